@@ -1,0 +1,224 @@
+/*
+ * zmpm.h -- C ABI of the MI355X explicit APIC MPM substep (+ AnisoMPM damage).
+ *
+ * This is the drop-in boundary that replaces the explicit-substep body of
+ * Ziran's MpmSimulationBase<T,3>.  Every declaration names the reference
+ * interface (file:line under the Ziran tree) whose behaviour it carries.
+ * Plain pointers and sizes only; no C++ or torch types cross this boundary.
+ *
+ * Conventions
+ *   - dim = 3 only.  Vectors are 3 doubles, matrices 9 doubles ROW-major
+ *     (element (r,c) at [3*r+c]).  Host arrays are caller-owned and copied.
+ *   - Every call returns a zmpm_status; zmpm_last_error() gives the message.
+ *   - zmpm_substep()/zmpm_run() are asynchronous on the handle's HIP stream.
+ *     Device-side conditions (CFL violation, NaN, domain exit, fixed-point
+ *     overflow) are latched on the device and reported by the next
+ *     zmpm_synchronize(), zmpm_download(), zmpm_get_binning() or
+ *     zmpm_get_grid().
+ *   - Calls on one handle are not thread-safe; separate handles are
+ *     independent (one HIP stream each).
+ */
+#ifndef ZMPM_H
+#define ZMPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZMPM_ABI_VERSION 1
+
+/* Error model.  Replaces ZIRAN_ASSERT -> std::runtime_error
+ * (Lib/Ziran/CS/Util/Debug.h:19-37) and the CFL RestartException
+ * (Lib/MPM/MpmSimulationBase.cpp:1480-1499). */
+typedef enum {
+    ZMPM_OK = 0,
+    ZMPM_E_ARG = 1,      /* bad argument / size / state                          */
+    ZMPM_E_HIP = 2,      /* HIP runtime error                                     */
+    ZMPM_E_DOMAIN = 3,   /* a particle's base node left [0, domain_nodes-5]      */
+    ZMPM_E_CFL = 4,      /* a particle moved more than dx in one substep          */
+    ZMPM_E_NAN = 5,      /* non-finite particle position                          */
+    ZMPM_E_OVERFLOW = 6, /* 64-bit fixed-point P2G accumulator out of range       */
+    ZMPM_E_NOMEM = 7     /* device allocation failed                              */
+} zmpm_status;
+
+/* Precision form chosen at create time. */
+typedef enum {
+    ZMPM_PREC_FP64 = 0, /* plain form: everything in double, f64 atomics          */
+    ZMPM_PREC_FP32 = 1  /* particle state + stress in float, P2G in int64 fixed
+                           point (deterministic), FP64 polar only near inversion */
+} zmpm_precision;
+
+/* Constitutive models on the path.
+ *  FIXED_COROTATED          Physics/ConstitutiveModel/CorotatedElasticity.cpp:55-76
+ *  QR_STABLE_NEO_HOOKEAN    Physics/ConstitutiveModel/QRStableNeoHookean.h:54-213
+ *  NEO_HOOKEAN              Physics/ConstitutiveModel/NeoHookean.cpp:15-48
+ *  EQUATION_OF_STATE (J-based helper)  Physics/ConstitutiveModel/EquationOfState.h:47-120,
+ *                           MPM/Force/JBasedMpmForceHelper.cpp:28-50,79-94        */
+typedef enum {
+    ZMPM_MODEL_FIXED_COROTATED = 0,
+    ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN = 1,
+    ZMPM_MODEL_NEO_HOOKEAN = 2,
+    ZMPM_MODEL_EQUATION_OF_STATE = 3
+} zmpm_model;
+
+/* Collision object types; values equal AnalyticCollisionObject::COLLISION_OBJECT_TYPE
+ * (Lib/Ziran/Math/Geometry/CollisionObject.h:47-52). */
+typedef enum {
+    ZMPM_STICKY = 1,
+    ZMPM_SLIP = 2,
+    ZMPM_SEPARATE = 3,
+    ZMPM_GHOST = 4
+} zmpm_collider_type;
+
+/* Level-set shapes: HalfSpace (Math/Geometry/AnalyticLevelSet.cpp:318-363) and
+ * AxisAlignedAnalyticBox (AnalyticLevelSet.cpp:412-428 over AnalyticBox :545-600). */
+typedef enum {
+    ZMPM_SHAPE_HALF_SPACE = 0,
+    ZMPM_SHAPE_BOX = 1
+} zmpm_shape;
+
+/* Scene-wide parameters.  Mirrors the public members of MpmSimulationBase
+ * (Lib/MPM/MpmSimulationBase.h:47-76) that the explicit step reads. */
+typedef struct {
+    int32_t precision;       /* zmpm_precision                                            */
+    int32_t domain_nodes;    /* grid nodes per axis that may be addressed: power of two in
+                                [64, 1024] (the reference SPGrid is 4096^3,
+                                MpmGrid.h:111-112).  Sizes the dense page table.         */
+    double dx;               /* MpmSimulationBase::dx                                     */
+    double gravity[3];       /* MpmSimulationBase::gravity                                */
+    double apic_rpic_ratio;  /* MpmSimulationBase::apic_rpic_ratio (default 1)            */
+    double cfl;              /* MpmSimulationBase::cfl (half-cell flag only)              */
+    int32_t device;          /* HIP device ordinal                                        */
+    int32_t reserved;
+} zmpm_scene_desc;
+
+/* One material = one constitutive-model parameter set shared by its particles,
+ * plus the optional AnisotropicPhaseField (Projects/anisofracture/
+ * AnisotropicPhaseField.h:10-33) attached by
+ * MpmParticleHandleBase::addFBasedMpmForceWithAnisotropicPhaseField
+ * (Lib/MPM/MpmParticleHandleBase.cpp:175-222). */
+typedef struct {
+    int32_t model;           /* zmpm_model                                                */
+    int32_t has_damage;      /* 1: phase-field damage updated each substep                */
+    double mu, lambda;       /* Lame parameters (corotated / QR-NH / NH)                  */
+    double kx, ky;           /* QRStableNeoHookean extra fibre stiffness                  */
+    double bulk, gamma;      /* EquationOfState                                           */
+    int32_t num_directors;   /* 1 or 2 structural directors a_0                           */
+    int32_t reserved;
+    double a0[2][3];         /* structural directors (reference uses unit e1, e2)         */
+    double alphas[2];
+    double residual_phase;   /* k in g = (1-d)^2 (1-k) + k                                */
+    double l0, sigma_crit, eta, zeta;
+} zmpm_material;
+
+/* Analytic collision object with rigid translation b(t) = b0 + dbdt * t
+ * (CollisionObject.h:144-148 setTranslation; R = I, s = 1, omega = 0). */
+typedef struct {
+    int32_t shape;           /* zmpm_shape                                                */
+    int32_t type;            /* zmpm_collider_type                                        */
+    double origin[3];        /* half-space origin                                          */
+    double normal[3];        /* half-space outward normal (normalised by the library)     */
+    double box_min[3];       /* box corners in object space                               */
+    double box_max[3];
+    double b0[3];            /* translation at t = 0                                      */
+    double dbdt[3];          /* translation velocity                                      */
+    double friction;         /* SLIP / SEPARATE kinematic friction                        */
+} zmpm_collider;
+
+/* Host particle arrays (caller-owned).  Stands for the Particles/DataManager
+ * attributes the explicit step touches (Math/Geometry/Particles.h:7-45;
+ * "F" FBasedMpmForceHelper.h:219-222; "element measure" :215-218; C added at
+ * MpmSimulationBase.cpp:118; phase field AnisotropicPhaseField.h:13-25). */
+typedef struct {
+    int64_t n;
+    double* x;          /* [n][3]                                                         */
+    double* v;          /* [n][3]                                                         */
+    double* C;          /* [n][9] APIC affine matrix                                      */
+    double* F;          /* [n][9] deformation gradient; EOS particles keep J in F[0]      */
+    double* m;          /* [n] mass                                                       */
+    double* V0;         /* [n] element measure (initial volume)                           */
+    int32_t* material;  /* [n] index into the material table                              */
+    double* d;          /* [n] damage, may be NULL (0 on upload)                          */
+    double* g;          /* [n] degradation g of the QR model, may be NULL (1 on upload)   */
+    double* laplacian;  /* [n] damage Laplacian (download only), may be NULL              */
+} zmpm_particles;
+
+typedef struct zmpm_context* zmpm_handle;
+
+/* Library identity: ABI version and the gfx target the device code was built for. */
+int32_t zmpm_abi_version(void);
+const char* zmpm_build_info(void);
+
+/* Construct a simulation.  Replaces MpmSimulationBase::MpmSimulationBase +
+ * initialize() (MpmSimulationBase.cpp:37-133): D^-1 = 4/dx^2, C attribute. */
+zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out);
+zmpm_status zmpm_destroy(zmpm_handle h);
+const char* zmpm_last_error(zmpm_handle h);
+
+/* Material table (<= 256 entries).  Replaces the per-particle constitutive
+ * model attribute added by addFBasedMpmForce* / addJBasedMpmForce
+ * (MpmParticleHandleBase.cpp:175-222,796-809). */
+zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material* mats);
+
+/* Collision objects (<= 16).  Replaces MpmSimulationBase::addCollisionObject
+ * (MpmSimulationBase.cpp:87-91); evaluation order = insertion order, as in
+ * AnalyticCollisionObject::multiObjectCollision (CollisionObject.cpp:66-105). */
+zmpm_status zmpm_add_collider(zmpm_handle h, const zmpm_collider* c, int32_t* id_out);
+zmpm_status zmpm_clear_colliders(zmpm_handle h);
+
+/* Particle state in / out.  Upload replaces the scene-building writes of
+ * MpmInitializationHelper::sampleInAnalyticLevelSet (MpmInitializationHelper.h:637-647);
+ * download returns particles in their upload order (the reference never
+ * reorders the DataManager arrays). */
+zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p);
+zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p);
+
+/* One explicit substep at time t with step dt.  Replaces
+ * MpmSimulationBase::advanceOneTimeStep (MpmSimulationBase.cpp:262-357) with
+ * symplectic = true, APIC, quadratic B-splines, including the AnisoMPM
+ * particlesToGrid override (Projects/anisofracture/AnisoFractureSimulation.h:129-141)
+ * and the collision-object updateState at t and t+dt (MpmSimulationBase.cpp:291-293). */
+zmpm_status zmpm_substep(zmpm_handle h, double dt, double t);
+
+/* `steps` substeps t0, t0+dt, ...; the GPU library replays a captured HIP graph. */
+zmpm_status zmpm_run(zmpm_handle h, int32_t steps, double dt, double t0);
+
+/* Wait for queued work; returns the first latched device condition. */
+zmpm_status zmpm_synchronize(zmpm_handle h);
+
+/* Binning of the LAST substep, as built by
+ * MpmSimulationBase::sortParticlesAndPolluteGrid (MpmSimulationBase.cpp:1531-1573):
+ *   keys[n]         sorted particle_sorter values ((offset>>data_bits)<<(32-block_bits)) + i
+ *   order[n]        particle_order
+ *   groups[2*G]     particle_group (first, last) pairs
+ *   block_offset[G] page index of each group
+ * Pass NULL arrays to query n_groups only. */
+zmpm_status zmpm_get_binning(zmpm_handle h, int64_t* n_groups, uint64_t* keys,
+                             int32_t* order, int32_t* groups, uint64_t* block_offset);
+
+/* Grid nodes with m != 0 (the reference's idx >= 0 nodes) after the LAST
+ * substep, sorted by (i, j, k):
+ *   coords[3N] int32, m[N], v[3N] = mv/m (MpmSimulationBase.cpp:655-666),
+ *   new_v[3N] after gridVelocityExplicitUpdate (:1179-1228),
+ *   pf[N] normalised phase field (AnisoFractureSimulation.h:177-179; 0 if no damage).
+ * Pass NULL arrays to query the count only. */
+zmpm_status zmpm_get_grid(zmpm_handle h, int64_t* n_nodes, int32_t* coords,
+                          double* m, double* v, double* new_v, double* pf);
+
+/* Per-stage device timing (HIP events on the handle's stream) of the substeps
+ * run while enabled; off by default.  No counterpart in the reference, whose
+ * ZIRAN_TIMER scopes (Lib/Ziran/CS/Util/Timer.h:35-60) play this role on the CPU. */
+zmpm_status zmpm_set_timing(zmpm_handle h, int32_t enable);
+
+/* Mean per-substep stage times (ms) of the last timed zmpm_run/zmpm_substep.
+ * *count: capacity of names/ms on entry, entries written on return.
+ * names[] receives static strings.  The oracle reports none. */
+zmpm_status zmpm_get_timings(zmpm_handle h, int32_t* count, const char** names, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZMPM_H */

@@ -1,0 +1,1541 @@
+/*
+ * zmpm_oracle.cpp -- FP64 CPU restatement of Ziran's explicit APIC MPM substep
+ * and AnisoMPM explicit damage update.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the GPU library is
+ * compared against.  Only tests/, the smoke check and the bench's CPU-baseline
+ * leg may load it; the product library never links or calls it.
+ *
+ * It implements the same C ABI as the GPU library (include/zmpm.h) so a test
+ * can drive both with identical calls, plus a few zmpm_oracle_* unit entry
+ * points used to pin individual pieces against closed forms.
+ *
+ * Faithfulness: every routine cites the reference lines it restates.  The
+ * reference cannot be built here (Eigen3 / TBB headers and OpenVDB are absent),
+ * so bit-level agreement with Eigen's evaluation order is not claimed; the
+ * algorithms, operation order inside each routine, accumulation order of P2G
+ * (8-colour block schedule, sorted particle order) and all branch conventions
+ * follow the reference.  Arithmetic is double; when the handle is created in
+ * the FP32 layout the binning arithmetic (x * (1/dx) - 0.5, floor) is done in
+ * float, as MpmSimulationBase<float,3> would, so keys can be compared bit-exactly.
+ *
+ * Compiled with -ffp-contract=off: the reference's base-node flooring must not
+ * be FMA-contracted (MpmSimulationBase.cpp:1545).
+ */
+#include "zmpm.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------- 3x3 algebra
+struct M3 {
+    double a[3][3];
+};
+
+inline M3 mzero()
+{
+    M3 r;
+    std::memset(&r, 0, sizeof(r));
+    return r;
+}
+inline M3 meye()
+{
+    M3 r = mzero();
+    r.a[0][0] = r.a[1][1] = r.a[2][2] = 1;
+    return r;
+}
+inline M3 mmul(const M3& A, const M3& B)
+{
+    M3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            r.a[i][j] = A.a[i][0] * B.a[0][j] + A.a[i][1] * B.a[1][j] + A.a[i][2] * B.a[2][j];
+    return r;
+}
+inline M3 mtrans(const M3& A)
+{
+    M3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            r.a[i][j] = A.a[j][i];
+    return r;
+}
+inline M3 mload(const double* p)
+{
+    M3 r;
+    for (int i = 0; i < 9; i++)
+        r.a[i / 3][i % 3] = p[i];
+    return r;
+}
+inline void mstore(const M3& A, double* p)
+{
+    for (int i = 0; i < 9; i++)
+        p[i] = A.a[i / 3][i % 3];
+}
+
+// Eigen's 3x3 determinant expansion along the first column
+// (used by F.determinant() at CorotatedElasticity.cpp:60, NeoHookean.cpp:27).
+inline double mdet(const M3& m)
+{
+    const double(*a)[3] = m.a;
+    return a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1])
+        - a[1][0] * (a[0][1] * a[2][2] - a[0][2] * a[2][1])
+        + a[2][0] * (a[0][1] * a[1][2] - a[0][2] * a[1][1]);
+}
+
+// cofactorMatrix: J F^{-T}  (Lib/Ziran/Math/Linear/DenseExt.h:240-252)
+inline M3 mcofactor(const M3& Fm)
+{
+    const double(*F)[3] = Fm.a;
+    M3 A;
+    A.a[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+    A.a[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+    A.a[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+    A.a[1][0] = F[0][2] * F[2][1] - F[0][1] * F[2][2];
+    A.a[1][1] = F[0][0] * F[2][2] - F[0][2] * F[2][0];
+    A.a[1][2] = F[0][1] * F[2][0] - F[0][0] * F[2][1];
+    A.a[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+    A.a[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+    A.a[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+    return A;
+}
+
+// 3x3 inverse through the cofactor matrix (Eigen's compute_inverse for size 3).
+inline M3 minverse(const M3& A)
+{
+    M3 cof = mcofactor(A); // cof = det * A^{-T}
+    double det = cof.a[0][0] * A.a[0][0] + cof.a[0][1] * A.a[0][1] + cof.a[0][2] * A.a[0][2];
+    double inv = 1.0 / det;
+    M3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            r.a[i][j] = cof.a[j][i] * inv;
+    return r;
+}
+
+// ------------------------------------------------------- Givens / implicit QR SVD
+// Givens rotation with the conventions of Lib/Ziran/Math/Linear/Givens.h:32-181.
+struct Givens {
+    int i, k;
+    double c, s;
+    Givens(int i_, int k_)
+        : i(i_), k(k_), c(1), s(0)
+    {
+    }
+    // Givens.h:72-85
+    void compute(double a, double b)
+    {
+        double d = a * a + b * b;
+        c = 1;
+        s = 0;
+        double sqrtd = std::sqrt(d);
+        if (sqrtd) {
+            double t = 1 / sqrtd;
+            c = a * t;
+            s = -b * t;
+        }
+    }
+    // Givens.h:92-105
+    void computeUnconventional(double a, double b)
+    {
+        double d = a * a + b * b;
+        c = 0;
+        s = 1;
+        double sqrtd = std::sqrt(d);
+        if (sqrtd) {
+            double t = 1 / sqrtd;
+            s = a * t;
+            c = b * t;
+        }
+    }
+    // Givens.h:131-140 (rows i,k of A)
+    void rowRotation(M3& A) const
+    {
+        for (int j = 0; j < 3; j++) {
+            double t1 = A.a[i][j], t2 = A.a[k][j];
+            A.a[i][j] = c * t1 - s * t2;
+            A.a[k][j] = s * t1 + c * t2;
+        }
+    }
+    // Givens.h:150-159 (columns i,k of A)
+    void columnRotation(M3& A) const
+    {
+        for (int j = 0; j < 3; j++) {
+            double t1 = A.a[j][i], t2 = A.a[j][k];
+            A.a[j][i] = c * t1 - s * t2;
+            A.a[j][k] = s * t1 + c * t2;
+        }
+    }
+    // Givens.h:164-170
+    void mulAssign(const Givens& B)
+    {
+        double nc = c * B.c - s * B.s;
+        double ns = s * B.c + c * B.s;
+        c = nc;
+        s = ns;
+    }
+};
+
+// zeroChase (Givens.h:195-240)
+void zeroChase(M3& H, M3& U, M3& V)
+{
+    Givens r1(0, 1);
+    r1.compute(H.a[0][0], H.a[1][0]);
+    Givens r2(1, 2);
+    if (H.a[1][0] != 0)
+        r2.compute(H.a[0][0] * H.a[0][1] + H.a[1][0] * H.a[1][1], H.a[0][0] * H.a[0][2] + H.a[1][0] * H.a[1][2]);
+    else
+        r2.compute(H.a[0][1], H.a[0][2]);
+    r1.rowRotation(H);
+    r2.columnRotation(H);
+    r2.columnRotation(V);
+    Givens r3(1, 2);
+    r3.compute(H.a[1][1], H.a[2][1]);
+    r3.rowRotation(H);
+    r1.columnRotation(U);
+    r3.columnRotation(U);
+}
+
+// makeUpperBidiag (Givens.h:252-271)
+void makeUpperBidiag(M3& H, M3& U, M3& V)
+{
+    U = meye();
+    V = meye();
+    Givens r(1, 2);
+    r.compute(H.a[1][0], H.a[2][0]);
+    r.rowRotation(H);
+    r.columnRotation(U);
+    zeroChase(H, U, V);
+}
+
+// wilkinsonShift (ImplicitQRSVD.h:260-272)
+double wilkinsonShift(double a1, double b1, double a2)
+{
+    double d = 0.5 * (a1 - a2);
+    double bs = b1 * b1;
+    return a2 - std::copysign(bs / (std::fabs(d) + std::sqrt(d * d + bs)), d);
+}
+
+// 2x2 SVD of the block B[t..t+1][t..t+1] in Givens form
+// (polarDecomposition 2x2 ImplicitQRSVD.h:45-66, singularValueDecomposition 2x2 :116-175)
+void svd2x2(const M3& B, int t, Givens& U, double& s0, double& s1, Givens& V)
+{
+    double A00 = B.a[t][t], A01 = B.a[t][t + 1], A10 = B.a[t + 1][t], A11 = B.a[t + 1][t + 1];
+    double x0 = A00 + A11, x1 = A10 - A01;
+    double denominator = std::sqrt(x0 * x0 + x1 * x1);
+    U.c = 1;
+    U.s = 0;
+    if (denominator != 0) {
+        U.c = x0 / denominator;
+        U.s = -x1 / denominator;
+    }
+    // S = A; U.rowRotation(S) on a 2x2
+    double S00 = U.c * A00 - U.s * A10;
+    double S01 = U.c * A01 - U.s * A11;
+    double S11 = U.s * A01 + U.c * A11;
+    double cosine, sine;
+    double x = S00, y = S01, z = S11;
+    double y2 = y * y;
+    if (y2 == 0) {
+        cosine = 1;
+        sine = 0;
+        s0 = x;
+        s1 = z;
+    }
+    else {
+        double tau = 0.5 * (x - z);
+        double w = std::sqrt(tau * tau + y2);
+        double tt;
+        if (tau > 0)
+            tt = y / (tau + w);
+        else
+            tt = y / (tau - w);
+        cosine = 1 / std::sqrt(tt * tt + 1);
+        sine = -tt * cosine;
+        double c2 = cosine * cosine;
+        double csy = 2 * cosine * sine * y;
+        double s2 = sine * sine;
+        s0 = c2 * x - csy + s2 * z;
+        s1 = s2 * x + csy + c2 * z;
+    }
+    if (s0 < s1) {
+        std::swap(s0, s1);
+        V.c = -sine;
+        V.s = cosine;
+    }
+    else {
+        V.c = cosine;
+        V.s = sine;
+    }
+    U.mulAssign(V);
+}
+
+// process<t> (ImplicitQRSVD.h:277-293)
+void process(int t, M3& B, M3& U, double* sigma, M3& V)
+{
+    int other = (t == 1) ? 0 : 2;
+    Givens u(0, 1), v(0, 1);
+    sigma[other] = B.a[other][other];
+    svd2x2(B, t, u, sigma[t], sigma[t + 1], v);
+    u.i += t;
+    u.k += t;
+    v.i += t;
+    v.k += t;
+    u.columnRotation(U);
+    v.columnRotation(V);
+}
+
+// flipSign (ImplicitQRSVD.h:298-303)
+void flipSign(int i, M3& U, double* sigma)
+{
+    sigma[i] = -sigma[i];
+    for (int r = 0; r < 3; r++)
+        U.a[r][i] = -U.a[r][i];
+}
+void swapCols(M3& A, int i, int j)
+{
+    for (int r = 0; r < 3; r++)
+        std::swap(A.a[r][i], A.a[r][j]);
+}
+void negCol(M3& A, int i)
+{
+    for (int r = 0; r < 3; r++)
+        A.a[r][i] = -A.a[r][i];
+}
+
+// sort<0> (ImplicitQRSVD.h:308-341)
+void sort0(M3& U, double* sigma, M3& V)
+{
+    if (std::fabs(sigma[1]) >= std::fabs(sigma[2])) {
+        if (sigma[1] < 0) {
+            flipSign(1, U, sigma);
+            flipSign(2, U, sigma);
+        }
+        return;
+    }
+    if (sigma[2] < 0) {
+        flipSign(1, U, sigma);
+        flipSign(2, U, sigma);
+    }
+    std::swap(sigma[1], sigma[2]);
+    swapCols(U, 1, 2);
+    swapCols(V, 1, 2);
+    if (sigma[1] > sigma[0]) {
+        std::swap(sigma[0], sigma[1]);
+        swapCols(U, 0, 1);
+        swapCols(V, 0, 1);
+    }
+    else {
+        negCol(U, 2);
+        negCol(V, 2);
+    }
+}
+
+// sort<1> (ImplicitQRSVD.h:307-345 second overload)
+void sort1(M3& U, double* sigma, M3& V)
+{
+    if (std::fabs(sigma[0]) >= sigma[1]) {
+        if (sigma[0] < 0) {
+            flipSign(0, U, sigma);
+            flipSign(2, U, sigma);
+        }
+        return;
+    }
+    std::swap(sigma[0], sigma[1]);
+    swapCols(U, 0, 1);
+    swapCols(V, 0, 1);
+    if (std::fabs(sigma[1]) < std::fabs(sigma[2])) {
+        std::swap(sigma[1], sigma[2]);
+        swapCols(U, 1, 2);
+        swapCols(V, 1, 2);
+    }
+    else {
+        negCol(U, 1);
+        negCol(V, 1);
+    }
+    if (sigma[1] < 0) {
+        flipSign(1, U, sigma);
+        flipSign(2, U, sigma);
+    }
+}
+
+// singularValueDecompositionHelper (ImplicitQRSVD.h:354-517): the implicit-shift
+// symmetric QR SVD of Gast, Fu, Jiang and Teran (UCLA CAM 16-04, 2016).
+int svd3(const M3& A, M3& U, double* sigma, M3& V, double tol)
+{
+    M3 B = A;
+    U = meye();
+    V = meye();
+    makeUpperBidiag(B, U, V);
+    int count = 0;
+    double mu = 0;
+    Givens r(0, 1);
+    double alpha_1 = B.a[0][0], beta_1 = B.a[0][1], alpha_2 = B.a[1][1], alpha_3 = B.a[2][2], beta_2 = B.a[1][2];
+    double gamma_1 = alpha_1 * beta_1, gamma_2 = alpha_2 * beta_2;
+    tol *= std::max(0.5 * std::sqrt(alpha_1 * alpha_1 + alpha_2 * alpha_2 + alpha_3 * alpha_3 + beta_1 * beta_1 + beta_2 * beta_2), 1.0);
+    while (std::fabs(beta_2) > tol && std::fabs(beta_1) > tol && std::fabs(alpha_1) > tol
+        && std::fabs(alpha_2) > tol && std::fabs(alpha_3) > tol) {
+        mu = wilkinsonShift(alpha_2 * alpha_2 + beta_1 * beta_1, gamma_2, alpha_3 * alpha_3 + beta_2 * beta_2);
+        r.compute(alpha_1 * alpha_1 - mu, gamma_1);
+        r.columnRotation(B);
+        r.columnRotation(V);
+        zeroChase(B, U, V);
+        alpha_1 = B.a[0][0];
+        beta_1 = B.a[0][1];
+        alpha_2 = B.a[1][1];
+        alpha_3 = B.a[2][2];
+        beta_2 = B.a[1][2];
+        gamma_1 = alpha_1 * beta_1;
+        gamma_2 = alpha_2 * beta_2;
+        count++;
+    }
+    if (std::fabs(beta_2) <= tol) {
+        process(0, B, U, sigma, V);
+        sort0(U, sigma, V);
+    }
+    else if (std::fabs(beta_1) <= tol) {
+        process(1, B, U, sigma, V);
+        sort1(U, sigma, V);
+    }
+    else if (std::fabs(alpha_2) <= tol) {
+        Givens r1(1, 2);
+        r1.computeUnconventional(B.a[1][2], B.a[2][2]);
+        r1.rowRotation(B);
+        r1.columnRotation(U);
+        process(0, B, U, sigma, V);
+        sort0(U, sigma, V);
+    }
+    else if (std::fabs(alpha_3) <= tol) {
+        Givens r1(1, 2);
+        r1.compute(B.a[1][1], B.a[1][2]);
+        r1.columnRotation(B);
+        r1.columnRotation(V);
+        Givens r2(0, 2);
+        r2.compute(B.a[0][0], B.a[0][2]);
+        r2.columnRotation(B);
+        r2.columnRotation(V);
+        process(0, B, U, sigma, V);
+        sort0(U, sigma, V);
+    }
+    else if (std::fabs(alpha_1) <= tol) {
+        Givens r1(0, 1);
+        r1.computeUnconventional(B.a[0][1], B.a[1][1]);
+        r1.rowRotation(B);
+        r1.columnRotation(U);
+        Givens r2(0, 2);
+        r2.computeUnconventional(B.a[0][2], B.a[2][2]);
+        r2.rowRotation(B);
+        r2.columnRotation(U);
+        process(1, B, U, sigma, V);
+        sort1(U, sigma, V);
+    }
+    return count;
+}
+
+const double kSvdTol = 1024 * std::numeric_limits<double>::epsilon(); // ImplicitQRSVD.h:526-533
+
+// polarDecomposition rotation R = U V^T (ImplicitQRSVD.h:546-558)
+M3 polarR(const M3& F)
+{
+    M3 U, V;
+    double s[3];
+    svd3(F, U, s, V, kSvdTol);
+    return mmul(U, mtrans(V));
+}
+
+// GivensQR (GivensQR.h:20-48): R <- G_n..G_1 A, Q = (G_n..G_1)^T
+void givensQR(const M3& A, M3& Q, M3& R)
+{
+    R = A;
+    Q = meye();
+    for (int j = 0; j < 3; j++)
+        for (int i = 2; i > j; i--) {
+            Givens r(i - 1, i);
+            r.compute(R.a[i - 1][j], R.a[i][j]);
+            r.rowRotation(R);
+            r.rowRotation(Q);
+        }
+    Q = mtrans(Q);
+}
+
+// -------------------------------------------------------------- constitutive
+struct Material {
+    zmpm_material p;
+};
+
+// QRStableNeoHookean::updateScratch + firstPiola (QRStableNeoHookean.h:108-213,
+// QrBasedHelper.h:28-40).  Returns P; J and energy through pointers.
+M3 qrStablePiola(const zmpm_material& mat, double g, double kx, double ky, const M3& F, double* J_out, double* psi_out)
+{
+    const double mu = mat.mu, lambda = mat.lambda;
+    M3 Q, R;
+    givensQR(F, Q, R);
+    if (R.a[0][0] < 0) { // sign convention, QRStableNeoHookean.h:125-139
+        R.a[0][0] *= -1;
+        for (int r = 0; r < 3; r++) Q.a[r][0] *= -1;
+        R.a[0][1] *= -1;
+        R.a[0][2] *= -1;
+        R.a[2][2] *= -1;
+        for (int r = 0; r < 3; r++) Q.a[r][2] *= -1;
+    }
+    if (R.a[1][1] < 0) {
+        R.a[1][1] *= -1;
+        for (int r = 0; r < 3; r++) Q.a[r][1] *= -1;
+        R.a[1][2] *= -1;
+        R.a[2][2] *= -1;
+        for (int r = 0; r < 3; r++) Q.a[r][2] *= -1;
+    }
+    M3 Rinv = minverse(R);
+    double J = 1;
+    for (int i = 0; i < 3; i++) J *= R.a[i][i];
+    auto sqr = [](double v) { return v * v; };
+    double energy = 0;
+    double vol = (J >= 1) ? g * 0.5 * lambda * sqr(J - 1) : 0.5 * lambda * sqr(J - 1);
+    energy += vol;
+    double rss = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = i; j < 3; j++) rss += sqr(R.a[i][j]);
+    energy += g * (mu / 2 * (rss - 3) - mu * (J - 1));
+    energy += kx / 2 * sqr(R.a[0][0] - 1);
+    energy += ky / 2 * sqr(std::sqrt(sqr(R.a[0][1]) + sqr(R.a[1][1])) - 1);
+    M3 D = mzero(); // dPsiHat/dR
+    if (J >= 1)
+        for (int i = 0; i < 3; i++) D.a[i][i] += g * lambda * (J - 1) * J / R.a[i][i];
+    else
+        for (int i = 0; i < 3; i++) D.a[i][i] += lambda * (J - 1) * J / R.a[i][i];
+    for (int i = 0; i < 3; i++)
+        for (int j = i; j < 3; j++) D.a[i][j] += g * mu * R.a[i][j];
+    for (int i = 0; i < 3; i++) D.a[i][i] -= g * mu * J / R.a[i][i];
+    D.a[0][0] += kx * (R.a[0][0] - 1);
+    {
+        double ss = std::sqrt(sqr(R.a[0][1]) + sqr(R.a[1][1]));
+        D.a[0][1] += ky * (ss - 1) * R.a[0][1] / ss;
+        D.a[1][1] += ky * (ss - 1) * R.a[1][1] / ss;
+    }
+    // evaluateP: A = D R^T, symmetrised from its upper triangle; P = Q A R^{-T}
+    M3 A = mmul(D, mtrans(R));
+    for (int i = 1; i < 3; i++)
+        for (int j = 0; j < i; j++) A.a[i][j] = A.a[j][i];
+    if (J_out) *J_out = J;
+    if (psi_out) *psi_out = energy;
+    return mmul(mmul(Q, A), mtrans(Rinv));
+}
+
+// First Piola stress of a material at F with degradation g.
+M3 firstPiola(const zmpm_material& mat, const M3& F, double g)
+{
+    switch (mat.model) {
+    case ZMPM_MODEL_FIXED_COROTATED: {
+        // CorotatedElasticity::updateScratch / firstPiola (CorotatedElasticity.cpp:55-76)
+        double J = mdet(F);
+        M3 JFinvT = mcofactor(F);
+        M3 R = polarR(F);
+        M3 P;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                P.a[i][j] = 2 * mat.mu * (F.a[i][j] - R.a[i][j]) + mat.lambda * (J - 1) * JFinvT.a[i][j];
+        return P;
+    }
+    case ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN:
+        return qrStablePiola(mat, g, mat.kx, mat.ky, F, nullptr, nullptr);
+    case ZMPM_MODEL_NEO_HOOKEAN: {
+        // P = tau F^{-T} with tau from NeoHookean::kirchhoff (NeoHookean.cpp:23-48)
+        double J = mdet(F);
+        M3 FinvT = mcofactor(F);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) FinvT.a[i][j] *= 1.0 / J;
+        double scale = mat.lambda * std::log(J) - mat.mu;
+        M3 P;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) P.a[i][j] = mat.mu * F.a[i][j] + scale * FinvT.a[i][j];
+        return P;
+    }
+    default:
+        return mzero();
+    }
+}
+
+// Strain energy density (for finite-difference pinning only).
+double psiOf(const zmpm_material& mat, const M3& F, double g)
+{
+    switch (mat.model) {
+    case ZMPM_MODEL_FIXED_COROTATED: { // CorotatedElasticity.cpp:65-70
+        double J = mdet(F);
+        M3 R = polarR(F);
+        double s = 0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) s += (F.a[i][j] - R.a[i][j]) * (F.a[i][j] - R.a[i][j]);
+        return mat.mu * s + 0.5 * mat.lambda * (J - 1) * (J - 1);
+    }
+    case ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN: {
+        double e = 0;
+        qrStablePiola(mat, g, mat.kx, mat.ky, F, nullptr, &e);
+        return e;
+    }
+    case ZMPM_MODEL_NEO_HOOKEAN: { // NeoHookean.cpp:31-35
+        double J = mdet(F), lj = std::log(J), I1 = 0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) I1 += F.a[i][j] * F.a[i][j];
+        return 0.5 * mat.mu * (I1 - 3) - mat.mu * lj + 0.5 * mat.lambda * lj * lj;
+    }
+    default:
+        return 0;
+    }
+}
+
+// V0 * kirchhoff stress: FBasedMpmForceHelper::updateState (FBasedMpmForceHelper.cpp:41-63)
+// with kirchhoff = P F^T (HyperelasticConstitutiveModel.h:75-80); J-based:
+// JBasedMpmForceHelper::updateState (JBasedMpmForceHelper.cpp:28-50).
+M3 vtauOf(const zmpm_material& mat, const double* Fp, double V0, double g)
+{
+    M3 tau;
+    if (mat.model == ZMPM_MODEL_EQUATION_OF_STATE) {
+        double J = Fp[0];
+        double dpsi_dJ = -mat.bulk * (std::pow(J, -mat.gamma) - 1.0); // EquationOfState.h:83-86
+        double s = V0 * dpsi_dJ * J;
+        tau = mzero();
+        tau.a[0][0] = tau.a[1][1] = tau.a[2][2] = s;
+        return tau;
+    }
+    M3 F = mload(Fp);
+    if (mat.model == ZMPM_MODEL_NEO_HOOKEAN) {
+        // NeoHookean::kirchhoff (NeoHookean.cpp:38-43): tau = mu F F^T + (lambda logJ - mu) I
+        double J = mdet(F);
+        double scale = mat.lambda * std::log(J) - mat.mu;
+        M3 FFt = mmul(F, mtrans(F));
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) tau.a[i][j] = mat.mu * FFt.a[i][j] + (i == j ? scale : 0.0);
+    }
+    else {
+        M3 P = firstPiola(mat, F, g);
+        tau = mmul(P, mtrans(F));
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) tau.a[i][j] *= V0;
+    return tau;
+}
+
+// Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (columns of Vec
+// are unit eigenvectors).  Stands in for Eigen::EigenSolver at
+// AnisoFractureSimulation.h:244-254: sigma^+ is invariant to eigenvector sign and
+// to the basis chosen inside a degenerate eigenspace.
+void symEigen(const M3& S_in, double* lam, M3& Vec)
+{
+    M3 S = S_in;
+    Vec = meye();
+    for (int sweep = 0; sweep < 50; sweep++) {
+        double off = S.a[0][1] * S.a[0][1] + S.a[0][2] * S.a[0][2] + S.a[1][2] * S.a[1][2];
+        double diag = S.a[0][0] * S.a[0][0] + S.a[1][1] * S.a[1][1] + S.a[2][2] * S.a[2][2];
+        if (off <= 1e-36 * diag || off == 0) break;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                double apq = S.a[p][q];
+                if (apq == 0) continue;
+                double theta = (S.a[q][q] - S.a[p][p]) / (2 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 3; k++) { // S <- J^T S J
+                    double skp = S.a[k][p], skq = S.a[k][q];
+                    S.a[k][p] = c * skp - s * skq;
+                    S.a[k][q] = s * skp + c * skq;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double spk = S.a[p][k], sqk = S.a[q][k];
+                    S.a[p][k] = c * spk - s * sqk;
+                    S.a[q][k] = s * spk + c * sqk;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double vkp = Vec.a[k][p], vkq = Vec.a[k][q];
+                    Vec.a[k][p] = c * vkp - s * vkq;
+                    Vec.a[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < 3; i++) lam[i] = S.a[i][i];
+}
+
+inline double macaulay(double x) { return (x + std::fabs(x)) / 2; } // MathTools.h:276-281
+
+// updateDamageHelper for one particle (AnisoFractureSimulation.h:210-283) with
+// constructStructuralTensor (:365-391) and evaluateEffectiveCauchyStress
+// (QRStableNeoHookean.h:83-93).  dt is the substep.
+void damageUpdate(const zmpm_material& mat, const M3& F, double lap, double dt, double& d, double& g)
+{
+    double geometricResist = d - (mat.l0 * mat.l0 * lap);
+    // undegraded isotropic Cauchy stress: g = 1, kx = ky = 0
+    double J = 1;
+    M3 P = qrStablePiola(mat, 1.0, 0.0, 0.0, F, &J, nullptr);
+    M3 cauchy = mmul(P, mtrans(F));
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) cauchy.a[i][j] /= J;
+    M3 sym;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) sym.a[i][j] = 0.5 * (cauchy.a[i][j] + cauchy.a[j][i]);
+    double lam[3];
+    M3 vec;
+    symEigen(sym, lam, vec);
+    M3 sp = mzero();
+    for (int e = 0; e < 3; e++) {
+        double l = macaulay(lam[e]);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) sp.a[i][j] += l * (vec.a[i][e] * vec.a[j][e]);
+    }
+    // structural tensor A = I + sum_k alpha_k (R a0_k)(R a0_k)^T, R from the SVD of F
+    M3 R = polarR(F);
+    M3 A = meye();
+    int nd = mat.num_directors == 2 ? 2 : 1;
+    for (int k = 0; k < nd; k++) {
+        double a1[3];
+        for (int i = 0; i < 3; i++) a1[i] = R.a[i][0] * mat.a0[k][0] + R.a[i][1] * mat.a0[k][1] + R.a[i][2] * mat.a0[k][2];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) A.a[i][j] += mat.alphas[k] * (a1[i] * a1[j]);
+    }
+    M3 Asig = mmul(A, sp), sigA = mmul(sp, A);
+    double contraction = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) contraction += Asig.a[i][j] * sigA.a[i][j];
+    double phi = (1.0 / (mat.sigma_crit * mat.sigma_crit)) * contraction;
+    double dTilde = mat.zeta * macaulay(phi - 1);
+    double diff = ((1 - d) * dTilde) - geometricResist;
+    double newD = d + ((dt / mat.eta) * macaulay(diff));
+    d = std::min(newD, 1.0);
+    double k = mat.residual_phase;
+    g = ((1 - d) * (1 - d) * (1 - k)) + k;
+}
+
+// ------------------------------------------------------------ SPGrid layout
+// SPGrid_Mask_base / SPGrid_Mask (Lib/SPGrid/Core/SPGrid_Mask.h:22-129) for
+// log2_page = 12 and the GridState<T,3> struct size (MpmGrid.h:16-36):
+// float -> 64 B (log2 6), double -> 128 B (log2 7).
+struct Layout {
+    int data_bits, block_bits, xbits, ybits, zbits;
+    uint64_t xmask, ymask, zmask;
+};
+
+Layout makeLayout(int log2_struct)
+{
+    Layout L;
+    const int page_bits = 12;
+    L.data_bits = log2_struct;
+    L.block_bits = page_bits - L.data_bits;
+    L.zbits = L.block_bits / 3 + (L.block_bits % 3 > 0);
+    L.ybits = L.block_bits / 3 + (L.block_bits % 3 > 1);
+    L.xbits = L.block_bits / 3;
+    const int sh = 3 - L.block_bits % 3 + (page_bits - 12);
+    const uint64_t hi = ~0ull << page_bits;
+    uint64_t page_z = (0x9249249249249249ull << sh) & hi;
+    uint64_t page_y = (0x2492492492492492ull << sh) & hi;
+    uint64_t page_x = (0x4924924924924924ull << sh) & hi;
+    const int log2_field = log2_struct; // Array_type<> covers the whole struct
+    uint64_t ez = (uint64_t)((1 << L.zbits) - 1) << log2_field;
+    uint64_t ey = (uint64_t)((1 << L.ybits) - 1) << (log2_field + L.zbits);
+    uint64_t ex = (uint64_t)((1 << L.xbits) - 1) << (log2_field + L.zbits + L.ybits);
+    L.zmask = page_z | ez;
+    L.ymask = page_y | ey;
+    L.xmask = page_x | ex;
+    return L;
+}
+
+// Bit_Spread_u64 (SPGrid_Utilities.h:81-131): deposit the low bits of `data`
+// (sign-extended to 64 bits) into the set bits of `mask`, low to high.
+uint64_t bitSpread(int64_t data, uint64_t mask)
+{
+    uint64_t u = (uint64_t)data, result = 0;
+    for (int b = 0; b < 64; b++) {
+        uint64_t bit = 1ull << b;
+        if (mask & bit)
+            result |= u & bit;
+        else
+            u <<= 1;
+    }
+    return result;
+}
+
+// Bit_Pack (SPGrid_Utilities.cpp:43-62): gather the bits of `data` under `mask`.
+int64_t bitPack(uint64_t data, uint64_t mask)
+{
+    int64_t r = 0;
+    int out = 0;
+    for (int b = 0; b < 64; b++)
+        if (mask & (1ull << b)) {
+            if (data & (1ull << b)) r |= (int64_t)1 << out;
+            out++;
+        }
+    return r;
+}
+
+inline uint64_t linearOffset(const Layout& L, int i, int j, int k)
+{
+    return bitSpread(i, L.xmask) | bitSpread(j, L.ymask) | bitSpread(k, L.zmask);
+}
+
+// Packed_Add (SPGrid_Mask.h:237-245)
+inline uint64_t packedAdd(const Layout& L, uint64_t a, uint64_t b)
+{
+    uint64_t MX = ~L.xmask, MY = ~L.ymask, MZ = ~L.zmask, W = L.xmask | L.ymask | L.zmask;
+    uint64_t x = ((a | MX) + (b & ~MX)) & ~MX;
+    uint64_t y = ((a | MY) + (b & ~MY)) & ~MY;
+    uint64_t z = ((a | MZ) + (b & ~MZ)) & ~MZ;
+    uint64_t w = ((a | W) + (b & ~W)) & ~W;
+    return x | y | z | w;
+}
+
+// int_floor (Math/MathTools.h:15-25)
+inline int intFloor(double x)
+{
+    int i = (int)x;
+    return i - (i > x);
+}
+inline int intFloorF(float x)
+{
+    int i = (int)x;
+    return i - (i > x);
+}
+
+// Quadratic B-spline (BSplines.h:72-98): base node, weights, gradients, Hessian diag.
+inline void bspline(double x, int& base, double* w, double* dw, double* ddw)
+{
+    base = intFloor(x - 0.5);
+    double d0 = x - base;
+    double z = 1.5 - d0;
+    double z2 = z * z;
+    w[0] = 0.5 * z2;
+    double d1 = d0 - 1;
+    w[1] = 0.75 - d1 * d1;
+    double d2 = 1 - d1;
+    double zz = 1.5 - d2;
+    double zz2 = zz * zz;
+    w[2] = 0.5 * zz2;
+    if (dw) {
+        dw[0] = -z;
+        dw[1] = -2 * d1;
+        dw[2] = zz;
+    }
+    if (ddw) {
+        ddw[0] = 1;
+        ddw[1] = -2;
+        ddw[2] = 1;
+    }
+}
+
+// ------------------------------------------------------------------- grid
+struct Node {
+    double v[3];
+    double m;
+    double new_v[3];
+    int64_t idx;
+    double pf, pfm;
+};
+
+struct Collider {
+    zmpm_collider c;
+    double n[3]; // normalised half-space normal
+};
+
+} // namespace
+
+// --------------------------------------------------------------- the handle
+struct zmpm_context {
+    zmpm_scene_desc desc;
+    Layout L;
+    std::string err;
+    std::vector<zmpm_material> mats;
+    std::vector<Collider> colliders;
+    bool any_damage = false;
+    // particle state (index order, never reordered)
+    int64_t n = 0;
+    std::vector<double> x, v, C, F, m, V0, d, g, lap;
+    std::vector<int32_t> mat;
+    // binning of the last substep
+    std::vector<uint64_t> keys;
+    std::vector<int32_t> order;
+    std::vector<std::pair<int, int>> groups;
+    std::vector<uint64_t> block_offset;
+    std::vector<uint64_t> base_offset;
+    // grid (page map in first-touch order, MpmGrid::pollute)
+    std::unordered_map<uint64_t, int> page_to_block;
+    std::vector<uint64_t> block_pages;
+    std::vector<Node> nodes;
+    int NB = 0;
+    int latched = ZMPM_OK;
+    bool stepped = false;
+};
+
+namespace {
+
+zmpm_status fail(zmpm_context* h, zmpm_status s, const std::string& msg)
+{
+    if (h) h->err = msg;
+    return s;
+}
+
+// Base node of particle i in the handle's precision arithmetic; false if the
+// node leaves the addressable domain.
+bool baseNodeOf(const zmpm_context* h, int64_t i, int* b, bool* nan_out)
+{
+    bool ok = true;
+    for (int a = 0; a < 3; a++) {
+        double xv = h->x[3 * i + a];
+        if (!std::isfinite(xv)) {
+            *nan_out = true;
+            b[a] = 0;
+            ok = false;
+            continue;
+        }
+        double s;
+        if (h->desc.precision == ZMPM_PREC_FP32) {
+            float odx = (float)1 / (float)h->desc.dx;
+            float xs = (float)xv * odx;
+            float t = xs - 0.5f;
+            if (!(t > -1e6f && t < 1e6f)) { ok = false; b[a] = 0; continue; }
+            b[a] = intFloorF(t);
+        }
+        else {
+            double odx = 1 / h->desc.dx;
+            s = xv * odx;
+            double t = s - 0.5;
+            if (!(t > -1e9 && t < 1e9)) { ok = false; b[a] = 0; continue; }
+            b[a] = intFloor(t);
+        }
+        if (b[a] < 0 || b[a] > h->desc.domain_nodes - 5) ok = false;
+    }
+    return ok;
+}
+
+// Index-space position in the handle's precision (BSplineWeights::compute, MpmGrid.h:74-79).
+inline double indexSpace(const zmpm_context* h, double xv)
+{
+    if (h->desc.precision == ZMPM_PREC_FP32) {
+        float odx = (float)1 / (float)h->desc.dx;
+        return (double)((float)xv * odx);
+    }
+    return xv * (1 / h->desc.dx);
+}
+
+inline Node* nodeAt(zmpm_context* h, int i, int j, int k)
+{
+    uint64_t off = linearOffset(h->L, i, j, k);
+    auto it = h->page_to_block.find(off >> 12);
+    if (it == h->page_to_block.end()) return nullptr;
+    int elem = (int)((off >> h->L.data_bits) & (uint64_t)(h->NB - 1));
+    return &h->nodes[(size_t)it->second * h->NB + elem];
+}
+
+void setPage(zmpm_context* h, uint64_t offset)
+{
+    uint64_t page = offset >> 12;
+    if (h->page_to_block.find(page) == h->page_to_block.end()) {
+        h->page_to_block.emplace(page, (int)h->block_pages.size());
+        h->block_pages.push_back(page);
+    }
+}
+
+// sortParticlesAndPolluteGrid (MpmSimulationBase.cpp:1531-1573) + MpmGrid::pollute (MpmGrid.h:153-193)
+zmpm_status sortAndPollute(zmpm_context* h)
+{
+    const Layout& L = h->L;
+    const int64_t n = h->n;
+    const int index_bits = 32 - L.block_bits;
+    if (n >= ((int64_t)1 << index_bits)) return fail(h, ZMPM_E_ARG, "too many particles for the key layout");
+    h->keys.resize(n);
+    h->base_offset.resize(n);
+    h->order.resize(n);
+    bool bad = false, nan = false;
+    for (int64_t i = 0; i < n; i++) {
+        int b[3];
+        bool isnan = false;
+        if (!baseNodeOf(h, i, b, &isnan)) {
+            bad = true;
+            nan = nan || isnan;
+            continue;
+        }
+        uint64_t off = linearOffset(L, b[0], b[1], b[2]);
+        h->base_offset[i] = off;
+        h->keys[i] = ((off >> L.data_bits) << index_bits) + (uint64_t)i;
+    }
+    if (bad) {
+        h->latched = nan ? ZMPM_E_NAN : ZMPM_E_DOMAIN;
+        return fail(h, (zmpm_status)h->latched, nan ? "non-finite particle position" : "particle outside the addressable domain");
+    }
+    std::sort(h->keys.begin(), h->keys.end());
+    h->groups.clear();
+    h->block_offset.clear();
+    int last = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (i == n - 1 || (h->keys[i] >> 32) != (h->keys[i + 1] >> 32)) {
+            h->groups.emplace_back(last, (int)i);
+            h->block_offset.push_back(h->keys[i] >> 32);
+            last = (int)i + 1;
+        }
+        h->order[i] = (int32_t)(h->keys[i] & ((1ull << index_bits) - 1));
+    }
+    // pollute: first-touch page order over particles in index order
+    h->page_to_block.clear();
+    h->block_pages.clear();
+    const uint64_t bx = 1u << L.xbits, by = 1u << L.ybits, bz = 1u << L.zbits;
+    for (int64_t i = 0; i < n; i++) {
+        uint64_t off = h->base_offset[i];
+        if (!i || off != h->base_offset[i - 1]) {
+            setPage(h, off);
+            for (int a = 0; a <= 1; a++)
+                for (int b = 0; b <= 1; b++)
+                    for (int c = 0; c <= 1; c++)
+                        setPage(h, packedAdd(L, off, linearOffset(L, (int)(bx * a), (int)(by * b), (int)(bz * c))));
+        }
+    }
+    h->nodes.assign(h->block_pages.size() * (size_t)h->NB, Node());
+    for (auto& nd : h->nodes) {
+        std::memset(&nd, 0, sizeof(Node));
+        nd.idx = -1;
+    }
+    return ZMPM_OK;
+}
+
+// Weights of the 27-node stencil in MpmGrid::iterateKernel order (MpmGrid.h:437-488).
+struct Stencil {
+    int base[3];
+    double w[3][3], dw[3][3], ddw[3][3];
+    double odx;
+};
+
+inline void makeStencil(const zmpm_context* h, const double* xp, Stencil& s)
+{
+    s.odx = (h->desc.precision == ZMPM_PREC_FP32) ? (double)((float)1 / (float)h->desc.dx) : 1 / h->desc.dx;
+    for (int a = 0; a < 3; a++) bspline(indexSpace(h, xp[a]), s.base[a], s.w[a], s.dw[a], s.ddw[a]);
+}
+
+template <class OP>
+inline void iterateKernel(zmpm_context* h, const Stencil& s, const OP& op)
+{
+    const double odx = s.odx;
+    for (int i = 0; i < 3; i++) {
+        double wi = s.w[0][i];
+        double dwidxi = odx * s.dw[0][i];
+        for (int j = 0; j < 3; j++) {
+            double wj = s.w[1][j];
+            double wij = wi * wj;
+            double dwijdxi = dwidxi * wj;
+            double dwijdxj = wi * odx * s.dw[1][j];
+            for (int k = 0; k < 3; k++) {
+                double wk = s.w[2][k];
+                double wijk = wij * wk;
+                double dw3[3] = { dwijdxi * wk, dwijdxj * wk, wij * odx * s.dw[2][k] };
+                // Laplacian weight with the 3D index quirk (MpmGrid.h:608-613):
+                // ddw[1] and ddw[2] are indexed by i, not j and k.
+                double t1 = odx * odx * s.ddw[0][i] * wj * wk;
+                double t2 = odx * odx * s.ddw[1][i] * wi * wk;
+                double t3 = odx * odx * s.ddw[2][i] * wi * wj;
+                int node[3] = { s.base[0] + i, s.base[1] + j, s.base[2] + k };
+                Node* g = nodeAt(h, node[0], node[1], node[2]);
+                op(node, wijk, dw3, t1 + t2 + t3, *g);
+            }
+        }
+    }
+}
+
+const zmpm_material& matOf(const zmpm_context* h, int64_t i) { return h->mats[(size_t)h->mat[i]]; }
+
+// One explicit substep (MpmSimulationBase::advanceOneTimeStep, MpmSimulationBase.cpp:262-357).
+zmpm_status substep(zmpm_context* h, double dt, double t)
+{
+    if (h->n == 0) return ZMPM_OK;
+    zmpm_status st = sortAndPollute(h);
+    if (st != ZMPM_OK) return st;
+    const int64_t n = h->n;
+    const double dx = h->desc.dx;
+
+    // forcesUpdateParticleState (MpmSimulationBase.cpp:953-979)
+    std::vector<double> vtau((size_t)n * 9);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        M3 tau = vtauOf(matOf(h, i), &h->F[9 * i], h->V0[i], h->g[i]);
+        mstore(tau, &vtau[9 * i]);
+    }
+
+    // particlesToGridWithForceHelper<APIC, deg 2, !MLS> generic path (MpmSimulationBase.cpp:791-854)
+    const int ng = (int)h->groups.size();
+    for (uint64_t color = 0; color < 8; color++) {
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int gi = 0; gi < ng; gi++) {
+            if ((h->block_offset[gi] & 7) != color) continue;
+            for (int idx = h->groups[gi].first; idx <= h->groups[gi].second; idx++) {
+                int i = h->order[idx];
+                const double* Xp = &h->x[3 * i];
+                double mass = h->m[i];
+                double momentum[3] = { mass * h->v[3 * i], mass * h->v[3 * i + 1], mass * h->v[3 * i + 2] };
+                double Cm[3][3];
+                for (int r = 0; r < 3; r++)
+                    for (int c = 0; c < 3; c++) Cm[r][c] = mass * h->C[9 * i + 3 * r + c];
+                const double* S = &vtau[9 * i];
+                Stencil s;
+                makeStencil(h, Xp, s);
+                iterateKernel(h, s, [&](const int* node, double w, const double* dw, double, Node& g) {
+                    double xi_xp[3];
+                    for (int a = 0; a < 3; a++) xi_xp[a] = (double)node[a] * dx - Xp[a];
+                    double vd[3], fd[3];
+                    for (int r = 0; r < 3; r++) {
+                        // (velocity_density * [xi-xp; 1]) * w, stress_density * [dw; w] with fp = 0
+                        vd[r] = (Cm[r][0] * xi_xp[0] + Cm[r][1] * xi_xp[1] + Cm[r][2] * xi_xp[2] + momentum[r]) * w;
+                        fd[r] = S[3 * r + 0] * dw[0] + S[3 * r + 1] * dw[1] + S[3 * r + 2] * dw[2] + (-0.0) * w;
+                    }
+                    g.m += mass * w;
+                    for (int r = 0; r < 3; r++) {
+                        g.v[r] += vd[r];
+                        g.new_v[r] -= fd[r];
+                    }
+                });
+            }
+        }
+    }
+    // getNumNodes (MpmGrid.h:195-208) and v /= m (MpmSimulationBase.cpp:655-666)
+    {
+        int64_t cnt = 0;
+        for (size_t b = 0; b < h->block_pages.size(); b++)
+            for (int e = 0; e < h->NB; e++) {
+                Node& g = h->nodes[b * h->NB + e];
+                if (g.m != 0) g.idx = cnt++;
+            }
+#pragma omp parallel for schedule(static)
+        for (int64_t q = 0; q < (int64_t)h->nodes.size(); q++) {
+            Node& g = h->nodes[q];
+            if (g.idx < 0) continue;
+            for (int a = 0; a < 3; a++) g.v[a] /= g.m;
+        }
+    }
+
+    // AnisoMPM explicit damage (AnisoFractureSimulation.h:129-141): computeLaplacians then updateDamage
+    if (h->any_damage) {
+        for (uint64_t color = 0; color < 8; color++) {
+#pragma omp parallel for schedule(dynamic, 4)
+            for (int gi = 0; gi < ng; gi++) {
+                if ((h->block_offset[gi] & 7) != color) continue;
+                for (int idx = h->groups[gi].first; idx <= h->groups[gi].second; idx++) {
+                    int i = h->order[idx];
+                    double dp = matOf(h, i).has_damage ? h->d[i] : 0.0;
+                    Stencil s;
+                    makeStencil(h, &h->x[3 * i], s);
+                    iterateKernel(h, s, [&](const int*, double w, const double*, double, Node& g) {
+                        g.pfm += w;
+                        g.pf += dp * w;
+                    });
+                }
+            }
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t q = 0; q < (int64_t)h->nodes.size(); q++) {
+            Node& g = h->nodes[q];
+            if (g.idx >= 0) g.pf /= g.pfm;
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++) {
+            Stencil s;
+            makeStencil(h, &h->x[3 * i], s);
+            double lap = 0;
+            iterateKernel(h, s, [&](const int*, double, const double*, double L, Node& g) {
+                if (g.idx >= 0) lap += g.pf * L;
+            });
+            h->lap[i] = lap;
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++) {
+            const zmpm_material& mt = matOf(h, i);
+            if (!mt.has_damage || mt.model != ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) continue;
+            damageUpdate(mt, mload(&h->F[9 * i]), h->lap[i], dt, h->d[i], h->g[i]);
+        }
+    }
+
+    // gridVelocityExplicitUpdate (MpmSimulationBase.cpp:1179-1228) with
+    // multiObjectCollision (CollisionObject.cpp:66-105) at time t
+    {
+        double dvg[3] = { h->desc.gravity[0] * dt, h->desc.gravity[1] * dt, h->desc.gravity[2] * dt };
+        const std::vector<Collider>& cols = h->colliders;
+        const int NB = h->NB;
+        const Layout L = h->L;
+#pragma omp parallel for schedule(static)
+        for (int64_t b = 0; b < (int64_t)h->block_pages.size(); b++) {
+            for (int e = 0; e < NB; e++) {
+                Node& g = h->nodes[(size_t)b * NB + e];
+                if (g.idx < 0) continue;
+                uint64_t off = (h->block_pages[b] << 12) | ((uint64_t)e << L.data_bits);
+                int node[3] = { (int)bitPack(off, L.xmask), (int)bitPack(off, L.ymask), (int)bitPack(off, L.zmask) };
+                double xi[3] = { node[0] * dx, node[1] * dx, node[2] * dx };
+                double dt_over_mi = dt / g.m;
+                double vi[3];
+                for (int a = 0; a < 3; a++) vi[a] = g.v[a] + dvg[a] + g.new_v[a] * dt_over_mi;
+                for (const Collider& co : cols) {
+                    if (co.c.type == ZMPM_GHOST) continue;
+                    // detectAndResolveCollision (CollisionObject.cpp:341-406) with R = I, s = 1, omega = 0
+                    double bt[3], X[3];
+                    for (int a = 0; a < 3; a++) {
+                        bt[a] = co.c.b0[a] + co.c.dbdt[a] * t;
+                        X[a] = xi[a] - bt[a];
+                    }
+                    double phi, nrm[3] = { 0, 0, 0 };
+                    if (co.c.shape == ZMPM_SHAPE_HALF_SPACE) {
+                        phi = co.n[0] * (X[0] - co.c.origin[0]) + co.n[1] * (X[1] - co.c.origin[1]) + co.n[2] * (X[2] - co.c.origin[2]);
+                        for (int a = 0; a < 3; a++) nrm[a] = co.n[a];
+                    }
+                    else {
+                        // AnalyticBox::signedDistancePrimitive (AnalyticLevelSet.cpp:562-580)
+                        double dd[3], Xp[3];
+                        for (int a = 0; a < 3; a++) {
+                            double c = (co.c.box_min[a] + co.c.box_max[a]) / 2;
+                            double he = (co.c.box_max[a] - co.c.box_min[a]) / 2;
+                            Xp[a] = X[a] - c;
+                            dd[a] = std::fabs(Xp[a]) - he;
+                        }
+                        int am = 0;
+                        for (int a = 1; a < 3; a++)
+                            if (dd[a] > dd[am]) am = a;
+                        double q2 = 0;
+                        for (int a = 0; a < 3; a++) q2 += dd[a] > 0 ? dd[a] * dd[a] : 0.0;
+                        phi = std::min(dd[am], 0.0) + std::sqrt(q2);
+                        if (q2 > 0) {
+                            double qn = std::sqrt(q2);
+                            for (int a = 0; a < 3; a++) nrm[a] = (dd[a] > 0 ? dd[a] / qn : 0.0) * (Xp[a] >= 0 ? 1.0 : -1.0);
+                        }
+                        else
+                            nrm[am] = Xp[am] >= 0 ? 1.0 : -1.0;
+                    }
+                    if (!(phi <= 0)) continue;
+                    const double* vo = co.c.dbdt;
+                    if (co.c.type == ZMPM_STICKY) {
+                        for (int a = 0; a < 3; a++) vi[a] = vo[a];
+                        break; // the first sticky hit ends the loop
+                    }
+                    double vr[3] = { vi[0] - vo[0], vi[1] - vo[1], vi[2] - vo[2] };
+                    double dot = vr[0] * nrm[0] + vr[1] * nrm[1] + vr[2] * nrm[2];
+                    bool project = (co.c.type == ZMPM_SLIP) || (dot < 0);
+                    if (project) {
+                        for (int a = 0; a < 3; a++) vr[a] -= nrm[a] * dot;
+                        if (co.c.friction != 0 && dot < 0) {
+                            double vn = std::sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+                            if (-dot * co.c.friction < vn) {
+                                for (int a = 0; a < 3; a++) vr[a] += (vr[a] / vn) * dot * co.c.friction;
+                            }
+                            else
+                                vr[0] = vr[1] = vr[2] = 0;
+                        }
+                    }
+                    for (int a = 0; a < 3; a++) vi[a] = vr[a] + vo[a];
+                }
+                for (int a = 0; a < 3; a++) g.new_v[a] = vi[a];
+            }
+        }
+    }
+
+    // gridToParticlesHelper<APIC, deg 2, !MLS> generic path (MpmSimulationBase.cpp:1412-1478)
+    const double dx2 = dx * dx;
+    const double D_inverse = 4 / dx2;
+    const double r = h->desc.apic_rpic_ratio;
+    int cfl_flag = 0;
+    std::vector<double> gradV((size_t)n * 9);
+#pragma omp parallel for schedule(static) reduction(| : cfl_flag)
+    for (int64_t i = 0; i < n; i++) {
+        double* Xp = &h->x[3 * i];
+        double picV[3] = { 0, 0, 0 }, Bp[3][3] = {}, gv[3][3] = {};
+        Stencil s;
+        makeStencil(h, Xp, s);
+        iterateKernel(h, s, [&](const int* node, double w, const double* dw, double, Node& g) {
+            double xi_xp[3];
+            for (int a = 0; a < 3; a++) xi_xp[a] = (double)node[a] * dx - Xp[a];
+            for (int a = 0; a < 3; a++) picV[a] += w * g.new_v[a];
+            for (int a = 0; a < 3; a++) {
+                double wv = w * g.new_v[a];
+                for (int c = 0; c < 3; c++) {
+                    Bp[a][c] += wv * xi_xp[c];
+                    gv[a][c] += g.new_v[a] * dw[c];
+                }
+            }
+        });
+        for (int a = 0; a < 3; a++) h->v[3 * i + a] = picV[a];
+        for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++) {
+                double CC = Bp[a][c] * D_inverse, CCt = Bp[c][a] * D_inverse;
+                h->C[9 * i + 3 * a + c] = ((r + 1) * 0.5) * CC + ((r - 1) * 0.5) * CCt;
+                gradV[9 * i + 3 * a + c] = gv[a][c];
+            }
+        double inc[3] = { dt * picV[0], dt * picV[1], dt * picV[2] };
+        for (int a = 0; a < 3; a++) Xp[a] += inc[a];
+        double incn = inc[0] * inc[0] + inc[1] * inc[1] + inc[2] * inc[2];
+        if (incn > dx2) cfl_flag |= 1;
+    }
+
+    // evolveStrain: F <- (I + dt gradV) F (FBasedMpmForceHelper.cpp:93-108);
+    // J <- (1 + dt tr gradV) J (JBasedMpmForceHelper.cpp:79-94)
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        const double* gv = &gradV[9 * i];
+        double* Fp = &h->F[9 * i];
+        if (matOf(h, i).model == ZMPM_MODEL_EQUATION_OF_STATE) {
+            Fp[0] = (1 + dt * (gv[0] + gv[4] + gv[8])) * Fp[0];
+            continue;
+        }
+        M3 A;
+        for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++) A.a[a][c] = (a == c ? 1.0 : 0.0) + dt * gv[3 * a + c];
+        M3 Fn = mmul(A, mload(Fp));
+        mstore(Fn, Fp);
+    }
+    h->stepped = true;
+    if (cfl_flag && h->latched == ZMPM_OK) h->latched = ZMPM_E_CFL;
+    return ZMPM_OK;
+}
+
+} // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+int32_t zmpm_abi_version(void) { return ZMPM_ABI_VERSION; }
+const char* zmpm_build_info(void) { return "zmpm FP64 CPU oracle (test infrastructure)"; }
+
+zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
+{
+    if (!desc || !out) return ZMPM_E_ARG;
+    int dn = desc->domain_nodes;
+    if (dn < 64 || dn > 1024 || (dn & (dn - 1)) || !(desc->dx > 0)) return ZMPM_E_ARG;
+    if (desc->precision != ZMPM_PREC_FP32 && desc->precision != ZMPM_PREC_FP64) return ZMPM_E_ARG;
+    auto* h = new zmpm_context();
+    h->desc = *desc;
+    h->L = makeLayout(desc->precision == ZMPM_PREC_FP32 ? 6 : 7);
+    h->NB = 1 << h->L.block_bits;
+    *out = h;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_destroy(zmpm_handle h)
+{
+    delete h;
+    return ZMPM_OK;
+}
+
+const char* zmpm_last_error(zmpm_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material* mats)
+{
+    if (!h || count <= 0 || count > 256 || !mats) return fail(h, ZMPM_E_ARG, "bad material table");
+    h->mats.assign(mats, mats + count);
+    h->any_damage = false;
+    for (auto& m : h->mats) h->any_damage = h->any_damage || m.has_damage;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_add_collider(zmpm_handle h, const zmpm_collider* c, int32_t* id_out)
+{
+    if (!h || !c) return ZMPM_E_ARG;
+    if (h->colliders.size() >= 16) return fail(h, ZMPM_E_ARG, "too many colliders");
+    Collider co;
+    co.c = *c;
+    double nn = std::sqrt(c->normal[0] * c->normal[0] + c->normal[1] * c->normal[1] + c->normal[2] * c->normal[2]);
+    for (int a = 0; a < 3; a++) co.n[a] = nn > 0 ? c->normal[a] / nn : 0.0; // HalfSpace ctor normalises
+    h->colliders.push_back(co);
+    if (id_out) *id_out = (int32_t)h->colliders.size() - 1;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_clear_colliders(zmpm_handle h)
+{
+    if (!h) return ZMPM_E_ARG;
+    h->colliders.clear();
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
+{
+    if (!h || !p || p->n < 0) return ZMPM_E_ARG;
+    if (p->n > 0 && (!p->x || !p->v || !p->C || !p->F || !p->m || !p->V0 || !p->material))
+        return fail(h, ZMPM_E_ARG, "missing particle array");
+    if (h->mats.empty()) return fail(h, ZMPM_E_ARG, "set materials before upload");
+    const int64_t n = p->n;
+    for (int64_t i = 0; i < n; i++)
+        if (p->material[i] < 0 || p->material[i] >= (int32_t)h->mats.size()) return fail(h, ZMPM_E_ARG, "material index out of range");
+    h->n = n;
+    h->x.assign(p->x, p->x + 3 * n);
+    h->v.assign(p->v, p->v + 3 * n);
+    h->C.assign(p->C, p->C + 9 * n);
+    h->F.assign(p->F, p->F + 9 * n);
+    h->m.assign(p->m, p->m + n);
+    h->V0.assign(p->V0, p->V0 + n);
+    h->mat.assign(p->material, p->material + n);
+    h->d.assign(n, 0.0);
+    h->g.assign(n, 1.0);
+    h->lap.assign(n, 0.0);
+    if (p->d) h->d.assign(p->d, p->d + n);
+    if (p->g) h->g.assign(p->g, p->g + n);
+    h->latched = ZMPM_OK;
+    h->stepped = false;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
+{
+    if (!h || !p || p->n != h->n) return fail(h, ZMPM_E_ARG, "particle count mismatch");
+    const int64_t n = h->n;
+    if (p->x) std::copy(h->x.begin(), h->x.end(), p->x);
+    if (p->v) std::copy(h->v.begin(), h->v.end(), p->v);
+    if (p->C) std::copy(h->C.begin(), h->C.end(), p->C);
+    if (p->F) std::copy(h->F.begin(), h->F.end(), p->F);
+    if (p->m) std::copy(h->m.begin(), h->m.end(), p->m);
+    if (p->V0) std::copy(h->V0.begin(), h->V0.end(), p->V0);
+    if (p->material) std::copy(h->mat.begin(), h->mat.end(), p->material);
+    if (p->d) std::copy(h->d.begin(), h->d.end(), p->d);
+    if (p->g) std::copy(h->g.begin(), h->g.end(), p->g);
+    if (p->laplacian) std::copy(h->lap.begin(), h->lap.end(), p->laplacian);
+    (void)n;
+    zmpm_status s = (zmpm_status)h->latched;
+    return s;
+}
+
+zmpm_status zmpm_substep(zmpm_handle h, double dt, double t)
+{
+    if (!h) return ZMPM_E_ARG;
+    if (h->latched != ZMPM_OK && h->latched != ZMPM_E_CFL) return (zmpm_status)h->latched;
+    return substep(h, dt, t);
+}
+
+zmpm_status zmpm_run(zmpm_handle h, int32_t steps, double dt, double t0)
+{
+    if (!h || steps < 0) return ZMPM_E_ARG;
+    for (int32_t s = 0; s < steps; s++) {
+        zmpm_status st = zmpm_substep(h, dt, t0 + s * dt);
+        if (st != ZMPM_OK) return st;
+    }
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_synchronize(zmpm_handle h)
+{
+    if (!h) return ZMPM_E_ARG;
+    return (zmpm_status)h->latched;
+}
+
+zmpm_status zmpm_get_binning(zmpm_handle h, int64_t* n_groups, uint64_t* keys, int32_t* order, int32_t* groups, uint64_t* block_offset)
+{
+    if (!h || !n_groups) return ZMPM_E_ARG;
+    *n_groups = (int64_t)h->groups.size();
+    if (keys) std::copy(h->keys.begin(), h->keys.end(), keys);
+    if (order) std::copy(h->order.begin(), h->order.end(), order);
+    if (groups)
+        for (size_t gi = 0; gi < h->groups.size(); gi++) {
+            groups[2 * gi] = h->groups[gi].first;
+            groups[2 * gi + 1] = h->groups[gi].second;
+        }
+    if (block_offset) std::copy(h->block_offset.begin(), h->block_offset.end(), block_offset);
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_get_grid(zmpm_handle h, int64_t* n_nodes, int32_t* coords, double* m, double* v, double* new_v, double* pf)
+{
+    if (!h || !n_nodes) return ZMPM_E_ARG;
+    struct Rec {
+        int c[3];
+        const Node* g;
+    };
+    std::vector<Rec> recs;
+    for (size_t b = 0; b < h->block_pages.size(); b++)
+        for (int e = 0; e < h->NB; e++) {
+            const Node& g = h->nodes[b * h->NB + e];
+            if (g.idx < 0) continue;
+            uint64_t off = (h->block_pages[b] << 12) | ((uint64_t)e << h->L.data_bits);
+            Rec r { { (int)bitPack(off, h->L.xmask), (int)bitPack(off, h->L.ymask), (int)bitPack(off, h->L.zmask) }, &g };
+            recs.push_back(r);
+        }
+    std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) {
+        if (a.c[0] != b.c[0]) return a.c[0] < b.c[0];
+        if (a.c[1] != b.c[1]) return a.c[1] < b.c[1];
+        return a.c[2] < b.c[2];
+    });
+    *n_nodes = (int64_t)recs.size();
+    for (size_t q = 0; q < recs.size(); q++) {
+        const Node& g = *recs[q].g;
+        if (coords)
+            for (int a = 0; a < 3; a++) coords[3 * q + a] = recs[q].c[a];
+        if (m) m[q] = g.m;
+        if (v)
+            for (int a = 0; a < 3; a++) v[3 * q + a] = g.v[a];
+        if (new_v)
+            for (int a = 0; a < 3; a++) new_v[3 * q + a] = g.new_v[a];
+        if (pf) pf[q] = h->any_damage ? g.pf : 0.0;
+    }
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_set_timing(zmpm_handle h, int32_t)
+{
+    return h ? ZMPM_OK : ZMPM_E_ARG;
+}
+
+zmpm_status zmpm_get_timings(zmpm_handle h, int32_t* count, const char**, double*)
+{
+    if (!h || !count) return ZMPM_E_ARG;
+    *count = 0;
+    return ZMPM_OK;
+}
+
+// ------------------------------------------------ unit entry points (pinning)
+uint64_t zmpm_oracle_linear_offset(int32_t log2_struct, int32_t i, int32_t j, int32_t k)
+{
+    return linearOffset(makeLayout(log2_struct), i, j, k);
+}
+
+void zmpm_oracle_masks(int32_t log2_struct, uint64_t* xyz)
+{
+    Layout L = makeLayout(log2_struct);
+    xyz[0] = L.xmask;
+    xyz[1] = L.ymask;
+    xyz[2] = L.zmask;
+}
+
+void zmpm_oracle_bspline(double x, int32_t* base, double* w, double* dw, double* ddw)
+{
+    int b;
+    bspline(x, b, w, dw, ddw);
+    *base = b;
+}
+
+int32_t zmpm_oracle_svd(const double* A, double* U, double* sigma, double* V)
+{
+    M3 a = mload(A), u, v;
+    int c = svd3(a, u, sigma, v, kSvdTol);
+    mstore(u, U);
+    mstore(v, V);
+    return c;
+}
+
+void zmpm_oracle_first_piola(const zmpm_material* mat, const double* F, double g, double* P)
+{
+    mstore(firstPiola(*mat, mload(F), g), P);
+}
+
+double zmpm_oracle_psi(const zmpm_material* mat, const double* F, double g)
+{
+    return psiOf(*mat, mload(F), g);
+}
+
+void zmpm_oracle_vtau(const zmpm_material* mat, const double* F, double V0, double g, double* vtau)
+{
+    mstore(vtauOf(*mat, F, V0, g), vtau);
+}
+
+void zmpm_oracle_damage(const zmpm_material* mat, const double* F, double lap, double dt, double* d, double* g)
+{
+    damageUpdate(*mat, mload(F), lap, dt, *d, *g);
+}
+
+void zmpm_oracle_sym_eigen(const double* S, double* lam, double* vecs)
+{
+    M3 vv;
+    symEigen(mload(S), lam, vv);
+    mstore(vv, vecs);
+}
+
+} // extern "C"

@@ -1,0 +1,219 @@
+"""GPU parity: the HIP library against the FP64 CPU oracle on identical seeded inputs.
+
+Bars: binning (keys, particle order, groups, block offsets) bit-exact; FP64 plain
+form within 1e-9 normwise of the oracle; FP32 form within the stated tolerance
+of the FP64 oracle; FP32 form bit-reproducible run to run.
+"""
+import numpy as np
+import pytest
+
+from ziran2020_amd import _abi as A, scenes
+from helpers import normwise, run
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-9     # FP64 plain form vs FP64 oracle (atomic order only)
+FP32_TOL = 1e-4     # FP32 form vs FP64 oracle, normwise per field
+
+
+def small_cube(seed=1, cells=8):
+    return scenes.elastic_cube(n_side_cells=cells, seed=seed)
+
+
+def small_slab(n=20000, sigma_crit=20.0, seed=2):
+    # low critical stress so the phase field actually evolves within a few steps
+    return scenes.damage_slab(n=n, dx=1.0 / 64, size=(0.5, 0.15, 0.15), domain_nodes=128,
+                              sigma_crit=sigma_crit, pull_speed=0.5, seed=seed)
+
+
+def compare_binning(a, b):
+    assert np.array_equal(a["keys"], b["keys"])
+    assert np.array_equal(a["order"], b["order"])
+    assert np.array_equal(a["groups"], b["groups"])
+    assert np.array_equal(a["block_offset"], b["block_offset"])
+
+
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_binning_bit_exact(gpu, oracle, precision):
+    sc = small_cube(cells=12)
+    g = run(gpu, sc, precision, 1)
+    o = run(oracle, sc, precision, 1)
+    compare_binning(g.binning(), o.binning())
+
+
+def test_binning_bit_exact_after_motion(gpu, oracle):
+    # the sort of a later substep runs on positions both sides computed (FP64)
+    sc = small_cube(cells=10, seed=7)
+    sc.v[:] = np.random.default_rng(3).normal(scale=0.3, size=sc.v.shape)
+    g = run(gpu, sc, A.PREC_FP64, 5)
+    o = run(oracle, sc, A.PREC_FP64, 5)
+    xg, xo = g.download()["x"], o.download()["x"]
+    assert normwise(xg, xo) < 1e-12
+    g.substep(sc.dt, 5 * sc.dt)
+    o.substep(sc.dt, 5 * sc.dt)
+    compare_binning(g.binning(), o.binning())
+
+
+def test_binning_scattered_blocks(gpu, oracle):
+    # sparse random particles: many single-particle blocks and crowded cells
+    rng = np.random.default_rng(11)
+    sc = small_cube(cells=4)
+    n = 3000
+    x = rng.uniform(0.2, 0.8, size=(n, 3))
+    x[:500] = x[0] + rng.uniform(0, 1e-3, size=(500, 3))  # 500 particles in one cell
+    sc.x, sc.v, sc.C = x, np.zeros((n, 3)), np.zeros((n, 9))
+    sc.F, sc.V0 = scenes.identity_F(n), np.full(n, 1e-7)
+    sc.material, sc.rho = np.zeros(n, np.int32), np.full(n, 1000.0)
+    sc.m = sc.rho * sc.V0
+    for prec in (A.PREC_FP64, A.PREC_FP32):
+        compare_binning(run(gpu, sc, prec, 1).binning(), run(oracle, sc, prec, 1).binning())
+
+
+def test_grid_fp64_one_step(gpu, oracle):
+    sc = small_cube(cells=12)
+    sc.v[:] = np.random.default_rng(5).normal(scale=0.1, size=sc.v.shape)
+    g = run(gpu, sc, A.PREC_FP64, 1).grid()
+    o = run(oracle, sc, A.PREC_FP64, 1).grid()
+    assert np.array_equal(g["coords"], o["coords"])
+    for k in ("m", "v", "new_v"):
+        assert normwise(g[k], o[k]) < 1e-12, k
+
+
+def test_particles_fp64_elastic_cube(gpu, oracle):
+    sc = small_cube(cells=12)
+    g = run(gpu, sc, A.PREC_FP64, 30).download()
+    o = run(oracle, sc, A.PREC_FP64, 30).download()
+    for k in ("x", "v", "C", "F"):
+        assert normwise(g[k], o[k]) < FP64_TOL, k
+
+
+def test_particles_fp32_elastic_cube(gpu, oracle):
+    sc = small_cube(cells=12)
+    g = run(gpu, sc, A.PREC_FP32, 30).download()
+    o = run(oracle, sc, A.PREC_FP64, 30).download()
+    for k in ("x", "v", "C", "F"):
+        assert normwise(g[k], o[k]) < FP32_TOL, k
+
+
+def test_fp32_deterministic(gpu):
+    sc = small_cube(cells=10)
+    sc.v[:] = np.random.default_rng(9).normal(scale=0.2, size=sc.v.shape)
+    a = run(gpu, sc, A.PREC_FP32, 10).download()
+    b = run(gpu, sc, A.PREC_FP32, 10).download()
+    for k in ("x", "v", "C", "F"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_damage_slab(gpu, oracle, precision):
+    sc = small_slab()
+    steps = 20
+    g = run(gpu, sc, precision, steps).download()
+    o = run(oracle, sc, A.PREC_FP64, steps).download()
+    assert o["d"].max() > 0.01, "scene must exercise the damage update"
+    tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+    for k in ("x", "v", "F", "d", "g"):
+        assert normwise(g[k], o[k]) < (tol if k != "d" else 10 * tol), k
+
+
+def test_damage_grid_phase_field(gpu, oracle):
+    sc = small_slab()
+    sc_d = np.random.default_rng(1).uniform(0, 0.5, sc.n)
+    gs = sc.make(gpu, A.PREC_FP64)
+    os_ = sc.make(oracle, A.PREC_FP64)
+    for s in (gs, os_):
+        s.upload(sc.x, sc.v, sc.C, sc.F, sc.m, sc.V0, sc.material, d=sc_d)
+        s.substep(sc.dt, 0.0)
+    g, o = gs.grid(), os_.grid()
+    assert np.array_equal(g["coords"], o["coords"])
+    assert normwise(g["pf"], o["pf"]) < 1e-12
+    gd, od = gs.download(), os_.download()
+    assert normwise(gd["laplacian"], od["laplacian"]) < 1e-9
+    assert normwise(gd["d"], od["d"]) < 1e-9
+
+
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_two_material(gpu, oracle, precision):
+    sc = scenes.two_material(n=20000, dx=1.0 / 64, domain_nodes=128)
+    g = run(gpu, sc, precision, 10).download()
+    o = run(oracle, sc, A.PREC_FP64, 10).download()
+    tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+    for k in ("x", "v", "F"):
+        assert normwise(g[k], o[k]) < tol, k
+
+
+def test_batched_scenes_small(gpu, oracle):
+    sc = scenes.batched_scenes(n_scenes=8, n_range=(500, 4000), domain_nodes=256)
+    g = run(gpu, sc, A.PREC_FP64, 5).download()
+    o = run(oracle, sc, A.PREC_FP64, 5).download()
+    for k in ("x", "v", "F", "d"):
+        assert normwise(g[k], o[k]) < FP64_TOL, k
+
+
+def test_sparse_spheres_small(gpu, oracle):
+    sc = scenes.sparse_spheres(n=30000, n_spheres=5, dx=1.0 / 64, domain_nodes=128)
+    g = run(gpu, sc, A.PREC_FP32, 10).download()
+    o = run(oracle, sc, A.PREC_FP64, 10).download()
+    for k in ("x", "v", "F"):
+        assert normwise(g[k], o[k]) < FP32_TOL, k
+
+
+def test_slip_and_separate_colliders(gpu, oracle):
+    sc = small_cube(cells=8)
+    sc.v[:, 0] = 1.0
+    sc.colliders = [A.half_space((0, 8 * sc.dx, 0), (0, 1, 0), A.SLIP, friction=0.3),
+                    A.half_space((30 * sc.dx, 0, 0), (-1, 0, 0), A.SEPARATE),
+                    A.box((0, 0, 0), (20 * sc.dx, 30 * sc.dx, 20 * sc.dx), A.SLIP, b0=(0.01, 0, 0),
+                          dbdt=(0.2, 0, 0))]
+    g = run(gpu, sc, A.PREC_FP64, 10).download()
+    o = run(oracle, sc, A.PREC_FP64, 10).download()
+    for k in ("x", "v"):
+        assert normwise(g[k], o[k]) < FP64_TOL, k
+
+
+def test_empty_scene(gpu):
+    sc = small_cube(cells=4)
+    sim = sc.make(gpu, A.PREC_FP32)
+    z3 = np.zeros((0, 3))
+    sim.upload(z3, z3, np.zeros((0, 9)), np.zeros((0, 9)), np.zeros(0), np.zeros(0), np.zeros(0, np.int32))
+    sim.run(3, 1e-4)
+    assert sim.synchronize() == A.OK
+    g = sim.binning()
+    assert g["keys"].size == 0
+
+
+def step_status(sim, dt):
+    """Status of one substep: reported by the call itself or latched until the next sync."""
+    st = sim.L.zmpm_substep(sim.h, dt, 0.0)
+    return st if st != A.OK else sim.synchronize(check=False)
+
+
+def test_cfl_violation_reported(gpu):
+    sc = small_cube(cells=4)
+    sc.v[:] = (200.0, 0.0, 0.0)  # 200 m/s * 1e-4 s > dx = 1/64 m
+    sim = sc.make(gpu, A.PREC_FP64)
+    assert step_status(sim, sc.dt) == A.E_CFL
+
+
+def test_domain_and_nan_rejected(gpu):
+    sc = small_cube(cells=4)
+    sc.x[3] = (-0.5, 0.3, 0.3)
+    sim = sc.make(gpu, A.PREC_FP32)
+    assert step_status(sim, sc.dt) == A.E_DOMAIN
+    sc = small_cube(cells=4)
+    sc.x[5, 1] = np.nan
+    sim = sc.make(gpu, A.PREC_FP64)
+    assert step_status(sim, sc.dt) == A.E_NAN
+
+
+def test_full_size_damage_slab_conservation(gpu):
+    """1M-particle FP32 workload: P2G conserves mass and the fixed-point sums are exact."""
+    sc = scenes.damage_slab()
+    sim = sc.make(gpu, A.PREC_FP32)
+    sim.run(3, sc.dt)
+    grid = sim.grid()
+    mp = sim.download()["m"].sum()
+    assert abs(grid["m"].sum() - mp) / mp < 1e-6
+    b = sim.binning()
+    assert np.all(np.diff(b["keys"].astype(np.uint64)) > 0)  # strictly increasing keys
+    assert np.array_equal(np.sort(b["order"]), np.arange(sc.n))

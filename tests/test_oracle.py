@@ -1,0 +1,460 @@
+"""Pins the FP64 CPU oracle (oracle/zmpm_oracle.cpp) without running the reference.
+
+Every check is chosen so that a plausible restatement mistake fails it: closed
+forms, invariants, finite differences of the strain energy, brute force on
+tiny inputs and the SPGrid known answers in tests/golden/spgrid_kat.json.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from ziran2020_amd import _abi as A, scenes
+from helpers import normwise
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DP = C.POINTER(C.c_double)
+
+
+@pytest.fixture(scope="module")
+def O(oracle):
+    L = oracle.lib
+    L.zmpm_oracle_linear_offset.restype = C.c_uint64
+    L.zmpm_oracle_linear_offset.argtypes = [C.c_int32] * 4
+    L.zmpm_oracle_masks.argtypes = [C.c_int32, C.POINTER(C.c_uint64)]
+    L.zmpm_oracle_bspline.argtypes = [C.c_double, C.POINTER(C.c_int32), DP, DP, DP]
+    L.zmpm_oracle_svd.restype = C.c_int32
+    L.zmpm_oracle_svd.argtypes = [DP, DP, DP, DP]
+    L.zmpm_oracle_first_piola.argtypes = [C.POINTER(A.Material), DP, C.c_double, DP]
+    L.zmpm_oracle_psi.restype = C.c_double
+    L.zmpm_oracle_psi.argtypes = [C.POINTER(A.Material), DP, C.c_double]
+    L.zmpm_oracle_vtau.argtypes = [C.POINTER(A.Material), DP, C.c_double, C.c_double, DP]
+    L.zmpm_oracle_damage.argtypes = [C.POINTER(A.Material), DP, C.c_double, C.c_double, DP, DP]
+    L.zmpm_oracle_sym_eigen.argtypes = [DP, DP, DP]
+    return L
+
+
+def p(a):
+    return a.ctypes.data_as(DP)
+
+
+# ------------------------------------------------------------------ SPGrid
+def pdep(v, mask):
+    """Independent parallel bit deposit (BMI2 _pdep_u64 semantics)."""
+    out, bit = 0, 0
+    for b in range(64):
+        if mask >> b & 1:
+            if v >> bit & 1:
+                out |= 1 << b
+            bit += 1
+    return out
+
+
+def test_masks_and_known_answers(O):
+    kat = json.load(open(os.path.join(HERE, "golden", "spgrid_kat.json")))
+    for layout in kat["layouts"]:
+        m = (C.c_uint64 * 3)()
+        O.zmpm_oracle_masks(layout["log2_struct"], m)
+        assert [hex(v) for v in m] == layout["masks"]
+        for (i, j, k), off in layout["offsets"]:
+            got = O.zmpm_oracle_linear_offset(layout["log2_struct"], i, j, k)
+            assert got == int(off, 16), (i, j, k)
+
+
+@pytest.mark.parametrize("log2_struct", [6, 7])
+def test_linear_offset_is_pdep(O, log2_struct):
+    m = (C.c_uint64 * 3)()
+    O.zmpm_oracle_masks(log2_struct, m)
+    rng = np.random.default_rng(0)
+    for i, j, k in rng.integers(0, 1024, size=(200, 3)):
+        want = pdep(int(i), m[0]) | pdep(int(j), m[1]) | pdep(int(k), m[2])
+        assert O.zmpm_oracle_linear_offset(log2_struct, int(i), int(j), int(k)) == want
+
+
+def test_block_shapes(O):
+    # float GridState (64 B): 4x4x4 blocks; double (128 B): 2x4x4 (x,y,z)
+    for log2, shape in ((6, (4, 4, 4)), (7, (2, 4, 4))):
+        pages = set()
+        for i in range(shape[0]):
+            for j in range(shape[1]):
+                for k in range(shape[2]):
+                    pages.add(O.zmpm_oracle_linear_offset(log2, i, j, k) >> 12)
+        assert pages == {0}
+        for a in range(3):
+            c = [0, 0, 0]
+            c[a] = shape[a]
+            assert O.zmpm_oracle_linear_offset(log2, *c) >> 12 != 0
+
+
+# ---------------------------------------------------------------- B-spline
+def bspline(O, x):
+    b = C.c_int32()
+    w, dw, ddw = np.zeros(3), np.zeros(3), np.zeros(3)
+    O.zmpm_oracle_bspline(x, C.byref(b), p(w), p(dw), p(ddw))
+    return b.value, w, dw, ddw
+
+
+def test_bspline_invariants(O):
+    rng = np.random.default_rng(1)
+    for x in rng.uniform(3.0, 200.0, 500):
+        b, w, dw, ddw = bspline(O, x)
+        assert b == int(np.floor(x - 0.5))
+        off = b + np.arange(3) - x
+        assert abs(w.sum() - 1) < 1e-14                 # partition of unity
+        assert abs((w * off).sum()) < 1e-12             # linear reproduction
+        assert abs((w * off * off).sum() - 0.25) < 1e-12  # D = dx^2/4 (quadratic)
+        assert abs(dw.sum()) < 1e-14                    # gradients of a constant
+        assert abs((dw * off).sum() - 1) < 1e-12        # gradient of a linear field
+        assert list(ddw) == [1, -2, 1]
+
+
+def test_bspline_closed_form(O):
+    # d0 = x - base = 1.0 -> (1/8, 3/4, 1/8); d0 = 0.5 -> (1/2, 1/2, 0)
+    b, w, dw, _ = bspline(O, 10.0)
+    assert b == 9 and np.allclose(w, [0.125, 0.75, 0.125], atol=0) and np.allclose(dw, [-0.5, 0, 0.5])
+    b, w, dw, _ = bspline(O, 10.5)
+    assert b == 10 and np.allclose(w, [0.5, 0.5, 0.0], atol=0) and np.allclose(dw, [-1, 1, 0])
+
+
+# ---------------------------------------------------------------------- SVD
+def svd(O, M):
+    U, S, V = np.zeros(9), np.zeros(3), np.zeros(9)
+    it = O.zmpm_oracle_svd(p(np.ascontiguousarray(M, float).ravel()), p(U), p(S), p(V))
+    return U.reshape(3, 3), S, V.reshape(3, 3), it
+
+
+def check_svd(O, M):
+    U, S, V, _ = svd(O, M)
+    assert np.abs(U @ np.diag(S) @ V.T - M).max() < 1e-12 * max(1, np.abs(M).max())
+    for R in (U, V):
+        assert np.abs(R.T @ R - np.eye(3)).max() < 1e-13
+        assert abs(np.linalg.det(R) - 1) < 1e-13
+    assert S[0] >= abs(S[1]) - 1e-15 and abs(S[1]) >= abs(S[2]) - 1e-15
+    assert S[0] >= 0 and S[1] >= 0
+    assert np.allclose(np.sort(np.abs(S)), np.sort(np.linalg.svd(M, compute_uv=False)), rtol=1e-12, atol=1e-14)
+    assert np.sign(S[2]) == np.sign(np.linalg.det(M)) or abs(np.linalg.det(M)) < 1e-12
+
+
+def test_svd_random_and_branches(O):
+    rng = np.random.default_rng(2)
+    mats = [rng.normal(size=(3, 3)) for _ in range(300)]
+    mats += [np.eye(3), np.diag([3.0, -2.0, 1.0]), np.diag([1.0, 1.0, -1.0]), np.zeros((3, 3)),
+             np.array([[1, 2, 0], [0, 0, 3], [0, 0, 4.0]]),   # alpha_2 == 0
+             np.array([[1, 2, 0], [0, 3, 4], [0, 0, 0.0]]),   # alpha_3 == 0
+             np.array([[0, 2, 0], [0, 3, 4], [0, 0, 5.0]]),   # alpha_1 == 0
+             np.array([[1, 0, 0], [0, 3, 4], [0, 0, 5.0]]),   # beta_1 == 0
+             np.outer([1, 2, 3.0], [4, 5, 6.0])]                 # rank one
+    Q = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    mats += [Q @ np.diag([1 + 1e-9, 1.0, 1 - 1e-9]) @ Q.T, Q @ np.diag([2.0, 2.0, 2.0])]
+    for M in mats:
+        check_svd(O, M)
+
+
+# ----------------------------------------------------------- constitutive
+def mat(model, E=1e4, nu=0.3, **kw):
+    return A.make_material(model, E, nu, **kw)
+
+
+def piola(O, m, F, g=1.0):
+    P = np.zeros(9)
+    O.zmpm_oracle_first_piola(C.byref(m), p(np.ascontiguousarray(F, float).ravel()), g, p(P))
+    return P.reshape(3, 3)
+
+
+def psi(O, m, F, g=1.0):
+    return O.zmpm_oracle_psi(C.byref(m), p(np.ascontiguousarray(F, float).ravel()), g)
+
+
+def fd_piola(O, m, F, g=1.0, h=1e-6):
+    G = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            E = np.zeros((3, 3))
+            E[i, j] = h
+            G[i, j] = (psi(O, m, F + E, g) - psi(O, m, F - E, g)) / (2 * h)
+    return G
+
+
+def rand_F(rng, s=0.2):
+    return np.eye(3) + s * rng.normal(size=(3, 3))
+
+
+@pytest.mark.parametrize("model", [A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN])
+def test_piola_is_energy_gradient(O, model):
+    rng = np.random.default_rng(3)
+    m = mat(model, kx=0.0)
+    for _ in range(20):
+        F = rand_F(rng)
+        if np.linalg.det(F) <= 0.2:
+            continue
+        for g in ((1.0, 0.4) if model == A.MODEL_QR_STABLE_NEO_HOOKEAN else (1.0,)):
+            P, G = piola(O, m, F, g), fd_piola(O, m, F, g)
+            assert np.abs(P - G).max() < 1e-5 * max(1, np.abs(P).max()), (model, g)
+
+
+def test_qr_neo_hookean_fibre_stiffness_gradient(O):
+    rng = np.random.default_rng(4)
+    m = mat(A.MODEL_QR_STABLE_NEO_HOOKEAN, kx=500.0, ky=300.0)
+    for _ in range(10):
+        F = rand_F(rng)
+        if np.linalg.det(F) <= 0.2:
+            continue
+        P, G = piola(O, m, F), fd_piola(O, m, F)
+        assert np.abs(P - G).max() < 1e-5 * max(1, np.abs(P).max())
+
+
+def test_corotated_diagonal_closed_form(O):
+    m = mat(A.MODEL_FIXED_COROTATED, 2e4, 0.25)
+    for sig in ([1.2, 0.9, 1.05], [0.7, 1.3, 1.0], [2.0, 1.0, 0.5]):
+        S = np.array(sig)
+        J = S.prod()
+        want = 2 * m.mu * np.diag(S - 1) + m.lam * (J - 1) * J * np.diag(1 / S)
+        assert np.abs(piola(O, m, np.diag(S)) - want).max() < 1e-9
+
+
+@pytest.mark.parametrize("model", [A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN])
+def test_rotation_invariance(O, model):
+    # P(QF) = Q P(F) for isotropic hyperelasticity (QR-NH is left-rotation invariant too)
+    rng = np.random.default_rng(5)
+    m = mat(model)
+    for _ in range(10):
+        F = rand_F(rng)
+        Q = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        if np.linalg.det(Q) < 0:
+            Q[:, 0] *= -1
+        assert np.abs(piola(O, m, Q @ F) - Q @ piola(O, m, F)).max() < 1e-8 * max(1, np.abs(piola(O, m, F)).max())
+
+
+def test_rest_state_is_stress_free(O):
+    for model in (A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN):
+        assert np.abs(piola(O, mat(model), np.eye(3))).max() < 1e-10
+
+
+def test_neo_hookean_and_eos_kirchhoff(O):
+    m = mat(A.MODEL_NEO_HOOKEAN, 1e5, 0.4)
+    F = np.diag([1.1, 0.95, 1.02])
+    tau = np.zeros(9)
+    O.zmpm_oracle_vtau(C.byref(m), p(F.ravel().copy()), 2.0, 1.0, p(tau))
+    J = np.linalg.det(F)
+    want = 2.0 * (m.mu * F @ F.T + (m.lam * np.log(J) - m.mu) * np.eye(3))
+    assert np.abs(tau.reshape(3, 3) - want).max() < 1e-9 * np.abs(want).max()
+    e = mat(A.MODEL_EQUATION_OF_STATE, 1.0, 0.0, bulk=1e4, gamma=7.0)
+    Fj = np.zeros(9)
+    Fj[0] = 0.97
+    O.zmpm_oracle_vtau(C.byref(e), p(Fj), 0.5, 1.0, p(tau))
+    s = 0.5 * (-1e4 * (0.97 ** -7 - 1)) * 0.97
+    assert np.allclose(tau.reshape(3, 3), s * np.eye(3), rtol=1e-13)
+
+
+# ------------------------------------------------------------------ damage
+def damage(O, m, F, d, lap, dt=1e-4):
+    dd, gg = C.c_double(d), C.c_double(0.0)
+    O.zmpm_oracle_damage(C.byref(m), p(np.ascontiguousarray(F, float).ravel()), lap, dt, C.byref(dd), C.byref(gg))
+    return dd.value, gg.value
+
+
+def dmat(alpha=-0.9, sigma_crit=50.0, eta=0.1, zeta=1.0, k=1e-3, l0=0.01):
+    return mat(A.MODEL_QR_STABLE_NEO_HOOKEAN, 1e4, 0.3, has_damage=True, alphas=(alpha,), sigma_crit=sigma_crit,
+               eta=eta, zeta=zeta, residual_phase=k, l0=l0)
+
+
+def test_damage_uniaxial_closed_form(O):
+    m = dmat()
+    s, alpha = 1.01, -0.9
+    lam, mu = m.lam, m.mu
+    sxx = (lam + mu) * (s - 1)
+    syy = (s - 1) * (lam * s - mu) / s
+    contraction = (1 + alpha) ** 2 * max(sxx, 0) ** 2 + 2 * max(syy, 0) ** 2
+    phi = contraction / m.sigma_crit ** 2
+    d0, lap, dt = 0.1, 3.0, 1e-4
+    geo = d0 - m.l0 ** 2 * lap
+    diff = (1 - d0) * m.zeta * max(phi - 1, 0) - geo
+    want_d = min(d0 + dt / m.eta * max(diff, 0), 1.0)
+    d, g = damage(O, m, np.diag([s, 1, 1]), d0, lap, dt)
+    assert abs(d - want_d) < 1e-12
+    assert abs(g - ((1 - d) ** 2 * (1 - m.residual_phase) + m.residual_phase)) < 1e-14
+
+
+def test_damage_no_growth_below_threshold_and_clamp(O):
+    m = dmat(sigma_crit=1e9)
+    d, g = damage(O, m, np.diag([1.01, 1, 1]), 0.0, 0.0)
+    assert d == 0.0 and g == 1.0
+    m = dmat(sigma_crit=1e-3, eta=1e-9)
+    d, g = damage(O, m, np.diag([1.2, 1, 1]), 0.5, 0.0)
+    assert d == 1.0 and abs(g - m.residual_phase) < 1e-15
+    # compression produces no tensile stress, hence no growth (fibre along x)
+    m = dmat(sigma_crit=1.0)
+    d, _ = damage(O, m, np.diag([0.99, 0.99, 0.99]), 0.0, 0.0)
+    assert d == 0.0
+
+
+def test_damage_structural_tensor_follows_rotation(O):
+    # rotating the body rotates the fibre: phi is invariant under F -> Q F
+    m = dmat(sigma_crit=20.0)
+    rng = np.random.default_rng(6)
+    F = np.diag([1.02, 1.0, 0.99]) + 0.003 * rng.normal(size=(3, 3))
+    Q = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    if np.linalg.det(Q) < 0:
+        Q[:, 0] *= -1
+    a = damage(O, m, F, 0.05, 1.0)
+    b = damage(O, m, Q @ F, 0.05, 1.0)
+    assert abs(a[0] - b[0]) < 1e-12
+
+
+def test_sym_eigen(O):
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        M = rng.normal(size=(3, 3))
+        S = M + M.T
+        lam, V = np.zeros(3), np.zeros(9)
+        O.zmpm_oracle_sym_eigen(p(S.ravel().copy()), p(lam), p(V))
+        V = V.reshape(3, 3)
+        assert np.abs(V @ np.diag(lam) @ V.T - S).max() < 1e-12
+        assert np.allclose(np.sort(lam), np.linalg.eigvalsh(S), atol=1e-12)
+
+
+# ------------------------------------------------------- whole-step checks
+def cube_scene(cells=6, seed=0, E=1e4):
+    sc = scenes.elastic_cube(n_side_cells=cells, seed=seed, E=E)
+    return sc
+
+
+def test_apic_affine_round_trip(oracle):
+    """v = a + B x with C = B survives P2G -> G2P exactly; grad v = B so F <- (I + dt B) F."""
+    sc = cube_scene(cells=6)
+    sc.materials = [A.make_material(A.MODEL_FIXED_COROTATED, 0.0, 0.0)]
+    sc.colliders = []
+    sc.gravity = (0.0, 0.0, 0.0)
+    a = np.array([0.1, -0.2, 0.05])
+    B = np.array([[0.3, -0.1, 0.2], [0.05, -0.4, 0.1], [0.2, 0.0, 0.1]])
+    sc.v = a + sc.x @ B.T
+    sc.C = np.tile(B.ravel(), (sc.n, 1))
+    sim = sc.make(oracle)
+    grid_x = None
+    sim.substep(sc.dt, 0.0)
+    g = sim.grid()
+    xi = g["coords"] * sc.dx
+    assert np.abs(g["v"] - (a + xi @ B.T)).max() < 1e-12
+    out = sim.download()
+    x_new = sc.x + sc.dt * (a + sc.x @ B.T)
+    assert np.abs(out["x"] - x_new).max() < 1e-13
+    assert np.abs(out["v"] - (a + sc.x @ B.T)).max() < 1e-12
+    assert np.abs(out["C"] - B.ravel()).max() < 1e-10
+    F_want = (np.eye(3) + sc.dt * B).ravel()
+    assert np.abs(out["F"] - F_want).max() < 1e-12
+    del grid_x
+
+
+def test_p2g_conserves_mass_and_momentum(oracle):
+    sc = cube_scene(cells=6, seed=3)
+    rng = np.random.default_rng(8)
+    sc.v = rng.normal(size=sc.v.shape)
+    sc.C = rng.normal(size=sc.C.shape)
+    sim = sc.make(oracle)
+    sim.substep(sc.dt, 0.0)
+    g = sim.grid()
+    assert abs(g["m"].sum() - sc.m.sum()) < 1e-13 * sc.m.sum()
+    P_grid = (g["m"][:, None] * g["v"]).sum(0)
+    P_part = (sc.m[:, None] * sc.v).sum(0)
+    assert np.abs(P_grid - P_part).max() < 1e-12 * np.abs(sc.m[:, None] * sc.v).sum()
+
+
+def test_grid_update_gravity_and_sticky_floor(oracle):
+    sc = cube_scene(cells=6)
+    sc.materials = [A.make_material(A.MODEL_FIXED_COROTATED, 0.0, 0.0)]
+    sc.v[:] = (0.3, 0.0, -0.1)
+    sim = sc.make(oracle)
+    sim.substep(sc.dt, 0.0)
+    g = sim.grid()
+    y = g["coords"][:, 1]
+    floor = y <= 8  # nodes inside the sticky floor half-space (y <= 8 dx)
+    assert floor.any()
+    assert np.all(g["new_v"][floor] == 0.0)
+    free = ~floor
+    want = g["v"][free] + np.array(sc.gravity) * sc.dt
+    assert np.abs(g["new_v"][free] - want).max() < 1e-12
+
+
+def test_moving_sticky_box_sets_object_velocity(oracle):
+    sc = cube_scene(cells=6)
+    sc.colliders = [A.box((0, 0, 0), (20 * sc.dx, 12 * sc.dx, 20 * sc.dx), A.STICKY, b0=(0.0, 0.001, 0.0),
+                          dbdt=(0.0, 0.5, 0.0))]
+    t = 0.002  # b(t) = (0, 0.002, 0): the box top moves to 12 dx + 0.002
+    sim = sc.make(oracle)
+    sim.substep(sc.dt, t)
+    g = sim.grid()
+    xi = g["coords"] * sc.dx
+    inside = (xi[:, 0] <= 20 * sc.dx) & (xi[:, 1] <= 12 * sc.dx + 0.002) & (xi[:, 2] <= 20 * sc.dx)
+    assert inside.any()
+    assert np.all(g["new_v"][inside] == np.array([0.0, 0.5, 0.0]))
+
+
+def test_laplacian_quirk_known_answer(oracle):
+    """Uniform nodal d: the reference's 3D Laplacian weight uses ddw[1](i), ddw[2](i)
+    (MpmGrid.h:610-612), giving lap = 6 d (3 t^2 - 1.25) / dx^2 with t = x/dx - base_x - 1."""
+    sc = scenes.damage_slab(n=8000, dx=1.0 / 64, size=(0.2, 0.2, 0.2), domain_nodes=128, sigma_crit=1e12)
+    sc.colliders = []
+    sim = sc.make(oracle)
+    d0 = 0.3
+    sim.upload(sc.x, sc.v, sc.C, sc.F, sc.m, sc.V0, sc.material, d=np.full(sc.n, d0))
+    sim.substep(sc.dt, 0.0)
+    out = sim.download()
+    xs = sc.x / sc.dx
+    base = np.floor(xs - 0.5)
+    t = xs[:, 0] - base[:, 0] - 1
+    lo, hi = sc.x.min(0) / sc.dx + 3, sc.x.max(0) / sc.dx - 3
+    interior = np.all((xs > lo) & (xs < hi), axis=1)
+    assert interior.sum() > 100
+    want = 6 * d0 * (3 * t ** 2 - 1.25) / sc.dx ** 2
+    assert np.abs(out["laplacian"][interior] - want[interior]).max() < 1e-9 * np.abs(want).max()
+
+
+def test_sort_matches_brute_force(oracle):
+    rng = np.random.default_rng(9)
+    for prec, log2 in ((A.PREC_FP64, 7), (A.PREC_FP32, 6)):
+        sc = cube_scene(cells=2)
+        n = 40
+        sc.x = rng.uniform(0.15, 0.6, size=(n, 3))
+        sc.v, sc.C, sc.F = np.zeros((n, 3)), np.zeros((n, 9)), scenes.identity_F(n)
+        sc.V0, sc.material, sc.rho = np.full(n, 1e-6), np.zeros(n, np.int32), np.full(n, 1.0)
+        sc.m = sc.V0.copy()
+        sim = sc.make(oracle, prec)
+        sim.substep(1e-6, 0.0)
+        b = sim.binning()
+        masks = (C.c_uint64 * 3)()
+        oracle.lib.zmpm_oracle_masks(log2, masks)
+        if prec == A.PREC_FP32:
+            xs = (sc.x.astype(np.float32) * np.float32(np.float32(1) / np.float32(sc.dx))) - np.float32(0.5)
+        else:
+            xs = sc.x * (1 / sc.dx) - 0.5
+        base = np.floor(xs).astype(int)
+        block_bits = 12 - log2
+        keys = []
+        for i in range(n):
+            off = pdep(int(base[i, 0]), masks[0]) | pdep(int(base[i, 1]), masks[1]) | pdep(int(base[i, 2]), masks[2])
+            keys.append(((off >> log2) << (32 - block_bits)) + i)
+        keys.sort()
+        assert [int(k) for k in b["keys"]] == keys
+        assert list(b["order"]) == [k & ((1 << (32 - block_bits)) - 1) for k in keys]
+        pages = [k >> 32 for k in keys]
+        assert list(b["block_offset"]) == sorted(set(pages))
+
+
+def test_oracle_thread_count_invariance(oracle):
+    """The 8-colour schedule makes the oracle's P2G order independent of threads."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path[:0] = {path!r}; import numpy as np; from ziran2020_amd import _abi as A, scenes;"
+            "lib = A.ZmpmLib({lib!r}); sc = scenes.elastic_cube(n_side_cells=6, seed=4);"
+            "s = sc.make(lib); s.run(5, sc.dt); np.save(sys.argv[1], s.download()['x'])")
+    root = os.path.dirname(HERE)
+    outs = []
+    for th in ("1", "4"):
+        f = os.path.join("/tmp", f"zmpm_oracle_thr{th}_{os.getpid()}.npy")
+        env = dict(os.environ, OMP_NUM_THREADS=th)
+        subprocess.run([sys.executable, "-c", code.format(path=[root, HERE], lib=oracle.path), f], env=env, check=True)
+        outs.append(np.load(f))
+        os.remove(f)
+    assert np.array_equal(outs[0], outs[1])

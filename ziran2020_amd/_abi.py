@@ -1,0 +1,311 @@
+"""ctypes mirror of include/zmpm.h.
+
+The same declarations bind any shared library that exports the zmpm C ABI.
+The package itself only ever loads the HIP library (``libzmpm.so``); tests may
+bind a second library through :class:`ZmpmLib` to compare against it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ZMPM_ABI_VERSION = 1
+
+# zmpm_status
+OK, E_ARG, E_HIP, E_DOMAIN, E_CFL, E_NAN, E_OVERFLOW, E_NOMEM = range(8)
+STATUS_NAMES = {
+    OK: "ZMPM_OK", E_ARG: "ZMPM_E_ARG", E_HIP: "ZMPM_E_HIP", E_DOMAIN: "ZMPM_E_DOMAIN",
+    E_CFL: "ZMPM_E_CFL", E_NAN: "ZMPM_E_NAN", E_OVERFLOW: "ZMPM_E_OVERFLOW",
+    E_NOMEM: "ZMPM_E_NOMEM",
+}
+
+PREC_FP64, PREC_FP32 = 0, 1
+MODEL_FIXED_COROTATED, MODEL_QR_STABLE_NEO_HOOKEAN, MODEL_NEO_HOOKEAN, MODEL_EQUATION_OF_STATE = range(4)
+STICKY, SLIP, SEPARATE, GHOST = 1, 2, 3, 4
+SHAPE_HALF_SPACE, SHAPE_BOX = 0, 1
+
+D3 = C.c_double * 3
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32),
+        ("domain_nodes", C.c_int32),
+        ("dx", C.c_double),
+        ("gravity", D3),
+        ("apic_rpic_ratio", C.c_double),
+        ("cfl", C.c_double),
+        ("device", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class Material(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32),
+        ("has_damage", C.c_int32),
+        ("mu", C.c_double), ("lam", C.c_double),
+        ("kx", C.c_double), ("ky", C.c_double),
+        ("bulk", C.c_double), ("gamma", C.c_double),
+        ("num_directors", C.c_int32),
+        ("reserved", C.c_int32),
+        ("a0", D3 * 2),
+        ("alphas", C.c_double * 2),
+        ("residual_phase", C.c_double),
+        ("l0", C.c_double), ("sigma_crit", C.c_double), ("eta", C.c_double), ("zeta", C.c_double),
+    ]
+
+
+class Collider(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int32),
+        ("type", C.c_int32),
+        ("origin", D3), ("normal", D3),
+        ("box_min", D3), ("box_max", D3),
+        ("b0", D3), ("dbdt", D3),
+        ("friction", C.c_double),
+    ]
+
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+
+
+class Particles(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("x", _DP), ("v", _DP), ("C", _DP), ("F", _DP),
+        ("m", _DP), ("V0", _DP), ("material", _IP),
+        ("d", _DP), ("g", _DP), ("laplacian", _DP),
+    ]
+
+
+def _ptr(a, ctype=C.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class ZmpmError(RuntimeError):
+    def __init__(self, status: int, where: str, message: str = ""):
+        self.status = status
+        name = STATUS_NAMES.get(status, str(status))
+        super().__init__(f"{where}: {name}" + (f" ({message})" if message else ""))
+
+
+def make_material(model=MODEL_FIXED_COROTATED, E=1e4, nu=0.3, *, has_damage=False, kx=0.0, ky=0.0,
+                  bulk=0.0, gamma=7.0, directors=((1.0, 0.0, 0.0),), alphas=(0.0,),
+                  residual_phase=0.001, l0=0.0, sigma_crit=1.0, eta=0.1, zeta=1.0) -> Material:
+    """Material from Young's modulus and Poisson ratio with the reference's Lame
+    conversion (CorotatedElasticity.cpp:15-20, QRStableNeoHookean.h:61-65)."""
+    m = Material()
+    m.model = model
+    m.has_damage = 1 if has_damage else 0
+    m.lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    m.mu = E / (2.0 * (1.0 + nu))
+    m.kx, m.ky, m.bulk, m.gamma = kx, ky, bulk, gamma
+    m.num_directors = len(directors)
+    for k, a in enumerate(directors):
+        for c in range(3):
+            m.a0[k][c] = a[c]
+    for k, a in enumerate(alphas):
+        m.alphas[k] = a
+    m.residual_phase, m.l0, m.sigma_crit, m.eta, m.zeta = residual_phase, l0, sigma_crit, eta, zeta
+    return m
+
+
+def half_space(origin, normal, type_=STICKY, friction=0.0) -> Collider:
+    c = Collider()
+    c.shape, c.type, c.friction = SHAPE_HALF_SPACE, type_, friction
+    for a in range(3):
+        c.origin[a], c.normal[a] = origin[a], normal[a]
+    return c
+
+
+def box(box_min, box_max, type_=STICKY, b0=(0, 0, 0), dbdt=(0, 0, 0), friction=0.0) -> Collider:
+    c = Collider()
+    c.shape, c.type, c.friction = SHAPE_BOX, type_, friction
+    for a in range(3):
+        c.box_min[a], c.box_max[a] = box_min[a], box_max[a]
+        c.b0[a], c.dbdt[a] = b0[a], dbdt[a]
+    return c
+
+
+class ZmpmLib:
+    """Bindings of one shared library exporting the zmpm C ABI."""
+
+    EXPORTS = (
+        "zmpm_abi_version", "zmpm_build_info", "zmpm_create", "zmpm_destroy", "zmpm_last_error",
+        "zmpm_set_materials", "zmpm_add_collider", "zmpm_clear_colliders", "zmpm_upload",
+        "zmpm_download", "zmpm_substep", "zmpm_run", "zmpm_synchronize", "zmpm_get_binning",
+        "zmpm_get_grid", "zmpm_set_timing", "zmpm_get_timings",
+    )
+
+    def __init__(self, path: str):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.path = path
+        self.lib = C.CDLL(path)
+        L = self.lib
+        L.zmpm_abi_version.restype = C.c_int32
+        L.zmpm_build_info.restype = C.c_char_p
+        L.zmpm_last_error.restype = C.c_char_p
+        L.zmpm_last_error.argtypes = [C.c_void_p]
+        for name in self.EXPORTS:
+            fn = getattr(L, name)
+            if name not in ("zmpm_abi_version", "zmpm_build_info", "zmpm_last_error"):
+                fn.restype = C.c_int32
+        L.zmpm_create.argtypes = [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]
+        L.zmpm_destroy.argtypes = [C.c_void_p]
+        L.zmpm_set_materials.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Material)]
+        L.zmpm_add_collider.argtypes = [C.c_void_p, C.POINTER(Collider), C.POINTER(C.c_int32)]
+        L.zmpm_clear_colliders.argtypes = [C.c_void_p]
+        L.zmpm_upload.argtypes = [C.c_void_p, C.POINTER(Particles)]
+        L.zmpm_download.argtypes = [C.c_void_p, C.POINTER(Particles)]
+        L.zmpm_substep.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.zmpm_run.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double]
+        L.zmpm_synchronize.argtypes = [C.c_void_p]
+        L.zmpm_get_binning.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
+        L.zmpm_get_grid.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), _DP, _DP, _DP, _DP]
+        L.zmpm_set_timing.argtypes = [C.c_void_p, C.c_int32]
+        L.zmpm_get_timings.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_char_p), _DP]
+        if L.zmpm_abi_version() != ZMPM_ABI_VERSION:
+            raise RuntimeError(f"{path}: ABI version {L.zmpm_abi_version()} != {ZMPM_ABI_VERSION}")
+
+    @property
+    def build_info(self) -> str:
+        return self.lib.zmpm_build_info().decode()
+
+
+class Simulation:
+    """One zmpm handle: scene, materials, colliders and particle state."""
+
+    def __init__(self, lib: ZmpmLib, *, dx, precision=PREC_FP64, domain_nodes=256, gravity=(0, 0, 0),
+                 apic_rpic_ratio=1.0, cfl=0.6, device=0):
+        self.lib = lib
+        self.L = lib.lib
+        d = SceneDesc()
+        d.precision, d.domain_nodes, d.dx = precision, domain_nodes, dx
+        for a in range(3):
+            d.gravity[a] = gravity[a]
+        d.apic_rpic_ratio, d.cfl, d.device = apic_rpic_ratio, cfl, device
+        self.desc = d
+        h = C.c_void_p()
+        self._check(self.L.zmpm_create(C.byref(d), C.byref(h)), "zmpm_create", handle=False)
+        self.h = h
+        self.n = 0
+
+    def _check(self, st, where, handle=True):
+        if st != OK:
+            msg = self.L.zmpm_last_error(self.h).decode() if handle else ""
+            raise ZmpmError(st, where, msg)
+        return st
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.zmpm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_materials(self, mats):
+        arr = (Material * len(mats))(*mats)
+        self._check(self.L.zmpm_set_materials(self.h, len(mats), arr), "zmpm_set_materials")
+
+    def add_collider(self, c: Collider) -> int:
+        i = C.c_int32()
+        self._check(self.L.zmpm_add_collider(self.h, C.byref(c), C.byref(i)), "zmpm_add_collider")
+        return i.value
+
+    def upload(self, x, v, Cm, F, m, V0, material, d=None, g=None):
+        n = int(x.shape[0])
+        arrs = dict(
+            x=np.ascontiguousarray(x, np.float64).reshape(n, 3),
+            v=np.ascontiguousarray(v, np.float64).reshape(n, 3),
+            C=np.ascontiguousarray(Cm, np.float64).reshape(n, 9),
+            F=np.ascontiguousarray(F, np.float64).reshape(n, 9),
+            m=np.ascontiguousarray(m, np.float64),
+            V0=np.ascontiguousarray(V0, np.float64),
+            material=np.ascontiguousarray(material, np.int32),
+            d=None if d is None else np.ascontiguousarray(d, np.float64),
+            g=None if g is None else np.ascontiguousarray(g, np.float64),
+        )
+        p = Particles()
+        p.n = n
+        for k in ("x", "v", "C", "F", "m", "V0", "d", "g"):
+            setattr(p, k, _ptr(arrs[k]))
+        p.material = _ptr(arrs["material"], C.c_int32)
+        p.laplacian = None
+        self._keep = arrs
+        self._check(self.L.zmpm_upload(self.h, C.byref(p)), "zmpm_upload")
+        self.n = n
+
+    def download(self, check=True):
+        n = self.n
+        out = dict(x=np.zeros((n, 3)), v=np.zeros((n, 3)), C=np.zeros((n, 9)), F=np.zeros((n, 9)),
+                   m=np.zeros(n), V0=np.zeros(n), material=np.zeros(n, np.int32), d=np.zeros(n),
+                   g=np.zeros(n), laplacian=np.zeros(n))
+        p = Particles()
+        p.n = n
+        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "laplacian"):
+            setattr(p, k, _ptr(out[k]))
+        p.material = _ptr(out["material"], C.c_int32)
+        st = self.L.zmpm_download(self.h, C.byref(p))
+        out["status"] = st
+        if check and st != OK:
+            self._check(st, "zmpm_download")
+        return out
+
+    def substep(self, dt, t):
+        return self._check(self.L.zmpm_substep(self.h, dt, t), "zmpm_substep")
+
+    def run(self, steps, dt, t0=0.0):
+        return self._check(self.L.zmpm_run(self.h, steps, dt, t0), "zmpm_run")
+
+    def synchronize(self, check=True):
+        st = self.L.zmpm_synchronize(self.h)
+        if check:
+            self._check(st, "zmpm_synchronize")
+        return st
+
+    def binning(self):
+        G = C.c_int64()
+        self._check(self.L.zmpm_get_binning(self.h, C.byref(G), None, None, None, None), "zmpm_get_binning")
+        n, g = self.n, G.value
+        keys = np.zeros(n, np.uint64)
+        order = np.zeros(n, np.int32)
+        groups = np.zeros((g, 2), np.int32)
+        blocks = np.zeros(g, np.uint64)
+        self._check(self.L.zmpm_get_binning(self.h, C.byref(G), _ptr(keys, C.c_uint64), _ptr(order, C.c_int32),
+                                            _ptr(groups, C.c_int32), _ptr(blocks, C.c_uint64)), "zmpm_get_binning")
+        return dict(keys=keys, order=order, groups=groups, block_offset=blocks)
+
+    def grid(self):
+        N = C.c_int64()
+        self._check(self.L.zmpm_get_grid(self.h, C.byref(N), None, None, None, None, None), "zmpm_get_grid")
+        k = N.value
+        out = dict(coords=np.zeros((k, 3), np.int32), m=np.zeros(k), v=np.zeros((k, 3)),
+                   new_v=np.zeros((k, 3)), pf=np.zeros(k))
+        self._check(self.L.zmpm_get_grid(self.h, C.byref(N), _ptr(out["coords"], C.c_int32), _ptr(out["m"]),
+                                         _ptr(out["v"]), _ptr(out["new_v"]), _ptr(out["pf"])), "zmpm_get_grid")
+        return out
+
+    def timings(self):
+        cnt = C.c_int32(64)
+        names = (C.c_char_p * 64)()
+        ms = np.zeros(64)
+        self._check(self.L.zmpm_get_timings(self.h, C.byref(cnt), names, _ptr(ms)), "zmpm_get_timings")
+        return {names[i].decode(): float(ms[i]) for i in range(cnt.value)}

@@ -1,0 +1,739 @@
+// zmpm_api.hip -- host side of the C ABI declared in include/zmpm.h: device
+// memory, the substep launch sequence on the handle's HIP stream, uploads,
+// downloads and the binning / grid exports used for parity checks.
+#include "zmpm_device.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace zmpm;
+
+namespace {
+
+constexpr int kMaxMaterials = 256;
+constexpr int kMaxColliders = 16;
+constexpr int kTimedGroups = 4;
+const char* const kTimedNames[kTimedGroups + 1] = { "bin_sort", "p2g", "grid_update", "g2p", "substep" };
+
+uint64_t spreadMask(int log2_struct, int axis)
+{
+    // SPGrid_Mask (Lib/SPGrid/Core/SPGrid_Mask.h:22-123) aggregate masks for log2_page = 12
+    const int page_bits = 12, data_bits = log2_struct, block_bits = page_bits - data_bits;
+    const int zb = block_bits / 3 + (block_bits % 3 > 0), yb = block_bits / 3 + (block_bits % 3 > 1), xb = block_bits / 3;
+    const int sh = 3 - block_bits % 3;
+    const uint64_t hi = ~0ull << page_bits;
+    if (axis == 0) return ((0x4924924924924924ull << sh) & hi) | ((uint64_t)((1 << xb) - 1) << (data_bits + zb + yb));
+    if (axis == 1) return ((0x2492492492492492ull << sh) & hi) | ((uint64_t)((1 << yb) - 1) << (data_bits + zb));
+    return ((0x9249249249249249ull << sh) & hi) | ((uint64_t)((1 << zb) - 1) << data_bits);
+}
+
+uint32_t hostBitPack(uint64_t data, uint64_t mask)
+{
+    uint32_t r = 0, bit = 0;
+    for (int b = 0; b < 64 && bit < 32; b++)
+        if (mask & (1ull << b)) {
+            if (data & (1ull << b)) r |= 1u << bit;
+            bit++;
+        }
+    return r;
+}
+
+} // namespace
+
+struct zmpm_context {
+    zmpm_scene_desc desc {};
+    std::string err;
+    hipStream_t stream = nullptr;
+    bool fp32 = false;
+    bool any_damage = false;
+    std::vector<zmpm_material> mats;
+    std::vector<zmpm_collider> cols;
+    Dev<float, long long> Df {};
+    Dev<double, double> Dd {};
+    std::vector<void*> allocs; // particle + grid buffers (freed on reallocation)
+    void* mats_dev = nullptr;
+    void* cols_dev = nullptr;
+    int* tile_sums = nullptr;
+    int64_t max_tiles = 0;
+    int n = 0;
+    int64_t cap = 0;
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    int timed_steps = 0;
+    double timed_ms[kTimedGroups + 1] = {};
+    bool stepped = false;
+};
+
+namespace {
+
+zmpm_status fail(zmpm_context* h, zmpm_status s, const std::string& msg)
+{
+    if (h) h->err = msg;
+    return s;
+}
+
+#define HIPCHK(h, x)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess) return fail((h), ZMPM_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class X>
+zmpm_status dalloc(zmpm_context* h, X** out, size_t count)
+{
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(X));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(h, ZMPM_E_NOMEM, std::string("hipMalloc of ") + std::to_string(count * sizeof(X)) + " bytes failed");
+    }
+    h->allocs.push_back(p);
+    *out = (X*)p;
+    return ZMPM_OK;
+}
+
+void freeAll(zmpm_context* h)
+{
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void* p : h->allocs) (void)hipFree(p);
+    h->allocs.clear();
+    h->Df = Dev<float, long long> {};
+    h->Dd = Dev<double, double> {};
+    h->tile_sums = nullptr;
+    h->cap = 0;
+    h->n = 0;
+}
+
+template <class T, class AccT>
+Dev<T, AccT>& devOf(zmpm_context* h);
+template <>
+Dev<float, long long>& devOf<float, long long>(zmpm_context* h) { return h->Df; }
+template <>
+Dev<double, double>& devOf<double, double>(zmpm_context* h) { return h->Dd; }
+
+template <class T, class AccT>
+zmpm_status allocate(zmpm_context* h, int n)
+{
+    freeAll(h);
+    Dev<T, AccT> D {};
+    using B = Blk<T>;
+    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
+    const int64_t dom = h->desc.domain_nodes;
+    const int64_t cap = std::max(n, 1);
+    D.cap = cap;
+    D.domain = (int)dom;
+    D.num_pages = dom * dom * dom / NB;
+    int64_t cap_blocks = (int64_t)n / 16 + 16384;
+    if (const char* e = std::getenv("ZMPM_BLOCK_CAPACITY")) cap_blocks = std::max<int64_t>(64, std::atoll(e));
+    cap_blocks = std::min<int64_t>(cap_blocks, D.num_pages);
+    cap_blocks = std::min<int64_t>(cap_blocks, (int64_t)(0x7fffffff / NB) - 1);
+    D.cap_blocks = (int)cap_blocks;
+    D.cap_nodes = cap_blocks * NB;
+    zmpm_status s;
+#define ALLOC(ptr, cnt) \
+    if ((s = dalloc(h, &(ptr), (size_t)(cnt))) != ZMPM_OK) return s
+    ALLOC(D.P, NPF * cap);
+    ALLOC(D.Q, NPF * cap);
+    ALLOC(D.pid, cap);
+    ALLOC(D.pmat, cap);
+    ALLOC(D.qid, cap);
+    ALLOC(D.qmat, cap);
+    ALLOC(D.key, cap);
+    ALLOC(D.rank, cap);
+    ALLOC(D.perm, cap);
+    ALLOC(D.sorted_cell, cap);
+    ALLOC(D.touched, D.num_pages);
+    ALLOC(D.scan_tmp, D.num_pages);
+    ALLOC(D.page_table, D.num_pages);
+    ALLOC(D.block_page, cap_blocks);
+    ALLOC(D.nbr, cap_blocks * 8);
+    ALLOC(D.cell_count, D.cap_nodes);
+    ALLOC(D.cell_start, D.cap_nodes);
+    ALLOC(D.counters, 8);
+    ALLOC(D.acc, (int64_t)NGF * D.cap_nodes);
+    ALLOC(D.newv, 3 * D.cap_nodes);
+    ALLOC(D.pfn, D.cap_nodes);
+    ALLOC(D.scales, NSC);
+    ALLOC(D.maxbits, NSC);
+    h->max_tiles = (std::max<int64_t>(D.num_pages, D.cap_nodes) + SCAN_TILE - 1) / SCAN_TILE + 1;
+    ALLOC(h->tile_sums, h->max_tiles);
+#undef ALLOC
+    HIPCHK(h, hipMemsetAsync(D.touched, 0, D.num_pages * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.perm, 0, cap * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.sorted_cell, 0, cap * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.counters, 0, 8 * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * sizeof(unsigned int), h->stream));
+    float ones[NSC] = { 1.f, 1.f, 1.f, 1.f, 1.f };
+    HIPCHK(h, hipMemcpyAsync(D.scales, ones, sizeof(ones), hipMemcpyHostToDevice, h->stream));
+    D.masks.x = spreadMask(B::data_bits, 0);
+    D.masks.y = spreadMask(B::data_bits, 1);
+    D.masks.z = spreadMask(B::data_bits, 2);
+    devOf<T, AccT>(h) = D;
+    h->cap = cap;
+    return ZMPM_OK;
+}
+
+template <class T, class AccT>
+void refreshTables(zmpm_context* h)
+{
+    Dev<T, AccT>& D = devOf<T, AccT>(h);
+    D.mats = (const MatDev<T>*)h->mats_dev;
+    D.cols = (const ColDev*)h->cols_dev;
+    D.num_cols = (int)h->cols.size();
+    D.any_damage = h->any_damage ? 1 : 0;
+    D.dx = (T)h->desc.dx;
+    D.odx = T(1) / (T)h->desc.dx;
+    for (int a = 0; a < 3; a++) D.gravity[a] = (T)h->desc.gravity[a];
+    D.rpic = (T)h->desc.apic_rpic_ratio;
+    D.cfl = (T)h->desc.cfl;
+}
+
+template <class T>
+zmpm_status uploadMaterials(zmpm_context* h)
+{
+    std::vector<MatDev<T>> md(kMaxMaterials);
+    for (size_t i = 0; i < h->mats.size(); i++) {
+        const zmpm_material& m = h->mats[i];
+        MatDev<T>& d = md[i];
+        d.model = m.model;
+        d.has_damage = m.has_damage;
+        d.nd = m.num_directors == 2 ? 2 : 1;
+        d.mu = (T)m.mu;
+        d.lambda = (T)m.lambda;
+        d.kx = (T)m.kx;
+        d.ky = (T)m.ky;
+        d.bulk = (T)m.bulk;
+        d.gamma = (T)m.gamma;
+        for (int k = 0; k < 2; k++) {
+            for (int c = 0; c < 3; c++) d.a0[k][c] = (T)m.a0[k][c];
+            d.alphas[k] = (T)m.alphas[k];
+        }
+        d.k = (T)m.residual_phase;
+        d.l0 = (T)m.l0;
+        d.sigma_crit = (T)m.sigma_crit;
+        d.eta = (T)m.eta;
+        d.zeta = (T)m.zeta;
+    }
+    HIPCHK(h, hipMemcpyAsync(h->mats_dev, md.data(), sizeof(MatDev<T>) * kMaxMaterials, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ZMPM_OK;
+}
+
+zmpm_status uploadColliders(zmpm_context* h)
+{
+    std::vector<ColDev> cd(kMaxColliders);
+    for (size_t i = 0; i < h->cols.size(); i++) {
+        const zmpm_collider& c = h->cols[i];
+        ColDev& d = cd[i];
+        d.shape = c.shape;
+        d.type = c.type;
+        double nn = std::sqrt(c.normal[0] * c.normal[0] + c.normal[1] * c.normal[1] + c.normal[2] * c.normal[2]);
+        for (int a = 0; a < 3; a++) {
+            d.origin[a] = c.origin[a];
+            d.n[a] = nn > 0 ? c.normal[a] / nn : 0.0; // HalfSpace ctor normalises (AnalyticLevelSet.cpp:318-323)
+            d.bmin[a] = c.box_min[a];
+            d.bmax[a] = c.box_max[a];
+            d.b0[a] = c.b0[a];
+            d.dbdt[a] = c.dbdt[a];
+        }
+        d.friction = c.friction;
+    }
+    HIPCHK(h, hipMemcpyAsync(h->cols_dev, cd.data(), sizeof(ColDev) * kMaxColliders, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ZMPM_OK;
+}
+
+int gridFor(int64_t items, int per, int maxg)
+{
+    int64_t g = (items + per - 1) / per;
+    if (g < 1) g = 1;
+    return (int)std::min<int64_t>(g, maxg);
+}
+
+template <class T, class AccT>
+void launchScan(zmpm_context* h, const int* in, const int* len_dev, int64_t mul, int64_t len_host, int* out,
+                int* total_out, int cap, int* counters)
+{
+    const int g = (int)std::min<int64_t>(h->max_tiles, 2048);
+    k_scan_reduce<<<g, WG, 0, h->stream>>>(in, len_dev, mul, len_host, h->tile_sums);
+    k_scan_top<<<1, WG, 0, h->stream>>>(h->tile_sums, len_dev, mul, len_host, total_out, cap, counters);
+    k_scan_down<<<g, WG, 0, h->stream>>>(in, len_dev, mul, len_host, h->tile_sums, out);
+}
+
+template <class T, class AccT>
+void launchStep(zmpm_context* h, double dt, double t, hipEvent_t* ev)
+{
+    Dev<T, AccT>& D = devOf<T, AccT>(h);
+    using B = Blk<T>;
+    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
+    hipStream_t st = h->stream;
+    const int gp = gridFor(D.n, WG, 1 << 30);
+    if (ev) (void)hipEventRecord(ev[0], st);
+    // fixed-point scales of this substep from the previous one's maxima
+    if (std::is_same<AccT, long long>::value) k_update_scales<<<1, 64, 0, st>>>(D.scales, D.maxbits);
+    k_bin<T, AccT><<<gp, WG, 0, st>>>(D);
+    launchScan<T, AccT>(h, D.touched, nullptr, 1, D.num_pages, D.scan_tmp, D.counters, D.cap_blocks, D.counters);
+    k_assign<T, AccT><<<gridFor(D.num_pages, WG, 8192), WG, 0, st>>>(D);
+    k_prepare<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, 16384), 64, 0, st>>>(D);
+    k_count<T, AccT><<<gp, WG, 0, st>>>(D);
+    launchScan<T, AccT>(h, D.cell_count, D.counters, NB, 0, D.cell_start, nullptr, -1, D.counters);
+    k_scatter<T, AccT><<<gp, WG, 0, st>>>(D);
+    k_cellsort<T, AccT><<<gridFor(D.cap_nodes, WG, 8192), WG, 0, st>>>(D);
+    if (ev) (void)hipEventRecord(ev[1], st);
+    k_p2g<T, AccT><<<gridFor(D.n, Tile<T>::CHUNK, 1 << 30), WG, 0, st>>>(D);
+    if (ev) (void)hipEventRecord(ev[2], st);
+    k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, 8192), WG, 0, st>>>(D, (T)dt, t);
+    if (ev) (void)hipEventRecord(ev[3], st);
+    k_g2p<T, AccT><<<gp, WG, 0, st>>>(D, (T)dt);
+    if (ev) (void)hipEventRecord(ev[4], st);
+    std::swap(D.P, D.Q);
+    std::swap(D.pid, D.qid);
+    std::swap(D.pmat, D.qmat);
+}
+
+zmpm_status deviceStatus(zmpm_context* h, bool sync)
+{
+    if (sync) HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipGetLastError());
+    int* counters = h->fp32 ? h->Df.counters : h->Dd.counters;
+    if (!counters) return ZMPM_OK;
+    int c[2] = { 0, 0 };
+    HIPCHK(h, hipMemcpy(c, counters, sizeof(c), hipMemcpyDeviceToHost));
+    int bits = c[1];
+    if (bits & ST_NAN) return fail(h, ZMPM_E_NAN, "non-finite particle position");
+    if (bits & ST_DOMAIN) return fail(h, ZMPM_E_DOMAIN, "particle base node outside [0, domain_nodes-5]");
+    if (bits & ST_OVERFLOW) return fail(h, ZMPM_E_OVERFLOW, "fixed-point P2G contribution out of range");
+    if (bits & ST_CAPACITY) return fail(h, ZMPM_E_NOMEM, "grid block capacity exceeded (set ZMPM_BLOCK_CAPACITY)");
+    if (bits & ST_CFL) return fail(h, ZMPM_E_CFL, "a particle moved more than dx in one substep");
+    return ZMPM_OK;
+}
+
+template <class T, class AccT>
+zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
+{
+    const int n = (int)p->n;
+    zmpm_status s;
+    if (n > h->cap || devOf<T, AccT>(h).P == nullptr)
+        if ((s = allocate<T, AccT>(h, n)) != ZMPM_OK) return s;
+    Dev<T, AccT>& D = devOf<T, AccT>(h);
+    D.n = n;
+    h->n = n;
+    refreshTables<T, AccT>(h);
+    const int64_t cap = D.cap;
+    std::vector<T> buf((size_t)NPF * cap, T(0));
+    for (int i = 0; i < n; i++) {
+        for (int a = 0; a < 3; a++) {
+            buf[(PX + a) * cap + i] = (T)p->x[3 * i + a];
+            buf[(PV + a) * cap + i] = (T)p->v[3 * i + a];
+        }
+        for (int k = 0; k < 9; k++) {
+            buf[(PC + k) * cap + i] = (T)p->C[9 * i + k];
+            buf[(PF + k) * cap + i] = (T)p->F[9 * i + k];
+        }
+        buf[PM * cap + i] = (T)p->m[i];
+        buf[PVOL * cap + i] = (T)p->V0[i];
+        buf[PD * cap + i] = p->d ? (T)p->d[i] : T(0);
+        buf[PG * cap + i] = p->g ? (T)p->g[i] : T(1);
+    }
+    std::vector<int> ids(cap, 0), mat(cap, 0);
+    for (int i = 0; i < n; i++) {
+        ids[i] = i;
+        mat[i] = p->material[i];
+    }
+    HIPCHK(h, hipMemcpyAsync(D.P, buf.data(), buf.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(D.pid, ids.data(), cap * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(D.pmat, mat.data(), cap * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemsetAsync(D.counters, 0, 8 * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * sizeof(unsigned int), h->stream));
+    if (n > 0) {
+        // stress of the uploaded state and the first substep's fixed-point bounds
+        k_stress_init<T, AccT><<<gridFor(n, WG, 1 << 30), WG, 0, h->stream>>>(D);
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipGetLastError());
+    h->stepped = false;
+    return ZMPM_OK;
+}
+
+template <class T, class AccT>
+zmpm_status downloadT(zmpm_context* h, zmpm_particles* p)
+{
+    Dev<T, AccT>& D = devOf<T, AccT>(h);
+    const int n = h->n;
+    const int64_t cap = D.cap;
+    std::vector<T> buf((size_t)NPF * cap);
+    std::vector<int> ids(cap), mat(cap);
+    HIPCHK(h, hipMemcpyAsync(buf.data(), D.P, buf.size() * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(ids.data(), D.pid, cap * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(mat.data(), D.pmat, cap * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int s = 0; s < n; s++) {
+        const int i = ids[s];
+        if (i < 0 || i >= n) return fail(h, ZMPM_E_HIP, "corrupt particle id table");
+        for (int a = 0; a < 3; a++) {
+            if (p->x) p->x[3 * i + a] = (double)buf[(PX + a) * cap + s];
+            if (p->v) p->v[3 * i + a] = (double)buf[(PV + a) * cap + s];
+        }
+        for (int k = 0; k < 9; k++) {
+            if (p->C) p->C[9 * i + k] = (double)buf[(PC + k) * cap + s];
+            if (p->F) p->F[9 * i + k] = (double)buf[(PF + k) * cap + s];
+        }
+        if (p->m) p->m[i] = (double)buf[PM * cap + s];
+        if (p->V0) p->V0[i] = (double)buf[PVOL * cap + s];
+        if (p->material) p->material[i] = mat[s];
+        if (p->d) p->d[i] = (double)buf[PD * cap + s];
+        if (p->g) p->g[i] = (double)buf[PG * cap + s];
+        if (p->laplacian) p->laplacian[i] = (double)buf[PLAP * cap + s];
+    }
+    return ZMPM_OK;
+}
+
+template <class T, class AccT>
+zmpm_status binningT(zmpm_context* h, int64_t* n_groups, uint64_t* keys, int32_t* order, int32_t* groups,
+                     uint64_t* block_offset)
+{
+    Dev<T, AccT>& D = devOf<T, AccT>(h);
+    using B = Blk<T>;
+    constexpr int bbits = B::xbits + B::ybits + B::zbits;
+    constexpr int NB = 1 << bbits;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (!h->stepped) {
+        *n_groups = 0;
+        return ZMPM_OK;
+    }
+    int nb = 0;
+    HIPCHK(h, hipMemcpy(&nb, D.counters, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> bpage(nb), cstart((size_t)nb * NB), ccount((size_t)nb * NB);
+    HIPCHK(h, hipMemcpy(bpage.data(), D.block_page, nb * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(cstart.data(), D.cell_start, cstart.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(ccount.data(), D.cell_count, ccount.size() * sizeof(int), hipMemcpyDeviceToHost));
+    int64_t G = 0;
+    for (int b = 0; b < nb; b++) {
+        int tot = 0;
+        for (int e = 0; e < NB; e++) tot += ccount[(size_t)b * NB + e];
+        if (tot == 0) continue;
+        if (groups) {
+            groups[2 * G] = cstart[(size_t)b * NB];
+            groups[2 * G + 1] = cstart[(size_t)b * NB] + tot - 1;
+        }
+        if (block_offset) block_offset[G] = (uint64_t)(uint32_t)bpage[b];
+        G++;
+    }
+    *n_groups = G;
+    if (keys || order) {
+        const int n = h->n;
+        std::vector<int> ids(n), scell(n);
+        HIPCHK(h, hipMemcpy(ids.data(), D.pid, n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(scell.data(), D.sorted_cell, n * sizeof(int), hipMemcpyDeviceToHost));
+        const int index_bits = 32 - bbits;
+        for (int s = 0; s < n; s++) {
+            if (order) order[s] = ids[s];
+            if (keys) {
+                const int c = scell[s];
+                const uint64_t page = (uint64_t)(uint32_t)bpage[c / NB];
+                const uint64_t elem = (uint64_t)(c % NB);
+                keys[s] = (((page << bbits) | elem) << index_bits) + (uint64_t)ids[s];
+            }
+        }
+    }
+    return ZMPM_OK;
+}
+
+template <class T, class AccT>
+zmpm_status gridT(zmpm_context* h, int64_t* n_nodes, int32_t* coords, double* m, double* v, double* new_v, double* pf)
+{
+    Dev<T, AccT>& D = devOf<T, AccT>(h);
+    using B = Blk<T>;
+    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (!h->stepped) {
+        *n_nodes = 0;
+        return ZMPM_OK;
+    }
+    int nb = 0;
+    HIPCHK(h, hipMemcpy(&nb, D.counters, sizeof(int), hipMemcpyDeviceToHost));
+    const int64_t nn = (int64_t)nb * NB;
+    std::vector<int> bpage(nb);
+    std::vector<AccT> acc((size_t)7 * nn);
+    std::vector<T> nv((size_t)3 * nn), pfn(nn);
+    float sc[NSC] = { 1, 1, 1, 1, 1 };
+    HIPCHK(h, hipMemcpy(bpage.data(), D.block_page, nb * sizeof(int), hipMemcpyDeviceToHost));
+    for (int f = 0; f < 7; f++)
+        HIPCHK(h, hipMemcpy(acc.data() + f * nn, D.acc + f * D.cap_nodes, nn * sizeof(AccT), hipMemcpyDeviceToHost));
+    for (int a = 0; a < 3; a++)
+        HIPCHK(h, hipMemcpy(nv.data() + a * nn, D.newv + a * D.cap_nodes, nn * sizeof(T), hipMemcpyDeviceToHost));
+    if (h->any_damage) HIPCHK(h, hipMemcpy(pfn.data(), D.pfn, nn * sizeof(T), hipMemcpyDeviceToHost));
+    // scales of the last substep's P2G (they are replaced only at the start of the next one)
+    if (std::is_same<AccT, long long>::value) HIPCHK(h, hipMemcpy(sc, D.scales, sizeof(sc), hipMemcpyDeviceToHost));
+    auto val = [&](int f, int64_t node, int g) -> double {
+        if (std::is_same<AccT, long long>::value) return (double)acc[f * nn + node] / (double)sc[g];
+        return (double)acc[f * nn + node];
+    };
+    struct Rec {
+        uint32_t c[3];
+        int64_t node;
+    };
+    std::vector<Rec> recs;
+    for (int64_t node = 0; node < nn; node++) {
+        double mm = val(GM, node, SC_M);
+        if (mm == 0) continue;
+        const int b = (int)(node / NB), e = (int)(node % NB);
+        uint64_t off = ((uint64_t)(uint32_t)bpage[b] << 12) | ((uint64_t)e << B::data_bits);
+        recs.push_back(Rec { { hostBitPack(off, D.masks.x), hostBitPack(off, D.masks.y), hostBitPack(off, D.masks.z) }, node });
+    }
+    std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) {
+        if (a.c[0] != b.c[0]) return a.c[0] < b.c[0];
+        if (a.c[1] != b.c[1]) return a.c[1] < b.c[1];
+        return a.c[2] < b.c[2];
+    });
+    *n_nodes = (int64_t)recs.size();
+    for (size_t q = 0; q < recs.size(); q++) {
+        const int64_t node = recs[q].node;
+        const double mm = val(GM, node, SC_M);
+        if (coords)
+            for (int a = 0; a < 3; a++) coords[3 * q + a] = (int32_t)recs[q].c[a];
+        if (m) m[q] = mm;
+        if (v)
+            for (int a = 0; a < 3; a++) v[3 * q + a] = val(GMV + a, node, SC_MV) / mm;
+        if (new_v)
+            for (int a = 0; a < 3; a++) new_v[3 * q + a] = (double)nv[a * nn + node];
+        if (pf) pf[q] = h->any_damage ? (double)pfn[node] : 0.0;
+    }
+    return ZMPM_OK;
+}
+
+} // namespace
+
+// ================================================================= C ABI
+extern "C" {
+
+int32_t zmpm_abi_version(void) { return ZMPM_ABI_VERSION; }
+
+const char* zmpm_build_info(void)
+{
+    return "zmpm HIP library: explicit APIC MPM substep + AnisoMPM damage; device code gfx950";
+}
+
+zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
+{
+    if (!desc || !out) return ZMPM_E_ARG;
+    const int dn = desc->domain_nodes;
+    if (dn < 64 || dn > 1024 || (dn & (dn - 1)) || !(desc->dx > 0)) return ZMPM_E_ARG;
+    if (desc->precision != ZMPM_PREC_FP32 && desc->precision != ZMPM_PREC_FP64) return ZMPM_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ZMPM_E_HIP;
+    if (desc->device < 0 || desc->device >= ndev) return ZMPM_E_ARG;
+    if (hipSetDevice(desc->device) != hipSuccess) return ZMPM_E_HIP;
+    auto* h = new zmpm_context();
+    h->desc = *desc;
+    h->fp32 = desc->precision == ZMPM_PREC_FP32;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return ZMPM_E_HIP;
+    }
+    const size_t msz = h->fp32 ? sizeof(MatDev<float>) : sizeof(MatDev<double>);
+    if (hipMalloc(&h->mats_dev, msz * kMaxMaterials) != hipSuccess
+        || hipMalloc(&h->cols_dev, sizeof(ColDev) * kMaxColliders) != hipSuccess) {
+        if (h->mats_dev) (void)hipFree(h->mats_dev);
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return ZMPM_E_NOMEM;
+    }
+    *out = h;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_destroy(zmpm_handle h)
+{
+    if (!h) return ZMPM_E_ARG;
+    (void)hipSetDevice(h->desc.device);
+    freeAll(h);
+    for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
+    if (h->mats_dev) (void)hipFree(h->mats_dev);
+    if (h->cols_dev) (void)hipFree(h->cols_dev);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return ZMPM_OK;
+}
+
+const char* zmpm_last_error(zmpm_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material* mats)
+{
+    if (!h) return ZMPM_E_ARG;
+    if (count <= 0 || count > kMaxMaterials || !mats) return fail(h, ZMPM_E_ARG, "bad material table");
+    for (int i = 0; i < count; i++)
+        if (mats[i].model < 0 || mats[i].model > ZMPM_MODEL_EQUATION_OF_STATE)
+            return fail(h, ZMPM_E_ARG, "unknown constitutive model");
+    (void)hipSetDevice(h->desc.device);
+    h->mats.assign(mats, mats + count);
+    h->any_damage = false;
+    for (auto& m : h->mats) h->any_damage = h->any_damage || m.has_damage;
+    zmpm_status s = h->fp32 ? uploadMaterials<float>(h) : uploadMaterials<double>(h);
+    if (s != ZMPM_OK) return s;
+    if (h->fp32)
+        refreshTables<float, long long>(h);
+    else
+        refreshTables<double, double>(h);
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_add_collider(zmpm_handle h, const zmpm_collider* c, int32_t* id_out)
+{
+    if (!h || !c) return ZMPM_E_ARG;
+    if ((int)h->cols.size() >= kMaxColliders) return fail(h, ZMPM_E_ARG, "too many colliders");
+    if (c->shape != ZMPM_SHAPE_HALF_SPACE && c->shape != ZMPM_SHAPE_BOX) return fail(h, ZMPM_E_ARG, "unknown shape");
+    if (c->type < ZMPM_STICKY || c->type > ZMPM_GHOST) return fail(h, ZMPM_E_ARG, "unknown collider type");
+    (void)hipSetDevice(h->desc.device);
+    h->cols.push_back(*c);
+    zmpm_status s = uploadColliders(h);
+    if (s != ZMPM_OK) return s;
+    if (h->fp32)
+        refreshTables<float, long long>(h);
+    else
+        refreshTables<double, double>(h);
+    if (id_out) *id_out = (int32_t)h->cols.size() - 1;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_clear_colliders(zmpm_handle h)
+{
+    if (!h) return ZMPM_E_ARG;
+    h->cols.clear();
+    if (h->fp32)
+        refreshTables<float, long long>(h);
+    else
+        refreshTables<double, double>(h);
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
+{
+    if (!h || !p || p->n < 0) return ZMPM_E_ARG;
+    if (p->n > 0 && (!p->x || !p->v || !p->C || !p->F || !p->m || !p->V0 || !p->material))
+        return fail(h, ZMPM_E_ARG, "missing particle array");
+    if (h->mats.empty()) return fail(h, ZMPM_E_ARG, "set materials before upload");
+    const int bbits = h->fp32 ? 6 : 5;
+    if (p->n >= ((int64_t)1 << (32 - bbits))) return fail(h, ZMPM_E_ARG, "too many particles for the key layout");
+    for (int64_t i = 0; i < p->n; i++)
+        if (p->material[i] < 0 || p->material[i] >= (int32_t)h->mats.size())
+            return fail(h, ZMPM_E_ARG, "material index out of range");
+    (void)hipSetDevice(h->desc.device);
+    return h->fp32 ? uploadT<float, long long>(h, p) : uploadT<double, double>(h, p);
+}
+
+zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
+{
+    if (!h || !p) return ZMPM_E_ARG;
+    if (p->n != h->n) return fail(h, ZMPM_E_ARG, "particle count mismatch");
+    (void)hipSetDevice(h->desc.device);
+    zmpm_status st = deviceStatus(h, true);
+    if (st == ZMPM_E_HIP) return st;
+    zmpm_status s = h->fp32 ? downloadT<float, long long>(h, p) : downloadT<double, double>(h, p);
+    return s != ZMPM_OK ? s : st;
+}
+
+static zmpm_status runSteps(zmpm_handle h, int32_t steps, double dt, double t0)
+{
+    if (h->n == 0) return ZMPM_OK;
+    (void)hipSetDevice(h->desc.device);
+    hipEvent_t* ev = nullptr;
+    if (h->timing) {
+        const size_t need = (size_t)steps * (kTimedGroups + 1);
+        while (h->events.size() < need) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return fail(h, ZMPM_E_HIP, "hipEventCreate failed");
+            h->events.push_back(e);
+        }
+    }
+    for (int32_t s = 0; s < steps; s++) {
+        if (h->timing) ev = &h->events[(size_t)s * (kTimedGroups + 1)];
+        if (h->fp32)
+            launchStep<float, long long>(h, dt, t0 + s * dt, ev);
+        else
+            launchStep<double, double>(h, dt, t0 + s * dt, ev);
+    }
+    h->stepped = true;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(h, ZMPM_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    if (h->timing) {
+        if (hipStreamSynchronize(h->stream) != hipSuccess) return fail(h, ZMPM_E_HIP, "stream sync");
+        for (int g = 0; g <= kTimedGroups; g++) h->timed_ms[g] = 0;
+        for (int32_t s = 0; s < steps; s++) {
+            hipEvent_t* e5 = &h->events[(size_t)s * (kTimedGroups + 1)];
+            for (int g = 0; g < kTimedGroups; g++) {
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e5[g], e5[g + 1]);
+                h->timed_ms[g] += ms;
+            }
+            float tot = 0;
+            (void)hipEventElapsedTime(&tot, e5[0], e5[kTimedGroups]);
+            h->timed_ms[kTimedGroups] += tot;
+        }
+        h->timed_steps = steps;
+    }
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_substep(zmpm_handle h, double dt, double t)
+{
+    if (!h || !(dt > 0)) return ZMPM_E_ARG;
+    return runSteps(h, 1, dt, t);
+}
+
+zmpm_status zmpm_run(zmpm_handle h, int32_t steps, double dt, double t0)
+{
+    if (!h || steps < 0 || !(dt > 0)) return ZMPM_E_ARG;
+    return runSteps(h, steps, dt, t0);
+}
+
+zmpm_status zmpm_synchronize(zmpm_handle h)
+{
+    if (!h) return ZMPM_E_ARG;
+    (void)hipSetDevice(h->desc.device);
+    return deviceStatus(h, true);
+}
+
+zmpm_status zmpm_get_binning(zmpm_handle h, int64_t* n_groups, uint64_t* keys, int32_t* order, int32_t* groups,
+                             uint64_t* block_offset)
+{
+    if (!h || !n_groups) return ZMPM_E_ARG;
+    (void)hipSetDevice(h->desc.device);
+    return h->fp32 ? binningT<float, long long>(h, n_groups, keys, order, groups, block_offset)
+                   : binningT<double, double>(h, n_groups, keys, order, groups, block_offset);
+}
+
+zmpm_status zmpm_get_grid(zmpm_handle h, int64_t* n_nodes, int32_t* coords, double* m, double* v, double* new_v,
+                          double* pf)
+{
+    if (!h || !n_nodes) return ZMPM_E_ARG;
+    (void)hipSetDevice(h->desc.device);
+    return h->fp32 ? gridT<float, long long>(h, n_nodes, coords, m, v, new_v, pf)
+                   : gridT<double, double>(h, n_nodes, coords, m, v, new_v, pf);
+}
+
+zmpm_status zmpm_set_timing(zmpm_handle h, int32_t enable)
+{
+    if (!h) return ZMPM_E_ARG;
+    h->timing = enable != 0;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_get_timings(zmpm_handle h, int32_t* count, const char** names, double* ms)
+{
+    if (!h || !count) return ZMPM_E_ARG;
+    const int cap = *count;
+    const int k = std::min(cap, kTimedGroups + 1);
+    for (int g = 0; g < k; g++) {
+        if (names) names[g] = kTimedNames[g];
+        if (ms) ms[g] = h->timed_steps ? h->timed_ms[g] / h->timed_steps : 0.0;
+    }
+    *count = k;
+    return ZMPM_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,1034 @@
+// zmpm_device.hpp -- gfx950 kernels of the explicit
+// APIC MPM substep with AnisoMPM damage.
+//
+// One substep (Ziran MpmSimulationBase::advanceOneTimeStep,
+// Lib/MPM/MpmSimulationBase.cpp:262-357) is this launch sequence on the
+// handle's stream:
+//   k_bin            base node, SPGrid page/element key, mark touched pages
+//   scan + k_assign  dense block ids in page (Morton) order
+//   k_prepare        zero block accumulators, forward-neighbour block table
+//   k_count          per-cell occupancy (atomic slot)
+//   scan             cell start offsets
+//   k_scatter        counting-sort scatter
+//   k_cellsort       order inside each cell by original index -> reference order
+//   k_p2g            APIC scatter of m, mv, f (stress read from the particle)
+//                    and of the damage phase field
+//   k_grid           v = mv/m, gravity + force, collision objects, phase-field
+//                    normalisation
+//   k_g2p            APIC gather, x update, CFL flag, damage Laplacian and
+//                    update, F update, stress of the new state (SVD / QR) and
+//                    the next P2G's fixed-point bounds; writes particles in
+//                    sorted order
+#pragma once
+#include "zmpm_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+namespace zmpm {
+
+constexpr int WG = 256;
+constexpr int SCAN_TILE = 2048; // 256 threads x 8 elements
+constexpr float FIXED_TARGET_EXP = 40.0f;
+constexpr float FIXED_LIMIT = 2251799813685248.0f; // 2^51: per-contribution bound
+
+__device__ __forceinline__ void latch(int* counters, int bits) { atomicOr(&counters[1], bits); }
+
+__device__ __forceinline__ uint32_t bitPack(uint64_t data, uint64_t mask)
+{
+    uint32_t r = 0, bit = 0;
+    while (mask) {
+        uint64_t low = mask & (~mask + 1);
+        if (data & low) r |= 1u << bit;
+        mask ^= low;
+        bit++;
+        if (bit >= 32) break;
+    }
+    return r;
+}
+
+template <class T>
+__device__ __forceinline__ uint32_t pageOfBlock(const Masks& M, uint32_t Bx, uint32_t By, uint32_t Bz)
+{
+    using B = Blk<T>;
+    uint64_t off = bitSpread(Bx << B::xbits, M.x) | bitSpread(By << B::ybits, M.y) | bitSpread(Bz << B::zbits, M.z);
+    return (uint32_t)(off >> 12);
+}
+
+// ---------------------------------------------------------------- block scan
+__device__ __forceinline__ int waveInclusiveScan(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int o = __shfl_up(v, d);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// exclusive scan over a 256-thread block; returns the block total through *total
+__device__ __forceinline__ int blockExclusiveScan256(int v, int* total)
+{
+    __shared__ int wsum[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = waveInclusiveScan(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (i < w) off += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+// length of a scan: device count * mul, or the host length
+__device__ __forceinline__ int64_t scanLength(const int* len_dev, int64_t mul, int64_t len_host)
+{
+    return len_dev ? (int64_t)(*len_dev) * mul : len_host;
+}
+
+__global__ __launch_bounds__(WG) void k_scan_reduce(const int* __restrict__ in, const int* len_dev, int64_t mul,
+                                                   int64_t len_host, int* __restrict__ tile_sums)
+{
+    const int64_t len = scanLength(len_dev, mul, len_host);
+    const int64_t ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int s = 0;
+        const int64_t base = tile * SCAN_TILE + (int64_t)threadIdx.x * 8;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (base + k < len) s += in[base + k];
+        int tot;
+        blockExclusiveScan256(s, &tot);
+        if (threadIdx.x == 0) tile_sums[tile] = tot;
+    }
+}
+
+// exclusive scan of the tile sums in one workgroup; total -> *total_out
+// (clamped to cap if cap >= 0, with the capacity status latched)
+__global__ __launch_bounds__(WG) void k_scan_top(int* __restrict__ tile_sums, const int* len_dev, int64_t mul,
+                                                int64_t len_host, int* total_out, int cap, int* counters)
+{
+    const int64_t len = scanLength(len_dev, mul, len_host);
+    const int64_t ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
+    int carry = 0;
+    for (int64_t base = 0; base < ntiles; base += WG) {
+        int64_t i = base + threadIdx.x;
+        int v = (i < ntiles) ? tile_sums[i] : 0;
+        int tot;
+        int ex = blockExclusiveScan256(v, &tot);
+        if (i < ntiles) tile_sums[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total_out) {
+        int t = carry;
+        if (cap >= 0 && t > cap) {
+            t = cap;
+            atomicOr(&counters[1], ST_CAPACITY);
+        }
+        *total_out = t;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_scan_down(const int* __restrict__ in, const int* len_dev, int64_t mul,
+                                                 int64_t len_host, const int* __restrict__ tile_off,
+                                                 int* __restrict__ out)
+{
+    const int64_t len = scanLength(len_dev, mul, len_host);
+    const int64_t ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t base = tile * SCAN_TILE + (int64_t)threadIdx.x * 8;
+        int v[8], s = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            v[k] = (base + k < len) ? in[base + k] : 0;
+            s += v[k];
+        }
+        int tot;
+        int ex = blockExclusiveScan256(s, &tot) + tile_off[tile];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (base + k < len) out[base + k] = ex;
+            ex += v[k];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ binning
+// sortParticlesAndPolluteGrid key build (MpmSimulationBase.cpp:1545-1550) and
+// MpmGrid::pollute page activation (MpmGrid.h:153-193).
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_bin(Dev<T, AccT> D)
+{
+    using B = Blk<T>;
+    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
+    const int p = blockIdx.x * WG + threadIdx.x;
+    const bool active = p < D.n;
+    uint32_t page = 0xffffffffu;
+    int bb[3] = { 0, 0, 0 };
+    if (active) {
+        int bits = 0;
+        const T hi = T(D.domain - 4);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            T x = D.P[(PX + a) * D.cap + p];
+            T t = sub_rn(mul_rn(x, D.odx), T(0.5));
+            if (!isfinite(t)) {
+                bits |= ST_NAN;
+                t = T(0);
+            }
+            if (!(t >= T(0) && t < hi)) {
+                bits |= ST_DOMAIN;
+                t = (t < T(0)) ? T(0) : hi - T(1);
+            }
+            bb[a] = (int)t; // t >= 0: truncation is int_floor (MathTools.h:15-25)
+        }
+        if (bits) latch(D.counters, bits);
+        uint64_t off = bitSpread((uint32_t)bb[0], D.masks.x) | bitSpread((uint32_t)bb[1], D.masks.y)
+            | bitSpread((uint32_t)bb[2], D.masks.z);
+        page = (uint32_t)(off >> 12);
+        uint32_t elem = (uint32_t)((off >> B::data_bits) & (NB - 1));
+        D.key[p] = ((uint64_t)page << (B::xbits + B::ybits + B::zbits)) | elem;
+    }
+    // one activation per run of equal pages inside the wavefront
+    uint32_t prev = (uint32_t)__shfl_up((int)page, 1);
+    const int lane = threadIdx.x & 63;
+    if (active && (lane == 0 || prev != page)) {
+        uint32_t Bx = bb[0] >> B::xbits, By = bb[1] >> B::ybits, Bz = bb[2] >> B::zbits;
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int c = 0; c < 2; c++) D.touched[pageOfBlock<T>(D.masks, Bx + a, By + b, Bz + c)] = 1;
+    }
+}
+
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_assign(Dev<T, AccT> D)
+{
+    const int cap = D.cap_blocks;
+    for (int64_t q = (int64_t)blockIdx.x * WG + threadIdx.x; q < D.num_pages; q += (int64_t)gridDim.x * WG) {
+        int t = D.touched[q];
+        int id = -1;
+        if (t) {
+            id = D.scan_tmp[q];
+            if (id < cap)
+                D.block_page[id] = (int)q;
+            else
+                id = -1;
+            D.touched[q] = 0;
+        }
+        D.page_table[q] = id;
+    }
+}
+
+template <class T, class AccT>
+__global__ __launch_bounds__(64) void k_prepare(Dev<T, AccT> D)
+{
+    using B = Blk<T>;
+    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
+    const int nb = D.counters[0];
+    const int nf = D.any_damage ? NGF : GPF;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int t = threadIdx.x;
+        if (t < NB) {
+            const int64_t node = (int64_t)b * NB + t;
+            for (int f = 0; f < nf; f++) D.acc[f * D.cap_nodes + node] = AccT(0);
+            D.cell_count[node] = 0;
+        }
+        if (t < 8) {
+            uint64_t off = (uint64_t)(uint32_t)D.block_page[b] << 12;
+            uint32_t Bx = bitPack(off, D.masks.x) >> B::xbits;
+            uint32_t By = bitPack(off, D.masks.y) >> B::ybits;
+            uint32_t Bz = bitPack(off, D.masks.z) >> B::zbits;
+            uint32_t nx = Bx + ((t >> 2) & 1), ny = By + ((t >> 1) & 1), nz = Bz + (t & 1);
+            int id = -1;
+            if ((nx << B::xbits) < (uint32_t)D.domain && (ny << B::ybits) < (uint32_t)D.domain
+                && (nz << B::zbits) < (uint32_t)D.domain)
+                id = D.page_table[pageOfBlock<T>(D.masks, nx, ny, nz)];
+            D.nbr[b * 8 + t] = id;
+        }
+    }
+}
+
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_count(Dev<T, AccT> D)
+{
+    using B = Blk<T>;
+    constexpr int bbits = B::xbits + B::ybits + B::zbits;
+    const int p = blockIdx.x * WG + threadIdx.x;
+    if (p >= D.n) return;
+    uint64_t key = D.key[p];
+    int bid = D.page_table[key >> bbits];
+    int r = -1;
+    if (bid >= 0)
+        r = atomicAdd(&D.cell_count[(int64_t)bid * (1 << bbits) + (int)(key & ((1 << bbits) - 1))], 1);
+    else
+        latch(D.counters, ST_CAPACITY);
+    D.rank[p] = r;
+}
+
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D)
+{
+    using B = Blk<T>;
+    constexpr int bbits = B::xbits + B::ybits + B::zbits;
+    const int p = blockIdx.x * WG + threadIdx.x;
+    if (p >= D.n) return;
+    int r = D.rank[p];
+    if (r < 0) return;
+    uint64_t key = D.key[p];
+    int bid = D.page_table[key >> bbits];
+    int64_t cell = (int64_t)bid * (1 << bbits) + (int)(key & ((1 << bbits) - 1));
+    D.perm[D.cell_start[cell] + r] = p;
+}
+
+// Order inside a cell by original particle index: together with the page-ordered
+// block ids and element-ordered cells this reproduces std::sort of the
+// reference's unique 64-bit keys (MpmSimulationBase.cpp:1552).  Shell sort keeps
+// crowded cells bounded.
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_cellsort(Dev<T, AccT> D)
+{
+    using B = Blk<T>;
+    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
+    const int64_t ncells = (int64_t)D.counters[0] * NB;
+    for (int64_t c = (int64_t)blockIdx.x * WG + threadIdx.x; c < ncells; c += (int64_t)gridDim.x * WG) {
+        const int cnt = D.cell_count[c];
+        if (cnt == 0) continue;
+        int* a = D.perm + D.cell_start[c];
+        int gap = 1;
+        while (gap < cnt / 3) gap = 3 * gap + 1;
+        for (; gap >= 1; gap /= 3)
+            for (int i = gap; i < cnt; i++) {
+                int v = a[i];
+                int kv = D.pid[v];
+                int j = i;
+                while (j >= gap && D.pid[a[j - gap]] > kv) {
+                    a[j] = a[j - gap];
+                    j -= gap;
+                }
+                a[j] = v;
+            }
+        const int s0 = D.cell_start[c];
+        for (int i = 0; i < cnt; i++) D.sorted_cell[s0 + i] = (int)c;
+    }
+}
+
+// ------------------------------------------------------------- P2G helpers
+template <class T, class AccT>
+struct Scatter {
+    // FP64 plain form: hardware f64 atomics
+    __device__ __forceinline__ static void add(AccT* p, T v, float, int&) { unsafeAtomicAdd(p, v); }
+};
+
+template <>
+struct Scatter<float, long long> {
+    // FP32 form: 64-bit fixed point, exact and order-independent
+    __device__ __forceinline__ static void add(long long* p, float v, float scale, int& ovf)
+    {
+        float q = rintf(v * scale);
+        if (!(fabsf(q) <= FIXED_LIMIT)) {
+            ovf = 1;
+            q = 0.f;
+        }
+        atomicAdd((unsigned long long*)p, (unsigned long long)(long long)q);
+    }
+};
+
+template <class T>
+__device__ __forceinline__ Mat3<T> loadM(const T* P, int64_t cap, int f, int p)
+{
+    Mat3<T> M;
+#pragma unroll
+    for (int i = 0; i < 9; i++) M.a[i / 3][i % 3] = P[(f + i) * cap + p];
+    return M;
+}
+
+// V0 * Kirchhoff stress of one particle, symmetric part (xx xy xz yy yz zz):
+// FBasedMpmForceHelper::updateState (FBasedMpmForceHelper.cpp:41-63) with
+// kirchhoff = P F^T (HyperelasticConstitutiveModel.h:75-80), or
+// JBasedMpmForceHelper::updateState (JBasedMpmForceHelper.cpp:28-50).  tau is
+// symmetric for every model on the path; its rounding-level skew is dropped.
+template <class T>
+__device__ __forceinline__ void vtauSym(const MatDev<T>& mt, const Mat3<T>& F, T V0, T g, T* S)
+{
+    Mat3<T> tau;
+    if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE) {
+        const T J = F.a[0][0];
+        const T s = V0 * (-mt.bulk * (dpow(J, -mt.gamma) - T(1))) * J;
+        S[0] = S[3] = S[5] = s;
+        S[1] = S[2] = S[4] = T(0);
+        return;
+    }
+    if (mt.model == ZMPM_MODEL_FIXED_COROTATED)
+        tau = corotatedTau(F, mt.mu, mt.lambda);
+    else if (mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
+        T J;
+        Mat3<T> Pk = qrStablePiola(F, mt.mu, mt.lambda, g, mt.kx, mt.ky, &J);
+        tau = mulT(Pk, F);
+    }
+    else
+        tau = neoHookeanTau(F, mt.mu, mt.lambda);
+    S[0] = V0 * tau.a[0][0];
+    S[1] = V0 * (T(0.5) * (tau.a[0][1] + tau.a[1][0]));
+    S[2] = V0 * (T(0.5) * (tau.a[0][2] + tau.a[2][0]));
+    S[3] = V0 * tau.a[1][1];
+    S[4] = V0 * (T(0.5) * (tau.a[1][2] + tau.a[2][1]));
+    S[5] = V0 * tau.a[2][2];
+}
+
+__device__ __forceinline__ float waveMax(float v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d));
+    return v;
+}
+
+// Upper bounds of one particle's P2G contributions per scale group: mass w m;
+// momentum w (m v + m C (x_i - x_p)) with |x_i - x_p| <= 1.5 dx; force
+// tau grad(w) with |grad w| <= 1/dx per axis; phase field w d and w (w <= 1).
+// Reduced over the wavefront and folded into the device maxima.
+template <class T>
+__device__ __forceinline__ void recordBounds(unsigned int* maxbits, bool live, T m, const T* v, const T* Cm,
+                                             const T* S, T d, T dx, T odx, bool dmg)
+{
+    float b[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
+    if (live) {
+        b[SC_M] = fabsf((float)m);
+        float mv = 0.f, f = 0.f;
+        const int sidx[3][3] = { { 0, 1, 2 }, { 1, 3, 4 }, { 2, 4, 5 } };
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            float cr = fabsf((float)Cm[3 * r]) + fabsf((float)Cm[3 * r + 1]) + fabsf((float)Cm[3 * r + 2]);
+            mv = fmaxf(mv, fabsf((float)m) * (fabsf((float)v[r]) + 1.5f * (float)dx * cr));
+            f = fmaxf(f, (float)odx * (fabsf((float)S[sidx[r][0]]) + fabsf((float)S[sidx[r][1]]) + fabsf((float)S[sidx[r][2]])));
+        }
+        b[SC_MV] = mv * 1.0001f;
+        b[SC_F] = f * 1.0001f;
+        if (dmg) {
+            b[SC_PF] = fabsf((float)d);
+            b[SC_PW] = 1.f;
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < NSC; g++) {
+        float mx = waveMax(b[g]);
+        if ((threadIdx.x & 63) == 0 && mx > 0.f) atomicMax(&maxbits[g], __float_as_uint(mx));
+    }
+}
+
+// Stress of the uploaded state and the P2G bounds of the first substep.
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D)
+{
+    const int p = blockIdx.x * WG + threadIdx.x;
+    const bool live = p < D.n;
+    T S[6] = { T(0), T(0), T(0), T(0), T(0), T(0) }, v[3] = { T(0), T(0), T(0) }, Cm[9], m = T(0), d = T(0);
+    bool dmg = false;
+    if (live) {
+        const int64_t cap = D.cap;
+        const MatDev<T>& mt = D.mats[D.pmat[p]];
+        Mat3<T> F = loadM(D.P, cap, PF, p);
+        vtauSym(mt, F, D.P[PVOL * cap + p], D.P[PG * cap + p], S);
+#pragma unroll
+        for (int i = 0; i < 6; i++) D.P[(PS + i) * cap + p] = S[i];
+        m = D.P[PM * cap + p];
+#pragma unroll
+        for (int a = 0; a < 3; a++) v[a] = D.P[(PV + a) * cap + p];
+#pragma unroll
+        for (int i = 0; i < 9; i++) Cm[i] = D.P[(PC + i) * cap + p];
+        dmg = D.any_damage && mt.has_damage;
+        d = D.P[PD * cap + p];
+    }
+    if constexpr (std::is_same<AccT, long long>::value)
+        recordBounds(D.maxbits, live, m, v, Cm, S, dmg ? d : T(0), D.dx, D.odx, D.any_damage != 0);
+}
+
+// Block-local LDS tile geometry: a block of BX x BY x BZ nodes plus the +2
+// halo its particles' quadratic stencils reach.
+template <class T>
+struct Tile {
+    using B = Blk<T>;
+    static constexpr int BX = 1 << B::xbits, BY = 1 << B::ybits, BZ = 1 << B::zbits;
+    static constexpr int NB = BX * BY * BZ;
+    static constexpr int TX = BX + 2, TY = BY + 2, TZ = BZ + 2, TN = TX * TY * TZ;
+    static constexpr int CHUNK = sizeof(T) == 4 ? 256 : 128; // sorted particles per workgroup
+};
+
+template <class T>
+struct alignas(4 * sizeof(T)) V4 {
+    T x, y, z, w;
+};
+
+template <class T, class AccT>
+__device__ __forceinline__ void ldsAdd(AccT* a, T v, float scale, int& ovf)
+{
+    if constexpr (std::is_same<AccT, long long>::value) {
+        float q = rintf((float)v * scale);
+        if (!(fabsf(q) <= FIXED_LIMIT)) {
+            ovf = 1;
+            q = 0.f;
+        }
+        if (q != 0.f) atomicAdd((unsigned long long*)a, (unsigned long long)(long long)q);
+    }
+    else {
+        if (v != T(0)) unsafeAtomicAdd(a, v);
+    }
+}
+
+// particlesToGridWithForceHelper<APIC, deg 2, !MLS> (MpmSimulationBase.cpp:791-854)
+// reading the stress of forcesUpdateParticleState (:953-979) from the particle,
+// plus the AnisoMPM phase-field transfer (AnisoFractureSimulation.h:161-176).
+//
+// One workgroup takes CHUNK consecutive sorted particles.  Phase A: one thread
+// per particle builds its P2G payload in LDS.  Phase B, per block segment of
+// the chunk: lanes are (cell, i, j) columns of the 27-node stencil; each lane
+// walks the particles of its cell (all share one base node) and accumulates
+// its 3 nodes x 9 quantities in registers, then adds them into an LDS tile of
+// the block + halo.  The tile is flushed to HBM with one atomic per non-zero
+// node value.  FP32 form: per-lane float partial sums in sorted order, then
+// int64 fixed point in LDS and HBM (result independent of scheduling).
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
+{
+    using B = Blk<T>;
+    using TL = Tile<T>;
+    constexpr int NB = TL::NB, CH = TL::CHUNK;
+    constexpr bool FIXED = std::is_same<AccT, long long>::value;
+    __shared__ V4<T> q[7][CH];
+    __shared__ T wl[4][CH][3];
+    __shared__ AccT tile[NGF][TL::TN];
+    __shared__ int cs[NB], ce[NB];
+
+    const int lo = blockIdx.x * CH;
+    const int hi = min(lo + CH, D.n);
+    if (lo >= hi) return;
+    const int t = threadIdx.x;
+    const bool dmg = D.any_damage != 0;
+    const int nq = dmg ? NGF : GPF;
+    const int64_t cap = D.cap;
+
+    // ---- phase A: payload of particle lo + t
+    if (t < CH && lo + t < hi) {
+        int p = D.perm[lo + t];
+        if ((unsigned)p >= (unsigned)D.n) p = 0;
+        const T* P = D.P;
+        const T mass = P[PM * cap + p];
+        T X[3], w[3][3], dw[3][3], xb[3];
+        int base[3];
+        const T hi_t = T(D.domain - 4);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            X[a] = P[(PX + a) * cap + p];
+            T xs = mul_rn(X[a], D.odx);
+            T tt = sub_rn(xs, T(0.5));
+            if (!(tt >= T(0) && tt < hi_t)) tt = (tt >= hi_t) ? hi_t - T(1) : T(0); // latched by k_bin
+            base[a] = (int)tt;
+            bspline(xs, base[a], w[a], dw[a]);
+            xb[a] = T(base[a]) * D.dx - X[a];
+        }
+        T mC[9], a3[3];
+#pragma unroll
+        for (int i = 0; i < 9; i++) mC[i] = mass * P[(PC + i) * cap + p];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            a3[r] = mass * P[(PV + r) * cap + p] + mC[3 * r] * xb[0] + mC[3 * r + 1] * xb[1] + mC[3 * r + 2] * xb[2];
+        T Sv[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) Sv[i] = P[(PS + i) * cap + p] * D.odx;
+        const T dp = dmg && D.mats[D.pmat[p]].has_damage ? P[PD * cap + p] : T(0);
+        const T dx = D.dx;
+        q[0][t] = V4<T> { mass, a3[0], a3[1], a3[2] };
+        q[1][t] = V4<T> { mC[0] * dx, mC[1] * dx, mC[2] * dx, mC[3] * dx };
+        q[2][t] = V4<T> { mC[4] * dx, mC[5] * dx, mC[6] * dx, mC[7] * dx };
+        q[3][t] = V4<T> { mC[8] * dx, Sv[0], Sv[1], Sv[2] };
+        q[4][t] = V4<T> { Sv[3], Sv[4], Sv[5], dp };
+        q[5][t] = V4<T> { w[2][0], w[2][1], w[2][2], T(0) };
+        q[6][t] = V4<T> { dw[2][0], dw[2][1], dw[2][2], T(0) };
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            wl[0][t][i] = w[0][i];
+            wl[1][t][i] = dw[0][i];
+            wl[2][t][i] = w[1][i];
+            wl[3][t][i] = dw[1][i];
+        }
+    }
+    int ovf = 0;
+    int seg_lo = lo;
+    while (seg_lo < hi) {
+        const int b = D.sorted_cell[seg_lo] / NB;
+        __syncthreads(); // payload written / previous segment's tile flushed
+        if (t < NB) {
+            const int c = b * NB + t;
+            cs[t] = D.cell_start[c];
+            ce[t] = D.cell_start[c] + D.cell_count[c];
+        }
+        for (int e = t; e < nq * TL::TN; e += WG) (&tile[0][0])[e] = AccT(0);
+        __syncthreads();
+        const int seg_hi = min(hi, ce[NB - 1]);
+        // ---- phase B: (cell, i, j) columns
+        for (int slot = t; slot < NB * 9; slot += WG) {
+            const int cell = slot / 9, col = slot - 9 * (slot / 9);
+            const int ci = col / 3, cj = col - 3 * (col / 3);
+            const int p0 = max(cs[cell], seg_lo) - lo, p1 = min(ce[cell], seg_hi) - lo;
+            if (p0 >= p1) continue;
+            const T fi = T(ci), fj = T(cj);
+            T acc[3][NGF];
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int f = 0; f < NGF; f++) acc[k][f] = T(0);
+            for (int pp = p0; pp < p1; pp++) {
+                const V4<T> q0 = q[0][pp], q1 = q[1][pp], q2 = q[2][pp], q3 = q[3][pp], q4 = q[4][pp];
+                const V4<T> q5 = q[5][pp], q6 = q[6][pp];
+                const T wx = wl[0][pp][ci], dwx = wl[1][pp][ci], wy = wl[2][pp][cj], dwy = wl[3][pp][cj];
+                const T wxy = wx * wy, gx = dwx * wy, gy = wx * dwy;
+                // a + B (i, j, 0): momentum + m C (x_i - x_p) at the column base
+                const T a0 = q0.y + q1.x * fi + q1.y * fj;
+                const T a1 = q0.z + q1.w * fi + q2.x * fj;
+                const T a2 = q0.w + q2.z * fi + q2.w * fj;
+                const T b0 = q1.z, b1 = q2.y, b2 = q3.x; // B column z
+                const T s00 = q3.y, s01 = q3.z, s02 = q3.w, s11 = q4.x, s12 = q4.y, s22 = q4.z;
+                const T wz[3] = { q5.x, q5.y, q5.z }, dwz[3] = { q6.x, q6.y, q6.z };
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const T wk = wxy * wz[k];
+                    const T g0 = gx * wz[k], g1 = gy * wz[k], g2 = wxy * dwz[k];
+                    const T fk = T(k);
+                    acc[k][GM] += q0.x * wk;
+                    acc[k][GMV + 0] += wk * (a0 + b0 * fk);
+                    acc[k][GMV + 1] += wk * (a1 + b1 * fk);
+                    acc[k][GMV + 2] += wk * (a2 + b2 * fk);
+                    acc[k][GFO + 0] -= s00 * g0 + s01 * g1 + s02 * g2;
+                    acc[k][GFO + 1] -= s01 * g0 + s11 * g1 + s12 * g2;
+                    acc[k][GFO + 2] -= s02 * g0 + s12 * g1 + s22 * g2;
+                    acc[k][GPF] += q4.w * wk;
+                    acc[k][GPW] += wk;
+                }
+            }
+            const int cx = cell >> (B::ybits + B::zbits), cy = (cell >> B::zbits) & (TL::BY - 1), cz = cell & (TL::BZ - 1);
+            const int nbase = ((cx + ci) * TL::TY + (cy + cj)) * TL::TZ + cz;
+            const float* sc = D.scales;
+            const float scl[NGF] = { sc[SC_M], sc[SC_MV], sc[SC_MV], sc[SC_MV], sc[SC_F], sc[SC_F], sc[SC_F], sc[SC_PF], sc[SC_PW] };
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int f = 0; f < NGF; f++)
+                    if (f < nq) ldsAdd<T, AccT>(&tile[f][nbase + k], acc[k][f], scl[f], ovf);
+        }
+        __syncthreads();
+        // ---- flush the tile: one HBM atomic per non-zero node value
+        const int* nb = D.nbr + b * 8;
+        const int64_t cn = D.cap_nodes;
+        for (int e = t; e < nq * TL::TN; e += WG) {
+            const int f = e / TL::TN, nd = e - TL::TN * (e / TL::TN);
+            const AccT v = tile[f][nd];
+            if (v == AccT(0)) continue;
+            const int tx = nd / (TL::TY * TL::TZ), ty = (nd / TL::TZ) % TL::TY, tz = nd % TL::TZ;
+            const int which = ((tx >= TL::BX) << 2) | ((ty >= TL::BY) << 1) | (tz >= TL::BZ);
+            const int tb = nb[which];
+            if (tb < 0) {
+                ovf |= 2;
+                continue;
+            }
+            const int elem = ((tx & (TL::BX - 1)) << (B::ybits + B::zbits)) | ((ty & (TL::BY - 1)) << B::zbits) | (tz & (TL::BZ - 1));
+            AccT* dst = D.acc + f * cn + (int64_t)tb * NB + elem;
+            if constexpr (FIXED)
+                atomicAdd((unsigned long long*)dst, (unsigned long long)v);
+            else
+                unsafeAtomicAdd(dst, v);
+        }
+        seg_lo = seg_hi;
+    }
+    if (ovf & 1) latch(D.counters, ST_OVERFLOW);
+    if (ovf & 2) latch(D.counters, ST_CAPACITY);
+}
+
+// Next substep's fixed-point scales: 2^(40 - e) with max = f 2^e, f in [0.5,1).
+__global__ void k_update_scales(float* scales, unsigned int* maxbits)
+{
+    const int g = threadIdx.x;
+    if (g < NSC) {
+        float m = __uint_as_float(maxbits[g]);
+        if (m > 0.f && isfinite(m)) {
+            int e;
+            frexpf(m, &e);
+            int se = (int)FIXED_TARGET_EXP - e;
+            se = se < -120 ? -120 : (se > 120 ? 120 : se);
+            scales[g] = ldexpf(1.0f, se);
+        }
+        maxbits[g] = 0u;
+    }
+}
+
+template <class T, class AccT>
+__device__ __forceinline__ T accValue(const Dev<T, AccT>& D, int f, int64_t node, int sg)
+{
+    if constexpr (std::is_same<AccT, long long>::value)
+        return (T)((double)D.acc[f * D.cap_nodes + node] * (1.0 / (double)D.scales[sg]));
+    else
+        return D.acc[f * D.cap_nodes + node];
+}
+
+// v = mv/m (MpmSimulationBase.cpp:655-666) and gridVelocityExplicitUpdate
+// (:1179-1228) with multiObjectCollision / detectAndResolveCollision
+// (CollisionObject.cpp:66-105, 341-406) at time t; phase-field normalisation
+// (AnisoFractureSimulation.h:177-179).
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt, double t)
+{
+    using B = Blk<T>;
+    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
+    const int64_t nn = (int64_t)D.counters[0] * NB;
+    const int64_t cn = D.cap_nodes;
+    for (int64_t node = (int64_t)blockIdx.x * WG + threadIdx.x; node < nn; node += (int64_t)gridDim.x * WG) {
+        const T m = accValue(D, GM, node, SC_M);
+        T f[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) f[a] = accValue(D, GFO + a, node, SC_F);
+        if (m == T(0)) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) D.newv[a * cn + node] = f[a];
+            if (D.any_damage) D.pfn[node] = T(0);
+            continue;
+        }
+        const int b = (int)(node / NB), e = (int)(node % NB);
+        uint64_t off = ((uint64_t)(uint32_t)D.block_page[b] << 12) | ((uint64_t)e << B::data_bits);
+        const T xi[3] = { T(bitPack(off, D.masks.x)) * D.dx, T(bitPack(off, D.masks.y)) * D.dx,
+            T(bitPack(off, D.masks.z)) * D.dx };
+        const T dt_over_mi = dt / m;
+        T vi[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            T v = accValue(D, GMV + a, node, SC_MV) / m;
+            vi[a] = v + D.gravity[a] * dt + f[a] * dt_over_mi;
+        }
+        for (int c = 0; c < D.num_cols; c++) {
+            const ColDev& co = D.cols[c];
+            if (co.type == ZMPM_GHOST) continue;
+            T X[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) X[a] = xi[a] - (T)(co.b0[a] + co.dbdt[a] * t);
+            T phi, nrm[3] = { T(0), T(0), T(0) };
+            if (co.shape == ZMPM_SHAPE_HALF_SPACE) {
+                phi = (T)co.n[0] * (X[0] - (T)co.origin[0]) + (T)co.n[1] * (X[1] - (T)co.origin[1])
+                    + (T)co.n[2] * (X[2] - (T)co.origin[2]);
+#pragma unroll
+                for (int a = 0; a < 3; a++) nrm[a] = (T)co.n[a];
+            }
+            else {
+                T dd[3], Xc[3];
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    T c0 = (T)((co.bmin[a] + co.bmax[a]) / 2), he = (T)((co.bmax[a] - co.bmin[a]) / 2);
+                    Xc[a] = X[a] - c0;
+                    dd[a] = dabs(Xc[a]) - he;
+                }
+                int am = 0;
+                if (dd[1] > dd[am]) am = 1;
+                if (dd[2] > dd[am]) am = 2;
+                T q2 = T(0);
+#pragma unroll
+                for (int a = 0; a < 3; a++) q2 += dd[a] > T(0) ? dd[a] * dd[a] : T(0);
+                T dmax = dd[am];
+                phi = (dmax < T(0) ? dmax : T(0)) + dsqrt(q2);
+                if (co.type != ZMPM_STICKY) {
+                    if (q2 > T(0)) {
+                        T qn = dsqrt(q2);
+#pragma unroll
+                        for (int a = 0; a < 3; a++)
+                            nrm[a] = (dd[a] > T(0) ? dd[a] / qn : T(0)) * (Xc[a] >= T(0) ? T(1) : T(-1));
+                    }
+                    else
+                        nrm[am] = Xc[am] >= T(0) ? T(1) : T(-1);
+                }
+            }
+            if (!(phi <= T(0))) continue;
+            const T vo[3] = { (T)co.dbdt[0], (T)co.dbdt[1], (T)co.dbdt[2] };
+            if (co.type == ZMPM_STICKY) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) vi[a] = vo[a];
+                break;
+            }
+            T vr[3] = { vi[0] - vo[0], vi[1] - vo[1], vi[2] - vo[2] };
+            T dot = vr[0] * nrm[0] + vr[1] * nrm[1] + vr[2] * nrm[2];
+            if (co.type == ZMPM_SLIP || dot < T(0)) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) vr[a] -= nrm[a] * dot;
+                T fr = (T)co.friction;
+                if (fr != T(0) && dot < T(0)) {
+                    T vn = dsqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+                    if (-dot * fr < vn) {
+#pragma unroll
+                        for (int a = 0; a < 3; a++) vr[a] += (vr[a] / vn) * dot * fr;
+                    }
+                    else
+                        vr[0] = vr[1] = vr[2] = T(0);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++) vi[a] = vr[a] + vo[a];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++) D.newv[a * cn + node] = vi[a];
+        if (D.any_damage) {
+            const T pw = accValue(D, GPW, node, SC_PW);
+            D.pfn[node] = pw != T(0) ? accValue(D, GPF, node, SC_PF) / pw : T(0);
+        }
+    }
+}
+
+// AnisoMPM explicit damage update of one particle (AnisoFractureSimulation.h:210-283,
+// structural tensor :365-391, undegraded Cauchy stress QRStableNeoHookean.h:83-93).
+template <class T>
+__device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>& F, T lap, T dt, T& d, T& g)
+{
+    const T geometricResist = d - (mt.l0 * mt.l0 * lap);
+    T J;
+    Mat3<T> Pk = qrStablePiola(F, mt.mu, mt.lambda, T(1), T(0), T(0), &J);
+    Mat3<T> cs = mulT(Pk, F);
+    Mat3<T> sym;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) sym.a[i][j] = T(0.5) * (cs.a[i][j] + cs.a[j][i]) / J;
+    T lam[3];
+    Mat3<T> Qe;
+    symEigen(sym, lam, Qe);
+    Mat3<T> sp;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            sp.a[i][j] = macaulay(lam[0]) * (Qe.a[i][0] * Qe.a[j][0]) + macaulay(lam[1]) * (Qe.a[i][1] * Qe.a[j][1])
+                + macaulay(lam[2]) * (Qe.a[i][2] * Qe.a[j][2]);
+    Mat3<T> R = polarRotation(F);
+    Mat3<T> A;
+    setIdentity(A);
+    for (int k = 0; k < mt.nd; k++) {
+        T a1[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) a1[i] = R.a[i][0] * mt.a0[k][0] + R.a[i][1] * mt.a0[k][1] + R.a[i][2] * mt.a0[k][2];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) A.a[i][j] += mt.alphas[k] * (a1[i] * a1[j]);
+    }
+    Mat3<T> As = mul(A, sp), sA = mul(sp, A);
+    T contraction = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) contraction += As.a[i][j] * sA.a[i][j];
+    const T phi = (T(1) / (mt.sigma_crit * mt.sigma_crit)) * contraction;
+    const T dTilde = mt.zeta * macaulay(phi - T(1));
+    const T diff = ((T(1) - d) * dTilde) - geometricResist;
+    const T newD = d + ((dt / mt.eta) * macaulay(diff));
+    d = newD < T(1) ? newD : T(1);
+    const T k = mt.k;
+    g = ((T(1) - d) * (T(1) - d) * (T(1) - k)) + k;
+}
+
+// gridToParticlesHelper<APIC, deg 2, !MLS> (MpmSimulationBase.cpp:1412-1478),
+// CFL flags (:1466-1499), evolveStrain (FBasedMpmForceHelper.cpp:93-108,
+// JBasedMpmForceHelper.cpp:79-94), plus the AnisoMPM Laplacian gather
+// (iterateKernelWithLaplacian, MpmGrid.h:552-623, with its 3D index quirk) and
+// damage update on F_n.  Output goes to sorted position s of the other buffer.
+// gridToParticlesHelper<APIC, deg 2, !MLS> (MpmSimulationBase.cpp:1412-1478),
+// CFL flags (:1466-1499), evolveStrain (FBasedMpmForceHelper.cpp:93-108,
+// JBasedMpmForceHelper.cpp:79-94), the AnisoMPM Laplacian gather
+// (iterateKernelWithLaplacian, MpmGrid.h:552-623, with its 3D index quirk) and
+// damage update on F_n, then the stress of (F_{n+1}, g_{n+1}) that the next
+// substep's forcesUpdateParticleState would compute, and the next P2G's
+// fixed-point bounds.  One thread per sorted particle; the grid velocities of
+// each block segment (block + halo) are staged in LDS first.  Output goes to
+// sorted position s of the other buffer.
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
+{
+    using B = Blk<T>;
+    using TL = Tile<T>;
+    constexpr int NB = TL::NB;
+    __shared__ T gt[4][TL::TN];
+    __shared__ int seg_end;
+    const int lo = blockIdx.x * WG;
+    const int hi = min(lo + WG, D.n);
+    if (lo >= hi) return;
+    const int t = threadIdx.x, s = lo + t;
+    const bool live = s < hi;
+    const int64_t cap = D.cap, cn = D.cap_nodes;
+    const T* P = D.P;
+    T* Q = D.Q;
+    int p = 0, mid = 0;
+    T Xp[3] = { T(0), T(0), T(0) }, w[3][3], dw[3][3];
+    int base[3] = { 0, 0, 0 };
+    if (live) {
+        p = D.perm[s];
+        if ((unsigned)p >= (unsigned)D.n) p = 0;
+        mid = D.pmat[p];
+        const T hi_t = T(D.domain - 4);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            Xp[a] = P[(PX + a) * cap + p];
+            T xs = mul_rn(Xp[a], D.odx);
+            T tt = sub_rn(xs, T(0.5));
+            if (!(tt >= T(0) && tt < hi_t)) tt = (tt >= hi_t) ? hi_t - T(1) : T(0);
+            base[a] = (int)tt;
+            bspline(xs, base[a], w[a], dw[a]);
+        }
+    }
+    const MatDev<T>& mt = D.mats[mid];
+    const bool dmg = live && D.any_damage && mt.has_damage;
+    const T odx = D.odx, dx = D.dx, odx2 = odx * odx;
+    T picV[3] = { T(0), T(0), T(0) }, Bp[3][3], gv[3][3], lap = T(0);
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        Bp[i / 3][i % 3] = T(0);
+        gv[i / 3][i % 3] = T(0);
+    }
+    const int nfield = D.any_damage ? 4 : 3;
+    int seg_lo = lo;
+    while (seg_lo < hi) {
+        const int b = D.sorted_cell[seg_lo] / NB;
+        __syncthreads(); // previous segment's gathers done
+        if (t == 0) {
+            const int c = b * NB + NB - 1;
+            seg_end = D.cell_start[c] + D.cell_count[c];
+        }
+        const int* nb = D.nbr + b * 8;
+        for (int e = t; e < nfield * TL::TN; e += WG) {
+            const int f = e / TL::TN, nd = e - TL::TN * (e / TL::TN);
+            const int tx = nd / (TL::TY * TL::TZ), ty = (nd / TL::TZ) % TL::TY, tz = nd % TL::TZ;
+            const int which = ((tx >= TL::BX) << 2) | ((ty >= TL::BY) << 1) | (tz >= TL::BZ);
+            const int tb = nb[which];
+            T v = T(0);
+            if (tb >= 0) {
+                const int elem = ((tx & (TL::BX - 1)) << (B::ybits + B::zbits)) | ((ty & (TL::BY - 1)) << B::zbits) | (tz & (TL::BZ - 1));
+                const int64_t node = (int64_t)tb * NB + elem;
+                v = f < 3 ? D.newv[f * cn + node] : D.pfn[node];
+            }
+            gt[f][nd] = v;
+        }
+        __syncthreads();
+        const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
+        if (live && s >= seg_lo && s < seg_hi) {
+            const int lx0 = base[0] & (TL::BX - 1), ly0 = base[1] & (TL::BY - 1), lz0 = base[2] & (TL::BZ - 1);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const T wi = w[0][i];
+                const T dwidxi = odx * dw[0][i];
+                const T ddwi = (i == 1) ? T(-2) : T(1);
+                const T xi = T(base[0] + i) * dx - Xp[0];
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const T wj = w[1][j];
+                    const T wij = wi * wj;
+                    const T dwijdxi = dwidxi * wj;
+                    const T dwijdxj = wi * odx * dw[1][j];
+                    const T yj = T(base[1] + j) * dx - Xp[1];
+                    const int nrow = ((lx0 + i) * TL::TY + (ly0 + j)) * TL::TZ + lz0;
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        const T wk = w[2][k];
+                        const T wijk = wij * wk;
+                        const T g3[3] = { dwijdxi * wk, dwijdxj * wk, wij * odx * dw[2][k] };
+                        const T zk = T(base[2] + k) * dx - Xp[2];
+                        const T nv[3] = { gt[0][nrow + k], gt[1][nrow + k], gt[2][nrow + k] };
+                        const T dxv[3] = { xi, yj, zk };
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            picV[a] += wijk * nv[a];
+                            const T wv = wijk * nv[a];
+#pragma unroll
+                            for (int c = 0; c < 3; c++) {
+                                Bp[a][c] += wv * dxv[c];
+                                gv[a][c] += nv[a] * g3[c];
+                            }
+                        }
+                        if (dmg) {
+                            // quirk: ddw[1](i), ddw[2](i) instead of (j), (k) (MpmGrid.h:610-612)
+                            const T L = odx2 * ddwi * wj * wk + odx2 * ddwi * wi * wk + odx2 * ddwi * wi * wj;
+                            lap += gt[3][nrow + k] * L;
+                        }
+                    }
+                }
+            }
+        }
+        seg_lo = seg_hi;
+    }
+    T m = T(0), Cn[9], S[6] = { T(0), T(0), T(0), T(0), T(0), T(0) }, dout = T(0);
+#pragma unroll
+    for (int i = 0; i < 9; i++) Cn[i] = T(0);
+    if (live) {
+        const T Dinv = T(4) / (dx * dx);
+        const T r = D.rpic;
+        const T ca = (r + T(1)) * T(0.5), cb = (r - T(1)) * T(0.5);
+#pragma unroll
+        for (int a = 0; a < 3; a++) Q[(PV + a) * cap + s] = picV[a];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                Cn[3 * a + c] = ca * (Bp[a][c] * Dinv) + cb * (Bp[c][a] * Dinv);
+                Q[(PC + 3 * a + c) * cap + s] = Cn[3 * a + c];
+            }
+        T inc2 = T(0);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            T inc = dt * picV[a];
+            inc2 += inc * inc;
+            Q[(PX + a) * cap + s] = Xp[a] + inc;
+        }
+        if (inc2 > dx * dx) latch(D.counters, ST_CFL);
+        T d = P[PD * cap + p], g = P[PG * cap + p];
+        const T V0 = P[PVOL * cap + p];
+        Mat3<T> Fn = loadM(P, cap, PF, p);
+        if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE)
+            Fn.a[0][0] = (T(1) + dt * (gv[0][0] + gv[1][1] + gv[2][2])) * Fn.a[0][0];
+        else {
+            const Mat3<T> F = Fn;
+            if (dmg && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) damageUpdate(mt, F, lap, dt, d, g);
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    T v = F.a[a][c];
+#pragma unroll
+                    for (int k = 0; k < 3; k++) v += dt * gv[a][k] * F.a[k][c];
+                    Fn.a[a][c] = v;
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < 9; i++) Q[(PF + i) * cap + s] = Fn.a[i / 3][i % 3];
+        // stress of the new state for the next substep's P2G
+        vtauSym(mt, Fn, V0, g, S);
+#pragma unroll
+        for (int i = 0; i < 6; i++) Q[(PS + i) * cap + s] = S[i];
+        m = P[PM * cap + p];
+        Q[PM * cap + s] = m;
+        Q[PVOL * cap + s] = V0;
+        Q[PD * cap + s] = d;
+        Q[PG * cap + s] = g;
+        Q[PLAP * cap + s] = dmg ? lap : T(0);
+        D.qid[s] = D.pid[p];
+        D.qmat[s] = mid;
+        dout = dmg ? d : T(0);
+    }
+    if constexpr (std::is_same<AccT, long long>::value)
+        recordBounds(D.maxbits, live, m, picV, Cn, S, dout, D.dx, D.odx, D.any_damage != 0);
+}
+
+} // namespace zmpm

@@ -1,0 +1,138 @@
+// zmpm_kernels.hpp -- gfx950 kernels of the explicit MPM substep.
+//
+// Data layout in HBM (per handle, one GPU):
+//   particles  SoA, field-major: P[f * cap + s] for the NPF scalar fields below,
+//              double-buffered; after each substep the live buffer holds the
+//              particles in the sorted order of that substep (G2P writes them
+//              there), so the next substep's gathers are nearly sequential.
+//              The stress V0*tau(F, g) of the current state is a particle field:
+//              G2P computes it right after F_{n+1} and g_{n+1} exist, so P2G reads
+//              6 stress components instead of F and every P2G contribution is
+//              bounded before P2G runs (exact fixed-point scales).
+//   page table int32[domain^3 / NB]: SPGrid page index -> dense block id or -1,
+//              rebuilt every substep by a scan over "touched" flags.
+//   grid       per dense block, NB nodes (4x4x4 for float, 2x4x4 for double, the
+//              reference's SPGrid block shapes), field-major accumulators:
+//              double (FP64 form) or int64 fixed point (FP32 form).
+#pragma once
+
+#include "zmpm_math.hpp"
+#include "zmpm.h"
+
+namespace zmpm {
+
+// particle scalar fields (field-major SoA)
+enum : int {
+    PX = 0,   // x[3]
+    PV = 3,   // v[3]
+    PC = 6,   // C[9] row-major
+    PF = 15,  // F[9] row-major (EOS: J in PF+0)
+    PM = 24,  // mass
+    PVOL = 25, // element measure V0
+    PD = 26,  // damage d
+    PG = 27,  // degradation g
+    PLAP = 28, // damage Laplacian (output)
+    PS = 29,   // V0 * Kirchhoff stress of the current F, symmetric: xx xy xz yy yz zz
+    NPF = 35
+};
+
+// grid accumulator fields
+enum : int {
+    GM = 0,    // mass
+    GMV = 1,   // momentum [3]
+    GFO = 4,   // force [3]
+    GPF = 7,   // sum w d
+    GPW = 8,   // sum w
+    NGF = 9
+};
+
+// fixed-point scale groups
+enum : int { SC_M = 0, SC_MV = 1, SC_F = 2, SC_PF = 3, SC_PW = 4, NSC = 5 };
+
+// latched device status bits
+enum : int {
+    ST_DOMAIN = 1,
+    ST_NAN = 2,
+    ST_CFL = 4,
+    ST_OVERFLOW = 8,
+    ST_CAPACITY = 16,
+    ST_HALF_CFL = 32
+};
+
+// SPGrid block shape of GridState<T,3> (64 B float -> 4x4x4, 128 B double -> 2x4x4).
+template <class T> struct Blk;
+template <> struct Blk<float> {
+    static constexpr int xbits = 2, ybits = 2, zbits = 2, data_bits = 6;
+};
+template <> struct Blk<double> {
+    static constexpr int xbits = 1, ybits = 2, zbits = 2, data_bits = 7;
+};
+
+template <class T>
+struct MatDev {
+    int model, has_damage, nd, pad;
+    T mu, lambda, kx, ky, bulk, gamma;
+    T a0[2][3];
+    T alphas[2];
+    T k, l0, sigma_crit, eta, zeta;
+};
+
+struct ColDev {
+    int shape, type;
+    double origin[3], n[3], bmin[3], bmax[3], b0[3], dbdt[3], friction;
+};
+
+struct Masks {
+    uint64_t x, y, z;
+};
+
+// Kernel-visible state of one handle.
+template <class T, class AccT>
+struct Dev {
+    // particles
+    T* P;          // live buffer (sorted order of the previous substep)
+    T* Q;          // G2P output buffer
+    int* pid;      // original index, live order
+    int* pmat;     // material index, live order
+    int* qid;
+    int* qmat;
+    int64_t cap;   // particle capacity (field stride)
+    int n;
+    // binning
+    uint64_t* key;     // (page << block_bits | elem) per live particle
+    int* rank;         // slot inside the cell
+    int* perm;         // sorted position -> live index
+    int* sorted_cell;  // sorted position -> dense cell id
+    int* touched;      // per page: 1 if a particle block or its -1 neighbour has particles
+    int* scan_tmp;     // per page exclusive scan / per cell scan scratch
+    int* page_table;   // per page: dense block id or -1
+    int64_t num_pages;
+    int* block_page;   // dense block id -> page index
+    int* nbr;          // [cap_blocks][8] forward neighbour block ids
+    int* cell_count;   // [cap_blocks*NB]
+    int* cell_start;   // [cap_blocks*NB]
+    int* counters;     // [0] num_blocks, [1] status bits
+    int cap_blocks;
+    int domain;
+    Masks masks;
+    // grid
+    AccT* acc;         // [NGF][cap_nodes]
+    T* newv;           // [3][cap_nodes]
+    T* pfn;            // [cap_nodes]
+    int64_t cap_nodes;
+    // fixed-point scales (float form only)
+    float* scales;     // [NSC] current scale 2^e
+    unsigned int* maxbits; // [NSC] max |contribution| of the running substep (float bits)
+    // tables
+    const MatDev<T>* mats;
+    const ColDev* cols;
+    int num_cols;
+    int any_damage;
+    // scene
+    T dx, odx;
+    T gravity[3];
+    T rpic;
+    T cfl;
+};
+
+} // namespace zmpm

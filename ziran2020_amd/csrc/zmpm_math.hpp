@@ -1,0 +1,668 @@
+// zmpm_math.hpp -- device math of the explicit MPM substep, templated on the
+// particle scalar T (float for the FP32 form, double for the FP64 plain form).
+//
+// Each routine names the reference lines whose behaviour it reproduces.  The
+// 3x3 SVD is the implicit-shift symmetric QR method of Gast, Fu, Jiang and
+// Teran (UCLA CAM report 16-04, 2016) that Ziran uses
+// (Lib/Ziran/Math/Linear/ImplicitQRSVD.h:354-517); its sign and ordering
+// conventions define U, sigma and V, so the steps follow the same sequence.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zmpm {
+
+template <class T> struct Lim;
+template <> struct Lim<float> {
+    static constexpr float eps = 1.1920928955078125e-7f;
+    static constexpr float svd_tol = 128 * eps; // ImplicitQRSVD.h:518-525
+};
+template <> struct Lim<double> {
+    static constexpr double eps = 2.220446049250313e-16;
+    static constexpr double svd_tol = 1024 * eps; // ImplicitQRSVD.h:526-533
+};
+
+__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float dabs(float x) { return fabsf(x); }
+__device__ __forceinline__ double dabs(double x) { return fabs(x); }
+__device__ __forceinline__ float dcopysign(float a, float b) { return copysignf(a, b); }
+__device__ __forceinline__ double dcopysign(double a, double b) { return copysign(a, b); }
+__device__ __forceinline__ float dlog(float x) { return logf(x); }
+__device__ __forceinline__ double dlog(double x) { return log(x); }
+__device__ __forceinline__ float dpow(float a, float b) { return powf(a, b); }
+__device__ __forceinline__ double dpow(double a, double b) { return pow(a, b); }
+// Products and differences that must not be fused into an FMA (base-node
+// flooring, MpmSimulationBase.cpp:1545 and BSplines.h:16-20).
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float sub_rn(float a, float b) { return __fsub_rn(a, b); }
+__device__ __forceinline__ double sub_rn(double a, double b) { return __dsub_rn(a, b); }
+
+// --------------------------------------------------------------- 3x3 matrices
+template <class T>
+struct Mat3 {
+    T a[3][3];
+    __device__ __forceinline__ T& operator()(int i, int j) { return a[i][j]; }
+    __device__ __forceinline__ const T& operator()(int i, int j) const { return a[i][j]; }
+};
+
+template <class T>
+__device__ __forceinline__ void setIdentity(Mat3<T>& M)
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) M.a[i][j] = (i == j) ? T(1) : T(0);
+}
+
+template <class T>
+__device__ __forceinline__ Mat3<T> mul(const Mat3<T>& A, const Mat3<T>& B)
+{
+    Mat3<T> r;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) r.a[i][j] = A.a[i][0] * B.a[0][j] + A.a[i][1] * B.a[1][j] + A.a[i][2] * B.a[2][j];
+    return r;
+}
+
+// A * B^T
+template <class T>
+__device__ __forceinline__ Mat3<T> mulT(const Mat3<T>& A, const Mat3<T>& B)
+{
+    Mat3<T> r;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) r.a[i][j] = A.a[i][0] * B.a[j][0] + A.a[i][1] * B.a[j][1] + A.a[i][2] * B.a[j][2];
+    return r;
+}
+
+template <class T>
+__device__ __forceinline__ T det3(const Mat3<T>& m)
+{
+    return m.a[0][0] * (m.a[1][1] * m.a[2][2] - m.a[1][2] * m.a[2][1])
+        - m.a[1][0] * (m.a[0][1] * m.a[2][2] - m.a[0][2] * m.a[2][1])
+        + m.a[2][0] * (m.a[0][1] * m.a[1][2] - m.a[0][2] * m.a[1][1]);
+}
+
+// J F^{-T} (cofactorMatrix, DenseExt.h:240-252)
+template <class T>
+__device__ __forceinline__ Mat3<T> cofactor(const Mat3<T>& Fm)
+{
+    const T(*F)[3] = Fm.a;
+    Mat3<T> A;
+    A.a[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+    A.a[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+    A.a[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+    A.a[1][0] = F[0][2] * F[2][1] - F[0][1] * F[2][2];
+    A.a[1][1] = F[0][0] * F[2][2] - F[0][2] * F[2][0];
+    A.a[1][2] = F[0][1] * F[2][0] - F[0][0] * F[2][1];
+    A.a[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+    A.a[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+    A.a[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+    return A;
+}
+
+// ------------------------------------------------------- Givens rotations
+// Conventions of Givens.h:32-181: c,s with (c -s; s c)(a; b) = (*; 0).
+template <class T>
+struct Giv {
+    T c, s;
+};
+
+template <class T>
+__device__ __forceinline__ Giv<T> givens(T a, T b) // Givens.h:72-85
+{
+    Giv<T> g { T(1), T(0) };
+    T d = a * a + b * b;
+    T sq = dsqrt(d);
+    if (sq != T(0)) {
+        T t = T(1) / sq;
+        g.c = a * t;
+        g.s = -b * t;
+    }
+    return g;
+}
+
+template <class T>
+__device__ __forceinline__ Giv<T> givensUnconventional(T a, T b) // Givens.h:92-105
+{
+    Giv<T> g { T(0), T(1) };
+    T d = a * a + b * b;
+    T sq = dsqrt(d);
+    if (sq != T(0)) {
+        T t = T(1) / sq;
+        g.s = a * t;
+        g.c = b * t;
+    }
+    return g;
+}
+
+template <class T, int I, int K>
+__device__ __forceinline__ void rowRot(Mat3<T>& A, Giv<T> g) // Givens.h:131-140
+{
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        T t1 = A.a[I][j], t2 = A.a[K][j];
+        A.a[I][j] = g.c * t1 - g.s * t2;
+        A.a[K][j] = g.s * t1 + g.c * t2;
+    }
+}
+
+template <class T, int I, int K>
+__device__ __forceinline__ void colRot(Mat3<T>& A, Giv<T> g) // Givens.h:150-159
+{
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        T t1 = A.a[j][I], t2 = A.a[j][K];
+        A.a[j][I] = g.c * t1 - g.s * t2;
+        A.a[j][K] = g.s * t1 + g.c * t2;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void zeroChase(Mat3<T>& H, Mat3<T>& U, Mat3<T>& V) // Givens.h:195-240
+{
+    Giv<T> r1 = givens(H.a[0][0], H.a[1][0]);
+    Giv<T> r2;
+    if (H.a[1][0] != T(0))
+        r2 = givens(H.a[0][0] * H.a[0][1] + H.a[1][0] * H.a[1][1], H.a[0][0] * H.a[0][2] + H.a[1][0] * H.a[1][2]);
+    else
+        r2 = givens(H.a[0][1], H.a[0][2]);
+    rowRot<T, 0, 1>(H, r1);
+    colRot<T, 1, 2>(H, r2);
+    colRot<T, 1, 2>(V, r2);
+    Giv<T> r3 = givens(H.a[1][1], H.a[2][1]);
+    rowRot<T, 1, 2>(H, r3);
+    colRot<T, 0, 1>(U, r1);
+    colRot<T, 1, 2>(U, r3);
+}
+
+template <class T>
+__device__ __forceinline__ T wilkinsonShift(T a1, T b1, T a2) // ImplicitQRSVD.h:260-272
+{
+    T d = T(0.5) * (a1 - a2);
+    T bs = b1 * b1;
+    return a2 - dcopysign(bs / (dabs(d) + dsqrt(d * d + bs)), d);
+}
+
+// 2x2 SVD of B[t..t+1][t..t+1] (ImplicitQRSVD.h:45-66, 116-175), then the
+// process<t> bookkeeping (:277-293).
+template <class T, int t>
+__device__ __forceinline__ void process(const Mat3<T>& B, Mat3<T>& U, T* sigma, Mat3<T>& V)
+{
+    constexpr int other = (t == 1) ? 0 : 2;
+    sigma[other] = B.a[other][other];
+    T A00 = B.a[t][t], A01 = B.a[t][t + 1], A10 = B.a[t + 1][t], A11 = B.a[t + 1][t + 1];
+    T x0 = A00 + A11, x1 = A10 - A01;
+    T den = dsqrt(x0 * x0 + x1 * x1);
+    Giv<T> u { T(1), T(0) };
+    if (den != T(0)) {
+        u.c = x0 / den;
+        u.s = -x1 / den;
+    }
+    T S00 = u.c * A00 - u.s * A10;
+    T S01 = u.c * A01 - u.s * A11;
+    T S11 = u.s * A01 + u.c * A11;
+    T cosine, sine, s0, s1;
+    T x = S00, y = S01, z = S11;
+    T y2 = y * y;
+    if (y2 == T(0)) {
+        cosine = T(1);
+        sine = T(0);
+        s0 = x;
+        s1 = z;
+    }
+    else {
+        T tau = T(0.5) * (x - z);
+        T w = dsqrt(tau * tau + y2);
+        T tt = (tau > T(0)) ? y / (tau + w) : y / (tau - w);
+        cosine = T(1) / dsqrt(tt * tt + T(1));
+        sine = -tt * cosine;
+        T c2 = cosine * cosine;
+        T csy = T(2) * cosine * sine * y;
+        T s2 = sine * sine;
+        s0 = c2 * x - csy + s2 * z;
+        s1 = s2 * x + csy + c2 * z;
+    }
+    Giv<T> v;
+    if (s0 < s1) {
+        T tmp = s0;
+        s0 = s1;
+        s1 = tmp;
+        v.c = -sine;
+        v.s = cosine;
+    }
+    else {
+        v.c = cosine;
+        v.s = sine;
+    }
+    Giv<T> uu { u.c * v.c - u.s * v.s, u.s * v.c + u.c * v.s };
+    sigma[t] = s0;
+    sigma[t + 1] = s1;
+    colRot<T, t, t + 1>(U, uu);
+    colRot<T, t, t + 1>(V, v);
+}
+
+template <class T>
+__device__ __forceinline__ void flipSign(int i, Mat3<T>& U, T* sigma)
+{
+    sigma[i] = -sigma[i];
+#pragma unroll
+    for (int r = 0; r < 3; r++) U.a[r][i] = -U.a[r][i];
+}
+
+template <class T, int I, int J>
+__device__ __forceinline__ void swapCols(Mat3<T>& A)
+{
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        T t = A.a[r][I];
+        A.a[r][I] = A.a[r][J];
+        A.a[r][J] = t;
+    }
+}
+
+template <class T, int I>
+__device__ __forceinline__ void negCol(Mat3<T>& A)
+{
+#pragma unroll
+    for (int r = 0; r < 3; r++) A.a[r][I] = -A.a[r][I];
+}
+
+template <class T>
+__device__ __forceinline__ void sort0(Mat3<T>& U, T* sigma, Mat3<T>& V) // ImplicitQRSVD.h:308-341
+{
+    if (dabs(sigma[1]) >= dabs(sigma[2])) {
+        if (sigma[1] < T(0)) {
+            flipSign(1, U, sigma);
+            flipSign(2, U, sigma);
+        }
+        return;
+    }
+    if (sigma[2] < T(0)) {
+        flipSign(1, U, sigma);
+        flipSign(2, U, sigma);
+    }
+    T t = sigma[1];
+    sigma[1] = sigma[2];
+    sigma[2] = t;
+    swapCols<T, 1, 2>(U);
+    swapCols<T, 1, 2>(V);
+    if (sigma[1] > sigma[0]) {
+        t = sigma[0];
+        sigma[0] = sigma[1];
+        sigma[1] = t;
+        swapCols<T, 0, 1>(U);
+        swapCols<T, 0, 1>(V);
+    }
+    else {
+        negCol<T, 2>(U);
+        negCol<T, 2>(V);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void sort1(Mat3<T>& U, T* sigma, Mat3<T>& V) // second sort<1> overload
+{
+    if (dabs(sigma[0]) >= sigma[1]) {
+        if (sigma[0] < T(0)) {
+            flipSign(0, U, sigma);
+            flipSign(2, U, sigma);
+        }
+        return;
+    }
+    T t = sigma[0];
+    sigma[0] = sigma[1];
+    sigma[1] = t;
+    swapCols<T, 0, 1>(U);
+    swapCols<T, 0, 1>(V);
+    if (dabs(sigma[1]) < dabs(sigma[2])) {
+        t = sigma[1];
+        sigma[1] = sigma[2];
+        sigma[2] = t;
+        swapCols<T, 1, 2>(U);
+        swapCols<T, 1, 2>(V);
+    }
+    else {
+        negCol<T, 1>(U);
+        negCol<T, 1>(V);
+    }
+    if (sigma[1] < T(0)) {
+        flipSign(1, U, sigma);
+        flipSign(2, U, sigma);
+    }
+}
+
+// A = U diag(sigma) V^T with U, V rotations; sigma sorted by decreasing
+// magnitude, only sigma[2] may be negative.  Returns the QR sweep count
+// (bounded: the loop also stops after 64 sweeps so every lane terminates).
+template <class T>
+__device__ __forceinline__ int svd3(const Mat3<T>& A, Mat3<T>& U, T* sigma, Mat3<T>& V)
+{
+    Mat3<T> B = A;
+    setIdentity(U);
+    setIdentity(V);
+    { // makeUpperBidiag (Givens.h:252-271)
+        Giv<T> r = givens(B.a[1][0], B.a[2][0]);
+        rowRot<T, 1, 2>(B, r);
+        colRot<T, 1, 2>(U, r);
+        zeroChase(B, U, V);
+    }
+    int count = 0;
+    T alpha_1 = B.a[0][0], beta_1 = B.a[0][1], alpha_2 = B.a[1][1], alpha_3 = B.a[2][2], beta_2 = B.a[1][2];
+    T gamma_1 = alpha_1 * beta_1, gamma_2 = alpha_2 * beta_2;
+    T tol = Lim<T>::svd_tol;
+    T sc = T(0.5) * dsqrt(alpha_1 * alpha_1 + alpha_2 * alpha_2 + alpha_3 * alpha_3 + beta_1 * beta_1 + beta_2 * beta_2);
+    tol *= (sc > T(1) ? sc : T(1));
+    while (dabs(beta_2) > tol && dabs(beta_1) > tol && dabs(alpha_1) > tol && dabs(alpha_2) > tol
+        && dabs(alpha_3) > tol && count < 64) {
+        T mu = wilkinsonShift(alpha_2 * alpha_2 + beta_1 * beta_1, gamma_2, alpha_3 * alpha_3 + beta_2 * beta_2);
+        Giv<T> r = givens(alpha_1 * alpha_1 - mu, gamma_1);
+        colRot<T, 0, 1>(B, r);
+        colRot<T, 0, 1>(V, r);
+        zeroChase(B, U, V);
+        alpha_1 = B.a[0][0];
+        beta_1 = B.a[0][1];
+        alpha_2 = B.a[1][1];
+        alpha_3 = B.a[2][2];
+        beta_2 = B.a[1][2];
+        gamma_1 = alpha_1 * beta_1;
+        gamma_2 = alpha_2 * beta_2;
+        count++;
+    }
+    if (dabs(beta_2) <= tol) {
+        process<T, 0>(B, U, sigma, V);
+        sort0(U, sigma, V);
+    }
+    else if (dabs(beta_1) <= tol) {
+        process<T, 1>(B, U, sigma, V);
+        sort1(U, sigma, V);
+    }
+    else if (dabs(alpha_2) <= tol) {
+        Giv<T> r1 = givensUnconventional(B.a[1][2], B.a[2][2]);
+        rowRot<T, 1, 2>(B, r1);
+        colRot<T, 1, 2>(U, r1);
+        process<T, 0>(B, U, sigma, V);
+        sort0(U, sigma, V);
+    }
+    else if (dabs(alpha_3) <= tol) {
+        Giv<T> r1 = givens(B.a[1][1], B.a[1][2]);
+        colRot<T, 1, 2>(B, r1);
+        colRot<T, 1, 2>(V, r1);
+        Giv<T> r2 = givens(B.a[0][0], B.a[0][2]);
+        colRot<T, 0, 2>(B, r2);
+        colRot<T, 0, 2>(V, r2);
+        process<T, 0>(B, U, sigma, V);
+        sort0(U, sigma, V);
+    }
+    else {
+        // alpha_1 ~ 0 (also reached if the sweep cap hits, which leaves B nearly diagonal)
+        Giv<T> r1 = givensUnconventional(B.a[0][1], B.a[1][1]);
+        rowRot<T, 0, 1>(B, r1);
+        colRot<T, 0, 1>(U, r1);
+        Giv<T> r2 = givensUnconventional(B.a[0][2], B.a[2][2]);
+        rowRot<T, 0, 2>(B, r2);
+        colRot<T, 0, 2>(U, r2);
+        process<T, 1>(B, U, sigma, V);
+        sort1(U, sigma, V);
+    }
+    return count;
+}
+
+// Rotation of the polar decomposition, R = U V^T (ImplicitQRSVD.h:546-558).
+// The float form switches to double when the two smallest signed singular
+// values nearly cancel (F close to inversion), where R is ill-conditioned.
+template <class T>
+__device__ __forceinline__ Mat3<T> polarRotation(const Mat3<T>& F)
+{
+    Mat3<T> U, V;
+    T s[3];
+    svd3(F, U, s, V);
+    return mulT(U, V);
+}
+
+template <>
+__device__ __forceinline__ Mat3<float> polarRotation<float>(const Mat3<float>& F)
+{
+    Mat3<float> U, V;
+    float s[3];
+    svd3(F, U, s, V);
+    if (!(s[1] + s[2] > 1e-2f * s[0])) {
+        Mat3<double> Fd, Ud, Vd;
+        double sd[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) Fd.a[i][j] = F.a[i][j];
+        svd3(Fd, Ud, sd, Vd);
+        Mat3<double> Rd = mulT(Ud, Vd);
+        Mat3<float> R;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) R.a[i][j] = (float)Rd.a[i][j];
+        return R;
+    }
+    return mulT(U, V);
+}
+
+// ------------------------------------------------------ constitutive models
+// Fixed corotated Kirchhoff stress tau = P F^T with P = 2 mu (F - R) + lambda (J-1) J F^{-T}
+// (CorotatedElasticity.cpp:55-76, HyperelasticConstitutiveModel.h:75-80).
+template <class T>
+__device__ __forceinline__ Mat3<T> corotatedTau(const Mat3<T>& F, T mu, T lambda)
+{
+    T J = det3(F);
+    Mat3<T> JFinvT = cofactor(F);
+    Mat3<T> R = polarRotation(F);
+    Mat3<T> P;
+    T two_mu = T(2) * mu, lj = lambda * (J - T(1));
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) P.a[i][j] = two_mu * (F.a[i][j] - R.a[i][j]) + lj * JFinvT.a[i][j];
+    return mulT(P, F);
+}
+
+// QRStableNeoHookean first Piola stress (QRStableNeoHookean.h:108-213 with
+// QrBasedHelper::evaluateP, QrBasedHelper.h:28-40).  J returned through Jout.
+template <class T>
+__device__ __forceinline__ Mat3<T> qrStablePiola(const Mat3<T>& F, T mu, T lambda, T g, T kx, T ky, T* Jout)
+{
+    // GivensQR (GivensQR.h:20-48): R = G F, Qt accumulates G
+    Mat3<T> R = F, Qt;
+    setIdentity(Qt);
+    {
+        Giv<T> r = givens(R.a[1][0], R.a[2][0]);
+        rowRot<T, 1, 2>(R, r);
+        rowRot<T, 1, 2>(Qt, r);
+        r = givens(R.a[0][0], R.a[1][0]);
+        rowRot<T, 0, 1>(R, r);
+        rowRot<T, 0, 1>(Qt, r);
+        r = givens(R.a[1][1], R.a[2][1]);
+        rowRot<T, 1, 2>(R, r);
+        rowRot<T, 1, 2>(Qt, r);
+    }
+    // Q = Qt^T; sign convention on columns of Q == rows of Qt (QRStableNeoHookean.h:125-139)
+    if (R.a[0][0] < T(0)) {
+        R.a[0][0] = -R.a[0][0];
+        R.a[0][1] = -R.a[0][1];
+        R.a[0][2] = -R.a[0][2];
+        R.a[2][2] = -R.a[2][2];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            Qt.a[0][c] = -Qt.a[0][c];
+            Qt.a[2][c] = -Qt.a[2][c];
+        }
+    }
+    if (R.a[1][1] < T(0)) {
+        R.a[1][1] = -R.a[1][1];
+        R.a[1][2] = -R.a[1][2];
+        R.a[2][2] = -R.a[2][2];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            Qt.a[1][c] = -Qt.a[1][c];
+            Qt.a[2][c] = -Qt.a[2][c];
+        }
+    }
+    T r00 = R.a[0][0], r01 = R.a[0][1], r02 = R.a[0][2], r11 = R.a[1][1], r12 = R.a[1][2], r22 = R.a[2][2];
+    T J = r00 * r11 * r22;
+    // dPsiHat/dR (upper triangular)
+    T gl = (J >= T(1)) ? g * lambda : lambda;
+    T vol = gl * (J - T(1)) * J;
+    T gm = g * mu, gmJ = g * mu * J;
+    T D00 = vol / r00 + gm * r00 - gmJ / r00 + kx * (r00 - T(1));
+    T D01 = gm * r01;
+    T D02 = gm * r02;
+    T D11 = vol / r11 + gm * r11 - gmJ / r11;
+    T D12 = gm * r12;
+    T D22 = vol / r22 + gm * r22 - gmJ / r22;
+    if (ky != T(0)) {
+        T ss = dsqrt(r01 * r01 + r11 * r11);
+        D01 += ky * (ss - T(1)) * r01 / ss;
+        D11 += ky * (ss - T(1)) * r11 / ss;
+    }
+    // A = D R^T (upper triangle, symmetrised)
+    T A00 = D00 * r00 + D01 * r01 + D02 * r02;
+    T A01 = D01 * r11 + D02 * r12;
+    T A02 = D02 * r22;
+    T A11 = D11 * r11 + D12 * r12;
+    T A12 = D12 * r22;
+    T A22 = D22 * r22;
+    // R^{-1} (upper triangular)
+    T i00 = T(1) / r00, i11 = T(1) / r11, i22 = T(1) / r22;
+    T i01 = -r01 * i00 * i11;
+    T i12 = -r12 * i11 * i22;
+    T i02 = (r01 * r12 - r02 * r11) * i00 * i11 * i22;
+    // M = A R^{-T}:  (R^{-T})_{kj} = Rinv_{jk}
+    Mat3<T> M;
+    T Am[3][3] = { { A00, A01, A02 }, { A01, A11, A12 }, { A02, A12, A22 } };
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        M.a[i][0] = Am[i][0] * i00 + Am[i][1] * i01 + Am[i][2] * i02;
+        M.a[i][1] = Am[i][1] * i11 + Am[i][2] * i12;
+        M.a[i][2] = Am[i][2] * i22;
+    }
+    // P = Q M = Qt^T M
+    Mat3<T> P;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) P.a[i][j] = Qt.a[0][i] * M.a[0][j] + Qt.a[1][i] * M.a[1][j] + Qt.a[2][i] * M.a[2][j];
+    *Jout = J;
+    return P;
+}
+
+// Neo-Hookean Kirchhoff stress tau = mu F F^T + (lambda log J - mu) I (NeoHookean.cpp:38-43).
+template <class T>
+__device__ __forceinline__ Mat3<T> neoHookeanTau(const Mat3<T>& F, T mu, T lambda)
+{
+    T J = det3(F);
+    T scale = lambda * dlog(J) - mu;
+    Mat3<T> tau = mulT(F, F);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) tau.a[i][j] = mu * tau.a[i][j] + (i == j ? scale : T(0));
+    return tau;
+}
+
+// ------------------------------------------------------ symmetric eigen (Jacobi)
+// Unit eigenvectors in the columns of Q.  Stands in for Eigen::EigenSolver at
+// AnisoFractureSimulation.h:244-254 (sigma^+ does not depend on the basis
+// chosen in a degenerate eigenspace or on eigenvector signs).
+template <class T>
+__device__ __forceinline__ void jacobiRot(Mat3<T>& S, Mat3<T>& Q, int p, int q)
+{
+    T apq = S.a[p][q];
+    if (apq == T(0)) return;
+    T theta = (S.a[q][q] - S.a[p][p]) / (T(2) * apq);
+    T t = (theta >= T(0) ? T(1) : T(-1)) / (dabs(theta) + dsqrt(theta * theta + T(1)));
+    T c = T(1) / dsqrt(t * t + T(1)), s = t * c;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        T skp = S.a[k][p], skq = S.a[k][q];
+        S.a[k][p] = c * skp - s * skq;
+        S.a[k][q] = s * skp + c * skq;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        T spk = S.a[p][k], sqk = S.a[q][k];
+        S.a[p][k] = c * spk - s * sqk;
+        S.a[q][k] = s * spk + c * sqk;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        T qkp = Q.a[k][p], qkq = Q.a[k][q];
+        Q.a[k][p] = c * qkp - s * qkq;
+        Q.a[k][q] = s * qkp + c * qkq;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void symEigen(Mat3<T> S, T* lam, Mat3<T>& Q)
+{
+    setIdentity(Q);
+    for (int sweep = 0; sweep < 12; sweep++) {
+        T off = S.a[0][1] * S.a[0][1] + S.a[0][2] * S.a[0][2] + S.a[1][2] * S.a[1][2];
+        T diag = S.a[0][0] * S.a[0][0] + S.a[1][1] * S.a[1][1] + S.a[2][2] * S.a[2][2];
+        if (off <= T(1e-2) * Lim<T>::eps * Lim<T>::eps * diag || off == T(0)) break;
+        jacobiRot(S, Q, 0, 1);
+        jacobiRot(S, Q, 0, 2);
+        jacobiRot(S, Q, 1, 2);
+    }
+    lam[0] = S.a[0][0];
+    lam[1] = S.a[1][1];
+    lam[2] = S.a[2][2];
+}
+
+template <class T>
+__device__ __forceinline__ T macaulay(T x) { return (x + dabs(x)) / T(2); } // MathTools.h:276-281
+
+// ------------------------------------------------------ quadratic B-spline
+// BSplines.h:16-20 (base node) and :72-98 (weights); base computed without FMA.
+template <class T>
+__device__ __forceinline__ int baseNodeOf(T x_index)
+{
+    T t = sub_rn(x_index, T(0.5));
+    int i = (int)t;
+    return i - (T(i) > t);
+}
+
+template <class T>
+__device__ __forceinline__ void bspline(T x_index, int base, T* w, T* dw)
+{
+    T d0 = x_index - T(base);
+    T z = T(1.5) - d0;
+    w[0] = T(0.5) * z * z;
+    T d1 = d0 - T(1);
+    w[1] = T(0.75) - d1 * d1;
+    T d2 = T(1) - d1;
+    T zz = T(1.5) - d2;
+    w[2] = T(0.5) * zz * zz;
+    dw[0] = -z;
+    dw[1] = T(-2) * d1;
+    dw[2] = zz;
+}
+
+// ------------------------------------------------------ SPGrid bit layout
+// Bit_Spread_u64 (SPGrid_Utilities.h:81-131) for non-negative coordinates:
+// deposit the low bits of v into the set bits of mask, low to high.
+__device__ __forceinline__ uint64_t bitSpread(uint32_t v, uint64_t mask)
+{
+    uint64_t r = 0;
+    uint32_t bit = 0;
+    while (mask && bit < 13) {
+        uint64_t low = mask & (~mask + 1);
+        if (v & (1u << bit)) r |= low;
+        mask ^= low;
+        bit++;
+    }
+    return r;
+}
+
+} // namespace zmpm

@@ -1,0 +1,264 @@
+"""Seeded synthetic scenes for the five benchmark workloads and the tests.
+
+Scene-building semantics follow the reference's initialisation helpers:
+particle mass = rho * V / N and element measure = V / N
+(MpmInitializationHelper.h:637-647, MpmParticleHandleBase.cpp:1211-1213); the
+fibre of an AnisoMPM particle lives in its initial F = [a, a_perp, a x a_perp]
+with a_0 = e1 (MpmParticleHandleBase.cpp:196-268); domain walls are six sticky
+half-spaces at 5 dx from the faces of the addressable domain
+(MpmInitializationHelper.h:1498-1513, examples/paramEffects.h:85).
+Sampling uses numpy's PCG64 generator with the stated seed (the reference's
+PoissonDisk sampler is not restated).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi as A
+
+
+@dataclass
+class Scene:
+    name: str
+    dx: float
+    domain_nodes: int
+    dt: float
+    steps: int
+    gravity: tuple
+    materials: list
+    colliders: list
+    x: np.ndarray
+    material: np.ndarray
+    rho: np.ndarray  # per-particle density used for mass
+    V0: np.ndarray
+    F: np.ndarray
+    v: np.ndarray = None
+    C: np.ndarray = None
+    info: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        n = self.x.shape[0]
+        if self.v is None:
+            self.v = np.zeros((n, 3))
+        if self.C is None:
+            self.C = np.zeros((n, 9))
+        self.m = self.rho * self.V0
+
+    @property
+    def n(self):
+        return int(self.x.shape[0])
+
+    def make(self, lib, precision=A.PREC_FP64, device=0, apic_rpic_ratio=1.0):
+        sim = A.Simulation(lib, dx=self.dx, precision=precision, domain_nodes=self.domain_nodes,
+                           gravity=self.gravity, apic_rpic_ratio=apic_rpic_ratio, device=device)
+        sim.set_materials(self.materials)
+        for c in self.colliders:
+            sim.add_collider(c)
+        sim.upload(self.x, self.v, self.C, self.F, self.m, self.V0, self.material)
+        return sim
+
+
+def domain_walls(domain_nodes, dx, type_=A.STICKY):
+    """addAllWallsInDomain(domain*dx, 5*dx, type) (MpmInitializationHelper.h:1498-1513)."""
+    L, off = domain_nodes * dx, 5 * dx
+    out = []
+    for d in range(3):
+        for s in (-1, 1):
+            o = [0.0, 0.0, 0.0]
+            nrm = [0.0, 0.0, 0.0]
+            o[d] = off if s == 1 else L - off
+            nrm[d] = float(s)
+            out.append(A.half_space(o, nrm, type_))
+    return out
+
+
+def stratified_box(rng, lo, hi, dx, ppc_axis):
+    """ppc_axis^3 particles per cell, one per sub-cell, uniformly jittered inside it."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    ncell = np.round((hi - lo) / dx).astype(int)
+    h = dx / ppc_axis
+    nsub = ncell * ppc_axis
+    g = np.stack(np.meshgrid(*[np.arange(k) for k in nsub], indexing="ij"), -1).reshape(-1, 3)
+    x = lo + (g + rng.random(g.shape)) * h
+    return x
+
+
+def uniform_box(rng, lo, hi, n):
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    return lo + rng.random((n, 3)) * (hi - lo)
+
+
+def fibre_F(rng, n, max_deg=15.0):
+    """Per-particle fibre a = x-hat rotated by U(-max,max) degrees about a random axis
+    perpendicular to x-hat, stored as F0 = [a, a_perp, a x a_perp]
+    (MpmParticleHandleBase::initializeRotatedFHelper, MpmParticleHandleBase.cpp:224-268)."""
+    th = np.deg2rad(rng.uniform(-max_deg, max_deg, n))
+    ph = rng.uniform(0, 2 * np.pi, n)
+    a = np.stack([np.cos(th), np.sin(th) * np.sin(ph), -np.sin(th) * np.cos(ph)], 1)
+    second = np.stack([-a[:, 1], a[:, 0], np.zeros(n)], 1)
+    second /= np.linalg.norm(second, axis=1, keepdims=True)
+    third = np.cross(a, second)
+    F = np.stack([a, second, third], 2)  # columns
+    return F.reshape(n, 9)
+
+
+def identity_F(n):
+    return np.tile(np.eye(3).reshape(1, 9), (n, 1))
+
+
+# ------------------------------------------------------------------ workloads
+def elastic_cube(n_side_cells=32, dx=1.0 / 64, ppc_axis=2, seed=1, steps=100, dt=1e-4, domain_nodes=64,
+                 E=1e4, nu=0.3, rho=1000.0):
+    """Plain explicit APIC, FP64: elastic cube (0.5 m side at the defaults) on a
+    sticky floor at y = 8 dx, fixed-corotated E=1e4 nu=0.3 rho=1000, g=-9.8,
+    8 ppc stratified jitter -> 262,144 particles."""
+    rng = np.random.default_rng(seed)
+    lo = np.array([16 * dx, 8 * dx, 16 * dx])
+    hi = lo + n_side_cells * dx
+    x = stratified_box(rng, lo, hi, dx, ppc_axis)
+    n = x.shape[0]
+    V0 = np.full(n, (n_side_cells * dx) ** 3 / n)
+    cols = domain_walls(domain_nodes, dx) + [A.half_space((0, 8 * dx, 0), (0, 1, 0), A.STICKY)]
+    return Scene("elastic_cube", dx, domain_nodes, dt, steps, (0.0, -9.8, 0.0),
+                 [A.make_material(A.MODEL_FIXED_COROTATED, E, nu)], cols, x,
+                 np.zeros(n, np.int32), np.full(n, rho), V0, identity_F(n))
+
+
+def damage_slab(n=1_000_000, dx=1.0 / 128, size=(1.0, 0.2, 0.2), seed=2, steps=200, dt=1e-4,
+                domain_nodes=256, E=1e4, nu=0.3, rho=1000.0, alpha=-0.9, sigma_crit=5e3,
+                eta=0.1, zeta=1.0, residual=0.001, pull_speed=0.05):
+    """AnisoMPM phase-field damage: n particles uniformly in a slab, per-particle fibre
+    +-15 deg about x in F0, A = I + alpha (a x a), l0 = 2 dx, sigma_c = 5e3 Pa,
+    QRStableNeoHookean; bottom face held by a fixed sticky box, top face pulled by
+    a sticky box moving +y (examples/paramEffects.h:129-154)."""
+    rng = np.random.default_rng(seed)
+    size = np.asarray(size, float)
+    lo = np.array([16 * dx, 16 * dx, 16 * dx])
+    hi = lo + size
+    x = uniform_box(rng, lo, hi, n)
+    V0 = np.full(n, float(np.prod(size)) / n)
+    mat = A.make_material(A.MODEL_QR_STABLE_NEO_HOOKEAN, E, nu, has_damage=True, directors=((1, 0, 0),),
+                          alphas=(alpha,), residual_phase=residual, l0=2 * dx, sigma_crit=sigma_crit,
+                          eta=eta, zeta=zeta)
+    m = 4 * dx
+    top = A.box((lo[0] - m, hi[1] - 2 * dx, lo[2] - m), (hi[0] + m, hi[1] + 8 * dx, hi[2] + m),
+                A.STICKY, dbdt=(0.0, pull_speed, 0.0))
+    bottom = A.box((lo[0] - m, lo[1] - 8 * dx, lo[2] - m), (hi[0] + m, lo[1] + 2 * dx, hi[2] + m), A.STICKY)
+    cols = domain_walls(domain_nodes, dx) + [top, bottom]
+    return Scene("damage_slab", dx, domain_nodes, dt, steps, (0.0, 0.0, 0.0), [mat], cols, x,
+                 np.zeros(n, np.int32), np.full(n, rho), V0, fibre_F(rng, n))
+
+
+def sparse_spheres(n=8_000_000, n_spheres=20, dx=1.0 / 256, seed=3, steps=100, dt=1e-4, domain_nodes=512,
+                   E_range=(5e3, 5e4), nu=0.3, rho=1000.0, r_range=(0.05, 0.15)):
+    """Large sparse scene: n particles uniformly inside non-overlapping spheres
+    (radii U[0.05,0.15] m, centres in a 1 m^3 box), one fixed-corotated material
+    per sphere with E ~ U[5e3, 5e4], gravity, sticky floor."""
+    rng = np.random.default_rng(seed)
+    off = 16 * dx
+    cs, rs = [], []
+    while len(cs) < n_spheres:
+        r = rng.uniform(*r_range)
+        c = off + r + rng.random(3) * (1.0 - 2 * r)
+        if all(np.linalg.norm(c - c2) > r + r2 + 2 * dx for c2, r2 in zip(cs, rs)):
+            cs.append(c)
+            rs.append(r)
+    vol = np.array([4.0 / 3.0 * np.pi * r ** 3 for r in rs])
+    counts = np.floor(n * vol / vol.sum()).astype(int)
+    counts[: n - counts.sum()] += 1
+    xs, mats_idx, V0 = [], [], []
+    for s, (c, r, k) in enumerate(zip(cs, rs, counts)):
+        u = rng.normal(size=(k, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        rad = r * rng.random(k) ** (1.0 / 3.0)
+        xs.append(c + u * rad[:, None])
+        mats_idx.append(np.full(k, s, np.int32))
+        V0.append(np.full(k, vol[s] / k))
+    x = np.concatenate(xs)
+    Es = rng.uniform(*E_range, n_spheres)
+    mats = [A.make_material(A.MODEL_FIXED_COROTATED, float(E), nu) for E in Es]
+    cols = domain_walls(domain_nodes, dx) + [A.half_space((0, off, 0), (0, 1, 0), A.STICKY)]
+    nn = x.shape[0]
+    return Scene("sparse_spheres", dx, domain_nodes, dt, steps, (0.0, -9.8, 0.0), mats, cols, x,
+                 np.concatenate(mats_idx), np.full(nn, rho), np.concatenate(V0), identity_F(nn),
+                 info=dict(radii=rs, E=list(Es)))
+
+
+def batched_scenes(n_scenes=64, n_range=(10_000, 500_000), dx=1.0 / 128, seed=4, steps=100, dt=1e-4,
+                   domain_nodes=512, tile_cells=48, E=1e4, nu=0.3, rho=1000.0, sigma_crit=5e3):
+    """Batched uneven scenes in one launch: n_scenes independent cubes with particle
+    counts log-uniform over n_range, each in its own tile of one world grid on a
+    shared sticky floor; odd scenes fixed-corotated, even scenes QRStableNeoHookean
+    with AnisoMPM damage."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), n_scenes)).astype(int)
+    per_row = int(np.ceil(np.sqrt(n_scenes)))
+    off = 16 * dx
+    xs, mid, V0, Fs = [], [], [], []
+    mats = [A.make_material(A.MODEL_FIXED_COROTATED, E, nu),
+            A.make_material(A.MODEL_QR_STABLE_NEO_HOOKEAN, E, nu, has_damage=True, alphas=(-0.9,),
+                            l0=2 * dx, sigma_crit=sigma_crit)]
+    for s, k in enumerate(counts):
+        side = (k / 8.0) ** (1.0 / 3.0) * dx  # ~8 particles per cell
+        side = min(side, (tile_cells - 8) * dx)
+        ix, iz = s % per_row, s // per_row
+        lo = np.array([off + ix * tile_cells * dx, off, off + iz * tile_cells * dx])
+        xs.append(uniform_box(rng, lo, lo + side, k))
+        damaged = (s % 2 == 0)
+        mid.append(np.full(k, 1 if damaged else 0, np.int32))
+        V0.append(np.full(k, side ** 3 / k))
+        Fs.append(fibre_F(rng, k) if damaged else identity_F(k))
+    x = np.concatenate(xs)
+    n = x.shape[0]
+    cols = domain_walls(domain_nodes, dx) + [A.half_space((0, off, 0), (0, 1, 0), A.STICKY)]
+    return Scene("batched_scenes", dx, domain_nodes, dt, steps, (0.0, -9.8, 0.0), mats, cols, x,
+                 np.concatenate(mid), np.full(n, rho), np.concatenate(V0), np.concatenate(Fs),
+                 info=dict(counts=counts.tolist()))
+
+
+def two_material(n=16_000_000, dx=1.0 / 256, seed=5, steps=100, dt=1e-4, domain_nodes=512,
+                 E=1e5, nu=0.4, bulk=1e4, gamma=7.0, rho_solid=1000.0, rho_fluid=1000.0):
+    """Neo-Hookean block (8 ppc) inside a weakly compressible J-based EOS fluid column
+    (2 ppc), n particles in total."""
+    rng = np.random.default_rng(seed)
+    off = 16 * dx
+    col_lo = np.array([off, off, off])
+    col_size = np.array([0.9, 0.8, 0.9])
+    blk_size = np.array([0.25, 0.25, 0.25])
+    blk_lo = col_lo + np.array([0.325, 0.3, 0.325])
+    v_col = float(np.prod(col_size)) - float(np.prod(blk_size))
+    v_blk = float(np.prod(blk_size))
+    # 2 ppc fluid, 8 ppc solid, then scaled to hit n exactly
+    n_f0 = 2 * v_col / dx ** 3
+    n_s0 = 8 * v_blk / dx ** 3
+    n_s = int(round(n * n_s0 / (n_f0 + n_s0)))
+    n_f = n - n_s
+    xs = uniform_box(rng, blk_lo, blk_lo + blk_size, n_s)
+    xf = np.empty((0, 3))
+    while xf.shape[0] < n_f:
+        cand = uniform_box(rng, col_lo, col_lo + col_size, int(1.3 * (n_f - xf.shape[0])) + 16)
+        inside = np.all((cand >= blk_lo) & (cand <= blk_lo + blk_size), axis=1)
+        xf = np.concatenate([xf, cand[~inside]])
+    xf = xf[:n_f]
+    x = np.concatenate([xs, xf])
+    material = np.concatenate([np.zeros(n_s, np.int32), np.ones(n_f, np.int32)])
+    V0 = np.concatenate([np.full(n_s, v_blk / n_s), np.full(n_f, v_col / n_f)])
+    rho = np.concatenate([np.full(n_s, rho_solid), np.full(n_f, rho_fluid)])
+    F = identity_F(n)
+    F[n_s:, 0] = 1.0  # J = 1 for EOS particles
+    mats = [A.make_material(A.MODEL_NEO_HOOKEAN, E, nu),
+            A.make_material(A.MODEL_EQUATION_OF_STATE, 1.0, 0.0, bulk=bulk, gamma=gamma)]
+    cols = domain_walls(domain_nodes, dx) + [A.half_space((0, off, 0), (0, 1, 0), A.STICKY)]
+    return Scene("two_material", dx, domain_nodes, dt, steps, (0.0, -9.8, 0.0), mats, cols, x, material,
+                 rho, V0, F, info=dict(n_solid=n_s, n_fluid=n_f))
+
+
+WORKLOADS = {
+    "elastic_cube": elastic_cube,
+    "damage_slab": damage_slab,
+    "sparse_spheres": sparse_spheres,
+    "batched_scenes": batched_scenes,
+    "two_material": two_material,
+}
