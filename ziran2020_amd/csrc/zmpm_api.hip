@@ -146,6 +146,7 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.rank, cap);
     ALLOC(D.perm, cap);
     ALLOC(D.sorted_cell, cap);
+    ALLOC(D.spid, cap);
     ALLOC(D.touched, D.num_pages);
     ALLOC(D.scan_tmp, D.num_pages);
     ALLOC(D.page_table, D.num_pages);
@@ -158,15 +159,16 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.newv, 3 * D.cap_nodes);
     ALLOC(D.pfn, D.cap_nodes);
     ALLOC(D.scales, NSC);
-    ALLOC(D.maxbits, NSC);
+    ALLOC(D.maxbits, (int64_t)NSC * ((cap + WG - 1) / WG));
     h->max_tiles = (std::max<int64_t>(D.num_pages, D.cap_nodes) + SCAN_TILE - 1) / SCAN_TILE + 1;
     ALLOC(h->tile_sums, h->max_tiles);
 #undef ALLOC
     HIPCHK(h, hipMemsetAsync(D.touched, 0, D.num_pages * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.perm, 0, cap * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.sorted_cell, 0, cap * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.spid, 0, cap * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.counters, 0, 8 * sizeof(int), h->stream));
-    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * sizeof(unsigned int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * ((cap + WG - 1) / WG) * sizeof(unsigned int), h->stream));
     float ones[NSC] = { 1.f, 1.f, 1.f, 1.f, 1.f };
     HIPCHK(h, hipMemcpyAsync(D.scales, ones, sizeof(ones), hipMemcpyHostToDevice, h->stream));
     D.masks.x = spreadMask(B::data_bits, 0);
@@ -274,7 +276,7 @@ void launchStep(zmpm_context* h, double dt, double t, hipEvent_t* ev)
     const int gp = gridFor(D.n, WG, 1 << 30);
     if (ev) (void)hipEventRecord(ev[0], st);
     // fixed-point scales of this substep from the previous one's maxima
-    if (std::is_same<AccT, long long>::value) k_update_scales<<<1, 64, 0, st>>>(D.scales, D.maxbits);
+    if (std::is_same<AccT, long long>::value) k_update_scales<<<1, WG, 0, st>>>(D.scales, D.maxbits, D.max_slots);
     k_bin<T, AccT><<<gp, WG, 0, st>>>(D);
     launchScan<T, AccT>(h, D.touched, nullptr, 1, D.num_pages, D.scan_tmp, D.counters, D.cap_blocks, D.counters);
     k_assign<T, AccT><<<gridFor(D.num_pages, WG, 8192), WG, 0, st>>>(D);
@@ -282,7 +284,7 @@ void launchStep(zmpm_context* h, double dt, double t, hipEvent_t* ev)
     k_count<T, AccT><<<gp, WG, 0, st>>>(D);
     launchScan<T, AccT>(h, D.cell_count, D.counters, NB, 0, D.cell_start, nullptr, -1, D.counters);
     k_scatter<T, AccT><<<gp, WG, 0, st>>>(D);
-    k_cellsort<T, AccT><<<gridFor(D.cap_nodes, WG, 8192), WG, 0, st>>>(D);
+    k_cellsort<T, AccT><<<gridFor(D.cap_nodes, CELLSORT_WG, 16384), CELLSORT_WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[1], st);
     k_p2g<T, AccT><<<gridFor(D.n, Tile<T>::CHUNK, 1 << 30), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[2], st);
@@ -348,7 +350,7 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
     HIPCHK(h, hipMemcpyAsync(D.pid, ids.data(), cap * sizeof(int), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(D.pmat, mat.data(), cap * sizeof(int), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemsetAsync(D.counters, 0, 8 * sizeof(int), h->stream));
-    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * sizeof(unsigned int), h->stream));
+    D.max_slots = (n + WG - 1) / WG;
     if (n > 0) {
         // stress of the uploaded state and the first substep's fixed-point bounds
         k_stress_init<T, AccT><<<gridFor(n, WG, 1 << 30), WG, 0, h->stream>>>(D);

@@ -40,25 +40,56 @@ constexpr float FIXED_LIMIT = 2251799813685248.0f; // 2^51: per-contribution bou
 
 __device__ __forceinline__ void latch(int* counters, int bits) { atomicOr(&counters[1], bits); }
 
-__device__ __forceinline__ uint32_t bitPack(uint64_t data, uint64_t mask)
+// Every third bit: b9..b0 -> bit 3*i (block coordinates < 1024).
+__device__ __forceinline__ uint32_t part1by2(uint32_t x)
 {
-    uint32_t r = 0, bit = 0;
-    while (mask) {
-        uint64_t low = mask & (~mask + 1);
-        if (data & low) r |= 1u << bit;
-        mask ^= low;
-        bit++;
-        if (bit >= 32) break;
-    }
-    return r;
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t compact1by2(uint32_t x)
+{
+    x &= 0x09249249u;
+    x = (x | (x >> 2)) & 0x030c30c3u;
+    x = (x | (x >> 4)) & 0x0300f00fu;
+    x = (x | (x >> 8)) & 0x030000ffu;
+    x = (x | (x >> 16)) & 0x000003ffu;
+    return x;
+}
+
+// SPGrid page index of block coordinates: the page bits of Linear_Offset
+// (SPGrid_Mask.h:150-166) interleave the block coordinates, one bit per axis in
+// the order of Blk<T>::{x,y,z}pos.  Equal to bitSpread over the full masks
+// shifted right by 12 (pinned by the binning parity tests).
+template <class T>
+__device__ __forceinline__ uint32_t pageOfBlock(const Masks&, uint32_t Bx, uint32_t By, uint32_t Bz)
+{
+    using B = Blk<T>;
+    return (part1by2(Bx) << B::xpos) | (part1by2(By) << B::ypos) | (part1by2(Bz) << B::zpos);
 }
 
 template <class T>
-__device__ __forceinline__ uint32_t pageOfBlock(const Masks& M, uint32_t Bx, uint32_t By, uint32_t Bz)
+__device__ __forceinline__ void blockOfPage(uint32_t page, uint32_t& Bx, uint32_t& By, uint32_t& Bz)
 {
     using B = Blk<T>;
-    uint64_t off = bitSpread(Bx << B::xbits, M.x) | bitSpread(By << B::ybits, M.y) | bitSpread(Bz << B::zbits, M.z);
-    return (uint32_t)(off >> 12);
+    Bx = compact1by2(page >> B::xpos);
+    By = compact1by2(page >> B::ypos);
+    Bz = compact1by2(page >> B::zpos);
+}
+
+// (page, element) of a node: Linear_Offset(i, j, k) >> 12 and the element index
+// inside the page (x bits above y bits above z bits).
+template <class T>
+__device__ __forceinline__ void pageElem(uint32_t i, uint32_t j, uint32_t k, uint32_t& page, uint32_t& elem)
+{
+    using B = Blk<T>;
+    page = pageOfBlock<T>(Masks {}, i >> B::xbits, j >> B::ybits, k >> B::zbits);
+    elem = ((i & ((1u << B::xbits) - 1)) << (B::ybits + B::zbits)) | ((j & ((1u << B::ybits) - 1)) << B::zbits)
+        | (k & ((1u << B::zbits) - 1));
 }
 
 // ---------------------------------------------------------------- block scan
@@ -172,7 +203,6 @@ template <class T, class AccT>
 __global__ __launch_bounds__(WG) void k_bin(Dev<T, AccT> D)
 {
     using B = Blk<T>;
-    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
     const int p = blockIdx.x * WG + threadIdx.x;
     const bool active = p < D.n;
     uint32_t page = 0xffffffffu;
@@ -195,10 +225,8 @@ __global__ __launch_bounds__(WG) void k_bin(Dev<T, AccT> D)
             bb[a] = (int)t; // t >= 0: truncation is int_floor (MathTools.h:15-25)
         }
         if (bits) latch(D.counters, bits);
-        uint64_t off = bitSpread((uint32_t)bb[0], D.masks.x) | bitSpread((uint32_t)bb[1], D.masks.y)
-            | bitSpread((uint32_t)bb[2], D.masks.z);
-        page = (uint32_t)(off >> 12);
-        uint32_t elem = (uint32_t)((off >> B::data_bits) & (NB - 1));
+        uint32_t elem;
+        pageElem<T>((uint32_t)bb[0], (uint32_t)bb[1], (uint32_t)bb[2], page, elem);
         D.key[p] = ((uint64_t)page << (B::xbits + B::ybits + B::zbits)) | elem;
     }
     // one activation per run of equal pages inside the wavefront
@@ -249,10 +277,8 @@ __global__ __launch_bounds__(64) void k_prepare(Dev<T, AccT> D)
             D.cell_count[node] = 0;
         }
         if (t < 8) {
-            uint64_t off = (uint64_t)(uint32_t)D.block_page[b] << 12;
-            uint32_t Bx = bitPack(off, D.masks.x) >> B::xbits;
-            uint32_t By = bitPack(off, D.masks.y) >> B::ybits;
-            uint32_t Bz = bitPack(off, D.masks.z) >> B::zbits;
+            uint32_t Bx, By, Bz;
+            blockOfPage<T>((uint32_t)D.block_page[b], Bx, By, Bz);
             uint32_t nx = Bx + ((t >> 2) & 1), ny = By + ((t >> 1) & 1), nz = Bz + (t & 1);
             int id = -1;
             if ((nx << B::xbits) < (uint32_t)D.domain && (ny << B::ybits) < (uint32_t)D.domain
@@ -263,21 +289,36 @@ __global__ __launch_bounds__(64) void k_prepare(Dev<T, AccT> D)
     }
 }
 
+// Per-cell occupancy and each particle's slot in its cell.  Live order is the
+// previous substep's sorted order, so equal cells come in runs: one atomic per
+// run inside a wavefront, slots in live order within the run.
 template <class T, class AccT>
 __global__ __launch_bounds__(WG) void k_count(Dev<T, AccT> D)
 {
     using B = Blk<T>;
     constexpr int bbits = B::xbits + B::ybits + B::zbits;
     const int p = blockIdx.x * WG + threadIdx.x;
-    if (p >= D.n) return;
-    uint64_t key = D.key[p];
-    int bid = D.page_table[key >> bbits];
-    int r = -1;
-    if (bid >= 0)
-        r = atomicAdd(&D.cell_count[(int64_t)bid * (1 << bbits) + (int)(key & ((1 << bbits) - 1))], 1);
-    else
-        latch(D.counters, ST_CAPACITY);
-    D.rank[p] = r;
+    const int lane = threadIdx.x & 63;
+    int cell = -1 - lane; // distinct per inactive lane: never joins a run
+    if (p < D.n) {
+        const uint64_t key = D.key[p];
+        const int bid = D.page_table[key >> bbits];
+        if (bid >= 0)
+            cell = bid * (1 << bbits) + (int)(key & ((1 << bbits) - 1));
+        else
+            latch(D.counters, ST_CAPACITY);
+    }
+    const int prev = __shfl_up(cell, 1);
+    const bool head = (lane == 0) || (prev != cell);
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = heads & (~0ull >> (63 - lane));      // heads at lanes <= this lane
+    const int start = 63 - __clzll(upto);                       // this run's head lane
+    const uint64_t after = (lane == 63) ? 0ull : (heads & (~0ull << (lane + 1)));
+    const int next = after ? __ffsll((unsigned long long)after) - 1 : 64;
+    int base = 0;
+    if (head && cell >= 0) base = atomicAdd(&D.cell_count[cell], next - lane);
+    base = __shfl(base, start);
+    if (p < D.n) D.rank[p] = cell >= 0 ? base + (lane - start) : -1;
 }
 
 template <class T, class AccT>
@@ -292,37 +333,67 @@ __global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D)
     uint64_t key = D.key[p];
     int bid = D.page_table[key >> bbits];
     int64_t cell = (int64_t)bid * (1 << bbits) + (int)(key & ((1 << bbits) - 1));
-    D.perm[D.cell_start[cell] + r] = p;
+    const int pos = D.cell_start[cell] + r;
+    D.perm[pos] = p;
+    D.spid[pos] = D.pid[p];
 }
 
 // Order inside a cell by original particle index: together with the page-ordered
 // block ids and element-ordered cells this reproduces std::sort of the
-// reference's unique 64-bit keys (MpmSimulationBase.cpp:1552).  Shell sort keeps
-// crowded cells bounded.
+// reference's unique 64-bit keys (MpmSimulationBase.cpp:1552).  One thread per
+// cell sorts its (index, live slot) pairs in a private LDS slice; cells above
+// the slice size use a shell sort in place in HBM.
+constexpr int CELLSORT_WG = 64, CELLSORT_SLICE = 48;
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_cellsort(Dev<T, AccT> D)
+__global__ __launch_bounds__(CELLSORT_WG) void k_cellsort(Dev<T, AccT> D)
 {
     using B = Blk<T>;
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
+    __shared__ int kk[CELLSORT_SLICE][CELLSORT_WG];
+    __shared__ int vv[CELLSORT_SLICE][CELLSORT_WG];
+    const int t = threadIdx.x;
     const int64_t ncells = (int64_t)D.counters[0] * NB;
-    for (int64_t c = (int64_t)blockIdx.x * WG + threadIdx.x; c < ncells; c += (int64_t)gridDim.x * WG) {
+    for (int64_t c0 = (int64_t)blockIdx.x * CELLSORT_WG; c0 < ncells; c0 += (int64_t)gridDim.x * CELLSORT_WG) {
+        const int64_t c = c0 + t;
+        if (c >= ncells) continue;
         const int cnt = D.cell_count[c];
         if (cnt == 0) continue;
-        int* a = D.perm + D.cell_start[c];
-        int gap = 1;
-        while (gap < cnt / 3) gap = 3 * gap + 1;
-        for (; gap >= 1; gap /= 3)
-            for (int i = gap; i < cnt; i++) {
-                int v = a[i];
-                int kv = D.pid[v];
-                int j = i;
-                while (j >= gap && D.pid[a[j - gap]] > kv) {
-                    a[j] = a[j - gap];
-                    j -= gap;
-                }
-                a[j] = v;
-            }
         const int s0 = D.cell_start[c];
+        int* key = D.spid + s0;
+        int* val = D.perm + s0;
+        if (cnt <= CELLSORT_SLICE) {
+            for (int i = 0; i < cnt; i++) {
+                const int k = key[i], v = val[i];
+                int j = i;
+                while (j > 0 && kk[j - 1][t] > k) {
+                    kk[j][t] = kk[j - 1][t];
+                    vv[j][t] = vv[j - 1][t];
+                    j--;
+                }
+                kk[j][t] = k;
+                vv[j][t] = v;
+            }
+            for (int i = 0; i < cnt; i++) {
+                key[i] = kk[i][t];
+                val[i] = vv[i][t];
+            }
+        }
+        else {
+            int gap = 1;
+            while (gap < cnt / 3) gap = 3 * gap + 1;
+            for (; gap >= 1; gap /= 3)
+                for (int i = gap; i < cnt; i++) {
+                    const int k = key[i], v = val[i];
+                    int j = i;
+                    while (j >= gap && key[j - gap] > k) {
+                        key[j] = key[j - gap];
+                        val[j] = val[j - gap];
+                        j -= gap;
+                    }
+                    key[j] = k;
+                    val[j] = v;
+                }
+        }
         for (int i = 0; i < cnt; i++) D.sorted_cell[s0 + i] = (int)c;
     }
 }
@@ -365,29 +436,37 @@ __device__ __forceinline__ Mat3<T> loadM(const T* P, int64_t cap, int f, int p)
 template <class T>
 __device__ __forceinline__ void vtauSym(const MatDev<T>& mt, const Mat3<T>& F, T V0, T g, T* S)
 {
-    Mat3<T> tau;
+    T s0, s1, s2, s3, s4, s5;
     if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE) {
         const T J = F.a[0][0];
-        const T s = V0 * (-mt.bulk * (dpow(J, -mt.gamma) - T(1))) * J;
-        S[0] = S[3] = S[5] = s;
-        S[1] = S[2] = S[4] = T(0);
-        return;
+        const T p = V0 * (-mt.bulk * (dpow(J, -mt.gamma) - T(1))) * J;
+        s0 = s3 = s5 = p;
+        s1 = s2 = s4 = T(0);
     }
-    if (mt.model == ZMPM_MODEL_FIXED_COROTATED)
-        tau = corotatedTau(F, mt.mu, mt.lambda);
-    else if (mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
-        T J;
-        Mat3<T> Pk = qrStablePiola(F, mt.mu, mt.lambda, g, mt.kx, mt.ky, &J);
-        tau = mulT(Pk, F);
+    else {
+        Mat3<T> tau;
+        if (mt.model == ZMPM_MODEL_FIXED_COROTATED)
+            tau = corotatedTau(F, mt.mu, mt.lambda);
+        else if (mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
+            T J;
+            Mat3<T> Pk = qrStablePiola(F, mt.mu, mt.lambda, g, mt.kx, mt.ky, &J);
+            tau = mulT(Pk, F);
+        }
+        else
+            tau = neoHookeanTau(F, mt.mu, mt.lambda);
+        s0 = V0 * tau.a[0][0];
+        s1 = V0 * (T(0.5) * (tau.a[0][1] + tau.a[1][0]));
+        s2 = V0 * (T(0.5) * (tau.a[0][2] + tau.a[2][0]));
+        s3 = V0 * tau.a[1][1];
+        s4 = V0 * (T(0.5) * (tau.a[1][2] + tau.a[2][1]));
+        s5 = V0 * tau.a[2][2];
     }
-    else
-        tau = neoHookeanTau(F, mt.mu, mt.lambda);
-    S[0] = V0 * tau.a[0][0];
-    S[1] = V0 * (T(0.5) * (tau.a[0][1] + tau.a[1][0]));
-    S[2] = V0 * (T(0.5) * (tau.a[0][2] + tau.a[2][0]));
-    S[3] = V0 * tau.a[1][1];
-    S[4] = V0 * (T(0.5) * (tau.a[1][2] + tau.a[2][1]));
-    S[5] = V0 * tau.a[2][2];
+    S[0] = s0;
+    S[1] = s1;
+    S[2] = s2;
+    S[3] = s3;
+    S[4] = s4;
+    S[5] = s5;
 }
 
 __device__ __forceinline__ float waveMax(float v)
@@ -400,22 +479,23 @@ __device__ __forceinline__ float waveMax(float v)
 // Upper bounds of one particle's P2G contributions per scale group: mass w m;
 // momentum w (m v + m C (x_i - x_p)) with |x_i - x_p| <= 1.5 dx; force
 // tau grad(w) with |grad w| <= 1/dx per axis; phase field w d and w (w <= 1).
-// Reduced over the wavefront and folded into the device maxima.
+// Reduced over the workgroup into its own slot of the bound array.
 template <class T>
-__device__ __forceinline__ void recordBounds(unsigned int* maxbits, bool live, T m, const T* v, const T* Cm,
-                                             const T* S, T d, T dx, T odx, bool dmg)
+__device__ __forceinline__ void recordBounds(unsigned int* maxbits, int nslots, bool live, T m, const T* v,
+                                             const T* Cm, const T* S, T d, T dx, T odx, bool dmg)
 {
     float b[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
     if (live) {
         b[SC_M] = fabsf((float)m);
-        float mv = 0.f, f = 0.f;
-        const int sidx[3][3] = { { 0, 1, 2 }, { 1, 3, 4 }, { 2, 4, 5 } };
+        float mv = 0.f;
 #pragma unroll
         for (int r = 0; r < 3; r++) {
             float cr = fabsf((float)Cm[3 * r]) + fabsf((float)Cm[3 * r + 1]) + fabsf((float)Cm[3 * r + 2]);
             mv = fmaxf(mv, fabsf((float)m) * (fabsf((float)v[r]) + 1.5f * (float)dx * cr));
-            f = fmaxf(f, (float)odx * (fabsf((float)S[sidx[r][0]]) + fabsf((float)S[sidx[r][1]]) + fabsf((float)S[sidx[r][2]])));
         }
+        const float a0 = fabsf((float)S[0]), a1 = fabsf((float)S[1]), a2 = fabsf((float)S[2]);
+        const float a3 = fabsf((float)S[3]), a4 = fabsf((float)S[4]), a5 = fabsf((float)S[5]);
+        const float f = (float)odx * fmaxf(a0 + a1 + a2, fmaxf(a1 + a3 + a4, a2 + a4 + a5));
         b[SC_MV] = mv * 1.0001f;
         b[SC_F] = f * 1.0001f;
         if (dmg) {
@@ -423,10 +503,20 @@ __device__ __forceinline__ void recordBounds(unsigned int* maxbits, bool live, T
             b[SC_PW] = 1.f;
         }
     }
+    // workgroup maximum -> this workgroup's slot (no contended atomics)
+    __shared__ float wmax[NSC][WG / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int g = 0; g < NSC; g++) {
-        float mx = waveMax(b[g]);
-        if ((threadIdx.x & 63) == 0 && mx > 0.f) atomicMax(&maxbits[g], __float_as_uint(mx));
+        const float mx = waveMax(b[g]);
+        if (lane == 0) wmax[g][wv] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x < NSC) {
+        float mx = 0.f;
+#pragma unroll
+        for (int w = 0; w < WG / 64; w++) mx = fmaxf(mx, wmax[threadIdx.x][w]);
+        maxbits[threadIdx.x * nslots + blockIdx.x] = __float_as_uint(mx);
     }
 }
 
@@ -454,7 +544,7 @@ __global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D)
         d = D.P[PD * cap + p];
     }
     if constexpr (std::is_same<AccT, long long>::value)
-        recordBounds(D.maxbits, live, m, v, Cm, S, dmg ? d : T(0), D.dx, D.odx, D.any_damage != 0);
+        recordBounds(D.maxbits, D.max_slots, live, m, v, Cm, S, dmg ? d : T(0), D.dx, D.odx, D.any_damage != 0);
 }
 
 // Block-local LDS tile geometry: a block of BX x BY x BZ nodes plus the +2
@@ -511,7 +601,8 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
     __shared__ V4<T> q[7][CH];
     __shared__ T wl[4][CH][3];
     __shared__ AccT tile[NGF][TL::TN];
-    __shared__ int cs[NB], ce[NB];
+    __shared__ int cs[NB], ce[NB], cl[NB];
+    __shared__ int ncl;
 
     const int lo = blockIdx.x * CH;
     const int hi = min(lo + CH, D.n);
@@ -578,12 +669,25 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
         }
         for (int e = t; e < nq * TL::TN; e += WG) (&tile[0][0])[e] = AccT(0);
         __syncthreads();
-        const int seg_hi = min(hi, ce[NB - 1]);
-        // ---- phase B: (cell, i, j) columns
-        for (int slot = t; slot < NB * 9; slot += WG) {
-            const int cell = slot / 9, col = slot - 9 * (slot / 9);
+        const int seg_hi = max(seg_lo + 1, min(hi, ce[NB - 1]));
+        // ---- compact the cells holding particles of this segment (NB <= 64: one wavefront)
+        if (t < 64) {
+            bool act = false;
+            if (t < NB) act = min(ce[t], seg_hi) > max(cs[t], seg_lo);
+            const uint64_t m = __ballot(act);
+            if (act) cl[__popcll(m & ((1ull << t) - 1))] = t;
+            if (t == 0) ncl = __popcll(m);
+        }
+        __syncthreads();
+        // ---- phase B: lanes = (sub-lane, cell, i, j); R sub-lanes share a column
+        // and take every R-th particle of the cell
+        const int per = ncl * 9;
+        const int R = per > 0 ? max(1, WG / per) : 1;
+        for (int slot = t; slot < per * R; slot += WG) {
+            const int sub = slot / per, rem = slot - per * (slot / per);
+            const int cell = cl[rem / 9], col = rem - 9 * (rem / 9);
             const int ci = col / 3, cj = col - 3 * (col / 3);
-            const int p0 = max(cs[cell], seg_lo) - lo, p1 = min(ce[cell], seg_hi) - lo;
+            const int p0 = max(cs[cell], seg_lo) - lo + sub, p1 = min(ce[cell], seg_hi) - lo;
             if (p0 >= p1) continue;
             const T fi = T(ci), fj = T(cj);
             T acc[3][NGF];
@@ -591,7 +695,7 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
             for (int k = 0; k < 3; k++)
 #pragma unroll
                 for (int f = 0; f < NGF; f++) acc[k][f] = T(0);
-            for (int pp = p0; pp < p1; pp++) {
+            for (int pp = p0; pp < p1; pp += R) {
                 const V4<T> q0 = q[0][pp], q1 = q[1][pp], q2 = q[2][pp], q3 = q[3][pp], q4 = q[4][pp];
                 const V4<T> q5 = q[5][pp], q6 = q[6][pp];
                 const T wx = wl[0][pp][ci], dwx = wl[1][pp][ci], wy = wl[2][pp][cj], dwy = wl[3][pp][cj];
@@ -657,20 +761,33 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
     if (ovf & 2) latch(D.counters, ST_CAPACITY);
 }
 
-// Next substep's fixed-point scales: 2^(40 - e) with max = f 2^e, f in [0.5,1).
-__global__ void k_update_scales(float* scales, unsigned int* maxbits)
+// This substep's fixed-point scales from the bounds the previous G2P (or the
+// upload) wrote, one slot per workgroup: 2^(40 - e) with max = f 2^e, f in [0.5,1).
+__global__ __launch_bounds__(WG) void k_update_scales(float* scales, const unsigned int* maxbits, int nslots)
 {
-    const int g = threadIdx.x;
-    if (g < NSC) {
-        float m = __uint_as_float(maxbits[g]);
+    __shared__ float red[NSC][WG];
+    float mx[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
+    for (int i = threadIdx.x; i < nslots; i += WG)
+#pragma unroll
+        for (int g = 0; g < NSC; g++) mx[g] = fmaxf(mx[g], __uint_as_float(maxbits[g * nslots + i]));
+#pragma unroll
+    for (int g = 0; g < NSC; g++) red[g][threadIdx.x] = mx[g];
+    __syncthreads();
+    for (int w = WG / 2; w >= 1; w >>= 1) {
+        if (threadIdx.x < w)
+#pragma unroll
+            for (int g = 0; g < NSC; g++) red[g][threadIdx.x] = fmaxf(red[g][threadIdx.x], red[g][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < NSC) {
+        const float m = red[threadIdx.x][0];
         if (m > 0.f && isfinite(m)) {
             int e;
             frexpf(m, &e);
             int se = (int)FIXED_TARGET_EXP - e;
             se = se < -120 ? -120 : (se > 120 ? 120 : se);
-            scales[g] = ldexpf(1.0f, se);
+            scales[threadIdx.x] = ldexpf(1.0f, se);
         }
-        maxbits[g] = 0u;
     }
 }
 
@@ -706,9 +823,12 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt, double t)
             continue;
         }
         const int b = (int)(node / NB), e = (int)(node % NB);
-        uint64_t off = ((uint64_t)(uint32_t)D.block_page[b] << 12) | ((uint64_t)e << B::data_bits);
-        const T xi[3] = { T(bitPack(off, D.masks.x)) * D.dx, T(bitPack(off, D.masks.y)) * D.dx,
-            T(bitPack(off, D.masks.z)) * D.dx };
+        uint32_t Bx, By, Bz;
+        blockOfPage<T>((uint32_t)D.block_page[b], Bx, By, Bz);
+        const uint32_t ix = (Bx << B::xbits) | (uint32_t)(e >> (B::ybits + B::zbits));
+        const uint32_t iy = (By << B::ybits) | (uint32_t)((e >> B::zbits) & ((1 << B::ybits) - 1));
+        const uint32_t iz = (Bz << B::zbits) | (uint32_t)(e & ((1 << B::zbits) - 1));
+        const T xi[3] = { T(ix) * D.dx, T(iy) * D.dx, T(iz) * D.dx };
         const T dt_over_mi = dt / m;
         T vi[3];
 #pragma unroll
@@ -926,7 +1046,7 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
         const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
         if (live && s >= seg_lo && s < seg_hi) {
             const int lx0 = base[0] & (TL::BX - 1), ly0 = base[1] & (TL::BY - 1), lz0 = base[2] & (TL::BZ - 1);
-#pragma unroll
+#pragma unroll 1
             for (int i = 0; i < 3; i++) {
                 const T wi = w[0][i];
                 const T dwidxi = odx * dw[0][i];
@@ -1023,12 +1143,12 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
         Q[PD * cap + s] = d;
         Q[PG * cap + s] = g;
         Q[PLAP * cap + s] = dmg ? lap : T(0);
-        D.qid[s] = D.pid[p];
+        D.qid[s] = D.spid[s];
         D.qmat[s] = mid;
         dout = dmg ? d : T(0);
     }
     if constexpr (std::is_same<AccT, long long>::value)
-        recordBounds(D.maxbits, live, m, picV, Cn, S, dout, D.dx, D.odx, D.any_damage != 0);
+        recordBounds(D.maxbits, D.max_slots, live, m, picV, Cn, S, dout, D.dx, D.odx, D.any_damage != 0);
 }
 
 } // namespace zmpm

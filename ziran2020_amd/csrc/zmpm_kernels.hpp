@@ -59,13 +59,17 @@ enum : int {
     ST_HALF_CFL = 32
 };
 
-// SPGrid block shape of GridState<T,3> (64 B float -> 4x4x4, 128 B double -> 2x4x4).
+// SPGrid block shape of GridState<T,3> (64 B float -> 4x4x4, 128 B double -> 2x4x4)
+// and the axis that fills page bit 0, 1, 2 of the interleave (SPGrid_Mask.h:31-47:
+// float z/y/x, double x/z/y).
 template <class T> struct Blk;
 template <> struct Blk<float> {
     static constexpr int xbits = 2, ybits = 2, zbits = 2, data_bits = 6;
+    static constexpr int xpos = 2, ypos = 1, zpos = 0;
 };
 template <> struct Blk<double> {
     static constexpr int xbits = 1, ybits = 2, zbits = 2, data_bits = 7;
+    static constexpr int xpos = 0, ypos = 2, zpos = 1;
 };
 
 template <class T>
@@ -103,6 +107,7 @@ struct Dev {
     int* rank;         // slot inside the cell
     int* perm;         // sorted position -> live index
     int* sorted_cell;  // sorted position -> dense cell id
+    int* spid;         // sorted position -> original index (scatter output, sorted in place)
     int* touched;      // per page: 1 if a particle block or its -1 neighbour has particles
     int* scan_tmp;     // per page exclusive scan / per cell scan scratch
     int* page_table;   // per page: dense block id or -1
@@ -122,7 +127,8 @@ struct Dev {
     int64_t cap_nodes;
     // fixed-point scales (float form only)
     float* scales;     // [NSC] current scale 2^e
-    unsigned int* maxbits; // [NSC] max |contribution| of the running substep (float bits)
+    unsigned int* maxbits; // [NSC][max_slots] per-workgroup max |contribution| bound (float bits)
+    int max_slots;         // workgroups of the last particle-order launch (G2P / upload)
     // tables
     const MatDev<T>* mats;
     const ColDev* cols;

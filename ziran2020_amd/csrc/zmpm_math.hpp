@@ -425,29 +425,66 @@ __device__ __forceinline__ Mat3<T> polarRotation(const Mat3<T>& F)
     return mulT(U, V);
 }
 
+// Float form: the same rotation from the scaled Newton iteration
+// X <- (g X + X^{-T} / g) / 2, g = |det X|^{-1/3} (Higham, "Computing the polar
+// decomposition -- with applications", SIAM J. Sci. Stat. Comput. 7, 1986),
+// which converges quadratically to U V^T for det F > 0 without the branchy QR
+// sweeps.  Near inversion (small or negative det relative to |F|^3, where R is
+// ill-conditioned or the orthogonal factor is a reflection) the implicit-QR
+// SVD runs in double.
+__device__ __forceinline__ Mat3<float> polarRotationSvd64(const Mat3<float>& F)
+{
+    Mat3<double> Fd, Ud, Vd;
+    double sd[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) Fd.a[i][j] = F.a[i][j];
+    svd3(Fd, Ud, sd, Vd);
+    Mat3<double> Rd = mulT(Ud, Vd);
+    Mat3<float> R;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) R.a[i][j] = (float)Rd.a[i][j];
+    return R;
+}
+
 template <>
 __device__ __forceinline__ Mat3<float> polarRotation<float>(const Mat3<float>& F)
 {
-    Mat3<float> U, V;
-    float s[3];
-    svd3(F, U, s, V);
-    if (!(s[1] + s[2] > 1e-2f * s[0])) {
-        Mat3<double> Fd, Ud, Vd;
-        double sd[3];
+    float n2 = 0.f;
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+    for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) Fd.a[i][j] = F.a[i][j];
-        svd3(Fd, Ud, sd, Vd);
-        Mat3<double> Rd = mulT(Ud, Vd);
-        Mat3<float> R;
+        for (int j = 0; j < 3; j++) n2 += F.a[i][j] * F.a[i][j];
+    const float J = det3(F);
+    // det F >= 1e-3 (|F|_F^2 / 3)^{3/2}: smallest singular value bounded away from 0
+    const float ref = n2 * (1.f / 3.f);
+    bool ok = J > 1e-3f * ref * sqrtf(ref);
+    Mat3<float> X = F;
+    if (ok) {
+        int it = 0;
+        for (; it < 12; it++) {
+            Mat3<float> C = cofactor(X);
+            const float d = X.a[0][0] * C.a[0][0] + X.a[0][1] * C.a[0][1] + X.a[0][2] * C.a[0][2];
+            const float g = exp2f(-log2f(fabsf(d)) * (1.f / 3.f));
+            const float a = 0.5f * g, b = 0.5f / (g * d);
+            float delta = 0.f;
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+            for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) R.a[i][j] = (float)Rd.a[i][j];
-        return R;
+                for (int j = 0; j < 3; j++) {
+                    const float xn = a * X.a[i][j] + b * C.a[i][j];
+                    const float e = xn - X.a[i][j];
+                    delta += e * e;
+                    X.a[i][j] = xn;
+                }
+            if (delta < 1e-12f) break;
+        }
+        ok = it < 12;
     }
-    return mulT(U, V);
+    return ok ? X : polarRotationSvd64(F);
 }
 
 // ------------------------------------------------------ constitutive models
@@ -607,10 +644,12 @@ template <class T>
 __device__ __forceinline__ void symEigen(Mat3<T> S, T* lam, Mat3<T>& Q)
 {
     setIdentity(Q);
-    for (int sweep = 0; sweep < 12; sweep++) {
+    constexpr int max_sweeps = sizeof(T) == 4 ? 6 : 10;
+    for (int sweep = 0; sweep < max_sweeps; sweep++) {
         T off = S.a[0][1] * S.a[0][1] + S.a[0][2] * S.a[0][2] + S.a[1][2] * S.a[1][2];
         T diag = S.a[0][0] * S.a[0][0] + S.a[1][1] * S.a[1][1] + S.a[2][2] * S.a[2][2];
-        if (off <= T(1e-2) * Lim<T>::eps * Lim<T>::eps * diag || off == T(0)) break;
+        // off-diagonal mass down to rounding level: (4 eps)^2 of the diagonal
+        if (off <= T(16) * Lim<T>::eps * Lim<T>::eps * diag || off == T(0)) break;
         jacobiRot(S, Q, 0, 1);
         jacobiRot(S, Q, 0, 2);
         jacobiRot(S, Q, 1, 2);
