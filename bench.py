@@ -28,22 +28,25 @@ METRIC = "particle-substeps/sec FP32-batched vs FP64-plain on 1 MI355X; % HBM pe
 
 
 def algorithmic_bytes(stage, precision, n_particles, n_nodes, damage):
-    """Compulsory HBM bytes of one launch of a stage (roofline)."""
+    """Compulsory HBM bytes of one launch of a stage: every particle field the
+    stage must read or write once, plus its grid accumulators / velocities once
+    per active node (index and sort arrays excluded)."""
     w = 4 if precision == "fp32" else 8
-    acc = 8  # int64 fixed point or double accumulator
+    acc = 8  # int64 fixed point (FP32 form) or double accumulator
     nacc = 9 if damage else 7
+    dmg = 1 if damage else 0
     if stage == "p2g":
-        # read x v C F (24 scalars) + m V0 (+ d g) + material id; write nacc accumulators per node
-        per_p = (24 + 2 + (2 if damage else 0)) * w + 4
+        # read x v C (15) + symmetric stress (6) + m (+ d) + material id; nacc accumulators per node
+        per_p = (15 + 6 + 1 + dmg) * w + 4
         per_n = nacc * acc
     elif stage == "g2p":
-        # read x F m V0 (+ d g) + id + material; write x v C F m V0 (+ d g lap) + id + material;
+        # read x F m V0 d g (16) + id + material; write x v C F stress m V0 d g lap (35) + id + material;
         # read new_v (+ phase field) per node
-        per_p = (12 + 2 + (2 if damage else 0)) * w + 8 + (24 + 2 + (3 if damage else 0)) * w + 8
-        per_n = (3 + (1 if damage else 0)) * w
+        per_p = 16 * w + 8 + 35 * w + 8
+        per_n = (3 + dmg) * w
     elif stage == "grid_update":
         per_p = 0
-        per_n = nacc * acc + (3 + (1 if damage else 0)) * w
+        per_n = nacc * acc + (3 + dmg) * w
     else:
         raise ValueError(stage)
     return per_p * n_particles + per_n * n_nodes
@@ -68,18 +71,23 @@ def cpu_baseline(scene, budget_s=12.0, max_steps=10):
                       f"FP64 oracle (C++/OpenMP restatement of the reference CPU step), {threads} threads"}
 
 
-def timed(scene, lib, precision, steps, warmup):
-    from ziran2020_amd import _abi as A
+def timed(scene, lib, precision, steps, warmup, timing_steps=50):
+    """Throughput of `steps` substeps (HIP-graph replays, no events inside), then
+    per-stage kernel times from HIP events on the handle's stream over the next
+    min(steps, timing_steps) substeps launched one kernel at a time."""
     sim = scene.make(lib, precision)
     sim.run(warmup, scene.dt, 0.0)
     sim.synchronize()
-    sim.L.zmpm_set_timing(sim.h, 1)
     t_start = warmup * scene.dt
-    sim.synchronize()
     t0 = time.perf_counter()
     sim.run(steps, scene.dt, t_start)
     sim.synchronize()
     el = time.perf_counter() - t0
+    k = min(steps, timing_steps)
+    sim.L.zmpm_set_timing(sim.h, 1)
+    sim.run(k, scene.dt, t_start + steps * scene.dt)
+    sim.synchronize()
+    sim.L.zmpm_set_timing(sim.h, 0)
     stages = sim.timings()
     grid = sim.grid()
     n_nodes = int(grid["m"].shape[0])

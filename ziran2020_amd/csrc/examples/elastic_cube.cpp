@@ -1,0 +1,57 @@
+// elastic_cube.cpp -- C++ driver of the explicit MPM substep through the
+// MpmSimulationHip facade: a fixed-corotated cube (8 particles per cell on
+// the sub-cell centres) dropping onto a sticky floor, as a scene written
+// against the reference's MpmSimulationBase member names.
+//
+// usage: zmpm_elastic_cube <cells per side> <substeps> <fp32: 0|1>
+// prints one JSON line: particle count, centre of mass, mean velocity.
+#include "zmpm_simulation.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+
+int main(int argc, char** argv)
+{
+    const int cells = argc > 1 ? std::atoi(argv[1]) : 16;
+    const int steps = argc > 2 ? std::atoi(argv[2]) : 20;
+    const bool fp32 = argc > 3 && std::atoi(argv[3]) != 0;
+    zmpm::MpmSimulationHip sim;
+    sim.dx = 1.0 / 64;
+    sim.gravity = { 0, -9.8, 0 };
+    sim.precision = fp32 ? ZMPM_PREC_FP32 : ZMPM_PREC_FP64;
+    sim.domain_nodes = 64;
+    sim.autorestart = false;
+    const double E = 1e4, nu = 0.3, rho = 1000;
+    zmpm_material m {};
+    m.model = ZMPM_MODEL_FIXED_COROTATED;
+    m.mu = E / (2 * (1 + nu));                     // CorotatedElasticity.cpp:15-20
+    m.lambda = E * nu / ((1 + nu) * (1 - 2 * nu));
+    m.num_directors = 1;
+    const int mat = sim.addMaterial(m);
+    zmpm_collider floor {};
+    floor.shape = ZMPM_SHAPE_HALF_SPACE;
+    floor.type = ZMPM_STICKY;
+    floor.origin[1] = 8 * sim.dx;
+    floor.normal[1] = 1;
+    sim.addCollisionObject(floor);
+    const double h = sim.dx / 2, V0 = h * h * h;
+    const double x0 = 16 * sim.dx, y0 = 8 * sim.dx, z0 = 16 * sim.dx;
+    for (int i = 0; i < 2 * cells; i++)
+        for (int j = 0; j < 2 * cells; j++)
+            for (int k = 0; k < 2 * cells; k++)
+                sim.addParticle({ x0 + (i + 0.5) * h, y0 + (j + 0.5) * h, z0 + (k + 0.5) * h }, rho * V0, V0, mat);
+    sim.initialize();
+    sim.advance(steps, 1e-4);
+    sim.sync();
+    double cm[3] = { 0, 0, 0 }, vm[3] = { 0, 0, 0 }, M = 0;
+    for (size_t p = 0; p < sim.X.size(); p++) {
+        for (int a = 0; a < 3; a++) {
+            cm[a] += sim.mass[p] * sim.X[p][a];
+            vm[a] += sim.mass[p] * sim.V[p][a];
+        }
+        M += sim.mass[p];
+    }
+    std::printf("{\"n\": %zu, \"com\": [%.17g, %.17g, %.17g], \"mean_v\": [%.17g, %.17g, %.17g]}\n", sim.X.size(),
+                cm[0] / M, cm[1] / M, cm[2] / M, vm[0] / M, vm[1] / M, vm[2] / M);
+    return 0;
+}

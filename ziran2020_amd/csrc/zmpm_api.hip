@@ -65,6 +65,15 @@ struct zmpm_context {
     int timed_steps = 0;
     double timed_ms[kTimedGroups + 1] = {};
     bool stepped = false;
+    double clock_host = 0;
+    // captured two-substep graphs, keyed by live buffer and dt
+    struct Graph {
+        const void* live;
+        double dt;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+    bool use_graphs = true;
 };
 
 namespace {
@@ -96,9 +105,16 @@ zmpm_status dalloc(zmpm_context* h, X** out, size_t count)
     return ZMPM_OK;
 }
 
+void dropGraphs(zmpm_context* h)
+{
+    for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+    h->graphs.clear();
+}
+
 void freeAll(zmpm_context* h)
 {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    dropGraphs(h);
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
     h->Df = Dev<float, long long> {};
@@ -155,6 +171,7 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.cell_count, D.cap_nodes);
     ALLOC(D.cell_start, D.cap_nodes);
     ALLOC(D.counters, 8);
+    ALLOC(D.clock, 2);
     ALLOC(D.acc, (int64_t)NGF * D.cap_nodes);
     ALLOC(D.newv, 3 * D.cap_nodes);
     ALLOC(D.pfn, D.cap_nodes);
@@ -267,7 +284,7 @@ void launchScan(zmpm_context* h, const int* in, const int* len_dev, int64_t mul,
 }
 
 template <class T, class AccT>
-void launchStep(zmpm_context* h, double dt, double t, hipEvent_t* ev)
+void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
 {
     Dev<T, AccT>& D = devOf<T, AccT>(h);
     using B = Blk<T>;
@@ -275,8 +292,9 @@ void launchStep(zmpm_context* h, double dt, double t, hipEvent_t* ev)
     hipStream_t st = h->stream;
     const int gp = gridFor(D.n, WG, 1 << 30);
     if (ev) (void)hipEventRecord(ev[0], st);
-    // fixed-point scales of this substep from the previous one's maxima
-    if (std::is_same<AccT, long long>::value) k_update_scales<<<1, WG, 0, st>>>(D.scales, D.maxbits, D.max_slots);
+    // device clock; fixed-point scales of this substep from the previous one's bounds
+    k_begin_step<<<1, WG, 0, st>>>(std::is_same<AccT, long long>::value ? D.scales : nullptr, D.maxbits, D.max_slots,
+                                   D.clock, dt);
     k_bin<T, AccT><<<gp, WG, 0, st>>>(D);
     launchScan<T, AccT>(h, D.touched, nullptr, 1, D.num_pages, D.scan_tmp, D.counters, D.cap_blocks, D.counters);
     k_assign<T, AccT><<<gridFor(D.num_pages, WG, 8192), WG, 0, st>>>(D);
@@ -288,7 +306,7 @@ void launchStep(zmpm_context* h, double dt, double t, hipEvent_t* ev)
     if (ev) (void)hipEventRecord(ev[1], st);
     k_p2g<T, AccT><<<gridFor(D.n, Tile<T>::CHUNK, 1 << 30), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[2], st);
-    k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, 8192), WG, 0, st>>>(D, (T)dt, t);
+    k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, 8192), WG, 0, st>>>(D, (T)dt);
     if (ev) (void)hipEventRecord(ev[3], st);
     k_g2p<T, AccT><<<gp, WG, 0, st>>>(D, (T)dt);
     if (ev) (void)hipEventRecord(ev[4], st);
@@ -533,6 +551,7 @@ zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
     auto* h = new zmpm_context();
     h->desc = *desc;
     h->fp32 = desc->precision == ZMPM_PREC_FP32;
+    if (const char* e = std::getenv("ZMPM_NO_GRAPH")) h->use_graphs = std::atoi(e) == 0;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return ZMPM_E_HIP;
@@ -572,6 +591,7 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
         if (mats[i].model < 0 || mats[i].model > ZMPM_MODEL_EQUATION_OF_STATE)
             return fail(h, ZMPM_E_ARG, "unknown constitutive model");
     (void)hipSetDevice(h->desc.device);
+    dropGraphs(h);
     h->mats.assign(mats, mats + count);
     h->any_damage = false;
     for (auto& m : h->mats) h->any_damage = h->any_damage || m.has_damage;
@@ -591,6 +611,7 @@ zmpm_status zmpm_add_collider(zmpm_handle h, const zmpm_collider* c, int32_t* id
     if (c->shape != ZMPM_SHAPE_HALF_SPACE && c->shape != ZMPM_SHAPE_BOX) return fail(h, ZMPM_E_ARG, "unknown shape");
     if (c->type < ZMPM_STICKY || c->type > ZMPM_GHOST) return fail(h, ZMPM_E_ARG, "unknown collider type");
     (void)hipSetDevice(h->desc.device);
+    dropGraphs(h);
     h->cols.push_back(*c);
     zmpm_status s = uploadColliders(h);
     if (s != ZMPM_OK) return s;
@@ -605,6 +626,7 @@ zmpm_status zmpm_add_collider(zmpm_handle h, const zmpm_collider* c, int32_t* id
 zmpm_status zmpm_clear_colliders(zmpm_handle h)
 {
     if (!h) return ZMPM_E_ARG;
+    dropGraphs(h);
     h->cols.clear();
     if (h->fp32)
         refreshTables<float, long long>(h);
@@ -625,6 +647,7 @@ zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
         if (p->material[i] < 0 || p->material[i] >= (int32_t)h->mats.size())
             return fail(h, ZMPM_E_ARG, "material index out of range");
     (void)hipSetDevice(h->desc.device);
+    dropGraphs(h);
     return h->fp32 ? uploadT<float, long long>(h, p) : uploadT<double, double>(h, p);
 }
 
@@ -637,6 +660,36 @@ zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
     if (st == ZMPM_E_HIP) return st;
     zmpm_status s = h->fp32 ? downloadT<float, long long>(h, p) : downloadT<double, double>(h, p);
     return s != ZMPM_OK ? s : st;
+}
+
+// Two substeps captured into one HIP graph (the particle buffers ping-pong, so
+// a pair returns to the starting buffer); cached per live buffer and dt.
+static zmpm_status graphFor(zmpm_handle h, double dt, hipGraphExec_t* out)
+{
+    const void* live = h->fp32 ? (const void*)h->Df.P : (const void*)h->Dd.P;
+    for (auto& g : h->graphs)
+        if (g.live == live && g.dt == dt) {
+            *out = g.exec;
+            return ZMPM_OK;
+        }
+    if (h->graphs.size() >= 4) dropGraphs(h);
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return fail(h, ZMPM_E_HIP, "hipStreamBeginCapture");
+    for (int k = 0; k < 2; k++) {
+        if (h->fp32)
+            launchStep<float, long long>(h, dt, nullptr);
+        else
+            launchStep<double, double>(h, dt, nullptr);
+    }
+    if (hipStreamEndCapture(h->stream, &graph) != hipSuccess) return fail(h, ZMPM_E_HIP, "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(h, ZMPM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    h->graphs.push_back({ live, dt, exec });
+    *out = exec;
+    return ZMPM_OK;
 }
 
 static zmpm_status runSteps(zmpm_handle h, int32_t steps, double dt, double t0)
@@ -652,12 +705,25 @@ static zmpm_status runSteps(zmpm_handle h, int32_t steps, double dt, double t0)
             h->events.push_back(e);
         }
     }
-    for (int32_t s = 0; s < steps; s++) {
+    // the device clock starts at t0; every substep advances it by dt
+    h->clock_host = t0;
+    double* clock = h->fp32 ? h->Df.clock : h->Dd.clock;
+    if (hipMemcpyAsync(clock, &h->clock_host, sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        return fail(h, ZMPM_E_HIP, "clock upload");
+    int32_t s = 0;
+    if (h->use_graphs && !h->timing && steps >= 2) {
+        hipGraphExec_t exec = nullptr;
+        zmpm_status st = graphFor(h, dt, &exec);
+        if (st != ZMPM_OK) return st;
+        for (; s + 2 <= steps; s += 2)
+            if (hipGraphLaunch(exec, h->stream) != hipSuccess) return fail(h, ZMPM_E_HIP, "hipGraphLaunch");
+    }
+    for (; s < steps; s++) {
         if (h->timing) ev = &h->events[(size_t)s * (kTimedGroups + 1)];
         if (h->fp32)
-            launchStep<float, long long>(h, dt, t0 + s * dt, ev);
+            launchStep<float, long long>(h, dt, ev);
         else
-            launchStep<double, double>(h, dt, t0 + s * dt, ev);
+            launchStep<double, double>(h, dt, ev);
     }
     h->stepped = true;
     hipError_t e = hipGetLastError();

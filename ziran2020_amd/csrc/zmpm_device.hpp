@@ -761,10 +761,18 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
     if (ovf & 2) latch(D.counters, ST_CAPACITY);
 }
 
-// This substep's fixed-point scales from the bounds the previous G2P (or the
-// upload) wrote, one slot per workgroup: 2^(40 - e) with max = f 2^e, f in [0.5,1).
-__global__ __launch_bounds__(WG) void k_update_scales(float* scales, const unsigned int* maxbits, int nslots)
+// Start of a substep: advance the device clock and (float form) set this
+// substep's fixed-point scales from the bounds the previous G2P (or the upload)
+// wrote, one slot per workgroup: 2^(40 - e) with max = f 2^e, f in [0.5,1).
+__global__ __launch_bounds__(WG) void k_begin_step(float* scales, const unsigned int* maxbits, int nslots, double* clock,
+                                                  double dt)
 {
+    // device clock: clock[0] = time of the next substep, clock[1] = this substep's t
+    if (threadIdx.x == 0) {
+        clock[1] = clock[0];
+        clock[0] = clock[0] + dt;
+    }
+    if (!scales) return;
     __shared__ float red[NSC][WG];
     float mx[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
     for (int i = threadIdx.x; i < nslots; i += WG)
@@ -805,8 +813,9 @@ __device__ __forceinline__ T accValue(const Dev<T, AccT>& D, int f, int64_t node
 // (CollisionObject.cpp:66-105, 341-406) at time t; phase-field normalisation
 // (AnisoFractureSimulation.h:177-179).
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt, double t)
+__global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
 {
+    const double t = D.clock[1]; // collision objects at t_n (MpmSimulationBase.cpp:291-293)
     using B = Blk<T>;
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
     const int64_t nn = (int64_t)D.counters[0] * NB;
@@ -925,16 +934,7 @@ __device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>&
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) sym.a[i][j] = T(0.5) * (cs.a[i][j] + cs.a[j][i]) / J;
-    T lam[3];
-    Mat3<T> Qe;
-    symEigen(sym, lam, Qe);
-    Mat3<T> sp;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-            sp.a[i][j] = macaulay(lam[0]) * (Qe.a[i][0] * Qe.a[j][0]) + macaulay(lam[1]) * (Qe.a[i][1] * Qe.a[j][1])
-                + macaulay(lam[2]) * (Qe.a[i][2] * Qe.a[j][2]);
+    const Mat3<T> sp = positivePart(sym);
     Mat3<T> R = polarRotation(F);
     Mat3<T> A;
     setIdentity(A);

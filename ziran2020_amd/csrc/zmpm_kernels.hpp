@@ -117,6 +117,7 @@ struct Dev {
     int* cell_count;   // [cap_blocks*NB]
     int* cell_start;   // [cap_blocks*NB]
     int* counters;     // [0] num_blocks, [1] status bits
+    double* clock;     // [0] time of the next substep, [1] time of the running substep
     int cap_blocks;
     int domain;
     Masks masks;

@@ -659,6 +659,105 @@ __device__ __forceinline__ void symEigen(Mat3<T> S, T* lam, Mat3<T>& Q)
     lam[2] = S.a[2][2];
 }
 
+// sigma^+ = sum_i <lambda_i>_+ v_i v_i^T of a symmetric 3x3.  Generic form: Jacobi.
+template <class T>
+__device__ __forceinline__ Mat3<T> positivePart(const Mat3<T>& S)
+{
+    T lam[3];
+    Mat3<T> Qe, sp;
+    symEigen(S, lam, Qe);
+#pragma unroll
+    for (int e = 0; e < 3; e++) lam[e] = (lam[e] + dabs(lam[e])) / T(2);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            sp.a[i][j] = lam[0] * (Qe.a[i][0] * Qe.a[j][0]) + lam[1] * (Qe.a[i][1] * Qe.a[j][1]) + lam[2] * (Qe.a[i][2] * Qe.a[j][2]);
+    return sp;
+}
+
+// Unit eigenvector of symmetric S for the simple eigenvalue l: the largest cross
+// product of two rows of S - l I.
+__device__ __forceinline__ void eigvecOf(const Mat3<float>& S, float l, float* v)
+{
+    const float r0[3] = { S.a[0][0] - l, S.a[0][1], S.a[0][2] };
+    const float r1[3] = { S.a[1][0], S.a[1][1] - l, S.a[1][2] };
+    const float r2[3] = { S.a[2][0], S.a[2][1], S.a[2][2] - l };
+    float c[3][3];
+    const float* A[3] = { r0, r0, r1 };
+    const float* Bv[3] = { r1, r2, r2 };
+    float best = -1.f;
+    int bi = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        c[k][0] = A[k][1] * Bv[k][2] - A[k][2] * Bv[k][1];
+        c[k][1] = A[k][2] * Bv[k][0] - A[k][0] * Bv[k][2];
+        c[k][2] = A[k][0] * Bv[k][1] - A[k][1] * Bv[k][0];
+        const float n = c[k][0] * c[k][0] + c[k][1] * c[k][1] + c[k][2] * c[k][2];
+        if (n > best) {
+            best = n;
+            bi = k;
+        }
+    }
+    const float inv = best > 0.f ? rsqrtf(best) : 0.f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) v[a] = (bi == 0 ? c[0][a] : (bi == 1 ? c[1][a] : c[2][a])) * inv;
+}
+
+// Float form: eigenvalues from the trigonometric solution of the characteristic
+// cubic (O. K. Smith, "Eigenvalues of a symmetric 3x3 matrix", CACM 4, 1961) and
+// at most one eigenvector -- of the eigenvalue on the other side of zero from
+// the rest -- so no iteration.  When that eigenvalue is nearly degenerate with a
+// neighbour it is itself near zero and its term in sigma^+ is negligible.
+template <>
+__device__ __forceinline__ Mat3<float> positivePart<float>(const Mat3<float>& S)
+{
+    const float a01 = S.a[0][1], a02 = S.a[0][2], a12 = S.a[1][2];
+    const float q = (S.a[0][0] + S.a[1][1] + S.a[2][2]) * (1.f / 3.f);
+    const float b00 = S.a[0][0] - q, b11 = S.a[1][1] - q, b22 = S.a[2][2] - q;
+    const float p1 = a01 * a01 + a02 * a02 + a12 * a12;
+    const float p2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.f * p1;
+    Mat3<float> out;
+    float l1, l2, l3;
+    if (!(p2 > 1e-30f * (q * q + 1e-30f))) {
+        l1 = l2 = l3 = q; // isotropic
+    }
+    else {
+        const float p = sqrtf(p2 * (1.f / 6.f));
+        const float ip = 1.f / p;
+        const float c00 = b00 * ip, c11 = b11 * ip, c22 = b22 * ip, c01 = a01 * ip, c02 = a02 * ip, c12 = a12 * ip;
+        const float detB = c00 * (c11 * c22 - c12 * c12) - c01 * (c01 * c22 - c12 * c02) + c02 * (c01 * c12 - c11 * c02);
+        const float r = fminf(1.f, fmaxf(-1.f, 0.5f * detB));
+        const float phi = acosf(r) * (1.f / 3.f);
+        l1 = q + 2.f * p * cosf(phi);
+        l3 = q + 2.f * p * cosf(phi + 2.0943951023931953f);
+        l2 = 3.f * q - l1 - l3;
+    }
+    if (l3 >= 0.f) return S;
+    if (l1 <= 0.f) {
+        setIdentity(out);
+#pragma unroll
+        for (int i = 0; i < 3; i++) out.a[i][i] = 0.f;
+        return out;
+    }
+    float v[3];
+    if (l2 <= 0.f) { // only l1 > 0
+        eigvecOf(S, l1, v);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) out.a[i][j] = l1 * (v[i] * v[j]);
+    }
+    else { // l1, l2 > 0 >= l3
+        eigvecOf(S, l3, v);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) out.a[i][j] = S.a[i][j] - l3 * (v[i] * v[j]);
+    }
+    return out;
+}
+
 template <class T>
 __device__ __forceinline__ T macaulay(T x) { return (x + dabs(x)) / T(2); } // MathTools.h:276-281
 

@@ -1,0 +1,257 @@
+// zmpm_simulation.hpp -- C++ host facade over the zmpm C ABI with the public
+// members and virtual step structure of Ziran's MpmSimulationBase<T,3>
+// (Lib/MPM/MpmSimulationBase.h:47-76 members, :150-231 virtuals) and the frame
+// driver of SimulationBase (Lib/Ziran/Sim/SimulationBase.h:288-354), so scene
+// code written against those names drives the gfx950 substep.
+//
+// Header-only; link with libzmpm.so.
+#pragma once
+
+#include "zmpm.h"
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <ostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace zmpm {
+
+// SimulationBase.h:21-29: thrown when a substep moved a particle more than dx
+// and autorestart is on (MpmSimulationBase.cpp:1480-1499).
+struct RestartException : public std::runtime_error {
+    int frame;
+    explicit RestartException(int f)
+        : std::runtime_error("particle traveled more than a grid cell"), frame(f) {}
+};
+
+struct ZmpmError : public std::runtime_error {
+    zmpm_status status;
+    ZmpmError(zmpm_status s, const std::string& what)
+        : std::runtime_error(what), status(s) {}
+};
+
+using TV = std::array<double, 3>;
+using TM = std::array<double, 9>; // row-major
+
+class MpmSimulationHip {
+public:
+    // ---- MpmSimulationBase public parameters
+    double dx = 1;
+    double dt = 1.0 / 24;
+    double cfl = 0.6;
+    double apic_rpic_ratio = 1;
+    TV gravity { 0, 0, 0 };
+    bool autorestart = true;
+    // ---- device configuration (no counterpart in the reference)
+    zmpm_precision precision = ZMPM_PREC_FP64;
+    int domain_nodes = 256;
+    int device = 0;
+    // ---- particle attributes (Particles m/X/V, "C", "F", "element measure",
+    //      AnisotropicPhaseField d, model g), host copies in upload order
+    std::vector<TV> X, V;
+    std::vector<double> mass, element_measure;
+    std::vector<TM> C, F;
+    std::vector<int32_t> material;
+    std::vector<double> damage, degradation, laplacian;
+    // ---- scene
+    std::vector<zmpm_material> materials;
+    std::vector<zmpm_collider> collision_objects;
+    // ---- SimulationBase frame driver state
+    int frame = 0;
+    double time = 0;
+    double frame_dt = 1.0 / 24;
+    int end_frame = 0;
+    std::vector<std::function<void(int, int, double, double)>> begin_time_step_callbacks;
+
+    MpmSimulationHip() = default;
+    MpmSimulationHip(const MpmSimulationHip&) = delete;
+    MpmSimulationHip& operator=(const MpmSimulationHip&) = delete;
+    virtual ~MpmSimulationHip()
+    {
+        if (h_) zmpm_destroy(h_);
+    }
+
+    // MpmSimulationBase::addCollisionObject (MpmSimulationBase.cpp:87-91)
+    void addCollisionObject(const zmpm_collider& c)
+    {
+        collision_objects.push_back(c);
+        dirty_ = true;
+    }
+
+    int addMaterial(const zmpm_material& m)
+    {
+        materials.push_back(m);
+        dirty_ = true;
+        return (int)materials.size() - 1;
+    }
+
+    // Particle with mass rho V / N and element measure V / N (MpmInitializationHelper.h:637-647).
+    void addParticle(const TV& x, double m, double V0, int mat, const TM& F0 = { 1, 0, 0, 0, 1, 0, 0, 0, 1 })
+    {
+        X.push_back(x);
+        V.push_back({ 0, 0, 0 });
+        mass.push_back(m);
+        element_measure.push_back(V0);
+        C.push_back({ 0, 0, 0, 0, 0, 0, 0, 0, 0 });
+        F.push_back(F0);
+        material.push_back(mat);
+        damage.push_back(0);
+        degradation.push_back(1);
+        laplacian.push_back(0);
+        dirty_ = true;
+    }
+
+    // MpmSimulationBase::initialize (MpmSimulationBase.cpp:98-146): D^-1 = 4/dx^2
+    // lives on the device; here the handle is created and the state uploaded.
+    virtual void initialize()
+    {
+        if (h_) {
+            zmpm_destroy(h_);
+            h_ = nullptr;
+        }
+        zmpm_scene_desc d {};
+        d.precision = precision;
+        d.domain_nodes = domain_nodes;
+        d.dx = dx;
+        for (int a = 0; a < 3; a++) d.gravity[a] = gravity[a];
+        d.apic_rpic_ratio = apic_rpic_ratio;
+        d.cfl = cfl;
+        d.device = device;
+        check(zmpm_create(&d, &h_), "zmpm_create");
+        check(zmpm_set_materials(h_, (int32_t)materials.size(), materials.data()), "zmpm_set_materials");
+        for (const auto& c : collision_objects) check(zmpm_add_collider(h_, &c, nullptr), "zmpm_add_collider");
+        upload();
+    }
+
+    // Binning, sort and grid activation run on the device at the start of every
+    // substep (sortParticlesAndPolluteGrid, MpmSimulationBase.cpp:1531-1573).
+    virtual void reinitialize()
+    {
+        if (dirty_) initialize();
+    }
+
+    // MpmSimulationBase::beginTimeStep (MpmSimulationBase.cpp:238-245)
+    virtual void beginTimeStep(int frame_, int substep, double time_, double dt_)
+    {
+        dt = dt_;
+        for (auto& f : begin_time_step_callbacks) f(frame_, substep, time_, dt_);
+    }
+
+    // MpmSimulationBase::advanceOneTimeStep (MpmSimulationBase.cpp:262-357):
+    // one zmpm_substep on the device.  With autorestart the CFL condition is
+    // checked after the substep (one device synchronisation).
+    virtual void advanceOneTimeStep(double dt_)
+    {
+        reinitialize();
+        check(zmpm_substep(h_, dt_, time), "zmpm_substep");
+        if (autorestart) {
+            zmpm_status s = zmpm_synchronize(h_);
+            if (s == ZMPM_E_CFL) throw RestartException(frame - 1);
+            check(s, "zmpm_synchronize");
+        }
+        host_stale_ = true;
+    }
+
+    // `steps` substeps of size dt_ without host round trips.
+    void advance(int steps, double dt_)
+    {
+        reinitialize();
+        check(zmpm_run(h_, steps, dt_, time), "zmpm_run");
+        time += steps * dt_;
+        host_stale_ = true;
+    }
+
+    // SimulationBase::advanceOneFrame (SimulationBase.h:288-322) with a fixed dt.
+    virtual void advanceOneFrame()
+    {
+        double t_end = time + frame_dt;
+        int substep = 0;
+        while (time < t_end - 1e-12 * frame_dt) {
+            double h = std::min(dt, t_end - time);
+            beginTimeStep(frame, substep++, time, h);
+            advanceOneTimeStep(h);
+            time += h;
+        }
+        frame++;
+    }
+
+    // SimulationBase::simulate (SimulationBase.h:327-354) without restart files.
+    virtual void simulate()
+    {
+        while (frame < end_frame) advanceOneFrame();
+        sync();
+    }
+
+    // Copies the device state back into the host attribute arrays.
+    void sync()
+    {
+        if (!h_ || !host_stale_) return;
+        zmpm_particles p = view();
+        zmpm_status s = zmpm_download(h_, &p);
+        if (s != ZMPM_OK && s != ZMPM_E_CFL) check(s, "zmpm_download");
+        host_stale_ = false;
+    }
+
+    // Flat binary state in upload order (the reference streams DataManager
+    // attributes, MpmSimulationBase.cpp:1060-1098).
+    virtual void writeState(std::ostream& out)
+    {
+        sync();
+        const int64_t n = (int64_t)X.size();
+        out.write((const char*)&n, sizeof(n));
+        out.write((const char*)X.data(), n * sizeof(TV));
+        out.write((const char*)V.data(), n * sizeof(TV));
+        out.write((const char*)C.data(), n * sizeof(TM));
+        out.write((const char*)F.data(), n * sizeof(TM));
+        out.write((const char*)mass.data(), n * sizeof(double));
+        out.write((const char*)element_measure.data(), n * sizeof(double));
+        out.write((const char*)material.data(), n * sizeof(int32_t));
+        out.write((const char*)damage.data(), n * sizeof(double));
+        out.write((const char*)degradation.data(), n * sizeof(double));
+    }
+
+    zmpm_handle handle() const { return h_; }
+
+private:
+    zmpm_handle h_ = nullptr;
+    bool dirty_ = true;
+    bool host_stale_ = false;
+
+    zmpm_particles view()
+    {
+        zmpm_particles p {};
+        p.n = (int64_t)X.size();
+        p.x = X.empty() ? nullptr : X[0].data();
+        p.v = V.empty() ? nullptr : V[0].data();
+        p.C = C.empty() ? nullptr : C[0].data();
+        p.F = F.empty() ? nullptr : F[0].data();
+        p.m = mass.data();
+        p.V0 = element_measure.data();
+        p.material = material.data();
+        p.d = damage.data();
+        p.g = degradation.data();
+        p.laplacian = laplacian.data();
+        return p;
+    }
+
+    void upload()
+    {
+        zmpm_particles p = view();
+        check(zmpm_upload(h_, &p), "zmpm_upload");
+        dirty_ = false;
+        host_stale_ = false;
+    }
+
+    void check(zmpm_status s, const char* where)
+    {
+        if (s == ZMPM_OK) return;
+        std::string msg = std::string(where) + ": status " + std::to_string((int)s);
+        if (h_) msg += std::string(" (") + zmpm_last_error(h_) + ")";
+        throw ZmpmError(s, msg);
+    }
+};
+
+} // namespace zmpm
