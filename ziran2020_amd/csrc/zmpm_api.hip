@@ -164,7 +164,6 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.sorted_cell, cap);
     ALLOC(D.spid, cap);
     ALLOC(D.touched, D.num_pages);
-    ALLOC(D.scan_tmp, D.num_pages);
     ALLOC(D.page_table, D.num_pages);
     ALLOC(D.block_page, cap_blocks);
     ALLOC(D.nbr, cap_blocks * 8);
@@ -273,14 +272,14 @@ int gridFor(int64_t items, int per, int maxg)
     return (int)std::min<int64_t>(g, maxg);
 }
 
-template <class T, class AccT>
-void launchScan(zmpm_context* h, const int* in, const int* len_dev, int64_t mul, int64_t len_host, int* out,
-                int* total_out, int cap, int* counters)
+template <bool ASSIGN>
+void launchScan(zmpm_context* h, int* in, const int* len_dev, int64_t mul, int64_t len_host, int* out,
+                int* total_out, int cap, int* counters, int* block_page)
 {
     const int g = (int)std::min<int64_t>(h->max_tiles, 2048);
     k_scan_reduce<<<g, WG, 0, h->stream>>>(in, len_dev, mul, len_host, h->tile_sums);
     k_scan_top<<<1, WG, 0, h->stream>>>(h->tile_sums, len_dev, mul, len_host, total_out, cap, counters);
-    k_scan_down<<<g, WG, 0, h->stream>>>(in, len_dev, mul, len_host, h->tile_sums, out);
+    k_scan_down<ASSIGN><<<g, WG, 0, h->stream>>>(in, len_dev, mul, len_host, h->tile_sums, out, block_page, cap);
 }
 
 template <class T, class AccT>
@@ -293,22 +292,22 @@ void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
     const int gp = gridFor(D.n, WG, 1 << 30);
     if (ev) (void)hipEventRecord(ev[0], st);
     // device clock; fixed-point scales of this substep from the previous one's bounds
-    k_begin_step<<<1, WG, 0, st>>>(std::is_same<AccT, long long>::value ? D.scales : nullptr, D.maxbits, D.max_slots,
+    k_begin_step<<<1, BEGIN_WG, 0, st>>>(std::is_same<AccT, long long>::value ? D.scales : nullptr, D.maxbits, D.max_slots,
                                    D.clock, dt);
     k_bin<T, AccT><<<gp, WG, 0, st>>>(D);
-    launchScan<T, AccT>(h, D.touched, nullptr, 1, D.num_pages, D.scan_tmp, D.counters, D.cap_blocks, D.counters);
-    k_assign<T, AccT><<<gridFor(D.num_pages, WG, 8192), WG, 0, st>>>(D);
+    launchScan<true>(h, D.touched, nullptr, 1, D.num_pages, D.page_table, D.counters, D.cap_blocks, D.counters,
+                     D.block_page);
     k_prepare<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, 16384), 64, 0, st>>>(D);
     k_count<T, AccT><<<gp, WG, 0, st>>>(D);
-    launchScan<T, AccT>(h, D.cell_count, D.counters, NB, 0, D.cell_start, nullptr, -1, D.counters);
+    launchScan<false>(h, D.cell_count, D.counters, NB, 0, D.cell_start, nullptr, -1, D.counters, nullptr);
     k_scatter<T, AccT><<<gp, WG, 0, st>>>(D);
-    k_cellsort<T, AccT><<<gridFor(D.cap_nodes, CELLSORT_WG, 16384), CELLSORT_WG, 0, st>>>(D);
+    k_cellsort<T, AccT><<<gridFor(D.cap_nodes, CELLSORT_CELLS, 8192), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[1], st);
-    k_p2g<T, AccT><<<gridFor(D.n, Tile<T>::CHUNK, 1 << 30), WG, 0, st>>>(D);
+    k_p2g<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, 4096), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[2], st);
     k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, 8192), WG, 0, st>>>(D, (T)dt);
     if (ev) (void)hipEventRecord(ev[3], st);
-    k_g2p<T, AccT><<<gp, WG, 0, st>>>(D, (T)dt);
+    k_g2p<T, AccT><<<D.max_slots, WG, 0, st>>>(D, (T)dt);
     if (ev) (void)hipEventRecord(ev[4], st);
     std::swap(D.P, D.Q);
     std::swap(D.pid, D.qid);
@@ -368,10 +367,11 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
     HIPCHK(h, hipMemcpyAsync(D.pid, ids.data(), cap * sizeof(int), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(D.pmat, mat.data(), cap * sizeof(int), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemsetAsync(D.counters, 0, 8 * sizeof(int), h->stream));
-    D.max_slots = (n + WG - 1) / WG;
+    // one bound slot per G2P workgroup (WG particles each)
+    D.max_slots = std::max(1, (n + WG - 1) / WG);
     if (n > 0) {
         // stress of the uploaded state and the first substep's fixed-point bounds
-        k_stress_init<T, AccT><<<gridFor(n, WG, 1 << 30), WG, 0, h->stream>>>(D);
+        k_stress_init<T, AccT><<<D.max_slots, WG, 0, h->stream>>>(D);
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipGetLastError());

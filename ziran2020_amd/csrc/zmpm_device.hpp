@@ -4,8 +4,9 @@
 // One substep (Ziran MpmSimulationBase::advanceOneTimeStep,
 // Lib/MPM/MpmSimulationBase.cpp:262-357) is this launch sequence on the
 // handle's stream:
+//   k_begin_step     device clock, fixed-point scales (float form)
 //   k_bin            base node, SPGrid page/element key, mark touched pages
-//   scan + k_assign  dense block ids in page (Morton) order
+//   scan             dense block ids in page (Morton) order -> page table
 //   k_prepare        zero block accumulators, forward-neighbour block table
 //   k_count          per-cell occupancy (atomic slot)
 //   scan             cell start offsets
@@ -172,9 +173,13 @@ __global__ __launch_bounds__(WG) void k_scan_top(int* __restrict__ tile_sums, co
     }
 }
 
-__global__ __launch_bounds__(WG) void k_scan_down(const int* __restrict__ in, const int* len_dev, int64_t mul,
+// Down-sweep.  ASSIGN: the input is the per-page "touched" flag and the
+// epilogue writes the page table (dense block id in page order, or -1), the
+// block -> page list, and clears the flag for the next substep.
+template <bool ASSIGN>
+__global__ __launch_bounds__(WG) void k_scan_down(int* __restrict__ in, const int* len_dev, int64_t mul,
                                                  int64_t len_host, const int* __restrict__ tile_off,
-                                                 int* __restrict__ out)
+                                                 int* __restrict__ out, int* __restrict__ block_page, int cap)
 {
     const int64_t len = scanLength(len_dev, mul, len_host);
     const int64_t ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
@@ -190,7 +195,21 @@ __global__ __launch_bounds__(WG) void k_scan_down(const int* __restrict__ in, co
         int ex = blockExclusiveScan256(s, &tot) + tile_off[tile];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            if (base + k < len) out[base + k] = ex;
+            if (base + k < len) {
+                if (ASSIGN) {
+                    int id = -1;
+                    if (v[k]) {
+                        in[base + k] = 0;
+                        if (ex < cap) {
+                            id = ex;
+                            block_page[ex] = (int)(base + k);
+                        }
+                    }
+                    out[base + k] = id;
+                }
+                else
+                    out[base + k] = ex;
+            }
             ex += v[k];
         }
     }
@@ -240,25 +259,6 @@ __global__ __launch_bounds__(WG) void k_bin(Dev<T, AccT> D)
             for (int b = 0; b < 2; b++)
 #pragma unroll
                 for (int c = 0; c < 2; c++) D.touched[pageOfBlock<T>(D.masks, Bx + a, By + b, Bz + c)] = 1;
-    }
-}
-
-template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_assign(Dev<T, AccT> D)
-{
-    const int cap = D.cap_blocks;
-    for (int64_t q = (int64_t)blockIdx.x * WG + threadIdx.x; q < D.num_pages; q += (int64_t)gridDim.x * WG) {
-        int t = D.touched[q];
-        int id = -1;
-        if (t) {
-            id = D.scan_tmp[q];
-            if (id < cap)
-                D.block_page[id] = (int)q;
-            else
-                id = -1;
-            D.touched[q] = 0;
-        }
-        D.page_table[q] = id;
     }
 }
 
@@ -340,61 +340,82 @@ __global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D)
 
 // Order inside a cell by original particle index: together with the page-ordered
 // block ids and element-ordered cells this reproduces std::sort of the
-// reference's unique 64-bit keys (MpmSimulationBase.cpp:1552).  One thread per
-// cell sorts its (index, live slot) pairs in a private LDS slice; cells above
-// the slice size use a shell sort in place in HBM.
-constexpr int CELLSORT_WG = 64, CELLSORT_SLICE = 48;
+// reference's unique 64-bit keys (MpmSimulationBase.cpp:1552).  A workgroup
+// takes 256 consecutive cells; their particles are contiguous, so (index,
+// live slot) pairs are staged in LDS with coalesced loads, each thread
+// insertion-sorts its cell there, and the result and the cell ids go back
+// coalesced.  Ranges larger than the LDS stage fall back to a per-cell shell
+// sort in HBM.
+constexpr int CELLSORT_CELLS = 256, CELLSORT_STAGE = 4096;
 template <class T, class AccT>
-__global__ __launch_bounds__(CELLSORT_WG) void k_cellsort(Dev<T, AccT> D)
+__global__ __launch_bounds__(WG) void k_cellsort(Dev<T, AccT> D)
 {
     using B = Blk<T>;
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
-    __shared__ int kk[CELLSORT_SLICE][CELLSORT_WG];
-    __shared__ int vv[CELLSORT_SLICE][CELLSORT_WG];
+    __shared__ int kk[CELLSORT_STAGE];
+    __shared__ int vv[CELLSORT_STAGE];
+    __shared__ int cc[CELLSORT_STAGE];
     const int t = threadIdx.x;
     const int64_t ncells = (int64_t)D.counters[0] * NB;
-    for (int64_t c0 = (int64_t)blockIdx.x * CELLSORT_WG; c0 < ncells; c0 += (int64_t)gridDim.x * CELLSORT_WG) {
+    for (int64_t c0 = (int64_t)blockIdx.x * CELLSORT_CELLS; c0 < ncells; c0 += (int64_t)gridDim.x * CELLSORT_CELLS) {
+        const int64_t clast = min(c0 + CELLSORT_CELLS, ncells) - 1;
+        const int S0 = D.cell_start[c0];
+        const int S1 = D.cell_start[clast] + D.cell_count[clast];
         const int64_t c = c0 + t;
-        if (c >= ncells) continue;
-        const int cnt = D.cell_count[c];
-        if (cnt == 0) continue;
-        const int s0 = D.cell_start[c];
-        int* key = D.spid + s0;
-        int* val = D.perm + s0;
-        if (cnt <= CELLSORT_SLICE) {
-            for (int i = 0; i < cnt; i++) {
-                const int k = key[i], v = val[i];
-                int j = i;
-                while (j > 0 && kk[j - 1][t] > k) {
-                    kk[j][t] = kk[j - 1][t];
-                    vv[j][t] = vv[j - 1][t];
-                    j--;
+        const bool mine = c <= clast;
+        const int cs = mine ? D.cell_start[c] : 0;
+        const int cnt = mine ? D.cell_count[c] : 0;
+        if (S1 - S0 <= CELLSORT_STAGE) {
+            for (int i = t; i < S1 - S0; i += WG) {
+                kk[i] = D.spid[S0 + i];
+                vv[i] = D.perm[S0 + i];
+            }
+            __syncthreads();
+            if (cnt > 1) {
+                int* k = kk + (cs - S0);
+                int* v = vv + (cs - S0);
+                for (int i = 1; i < cnt; i++) {
+                    const int kv = k[i], vv_i = v[i];
+                    int j = i;
+                    while (j > 0 && k[j - 1] > kv) {
+                        k[j] = k[j - 1];
+                        v[j] = v[j - 1];
+                        j--;
+                    }
+                    k[j] = kv;
+                    v[j] = vv_i;
                 }
-                kk[j][t] = k;
-                vv[j][t] = v;
             }
-            for (int i = 0; i < cnt; i++) {
-                key[i] = kk[i][t];
-                val[i] = vv[i][t];
+            for (int i = 0; i < cnt; i++) cc[cs - S0 + i] = (int)c;
+            __syncthreads();
+            for (int i = t; i < S1 - S0; i += WG) {
+                D.spid[S0 + i] = kk[i];
+                D.perm[S0 + i] = vv[i];
+                D.sorted_cell[S0 + i] = cc[i];
             }
+            __syncthreads();
         }
         else {
-            int gap = 1;
-            while (gap < cnt / 3) gap = 3 * gap + 1;
-            for (; gap >= 1; gap /= 3)
-                for (int i = gap; i < cnt; i++) {
-                    const int k = key[i], v = val[i];
-                    int j = i;
-                    while (j >= gap && key[j - gap] > k) {
-                        key[j] = key[j - gap];
-                        val[j] = val[j - gap];
-                        j -= gap;
+            if (cnt > 0) {
+                int* key = D.spid + cs;
+                int* val = D.perm + cs;
+                int gap = 1;
+                while (gap < cnt / 3) gap = 3 * gap + 1;
+                for (; gap >= 1; gap /= 3)
+                    for (int i = gap; i < cnt; i++) {
+                        const int kv = key[i], v = val[i];
+                        int j = i;
+                        while (j >= gap && key[j - gap] > kv) {
+                            key[j] = key[j - gap];
+                            val[j] = val[j - gap];
+                            j -= gap;
+                        }
+                        key[j] = kv;
+                        val[j] = v;
                     }
-                    key[j] = k;
-                    val[j] = v;
-                }
+                for (int i = 0; i < cnt; i++) D.sorted_cell[cs + i] = (int)c;
+            }
         }
-        for (int i = 0; i < cnt; i++) D.sorted_cell[s0 + i] = (int)c;
     }
 }
 
@@ -476,34 +497,36 @@ __device__ __forceinline__ float waveMax(float v)
     return v;
 }
 
-// Upper bounds of one particle's P2G contributions per scale group: mass w m;
-// momentum w (m v + m C (x_i - x_p)) with |x_i - x_p| <= 1.5 dx; force
-// tau grad(w) with |grad w| <= 1/dx per axis; phase field w d and w (w <= 1).
-// Reduced over the workgroup into its own slot of the bound array.
+// Upper bounds of one particle's P2G contributions per scale group, folded
+// into the running per-thread maxima b: mass w m; momentum w (m v + m C (x_i - x_p))
+// with |x_i - x_p| <= 1.5 dx; force tau grad(w) with |grad w| <= 1/dx per axis;
+// phase field w d and w (w <= 1).
 template <class T>
-__device__ __forceinline__ void recordBounds(unsigned int* maxbits, int nslots, bool live, T m, const T* v,
-                                             const T* Cm, const T* S, T d, T dx, T odx, bool dmg)
+__device__ __forceinline__ void particleBounds(float* b, T m, const T* v, const T* Cm, const T* S, T d, T dx, T odx,
+                                               bool dmg)
 {
-    float b[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
-    if (live) {
-        b[SC_M] = fabsf((float)m);
-        float mv = 0.f;
+    b[SC_M] = fmaxf(b[SC_M], fabsf((float)m));
+    float mv = 0.f;
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
-            float cr = fabsf((float)Cm[3 * r]) + fabsf((float)Cm[3 * r + 1]) + fabsf((float)Cm[3 * r + 2]);
-            mv = fmaxf(mv, fabsf((float)m) * (fabsf((float)v[r]) + 1.5f * (float)dx * cr));
-        }
-        const float a0 = fabsf((float)S[0]), a1 = fabsf((float)S[1]), a2 = fabsf((float)S[2]);
-        const float a3 = fabsf((float)S[3]), a4 = fabsf((float)S[4]), a5 = fabsf((float)S[5]);
-        const float f = (float)odx * fmaxf(a0 + a1 + a2, fmaxf(a1 + a3 + a4, a2 + a4 + a5));
-        b[SC_MV] = mv * 1.0001f;
-        b[SC_F] = f * 1.0001f;
-        if (dmg) {
-            b[SC_PF] = fabsf((float)d);
-            b[SC_PW] = 1.f;
-        }
+    for (int r = 0; r < 3; r++) {
+        float cr = fabsf((float)Cm[3 * r]) + fabsf((float)Cm[3 * r + 1]) + fabsf((float)Cm[3 * r + 2]);
+        mv = fmaxf(mv, fabsf((float)m) * (fabsf((float)v[r]) + 1.5f * (float)dx * cr));
     }
-    // workgroup maximum -> this workgroup's slot (no contended atomics)
+    const float a0 = fabsf((float)S[0]), a1 = fabsf((float)S[1]), a2 = fabsf((float)S[2]);
+    const float a3 = fabsf((float)S[3]), a4 = fabsf((float)S[4]), a5 = fabsf((float)S[5]);
+    const float f = (float)odx * fmaxf(a0 + a1 + a2, fmaxf(a1 + a3 + a4, a2 + a4 + a5));
+    b[SC_MV] = fmaxf(b[SC_MV], mv * 1.0001f);
+    b[SC_F] = fmaxf(b[SC_F], f * 1.0001f);
+    if (dmg) {
+        b[SC_PF] = fmaxf(b[SC_PF], fabsf((float)d));
+        b[SC_PW] = 1.f;
+    }
+}
+
+// Workgroup maximum of the per-thread bounds -> this workgroup's slot
+// (one slot per workgroup of a fixed-size launch, no contended atomics).
+__device__ __forceinline__ void writeBounds(unsigned int* maxbits, int nslots, const float* b)
+{
     __shared__ float wmax[NSC][WG / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -520,31 +543,28 @@ __device__ __forceinline__ void recordBounds(unsigned int* maxbits, int nslots, 
     }
 }
 
-// Stress of the uploaded state and the P2G bounds of the first substep.
+// Stress of the uploaded state and the P2G bounds of the first substep
+// (grid of exactly max_slots workgroups, striding over particles).
 template <class T, class AccT>
 __global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D)
 {
-    const int p = blockIdx.x * WG + threadIdx.x;
-    const bool live = p < D.n;
-    T S[6] = { T(0), T(0), T(0), T(0), T(0), T(0) }, v[3] = { T(0), T(0), T(0) }, Cm[9], m = T(0), d = T(0);
-    bool dmg = false;
-    if (live) {
-        const int64_t cap = D.cap;
+    float b[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
+    const int64_t cap = D.cap;
+    for (int p = blockIdx.x * WG + threadIdx.x; p < D.n; p += gridDim.x * WG) {
         const MatDev<T>& mt = D.mats[D.pmat[p]];
         Mat3<T> F = loadM(D.P, cap, PF, p);
+        T S[6], v[3], Cm[9];
         vtauSym(mt, F, D.P[PVOL * cap + p], D.P[PG * cap + p], S);
 #pragma unroll
         for (int i = 0; i < 6; i++) D.P[(PS + i) * cap + p] = S[i];
-        m = D.P[PM * cap + p];
 #pragma unroll
         for (int a = 0; a < 3; a++) v[a] = D.P[(PV + a) * cap + p];
 #pragma unroll
         for (int i = 0; i < 9; i++) Cm[i] = D.P[(PC + i) * cap + p];
-        dmg = D.any_damage && mt.has_damage;
-        d = D.P[PD * cap + p];
+        const bool dmg = D.any_damage && mt.has_damage;
+        particleBounds(b, D.P[PM * cap + p], v, Cm, S, dmg ? D.P[PD * cap + p] : T(0), D.dx, D.odx, D.any_damage != 0);
     }
-    if constexpr (std::is_same<AccT, long long>::value)
-        recordBounds(D.maxbits, D.max_slots, live, m, v, Cm, S, dmg ? d : T(0), D.dx, D.odx, D.any_damage != 0);
+    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, D.max_slots, b);
 }
 
 // Block-local LDS tile geometry: a block of BX x BY x BZ nodes plus the +2
@@ -583,14 +603,16 @@ __device__ __forceinline__ void ldsAdd(AccT* a, T v, float scale, int& ovf)
 // reading the stress of forcesUpdateParticleState (:953-979) from the particle,
 // plus the AnisoMPM phase-field transfer (AnisoFractureSimulation.h:161-176).
 //
-// One workgroup takes CHUNK consecutive sorted particles.  Phase A: one thread
-// per particle builds its P2G payload in LDS.  Phase B, per block segment of
-// the chunk: lanes are (cell, i, j) columns of the 27-node stencil; each lane
-// walks the particles of its cell (all share one base node) and accumulates
-// its 3 nodes x 9 quantities in registers, then adds them into an LDS tile of
-// the block + halo.  The tile is flushed to HBM with one atomic per non-zero
+// One workgroup per grid block (grid-stride over blocks).  The block's sorted
+// particles are taken CHUNK at a time, counted from the block's first particle.
+// Phase A: one thread per particle writes its P2G payload to LDS.  Phase B: the
+// cells holding particles of the chunk are compacted; lanes are (sub-lane,
+// cell, i, j) columns of the 27-node stencil; each lane walks every R-th
+// particle of its cell (all share one base node), accumulates 3 nodes x 9
+// quantities in registers and adds them into an LDS tile of the block + halo.
+// After the last chunk the tile is flushed to HBM with one atomic per non-zero
 // node value.  FP32 form: per-lane float partial sums in sorted order, then
-// int64 fixed point in LDS and HBM (result independent of scheduling).
+// int64 fixed point in LDS and HBM, so the result does not depend on scheduling.
 template <class T, class AccT>
 __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
 {
@@ -604,139 +626,137 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
     __shared__ int cs[NB], ce[NB], cl[NB];
     __shared__ int ncl;
 
-    const int lo = blockIdx.x * CH;
-    const int hi = min(lo + CH, D.n);
-    if (lo >= hi) return;
     const int t = threadIdx.x;
     const bool dmg = D.any_damage != 0;
     const int nq = dmg ? NGF : GPF;
-    const int64_t cap = D.cap;
-
-    // ---- phase A: payload of particle lo + t
-    if (t < CH && lo + t < hi) {
-        int p = D.perm[lo + t];
-        if ((unsigned)p >= (unsigned)D.n) p = 0;
-        const T* P = D.P;
-        const T mass = P[PM * cap + p];
-        T X[3], w[3][3], dw[3][3], xb[3];
-        int base[3];
-        const T hi_t = T(D.domain - 4);
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            X[a] = P[(PX + a) * cap + p];
-            T xs = mul_rn(X[a], D.odx);
-            T tt = sub_rn(xs, T(0.5));
-            if (!(tt >= T(0) && tt < hi_t)) tt = (tt >= hi_t) ? hi_t - T(1) : T(0); // latched by k_bin
-            base[a] = (int)tt;
-            bspline(xs, base[a], w[a], dw[a]);
-            xb[a] = T(base[a]) * D.dx - X[a];
-        }
-        T mC[9], a3[3];
-#pragma unroll
-        for (int i = 0; i < 9; i++) mC[i] = mass * P[(PC + i) * cap + p];
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-            a3[r] = mass * P[(PV + r) * cap + p] + mC[3 * r] * xb[0] + mC[3 * r + 1] * xb[1] + mC[3 * r + 2] * xb[2];
-        T Sv[6];
-#pragma unroll
-        for (int i = 0; i < 6; i++) Sv[i] = P[(PS + i) * cap + p] * D.odx;
-        const T dp = dmg && D.mats[D.pmat[p]].has_damage ? P[PD * cap + p] : T(0);
-        const T dx = D.dx;
-        q[0][t] = V4<T> { mass, a3[0], a3[1], a3[2] };
-        q[1][t] = V4<T> { mC[0] * dx, mC[1] * dx, mC[2] * dx, mC[3] * dx };
-        q[2][t] = V4<T> { mC[4] * dx, mC[5] * dx, mC[6] * dx, mC[7] * dx };
-        q[3][t] = V4<T> { mC[8] * dx, Sv[0], Sv[1], Sv[2] };
-        q[4][t] = V4<T> { Sv[3], Sv[4], Sv[5], dp };
-        q[5][t] = V4<T> { w[2][0], w[2][1], w[2][2], T(0) };
-        q[6][t] = V4<T> { dw[2][0], dw[2][1], dw[2][2], T(0) };
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            wl[0][t][i] = w[0][i];
-            wl[1][t][i] = dw[0][i];
-            wl[2][t][i] = w[1][i];
-            wl[3][t][i] = dw[1][i];
-        }
-    }
+    const int64_t cap = D.cap, cn = D.cap_nodes;
+    const int nblocks = D.counters[0];
+    const float* sc = D.scales;
+    const float scl[NGF] = { sc[SC_M], sc[SC_MV], sc[SC_MV], sc[SC_MV], sc[SC_F], sc[SC_F], sc[SC_F], sc[SC_PF], sc[SC_PW] };
     int ovf = 0;
-    int seg_lo = lo;
-    while (seg_lo < hi) {
-        const int b = D.sorted_cell[seg_lo] / NB;
-        __syncthreads(); // payload written / previous segment's tile flushed
+    for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        __syncthreads(); // previous block flushed
         if (t < NB) {
             const int c = b * NB + t;
             cs[t] = D.cell_start[c];
             ce[t] = D.cell_start[c] + D.cell_count[c];
         }
+        __syncthreads();
+        const int bs = cs[0], be = ce[NB - 1];
+        if (be <= bs) continue; // halo-only block
         for (int e = t; e < nq * TL::TN; e += WG) (&tile[0][0])[e] = AccT(0);
-        __syncthreads();
-        const int seg_hi = max(seg_lo + 1, min(hi, ce[NB - 1]));
-        // ---- compact the cells holding particles of this segment (NB <= 64: one wavefront)
-        if (t < 64) {
-            bool act = false;
-            if (t < NB) act = min(ce[t], seg_hi) > max(cs[t], seg_lo);
-            const uint64_t m = __ballot(act);
-            if (act) cl[__popcll(m & ((1ull << t) - 1))] = t;
-            if (t == 0) ncl = __popcll(m);
-        }
-        __syncthreads();
-        // ---- phase B: lanes = (sub-lane, cell, i, j); R sub-lanes share a column
-        // and take every R-th particle of the cell
-        const int per = ncl * 9;
-        const int R = per > 0 ? max(1, WG / per) : 1;
-        for (int slot = t; slot < per * R; slot += WG) {
-            const int sub = slot / per, rem = slot - per * (slot / per);
-            const int cell = cl[rem / 9], col = rem - 9 * (rem / 9);
-            const int ci = col / 3, cj = col - 3 * (col / 3);
-            const int p0 = max(cs[cell], seg_lo) - lo + sub, p1 = min(ce[cell], seg_hi) - lo;
-            if (p0 >= p1) continue;
-            const T fi = T(ci), fj = T(cj);
-            T acc[3][NGF];
+        for (int lo = bs; lo < be; lo += CH) {
+            const int hi = min(lo + CH, be);
+            __syncthreads(); // previous chunk's phase B done with the payload
+            // ---- phase A: payload of particle lo + t
+            if (t < CH && lo + t < hi) {
+                int p = D.perm[lo + t];
+                if ((unsigned)p >= (unsigned)D.n) p = 0;
+                const T* P = D.P;
+                const T mass = P[PM * cap + p];
+                T X[3], w[3][3], dw[3][3], xb[3];
+                int base[3];
+                const T hi_t = T(D.domain - 4);
 #pragma unroll
-            for (int k = 0; k < 3; k++)
+                for (int a = 0; a < 3; a++) {
+                    X[a] = P[(PX + a) * cap + p];
+                    T xs = mul_rn(X[a], D.odx);
+                    T tt = sub_rn(xs, T(0.5));
+                    if (!(tt >= T(0) && tt < hi_t)) tt = (tt >= hi_t) ? hi_t - T(1) : T(0); // latched by k_bin
+                    base[a] = (int)tt;
+                    bspline(xs, base[a], w[a], dw[a]);
+                    xb[a] = T(base[a]) * D.dx - X[a];
+                }
+                T mC[9], a3[3];
 #pragma unroll
-                for (int f = 0; f < NGF; f++) acc[k][f] = T(0);
-            for (int pp = p0; pp < p1; pp += R) {
-                const V4<T> q0 = q[0][pp], q1 = q[1][pp], q2 = q[2][pp], q3 = q[3][pp], q4 = q[4][pp];
-                const V4<T> q5 = q[5][pp], q6 = q[6][pp];
-                const T wx = wl[0][pp][ci], dwx = wl[1][pp][ci], wy = wl[2][pp][cj], dwy = wl[3][pp][cj];
-                const T wxy = wx * wy, gx = dwx * wy, gy = wx * dwy;
-                // a + B (i, j, 0): momentum + m C (x_i - x_p) at the column base
-                const T a0 = q0.y + q1.x * fi + q1.y * fj;
-                const T a1 = q0.z + q1.w * fi + q2.x * fj;
-                const T a2 = q0.w + q2.z * fi + q2.w * fj;
-                const T b0 = q1.z, b1 = q2.y, b2 = q3.x; // B column z
-                const T s00 = q3.y, s01 = q3.z, s02 = q3.w, s11 = q4.x, s12 = q4.y, s22 = q4.z;
-                const T wz[3] = { q5.x, q5.y, q5.z }, dwz[3] = { q6.x, q6.y, q6.z };
+                for (int i = 0; i < 9; i++) mC[i] = mass * P[(PC + i) * cap + p];
 #pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const T wk = wxy * wz[k];
-                    const T g0 = gx * wz[k], g1 = gy * wz[k], g2 = wxy * dwz[k];
-                    const T fk = T(k);
-                    acc[k][GM] += q0.x * wk;
-                    acc[k][GMV + 0] += wk * (a0 + b0 * fk);
-                    acc[k][GMV + 1] += wk * (a1 + b1 * fk);
-                    acc[k][GMV + 2] += wk * (a2 + b2 * fk);
-                    acc[k][GFO + 0] -= s00 * g0 + s01 * g1 + s02 * g2;
-                    acc[k][GFO + 1] -= s01 * g0 + s11 * g1 + s12 * g2;
-                    acc[k][GFO + 2] -= s02 * g0 + s12 * g1 + s22 * g2;
-                    acc[k][GPF] += q4.w * wk;
-                    acc[k][GPW] += wk;
+                for (int r = 0; r < 3; r++)
+                    a3[r] = mass * P[(PV + r) * cap + p] + mC[3 * r] * xb[0] + mC[3 * r + 1] * xb[1] + mC[3 * r + 2] * xb[2];
+                T Sv[6];
+#pragma unroll
+                for (int i = 0; i < 6; i++) Sv[i] = P[(PS + i) * cap + p] * D.odx;
+                const T dp = dmg && D.mats[D.pmat[p]].has_damage ? P[PD * cap + p] : T(0);
+                const T dx = D.dx;
+                q[0][t] = V4<T> { mass, a3[0], a3[1], a3[2] };
+                q[1][t] = V4<T> { mC[0] * dx, mC[1] * dx, mC[2] * dx, mC[3] * dx };
+                q[2][t] = V4<T> { mC[4] * dx, mC[5] * dx, mC[6] * dx, mC[7] * dx };
+                q[3][t] = V4<T> { mC[8] * dx, Sv[0], Sv[1], Sv[2] };
+                q[4][t] = V4<T> { Sv[3], Sv[4], Sv[5], dp };
+                q[5][t] = V4<T> { w[2][0], w[2][1], w[2][2], T(0) };
+                q[6][t] = V4<T> { dw[2][0], dw[2][1], dw[2][2], T(0) };
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    wl[0][t][i] = w[0][i];
+                    wl[1][t][i] = dw[0][i];
+                    wl[2][t][i] = w[1][i];
+                    wl[3][t][i] = dw[1][i];
                 }
             }
-            const int cx = cell >> (B::ybits + B::zbits), cy = (cell >> B::zbits) & (TL::BY - 1), cz = cell & (TL::BZ - 1);
-            const int nbase = ((cx + ci) * TL::TY + (cy + cj)) * TL::TZ + cz;
-            const float* sc = D.scales;
-            const float scl[NGF] = { sc[SC_M], sc[SC_MV], sc[SC_MV], sc[SC_MV], sc[SC_F], sc[SC_F], sc[SC_F], sc[SC_PF], sc[SC_PW] };
+            // ---- cells holding particles of this chunk (NB <= 64: one wavefront)
+            if (t < 64) {
+                bool act = false;
+                if (t < NB) act = min(ce[t], hi) > max(cs[t], lo);
+                const uint64_t m = __ballot(act);
+                if (act) cl[__popcll(m & ((1ull << t) - 1))] = t;
+                if (t == 0) ncl = __popcll(m);
+            }
+            __syncthreads();
+            // ---- phase B: lanes = (sub-lane, cell, i, j); R sub-lanes share a column
+            const int per = ncl * 9;
+            const int R = per > 0 ? max(1, WG / per) : 1;
+            for (int slot = t; slot < per * R; slot += WG) {
+                const int sub = slot / per, rem = slot - per * (slot / per);
+                const int cell = cl[rem / 9], col = rem - 9 * (rem / 9);
+                const int ci = col / 3, cj = col - 3 * (col / 3);
+                const int p0 = max(cs[cell], lo) - lo + sub, p1 = min(ce[cell], hi) - lo;
+                if (p0 >= p1) continue;
+                const T fi = T(ci), fj = T(cj);
+                T acc[3][NGF];
 #pragma unroll
-            for (int k = 0; k < 3; k++)
+                for (int k = 0; k < 3; k++)
 #pragma unroll
-                for (int f = 0; f < NGF; f++)
-                    if (f < nq) ldsAdd<T, AccT>(&tile[f][nbase + k], acc[k][f], scl[f], ovf);
+                    for (int f = 0; f < NGF; f++) acc[k][f] = T(0);
+                for (int pp = p0; pp < p1; pp += R) {
+                    const V4<T> q0 = q[0][pp], q1 = q[1][pp], q2 = q[2][pp], q3 = q[3][pp], q4 = q[4][pp];
+                    const V4<T> q5 = q[5][pp], q6 = q[6][pp];
+                    const T wx = wl[0][pp][ci], dwx = wl[1][pp][ci], wy = wl[2][pp][cj], dwy = wl[3][pp][cj];
+                    const T wxy = wx * wy, gx = dwx * wy, gy = wx * dwy;
+                    // momentum + m C (x_i - x_p) at the column base node (k = 0)
+                    const T a0 = q0.y + q1.x * fi + q1.y * fj;
+                    const T a1 = q0.z + q1.w * fi + q2.x * fj;
+                    const T a2 = q0.w + q2.z * fi + q2.w * fj;
+                    const T b0 = q1.z, b1 = q2.y, b2 = q3.x; // m C dx, z column
+                    const T s00 = q3.y, s01 = q3.z, s02 = q3.w, s11 = q4.x, s12 = q4.y, s22 = q4.z;
+                    const T wz[3] = { q5.x, q5.y, q5.z }, dwz[3] = { q6.x, q6.y, q6.z };
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        const T wk = wxy * wz[k];
+                        const T g0 = gx * wz[k], g1 = gy * wz[k], g2 = wxy * dwz[k];
+                        const T fk = T(k);
+                        acc[k][GM] += q0.x * wk;
+                        acc[k][GMV + 0] += wk * (a0 + b0 * fk);
+                        acc[k][GMV + 1] += wk * (a1 + b1 * fk);
+                        acc[k][GMV + 2] += wk * (a2 + b2 * fk);
+                        acc[k][GFO + 0] -= s00 * g0 + s01 * g1 + s02 * g2;
+                        acc[k][GFO + 1] -= s01 * g0 + s11 * g1 + s12 * g2;
+                        acc[k][GFO + 2] -= s02 * g0 + s12 * g1 + s22 * g2;
+                        acc[k][GPF] += q4.w * wk;
+                        acc[k][GPW] += wk;
+                    }
+                }
+                const int cx = cell >> (B::ybits + B::zbits), cy = (cell >> B::zbits) & (TL::BY - 1), cz = cell & (TL::BZ - 1);
+                const int nbase = ((cx + ci) * TL::TY + (cy + cj)) * TL::TZ + cz;
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+#pragma unroll
+                    for (int f = 0; f < NGF; f++)
+                        if (f < nq) ldsAdd<T, AccT>(&tile[f][nbase + k], acc[k][f], scl[f], ovf);
+            }
         }
         __syncthreads();
         // ---- flush the tile: one HBM atomic per non-zero node value
         const int* nb = D.nbr + b * 8;
-        const int64_t cn = D.cap_nodes;
         for (int e = t; e < nq * TL::TN; e += WG) {
             const int f = e / TL::TN, nd = e - TL::TN * (e / TL::TN);
             const AccT v = tile[f][nd];
@@ -755,7 +775,6 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
             else
                 unsafeAtomicAdd(dst, v);
         }
-        seg_lo = seg_hi;
     }
     if (ovf & 1) latch(D.counters, ST_OVERFLOW);
     if (ovf & 2) latch(D.counters, ST_CAPACITY);
@@ -764,7 +783,8 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
 // Start of a substep: advance the device clock and (float form) set this
 // substep's fixed-point scales from the bounds the previous G2P (or the upload)
 // wrote, one slot per workgroup: 2^(40 - e) with max = f 2^e, f in [0.5,1).
-__global__ __launch_bounds__(WG) void k_begin_step(float* scales, const unsigned int* maxbits, int nslots, double* clock,
+constexpr int BEGIN_WG = 1024;
+__global__ __launch_bounds__(BEGIN_WG) void k_begin_step(float* scales, const unsigned int* maxbits, int nslots, double* clock,
                                                   double dt)
 {
     // device clock: clock[0] = time of the next substep, clock[1] = this substep's t
@@ -773,22 +793,20 @@ __global__ __launch_bounds__(WG) void k_begin_step(float* scales, const unsigned
         clock[0] = clock[0] + dt;
     }
     if (!scales) return;
-    __shared__ float red[NSC][WG];
+    __shared__ float red[NSC][BEGIN_WG / 64];
     float mx[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
-    for (int i = threadIdx.x; i < nslots; i += WG)
+    for (int i = threadIdx.x; i < nslots; i += BEGIN_WG)
 #pragma unroll
         for (int g = 0; g < NSC; g++) mx[g] = fmaxf(mx[g], __uint_as_float(maxbits[g * nslots + i]));
 #pragma unroll
-    for (int g = 0; g < NSC; g++) red[g][threadIdx.x] = mx[g];
-    __syncthreads();
-    for (int w = WG / 2; w >= 1; w >>= 1) {
-        if (threadIdx.x < w)
-#pragma unroll
-            for (int g = 0; g < NSC; g++) red[g][threadIdx.x] = fmaxf(red[g][threadIdx.x], red[g][threadIdx.x + w]);
-        __syncthreads();
+    for (int g = 0; g < NSC; g++) {
+        const float w = waveMax(mx[g]);
+        if ((threadIdx.x & 63) == 0) red[g][threadIdx.x >> 6] = w;
     }
+    __syncthreads();
     if (threadIdx.x < NSC) {
-        const float m = red[threadIdx.x][0];
+        float m = 0.f;
+        for (int w = 0; w < BEGIN_WG / 64; w++) m = fmaxf(m, red[threadIdx.x][w]);
         if (m > 0.f && isfinite(m)) {
             int e;
             frexpf(m, &e);
@@ -964,18 +982,152 @@ __device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>&
 
 // gridToParticlesHelper<APIC, deg 2, !MLS> (MpmSimulationBase.cpp:1412-1478),
 // CFL flags (:1466-1499), evolveStrain (FBasedMpmForceHelper.cpp:93-108,
-// JBasedMpmForceHelper.cpp:79-94), plus the AnisoMPM Laplacian gather
-// (iterateKernelWithLaplacian, MpmGrid.h:552-623, with its 3D index quirk) and
-// damage update on F_n.  Output goes to sorted position s of the other buffer.
-// gridToParticlesHelper<APIC, deg 2, !MLS> (MpmSimulationBase.cpp:1412-1478),
-// CFL flags (:1466-1499), evolveStrain (FBasedMpmForceHelper.cpp:93-108,
 // JBasedMpmForceHelper.cpp:79-94), the AnisoMPM Laplacian gather
 // (iterateKernelWithLaplacian, MpmGrid.h:552-623, with its 3D index quirk) and
 // damage update on F_n, then the stress of (F_{n+1}, g_{n+1}) that the next
 // substep's forcesUpdateParticleState would compute, and the next P2G's
-// fixed-point bounds.  One thread per sorted particle; the grid velocities of
-// each block segment (block + halo) are staged in LDS first.  Output goes to
-// sorted position s of the other buffer.
+// fixed-point bounds.  One thread per sorted particle (max_slots workgroups of
+// WG particles); the grid velocities of each block segment of the workgroup's
+// particles (block + halo) are staged in LDS first.  Output goes to sorted
+// position s of the other buffer.
+// Per-particle G2P state carried from the gather to the finish.
+template <class T>
+struct G2PAcc {
+    T picV[3], Bp[3][3], gv[3][3], lap;
+};
+
+// APIC gather of one particle from the staged block tile.
+template <class T, class AccT>
+__device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[Tile<T>::TN], int p, bool dmg, G2PAcc<T>& A)
+{
+    using TL = Tile<T>;
+    const int64_t cap = D.cap;
+    const T* P = D.P;
+    const T odx = D.odx, dx = D.dx, odx2 = odx * odx;
+    T Xp[3], w[3][3], dw[3][3];
+    int base[3];
+    const T hi_t = T(D.domain - 4);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        Xp[a] = P[(PX + a) * cap + p];
+        T xs = mul_rn(Xp[a], odx);
+        T tt = sub_rn(xs, T(0.5));
+        if (!(tt >= T(0) && tt < hi_t)) tt = (tt >= hi_t) ? hi_t - T(1) : T(0);
+        base[a] = (int)tt;
+        bspline(xs, base[a], w[a], dw[a]);
+    }
+    const int lx0 = base[0] & (TL::BX - 1), ly0 = base[1] & (TL::BY - 1), lz0 = base[2] & (TL::BZ - 1);
+#pragma unroll 1
+    for (int i = 0; i < 3; i++) {
+        const T wi = w[0][i];
+        const T dwidxi = odx * dw[0][i];
+        const T ddwi = (i == 1) ? T(-2) : T(1);
+        const T xi = T(base[0] + i) * dx - Xp[0];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const T wj = w[1][j];
+            const T wij = wi * wj;
+            const T dwijdxi = dwidxi * wj;
+            const T dwijdxj = wi * odx * dw[1][j];
+            const T yj = T(base[1] + j) * dx - Xp[1];
+            const int nrow = ((lx0 + i) * TL::TY + (ly0 + j)) * TL::TZ + lz0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const T wk = w[2][k];
+                const T wijk = wij * wk;
+                const T g3[3] = { dwijdxi * wk, dwijdxj * wk, wij * odx * dw[2][k] };
+                const T zk = T(base[2] + k) * dx - Xp[2];
+                const T nv[3] = { gt[0][nrow + k], gt[1][nrow + k], gt[2][nrow + k] };
+                const T dxv[3] = { xi, yj, zk };
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    A.picV[a] += wijk * nv[a];
+                    const T wv = wijk * nv[a];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        A.Bp[a][c] += wv * dxv[c];
+                        A.gv[a][c] += nv[a] * g3[c];
+                    }
+                }
+                if (dmg) {
+                    // quirk: ddw[1](i), ddw[2](i) instead of (j), (k) (MpmGrid.h:610-612)
+                    const T L = odx2 * ddwi * wj * wk + odx2 * ddwi * wi * wk + odx2 * ddwi * wi * wj;
+                    A.lap += gt[3][nrow + k] * L;
+                }
+            }
+        }
+    }
+}
+
+// Particle update after the gather; writes sorted position s.
+template <class T, class AccT>
+__device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, int p, T dt, const G2PAcc<T>& A, float* bnd)
+{
+    const int64_t cap = D.cap;
+    const T* P = D.P;
+    T* Q = D.Q;
+    const int mid = D.pmat[p];
+    const MatDev<T>& mt = D.mats[mid];
+    const bool dmg = D.any_damage && mt.has_damage;
+    const T dx = D.dx;
+    const T Dinv = T(4) / (dx * dx);
+    const T r = D.rpic;
+    const T ca = (r + T(1)) * T(0.5), cb = (r - T(1)) * T(0.5);
+    T Cn[9];
+#pragma unroll
+    for (int a = 0; a < 3; a++) Q[(PV + a) * cap + s] = A.picV[a];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            Cn[3 * a + c] = ca * (A.Bp[a][c] * Dinv) + cb * (A.Bp[c][a] * Dinv);
+            Q[(PC + 3 * a + c) * cap + s] = Cn[3 * a + c];
+        }
+    T inc2 = T(0);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        T inc = dt * A.picV[a];
+        inc2 += inc * inc;
+        Q[(PX + a) * cap + s] = P[(PX + a) * cap + p] + inc;
+    }
+    if (inc2 > dx * dx) latch(D.counters, ST_CFL);
+    T d = P[PD * cap + p], g = P[PG * cap + p];
+    const T V0 = P[PVOL * cap + p];
+    Mat3<T> Fn = loadM(P, cap, PF, p);
+    if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE)
+        Fn.a[0][0] = (T(1) + dt * (A.gv[0][0] + A.gv[1][1] + A.gv[2][2])) * Fn.a[0][0];
+    else {
+        const Mat3<T> F = Fn;
+        if (dmg && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) damageUpdate(mt, F, A.lap, dt, d, g);
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                T v = F.a[a][c];
+#pragma unroll
+                for (int k = 0; k < 3; k++) v += dt * A.gv[a][k] * F.a[k][c];
+                Fn.a[a][c] = v;
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; i++) Q[(PF + i) * cap + s] = Fn.a[i / 3][i % 3];
+    // stress of the new state for the next substep's P2G
+    T S[6];
+    vtauSym(mt, Fn, V0, g, S);
+#pragma unroll
+    for (int i = 0; i < 6; i++) Q[(PS + i) * cap + s] = S[i];
+    const T m = P[PM * cap + p];
+    Q[PM * cap + s] = m;
+    Q[PVOL * cap + s] = V0;
+    Q[PD * cap + s] = d;
+    Q[PG * cap + s] = g;
+    Q[PLAP * cap + s] = dmg ? A.lap : T(0);
+    D.qid[s] = D.spid[s];
+    D.qmat[s] = mid;
+    if constexpr (std::is_same<AccT, long long>::value)
+        particleBounds(bnd, m, A.picV, Cn, S, dmg ? d : T(0), dx, D.odx, D.any_damage != 0);
+}
+
 template <class T, class AccT>
 __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
 {
@@ -986,40 +1138,26 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
     __shared__ int seg_end;
     const int lo = blockIdx.x * WG;
     const int hi = min(lo + WG, D.n);
-    if (lo >= hi) return;
     const int t = threadIdx.x, s = lo + t;
     const bool live = s < hi;
-    const int64_t cap = D.cap, cn = D.cap_nodes;
-    const T* P = D.P;
-    T* Q = D.Q;
-    int p = 0, mid = 0;
-    T Xp[3] = { T(0), T(0), T(0) }, w[3][3], dw[3][3];
-    int base[3] = { 0, 0, 0 };
+    const int64_t cn = D.cap_nodes;
+    const int nfield = D.any_damage ? 4 : 3;
+    int p = 0;
+    bool dmg = false;
     if (live) {
         p = D.perm[s];
         if ((unsigned)p >= (unsigned)D.n) p = 0;
-        mid = D.pmat[p];
-        const T hi_t = T(D.domain - 4);
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            Xp[a] = P[(PX + a) * cap + p];
-            T xs = mul_rn(Xp[a], D.odx);
-            T tt = sub_rn(xs, T(0.5));
-            if (!(tt >= T(0) && tt < hi_t)) tt = (tt >= hi_t) ? hi_t - T(1) : T(0);
-            base[a] = (int)tt;
-            bspline(xs, base[a], w[a], dw[a]);
-        }
+        dmg = D.any_damage && D.mats[D.pmat[p]].has_damage;
     }
-    const MatDev<T>& mt = D.mats[mid];
-    const bool dmg = live && D.any_damage && mt.has_damage;
-    const T odx = D.odx, dx = D.dx, odx2 = odx * odx;
-    T picV[3] = { T(0), T(0), T(0) }, Bp[3][3], gv[3][3], lap = T(0);
+    G2PAcc<T> A;
+#pragma unroll
+    for (int a = 0; a < 3; a++) A.picV[a] = T(0);
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-        Bp[i / 3][i % 3] = T(0);
-        gv[i / 3][i % 3] = T(0);
+        A.Bp[i / 3][i % 3] = T(0);
+        A.gv[i / 3][i % 3] = T(0);
     }
-    const int nfield = D.any_damage ? 4 : 3;
+    A.lap = T(0);
     int seg_lo = lo;
     while (seg_lo < hi) {
         const int b = D.sorted_cell[seg_lo] / NB;
@@ -1044,111 +1182,12 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
         }
         __syncthreads();
         const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
-        if (live && s >= seg_lo && s < seg_hi) {
-            const int lx0 = base[0] & (TL::BX - 1), ly0 = base[1] & (TL::BY - 1), lz0 = base[2] & (TL::BZ - 1);
-#pragma unroll 1
-            for (int i = 0; i < 3; i++) {
-                const T wi = w[0][i];
-                const T dwidxi = odx * dw[0][i];
-                const T ddwi = (i == 1) ? T(-2) : T(1);
-                const T xi = T(base[0] + i) * dx - Xp[0];
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const T wj = w[1][j];
-                    const T wij = wi * wj;
-                    const T dwijdxi = dwidxi * wj;
-                    const T dwijdxj = wi * odx * dw[1][j];
-                    const T yj = T(base[1] + j) * dx - Xp[1];
-                    const int nrow = ((lx0 + i) * TL::TY + (ly0 + j)) * TL::TZ + lz0;
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        const T wk = w[2][k];
-                        const T wijk = wij * wk;
-                        const T g3[3] = { dwijdxi * wk, dwijdxj * wk, wij * odx * dw[2][k] };
-                        const T zk = T(base[2] + k) * dx - Xp[2];
-                        const T nv[3] = { gt[0][nrow + k], gt[1][nrow + k], gt[2][nrow + k] };
-                        const T dxv[3] = { xi, yj, zk };
-#pragma unroll
-                        for (int a = 0; a < 3; a++) {
-                            picV[a] += wijk * nv[a];
-                            const T wv = wijk * nv[a];
-#pragma unroll
-                            for (int c = 0; c < 3; c++) {
-                                Bp[a][c] += wv * dxv[c];
-                                gv[a][c] += nv[a] * g3[c];
-                            }
-                        }
-                        if (dmg) {
-                            // quirk: ddw[1](i), ddw[2](i) instead of (j), (k) (MpmGrid.h:610-612)
-                            const T L = odx2 * ddwi * wj * wk + odx2 * ddwi * wi * wk + odx2 * ddwi * wi * wj;
-                            lap += gt[3][nrow + k] * L;
-                        }
-                    }
-                }
-            }
-        }
+        if (live && s >= seg_lo && s < seg_hi) g2pGather(D, gt, p, dmg, A);
         seg_lo = seg_hi;
     }
-    T m = T(0), Cn[9], S[6] = { T(0), T(0), T(0), T(0), T(0), T(0) }, dout = T(0);
-#pragma unroll
-    for (int i = 0; i < 9; i++) Cn[i] = T(0);
-    if (live) {
-        const T Dinv = T(4) / (dx * dx);
-        const T r = D.rpic;
-        const T ca = (r + T(1)) * T(0.5), cb = (r - T(1)) * T(0.5);
-#pragma unroll
-        for (int a = 0; a < 3; a++) Q[(PV + a) * cap + s] = picV[a];
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                Cn[3 * a + c] = ca * (Bp[a][c] * Dinv) + cb * (Bp[c][a] * Dinv);
-                Q[(PC + 3 * a + c) * cap + s] = Cn[3 * a + c];
-            }
-        T inc2 = T(0);
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            T inc = dt * picV[a];
-            inc2 += inc * inc;
-            Q[(PX + a) * cap + s] = Xp[a] + inc;
-        }
-        if (inc2 > dx * dx) latch(D.counters, ST_CFL);
-        T d = P[PD * cap + p], g = P[PG * cap + p];
-        const T V0 = P[PVOL * cap + p];
-        Mat3<T> Fn = loadM(P, cap, PF, p);
-        if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE)
-            Fn.a[0][0] = (T(1) + dt * (gv[0][0] + gv[1][1] + gv[2][2])) * Fn.a[0][0];
-        else {
-            const Mat3<T> F = Fn;
-            if (dmg && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) damageUpdate(mt, F, lap, dt, d, g);
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    T v = F.a[a][c];
-#pragma unroll
-                    for (int k = 0; k < 3; k++) v += dt * gv[a][k] * F.a[k][c];
-                    Fn.a[a][c] = v;
-                }
-        }
-#pragma unroll
-        for (int i = 0; i < 9; i++) Q[(PF + i) * cap + s] = Fn.a[i / 3][i % 3];
-        // stress of the new state for the next substep's P2G
-        vtauSym(mt, Fn, V0, g, S);
-#pragma unroll
-        for (int i = 0; i < 6; i++) Q[(PS + i) * cap + s] = S[i];
-        m = P[PM * cap + p];
-        Q[PM * cap + s] = m;
-        Q[PVOL * cap + s] = V0;
-        Q[PD * cap + s] = d;
-        Q[PG * cap + s] = g;
-        Q[PLAP * cap + s] = dmg ? lap : T(0);
-        D.qid[s] = D.spid[s];
-        D.qmat[s] = mid;
-        dout = dmg ? d : T(0);
-    }
-    if constexpr (std::is_same<AccT, long long>::value)
-        recordBounds(D.maxbits, D.max_slots, live, m, picV, Cn, S, dout, D.dx, D.odx, D.any_damage != 0);
+    float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
+    if (live) g2pFinish(D, s, p, dt, A, bnd);
+    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, D.max_slots, bnd);
 }
 
 } // namespace zmpm

@@ -109,7 +109,6 @@ struct Dev {
     int* sorted_cell;  // sorted position -> dense cell id
     int* spid;         // sorted position -> original index (scatter output, sorted in place)
     int* touched;      // per page: 1 if a particle block or its -1 neighbour has particles
-    int* scan_tmp;     // per page exclusive scan / per cell scan scratch
     int* page_table;   // per page: dense block id or -1
     int64_t num_pages;
     int* block_page;   // dense block id -> page index
