@@ -205,6 +205,7 @@ void refreshTables(zmpm_context* h)
     D.any_damage = h->any_damage ? 1 : 0;
     D.dx = (T)h->desc.dx;
     D.odx = T(1) / (T)h->desc.dx;
+    D.Dinv = T(4) / ((T)h->desc.dx * (T)h->desc.dx);
     for (int a = 0; a < 3; a++) D.gravity[a] = (T)h->desc.gravity[a];
     D.rpic = (T)h->desc.apic_rpic_ratio;
     D.cfl = (T)h->desc.cfl;
@@ -235,6 +236,7 @@ zmpm_status uploadMaterials(zmpm_context* h)
         d.sigma_crit = (T)m.sigma_crit;
         d.eta = (T)m.eta;
         d.zeta = (T)m.zeta;
+        d.inv_sigma_crit2 = (T)(1.0 / (m.sigma_crit * m.sigma_crit));
     }
     HIPCHK(h, hipMemcpyAsync(h->mats_dev, md.data(), sizeof(MatDev<T>) * kMaxMaterials, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));

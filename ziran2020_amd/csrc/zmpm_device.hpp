@@ -575,7 +575,7 @@ struct Tile {
     static constexpr int BX = 1 << B::xbits, BY = 1 << B::ybits, BZ = 1 << B::zbits;
     static constexpr int NB = BX * BY * BZ;
     static constexpr int TX = BX + 2, TY = BY + 2, TZ = BZ + 2, TN = TX * TY * TZ;
-    static constexpr int CHUNK = sizeof(T) == 4 ? 256 : 128; // sorted particles per workgroup
+    static constexpr int CHUNK = 128; // sorted particles per P2G payload stage
 };
 
 template <class T>
@@ -597,6 +597,32 @@ __device__ __forceinline__ void ldsAdd(AccT* a, T v, float scale, int& ovf)
     else {
         if (v != T(0)) unsafeAtomicAdd(a, v);
     }
+}
+
+// Particle fields P2G reads, loaded ahead of use.
+template <class T>
+struct RawP2G {
+    T x[3], v[3], C[9], S[6], m, d;
+};
+
+template <class T, class AccT>
+__device__ __forceinline__ void loadP2G(const Dev<T, AccT>& D, int s, bool dmg, RawP2G<T>& r)
+{
+    int p = D.perm[s];
+    if ((unsigned)p >= (unsigned)D.n) p = 0;
+    const int64_t cap = D.cap;
+    const T* P = D.P;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        r.x[a] = P[(PX + a) * cap + p];
+        r.v[a] = P[(PV + a) * cap + p];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.C[i] = P[(PC + i) * cap + p];
+#pragma unroll
+    for (int i = 0; i < 6; i++) r.S[i] = P[(PS + i) * cap + p];
+    r.m = P[PM * cap + p];
+    r.d = dmg && D.mats[D.pmat[p]].has_damage ? P[PD * cap + p] : T(0);
 }
 
 // particlesToGridWithForceHelper<APIC, deg 2, !MLS> (MpmSimulationBase.cpp:791-854)
@@ -645,44 +671,42 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
         const int bs = cs[0], be = ce[NB - 1];
         if (be <= bs) continue; // halo-only block
         for (int e = t; e < nq * TL::TN; e += WG) (&tile[0][0])[e] = AccT(0);
+        bool have = false;
+        RawP2G<T> raw;
         for (int lo = bs; lo < be; lo += CH) {
             const int hi = min(lo + CH, be);
             __syncthreads(); // previous chunk's phase B done with the payload
-            // ---- phase A: payload of particle lo + t
+            // ---- phase A: payload of particle lo + t from the prefetched fields
             if (t < CH && lo + t < hi) {
-                int p = D.perm[lo + t];
-                if ((unsigned)p >= (unsigned)D.n) p = 0;
-                const T* P = D.P;
-                const T mass = P[PM * cap + p];
-                T X[3], w[3][3], dw[3][3], xb[3];
+                if (!have) loadP2G(D, lo + t, dmg, raw);
+                const T mass = raw.m;
+                T w[3][3], dw[3][3], xb[3];
                 int base[3];
                 const T hi_t = T(D.domain - 4);
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    X[a] = P[(PX + a) * cap + p];
-                    T xs = mul_rn(X[a], D.odx);
+                    T xs = mul_rn(raw.x[a], D.odx);
                     T tt = sub_rn(xs, T(0.5));
                     if (!(tt >= T(0) && tt < hi_t)) tt = (tt >= hi_t) ? hi_t - T(1) : T(0); // latched by k_bin
                     base[a] = (int)tt;
                     bspline(xs, base[a], w[a], dw[a]);
-                    xb[a] = T(base[a]) * D.dx - X[a];
+                    xb[a] = T(base[a]) * D.dx - raw.x[a];
                 }
                 T mC[9], a3[3];
 #pragma unroll
-                for (int i = 0; i < 9; i++) mC[i] = mass * P[(PC + i) * cap + p];
+                for (int i = 0; i < 9; i++) mC[i] = mass * raw.C[i];
 #pragma unroll
                 for (int r = 0; r < 3; r++)
-                    a3[r] = mass * P[(PV + r) * cap + p] + mC[3 * r] * xb[0] + mC[3 * r + 1] * xb[1] + mC[3 * r + 2] * xb[2];
+                    a3[r] = mass * raw.v[r] + mC[3 * r] * xb[0] + mC[3 * r + 1] * xb[1] + mC[3 * r + 2] * xb[2];
                 T Sv[6];
 #pragma unroll
-                for (int i = 0; i < 6; i++) Sv[i] = P[(PS + i) * cap + p] * D.odx;
-                const T dp = dmg && D.mats[D.pmat[p]].has_damage ? P[PD * cap + p] : T(0);
+                for (int i = 0; i < 6; i++) Sv[i] = raw.S[i] * D.odx;
                 const T dx = D.dx;
                 q[0][t] = V4<T> { mass, a3[0], a3[1], a3[2] };
                 q[1][t] = V4<T> { mC[0] * dx, mC[1] * dx, mC[2] * dx, mC[3] * dx };
                 q[2][t] = V4<T> { mC[4] * dx, mC[5] * dx, mC[6] * dx, mC[7] * dx };
                 q[3][t] = V4<T> { mC[8] * dx, Sv[0], Sv[1], Sv[2] };
-                q[4][t] = V4<T> { Sv[3], Sv[4], Sv[5], dp };
+                q[4][t] = V4<T> { Sv[3], Sv[4], Sv[5], raw.d };
                 q[5][t] = V4<T> { w[2][0], w[2][1], w[2][2], T(0) };
                 q[6][t] = V4<T> { dw[2][0], dw[2][1], dw[2][2], T(0) };
 #pragma unroll
@@ -702,6 +726,13 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
                 if (t == 0) ncl = __popcll(m);
             }
             __syncthreads();
+            // float form: prefetch the next chunk of this block while phase B runs
+            // (the double form keeps its registers for phase B)
+            have = false;
+            if (sizeof(T) == 4 && t < CH && hi + t < be) {
+                loadP2G(D, hi + t, dmg, raw);
+                have = true;
+            }
             // ---- phase B: lanes = (sub-lane, cell, i, j); R sub-lanes share a column
             const int per = ncl * 9;
             const int R = per > 0 ? max(1, WG / per) : 1;
@@ -951,7 +982,7 @@ __device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>&
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) sym.a[i][j] = T(0.5) * (cs.a[i][j] + cs.a[j][i]) / J;
+        for (int j = 0; j < 3; j++) sym.a[i][j] = ddiv(T(0.5) * (cs.a[i][j] + cs.a[j][i]), J);
     const Mat3<T> sp = positivePart(sym);
     Mat3<T> R = polarRotation(F);
     Mat3<T> A;
@@ -971,10 +1002,10 @@ __device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>&
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) contraction += As.a[i][j] * sA.a[i][j];
-    const T phi = (T(1) / (mt.sigma_crit * mt.sigma_crit)) * contraction;
+    const T phi = mt.inv_sigma_crit2 * contraction;
     const T dTilde = mt.zeta * macaulay(phi - T(1));
     const T diff = ((T(1) - d) * dTilde) - geometricResist;
-    const T newD = d + ((dt / mt.eta) * macaulay(diff));
+    const T newD = d + (ddiv(dt, mt.eta) * macaulay(diff));
     d = newD < T(1) ? newD : T(1);
     const T k = mt.k;
     g = ((T(1) - d) * (T(1) - d) * (T(1) - k)) + k;
@@ -998,18 +1029,16 @@ struct G2PAcc {
 
 // APIC gather of one particle from the staged block tile.
 template <class T, class AccT>
-__device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[Tile<T>::TN], int p, bool dmg, G2PAcc<T>& A)
+__device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[Tile<T>::TN], const T* Xp, bool dmg,
+                                          G2PAcc<T>& A)
 {
     using TL = Tile<T>;
-    const int64_t cap = D.cap;
-    const T* P = D.P;
     const T odx = D.odx, dx = D.dx, odx2 = odx * odx;
-    T Xp[3], w[3][3], dw[3][3];
+    T w[3][3], dw[3][3];
     int base[3];
     const T hi_t = T(D.domain - 4);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        Xp[a] = P[(PX + a) * cap + p];
         T xs = mul_rn(Xp[a], odx);
         T tt = sub_rn(xs, T(0.5));
         if (!(tt >= T(0) && tt < hi_t)) tt = (tt >= hi_t) ? hi_t - T(1) : T(0);
@@ -1060,17 +1089,41 @@ __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[T
 }
 
 // Particle update after the gather; writes sorted position s.
+// Particle fields G2P reads after the gather, loaded before it.
+template <class T>
+struct RawG2P {
+    T x[3], F[9], m, V0, d, g;
+    int mid, id;
+};
+
 template <class T, class AccT>
-__device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, int p, T dt, const G2PAcc<T>& A, float* bnd)
+__device__ __forceinline__ void loadG2P(const Dev<T, AccT>& D, int s, int p, RawG2P<T>& r)
 {
     const int64_t cap = D.cap;
     const T* P = D.P;
+#pragma unroll
+    for (int a = 0; a < 3; a++) r.x[a] = P[(PX + a) * cap + p];
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.F[i] = P[(PF + i) * cap + p];
+    r.m = P[PM * cap + p];
+    r.V0 = P[PVOL * cap + p];
+    r.d = P[PD * cap + p];
+    r.g = P[PG * cap + p];
+    r.mid = D.pmat[p];
+    r.id = D.spid[s];
+}
+
+template <class T, class AccT>
+__device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const RawG2P<T>& R, T dt, const G2PAcc<T>& A,
+                                          float* bnd)
+{
+    const int64_t cap = D.cap;
     T* Q = D.Q;
-    const int mid = D.pmat[p];
+    const int mid = R.mid;
     const MatDev<T>& mt = D.mats[mid];
     const bool dmg = D.any_damage && mt.has_damage;
     const T dx = D.dx;
-    const T Dinv = T(4) / (dx * dx);
+    const T Dinv = D.Dinv;
     const T r = D.rpic;
     const T ca = (r + T(1)) * T(0.5), cb = (r - T(1)) * T(0.5);
     T Cn[9];
@@ -1088,12 +1141,14 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, int p, T
     for (int a = 0; a < 3; a++) {
         T inc = dt * A.picV[a];
         inc2 += inc * inc;
-        Q[(PX + a) * cap + s] = P[(PX + a) * cap + p] + inc;
+        Q[(PX + a) * cap + s] = R.x[a] + inc;
     }
     if (inc2 > dx * dx) latch(D.counters, ST_CFL);
-    T d = P[PD * cap + p], g = P[PG * cap + p];
-    const T V0 = P[PVOL * cap + p];
-    Mat3<T> Fn = loadM(P, cap, PF, p);
+    T d = R.d, g = R.g;
+    const T V0 = R.V0;
+    Mat3<T> Fn;
+#pragma unroll
+    for (int i = 0; i < 9; i++) Fn.a[i / 3][i % 3] = R.F[i];
     if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE)
         Fn.a[0][0] = (T(1) + dt * (A.gv[0][0] + A.gv[1][1] + A.gv[2][2])) * Fn.a[0][0];
     else {
@@ -1116,13 +1171,13 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, int p, T
     vtauSym(mt, Fn, V0, g, S);
 #pragma unroll
     for (int i = 0; i < 6; i++) Q[(PS + i) * cap + s] = S[i];
-    const T m = P[PM * cap + p];
+    const T m = R.m;
     Q[PM * cap + s] = m;
     Q[PVOL * cap + s] = V0;
     Q[PD * cap + s] = d;
     Q[PG * cap + s] = g;
     Q[PLAP * cap + s] = dmg ? A.lap : T(0);
-    D.qid[s] = D.spid[s];
+    D.qid[s] = R.id;
     D.qmat[s] = mid;
     if constexpr (std::is_same<AccT, long long>::value)
         particleBounds(bnd, m, A.picV, Cn, S, dmg ? d : T(0), dx, D.odx, D.any_damage != 0);
@@ -1142,12 +1197,13 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
     const bool live = s < hi;
     const int64_t cn = D.cap_nodes;
     const int nfield = D.any_damage ? 4 : 3;
-    int p = 0;
+    RawG2P<T> R;
     bool dmg = false;
     if (live) {
-        p = D.perm[s];
+        int p = D.perm[s];
         if ((unsigned)p >= (unsigned)D.n) p = 0;
-        dmg = D.any_damage && D.mats[D.pmat[p]].has_damage;
+        loadG2P(D, s, p, R);
+        dmg = D.any_damage && D.mats[R.mid].has_damage;
     }
     G2PAcc<T> A;
 #pragma unroll
@@ -1182,11 +1238,11 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
         }
         __syncthreads();
         const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
-        if (live && s >= seg_lo && s < seg_hi) g2pGather(D, gt, p, dmg, A);
+        if (live && s >= seg_lo && s < seg_hi) g2pGather(D, gt, R.x, dmg, A);
         seg_lo = seg_hi;
     }
     float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
-    if (live) g2pFinish(D, s, p, dt, A, bnd);
+    if (live) g2pFinish(D, s, R, dt, A, bnd);
     if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, D.max_slots, bnd);
 }
 

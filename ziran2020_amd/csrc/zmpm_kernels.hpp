@@ -79,6 +79,7 @@ struct MatDev {
     T a0[2][3];
     T alphas[2];
     T k, l0, sigma_crit, eta, zeta;
+    T inv_sigma_crit2; // 1 / sigma_c^2 (AnisoFractureSimulation.h:262)
 };
 
 struct ColDev {
@@ -136,6 +137,7 @@ struct Dev {
     int any_damage;
     // scene
     T dx, odx;
+    T Dinv; // 4 / dx^2 (MpmSimulationBase.cpp:119-122)
     T gravity[3];
     T rpic;
     T cfl;

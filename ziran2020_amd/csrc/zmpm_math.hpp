@@ -23,8 +23,16 @@ template <> struct Lim<double> {
     static constexpr double svd_tol = 1024 * eps; // ImplicitQRSVD.h:526-533
 };
 
-__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+// Square root, reciprocal square root and division.  The float form uses the
+// hardware v_sqrt/v_rsq/v_rcp (about 1 ulp), well inside its error budget;
+// the double form keeps correctly rounded operations (parity with the FP64
+// oracle).
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float drsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double drsqrt(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float ddiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double ddiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float dabs(float x) { return fabsf(x); }
 __device__ __forceinline__ double dabs(double x) { return fabs(x); }
 __device__ __forceinline__ float dcopysign(float a, float b) { return copysignf(a, b); }
@@ -118,9 +126,8 @@ __device__ __forceinline__ Giv<T> givens(T a, T b) // Givens.h:72-85
 {
     Giv<T> g { T(1), T(0) };
     T d = a * a + b * b;
-    T sq = dsqrt(d);
-    if (sq != T(0)) {
-        T t = T(1) / sq;
+    if (d != T(0)) {
+        T t = drsqrt(d);
         g.c = a * t;
         g.s = -b * t;
     }
@@ -132,9 +139,8 @@ __device__ __forceinline__ Giv<T> givensUnconventional(T a, T b) // Givens.h:92-
 {
     Giv<T> g { T(0), T(1) };
     T d = a * a + b * b;
-    T sq = dsqrt(d);
-    if (sq != T(0)) {
-        T t = T(1) / sq;
+    if (d != T(0)) {
+        T t = drsqrt(d);
         g.s = a * t;
         g.c = b * t;
     }
@@ -468,8 +474,8 @@ __device__ __forceinline__ Mat3<float> polarRotation<float>(const Mat3<float>& F
         for (; it < 12; it++) {
             Mat3<float> C = cofactor(X);
             const float d = X.a[0][0] * C.a[0][0] + X.a[0][1] * C.a[0][1] + X.a[0][2] * C.a[0][2];
-            const float g = exp2f(-log2f(fabsf(d)) * (1.f / 3.f));
-            const float a = 0.5f * g, b = 0.5f / (g * d);
+            const float g = __builtin_amdgcn_exp2f(-__builtin_amdgcn_logf(fabsf(d)) * (1.f / 3.f));
+            const float a = 0.5f * g, b = 0.5f * __builtin_amdgcn_rcpf(g * d);
             float delta = 0.f;
 #pragma unroll
             for (int i = 0; i < 3; i++)
@@ -552,16 +558,16 @@ __device__ __forceinline__ Mat3<T> qrStablePiola(const Mat3<T>& F, T mu, T lambd
     T gl = (J >= T(1)) ? g * lambda : lambda;
     T vol = gl * (J - T(1)) * J;
     T gm = g * mu, gmJ = g * mu * J;
-    T D00 = vol / r00 + gm * r00 - gmJ / r00 + kx * (r00 - T(1));
+    T D00 = ddiv(vol, r00) + gm * r00 - ddiv(gmJ, r00) + kx * (r00 - T(1));
     T D01 = gm * r01;
     T D02 = gm * r02;
-    T D11 = vol / r11 + gm * r11 - gmJ / r11;
+    T D11 = ddiv(vol, r11) + gm * r11 - ddiv(gmJ, r11);
     T D12 = gm * r12;
-    T D22 = vol / r22 + gm * r22 - gmJ / r22;
+    T D22 = ddiv(vol, r22) + gm * r22 - ddiv(gmJ, r22);
     if (ky != T(0)) {
         T ss = dsqrt(r01 * r01 + r11 * r11);
-        D01 += ky * (ss - T(1)) * r01 / ss;
-        D11 += ky * (ss - T(1)) * r11 / ss;
+        D01 += ddiv(ky * (ss - T(1)) * r01, ss);
+        D11 += ddiv(ky * (ss - T(1)) * r11, ss);
     }
     // A = D R^T (upper triangle, symmetrised)
     T A00 = D00 * r00 + D01 * r01 + D02 * r02;
@@ -571,7 +577,7 @@ __device__ __forceinline__ Mat3<T> qrStablePiola(const Mat3<T>& F, T mu, T lambd
     T A12 = D12 * r22;
     T A22 = D22 * r22;
     // R^{-1} (upper triangular)
-    T i00 = T(1) / r00, i11 = T(1) / r11, i22 = T(1) / r22;
+    T i00 = ddiv(T(1), r00), i11 = ddiv(T(1), r11), i22 = ddiv(T(1), r22);
     T i01 = -r01 * i00 * i11;
     T i12 = -r12 * i11 * i22;
     T i02 = (r01 * r12 - r02 * r11) * i00 * i11 * i22;
@@ -699,7 +705,7 @@ __device__ __forceinline__ void eigvecOf(const Mat3<float>& S, float l, float* v
             bi = k;
         }
     }
-    const float inv = best > 0.f ? rsqrtf(best) : 0.f;
+    const float inv = best > 0.f ? __builtin_amdgcn_rsqf(best) : 0.f;
 #pragma unroll
     for (int a = 0; a < 3; a++) v[a] = (bi == 0 ? c[0][a] : (bi == 1 ? c[1][a] : c[2][a])) * inv;
 }
@@ -723,14 +729,14 @@ __device__ __forceinline__ Mat3<float> positivePart<float>(const Mat3<float>& S)
         l1 = l2 = l3 = q; // isotropic
     }
     else {
-        const float p = sqrtf(p2 * (1.f / 6.f));
-        const float ip = 1.f / p;
+        const float p = __builtin_amdgcn_sqrtf(p2 * (1.f / 6.f));
+        const float ip = __builtin_amdgcn_rcpf(p);
         const float c00 = b00 * ip, c11 = b11 * ip, c22 = b22 * ip, c01 = a01 * ip, c02 = a02 * ip, c12 = a12 * ip;
         const float detB = c00 * (c11 * c22 - c12 * c12) - c01 * (c01 * c22 - c12 * c02) + c02 * (c01 * c12 - c11 * c02);
         const float r = fminf(1.f, fmaxf(-1.f, 0.5f * detB));
         const float phi = acosf(r) * (1.f / 3.f);
-        l1 = q + 2.f * p * cosf(phi);
-        l3 = q + 2.f * p * cosf(phi + 2.0943951023931953f);
+        l1 = q + 2.f * p * __cosf(phi);
+        l3 = q + 2.f * p * __cosf(phi + 2.0943951023931953f);
         l2 = 3.f * q - l1 - l3;
     }
     if (l3 >= 0.f) return S;
