@@ -96,6 +96,20 @@ def timed(scene, lib, precision, steps, warmup, timing_steps=50):
     return el, stages, n_nodes, out
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of `mib` MiB, ten times: the reference
+    kernel whose FETCH_SIZE / WRITE_SIZE counters (known bytes per dispatch)
+    calibrate the per-dispatch HBM traffic figures of the substep kernels."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.rand(n, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "dispatches": 10}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,7 +118,12 @@ def main():
     ap.add_argument("--workload", default="damage_slab")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain MiB-sized device copy (traffic-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs the substep on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
