@@ -205,6 +205,28 @@ def test_qr_neo_hookean_fibre_stiffness_gradient(O):
         assert np.abs(P - G).max() < 1e-5 * max(1, np.abs(P).max())
 
 
+@pytest.mark.parametrize("g", [1.0, 0.3])
+def test_qr_neo_hookean_kirchhoff_closed_form(O, g):
+    # The QR route (P = Q A R^-T) gives tau = P F^T = Q A Q^T, which expands to
+    # g mu (F F^T - J I) + g_l lambda (J-1) J I + kx (|f0|-1)/|f0| f0 f0^T
+    # + ky (|f1|-1)/|f1| f1 f1^T (f_k = column k of F, g_l = g if J >= 1 else 1),
+    # including inverted F and both sides of J = 1 (the degraded volume term).
+    rng = np.random.default_rng(11)
+    m = mat(A.MODEL_QR_STABLE_NEO_HOOKEAN, kx=500.0, ky=300.0)
+    Fs = [rand_F(rng, 0.3) for _ in range(40)] + [np.diag([1.2, 1.1, 1.05]), np.diag([0.9, 0.8, 0.95]),
+                                                  np.diag([1.1, 0.9, -0.7]), rand_F(rng, 0.3) @ np.diag([1, 1, -1.0])]
+    for F in Fs:
+        J = np.linalg.det(F)
+        gl = g if J >= 1 else 1.0
+        tau = g * m.mu * (F @ F.T - J * np.eye(3)) + gl * m.lam * (J - 1) * J * np.eye(3)
+        for k, kk in ((0, m.kx), (1, m.ky)):
+            f = F[:, k]
+            n = np.linalg.norm(f)
+            tau += kk * (n - 1) / n * np.outer(f, f)
+        got = piola(O, m, F, g) @ F.T
+        assert np.abs(got - tau).max() < 1e-10 * max(1.0, np.abs(tau).max())
+
+
 def test_corotated_diagonal_closed_form(O):
     m = mat(A.MODEL_FIXED_COROTATED, 2e4, 0.25)
     for sig in ([1.2, 0.9, 1.05], [0.7, 1.3, 1.0], [2.0, 1.0, 0.5]):

@@ -74,6 +74,7 @@ struct zmpm_context {
     };
     std::vector<Graph> graphs;
     bool use_graphs = true;
+    int p2g_split = 64;
 };
 
 namespace {
@@ -209,6 +210,7 @@ void refreshTables(zmpm_context* h)
     for (int a = 0; a < 3; a++) D.gravity[a] = (T)h->desc.gravity[a];
     D.rpic = (T)h->desc.apic_rpic_ratio;
     D.cfl = (T)h->desc.cfl;
+    D.p2g_split = h->p2g_split;
 }
 
 template <class T>
@@ -554,6 +556,7 @@ zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
     h->desc = *desc;
     h->fp32 = desc->precision == ZMPM_PREC_FP32;
     if (const char* e = std::getenv("ZMPM_NO_GRAPH")) h->use_graphs = std::atoi(e) == 0;
+    if (const char* e = std::getenv("ZMPM_P2G_SPLIT")) h->p2g_split = std::max(1, std::atoi(e));
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return ZMPM_E_HIP;

@@ -37,7 +37,10 @@ namespace zmpm {
 constexpr int WG = 256;
 constexpr int SCAN_TILE = 2048; // 256 threads x 8 elements
 constexpr float FIXED_TARGET_EXP = 40.0f;
-constexpr float FIXED_LIMIT = 2251799813685248.0f; // 2^51: per-contribution bound
+// Per-lane bound on the summed |fixed-point values| one P2G lane adds to its tile:
+// by construction a lane adds at most CHUNK particles x 27 node-quantities of
+// |q| < 2^40 (< 2^52); 2^56 leaves int64 tile sums far from overflow.
+constexpr float FIXED_LANE_LIMIT = 72057594037927936.0f; // 2^56
 
 __device__ __forceinline__ void latch(int* counters, int bits) { atomicOr(&counters[1], bits); }
 
@@ -420,26 +423,6 @@ __global__ __launch_bounds__(WG) void k_cellsort(Dev<T, AccT> D)
 }
 
 // ------------------------------------------------------------- P2G helpers
-template <class T, class AccT>
-struct Scatter {
-    // FP64 plain form: hardware f64 atomics
-    __device__ __forceinline__ static void add(AccT* p, T v, float, int&) { unsafeAtomicAdd(p, v); }
-};
-
-template <>
-struct Scatter<float, long long> {
-    // FP32 form: 64-bit fixed point, exact and order-independent
-    __device__ __forceinline__ static void add(long long* p, float v, float scale, int& ovf)
-    {
-        float q = rintf(v * scale);
-        if (!(fabsf(q) <= FIXED_LIMIT)) {
-            ovf = 1;
-            q = 0.f;
-        }
-        atomicAdd((unsigned long long*)p, (unsigned long long)(long long)q);
-    }
-};
-
 template <class T>
 __device__ __forceinline__ Mat3<T> loadM(const T* P, int64_t cap, int f, int p)
 {
@@ -469,9 +452,15 @@ __device__ __forceinline__ void vtauSym(const MatDev<T>& mt, const Mat3<T>& F, T
         if (mt.model == ZMPM_MODEL_FIXED_COROTATED)
             tau = corotatedTau(F, mt.mu, mt.lambda);
         else if (mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
-            T J;
-            Mat3<T> Pk = qrStablePiola(F, mt.mu, mt.lambda, g, mt.kx, mt.ky, &J);
-            tau = mulT(Pk, F);
+            T J, t6[6];
+            qrStableTau(F, mt.mu, mt.lambda, g, mt.kx, mt.ky, t6, &J);
+            S[0] = V0 * t6[0];
+            S[1] = V0 * t6[1];
+            S[2] = V0 * t6[2];
+            S[3] = V0 * t6[3];
+            S[4] = V0 * t6[4];
+            S[5] = V0 * t6[5];
+            return;
         }
         else
             tau = neoHookeanTau(F, mt.mu, mt.lambda);
@@ -583,20 +572,19 @@ struct alignas(4 * sizeof(T)) V4 {
     T x, y, z, w;
 };
 
+// Adds a lane's partial sum into the LDS tile.  Float form: rounded to int64
+// fixed point; |q| is summed into chk, so one test of chk <= FIXED_LANE_LIMIT per lane
+// bounds every value added (and catches NaN / Inf) without a branch per value.
 template <class T, class AccT>
-__device__ __forceinline__ void ldsAdd(AccT* a, T v, float scale, int& ovf)
+__device__ __forceinline__ void ldsAdd(AccT* a, T v, float scale, float& chk)
 {
     if constexpr (std::is_same<AccT, long long>::value) {
-        float q = rintf((float)v * scale);
-        if (!(fabsf(q) <= FIXED_LIMIT)) {
-            ovf = 1;
-            q = 0.f;
-        }
-        if (q != 0.f) atomicAdd((unsigned long long*)a, (unsigned long long)(long long)q);
+        const float q = rintf((float)v * scale);
+        chk += fabsf(q);
+        atomicAdd((unsigned long long*)a, (unsigned long long)(long long)q);
     }
-    else {
-        if (v != T(0)) unsafeAtomicAdd(a, v);
-    }
+    else
+        unsafeAtomicAdd(a, v);
 }
 
 // Particle fields P2G reads, loaded ahead of use.
@@ -735,7 +723,7 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
             }
             // ---- phase B: lanes = (sub-lane, cell, i, j); R sub-lanes share a column
             const int per = ncl * 9;
-            const int R = per > 0 ? max(1, WG / per) : 1;
+            const int R = per > 0 ? max(1, min(D.p2g_split, WG / per)) : 1;
             for (int slot = t; slot < per * R; slot += WG) {
                 const int sub = slot / per, rem = slot - per * (slot / per);
                 const int cell = cl[rem / 9], col = rem - 9 * (rem / 9);
@@ -778,11 +766,13 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
                 }
                 const int cx = cell >> (B::ybits + B::zbits), cy = (cell >> B::zbits) & (TL::BY - 1), cz = cell & (TL::BZ - 1);
                 const int nbase = ((cx + ci) * TL::TY + (cy + cj)) * TL::TZ + cz;
+                float chk = 0.f;
 #pragma unroll
                 for (int k = 0; k < 3; k++)
 #pragma unroll
                     for (int f = 0; f < NGF; f++)
-                        if (f < nq) ldsAdd<T, AccT>(&tile[f][nbase + k], acc[k][f], scl[f], ovf);
+                        if (f < nq) ldsAdd<T, AccT>(&tile[f][nbase + k], acc[k][f], scl[f], chk);
+                if (FIXED && !(chk <= FIXED_LANE_LIMIT)) ovf = 1;
             }
         }
         __syncthreads();
@@ -975,14 +965,18 @@ template <class T>
 __device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>& F, T lap, T dt, T& d, T& g)
 {
     const T geometricResist = d - (mt.l0 * mt.l0 * lap);
-    T J;
-    Mat3<T> Pk = qrStablePiola(F, mt.mu, mt.lambda, T(1), T(0), T(0), &J);
-    Mat3<T> cs = mulT(Pk, F);
+    // undegraded Cauchy stress tau(g = 1, no fibres) / J (evaluateEffectiveCauchyStress,
+    // QRStableNeoHookean.h:83-93); tau is exactly symmetric in its closed form
+    T J, t6[6];
+    qrStableTau(F, mt.mu, mt.lambda, T(1), T(0), T(0), t6, &J);
+    const T iJ = ddiv(T(1), J);
     Mat3<T> sym;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) sym.a[i][j] = ddiv(T(0.5) * (cs.a[i][j] + cs.a[j][i]), J);
+    sym.a[0][0] = t6[0] * iJ;
+    sym.a[0][1] = sym.a[1][0] = t6[1] * iJ;
+    sym.a[0][2] = sym.a[2][0] = t6[2] * iJ;
+    sym.a[1][1] = t6[3] * iJ;
+    sym.a[1][2] = sym.a[2][1] = t6[4] * iJ;
+    sym.a[2][2] = t6[5] * iJ;
     const Mat3<T> sp = positivePart(sym);
     Mat3<T> R = polarRotation(F);
     Mat3<T> A;

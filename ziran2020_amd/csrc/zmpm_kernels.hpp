@@ -135,6 +135,8 @@ struct Dev {
     const ColDev* cols;
     int num_cols;
     int any_damage;
+    // launch parameters
+    int p2g_split; // most lanes sharing one stencil column of a cell in P2G (>= 1)
     // scene
     T dx, odx;
     T Dinv; // 4 / dx^2 (MpmSimulationBase.cpp:119-122)

@@ -511,93 +511,54 @@ __device__ __forceinline__ Mat3<T> corotatedTau(const Mat3<T>& F, T mu, T lambda
     return mulT(P, F);
 }
 
-// QRStableNeoHookean first Piola stress (QRStableNeoHookean.h:108-213 with
-// QrBasedHelper::evaluateP, QrBasedHelper.h:28-40).  J returned through Jout.
+// QRStableNeoHookean Kirchhoff stress tau = P F^T, symmetric (xx xy xz yy yz zz).
+// The reference evaluates P = Q A R^{-T} from the Givens QR F = Q R, with A the
+// symmetric matrix whose upper triangle is dPsiHat/dR R^T
+// (QRStableNeoHookean.h:108-213, QrBasedHelper::evaluateP, QrBasedHelper.h:28-40).
+// Then tau = Q A R^{-T} R^T Q^T = Q A Q^T, and with the entries of dPsiHat/dR
+// (upper triangular: g mu R_ij off the diagonal; (vol - g mu J) / R_ii + g mu R_ii
+// on it, vol = g_l lambda (J - 1) J, g_l = g for J >= 1 else 1; fibre terms
+// kx (R_00 - 1) on (0,0) and ky (s - 1) R_i1 / s on (0,1), (1,1), s = |(R_01, R_11)|)
+//   A = (vol - g mu J) I + g mu R R^T + kx (R_00 - 1) R_00 e0 e0^T + c r r^T,
+//   r = (R_01, R_11, 0), c = ky (s - 1) / s.
+// Since Q R R^T Q^T = F F^T, Q e0 R_00 = f0 and Q r = f1 (the first two columns
+// of F, |f0| = R_00 >= 0 and |f1| = s under the sign convention, det Q = 1 so
+// J = R_00 R_11 R_22 = det F):
+//   tau = g mu F F^T + (vol - g mu J) I + kx (|f0| - 1) / |f0| f0 f0^T
+//         + ky (|f1| - 1) / |f1| f1 f1^T.
+// Same stress as the reference up to rounding, without the QR sweep.
 template <class T>
-__device__ __forceinline__ Mat3<T> qrStablePiola(const Mat3<T>& F, T mu, T lambda, T g, T kx, T ky, T* Jout)
+__device__ __forceinline__ void qrStableTau(const Mat3<T>& F, T mu, T lambda, T g, T kx, T ky, T* tau, T* Jout)
 {
-    // GivensQR (GivensQR.h:20-48): R = G F, Qt accumulates G
-    Mat3<T> R = F, Qt;
-    setIdentity(Qt);
-    {
-        Giv<T> r = givens(R.a[1][0], R.a[2][0]);
-        rowRot<T, 1, 2>(R, r);
-        rowRot<T, 1, 2>(Qt, r);
-        r = givens(R.a[0][0], R.a[1][0]);
-        rowRot<T, 0, 1>(R, r);
-        rowRot<T, 0, 1>(Qt, r);
-        r = givens(R.a[1][1], R.a[2][1]);
-        rowRot<T, 1, 2>(R, r);
-        rowRot<T, 1, 2>(Qt, r);
-    }
-    // Q = Qt^T; sign convention on columns of Q == rows of Qt (QRStableNeoHookean.h:125-139)
-    if (R.a[0][0] < T(0)) {
-        R.a[0][0] = -R.a[0][0];
-        R.a[0][1] = -R.a[0][1];
-        R.a[0][2] = -R.a[0][2];
-        R.a[2][2] = -R.a[2][2];
+    const T J = det3(F);
+    const T gl = (J >= T(1)) ? g * lambda : lambda;
+    const T gm = g * mu;
+    const T iso = gl * (J - T(1)) * J - gm * J;
+    const T r0[3] = { F.a[0][0], F.a[0][1], F.a[0][2] };
+    const T r1[3] = { F.a[1][0], F.a[1][1], F.a[1][2] };
+    const T r2[3] = { F.a[2][0], F.a[2][1], F.a[2][2] };
+    tau[0] = gm * (r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2]) + iso;
+    tau[1] = gm * (r0[0] * r1[0] + r0[1] * r1[1] + r0[2] * r1[2]);
+    tau[2] = gm * (r0[0] * r2[0] + r0[1] * r2[1] + r0[2] * r2[2]);
+    tau[3] = gm * (r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]) + iso;
+    tau[4] = gm * (r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2]);
+    tau[5] = gm * (r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]) + iso;
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            Qt.a[0][c] = -Qt.a[0][c];
-            Qt.a[2][c] = -Qt.a[2][c];
+    for (int c = 0; c < 2; c++) {
+        const T k = c == 0 ? kx : ky;
+        if (k != T(0)) {
+            const T f[3] = { F.a[0][c], F.a[1][c], F.a[2][c] };
+            const T n = dsqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+            const T cc = ddiv(k * (n - T(1)), n);
+            tau[0] += cc * f[0] * f[0];
+            tau[1] += cc * f[0] * f[1];
+            tau[2] += cc * f[0] * f[2];
+            tau[3] += cc * f[1] * f[1];
+            tau[4] += cc * f[1] * f[2];
+            tau[5] += cc * f[2] * f[2];
         }
     }
-    if (R.a[1][1] < T(0)) {
-        R.a[1][1] = -R.a[1][1];
-        R.a[1][2] = -R.a[1][2];
-        R.a[2][2] = -R.a[2][2];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            Qt.a[1][c] = -Qt.a[1][c];
-            Qt.a[2][c] = -Qt.a[2][c];
-        }
-    }
-    T r00 = R.a[0][0], r01 = R.a[0][1], r02 = R.a[0][2], r11 = R.a[1][1], r12 = R.a[1][2], r22 = R.a[2][2];
-    T J = r00 * r11 * r22;
-    // dPsiHat/dR (upper triangular)
-    T gl = (J >= T(1)) ? g * lambda : lambda;
-    T vol = gl * (J - T(1)) * J;
-    T gm = g * mu, gmJ = g * mu * J;
-    T D00 = ddiv(vol, r00) + gm * r00 - ddiv(gmJ, r00) + kx * (r00 - T(1));
-    T D01 = gm * r01;
-    T D02 = gm * r02;
-    T D11 = ddiv(vol, r11) + gm * r11 - ddiv(gmJ, r11);
-    T D12 = gm * r12;
-    T D22 = ddiv(vol, r22) + gm * r22 - ddiv(gmJ, r22);
-    if (ky != T(0)) {
-        T ss = dsqrt(r01 * r01 + r11 * r11);
-        D01 += ddiv(ky * (ss - T(1)) * r01, ss);
-        D11 += ddiv(ky * (ss - T(1)) * r11, ss);
-    }
-    // A = D R^T (upper triangle, symmetrised)
-    T A00 = D00 * r00 + D01 * r01 + D02 * r02;
-    T A01 = D01 * r11 + D02 * r12;
-    T A02 = D02 * r22;
-    T A11 = D11 * r11 + D12 * r12;
-    T A12 = D12 * r22;
-    T A22 = D22 * r22;
-    // R^{-1} (upper triangular)
-    T i00 = ddiv(T(1), r00), i11 = ddiv(T(1), r11), i22 = ddiv(T(1), r22);
-    T i01 = -r01 * i00 * i11;
-    T i12 = -r12 * i11 * i22;
-    T i02 = (r01 * r12 - r02 * r11) * i00 * i11 * i22;
-    // M = A R^{-T}:  (R^{-T})_{kj} = Rinv_{jk}
-    Mat3<T> M;
-    T Am[3][3] = { { A00, A01, A02 }, { A01, A11, A12 }, { A02, A12, A22 } };
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        M.a[i][0] = Am[i][0] * i00 + Am[i][1] * i01 + Am[i][2] * i02;
-        M.a[i][1] = Am[i][1] * i11 + Am[i][2] * i12;
-        M.a[i][2] = Am[i][2] * i22;
-    }
-    // P = Q M = Qt^T M
-    Mat3<T> P;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) P.a[i][j] = Qt.a[0][i] * M.a[0][j] + Qt.a[1][i] * M.a[1][j] + Qt.a[2][i] * M.a[2][j];
     *Jout = J;
-    return P;
 }
 
 // Neo-Hookean Kirchhoff stress tau = mu F F^T + (lambda log J - mu) I (NeoHookean.cpp:38-43).
@@ -682,13 +643,16 @@ __device__ __forceinline__ Mat3<T> positivePart(const Mat3<T>& S)
     return sp;
 }
 
-// Unit eigenvector of symmetric S for the simple eigenvalue l: the largest cross
-// product of two rows of S - l I.
-__device__ __forceinline__ void eigvecOf(const Mat3<float>& S, float l, float* v)
+// Unit eigenvector of the symmetric matrix with diagonal d and off-diagonal
+// (o01, o02, o12) for its simple eigenvalue l: the largest cross product of two
+// rows of M - l I.  Called on the normalised deviator (entries O(1)), so the
+// squared norms stay far from the float underflow range; a vanishing cross
+// product (l degenerate to rounding) gives v = 0.
+__device__ __forceinline__ void eigvecOf(const float* dg, float o01, float o02, float o12, float l, float* v)
 {
-    const float r0[3] = { S.a[0][0] - l, S.a[0][1], S.a[0][2] };
-    const float r1[3] = { S.a[1][0], S.a[1][1] - l, S.a[1][2] };
-    const float r2[3] = { S.a[2][0], S.a[2][1], S.a[2][2] - l };
+    const float r0[3] = { dg[0] - l, o01, o02 };
+    const float r1[3] = { o01, dg[1] - l, o12 };
+    const float r2[3] = { o02, o12, dg[2] - l };
     float c[3][3];
     const float* A[3] = { r0, r0, r1 };
     const float* Bv[3] = { r1, r2, r2 };
@@ -705,16 +669,19 @@ __device__ __forceinline__ void eigvecOf(const Mat3<float>& S, float l, float* v
             bi = k;
         }
     }
-    const float inv = best > 0.f ? __builtin_amdgcn_rsqf(best) : 0.f;
+    const float inv = best > 1e-30f ? __builtin_amdgcn_rsqf(best) : 0.f;
 #pragma unroll
     for (int a = 0; a < 3; a++) v[a] = (bi == 0 ? c[0][a] : (bi == 1 ? c[1][a] : c[2][a])) * inv;
 }
 
 // Float form: eigenvalues from the trigonometric solution of the characteristic
-// cubic (O. K. Smith, "Eigenvalues of a symmetric 3x3 matrix", CACM 4, 1961) and
-// at most one eigenvector -- of the eigenvalue on the other side of zero from
-// the rest -- so no iteration.  When that eigenvalue is nearly degenerate with a
-// neighbour it is itself near zero and its term in sigma^+ is negligible.
+// cubic (O. K. Smith, "Eigenvalues of a symmetric 3x3 matrix", CACM 4, 1961) of
+// the normalised deviator B = (S - q I) / p, and at most one eigenvector (of B,
+// the same as S's) -- of the eigenvalue on the other side of zero from the
+// rest -- so no iteration.  When that eigenvalue is nearly degenerate with a
+// neighbour it is itself near zero and its term in sigma^+ is negligible.  A
+// deviator below 1e-6 of the mean stress (or 1e-15 absolute) counts as
+// isotropic: every eigenvalue then has the sign of q.
 template <>
 __device__ __forceinline__ Mat3<float> positivePart<float>(const Mat3<float>& S)
 {
@@ -724,21 +691,23 @@ __device__ __forceinline__ Mat3<float> positivePart<float>(const Mat3<float>& S)
     const float p1 = a01 * a01 + a02 * a02 + a12 * a12;
     const float p2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.f * p1;
     Mat3<float> out;
-    float l1, l2, l3;
-    if (!(p2 > 1e-30f * (q * q + 1e-30f))) {
-        l1 = l2 = l3 = q; // isotropic
+    if (!(p2 > fmaxf(1e-30f, 1e-12f * q * q))) { // isotropic
+        if (q >= 0.f) return S;
+        setIdentity(out);
+#pragma unroll
+        for (int i = 0; i < 3; i++) out.a[i][i] = 0.f;
+        return out;
     }
-    else {
-        const float p = __builtin_amdgcn_sqrtf(p2 * (1.f / 6.f));
-        const float ip = __builtin_amdgcn_rcpf(p);
-        const float c00 = b00 * ip, c11 = b11 * ip, c22 = b22 * ip, c01 = a01 * ip, c02 = a02 * ip, c12 = a12 * ip;
-        const float detB = c00 * (c11 * c22 - c12 * c12) - c01 * (c01 * c22 - c12 * c02) + c02 * (c01 * c12 - c11 * c02);
-        const float r = fminf(1.f, fmaxf(-1.f, 0.5f * detB));
-        const float phi = acosf(r) * (1.f / 3.f);
-        l1 = q + 2.f * p * __cosf(phi);
-        l3 = q + 2.f * p * __cosf(phi + 2.0943951023931953f);
-        l2 = 3.f * q - l1 - l3;
-    }
+    const float p = __builtin_amdgcn_sqrtf(p2 * (1.f / 6.f));
+    const float ip = __builtin_amdgcn_rcpf(p);
+    const float dg[3] = { b00 * ip, b11 * ip, b22 * ip };
+    const float c01 = a01 * ip, c02 = a02 * ip, c12 = a12 * ip;
+    const float detB = dg[0] * (dg[1] * dg[2] - c12 * c12) - c01 * (c01 * dg[2] - c12 * c02) + c02 * (c01 * c12 - dg[1] * c02);
+    const float r = fminf(1.f, fmaxf(-1.f, 0.5f * detB));
+    const float phi = acosf(r) * (1.f / 3.f);
+    const float e1 = 2.f * __cosf(phi), e3 = 2.f * __cosf(phi + 2.0943951023931953f);
+    const float l1 = q + p * e1, l3 = q + p * e3;
+    const float l2 = 3.f * q - l1 - l3;
     if (l3 >= 0.f) return S;
     if (l1 <= 0.f) {
         setIdentity(out);
@@ -748,14 +717,14 @@ __device__ __forceinline__ Mat3<float> positivePart<float>(const Mat3<float>& S)
     }
     float v[3];
     if (l2 <= 0.f) { // only l1 > 0
-        eigvecOf(S, l1, v);
+        eigvecOf(dg, c01, c02, c12, e1, v);
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) out.a[i][j] = l1 * (v[i] * v[j]);
     }
     else { // l1, l2 > 0 >= l3
-        eigvecOf(S, l3, v);
+        eigvecOf(dg, c01, c02, c12, e3, v);
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
