@@ -56,8 +56,6 @@ struct zmpm_context {
     std::vector<void*> allocs; // particle + grid buffers (freed on reallocation)
     void* mats_dev = nullptr;
     void* cols_dev = nullptr;
-    int* tile_sums = nullptr;
-    int64_t max_tiles = 0;
     int n = 0;
     int64_t cap = 0;
     bool timing = false;
@@ -120,7 +118,6 @@ void freeAll(zmpm_context* h)
     h->allocs.clear();
     h->Df = Dev<float, long long> {};
     h->Dd = Dev<double, double> {};
-    h->tile_sums = nullptr;
     h->cap = 0;
     h->n = 0;
 }
@@ -161,12 +158,18 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.qmat, cap);
     ALLOC(D.key, cap);
     ALLOC(D.rank, cap);
+    ALLOC(D.tmp, cap);
     ALLOC(D.perm, cap);
     ALLOC(D.sorted_cell, cap);
     ALLOC(D.spid, cap);
     ALLOC(D.touched, D.num_pages);
+    ALLOC(D.page_count, D.num_pages);
     ALLOC(D.page_table, D.num_pages);
+    const int64_t scan_tiles = (D.num_pages + PSCAN_TILE - 1) / PSCAN_TILE;
+    ALLOC(D.scan_state, scan_tiles);
     ALLOC(D.block_page, cap_blocks);
+    ALLOC(D.block_start, cap_blocks);
+    ALLOC(D.block_count, cap_blocks);
     ALLOC(D.nbr, cap_blocks * 8);
     ALLOC(D.cell_count, D.cap_nodes);
     ALLOC(D.cell_start, D.cap_nodes);
@@ -176,16 +179,17 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.newv, 3 * D.cap_nodes);
     ALLOC(D.pfn, D.cap_nodes);
     ALLOC(D.scales, NSC);
-    ALLOC(D.maxbits, (int64_t)NSC * ((cap + WG - 1) / WG));
-    h->max_tiles = (std::max<int64_t>(D.num_pages, D.cap_nodes) + SCAN_TILE - 1) / SCAN_TILE + 1;
-    ALLOC(h->tile_sums, h->max_tiles);
+    ALLOC(D.maxbits, (int64_t)NSC * BOUND_SLOTS);
 #undef ALLOC
     HIPCHK(h, hipMemsetAsync(D.touched, 0, D.num_pages * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.page_count, 0, D.num_pages * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.scan_state, 0, scan_tiles * sizeof(uint64_t), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.tmp, 0, cap * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.perm, 0, cap * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.sorted_cell, 0, cap * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.spid, 0, cap * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.counters, 0, 8 * sizeof(int), h->stream));
-    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * ((cap + WG - 1) / WG) * sizeof(unsigned int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * BOUND_SLOTS * sizeof(unsigned int), h->stream));
     float ones[NSC] = { 1.f, 1.f, 1.f, 1.f, 1.f };
     HIPCHK(h, hipMemcpyAsync(D.scales, ones, sizeof(ones), hipMemcpyHostToDevice, h->stream));
     D.masks.x = spreadMask(B::data_bits, 0);
@@ -276,36 +280,17 @@ int gridFor(int64_t items, int per, int maxg)
     return (int)std::min<int64_t>(g, maxg);
 }
 
-template <bool ASSIGN>
-void launchScan(zmpm_context* h, int* in, const int* len_dev, int64_t mul, int64_t len_host, int* out,
-                int* total_out, int cap, int* counters, int* block_page)
-{
-    const int g = (int)std::min<int64_t>(h->max_tiles, 2048);
-    k_scan_reduce<<<g, WG, 0, h->stream>>>(in, len_dev, mul, len_host, h->tile_sums);
-    k_scan_top<<<1, WG, 0, h->stream>>>(h->tile_sums, len_dev, mul, len_host, total_out, cap, counters);
-    k_scan_down<ASSIGN><<<g, WG, 0, h->stream>>>(in, len_dev, mul, len_host, h->tile_sums, out, block_page, cap);
-}
-
 template <class T, class AccT>
 void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
 {
     Dev<T, AccT>& D = devOf<T, AccT>(h);
-    using B = Blk<T>;
-    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
     hipStream_t st = h->stream;
     const int gp = gridFor(D.n, WG, 1 << 30);
     if (ev) (void)hipEventRecord(ev[0], st);
-    // device clock; fixed-point scales of this substep from the previous one's bounds
-    k_begin_step<<<1, BEGIN_WG, 0, st>>>(std::is_same<AccT, long long>::value ? D.scales : nullptr, D.maxbits, D.max_slots,
-                                   D.clock, dt);
     k_bin<T, AccT><<<gp, WG, 0, st>>>(D);
-    launchScan<true>(h, D.touched, nullptr, 1, D.num_pages, D.page_table, D.counters, D.cap_blocks, D.counters,
-                     D.block_page);
-    k_prepare<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, 16384), 64, 0, st>>>(D);
-    k_count<T, AccT><<<gp, WG, 0, st>>>(D);
-    launchScan<false>(h, D.cell_count, D.counters, NB, 0, D.cell_start, nullptr, -1, D.counters, nullptr);
-    k_scatter<T, AccT><<<gp, WG, 0, st>>>(D);
-    k_cellsort<T, AccT><<<gridFor(D.cap_nodes, CELLSORT_CELLS, 8192), WG, 0, st>>>(D);
+    k_page_scan<T, AccT><<<(int)((D.num_pages + PSCAN_TILE - 1) / PSCAN_TILE), WG, 0, st>>>(D);
+    k_scatter<T, AccT><<<gp + PREP_WGS + 1, WG, 0, st>>>(D, dt, gp);
+    k_blocksort<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, 8192), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[1], st);
     k_p2g<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, 4096), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[2], st);
@@ -331,6 +316,7 @@ zmpm_status deviceStatus(zmpm_context* h, bool sync)
     if (bits & ST_DOMAIN) return fail(h, ZMPM_E_DOMAIN, "particle base node outside [0, domain_nodes-5]");
     if (bits & ST_OVERFLOW) return fail(h, ZMPM_E_OVERFLOW, "fixed-point P2G contribution out of range");
     if (bits & ST_CAPACITY) return fail(h, ZMPM_E_NOMEM, "grid block capacity exceeded (set ZMPM_BLOCK_CAPACITY)");
+    if (bits & ST_SCAN) return fail(h, ZMPM_E_HIP, "binning page scan did not complete");
     if (bits & ST_CFL) return fail(h, ZMPM_E_CFL, "a particle moved more than dx in one substep");
     return ZMPM_OK;
 }
@@ -370,9 +356,11 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
     HIPCHK(h, hipMemcpyAsync(D.P, buf.data(), buf.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(D.pid, ids.data(), cap * sizeof(int), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(D.pmat, mat.data(), cap * sizeof(int), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemsetAsync(D.counters, 0, 8 * sizeof(int), h->stream));
-    // one bound slot per G2P workgroup (WG particles each)
+    // block count and status; the scan epoch and ticket (slots 2, 3) keep counting
+    HIPCHK(h, hipMemsetAsync(D.counters, 0, 2 * sizeof(int), h->stream));
+    // G2P workgroups (WG particles each); the bound slots start cleared
     D.max_slots = std::max(1, (n + WG - 1) / WG);
+    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * BOUND_SLOTS * sizeof(unsigned int), h->stream));
     if (n > 0) {
         // stress of the uploaded state and the first substep's fixed-point bounds
         k_stress_init<T, AccT><<<D.max_slots, WG, 0, h->stream>>>(D);

@@ -4,22 +4,22 @@
 // One substep (Ziran MpmSimulationBase::advanceOneTimeStep,
 // Lib/MPM/MpmSimulationBase.cpp:262-357) is this launch sequence on the
 // handle's stream:
-//   k_begin_step     device clock, fixed-point scales (float form)
-//   k_bin            base node, SPGrid page/element key, mark touched pages
-//   scan             dense block ids in page (Morton) order -> page table
-//   k_prepare        zero block accumulators, forward-neighbour block table
-//   k_count          per-cell occupancy (atomic slot)
-//   scan             cell start offsets
-//   k_scatter        counting-sort scatter
-//   k_cellsort       order inside each cell by original index -> reference order
+//   k_bin            base node, SPGrid page/element key, page activation,
+//                    per-page particle counts
+//   k_page_scan      dense block ids in page (Morton) order, page table, block
+//                    ranges (single-pass scan)
+//   k_scatter        particles to their block ranges; zero block accumulators,
+//                    forward-neighbour block table, device clock and
+//                    fixed-point scales (float form)
+//   k_blocksort      order inside each block by cell, then original index
+//                    -> reference order
 //   k_p2g            APIC scatter of m, mv, f (stress read from the particle)
 //                    and of the damage phase field
 //   k_grid           v = mv/m, gravity + force, collision objects, phase-field
 //                    normalisation
 //   k_g2p            APIC gather, x update, CFL flag, damage Laplacian and
-//                    update, F update, stress of the new state (SVD / QR) and
-//                    the next P2G's fixed-point bounds; writes particles in
-//                    sorted order
+//                    update, F update, stress of the new state and the next
+//                    P2G's fixed-point bounds; writes particles in sorted order
 #pragma once
 #include "zmpm_kernels.hpp"
 
@@ -35,8 +35,8 @@
 namespace zmpm {
 
 constexpr int WG = 256;
-constexpr int SCAN_TILE = 2048; // 256 threads x 8 elements
 constexpr float FIXED_TARGET_EXP = 40.0f;
+constexpr int BOUND_SLOTS = 64; // atomic-max slots per fixed-point scale group
 // Per-lane bound on the summed |fixed-point values| one P2G lane adds to its tile:
 // by construction a lane adds at most CHUNK particles x 27 node-quantities of
 // |q| < 2^40 (< 2^52); 2^56 leaves int64 tile sums far from overflow.
@@ -96,7 +96,14 @@ __device__ __forceinline__ void pageElem(uint32_t i, uint32_t j, uint32_t k, uin
         | (k & ((1u << B::zbits) - 1));
 }
 
-// ---------------------------------------------------------------- block scan
+// ---------------------------------------------------------------- wave scans
+__device__ __forceinline__ float waveMax(float v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d));
+    return v;
+}
+
 __device__ __forceinline__ int waveInclusiveScan(int v)
 {
     const int lane = threadIdx.x & 63;
@@ -108,126 +115,68 @@ __device__ __forceinline__ int waveInclusiveScan(int v)
     return v;
 }
 
-// exclusive scan over a 256-thread block; returns the block total through *total
-__device__ __forceinline__ int blockExclusiveScan256(int v, int* total)
+__device__ __forceinline__ uint64_t waveInclusiveScan64(uint64_t v)
 {
-    __shared__ int wsum[4];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int inc = waveInclusiveScan(v);
-    if (lane == 63) wsum[w] = inc;
-    __syncthreads();
-    int off = 0, tot = 0;
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        if (i < w) off += wsum[i];
-        tot += wsum[i];
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t o = __shfl_up(v, d);
+        if (lane >= d) v += o;
     }
-    __syncthreads();
-    *total = tot;
-    return off + inc - v;
-}
-
-// length of a scan: device count * mul, or the host length
-__device__ __forceinline__ int64_t scanLength(const int* len_dev, int64_t mul, int64_t len_host)
-{
-    return len_dev ? (int64_t)(*len_dev) * mul : len_host;
-}
-
-__global__ __launch_bounds__(WG) void k_scan_reduce(const int* __restrict__ in, const int* len_dev, int64_t mul,
-                                                   int64_t len_host, int* __restrict__ tile_sums)
-{
-    const int64_t len = scanLength(len_dev, mul, len_host);
-    const int64_t ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        int s = 0;
-        const int64_t base = tile * SCAN_TILE + (int64_t)threadIdx.x * 8;
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-            if (base + k < len) s += in[base + k];
-        int tot;
-        blockExclusiveScan256(s, &tot);
-        if (threadIdx.x == 0) tile_sums[tile] = tot;
-    }
-}
-
-// exclusive scan of the tile sums in one workgroup; total -> *total_out
-// (clamped to cap if cap >= 0, with the capacity status latched)
-__global__ __launch_bounds__(WG) void k_scan_top(int* __restrict__ tile_sums, const int* len_dev, int64_t mul,
-                                                int64_t len_host, int* total_out, int cap, int* counters)
-{
-    const int64_t len = scanLength(len_dev, mul, len_host);
-    const int64_t ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
-    int carry = 0;
-    for (int64_t base = 0; base < ntiles; base += WG) {
-        int64_t i = base + threadIdx.x;
-        int v = (i < ntiles) ? tile_sums[i] : 0;
-        int tot;
-        int ex = blockExclusiveScan256(v, &tot);
-        if (i < ntiles) tile_sums[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0 && total_out) {
-        int t = carry;
-        if (cap >= 0 && t > cap) {
-            t = cap;
-            atomicOr(&counters[1], ST_CAPACITY);
-        }
-        *total_out = t;
-    }
-}
-
-// Down-sweep.  ASSIGN: the input is the per-page "touched" flag and the
-// epilogue writes the page table (dense block id in page order, or -1), the
-// block -> page list, and clears the flag for the next substep.
-template <bool ASSIGN>
-__global__ __launch_bounds__(WG) void k_scan_down(int* __restrict__ in, const int* len_dev, int64_t mul,
-                                                 int64_t len_host, const int* __restrict__ tile_off,
-                                                 int* __restrict__ out, int* __restrict__ block_page, int cap)
-{
-    const int64_t len = scanLength(len_dev, mul, len_host);
-    const int64_t ntiles = (len + SCAN_TILE - 1) / SCAN_TILE;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int64_t base = tile * SCAN_TILE + (int64_t)threadIdx.x * 8;
-        int v[8], s = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            v[k] = (base + k < len) ? in[base + k] : 0;
-            s += v[k];
-        }
-        int tot;
-        int ex = blockExclusiveScan256(s, &tot) + tile_off[tile];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            if (base + k < len) {
-                if (ASSIGN) {
-                    int id = -1;
-                    if (v[k]) {
-                        in[base + k] = 0;
-                        if (ex < cap) {
-                            id = ex;
-                            block_page[ex] = (int)(base + k);
-                        }
-                    }
-                    out[base + k] = id;
-                }
-                else
-                    out[base + k] = ex;
-            }
-            ex += v[k];
-        }
-    }
+    return v;
 }
 
 // ------------------------------------------------------------------ binning
-// sortParticlesAndPolluteGrid key build (MpmSimulationBase.cpp:1545-1550) and
-// MpmGrid::pollute page activation (MpmGrid.h:153-193).
+// sortParticlesAndPolluteGrid (MpmSimulationBase.cpp:1531-1573) and
+// MpmGrid::pollute (MpmGrid.h:153-193), as four launches:
+//   k_bin        key of every particle, page activation, per-page particle
+//                count and each particle's slot in its page
+//   k_page_scan  one single-pass scan over the pages of (activated, count):
+//                dense block ids in page (Morton) order, the page table, and
+//                each block's range of sorted positions; clears the page flags
+//   k_scatter    particle -> its block's range; zeroes the blocks' grid
+//                accumulators, builds the neighbour table, starts the substep
+//                (device clock, fixed-point scales)
+//   k_blocksort  inside each block: counting sort by cell, then by original
+//                index inside the cell, in LDS
+// Sorting by (block in page order, cell, original index) is std::sort of the
+// reference's unique 64-bit keys (MpmSimulationBase.cpp:1552).
+
+// counters[] slots
+enum : int { CNT_BLOCKS = 0, CNT_STATUS = 1, CNT_EPOCH = 2, CNT_TICKET = 3 };
+
+constexpr int PSCAN_ITEMS = 8;
+constexpr int PSCAN_TILE = WG * PSCAN_ITEMS; // pages per scan tile
+constexpr int SCAN_SPIN_LIMIT = 1 << 16;
+
+// Page-scan tile status word: [63:55] epoch of the substep (1..511),
+// [54:53] 1 = tile aggregate / 2 = inclusive prefix, [52:27] activated pages
+// (< 2^26), [26:0] particles (< 2^27).  Every tile publishes every substep, so
+// a word whose epoch differs is the previous substep's and is waited on.
+__device__ __forceinline__ uint64_t scanWord(unsigned epoch, unsigned flag, uint64_t pair)
+{
+    const uint64_t pages = pair >> 32, parts = pair & 0xffffffffull;
+    return ((uint64_t)epoch << 55) | ((uint64_t)flag << 53) | (pages << 27) | parts;
+}
+
+__device__ __forceinline__ uint64_t scanPair(uint64_t w)
+{
+    return (((w >> 27) & ((1ull << 26) - 1)) << 32) | (w & ((1ull << 27) - 1));
+}
+
 template <class T, class AccT>
 __global__ __launch_bounds__(WG) void k_bin(Dev<T, AccT> D)
 {
     using B = Blk<T>;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int e = D.counters[CNT_EPOCH];
+        D.counters[CNT_EPOCH] = e % 511 + 1; // this substep's page-scan epoch
+        D.counters[CNT_TICKET] = 0;          // page-scan tile tickets
+    }
     const int p = blockIdx.x * WG + threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const bool active = p < D.n;
-    uint32_t page = 0xffffffffu;
+    uint32_t page = 0xffffffffu - (uint32_t)lane; // inactive lanes never join a run
     int bb[3] = { 0, 0, 0 };
     if (active) {
         int bits = 0;
@@ -251,11 +200,19 @@ __global__ __launch_bounds__(WG) void k_bin(Dev<T, AccT> D)
         pageElem<T>((uint32_t)bb[0], (uint32_t)bb[1], (uint32_t)bb[2], page, elem);
         D.key[p] = ((uint64_t)page << (B::xbits + B::ybits + B::zbits)) | elem;
     }
-    // one activation per run of equal pages inside the wavefront
-    uint32_t prev = (uint32_t)__shfl_up((int)page, 1);
-    const int lane = threadIdx.x & 63;
-    if (active && (lane == 0 || prev != page)) {
-        uint32_t Bx = bb[0] >> B::xbits, By = bb[1] >> B::ybits, Bz = bb[2] >> B::zbits;
+    // runs of equal pages inside the wavefront (live order is the previous
+    // substep's sorted order): one count atomic and one activation per run
+    const uint32_t prev = (uint32_t)__shfl_up((int)page, 1);
+    const bool head = (lane == 0) || (prev != page);
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = heads & (~0ull >> (63 - lane));
+    const int start = 63 - __clzll(upto);
+    const uint64_t after = (lane == 63) ? 0ull : (heads & (~0ull << (lane + 1)));
+    const int next = after ? __ffsll((unsigned long long)after) - 1 : 64;
+    int slot = 0;
+    if (active && head) {
+        slot = atomicAdd(&D.page_count[page], next - lane);
+        const uint32_t Bx = bb[0] >> B::xbits, By = bb[1] >> B::ybits, Bz = bb[2] >> B::zbits;
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -263,26 +220,175 @@ __global__ __launch_bounds__(WG) void k_bin(Dev<T, AccT> D)
 #pragma unroll
                 for (int c = 0; c < 2; c++) D.touched[pageOfBlock<T>(D.masks, Bx + a, By + b, Bz + c)] = 1;
     }
+    slot = __shfl(slot, start);
+    if (active) D.rank[p] = slot + (lane - start);
 }
 
+// Single-pass scan (tiles taken in ticket order, each adding its predecessors'
+// published prefixes: decoupled look-back) over the pages of the pair
+// (activated, particle count).  Epilogue per page: dense block id or -1 in the
+// page table, and for each block its page, first sorted position and particle
+// count.  The flags and counts are cleared for the next substep.  A tile waits
+// only on tiles that took earlier tickets, which are running; the wait is also
+// bounded (SCAN_SPIN_LIMIT polls, then ST_SCAN is latched).
 template <class T, class AccT>
-__global__ __launch_bounds__(64) void k_prepare(Dev<T, AccT> D)
+__global__ __launch_bounds__(WG) void k_page_scan(Dev<T, AccT> D)
+{
+    __shared__ int s_tile;
+    __shared__ uint64_t s_wsum[WG / 64];
+    __shared__ uint64_t s_prefix;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int64_t np = D.num_pages;
+    const int ntiles = (int)((np + PSCAN_TILE - 1) / PSCAN_TILE);
+    const unsigned epoch = (unsigned)D.counters[CNT_EPOCH];
+    if (t == 0) s_tile = atomicAdd(&D.counters[CNT_TICKET], 1);
+    __syncthreads();
+    const int tile = s_tile;
+    if (tile >= ntiles) return;
+    const int64_t base = (int64_t)tile * PSCAN_TILE + (int64_t)t * PSCAN_ITEMS;
+    uint32_t fl[PSCAN_ITEMS], ct[PSCAN_ITEMS];
+    uint64_t mine = 0;
+#pragma unroll
+    for (int k = 0; k < PSCAN_ITEMS; k++) {
+        const int64_t pg = base + k;
+        fl[k] = 0;
+        ct[k] = 0;
+        if (pg < np) {
+            fl[k] = D.touched[pg] != 0;
+            ct[k] = (uint32_t)D.page_count[pg];
+        }
+        mine += ((uint64_t)fl[k] << 32) | ct[k];
+    }
+    const uint64_t inc = waveInclusiveScan64(mine);
+    if (lane == 63) s_wsum[wv] = inc;
+    __syncthreads();
+    uint64_t off = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < WG / 64; w++) {
+        if (w < wv) off += s_wsum[w];
+        total += s_wsum[w];
+    }
+    if (t == 0) {
+        uint64_t pre = 0;
+        if (tile == 0)
+            __hip_atomic_store(&D.scan_state[0], scanWord(epoch, 2, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else {
+            __hip_atomic_store(&D.scan_state[tile], scanWord(epoch, 1, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int j = tile - 1, spins = 0;
+            while (j >= 0) {
+                const uint64_t w = __hip_atomic_load(&D.scan_state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned we = (unsigned)(w >> 55), wf = (unsigned)(w >> 53) & 3u;
+                if (we != epoch || wf == 0) {
+                    if (++spins > SCAN_SPIN_LIMIT) {
+                        latch(D.counters, ST_SCAN);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                pre += scanPair(w);
+                if (wf == 2) break;
+                j--;
+            }
+            __hip_atomic_store(&D.scan_state[tile], scanWord(epoch, 2, pre + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = pre;
+        if (tile == ntiles - 1) {
+            const int64_t blocks = (int64_t)((pre + total) >> 32);
+            if (blocks > D.cap_blocks) latch(D.counters, ST_CAPACITY);
+            D.counters[CNT_BLOCKS] = (int)min<int64_t>(blocks, D.cap_blocks);
+        }
+    }
+    __syncthreads();
+    uint64_t ex = s_prefix + off + inc - mine;
+#pragma unroll
+    for (int k = 0; k < PSCAN_ITEMS; k++) {
+        const int64_t pg = base + k;
+        if (pg < np) {
+            int id = -1;
+            if (fl[k]) {
+                const int bid = (int)(ex >> 32);
+                D.touched[pg] = 0;
+                if (bid < D.cap_blocks) {
+                    id = bid;
+                    D.block_page[bid] = (int)pg;
+                    D.block_start[bid] = (int)(ex & 0xffffffffull);
+                    D.block_count[bid] = (int)ct[k];
+                }
+            }
+            if (ct[k]) D.page_count[pg] = 0;
+            D.page_table[pg] = id;
+        }
+        ex += ((uint64_t)fl[k] << 32) | ct[k];
+    }
+}
+
+// Start of a substep: device clock and (float form) the fixed-point scales of
+// this substep's P2G from the bounds the previous G2P (or the upload) wrote, one
+// slot per workgroup: 2^(40 - e) with max = f 2^e, f in [0.5, 1).
+template <class T, class AccT>
+__device__ __forceinline__ void beginStep(const Dev<T, AccT>& D, double dt)
+{
+    // clock[0] = time of the next substep, clock[1] = this substep's t
+    if (threadIdx.x == 0) {
+        D.clock[1] = D.clock[0];
+        D.clock[0] = D.clock[0] + dt;
+    }
+    if constexpr (!std::is_same<AccT, long long>::value) return;
+    // one wavefront per scale group over the BOUND_SLOTS slots, which are
+    // cleared for this substep's G2P
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    float m = 0.f;
+    for (int k = g; k < NSC; k += WG / 64) {
+        const float v = __uint_as_float(D.maxbits[k * BOUND_SLOTS + lane]);
+        D.maxbits[k * BOUND_SLOTS + lane] = 0u;
+        m = waveMax(v);
+        if (lane == 0 && m > 0.f && isfinite(m)) {
+            int e;
+            frexpf(m, &e);
+            int se = (int)FIXED_TARGET_EXP - e;
+            se = se < -120 ? -120 : (se > 120 ? 120 : se);
+            D.scales[k] = ldexpf(1.0f, se);
+        }
+    }
+}
+
+// Workgroups [0, part_wgs): particle p -> block_start[block] + its slot
+// (order inside a block is fixed by k_blocksort).  The next PREP_WGS
+// workgroups: zero the accumulators of every block's nodes and build the
+// forward-neighbour block table.  The last workgroup starts the substep.
+constexpr int PREP_WGS = 2048;
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D, double dt, int part_wgs)
 {
     using B = Blk<T>;
-    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
-    const int nb = D.counters[0];
+    constexpr int bbits = B::xbits + B::ybits + B::zbits;
+    constexpr int NB = 1 << bbits;
+    const int t = threadIdx.x;
+    if ((int)blockIdx.x < part_wgs) {
+        const int p = blockIdx.x * WG + t;
+        if (p >= D.n) return;
+        const int bid = D.page_table[D.key[p] >> bbits];
+        if (bid < 0) return; // capacity exceeded (latched by the scan)
+        const int pos = D.block_start[bid] + D.rank[p];
+        if ((unsigned)pos < (unsigned)D.n) D.tmp[pos] = p;
+        return;
+    }
+    if ((int)blockIdx.x == part_wgs + PREP_WGS) {
+        beginStep(D, dt);
+        return;
+    }
+    const int nb = D.counters[CNT_BLOCKS];
     const int nf = D.any_damage ? NGF : GPF;
-    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-        const int t = threadIdx.x;
-        if (t < NB) {
-            const int64_t node = (int64_t)b * NB + t;
-            for (int f = 0; f < nf; f++) D.acc[f * D.cap_nodes + node] = AccT(0);
-            D.cell_count[node] = 0;
+    for (int b = (int)blockIdx.x - part_wgs; b < nb; b += PREP_WGS) {
+        for (int e = t; e < nf * NB; e += WG) {
+            const int f = e / NB, nd = e % NB;
+            D.acc[f * D.cap_nodes + (int64_t)b * NB + nd] = AccT(0);
         }
         if (t < 8) {
             uint32_t Bx, By, Bz;
             blockOfPage<T>((uint32_t)D.block_page[b], Bx, By, Bz);
-            uint32_t nx = Bx + ((t >> 2) & 1), ny = By + ((t >> 1) & 1), nz = Bz + (t & 1);
+            const uint32_t nx = Bx + ((t >> 2) & 1), ny = By + ((t >> 1) & 1), nz = Bz + (t & 1);
             int id = -1;
             if ((nx << B::xbits) < (uint32_t)D.domain && (ny << B::ybits) < (uint32_t)D.domain
                 && (nz << B::zbits) < (uint32_t)D.domain)
@@ -292,132 +398,113 @@ __global__ __launch_bounds__(64) void k_prepare(Dev<T, AccT> D)
     }
 }
 
-// Per-cell occupancy and each particle's slot in its cell.  Live order is the
-// previous substep's sorted order, so equal cells come in runs: one atomic per
-// run inside a wavefront, slots in live order within the run.
+// Order inside each block.  The block's particles are staged once in LDS as
+// key = cell << (32 - block_bits) | original index (the low 32 bits of the
+// reference key, unique) with their live index; a histogram and cursors group
+// them by cell; then every particle's position is its cell's start plus the
+// number of keys in its cell below its own (all particles ranked in parallel).
+// Blocks larger than the LDS stage take the same steps with the grouped
+// arrays in HBM.  Writes perm, spid, sorted_cell and every cell's start and
+// count.
+constexpr int BSORT_STAGE = 2048;
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_count(Dev<T, AccT> D)
+__global__ __launch_bounds__(WG) void k_blocksort(Dev<T, AccT> D)
 {
     using B = Blk<T>;
     constexpr int bbits = B::xbits + B::ybits + B::zbits;
-    const int p = blockIdx.x * WG + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    int cell = -1 - lane; // distinct per inactive lane: never joins a run
-    if (p < D.n) {
-        const uint64_t key = D.key[p];
-        const int bid = D.page_table[key >> bbits];
-        if (bid >= 0)
-            cell = bid * (1 << bbits) + (int)(key & ((1 << bbits) - 1));
-        else
-            latch(D.counters, ST_CAPACITY);
-    }
-    const int prev = __shfl_up(cell, 1);
-    const bool head = (lane == 0) || (prev != cell);
-    const uint64_t heads = __ballot(head);
-    const uint64_t upto = heads & (~0ull >> (63 - lane));      // heads at lanes <= this lane
-    const int start = 63 - __clzll(upto);                       // this run's head lane
-    const uint64_t after = (lane == 63) ? 0ull : (heads & (~0ull << (lane + 1)));
-    const int next = after ? __ffsll((unsigned long long)after) - 1 : 64;
-    int base = 0;
-    if (head && cell >= 0) base = atomicAdd(&D.cell_count[cell], next - lane);
-    base = __shfl(base, start);
-    if (p < D.n) D.rank[p] = cell >= 0 ? base + (lane - start) : -1;
-}
-
-template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D)
-{
-    using B = Blk<T>;
-    constexpr int bbits = B::xbits + B::ybits + B::zbits;
-    const int p = blockIdx.x * WG + threadIdx.x;
-    if (p >= D.n) return;
-    int r = D.rank[p];
-    if (r < 0) return;
-    uint64_t key = D.key[p];
-    int bid = D.page_table[key >> bbits];
-    int64_t cell = (int64_t)bid * (1 << bbits) + (int)(key & ((1 << bbits) - 1));
-    const int pos = D.cell_start[cell] + r;
-    D.perm[pos] = p;
-    D.spid[pos] = D.pid[p];
-}
-
-// Order inside a cell by original particle index: together with the page-ordered
-// block ids and element-ordered cells this reproduces std::sort of the
-// reference's unique 64-bit keys (MpmSimulationBase.cpp:1552).  A workgroup
-// takes 256 consecutive cells; their particles are contiguous, so (index,
-// live slot) pairs are staged in LDS with coalesced loads, each thread
-// insertion-sorts its cell there, and the result and the cell ids go back
-// coalesced.  Ranges larger than the LDS stage fall back to a per-cell shell
-// sort in HBM.
-constexpr int CELLSORT_CELLS = 256, CELLSORT_STAGE = 4096;
-template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_cellsort(Dev<T, AccT> D)
-{
-    using B = Blk<T>;
-    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
-    __shared__ int kk[CELLSORT_STAGE];
-    __shared__ int vv[CELLSORT_STAGE];
-    __shared__ int cc[CELLSORT_STAGE];
+    constexpr int NB = 1 << bbits;
+    constexpr int idbits = 32 - bbits;
+    constexpr uint32_t idmask = (1u << idbits) - 1u;
+    __shared__ uint32_t ska[BSORT_STAGE], skb[BSORT_STAGE];
+    __shared__ int pa[BSORT_STAGE], pb[BSORT_STAGE];
+    __shared__ int hist[NB], cur[NB], cst[NB];
     const int t = threadIdx.x;
-    const int64_t ncells = (int64_t)D.counters[0] * NB;
-    for (int64_t c0 = (int64_t)blockIdx.x * CELLSORT_CELLS; c0 < ncells; c0 += (int64_t)gridDim.x * CELLSORT_CELLS) {
-        const int64_t clast = min(c0 + CELLSORT_CELLS, ncells) - 1;
-        const int S0 = D.cell_start[c0];
-        const int S1 = D.cell_start[clast] + D.cell_count[clast];
-        const int64_t c = c0 + t;
-        const bool mine = c <= clast;
-        const int cs = mine ? D.cell_start[c] : 0;
-        const int cnt = mine ? D.cell_count[c] : 0;
-        if (S1 - S0 <= CELLSORT_STAGE) {
-            for (int i = t; i < S1 - S0; i += WG) {
-                kk[i] = D.spid[S0 + i];
-                vv[i] = D.perm[S0 + i];
+    const int nb = D.counters[CNT_BLOCKS];
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int start = D.block_start[b], cnt = D.block_count[b];
+        const bool staged = cnt <= BSORT_STAGE;
+        __syncthreads(); // LDS of the previous block done
+        if (t < NB) hist[t] = 0;
+        __syncthreads();
+        // stage (key, live index) and count cells
+        for (int i = t; i < cnt; i += WG) {
+            int p = D.tmp[start + i];
+            if ((unsigned)p >= (unsigned)D.n) p = 0;
+            const int c = (int)(D.key[p] & (NB - 1));
+            atomicAdd(&hist[c], 1);
+            if (staged) {
+                ska[i] = ((uint32_t)c << idbits) | (uint32_t)D.pid[p];
+                pa[i] = p;
+            }
+        }
+        __syncthreads();
+        if (t < 64) {
+            const int v = t < NB ? hist[t] : 0;
+            const int ex = waveInclusiveScan(v) - v;
+            if (t < NB) {
+                cst[t] = ex;
+                cur[t] = ex;
+            }
+        }
+        __syncthreads();
+        if (staged) {
+            // group by cell
+            for (int i = t; i < cnt; i += WG) {
+                const uint32_t k = ska[i];
+                const int dst = atomicAdd(&cur[k >> idbits], 1);
+                skb[dst] = k;
+                pb[dst] = pa[i];
             }
             __syncthreads();
-            if (cnt > 1) {
-                int* k = kk + (cs - S0);
-                int* v = vv + (cs - S0);
-                for (int i = 1; i < cnt; i++) {
-                    const int kv = k[i], vv_i = v[i];
-                    int j = i;
-                    while (j > 0 && k[j - 1] > kv) {
-                        k[j] = k[j - 1];
-                        v[j] = v[j - 1];
-                        j--;
-                    }
-                    k[j] = kv;
-                    v[j] = vv_i;
-                }
+            // rank inside the cell; write the sorted block
+            for (int i = t; i < cnt; i += WG) {
+                const uint32_t k = skb[i];
+                const int c = (int)(k >> idbits);
+                const int c0 = cst[c], c1 = c0 + hist[c];
+                int pos = c0;
+                for (int j = c0; j < c1; j++) pos += skb[j] < k;
+                D.perm[start + pos] = pb[i];
+                D.spid[start + pos] = (int)(k & idmask);
+                D.sorted_cell[start + pos] = b * NB + c;
             }
-            for (int i = 0; i < cnt; i++) cc[cs - S0 + i] = (int)c;
-            __syncthreads();
-            for (int i = t; i < S1 - S0; i += WG) {
-                D.spid[S0 + i] = kk[i];
-                D.perm[S0 + i] = vv[i];
-                D.sorted_cell[S0 + i] = cc[i];
-            }
-            __syncthreads();
         }
         else {
-            if (cnt > 0) {
-                int* key = D.spid + cs;
-                int* val = D.perm + cs;
-                int gap = 1;
-                while (gap < cnt / 3) gap = 3 * gap + 1;
-                for (; gap >= 1; gap /= 3)
-                    for (int i = gap; i < cnt; i++) {
-                        const int kv = key[i], v = val[i];
-                        int j = i;
-                        while (j >= gap && key[j - gap] > kv) {
-                            key[j] = key[j - gap];
-                            val[j] = val[j - gap];
-                            j -= gap;
-                        }
-                        key[j] = kv;
-                        val[j] = v;
-                    }
-                for (int i = 0; i < cnt; i++) D.sorted_cell[cs + i] = (int)c;
+            // the same in HBM: group into perm/spid, then rank through tmp
+            for (int i = t; i < cnt; i += WG) {
+                int p = D.tmp[start + i];
+                if ((unsigned)p >= (unsigned)D.n) p = 0;
+                const int c = (int)(D.key[p] & (NB - 1));
+                const int dst = atomicAdd(&cur[c], 1);
+                D.perm[start + dst] = p;
+                D.spid[start + dst] = D.pid[p];
             }
+            __syncthreads();
+            __threadfence_block();
+            for (int i = t; i < cnt; i += WG) D.tmp[start + i] = D.perm[start + i];
+            __syncthreads();
+            for (int i = t; i < cnt; i += WG) D.sorted_cell[start + i] = D.spid[start + i];
+            __syncthreads();
+            for (int i = t; i < cnt; i += WG) {
+                const int p = D.tmp[start + i];
+                const int id = D.sorted_cell[start + i];
+                int c = NB - 1;
+                while (c > 0 && cst[c] > i) c--;
+                const int c0 = cst[c], c1 = c0 + hist[c];
+                int pos = c0;
+                for (int j = c0; j < c1; j++) pos += D.sorted_cell[start + j] < id;
+                D.perm[start + pos] = p;
+                D.spid[start + pos] = id;
+            }
+            __syncthreads();
+            for (int i = t; i < cnt; i += WG) {
+                int c = NB - 1;
+                while (c > 0 && cst[c] > i) c--;
+                D.sorted_cell[start + i] = b * NB + c;
+            }
+        }
+        if (t < NB) {
+            D.cell_start[b * NB + t] = start + cst[t];
+            D.cell_count[b * NB + t] = hist[t];
         }
     }
 }
@@ -479,13 +566,6 @@ __device__ __forceinline__ void vtauSym(const MatDev<T>& mt, const Mat3<T>& F, T
     S[5] = s5;
 }
 
-__device__ __forceinline__ float waveMax(float v)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d));
-    return v;
-}
-
 // Upper bounds of one particle's P2G contributions per scale group, folded
 // into the running per-thread maxima b: mass w m; momentum w (m v + m C (x_i - x_p))
 // with |x_i - x_p| <= 1.5 dx; force tau grad(w) with |grad w| <= 1/dx per axis;
@@ -512,9 +592,10 @@ __device__ __forceinline__ void particleBounds(float* b, T m, const T* v, const 
     }
 }
 
-// Workgroup maximum of the per-thread bounds -> this workgroup's slot
-// (one slot per workgroup of a fixed-size launch, no contended atomics).
-__device__ __forceinline__ void writeBounds(unsigned int* maxbits, int nslots, const float* b)
+// Workgroup maximum of the per-thread bounds, folded into slot
+// blockIdx % BOUND_SLOTS of each scale group with an atomic max on the float
+// bits (non-negative floats order like their bit patterns).
+__device__ __forceinline__ void writeBounds(unsigned int* maxbits, const float* b)
 {
     __shared__ float wmax[NSC][WG / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -528,12 +609,13 @@ __device__ __forceinline__ void writeBounds(unsigned int* maxbits, int nslots, c
         float mx = 0.f;
 #pragma unroll
         for (int w = 0; w < WG / 64; w++) mx = fmaxf(mx, wmax[threadIdx.x][w]);
-        maxbits[threadIdx.x * nslots + blockIdx.x] = __float_as_uint(mx);
+        if (mx > 0.f || !(mx == mx))
+            atomicMax(&maxbits[threadIdx.x * BOUND_SLOTS + (int)(blockIdx.x % BOUND_SLOTS)], __float_as_uint(mx));
     }
 }
 
 // Stress of the uploaded state and the P2G bounds of the first substep
-// (grid of exactly max_slots workgroups, striding over particles).
+// (grid of max_slots workgroups, striding over particles).
 template <class T, class AccT>
 __global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D)
 {
@@ -553,7 +635,7 @@ __global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D)
         const bool dmg = D.any_damage && mt.has_damage;
         particleBounds(b, D.P[PM * cap + p], v, Cm, S, dmg ? D.P[PD * cap + p] : T(0), D.dx, D.odx, D.any_damage != 0);
     }
-    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, D.max_slots, b);
+    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, b);
 }
 
 // Block-local LDS tile geometry: a block of BX x BY x BZ nodes plus the +2
@@ -799,43 +881,6 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
     }
     if (ovf & 1) latch(D.counters, ST_OVERFLOW);
     if (ovf & 2) latch(D.counters, ST_CAPACITY);
-}
-
-// Start of a substep: advance the device clock and (float form) set this
-// substep's fixed-point scales from the bounds the previous G2P (or the upload)
-// wrote, one slot per workgroup: 2^(40 - e) with max = f 2^e, f in [0.5,1).
-constexpr int BEGIN_WG = 1024;
-__global__ __launch_bounds__(BEGIN_WG) void k_begin_step(float* scales, const unsigned int* maxbits, int nslots, double* clock,
-                                                  double dt)
-{
-    // device clock: clock[0] = time of the next substep, clock[1] = this substep's t
-    if (threadIdx.x == 0) {
-        clock[1] = clock[0];
-        clock[0] = clock[0] + dt;
-    }
-    if (!scales) return;
-    __shared__ float red[NSC][BEGIN_WG / 64];
-    float mx[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
-    for (int i = threadIdx.x; i < nslots; i += BEGIN_WG)
-#pragma unroll
-        for (int g = 0; g < NSC; g++) mx[g] = fmaxf(mx[g], __uint_as_float(maxbits[g * nslots + i]));
-#pragma unroll
-    for (int g = 0; g < NSC; g++) {
-        const float w = waveMax(mx[g]);
-        if ((threadIdx.x & 63) == 0) red[g][threadIdx.x >> 6] = w;
-    }
-    __syncthreads();
-    if (threadIdx.x < NSC) {
-        float m = 0.f;
-        for (int w = 0; w < BEGIN_WG / 64; w++) m = fmaxf(m, red[threadIdx.x][w]);
-        if (m > 0.f && isfinite(m)) {
-            int e;
-            frexpf(m, &e);
-            int se = (int)FIXED_TARGET_EXP - e;
-            se = se < -120 ? -120 : (se > 120 ? 120 : se);
-            scales[threadIdx.x] = ldexpf(1.0f, se);
-        }
-    }
 }
 
 template <class T, class AccT>
@@ -1237,7 +1282,7 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
     }
     float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
     if (live) g2pFinish(D, s, R, dt, A, bnd);
-    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, D.max_slots, bnd);
+    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd);
 }
 
 } // namespace zmpm

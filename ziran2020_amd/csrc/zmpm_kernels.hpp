@@ -10,7 +10,8 @@
 //              6 stress components instead of F and every P2G contribution is
 //              bounded before P2G runs (exact fixed-point scales).
 //   page table int32[domain^3 / NB]: SPGrid page index -> dense block id or -1,
-//              rebuilt every substep by a scan over "touched" flags.
+//              rebuilt every substep by one scan over the pages' activation
+//              flags and particle counts.
 //   grid       per dense block, NB nodes (4x4x4 for float, 2x4x4 for double, the
 //              reference's SPGrid block shapes), field-major accumulators:
 //              double (FP64 form) or int64 fixed point (FP32 form).
@@ -56,7 +57,8 @@ enum : int {
     ST_CFL = 4,
     ST_OVERFLOW = 8,
     ST_CAPACITY = 16,
-    ST_HALF_CFL = 32
+    ST_HALF_CFL = 32,
+    ST_SCAN = 64 // a page-scan tile's look-back hit its poll limit
 };
 
 // SPGrid block shape of GridState<T,3> (64 B float -> 4x4x4, 128 B double -> 2x4x4)
@@ -105,18 +107,23 @@ struct Dev {
     int n;
     // binning
     uint64_t* key;     // (page << block_bits | elem) per live particle
-    int* rank;         // slot inside the cell
+    int* rank;         // slot inside the particle's page (k_bin)
+    int* tmp;          // sorted position -> live index, before the in-block sort
     int* perm;         // sorted position -> live index
     int* sorted_cell;  // sorted position -> dense cell id
-    int* spid;         // sorted position -> original index (scatter output, sorted in place)
+    int* spid;         // sorted position -> original index
     int* touched;      // per page: 1 if a particle block or its -1 neighbour has particles
+    int* page_count;   // per page: particles whose base node is in the page
     int* page_table;   // per page: dense block id or -1
     int64_t num_pages;
+    uint64_t* scan_state; // per page-scan tile: published prefix (see k_page_scan)
     int* block_page;   // dense block id -> page index
+    int* block_start;  // dense block id -> first sorted position
+    int* block_count;  // dense block id -> particles
     int* nbr;          // [cap_blocks][8] forward neighbour block ids
     int* cell_count;   // [cap_blocks*NB]
     int* cell_start;   // [cap_blocks*NB]
-    int* counters;     // [0] num_blocks, [1] status bits
+    int* counters;     // [0] num_blocks, [1] status bits, [2] scan epoch, [3] scan ticket
     double* clock;     // [0] time of the next substep, [1] time of the running substep
     int cap_blocks;
     int domain;
@@ -128,8 +135,8 @@ struct Dev {
     int64_t cap_nodes;
     // fixed-point scales (float form only)
     float* scales;     // [NSC] current scale 2^e
-    unsigned int* maxbits; // [NSC][max_slots] per-workgroup max |contribution| bound (float bits)
-    int max_slots;         // workgroups of the last particle-order launch (G2P / upload)
+    unsigned int* maxbits; // [NSC][64] max |contribution| bound (float bits), atomic-max slots
+    int max_slots;         // workgroups of the particle-order launches (G2P / upload)
     // tables
     const MatDev<T>* mats;
     const ColDev* cols;
