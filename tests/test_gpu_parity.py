@@ -263,3 +263,39 @@ def test_graph_replays_match_eager_substeps(gpu, precision):
         assert np.array_equal(a["x"], b["x"])
     else:
         assert normwise(a["x"], b["x"]) < 1e-12
+
+
+def _fields(out):
+    return {k: out[k] for k in ("x", "v", "C", "F", "d", "g")}
+
+
+def test_reupload_after_run_matches_fresh_handle(gpu):
+    # The substep that writes positions also bins them for the next substep;
+    # a re-upload must discard that binning (FP32 form: bit-exact).
+    sc = small_slab(n=8000)
+    sim = run(gpu, sc, A.PREC_FP32, 4)
+    sim.upload(sc.x, sc.v, sc.C, sc.F, sc.m, sc.V0, sc.material)
+    sim.run(3, sc.dt, 0.0)
+    a = _fields(sim.download())
+    b = _fields(run(gpu, sc, A.PREC_FP32, 3).download())
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_material_change_after_upload_restresses(gpu):
+    # New materials after the upload: the stored stress and the fixed-point
+    # bounds follow them (here damage switches on, which adds the phase-field
+    # quantities to P2G).
+    sc = small_slab(n=8000)
+    plain = [A.make_material(A.MODEL_QR_STABLE_NEO_HOOKEAN, 2e4, 0.3)]
+    sim = A.Simulation(gpu, dx=sc.dx, precision=A.PREC_FP32, domain_nodes=sc.domain_nodes, gravity=sc.gravity)
+    sim.set_materials(plain)
+    for c in sc.colliders:
+        sim.add_collider(c)
+    sim.upload(sc.x, sc.v, sc.C, sc.F, sc.m, sc.V0, sc.material)
+    sim.set_materials(sc.materials)
+    sim.run(3, sc.dt, 0.0)
+    a = _fields(sim.download())
+    b = _fields(run(gpu, sc, A.PREC_FP32, 3).download())
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k

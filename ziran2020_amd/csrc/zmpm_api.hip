@@ -287,7 +287,6 @@ void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
     hipStream_t st = h->stream;
     const int gp = gridFor(D.n, WG, 1 << 30);
     if (ev) (void)hipEventRecord(ev[0], st);
-    k_bin<T, AccT><<<gp, WG, 0, st>>>(D);
     k_page_scan<T, AccT><<<(int)((D.num_pages + PSCAN_TILE - 1) / PSCAN_TILE), WG, 0, st>>>(D);
     k_scatter<T, AccT><<<gp + PREP_WGS + 1, WG, 0, st>>>(D, dt, gp);
     k_blocksort<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, 8192), WG, 0, st>>>(D);
@@ -358,16 +357,31 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
     HIPCHK(h, hipMemcpyAsync(D.pmat, mat.data(), cap * sizeof(int), hipMemcpyHostToDevice, h->stream));
     // block count and status; the scan epoch and ticket (slots 2, 3) keep counting
     HIPCHK(h, hipMemsetAsync(D.counters, 0, 2 * sizeof(int), h->stream));
-    // G2P workgroups (WG particles each); the bound slots start cleared
+    // G2P workgroups (WG particles each); the bound slots start cleared, and so do
+    // the page flags and counts the last G2P may have left for a substep that
+    // did not run
     D.max_slots = std::max(1, (n + WG - 1) / WG);
     HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * BOUND_SLOTS * sizeof(unsigned int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.touched, 0, D.num_pages * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.page_count, 0, D.num_pages * sizeof(int), h->stream));
     if (n > 0) {
         // stress of the uploaded state and the first substep's fixed-point bounds
-        k_stress_init<T, AccT><<<D.max_slots, WG, 0, h->stream>>>(D);
+        k_stress_init<T, AccT><<<D.max_slots, WG, 0, h->stream>>>(D, 1);
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipGetLastError());
     h->stepped = false;
+    return ZMPM_OK;
+}
+
+template <class T, class AccT>
+zmpm_status restress(zmpm_context* h)
+{
+    Dev<T, AccT>& D = devOf<T, AccT>(h);
+    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * BOUND_SLOTS * sizeof(unsigned int), h->stream));
+    k_stress_init<T, AccT><<<D.max_slots, WG, 0, h->stream>>>(D, 0);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipGetLastError());
     return ZMPM_OK;
 }
 
@@ -594,6 +608,9 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
         refreshTables<float, long long>(h);
     else
         refreshTables<double, double>(h);
+    // particles already on the device: their stored stress and the next P2G's
+    // fixed-point bounds follow the new table (the binning stays)
+    if (h->n > 0) return h->fp32 ? restress<float, long long>(h) : restress<double, double>(h);
     return ZMPM_OK;
 }
 

@@ -4,8 +4,6 @@
 // One substep (Ziran MpmSimulationBase::advanceOneTimeStep,
 // Lib/MPM/MpmSimulationBase.cpp:262-357) is this launch sequence on the
 // handle's stream:
-//   k_bin            base node, SPGrid page/element key, page activation,
-//                    per-page particle counts
 //   k_page_scan      dense block ids in page (Morton) order, page table, block
 //                    ranges (single-pass scan)
 //   k_scatter        particles to their block ranges; zero block accumulators,
@@ -20,6 +18,8 @@
 //   k_g2p            APIC gather, x update, CFL flag, damage Laplacian and
 //                    update, F update, stress of the new state and the next
 //                    P2G's fixed-point bounds; writes particles in sorted order
+//                    and bins them for the next substep (binParticle: key,
+//                    page activation, per-page counts)
 #pragma once
 #include "zmpm_kernels.hpp"
 
@@ -128,9 +128,10 @@ __device__ __forceinline__ uint64_t waveInclusiveScan64(uint64_t v)
 
 // ------------------------------------------------------------------ binning
 // sortParticlesAndPolluteGrid (MpmSimulationBase.cpp:1531-1573) and
-// MpmGrid::pollute (MpmGrid.h:153-193), as four launches:
-//   k_bin        key of every particle, page activation, per-page particle
-//                count and each particle's slot in its page
+// MpmGrid::pollute (MpmGrid.h:153-193):
+//   binParticle  key of every particle, page activation, per-page particle
+//                count and each particle's slot in its page -- run by the
+//                kernel that wrote the positions (G2P, or the upload)
 //   k_page_scan  one single-pass scan over the pages of (activated, count):
 //                dense block ids in page (Morton) order, the page table, and
 //                each block's range of sorted positions; clears the page flags
@@ -164,18 +165,17 @@ __device__ __forceinline__ uint64_t scanPair(uint64_t w)
     return (((w >> 27) & ((1ull << 26) - 1)) << 32) | (w & ((1ull << 27) - 1));
 }
 
+// Binning of the particle at live index p for the coming substep: base node
+// (MpmSimulationBase.cpp:1545, without FMA), SPGrid key, activation of its page
+// and the +1 neighbours (MpmGrid.h:153-193), and its slot in its page's count.
+// Live order is the previous substep's sorted order, so equal pages come in
+// runs inside a wavefront: one count atomic and one activation per run.  Every
+// lane of the wavefront calls it (active or not).
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_bin(Dev<T, AccT> D)
+__device__ __forceinline__ void binParticle(const Dev<T, AccT>& D, int p, bool active, const T* x)
 {
     using B = Blk<T>;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const int e = D.counters[CNT_EPOCH];
-        D.counters[CNT_EPOCH] = e % 511 + 1; // this substep's page-scan epoch
-        D.counters[CNT_TICKET] = 0;          // page-scan tile tickets
-    }
-    const int p = blockIdx.x * WG + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const bool active = p < D.n;
     uint32_t page = 0xffffffffu - (uint32_t)lane; // inactive lanes never join a run
     int bb[3] = { 0, 0, 0 };
     if (active) {
@@ -183,8 +183,7 @@ __global__ __launch_bounds__(WG) void k_bin(Dev<T, AccT> D)
         const T hi = T(D.domain - 4);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            T x = D.P[(PX + a) * D.cap + p];
-            T t = sub_rn(mul_rn(x, D.odx), T(0.5));
+            T t = sub_rn(mul_rn(x[a], D.odx), T(0.5));
             if (!isfinite(t)) {
                 bits |= ST_NAN;
                 t = T(0);
@@ -200,8 +199,6 @@ __global__ __launch_bounds__(WG) void k_bin(Dev<T, AccT> D)
         pageElem<T>((uint32_t)bb[0], (uint32_t)bb[1], (uint32_t)bb[2], page, elem);
         D.key[p] = ((uint64_t)page << (B::xbits + B::ybits + B::zbits)) | elem;
     }
-    // runs of equal pages inside the wavefront (live order is the previous
-    // substep's sorted order): one count atomic and one activation per run
     const uint32_t prev = (uint32_t)__shfl_up((int)page, 1);
     const bool head = (lane == 0) || (prev != page);
     const uint64_t heads = __ballot(head);
@@ -222,6 +219,19 @@ __global__ __launch_bounds__(WG) void k_bin(Dev<T, AccT> D)
     }
     slot = __shfl(slot, start);
     if (active) D.rank[p] = slot + (lane - start);
+}
+
+// The binning of a substep is done by whatever wrote the particle positions
+// (G2P of the previous substep, or the upload): it also opens the page scan of
+// the coming substep.
+template <class T, class AccT>
+__device__ __forceinline__ void openPageScan(const Dev<T, AccT>& D)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int e = D.counters[CNT_EPOCH];
+        D.counters[CNT_EPOCH] = e % 511 + 1; // the coming substep's page-scan epoch
+        D.counters[CNT_TICKET] = 0;          // page-scan tile tickets
+    }
 }
 
 // Single-pass scan (tiles taken in ticket order, each adding its predecessors'
@@ -614,26 +624,37 @@ __device__ __forceinline__ void writeBounds(unsigned int* maxbits, const float* 
     }
 }
 
-// Stress of the uploaded state and the P2G bounds of the first substep
-// (grid of max_slots workgroups, striding over particles).
+// Stress of the uploaded state and the P2G bounds of the first substep; with
+// bin, also the binning of the first substep (grid of max_slots workgroups,
+// striding over particles).  Run again without bin after a material change.
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D)
+__global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D, int bin)
 {
     float b[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
     const int64_t cap = D.cap;
-    for (int p = blockIdx.x * WG + threadIdx.x; p < D.n; p += gridDim.x * WG) {
-        const MatDev<T>& mt = D.mats[D.pmat[p]];
-        Mat3<T> F = loadM(D.P, cap, PF, p);
-        T S[6], v[3], Cm[9];
-        vtauSym(mt, F, D.P[PVOL * cap + p], D.P[PG * cap + p], S);
+    if (bin) openPageScan(D);
+    for (int base = blockIdx.x * WG; base < D.n; base += gridDim.x * WG) {
+        const int p = base + threadIdx.x;
+        const bool active = p < D.n;
+        T x[3] = { T(0), T(0), T(0) };
+        if (active) {
+            const MatDev<T>& mt = D.mats[D.pmat[p]];
+            Mat3<T> F = loadM(D.P, cap, PF, p);
+            T S[6], v[3], Cm[9];
+            vtauSym(mt, F, D.P[PVOL * cap + p], D.P[PG * cap + p], S);
 #pragma unroll
-        for (int i = 0; i < 6; i++) D.P[(PS + i) * cap + p] = S[i];
+            for (int i = 0; i < 6; i++) D.P[(PS + i) * cap + p] = S[i];
 #pragma unroll
-        for (int a = 0; a < 3; a++) v[a] = D.P[(PV + a) * cap + p];
+            for (int a = 0; a < 3; a++) {
+                v[a] = D.P[(PV + a) * cap + p];
+                x[a] = D.P[(PX + a) * cap + p];
+            }
 #pragma unroll
-        for (int i = 0; i < 9; i++) Cm[i] = D.P[(PC + i) * cap + p];
-        const bool dmg = D.any_damage && mt.has_damage;
-        particleBounds(b, D.P[PM * cap + p], v, Cm, S, dmg ? D.P[PD * cap + p] : T(0), D.dx, D.odx, D.any_damage != 0);
+            for (int i = 0; i < 9; i++) Cm[i] = D.P[(PC + i) * cap + p];
+            const bool dmg = D.any_damage && mt.has_damage;
+            particleBounds(b, D.P[PM * cap + p], v, Cm, S, dmg ? D.P[PD * cap + p] : T(0), D.dx, D.odx, D.any_damage != 0);
+        }
+        if (bin) binParticle(D, p, active, x);
     }
     if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, b);
 }
@@ -1154,7 +1175,7 @@ __device__ __forceinline__ void loadG2P(const Dev<T, AccT>& D, int s, int p, Raw
 
 template <class T, class AccT>
 __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const RawG2P<T>& R, T dt, const G2PAcc<T>& A,
-                                          float* bnd)
+                                          float* bnd, T* xnew)
 {
     const int64_t cap = D.cap;
     T* Q = D.Q;
@@ -1180,7 +1201,8 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     for (int a = 0; a < 3; a++) {
         T inc = dt * A.picV[a];
         inc2 += inc * inc;
-        Q[(PX + a) * cap + s] = R.x[a] + inc;
+        xnew[a] = R.x[a] + inc;
+        Q[(PX + a) * cap + s] = xnew[a];
     }
     if (inc2 > dx * dx) latch(D.counters, ST_CFL);
     T d = R.d, g = R.g;
@@ -1281,7 +1303,11 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
         seg_lo = seg_hi;
     }
     float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
-    if (live) g2pFinish(D, s, R, dt, A, bnd);
+    T xnew[3] = { T(0), T(0), T(0) };
+    if (live) g2pFinish(D, s, R, dt, A, bnd, xnew);
+    // binning of the coming substep at the new positions (live index s)
+    openPageScan(D);
+    binParticle(D, s, live, xnew);
     if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd);
 }
 
