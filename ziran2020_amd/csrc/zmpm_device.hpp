@@ -37,10 +37,16 @@ namespace zmpm {
 constexpr int WG = 256;
 constexpr float FIXED_TARGET_EXP = 40.0f;
 constexpr int BOUND_SLOTS = 64; // atomic-max slots per fixed-point scale group
-// Per-lane bound on the summed |fixed-point values| one P2G lane adds to its tile:
-// by construction a lane adds at most CHUNK particles x 27 node-quantities of
-// |q| < 2^40 (< 2^52); 2^56 leaves int64 tile sums far from overflow.
-constexpr float FIXED_LANE_LIMIT = 72057594037927936.0f; // 2^56
+// Per-lane bound on the summed |fixed-point values| one P2G lane adds to its
+// tile.  By construction one particle's contributions to a stencil column are
+// below 15 x 2^40 summed over the 9 quantities and 3 nodes (each quantity's
+// per-particle bound is scaled to < 2^40; the force counts 3 components x 3
+// nodes), and a lane adds at most CHUNK = 128 particles, so a correct lane stays
+// below 1920 x 2^40 < 2^51; checking 2^51 also keeps every value in the range
+// of the exact double-to-int64 conversion used for it.
+constexpr float FIXED_LANE_LIMIT = 2251799813685248.0f; // 2^51
+constexpr double FIXED_MAGIC = 6755399441055744.0;      // 1.5 x 2^52
+constexpr long long FIXED_MAGIC_BITS = 0x4338000000000000LL;
 
 __device__ __forceinline__ void latch(int* counters, int bits) { atomicOr(&counters[1], bits); }
 
@@ -675,16 +681,20 @@ struct alignas(4 * sizeof(T)) V4 {
     T x, y, z, w;
 };
 
-// Adds a lane's partial sum into the LDS tile.  Float form: rounded to int64
-// fixed point; |q| is summed into chk, so one test of chk <= FIXED_LANE_LIMIT per lane
-// bounds every value added (and catches NaN / Inf) without a branch per value.
+// Adds a lane's partial sum into the LDS tile.  Float form: v * scale (exact:
+// the scale is a power of two) rounded to the nearest integer as int64 -- added
+// in double onto 1.5 x 2^52 it lands in the low mantissa bits (exact for
+// |v * scale| < 2^51), minus the constant's bit pattern; |v * scale| is summed
+// into chk, so one test of chk <= FIXED_LANE_LIMIT per lane bounds every value
+// added (and catches NaN / Inf) without a branch per value.
 template <class T, class AccT>
 __device__ __forceinline__ void ldsAdd(AccT* a, T v, float scale, float& chk)
 {
     if constexpr (std::is_same<AccT, long long>::value) {
-        const float q = rintf((float)v * scale);
-        chk += fabsf(q);
-        atomicAdd((unsigned long long*)a, (unsigned long long)(long long)q);
+        const float vs = (float)v * scale;
+        chk += fabsf(vs);
+        const double q = (double)vs + FIXED_MAGIC;
+        atomicAdd((unsigned long long*)a, (unsigned long long)(__double_as_longlong(q) - FIXED_MAGIC_BITS));
     }
     else
         unsafeAtomicAdd(a, v);
@@ -750,6 +760,7 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
     const int nblocks = D.counters[0];
     const float* sc = D.scales;
     const float scl[NGF] = { sc[SC_M], sc[SC_MV], sc[SC_MV], sc[SC_MV], sc[SC_F], sc[SC_F], sc[SC_F], sc[SC_PF], sc[SC_PW] };
+
     int ovf = 0;
     for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
         __syncthreads(); // previous block flushed
@@ -851,18 +862,37 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
                     const T b0 = q1.z, b1 = q2.y, b2 = q3.x; // m C dx, z column
                     const T s00 = q3.y, s01 = q3.z, s02 = q3.w, s11 = q4.x, s12 = q4.y, s22 = q4.z;
                     const T wz[3] = { q5.x, q5.y, q5.z }, dwz[3] = { q6.x, q6.y, q6.z };
+                    // V0 tau grad(w) at node k: float form as wz_k (S (gx, gy, 0)) +
+                    // dwz_k (S e_z wxy); the double form keeps its registers for
+                    // the per-node products
+                    T u0, u1, u2, v0, v1, v2;
+                    if constexpr (sizeof(T) == 4) {
+                        u0 = s00 * gx + s01 * gy;
+                        u1 = s01 * gx + s11 * gy;
+                        u2 = s02 * gx + s12 * gy;
+                        v0 = s02 * wxy;
+                        v1 = s12 * wxy;
+                        v2 = s22 * wxy;
+                    }
 #pragma unroll
                     for (int k = 0; k < 3; k++) {
                         const T wk = wxy * wz[k];
-                        const T g0 = gx * wz[k], g1 = gy * wz[k], g2 = wxy * dwz[k];
                         const T fk = T(k);
                         acc[k][GM] += q0.x * wk;
                         acc[k][GMV + 0] += wk * (a0 + b0 * fk);
                         acc[k][GMV + 1] += wk * (a1 + b1 * fk);
                         acc[k][GMV + 2] += wk * (a2 + b2 * fk);
-                        acc[k][GFO + 0] -= s00 * g0 + s01 * g1 + s02 * g2;
-                        acc[k][GFO + 1] -= s01 * g0 + s11 * g1 + s12 * g2;
-                        acc[k][GFO + 2] -= s02 * g0 + s12 * g1 + s22 * g2;
+                        if constexpr (sizeof(T) == 4) {
+                            acc[k][GFO + 0] -= wz[k] * u0 + dwz[k] * v0;
+                            acc[k][GFO + 1] -= wz[k] * u1 + dwz[k] * v1;
+                            acc[k][GFO + 2] -= wz[k] * u2 + dwz[k] * v2;
+                        }
+                        else {
+                            const T g0 = gx * wz[k], g1 = gy * wz[k], g2 = wxy * dwz[k];
+                            acc[k][GFO + 0] -= s00 * g0 + s01 * g1 + s02 * g2;
+                            acc[k][GFO + 1] -= s01 * g0 + s11 * g1 + s12 * g2;
+                            acc[k][GFO + 2] -= s02 * g0 + s12 * g1 + s22 * g2;
+                        }
                         acc[k][GPF] += q4.w * wk;
                         acc[k][GPW] += wk;
                     }
