@@ -1117,14 +1117,21 @@ struct G2PAcc {
     T picV[3], Bp[3][3], gv[3][3], lap;
 };
 
-// APIC gather of one particle from the staged block tile.
+// APIC gather of one particle from the staged block tile.  The 27-node sums
+// (gridToParticlesHelper, MpmSimulationBase.cpp:1412-1478) factor over the
+// tensor-product weights, so they run as sums over k (z), then j (y), then i
+// (x):  v = sum w v_i,  B = sum w v_i (x_i - x_p)^T,  grad v = sum v_i grad(w)^T
+// with grad(w) = (dw_x w_y w_z, w_x dw_y w_z, w_x w_y dw_z) / dx, and the
+// damage Laplacian sum d_i L_i with the reference's 3D weights
+// L = (ddw_x(i) w_y w_z + w_x ddw_x(i) w_z + w_x w_y ddw_x(i)) / dx^2
+// (MpmGrid.h:610-612 index ddw[1], ddw[2] with i as well).
 template <class T, class AccT>
 __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[Tile<T>::TN], const T* Xp, bool dmg,
                                           G2PAcc<T>& A)
 {
     using TL = Tile<T>;
-    const T odx = D.odx, dx = D.dx, odx2 = odx * odx;
-    T w[3][3], dw[3][3];
+    const T odx = D.odx, dx = D.dx;
+    T w[3][3], dw[3][3], off[3][3];
     int base[3];
     const T hi_t = T(D.domain - 4);
 #pragma unroll
@@ -1134,48 +1141,74 @@ __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[T
         if (!(tt >= T(0) && tt < hi_t)) tt = (tt >= hi_t) ? hi_t - T(1) : T(0);
         base[a] = (int)tt;
         bspline(xs, base[a], w[a], dw[a]);
+#pragma unroll
+        for (int i = 0; i < 3; i++) off[a][i] = T(base[a] + i) * dx - Xp[a]; // x_i - x_p
     }
+    const T wzo[3] = { w[2][0] * off[2][0], w[2][1] * off[2][1], w[2][2] * off[2][2] };
     const int lx0 = base[0] & (TL::BX - 1), ly0 = base[1] & (TL::BY - 1), lz0 = base[2] & (TL::BZ - 1);
+    T lap = T(0);
 #pragma unroll 1
     for (int i = 0; i < 3; i++) {
-        const T wi = w[0][i];
-        const T dwidxi = odx * dw[0][i];
-        const T ddwi = (i == 1) ? T(-2) : T(1);
-        const T xi = T(base[0] + i) * dx - Xp[0];
+        // sums over j of the z-sums: tv = sum wy s0, ty = sum wy (y_j - y_p) s0,
+        // tdy = sum dwy s0, tz = sum wy s2, tdz = sum wy s1
+        T tv[3] = { T(0), T(0), T(0) }, ty[3] = { T(0), T(0), T(0) }, tdy[3] = { T(0), T(0), T(0) };
+        T tz[3] = { T(0), T(0), T(0) }, tdz[3] = { T(0), T(0), T(0) };
+        T q0 = T(0), q1 = T(0), q2 = T(0);
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            const T wj = w[1][j];
-            const T wij = wi * wj;
-            const T dwijdxi = dwidxi * wj;
-            const T dwijdxj = wi * odx * dw[1][j];
-            const T yj = T(base[1] + j) * dx - Xp[1];
             const int nrow = ((lx0 + i) * TL::TY + (ly0 + j)) * TL::TZ + lz0;
+            // s0 = sum wz v, s1 = sum dwz v, s2 = sum wz (z_k - z_p) v; p0 = sum wz d, p1 = sum d
+            T s0[3] = { T(0), T(0), T(0) }, s1[3] = { T(0), T(0), T(0) }, s2[3] = { T(0), T(0), T(0) };
+            T p0 = T(0), p1 = T(0);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const T wk = w[2][k];
-                const T wijk = wij * wk;
-                const T g3[3] = { dwijdxi * wk, dwijdxj * wk, wij * odx * dw[2][k] };
-                const T zk = T(base[2] + k) * dx - Xp[2];
-                const T nv[3] = { gt[0][nrow + k], gt[1][nrow + k], gt[2][nrow + k] };
-                const T dxv[3] = { xi, yj, zk };
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    A.picV[a] += wijk * nv[a];
-                    const T wv = wijk * nv[a];
-#pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        A.Bp[a][c] += wv * dxv[c];
-                        A.gv[a][c] += nv[a] * g3[c];
-                    }
+                    const T nv = gt[a][nrow + k];
+                    s0[a] += w[2][k] * nv;
+                    s1[a] += dw[2][k] * nv;
+                    s2[a] += wzo[k] * nv;
                 }
                 if (dmg) {
-                    // quirk: ddw[1](i), ddw[2](i) instead of (j), (k) (MpmGrid.h:610-612)
-                    const T L = odx2 * ddwi * wj * wk + odx2 * ddwi * wi * wk + odx2 * ddwi * wi * wj;
-                    A.lap += gt[3][nrow + k] * L;
+                    const T ph = gt[3][nrow + k];
+                    p0 += w[2][k] * ph;
+                    p1 += ph;
                 }
             }
+            const T wy = w[1][j], dwy = dw[1][j], wyo = w[1][j] * off[1][j];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                tv[a] += wy * s0[a];
+                ty[a] += wyo * s0[a];
+                tdy[a] += dwy * s0[a];
+                tz[a] += wy * s2[a];
+                tdz[a] += wy * s1[a];
+            }
+            if (dmg) {
+                q0 += wy * p0;
+                q1 += p0;
+                q2 += wy * p1;
+            }
         }
+        const T wx = w[0][i], dwx = dw[0][i], wxo = w[0][i] * off[0][i];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            A.picV[a] += wx * tv[a];
+            A.Bp[a][0] += wxo * tv[a];
+            A.Bp[a][1] += wx * ty[a];
+            A.Bp[a][2] += wx * tz[a];
+            A.gv[a][0] += dwx * tv[a];
+            A.gv[a][1] += wx * tdy[a];
+            A.gv[a][2] += wx * tdz[a];
+        }
+        if (dmg) lap += ((i == 1) ? T(-2) : T(1)) * (q0 + wx * (q1 + q2));
     }
+    // (called once per particle: the particle's stencil lies in one block tile)
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) A.gv[a][c] *= odx;
+    A.lap += lap * (odx * odx);
 }
 
 // Particle update after the gather; writes sorted position s.
@@ -1289,12 +1322,14 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
     const int64_t cn = D.cap_nodes;
     const int nfield = D.any_damage ? 4 : 3;
     RawG2P<T> R;
-    bool dmg = false;
+    // the Laplacian is gathered for every particle when any material has damage
+    // (stored only for damaged ones), so no per-particle material load has to
+    // complete before the gathers
+    const bool dmg = D.any_damage != 0;
     if (live) {
         int p = D.perm[s];
         if ((unsigned)p >= (unsigned)D.n) p = 0;
         loadG2P(D, s, p, R);
-        dmg = D.any_damage && D.mats[R.mid].has_damage;
     }
     G2PAcc<T> A;
 #pragma unroll
