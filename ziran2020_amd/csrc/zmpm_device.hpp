@@ -693,8 +693,11 @@ __device__ __forceinline__ void ldsAdd(AccT* a, T v, float scale, float& chk)
     if constexpr (std::is_same<AccT, long long>::value) {
         const float vs = (float)v * scale;
         chk += fabsf(vs);
+        // the constant's low word is zero, so only the high word needs the offset
         const double q = (double)vs + FIXED_MAGIC;
-        atomicAdd((unsigned long long*)a, (unsigned long long)(__double_as_longlong(q) - FIXED_MAGIC_BITS));
+        const unsigned lo = (unsigned)__double2loint(q);
+        const unsigned hi = (unsigned)__double2hiint(q) - (unsigned)(FIXED_MAGIC_BITS >> 32);
+        atomicAdd((unsigned long long*)a, ((unsigned long long)hi << 32) | lo);
     }
     else
         unsafeAtomicAdd(a, v);
@@ -802,13 +805,16 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
                     a3[r] = mass * raw.v[r] + mC[3 * r] * xb[0] + mC[3 * r + 1] * xb[1] + mC[3 * r + 2] * xb[2];
                 T Sv[6];
 #pragma unroll
-                for (int i = 0; i < 6; i++) Sv[i] = raw.S[i] * D.odx;
+                for (int i = 0; i < 6; i++) Sv[i] = raw.S[i] * D.odx; // float form stores -S below
                 const T dx = D.dx;
                 q[0][t] = V4<T> { mass, a3[0], a3[1], a3[2] };
                 q[1][t] = V4<T> { mC[0] * dx, mC[1] * dx, mC[2] * dx, mC[3] * dx };
                 q[2][t] = V4<T> { mC[4] * dx, mC[5] * dx, mC[6] * dx, mC[7] * dx };
-                q[3][t] = V4<T> { mC[8] * dx, Sv[0], Sv[1], Sv[2] };
-                q[4][t] = V4<T> { Sv[3], Sv[4], Sv[5], raw.d };
+                // the force is -V0 tau grad(w): the float form accumulates it
+                // with FMAs of the negated stress
+                const T sg = sizeof(T) == 4 ? T(-1) : T(1);
+                q[3][t] = V4<T> { mC[8] * dx, sg * Sv[0], sg * Sv[1], sg * Sv[2] };
+                q[4][t] = V4<T> { sg * Sv[3], sg * Sv[4], sg * Sv[5], raw.d };
                 q[5][t] = V4<T> { w[2][0], w[2][1], w[2][2], T(0) };
                 q[6][t] = V4<T> { dw[2][0], dw[2][1], dw[2][2], T(0) };
 #pragma unroll
@@ -879,13 +885,16 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
                         const T wk = wxy * wz[k];
                         const T fk = T(k);
                         acc[k][GM] += q0.x * wk;
-                        acc[k][GMV + 0] += wk * (a0 + b0 * fk);
-                        acc[k][GMV + 1] += wk * (a1 + b1 * fk);
-                        acc[k][GMV + 2] += wk * (a2 + b2 * fk);
-                        if constexpr (sizeof(T) == 4) {
-                            acc[k][GFO + 0] -= wz[k] * u0 + dwz[k] * v0;
-                            acc[k][GFO + 1] -= wz[k] * u1 + dwz[k] * v1;
-                            acc[k][GFO + 2] -= wz[k] * u2 + dwz[k] * v2;
+                        acc[k][GMV + 0] += wk * (k == 0 ? a0 : a0 + b0 * fk);
+                        acc[k][GMV + 1] += wk * (k == 0 ? a1 : a1 + b1 * fk);
+                        acc[k][GMV + 2] += wk * (k == 0 ? a2 : a2 + b2 * fk);
+                        if constexpr (sizeof(T) == 4) { // s.. hold -V0 tau / dx here
+                            acc[k][GFO + 0] += wz[k] * u0;
+                            acc[k][GFO + 0] += dwz[k] * v0;
+                            acc[k][GFO + 1] += wz[k] * u1;
+                            acc[k][GFO + 1] += dwz[k] * v1;
+                            acc[k][GFO + 2] += wz[k] * u2;
+                            acc[k][GFO + 2] += dwz[k] * v2;
                         }
                         else {
                             const T g0 = gx * wz[k], g1 = gy * wz[k], g2 = wxy * dwz[k];
