@@ -73,6 +73,8 @@ struct zmpm_context {
     std::vector<Graph> graphs;
     bool use_graphs = true;
     int p2g_split = 64;
+    // workgroups of the grid-stride launches (results do not depend on them)
+    int sort_wgs = 2048, p2g_wgs = 2048, grid_wgs = 2048, prep_wgs = 2048;
 };
 
 namespace {
@@ -288,12 +290,13 @@ void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
     const int gp = gridFor(D.n, WG, 1 << 30);
     if (ev) (void)hipEventRecord(ev[0], st);
     k_page_scan<T, AccT><<<(int)((D.num_pages + PSCAN_TILE - 1) / PSCAN_TILE), WG, 0, st>>>(D);
-    k_scatter<T, AccT><<<gp + PREP_WGS + 1, WG, 0, st>>>(D, dt, gp);
-    k_blocksort<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, 8192), WG, 0, st>>>(D);
+    const int prep = (int)std::min<int64_t>(D.cap_blocks, h->prep_wgs);
+    k_scatter<T, AccT><<<gp + prep + 1, WG, 0, st>>>(D, dt, gp, prep);
+    k_blocksort<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, h->sort_wgs), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[1], st);
-    k_p2g<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, 4096), WG, 0, st>>>(D);
+    k_p2g<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[2], st);
-    k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, 8192), WG, 0, st>>>(D, (T)dt);
+    k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D, (T)dt);
     if (ev) (void)hipEventRecord(ev[3], st);
     k_g2p<T, AccT><<<D.max_slots, WG, 0, st>>>(D, (T)dt);
     if (ev) (void)hipEventRecord(ev[4], st);
@@ -559,6 +562,10 @@ zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
     h->fp32 = desc->precision == ZMPM_PREC_FP32;
     if (const char* e = std::getenv("ZMPM_NO_GRAPH")) h->use_graphs = std::atoi(e) == 0;
     if (const char* e = std::getenv("ZMPM_P2G_SPLIT")) h->p2g_split = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("ZMPM_SORT_WGS")) h->sort_wgs = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("ZMPM_P2G_WGS")) h->p2g_wgs = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("ZMPM_GRID_WGS")) h->grid_wgs = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("ZMPM_PREP_WGS")) h->prep_wgs = std::max(1, std::atoi(e));
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return ZMPM_E_HIP;

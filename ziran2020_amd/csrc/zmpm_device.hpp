@@ -370,12 +370,11 @@ __device__ __forceinline__ void beginStep(const Dev<T, AccT>& D, double dt)
 }
 
 // Workgroups [0, part_wgs): particle p -> block_start[block] + its slot
-// (order inside a block is fixed by k_blocksort).  The next PREP_WGS
+// (order inside a block is fixed by k_blocksort).  The next prep_wgs
 // workgroups: zero the accumulators of every block's nodes and build the
 // forward-neighbour block table.  The last workgroup starts the substep.
-constexpr int PREP_WGS = 2048;
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D, double dt, int part_wgs)
+__global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D, double dt, int part_wgs, int prep_wgs)
 {
     using B = Blk<T>;
     constexpr int bbits = B::xbits + B::ybits + B::zbits;
@@ -390,13 +389,13 @@ __global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D, double dt, int p
         if ((unsigned)pos < (unsigned)D.n) D.tmp[pos] = p;
         return;
     }
-    if ((int)blockIdx.x == part_wgs + PREP_WGS) {
+    if ((int)blockIdx.x == part_wgs + prep_wgs) {
         beginStep(D, dt);
         return;
     }
     const int nb = D.counters[CNT_BLOCKS];
     const int nf = D.any_damage ? NGF : GPF;
-    for (int b = (int)blockIdx.x - part_wgs; b < nb; b += PREP_WGS) {
+    for (int b = (int)blockIdx.x - part_wgs; b < nb; b += prep_wgs) {
         for (int e = t; e < nf * NB; e += WG) {
             const int f = e / NB, nd = e % NB;
             D.acc[f * D.cap_nodes + (int64_t)b * NB + nd] = AccT(0);
@@ -1247,12 +1246,11 @@ __device__ __forceinline__ void loadG2P(const Dev<T, AccT>& D, int s, int p, Raw
 
 template <class T, class AccT>
 __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const RawG2P<T>& R, T dt, const G2PAcc<T>& A,
-                                          float* bnd, T* xnew)
+                                          const MatDev<T>& mt, float* bnd, T* xnew)
 {
     const int64_t cap = D.cap;
     T* Q = D.Q;
     const int mid = R.mid;
-    const MatDev<T>& mt = D.mats[mid];
     const bool dmg = D.any_damage && mt.has_damage;
     const T dx = D.dx;
     const T Dinv = D.Dinv;
@@ -1378,7 +1376,7 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
     }
     float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
     T xnew[3] = { T(0), T(0), T(0) };
-    if (live) g2pFinish(D, s, R, dt, A, bnd, xnew);
+    if (live) g2pFinish(D, s, R, dt, A, D.mats[R.mid], bnd, xnew);
     // binning of the coming substep at the new positions (live index s)
     openPageScan(D);
     binParticle(D, s, live, xnew);
