@@ -284,35 +284,52 @@ __global__ __launch_bounds__(WG) void k_page_scan(Dev<T, AccT> D)
         if (w < wv) off += s_wsum[w];
         total += s_wsum[w];
     }
-    if (t == 0) {
+    // wavefront 0 publishes and looks back 64 predecessors per poll: lane l reads
+    // tile - 1 - l; the prefix is the sum up to the nearest inclusive word
+    if (wv == 0) {
         uint64_t pre = 0;
-        if (tile == 0)
-            __hip_atomic_store(&D.scan_state[0], scanWord(epoch, 2, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tile == 0) {
+            if (lane == 0)
+                __hip_atomic_store(&D.scan_state[0], scanWord(epoch, 2, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         else {
-            __hip_atomic_store(&D.scan_state[tile], scanWord(epoch, 1, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int j = tile - 1, spins = 0;
-            while (j >= 0) {
-                const uint64_t w = __hip_atomic_load(&D.scan_state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&D.scan_state[tile], scanWord(epoch, 1, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int base_j = tile - 1, spins = 0;
+            while (true) {
+                const int j = base_j - lane;
+                uint64_t w = scanWord(epoch, 2, 0); // before tile 0: an inclusive zero
+                if (j >= 0) w = __hip_atomic_load(&D.scan_state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const unsigned we = (unsigned)(w >> 55), wf = (unsigned)(w >> 53) & 3u;
-                if (we != epoch || wf == 0) {
+                const bool valid = we == epoch && wf != 0;
+                const uint64_t incl = __ballot(valid && wf == 2);
+                const int first = incl ? __ffsll((unsigned long long)incl) - 1 : 63;
+                const uint64_t need = (first == 63) ? ~0ull : ((2ull << first) - 1ull);
+                if (__ballot(!valid) & need) { // a predecessor up to `first` is not published yet
                     if (++spins > SCAN_SPIN_LIMIT) {
-                        latch(D.counters, ST_SCAN);
+                        if (lane == 0) latch(D.counters, ST_SCAN);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                pre += scanPair(w);
-                if (wf == 2) break;
-                j--;
+                uint64_t v = lane <= first ? scanPair(w) : 0ull;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+                pre += v;
+                if (incl) break;
+                base_j -= 64;
             }
-            __hip_atomic_store(&D.scan_state[tile], scanWord(epoch, 2, pre + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&D.scan_state[tile], scanWord(epoch, 2, pre + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_prefix = pre;
-        if (tile == ntiles - 1) {
-            const int64_t blocks = (int64_t)((pre + total) >> 32);
-            if (blocks > D.cap_blocks) latch(D.counters, ST_CAPACITY);
-            D.counters[CNT_BLOCKS] = (int)min<int64_t>(blocks, D.cap_blocks);
+        if (lane == 0) {
+            s_prefix = pre;
+            if (tile == ntiles - 1) {
+                const int64_t blocks = (int64_t)((pre + total) >> 32);
+                if (blocks > D.cap_blocks) latch(D.counters, ST_CAPACITY);
+                D.counters[CNT_BLOCKS] = (int)min<int64_t>(blocks, D.cap_blocks);
+            }
         }
     }
     __syncthreads();
