@@ -603,8 +603,7 @@ __device__ __forceinline__ void vtauSym(const MatDev<T>& mt, const Mat3<T>& F, T
 // with |x_i - x_p| <= 1.5 dx; force tau grad(w) with |grad w| <= 1/dx per axis;
 // phase field w d and w (w <= 1).
 template <class T>
-__device__ __forceinline__ void particleBounds(float* b, T m, const T* v, const T* Cm, const T* S, T d, T dx, T odx,
-                                               bool dmg)
+__device__ __forceinline__ void momentumBounds(float* b, T m, const T* v, const T* Cm, T dx)
 {
     b[SC_M] = fmaxf(b[SC_M], fabsf((float)m));
     float mv = 0.f;
@@ -613,15 +612,28 @@ __device__ __forceinline__ void particleBounds(float* b, T m, const T* v, const 
         float cr = fabsf((float)Cm[3 * r]) + fabsf((float)Cm[3 * r + 1]) + fabsf((float)Cm[3 * r + 2]);
         mv = fmaxf(mv, fabsf((float)m) * (fabsf((float)v[r]) + 1.5f * (float)dx * cr));
     }
+    b[SC_MV] = fmaxf(b[SC_MV], mv * 1.0001f);
+}
+
+template <class T>
+__device__ __forceinline__ void forceBounds(float* b, const T* S, T d, T odx, bool dmg)
+{
     const float a0 = fabsf((float)S[0]), a1 = fabsf((float)S[1]), a2 = fabsf((float)S[2]);
     const float a3 = fabsf((float)S[3]), a4 = fabsf((float)S[4]), a5 = fabsf((float)S[5]);
     const float f = (float)odx * fmaxf(a0 + a1 + a2, fmaxf(a1 + a3 + a4, a2 + a4 + a5));
-    b[SC_MV] = fmaxf(b[SC_MV], mv * 1.0001f);
     b[SC_F] = fmaxf(b[SC_F], f * 1.0001f);
     if (dmg) {
         b[SC_PF] = fmaxf(b[SC_PF], fabsf((float)d));
         b[SC_PW] = 1.f;
     }
+}
+
+template <class T>
+__device__ __forceinline__ void particleBounds(float* b, T m, const T* v, const T* Cm, const T* S, T d, T dx, T odx,
+                                               bool dmg)
+{
+    momentumBounds(b, m, v, Cm, dx);
+    forceBounds(b, S, d, odx, dmg);
 }
 
 // Workgroup maximum of the per-thread bounds, folded into slot
@@ -1265,35 +1277,43 @@ template <class T, class AccT>
 __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const RawG2P<T>& R, T dt, const G2PAcc<T>& A,
                                           const MatDev<T>& mt, float* bnd, T* xnew)
 {
+    constexpr bool FIXED = std::is_same<AccT, long long>::value;
     const int64_t cap = D.cap;
     T* Q = D.Q;
-    const int mid = R.mid;
     const bool dmg = D.any_damage && mt.has_damage;
     const T dx = D.dx;
     const T Dinv = D.Dinv;
     const T r = D.rpic;
     const T ca = (r + T(1)) * T(0.5), cb = (r - T(1)) * T(0.5);
-    T Cn[9];
+    // v, C, x first (values die early: the momentum bound is taken here)
+    {
+        T Cn[9];
 #pragma unroll
-    for (int a = 0; a < 3; a++) Q[(PV + a) * cap + s] = A.picV[a];
+        for (int a = 0; a < 3; a++) Q[(PV + a) * cap + s] = A.picV[a];
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+        for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            Cn[3 * a + c] = ca * (A.Bp[a][c] * Dinv) + cb * (A.Bp[c][a] * Dinv);
-            Q[(PC + 3 * a + c) * cap + s] = Cn[3 * a + c];
+            for (int c = 0; c < 3; c++) {
+                Cn[3 * a + c] = ca * (A.Bp[a][c] * Dinv) + cb * (A.Bp[c][a] * Dinv);
+                Q[(PC + 3 * a + c) * cap + s] = Cn[3 * a + c];
+            }
+        T inc2 = T(0);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            T inc = dt * A.picV[a];
+            inc2 += inc * inc;
+            xnew[a] = R.x[a] + inc;
+            Q[(PX + a) * cap + s] = xnew[a];
         }
-    T inc2 = T(0);
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        T inc = dt * A.picV[a];
-        inc2 += inc * inc;
-        xnew[a] = R.x[a] + inc;
-        Q[(PX + a) * cap + s] = xnew[a];
+        if (inc2 > dx * dx) latch(D.counters, ST_CFL);
+        if constexpr (FIXED) momentumBounds(bnd, R.m, A.picV, Cn, dx);
     }
-    if (inc2 > dx * dx) latch(D.counters, ST_CFL);
+    Q[PM * cap + s] = R.m;
+    Q[PVOL * cap + s] = R.V0;
+    Q[PLAP * cap + s] = dmg ? A.lap : T(0);
+    D.qid[s] = R.id;
+    D.qmat[s] = R.mid;
     T d = R.d, g = R.g;
-    const T V0 = R.V0;
     Mat3<T> Fn;
 #pragma unroll
     for (int i = 0; i < 9; i++) Fn.a[i / 3][i % 3] = R.F[i];
@@ -1312,23 +1332,16 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
                 Fn.a[a][c] = v;
             }
     }
+    Q[PD * cap + s] = d;
+    Q[PG * cap + s] = g;
 #pragma unroll
     for (int i = 0; i < 9; i++) Q[(PF + i) * cap + s] = Fn.a[i / 3][i % 3];
     // stress of the new state for the next substep's P2G
     T S[6];
-    vtauSym(mt, Fn, V0, g, S);
+    vtauSym(mt, Fn, R.V0, g, S);
 #pragma unroll
     for (int i = 0; i < 6; i++) Q[(PS + i) * cap + s] = S[i];
-    const T m = R.m;
-    Q[PM * cap + s] = m;
-    Q[PVOL * cap + s] = V0;
-    Q[PD * cap + s] = d;
-    Q[PG * cap + s] = g;
-    Q[PLAP * cap + s] = dmg ? A.lap : T(0);
-    D.qid[s] = R.id;
-    D.qmat[s] = mid;
-    if constexpr (std::is_same<AccT, long long>::value)
-        particleBounds(bnd, m, A.picV, Cn, S, dmg ? d : T(0), dx, D.odx, D.any_damage != 0);
+    if constexpr (FIXED) forceBounds(bnd, S, dmg ? d : T(0), D.odx, D.any_damage != 0);
 }
 
 template <class T, class AccT>
