@@ -245,6 +245,11 @@ zmpm_status uploadMaterials(zmpm_context* h)
         d.eta = (T)m.eta;
         d.zeta = (T)m.zeta;
         d.inv_sigma_crit2 = (T)(1.0 / (m.sigma_crit * m.sigma_crit));
+        double na = 1.0;
+        for (int k = 0; k < d.nd; k++)
+            na += std::fabs(m.alphas[k]) * (m.a0[k][0] * m.a0[k][0] + m.a0[k][1] * m.a0[k][1] + m.a0[k][2] * m.a0[k][2]);
+        // rounded up so the float form never under-estimates the bound
+        d.phi_bound_scale = (T)(na * na / (m.sigma_crit * m.sigma_crit) * (1.0 + 1e-6));
     }
     HIPCHK(h, hipMemcpyAsync(h->mats_dev, md.data(), sizeof(MatDev<T>) * kMaxMaterials, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -1110,27 +1110,36 @@ __device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>&
     sym.a[1][1] = t6[3] * iJ;
     sym.a[1][2] = sym.a[2][1] = t6[4] * iJ;
     sym.a[2][2] = t6[5] * iJ;
-    const Mat3<T> sp = positivePart(sym);
-    Mat3<T> R = polarRotation(F);
-    Mat3<T> A;
-    setIdentity(A);
-    for (int k = 0; k < mt.nd; k++) {
-        T a1[3];
+    // phi = tr(A s+ A s+) / sigma_c^2 <= |A|_2^2 |s+|_F^2 / sigma_c^2
+    //     <= (1 + sum |alpha_k| |a0_k|^2)^2 |s|_F^2 / sigma_c^2 (phi_bound_scale):
+    // when that is below 1 (with a rounding margin), <phi - 1>_+ is exactly 0 and
+    // sigma^+ and the polar rotation contribute nothing
+    const T sf2 = sym.a[0][0] * sym.a[0][0] + sym.a[1][1] * sym.a[1][1] + sym.a[2][2] * sym.a[2][2]
+        + T(2) * (sym.a[0][1] * sym.a[0][1] + sym.a[0][2] * sym.a[0][2] + sym.a[1][2] * sym.a[1][2]);
+    T dTilde = T(0);
+    if (!(mt.phi_bound_scale * sf2 < T(0.99))) {
+        const Mat3<T> sp = positivePart(sym);
+        Mat3<T> R = polarRotation(F);
+        Mat3<T> A;
+        setIdentity(A);
+        for (int k = 0; k < mt.nd; k++) {
+            T a1[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) a1[i] = R.a[i][0] * mt.a0[k][0] + R.a[i][1] * mt.a0[k][1] + R.a[i][2] * mt.a0[k][2];
+            for (int i = 0; i < 3; i++) a1[i] = R.a[i][0] * mt.a0[k][0] + R.a[i][1] * mt.a0[k][1] + R.a[i][2] * mt.a0[k][2];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) A.a[i][j] += mt.alphas[k] * (a1[i] * a1[j]);
+        }
+        Mat3<T> As = mul(A, sp), sA = mul(sp, A);
+        T contraction = T(0);
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) A.a[i][j] += mt.alphas[k] * (a1[i] * a1[j]);
+            for (int j = 0; j < 3; j++) contraction += As.a[i][j] * sA.a[i][j];
+        const T phi = mt.inv_sigma_crit2 * contraction;
+        dTilde = mt.zeta * macaulay(phi - T(1));
     }
-    Mat3<T> As = mul(A, sp), sA = mul(sp, A);
-    T contraction = T(0);
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) contraction += As.a[i][j] * sA.a[i][j];
-    const T phi = mt.inv_sigma_crit2 * contraction;
-    const T dTilde = mt.zeta * macaulay(phi - T(1));
     const T diff = ((T(1) - d) * dTilde) - geometricResist;
     const T newD = d + (ddiv(dt, mt.eta) * macaulay(diff));
     d = newD < T(1) ? newD : T(1);

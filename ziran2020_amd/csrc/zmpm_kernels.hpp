@@ -82,6 +82,7 @@ struct MatDev {
     T alphas[2];
     T k, l0, sigma_crit, eta, zeta;
     T inv_sigma_crit2; // 1 / sigma_c^2 (AnisoFractureSimulation.h:262)
+    T phi_bound_scale; // (1 + sum |alpha_k| |a0_k|^2)^2 / sigma_c^2: phi <= this * |sigma|_F^2
 };
 
 struct ColDev {

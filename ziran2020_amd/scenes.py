@@ -255,9 +255,18 @@ def two_material(n=16_000_000, dx=1.0 / 256, seed=5, steps=100, dt=1e-4, domain_
                  rho, V0, F, info=dict(n_solid=n_s, n_fluid=n_f))
 
 
+def damage_slab_active(**kw):
+    """The damage slab with sigma_c = 20 Pa: the phase field grows from the first
+    substeps, so every substep runs the full damage update (sigma^+ and the
+    structural tensor) for the loaded particles."""
+    kw.setdefault("sigma_crit", 20.0)
+    return damage_slab(**kw)
+
+
 WORKLOADS = {
     "elastic_cube": elastic_cube,
     "damage_slab": damage_slab,
+    "damage_slab_active": damage_slab_active,
     "sparse_spheres": sparse_spheres,
     "batched_scenes": batched_scenes,
     "two_material": two_material,
