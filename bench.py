@@ -71,11 +71,13 @@ def cpu_baseline(scene, budget_s=12.0, max_steps=10):
                       f"FP64 oracle (C++/OpenMP restatement of the reference CPU step), {threads} threads"}
 
 
-def timed(scene, lib, precision, steps, warmup, timing_steps=50):
-    """Throughput of `steps` substeps (HIP-graph replays, no events inside), then
+def timed(scene, lib, precision, steps, warmup, timing_steps=50, autotune=True):
+    """Launch configuration from the autotuner (state left unchanged), then
+    throughput of `steps` substeps (HIP-graph replays, no events inside), then
     per-stage kernel times from HIP events on the handle's stream over the next
     min(steps, timing_steps) substeps launched one kernel at a time."""
     sim = scene.make(lib, precision)
+    timed.launch_config = sim.autotune(scene.dt, 0.0, steps=4) if autotune else sim.launch_config()
     sim.run(warmup, scene.dt, 0.0)
     sim.synchronize()
     t_start = warmup * scene.dt
@@ -137,6 +139,7 @@ def main():
     ap.add_argument("--workload", default="damage_slab")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain MiB-sized device copy (traffic-counter calibration)")
     args = ap.parse_args()
@@ -155,7 +158,8 @@ def main():
     scene = scenes.WORKLOADS[args.workload]()
     damage = any(m.has_damage for m in scene.materials)
 
-    el32, st32, nn32, out32 = timed(scene, lib, A.PREC_FP32, args.steps, args.warmup)
+    el32, st32, nn32, out32 = timed(scene, lib, A.PREC_FP32, args.steps, args.warmup, autotune=not args.no_autotune)
+    cfg32 = timed.launch_config
     value = scene.n * args.steps / el32
     result = {
         "metric": METRIC,
@@ -171,7 +175,8 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": args.workload, "particles": scene.n, "dx": scene.dx, "dt": scene.dt,
-                   "form": "FP32 particle state, int64 fixed-point P2G", "active_nodes": nn32},
+                   "form": "FP32 particle state, int64 fixed-point P2G", "active_nodes": nn32,
+                   "launch": cfg32, "autotuned": not args.no_autotune},
     }
     # roofline of the dominant kernel stage (HIP events on the handle's stream)
     stage_ms = {k: v for k, v in st32.items() if k in ("p2g", "g2p", "grid_update")}
@@ -188,14 +193,15 @@ def main():
                           "algorithmic_bytes": byt, "avg_ms": stage_ms[dom]}
     result["stage_ms"] = st32
     if not args.no_fp64:
-        el64, st64, nn64, out64 = timed(scene, lib, A.PREC_FP64, args.steps, args.warmup)
+        el64, st64, nn64, out64 = timed(scene, lib, A.PREC_FP64, args.steps, args.warmup,
+                                        autotune=not args.no_autotune)
         v64 = scene.n * args.steps / el64
         errs = {}
         for k in ("x", "v", "C", "F", "d"):
             den = np.abs(out64[k]).max()
             errs[k] = float(np.abs(out32[k] - out64[k]).max() / den) if den > 0 else float(np.abs(out32[k]).max())
         result["fp64_plain"] = {"value": v64, "ms_per_step": 1e3 * el64 / args.steps, "stage_ms": st64,
-                                "dtype": "fp64"}
+                                "dtype": "fp64", "launch": timed.launch_config}
         result["fp32_over_fp64"] = value / v64
         result["fp32_vs_fp64_normwise_err"] = {"substeps": args.warmup + args.steps + min(args.steps, 50), **errs}
         result["fp32_vs_fp64_after_100_substeps"] = accuracy_after(scene, lib, 100)

@@ -217,6 +217,35 @@ zmpm_status zmpm_set_timing(zmpm_handle h, int32_t enable);
  * names[] receives static strings.  The oracle reports none. */
 zmpm_status zmpm_get_timings(zmpm_handle h, int32_t* count, const char** names, double* ms);
 
+/* Launch configuration of the grid-stride kernels (workgroups of 256 threads)
+ * and the P2G lane split.  The workgroup counts do not change any result; the
+ * P2G split cap (most lanes sharing one stencil column of a cell) changes how
+ * a cell's particles are grouped into float partial sums before the 64-bit
+ * fixed-point accumulation (results deterministic for a given value).
+ * No counterpart in the reference (launch configuration of this build). */
+typedef struct {
+    int32_t p2g_wgs;     /* P2G: workgroups over grid blocks                      */
+    int32_t sort_wgs;    /* in-block sort: workgroups over grid blocks            */
+    int32_t grid_wgs;    /* grid update: workgroups over nodes                    */
+    int32_t prep_wgs;    /* accumulator zeroing / neighbour table                 */
+    int32_t p2g_split;   /* P2G lanes per stencil column, cap (>= 1)              */
+    int32_t reserved[3];
+} zmpm_launch_config;
+
+zmpm_status zmpm_get_launch_config(zmpm_handle h, zmpm_launch_config* cfg);
+zmpm_status zmpm_set_launch_config(zmpm_handle h, const zmpm_launch_config* cfg);
+
+/* Autotuner: times candidate workgroup counts for each grid-stride kernel on
+ * the uploaded scene (coordinate descent, `steps` event-timed substeps per
+ * candidate, colliders at times from t0) and keeps the fastest; with
+ * tune_split != 0 also the P2G split cap.  The particle state is saved before
+ * and restored after, so the simulation continues from where it was (the grid
+ * and binning queries then describe the tuner's last substep).  The chosen
+ * configuration is returned in *cfg (may be NULL).
+ * No counterpart in the reference (launch configuration of this build). */
+zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, int32_t tune_split,
+                          zmpm_launch_config* cfg);
+
 #ifdef __cplusplus
 }
 #endif

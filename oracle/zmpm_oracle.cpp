@@ -847,6 +847,7 @@ struct zmpm_context {
     zmpm_scene_desc desc;
     Layout L;
     std::string err;
+    zmpm_launch_config launch { 1, 1, 1, 1, 1, { 0, 0, 0 } };
     std::vector<zmpm_material> mats;
     std::vector<Collider> colliders;
     bool any_damage = false;
@@ -1478,6 +1479,31 @@ zmpm_status zmpm_get_timings(zmpm_handle h, int32_t* count, const char**, double
 {
     if (!h || !count) return ZMPM_E_ARG;
     *count = 0;
+    return ZMPM_OK;
+}
+
+// Launch configuration: the oracle has no launches; it keeps and returns the
+// values it is given (all ones by default) so callers can run unchanged.
+zmpm_status zmpm_get_launch_config(zmpm_handle h, zmpm_launch_config* cfg)
+{
+    if (!h || !cfg) return ZMPM_E_ARG;
+    *cfg = h->launch;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_set_launch_config(zmpm_handle h, const zmpm_launch_config* cfg)
+{
+    if (!h || !cfg) return ZMPM_E_ARG;
+    if (cfg->p2g_wgs < 1 || cfg->sort_wgs < 1 || cfg->grid_wgs < 1 || cfg->prep_wgs < 1 || cfg->p2g_split < 1)
+        return ZMPM_E_ARG;
+    h->launch = *cfg;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_autotune(zmpm_handle h, double dt, double, int32_t steps, int32_t, zmpm_launch_config* cfg)
+{
+    if (!h || !(dt > 0) || steps < 1) return ZMPM_E_ARG;
+    if (cfg) *cfg = h->launch;
     return ZMPM_OK;
 }
 

@@ -49,8 +49,9 @@ def test_struct_layouts_match_header():
     import subprocess
     import tempfile
     from ziran2020_amd import _abi as A
-    src = ('#include "zmpm.h"\n#include <stdio.h>\nint main(){printf("%zu %zu %zu %zu\\n", sizeof(zmpm_scene_desc),'
-           ' sizeof(zmpm_material), sizeof(zmpm_collider), sizeof(zmpm_particles)); return 0;}\n')
+    src = ('#include "zmpm.h"\n#include <stdio.h>\nint main(){printf("%zu %zu %zu %zu %zu\\n", sizeof(zmpm_scene_desc),'
+           ' sizeof(zmpm_material), sizeof(zmpm_collider), sizeof(zmpm_particles), sizeof(zmpm_launch_config));'
+           ' return 0;}\n')
     with tempfile.TemporaryDirectory() as d:
         c = os.path.join(d, "s.c")
         open(c, "w").write(src)
@@ -58,7 +59,7 @@ def test_struct_layouts_match_header():
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
         sizes = list(map(int, subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()))
     assert sizes == [ctypes.sizeof(A.SceneDesc), ctypes.sizeof(A.Material), ctypes.sizeof(A.Collider),
-                     ctypes.sizeof(A.Particles)]
+                     ctypes.sizeof(A.Particles), ctypes.sizeof(A.LaunchConfig)]
 
 
 def test_library_loads_without_gpu_and_fails_loudly():

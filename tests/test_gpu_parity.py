@@ -299,3 +299,29 @@ def test_material_change_after_upload_restresses(gpu):
     b = _fields(run(gpu, sc, A.PREC_FP32, 3).download())
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_launch_config_does_not_change_results(gpu):
+    # the workgroup counts of the grid-stride kernels are free: FP32 bit-exact
+    sc = small_slab(n=8000)
+    a = _fields(run(gpu, sc, A.PREC_FP32, 4).download())
+    sim = sc.make(gpu, A.PREC_FP32)
+    sim.set_launch_config(p2g_wgs=37, sort_wgs=13, grid_wgs=5, prep_wgs=3)
+    assert sim.launch_config()["p2g_wgs"] == 37
+    sim.run(4, sc.dt, 0.0)
+    b = _fields(sim.download())
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_autotune_restores_the_state(gpu):
+    # tuning runs substeps on the live scene, then puts the state back
+    sc = small_slab(n=8000)
+    sim = sc.make(gpu, A.PREC_FP32)
+    cfg = sim.autotune(sc.dt, 0.0, steps=2)
+    assert all(v >= 1 for v in cfg.values())
+    sim.run(4, sc.dt, 0.0)
+    a = _fields(sim.download())
+    b = _fields(run(gpu, sc, A.PREC_FP32, 4).download())
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k

@@ -133,6 +133,15 @@ def box(box_min, box_max, type_=STICKY, b0=(0, 0, 0), dbdt=(0, 0, 0), friction=0
     return c
 
 
+class LaunchConfig(C.Structure):
+    """zmpm_launch_config (include/zmpm.h)."""
+    _fields_ = [("p2g_wgs", C.c_int32), ("sort_wgs", C.c_int32), ("grid_wgs", C.c_int32),
+                ("prep_wgs", C.c_int32), ("p2g_split", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k in ("p2g_wgs", "sort_wgs", "grid_wgs", "prep_wgs", "p2g_split")}
+
+
 class ZmpmLib:
     """Bindings of one shared library exporting the zmpm C ABI."""
 
@@ -140,7 +149,8 @@ class ZmpmLib:
         "zmpm_abi_version", "zmpm_build_info", "zmpm_create", "zmpm_destroy", "zmpm_last_error",
         "zmpm_set_materials", "zmpm_add_collider", "zmpm_clear_colliders", "zmpm_upload",
         "zmpm_download", "zmpm_substep", "zmpm_run", "zmpm_synchronize", "zmpm_get_binning",
-        "zmpm_get_grid", "zmpm_set_timing", "zmpm_get_timings",
+        "zmpm_get_grid", "zmpm_set_timing", "zmpm_get_timings", "zmpm_get_launch_config",
+        "zmpm_set_launch_config", "zmpm_autotune",
     )
 
     def __init__(self, path: str):
@@ -172,6 +182,10 @@ class ZmpmLib:
         L.zmpm_get_grid.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), _DP, _DP, _DP, _DP]
         L.zmpm_set_timing.argtypes = [C.c_void_p, C.c_int32]
         L.zmpm_get_timings.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_char_p), _DP]
+        L.zmpm_get_launch_config.argtypes = [C.c_void_p, C.POINTER(LaunchConfig)]
+        L.zmpm_set_launch_config.argtypes = [C.c_void_p, C.POINTER(LaunchConfig)]
+        L.zmpm_autotune.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                    C.POINTER(LaunchConfig)]
         if L.zmpm_abi_version() != ZMPM_ABI_VERSION:
             raise RuntimeError(f"{path}: ABI version {L.zmpm_abi_version()} != {ZMPM_ABI_VERSION}")
 
@@ -302,6 +316,24 @@ class Simulation:
         self._check(self.L.zmpm_get_grid(self.h, C.byref(N), _ptr(out["coords"], C.c_int32), _ptr(out["m"]),
                                          _ptr(out["v"]), _ptr(out["new_v"]), _ptr(out["pf"])), "zmpm_get_grid")
         return out
+
+    def launch_config(self):
+        cfg = LaunchConfig()
+        self._check(self.L.zmpm_get_launch_config(self.h, C.byref(cfg)), "zmpm_get_launch_config")
+        return cfg.as_dict()
+
+    def set_launch_config(self, **kw):
+        cfg = LaunchConfig()
+        self._check(self.L.zmpm_get_launch_config(self.h, C.byref(cfg)), "zmpm_get_launch_config")
+        for k, v in kw.items():
+            setattr(cfg, k, int(v))
+        self._check(self.L.zmpm_set_launch_config(self.h, C.byref(cfg)), "zmpm_set_launch_config")
+
+    def autotune(self, dt, t0=0.0, steps=4, tune_split=False):
+        """Tune the launch configuration on this scene (advances the simulation)."""
+        cfg = LaunchConfig()
+        self._check(self.L.zmpm_autotune(self.h, dt, t0, steps, int(bool(tune_split)), C.byref(cfg)), "zmpm_autotune")
+        return cfg.as_dict()
 
     def timings(self):
         cnt = C.c_int32(64)

@@ -543,6 +543,31 @@ zmpm_status gridT(zmpm_context* h, int64_t* n_nodes, int32_t* coords, double* m,
 } // namespace
 
 // ================================================================= C ABI
+namespace {
+// Device state a substep reads or leaves for the next one: live particle
+// buffers, the binning the last G2P (or the upload) did (keys, slots in the
+// pages, page flags and counts), bound slots, scales, counters and clock.
+template <class T, class AccT>
+std::vector<std::pair<void*, size_t>> liveState(zmpm_context* h)
+{
+    Dev<T, AccT>& D = devOf<T, AccT>(h);
+    return { { D.P, (size_t)NPF * D.cap * sizeof(T) },      { D.pid, (size_t)D.cap * sizeof(int) },
+             { D.pmat, (size_t)D.cap * sizeof(int) },        { D.key, (size_t)D.cap * sizeof(uint64_t) },
+             { D.rank, (size_t)D.cap * sizeof(int) },
+             { D.touched, (size_t)D.num_pages * sizeof(int) }, { D.page_count, (size_t)D.num_pages * sizeof(int) },
+             { D.maxbits, (size_t)NSC * BOUND_SLOTS * sizeof(unsigned) }, { D.scales, NSC * sizeof(float) },
+             { D.counters, 8 * sizeof(int) },                { D.clock, 2 * sizeof(double) } };
+}
+
+template <class T, class AccT>
+void clearScanState(zmpm_context* h)
+{
+    Dev<T, AccT>& D = devOf<T, AccT>(h);
+    (void)hipMemsetAsync(D.scan_state, 0, (D.num_pages + PSCAN_TILE - 1) / PSCAN_TILE * sizeof(uint64_t), h->stream);
+}
+
+} // namespace
+
 extern "C" {
 
 int32_t zmpm_abi_version(void) { return ZMPM_ABI_VERSION; }
@@ -823,6 +848,123 @@ zmpm_status zmpm_get_timings(zmpm_handle h, int32_t* count, const char** names, 
         if (ms) ms[g] = h->timed_steps ? h->timed_ms[g] / h->timed_steps : 0.0;
     }
     *count = k;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_get_launch_config(zmpm_handle h, zmpm_launch_config* cfg)
+{
+    if (!h || !cfg) return ZMPM_E_ARG;
+    *cfg = zmpm_launch_config {};
+    cfg->p2g_wgs = h->p2g_wgs;
+    cfg->sort_wgs = h->sort_wgs;
+    cfg->grid_wgs = h->grid_wgs;
+    cfg->prep_wgs = h->prep_wgs;
+    cfg->p2g_split = h->p2g_split;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_set_launch_config(zmpm_handle h, const zmpm_launch_config* cfg)
+{
+    if (!h || !cfg) return ZMPM_E_ARG;
+    if (cfg->p2g_wgs < 1 || cfg->sort_wgs < 1 || cfg->grid_wgs < 1 || cfg->prep_wgs < 1 || cfg->p2g_split < 1)
+        return fail(h, ZMPM_E_ARG, "launch configuration entries must be >= 1");
+    (void)hipSetDevice(h->desc.device);
+    dropGraphs(h);
+    h->p2g_wgs = cfg->p2g_wgs;
+    h->sort_wgs = cfg->sort_wgs;
+    h->grid_wgs = cfg->grid_wgs;
+    h->prep_wgs = cfg->prep_wgs;
+    h->p2g_split = cfg->p2g_split;
+    if (h->fp32)
+        refreshTables<float, long long>(h);
+    else
+        refreshTables<double, double>(h);
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, int32_t tune_split,
+                          zmpm_launch_config* cfg)
+{
+    if (!h || !(dt > 0) || steps < 1) return ZMPM_E_ARG;
+    if (h->n == 0) return cfg ? zmpm_get_launch_config(h, cfg) : ZMPM_OK;
+    (void)hipSetDevice(h->desc.device);
+    // snapshot of the state so tuning leaves the simulation where it was
+    const auto state = h->fp32 ? liveState<float, long long>(h) : liveState<double, double>(h);
+    std::vector<void*> saved(state.size(), nullptr);
+    for (size_t i = 0; i < state.size(); i++) {
+        if (hipMalloc(&saved[i], state[i].second) != hipSuccess) {
+            (void)hipGetLastError();
+            for (void* q : saved)
+                if (q) (void)hipFree(q);
+            return fail(h, ZMPM_E_NOMEM, "autotune: no memory for the state snapshot");
+        }
+        HIPCHK(h, hipMemcpyAsync(saved[i], state[i].first, state[i].second, hipMemcpyDeviceToDevice, h->stream));
+    }
+    const bool timing = h->timing, stepped = h->stepped;
+    // substeps swap the live and output particle buffers: keep which is live
+    const Dev<float, long long> df = h->Df;
+    const Dev<double, double> dd = h->Dd;
+    double t = t0;
+    // stage (0 bin/sort, 1 p2g, 2 grid update) each knob is charged with
+    struct Knob {
+        int* value;
+        int stage;
+        std::vector<int> candidates;
+    };
+    std::vector<Knob> knobs = {
+        { &h->p2g_wgs, 1, { 512, 1024, 2048, 4096 } },
+        { &h->sort_wgs, 0, { 512, 1024, 2048, 4096 } },
+        { &h->prep_wgs, 0, { 256, 512, 1024, 2048 } },
+        { &h->grid_wgs, 2, { 256, 512, 1024, 2048, 4096 } },
+    };
+    if (tune_split) knobs.push_back({ &h->p2g_split, 1, { 1, 2, 4, 64 } });
+    h->timing = true;
+    zmpm_status st = ZMPM_OK;
+    // one untimed substep first (first-touch of the configuration's launches)
+    if ((st = runSteps(h, 1, dt, t)) != ZMPM_OK) goto done;
+    t += dt;
+    for (Knob& k : knobs) {
+        int best = *k.value;
+        double best_ms = 1e300;
+        for (int c : k.candidates) {
+            *k.value = c;
+            if (h->fp32)
+                refreshTables<float, long long>(h);
+            else
+                refreshTables<double, double>(h);
+            if ((st = runSteps(h, steps, dt, t)) != ZMPM_OK) goto done;
+            t += steps * dt;
+            const double ms = h->timed_ms[k.stage] / h->timed_steps;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        *k.value = best;
+    }
+done:
+    h->timing = timing;
+    if (st == ZMPM_OK) st = deviceStatus(h, true);
+    // restore the state (into the buffers live before tuning); page-scan tile
+    // words of the tuning substeps are cleared
+    h->Df = df;
+    h->Dd = dd;
+    for (size_t i = 0; i < state.size(); i++)
+        (void)hipMemcpyAsync(state[i].first, saved[i], state[i].second, hipMemcpyDeviceToDevice, h->stream);
+    if (h->fp32)
+        clearScanState<float, long long>(h);
+    else
+        clearScanState<double, double>(h);
+    (void)hipStreamSynchronize(h->stream);
+    for (void* q : saved) (void)hipFree(q);
+    h->stepped = stepped;
+    if (h->fp32)
+        refreshTables<float, long long>(h);
+    else
+        refreshTables<double, double>(h);
+    dropGraphs(h);
+    if (st != ZMPM_OK) return st;
+    if (cfg) return zmpm_get_launch_config(h, cfg);
     return ZMPM_OK;
 }
 
