@@ -15,9 +15,12 @@
  * so bit-level agreement with Eigen's evaluation order is not claimed; the
  * algorithms, operation order inside each routine, accumulation order of P2G
  * (8-colour block schedule, sorted particle order) and all branch conventions
- * follow the reference.  Arithmetic is double; when the handle is created in
- * the FP32 layout the binning arithmetic (x * (1/dx) - 0.5, floor) is done in
- * float, as MpmSimulationBase<float,3> would, so keys can be compared bit-exactly.
+ * follow the reference.  Arithmetic is double.  When the handle is created in
+ * the FP32 layout (4x4x4 blocks), positions are held on the GPU library's FP32
+ * position grid -- x / dx in units of 2^-21, rounded at upload and after every
+ * advection -- and the base node is the exact floor(x / dx - 1/2) of that
+ * fixed-point value, as the FP32 form computes it, so keys can be compared
+ * bit-exactly.
  *
  * Compiled with -ffp-contract=off: the reference's base-node flooring must not
  * be FMA-contracted (MpmSimulationBase.cpp:1545).
@@ -794,11 +797,6 @@ inline int intFloor(double x)
     int i = (int)x;
     return i - (i > x);
 }
-inline int intFloorF(float x)
-{
-    int i = (int)x;
-    return i - (i > x);
-}
 
 // Quadratic B-spline (BSplines.h:72-98): base node, weights, gradients, Hessian diag.
 inline void bspline(double x, int& base, double* w, double* dw, double* ddw)
@@ -878,6 +876,22 @@ zmpm_status fail(zmpm_context* h, zmpm_status s, const std::string& msg)
     return s;
 }
 
+// FP32 layout position grid: x / dx in units of 2^-kPosBits, clamped to
+// [-2^30, 2^31 - 2] units (outside the domain either way).
+constexpr int kPosBits = 21;
+
+inline double posUnits(double x, double dx)
+{
+    const double u = std::nearbyint(x / dx * (double)(1 << kPosBits));
+    return std::min(std::max(u, -1073741824.0), 2147483646.0);
+}
+
+inline double quantizePos(double x, double dx)
+{
+    if (!std::isfinite(x)) return x;
+    return posUnits(x, dx) * dx / (double)(1 << kPosBits);
+}
+
 // Base node of particle i in the handle's precision arithmetic; false if the
 // node leaves the addressable domain.
 bool baseNodeOf(const zmpm_context* h, int64_t i, int* b, bool* nan_out)
@@ -893,11 +907,10 @@ bool baseNodeOf(const zmpm_context* h, int64_t i, int* b, bool* nan_out)
         }
         double s;
         if (h->desc.precision == ZMPM_PREC_FP32) {
-            float odx = (float)1 / (float)h->desc.dx;
-            float xs = (float)xv * odx;
-            float t = xs - 0.5f;
-            if (!(t > -1e6f && t < 1e6f)) { ok = false; b[a] = 0; continue; }
-            b[a] = intFloorF(t);
+            // floor(X / 2^21 - 1/2) of the integer X, exactly
+            const int64_t X = (int64_t)posUnits(xv, h->desc.dx);
+            const int64_t t = X - (int64_t(1) << (kPosBits - 1));
+            b[a] = (int)((t >= 0) ? (t >> kPosBits) : -((-t + (int64_t(1) << kPosBits) - 1) >> kPosBits));
         }
         else {
             double odx = 1 / h->desc.dx;
@@ -914,10 +927,7 @@ bool baseNodeOf(const zmpm_context* h, int64_t i, int* b, bool* nan_out)
 // Index-space position in the handle's precision (BSplineWeights::compute, MpmGrid.h:74-79).
 inline double indexSpace(const zmpm_context* h, double xv)
 {
-    if (h->desc.precision == ZMPM_PREC_FP32) {
-        float odx = (float)1 / (float)h->desc.dx;
-        return (double)((float)xv * odx);
-    }
+    if (h->desc.precision == ZMPM_PREC_FP32) return posUnits(xv, h->desc.dx) / (double)(1 << kPosBits);
     return xv * (1 / h->desc.dx);
 }
 
@@ -1009,7 +1019,7 @@ struct Stencil {
 
 inline void makeStencil(const zmpm_context* h, const double* xp, Stencil& s)
 {
-    s.odx = (h->desc.precision == ZMPM_PREC_FP32) ? (double)((float)1 / (float)h->desc.dx) : 1 / h->desc.dx;
+    s.odx = 1 / h->desc.dx;
     for (int a = 0; a < 3; a++) bspline(indexSpace(h, xp[a]), s.base[a], s.w[a], s.dw[a], s.ddw[a]);
 }
 
@@ -1238,6 +1248,7 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
     const double D_inverse = 4 / dx2;
     const double r = h->desc.apic_rpic_ratio;
     int cfl_flag = 0;
+    const bool fp32_layout = h->desc.precision == ZMPM_PREC_FP32;
     std::vector<double> gradV((size_t)n * 9);
 #pragma omp parallel for schedule(static) reduction(| : cfl_flag)
     for (int64_t i = 0; i < n; i++) {
@@ -1266,6 +1277,8 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
             }
         double inc[3] = { dt * picV[0], dt * picV[1], dt * picV[2] };
         for (int a = 0; a < 3; a++) Xp[a] += inc[a];
+        if (fp32_layout)
+            for (int a = 0; a < 3; a++) Xp[a] = quantizePos(Xp[a], dx);
         double incn = inc[0] * inc[0] + inc[1] * inc[1] + inc[2] * inc[2];
         if (incn > dx2) cfl_flag |= 1;
     }
@@ -1361,6 +1374,8 @@ zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
         if (p->material[i] < 0 || p->material[i] >= (int32_t)h->mats.size()) return fail(h, ZMPM_E_ARG, "material index out of range");
     h->n = n;
     h->x.assign(p->x, p->x + 3 * n);
+    if (h->desc.precision == ZMPM_PREC_FP32)
+        for (double& xv : h->x) xv = quantizePos(xv, h->desc.dx);
     h->v.assign(p->v, p->v + 3 * n);
     h->C.assign(p->C, p->C + 9 * n);
     h->F.assign(p->F, p->F + 9 * n);

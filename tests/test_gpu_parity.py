@@ -215,10 +215,11 @@ def test_domain_and_nan_rejected(gpu):
     sc.x[3] = (-0.5, 0.3, 0.3)
     sim = sc.make(gpu, A.PREC_FP32)
     assert step_status(sim, sc.dt) == A.E_DOMAIN
-    sc = small_cube(cells=4)
-    sc.x[5, 1] = np.nan
-    sim = sc.make(gpu, A.PREC_FP64)
-    assert step_status(sim, sc.dt) == A.E_NAN
+    for prec in (A.PREC_FP64, A.PREC_FP32):
+        sc = small_cube(cells=4)
+        sc.x[5, 1] = np.nan
+        sim = sc.make(gpu, prec)
+        assert step_status(sim, sc.dt) == A.E_NAN
 
 
 def test_full_size_damage_slab_conservation(gpu):

@@ -448,10 +448,11 @@ def test_sort_matches_brute_force(oracle):
         masks = (C.c_uint64 * 3)()
         oracle.lib.zmpm_oracle_masks(log2, masks)
         if prec == A.PREC_FP32:
-            xs = (sc.x.astype(np.float32) * np.float32(np.float32(1) / np.float32(sc.dx))) - np.float32(0.5)
+            # FP32 layout: x / dx on a 2^-21 fixed-point grid, floored exactly
+            X = np.rint(sc.x / sc.dx * 2.0 ** 21).astype(np.int64)
+            base = np.floor_divide(X - 2 ** 20, 2 ** 21)
         else:
-            xs = sc.x * (1 / sc.dx) - 0.5
-        base = np.floor(xs).astype(int)
+            base = np.floor(sc.x * (1 / sc.dx) - 0.5).astype(int)
         block_bits = 12 - log2
         keys = []
         for i in range(n):

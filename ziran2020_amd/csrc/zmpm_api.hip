@@ -4,8 +4,10 @@
 #include "zmpm_device.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -212,6 +214,7 @@ void refreshTables(zmpm_context* h)
     D.any_damage = h->any_damage ? 1 : 0;
     D.dx = (T)h->desc.dx;
     D.odx = T(1) / (T)h->desc.dx;
+    D.pos_scale = (T)((double)(1 << POS_BITS) / h->desc.dx);
     D.Dinv = T(4) / ((T)h->desc.dx * (T)h->desc.dx);
     for (int a = 0; a < 3; a++) D.gravity[a] = (T)h->desc.gravity[a];
     D.rpic = (T)h->desc.apic_rpic_ratio;
@@ -328,6 +331,24 @@ zmpm_status deviceStatus(zmpm_context* h, bool sync)
     return ZMPM_OK;
 }
 
+// FP32-form position in fixed point, X = round(x / dx * 2^POS_BITS): POS_NAN
+// for a non-finite x, clamped to [POS_MIN, POS_MAX] (outside the domain either
+// way, which the first substep reports).
+int encodePos(double x, double dx)
+{
+    if (!std::isfinite(x)) return POS_NAN;
+    const double u = std::nearbyint(x / dx * (double)(1 << POS_BITS));
+    if (!(u >= (double)POS_MIN)) return POS_MIN;
+    if (!(u <= (double)POS_MAX)) return POS_MAX;
+    return (int)u;
+}
+
+double decodePos(int X, double dx)
+{
+    if (X == POS_NAN) return std::numeric_limits<double>::quiet_NaN();
+    return (double)X * dx / (double)(1 << POS_BITS);
+}
+
 template <class T, class AccT>
 zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
 {
@@ -340,15 +361,22 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
     h->n = n;
     refreshTables<T, AccT>(h);
     const int64_t cap = D.cap;
+    constexpr bool F32 = std::is_same<T, float>::value;
     std::vector<T> buf((size_t)NPF * cap, T(0));
     for (int i = 0; i < n; i++) {
         for (int a = 0; a < 3; a++) {
-            buf[(PX + a) * cap + i] = (T)p->x[3 * i + a];
+            if constexpr (F32) {
+                const int X = encodePos(p->x[3 * i + a], h->desc.dx);
+                std::memcpy(&buf[(PX + a) * cap + i], &X, sizeof(int));
+            }
+            else
+                buf[(PX + a) * cap + i] = (T)p->x[3 * i + a];
             buf[(PV + a) * cap + i] = (T)p->v[3 * i + a];
         }
         for (int k = 0; k < 9; k++) {
             buf[(PC + k) * cap + i] = (T)p->C[9 * i + k];
-            buf[(PF + k) * cap + i] = (T)p->F[9 * i + k];
+            // FP32 form: H = F - I, formed in double
+            buf[(PF + k) * cap + i] = F32 ? (T)(p->F[9 * i + k] - (k % 4 == 0 ? 1.0 : 0.0)) : (T)p->F[9 * i + k];
         }
         buf[PM * cap + i] = (T)p->m[i];
         buf[PVOL * cap + i] = (T)p->V0[i];
@@ -409,12 +437,22 @@ zmpm_status downloadT(zmpm_context* h, zmpm_particles* p)
         const int i = ids[s];
         if (i < 0 || i >= n) return fail(h, ZMPM_E_HIP, "corrupt particle id table");
         for (int a = 0; a < 3; a++) {
-            if (p->x) p->x[3 * i + a] = (double)buf[(PX + a) * cap + s];
+            if (p->x) {
+                if constexpr (std::is_same<T, float>::value) {
+                    int X;
+                    std::memcpy(&X, &buf[(PX + a) * cap + s], sizeof(int));
+                    p->x[3 * i + a] = decodePos(X, h->desc.dx);
+                }
+                else
+                    p->x[3 * i + a] = (double)buf[(PX + a) * cap + s];
+            }
             if (p->v) p->v[3 * i + a] = (double)buf[(PV + a) * cap + s];
         }
         for (int k = 0; k < 9; k++) {
             if (p->C) p->C[9 * i + k] = (double)buf[(PC + k) * cap + s];
-            if (p->F) p->F[9 * i + k] = (double)buf[(PF + k) * cap + s];
+            if (p->F)
+                p->F[9 * i + k] = (double)buf[(PF + k) * cap + s]
+                    + (std::is_same<T, float>::value && k % 4 == 0 ? 1.0 : 0.0);
         }
         if (p->m) p->m[i] = (double)buf[PM * cap + s];
         if (p->V0) p->V0[i] = (double)buf[PVOL * cap + s];

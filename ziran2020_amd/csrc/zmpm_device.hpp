@@ -50,6 +50,55 @@ constexpr long long FIXED_MAGIC_BITS = 0x4338000000000000LL;
 
 __device__ __forceinline__ void latch(int* counters, int bits) { atomicOr(&counters[1], bits); }
 
+// ---------------------------------------------------------------- positions
+__device__ __forceinline__ int loadPos(const float* P, int64_t i) { return __float_as_int(P[i]); }
+__device__ __forceinline__ double loadPos(const double* P, int64_t i) { return P[i]; }
+__device__ __forceinline__ void storePos(float* P, int64_t i, int X) { P[i] = __int_as_float(X); }
+__device__ __forceinline__ void storePos(double* P, int64_t i, double x) { P[i] = x; }
+
+// Base node of one axis, floor(x / dx - 1/2) (baseNode<2>, BSplines.h:16-20;
+// the key build of MpmSimulationBase.cpp:1545), and the offset from it in
+// cells, d0 = x / dx - base in [0.5, 1.5).  Returns ST_NAN / ST_DOMAIN when the
+// position is not finite or the base node leaves [0, domain - 4); base is then
+// clamped into that range (and d0 set to 1) so callers index inside the grid.
+// FP64 form: x * (1/dx) - 0.5 in double without FMA, truncated (int_floor for
+// t >= 0, MathTools.h:15-25).
+__device__ __forceinline__ int cellOf(double x, double odx, int domain, int& base, double& d0)
+{
+    const double xs = mul_rn(x, odx);
+    double t = sub_rn(xs, 0.5);
+    int bits = 0;
+    if (!isfinite(t)) {
+        bits = ST_NAN;
+        t = 0.0;
+    }
+    const double hi = double(domain - 4);
+    if (!(t >= 0.0 && t < hi)) {
+        bits |= ST_DOMAIN;
+        t = (t < 0.0) ? 0.0 : hi - 1.0;
+    }
+    base = (int)t;
+    d0 = bits ? 1.0 : xs - double(base);
+    return bits;
+}
+
+// FP32 form: the same floor, exact on the fixed-point position; d0 has at most
+// POS_BITS + 1 significant bits, so it is exact in float too.
+__device__ __forceinline__ int cellOf(int X, float, int domain, int& base, float& d0)
+{
+    const int bits0 = (X == POS_NAN) ? ST_NAN : 0;
+    int bits = bits0;
+    const int t = bits0 ? 0 : X - POS_HALF;
+    int b = t >> POS_BITS; // arithmetic shift: floor
+    if (!(b >= 0 && b < domain - 4)) {
+        bits |= ST_DOMAIN;
+        b = (b < 0) ? 0 : domain - 5;
+    }
+    base = b;
+    d0 = bits ? 1.f : (float)(X - (b << POS_BITS)) * (1.f / (float)(1 << POS_BITS));
+    return bits;
+}
+
 // Every third bit: b9..b0 -> bit 3*i (block coordinates < 1024).
 __device__ __forceinline__ uint32_t part1by2(uint32_t x)
 {
@@ -178,7 +227,7 @@ __device__ __forceinline__ uint64_t scanPair(uint64_t w)
 // runs inside a wavefront: one count atomic and one activation per run.  Every
 // lane of the wavefront calls it (active or not).
 template <class T, class AccT>
-__device__ __forceinline__ void binParticle(const Dev<T, AccT>& D, int p, bool active, const T* x)
+__device__ __forceinline__ void binParticle(const Dev<T, AccT>& D, int p, bool active, const PosT<T>* x)
 {
     using B = Blk<T>;
     const int lane = threadIdx.x & 63;
@@ -186,19 +235,10 @@ __device__ __forceinline__ void binParticle(const Dev<T, AccT>& D, int p, bool a
     int bb[3] = { 0, 0, 0 };
     if (active) {
         int bits = 0;
-        const T hi = T(D.domain - 4);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            T t = sub_rn(mul_rn(x[a], D.odx), T(0.5));
-            if (!isfinite(t)) {
-                bits |= ST_NAN;
-                t = T(0);
-            }
-            if (!(t >= T(0) && t < hi)) {
-                bits |= ST_DOMAIN;
-                t = (t < T(0)) ? T(0) : hi - T(1);
-            }
-            bb[a] = (int)t; // t >= 0: truncation is int_floor (MathTools.h:15-25)
+            T d0;
+            bits |= cellOf(x[a], D.odx, D.domain, bb[a], d0);
         }
         if (bits) latch(D.counters, bits);
         uint32_t elem;
@@ -556,9 +596,34 @@ __device__ __forceinline__ Mat3<T> loadM(const T* P, int64_t cap, int f, int p)
 // kirchhoff = P F^T (HyperelasticConstitutiveModel.h:75-80), or
 // JBasedMpmForceHelper::updateState (JBasedMpmForceHelper.cpp:28-50).  tau is
 // symmetric for every model on the path; its rounding-level skew is dropped.
-template <class T>
-__device__ __forceinline__ void vtauSym(const MatDev<T>& mt, const Mat3<T>& F, T V0, T g, T* S)
+// Fs is the stored deformation: F (FP64 form) or H = F - I (FP32 form, whose
+// EOS particles hold J - 1 in H_00; the EOS pressure is then
+// -k expm1(-gamma log1p(J - 1)) J).
+__device__ __forceinline__ void vtauSym(const MatDev<float>& mt, const Mat3<float>& H, float V0, float g, float* S)
 {
+    float t6[6];
+    if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE) {
+        const float jm1 = H.a[0][0];
+        const float p = V0 * (-mt.bulk * expm1f(-mt.gamma * log1pf(jm1))) * (1.f + jm1);
+        S[0] = S[3] = S[5] = p;
+        S[1] = S[2] = S[4] = 0.f;
+        return;
+    }
+    if (mt.model == ZMPM_MODEL_FIXED_COROTATED)
+        corotatedTauH(H, mt.mu, mt.lambda, t6);
+    else if (mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
+        float J;
+        qrStableTauH(H, mt.mu, mt.lambda, g, mt.kx, mt.ky, t6, &J);
+    }
+    else
+        neoHookeanTauH(H, mt.mu, mt.lambda, t6);
+#pragma unroll
+    for (int i = 0; i < 6; i++) S[i] = V0 * t6[i];
+}
+
+__device__ __forceinline__ void vtauSym(const MatDev<double>& mt, const Mat3<double>& F, double V0, double g, double* S)
+{
+    using T = double;
     T s0, s1, s2, s3, s4, s5;
     if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE) {
         const T J = F.a[0][0];
@@ -670,7 +735,7 @@ __global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D, int bin)
     for (int base = blockIdx.x * WG; base < D.n; base += gridDim.x * WG) {
         const int p = base + threadIdx.x;
         const bool active = p < D.n;
-        T x[3] = { T(0), T(0), T(0) };
+        PosT<T> x[3] = { 0, 0, 0 };
         if (active) {
             const MatDev<T>& mt = D.mats[D.pmat[p]];
             Mat3<T> F = loadM(D.P, cap, PF, p);
@@ -681,7 +746,7 @@ __global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D, int bin)
 #pragma unroll
             for (int a = 0; a < 3; a++) {
                 v[a] = D.P[(PV + a) * cap + p];
-                x[a] = D.P[(PX + a) * cap + p];
+                x[a] = loadPos(D.P, (PX + a) * cap + p);
             }
 #pragma unroll
             for (int i = 0; i < 9; i++) Cm[i] = D.P[(PC + i) * cap + p];
@@ -734,7 +799,8 @@ __device__ __forceinline__ void ldsAdd(AccT* a, T v, float scale, float& chk)
 // Particle fields P2G reads, loaded ahead of use.
 template <class T>
 struct RawP2G {
-    T x[3], v[3], C[9], S[6], m, d;
+    PosT<T> x[3];
+    T v[3], C[9], S[6], m, d;
 };
 
 template <class T, class AccT>
@@ -746,7 +812,7 @@ __device__ __forceinline__ void loadP2G(const Dev<T, AccT>& D, int s, bool dmg, 
     const T* P = D.P;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        r.x[a] = P[(PX + a) * cap + p];
+        r.x[a] = loadPos(P, (PX + a) * cap + p);
         r.v[a] = P[(PV + a) * cap + p];
     }
 #pragma unroll
@@ -815,15 +881,15 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
                 const T mass = raw.m;
                 T w[3][3], dw[3][3], xb[3];
                 int base[3];
-                const T hi_t = T(D.domain - 4);
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    T xs = mul_rn(raw.x[a], D.odx);
-                    T tt = sub_rn(xs, T(0.5));
-                    if (!(tt >= T(0) && tt < hi_t)) tt = (tt >= hi_t) ? hi_t - T(1) : T(0); // latched by k_bin
-                    base[a] = (int)tt;
-                    bspline(xs, base[a], w[a], dw[a]);
-                    xb[a] = T(base[a]) * D.dx - raw.x[a];
+                    T d0;
+                    cellOf(raw.x[a], D.odx, D.domain, base[a], d0); // errors latched by the binning
+                    bsplineFrac(d0, w[a], dw[a]);
+                    if constexpr (sizeof(T) == 4)
+                        xb[a] = -d0 * D.dx; // x_base - x_p
+                    else
+                        xb[a] = T(base[a]) * D.dx - raw.x[a];
                 }
                 T mC[9], a3[3];
 #pragma unroll
@@ -1094,14 +1160,18 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
 
 // AnisoMPM explicit damage update of one particle (AnisoFractureSimulation.h:210-283,
 // structural tensor :365-391, undegraded Cauchy stress QRStableNeoHookean.h:83-93).
+// Fs is the stored deformation: F (FP64 form) or H = F - I (FP32 form).
 template <class T>
-__device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>& F, T lap, T dt, T& d, T& g)
+__device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>& Fs, T lap, T dt, T& d, T& g)
 {
     const T geometricResist = d - (mt.l0 * mt.l0 * lap);
     // undegraded Cauchy stress tau(g = 1, no fibres) / J (evaluateEffectiveCauchyStress,
     // QRStableNeoHookean.h:83-93); tau is exactly symmetric in its closed form
     T J, t6[6];
-    qrStableTau(F, mt.mu, mt.lambda, T(1), T(0), T(0), t6, &J);
+    if constexpr (sizeof(T) == 4)
+        qrStableTauH(Fs, mt.mu, mt.lambda, 1.f, 0.f, 0.f, t6, &J);
+    else
+        qrStableTau(Fs, mt.mu, mt.lambda, T(1), T(0), T(0), t6, &J);
     const T iJ = ddiv(T(1), J);
     Mat3<T> sym;
     sym.a[0][0] = t6[0] * iJ;
@@ -1119,7 +1189,11 @@ __device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>&
     T dTilde = T(0);
     if (!(mt.phi_bound_scale * sf2 < T(0.99))) {
         const Mat3<T> sp = positivePart(sym);
-        Mat3<T> R = polarRotation(F);
+        Mat3<T> R;
+        if constexpr (sizeof(T) == 4)
+            R = polarRotation(identityPlus(Fs));
+        else
+            R = polarRotation(Fs);
         Mat3<T> A;
         setIdentity(A);
         for (int k = 0; k < mt.nd; k++) {
@@ -1172,23 +1246,25 @@ struct G2PAcc {
 // L = (ddw_x(i) w_y w_z + w_x ddw_x(i) w_z + w_x w_y ddw_x(i)) / dx^2
 // (MpmGrid.h:610-612 index ddw[1], ddw[2] with i as well).
 template <class T, class AccT>
-__device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[Tile<T>::TN], const T* Xp, bool dmg,
-                                          G2PAcc<T>& A)
+__device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[Tile<T>::TN], const PosT<T>* Xp,
+                                          bool dmg, G2PAcc<T>& A)
 {
     using TL = Tile<T>;
     const T odx = D.odx, dx = D.dx;
     T w[3][3], dw[3][3], off[3][3];
     int base[3];
-    const T hi_t = T(D.domain - 4);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        T xs = mul_rn(Xp[a], odx);
-        T tt = sub_rn(xs, T(0.5));
-        if (!(tt >= T(0) && tt < hi_t)) tt = (tt >= hi_t) ? hi_t - T(1) : T(0);
-        base[a] = (int)tt;
-        bspline(xs, base[a], w[a], dw[a]);
+        T d0;
+        cellOf(Xp[a], odx, D.domain, base[a], d0);
+        bsplineFrac(d0, w[a], dw[a]);
 #pragma unroll
-        for (int i = 0; i < 3; i++) off[a][i] = T(base[a] + i) * dx - Xp[a]; // x_i - x_p
+        for (int i = 0; i < 3; i++) { // x_i - x_p
+            if constexpr (sizeof(T) == 4)
+                off[a][i] = (T(i) - d0) * dx;
+            else
+                off[a][i] = T(base[a] + i) * dx - Xp[a];
+        }
     }
     const T wzo[3] = { w[2][0] * off[2][0], w[2][1] * off[2][1], w[2][2] * off[2][2] };
     const int lx0 = base[0] & (TL::BX - 1), ly0 = base[1] & (TL::BY - 1), lz0 = base[2] & (TL::BZ - 1);
@@ -1261,7 +1337,8 @@ __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[T
 // Particle fields G2P reads after the gather, loaded before it.
 template <class T>
 struct RawG2P {
-    T x[3], F[9], m, V0, d, g;
+    PosT<T> x[3];
+    T F[9], m, V0, d, g;
     int mid, id;
 };
 
@@ -1271,7 +1348,7 @@ __device__ __forceinline__ void loadG2P(const Dev<T, AccT>& D, int s, int p, Raw
     const int64_t cap = D.cap;
     const T* P = D.P;
 #pragma unroll
-    for (int a = 0; a < 3; a++) r.x[a] = P[(PX + a) * cap + p];
+    for (int a = 0; a < 3; a++) r.x[a] = loadPos(P, (PX + a) * cap + p);
 #pragma unroll
     for (int i = 0; i < 9; i++) r.F[i] = P[(PF + i) * cap + p];
     r.m = P[PM * cap + p];
@@ -1284,7 +1361,7 @@ __device__ __forceinline__ void loadG2P(const Dev<T, AccT>& D, int s, int p, Raw
 
 template <class T, class AccT>
 __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const RawG2P<T>& R, T dt, const G2PAcc<T>& A,
-                                          const MatDev<T>& mt, float* bnd, T* xnew)
+                                          const MatDev<T>& mt, float* bnd, PosT<T>* xnew)
 {
     constexpr bool FIXED = std::is_same<AccT, long long>::value;
     const int64_t cap = D.cap;
@@ -1306,15 +1383,31 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
                 Cn[3 * a + c] = ca * (A.Bp[a][c] * Dinv) + cb * (A.Bp[c][a] * Dinv);
                 Q[(PC + 3 * a + c) * cap + s] = Cn[3 * a + c];
             }
-        T inc2 = T(0);
+        T inc[3], inc2 = T(0);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            T inc = dt * A.picV[a];
-            inc2 += inc * inc;
-            xnew[a] = R.x[a] + inc;
-            Q[(PX + a) * cap + s] = xnew[a];
+            inc[a] = dt * A.picV[a];
+            inc2 += inc[a] * inc[a];
         }
         if (inc2 > dx * dx) latch(D.counters, ST_CFL);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            if constexpr (sizeof(T) == 4) {
+                // fixed point: exact integer add of the rounded increment; a
+                // non-finite increment marks the position (the binning latches
+                // it), an out-of-range one saturates (the domain check rejects it)
+                const float u = inc[a] * D.pos_scale;
+                int xn = POS_NAN;
+                if (u == u && R.x[a] != POS_NAN) {
+                    const int du = __float2int_rn(fminf(fmaxf(u, -1073741824.f), 1073741824.f));
+                    xn = min(max(__builtin_elementwise_add_sat(R.x[a], du), POS_MIN), POS_MAX);
+                }
+                xnew[a] = xn;
+            }
+            else
+                xnew[a] = R.x[a] + inc[a];
+            storePos(Q, (PX + a) * cap + s, xnew[a]);
+        }
         if constexpr (FIXED) momentumBounds(bnd, R.m, A.picV, Cn, dx);
     }
     Q[PM * cap + s] = R.m;
@@ -1326,8 +1419,15 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     Mat3<T> Fn;
 #pragma unroll
     for (int i = 0; i < 9; i++) Fn.a[i / 3][i % 3] = R.F[i];
-    if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE)
-        Fn.a[0][0] = (T(1) + dt * (A.gv[0][0] + A.gv[1][1] + A.gv[2][2])) * Fn.a[0][0];
+    // F <- (I + dt grad v) F; J <- (1 + dt tr grad v) J.  FP32 form on H = F - I:
+    // H <- H + G + G H with G = dt grad v; J - 1 <- (J - 1) + dt tr(grad v) J.
+    const T dtr = dt * (A.gv[0][0] + A.gv[1][1] + A.gv[2][2]);
+    if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE) {
+        if constexpr (sizeof(T) == 4)
+            Fn.a[0][0] = Fn.a[0][0] + dtr * (1.f + Fn.a[0][0]);
+        else
+            Fn.a[0][0] = (T(1) + dtr) * Fn.a[0][0];
+    }
     else {
         const Mat3<T> F = Fn;
         if (dmg && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) damageUpdate(mt, F, A.lap, dt, d, g);
@@ -1336,6 +1436,7 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 T v = F.a[a][c];
+                if constexpr (sizeof(T) == 4) v += dt * A.gv[a][c];
 #pragma unroll
                 for (int k = 0; k < 3; k++) v += dt * A.gv[a][k] * F.a[k][c];
                 Fn.a[a][c] = v;
@@ -1414,7 +1515,7 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
         seg_lo = seg_hi;
     }
     float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
-    T xnew[3] = { T(0), T(0), T(0) };
+    PosT<T> xnew[3] = { 0, 0, 0 };
     if (live) g2pFinish(D, s, R, dt, A, D.mats[R.mid], bnd, xnew);
     // binning of the coming substep at the new positions (live index s)
     openPageScan(D);

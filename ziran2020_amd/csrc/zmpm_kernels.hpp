@@ -23,11 +23,20 @@
 namespace zmpm {
 
 // particle scalar fields (field-major SoA)
+//   FP64 form: x in metres and F, as the reference stores them.
+//   FP32 form: x as 32-bit fixed point in units of dx / 2^POS_BITS (the bits of
+//   an int32 in the float slot), and H = F - I instead of F (EOS: J - 1 in
+//   PF+0).  A float x carries ~|x| 2^-24 of rounding, which near 1 m is 1e-5 of
+//   a cell at dx = 1/256, and a float F near I loses the strain F - I to
+//   cancellation; both fed O(1e-4) per-field differences from the FP64 form
+//   after 100 substeps.  The fixed-point position resolves dx / 2^21 everywhere
+//   in the domain, advects by exact integer adds, and floors to its base node
+//   without rounding; H keeps the strain to float precision.
 enum : int {
-    PX = 0,   // x[3]
+    PX = 0,   // x[3] (FP32 form: fixed point, see above)
     PV = 3,   // v[3]
     PC = 6,   // C[9] row-major
-    PF = 15,  // F[9] row-major (EOS: J in PF+0)
+    PF = 15,  // F[9] row-major (EOS: J in PF+0); FP32 form: F - I (EOS: J - 1)
     PM = 24,  // mass
     PVOL = 25, // element measure V0
     PD = 26,  // damage d
@@ -36,6 +45,23 @@ enum : int {
     PS = 29,   // V0 * Kirchhoff stress of the current F, symmetric: xx xy xz yy yz zz
     NPF = 35
 };
+
+// FP32-form position: X = round(x / dx * 2^POS_BITS) as int32.  Domains are at
+// most 1024 nodes per axis, so every in-domain X is below 2^31.  POS_NAN marks
+// a non-finite position (an upload of NaN / Inf or a non-finite advection);
+// out-of-range values are clamped to [POS_MIN, POS_MAX], where the binning's
+// domain check rejects them.
+constexpr int POS_BITS = 21;
+constexpr int POS_HALF = 1 << (POS_BITS - 1); // half a cell: base = floor(X / 2^POS_BITS - 1/2)
+constexpr int POS_NAN = (int)0x80000000;
+constexpr int POS_MIN = -(1 << 30);
+constexpr int POS_MAX = 0x7ffffffe;
+
+// Stored position of one axis: fixed point (FP32 form) or metres (FP64 form).
+template <class T> struct PosRep;
+template <> struct PosRep<float> { using type = int; };
+template <> struct PosRep<double> { using type = double; };
+template <class T> using PosT = typename PosRep<T>::type;
 
 // grid accumulator fields
 enum : int {
@@ -147,6 +173,7 @@ struct Dev {
     int p2g_split; // most lanes sharing one stencil column of a cell in P2G (>= 1)
     // scene
     T dx, odx;
+    T pos_scale; // FP32 form: metres -> position units, odx * 2^POS_BITS
     T Dinv; // 4 / dx^2 (MpmSimulationBase.cpp:119-122)
     T gravity[3];
     T rpic;

@@ -575,6 +575,177 @@ __device__ __forceinline__ Mat3<T> neoHookeanTau(const Mat3<T>& F, T mu, T lambd
     return tau;
 }
 
+// ------------------------------------------- FP32 form: H = F - I stored
+// The float form keeps the deformation as H = F - I, and every stress below is
+// written in terms of the strain-sized quantities H, J - 1 and E = F F^T - I,
+// so no difference of two O(1) floats (det F - 1, F F^T - I, F - R) loses the
+// strain to cancellation.  The models are the same functions of F as above.
+
+// J - 1 = det(I + H) - 1 = tr H + (principal 2x2 minors of H) + det H.
+__device__ __forceinline__ float jm1OfH(const Mat3<float>& H)
+{
+    const float h00 = H.a[0][0], h01 = H.a[0][1], h02 = H.a[0][2];
+    const float h10 = H.a[1][0], h11 = H.a[1][1], h12 = H.a[1][2];
+    const float h20 = H.a[2][0], h21 = H.a[2][1], h22 = H.a[2][2];
+    const float m2 = (h00 * h11 - h01 * h10) + (h00 * h22 - h02 * h20) + (h11 * h22 - h12 * h21);
+    return (h00 + h11 + h22) + m2 + det3(H);
+}
+
+// E = F F^T - I = H + H^T + H H^T, symmetric (xx xy xz yy yz zz).
+__device__ __forceinline__ void strainOfH(const Mat3<float>& H, float* E)
+{
+    const float* r0 = H.a[0];
+    const float* r1 = H.a[1];
+    const float* r2 = H.a[2];
+    E[0] = 2.f * r0[0] + (r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2]);
+    E[1] = (r0[1] + r1[0]) + (r0[0] * r1[0] + r0[1] * r1[1] + r0[2] * r1[2]);
+    E[2] = (r0[2] + r2[0]) + (r0[0] * r2[0] + r0[1] * r2[1] + r0[2] * r2[2]);
+    E[3] = 2.f * r1[1] + (r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
+    E[4] = (r1[2] + r2[1]) + (r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2]);
+    E[5] = 2.f * r2[2] + (r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
+}
+
+__device__ __forceinline__ Mat3<float> identityPlus(const Mat3<float>& H)
+{
+    Mat3<float> F = H;
+#pragma unroll
+    for (int i = 0; i < 3; i++) F.a[i][i] += 1.f;
+    return F;
+}
+
+// Eigenvalues l1 >= l2 >= l3 of a symmetric 3x3 (xx xy xz yy yz zz) from the
+// trigonometric solution of its characteristic cubic (O. K. Smith, "Eigenvalues
+// of a symmetric 3x3 matrix", CACM 4, 1961), on the normalised deviator.
+__device__ __forceinline__ void symEigvals3(const float* S, float& l1, float& l2, float& l3)
+{
+    const float q = (S[0] + S[3] + S[5]) * (1.f / 3.f);
+    const float b0 = S[0] - q, b1 = S[3] - q, b2 = S[5] - q;
+    const float p2 = b0 * b0 + b1 * b1 + b2 * b2 + 2.f * (S[1] * S[1] + S[2] * S[2] + S[4] * S[4]);
+    if (!(p2 > 1e-30f)) {
+        l1 = l2 = l3 = q;
+        return;
+    }
+    const float p = __builtin_amdgcn_sqrtf(p2 * (1.f / 6.f));
+    const float ip = __builtin_amdgcn_rcpf(p);
+    const float c0 = b0 * ip, c1 = b1 * ip, c2 = b2 * ip, o01 = S[1] * ip, o02 = S[2] * ip, o12 = S[4] * ip;
+    const float detB = c0 * (c1 * c2 - o12 * o12) - o01 * (o01 * c2 - o12 * o02) + o02 * (o01 * o12 - c1 * o02);
+    const float r = fminf(1.f, fmaxf(-1.f, 0.5f * detB));
+    const float phi = acosf(r) * (1.f / 3.f);
+    l1 = q + 2.f * p * __cosf(phi);
+    l3 = q + 2.f * p * __cosf(phi + 2.0943951023931953f);
+    l2 = 3.f * q - l1 - l3;
+}
+
+// Fixed corotated Kirchhoff stress (the corotatedTau model above) from H.
+// With B = F F^T = V^2 (left polar decomposition F = V R, J > 0), the reference's
+// (F - R) F^T equals V^2 - V = f(B) for f(b) = b - sqrt(b), so
+//   tau = 2 mu f(B) + lambda (J - 1) J I.
+// f(B) is evaluated from the eigenvalues l_i of E = B - I (s_i = sqrt(1 + l_i)
+// are the singular values of F) in Newton's divided-difference form, which
+// stays exact for coincident eigenvalues and needs no eigenvectors:
+//   f(B) = f1 I + f[1,2] (E - l1 I) + f[1,2,3] (E - l1 I)(E - l2 I),
+//   f1 = s1 l1 / (s1 + 1),  f[1,2] = 1 - 1 / (s1 + s2),
+//   f[1,2,3] = 1 / ((s1 + s2)(s2 + s3)(s1 + s3)).
+// Below J = 1/20 (inverted or nearly, where the reference's signed SVD makes R
+// differ from V^{-1} F) the reference formula runs on F = I + H.
+__device__ __forceinline__ void corotatedTauH(const Mat3<float>& H, float mu, float lambda, float* tau)
+{
+    const float jm1 = jm1OfH(H);
+    const float J = 1.f + jm1;
+    if (!(J > 0.05f)) {
+        const Mat3<float> t = corotatedTau(identityPlus(H), mu, lambda);
+        tau[0] = t.a[0][0];
+        tau[1] = 0.5f * (t.a[0][1] + t.a[1][0]);
+        tau[2] = 0.5f * (t.a[0][2] + t.a[2][0]);
+        tau[3] = t.a[1][1];
+        tau[4] = 0.5f * (t.a[1][2] + t.a[2][1]);
+        tau[5] = t.a[2][2];
+        return;
+    }
+    float E[6];
+    strainOfH(H, E);
+    float l1, l2, l3;
+    symEigvals3(E, l1, l2, l3);
+    const float s1 = __builtin_amdgcn_sqrtf(1.f + l1), s2 = __builtin_amdgcn_sqrtf(1.f + l2);
+    const float s3 = __builtin_amdgcn_sqrtf(fmaxf(1.f + l3, 0.f));
+    const float f1 = s1 * l1 * __builtin_amdgcn_rcpf(s1 + 1.f);
+    const float d12 = 1.f - __builtin_amdgcn_rcpf(s1 + s2);
+    const float d123 = __builtin_amdgcn_rcpf((s1 + s2) * (s2 + s3) * (s1 + s3));
+    // M1 = E - l1 I, M2 = E - l2 I (they commute: M1 M2 is symmetric)
+    const float a00 = E[0] - l1, a11 = E[3] - l1, a22 = E[5] - l1;
+    const float b00 = E[0] - l2, b11 = E[3] - l2, b22 = E[5] - l2;
+    const float e01 = E[1], e02 = E[2], e12 = E[4];
+    const float p00 = a00 * b00 + e01 * e01 + e02 * e02;
+    const float p01 = a00 * e01 + e01 * b11 + e02 * e12;
+    const float p02 = a00 * e02 + e01 * e12 + e02 * b22;
+    const float p11 = e01 * e01 + a11 * b11 + e12 * e12;
+    const float p12 = e01 * e02 + a11 * e12 + e12 * b22;
+    const float p22 = e02 * e02 + e12 * e12 + a22 * b22;
+    const float two_mu = 2.f * mu;
+    const float iso = lambda * jm1 * J;
+    tau[0] = two_mu * (f1 + d12 * a00 + d123 * p00) + iso;
+    tau[1] = two_mu * (d12 * e01 + d123 * p01);
+    tau[2] = two_mu * (d12 * e02 + d123 * p02);
+    tau[3] = two_mu * (f1 + d12 * a11 + d123 * p11) + iso;
+    tau[4] = two_mu * (d12 * e12 + d123 * p12);
+    tau[5] = two_mu * (f1 + d12 * a22 + d123 * p22) + iso;
+}
+
+// QRStableNeoHookean Kirchhoff stress (qrStableTau above) from H:
+// g mu F F^T + (vol - g mu J) I = g mu E + (J - 1)(g_l lambda J - g mu) I, and the
+// fibre factor (|f| - 1) / |f| with |f| - 1 = (|f|^2 - 1) / (|f| + 1),
+// |f_c|^2 - 1 = 2 H_cc + |H e_c|^2.
+__device__ __forceinline__ void qrStableTauH(const Mat3<float>& H, float mu, float lambda, float g, float kx, float ky,
+                                             float* tau, float* Jout)
+{
+    const float jm1 = jm1OfH(H);
+    const float J = 1.f + jm1;
+    const float gl = (jm1 >= 0.f) ? g * lambda : lambda;
+    const float gm = g * mu;
+    const float iso = jm1 * (gl * J - gm);
+    float E[6];
+    strainOfH(H, E);
+    tau[0] = gm * E[0] + iso;
+    tau[1] = gm * E[1];
+    tau[2] = gm * E[2];
+    tau[3] = gm * E[3] + iso;
+    tau[4] = gm * E[4];
+    tau[5] = gm * E[5] + iso;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const float k = c == 0 ? kx : ky;
+        if (k != 0.f) {
+            const float h[3] = { H.a[0][c], H.a[1][c], H.a[2][c] };
+            const float f[3] = { h[0] + (c == 0 ? 1.f : 0.f), h[1] + (c == 1 ? 1.f : 0.f), h[2] };
+            const float n2m1 = 2.f * h[c] + (h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+            const float n = __builtin_amdgcn_sqrtf(1.f + n2m1);
+            const float cc = k * n2m1 * __builtin_amdgcn_rcpf((n + 1.f) * n);
+            tau[0] += cc * f[0] * f[0];
+            tau[1] += cc * f[0] * f[1];
+            tau[2] += cc * f[0] * f[2];
+            tau[3] += cc * f[1] * f[1];
+            tau[4] += cc * f[1] * f[2];
+            tau[5] += cc * f[2] * f[2];
+        }
+    }
+    *Jout = J;
+}
+
+// Neo-Hookean Kirchhoff stress (neoHookeanTau above) from H:
+// mu F F^T + (lambda log J - mu) I = mu E + lambda log1p(J - 1) I.
+__device__ __forceinline__ void neoHookeanTauH(const Mat3<float>& H, float mu, float lambda, float* tau)
+{
+    const float lj = lambda * log1pf(jm1OfH(H));
+    float E[6];
+    strainOfH(H, E);
+    tau[0] = mu * E[0] + lj;
+    tau[1] = mu * E[1];
+    tau[2] = mu * E[2];
+    tau[3] = mu * E[3] + lj;
+    tau[4] = mu * E[4];
+    tau[5] = mu * E[5] + lj;
+}
+
 // ------------------------------------------------------ symmetric eigen (Jacobi)
 // Unit eigenvectors in the columns of Q.  Stands in for Eigen::EigenSolver at
 // AnisoFractureSimulation.h:244-254 (sigma^+ does not depend on the basis
@@ -737,19 +908,11 @@ template <class T>
 __device__ __forceinline__ T macaulay(T x) { return (x + dabs(x)) / T(2); } // MathTools.h:276-281
 
 // ------------------------------------------------------ quadratic B-spline
-// BSplines.h:16-20 (base node) and :72-98 (weights); base computed without FMA.
+// Weights of computeBSplineWeights (BSplines.h:72-98) from the offset
+// d0 = x_index - base in [0.5, 1.5) (the base node is cellOf's, zmpm_device.hpp).
 template <class T>
-__device__ __forceinline__ int baseNodeOf(T x_index)
+__device__ __forceinline__ void bsplineFrac(T d0, T* w, T* dw)
 {
-    T t = sub_rn(x_index, T(0.5));
-    int i = (int)t;
-    return i - (T(i) > t);
-}
-
-template <class T>
-__device__ __forceinline__ void bspline(T x_index, int base, T* w, T* dw)
-{
-    T d0 = x_index - T(base);
     T z = T(1.5) - d0;
     w[0] = T(0.5) * z * z;
     T d1 = d0 - T(1);
