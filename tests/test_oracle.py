@@ -227,6 +227,72 @@ def test_qr_neo_hookean_kirchhoff_closed_form(O, g):
         assert np.abs(got - tau).max() < 1e-10 * max(1.0, np.abs(tau).max())
 
 
+def _tau_from_strain(model, m, H, g=1.0, dtype=np.float64):
+    """The FP32 form's stresses written in H = F - I (J - 1 from the minors of H,
+    E = H + H^T + H H^T; fixed corotated as 2 mu f(B) + lambda (J-1) J I with
+    f(b) = b - sqrt(b) in Newton divided-difference form over the eigenvalues
+    of E), evaluated in `dtype`."""
+    H = H.astype(dtype)
+    one = dtype(1)
+    minors = (H[0, 0] * H[1, 1] - H[0, 1] * H[1, 0]) + (H[0, 0] * H[2, 2] - H[0, 2] * H[2, 0]) \
+        + (H[1, 1] * H[2, 2] - H[1, 2] * H[2, 1])
+    jm1 = np.trace(H) + minors + dtype(np.linalg.det(H.astype(np.float64)))
+    J = one + jm1
+    E = H + H.T + H @ H.T
+    I = np.eye(3, dtype=dtype)
+    mu, lam = dtype(m.mu), dtype(m.lam)
+    if model == A.MODEL_FIXED_COROTATED:
+        l3, l2, l1 = np.linalg.eigvalsh(E)
+        s1, s2, s3 = np.sqrt(one + l1), np.sqrt(one + l2), np.sqrt(one + l3)
+        f1 = s1 * l1 / (s1 + one)
+        d12 = one - one / (s1 + s2)
+        d123 = one / ((s1 + s2) * (s2 + s3) * (s1 + s3))
+        M1, M2 = E - l1 * I, E - l2 * I
+        return dtype(2) * mu * (f1 * I + d12 * M1 + d123 * (M1 @ M2)) + lam * jm1 * J * I
+    if model == A.MODEL_QR_STABLE_NEO_HOOKEAN:
+        gl = dtype(g) * lam if jm1 >= 0 else lam
+        gm = dtype(g) * mu
+        tau = gm * E + jm1 * (gl * J - gm) * I
+        for c, k in ((0, m.kx), (1, m.ky)):
+            f = I[:, c] + H[:, c]
+            n2m1 = dtype(2) * H[c, c] + H[:, c] @ H[:, c]
+            n = np.sqrt(one + n2m1)
+            tau = tau + dtype(k) * n2m1 / ((n + one) * n) * np.outer(f, f)
+        return tau
+    return mu * E + lam * np.log1p(jm1) * I
+
+
+@pytest.mark.parametrize("model", [A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN])
+def test_stress_from_strain_matches_reference_route(O, model):
+    # The rewrite in H = F - I is the same function of F as the reference's
+    # route (SVD polar for corotated, QR for QR-NH, log det for NH): checked in
+    # double on rotated, stretched, sheared and nearly degenerate F with J > 0.
+    rng = np.random.default_rng(21)
+    m = mat(model, kx=500.0, ky=300.0) if model == A.MODEL_QR_STABLE_NEO_HOOKEAN else mat(model)
+    Fs = [rand_F(rng, 0.3) for _ in range(30)] + [np.eye(3), np.diag([1.2, 1.2, 1.2]), np.diag([1.1, 1.1, 0.9]),
+                                                  np.diag([1 + 1e-9, 1, 1 - 1e-9]), np.diag([3.0, 0.5, 0.4])]
+    Fs += [np.linalg.qr(rng.normal(size=(3, 3)))[0] @ np.diag(1 + 1e-4 * rng.normal(size=3)) for _ in range(5)]
+    for F in Fs:
+        if np.linalg.det(F) <= 0.05:
+            F = F @ np.diag([1, 1, -1.0])
+        for g in ((1.0, 0.4) if model == A.MODEL_QR_STABLE_NEO_HOOKEAN else (1.0,)):
+            want = piola(O, m, F, g) @ F.T
+            got = _tau_from_strain(model, m, F - np.eye(3), g)
+            assert np.abs(got - want).max() <= 1e-9 * max(1.0, np.abs(want).max()), (F, g)
+
+
+def test_stress_from_strain_keeps_float_precision_at_small_strain(O):
+    # At strain 1e-5 a float F loses the strain to cancellation (F F^T - I,
+    # F - R); the rewrite in H keeps float precision relative to the stress.
+    rng = np.random.default_rng(22)
+    m = mat(A.MODEL_FIXED_COROTATED)
+    for _ in range(20):
+        H = 1e-5 * rng.normal(size=(3, 3))
+        want = piola(O, m, np.eye(3) + H) @ (np.eye(3) + H).T
+        got32 = _tau_from_strain(A.MODEL_FIXED_COROTATED, m, H, dtype=np.float32).astype(np.float64)
+        assert np.abs(got32 - want).max() < 1e-5 * np.abs(want).max()
+
+
 def test_corotated_diagonal_closed_form(O):
     m = mat(A.MODEL_FIXED_COROTATED, 2e4, 0.25)
     for sig in ([1.2, 0.9, 1.05], [0.7, 1.3, 1.0], [2.0, 1.0, 0.5]):
