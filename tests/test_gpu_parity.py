@@ -165,6 +165,27 @@ def test_batched_scenes_small(gpu, oracle):
         assert normwise(g[k], o[k]) < FP64_TOL, k
 
 
+FP32_TOL_100 = 2e-5  # FP32 form vs FP64 form after 100 substeps, every field
+
+
+@pytest.mark.parametrize("name", ["elastic_cube", "two_material", "batched_scenes"])
+def test_fp32_error_after_100_substeps(gpu, name):
+    # The FP64 form equals the FP64 oracle to 1e-9 (tests above); the FP32
+    # form, with fixed-point positions and H = F - I, stays within 2e-5 of it
+    # per field after 100 substeps on each scene type (fluid + solid, damage,
+    # fixed corotated on a floor).
+    if name == "elastic_cube":
+        sc = scenes.elastic_cube()
+    elif name == "two_material":
+        sc = scenes.two_material(n=1_000_000, dx=1.0 / 128)
+    else:
+        sc = scenes.batched_scenes(n_scenes=8)
+    a = run(gpu, sc, A.PREC_FP32, 100).download()
+    b = run(gpu, sc, A.PREC_FP64, 100).download()
+    for k in ("x", "v", "C", "F", "d"):
+        assert normwise(a[k], b[k]) < FP32_TOL_100, (k, normwise(a[k], b[k]))
+
+
 def test_sparse_spheres_small(gpu, oracle):
     sc = scenes.sparse_spheres(n=30000, n_spheres=5, dx=1.0 / 64, domain_nodes=128)
     g = run(gpu, sc, A.PREC_FP32, 10).download()
