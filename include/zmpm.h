@@ -224,12 +224,11 @@ zmpm_status zmpm_get_timings(zmpm_handle h, int32_t* count, const char** names, 
  * fixed-point accumulation (results deterministic for a given value).
  * No counterpart in the reference (launch configuration of this build). */
 typedef struct {
-    int32_t p2g_wgs;     /* P2G: workgroups over grid blocks                      */
-    int32_t sort_wgs;    /* in-block sort: workgroups over grid blocks            */
+    int32_t p2g_wgs;     /* in-block sort + P2G: workgroups over grid blocks      */
     int32_t grid_wgs;    /* grid update: workgroups over nodes                    */
     int32_t prep_wgs;    /* accumulator zeroing / neighbour table                 */
     int32_t p2g_split;   /* P2G lanes per stencil column, cap (>= 1)              */
-    int32_t reserved[3];
+    int32_t reserved[4];
 } zmpm_launch_config;
 
 zmpm_status zmpm_get_launch_config(zmpm_handle h, zmpm_launch_config* cfg);

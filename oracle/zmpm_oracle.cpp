@@ -845,7 +845,7 @@ struct zmpm_context {
     zmpm_scene_desc desc;
     Layout L;
     std::string err;
-    zmpm_launch_config launch { 1, 1, 1, 1, 1, { 0, 0, 0 } };
+    zmpm_launch_config launch { 1, 1, 1, 1, { 0, 0, 0, 0 } };
     std::vector<zmpm_material> mats;
     std::vector<Collider> colliders;
     bool any_damage = false;
@@ -1509,7 +1509,7 @@ zmpm_status zmpm_get_launch_config(zmpm_handle h, zmpm_launch_config* cfg)
 zmpm_status zmpm_set_launch_config(zmpm_handle h, const zmpm_launch_config* cfg)
 {
     if (!h || !cfg) return ZMPM_E_ARG;
-    if (cfg->p2g_wgs < 1 || cfg->sort_wgs < 1 || cfg->grid_wgs < 1 || cfg->prep_wgs < 1 || cfg->p2g_split < 1)
+    if (cfg->p2g_wgs < 1 || cfg->grid_wgs < 1 || cfg->prep_wgs < 1 || cfg->p2g_split < 1)
         return ZMPM_E_ARG;
     h->launch = *cfg;
     return ZMPM_OK;

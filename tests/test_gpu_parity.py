@@ -328,7 +328,7 @@ def test_launch_config_does_not_change_results(gpu):
     sc = small_slab(n=8000)
     a = _fields(run(gpu, sc, A.PREC_FP32, 4).download())
     sim = sc.make(gpu, A.PREC_FP32)
-    sim.set_launch_config(p2g_wgs=37, sort_wgs=13, grid_wgs=5, prep_wgs=3)
+    sim.set_launch_config(p2g_wgs=37, grid_wgs=5, prep_wgs=3)
     assert sim.launch_config()["p2g_wgs"] == 37
     sim.run(4, sc.dt, 0.0)
     b = _fields(sim.download())

@@ -135,11 +135,11 @@ def box(box_min, box_max, type_=STICKY, b0=(0, 0, 0), dbdt=(0, 0, 0), friction=0
 
 class LaunchConfig(C.Structure):
     """zmpm_launch_config (include/zmpm.h)."""
-    _fields_ = [("p2g_wgs", C.c_int32), ("sort_wgs", C.c_int32), ("grid_wgs", C.c_int32),
-                ("prep_wgs", C.c_int32), ("p2g_split", C.c_int32), ("reserved", C.c_int32 * 3)]
+    _fields_ = [("p2g_wgs", C.c_int32), ("grid_wgs", C.c_int32),
+                ("prep_wgs", C.c_int32), ("p2g_split", C.c_int32), ("reserved", C.c_int32 * 4)]
 
     def as_dict(self):
-        return {k: int(getattr(self, k)) for k in ("p2g_wgs", "sort_wgs", "grid_wgs", "prep_wgs", "p2g_split")}
+        return {k: int(getattr(self, k)) for k in ("p2g_wgs", "grid_wgs", "prep_wgs", "p2g_split")}
 
 
 class ZmpmLib:

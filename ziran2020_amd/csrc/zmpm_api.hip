@@ -76,7 +76,7 @@ struct zmpm_context {
     bool use_graphs = true;
     int p2g_split = 64;
     // workgroups of the grid-stride launches (results do not depend on them)
-    int sort_wgs = 2048, p2g_wgs = 2048, grid_wgs = 2048, prep_wgs = 2048;
+    int p2g_wgs = 2048, grid_wgs = 2048, prep_wgs = 2048;
 };
 
 namespace {
@@ -163,6 +163,7 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.key, cap);
     ALLOC(D.rank, cap);
     ALLOC(D.tmp, cap);
+    ALLOC(D.tmpk, cap);
     ALLOC(D.perm, cap);
     ALLOC(D.sorted_cell, cap);
     ALLOC(D.spid, cap);
@@ -189,6 +190,7 @@ zmpm_status allocate(zmpm_context* h, int n)
     HIPCHK(h, hipMemsetAsync(D.page_count, 0, D.num_pages * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.scan_state, 0, scan_tiles * sizeof(uint64_t), h->stream));
     HIPCHK(h, hipMemsetAsync(D.tmp, 0, cap * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.tmpk, 0, cap * sizeof(uint32_t), h->stream));
     HIPCHK(h, hipMemsetAsync(D.perm, 0, cap * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.sorted_cell, 0, cap * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.spid, 0, cap * sizeof(int), h->stream));
@@ -300,7 +302,6 @@ void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
     k_page_scan<T, AccT><<<(int)((D.num_pages + PSCAN_TILE - 1) / PSCAN_TILE), WG, 0, st>>>(D);
     const int prep = (int)std::min<int64_t>(D.cap_blocks, h->prep_wgs);
     k_scatter<T, AccT><<<gp + prep + 1, WG, 0, st>>>(D, dt, gp, prep);
-    k_blocksort<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, h->sort_wgs), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[1], st);
     k_p2g<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[2], st);
@@ -630,7 +631,6 @@ zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
     h->fp32 = desc->precision == ZMPM_PREC_FP32;
     if (const char* e = std::getenv("ZMPM_NO_GRAPH")) h->use_graphs = std::atoi(e) == 0;
     if (const char* e = std::getenv("ZMPM_P2G_SPLIT")) h->p2g_split = std::max(1, std::atoi(e));
-    if (const char* e = std::getenv("ZMPM_SORT_WGS")) h->sort_wgs = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("ZMPM_P2G_WGS")) h->p2g_wgs = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("ZMPM_GRID_WGS")) h->grid_wgs = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("ZMPM_PREP_WGS")) h->prep_wgs = std::max(1, std::atoi(e));
@@ -894,7 +894,6 @@ zmpm_status zmpm_get_launch_config(zmpm_handle h, zmpm_launch_config* cfg)
     if (!h || !cfg) return ZMPM_E_ARG;
     *cfg = zmpm_launch_config {};
     cfg->p2g_wgs = h->p2g_wgs;
-    cfg->sort_wgs = h->sort_wgs;
     cfg->grid_wgs = h->grid_wgs;
     cfg->prep_wgs = h->prep_wgs;
     cfg->p2g_split = h->p2g_split;
@@ -904,12 +903,11 @@ zmpm_status zmpm_get_launch_config(zmpm_handle h, zmpm_launch_config* cfg)
 zmpm_status zmpm_set_launch_config(zmpm_handle h, const zmpm_launch_config* cfg)
 {
     if (!h || !cfg) return ZMPM_E_ARG;
-    if (cfg->p2g_wgs < 1 || cfg->sort_wgs < 1 || cfg->grid_wgs < 1 || cfg->prep_wgs < 1 || cfg->p2g_split < 1)
+    if (cfg->p2g_wgs < 1 || cfg->grid_wgs < 1 || cfg->prep_wgs < 1 || cfg->p2g_split < 1)
         return fail(h, ZMPM_E_ARG, "launch configuration entries must be >= 1");
     (void)hipSetDevice(h->desc.device);
     dropGraphs(h);
     h->p2g_wgs = cfg->p2g_wgs;
-    h->sort_wgs = cfg->sort_wgs;
     h->grid_wgs = cfg->grid_wgs;
     h->prep_wgs = cfg->prep_wgs;
     h->p2g_split = cfg->p2g_split;
@@ -951,7 +949,6 @@ zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, in
     };
     std::vector<Knob> knobs = {
         { &h->p2g_wgs, 1, { 512, 1024, 2048, 4096 } },
-        { &h->sort_wgs, 0, { 512, 1024, 2048, 4096 } },
         { &h->prep_wgs, 0, { 256, 512, 1024, 2048 } },
         { &h->grid_wgs, 2, { 256, 512, 1024, 2048, 4096 } },
     };

@@ -9,10 +9,10 @@
 //   k_scatter        particles to their block ranges; zero block accumulators,
 //                    forward-neighbour block table, device clock and
 //                    fixed-point scales (float form)
-//   k_blocksort      order inside each block by cell, then original index
-//                    -> reference order
-//   k_p2g            APIC scatter of m, mv, f (stress read from the particle)
-//                    and of the damage phase field
+//   k_p2g            order inside each block by cell, then original index
+//                    (-> reference order), then APIC scatter of m, mv, f
+//                    (stress read from the particle) and of the damage phase
+//                    field
 //   k_grid           v = mv/m, gravity + force, collision objects, phase-field
 //                    normalisation
 //   k_g2p            APIC gather, x update, CFL flag, damage Laplacian and
@@ -193,8 +193,8 @@ __device__ __forceinline__ uint64_t waveInclusiveScan64(uint64_t v)
 //   k_scatter    particle -> its block's range; zeroes the blocks' grid
 //                accumulators, builds the neighbour table, starts the substep
 //                (device clock, fixed-point scales)
-//   k_blocksort  inside each block: counting sort by cell, then by original
-//                index inside the cell, in LDS
+//   blockOrder   (in k_p2g) inside each block: counting sort by cell, then by
+//                original index inside the cell, in LDS
 // Sorting by (block in page order, cell, original index) is std::sort of the
 // reference's unique 64-bit keys (MpmSimulationBase.cpp:1552).
 
@@ -426,8 +426,9 @@ __device__ __forceinline__ void beginStep(const Dev<T, AccT>& D, double dt)
     }
 }
 
-// Workgroups [0, part_wgs): particle p -> block_start[block] + its slot
-// (order inside a block is fixed by k_blocksort).  The next prep_wgs
+// Workgroups [0, part_wgs): particle p -> block_start[block] + its slot, with
+// its in-block sort key (cell, original index) beside it (the order inside a
+// block is fixed by P2G's blockOrder).  The next prep_wgs
 // workgroups: zero the accumulators of every block's nodes and build the
 // forward-neighbour block table.  The last workgroup starts the substep.
 template <class T, class AccT>
@@ -440,10 +441,14 @@ __global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D, double dt, int p
     if ((int)blockIdx.x < part_wgs) {
         const int p = blockIdx.x * WG + t;
         if (p >= D.n) return;
-        const int bid = D.page_table[D.key[p] >> bbits];
+        const uint64_t key = D.key[p];
+        const int bid = D.page_table[key >> bbits];
         if (bid < 0) return; // capacity exceeded (latched by the scan)
         const int pos = D.block_start[bid] + D.rank[p];
-        if ((unsigned)pos < (unsigned)D.n) D.tmp[pos] = p;
+        if ((unsigned)pos < (unsigned)D.n) {
+            D.tmp[pos] = p;
+            D.tmpk[pos] = ((uint32_t)(key & (NB - 1)) << (32 - bbits)) | (uint32_t)D.pid[p];
+        }
         return;
     }
     if ((int)blockIdx.x == part_wgs + prep_wgs) {
@@ -470,115 +475,107 @@ __global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D, double dt, int p
     }
 }
 
-// Order inside each block.  The block's particles are staged once in LDS as
-// key = cell << (32 - block_bits) | original index (the low 32 bits of the
-// reference key, unique) with their live index; a histogram and cursors group
-// them by cell; then every particle's position is its cell's start plus the
+// Order inside block b (run by every thread of the P2G workgroup that owns the
+// block).  The block's particles are staged in LDS as key = cell << (32 -
+// block_bits) | original index (the low 32 bits of the reference key, unique)
+// with their live index (ska / pa); a histogram and cursors group them by cell
+// (skb / pb); then every particle's position is its cell's start plus the
 // number of keys in its cell below its own (all particles ranked in parallel).
-// Blocks larger than the LDS stage take the same steps with the grouped
-// arrays in HBM.  Writes perm, spid, sorted_cell and every cell's start and
-// count.
-constexpr int BSORT_STAGE = 2048;
+// Blocks larger than the LDS stage take the same steps with the grouped arrays
+// in HBM.  Writes perm, spid, sorted_cell and every cell's start and count, and
+// leaves the cell counts and starts (relative to the block) in hist / cst.
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_blocksort(Dev<T, AccT> D)
+__device__ __forceinline__ void blockOrder(const Dev<T, AccT>& D, int b, int start, int cnt, uint32_t* ska, int* pa,
+                                           uint32_t* skb, int* pb, int stage, int* hist, int* cur, int* cst)
 {
     using B = Blk<T>;
     constexpr int bbits = B::xbits + B::ybits + B::zbits;
     constexpr int NB = 1 << bbits;
     constexpr int idbits = 32 - bbits;
     constexpr uint32_t idmask = (1u << idbits) - 1u;
-    __shared__ uint32_t ska[BSORT_STAGE], skb[BSORT_STAGE];
-    __shared__ int pa[BSORT_STAGE], pb[BSORT_STAGE];
-    __shared__ int hist[NB], cur[NB], cst[NB];
     const int t = threadIdx.x;
-    const int nb = D.counters[CNT_BLOCKS];
-    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-        const int start = D.block_start[b], cnt = D.block_count[b];
-        const bool staged = cnt <= BSORT_STAGE;
-        __syncthreads(); // LDS of the previous block done
-        if (t < NB) hist[t] = 0;
-        __syncthreads();
-        // stage (key, live index) and count cells
-        for (int i = t; i < cnt; i += WG) {
-            int p = D.tmp[start + i];
-            if ((unsigned)p >= (unsigned)D.n) p = 0;
-            const int c = (int)(D.key[p] & (NB - 1));
-            atomicAdd(&hist[c], 1);
-            if (staged) {
-                ska[i] = ((uint32_t)c << idbits) | (uint32_t)D.pid[p];
-                pa[i] = p;
-            }
-        }
-        __syncthreads();
-        if (t < 64) {
-            const int v = t < NB ? hist[t] : 0;
-            const int ex = waveInclusiveScan(v) - v;
-            if (t < NB) {
-                cst[t] = ex;
-                cur[t] = ex;
-            }
-        }
-        __syncthreads();
+    const bool staged = cnt <= stage;
+    if (t < NB) hist[t] = 0;
+    __syncthreads();
+    // stage (key, live index) and count cells
+    for (int i = t; i < cnt; i += WG) {
+        const uint32_t k = D.tmpk[start + i];
+        atomicAdd(&hist[k >> idbits], 1);
         if (staged) {
-            // group by cell
-            for (int i = t; i < cnt; i += WG) {
-                const uint32_t k = ska[i];
-                const int dst = atomicAdd(&cur[k >> idbits], 1);
-                skb[dst] = k;
-                pb[dst] = pa[i];
-            }
-            __syncthreads();
-            // rank inside the cell; write the sorted block
-            for (int i = t; i < cnt; i += WG) {
-                const uint32_t k = skb[i];
-                const int c = (int)(k >> idbits);
-                const int c0 = cst[c], c1 = c0 + hist[c];
-                int pos = c0;
-                for (int j = c0; j < c1; j++) pos += skb[j] < k;
-                D.perm[start + pos] = pb[i];
-                D.spid[start + pos] = (int)(k & idmask);
-                D.sorted_cell[start + pos] = b * NB + c;
-            }
-        }
-        else {
-            // the same in HBM: group into perm/spid, then rank through tmp
-            for (int i = t; i < cnt; i += WG) {
-                int p = D.tmp[start + i];
-                if ((unsigned)p >= (unsigned)D.n) p = 0;
-                const int c = (int)(D.key[p] & (NB - 1));
-                const int dst = atomicAdd(&cur[c], 1);
-                D.perm[start + dst] = p;
-                D.spid[start + dst] = D.pid[p];
-            }
-            __syncthreads();
-            __threadfence_block();
-            for (int i = t; i < cnt; i += WG) D.tmp[start + i] = D.perm[start + i];
-            __syncthreads();
-            for (int i = t; i < cnt; i += WG) D.sorted_cell[start + i] = D.spid[start + i];
-            __syncthreads();
-            for (int i = t; i < cnt; i += WG) {
-                const int p = D.tmp[start + i];
-                const int id = D.sorted_cell[start + i];
-                int c = NB - 1;
-                while (c > 0 && cst[c] > i) c--;
-                const int c0 = cst[c], c1 = c0 + hist[c];
-                int pos = c0;
-                for (int j = c0; j < c1; j++) pos += D.sorted_cell[start + j] < id;
-                D.perm[start + pos] = p;
-                D.spid[start + pos] = id;
-            }
-            __syncthreads();
-            for (int i = t; i < cnt; i += WG) {
-                int c = NB - 1;
-                while (c > 0 && cst[c] > i) c--;
-                D.sorted_cell[start + i] = b * NB + c;
-            }
-        }
-        if (t < NB) {
-            D.cell_start[b * NB + t] = start + cst[t];
-            D.cell_count[b * NB + t] = hist[t];
+            ska[i] = k;
+            pa[i] = D.tmp[start + i];
         }
     }
+    __syncthreads();
+    if (t < 64) {
+        const int v = t < NB ? hist[t] : 0;
+        const int ex = waveInclusiveScan(v) - v;
+        if (t < NB) {
+            cst[t] = ex;
+            cur[t] = ex;
+        }
+    }
+    __syncthreads();
+    if (staged) {
+        // group by cell
+        for (int i = t; i < cnt; i += WG) {
+            const uint32_t k = ska[i];
+            const int dst = atomicAdd(&cur[k >> idbits], 1);
+            skb[dst] = k;
+            pb[dst] = pa[i];
+        }
+        __syncthreads();
+        // rank inside the cell; write the sorted block
+        for (int i = t; i < cnt; i += WG) {
+            const uint32_t k = skb[i];
+            const int c = (int)(k >> idbits);
+            const int c0 = cst[c], c1 = c0 + hist[c];
+            int pos = c0;
+            for (int j = c0; j < c1; j++) pos += skb[j] < k;
+            D.perm[start + pos] = pb[i];
+            D.spid[start + pos] = (int)(k & idmask);
+            D.sorted_cell[start + pos] = b * NB + c;
+        }
+    }
+    else {
+        // the same in HBM: group into perm/spid, then rank through tmp
+        for (int i = t; i < cnt; i += WG) {
+            const uint32_t k = D.tmpk[start + i];
+            const int dst = atomicAdd(&cur[k >> idbits], 1);
+            D.perm[start + dst] = D.tmp[start + i];
+            D.spid[start + dst] = (int)(k & idmask);
+        }
+        __syncthreads();
+        __threadfence_block();
+        for (int i = t; i < cnt; i += WG) D.tmp[start + i] = D.perm[start + i];
+        __syncthreads();
+        for (int i = t; i < cnt; i += WG) D.sorted_cell[start + i] = D.spid[start + i];
+        __syncthreads();
+        for (int i = t; i < cnt; i += WG) {
+            const int p = D.tmp[start + i];
+            const int id = D.sorted_cell[start + i];
+            int c = NB - 1;
+            while (c > 0 && cst[c] > i) c--;
+            const int c0 = cst[c], c1 = c0 + hist[c];
+            int pos = c0;
+            for (int j = c0; j < c1; j++) pos += D.sorted_cell[start + j] < id;
+            D.perm[start + pos] = p;
+            D.spid[start + pos] = id;
+        }
+        __syncthreads();
+        for (int i = t; i < cnt; i += WG) {
+            int c = NB - 1;
+            while (c > 0 && cst[c] > i) c--;
+            D.sorted_cell[start + i] = b * NB + c;
+        }
+    }
+    if (t < NB) {
+        D.cell_start[b * NB + t] = start + cst[t];
+        D.cell_count[b * NB + t] = hist[t];
+    }
+    // the sorted block is read back (perm) by this workgroup
+    __threadfence_block();
+    __syncthreads();
 }
 
 // ------------------------------------------------------------- P2G helpers
@@ -848,7 +845,16 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
     __shared__ T wl[4][CH][3];
     __shared__ AccT tile[NGF][TL::TN];
     __shared__ int cs[NB], ce[NB], cl[NB];
+    __shared__ int hist[NB], cur[NB], cst[NB];
     __shared__ int ncl;
+    // the in-block sort stages (key, live index) in the payload area and the
+    // grouped copy in the tile area, both free until the block's first chunk
+    constexpr int SA = (int)(sizeof(q) / 8), SB = (int)(sizeof(tile) / 8);
+    constexpr int STAGE = SA < SB ? SA : SB;
+    uint32_t* const ska = reinterpret_cast<uint32_t*>(&q[0][0]);
+    int* const pa = reinterpret_cast<int*>(&q[0][0]) + SA;
+    uint32_t* const skb = reinterpret_cast<uint32_t*>(&tile[0][0]);
+    int* const pb = reinterpret_cast<int*>(&tile[0][0]) + SB;
 
     const int t = threadIdx.x;
     const bool dmg = D.any_damage != 0;
@@ -858,6 +864,20 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
     const float* sc = D.scales;
     const float scl[NGF] = { sc[SC_M], sc[SC_MV], sc[SC_MV], sc[SC_MV], sc[SC_F], sc[SC_F], sc[SC_F], sc[SC_PF], sc[SC_PW] };
 
+    // in-block order of this workgroup's blocks first (a loop of its own, so
+    // its registers are free again for the scatter)
+    for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const int bs = D.block_start[b], cnt = D.block_count[b];
+        if (cnt <= 0) { // halo-only block
+            if (t < NB) {
+                D.cell_start[b * NB + t] = bs;
+                D.cell_count[b * NB + t] = 0;
+            }
+            continue;
+        }
+        __syncthreads(); // LDS of the previous block's order done
+        blockOrder(D, b, bs, cnt, ska, pa, skb, pb, STAGE, hist, cur, cst);
+    }
     int ovf = 0;
     for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
         __syncthreads(); // previous block flushed

@@ -136,6 +136,7 @@ struct Dev {
     uint64_t* key;     // (page << block_bits | elem) per live particle
     int* rank;         // slot inside the particle's page (k_bin)
     int* tmp;          // sorted position -> live index, before the in-block sort
+    uint32_t* tmpk;    // the same positions: cell << (32 - block bits) | original index
     int* perm;         // sorted position -> live index
     int* sorted_cell;  // sorted position -> dense cell id
     int* spid;         // sorted position -> original index
