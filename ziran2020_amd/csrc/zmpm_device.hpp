@@ -50,11 +50,41 @@ constexpr long long FIXED_MAGIC_BITS = 0x4338000000000000LL;
 
 __device__ __forceinline__ void latch(int* counters, int bits) { atomicOr(&counters[1], bits); }
 
+// ------------------------------------------------------- particle field access
+// Element i of a field whose base (buffer + field * capacity) is uniform: the
+// byte offset is formed in 32 bits (capacity < 2^27), so the access takes the
+// scalar-base + 32-bit vector-offset addressing form and needs no per-lane
+// 64-bit address arithmetic.
+// The base goes through readfirstlane (a no-op on a uniform value) so the
+// optimiser keeps it a scalar instead of folding the field offset into the
+// per-lane address; the access stays in the global address space.
+template <class T>
+using GlobalPtr = __attribute__((address_space(1))) T*;
+
+template <class T>
+__device__ __forceinline__ GlobalPtr<char> scalarBase(const T* p)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (GlobalPtr<char>)(((uint64_t)hi << 32) | lo);
+}
+template <class T>
+__device__ __forceinline__ T ldAt(const T* base, int i)
+{
+    return *(GlobalPtr<const T>)(scalarBase(base) + (uint32_t)i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void stAt(T* base, int i, T v)
+{
+    *(GlobalPtr<T>)(scalarBase(base) + (uint32_t)i * (uint32_t)sizeof(T)) = v;
+}
+
 // ---------------------------------------------------------------- positions
-__device__ __forceinline__ int loadPos(const float* P, int64_t i) { return __float_as_int(P[i]); }
-__device__ __forceinline__ double loadPos(const double* P, int64_t i) { return P[i]; }
-__device__ __forceinline__ void storePos(float* P, int64_t i, int X) { P[i] = __int_as_float(X); }
-__device__ __forceinline__ void storePos(double* P, int64_t i, double x) { P[i] = x; }
+__device__ __forceinline__ int loadPos(const float* base, int i) { return __float_as_int(ldAt(base, i)); }
+__device__ __forceinline__ double loadPos(const double* base, int i) { return ldAt(base, i); }
+__device__ __forceinline__ void storePos(float* base, int i, int X) { stAt(base, i, __int_as_float(X)); }
+__device__ __forceinline__ void storePos(double* base, int i, double x) { stAt(base, i, x); }
 
 // Base node of one axis, floor(x / dx - 1/2) (baseNode<2>, BSplines.h:16-20;
 // the key build of MpmSimulationBase.cpp:1545), and the offset from it in
@@ -243,7 +273,7 @@ __device__ __forceinline__ void binParticle(const Dev<T, AccT>& D, int p, bool a
         if (bits) latch(D.counters, bits);
         uint32_t elem;
         pageElem<T>((uint32_t)bb[0], (uint32_t)bb[1], (uint32_t)bb[2], page, elem);
-        D.key[p] = ((uint64_t)page << (B::xbits + B::ybits + B::zbits)) | elem;
+        stAt(D.key, p, ((uint64_t)page << (B::xbits + B::ybits + B::zbits)) | elem);
     }
     const uint32_t prev = (uint32_t)__shfl_up((int)page, 1);
     const bool head = (lane == 0) || (prev != page);
@@ -264,7 +294,7 @@ __device__ __forceinline__ void binParticle(const Dev<T, AccT>& D, int p, bool a
                 for (int c = 0; c < 2; c++) D.touched[pageOfBlock<T>(D.masks, Bx + a, By + b, Bz + c)] = 1;
     }
     slot = __shfl(slot, start);
-    if (active) D.rank[p] = slot + (lane - start);
+    if (active) stAt(D.rank, p, slot + (lane - start));
 }
 
 // The binning of a substep is done by whatever wrote the particle positions
@@ -584,7 +614,7 @@ __device__ __forceinline__ Mat3<T> loadM(const T* P, int64_t cap, int f, int p)
 {
     Mat3<T> M;
 #pragma unroll
-    for (int i = 0; i < 9; i++) M.a[i / 3][i % 3] = P[(f + i) * cap + p];
+    for (int i = 0; i < 9; i++) M.a[i / 3][i % 3] = ldAt(P + (f + i) * cap, p);
     return M;
 }
 
@@ -737,18 +767,18 @@ __global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D, int bin)
             const MatDev<T>& mt = D.mats[D.pmat[p]];
             Mat3<T> F = loadM(D.P, cap, PF, p);
             T S[6], v[3], Cm[9];
-            vtauSym(mt, F, D.P[PVOL * cap + p], D.P[PG * cap + p], S);
+            vtauSym(mt, F, ldAt(D.P + PVOL * cap, p), ldAt(D.P + PG * cap, p), S);
 #pragma unroll
-            for (int i = 0; i < 6; i++) D.P[(PS + i) * cap + p] = S[i];
+            for (int i = 0; i < 6; i++) stAt(D.P + (PS + i) * cap, p, S[i]);
 #pragma unroll
             for (int a = 0; a < 3; a++) {
-                v[a] = D.P[(PV + a) * cap + p];
-                x[a] = loadPos(D.P, (PX + a) * cap + p);
+                v[a] = ldAt(D.P + (PV + a) * cap, p);
+                x[a] = loadPos(D.P + (PX + a) * cap, p);
             }
 #pragma unroll
-            for (int i = 0; i < 9; i++) Cm[i] = D.P[(PC + i) * cap + p];
+            for (int i = 0; i < 9; i++) Cm[i] = ldAt(D.P + (PC + i) * cap, p);
             const bool dmg = D.any_damage && mt.has_damage;
-            particleBounds(b, D.P[PM * cap + p], v, Cm, S, dmg ? D.P[PD * cap + p] : T(0), D.dx, D.odx, D.any_damage != 0);
+            particleBounds(b, ldAt(D.P + PM * cap, p), v, Cm, S, dmg ? ldAt(D.P + PD * cap, p) : T(0), D.dx, D.odx, D.any_damage != 0);
         }
         if (bin) binParticle(D, p, active, x);
     }
@@ -803,21 +833,21 @@ struct RawP2G {
 template <class T, class AccT>
 __device__ __forceinline__ void loadP2G(const Dev<T, AccT>& D, int s, bool dmg, RawP2G<T>& r)
 {
-    int p = D.perm[s];
+    int p = ldAt(D.perm, s);
     if ((unsigned)p >= (unsigned)D.n) p = 0;
     const int64_t cap = D.cap;
     const T* P = D.P;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        r.x[a] = loadPos(P, (PX + a) * cap + p);
-        r.v[a] = P[(PV + a) * cap + p];
+        r.x[a] = loadPos(P + (PX + a) * cap, p);
+        r.v[a] = ldAt(P + (PV + a) * cap, p);
     }
 #pragma unroll
-    for (int i = 0; i < 9; i++) r.C[i] = P[(PC + i) * cap + p];
+    for (int i = 0; i < 9; i++) r.C[i] = ldAt(P + (PC + i) * cap, p);
 #pragma unroll
-    for (int i = 0; i < 6; i++) r.S[i] = P[(PS + i) * cap + p];
-    r.m = P[PM * cap + p];
-    r.d = dmg && D.mats[D.pmat[p]].has_damage ? P[PD * cap + p] : T(0);
+    for (int i = 0; i < 6; i++) r.S[i] = ldAt(P + (PS + i) * cap, p);
+    r.m = ldAt(P + PM * cap, p);
+    r.d = dmg && D.mats[D.pmat[p]].has_damage ? ldAt(P + PD * cap, p) : T(0);
 }
 
 // particlesToGridWithForceHelper<APIC, deg 2, !MLS> (MpmSimulationBase.cpp:791-854)
@@ -1368,15 +1398,15 @@ __device__ __forceinline__ void loadG2P(const Dev<T, AccT>& D, int s, int p, Raw
     const int64_t cap = D.cap;
     const T* P = D.P;
 #pragma unroll
-    for (int a = 0; a < 3; a++) r.x[a] = loadPos(P, (PX + a) * cap + p);
+    for (int a = 0; a < 3; a++) r.x[a] = loadPos(P + (PX + a) * cap, p);
 #pragma unroll
-    for (int i = 0; i < 9; i++) r.F[i] = P[(PF + i) * cap + p];
-    r.m = P[PM * cap + p];
-    r.V0 = P[PVOL * cap + p];
-    r.d = P[PD * cap + p];
-    r.g = P[PG * cap + p];
-    r.mid = D.pmat[p];
-    r.id = D.spid[s];
+    for (int i = 0; i < 9; i++) r.F[i] = ldAt(P + (PF + i) * cap, p);
+    r.m = ldAt(P + PM * cap, p);
+    r.V0 = ldAt(P + PVOL * cap, p);
+    r.d = ldAt(P + PD * cap, p);
+    r.g = ldAt(P + PG * cap, p);
+    r.mid = ldAt(D.pmat, p);
+    r.id = ldAt(D.spid, s);
 }
 
 template <class T, class AccT>
@@ -1395,13 +1425,13 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     {
         T Cn[9];
 #pragma unroll
-        for (int a = 0; a < 3; a++) Q[(PV + a) * cap + s] = A.picV[a];
+        for (int a = 0; a < 3; a++) stAt(Q + (PV + a) * cap, s, A.picV[a]);
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 Cn[3 * a + c] = ca * (A.Bp[a][c] * Dinv) + cb * (A.Bp[c][a] * Dinv);
-                Q[(PC + 3 * a + c) * cap + s] = Cn[3 * a + c];
+                stAt(Q + (PC + 3 * a + c) * cap, s, Cn[3 * a + c]);
             }
         T inc[3], inc2 = T(0);
 #pragma unroll
@@ -1426,15 +1456,15 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
             }
             else
                 xnew[a] = R.x[a] + inc[a];
-            storePos(Q, (PX + a) * cap + s, xnew[a]);
+            storePos(Q + (PX + a) * cap, s, xnew[a]);
         }
         if constexpr (FIXED) momentumBounds(bnd, R.m, A.picV, Cn, dx);
     }
-    Q[PM * cap + s] = R.m;
-    Q[PVOL * cap + s] = R.V0;
-    Q[PLAP * cap + s] = dmg ? A.lap : T(0);
-    D.qid[s] = R.id;
-    D.qmat[s] = R.mid;
+    stAt(Q + PM * cap, s, R.m);
+    stAt(Q + PVOL * cap, s, R.V0);
+    stAt(Q + PLAP * cap, s, dmg ? A.lap : T(0));
+    stAt(D.qid, s, R.id);
+    stAt(D.qmat, s, R.mid);
     T d = R.d, g = R.g;
     Mat3<T> Fn;
 #pragma unroll
@@ -1462,15 +1492,15 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
                 Fn.a[a][c] = v;
             }
     }
-    Q[PD * cap + s] = d;
-    Q[PG * cap + s] = g;
+    stAt(Q + PD * cap, s, d);
+    stAt(Q + PG * cap, s, g);
 #pragma unroll
-    for (int i = 0; i < 9; i++) Q[(PF + i) * cap + s] = Fn.a[i / 3][i % 3];
+    for (int i = 0; i < 9; i++) stAt(Q + (PF + i) * cap, s, Fn.a[i / 3][i % 3]);
     // stress of the new state for the next substep's P2G
     T S[6];
     vtauSym(mt, Fn, R.V0, g, S);
 #pragma unroll
-    for (int i = 0; i < 6; i++) Q[(PS + i) * cap + s] = S[i];
+    for (int i = 0; i < 6; i++) stAt(Q + (PS + i) * cap, s, S[i]);
     if constexpr (FIXED) forceBounds(bnd, S, dmg ? d : T(0), D.odx, D.any_damage != 0);
 }
 
@@ -1494,7 +1524,7 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
     // complete before the gathers
     const bool dmg = D.any_damage != 0;
     if (live) {
-        int p = D.perm[s];
+        int p = ldAt(D.perm, s);
         if ((unsigned)p >= (unsigned)D.n) p = 0;
         loadG2P(D, s, p, R);
     }
