@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define ZMPM_ABI_VERSION 1
+#define ZMPM_ABI_VERSION 2
 
 /* Error model.  Replaces ZIRAN_ASSERT -> std::runtime_error
  * (Lib/Ziran/CS/Util/Debug.h:19-37) and the CFL RestartException
@@ -79,6 +79,13 @@ typedef enum {
     ZMPM_SHAPE_BOX = 1
 } zmpm_shape;
 
+/* Particle-grid transfer scheme; values equal MpmSimulationBase::TransferScheme
+ * (Lib/MPM/MpmSimulationBase.h:47-51).  Quadratic B-splines (ZIRAN_MPM_DEGREE 2). */
+typedef enum {
+    ZMPM_APIC_BLEND_RPIC = 0, /* APIC, C blended with its skew part (apic_rpic_ratio)   */
+    ZMPM_FLIP_BLEND_PIC = 1   /* FLIP blended with PIC (flip_pic_ratio; 0 = PIC), no C   */
+} zmpm_transfer;
+
 /* Scene-wide parameters.  Mirrors the public members of MpmSimulationBase
  * (Lib/MPM/MpmSimulationBase.h:47-76) that the explicit step reads. */
 typedef struct {
@@ -91,6 +98,13 @@ typedef struct {
     double apic_rpic_ratio;  /* MpmSimulationBase::apic_rpic_ratio (default 1)            */
     double cfl;              /* MpmSimulationBase::cfl (half-cell flag only)              */
     int32_t device;          /* HIP device ordinal                                        */
+    int32_t transfer_scheme; /* zmpm_transfer, MpmSimulationBase::transfer_scheme
+                                (MpmSimulationBase.h:52; default APIC_blend_RPIC = 0)      */
+    double flip_pic_ratio;   /* MpmSimulationBase::flip_pic_ratio (FLIP_blend_PIC)        */
+    int32_t mls_mpm;         /* MpmSimulationBase::mls_mpm: MLS-MPM force and grad v = C
+                                (APIC only, MpmSimulationBase.cpp:108-112)                */
+    int32_t rpic_damping_iteration; /* MpmSimulationBase::rpic_damping_iteration (APIC)   */
+    int32_t pic_damping_iteration;  /* MpmSimulationBase::pic_damping_iteration (FLIP)    */
     int32_t reserved;
 } zmpm_scene_desc;
 

@@ -859,6 +859,7 @@ struct zmpm_context {
     std::vector<std::pair<int, int>> groups;
     std::vector<uint64_t> block_offset;
     std::vector<uint64_t> base_offset;
+    std::vector<int32_t> base_node; // 3 per particle: base node of the substep's sort
     // grid (page map in first-touch order, MpmGrid::pollute)
     std::unordered_map<uint64_t, int> page_to_block;
     std::vector<uint64_t> block_pages;
@@ -958,6 +959,7 @@ zmpm_status sortAndPollute(zmpm_context* h)
     if (n >= ((int64_t)1 << index_bits)) return fail(h, ZMPM_E_ARG, "too many particles for the key layout");
     h->keys.resize(n);
     h->base_offset.resize(n);
+    h->base_node.resize(3 * n);
     h->order.resize(n);
     bool bad = false, nan = false;
     for (int64_t i = 0; i < n; i++) {
@@ -970,6 +972,7 @@ zmpm_status sortAndPollute(zmpm_context* h)
         }
         uint64_t off = linearOffset(L, b[0], b[1], b[2]);
         h->base_offset[i] = off;
+        for (int a = 0; a < 3; a++) h->base_node[3 * i + a] = b[a];
         h->keys[i] = ((off >> L.data_bits) << index_bits) + (uint64_t)i;
     }
     if (bad) {
@@ -1023,9 +1026,14 @@ inline void makeStencil(const zmpm_context* h, const double* xp, Stencil& s)
     for (int a = 0; a < 3; a++) bspline(indexSpace(h, xp[a]), s.base[a], s.w[a], s.dw[a], s.ddw[a]);
 }
 
+// addr: the base node whose stencil the grid memory is taken from; null for
+// s.base.  The node coordinates passed to op are always s.base + (i, j, k):
+// iterateKernel reads memory at particle_base_offset but reports the spline's
+// own base node (MpmGrid.h:437-488).
 template <class OP>
-inline void iterateKernel(zmpm_context* h, const Stencil& s, const OP& op)
+inline void iterateKernel(zmpm_context* h, const Stencil& s, const OP& op, const int* addr = nullptr)
 {
+    const int* ab = addr ? addr : s.base;
     const double odx = s.odx;
     for (int i = 0; i < 3; i++) {
         double wi = s.w[0][i];
@@ -1045,7 +1053,7 @@ inline void iterateKernel(zmpm_context* h, const Stencil& s, const OP& op)
                 double t2 = odx * odx * s.ddw[1][i] * wi * wk;
                 double t3 = odx * odx * s.ddw[2][i] * wi * wj;
                 int node[3] = { s.base[0] + i, s.base[1] + j, s.base[2] + k };
-                Node* g = nodeAt(h, node[0], node[1], node[2]);
+                Node* g = nodeAt(h, ab[0] + i, ab[1] + j, ab[2] + k);
                 op(node, wijk, dw3, t1 + t2 + t3, *g);
             }
         }
@@ -1053,6 +1061,89 @@ inline void iterateKernel(zmpm_context* h, const Stencil& s, const OP& op)
 }
 
 const zmpm_material& matOf(const zmpm_context* h, int64_t i) { return h->mats[(size_t)h->mat[i]]; }
+
+// gridToParticlesHelper<scheme, deg 2, mls> generic path (MpmSimulationBase.cpp:1412-1478)
+// and evolveStrain: F <- (I + dt gradV) F (FBasedMpmForceHelper.cpp:93-108);
+// J <- (1 + dt tr gradV) J (JBasedMpmForceHelper.cpp:79-94).  APIC_blend_RPIC:
+// v = sum w v_i, C from B = sum w v_i (x_i - x_p)^T, grad v = sum v_i grad(w)^T
+// (MLS: grad v = C).  FLIP_blend_PIC: v <- ratio v + sum w v_i - ratio sum w v_i^old,
+// grad v = sum v_i grad(w)^T; C, which the reference does not keep for FLIP, is
+// stored as 0.  old_base: grid memory through the substep's base offsets (the
+// damping passes, whose particles have moved).  Returns the CFL flag.
+int gridToParticlesPass(zmpm_context* h, double dt, double r, double flip, bool old_base)
+{
+    const int64_t n = h->n;
+    const double dx = h->desc.dx;
+    const double dx2 = dx * dx;
+    const double D_inverse = 4 / dx2;
+    const bool apic = h->desc.transfer_scheme == ZMPM_APIC_BLEND_RPIC;
+    const bool mls = h->desc.mls_mpm != 0;
+    const bool fp32_layout = h->desc.precision == ZMPM_PREC_FP32;
+    int cfl_flag = 0;
+    std::vector<double> gradV((size_t)n * 9);
+#pragma omp parallel for schedule(static) reduction(| : cfl_flag)
+    for (int64_t i = 0; i < n; i++) {
+        double* Xp = &h->x[3 * i];
+        double picV[3] = { 0, 0, 0 }, oldV[3] = { 0, 0, 0 }, Bp[3][3] = {}, gv[3][3] = {};
+        Stencil s;
+        makeStencil(h, Xp, s);
+        iterateKernel(h, s, [&](const int* node, double w, const double* dw, double, Node& g) {
+            double xi_xp[3];
+            for (int a = 0; a < 3; a++) xi_xp[a] = (double)node[a] * dx - Xp[a];
+            for (int a = 0; a < 3; a++) picV[a] += w * g.new_v[a];
+            if (!apic)
+                for (int a = 0; a < 3; a++) oldV[a] += w * g.v[a];
+            for (int a = 0; a < 3; a++) {
+                double wv = w * g.new_v[a];
+                for (int c = 0; c < 3; c++) {
+                    Bp[a][c] += wv * xi_xp[c];
+                    gv[a][c] += g.new_v[a] * dw[c];
+                }
+            }
+        }, old_base ? &h->base_node[3 * i] : nullptr);
+        if (apic) {
+            for (int a = 0; a < 3; a++) h->v[3 * i + a] = picV[a];
+            for (int a = 0; a < 3; a++)
+                for (int c = 0; c < 3; c++) {
+                    double CC = Bp[a][c] * D_inverse, CCt = Bp[c][a] * D_inverse;
+                    h->C[9 * i + 3 * a + c] = ((r + 1) * 0.5) * CC + ((r - 1) * 0.5) * CCt;
+                    gradV[9 * i + 3 * a + c] = mls ? h->C[9 * i + 3 * a + c] : gv[a][c];
+                }
+        }
+        else {
+            for (int a = 0; a < 3; a++) {
+                double& va = h->v[3 * i + a];
+                va *= flip;
+                va += picV[a] - flip * oldV[a];
+            }
+            for (int k = 0; k < 9; k++) {
+                h->C[9 * i + k] = 0.0;
+                gradV[9 * i + k] = gv[k / 3][k % 3];
+            }
+        }
+        double inc[3] = { dt * picV[0], dt * picV[1], dt * picV[2] };
+        for (int a = 0; a < 3; a++) Xp[a] += inc[a];
+        if (fp32_layout)
+            for (int a = 0; a < 3; a++) Xp[a] = quantizePos(Xp[a], dx);
+        double incn = inc[0] * inc[0] + inc[1] * inc[1] + inc[2] * inc[2];
+        if (incn > dx2) cfl_flag |= 1;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        const double* gv = &gradV[9 * i];
+        double* Fp = &h->F[9 * i];
+        if (matOf(h, i).model == ZMPM_MODEL_EQUATION_OF_STATE) {
+            Fp[0] = (1 + dt * (gv[0] + gv[4] + gv[8])) * Fp[0];
+            continue;
+        }
+        M3 A;
+        for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++) A.a[a][c] = (a == c ? 1.0 : 0.0) + dt * gv[3 * a + c];
+        M3 Fn = mmul(A, mload(Fp));
+        mstore(Fn, Fp);
+    }
+    return cfl_flag;
+}
 
 // One explicit substep (MpmSimulationBase::advanceOneTimeStep, MpmSimulationBase.cpp:262-357).
 zmpm_status substep(zmpm_context* h, double dt, double t)
@@ -1071,8 +1162,12 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
         mstore(tau, &vtau[9 * i]);
     }
 
-    // particlesToGridWithForceHelper<APIC, deg 2, !MLS> generic path (MpmSimulationBase.cpp:791-854)
+    // particlesToGridWithForceHelper<scheme, deg 2, mls> generic path (MpmSimulationBase.cpp:791-854):
+    // C = mass C for APIC_blend_RPIC, 0 for FLIP_blend_PIC; MLS force S (x_i - x_p) w D^-1
     const int ng = (int)h->groups.size();
+    const bool apic = h->desc.transfer_scheme == ZMPM_APIC_BLEND_RPIC;
+    const bool mls = h->desc.mls_mpm != 0;
+    const double D_inverse = 4 / (dx * dx);
     for (uint64_t color = 0; color < 8; color++) {
 #pragma omp parallel for schedule(dynamic, 4)
         for (int gi = 0; gi < ng; gi++) {
@@ -1084,7 +1179,7 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
                 double momentum[3] = { mass * h->v[3 * i], mass * h->v[3 * i + 1], mass * h->v[3 * i + 2] };
                 double Cm[3][3];
                 for (int r = 0; r < 3; r++)
-                    for (int c = 0; c < 3; c++) Cm[r][c] = mass * h->C[9 * i + 3 * r + c];
+                    for (int c = 0; c < 3; c++) Cm[r][c] = apic ? mass * h->C[9 * i + 3 * r + c] : 0.0;
                 const double* S = &vtau[9 * i];
                 Stencil s;
                 makeStencil(h, Xp, s);
@@ -1095,7 +1190,11 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
                     for (int r = 0; r < 3; r++) {
                         // (velocity_density * [xi-xp; 1]) * w, stress_density * [dw; w] with fp = 0
                         vd[r] = (Cm[r][0] * xi_xp[0] + Cm[r][1] * xi_xp[1] + Cm[r][2] * xi_xp[2] + momentum[r]) * w;
-                        fd[r] = S[3 * r + 0] * dw[0] + S[3 * r + 1] * dw[1] + S[3 * r + 2] * dw[2] + (-0.0) * w;
+                        if (mls) // (stress_density * [xi - xp; 1]) * (w D^-1), fp = 0
+                            fd[r] = (S[3 * r + 0] * xi_xp[0] + S[3 * r + 1] * xi_xp[1] + S[3 * r + 2] * xi_xp[2] + (-0.0) * 1.0)
+                                * (w * D_inverse);
+                        else
+                            fd[r] = S[3 * r + 0] * dw[0] + S[3 * r + 1] * dw[1] + S[3 * r + 2] * dw[2] + (-0.0) * w;
                     }
                     g.m += mass * w;
                     for (int r = 0; r < 3; r++) {
@@ -1243,61 +1342,55 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
         }
     }
 
-    // gridToParticlesHelper<APIC, deg 2, !MLS> generic path (MpmSimulationBase.cpp:1412-1478)
-    const double dx2 = dx * dx;
-    const double D_inverse = 4 / dx2;
-    const double r = h->desc.apic_rpic_ratio;
-    int cfl_flag = 0;
-    const bool fp32_layout = h->desc.precision == ZMPM_PREC_FP32;
-    std::vector<double> gradV((size_t)n * 9);
-#pragma omp parallel for schedule(static) reduction(| : cfl_flag)
-    for (int64_t i = 0; i < n; i++) {
-        double* Xp = &h->x[3 * i];
-        double picV[3] = { 0, 0, 0 }, Bp[3][3] = {}, gv[3][3] = {};
-        Stencil s;
-        makeStencil(h, Xp, s);
-        iterateKernel(h, s, [&](const int* node, double w, const double* dw, double, Node& g) {
-            double xi_xp[3];
-            for (int a = 0; a < 3; a++) xi_xp[a] = (double)node[a] * dx - Xp[a];
-            for (int a = 0; a < 3; a++) picV[a] += w * g.new_v[a];
-            for (int a = 0; a < 3; a++) {
-                double wv = w * g.new_v[a];
-                for (int c = 0; c < 3; c++) {
-                    Bp[a][c] += wv * xi_xp[c];
-                    gv[a][c] += g.new_v[a] * dw[c];
+    int cfl_flag = gridToParticlesPass(h, dt, h->desc.apic_rpic_ratio, h->desc.flip_pic_ratio, false);
+
+    // damping passes (MpmSimulationBase.cpp:318-348): grid m, v cleared on the
+    // substep's active nodes, particlesToGridHelper (no force), v = mv/m, new_v = v,
+    // gridToParticles(0) with apic_rpic_ratio = 0 (RPIC) or flip_pic_ratio = 0 (PIC)
+    const int damping = apic ? h->desc.rpic_damping_iteration : h->desc.pic_damping_iteration;
+    for (int iter = 0; iter < damping; iter++) {
+        for (Node& g : h->nodes)
+            if (g.idx >= 0) {
+                g.m = 0;
+                g.v[0] = g.v[1] = g.v[2] = 0;
+            }
+        // particlesToGridHelper<scheme, deg 2> (MpmSimulationBase.cpp:860-900) at the
+        // moved positions, through the substep's base offsets
+        for (uint64_t color = 0; color < 8; color++) {
+#pragma omp parallel for schedule(dynamic, 4)
+            for (int gi = 0; gi < ng; gi++) {
+                if ((h->block_offset[gi] & 7) != color) continue;
+                for (int idx = h->groups[gi].first; idx <= h->groups[gi].second; idx++) {
+                    int i = h->order[idx];
+                    const double* Xp = &h->x[3 * i];
+                    double mass = h->m[i];
+                    double momentum[3] = { mass * h->v[3 * i], mass * h->v[3 * i + 1], mass * h->v[3 * i + 2] };
+                    double Cm[3][3];
+                    for (int r = 0; r < 3; r++)
+                        for (int c = 0; c < 3; c++) Cm[r][c] = apic ? mass * h->C[9 * i + 3 * r + c] : 0.0;
+                    Stencil s;
+                    makeStencil(h, Xp, s);
+                    iterateKernel(h, s, [&](const int* node, double w, const double*, double, Node& g) {
+                        double xi_xp[3];
+                        for (int a = 0; a < 3; a++) xi_xp[a] = (double)node[a] * dx - Xp[a];
+                        g.m += mass * w;
+                        for (int r = 0; r < 3; r++)
+                            g.v[r] += (Cm[r][0] * xi_xp[0] + Cm[r][1] * xi_xp[1] + Cm[r][2] * xi_xp[2] + momentum[r]) * w;
+                    }, &h->base_node[3 * i]);
                 }
             }
-        });
-        for (int a = 0; a < 3; a++) h->v[3 * i + a] = picV[a];
-        for (int a = 0; a < 3; a++)
-            for (int c = 0; c < 3; c++) {
-                double CC = Bp[a][c] * D_inverse, CCt = Bp[c][a] * D_inverse;
-                h->C[9 * i + 3 * a + c] = ((r + 1) * 0.5) * CC + ((r - 1) * 0.5) * CCt;
-                gradV[9 * i + 3 * a + c] = gv[a][c];
-            }
-        double inc[3] = { dt * picV[0], dt * picV[1], dt * picV[2] };
-        for (int a = 0; a < 3; a++) Xp[a] += inc[a];
-        if (fp32_layout)
-            for (int a = 0; a < 3; a++) Xp[a] = quantizePos(Xp[a], dx);
-        double incn = inc[0] * inc[0] + inc[1] * inc[1] + inc[2] * inc[2];
-        if (incn > dx2) cfl_flag |= 1;
-    }
-
-    // evolveStrain: F <- (I + dt gradV) F (FBasedMpmForceHelper.cpp:93-108);
-    // J <- (1 + dt tr gradV) J (JBasedMpmForceHelper.cpp:79-94)
-#pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < n; i++) {
-        const double* gv = &gradV[9 * i];
-        double* Fp = &h->F[9 * i];
-        if (matOf(h, i).model == ZMPM_MODEL_EQUATION_OF_STATE) {
-            Fp[0] = (1 + dt * (gv[0] + gv[4] + gv[8])) * Fp[0];
-            continue;
         }
-        M3 A;
-        for (int a = 0; a < 3; a++)
-            for (int c = 0; c < 3; c++) A.a[a][c] = (a == c ? 1.0 : 0.0) + dt * gv[3 * a + c];
-        M3 Fn = mmul(A, mload(Fp));
-        mstore(Fn, Fp);
+        int64_t cnt = 0;
+        for (Node& g : h->nodes)
+            if (g.m != 0) g.idx = cnt++;
+        for (Node& g : h->nodes) {
+            if (g.idx < 0) continue;
+            for (int a = 0; a < 3; a++) {
+                g.v[a] = g.m != 0 ? g.v[a] / g.m : 0.0;
+                g.new_v[a] = g.v[a];
+            }
+        }
+        cfl_flag |= gridToParticlesPass(h, 0.0, 0.0, 0.0, true);
     }
     h->stepped = true;
     if (cfl_flag && h->latched == ZMPM_OK) h->latched = ZMPM_E_CFL;
@@ -1318,6 +1411,13 @@ zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
     int dn = desc->domain_nodes;
     if (dn < 64 || dn > 1024 || (dn & (dn - 1)) || !(desc->dx > 0)) return ZMPM_E_ARG;
     if (desc->precision != ZMPM_PREC_FP32 && desc->precision != ZMPM_PREC_FP64) return ZMPM_E_ARG;
+    if (desc->transfer_scheme != ZMPM_APIC_BLEND_RPIC && desc->transfer_scheme != ZMPM_FLIP_BLEND_PIC) return ZMPM_E_ARG;
+    // MpmSimulationBase::initialize / advanceOneTimeStep assertions (MpmSimulationBase.cpp:108-128, 319, 335)
+    const bool apic = desc->transfer_scheme == ZMPM_APIC_BLEND_RPIC;
+    if (desc->mls_mpm && !apic) return ZMPM_E_ARG;
+    if (apic && !(desc->apic_rpic_ratio >= 0 && desc->apic_rpic_ratio <= 1)) return ZMPM_E_ARG;
+    if (desc->rpic_damping_iteration < 0 || desc->pic_damping_iteration < 0) return ZMPM_E_ARG;
+    if ((desc->rpic_damping_iteration > 0 && !apic) || (desc->pic_damping_iteration > 0 && apic)) return ZMPM_E_ARG;
     auto* h = new zmpm_context();
     h->desc = *desc;
     h->L = makeLayout(desc->precision == ZMPM_PREC_FP32 ? 6 : 7);

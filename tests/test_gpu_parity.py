@@ -256,14 +256,57 @@ def test_full_size_damage_slab_conservation(gpu):
     assert np.array_equal(np.sort(b["order"]), np.arange(sc.n))
 
 
+# transfer schemes (MpmSimulationBase::transfer_scheme, mls_mpm, damping passes)
+TRANSFER_CASES = {
+    "pic": dict(transfer_scheme=A.FLIP_BLEND_PIC, flip_pic_ratio=0.0),
+    "flip95": dict(transfer_scheme=A.FLIP_BLEND_PIC, flip_pic_ratio=0.95),
+    "mls": dict(mls_mpm=True),
+    "apic_rpic_half": dict(apic_rpic_ratio=0.5),
+    "rpic_damping": dict(rpic_damping_iteration=3),
+    "pic_damping": dict(transfer_scheme=A.FLIP_BLEND_PIC, flip_pic_ratio=0.0, pic_damping_iteration=2),
+}
+
+
+@pytest.mark.parametrize("case", sorted(TRANSFER_CASES))
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_transfer_schemes(gpu, oracle, case, precision):
+    # moving particles cross cells, so the damping passes' grid addressing through
+    # the substep's binning (weights from the moved positions) is exercised
+    sc = small_cube(cells=10, seed=4)
+    sc.v[:] = np.random.default_rng(8).normal(scale=0.3, size=sc.v.shape)
+    kw = TRANSFER_CASES[case]
+    g = run(gpu, sc, precision, 12, **kw).download()
+    o = run(oracle, sc, A.PREC_FP64, 12, **kw).download()
+    tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+    for k in ("x", "v", "C", "F"):
+        assert normwise(g[k], o[k]) < tol, k
+
+
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_damage_slab_pic(gpu, oracle, precision):
+    # AnisoMPM with FLIP_blend_PIC at flip_pic_ratio = 0, as in the reference's
+    # 3DTubeSmash / cheesePull / meatproxy examples
+    sc = small_slab()
+    kw = dict(transfer_scheme=A.FLIP_BLEND_PIC, flip_pic_ratio=0.0)
+    g = run(gpu, sc, precision, 20, **kw).download()
+    o = run(oracle, sc, A.PREC_FP64, 20, **kw).download()
+    assert o["d"].max() > 0.01
+    tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+    for k in ("x", "v", "F", "d", "g"):
+        assert normwise(g[k], o[k]) < (tol if k != "d" else 10 * tol), k
+
+
 @pytest.mark.parametrize("precision", [A.PREC_FP32, A.PREC_FP64])
-def test_graph_replays_match_eager_substeps(gpu, precision):
+@pytest.mark.parametrize("transfer", ["apic", "flip_damped"])
+def test_graph_replays_match_eager_substeps(gpu, precision, transfer):
     """zmpm_run replays a captured two-substep HIP graph; every replay, with host
     downloads in between, matches eager one-at-a-time substeps (bitwise for the
     deterministic FP32 form, to FP64 rounding otherwise)."""
     sc = small_slab(n=8000)
-    g = sc.make(gpu, precision)
-    e = sc.make(gpu, precision)
+    kw = {} if transfer == "apic" else dict(transfer_scheme=A.FLIP_BLEND_PIC, flip_pic_ratio=0.5,
+                                              pic_damping_iteration=2)
+    g = sc.make(gpu, precision, **kw)
+    e = sc.make(gpu, precision, **kw)
     t = 0.0
     for rep in range(5):
         g.run(2, sc.dt, t)

@@ -547,3 +547,112 @@ def test_oracle_thread_count_invariance(oracle):
         outs.append(np.load(f))
         os.remove(f)
     assert np.array_equal(outs[0], outs[1])
+
+
+# ------------------------------------------------------------ transfer schemes
+def _free_cube(cells=5, seed=3, E=0.0):
+    """Cube with no colliders and no gravity; E = 0 makes every stress zero."""
+    sc = cube_scene(cells=cells, seed=seed)
+    sc.materials = [A.make_material(A.MODEL_FIXED_COROTATED, E, 0.3)]
+    sc.colliders = []
+    sc.gravity = (0.0, 0.0, 0.0)
+    return sc
+
+
+def test_flip_ratio_one_keeps_particle_velocities(oracle):
+    # FLIP_blend_PIC with flip_pic_ratio = 1 (MpmSimulationBase.cpp:1431-1432):
+    # v <- v + sum w (v_new - v_old); with no force the grid update is the
+    # identity, so every particle keeps its velocity exactly, while x moves with
+    # the PIC velocity.
+    sc = _free_cube()
+    rng = np.random.default_rng(4)
+    sc.v = rng.normal(scale=0.2, size=sc.v.shape)
+    sim = sc.make(oracle, transfer_scheme=A.FLIP_BLEND_PIC, flip_pic_ratio=1.0)
+    sim.substep(sc.dt, 0.0)
+    out = sim.download()
+    assert np.array_equal(out["v"], sc.v)
+    assert np.all(out["C"] == 0.0)
+    assert np.abs(out["x"] - sc.x).max() > 0
+
+
+def test_pic_keeps_uniform_translation(oracle):
+    sc = _free_cube()
+    a = np.array([0.3, -0.1, 0.2])
+    sc.v = np.tile(a, (sc.n, 1))
+    sim = sc.make(oracle, transfer_scheme=A.FLIP_BLEND_PIC, flip_pic_ratio=0.0)
+    sim.substep(sc.dt, 0.0)
+    out = sim.download()
+    assert np.abs(out["v"] - a).max() < 1e-13
+    assert np.abs(out["x"] - (sc.x + sc.dt * a)).max() < 1e-15
+
+
+@pytest.mark.parametrize("mls", [False, True])
+def test_force_moments_of_one_particle(oracle, O, mls):
+    # f_i = -V0 tau grad(w)  or  -V0 tau (x_i - x_p) w D^-1 (MLS, MpmSimulationBase.cpp:823-833):
+    # sum_i f_i = 0 and sum_i f_i x_i^T = -V0 tau for both (quadratic B-spline
+    # with D = dx^2 / 4 I), and the two force fields differ node by node.
+    sc = _free_cube(E=1e4)
+    sc.x, sc.v, sc.C = sc.x[:1] + 0.0123, np.zeros((1, 3)), np.zeros((1, 9))
+    F = np.array([[1.1, 0.05, 0.0], [0.02, 0.95, 0.03], [0.0, -0.04, 1.05]])
+    sc.F, sc.V0, sc.material, sc.rho = F.reshape(1, 9), np.array([1e-6]), np.zeros(1, np.int32), np.array([1e3])
+    sc.m = sc.rho * sc.V0
+    sim = sc.make(oracle, mls_mpm=mls)
+    sim.substep(sc.dt, 0.0)
+    g = sim.grid()
+    f = (g["new_v"] - g["v"]) * g["m"][:, None] / sc.dt
+    xi = g["coords"] * sc.dx
+    tau = sc.V0[0] * (piola(O, sc.materials[0], F) @ F.T)
+    assert np.abs(f.sum(0)).max() < 1e-12 * np.abs(tau).max() / sc.dx
+    assert np.abs(f.T @ xi + tau).max() < 1e-9 * np.abs(tau).max()
+    if mls:
+        sim2 = sc.make(oracle, mls_mpm=False)
+        sim2.substep(sc.dt, 0.0)
+        g2 = sim2.grid()
+        assert np.abs(g2["new_v"] - g["new_v"]).max() > 1e-6 * np.abs(g["new_v"] - g["v"]).max()
+
+
+@pytest.mark.parametrize("scheme", ["rpic", "pic"])
+def test_damping_passes_conserve_momentum(oracle, scheme):
+    # P2G and G2P with the same weights conserve sum m v; with no force and no
+    # colliders the whole substep, damping passes included, keeps it.
+    sc = _free_cube()
+    rng = np.random.default_rng(6)
+    sc.v = rng.normal(scale=0.3, size=sc.v.shape)
+    if scheme == "rpic":
+        sc.C = rng.normal(scale=0.5, size=sc.C.shape)
+        kw = dict(rpic_damping_iteration=3)
+    else:
+        kw = dict(transfer_scheme=A.FLIP_BLEND_PIC, flip_pic_ratio=0.5, pic_damping_iteration=3)
+    sim = sc.make(oracle, **kw)
+    sim.substep(sc.dt, 0.0)
+    out = sim.download()
+    p0, p1 = (sc.m[:, None] * sc.v).sum(0), (out["m"][:, None] * out["v"]).sum(0)
+    assert np.abs(p1 - p0).max() < 1e-12 * np.abs(sc.m[:, None] * sc.v).sum()
+    # damping changes the velocities relative to the undamped substep
+    ref = sc.make(oracle, **{k: v for k, v in kw.items() if "damping" not in k})
+    ref.substep(sc.dt, 0.0)
+    assert np.abs(ref.download()["v"] - out["v"]).max() > 1e-6
+
+
+def test_damping_keeps_uniform_translation(oracle):
+    sc = _free_cube()
+    a = np.array([0.25, 0.1, -0.3])
+    sc.v = np.tile(a, (sc.n, 1))
+    sim = sc.make(oracle, rpic_damping_iteration=4)
+    sim.substep(sc.dt, 0.0)
+    out = sim.download()
+    assert np.abs(out["v"] - a).max() < 1e-13
+    assert np.abs(out["C"]).max() < 1e-10
+
+
+def test_transfer_options_validated(oracle):
+    sc = _free_cube()
+    bad = [dict(transfer_scheme=A.FLIP_BLEND_PIC, mls_mpm=True),
+           dict(transfer_scheme=A.FLIP_BLEND_PIC, rpic_damping_iteration=2),
+           dict(pic_damping_iteration=2),
+           dict(apic_rpic_ratio=1.5),
+           dict(rpic_damping_iteration=-1),
+           dict(transfer_scheme=7)]
+    for kw in bad:
+        with pytest.raises(A.ZmpmError):
+            sc.make(oracle, **kw)

@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-ZMPM_ABI_VERSION = 1
+ZMPM_ABI_VERSION = 2
 
 # zmpm_status
 OK, E_ARG, E_HIP, E_DOMAIN, E_CFL, E_NAN, E_OVERFLOW, E_NOMEM = range(8)
@@ -24,6 +24,7 @@ STATUS_NAMES = {
 PREC_FP64, PREC_FP32 = 0, 1
 MODEL_FIXED_COROTATED, MODEL_QR_STABLE_NEO_HOOKEAN, MODEL_NEO_HOOKEAN, MODEL_EQUATION_OF_STATE = range(4)
 STICKY, SLIP, SEPARATE, GHOST = 1, 2, 3, 4
+APIC_BLEND_RPIC, FLIP_BLEND_PIC = 0, 1
 SHAPE_HALF_SPACE, SHAPE_BOX = 0, 1
 
 D3 = C.c_double * 3
@@ -38,6 +39,11 @@ class SceneDesc(C.Structure):
         ("apic_rpic_ratio", C.c_double),
         ("cfl", C.c_double),
         ("device", C.c_int32),
+        ("transfer_scheme", C.c_int32),
+        ("flip_pic_ratio", C.c_double),
+        ("mls_mpm", C.c_int32),
+        ("rpic_damping_iteration", C.c_int32),
+        ("pic_damping_iteration", C.c_int32),
         ("reserved", C.c_int32),
     ]
 
@@ -198,7 +204,8 @@ class Simulation:
     """One zmpm handle: scene, materials, colliders and particle state."""
 
     def __init__(self, lib: ZmpmLib, *, dx, precision=PREC_FP64, domain_nodes=256, gravity=(0, 0, 0),
-                 apic_rpic_ratio=1.0, cfl=0.6, device=0):
+                 apic_rpic_ratio=1.0, cfl=0.6, device=0, transfer_scheme=APIC_BLEND_RPIC, flip_pic_ratio=0.0,
+                 mls_mpm=False, rpic_damping_iteration=0, pic_damping_iteration=0):
         self.lib = lib
         self.L = lib.lib
         d = SceneDesc()
@@ -206,6 +213,8 @@ class Simulation:
         for a in range(3):
             d.gravity[a] = gravity[a]
         d.apic_rpic_ratio, d.cfl, d.device = apic_rpic_ratio, cfl, device
+        d.transfer_scheme, d.flip_pic_ratio, d.mls_mpm = transfer_scheme, flip_pic_ratio, int(bool(mls_mpm))
+        d.rpic_damping_iteration, d.pic_damping_iteration = rpic_damping_iteration, pic_damping_iteration
         self.desc = d
         h = C.c_void_p()
         self._check(self.L.zmpm_create(C.byref(d), C.byref(h)), "zmpm_create", handle=False)

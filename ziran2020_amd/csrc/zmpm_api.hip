@@ -182,6 +182,7 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.clock, 2);
     ALLOC(D.acc, (int64_t)NGF * D.cap_nodes);
     ALLOC(D.newv, 3 * D.cap_nodes);
+    ALLOC(D.oldv, 3 * D.cap_nodes);
     ALLOC(D.pfn, D.cap_nodes);
     ALLOC(D.scales, NSC);
     ALLOC(D.maxbits, (int64_t)NSC * BOUND_SLOTS);
@@ -222,6 +223,9 @@ void refreshTables(zmpm_context* h)
     D.rpic = (T)h->desc.apic_rpic_ratio;
     D.cfl = (T)h->desc.cfl;
     D.p2g_split = h->p2g_split;
+    D.flip = h->desc.transfer_scheme == ZMPM_FLIP_BLEND_PIC ? 1 : 0;
+    D.mls = h->desc.mls_mpm ? 1 : 0;
+    D.flip_ratio = (T)h->desc.flip_pic_ratio;
 }
 
 template <class T>
@@ -303,15 +307,31 @@ void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
     const int prep = (int)std::min<int64_t>(D.cap_blocks, h->prep_wgs);
     k_scatter<T, AccT><<<gp + prep + 1, WG, 0, st>>>(D, dt, gp, prep);
     if (ev) (void)hipEventRecord(ev[1], st);
-    k_p2g<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
+    if (D.mls)
+        k_p2g<T, AccT, P2G_MLS><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
+    else
+        k_p2g<T, AccT, P2G_STD><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[2], st);
     k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D, (T)dt);
     if (ev) (void)hipEventRecord(ev[3], st);
-    k_g2p<T, AccT><<<D.max_slots, WG, 0, st>>>(D, (T)dt);
-    if (ev) (void)hipEventRecord(ev[4], st);
+    if (D.flip)
+        k_g2p<T, AccT, true><<<D.max_slots, WG, 0, st>>>(D, (T)dt);
+    else
+        k_g2p<T, AccT, false><<<D.max_slots, WG, 0, st>>>(D, (T)dt);
     std::swap(D.P, D.Q);
     std::swap(D.pid, D.qid);
     std::swap(D.pmat, D.qmat);
+    // RPIC / PIC damping passes (MpmSimulationBase.cpp:318-348) on the moved
+    // particles, through this substep's binning
+    const int damping = D.flip ? h->desc.pic_damping_iteration : h->desc.rpic_damping_iteration;
+    constexpr bool FIXED = std::is_same<AccT, long long>::value;
+    for (int it = 0; it < damping; it++) {
+        k_damp_prep<T, AccT><<<gridFor(D.cap_nodes, WG, h->prep_wgs) + (FIXED ? 1 : 0), WG, 0, st>>>(D);
+        k_p2g<T, AccT, P2G_DAMP><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
+        k_damp_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
+        k_damp_g2p<T, AccT><<<D.max_slots, WG, 0, st>>>(D);
+    }
+    if (ev) (void)hipEventRecord(ev[4], st);
 }
 
 zmpm_status deviceStatus(zmpm_context* h, bool sync)
@@ -622,6 +642,13 @@ zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
     const int dn = desc->domain_nodes;
     if (dn < 64 || dn > 1024 || (dn & (dn - 1)) || !(desc->dx > 0)) return ZMPM_E_ARG;
     if (desc->precision != ZMPM_PREC_FP32 && desc->precision != ZMPM_PREC_FP64) return ZMPM_E_ARG;
+    if (desc->transfer_scheme != ZMPM_APIC_BLEND_RPIC && desc->transfer_scheme != ZMPM_FLIP_BLEND_PIC) return ZMPM_E_ARG;
+    // MpmSimulationBase::initialize / advanceOneTimeStep assertions (MpmSimulationBase.cpp:108-128, 319, 335)
+    const bool apic = desc->transfer_scheme == ZMPM_APIC_BLEND_RPIC;
+    if (desc->mls_mpm && !apic) return ZMPM_E_ARG;
+    if (apic && !(desc->apic_rpic_ratio >= 0 && desc->apic_rpic_ratio <= 1)) return ZMPM_E_ARG;
+    if (desc->rpic_damping_iteration < 0 || desc->pic_damping_iteration < 0) return ZMPM_E_ARG;
+    if ((desc->rpic_damping_iteration > 0 && !apic) || (desc->pic_damping_iteration > 0 && apic)) return ZMPM_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ZMPM_E_HIP;
     if (desc->device < 0 || desc->device >= ndev) return ZMPM_E_ARG;

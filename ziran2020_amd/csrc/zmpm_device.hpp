@@ -830,10 +830,10 @@ struct RawP2G {
     T v[3], C[9], S[6], m, d;
 };
 
-template <class T, class AccT>
+template <class T, class AccT, bool DAMP = false>
 __device__ __forceinline__ void loadP2G(const Dev<T, AccT>& D, int s, bool dmg, RawP2G<T>& r)
 {
-    int p = ldAt(D.perm, s);
+    int p = DAMP ? s : ldAt(D.perm, s);
     if ((unsigned)p >= (unsigned)D.n) p = 0;
     const int64_t cap = D.cap;
     const T* P = D.P;
@@ -845,12 +845,19 @@ __device__ __forceinline__ void loadP2G(const Dev<T, AccT>& D, int s, bool dmg, 
 #pragma unroll
     for (int i = 0; i < 9; i++) r.C[i] = ldAt(P + (PC + i) * cap, p);
 #pragma unroll
-    for (int i = 0; i < 6; i++) r.S[i] = ldAt(P + (PS + i) * cap, p);
+    for (int i = 0; i < 6; i++) r.S[i] = DAMP ? T(0) : ldAt(P + (PS + i) * cap, p);
     r.m = ldAt(P + PM * cap, p);
-    r.d = dmg && D.mats[D.pmat[p]].has_damage ? ldAt(P + PD * cap, p) : T(0);
+    r.d = !DAMP && dmg && D.mats[D.pmat[p]].has_damage ? ldAt(P + PD * cap, p) : T(0);
 }
 
-// particlesToGridWithForceHelper<APIC, deg 2, !MLS> (MpmSimulationBase.cpp:791-854)
+// particlesToGridWithForceHelper<scheme, deg 2, mls> (MpmSimulationBase.cpp:791-854).
+// FLIP_blend_PIC: no affine term (C = 0).  MLS-MPM: the force weights grad(w)
+// become w D^-1 (x_i - x_p) (= 4 w (i - d0) / dx per axis), set in phase A.
+// DAMP: the damping passes' particlesToGridHelper (MpmSimulationBase.cpp:860-900):
+// mass and momentum only, particles at their sorted positions (written there by
+// G2P) binned by this substep's sort while their weights come from the moved
+// positions -- the reference reads grid memory through the substep's base
+// offsets (MpmGrid.h:437-488, particle_base_offset).
 // reading the stress of forcesUpdateParticleState (:953-979) from the particle,
 // plus the AnisoMPM phase-field transfer (AnisoFractureSimulation.h:161-176).
 //
@@ -864,9 +871,12 @@ __device__ __forceinline__ void loadP2G(const Dev<T, AccT>& D, int s, bool dmg, 
 // After the last chunk the tile is flushed to HBM with one atomic per non-zero
 // node value.  FP32 form: per-lane float partial sums in sorted order, then
 // int64 fixed point in LDS and HBM, so the result does not depend on scheduling.
-template <class T, class AccT>
+enum : int { P2G_STD = 0, P2G_MLS = 1, P2G_DAMP = 2 };
+
+template <class T, class AccT, int VAR = P2G_STD>
 __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
 {
+    constexpr bool DAMP = VAR == P2G_DAMP, MLS = VAR == P2G_MLS;
     using B = Blk<T>;
     using TL = Tile<T>;
     constexpr int NB = TL::NB, CH = TL::CHUNK;
@@ -887,8 +897,8 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
     int* const pb = reinterpret_cast<int*>(&tile[0][0]) + SB;
 
     const int t = threadIdx.x;
-    const bool dmg = D.any_damage != 0;
-    const int nq = dmg ? NGF : GPF;
+    const bool dmg = !DAMP && D.any_damage != 0;
+    const int nq = DAMP ? GFO : (dmg ? NGF : GPF);
     const int64_t cap = D.cap, cn = D.cap_nodes;
     const int nblocks = D.counters[0];
     const float* sc = D.scales;
@@ -896,7 +906,7 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
 
     // in-block order of this workgroup's blocks first (a loop of its own, so
     // its registers are free again for the scatter)
-    for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    for (int b = blockIdx.x; !DAMP && b < nblocks; b += gridDim.x) {
         const int bs = D.block_start[b], cnt = D.block_count[b];
         if (cnt <= 0) { // halo-only block
             if (t < NB) {
@@ -927,7 +937,7 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
             __syncthreads(); // previous chunk's phase B done with the payload
             // ---- phase A: payload of particle lo + t from the prefetched fields
             if (t < CH && lo + t < hi) {
-                if (!have) loadP2G(D, lo + t, dmg, raw);
+                if (!have) loadP2G<T, AccT, DAMP>(D, lo + t, dmg, raw);
                 const T mass = raw.m;
                 T w[3][3], dw[3][3], xb[3];
                 int base[3];
@@ -940,10 +950,13 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
                         xb[a] = -d0 * D.dx; // x_base - x_p
                     else
                         xb[a] = T(base[a]) * D.dx - raw.x[a];
+                    if constexpr (MLS) // D^-1 dx w (x_i - x_p) in place of dw
+#pragma unroll
+                        for (int i = 0; i < 3; i++) dw[a][i] = (D.Dinv * D.dx) * w[a][i] * (xb[a] + T(i) * D.dx);
                 }
                 T mC[9], a3[3];
 #pragma unroll
-                for (int i = 0; i < 9; i++) mC[i] = mass * raw.C[i];
+                for (int i = 0; i < 9; i++) mC[i] = D.flip ? T(0) : mass * raw.C[i];
 #pragma unroll
                 for (int r = 0; r < 3; r++)
                     a3[r] = mass * raw.v[r] + mC[3 * r] * xb[0] + mC[3 * r + 1] * xb[1] + mC[3 * r + 2] * xb[2];
@@ -982,7 +995,7 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
             // (the double form keeps its registers for phase B)
             have = false;
             if (sizeof(T) == 4 && t < CH && hi + t < be) {
-                loadP2G(D, hi + t, dmg, raw);
+                loadP2G<T, AccT, DAMP>(D, hi + t, dmg, raw);
                 have = true;
             }
             // ---- phase B: lanes = (sub-lane, cell, i, j); R sub-lanes share a column
@@ -1032,7 +1045,9 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
                         acc[k][GMV + 0] += wk * (k == 0 ? a0 : a0 + b0 * fk);
                         acc[k][GMV + 1] += wk * (k == 0 ? a1 : a1 + b1 * fk);
                         acc[k][GMV + 2] += wk * (k == 0 ? a2 : a2 + b2 * fk);
-                        if constexpr (sizeof(T) == 4) { // s.. hold -V0 tau / dx here
+                        if constexpr (DAMP) {
+                        }
+                        else if constexpr (sizeof(T) == 4) { // s.. hold -V0 tau / dx here
                             acc[k][GFO + 0] += wz[k] * u0;
                             acc[k][GFO + 0] += dwz[k] * v0;
                             acc[k][GFO + 1] += wz[k] * u1;
@@ -1046,8 +1061,10 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
                             acc[k][GFO + 1] -= s01 * g0 + s11 * g1 + s12 * g2;
                             acc[k][GFO + 2] -= s02 * g0 + s12 * g1 + s22 * g2;
                         }
-                        acc[k][GPF] += q4.w * wk;
-                        acc[k][GPW] += wk;
+                        if constexpr (!DAMP) {
+                            acc[k][GPF] += q4.w * wk;
+                            acc[k][GPW] += wk;
+                        }
                     }
                 }
                 const int cx = cell >> (B::ybits + B::zbits), cy = (cell >> B::zbits) & (TL::BY - 1), cz = cell & (TL::BZ - 1);
@@ -1116,6 +1133,9 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
         if (m == T(0)) {
 #pragma unroll
             for (int a = 0; a < 3; a++) D.newv[a * cn + node] = f[a];
+            if (D.flip)
+#pragma unroll
+                for (int a = 0; a < 3; a++) D.oldv[a * cn + node] = T(0);
             if (D.any_damage) D.pfn[node] = T(0);
             continue;
         }
@@ -1131,6 +1151,7 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             T v = accValue(D, GMV + a, node, SC_MV) / m;
+            if (D.flip) D.oldv[a * cn + node] = v; // the grid's v before the update (FLIP)
             vi[a] = v + D.gravity[a] * dt + f[a] * dt_over_mi;
         }
         for (int c = 0; c < D.num_cols; c++) {
@@ -1204,6 +1225,60 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
         if (D.any_damage) {
             const T pw = accValue(D, GPW, node, SC_PW);
             D.pfn[node] = pw != T(0) ? accValue(D, GPF, node, SC_PF) / pw : T(0);
+        }
+    }
+}
+
+// Damping pass (MpmSimulationBase.cpp:318-348), before its P2G: the substep's
+// blocks' mass and momentum accumulators cleared; float form: the mass and
+// momentum scales from the bound slots the last G2P wrote (read, not cleared:
+// the next substep's scales take the maximum over every pass).
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_damp_prep(Dev<T, AccT> D)
+{
+    using B = Blk<T>;
+    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
+    const int nb = D.counters[CNT_BLOCKS];
+    const int t = threadIdx.x;
+    if constexpr (std::is_same<AccT, long long>::value) {
+        if (blockIdx.x == gridDim.x - 1) {
+            const int lane = t & 63, g = t >> 6;
+            if (g == SC_M || g == SC_MV) {
+                const float m = waveMax(__uint_as_float(D.maxbits[g * BOUND_SLOTS + lane]));
+                if (lane == 0 && m > 0.f && isfinite(m)) {
+                    int e;
+                    frexpf(m, &e);
+                    int se = (int)FIXED_TARGET_EXP - e;
+                    se = se < -120 ? -120 : (se > 120 ? 120 : se);
+                    D.scales[g] = ldexpf(1.0f, se);
+                }
+            }
+            return;
+        }
+    }
+    const int64_t nn = (int64_t)nb * NB;
+    const int nw = std::is_same<AccT, long long>::value ? (int)gridDim.x - 1 : (int)gridDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * WG + t; e < nn; e += (int64_t)nw * WG)
+#pragma unroll
+        for (int f = GM; f < GFO; f++) D.acc[f * D.cap_nodes + e] = AccT(0);
+}
+
+// Damping pass grid: v = mv/m where m != 0, else 0, and new_v = v
+// (MpmSimulationBase.cpp:322-327 with the particlesToGrid tail :655-666).
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_damp_grid(Dev<T, AccT> D)
+{
+    using B = Blk<T>;
+    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
+    const int64_t nn = (int64_t)D.counters[0] * NB;
+    const int64_t cn = D.cap_nodes;
+    for (int64_t node = (int64_t)blockIdx.x * WG + threadIdx.x; node < nn; node += (int64_t)gridDim.x * WG) {
+        const T m = accValue(D, GM, node, SC_M);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const T v = m != T(0) ? accValue(D, GMV + a, node, SC_MV) / m : T(0);
+            D.newv[a * cn + node] = v;
+            if (D.flip) D.oldv[a * cn + node] = v;
         }
     }
 }
@@ -1285,6 +1360,7 @@ __device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>&
 template <class T>
 struct G2PAcc {
     T picV[3], Bp[3][3], gv[3][3], lap;
+    T oldV[3]; // sum w v_old (FLIP_blend_PIC)
 };
 
 // APIC gather of one particle from the staged block tile.  The 27-node sums
@@ -1295,9 +1371,12 @@ struct G2PAcc {
 // damage Laplacian sum d_i L_i with the reference's 3D weights
 // L = (ddw_x(i) w_y w_z + w_x ddw_x(i) w_z + w_x w_y ddw_x(i)) / dx^2
 // (MpmGrid.h:610-612 index ddw[1], ddw[2] with i as well).
-template <class T, class AccT>
+// FLIP: also sum w v_old over the staged old velocities (fields 4-6).  cell >= 0:
+// the stencil's tile position from that cell of the segment's block (the damping
+// passes: this substep's binning, moved particle); otherwise from the particle.
+template <class T, class AccT, bool FLIP = false>
 __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[Tile<T>::TN], const PosT<T>* Xp,
-                                          bool dmg, G2PAcc<T>& A)
+                                          bool dmg, G2PAcc<T>& A, int cell = -1)
 {
     using TL = Tile<T>;
     const T odx = D.odx, dx = D.dx;
@@ -1317,7 +1396,10 @@ __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[T
         }
     }
     const T wzo[3] = { w[2][0] * off[2][0], w[2][1] * off[2][1], w[2][2] * off[2][2] };
-    const int lx0 = base[0] & (TL::BX - 1), ly0 = base[1] & (TL::BY - 1), lz0 = base[2] & (TL::BZ - 1);
+    using B = Blk<T>;
+    const int lx0 = cell >= 0 ? cell >> (B::ybits + B::zbits) : base[0] & (TL::BX - 1);
+    const int ly0 = cell >= 0 ? (cell >> B::zbits) & (TL::BY - 1) : base[1] & (TL::BY - 1);
+    const int lz0 = cell >= 0 ? cell & (TL::BZ - 1) : base[2] & (TL::BZ - 1);
     T lap = T(0);
 #pragma unroll 1
     for (int i = 0; i < 3; i++) {
@@ -1325,12 +1407,14 @@ __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[T
         // tdy = sum dwy s0, tz = sum wy s2, tdz = sum wy s1
         T tv[3] = { T(0), T(0), T(0) }, ty[3] = { T(0), T(0), T(0) }, tdy[3] = { T(0), T(0), T(0) };
         T tz[3] = { T(0), T(0), T(0) }, tdz[3] = { T(0), T(0), T(0) };
+        T to[3] = { T(0), T(0), T(0) };
         T q0 = T(0), q1 = T(0), q2 = T(0);
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             const int nrow = ((lx0 + i) * TL::TY + (ly0 + j)) * TL::TZ + lz0;
             // s0 = sum wz v, s1 = sum dwz v, s2 = sum wz (z_k - z_p) v; p0 = sum wz d, p1 = sum d
             T s0[3] = { T(0), T(0), T(0) }, s1[3] = { T(0), T(0), T(0) }, s2[3] = { T(0), T(0), T(0) };
+            T so[3] = { T(0), T(0), T(0) };
             T p0 = T(0), p1 = T(0);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
@@ -1340,6 +1424,7 @@ __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[T
                     s0[a] += w[2][k] * nv;
                     s1[a] += dw[2][k] * nv;
                     s2[a] += wzo[k] * nv;
+                    if constexpr (FLIP) so[a] += w[2][k] * gt[4 + a][nrow + k];
                 }
                 if (dmg) {
                     const T ph = gt[3][nrow + k];
@@ -1355,6 +1440,7 @@ __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[T
                 tdy[a] += dwy * s0[a];
                 tz[a] += wy * s2[a];
                 tdz[a] += wy * s1[a];
+                if constexpr (FLIP) to[a] += wy * so[a];
             }
             if (dmg) {
                 q0 += wy * p0;
@@ -1372,6 +1458,7 @@ __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[T
             A.gv[a][0] += dwx * tv[a];
             A.gv[a][1] += wx * tdy[a];
             A.gv[a][2] += wx * tdz[a];
+            if constexpr (FLIP) A.oldV[a] += wx * to[a];
         }
         if (dmg) lap += ((i == 1) ? T(-2) : T(1)) * (q0 + wx * (q1 + q2));
     }
@@ -1389,10 +1476,11 @@ template <class T>
 struct RawG2P {
     PosT<T> x[3];
     T F[9], m, V0, d, g;
+    T v[3]; // FLIP_blend_PIC only
     int mid, id;
 };
 
-template <class T, class AccT>
+template <class T, class AccT, bool FLIP = false>
 __device__ __forceinline__ void loadG2P(const Dev<T, AccT>& D, int s, int p, RawG2P<T>& r)
 {
     const int64_t cap = D.cap;
@@ -1407,9 +1495,12 @@ __device__ __forceinline__ void loadG2P(const Dev<T, AccT>& D, int s, int p, Raw
     r.g = ldAt(P + PG * cap, p);
     r.mid = ldAt(D.pmat, p);
     r.id = ldAt(D.spid, s);
+    if constexpr (FLIP)
+#pragma unroll
+        for (int a = 0; a < 3; a++) r.v[a] = ldAt(P + (PV + a) * cap, p);
 }
 
-template <class T, class AccT>
+template <class T, class AccT, bool FLIP = false>
 __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const RawG2P<T>& R, T dt, const G2PAcc<T>& A,
                                           const MatDev<T>& mt, float* bnd, PosT<T>* xnew)
 {
@@ -1421,18 +1512,40 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     const T Dinv = D.Dinv;
     const T r = D.rpic;
     const T ca = (r + T(1)) * T(0.5), cb = (r - T(1)) * T(0.5);
+    // grad v of evolveStrain: sum v_i grad(w)^T, or C under MLS-MPM (APIC)
+    T gvF[3][3];
     // v, C, x first (values die early: the momentum bound is taken here)
     {
-        T Cn[9];
+        T Cn[9], vn[3];
+        if constexpr (FLIP) {
+            // FLIP_blend_PIC (MpmSimulationBase.cpp:1431-1432): v <- r v, v += pic - r old;
+            // no C (stored as 0)
 #pragma unroll
-        for (int a = 0; a < 3; a++) stAt(Q + (PV + a) * cap, s, A.picV[a]);
+            for (int a = 0; a < 3; a++) {
+                T va = D.flip_ratio * R.v[a];
+                va += A.picV[a] - D.flip_ratio * A.oldV[a];
+                vn[a] = va;
+            }
+#pragma unroll
+            for (int i = 0; i < 9; i++) Cn[i] = T(0);
+        }
+        else {
+#pragma unroll
+            for (int a = 0; a < 3; a++) vn[a] = A.picV[a];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) Cn[3 * a + c] = ca * (A.Bp[a][c] * Dinv) + cb * (A.Bp[c][a] * Dinv);
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++) stAt(Q + (PV + a) * cap, s, vn[a]);
+#pragma unroll
+        for (int i = 0; i < 9; i++) stAt(Q + (PC + i) * cap, s, Cn[i]);
+        const bool mls = !FLIP && D.mls;
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                Cn[3 * a + c] = ca * (A.Bp[a][c] * Dinv) + cb * (A.Bp[c][a] * Dinv);
-                stAt(Q + (PC + 3 * a + c) * cap, s, Cn[3 * a + c]);
-            }
+            for (int c = 0; c < 3; c++) gvF[a][c] = mls ? Cn[3 * a + c] : A.gv[a][c];
         T inc[3], inc2 = T(0);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
@@ -1458,7 +1571,7 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
                 xnew[a] = R.x[a] + inc[a];
             storePos(Q + (PX + a) * cap, s, xnew[a]);
         }
-        if constexpr (FIXED) momentumBounds(bnd, R.m, A.picV, Cn, dx);
+        if constexpr (FIXED) momentumBounds(bnd, R.m, vn, Cn, dx);
     }
     stAt(Q + PM * cap, s, R.m);
     stAt(Q + PVOL * cap, s, R.V0);
@@ -1471,7 +1584,7 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     for (int i = 0; i < 9; i++) Fn.a[i / 3][i % 3] = R.F[i];
     // F <- (I + dt grad v) F; J <- (1 + dt tr grad v) J.  FP32 form on H = F - I:
     // H <- H + G + G H with G = dt grad v; J - 1 <- (J - 1) + dt tr(grad v) J.
-    const T dtr = dt * (A.gv[0][0] + A.gv[1][1] + A.gv[2][2]);
+    const T dtr = dt * (gvF[0][0] + gvF[1][1] + gvF[2][2]);
     if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE) {
         if constexpr (sizeof(T) == 4)
             Fn.a[0][0] = Fn.a[0][0] + dtr * (1.f + Fn.a[0][0]);
@@ -1486,9 +1599,9 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 T v = F.a[a][c];
-                if constexpr (sizeof(T) == 4) v += dt * A.gv[a][c];
+                if constexpr (sizeof(T) == 4) v += dt * gvF[a][c];
 #pragma unroll
-                for (int k = 0; k < 3; k++) v += dt * A.gv[a][k] * F.a[k][c];
+                for (int k = 0; k < 3; k++) v += dt * gvF[a][k] * F.a[k][c];
                 Fn.a[a][c] = v;
             }
     }
@@ -1504,33 +1617,59 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     if constexpr (FIXED) forceBounds(bnd, S, dmg ? d : T(0), D.odx, D.any_damage != 0);
 }
 
+// Grid fields of block b's tile (the block and its forward halo, through the
+// neighbour table) into gt: field f < nf is 0-2 new velocity, 3 phase field
+// (staged only with dmg), 4-6 the grid's velocity before the update (FLIP).
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
+__device__ __forceinline__ void stageTile(const Dev<T, AccT>& D, int b, T (*gt)[Tile<T>::TN], int nf, bool dmg)
 {
     using B = Blk<T>;
     using TL = Tile<T>;
     constexpr int NB = TL::NB;
-    __shared__ T gt[4][TL::TN];
+    const int64_t cn = D.cap_nodes;
+    const int* nb = D.nbr + b * 8;
+    for (int e = threadIdx.x; e < nf * TL::TN; e += WG) {
+        const int f = e / TL::TN, nd = e - TL::TN * (e / TL::TN);
+        if (f == 3 && !dmg) continue;
+        const int tx = nd / (TL::TY * TL::TZ), ty = (nd / TL::TZ) % TL::TY, tz = nd % TL::TZ;
+        const int which = ((tx >= TL::BX) << 2) | ((ty >= TL::BY) << 1) | (tz >= TL::BZ);
+        const int tb = nb[which];
+        T v = T(0);
+        if (tb >= 0) {
+            const int elem = ((tx & (TL::BX - 1)) << (B::ybits + B::zbits)) | ((ty & (TL::BY - 1)) << B::zbits) | (tz & (TL::BZ - 1));
+            const int64_t node = (int64_t)tb * NB + elem;
+            v = f < 3 ? D.newv[f * cn + node] : (f == 3 ? D.pfn[node] : D.oldv[(f - 4) * cn + node]);
+        }
+        gt[f][nd] = v;
+    }
+}
+
+template <class T, class AccT, bool FLIP>
+__global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
+{
+    using TL = Tile<T>;
+    constexpr int NB = TL::NB;
+    constexpr int NGT = FLIP ? 7 : 4;
+    __shared__ T gt[NGT][TL::TN];
     __shared__ int seg_end;
     const int lo = blockIdx.x * WG;
     const int hi = min(lo + WG, D.n);
     const int t = threadIdx.x, s = lo + t;
     const bool live = s < hi;
-    const int64_t cn = D.cap_nodes;
-    const int nfield = D.any_damage ? 4 : 3;
     RawG2P<T> R;
     // the Laplacian is gathered for every particle when any material has damage
     // (stored only for damaged ones), so no per-particle material load has to
     // complete before the gathers
     const bool dmg = D.any_damage != 0;
+    const int nfield = FLIP ? 7 : (dmg ? 4 : 3);
     if (live) {
         int p = ldAt(D.perm, s);
         if ((unsigned)p >= (unsigned)D.n) p = 0;
-        loadG2P(D, s, p, R);
+        loadG2P<T, AccT, FLIP>(D, s, p, R);
     }
     G2PAcc<T> A;
 #pragma unroll
-    for (int a = 0; a < 3; a++) A.picV[a] = T(0);
+    for (int a = 0; a < 3; a++) A.picV[a] = A.oldV[a] = T(0);
 #pragma unroll
     for (int i = 0; i < 9; i++) {
         A.Bp[i / 3][i % 3] = T(0);
@@ -1545,31 +1684,89 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
             const int c = b * NB + NB - 1;
             seg_end = D.cell_start[c] + D.cell_count[c];
         }
-        const int* nb = D.nbr + b * 8;
-        for (int e = t; e < nfield * TL::TN; e += WG) {
-            const int f = e / TL::TN, nd = e - TL::TN * (e / TL::TN);
-            const int tx = nd / (TL::TY * TL::TZ), ty = (nd / TL::TZ) % TL::TY, tz = nd % TL::TZ;
-            const int which = ((tx >= TL::BX) << 2) | ((ty >= TL::BY) << 1) | (tz >= TL::BZ);
-            const int tb = nb[which];
-            T v = T(0);
-            if (tb >= 0) {
-                const int elem = ((tx & (TL::BX - 1)) << (B::ybits + B::zbits)) | ((ty & (TL::BY - 1)) << B::zbits) | (tz & (TL::BZ - 1));
-                const int64_t node = (int64_t)tb * NB + elem;
-                v = f < 3 ? D.newv[f * cn + node] : D.pfn[node];
-            }
-            gt[f][nd] = v;
-        }
+        stageTile(D, b, gt, nfield, dmg);
         __syncthreads();
         const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
-        if (live && s >= seg_lo && s < seg_hi) g2pGather(D, gt, R.x, dmg, A);
+        if (live && s >= seg_lo && s < seg_hi) g2pGather<T, AccT, FLIP>(D, gt, R.x, dmg, A);
         seg_lo = seg_hi;
     }
     float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
     PosT<T> xnew[3] = { 0, 0, 0 };
-    if (live) g2pFinish(D, s, R, dt, A, D.mats[R.mid], bnd, xnew);
+    if (live) g2pFinish<T, AccT, FLIP>(D, s, R, dt, A, D.mats[R.mid], bnd, xnew);
     // binning of the coming substep at the new positions (live index s)
     openPageScan(D);
     binParticle(D, s, live, xnew);
+    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd);
+}
+
+// Damping pass G2P, gridToParticles(0) (MpmSimulationBase.cpp:328, 344) with
+// apic_rpic_ratio = 0 (APIC_blend_RPIC: v = sum w v_i, C the skew part of
+// B D^-1) or flip_pic_ratio = 0 (FLIP_blend_PIC: v = sum w v_i); x and F are
+// unchanged (dt = 0).  The grid is read through this substep's binning (each
+// particle's sorted cell) with weights from its moved position, as the
+// reference's iterateKernel does with particle_base_offset; v and C are
+// rewritten in place (the particles are in sorted order after G2P).  Float
+// form: the momentum bounds of the new v, C join the bound slots.
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_damp_g2p(Dev<T, AccT> D)
+{
+    using TL = Tile<T>;
+    constexpr int NB = TL::NB;
+    __shared__ T gt[3][TL::TN];
+    __shared__ int seg_end;
+    const int lo = blockIdx.x * WG;
+    const int hi = min(lo + WG, D.n);
+    const int t = threadIdx.x, s = lo + t;
+    const bool live = s < hi;
+    const int64_t cap = D.cap;
+    T* P = D.P;
+    PosT<T> x[3] = { 0, 0, 0 };
+    T m = T(0);
+    int cell = 0;
+    if (live) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) x[a] = loadPos(P + (PX + a) * cap, s);
+        m = ldAt(P + PM * cap, s);
+        cell = ldAt(D.sorted_cell, s) & (NB - 1);
+    }
+    G2PAcc<T> A;
+#pragma unroll
+    for (int a = 0; a < 3; a++) A.picV[a] = A.oldV[a] = T(0);
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        A.Bp[i / 3][i % 3] = T(0);
+        A.gv[i / 3][i % 3] = T(0);
+    }
+    A.lap = T(0);
+    int seg_lo = lo;
+    while (seg_lo < hi) {
+        const int b = D.sorted_cell[seg_lo] / NB;
+        __syncthreads(); // previous segment's gathers done
+        if (t == 0) {
+            const int c = b * NB + NB - 1;
+            seg_end = D.cell_start[c] + D.cell_count[c];
+        }
+        stageTile(D, b, gt, 3, false);
+        __syncthreads();
+        const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
+        if (live && s >= seg_lo && s < seg_hi) g2pGather<T, AccT, false>(D, gt, x, false, A, cell);
+        seg_lo = seg_hi;
+    }
+    float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
+    if (live) {
+        T Cn[9];
+        const T Dinv = D.Dinv;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                Cn[3 * a + c] = D.flip ? T(0) : T(0.5) * (A.Bp[a][c] * Dinv) + T(-0.5) * (A.Bp[c][a] * Dinv);
+#pragma unroll
+        for (int a = 0; a < 3; a++) stAt(P + (PV + a) * cap, s, A.picV[a]);
+#pragma unroll
+        for (int i = 0; i < 9; i++) stAt(P + (PC + i) * cap, s, Cn[i]);
+        if constexpr (std::is_same<AccT, long long>::value) momentumBounds(bnd, m, A.picV, Cn, D.dx);
+    }
     if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd);
 }
 

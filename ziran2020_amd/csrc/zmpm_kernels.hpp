@@ -159,6 +159,7 @@ struct Dev {
     // grid
     AccT* acc;         // [NGF][cap_nodes]
     T* newv;           // [3][cap_nodes]
+    T* oldv;           // [3][cap_nodes] v = mv/m before the update (FLIP_blend_PIC)
     T* pfn;            // [cap_nodes]
     int64_t cap_nodes;
     // fixed-point scales (float form only)
@@ -172,6 +173,10 @@ struct Dev {
     int any_damage;
     // launch parameters
     int p2g_split; // most lanes sharing one stencil column of a cell in P2G (>= 1)
+    // transfer scheme (MpmSimulationBase::transfer_scheme, mls_mpm, flip_pic_ratio)
+    int flip;      // FLIP_blend_PIC: P2G without C, G2P v <- r v + sum w (v_new - r v_old)
+    int mls;       // MLS-MPM: force S (x_i - x_p) w D^-1, grad v = C
+    T flip_ratio;
     // scene
     T dx, odx;
     T pos_scale; // FP32 form: metres -> position units, odx * 2^POS_BITS

@@ -50,9 +50,11 @@ class Scene:
     def n(self):
         return int(self.x.shape[0])
 
-    def make(self, lib, precision=A.PREC_FP64, device=0, apic_rpic_ratio=1.0):
+    def make(self, lib, precision=A.PREC_FP64, device=0, apic_rpic_ratio=1.0, **transfer):
+        """transfer: transfer_scheme, flip_pic_ratio, mls_mpm, rpic_damping_iteration,
+        pic_damping_iteration (zmpm_scene_desc)."""
         sim = A.Simulation(lib, dx=self.dx, precision=precision, domain_nodes=self.domain_nodes,
-                           gravity=self.gravity, apic_rpic_ratio=apic_rpic_ratio, device=device)
+                           gravity=self.gravity, apic_rpic_ratio=apic_rpic_ratio, device=device, **transfer)
         sim.set_materials(self.materials)
         for c in self.colliders:
             sim.add_collider(c)
