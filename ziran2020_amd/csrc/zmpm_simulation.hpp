@@ -43,6 +43,12 @@ public:
     double dt = 1.0 / 24;
     double cfl = 0.6;
     double apic_rpic_ratio = 1;
+    // MpmSimulationBase::TransferScheme (MpmSimulationBase.h:47-52)
+    zmpm_transfer transfer_scheme = ZMPM_APIC_BLEND_RPIC;
+    double flip_pic_ratio = 0;
+    bool mls_mpm = false;
+    int rpic_damping_iteration = 0; // MpmSimulationBase.h:135-136
+    int pic_damping_iteration = 0;
     TV gravity { 0, 0, 0 };
     bool autorestart = true;
     // ---- device configuration (no counterpart in the reference)
@@ -120,6 +126,11 @@ public:
         d.apic_rpic_ratio = apic_rpic_ratio;
         d.cfl = cfl;
         d.device = device;
+        d.transfer_scheme = transfer_scheme;
+        d.flip_pic_ratio = flip_pic_ratio;
+        d.mls_mpm = mls_mpm ? 1 : 0;
+        d.rpic_damping_iteration = rpic_damping_iteration;
+        d.pic_damping_iteration = pic_damping_iteration;
         check(zmpm_create(&d, &h_), "zmpm_create");
         check(zmpm_set_materials(h_, (int32_t)materials.size(), materials.data()), "zmpm_set_materials");
         for (const auto& c : collision_objects) check(zmpm_add_collider(h_, &c, nullptr), "zmpm_add_collider");
