@@ -231,6 +231,15 @@ zmpm_status zmpm_set_timing(zmpm_handle h, int32_t enable);
  * names[] receives static strings.  The oracle reports none. */
 zmpm_status zmpm_get_timings(zmpm_handle h, int32_t* count, const char** names, double* ms);
 
+/* MpmSimulationBase::calculateDt (Lib/MPM/MpmSimulationBase.cpp:1127-1151) with
+ * evalMaxParticleSpeed (:1613-1638): the largest particle speed |v| and the
+ * particles' bounding box, reduced on the device (synchronous).  *dt = cfl dx /
+ * max speed, or max_dt when every particle is at rest (collision-object speeds
+ * are not included, as in the reference).  Output pointers may be NULL;
+ * min_corner / max_corner take 3 doubles. */
+zmpm_status zmpm_calculate_dt(zmpm_handle h, double max_dt, double* dt, double* max_speed, double* min_corner,
+                              double* max_corner);
+
 /* Launch configuration of the grid-stride kernels (workgroups of 256 threads)
  * and the P2G lane split.  The workgroup counts do not change any result; the
  * P2G split cap (most lanes sharing one stencil column of a cell) changes how

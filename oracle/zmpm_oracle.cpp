@@ -1590,6 +1590,30 @@ zmpm_status zmpm_set_timing(zmpm_handle h, int32_t)
     return h ? ZMPM_OK : ZMPM_E_ARG;
 }
 
+// calculateDt (MpmSimulationBase.cpp:1127-1151) over evalMaxParticleSpeed (:1613-1638)
+zmpm_status zmpm_calculate_dt(zmpm_handle h, double max_dt, double* dt, double* max_speed, double* min_corner,
+                              double* max_corner)
+{
+    if (!h || !(max_dt > 0)) return ZMPM_E_ARG;
+    const double lowest = (double)std::numeric_limits<float>::lowest();
+    double r[7] = { 0.0, lowest, lowest, lowest, lowest, lowest, lowest };
+    for (int64_t i = 0; i < h->n; i++) {
+        const double* v = &h->v[3 * i];
+        r[0] = std::max(r[0], std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+        for (int a = 0; a < 3; a++) {
+            r[1 + a] = std::max(r[1 + a], h->x[3 * i + a]);
+            r[4 + a] = std::max(r[4 + a], -h->x[3 * i + a]);
+        }
+    }
+    if (max_speed) *max_speed = r[0];
+    for (int a = 0; a < 3; a++) {
+        if (max_corner) max_corner[a] = r[1 + a];
+        if (min_corner) min_corner[a] = -r[4 + a];
+    }
+    if (dt) *dt = r[0] != 0 ? h->desc.cfl * h->desc.dx / r[0] : max_dt;
+    return ZMPM_OK;
+}
+
 zmpm_status zmpm_get_timings(zmpm_handle h, int32_t* count, const char**, double*)
 {
     if (!h || !count) return ZMPM_E_ARG;

@@ -44,3 +44,38 @@ def test_cpp_facade_matches_oracle(oracle, fp32):
     tol_x, tol_v = (1e-12, 1e-9) if not fp32 else (1e-6, 1e-4)
     assert np.abs(np.array(got["com"]) - com).max() < tol_x
     assert np.abs(np.array(got["mean_v"]) - vm).max() < tol_v * np.abs(vm).max()
+
+
+def test_cpp_frame_driver_matches_oracle(oracle):
+    # advanceOneFrame with nextDt(calculateDt()) (SimulationBase.h:288-322,
+    # TimeStepping.h:45-59, MpmSimulationBase.cpp:1127-1151) on the device,
+    # against the same driver loop over the oracle
+    cells, frames = 6, 2
+    out = subprocess.run([EXE, str(cells), "0", "0", str(frames)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    got = json.loads(out.stdout.strip().splitlines()[-1])
+    sc = lattice_cube(cells)
+    sc.v[:] = (0.0, -3.0, 0.0)
+    sim = sc.make(oracle)
+    frame_dt, max_dt, min_dt = 1.0 / 60, 1.0 / 100, 1e-6
+    t, substeps = 0.0, 0
+    for _ in range(frames):
+        since = 0.0
+        while True:
+            d = sim.calculate_dt(max_dt)["dt"]
+            d = min(max(d, min_dt), max_dt)
+            if since + d >= frame_dt:
+                d = frame_dt - since
+            elif since + 2 * d > frame_dt:
+                d = (frame_dt - since) / 2
+            sim.substep(d, t)
+            t += d
+            since += d
+            substeps += 1
+            if since >= frame_dt:
+                break
+    o = sim.download()
+    com = (o["m"][:, None] * o["x"]).sum(0) / o["m"].sum()
+    assert got["substeps"] == substeps and substeps > 2 * frames
+    assert abs(got["time"] - t) < 1e-15
+    assert np.abs(np.array(got["com"]) - com).max() < 1e-12

@@ -330,6 +330,24 @@ def test_graph_replays_match_eager_substeps(gpu, precision, transfer):
         assert normwise(a["x"], b["x"]) < 1e-12
 
 
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_calculate_dt(gpu, oracle, precision):
+    sc = small_cube(cells=8)
+    sc.v[:] = np.random.default_rng(12).normal(scale=0.4, size=sc.v.shape)
+    g = run(gpu, sc, precision, 3)
+    o = run(oracle, sc, A.PREC_FP64, 3)
+    rg, ro = g.calculate_dt(1e-2), o.calculate_dt(1e-2)
+    tol = 1e-12 if precision == A.PREC_FP64 else 1e-5
+    assert abs(rg["max_speed"] - ro["max_speed"]) <= tol * ro["max_speed"]
+    assert abs(rg["dt"] - ro["dt"]) <= tol * ro["dt"]
+    for k in ("min_corner", "max_corner"):
+        assert np.abs(rg[k] - ro[k]).max() <= tol * np.abs(ro[k]).max(), k
+    empty = sc.make(gpu, precision)
+    z3 = np.zeros((0, 3))
+    empty.upload(z3, z3, np.zeros((0, 9)), np.zeros((0, 9)), np.zeros(0), np.zeros(0), np.zeros(0, np.int32))
+    assert empty.calculate_dt(1e-2)["dt"] == 1e-2
+
+
 def _fields(out):
     return {k: out[k] for k in ("x", "v", "C", "F", "d", "g")}
 

@@ -656,3 +656,19 @@ def test_transfer_options_validated(oracle):
     for kw in bad:
         with pytest.raises(A.ZmpmError):
             sc.make(oracle, **kw)
+
+
+def test_calculate_dt(oracle):
+    # calculateDt (MpmSimulationBase.cpp:1127-1151): cfl dx / max |v|, the
+    # particle bounding box from evalMaxParticleSpeed; max_dt at rest
+    sc = cube_scene(cells=4)
+    sim = sc.make(oracle)
+    r = sim.calculate_dt(1e-2)
+    assert r["dt"] == 1e-2 and r["max_speed"] == 0.0
+    assert np.array_equal(r["min_corner"], sc.x.min(0)) and np.array_equal(r["max_corner"], sc.x.max(0))
+    sc.v = np.random.default_rng(2).normal(size=sc.v.shape)
+    sim = sc.make(oracle)
+    r = sim.calculate_dt(1e-2)
+    vmax = np.sqrt((sc.v ** 2).sum(1)).max()
+    assert abs(r["max_speed"] - vmax) <= 1e-15 * vmax
+    assert abs(r["dt"] - 0.6 * sc.dx / vmax) <= 1e-15 * r["dt"]

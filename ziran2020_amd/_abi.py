@@ -155,7 +155,7 @@ class ZmpmLib:
         "zmpm_abi_version", "zmpm_build_info", "zmpm_create", "zmpm_destroy", "zmpm_last_error",
         "zmpm_set_materials", "zmpm_add_collider", "zmpm_clear_colliders", "zmpm_upload",
         "zmpm_download", "zmpm_substep", "zmpm_run", "zmpm_synchronize", "zmpm_get_binning",
-        "zmpm_get_grid", "zmpm_set_timing", "zmpm_get_timings", "zmpm_get_launch_config",
+        "zmpm_get_grid", "zmpm_set_timing", "zmpm_get_timings", "zmpm_calculate_dt", "zmpm_get_launch_config",
         "zmpm_set_launch_config", "zmpm_autotune",
     )
 
@@ -188,6 +188,8 @@ class ZmpmLib:
         L.zmpm_get_grid.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), _DP, _DP, _DP, _DP]
         L.zmpm_set_timing.argtypes = [C.c_void_p, C.c_int32]
         L.zmpm_get_timings.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_char_p), _DP]
+        L.zmpm_calculate_dt.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.zmpm_get_launch_config.argtypes = [C.c_void_p, C.POINTER(LaunchConfig)]
         L.zmpm_set_launch_config.argtypes = [C.c_void_p, C.POINTER(LaunchConfig)]
         L.zmpm_autotune.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_int32,
@@ -325,6 +327,15 @@ class Simulation:
         self._check(self.L.zmpm_get_grid(self.h, C.byref(N), _ptr(out["coords"], C.c_int32), _ptr(out["m"]),
                                          _ptr(out["v"]), _ptr(out["new_v"]), _ptr(out["pf"])), "zmpm_get_grid")
         return out
+
+    def calculate_dt(self, max_dt):
+        """MpmSimulationBase::calculateDt: CFL step from the largest particle speed,
+        with the speed and the particles' bounding box."""
+        dt, sp = C.c_double(), C.c_double()
+        lo, hi = np.zeros(3), np.zeros(3)
+        self._check(self.L.zmpm_calculate_dt(self.h, max_dt, C.byref(dt), C.byref(sp), _ptr(lo), _ptr(hi)),
+                    "zmpm_calculate_dt")
+        return dict(dt=dt.value, max_speed=sp.value, min_corner=lo, max_corner=hi)
 
     def launch_config(self):
         cfg = LaunchConfig()

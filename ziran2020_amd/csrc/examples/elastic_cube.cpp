@@ -3,8 +3,11 @@
 // the sub-cell centres) dropping onto a sticky floor, as a scene written
 // against the reference's MpmSimulationBase member names.
 //
-// usage: zmpm_elastic_cube <cells per side> <substeps> <fp32: 0|1>
-// prints one JSON line: particle count, centre of mass, mean velocity.
+// usage: zmpm_elastic_cube <cells per side> <substeps> <fp32: 0|1> [frames]
+// With frames > 0 the cube starts at 3 m/s downwards and the frame driver runs
+// that many frames of 1/60 s with CFL substeps (calculateDt, TimeStepping::nextDt;
+// max_dt = 1/100 s) instead of the fixed substeps.
+// prints one JSON line: particle count, centre of mass, mean velocity, substeps.
 #include "zmpm_simulation.hpp"
 
 #include <cstdio>
@@ -15,6 +18,7 @@ int main(int argc, char** argv)
     const int cells = argc > 1 ? std::atoi(argv[1]) : 16;
     const int steps = argc > 2 ? std::atoi(argv[2]) : 20;
     const bool fp32 = argc > 3 && std::atoi(argv[3]) != 0;
+    const int frames = argc > 4 ? std::atoi(argv[4]) : 0;
     zmpm::MpmSimulationHip sim;
     sim.dx = 1.0 / 64;
     sim.gravity = { 0, -9.8, 0 };
@@ -40,8 +44,22 @@ int main(int argc, char** argv)
         for (int j = 0; j < 2 * cells; j++)
             for (int k = 0; k < 2 * cells; k++)
                 sim.addParticle({ x0 + (i + 0.5) * h, y0 + (j + 0.5) * h, z0 + (k + 0.5) * h }, rho * V0, V0, mat);
-    sim.initialize();
-    sim.advance(steps, 1e-4);
+    int substeps = steps;
+    if (frames > 0) {
+        for (auto& v : sim.V) v = { 0, -3, 0 };
+        sim.frame_dt = 1.0 / 60;
+        sim.max_dt = 1.0 / 100;
+        sim.initialize();
+        substeps = 0;
+        for (int f = 0; f < frames; f++) {
+            sim.advanceOneFrame();
+            substeps += sim.substeps_last_frame;
+        }
+    }
+    else {
+        sim.initialize();
+        sim.advance(steps, 1e-4);
+    }
     sim.sync();
     double cm[3] = { 0, 0, 0 }, vm[3] = { 0, 0, 0 }, M = 0;
     for (size_t p = 0; p < sim.X.size(); p++) {
@@ -51,7 +69,8 @@ int main(int argc, char** argv)
         }
         M += sim.mass[p];
     }
-    std::printf("{\"n\": %zu, \"com\": [%.17g, %.17g, %.17g], \"mean_v\": [%.17g, %.17g, %.17g]}\n", sim.X.size(),
-                cm[0] / M, cm[1] / M, cm[2] / M, vm[0] / M, vm[1] / M, vm[2] / M);
+    std::printf("{\"n\": %zu, \"com\": [%.17g, %.17g, %.17g], \"mean_v\": [%.17g, %.17g, %.17g], \"substeps\": %d, "
+                "\"time\": %.17g}\n",
+                sim.X.size(), cm[0] / M, cm[1] / M, cm[2] / M, vm[0] / M, vm[1] / M, vm[2] / M, substeps, sim.time);
     return 0;
 }

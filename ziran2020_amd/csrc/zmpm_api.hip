@@ -186,6 +186,7 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.pfn, D.cap_nodes);
     ALLOC(D.scales, NSC);
     ALLOC(D.maxbits, (int64_t)NSC * BOUND_SLOTS);
+    ALLOC(D.extent, (int64_t)(EXTENT_WGS + 1) * 8);
 #undef ALLOC
     HIPCHK(h, hipMemsetAsync(D.touched, 0, D.num_pages * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.page_count, 0, D.num_pages * sizeof(int), h->stream));
@@ -900,6 +901,36 @@ zmpm_status zmpm_set_timing(zmpm_handle h, int32_t enable)
 {
     if (!h) return ZMPM_E_ARG;
     h->timing = enable != 0;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_calculate_dt(zmpm_handle h, double max_dt, double* dt, double* max_speed, double* min_corner,
+                              double* max_corner)
+{
+    if (!h || !(max_dt > 0)) return ZMPM_E_ARG;
+    (void)hipSetDevice(h->desc.device);
+    double e[8] = { 0, -3.4028234663852886e38, -3.4028234663852886e38, -3.4028234663852886e38,
+                    -3.4028234663852886e38, -3.4028234663852886e38, -3.4028234663852886e38, 0 };
+    double* ext = h->fp32 ? h->Df.extent : h->Dd.extent;
+    if (h->n > 0 && ext) {
+        if (h->fp32) {
+            k_extent<float, long long><<<EXTENT_WGS, WG, 0, h->stream>>>(h->Df, 0);
+            k_extent<float, long long><<<1, WG, 0, h->stream>>>(h->Df, 1);
+        }
+        else {
+            k_extent<double, double><<<EXTENT_WGS, WG, 0, h->stream>>>(h->Dd, 0);
+            k_extent<double, double><<<1, WG, 0, h->stream>>>(h->Dd, 1);
+        }
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipMemcpyAsync(e, ext + EXTENT_WGS * 8, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    if (max_speed) *max_speed = e[0];
+    for (int a = 0; a < 3; a++) {
+        if (max_corner) max_corner[a] = e[1 + a];
+        if (min_corner) min_corner[a] = -e[4 + a];
+    }
+    if (dt) *dt = e[0] != 0 ? h->desc.cfl * h->desc.dx / e[0] : max_dt;
     return ZMPM_OK;
 }
 

@@ -1770,4 +1770,60 @@ __global__ __launch_bounds__(WG) void k_damp_g2p(Dev<T, AccT> D)
     if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd);
 }
 
+// evalMaxParticleSpeed (MpmSimulationBase.cpp:1613-1638): per workgroup the
+// maxima of |v| (in the particle precision, as the reference's velocity.norm()),
+// x and -x over a grid stride of the particles; the last launch folds the
+// EXTENT_WGS partials into row EXTENT_WGS.  Empty: speed 0, corners at the
+// reference's initial float lowest.
+constexpr int EXTENT_WGS = 256;
+
+__device__ __forceinline__ double posMetres(int X, double dx) { return (double)X * dx * (1.0 / (double)(1 << POS_BITS)); }
+__device__ __forceinline__ double posMetres(double x, double) { return x; }
+
+__device__ __forceinline__ double blockMax(double v)
+{
+    __shared__ double wm[WG / 64];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double m = wm[0];
+#pragma unroll
+    for (int w = 1; w < WG / 64; w++) m = fmax(m, wm[w]);
+    return m;
+}
+
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_extent(Dev<T, AccT> D, int fold)
+{
+    const double lowest = -3.4028234663852886e38; // std::numeric_limits<float>::lowest()
+    double r[7] = { 0.0, lowest, lowest, lowest, lowest, lowest, lowest };
+    if (fold) {
+        for (int w = threadIdx.x; w < EXTENT_WGS; w += WG)
+#pragma unroll
+            for (int k = 0; k < 7; k++) r[k] = fmax(r[k], D.extent[w * 8 + k]);
+    }
+    else {
+        const int64_t cap = D.cap;
+        for (int p = blockIdx.x * WG + threadIdx.x; p < D.n; p += gridDim.x * WG) {
+            const T v0 = ldAt(D.P + (PV + 0) * cap, p), v1 = ldAt(D.P + (PV + 1) * cap, p), v2 = ldAt(D.P + (PV + 2) * cap, p);
+            r[0] = fmax(r[0], (double)dsqrt(v0 * v0 + v1 * v1 + v2 * v2));
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const double x = posMetres(loadPos(D.P + (PX + a) * cap, p), (double)D.dx);
+                r[1 + a] = fmax(r[1 + a], x);
+                r[4 + a] = fmax(r[4 + a], -x);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) r[k] = blockMax(r[k]);
+    if (threadIdx.x == 0) {
+        const int row = fold ? EXTENT_WGS : blockIdx.x;
+#pragma unroll
+        for (int k = 0; k < 7; k++) D.extent[row * 8 + k] = r[k];
+    }
+}
+
 } // namespace zmpm

@@ -165,6 +165,7 @@ struct Dev {
     // fixed-point scales (float form only)
     float* scales;     // [NSC] current scale 2^e
     unsigned int* maxbits; // [NSC][64] max |contribution| bound (float bits), atomic-max slots
+    double* extent;        // [EXTENT_WGS + 1][8] particle speed / bounding-box partial maxima
     int max_slots;         // workgroups of the particle-order launches (G2P / upload)
     // tables
     const MatDev<T>* mats;

@@ -70,6 +70,10 @@ public:
     double time = 0;
     double frame_dt = 1.0 / 24;
     int end_frame = 0;
+    // TimeStepping (Lib/Ziran/Sim/TimeStepping.h:15-33): dt bounds of the frame driver
+    double min_dt = 1e-6;
+    double max_dt = 1.0 / 24;
+    double time_since_last_frame = 0;
     std::vector<std::function<void(int, int, double, double)>> begin_time_step_callbacks;
 
     MpmSimulationHip() = default;
@@ -175,19 +179,50 @@ public:
         host_stale_ = true;
     }
 
-    // SimulationBase::advanceOneFrame (SimulationBase.h:288-322) with a fixed dt.
+    // MpmSimulationBase::calculateDt (MpmSimulationBase.cpp:1127-1151): cfl dx /
+    // largest particle speed, max_dt when at rest (device reduction).
+    virtual double calculateDt()
+    {
+        if (!h_) return max_dt;
+        double d = max_dt;
+        check(zmpm_calculate_dt(h_, max_dt, &d, nullptr, nullptr, nullptr), "zmpm_calculate_dt");
+        return d;
+    }
+
+    // TimeStepping::nextDt (TimeStepping.h:45-59): clamp to [min_dt, max_dt] and
+    // land exactly on the frame boundary without a tiny last substep.
+    double nextDt(double d)
+    {
+        if (!(d > 0)) throw ZmpmError(ZMPM_E_ARG, "nextDt: dt must be positive");
+        d = (d < min_dt) ? min_dt : (d > max_dt) ? max_dt : d;
+        if (time_since_last_frame + d >= frame_dt)
+            d = frame_dt - time_since_last_frame;
+        else if (time_since_last_frame + 2 * d > frame_dt)
+            d = (frame_dt - time_since_last_frame) / 2;
+        return d;
+    }
+
+    // SimulationBase::advanceOneFrame (SimulationBase.h:288-322): substeps of
+    // nextDt(calculateDt()) until the frame time is reached (TimeStepping::advance).
     virtual void advanceOneFrame()
     {
-        double t_end = time + frame_dt;
         int substep = 0;
-        while (time < t_end - 1e-12 * frame_dt) {
-            double h = std::min(dt, t_end - time);
-            beginTimeStep(frame, substep++, time, h);
+        for (;;) {
+            const double h = nextDt(calculateDt());
+            beginTimeStep(frame, substep, time, h);
             advanceOneTimeStep(h);
             time += h;
+            time_since_last_frame += h;
+            substep++;
+            if (time_since_last_frame >= frame_dt) {
+                time_since_last_frame = 0;
+                break;
+            }
         }
+        substeps_last_frame = substep;
         frame++;
     }
+    int substeps_last_frame = 0;
 
     // SimulationBase::simulate (SimulationBase.h:327-354) without restart files.
     virtual void simulate()
