@@ -163,6 +163,7 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.key, cap);
     ALLOC(D.rank, cap);
     ALLOC(D.tmp, cap);
+    ALLOC(D.rare, cap);
     ALLOC(D.tmpk, cap);
     ALLOC(D.perm, cap);
     ALLOC(D.sorted_cell, cap);
@@ -319,6 +320,7 @@ void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
         k_g2p<T, AccT, true><<<D.max_slots, WG, 0, st>>>(D, (T)dt);
     else
         k_g2p<T, AccT, false><<<D.max_slots, WG, 0, st>>>(D, (T)dt);
+    k_rare<T, AccT><<<std::min(D.max_slots, 1024), WG, 0, st>>>(D, (T)dt);
     std::swap(D.P, D.Q);
     std::swap(D.pid, D.qid);
     std::swap(D.pmat, D.qmat);

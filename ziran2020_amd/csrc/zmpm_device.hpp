@@ -229,7 +229,7 @@ __device__ __forceinline__ uint64_t waveInclusiveScan64(uint64_t v)
 // reference's unique 64-bit keys (MpmSimulationBase.cpp:1552).
 
 // counters[] slots
-enum : int { CNT_BLOCKS = 0, CNT_STATUS = 1, CNT_EPOCH = 2, CNT_TICKET = 3 };
+enum : int { CNT_BLOCKS = 0, CNT_STATUS = 1, CNT_EPOCH = 2, CNT_TICKET = 3, CNT_RARE = 4 };
 
 constexpr int PSCAN_ITEMS = 8;
 constexpr int PSCAN_TILE = WG * PSCAN_ITEMS; // pages per scan tile
@@ -432,8 +432,10 @@ __global__ __launch_bounds__(WG) void k_page_scan(Dev<T, AccT> D)
 template <class T, class AccT>
 __device__ __forceinline__ void beginStep(const Dev<T, AccT>& D, double dt)
 {
-    // clock[0] = time of the next substep, clock[1] = this substep's t
+    // clock[0] = time of the next substep, clock[1] = this substep's t; the
+    // rare-path queue of this substep's G2P starts empty
     if (threadIdx.x == 0) {
+        D.counters[CNT_RARE] = 0;
         D.clock[1] = D.clock[0];
         D.clock[0] = D.clock[0] + dt;
     }
@@ -626,7 +628,10 @@ __device__ __forceinline__ Mat3<T> loadM(const T* P, int64_t cap, int f, int p)
 // Fs is the stored deformation: F (FP64 form) or H = F - I (FP32 form, whose
 // EOS particles hold J - 1 in H_00; the EOS pressure is then
 // -k expm1(-gamma log1p(J - 1)) J).
-__device__ __forceinline__ void vtauSym(const MatDev<float>& mt, const Mat3<float>& H, float V0, float g, float* S)
+// FAST (G2P): returns true, with no stress written, for a corotated particle
+// near inversion (see corotatedTauH).
+template <bool FAST = false>
+__device__ __forceinline__ bool vtauSym(const MatDev<float>& mt, const Mat3<float>& H, float V0, float g, float* S)
 {
     float t6[6];
     if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE) {
@@ -634,10 +639,11 @@ __device__ __forceinline__ void vtauSym(const MatDev<float>& mt, const Mat3<floa
         const float p = V0 * (-mt.bulk * expm1f(-mt.gamma * log1pf(jm1))) * (1.f + jm1);
         S[0] = S[3] = S[5] = p;
         S[1] = S[2] = S[4] = 0.f;
-        return;
+        return false;
     }
-    if (mt.model == ZMPM_MODEL_FIXED_COROTATED)
-        corotatedTauH(H, mt.mu, mt.lambda, t6);
+    if (mt.model == ZMPM_MODEL_FIXED_COROTATED) {
+        if (corotatedTauH<FAST>(H, mt.mu, mt.lambda, t6)) return true;
+    }
     else if (mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
         float J;
         qrStableTauH(H, mt.mu, mt.lambda, g, mt.kx, mt.ky, t6, &J);
@@ -646,9 +652,11 @@ __device__ __forceinline__ void vtauSym(const MatDev<float>& mt, const Mat3<floa
         neoHookeanTauH(H, mt.mu, mt.lambda, t6);
 #pragma unroll
     for (int i = 0; i < 6; i++) S[i] = V0 * t6[i];
+    return false;
 }
 
-__device__ __forceinline__ void vtauSym(const MatDev<double>& mt, const Mat3<double>& F, double V0, double g, double* S)
+template <bool FAST = false>
+__device__ __forceinline__ bool vtauSym(const MatDev<double>& mt, const Mat3<double>& F, double V0, double g, double* S)
 {
     using T = double;
     T s0, s1, s2, s3, s4, s5;
@@ -671,7 +679,7 @@ __device__ __forceinline__ void vtauSym(const MatDev<double>& mt, const Mat3<dou
             S[3] = V0 * t6[3];
             S[4] = V0 * t6[4];
             S[5] = V0 * t6[5];
-            return;
+            return false;
         }
         else
             tau = neoHookeanTau(F, mt.mu, mt.lambda);
@@ -688,6 +696,7 @@ __device__ __forceinline__ void vtauSym(const MatDev<double>& mt, const Mat3<dou
     S[3] = s3;
     S[4] = s4;
     S[5] = s5;
+    return false;
 }
 
 // Upper bounds of one particle's P2G contributions per scale group, folded
@@ -1286,8 +1295,12 @@ __global__ __launch_bounds__(WG) void k_damp_grid(Dev<T, AccT> D)
 // AnisoMPM explicit damage update of one particle (AnisoFractureSimulation.h:210-283,
 // structural tensor :365-391, undegraded Cauchy stress QRStableNeoHookean.h:83-93).
 // Fs is the stored deformation: F (FP64 form) or H = F - I (FP32 form).
-template <class T>
-__device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>& Fs, T lap, T dt, T& d, T& g)
+// FAST (G2P): a particle whose phi bound does not rule out <phi - 1>_+ > 0 is not
+// updated here; the function returns true and the particle is queued for
+// k_rare, which runs this update in full.  That keeps the eigen split and the
+// polar rotation, rarely needed, out of G2P's registers.
+template <class T, bool FAST = false>
+__device__ __forceinline__ bool damageUpdate(const MatDev<T>& mt, const Mat3<T>& Fs, T lap, T dt, T& d, T& g)
 {
     const T geometricResist = d - (mt.l0 * mt.l0 * lap);
     // undegraded Cauchy stress tau(g = 1, no fibres) / J (evaluateEffectiveCauchyStress,
@@ -1313,6 +1326,7 @@ __device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>&
         + T(2) * (sym.a[0][1] * sym.a[0][1] + sym.a[0][2] * sym.a[0][2] + sym.a[1][2] * sym.a[1][2]);
     T dTilde = T(0);
     if (!(mt.phi_bound_scale * sf2 < T(0.99))) {
+        if constexpr (FAST) return true;
         const Mat3<T> sp = positivePart(sym);
         Mat3<T> R;
         if constexpr (sizeof(T) == 4)
@@ -1344,6 +1358,7 @@ __device__ __forceinline__ void damageUpdate(const MatDev<T>& mt, const Mat3<T>&
     d = newD < T(1) ? newD : T(1);
     const T k = mt.k;
     g = ((T(1) - d) * (T(1) - d) * (T(1) - k)) + k;
+    return false;
 }
 
 // gridToParticlesHelper<APIC, deg 2, !MLS> (MpmSimulationBase.cpp:1412-1478),
@@ -1500,10 +1515,13 @@ __device__ __forceinline__ void loadG2P(const Dev<T, AccT>& D, int s, int p, Raw
         for (int a = 0; a < 3; a++) r.v[a] = ldAt(P + (PV + a) * cap, p);
 }
 
+// Returns true for a particle deferred to k_rare (its d, g and stress are
+// completed there).
 template <class T, class AccT, bool FLIP = false>
-__device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const RawG2P<T>& R, T dt, const G2PAcc<T>& A,
+__device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const RawG2P<T>& R, T dt, const G2PAcc<T>& A,
                                           const MatDev<T>& mt, float* bnd, PosT<T>* xnew)
 {
+    bool rare = false;
     constexpr bool FIXED = std::is_same<AccT, long long>::value;
     const int64_t cap = D.cap;
     T* Q = D.Q;
@@ -1593,7 +1611,7 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     }
     else {
         const Mat3<T> F = Fn;
-        if (dmg && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) damageUpdate(mt, F, A.lap, dt, d, g);
+        if (dmg && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) rare = damageUpdate<T, true>(mt, F, A.lap, dt, d, g);
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -1611,10 +1629,13 @@ __device__ __forceinline__ void g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     for (int i = 0; i < 9; i++) stAt(Q + (PF + i) * cap, s, Fn.a[i / 3][i % 3]);
     // stress of the new state for the next substep's P2G
     T S[6];
-    vtauSym(mt, Fn, R.V0, g, S);
+    rare = rare || vtauSym<true>(mt, Fn, R.V0, g, S);
+    if (!rare) {
 #pragma unroll
-    for (int i = 0; i < 6; i++) stAt(Q + (PS + i) * cap, s, S[i]);
-    if constexpr (FIXED) forceBounds(bnd, S, dmg ? d : T(0), D.odx, D.any_damage != 0);
+        for (int i = 0; i < 6; i++) stAt(Q + (PS + i) * cap, s, S[i]);
+        if constexpr (FIXED) forceBounds(bnd, S, dmg ? d : T(0), D.odx, D.any_damage != 0);
+    }
+    return rare;
 }
 
 // Grid fields of block b's tile (the block and its forward halo, through the
@@ -1692,10 +1713,54 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
     }
     float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
     PosT<T> xnew[3] = { 0, 0, 0 };
-    if (live) g2pFinish<T, AccT, FLIP>(D, s, R, dt, A, D.mats[R.mid], bnd, xnew);
+    bool rare = false;
+    if (live) rare = g2pFinish<T, AccT, FLIP>(D, s, R, dt, A, D.mats[R.mid], bnd, xnew);
+    // deferred particles to the rare-path queue, one atomic per wavefront
+    const uint64_t rm = __ballot(rare);
+    if (rm) {
+        const int lane = t & 63, first = __ffsll((unsigned long long)rm) - 1;
+        int at = 0;
+        if (lane == first) at = atomicAdd(&D.counters[CNT_RARE], __popcll(rm));
+        at = __shfl(at, first);
+        if (rare) stAt(D.rare, at + __popcll(rm & ((1ull << lane) - 1)), s);
+    }
     // binning of the coming substep at the new positions (live index s)
     openPageScan(D);
     binParticle(D, s, live, xnew);
+    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd);
+}
+
+// The particles G2P deferred (queued by sorted position): the AnisoMPM damage
+// update on F_n with its full path (sigma^+, structural tensor) and the stress of
+// (F_{n+1}, g_{n+1}) with every branch of the models, and its force bound.  Runs
+// after k_g2p of the same substep, before the buffers swap.
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_rare(Dev<T, AccT> D, T dt)
+{
+    const int cnt = D.counters[CNT_RARE];
+    const int64_t cap = D.cap;
+    float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
+    for (int i = blockIdx.x * WG + threadIdx.x; i < cnt; i += gridDim.x * WG) {
+        const int s = ldAt(D.rare, i);
+        if ((unsigned)s >= (unsigned)D.n) continue;
+        int p = ldAt(D.perm, s);
+        if ((unsigned)p >= (unsigned)D.n) p = 0;
+        const MatDev<T>& mt = D.mats[ldAt(D.qmat, s)];
+        const bool dmg = D.any_damage && mt.has_damage;
+        T d = ldAt(D.P + PD * cap, p), g = ldAt(D.P + PG * cap, p);
+        if (dmg && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
+            const Mat3<T> F = loadM(D.P, cap, PF, p); // F_n (FP32 form: H)
+            damageUpdate<T, false>(mt, F, ldAt(D.Q + PLAP * cap, s), dt, d, g);
+            stAt(D.Q + PD * cap, s, d);
+            stAt(D.Q + PG * cap, s, g);
+        }
+        const Mat3<T> Fn = loadM(D.Q, cap, PF, s);
+        T S[6];
+        vtauSym<false>(mt, Fn, ldAt(D.Q + PVOL * cap, s), g, S);
+#pragma unroll
+        for (int k = 0; k < 6; k++) stAt(D.Q + (PS + k) * cap, s, S[k]);
+        if constexpr (std::is_same<AccT, long long>::value) forceBounds(bnd, S, dmg ? d : T(0), D.odx, D.any_damage != 0);
+    }
     if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd);
 }
 

@@ -135,6 +135,7 @@ struct Dev {
     // binning
     uint64_t* key;     // (page << block_bits | elem) per live particle
     int* rank;         // slot inside the particle's page (k_bin)
+    int* rare;         // sorted positions G2P deferred to k_rare (count in counters[4])
     int* tmp;          // sorted position -> live index, before the in-block sort
     uint32_t* tmpk;    // the same positions: cell << (32 - block bits) | original index
     int* perm;         // sorted position -> live index
@@ -151,7 +152,7 @@ struct Dev {
     int* nbr;          // [cap_blocks][8] forward neighbour block ids
     int* cell_count;   // [cap_blocks*NB]
     int* cell_start;   // [cap_blocks*NB]
-    int* counters;     // [0] num_blocks, [1] status bits, [2] scan epoch, [3] scan ticket
+    int* counters;     // [0] num_blocks, [1] status bits, [2] scan epoch, [3] scan ticket, [4] rare count
     double* clock;     // [0] time of the next substep, [1] time of the running substep
     int cap_blocks;
     int domain;

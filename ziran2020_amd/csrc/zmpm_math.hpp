@@ -648,11 +648,15 @@ __device__ __forceinline__ void symEigvals3(const float* S, float& l1, float& l2
 //   f[1,2,3] = 1 / ((s1 + s2)(s2 + s3)(s1 + s3)).
 // Below J = 1/20 (inverted or nearly, where the reference's signed SVD makes R
 // differ from V^{-1} F) the reference formula runs on F = I + H.
-__device__ __forceinline__ void corotatedTauH(const Mat3<float>& H, float mu, float lambda, float* tau)
+// FAST: below J = 1/20 return true without a stress (the caller defers the
+// particle to a kernel that runs the reference formula).
+template <bool FAST = false>
+__device__ __forceinline__ bool corotatedTauH(const Mat3<float>& H, float mu, float lambda, float* tau)
 {
     const float jm1 = jm1OfH(H);
     const float J = 1.f + jm1;
     if (!(J > 0.05f)) {
+        if constexpr (FAST) return true;
         const Mat3<float> t = corotatedTau(identityPlus(H), mu, lambda);
         tau[0] = t.a[0][0];
         tau[1] = 0.5f * (t.a[0][1] + t.a[1][0]);
@@ -660,7 +664,7 @@ __device__ __forceinline__ void corotatedTauH(const Mat3<float>& H, float mu, fl
         tau[3] = t.a[1][1];
         tau[4] = 0.5f * (t.a[1][2] + t.a[2][1]);
         tau[5] = t.a[2][2];
-        return;
+        return false;
     }
     float E[6];
     strainOfH(H, E);
@@ -689,6 +693,7 @@ __device__ __forceinline__ void corotatedTauH(const Mat3<float>& H, float mu, fl
     tau[3] = two_mu * (f1 + d12 * a11 + d123 * p11) + iso;
     tau[4] = two_mu * (d12 * e12 + d123 * p12);
     tau[5] = two_mu * (f1 + d12 * a22 + d123 * p22) + iso;
+    return false;
 }
 
 // QRStableNeoHookean Kirchhoff stress (qrStableTau above) from H:
