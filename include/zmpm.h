@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define ZMPM_ABI_VERSION 2
+#define ZMPM_ABI_VERSION 3
 
 /* Error model.  Replaces ZIRAN_ASSERT -> std::runtime_error
  * (Lib/Ziran/CS/Util/Debug.h:19-37) and the CFL RestartException
@@ -62,6 +62,17 @@ typedef enum {
     ZMPM_MODEL_NEO_HOOKEAN = 2,
     ZMPM_MODEL_EQUATION_OF_STATE = 3
 } zmpm_model;
+
+/* Plasticity applied after the F update (MpmSimulationBase::applyPlasticity,
+ * MpmSimulationBase.cpp:1517-1529, PlasticityApplier.h:27-57):
+ *  VON_MISES_STVK_HENCKY  VonMisesStvkHencky::projectStrain (PlasticityApplier.cpp:1480-1516):
+ *                         radial return of the Hencky strain's deviator onto
+ *                         |eps_hat| = yield_stress / (2 mu), hardening
+ *                         yield_stress -= hardening * delta_gamma per particle */
+typedef enum {
+    ZMPM_PLASTICITY_NONE = 0,
+    ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY = 1
+} zmpm_plasticity;
 
 /* Collision object types; values equal AnalyticCollisionObject::COLLISION_OBJECT_TYPE
  * (Lib/Ziran/Math/Geometry/CollisionObject.h:47-52). */
@@ -120,11 +131,13 @@ typedef struct {
     double kx, ky;           /* QRStableNeoHookean extra fibre stiffness                  */
     double bulk, gamma;      /* EquationOfState                                           */
     int32_t num_directors;   /* 1 or 2 structural directors a_0                           */
-    int32_t reserved;
+    int32_t plasticity;      /* zmpm_plasticity (models with mu: corotated, QR-NH, NH)    */
     double a0[2][3];         /* structural directors (reference uses unit e1, e2)         */
     double alphas[2];
     double residual_phase;   /* k in g = (1-d)^2 (1-k) + k                                */
     double l0, sigma_crit, eta, zeta;
+    double yield_stress;     /* VonMisesStvkHencky::yield_stress (initial, per particle)  */
+    double hardening;        /* VonMisesStvkHencky::xi                                    */
 } zmpm_material;
 
 /* Analytic collision object with rigid translation b(t) = b0 + dbdt * t
@@ -157,6 +170,7 @@ typedef struct {
     double* d;          /* [n] damage, may be NULL (0 on upload)                          */
     double* g;          /* [n] degradation g of the QR model, may be NULL (1 on upload)   */
     double* laplacian;  /* [n] damage Laplacian (download only), may be NULL              */
+    double* yield_stress; /* [n] plastic yield stress, may be NULL (the material's on upload) */
 } zmpm_particles;
 
 typedef struct zmpm_context* zmpm_handle;

@@ -851,7 +851,7 @@ struct zmpm_context {
     bool any_damage = false;
     // particle state (index order, never reordered)
     int64_t n = 0;
-    std::vector<double> x, v, C, F, m, V0, d, g, lap;
+    std::vector<double> x, v, C, F, m, V0, d, g, lap, ys;
     std::vector<int32_t> mat;
     // binning of the last substep
     std::vector<uint64_t> keys;
@@ -1062,6 +1062,37 @@ inline void iterateKernel(zmpm_context* h, const Stencil& s, const OP& op, const
 
 const zmpm_material& matOf(const zmpm_context* h, int64_t i) { return h->mats[(size_t)h->mat[i]]; }
 
+// VonMisesStvkHencky::projectStrain (PlasticityApplier.cpp:1480-1516): SVD of
+// F, Hencky strain eps = log(max(sigma, 1e-4)), its deviator eps_hat; when
+// |eps_hat| exceeds yield / (2 mu) the strain returns radially onto it,
+// F = U exp(H) V^T, and the yield stress hardens by xi delta_gamma.  The
+// fail-stress bookkeeping (a crack normal the explicit step never reads) is
+// not kept.
+void vonMisesProject(double mu, double xi, double& yield_stress, M3& F)
+{
+    M3 U, V;
+    double sigma[3];
+    svd3(F, U, sigma, V, kSvdTol);
+    double eps[3], tr = 0;
+    for (int a = 0; a < 3; a++) {
+        eps[a] = std::log(std::max(sigma[a], 1e-4));
+        tr += eps[a];
+    }
+    double hat[3], n2 = 0;
+    for (int a = 0; a < 3; a++) {
+        hat[a] = eps[a] - (tr / 3.0);
+        n2 += hat[a] * hat[a];
+    }
+    const double nrm = std::sqrt(n2);
+    const double delta_gamma = nrm - yield_stress / (2 * mu);
+    if (delta_gamma <= 0) return;
+    yield_stress -= xi * delta_gamma;
+    double e[3];
+    for (int a = 0; a < 3; a++) e[a] = std::exp(eps[a] - (delta_gamma / nrm) * hat[a]);
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) F.a[r][c] = U.a[r][0] * e[0] * V.a[c][0] + U.a[r][1] * e[1] * V.a[c][1] + U.a[r][2] * e[2] * V.a[c][2];
+}
+
 // gridToParticlesHelper<scheme, deg 2, mls> generic path (MpmSimulationBase.cpp:1412-1478)
 // and evolveStrain: F <- (I + dt gradV) F (FBasedMpmForceHelper.cpp:93-108);
 // J <- (1 + dt tr gradV) J (JBasedMpmForceHelper.cpp:79-94).  APIC_blend_RPIC:
@@ -1140,6 +1171,9 @@ int gridToParticlesPass(zmpm_context* h, double dt, double r, double flip, bool 
         for (int a = 0; a < 3; a++)
             for (int c = 0; c < 3; c++) A.a[a][c] = (a == c ? 1.0 : 0.0) + dt * gv[3 * a + c];
         M3 Fn = mmul(A, mload(Fp));
+        // applyPlasticity (MpmSimulationBase.cpp:1517-1529)
+        const zmpm_material& mt = matOf(h, i);
+        if (mt.plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY) vonMisesProject(mt.mu, mt.hardening, h->ys[i], Fn);
         mstore(Fn, Fp);
     }
     return cfl_flag;
@@ -1437,6 +1471,13 @@ const char* zmpm_last_error(zmpm_handle h) { return h ? h->err.c_str() : "null h
 zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material* mats)
 {
     if (!h || count <= 0 || count > 256 || !mats) return fail(h, ZMPM_E_ARG, "bad material table");
+    for (int32_t i = 0; i < count; i++) {
+        const zmpm_material& m = mats[i];
+        if (m.plasticity != ZMPM_PLASTICITY_NONE && m.plasticity != ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY)
+            return fail(h, ZMPM_E_ARG, "unknown plasticity");
+        if (m.plasticity && m.model == ZMPM_MODEL_EQUATION_OF_STATE)
+            return fail(h, ZMPM_E_ARG, "plasticity needs a model with mu");
+    }
     h->mats.assign(mats, mats + count);
     h->any_damage = false;
     for (auto& m : h->mats) h->any_damage = h->any_damage || m.has_damage;
@@ -1484,6 +1525,8 @@ zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
     h->mat.assign(p->material, p->material + n);
     h->d.assign(n, 0.0);
     h->g.assign(n, 1.0);
+    h->ys.resize(n);
+    for (int64_t i = 0; i < n; i++) h->ys[i] = p->yield_stress ? p->yield_stress[i] : h->mats[(size_t)h->mat[i]].yield_stress;
     h->lap.assign(n, 0.0);
     if (p->d) h->d.assign(p->d, p->d + n);
     if (p->g) h->g.assign(p->g, p->g + n);
@@ -1506,6 +1549,7 @@ zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
     if (p->d) std::copy(h->d.begin(), h->d.end(), p->d);
     if (p->g) std::copy(h->g.begin(), h->g.end(), p->g);
     if (p->laplacian) std::copy(h->lap.begin(), h->lap.end(), p->laplacian);
+    if (p->yield_stress) std::copy(h->ys.begin(), h->ys.end(), p->yield_stress);
     (void)n;
     zmpm_status s = (zmpm_status)h->latched;
     return s;
@@ -1665,6 +1709,14 @@ void zmpm_oracle_bspline(double x, int32_t* base, double* w, double* dw, double*
     int b;
     bspline(x, b, w, dw, ddw);
     *base = b;
+}
+
+// VonMisesStvkHencky::projectStrain on one deformation gradient (row-major, in place)
+void zmpm_oracle_von_mises(double mu, double xi, double* yield_stress, double* F)
+{
+    M3 f = mload(F);
+    vonMisesProject(mu, xi, *yield_stress, f);
+    mstore(f, F);
 }
 
 int32_t zmpm_oracle_svd(const double* A, double* U, double* sigma, double* V)

@@ -330,6 +330,40 @@ def test_graph_replays_match_eager_substeps(gpu, precision, transfer):
         assert normwise(a["x"], b["x"]) < 1e-12
 
 
+@pytest.mark.parametrize("case", ["corotated", "corotated_hardening", "damage_slab"])
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_von_mises_plasticity(gpu, oracle, case, precision):
+    # VonMisesStvkHencky after the F update (MpmSimulationBase.cpp:1517-1529);
+    # the damage slab case is the reference's vonMisesTear setup (QR-stable
+    # Neo-Hookean with AnisoMPM damage and von Mises plasticity)
+    if case == "damage_slab":
+        sc = small_slab()
+        m = sc.materials[0]
+        m.plasticity, m.yield_stress, m.hardening = A.PLASTICITY_VON_MISES_STVK_HENCKY, 20.0, 0.0
+        steps = 20
+    else:
+        sc = small_cube(cells=10, seed=6)
+        sc.v[:, 1] = -1.0  # onto the sticky floor
+        sc.materials = [A.make_material(A.MODEL_FIXED_COROTATED, 1e4, 0.3, plasticity=A.PLASTICITY_VON_MISES_STVK_HENCKY,
+                                        yield_stress=30.0, hardening=-800.0 if case.endswith("hardening") else 0.0)]
+        steps = 25
+    g = run(gpu, sc, precision, steps).download()
+    o = run(oracle, sc, A.PREC_FP64, steps).download()
+    assert np.abs(o["yield_stress"] - sc.materials[0].yield_stress).max() >= 0.0
+    tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+    keys = ("x", "v", "F", "yield_stress") + (("d",) if case == "damage_slab" else ())
+    for k in keys:
+        assert normwise(g[k], o[k]) < (tol if k != "d" else 10 * tol), k
+    # the scene must actually yield: elastic runs differ
+    sc.materials[0].plasticity = A.PLASTICITY_NONE
+    e = run(oracle, sc, A.PREC_FP64, steps).download()
+    assert normwise(e["F"], o["F"]) > 1e-6
+    if case == "corotated_hardening":
+        # yield -= xi delta_gamma: xi < 0 hardens (PlasticityApplier.cpp:1503, "supposed
+        # to only increase"); xi > 0 softens, which localises and amplifies rounding
+        assert o["yield_stress"].max() > 30.0
+
+
 @pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
 def test_calculate_dt(gpu, oracle, precision):
     sc = small_cube(cells=8)

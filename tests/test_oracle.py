@@ -672,3 +672,53 @@ def test_calculate_dt(oracle):
     vmax = np.sqrt((sc.v ** 2).sum(1)).max()
     assert abs(r["max_speed"] - vmax) <= 1e-15 * vmax
     assert abs(r["dt"] - 0.6 * sc.dx / vmax) <= 1e-15 * r["dt"]
+
+
+# ---------------------------------------------------------------- plasticity
+def von_mises(O, mu, xi, ys, F):
+    O.zmpm_oracle_von_mises.argtypes = [C.c_double, C.c_double, DP, DP]
+    y = C.c_double(ys)
+    f = np.ascontiguousarray(F, float).ravel().copy()
+    O.zmpm_oracle_von_mises(mu, xi, C.byref(y), p(f))
+    return f.reshape(3, 3), y.value
+
+
+def hencky(F):
+    U, s, Vt = np.linalg.svd(F)
+    return U, np.log(s), Vt
+
+
+def test_von_mises_return_mapping(O):
+    # VonMisesStvkHencky::projectStrain (PlasticityApplier.cpp:1480-1516): beyond
+    # yield the Hencky strain's deviator returns radially onto |eps_hat| = y/(2 mu);
+    # the volumetric strain and the principal directions are kept; hardening
+    # lowers y by xi * delta_gamma; inside the yield surface nothing changes.
+    rng = np.random.default_rng(30)
+    mu, y0 = 3846.0, 50.0
+    for _ in range(20):
+        Q1 = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        Q2 = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        for Q in (Q1, Q2):
+            if np.linalg.det(Q) < 0:
+                Q[:, 0] *= -1
+        s = np.exp(rng.normal(scale=0.05, size=3))
+        F = Q1 @ np.diag(s) @ Q2.T
+        _, e, _ = hencky(F)
+        hat = e - e.mean()
+        n = np.linalg.norm(hat)
+        for xi in (0.0, 2000.0):
+            G, y = von_mises(O, mu, xi, y0, F)
+            if n <= y0 / (2 * mu):
+                assert np.array_equal(G, F) and y == y0
+                continue
+            dg = n - y0 / (2 * mu)
+            assert abs(y - (y0 - xi * dg)) < 1e-9 * y0
+            U, e2, Vt = hencky(G)
+            hat2 = e2 - e2.mean()
+            assert abs(np.linalg.norm(hat2) - y0 / (2 * mu)) < 1e-12
+            assert abs(e2.sum() - e.sum()) < 1e-12
+            # same principal frame: G = Q1 exp(H) Q2^T with H diagonal in that frame
+            Hd = Q1.T @ G @ Q2
+            assert np.abs(Hd - np.diag(np.diag(Hd))).max() < 1e-12
+    G, _ = von_mises(O, mu, 0.0, y0, np.eye(3))
+    assert np.array_equal(G, np.eye(3))

@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-ZMPM_ABI_VERSION = 2
+ZMPM_ABI_VERSION = 3
 
 # zmpm_status
 OK, E_ARG, E_HIP, E_DOMAIN, E_CFL, E_NAN, E_OVERFLOW, E_NOMEM = range(8)
@@ -25,6 +25,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 MODEL_FIXED_COROTATED, MODEL_QR_STABLE_NEO_HOOKEAN, MODEL_NEO_HOOKEAN, MODEL_EQUATION_OF_STATE = range(4)
 STICKY, SLIP, SEPARATE, GHOST = 1, 2, 3, 4
 APIC_BLEND_RPIC, FLIP_BLEND_PIC = 0, 1
+PLASTICITY_NONE, PLASTICITY_VON_MISES_STVK_HENCKY = 0, 1
 SHAPE_HALF_SPACE, SHAPE_BOX = 0, 1
 
 D3 = C.c_double * 3
@@ -56,11 +57,12 @@ class Material(C.Structure):
         ("kx", C.c_double), ("ky", C.c_double),
         ("bulk", C.c_double), ("gamma", C.c_double),
         ("num_directors", C.c_int32),
-        ("reserved", C.c_int32),
+        ("plasticity", C.c_int32),
         ("a0", D3 * 2),
         ("alphas", C.c_double * 2),
         ("residual_phase", C.c_double),
         ("l0", C.c_double), ("sigma_crit", C.c_double), ("eta", C.c_double), ("zeta", C.c_double),
+        ("yield_stress", C.c_double), ("hardening", C.c_double),
     ]
 
 
@@ -84,7 +86,7 @@ class Particles(C.Structure):
         ("n", C.c_int64),
         ("x", _DP), ("v", _DP), ("C", _DP), ("F", _DP),
         ("m", _DP), ("V0", _DP), ("material", _IP),
-        ("d", _DP), ("g", _DP), ("laplacian", _DP),
+        ("d", _DP), ("g", _DP), ("laplacian", _DP), ("yield_stress", _DP),
     ]
 
 
@@ -103,7 +105,8 @@ class ZmpmError(RuntimeError):
 
 def make_material(model=MODEL_FIXED_COROTATED, E=1e4, nu=0.3, *, has_damage=False, kx=0.0, ky=0.0,
                   bulk=0.0, gamma=7.0, directors=((1.0, 0.0, 0.0),), alphas=(0.0,),
-                  residual_phase=0.001, l0=0.0, sigma_crit=1.0, eta=0.1, zeta=1.0) -> Material:
+                  residual_phase=0.001, l0=0.0, sigma_crit=1.0, eta=0.1, zeta=1.0,
+                  plasticity=PLASTICITY_NONE, yield_stress=0.0, hardening=0.0) -> Material:
     """Material from Young's modulus and Poisson ratio with the reference's Lame
     conversion (CorotatedElasticity.cpp:15-20, QRStableNeoHookean.h:61-65)."""
     m = Material()
@@ -119,6 +122,7 @@ def make_material(model=MODEL_FIXED_COROTATED, E=1e4, nu=0.3, *, has_damage=Fals
     for k, a in enumerate(alphas):
         m.alphas[k] = a
     m.residual_phase, m.l0, m.sigma_crit, m.eta, m.zeta = residual_phase, l0, sigma_crit, eta, zeta
+    m.plasticity, m.yield_stress, m.hardening = plasticity, yield_stress, hardening
     return m
 
 
@@ -255,7 +259,7 @@ class Simulation:
         self._check(self.L.zmpm_add_collider(self.h, C.byref(c), C.byref(i)), "zmpm_add_collider")
         return i.value
 
-    def upload(self, x, v, Cm, F, m, V0, material, d=None, g=None):
+    def upload(self, x, v, Cm, F, m, V0, material, d=None, g=None, yield_stress=None):
         n = int(x.shape[0])
         arrs = dict(
             x=np.ascontiguousarray(x, np.float64).reshape(n, 3),
@@ -267,10 +271,11 @@ class Simulation:
             material=np.ascontiguousarray(material, np.int32),
             d=None if d is None else np.ascontiguousarray(d, np.float64),
             g=None if g is None else np.ascontiguousarray(g, np.float64),
+            yield_stress=None if yield_stress is None else np.ascontiguousarray(yield_stress, np.float64),
         )
         p = Particles()
         p.n = n
-        for k in ("x", "v", "C", "F", "m", "V0", "d", "g"):
+        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "yield_stress"):
             setattr(p, k, _ptr(arrs[k]))
         p.material = _ptr(arrs["material"], C.c_int32)
         p.laplacian = None
@@ -282,10 +287,10 @@ class Simulation:
         n = self.n
         out = dict(x=np.zeros((n, 3)), v=np.zeros((n, 3)), C=np.zeros((n, 9)), F=np.zeros((n, 9)),
                    m=np.zeros(n), V0=np.zeros(n), material=np.zeros(n, np.int32), d=np.zeros(n),
-                   g=np.zeros(n), laplacian=np.zeros(n))
+                   g=np.zeros(n), laplacian=np.zeros(n), yield_stress=np.zeros(n))
         p = Particles()
         p.n = n
-        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "laplacian"):
+        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "laplacian", "yield_stress"):
             setattr(p, k, _ptr(out[k]))
         p.material = _ptr(out["material"], C.c_int32)
         st = self.L.zmpm_download(self.h, C.byref(p))

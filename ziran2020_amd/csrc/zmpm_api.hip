@@ -217,6 +217,8 @@ void refreshTables(zmpm_context* h)
     D.cols = (const ColDev*)h->cols_dev;
     D.num_cols = (int)h->cols.size();
     D.any_damage = h->any_damage ? 1 : 0;
+    D.any_plastic = 0;
+    for (const auto& m : h->mats) D.any_plastic |= m.plasticity != ZMPM_PLASTICITY_NONE ? 1 : 0;
     D.dx = (T)h->desc.dx;
     D.odx = T(1) / (T)h->desc.dx;
     D.pos_scale = (T)((double)(1 << POS_BITS) / h->desc.dx);
@@ -261,6 +263,8 @@ zmpm_status uploadMaterials(zmpm_context* h)
             na += std::fabs(m.alphas[k]) * (m.a0[k][0] * m.a0[k][0] + m.a0[k][1] * m.a0[k][1] + m.a0[k][2] * m.a0[k][2]);
         // rounded up so the float form never under-estimates the bound
         d.phi_bound_scale = (T)(na * na / (m.sigma_crit * m.sigma_crit) * (1.0 + 1e-6));
+        d.plasticity = m.plasticity;
+        d.hardening = (T)m.hardening;
     }
     HIPCHK(h, hipMemcpyAsync(h->mats_dev, md.data(), sizeof(MatDev<T>) * kMaxMaterials, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -406,6 +410,9 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
         buf[PVOL * cap + i] = (T)p->V0[i];
         buf[PD * cap + i] = p->d ? (T)p->d[i] : T(0);
         buf[PG * cap + i] = p->g ? (T)p->g[i] : T(1);
+        const int mi = p->material[i];
+        buf[PYS * cap + i] = p->yield_stress ? (T)p->yield_stress[i]
+                                             : (mi >= 0 && mi < (int)h->mats.size() ? (T)h->mats[(size_t)mi].yield_stress : T(0));
     }
     std::vector<int> ids(cap, 0), mat(cap, 0);
     for (int i = 0; i < n; i++) {
@@ -484,6 +491,7 @@ zmpm_status downloadT(zmpm_context* h, zmpm_particles* p)
         if (p->d) p->d[i] = (double)buf[PD * cap + s];
         if (p->g) p->g[i] = (double)buf[PG * cap + s];
         if (p->laplacian) p->laplacian[i] = (double)buf[PLAP * cap + s];
+        if (p->yield_stress) p->yield_stress[i] = (double)buf[PYS * cap + s];
     }
     return ZMPM_OK;
 }
@@ -699,9 +707,14 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
 {
     if (!h) return ZMPM_E_ARG;
     if (count <= 0 || count > kMaxMaterials || !mats) return fail(h, ZMPM_E_ARG, "bad material table");
-    for (int i = 0; i < count; i++)
+    for (int i = 0; i < count; i++) {
         if (mats[i].model < 0 || mats[i].model > ZMPM_MODEL_EQUATION_OF_STATE)
             return fail(h, ZMPM_E_ARG, "unknown constitutive model");
+        if (mats[i].plasticity != ZMPM_PLASTICITY_NONE && mats[i].plasticity != ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY)
+            return fail(h, ZMPM_E_ARG, "unknown plasticity");
+        if (mats[i].plasticity && mats[i].model == ZMPM_MODEL_EQUATION_OF_STATE)
+            return fail(h, ZMPM_E_ARG, "plasticity needs a model with mu");
+    }
     (void)hipSetDevice(h->desc.device);
     dropGraphs(h);
     h->mats.assign(mats, mats + count);

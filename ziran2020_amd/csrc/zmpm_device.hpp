@@ -1492,6 +1492,7 @@ struct RawG2P {
     PosT<T> x[3];
     T F[9], m, V0, d, g;
     T v[3]; // FLIP_blend_PIC only
+    T ys;   // plastic yield stress (when any material has plasticity)
     int mid, id;
 };
 
@@ -1510,6 +1511,7 @@ __device__ __forceinline__ void loadG2P(const Dev<T, AccT>& D, int s, int p, Raw
     r.g = ldAt(P + PG * cap, p);
     r.mid = ldAt(D.pmat, p);
     r.id = ldAt(D.spid, s);
+    r.ys = D.any_plastic ? ldAt(P + PYS * cap, p) : T(0);
     if constexpr (FLIP)
 #pragma unroll
         for (int a = 0; a < 3; a++) r.v[a] = ldAt(P + (PV + a) * cap, p);
@@ -1627,6 +1629,12 @@ __device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     stAt(Q + PG * cap, s, g);
 #pragma unroll
     for (int i = 0; i < 9; i++) stAt(Q + (PF + i) * cap, s, Fn.a[i / 3][i % 3]);
+    // plasticity (applyPlasticity, MpmSimulationBase.cpp:1517-1529): the return
+    // mapping runs in k_rare for particles that may yield
+    if (D.any_plastic) {
+        stAt(Q + PYS * cap, s, R.ys);
+        if (mt.plasticity) rare = rare || plasticMayYield(mt, Fn, R.ys);
+    }
     // stress of the new state for the next substep's P2G
     T S[6];
     rare = rare || vtauSym<true>(mt, Fn, R.V0, g, S);
@@ -1730,6 +1738,74 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
     if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd);
 }
 
+// VonMisesStvkHencky::projectStrain (PlasticityApplier.cpp:1480-1516) on the
+// stored deformation (FP32 form: H = F - I, formed into F in double): SVD of F,
+// Hencky strain eps = log(max(sigma, 1e-4)) and its deviator eps_hat; when
+// |eps_hat| > yield / (2 mu) the strain returns radially, F = U exp(H) V^T, and
+// the yield stress hardens by xi delta_gamma.  In double for both forms (it runs
+// for the few particles G2P defers).  Returns true when F changed.
+template <class T>
+__device__ __noinline__ bool vonMisesProject(const MatDev<T>& mt, Mat3<T>& Fs, T& ys)
+{
+    constexpr double id = sizeof(T) == 4 ? 1.0 : 0.0;
+    Mat3<double> F, U, V;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) F.a[i][j] = (double)Fs.a[i][j] + (i == j ? id : 0.0);
+    double sg[3];
+    svd3(F, U, sg, V);
+    double eps[3], tr = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        eps[a] = log(fmax(sg[a], 1e-4));
+        tr += eps[a];
+    }
+    double hat[3], n2 = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        hat[a] = eps[a] - (tr / 3.0);
+        n2 += hat[a] * hat[a];
+    }
+    const double nrm = sqrt(n2);
+    const double dgam = nrm - (double)ys / (2 * (double)mt.mu);
+    if (dgam <= 0) return false;
+    ys = (T)((double)ys - (double)mt.hardening * dgam);
+    double e[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) e[a] = exp(eps[a] - (dgam / nrm) * hat[a]);
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            Fs.a[r][c] = (T)((U.a[r][0] * e[0] * V.a[c][0] + U.a[r][1] * e[1] * V.a[c][1] + U.a[r][2] * e[2] * V.a[c][2])
+                             - (r == c ? id : 0.0));
+    return true;
+}
+
+// Whether a plastic particle may be beyond its yield surface, so that G2P
+// defers it to k_rare.  FP64 form: always (the return mapping's own test decides
+// there, on the reference's SVD).  FP32 form: the Hencky strain from the
+// eigenvalues l of F F^T - I (eps = log1p(l) / 2, cancellation-free); a margin
+// covers the float rounding of that test, and inverted or nearly degenerate F
+// is always deferred.
+__device__ __forceinline__ bool plasticMayYield(const MatDev<double>&, const Mat3<double>&, double) { return true; }
+__device__ __forceinline__ bool plasticMayYield(const MatDev<float>& mt, const Mat3<float>& H, float ys)
+{
+    if (!(jm1OfH(H) > -0.95f)) return true;
+    float E[6];
+    strainOfH(H, E);
+    float l1, l2, l3;
+    symEigvals3(E, l1, l2, l3);
+    if (!(1.f + l3 > 1e-8f)) return true;
+    const float e0 = 0.5f * log1pf(l1), e1 = 0.5f * log1pf(l2), e2 = 0.5f * log1pf(l3);
+    const float tr = (e0 + e1 + e2) * (1.f / 3.f);
+    const float h0 = e0 - tr, h1 = e1 - tr, h2 = e2 - tr;
+    const float n = sqrtf(h0 * h0 + h1 * h1 + h2 * h2);
+    const float lim = ys / (2.f * mt.mu);
+    return n - lim > -(1e-4f * lim + 1e-6f);
+}
+
 // The particles G2P deferred (queued by sorted position): the AnisoMPM damage
 // update on F_n with its full path (sigma^+, structural tensor) and the stress of
 // (F_{n+1}, g_{n+1}) with every branch of the models, and its force bound.  Runs
@@ -1754,7 +1830,15 @@ __global__ __launch_bounds__(WG) void k_rare(Dev<T, AccT> D, T dt)
             stAt(D.Q + PD * cap, s, d);
             stAt(D.Q + PG * cap, s, g);
         }
-        const Mat3<T> Fn = loadM(D.Q, cap, PF, s);
+        Mat3<T> Fn = loadM(D.Q, cap, PF, s);
+        if (mt.plasticity) {
+            T ys = ldAt(D.Q + PYS * cap, s);
+            if (vonMisesProject(mt, Fn, ys)) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) stAt(D.Q + (PF + k) * cap, s, Fn.a[k / 3][k % 3]);
+                stAt(D.Q + PYS * cap, s, ys);
+            }
+        }
         T S[6];
         vtauSym<false>(mt, Fn, ldAt(D.Q + PVOL * cap, s), g, S);
 #pragma unroll

@@ -43,7 +43,8 @@ enum : int {
     PG = 27,  // degradation g
     PLAP = 28, // damage Laplacian (output)
     PS = 29,   // V0 * Kirchhoff stress of the current F, symmetric: xx xy xz yy yz zz
-    NPF = 35
+    PYS = 35,  // plastic yield stress (materials with plasticity)
+    NPF = 36
 };
 
 // FP32-form position: X = round(x / dx * 2^POS_BITS) as int32.  Domains are at
@@ -109,6 +110,9 @@ struct MatDev {
     T k, l0, sigma_crit, eta, zeta;
     T inv_sigma_crit2; // 1 / sigma_c^2 (AnisoFractureSimulation.h:262)
     T phi_bound_scale; // (1 + sum |alpha_k| |a0_k|^2)^2 / sigma_c^2: phi <= this * |sigma|_F^2
+    int plasticity;    // zmpm_plasticity
+    int pad2;
+    T hardening;       // VonMisesStvkHencky::xi
 };
 
 struct ColDev {
@@ -173,6 +177,7 @@ struct Dev {
     const ColDev* cols;
     int num_cols;
     int any_damage;
+    int any_plastic;
     // launch parameters
     int p2g_split; // most lanes sharing one stencil column of a cell in P2G (>= 1)
     // transfer scheme (MpmSimulationBase::transfer_scheme, mls_mpm, flip_pic_ratio)
