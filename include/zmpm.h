@@ -65,13 +65,18 @@ typedef enum {
 
 /* Plasticity applied after the F update (MpmSimulationBase::applyPlasticity,
  * MpmSimulationBase.cpp:1517-1529, PlasticityApplier.h:27-57):
- *  VON_MISES_STVK_HENCKY  VonMisesStvkHencky::projectStrain (PlasticityApplier.cpp:1480-1516):
- *                         radial return of the Hencky strain's deviator onto
- *                         |eps_hat| = yield_stress / (2 mu), hardening
- *                         yield_stress -= hardening * delta_gamma per particle */
+ *  VON_MISES_STVK_HENCKY     VonMisesStvkHencky::projectStrain (PlasticityApplier.cpp:1480-1516):
+ *                            radial return of the Hencky strain's deviator onto
+ *                            |eps_hat| = yield_stress / (2 mu), yield_stress -=
+ *                            hardening * delta_gamma; per-particle state: yield stress
+ *  DRUCKER_PRAGER_STVK_HENCKY DruckerPragerStvkHencky::projectStrain (PlasticityApplier.cpp:811-876):
+ *                            friction cone alpha = sqrt(2/3) 2 sin(phi) / (3 - sin(phi)),
+ *                            cohesion, tip projection with volume correction
+ *                            log Jp += beta tr(eps); per-particle state: log Jp */
 typedef enum {
     ZMPM_PLASTICITY_NONE = 0,
-    ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY = 1
+    ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY = 1,
+    ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY = 2
 } zmpm_plasticity;
 
 /* Collision object types; values equal AnalyticCollisionObject::COLLISION_OBJECT_TYPE
@@ -137,7 +142,11 @@ typedef struct {
     double residual_phase;   /* k in g = (1-d)^2 (1-k) + k                                */
     double l0, sigma_crit, eta, zeta;
     double yield_stress;     /* VonMisesStvkHencky::yield_stress (initial, per particle)  */
-    double hardening;        /* VonMisesStvkHencky::xi                                    */
+    double hardening;        /* VonMisesStvkHencky::xi / DruckerPragerStvkHencky::beta     */
+    double friction_angle;   /* DruckerPragerStvkHencky friction angle (degrees)          */
+    double cohesion;         /* DruckerPragerStvkHencky::cohesion                          */
+    int32_t volume_correction; /* DruckerPragerStvkHencky::volume_correction              */
+    int32_t reserved;
 } zmpm_material;
 
 /* Analytic collision object with rigid translation b(t) = b0 + dbdt * t
@@ -170,7 +179,8 @@ typedef struct {
     double* d;          /* [n] damage, may be NULL (0 on upload)                          */
     double* g;          /* [n] degradation g of the QR model, may be NULL (1 on upload)   */
     double* laplacian;  /* [n] damage Laplacian (download only), may be NULL              */
-    double* yield_stress; /* [n] plastic yield stress, may be NULL (the material's on upload) */
+    double* plastic_state; /* [n] plasticity state (von Mises: yield stress, Drucker-Prager:
+                              log Jp), may be NULL (upload: the material's yield stress / 0) */
 } zmpm_particles;
 
 typedef struct zmpm_context* zmpm_handle;

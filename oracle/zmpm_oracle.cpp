@@ -1093,6 +1093,51 @@ void vonMisesProject(double mu, double xi, double& yield_stress, M3& F)
         for (int c = 0; c < 3; c++) F.a[r][c] = U.a[r][0] * e[0] * V.a[c][0] + U.a[r][1] * e[1] * V.a[c][1] + U.a[r][2] * e[2] * V.a[c][2];
 }
 
+// DruckerPragerStvkHencky (PlasticityApplier.cpp:811-876): alpha from the
+// friction angle as its constructor computes it; projectStrain on the Hencky
+// strain eps = log(max(|sigma|, 1e-4)) - cohesion with the per-particle log Jp:
+// tip (tr >= 0: F = exp(cohesion) U V^T, log Jp += beta tr eps when volume
+// correction is on), inside (H = eps + cohesion) or onto the cone.
+double dpAlpha(double friction_angle)
+{
+    const double sin_phi = std::sin(friction_angle / 180.0 * 3.141592653);
+    return std::sqrt(2.0 / 3.0) * 2.0 * sin_phi / (3.0 - sin_phi);
+}
+
+void druckerPragerProject(const zmpm_material& mt, double& logJp, M3& F)
+{
+    M3 U, V;
+    double sigma[3];
+    svd3(F, U, sigma, V, kSvdTol);
+    const double coh = mt.cohesion;
+    double eps[3], sum = 0;
+    for (int a = 0; a < 3; a++) {
+        eps[a] = std::log(std::max(std::fabs(sigma[a]), 1e-4)) - coh;
+        sum += eps[a];
+    }
+    const double tr = sum + logJp;
+    double hat[3], n2 = 0;
+    for (int a = 0; a < 3; a++) {
+        hat[a] = eps[a] - (tr / 3.0);
+        n2 += hat[a] * hat[a];
+    }
+    double e[3];
+    if (tr >= 0) {
+        for (int a = 0; a < 3; a++) e[a] = std::exp(coh);
+        if (mt.volume_correction) logJp = mt.hardening * sum + logJp;
+    }
+    else if (mt.mu == 0)
+        return;
+    else {
+        logJp = 0;
+        const double nrm = std::sqrt(n2);
+        const double dgam = nrm + (3 * mt.lambda + 2 * mt.mu) / (2 * mt.mu) * tr * dpAlpha(mt.friction_angle);
+        for (int a = 0; a < 3; a++) e[a] = std::exp(dgam <= 0 ? eps[a] + coh : eps[a] - (dgam / nrm) * hat[a] + coh);
+    }
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) F.a[r][c] = U.a[r][0] * e[0] * V.a[c][0] + U.a[r][1] * e[1] * V.a[c][1] + U.a[r][2] * e[2] * V.a[c][2];
+}
+
 // gridToParticlesHelper<scheme, deg 2, mls> generic path (MpmSimulationBase.cpp:1412-1478)
 // and evolveStrain: F <- (I + dt gradV) F (FBasedMpmForceHelper.cpp:93-108);
 // J <- (1 + dt tr gradV) J (JBasedMpmForceHelper.cpp:79-94).  APIC_blend_RPIC:
@@ -1174,6 +1219,7 @@ int gridToParticlesPass(zmpm_context* h, double dt, double r, double flip, bool 
         // applyPlasticity (MpmSimulationBase.cpp:1517-1529)
         const zmpm_material& mt = matOf(h, i);
         if (mt.plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY) vonMisesProject(mt.mu, mt.hardening, h->ys[i], Fn);
+        if (mt.plasticity == ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY) druckerPragerProject(mt, h->ys[i], Fn);
         mstore(Fn, Fp);
     }
     return cfl_flag;
@@ -1473,7 +1519,7 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
     if (!h || count <= 0 || count > 256 || !mats) return fail(h, ZMPM_E_ARG, "bad material table");
     for (int32_t i = 0; i < count; i++) {
         const zmpm_material& m = mats[i];
-        if (m.plasticity != ZMPM_PLASTICITY_NONE && m.plasticity != ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY)
+        if (m.plasticity < ZMPM_PLASTICITY_NONE || m.plasticity > ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY)
             return fail(h, ZMPM_E_ARG, "unknown plasticity");
         if (m.plasticity && m.model == ZMPM_MODEL_EQUATION_OF_STATE)
             return fail(h, ZMPM_E_ARG, "plasticity needs a model with mu");
@@ -1526,7 +1572,11 @@ zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
     h->d.assign(n, 0.0);
     h->g.assign(n, 1.0);
     h->ys.resize(n);
-    for (int64_t i = 0; i < n; i++) h->ys[i] = p->yield_stress ? p->yield_stress[i] : h->mats[(size_t)h->mat[i]].yield_stress;
+    for (int64_t i = 0; i < n; i++) {
+        const zmpm_material& mt = h->mats[(size_t)h->mat[i]];
+        h->ys[i] = p->plastic_state ? p->plastic_state[i]
+                                    : (mt.plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY ? mt.yield_stress : 0.0);
+    }
     h->lap.assign(n, 0.0);
     if (p->d) h->d.assign(p->d, p->d + n);
     if (p->g) h->g.assign(p->g, p->g + n);
@@ -1549,7 +1599,7 @@ zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
     if (p->d) std::copy(h->d.begin(), h->d.end(), p->d);
     if (p->g) std::copy(h->g.begin(), h->g.end(), p->g);
     if (p->laplacian) std::copy(h->lap.begin(), h->lap.end(), p->laplacian);
-    if (p->yield_stress) std::copy(h->ys.begin(), h->ys.end(), p->yield_stress);
+    if (p->plastic_state) std::copy(h->ys.begin(), h->ys.end(), p->plastic_state);
     (void)n;
     zmpm_status s = (zmpm_status)h->latched;
     return s;
@@ -1709,6 +1759,14 @@ void zmpm_oracle_bspline(double x, int32_t* base, double* w, double* dw, double*
     int b;
     bspline(x, b, w, dw, ddw);
     *base = b;
+}
+
+// DruckerPragerStvkHencky::projectStrain on one deformation gradient (row-major, in place)
+void zmpm_oracle_drucker_prager(const zmpm_material* mat, double* logJp, double* F)
+{
+    M3 f = mload(F);
+    druckerPragerProject(*mat, *logJp, f);
+    mstore(f, F);
 }
 
 // VonMisesStvkHencky::projectStrain on one deformation gradient (row-major, in place)

@@ -349,9 +349,8 @@ def test_von_mises_plasticity(gpu, oracle, case, precision):
         steps = 25
     g = run(gpu, sc, precision, steps).download()
     o = run(oracle, sc, A.PREC_FP64, steps).download()
-    assert np.abs(o["yield_stress"] - sc.materials[0].yield_stress).max() >= 0.0
     tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
-    keys = ("x", "v", "F", "yield_stress") + (("d",) if case == "damage_slab" else ())
+    keys = ("x", "v", "F", "plastic_state") + (("d",) if case == "damage_slab" else ())
     for k in keys:
         assert normwise(g[k], o[k]) < (tol if k != "d" else 10 * tol), k
     # the scene must actually yield: elastic runs differ
@@ -361,7 +360,27 @@ def test_von_mises_plasticity(gpu, oracle, case, precision):
     if case == "corotated_hardening":
         # yield -= xi delta_gamma: xi < 0 hardens (PlasticityApplier.cpp:1503, "supposed
         # to only increase"); xi > 0 softens, which localises and amplifies rounding
-        assert o["yield_stress"].max() > 30.0
+        assert o["plastic_state"].max() > 30.0
+
+
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_drucker_prager_plasticity(gpu, oracle, precision):
+    # a sand-like cube (DruckerPragerStvkHencky, 30 degrees, volume correction)
+    # spreading on the sticky floor: tip, cone and inside cases all occur
+    sc = small_cube(cells=10, seed=7)
+    sc.v[:, 1] = -1.5
+    sc.v[:, 0] = np.random.default_rng(2).normal(scale=0.3, size=sc.n)
+    sc.materials = [A.make_material(A.MODEL_FIXED_COROTATED, 1e4, 0.3, plasticity=A.PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY,
+                                    friction_angle=30.0, cohesion=0.0, hardening=1.0, volume_correction=True)]
+    steps = 25
+    g = run(gpu, sc, precision, steps).download()
+    o = run(oracle, sc, A.PREC_FP64, steps).download()
+    assert np.abs(o["plastic_state"]).max() > 0  # tip projections happened
+    tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+    for k in ("x", "v", "F"):
+        assert normwise(g[k], o[k]) < tol, k
+    assert np.abs(g["plastic_state"] - o["plastic_state"]).max() < (1e-9 if precision == A.PREC_FP64 else 1e-4) * max(
+        1e-3, np.abs(o["plastic_state"]).max())
 
 
 @pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])

@@ -722,3 +722,52 @@ def test_von_mises_return_mapping(O):
             assert np.abs(Hd - np.diag(np.diag(Hd))).max() < 1e-12
     G, _ = von_mises(O, mu, 0.0, y0, np.eye(3))
     assert np.array_equal(G, np.eye(3))
+
+
+def drucker_prager(O, m, ljp, F):
+    O.zmpm_oracle_drucker_prager.argtypes = [C.POINTER(A.Material), DP, DP]
+    lj = C.c_double(ljp)
+    f = np.ascontiguousarray(F, float).ravel().copy()
+    O.zmpm_oracle_drucker_prager(C.byref(m), C.byref(lj), p(f))
+    return f.reshape(3, 3), lj.value
+
+
+def test_drucker_prager_return_mapping(O):
+    # DruckerPragerStvkHencky::projectStrain (PlasticityApplier.cpp:833-876)
+    m = mat(A.MODEL_FIXED_COROTATED, 1e4, 0.3, plasticity=A.PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY,
+            friction_angle=30.0, cohesion=0.0, hardening=1.0, volume_correction=True)
+    sin_phi = np.sin(30.0 / 180 * 3.141592653)
+    alpha = np.sqrt(2 / 3) * 2 * sin_phi / (3 - sin_phi)
+    rng = np.random.default_rng(31)
+    seen = set()
+    for _ in range(60):
+        Q1 = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        Q2 = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        for Q in (Q1, Q2):
+            if np.linalg.det(Q) < 0:
+                Q[:, 0] *= -1
+        s = np.exp(rng.normal(loc=-0.01, scale=0.03, size=3))
+        F = Q1 @ np.diag(s) @ Q2.T
+        e = np.log(s)
+        G, lj = drucker_prager(O, m, 0.0, F)
+        if e.sum() >= 0:   # tip: F = U V^T, log Jp += beta tr(eps)
+            seen.add("tip")
+            assert np.abs(G - Q1 @ Q2.T).max() < 1e-12 and abs(lj - e.sum()) < 1e-12
+            continue
+        hat = e - e.mean()
+        dgam = np.linalg.norm(hat) + (3 * m.lam + 2 * m.mu) / (2 * m.mu) * e.sum() * alpha
+        assert lj == 0.0
+        if dgam <= 0:      # inside the cone: F unchanged
+            seen.add("inside")
+            assert np.abs(G - F).max() < 1e-12
+        else:              # onto the cone: trace kept, on the yield surface after
+            seen.add("cone")
+            _, s2, _ = np.linalg.svd(G)
+            e2 = np.log(s2)
+            assert abs(e2.sum() - e.sum()) < 1e-12
+            d2 = np.linalg.norm(e2 - e2.mean()) + (3 * m.lam + 2 * m.mu) / (2 * m.mu) * e2.sum() * alpha
+            assert abs(d2) < 1e-12
+    assert seen == {"tip", "inside", "cone"}
+    # a previous tip projection's log Jp shifts the trace
+    G, lj = drucker_prager(O, m, 0.5, np.eye(3) * 0.99)
+    assert abs(lj - (0.5 + 3 * np.log(0.99))) < 1e-12

@@ -25,7 +25,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 MODEL_FIXED_COROTATED, MODEL_QR_STABLE_NEO_HOOKEAN, MODEL_NEO_HOOKEAN, MODEL_EQUATION_OF_STATE = range(4)
 STICKY, SLIP, SEPARATE, GHOST = 1, 2, 3, 4
 APIC_BLEND_RPIC, FLIP_BLEND_PIC = 0, 1
-PLASTICITY_NONE, PLASTICITY_VON_MISES_STVK_HENCKY = 0, 1
+PLASTICITY_NONE, PLASTICITY_VON_MISES_STVK_HENCKY, PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY = 0, 1, 2
 SHAPE_HALF_SPACE, SHAPE_BOX = 0, 1
 
 D3 = C.c_double * 3
@@ -63,6 +63,8 @@ class Material(C.Structure):
         ("residual_phase", C.c_double),
         ("l0", C.c_double), ("sigma_crit", C.c_double), ("eta", C.c_double), ("zeta", C.c_double),
         ("yield_stress", C.c_double), ("hardening", C.c_double),
+        ("friction_angle", C.c_double), ("cohesion", C.c_double),
+        ("volume_correction", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -86,7 +88,7 @@ class Particles(C.Structure):
         ("n", C.c_int64),
         ("x", _DP), ("v", _DP), ("C", _DP), ("F", _DP),
         ("m", _DP), ("V0", _DP), ("material", _IP),
-        ("d", _DP), ("g", _DP), ("laplacian", _DP), ("yield_stress", _DP),
+        ("d", _DP), ("g", _DP), ("laplacian", _DP), ("plastic_state", _DP),
     ]
 
 
@@ -106,7 +108,8 @@ class ZmpmError(RuntimeError):
 def make_material(model=MODEL_FIXED_COROTATED, E=1e4, nu=0.3, *, has_damage=False, kx=0.0, ky=0.0,
                   bulk=0.0, gamma=7.0, directors=((1.0, 0.0, 0.0),), alphas=(0.0,),
                   residual_phase=0.001, l0=0.0, sigma_crit=1.0, eta=0.1, zeta=1.0,
-                  plasticity=PLASTICITY_NONE, yield_stress=0.0, hardening=0.0) -> Material:
+                  plasticity=PLASTICITY_NONE, yield_stress=0.0, hardening=0.0, friction_angle=30.0, cohesion=0.0,
+                  volume_correction=True) -> Material:
     """Material from Young's modulus and Poisson ratio with the reference's Lame
     conversion (CorotatedElasticity.cpp:15-20, QRStableNeoHookean.h:61-65)."""
     m = Material()
@@ -123,6 +126,7 @@ def make_material(model=MODEL_FIXED_COROTATED, E=1e4, nu=0.3, *, has_damage=Fals
         m.alphas[k] = a
     m.residual_phase, m.l0, m.sigma_crit, m.eta, m.zeta = residual_phase, l0, sigma_crit, eta, zeta
     m.plasticity, m.yield_stress, m.hardening = plasticity, yield_stress, hardening
+    m.friction_angle, m.cohesion, m.volume_correction = friction_angle, cohesion, int(bool(volume_correction))
     return m
 
 
@@ -259,7 +263,7 @@ class Simulation:
         self._check(self.L.zmpm_add_collider(self.h, C.byref(c), C.byref(i)), "zmpm_add_collider")
         return i.value
 
-    def upload(self, x, v, Cm, F, m, V0, material, d=None, g=None, yield_stress=None):
+    def upload(self, x, v, Cm, F, m, V0, material, d=None, g=None, plastic_state=None):
         n = int(x.shape[0])
         arrs = dict(
             x=np.ascontiguousarray(x, np.float64).reshape(n, 3),
@@ -271,11 +275,11 @@ class Simulation:
             material=np.ascontiguousarray(material, np.int32),
             d=None if d is None else np.ascontiguousarray(d, np.float64),
             g=None if g is None else np.ascontiguousarray(g, np.float64),
-            yield_stress=None if yield_stress is None else np.ascontiguousarray(yield_stress, np.float64),
+            plastic_state=None if plastic_state is None else np.ascontiguousarray(plastic_state, np.float64),
         )
         p = Particles()
         p.n = n
-        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "yield_stress"):
+        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "plastic_state"):
             setattr(p, k, _ptr(arrs[k]))
         p.material = _ptr(arrs["material"], C.c_int32)
         p.laplacian = None
@@ -287,10 +291,10 @@ class Simulation:
         n = self.n
         out = dict(x=np.zeros((n, 3)), v=np.zeros((n, 3)), C=np.zeros((n, 9)), F=np.zeros((n, 9)),
                    m=np.zeros(n), V0=np.zeros(n), material=np.zeros(n, np.int32), d=np.zeros(n),
-                   g=np.zeros(n), laplacian=np.zeros(n), yield_stress=np.zeros(n))
+                   g=np.zeros(n), laplacian=np.zeros(n), plastic_state=np.zeros(n))
         p = Particles()
         p.n = n
-        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "laplacian", "yield_stress"):
+        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "laplacian", "plastic_state"):
             setattr(p, k, _ptr(out[k]))
         p.material = _ptr(out["material"], C.c_int32)
         st = self.L.zmpm_download(self.h, C.byref(p))

@@ -265,6 +265,11 @@ zmpm_status uploadMaterials(zmpm_context* h)
         d.phi_bound_scale = (T)(na * na / (m.sigma_crit * m.sigma_crit) * (1.0 + 1e-6));
         d.plasticity = m.plasticity;
         d.hardening = (T)m.hardening;
+        d.volume_correction = m.volume_correction ? 1 : 0;
+        d.cohesion = (T)m.cohesion;
+        // DruckerPragerStvkHencky::DruckerPragerStvkHencky (PlasticityApplier.cpp:811-819)
+        const double sin_phi = std::sin(m.friction_angle / 180.0 * 3.141592653);
+        d.dp_alpha = (T)(std::sqrt(2.0 / 3.0) * 2.0 * sin_phi / (3.0 - sin_phi));
     }
     HIPCHK(h, hipMemcpyAsync(h->mats_dev, md.data(), sizeof(MatDev<T>) * kMaxMaterials, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -411,8 +416,8 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
         buf[PD * cap + i] = p->d ? (T)p->d[i] : T(0);
         buf[PG * cap + i] = p->g ? (T)p->g[i] : T(1);
         const int mi = p->material[i];
-        buf[PYS * cap + i] = p->yield_stress ? (T)p->yield_stress[i]
-                                             : (mi >= 0 && mi < (int)h->mats.size() ? (T)h->mats[(size_t)mi].yield_stress : T(0));
+        const bool vm = mi >= 0 && mi < (int)h->mats.size() && h->mats[(size_t)mi].plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY;
+        buf[PYS * cap + i] = p->plastic_state ? (T)p->plastic_state[i] : (vm ? (T)h->mats[(size_t)mi].yield_stress : T(0));
     }
     std::vector<int> ids(cap, 0), mat(cap, 0);
     for (int i = 0; i < n; i++) {
@@ -491,7 +496,7 @@ zmpm_status downloadT(zmpm_context* h, zmpm_particles* p)
         if (p->d) p->d[i] = (double)buf[PD * cap + s];
         if (p->g) p->g[i] = (double)buf[PG * cap + s];
         if (p->laplacian) p->laplacian[i] = (double)buf[PLAP * cap + s];
-        if (p->yield_stress) p->yield_stress[i] = (double)buf[PYS * cap + s];
+        if (p->plastic_state) p->plastic_state[i] = (double)buf[PYS * cap + s];
     }
     return ZMPM_OK;
 }
@@ -710,7 +715,7 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
     for (int i = 0; i < count; i++) {
         if (mats[i].model < 0 || mats[i].model > ZMPM_MODEL_EQUATION_OF_STATE)
             return fail(h, ZMPM_E_ARG, "unknown constitutive model");
-        if (mats[i].plasticity != ZMPM_PLASTICITY_NONE && mats[i].plasticity != ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY)
+        if (mats[i].plasticity < ZMPM_PLASTICITY_NONE || mats[i].plasticity > ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY)
             return fail(h, ZMPM_E_ARG, "unknown plasticity");
         if (mats[i].plasticity && mats[i].model == ZMPM_MODEL_EQUATION_OF_STATE)
             return fail(h, ZMPM_E_ARG, "plasticity needs a model with mu");

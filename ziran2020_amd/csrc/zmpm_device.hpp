@@ -1492,7 +1492,7 @@ struct RawG2P {
     PosT<T> x[3];
     T F[9], m, V0, d, g;
     T v[3]; // FLIP_blend_PIC only
-    T ys;   // plastic yield stress (when any material has plasticity)
+    T ys;   // plasticity state (when any material has plasticity)
     int mid, id;
 };
 
@@ -1633,7 +1633,7 @@ __device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     // mapping runs in k_rare for particles that may yield
     if (D.any_plastic) {
         stAt(Q + PYS * cap, s, R.ys);
-        if (mt.plasticity) rare = rare || plasticMayYield(mt, Fn, R.ys);
+        if (mt.plasticity) rare = rare || plasticMayProject(mt, Fn, R.ys);
     }
     // stress of the new state for the next substep's P2G
     T S[6];
@@ -1783,26 +1783,92 @@ __device__ __noinline__ bool vonMisesProject(const MatDev<T>& mt, Mat3<T>& Fs, T
     return true;
 }
 
-// Whether a plastic particle may be beyond its yield surface, so that G2P
-// defers it to k_rare.  FP64 form: always (the return mapping's own test decides
-// there, on the reference's SVD).  FP32 form: the Hencky strain from the
-// eigenvalues l of F F^T - I (eps = log1p(l) / 2, cancellation-free); a margin
-// covers the float rounding of that test, and inverted or nearly degenerate F
-// is always deferred.
-__device__ __forceinline__ bool plasticMayYield(const MatDev<double>&, const Mat3<double>&, double) { return true; }
-__device__ __forceinline__ bool plasticMayYield(const MatDev<float>& mt, const Mat3<float>& H, float ys)
+// DruckerPragerStvkHencky::projectStrain (PlasticityApplier.cpp:833-876) on the
+// stored deformation, in double as vonMisesProject; ljp is log Jp.  F is
+// rebuilt from the SVD in every case (|sigma| for an inverted F), as the
+// reference does.  Returns true (F and ljp written back).
+template <class T>
+__device__ __noinline__ bool druckerPragerProject(const MatDev<T>& mt, Mat3<T>& Fs, T& ljp)
 {
+    constexpr double id = sizeof(T) == 4 ? 1.0 : 0.0;
+    Mat3<double> F, U, V;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) F.a[i][j] = (double)Fs.a[i][j] + (i == j ? id : 0.0);
+    double sg[3];
+    svd3(F, U, sg, V);
+    const double coh = (double)mt.cohesion;
+    double lj = (double)ljp;
+    double eps[3], sum = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        eps[a] = log(fmax(fabs(sg[a]), 1e-4)) - coh;
+        sum += eps[a];
+    }
+    const double tr = sum + lj;
+    double hat[3], n2 = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        hat[a] = eps[a] - (tr / 3.0);
+        n2 += hat[a] * hat[a];
+    }
+    double e[3];
+    if (tr >= 0) { // tip
+#pragma unroll
+        for (int a = 0; a < 3; a++) e[a] = exp(coh);
+        if (mt.volume_correction) lj = (double)mt.hardening * sum + lj;
+    }
+    else if ((double)mt.mu == 0)
+        return false;
+    else {
+        lj = 0;
+        const double nrm = sqrt(n2);
+        const double mu = (double)mt.mu, lam = (double)mt.lambda;
+        const double dgam = nrm + (3 * lam + 2 * mu) / (2 * mu) * tr * (double)mt.dp_alpha;
+#pragma unroll
+        for (int a = 0; a < 3; a++) e[a] = exp(dgam <= 0 ? eps[a] + coh : eps[a] - (dgam / nrm) * hat[a] + coh);
+    }
+    ljp = (T)lj;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            Fs.a[r][c] = (T)((U.a[r][0] * e[0] * V.a[c][0] + U.a[r][1] * e[1] * V.a[c][1] + U.a[r][2] * e[2] * V.a[c][2])
+                             - (r == c ? id : 0.0));
+    return true;
+}
+
+// Whether a plastic particle's return mapping may change it, so that G2P
+// defers it to k_rare.  FP64 form: always (the return mapping decides there,
+// on the reference's SVD).  FP32 form: the Hencky strain from the eigenvalues
+// l of F F^T - I (eps = log1p(l) / 2, cancellation-free); a margin covers the
+// float rounding of the test; inverted or nearly degenerate F is always
+// deferred.  von Mises: outside or near |eps_hat| = yield / (2 mu).  Drucker-
+// Prager: anything but strictly inside the cone with log Jp = 0 (where the
+// reference's rebuild of F from its SVD returns F).
+__device__ __forceinline__ bool plasticMayProject(const MatDev<double>&, const Mat3<double>&, double) { return true; }
+__device__ __forceinline__ bool plasticMayProject(const MatDev<float>& mt, const Mat3<float>& H, float state)
+{
+    const bool dp = mt.plasticity == ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY;
+    if (dp && state != 0.f) return true;
     if (!(jm1OfH(H) > -0.95f)) return true;
     float E[6];
     strainOfH(H, E);
     float l1, l2, l3;
     symEigvals3(E, l1, l2, l3);
     if (!(1.f + l3 > 1e-8f)) return true;
-    const float e0 = 0.5f * log1pf(l1), e1 = 0.5f * log1pf(l2), e2 = 0.5f * log1pf(l3);
-    const float tr = (e0 + e1 + e2) * (1.f / 3.f);
+    const float c = dp ? mt.cohesion : 0.f;
+    const float e0 = 0.5f * log1pf(l1) - c, e1 = 0.5f * log1pf(l2) - c, e2 = 0.5f * log1pf(l3) - c;
+    const float sum = e0 + e1 + e2, tr = sum * (1.f / 3.f);
     const float h0 = e0 - tr, h1 = e1 - tr, h2 = e2 - tr;
     const float n = sqrtf(h0 * h0 + h1 * h1 + h2 * h2);
-    const float lim = ys / (2.f * mt.mu);
+    if (dp) {
+        if (!(sum < -1e-6f)) return true;
+        const float dgam = n + (3.f * mt.lambda + 2.f * mt.mu) / (2.f * mt.mu) * sum * mt.dp_alpha;
+        return dgam > -1e-6f;
+    }
+    const float lim = state / (2.f * mt.mu);
     return n - lim > -(1e-4f * lim + 1e-6f);
 }
 
@@ -1833,7 +1899,9 @@ __global__ __launch_bounds__(WG) void k_rare(Dev<T, AccT> D, T dt)
         Mat3<T> Fn = loadM(D.Q, cap, PF, s);
         if (mt.plasticity) {
             T ys = ldAt(D.Q + PYS * cap, s);
-            if (vonMisesProject(mt, Fn, ys)) {
+            const bool changed = mt.plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY ? vonMisesProject(mt, Fn, ys)
+                                                                                       : druckerPragerProject(mt, Fn, ys);
+            if (changed) {
 #pragma unroll
                 for (int k = 0; k < 9; k++) stAt(D.Q + (PF + k) * cap, s, Fn.a[k / 3][k % 3]);
                 stAt(D.Q + PYS * cap, s, ys);

@@ -43,7 +43,7 @@ enum : int {
     PG = 27,  // degradation g
     PLAP = 28, // damage Laplacian (output)
     PS = 29,   // V0 * Kirchhoff stress of the current F, symmetric: xx xy xz yy yz zz
-    PYS = 35,  // plastic yield stress (materials with plasticity)
+    PYS = 35,  // plasticity state: von Mises yield stress / Drucker-Prager log Jp
     NPF = 36
 };
 
@@ -111,8 +111,10 @@ struct MatDev {
     T inv_sigma_crit2; // 1 / sigma_c^2 (AnisoFractureSimulation.h:262)
     T phi_bound_scale; // (1 + sum |alpha_k| |a0_k|^2)^2 / sigma_c^2: phi <= this * |sigma|_F^2
     int plasticity;    // zmpm_plasticity
-    int pad2;
-    T hardening;       // VonMisesStvkHencky::xi
+    int volume_correction; // DruckerPragerStvkHencky::volume_correction
+    T hardening;       // VonMisesStvkHencky::xi / DruckerPragerStvkHencky::beta
+    T dp_alpha;        // DruckerPragerStvkHencky::alpha (from the friction angle)
+    T cohesion;        // DruckerPragerStvkHencky::cohesion
 };
 
 struct ColDev {

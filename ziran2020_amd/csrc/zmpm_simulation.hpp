@@ -62,7 +62,7 @@ public:
     std::vector<TM> C, F;
     std::vector<int32_t> material;
     std::vector<double> damage, degradation, laplacian;
-    std::vector<double> yield_stress; // VonMisesStvkHencky::yield_stress per particle
+    std::vector<double> plastic_state; // von Mises yield stress / Drucker-Prager log Jp per particle
     // ---- scene
     std::vector<zmpm_material> materials;
     std::vector<zmpm_collider> collision_objects;
@@ -112,7 +112,10 @@ public:
         damage.push_back(0);
         degradation.push_back(1);
         laplacian.push_back(0);
-        yield_stress.push_back(mat >= 0 && mat < (int)materials.size() ? materials[(size_t)mat].yield_stress : 0.0);
+        plastic_state.push_back(mat >= 0 && mat < (int)materials.size()
+                                        && materials[(size_t)mat].plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY
+                                    ? materials[(size_t)mat].yield_stress
+                                    : 0.0);
         dirty_ = true;
     }
 
@@ -282,7 +285,7 @@ private:
         p.d = damage.data();
         p.g = degradation.data();
         p.laplacian = laplacian.data();
-        p.yield_stress = yield_stress.data();
+        p.plastic_state = plastic_state.data();
         return p;
     }
 
