@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define ZMPM_ABI_VERSION 3
+#define ZMPM_ABI_VERSION 4
 
 /* Error model.  Replaces ZIRAN_ASSERT -> std::runtime_error
  * (Lib/Ziran/CS/Util/Debug.h:19-37) and the CFL RestartException
@@ -121,7 +121,10 @@ typedef struct {
                                 (APIC only, MpmSimulationBase.cpp:108-112)                */
     int32_t rpic_damping_iteration; /* MpmSimulationBase::rpic_damping_iteration (APIC)   */
     int32_t pic_damping_iteration;  /* MpmSimulationBase::pic_damping_iteration (FLIP)    */
-    int32_t reserved;
+    int32_t implicit_damage; /* AnisoFractureSimulation::implicit (AnisoFractureSimulation.h:61,
+                                129-141): 1 = the phase field of the damaged particles by the
+                                matrix-free CG solve of solveDamageSystem (:435-632), 0 = the
+                                explicit Laplacian update                                 */
 } zmpm_scene_desc;
 
 /* One material = one constitutive-model parameter set shared by its particles,
@@ -181,6 +184,9 @@ typedef struct {
     double* laplacian;  /* [n] damage Laplacian (download only), may be NULL              */
     double* plastic_state; /* [n] plasticity state (von Mises: yield stress, Drucker-Prager:
                               log Jp), may be NULL (upload: the material's yield stress / 0) */
+    double* max_dtilde; /* [n] AnisotropicPhaseField::maxDTilde, the history maximum of the
+                           crack driving force the implicit damage solve reads
+                           (AnisoFractureSimulation.h:284-337), may be NULL (0 on upload)  */
 } zmpm_particles;
 
 typedef struct zmpm_context* zmpm_handle;

@@ -444,6 +444,7 @@ int svd3(const M3& A, M3& U, double* sigma, M3& V, double tol)
 }
 
 const double kSvdTol = 1024 * std::numeric_limits<double>::epsilon(); // ImplicitQRSVD.h:526-533
+constexpr int kDamageCgIterations = 100; // ConjugateGradient<..>(100) (AnisoFractureSimulation.h:70)
 
 // polarDecomposition rotation R = U V^T (ImplicitQRSVD.h:546-558)
 M3 polarR(const M3& F)
@@ -667,12 +668,12 @@ void symEigen(const M3& S_in, double* lam, M3& Vec)
 
 inline double macaulay(double x) { return (x + std::fabs(x)) / 2; } // MathTools.h:276-281
 
-// updateDamageHelper for one particle (AnisoFractureSimulation.h:210-283) with
+// The crack driving force D~ = zeta <phi - 1>_+ of one particle
+// (AnisoFractureSimulation.h:222-268, and :302-332 for the implicit solve) with
 // constructStructuralTensor (:365-391) and evaluateEffectiveCauchyStress
-// (QRStableNeoHookean.h:83-93).  dt is the substep.
-void damageUpdate(const zmpm_material& mat, const M3& F, double lap, double dt, double& d, double& g)
+// (QRStableNeoHookean.h:83-93).
+double dTildeOf(const zmpm_material& mat, const M3& F)
 {
-    double geometricResist = d - (mat.l0 * mat.l0 * lap);
     // undegraded isotropic Cauchy stress: g = 1, kx = ky = 0
     double J = 1;
     M3 P = qrStablePiola(mat, 1.0, 0.0, 0.0, F, &J, nullptr);
@@ -706,7 +707,15 @@ void damageUpdate(const zmpm_material& mat, const M3& F, double lap, double dt, 
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) contraction += Asig.a[i][j] * sigA.a[i][j];
     double phi = (1.0 / (mat.sigma_crit * mat.sigma_crit)) * contraction;
-    double dTilde = mat.zeta * macaulay(phi - 1);
+    return mat.zeta * macaulay(phi - 1);
+}
+
+// updateDamageHelper for one particle (AnisoFractureSimulation.h:210-283).  dt
+// is the substep.
+void damageUpdate(const zmpm_material& mat, const M3& F, double lap, double dt, double& d, double& g)
+{
+    double geometricResist = d - (mat.l0 * mat.l0 * lap);
+    double dTilde = dTildeOf(mat, F);
     double diff = ((1 - d) * dTilde) - geometricResist;
     double newD = d + ((dt / mat.eta) * macaulay(diff));
     d = std::min(newD, 1.0);
@@ -851,8 +860,9 @@ struct zmpm_context {
     bool any_damage = false;
     // particle state (index order, never reordered)
     int64_t n = 0;
-    std::vector<double> x, v, C, F, m, V0, d, g, lap, ys;
+    std::vector<double> x, v, C, F, m, V0, d, g, lap, ys, dtil;
     std::vector<int32_t> mat;
+    int cg_iterations = -1; // last implicit damage solve
     // binning of the last substep
     std::vector<uint64_t> keys;
     std::vector<int32_t> order;
@@ -1226,6 +1236,159 @@ int gridToParticlesPass(zmpm_context* h, double dt, double r, double flip, bool 
 }
 
 // One explicit substep (MpmSimulationBase::advanceOneTimeStep, MpmSimulationBase.cpp:262-357).
+// AnisoFractureSimulation::solveDamageSystem (AnisoFractureSimulation.h:435-612),
+// after damageP2G's normalised phase field (:396-433) is on the grid:
+//   vol = V0 det F and maxDTilde = max(maxDTilde, D~(F)) per damaged particle
+//   (damageP2G :425-432, updateDTilde :284-337);
+//   rhs_i = sum_p (dt D~_p / eta + pf_i) vol_p w_ip                      (:457-474)
+//   (A x)_i = sum_p vol_p (1 + dt/eta (D~_p + 1)) w_ip x_i
+//           + sum_p grad w_ip . (vol_p dt/eta l0^2 sum_j x_j grad w_jp)  (:476-519)
+//   Jacobi preconditioner with the first term's diagonal                  (:521-544)
+//   ConjugateGradient::solve from x = 0, tolerance 1e-6, 100 iterations  (ConjugateGradient.h:52-103)
+//   pf_i <- x_i - pf_i (:603-605), then damageG2P (:614-680):
+//   d <- min(max(d, d + sum_i w_ip pf_i), 1), g = (1 - d)^2 (1 - k) + k.
+// Only particles whose material carries the phase field take part (the
+// reference gives every particle one); a node no such particle reaches has a
+// zero diagonal, its preconditioned residual is taken as 0.  Sums in the
+// reference's colour order; nodes with m = 0 (idx < 0) are left out.
+void solveDamageSystem(zmpm_context* h, double dt)
+{
+    const int64_t n = h->n;
+    const int ng = (int)h->groups.size();
+    int64_t N = 0;
+    for (const Node& g : h->nodes)
+        if (g.idx >= 0) N = std::max<int64_t>(N, g.idx + 1);
+    std::vector<char> on((size_t)n, 0);
+    std::vector<double> vol((size_t)n, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        const zmpm_material& mt = matOf(h, i);
+        if (!mt.has_damage || mt.model != ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) continue;
+        on[i] = 1;
+        const M3 F = mload(&h->F[9 * i]);
+        vol[i] = h->V0[i] * mdet(F);
+        const double dTilde = dTildeOf(mt, F);
+        h->dtil[i] = std::max(dTilde, h->dtil[i]);
+    }
+    // colour sweeps over the damaged particles (deterministic accumulation order)
+    auto sweep = [&](auto&& perParticle) {
+        for (uint64_t color = 0; color < 8; color++) {
+#pragma omp parallel for schedule(dynamic, 4)
+            for (int gi = 0; gi < ng; gi++) {
+                if ((h->block_offset[gi] & 7) != color) continue;
+                for (int idx = h->groups[gi].first; idx <= h->groups[gi].second; idx++) {
+                    const int i = h->order[idx];
+                    if (on[i]) perParticle(i);
+                }
+            }
+        }
+    };
+    auto coef = [&](int i) {
+        const zmpm_material& mt = matOf(h, i);
+        return vol[i] * (1 + ((dt / mt.eta) * (h->dtil[i] + 1)));
+    };
+    std::vector<double> rhs((size_t)N, 0.0);
+    sweep([&](int i) {
+        const zmpm_material& mt = matOf(h, i);
+        Stencil st;
+        makeStencil(h, &h->x[3 * i], st);
+        iterateKernel(h, st, [&](const int*, double w, const double*, double, Node& g) {
+            if (g.idx < 0) return;
+            rhs[(size_t)g.idx] += (((dt * h->dtil[i]) / mt.eta) + g.pf) * vol[i] * w;
+        });
+    });
+    std::vector<double> gradc((size_t)3 * n);
+    auto multiply = [&](const std::vector<double>& xv, std::vector<double>& b) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++) {
+            if (!on[i]) continue;
+            const zmpm_material& mt = matOf(h, i);
+            double gc[3] = { 0, 0, 0 };
+            Stencil st;
+            makeStencil(h, &h->x[3 * i], st);
+            iterateKernel(h, st, [&](const int*, double, const double* dw, double, Node& g) {
+                if (g.idx < 0) return;
+                for (int a = 0; a < 3; a++) gc[a] += xv[(size_t)g.idx] * dw[a];
+            });
+            const double sc = vol[i] * (dt / mt.eta) * mt.l0 * mt.l0;
+            for (int a = 0; a < 3; a++) gradc[3 * i + a] = gc[a] * sc;
+        }
+        std::fill(b.begin(), b.end(), 0.0);
+        sweep([&](int i) {
+            const double c = coef(i);
+            Stencil st;
+            makeStencil(h, &h->x[3 * i], st);
+            iterateKernel(h, st, [&](const int*, double w, const double* dw, double, Node& g) {
+                if (g.idx < 0) return;
+                b[(size_t)g.idx] += c * w * xv[(size_t)g.idx];
+                b[(size_t)g.idx] += gradc[3 * i] * dw[0] + gradc[3 * i + 1] * dw[1] + gradc[3 * i + 2] * dw[2];
+            });
+        });
+    };
+    std::vector<double> diag((size_t)N, 0.0);
+    sweep([&](int i) {
+        const double c = coef(i);
+        Stencil st;
+        makeStencil(h, &h->x[3 * i], st);
+        iterateKernel(h, st, [&](const int*, double w, const double*, double, Node& g) {
+            if (g.idx < 0) return;
+            diag[(size_t)g.idx] += c * w;
+        });
+    });
+    auto precondition = [&](const std::vector<double>& in, std::vector<double>& out) {
+        for (int64_t q = 0; q < N; q++) out[(size_t)q] = diag[(size_t)q] != 0 ? in[(size_t)q] / diag[(size_t)q] : 0.0;
+    };
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
+        double r = 0;
+        for (int64_t q = 0; q < N; q++) r += a[(size_t)q] * b[(size_t)q];
+        return r;
+    };
+    // ConjugateGradient::solve (ConjugateGradient.h:52-103), relative tolerance
+    // 1e-12 of the first residual (double), absolute 1e-6
+    std::vector<double> xs((size_t)N, 0.0), r = rhs, q((size_t)N), mr((size_t)N), sdir((size_t)N, 0.0);
+    multiply(xs, q);
+    for (int64_t k = 0; k < N; k++) r[(size_t)k] -= q[(size_t)k];
+    double conv = std::sqrt(dot(r, r));
+    const double rel = conv * (h->desc.precision == ZMPM_PREC_FP32 ? 1e-6 : 1e-12);
+    double rho_old = std::numeric_limits<double>::max();
+    int it = 0;
+    for (it = 0;; it++) {
+        precondition(r, mr);
+        const double rho = dot(mr, r);
+        for (int64_t k = 0; k < N; k++) sdir[(size_t)k] = it == 0 ? mr[(size_t)k] : rho / rho_old * sdir[(size_t)k] + mr[(size_t)k];
+        multiply(sdir, q);
+        const double sq = dot(sdir, q);
+        const double alpha = sq != 0 ? rho / sq : std::numeric_limits<double>::max();
+        for (int64_t k = 0; k < N; k++) {
+            xs[(size_t)k] = alpha * sdir[(size_t)k] + xs[(size_t)k];
+            r[(size_t)k] = -alpha * q[(size_t)k] + r[(size_t)k];
+        }
+        rho_old = rho;
+        conv = std::sqrt(dot(r, r));
+        if (conv < 1e-6 || conv < rel || it == kDamageCgIterations) break;
+    }
+    h->cg_iterations = it;
+    for (Node& g : h->nodes)
+        if (g.idx >= 0) g.pf = xs[(size_t)g.idx] - g.pf;
+    // damageG2P
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        if (!on[i]) continue;
+        const zmpm_material& mt = matOf(h, i);
+        double pf = 0;
+        Stencil st;
+        makeStencil(h, &h->x[3 * i], st);
+        iterateKernel(h, st, [&](const int*, double w, const double*, double, Node& g) {
+            if (g.idx >= 0) pf += w * g.pf;
+        });
+        double& d = h->d[i];
+        const double nd = d + pf;
+        d = std::min(std::max(d, nd), 1.0);
+        const double k = mt.residual_phase;
+        h->g[i] = ((1 - d) * (1 - d) * (1 - k)) + k;
+    }
+}
+
 zmpm_status substep(zmpm_context* h, double dt, double t)
 {
     if (h->n == 0) return ZMPM_OK;
@@ -1301,7 +1464,9 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
         }
     }
 
-    // AnisoMPM explicit damage (AnisoFractureSimulation.h:129-141): computeLaplacians then updateDamage
+    // AnisoMPM damage (AnisoFractureSimulation.h:129-141): the normalised phase
+    // field on the grid (computeLaplacians :150-180 / damageP2G :396-420), then
+    // the explicit Laplacian update (updateDamage) or the implicit solve
     if (h->any_damage) {
         for (uint64_t color = 0; color < 8; color++) {
 #pragma omp parallel for schedule(dynamic, 4)
@@ -1324,6 +1489,10 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
             Node& g = h->nodes[q];
             if (g.idx >= 0) g.pf /= g.pfm;
         }
+    }
+    if (h->any_damage && h->desc.implicit_damage)
+        solveDamageSystem(h, dt);
+    else if (h->any_damage) {
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; i++) {
             Stencil s;
@@ -1498,6 +1667,7 @@ zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
     if (apic && !(desc->apic_rpic_ratio >= 0 && desc->apic_rpic_ratio <= 1)) return ZMPM_E_ARG;
     if (desc->rpic_damping_iteration < 0 || desc->pic_damping_iteration < 0) return ZMPM_E_ARG;
     if ((desc->rpic_damping_iteration > 0 && !apic) || (desc->pic_damping_iteration > 0 && apic)) return ZMPM_E_ARG;
+    if (desc->implicit_damage != 0 && desc->implicit_damage != 1) return ZMPM_E_ARG;
     auto* h = new zmpm_context();
     h->desc = *desc;
     h->L = makeLayout(desc->precision == ZMPM_PREC_FP32 ? 6 : 7);
@@ -1578,6 +1748,8 @@ zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
                                     : (mt.plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY ? mt.yield_stress : 0.0);
     }
     h->lap.assign(n, 0.0);
+    h->dtil.assign(n, 0.0);
+    if (p->max_dtilde) h->dtil.assign(p->max_dtilde, p->max_dtilde + n);
     if (p->d) h->d.assign(p->d, p->d + n);
     if (p->g) h->g.assign(p->g, p->g + n);
     h->latched = ZMPM_OK;
@@ -1600,6 +1772,7 @@ zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
     if (p->g) std::copy(h->g.begin(), h->g.end(), p->g);
     if (p->laplacian) std::copy(h->lap.begin(), h->lap.end(), p->laplacian);
     if (p->plastic_state) std::copy(h->ys.begin(), h->ys.end(), p->plastic_state);
+    if (p->max_dtilde) std::copy(h->dtil.begin(), h->dtil.end(), p->max_dtilde);
     (void)n;
     zmpm_status s = (zmpm_status)h->latched;
     return s;
@@ -1741,6 +1914,9 @@ zmpm_status zmpm_autotune(zmpm_handle h, double dt, double, int32_t steps, int32
 }
 
 // ------------------------------------------------ unit entry points (pinning)
+// CG iterations of the handle's last implicit damage solve (-1: none yet)
+int32_t zmpm_oracle_cg_iterations(zmpm_handle h) { return h ? h->cg_iterations : -1; }
+
 uint64_t zmpm_oracle_linear_offset(int32_t log2_struct, int32_t i, int32_t j, int32_t k)
 {
     return linearOffset(makeLayout(log2_struct), i, j, k);

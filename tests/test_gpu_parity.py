@@ -26,6 +26,13 @@ def small_slab(n=20000, sigma_crit=20.0, seed=2):
                               sigma_crit=sigma_crit, pull_speed=0.5, seed=seed)
 
 
+def implicit_slab(n=20000, eta=1e-3, seed=2):
+    # the small slab with a short damage time scale: the implicit solve runs a
+    # few CG iterations per substep and the phase field grows within a few substeps
+    return scenes.damage_slab(n=n, dx=1.0 / 64, size=(0.5, 0.15, 0.15), domain_nodes=128,
+                              sigma_crit=20.0, pull_speed=0.5, seed=seed, eta=eta)
+
+
 def compare_binning(a, b):
     assert np.array_equal(a["keys"], b["keys"])
     assert np.array_equal(a["order"], b["order"])
@@ -297,14 +304,19 @@ def test_damage_slab_pic(gpu, oracle, precision):
 
 
 @pytest.mark.parametrize("precision", [A.PREC_FP32, A.PREC_FP64])
-@pytest.mark.parametrize("transfer", ["apic", "flip_damped"])
+@pytest.mark.parametrize("transfer", ["apic", "flip_damped", "implicit_damage"])
 def test_graph_replays_match_eager_substeps(gpu, precision, transfer):
     """zmpm_run replays a captured two-substep HIP graph; every replay, with host
     downloads in between, matches eager one-at-a-time substeps (bitwise for the
-    deterministic FP32 form, to FP64 rounding otherwise)."""
-    sc = small_slab(n=8000)
-    kw = {} if transfer == "apic" else dict(transfer_scheme=A.FLIP_BLEND_PIC, flip_pic_ratio=0.5,
-                                              pic_damping_iteration=2)
+    deterministic FP32 form, to FP64 rounding otherwise; the implicit damage
+    solve sums in double with HBM atomics, so there the FP32 form agrees to float
+    rounding)."""
+    kw = {"apic": {}, "flip_damped": dict(transfer_scheme=A.FLIP_BLEND_PIC, flip_pic_ratio=0.5,
+                                          pic_damping_iteration=2),
+          "implicit_damage": dict(implicit_damage=True)}[transfer]
+    sc = small_slab(n=8000) if transfer != "implicit_damage" else implicit_slab(n=8000)
+    exact = precision == A.PREC_FP32 and transfer != "implicit_damage"
+    tol = 1e-12 if precision == A.PREC_FP64 else 1e-5
     g = sc.make(gpu, precision, **kw)
     e = sc.make(gpu, precision, **kw)
     t = 0.0
@@ -315,19 +327,19 @@ def test_graph_replays_match_eager_substeps(gpu, precision, transfer):
         t += 2 * sc.dt
         a, b = g.download(), e.download()
         for k in ("x", "v", "C", "F", "d"):
-            if precision == A.PREC_FP32:
+            if exact:
                 assert np.array_equal(a[k], b[k]), (rep, k)
             else:
-                assert normwise(a[k], b[k]) < 1e-12, (rep, k)
+                assert normwise(a[k], b[k]) < tol, (rep, k)
     # an odd step count ends on the other buffer; the next run captures its own graph
     g.run(3, sc.dt, t)
     for k in range(3):
         e.substep(sc.dt, t + k * sc.dt)
     a, b = g.download(), e.download()
-    if precision == A.PREC_FP32:
+    if exact:
         assert np.array_equal(a["x"], b["x"])
     else:
-        assert normwise(a["x"], b["x"]) < 1e-12
+        assert normwise(a["x"], b["x"]) < tol
 
 
 @pytest.mark.parametrize("case", ["corotated", "corotated_hardening", "damage_slab"])
@@ -461,3 +473,67 @@ def test_autotune_restores_the_state(gpu):
     b = _fields(run(gpu, sc, A.PREC_FP32, 4).download())
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+# ------------------------------------------------------- implicit damage solve
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_implicit_damage_slab(gpu, oracle, precision):
+    # AnisoFractureSimulation with implicit = true: the phase field by the
+    # matrix-free CG solve (AnisoFractureSimulation.h:435-680)
+    sc = implicit_slab()
+    steps = 20
+    g = run(gpu, sc, precision, steps, implicit_damage=True).download()
+    os_ = run(oracle, sc, A.PREC_FP64, steps, implicit_damage=True)
+    o = os_.download()
+    assert o["d"].max() > 0.5 and oracle.lib.zmpm_oracle_cg_iterations(os_.h) >= 1
+    tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+    for k in ("x", "v", "F", "d", "g", "max_dtilde"):
+        assert normwise(g[k], o[k]) < (tol if k not in ("d", "max_dtilde") else 10 * tol), k
+    assert g["laplacian"].max() == 0.0
+
+
+def test_implicit_damage_solution_many_iterations(gpu, oracle):
+    # V0 = 1 and eta = dt make the solve's absolute tolerance tight relative to
+    # its right-hand side, so the CG runs tens of iterations; the grid's x - pf
+    # and the particles' d follow the oracle
+    rng = np.random.default_rng(12)
+    dx = 1.0 / 32
+    lo = np.full(3, 16 * dx)
+    x = scenes.stratified_box(rng, lo, lo + 6 * dx, dx, 2)
+    n = x.shape[0]
+    F = (np.eye(3).reshape(1, 9) + 0.01 * rng.normal(size=(n, 9)))
+    mt = A.make_material(A.MODEL_QR_STABLE_NEO_HOOKEAN, 1e4, 0.3, has_damage=True, alphas=(-0.9,),
+                         sigma_crit=20.0, eta=1e-4, zeta=1.0, residual_phase=1e-3, l0=2 * dx)
+    sc = scenes.Scene("implicit_cube", dx, 64, 1e-4, 1, (0.0, 0.0, 0.0), [mt], [], x, np.zeros(n, np.int32),
+                      np.full(n, 1000.0), np.ones(n), F)
+    d0 = rng.uniform(0.0, 0.6, n)
+    sims = []
+    for lib in (gpu, oracle):
+        sim = sc.make(lib, A.PREC_FP64, implicit_damage=True)
+        sim.upload(sc.x, sc.v, sc.C, sc.F, sc.m, sc.V0, sc.material, d=d0)
+        sim.substep(sc.dt, 0.0)
+        sims.append(sim)
+    assert oracle.lib.zmpm_oracle_cg_iterations(sims[1].h) >= 10
+    g, o = sims[0].grid(), sims[1].grid()
+    assert np.array_equal(g["coords"], o["coords"])
+    assert np.abs(o["pf"]).max() > 1e-3
+    assert normwise(g["pf"], o["pf"]) < 1e-8
+    gd, od = sims[0].download(), sims[1].download()
+    assert (od["d"] > d0).sum() > 10
+    for k in ("d", "g", "max_dtilde", "x", "v", "F"):
+        assert normwise(gd[k], od[k]) < 1e-8, k
+
+
+def test_implicit_damage_many_blocks(gpu, oracle):
+    # more grid blocks than the solve has workgroups: its block loop strides
+    sc = scenes.damage_slab(n=200_000, dx=1.0 / 128, size=(1.0, 0.2, 0.2), sigma_crit=20.0, pull_speed=0.5,
+                            eta=1e-3, seed=7)
+    g = run(gpu, sc, A.PREC_FP64, 3, implicit_damage=True)
+    o = run(oracle, sc, A.PREC_FP64, 3, implicit_damage=True)
+    assert g.binning()["groups"].shape[0] > 1024
+    gd, od = g.download(), o.download()
+    assert od["d"].max() > 0.0
+    for k in ("x", "v", "F", "d", "g", "max_dtilde"):
+        assert normwise(gd[k], od[k]) < FP64_TOL * 10, k
+    gg, og = g.grid(), o.grid()
+    assert normwise(gg["pf"], og["pf"]) < 1e-8

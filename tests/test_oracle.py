@@ -771,3 +771,131 @@ def test_drucker_prager_return_mapping(O):
     # a previous tip projection's log Jp shifts the trace
     G, lj = drucker_prager(O, m, 0.5, np.eye(3) * 0.99)
     assert abs(lj - (0.5 + 3 * np.log(0.99))) < 1e-12
+
+
+# ------------------------------------------------------- implicit damage solve
+def _implicit_cube(F, d=None, V0=None, sigma_crit=20.0, eta=0.1, cells=4, seed=11, dx=1.0 / 32):
+    """A free cube of QR-stable neo-Hookean particles with the phase field, at
+    rest, no gravity, no colliders; per-particle F (n x 9 or one 3x3)."""
+    rng = np.random.default_rng(seed)
+    lo = np.full(3, 16 * dx)
+    x = scenes.stratified_box(rng, lo, lo + cells * dx, dx, 2)
+    n = x.shape[0]
+    Fa = np.broadcast_to(np.asarray(F, float).reshape(-1, 9), (n, 9)).copy()
+    mt = A.make_material(A.MODEL_QR_STABLE_NEO_HOOKEAN, 1e4, 0.3, has_damage=True, alphas=(-0.9,),
+                         sigma_crit=sigma_crit, eta=eta, zeta=1.0, residual_phase=1e-3, l0=2 * dx)
+    V = np.full(n, (cells * dx) ** 3 / n) if V0 is None else np.full(n, V0)
+    sc = scenes.Scene("implicit_cube", dx, 64, 1e-4, 1, (0.0, 0.0, 0.0), [mt], [], x, np.zeros(n, np.int32),
+                      np.full(n, 1000.0), V, Fa)
+    return sc, mt, (np.zeros(n) if d is None else d)
+
+
+def test_implicit_damage_uniform_driving_force_closed_form(oracle):
+    # Every particle with the same F has the same D~; with d = 0 the right-hand
+    # side and the lumped diagonal are the same multiple of sum vol w at every
+    # node and K 1 = 0 (sum_j grad w_jp = 0), so the solve returns the constant
+    # x = (dt D~ / eta) / (1 + dt (D~ + 1) / eta) after one CG iteration, and
+    # damageG2P gives every particle d = x (sum w = 1)
+    # (AnisoFractureSimulation.h:435-680, ConjugateGradient.h:52-103).
+    s = 1.01
+    sc, m, _ = _implicit_cube(np.diag([s, 1.0, 1.0]))
+    sim = sc.make(oracle, implicit_damage=True)
+    sim.substep(sc.dt, 0.0)
+    out = sim.download()
+    lam, mu, alpha = m.lam, m.mu, -0.9
+    sxx = (lam + mu) * (s - 1)
+    syy = (s - 1) * (lam * s - mu) / s
+    phi = ((1 + alpha) ** 2 * sxx ** 2 + 2 * syy ** 2) / m.sigma_crit ** 2
+    dtil = m.zeta * max(phi - 1, 0.0)
+    assert dtil > 1
+    r = sc.dt / m.eta
+    want = r * dtil / (1 + r * (dtil + 1))
+    assert np.abs(out["max_dtilde"] - dtil).max() < 1e-9 * dtil
+    assert np.abs(out["d"] - want).max() < 1e-12
+    k = m.residual_phase
+    assert np.abs(out["g"] - ((1 - want) ** 2 * (1 - k) + k)).max() < 1e-12
+    assert out["laplacian"].max() == 0.0
+    # below the threshold nothing grows, and the history keeps its maximum
+    sc2, _, _ = _implicit_cube(np.eye(3), sigma_crit=1e9)
+    sim2 = sc2.make(oracle, implicit_damage=True)
+    sim2.upload(sc2.x, sc2.v, sc2.C, sc2.F, sc2.m, sc2.V0, sc2.material, max_dtilde=np.full(sc2.n, 0.5))
+    sim2.substep(sc2.dt, 0.0)
+    o2 = sim2.download()
+    assert np.all(o2["max_dtilde"] == 0.5)
+    want2 = r * 0.5 / (1 + r * 1.5)
+    assert np.abs(o2["d"] - want2).max() < 1e-12
+
+
+def _bspline_np(xs):
+    """Quadratic B-spline of BSplines.h:72-98 in index space: base, w, dw per axis."""
+    base = np.floor(xs - 0.5)
+    d0 = xs - base
+    w = np.stack([0.5 * (1.5 - d0) ** 2, 0.75 - (d0 - 1) ** 2, 0.5 * (d0 - 0.5) ** 2], -1)
+    dw = np.stack([d0 - 1.5, -2 * (d0 - 1), d0 - 0.5], -1)
+    return base.astype(int), w, dw
+
+
+def test_implicit_damage_matches_dense_solve(oracle):
+    # The matrix-free operator and the CG against an assembled dense system,
+    # solved directly: A_ij = delta_ij sum_p vol_p (1 + dt/eta (D~_p + 1)) w_ip
+    # + sum_p vol_p dt/eta l0^2 grad w_ip . grad w_jp, rhs_i = sum_p (dt D~_p / eta
+    # + pf_i) vol_p w_ip, pf_i = sum_p w_ip d_p / sum_p w_ip.  V0 = 1 and eta = dt
+    # make the CG's absolute tolerance (1e-6) tight and the stiffness term O(1).
+    rng = np.random.default_rng(12)
+    n = 512
+    F = np.eye(3).reshape(1, 9) + 0.01 * rng.normal(size=(n, 9))
+    d0 = rng.uniform(0.0, 0.6, n)
+    sc, m, _ = _implicit_cube(F, V0=1.0, eta=1e-4)
+    assert sc.n == n
+    sim = sc.make(oracle, implicit_damage=True)
+    sim.upload(sc.x, sc.v, sc.C, sc.F, sc.m, sc.V0, sc.material, d=d0)
+    sim.substep(sc.dt, 0.0)
+    out = sim.download()
+    grid = sim.grid()
+    its = oracle.lib.zmpm_oracle_cg_iterations(sim.h)
+    assert 3 <= its < 100
+    dtil = out["max_dtilde"]
+    assert (dtil > 0).sum() > n // 4
+    dx, dt = sc.dx, sc.dt
+    vol = sc.V0 * np.linalg.det(sc.F.reshape(n, 3, 3))
+    base, w, dw = _bspline_np(sc.x / dx)
+    nodes = {tuple(c): i for i, c in enumerate(grid["coords"].tolist())}
+    N = len(nodes)
+    cd = vol * (1 + dt / m.eta * (dtil + 1))
+    kap = vol * dt / m.eta * m.l0 ** 2
+    Amat = np.zeros((N, N))
+    A1, rr, sw, swd = np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(N)
+    for p_ in range(n):
+        ids, ws, gs = [], [], []
+        for i in range(3):
+            for j in range(3):
+                for k in range(3):
+                    ids.append(nodes[(base[p_, 0] + i, base[p_, 1] + j, base[p_, 2] + k)])
+                    wt = w[p_, 0, i] * w[p_, 1, j] * w[p_, 2, k]
+                    ws.append(wt)
+                    gs.append(np.array([dw[p_, 0, i] * w[p_, 1, j] * w[p_, 2, k], w[p_, 0, i] * dw[p_, 1, j] * w[p_, 2, k],
+                                        w[p_, 0, i] * w[p_, 1, j] * dw[p_, 2, k]]) / dx)
+        ids, ws, gs = np.array(ids), np.array(ws), np.array(gs)
+        Amat[ids, ids] += cd[p_] * ws
+        Amat[np.ix_(ids, ids)] += kap[p_] * gs @ gs.T
+        A1[ids] += vol[p_] * ws
+        rr[ids] += dt * dtil[p_] / m.eta * vol[p_] * ws
+        sw[ids] += ws
+        swd[ids] += ws * d0[p_]
+    pf = swd / sw
+    xsol = np.linalg.solve(Amat, rr + pf * A1)
+    delta = xsol - pf
+    assert np.abs(delta).max() > 1e-3
+    assert np.abs(grid["pf"] - delta).max() < 1e-6 * np.abs(delta).max()
+    # damageG2P: d <- min(max(d, d + sum w (x - pf)), 1)
+    want = np.empty(n)
+    for p_ in range(n):
+        acc = 0.0
+        for i in range(3):
+            for j in range(3):
+                for k in range(3):
+                    acc += w[p_, 0, i] * w[p_, 1, j] * w[p_, 2, k] * delta[nodes[(base[p_, 0] + i, base[p_, 1] + j,
+                                                                                  base[p_, 2] + k)]]
+        want[p_] = min(max(d0[p_], d0[p_] + acc), 1.0)
+    assert np.abs(out["d"] - want).max() < 1e-6
+    assert (out["d"] > d0).sum() > 10

@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-ZMPM_ABI_VERSION = 3
+ZMPM_ABI_VERSION = 4
 
 # zmpm_status
 OK, E_ARG, E_HIP, E_DOMAIN, E_CFL, E_NAN, E_OVERFLOW, E_NOMEM = range(8)
@@ -45,7 +45,7 @@ class SceneDesc(C.Structure):
         ("mls_mpm", C.c_int32),
         ("rpic_damping_iteration", C.c_int32),
         ("pic_damping_iteration", C.c_int32),
-        ("reserved", C.c_int32),
+        ("implicit_damage", C.c_int32),
     ]
 
 
@@ -88,7 +88,7 @@ class Particles(C.Structure):
         ("n", C.c_int64),
         ("x", _DP), ("v", _DP), ("C", _DP), ("F", _DP),
         ("m", _DP), ("V0", _DP), ("material", _IP),
-        ("d", _DP), ("g", _DP), ("laplacian", _DP), ("plastic_state", _DP),
+        ("d", _DP), ("g", _DP), ("laplacian", _DP), ("plastic_state", _DP), ("max_dtilde", _DP),
     ]
 
 
@@ -215,7 +215,7 @@ class Simulation:
 
     def __init__(self, lib: ZmpmLib, *, dx, precision=PREC_FP64, domain_nodes=256, gravity=(0, 0, 0),
                  apic_rpic_ratio=1.0, cfl=0.6, device=0, transfer_scheme=APIC_BLEND_RPIC, flip_pic_ratio=0.0,
-                 mls_mpm=False, rpic_damping_iteration=0, pic_damping_iteration=0):
+                 mls_mpm=False, rpic_damping_iteration=0, pic_damping_iteration=0, implicit_damage=False):
         self.lib = lib
         self.L = lib.lib
         d = SceneDesc()
@@ -225,6 +225,7 @@ class Simulation:
         d.apic_rpic_ratio, d.cfl, d.device = apic_rpic_ratio, cfl, device
         d.transfer_scheme, d.flip_pic_ratio, d.mls_mpm = transfer_scheme, flip_pic_ratio, int(bool(mls_mpm))
         d.rpic_damping_iteration, d.pic_damping_iteration = rpic_damping_iteration, pic_damping_iteration
+        d.implicit_damage = int(implicit_damage)
         self.desc = d
         h = C.c_void_p()
         self._check(self.L.zmpm_create(C.byref(d), C.byref(h)), "zmpm_create", handle=False)
@@ -263,7 +264,7 @@ class Simulation:
         self._check(self.L.zmpm_add_collider(self.h, C.byref(c), C.byref(i)), "zmpm_add_collider")
         return i.value
 
-    def upload(self, x, v, Cm, F, m, V0, material, d=None, g=None, plastic_state=None):
+    def upload(self, x, v, Cm, F, m, V0, material, d=None, g=None, plastic_state=None, max_dtilde=None):
         n = int(x.shape[0])
         arrs = dict(
             x=np.ascontiguousarray(x, np.float64).reshape(n, 3),
@@ -276,10 +277,11 @@ class Simulation:
             d=None if d is None else np.ascontiguousarray(d, np.float64),
             g=None if g is None else np.ascontiguousarray(g, np.float64),
             plastic_state=None if plastic_state is None else np.ascontiguousarray(plastic_state, np.float64),
+            max_dtilde=None if max_dtilde is None else np.ascontiguousarray(max_dtilde, np.float64),
         )
         p = Particles()
         p.n = n
-        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "plastic_state"):
+        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "plastic_state", "max_dtilde"):
             setattr(p, k, _ptr(arrs[k]))
         p.material = _ptr(arrs["material"], C.c_int32)
         p.laplacian = None
@@ -291,10 +293,10 @@ class Simulation:
         n = self.n
         out = dict(x=np.zeros((n, 3)), v=np.zeros((n, 3)), C=np.zeros((n, 9)), F=np.zeros((n, 9)),
                    m=np.zeros(n), V0=np.zeros(n), material=np.zeros(n, np.int32), d=np.zeros(n),
-                   g=np.zeros(n), laplacian=np.zeros(n), plastic_state=np.zeros(n))
+                   g=np.zeros(n), laplacian=np.zeros(n), plastic_state=np.zeros(n), max_dtilde=np.zeros(n))
         p = Particles()
         p.n = n
-        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "laplacian", "plastic_state"):
+        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "laplacian", "plastic_state", "max_dtilde"):
             setattr(p, k, _ptr(out[k]))
         p.material = _ptr(out["material"], C.c_int32)
         st = self.L.zmpm_download(self.h, C.byref(p))

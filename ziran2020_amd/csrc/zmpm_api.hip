@@ -17,8 +17,8 @@ namespace {
 
 constexpr int kMaxMaterials = 256;
 constexpr int kMaxColliders = 16;
-constexpr int kTimedGroups = 4;
-const char* const kTimedNames[kTimedGroups + 1] = { "bin_sort", "p2g", "grid_update", "g2p", "substep" };
+constexpr int kTimedGroups = 5;
+const char* const kTimedNames[kTimedGroups + 1] = { "bin_sort", "p2g", "grid_update", "damage_solve", "g2p", "substep" };
 
 uint64_t spreadMask(int log2_struct, int axis)
 {
@@ -188,7 +188,23 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.scales, NSC);
     ALLOC(D.maxbits, (int64_t)NSC * BOUND_SLOTS);
     ALLOC(D.extent, (int64_t)(EXTENT_WGS + 1) * 8);
+    if (h->desc.implicit_damage) {
+        ALLOC(D.cgv, 5 * D.cap_nodes);
+        ALLOC(D.cgp, 4 * cap);
+        ALLOC(D.cg_part, 3 * CG_MAX_WGS);
+        ALLOC(D.cg_bar, 2);
+    }
 #undef ALLOC
+    if (h->desc.implicit_damage) {
+        // k_cg's grid barriers need every workgroup resident: at most two per CU,
+        // within what the occupancy calculator allows
+        int occ = 0, cus = 0;
+        HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cg<T, AccT>, WG, 0));
+        HIPCHK(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->desc.device));
+        D.cg_wgs = std::max(1, std::min(CG_MAX_WGS, cus * std::min(occ, 2)));
+        if (occ < 1) return fail(h, ZMPM_E_HIP, "damage solve kernel cannot be resident");
+        HIPCHK(h, hipMemsetAsync(D.cg_bar, 0, 2 * sizeof(unsigned), h->stream));
+    }
     HIPCHK(h, hipMemsetAsync(D.touched, 0, D.num_pages * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.page_count, 0, D.num_pages * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.scan_state, 0, scan_tiles * sizeof(uint64_t), h->stream));
@@ -230,6 +246,9 @@ void refreshTables(zmpm_context* h)
     D.flip = h->desc.transfer_scheme == ZMPM_FLIP_BLEND_PIC ? 1 : 0;
     D.mls = h->desc.mls_mpm ? 1 : 0;
     D.flip_ratio = (T)h->desc.flip_pic_ratio;
+    D.implicit = h->desc.implicit_damage ? 1 : 0;
+    // ConjugateGradient's relative tolerance for its scalar type (ConjugateGradient.h:70)
+    D.cg_rel = std::is_same<T, float>::value ? 1e-6 : 1e-12;
 }
 
 template <class T>
@@ -325,6 +344,12 @@ void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
     if (ev) (void)hipEventRecord(ev[2], st);
     k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D, (T)dt);
     if (ev) (void)hipEventRecord(ev[3], st);
+    if (D.implicit && D.any_damage) {
+        // AnisoFractureSimulation::solveDamageSystem (AnisoFractureSimulation.h:435-612)
+        k_dtilde<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D, (T)dt);
+        k_cg<T, AccT><<<D.cg_wgs, WG, 0, st>>>(D);
+    }
+    if (ev) (void)hipEventRecord(ev[4], st);
     if (D.flip)
         k_g2p<T, AccT, true><<<D.max_slots, WG, 0, st>>>(D, (T)dt);
     else
@@ -343,7 +368,7 @@ void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
         k_damp_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
         k_damp_g2p<T, AccT><<<D.max_slots, WG, 0, st>>>(D);
     }
-    if (ev) (void)hipEventRecord(ev[4], st);
+    if (ev) (void)hipEventRecord(ev[5], st);
 }
 
 zmpm_status deviceStatus(zmpm_context* h, bool sync)
@@ -360,6 +385,7 @@ zmpm_status deviceStatus(zmpm_context* h, bool sync)
     if (bits & ST_OVERFLOW) return fail(h, ZMPM_E_OVERFLOW, "fixed-point P2G contribution out of range");
     if (bits & ST_CAPACITY) return fail(h, ZMPM_E_NOMEM, "grid block capacity exceeded (set ZMPM_BLOCK_CAPACITY)");
     if (bits & ST_SCAN) return fail(h, ZMPM_E_HIP, "binning page scan did not complete");
+    if (bits & ST_BARRIER) return fail(h, ZMPM_E_HIP, "damage solve grid barrier did not complete");
     if (bits & ST_CFL) return fail(h, ZMPM_E_CFL, "a particle moved more than dx in one substep");
     return ZMPM_OK;
 }
@@ -418,6 +444,7 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
         const int mi = p->material[i];
         const bool vm = mi >= 0 && mi < (int)h->mats.size() && h->mats[(size_t)mi].plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY;
         buf[PYS * cap + i] = p->plastic_state ? (T)p->plastic_state[i] : (vm ? (T)h->mats[(size_t)mi].yield_stress : T(0));
+        buf[PDT * cap + i] = p->max_dtilde ? (T)p->max_dtilde[i] : T(0);
     }
     std::vector<int> ids(cap, 0), mat(cap, 0);
     for (int i = 0; i < n; i++) {
@@ -497,6 +524,7 @@ zmpm_status downloadT(zmpm_context* h, zmpm_particles* p)
         if (p->g) p->g[i] = (double)buf[PG * cap + s];
         if (p->laplacian) p->laplacian[i] = (double)buf[PLAP * cap + s];
         if (p->plastic_state) p->plastic_state[i] = (double)buf[PYS * cap + s];
+        if (p->max_dtilde) p->max_dtilde[i] = h->desc.implicit_damage ? (double)buf[PDT * cap + s] : 0.0;
     }
     return ZMPM_OK;
 }
@@ -665,6 +693,7 @@ zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
     if (apic && !(desc->apic_rpic_ratio >= 0 && desc->apic_rpic_ratio <= 1)) return ZMPM_E_ARG;
     if (desc->rpic_damping_iteration < 0 || desc->pic_damping_iteration < 0) return ZMPM_E_ARG;
     if ((desc->rpic_damping_iteration > 0 && !apic) || (desc->pic_damping_iteration > 0 && apic)) return ZMPM_E_ARG;
+    if (desc->implicit_damage != 0 && desc->implicit_damage != 1) return ZMPM_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ZMPM_E_HIP;
     if (desc->device < 0 || desc->device >= ndev) return ZMPM_E_ARG;

@@ -15,6 +15,8 @@
 //                    field
 //   k_grid           v = mv/m, gravity + force, collision objects, phase-field
 //                    normalisation
+//   k_dtilde, k_cg   (implicit damage only) crack driving force history and the
+//                    phase-field CG solve
 //   k_g2p            APIC gather, x update, CFL flag, damage Laplacian and
 //                    update, F update, stress of the new state and the next
 //                    P2G's fixed-point bounds; writes particles in sorted order
@@ -1134,7 +1136,13 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
     const int64_t nn = (int64_t)D.counters[0] * NB;
     const int64_t cn = D.cap_nodes;
+    const bool cg = D.implicit && D.any_damage;
     for (int64_t node = (int64_t)blockIdx.x * WG + threadIdx.x; node < nn; node += (int64_t)gridDim.x * WG) {
+        if (cg) { // the damage solve's accumulators, filled by k_dtilde
+            D.cgv[node] = 0.0;
+            D.cgv[cn + node] = 0.0;
+            D.cgv[4 * cn + node] = 0.0;
+        }
         const T m = accValue(D, GM, node, SC_M);
         T f[3];
 #pragma unroll
@@ -1292,17 +1300,17 @@ __global__ __launch_bounds__(WG) void k_damp_grid(Dev<T, AccT> D)
     }
 }
 
-// AnisoMPM explicit damage update of one particle (AnisoFractureSimulation.h:210-283,
-// structural tensor :365-391, undegraded Cauchy stress QRStableNeoHookean.h:83-93).
-// Fs is the stored deformation: F (FP64 form) or H = F - I (FP32 form).
-// FAST (G2P): a particle whose phi bound does not rule out <phi - 1>_+ > 0 is not
-// updated here; the function returns true and the particle is queued for
-// k_rare, which runs this update in full.  That keeps the eigen split and the
-// polar rotation, rarely needed, out of G2P's registers.
+// The crack driving force D~ = zeta <phi - 1>_+ of one particle
+// (updateDamageHelper / updateDTildeHelper, AnisoFractureSimulation.h:222-268,
+// 296-332; structural tensor :365-391, undegraded Cauchy stress
+// QRStableNeoHookean.h:83-93).  Fs is the stored deformation: F (FP64 form) or
+// H = F - I (FP32 form).  FAST (G2P): a particle whose phi bound does not rule
+// out <phi - 1>_+ > 0 is not evaluated; the function returns false and the
+// caller queues the particle for k_rare, which runs the full path.  That keeps
+// the eigen split and the polar rotation, rarely needed, out of G2P's registers.
 template <class T, bool FAST = false>
-__device__ __forceinline__ bool damageUpdate(const MatDev<T>& mt, const Mat3<T>& Fs, T lap, T dt, T& d, T& g)
+__device__ __forceinline__ bool dTildeOf(const MatDev<T>& mt, const Mat3<T>& Fs, T& dTilde)
 {
-    const T geometricResist = d - (mt.l0 * mt.l0 * lap);
     // undegraded Cauchy stress tau(g = 1, no fibres) / J (evaluateEffectiveCauchyStress,
     // QRStableNeoHookean.h:83-93); tau is exactly symmetric in its closed form
     T J, t6[6];
@@ -1324,9 +1332,9 @@ __device__ __forceinline__ bool damageUpdate(const MatDev<T>& mt, const Mat3<T>&
     // sigma^+ and the polar rotation contribute nothing
     const T sf2 = sym.a[0][0] * sym.a[0][0] + sym.a[1][1] * sym.a[1][1] + sym.a[2][2] * sym.a[2][2]
         + T(2) * (sym.a[0][1] * sym.a[0][1] + sym.a[0][2] * sym.a[0][2] + sym.a[1][2] * sym.a[1][2]);
-    T dTilde = T(0);
+    dTilde = T(0);
     if (!(mt.phi_bound_scale * sf2 < T(0.99))) {
-        if constexpr (FAST) return true;
+        if constexpr (FAST) return false;
         const Mat3<T> sp = positivePart(sym);
         Mat3<T> R;
         if constexpr (sizeof(T) == 4)
@@ -1353,6 +1361,18 @@ __device__ __forceinline__ bool damageUpdate(const MatDev<T>& mt, const Mat3<T>&
         const T phi = mt.inv_sigma_crit2 * contraction;
         dTilde = mt.zeta * macaulay(phi - T(1));
     }
+    return true;
+}
+
+// updateDamageHelper for one particle (AnisoFractureSimulation.h:210-283) on
+// F_n.  FAST (G2P): returns true, d and g unchanged, when the exact path is
+// needed (the particle goes to k_rare).
+template <class T, bool FAST = false>
+__device__ __forceinline__ bool damageUpdate(const MatDev<T>& mt, const Mat3<T>& Fs, T lap, T dt, T& d, T& g)
+{
+    const T geometricResist = d - (mt.l0 * mt.l0 * lap);
+    T dTilde;
+    if (!dTildeOf<T, FAST>(mt, Fs, dTilde)) return true;
     const T diff = ((T(1) - d) * dTilde) - geometricResist;
     const T newD = d + (ddiv(dt, mt.eta) * macaulay(diff));
     d = newD < T(1) ? newD : T(1);
@@ -1385,7 +1405,9 @@ struct G2PAcc {
 // with grad(w) = (dw_x w_y w_z, w_x dw_y w_z, w_x w_y dw_z) / dx, and the
 // damage Laplacian sum d_i L_i with the reference's 3D weights
 // L = (ddw_x(i) w_y w_z + w_x ddw_x(i) w_z + w_x w_y ddw_x(i)) / dx^2
-// (MpmGrid.h:610-612 index ddw[1], ddw[2] with i as well).
+// (MpmGrid.h:610-612 index ddw[1], ddw[2] with i as well); under the implicit
+// damage solve the staged field is x - pf and the sum is sum w (x_i - pf_i)
+// (damageG2PHelper, AnisoFractureSimulation.h:631-642).
 // FLIP: also sum w v_old over the staged old velocities (fields 4-6).  cell >= 0:
 // the stencil's tile position from that cell of the segment's block (the damping
 // passes: this substep's binning, moved particle); otherwise from the particle.
@@ -1475,14 +1497,14 @@ __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[T
             A.gv[a][2] += wx * tdz[a];
             if constexpr (FLIP) A.oldV[a] += wx * to[a];
         }
-        if (dmg) lap += ((i == 1) ? T(-2) : T(1)) * (q0 + wx * (q1 + q2));
+        if (dmg) lap += D.implicit ? wx * q0 : ((i == 1) ? T(-2) : T(1)) * (q0 + wx * (q1 + q2));
     }
     // (called once per particle: the particle's stencil lies in one block tile)
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int c = 0; c < 3; c++) A.gv[a][c] *= odx;
-    A.lap += lap * (odx * odx);
+    A.lap += D.implicit ? lap : lap * (odx * odx);
 }
 
 // Particle update after the gather; writes sorted position s.
@@ -1595,7 +1617,7 @@ __device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     }
     stAt(Q + PM * cap, s, R.m);
     stAt(Q + PVOL * cap, s, R.V0);
-    stAt(Q + PLAP * cap, s, dmg ? A.lap : T(0));
+    stAt(Q + PLAP * cap, s, dmg && !D.implicit ? A.lap : T(0));
     stAt(D.qid, s, R.id);
     stAt(D.qmat, s, R.mid);
     T d = R.d, g = R.g;
@@ -1613,7 +1635,18 @@ __device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     }
     else {
         const Mat3<T> F = Fn;
-        if (dmg && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) rare = damageUpdate<T, true>(mt, F, A.lap, dt, d, g);
+        if (dmg && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
+            if (D.implicit) {
+                // damageG2PHelper (AnisoFractureSimulation.h:644-678): the solve's
+                // change, never healing, clamped at 1
+                const T nd = d + A.lap;
+                const T mx = d < nd ? nd : d; // std::max(d, new_d)
+                d = T(1) < mx ? T(1) : mx;    // std::min(., 1)
+                g = ((T(1) - d) * (T(1) - d) * (T(1) - mt.k)) + mt.k;
+            }
+            else
+                rare = damageUpdate<T, true>(mt, F, A.lap, dt, d, g);
+        }
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -1890,7 +1923,7 @@ __global__ __launch_bounds__(WG) void k_rare(Dev<T, AccT> D, T dt)
         const MatDev<T>& mt = D.mats[ldAt(D.qmat, s)];
         const bool dmg = D.any_damage && mt.has_damage;
         T d = ldAt(D.P + PD * cap, p), g = ldAt(D.P + PG * cap, p);
-        if (dmg && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
+        if (dmg && !D.implicit && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
             const Mat3<T> F = loadM(D.P, cap, PF, p); // F_n (FP32 form: H)
             damageUpdate<T, false>(mt, F, ldAt(D.Q + PLAP * cap, s), dt, d, g);
             stAt(D.Q + PD * cap, s, d);
@@ -2041,6 +2074,352 @@ __global__ __launch_bounds__(WG) void k_extent(Dev<T, AccT> D, int fold)
 #pragma unroll
         for (int k = 0; k < 7; k++) D.extent[row * 8 + k] = r[k];
     }
+}
+
+// ------------------------------------------------- implicit damage solve
+// AnisoFractureSimulation::solveDamageSystem (AnisoFractureSimulation.h:435-612)
+// between the grid update and G2P:
+//   k_dtilde  per damaged particle (block by block, sorted order): vol = V0 det F_n,
+//             maxDTilde = max(D~(F_n), maxDTilde) (updateDTilde :284-337), the
+//             solve's per-particle weights, and the scatter of sum vol w,
+//             sum vol dt D~ / eta w (rhs :457-474) and the lumped diagonal
+//             sum vol (1 + dt / eta (D~ + 1)) w (multiply / preconditioner
+//             :476-544) through an LDS tile of the block and its halo
+//   k_cg      ConjugateGradient::solve (ConjugateGradient.h:52-103) from x = 0,
+//             Jacobi preconditioner, tolerance 1e-6, 100 iterations, as one
+//             persistent launch: all workgroups resident, grid barriers between
+//             the node phases and the block phase, every workgroup folding the
+//             same per-workgroup partial sums in the same order (so the stop
+//             decision is the same everywhere); then pf <- x - pf (:603-605),
+//             which G2P gathers for damageG2P (:614-680)
+// The stiffness product K s = sum_p grad w_p (vol dt / eta l0^2 sum_j s_j grad w_jp)
+// runs per block: s of the block and its halo staged in LDS, one thread per
+// particle gathers, then scatters into an LDS tile flushed with one HBM atomic
+// per node value.  s . A s is summed as sum_i diag_i s_i^2 + sum_p kappa_p |G_p|^2
+// (G_p = sum_j s_j grad w_jp), equal to the reference's dot product of s with
+// the assembled q, so alpha needs no pass over q.  The solve runs in double in
+// both forms; its HBM atomics make the last bits of x depend on scheduling.
+constexpr int CG_MAX_WGS = 2048;
+constexpr int CG_ITERATIONS = 100;       // ConjugateGradient<..>(100) (AnisoFractureSimulation.h:70)
+constexpr double CG_TOLERANCE = 1e-6;    // cg.setTolerance(1e-6) (AnisoFractureSimulation.h:600-601)
+constexpr int CG_SPIN_LIMIT = 1 << 18;   // barrier polls before the wait is given up (ST_BARRIER)
+
+// Barrier over every workgroup of a launch whose workgroups are all resident.
+// arrivals counts up through the launch (cleared before it); barrier number k
+// is passed when it reaches k * nwg.  Release before arriving and acquire after
+// leaving, at agent scope, make each workgroup's writes visible to the others
+// across the chip's L2 slices.  A wait that exceeds the poll limit latches
+// ST_BARRIER and goes on, and once that is latched no barrier waits any more,
+// so the launch always ends soon (with its results flagged as invalid).
+__device__ __forceinline__ void gridBarrier(unsigned* arrivals, unsigned nwg, unsigned& epoch, int* counters)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    epoch++;
+    if (threadIdx.x == 0) {
+        const unsigned target = epoch * nwg;
+        unsigned seen = __hip_atomic_fetch_add(arrivals, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        int polls = 0;
+        while (seen < target) {
+            if (++polls > CG_SPIN_LIMIT
+                || (__hip_atomic_load(&counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ST_BARRIER)) {
+                latch(counters, ST_BARRIER);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            seen = __hip_atomic_load(arrivals, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Sum over the workgroup, the same value in every thread (fixed order).
+__device__ __forceinline__ double wgSum(double v, double* red)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int w = 1; w < WG / 64; w++) r += red[w];
+    return r;
+}
+
+// Sum of the nwg per-workgroup partials part[0 .. nwg), identical in every
+// workgroup (each folds them in the same order).
+__device__ __forceinline__ double foldPartials(const double* part, int nwg, double* red)
+{
+    double v = 0.0;
+    for (int w = threadIdx.x; w < nwg; w += WG) v += part[w];
+    return wgSum(v, red);
+}
+
+// Flush an LDS tile of block b (block + forward halo) into a node array with
+// one HBM atomic per non-zero value.
+template <class T, class AccT>
+__device__ __forceinline__ void flushTile(const Dev<T, AccT>& D, int b, const double* tile, double* dst, int& ovf)
+{
+    using B = Blk<T>;
+    using TL = Tile<T>;
+    const int* nb = D.nbr + b * 8;
+    for (int nd = threadIdx.x; nd < TL::TN; nd += WG) {
+        const double v = tile[nd];
+        if (v == 0.0) continue;
+        const int tx = nd / (TL::TY * TL::TZ), ty = (nd / TL::TZ) % TL::TY, tz = nd % TL::TZ;
+        const int tb = nb[((tx >= TL::BX) << 2) | ((ty >= TL::BY) << 1) | (tz >= TL::BZ)];
+        if (tb < 0) {
+            ovf = 1;
+            continue;
+        }
+        const int elem = ((tx & (TL::BX - 1)) << (B::ybits + B::zbits)) | ((ty & (TL::BY - 1)) << B::zbits) | (tz & (TL::BZ - 1));
+        unsafeAtomicAdd(dst + (int64_t)tb * TL::NB + elem, v);
+    }
+}
+
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_dtilde(Dev<T, AccT> D, T dt)
+{
+    using B = Blk<T>;
+    using TL = Tile<T>;
+    constexpr int NB = TL::NB;
+    __shared__ double acc[3][TL::TN];
+    const int t = threadIdx.x;
+    if (blockIdx.x == 0 && t == 0) D.cg_bar[0] = 0u; // k_cg's barrier arrivals
+    const int64_t cap = D.cap, cn = D.cap_nodes;
+    const int nblocks = D.counters[CNT_BLOCKS];
+    int ovf = 0;
+    for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const int bs = D.block_start[b], cnt = D.block_count[b];
+        if (cnt <= 0) continue;
+        __syncthreads(); // previous block flushed
+        for (int e = t; e < 3 * TL::TN; e += WG) (&acc[0][0])[e] = 0.0;
+        __syncthreads();
+        for (int s = bs + t; s < bs + cnt; s += WG) {
+            int p = ldAt(D.perm, s);
+            if ((unsigned)p >= (unsigned)D.n) p = 0;
+            const MatDev<T>& mt = D.mats[ldAt(D.pmat, p)];
+            T hist = ldAt(D.P + PDT * cap, p);
+            T kap = T(0);
+            T d0[3] = { T(0), T(0), T(0) };
+            if (mt.has_damage && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
+                const Mat3<T> Fs = loadM(D.P, cap, PF, p);
+                T J;
+                if constexpr (sizeof(T) == 4)
+                    J = 1.f + jm1OfH(Fs);
+                else
+                    J = det3(Fs);
+                const T vol = ldAt(D.P + PVOL * cap, p) * J; // phase_field.vol (:425-432)
+                T dTilde;
+                dTildeOf<T, false>(mt, Fs, dTilde);
+                hist = dTilde < hist ? hist : dTilde; // std::max(dTilde, dTilde_history)
+                const T dte = dt / mt.eta;
+                kap = vol * dte * mt.l0 * mt.l0;
+                int base[3];
+                double w[3][3], dw[3][3];
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    cellOf(loadPos(D.P + (PX + a) * cap, p), D.odx, D.domain, base[a], d0[a]);
+                    bsplineFrac<double>((double)d0[a], w[a], dw[a]);
+                }
+                const double cw = (double)vol;                                   // sum vol w
+                const double cr = (double)((dt * hist) / mt.eta) * (double)vol; // D~ part of rhs
+                const double cd = (double)vol * (1.0 + (double)dte * ((double)hist + 1.0)); // diagonal
+                const int lx = base[0] & (TL::BX - 1), ly = base[1] & (TL::BY - 1), lz = base[2] & (TL::BZ - 1);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        const int nrow = ((lx + i) * TL::TY + (ly + j)) * TL::TZ + lz;
+                        const double wij = w[0][i] * w[1][j];
+#pragma unroll
+                        for (int k = 0; k < 3; k++) {
+                            const double wk = wij * w[2][k];
+                            unsafeAtomicAdd(&acc[0][nrow + k], cw * wk);
+                            unsafeAtomicAdd(&acc[1][nrow + k], cr * wk);
+                            unsafeAtomicAdd(&acc[2][nrow + k], cd * wk);
+                        }
+                    }
+            }
+            stAt(D.Q + PDT * cap, s, hist);
+#pragma unroll
+            for (int a = 0; a < 3; a++) stAt(D.cgp + a * cap, s, d0[a]);
+            stAt(D.cgp + 3 * cap, s, kap);
+        }
+        __syncthreads();
+        flushTile(D, b, acc[0], D.cgv, ovf);          // x slot: sum vol w
+        flushTile(D, b, acc[1], D.cgv + cn, ovf);     // r slot: sum vol dt D~ / eta w
+        flushTile(D, b, acc[2], D.cgv + 4 * cn, ovf); // diagonal
+    }
+    if (ovf) latch(D.counters, ST_CAPACITY);
+}
+
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_cg(Dev<T, AccT> D)
+{
+    using B = Blk<T>;
+    using TL = Tile<T>;
+    constexpr int NB = TL::NB;
+    __shared__ double st[TL::TN], qt[TL::TN];
+    __shared__ double red[WG / 64];
+    const int t = threadIdx.x;
+    const int nwg = (int)gridDim.x;
+    const int nblocks = D.counters[CNT_BLOCKS];
+    const int64_t cap = D.cap, cn = D.cap_nodes, nn = (int64_t)nblocks * NB;
+    double* const X = D.cgv;
+    double* const R = D.cgv + cn;
+    double* const S = D.cgv + 2 * cn;
+    double* const Qv = D.cgv + 3 * cn;
+    const double* const DG = D.cgv + 4 * cn;
+    double* const part = D.cg_part;
+    const int64_t first = (int64_t)blockIdx.x * WG + t, stride = (int64_t)nwg * WG;
+    const double odx = (double)D.odx;
+    unsigned epoch = 0;
+    int ovf = 0;
+    // r = b - A 0 = b with b_i = sum vol dt D~ / eta w + pf_i sum vol w; x = 0
+    double prr = 0.0, prho = 0.0;
+    for (int64_t node = first; node < nn; node += stride) {
+        const double dg = DG[node];
+        const double r = dg > 0.0 ? R[node] + (double)D.pfn[node] * X[node] : 0.0;
+        X[node] = 0.0;
+        R[node] = r;
+        const double mr = dg > 0.0 ? r / dg : 0.0;
+        prr += r * r;
+        prho += mr * r;
+    }
+    prr = wgSum(prr, red);
+    prho = wgSum(prho, red);
+    if (t == 0) {
+        part[blockIdx.x] = prr;
+        part[CG_MAX_WGS + blockIdx.x] = prho;
+    }
+    gridBarrier(D.cg_bar, (unsigned)nwg, epoch, D.counters);
+    double rr = foldPartials(part, nwg, red);
+    double rho = foldPartials(part + CG_MAX_WGS, nwg, red);
+    const double rel = sqrt(rr) * D.cg_rel;
+    double rho_old = 0.0;
+    int it = 0;
+    for (;; it++) {
+        // search direction s = M^-1 r (+ rho / rho_old s) and q = diag s
+        const double beta = it == 0 ? 0.0 : rho / rho_old;
+        double psq = 0.0;
+        for (int64_t node = first; node < nn; node += stride) {
+            const double dg = DG[node];
+            const double mr = dg > 0.0 ? R[node] / dg : 0.0;
+            const double sv = it == 0 ? mr : beta * S[node] + mr;
+            S[node] = sv;
+            Qv[node] = dg * sv;
+            psq += dg * sv * sv;
+        }
+        gridBarrier(D.cg_bar, (unsigned)nwg, epoch, D.counters);
+        // q += K s, block by block
+        for (int b = blockIdx.x; b < nblocks; b += nwg) {
+            const int bs = D.block_start[b], cnt = D.block_count[b];
+            if (cnt <= 0) continue;
+            __syncthreads(); // previous block flushed
+            const int* nb = D.nbr + b * 8;
+            for (int nd = t; nd < TL::TN; nd += WG) {
+                const int tx = nd / (TL::TY * TL::TZ), ty = (nd / TL::TZ) % TL::TY, tz = nd % TL::TZ;
+                const int tb = nb[((tx >= TL::BX) << 2) | ((ty >= TL::BY) << 1) | (tz >= TL::BZ)];
+                double v = 0.0;
+                if (tb >= 0) {
+                    const int elem = ((tx & (TL::BX - 1)) << (B::ybits + B::zbits)) | ((ty & (TL::BY - 1)) << B::zbits)
+                        | (tz & (TL::BZ - 1));
+                    v = S[(int64_t)tb * NB + elem];
+                }
+                st[nd] = v;
+                qt[nd] = 0.0;
+            }
+            __syncthreads();
+            for (int s = bs + t; s < bs + cnt; s += WG) {
+                const double kap = (double)ldAt(D.cgp + 3 * cap, s);
+                if (kap == 0.0) continue;
+                double w[3][3], dw[3][3];
+#pragma unroll
+                for (int a = 0; a < 3; a++) bsplineFrac<double>((double)ldAt(D.cgp + a * cap, s), w[a], dw[a]);
+                const int e = ldAt(D.sorted_cell, s) - b * NB;
+                const int lx = e >> (B::ybits + B::zbits), ly = (e >> B::zbits) & (TL::BY - 1), lz = e & (TL::BZ - 1);
+                double G0 = 0.0, G1 = 0.0, G2 = 0.0;
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    double ty = 0.0, tdy = 0.0, tz = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        const int nrow = ((lx + i) * TL::TY + (ly + j)) * TL::TZ + lz;
+                        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 3; k++) {
+                            const double sv = st[nrow + k];
+                            a0 += w[2][k] * sv;
+                            a1 += dw[2][k] * sv;
+                        }
+                        ty += w[1][j] * a0;
+                        tdy += dw[1][j] * a0;
+                        tz += w[1][j] * a1;
+                    }
+                    G0 += dw[0][i] * ty;
+                    G1 += w[0][i] * tdy;
+                    G2 += w[0][i] * tz;
+                }
+                G0 *= odx;
+                G1 *= odx;
+                G2 *= odx;
+                psq += kap * (G0 * G0 + G1 * G1 + G2 * G2);
+                const double g0 = kap * G0 * odx, g1 = kap * G1 * odx, g2 = kap * G2 * odx;
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        const int nrow = ((lx + i) * TL::TY + (ly + j)) * TL::TZ + lz;
+                        const double cxy = g0 * dw[0][i] * w[1][j] + g1 * w[0][i] * dw[1][j];
+                        const double cz = g2 * w[0][i] * w[1][j];
+#pragma unroll
+                        for (int k = 0; k < 3; k++) unsafeAtomicAdd(&qt[nrow + k], cxy * w[2][k] + cz * dw[2][k]);
+                    }
+            }
+            __syncthreads();
+            flushTile(D, b, qt, Qv, ovf);
+        }
+        psq = wgSum(psq, red);
+        if (t == 0) part[2 * CG_MAX_WGS + blockIdx.x] = psq;
+        gridBarrier(D.cg_bar, (unsigned)nwg, epoch, D.counters);
+        const double sq = foldPartials(part + 2 * CG_MAX_WGS, nwg, red);
+        const double alpha = sq != 0.0 ? rho / sq : 1.7976931348623157e308;
+        // x += alpha s, r -= alpha q, and the next M^-1 r . r
+        prr = 0.0;
+        prho = 0.0;
+        for (int64_t node = first; node < nn; node += stride) {
+            const double dg = DG[node];
+            if (!(dg > 0.0)) continue;
+            X[node] = alpha * S[node] + X[node];
+            const double r = -alpha * Qv[node] + R[node];
+            R[node] = r;
+            prr += r * r;
+            prho += (r / dg) * r;
+        }
+        prr = wgSum(prr, red);
+        prho = wgSum(prho, red);
+        if (t == 0) {
+            part[blockIdx.x] = prr;
+            part[CG_MAX_WGS + blockIdx.x] = prho;
+        }
+        gridBarrier(D.cg_bar, (unsigned)nwg, epoch, D.counters);
+        rr = foldPartials(part, nwg, red);
+        const double rho_new = foldPartials(part + CG_MAX_WGS, nwg, red);
+        const double conv = sqrt(rr);
+        if (conv < CG_TOLERANCE || conv < rel || it == CG_ITERATIONS) break;
+        rho_old = rho;
+        rho = rho_new;
+    }
+    // g.phase_field = x - phase_field: the change G2P transfers back
+    for (int64_t node = first; node < nn; node += stride) {
+        const double dg = DG[node];
+        D.pfn[node] = dg > 0.0 ? (T)(X[node] - (double)D.pfn[node]) : T(0);
+    }
+    if (blockIdx.x == 0 && t == 0) D.cg_bar[1] = (unsigned)it;
+    if (ovf) latch(D.counters, ST_CAPACITY);
 }
 
 } // namespace zmpm

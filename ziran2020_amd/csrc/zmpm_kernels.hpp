@@ -44,7 +44,8 @@ enum : int {
     PLAP = 28, // damage Laplacian (output)
     PS = 29,   // V0 * Kirchhoff stress of the current F, symmetric: xx xy xz yy yz zz
     PYS = 35,  // plasticity state: von Mises yield stress / Drucker-Prager log Jp
-    NPF = 36
+    PDT = 36,  // implicit damage: history maximum of the crack driving force D~
+    NPF = 37
 };
 
 // FP32-form position: X = round(x / dx * 2^POS_BITS) as int32.  Domains are at
@@ -85,7 +86,8 @@ enum : int {
     ST_OVERFLOW = 8,
     ST_CAPACITY = 16,
     ST_HALF_CFL = 32,
-    ST_SCAN = 64 // a page-scan tile's look-back hit its poll limit
+    ST_SCAN = 64, // a page-scan tile's look-back hit its poll limit
+    ST_BARRIER = 128 // a damage-solve grid barrier hit its poll limit
 };
 
 // SPGrid block shape of GridState<T,3> (64 B float -> 4x4x4, 128 B double -> 2x4x4)
@@ -186,6 +188,14 @@ struct Dev {
     int flip;      // FLIP_blend_PIC: P2G without C, G2P v <- r v + sum w (v_new - r v_old)
     int mls;       // MLS-MPM: force S (x_i - x_p) w D^-1, grad v = C
     T flip_ratio;
+    // implicit damage (AnisoFractureSimulation::implicit, solveDamageSystem)
+    int implicit;      // phase field by the CG solve instead of the explicit update
+    int cg_wgs;        // workgroups of k_cg (all resident at once)
+    double cg_rel;     // CG stop below this fraction of the first residual norm
+    double* cgv;       // [5][cap_nodes] x, r, s, q, diag (x and r first hold sum vol w and the D~ part of rhs)
+    T* cgp;            // [4][cap] per sorted particle: d0 x, y, z; vol dt / eta l0^2 (0: not in the solve)
+    double* cg_part;   // [3][CG_MAX_WGS] per-workgroup partial sums
+    unsigned* cg_bar;  // [0] barrier arrivals, [1] iterations of the last solve
     // scene
     T dx, odx;
     T pos_scale; // FP32 form: metres -> position units, odx * 2^POS_BITS

@@ -49,6 +49,9 @@ public:
     bool mls_mpm = false;
     int rpic_damping_iteration = 0; // MpmSimulationBase.h:135-136
     int pic_damping_iteration = 0;
+    // AnisoFractureSimulation::implicit (AnisoFractureSimulation.h:61): phase
+    // field by the CG solve of solveDamageSystem instead of the explicit update
+    bool implicit = false;
     TV gravity { 0, 0, 0 };
     bool autorestart = true;
     // ---- device configuration (no counterpart in the reference)
@@ -63,6 +66,7 @@ public:
     std::vector<int32_t> material;
     std::vector<double> damage, degradation, laplacian;
     std::vector<double> plastic_state; // von Mises yield stress / Drucker-Prager log Jp per particle
+    std::vector<double> max_dtilde;    // AnisotropicPhaseField::maxDTilde (implicit damage)
     // ---- scene
     std::vector<zmpm_material> materials;
     std::vector<zmpm_collider> collision_objects;
@@ -112,6 +116,7 @@ public:
         damage.push_back(0);
         degradation.push_back(1);
         laplacian.push_back(0);
+        max_dtilde.push_back(0);
         plastic_state.push_back(mat >= 0 && mat < (int)materials.size()
                                         && materials[(size_t)mat].plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY
                                     ? materials[(size_t)mat].yield_stress
@@ -140,6 +145,7 @@ public:
         d.mls_mpm = mls_mpm ? 1 : 0;
         d.rpic_damping_iteration = rpic_damping_iteration;
         d.pic_damping_iteration = pic_damping_iteration;
+        d.implicit_damage = implicit ? 1 : 0;
         check(zmpm_create(&d, &h_), "zmpm_create");
         check(zmpm_set_materials(h_, (int32_t)materials.size(), materials.data()), "zmpm_set_materials");
         for (const auto& c : collision_objects) check(zmpm_add_collider(h_, &c, nullptr), "zmpm_add_collider");
@@ -286,6 +292,7 @@ private:
         p.g = degradation.data();
         p.laplacian = laplacian.data();
         p.plastic_state = plastic_state.data();
+        p.max_dtilde = max_dtilde.data();
         return p;
     }
 
