@@ -79,3 +79,23 @@ def test_cpp_frame_driver_matches_oracle(oracle):
     assert got["substeps"] == substeps and substeps > 2 * frames
     assert abs(got["time"] - t) < 1e-15
     assert np.abs(np.array(got["com"]) - com).max() < 1e-12
+
+
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_cpp_restart_matches_continuous_run(tmp_path, fp32):
+    # write(outputFileName()) halfway, then a second simulation built from the
+    # same scene setup restarts from the file (SimulationBase.h:148-200, 265-285)
+    # and runs the other half: the same state as the run that never stopped
+    cells, steps = 6, 20
+    runs = []
+    for extra in ([], ["0", str(tmp_path)]):
+        out = subprocess.run([EXE, str(cells), str(steps), str(fp32), *extra], capture_output=True, text=True,
+                             timeout=120)
+        assert out.returncode == 0, out.stderr
+        runs.append(json.loads(out.stdout.strip().splitlines()[-1]))
+    assert (tmp_path / "restart_7.dat").exists()
+    a, b = runs
+    assert a["n"] == b["n"] and abs(a["time"] - b["time"]) < 1e-15
+    tol = 1e-12 if not fp32 else 1e-6
+    assert np.abs(np.array(a["com"]) - np.array(b["com"])).max() < tol
+    assert np.abs(np.array(a["mean_v"]) - np.array(b["mean_v"])).max() < tol * max(1.0, np.abs(a["mean_v"]).max())

@@ -3,23 +3,21 @@
 // the sub-cell centres) dropping onto a sticky floor, as a scene written
 // against the reference's MpmSimulationBase member names.
 //
-// usage: zmpm_elastic_cube <cells per side> <substeps> <fp32: 0|1> [frames]
+// usage: zmpm_elastic_cube <cells per side> <substeps> <fp32: 0|1> [frames] [restart dir]
 // With frames > 0 the cube starts at 3 m/s downwards and the frame driver runs
 // that many frames of 1/60 s with CFL substeps (calculateDt, TimeStepping::nextDt;
 // max_dt = 1/100 s) instead of the fixed substeps.
+// With a restart directory the fixed-substep run stops halfway, writes its
+// state there (write(outputFileName())), and a second simulation built from the
+// same scene setup restarts from that file and runs the other half.
 // prints one JSON line: particle count, centre of mass, mean velocity, substeps.
 #include "zmpm_simulation.hpp"
 
 #include <cstdio>
 #include <cstdlib>
 
-int main(int argc, char** argv)
+static void setup(zmpm::MpmSimulationHip& sim, int cells, bool fp32)
 {
-    const int cells = argc > 1 ? std::atoi(argv[1]) : 16;
-    const int steps = argc > 2 ? std::atoi(argv[2]) : 20;
-    const bool fp32 = argc > 3 && std::atoi(argv[3]) != 0;
-    const int frames = argc > 4 ? std::atoi(argv[4]) : 0;
-    zmpm::MpmSimulationHip sim;
     sim.dx = 1.0 / 64;
     sim.gravity = { 0, -9.8, 0 };
     sim.precision = fp32 ? ZMPM_PREC_FP32 : ZMPM_PREC_FP64;
@@ -44,22 +42,10 @@ int main(int argc, char** argv)
         for (int j = 0; j < 2 * cells; j++)
             for (int k = 0; k < 2 * cells; k++)
                 sim.addParticle({ x0 + (i + 0.5) * h, y0 + (j + 0.5) * h, z0 + (k + 0.5) * h }, rho * V0, V0, mat);
-    int substeps = steps;
-    if (frames > 0) {
-        for (auto& v : sim.V) v = { 0, -3, 0 };
-        sim.frame_dt = 1.0 / 60;
-        sim.max_dt = 1.0 / 100;
-        sim.initialize();
-        substeps = 0;
-        for (int f = 0; f < frames; f++) {
-            sim.advanceOneFrame();
-            substeps += sim.substeps_last_frame;
-        }
-    }
-    else {
-        sim.initialize();
-        sim.advance(steps, 1e-4);
-    }
+}
+
+static int report(zmpm::MpmSimulationHip& sim, int substeps)
+{
     sim.sync();
     double cm[3] = { 0, 0, 0 }, vm[3] = { 0, 0, 0 }, M = 0;
     for (size_t p = 0; p < sim.X.size(); p++) {
@@ -73,4 +59,47 @@ int main(int argc, char** argv)
                 "\"time\": %.17g}\n",
                 sim.X.size(), cm[0] / M, cm[1] / M, cm[2] / M, vm[0] / M, vm[1] / M, vm[2] / M, substeps, sim.time);
     return 0;
+}
+
+int main(int argc, char** argv)
+{
+    const int cells = argc > 1 ? std::atoi(argv[1]) : 16;
+    const int steps = argc > 2 ? std::atoi(argv[2]) : 20;
+    const bool fp32 = argc > 3 && std::atoi(argv[3]) != 0;
+    const int frames = argc > 4 ? std::atoi(argv[4]) : 0;
+    const char* restart_dir = argc > 5 ? argv[5] : nullptr;
+    zmpm::MpmSimulationHip sim;
+    setup(sim, cells, fp32);
+    int substeps = steps;
+    if (frames > 0) {
+        for (auto& v : sim.V) v = { 0, -3, 0 };
+        sim.frame_dt = 1.0 / 60;
+        sim.max_dt = 1.0 / 100;
+        sim.initialize();
+        substeps = 0;
+        for (int f = 0; f < frames; f++) {
+            sim.advanceOneFrame();
+            substeps += sim.substeps_last_frame;
+        }
+    }
+    else if (restart_dir) {
+        sim.initialize();
+        sim.advance(steps / 2, 1e-4);
+        sim.frame = 7;
+        sim.output_dir = restart_dir;
+        sim.write(sim.outputFileName());
+        zmpm::MpmSimulationHip again;
+        setup(again, cells, fp32);
+        again.output_dir = restart_dir;
+        again.restart(7);
+        again.initialize();
+        again.advance(steps - steps / 2, 1e-4);
+        again.sync();
+        return report(again, substeps);
+    }
+    else {
+        sim.initialize();
+        sim.advance(steps, 1e-4);
+    }
+    return report(sim, substeps);
 }

@@ -192,18 +192,19 @@ zmpm_status allocate(zmpm_context* h, int n)
         ALLOC(D.cgv, 5 * D.cap_nodes);
         ALLOC(D.cgp, 4 * cap);
         ALLOC(D.cg_part, 3 * CG_MAX_WGS);
-        ALLOC(D.cg_bar, 2);
+        ALLOC(D.cg_bar, 4);
     }
 #undef ALLOC
     if (h->desc.implicit_damage) {
-        // k_cg's grid barriers need every workgroup resident: at most two per CU,
-        // within what the occupancy calculator allows
+        // k_cg's grid barriers need every workgroup resident: one per CU (the
+        // occupancy calculator must allow at least that; fewer workgroups make
+        // each barrier cheaper, ~9 us at 256)
         int occ = 0, cus = 0;
-        HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cg<T, AccT>, WG, 0));
+        HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cg<T, AccT>, CGW, 0));
         HIPCHK(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->desc.device));
-        D.cg_wgs = std::max(1, std::min(CG_MAX_WGS, cus * std::min(occ, 2)));
         if (occ < 1) return fail(h, ZMPM_E_HIP, "damage solve kernel cannot be resident");
-        HIPCHK(h, hipMemsetAsync(D.cg_bar, 0, 2 * sizeof(unsigned), h->stream));
+        D.cg_wgs = std::max(1, std::min(CG_MAX_WGS, cus));
+        HIPCHK(h, hipMemsetAsync(D.cg_bar, 0, 4 * sizeof(unsigned), h->stream));
     }
     HIPCHK(h, hipMemsetAsync(D.touched, 0, D.num_pages * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.page_count, 0, D.num_pages * sizeof(int), h->stream));
@@ -347,7 +348,7 @@ void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
     if (D.implicit && D.any_damage) {
         // AnisoFractureSimulation::solveDamageSystem (AnisoFractureSimulation.h:435-612)
         k_dtilde<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D, (T)dt);
-        k_cg<T, AccT><<<D.cg_wgs, WG, 0, st>>>(D);
+        k_cg<T, AccT><<<D.cg_wgs, CGW, 0, st>>>(D);
     }
     if (ev) (void)hipEventRecord(ev[4], st);
     if (D.flip)

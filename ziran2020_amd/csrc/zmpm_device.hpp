@@ -2100,6 +2100,7 @@ __global__ __launch_bounds__(WG) void k_extent(Dev<T, AccT> D, int fold)
 // the assembled q, so alpha needs no pass over q.  The solve runs in double in
 // both forms; its HBM atomics make the last bits of x depend on scheduling.
 constexpr int CG_MAX_WGS = 2048;
+constexpr int CGW = 256;                 // k_cg workgroup size (one workgroup per CU)
 constexpr int CG_ITERATIONS = 100;       // ConjugateGradient<..>(100) (AnisoFractureSimulation.h:70)
 constexpr double CG_TOLERANCE = 1e-6;    // cg.setTolerance(1e-6) (AnisoFractureSimulation.h:600-601)
 constexpr int CG_SPIN_LIMIT = 1 << 18;   // barrier polls before the wait is given up (ST_BARRIER)
@@ -2108,15 +2109,23 @@ constexpr int CG_SPIN_LIMIT = 1 << 18;   // barrier polls before the wait is giv
 // arrivals counts up through the launch (cleared before it); barrier number k
 // is passed when it reaches k * nwg.  Release before arriving and acquire after
 // leaving, at agent scope, make each workgroup's writes visible to the others
-// across the chip's L2 slices.  A wait that exceeds the poll limit latches
+// across the chip's L2 slices (one per XCD).  The fences are cache-wide
+// operations, so one wave per workgroup issues them: with every wave fencing,
+// a barrier of 512 workgroups took ~52 us on MI355X, ~22 us with one wave per
+// workgroup, ~9 us with 256 workgroups.  A wait that exceeds the poll limit latches
 // ST_BARRIER and goes on, and once that is latched no barrier waits any more,
 // so the launch always ends soon (with its results flagged as invalid).
 __device__ __forceinline__ void gridBarrier(unsigned* arrivals, unsigned nwg, unsigned& epoch, int* counters)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // every wave's stores complete (in L2) before the workgroup barrier; then
+    // one wave writes the L2 back (agent-scope release) for the whole
+    // workgroup and, after the wait, invalidates it and the CU's vector cache
+    // (agent-scope acquire), which the other waves share
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     epoch++;
     if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const unsigned target = epoch * nwg;
         unsigned seen = __hip_atomic_fetch_add(arrivals, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
         int polls = 0;
@@ -2129,13 +2138,14 @@ __device__ __forceinline__ void gridBarrier(unsigned* arrivals, unsigned nwg, un
             __builtin_amdgcn_s_sleep(2);
             seen = __hip_atomic_load(arrivals, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// Sum over the workgroup, the same value in every thread (fixed order).
-__device__ __forceinline__ double wgSum(double v, double* red)
+// Sum over a workgroup of NT threads, the same value in every thread (fixed order).
+template <int NT>
+__device__ __forceinline__ double blockSum(double v, double* red)
 {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
@@ -2144,28 +2154,29 @@ __device__ __forceinline__ double wgSum(double v, double* red)
     __syncthreads();
     double r = red[0];
 #pragma unroll
-    for (int w = 1; w < WG / 64; w++) r += red[w];
+    for (int w = 1; w < NT / 64; w++) r += red[w];
     return r;
 }
 
 // Sum of the nwg per-workgroup partials part[0 .. nwg), identical in every
 // workgroup (each folds them in the same order).
+template <int NT>
 __device__ __forceinline__ double foldPartials(const double* part, int nwg, double* red)
 {
     double v = 0.0;
-    for (int w = threadIdx.x; w < nwg; w += WG) v += part[w];
-    return wgSum(v, red);
+    for (int w = threadIdx.x; w < nwg; w += NT) v += part[w];
+    return blockSum<NT>(v, red);
 }
 
 // Flush an LDS tile of block b (block + forward halo) into a node array with
 // one HBM atomic per non-zero value.
-template <class T, class AccT>
+template <class T, class AccT, int NT = WG>
 __device__ __forceinline__ void flushTile(const Dev<T, AccT>& D, int b, const double* tile, double* dst, int& ovf)
 {
     using B = Blk<T>;
     using TL = Tile<T>;
     const int* nb = D.nbr + b * 8;
-    for (int nd = threadIdx.x; nd < TL::TN; nd += WG) {
+    for (int nd = threadIdx.x; nd < TL::TN; nd += NT) {
         const double v = tile[nd];
         if (v == 0.0) continue;
         const int tx = nd / (TL::TY * TL::TZ), ty = (nd / TL::TZ) % TL::TY, tz = nd % TL::TZ;
@@ -2179,6 +2190,21 @@ __device__ __forceinline__ void flushTile(const Dev<T, AccT>& D, int b, const do
     }
 }
 
+// Cells of block b holding sorted positions in [lo, hi): their indices into cl[],
+// their count into *ncl (one wavefront; NB <= 64).
+template <int NB>
+__device__ __forceinline__ void activeCells(const int* cs, const int* ce, int lo, int hi, int* cl, int* ncl)
+{
+    const int t = threadIdx.x;
+    if (t < 64) {
+        bool act = false;
+        if (t < NB) act = min(ce[t], hi) > max(cs[t], lo);
+        const uint64_t m = __ballot(act);
+        if (act) cl[__popcll(m & ((1ull << t) - 1))] = t;
+        if (t == 0) *ncl = __popcll(m);
+    }
+}
+
 template <class T, class AccT>
 __global__ __launch_bounds__(WG) void k_dtilde(Dev<T, AccT> D, T dt)
 {
@@ -2186,8 +2212,15 @@ __global__ __launch_bounds__(WG) void k_dtilde(Dev<T, AccT> D, T dt)
     using TL = Tile<T>;
     constexpr int NB = TL::NB;
     __shared__ double acc[3][TL::TN];
+    __shared__ double pw[9][WG]; // per particle of the chunk: w_a(i) at [3 a + i]
+    __shared__ double pc[3][WG]; // vol, vol dt D~ / eta, vol (1 + dt / eta (D~ + 1)); 0 off the solve
+    __shared__ int cs[NB], ce[NB], cl[NB];
+    __shared__ int ncl;
     const int t = threadIdx.x;
-    if (blockIdx.x == 0 && t == 0) D.cg_bar[0] = 0u; // k_cg's barrier arrivals
+    if (blockIdx.x == 0 && t == 0) { // k_cg's barrier arrivals and block tickets
+        D.cg_bar[0] = 0u;
+        D.cg_bar[2] = 0u;
+    }
     const int64_t cap = D.cap, cn = D.cap_nodes;
     const int nblocks = D.counters[CNT_BLOCKS];
     int ovf = 0;
@@ -2196,57 +2229,84 @@ __global__ __launch_bounds__(WG) void k_dtilde(Dev<T, AccT> D, T dt)
         if (cnt <= 0) continue;
         __syncthreads(); // previous block flushed
         for (int e = t; e < 3 * TL::TN; e += WG) (&acc[0][0])[e] = 0.0;
-        __syncthreads();
-        for (int s = bs + t; s < bs + cnt; s += WG) {
-            int p = ldAt(D.perm, s);
-            if ((unsigned)p >= (unsigned)D.n) p = 0;
-            const MatDev<T>& mt = D.mats[ldAt(D.pmat, p)];
-            T hist = ldAt(D.P + PDT * cap, p);
-            T kap = T(0);
-            T d0[3] = { T(0), T(0), T(0) };
-            if (mt.has_damage && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
-                const Mat3<T> Fs = loadM(D.P, cap, PF, p);
-                T J;
-                if constexpr (sizeof(T) == 4)
-                    J = 1.f + jm1OfH(Fs);
-                else
-                    J = det3(Fs);
-                const T vol = ldAt(D.P + PVOL * cap, p) * J; // phase_field.vol (:425-432)
-                T dTilde;
-                dTildeOf<T, false>(mt, Fs, dTilde);
-                hist = dTilde < hist ? hist : dTilde; // std::max(dTilde, dTilde_history)
-                const T dte = dt / mt.eta;
-                kap = vol * dte * mt.l0 * mt.l0;
-                int base[3];
-                double w[3][3], dw[3][3];
+        if (t < NB) {
+            cs[t] = D.cell_start[b * NB + t];
+            ce[t] = cs[t] + D.cell_count[b * NB + t];
+        }
+        for (int lo = bs; lo < bs + cnt; lo += WG) {
+            const int hi = min(lo + WG, bs + cnt);
+            __syncthreads(); // previous chunk's columns done with pw, pc
+            // ---- one thread per particle: D~ history, the solve's per-particle data
+            const int s = lo + t;
+            if (s < hi) {
+                int p = ldAt(D.perm, s);
+                if ((unsigned)p >= (unsigned)D.n) p = 0;
+                const MatDev<T>& mt = D.mats[ldAt(D.pmat, p)];
+                T hist = ldAt(D.P + PDT * cap, p);
+                T kap = T(0);
+                T d0[3] = { T(0), T(0), T(0) };
+                double c3[3] = { 0.0, 0.0, 0.0 };
+                if (mt.has_damage && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
+                    const Mat3<T> Fs = loadM(D.P, cap, PF, p);
+                    T J;
+                    if constexpr (sizeof(T) == 4)
+                        J = 1.f + jm1OfH(Fs);
+                    else
+                        J = det3(Fs);
+                    const T vol = ldAt(D.P + PVOL * cap, p) * J; // phase_field.vol (:425-432)
+                    T dTilde;
+                    dTildeOf<T, false>(mt, Fs, dTilde);
+                    hist = dTilde < hist ? hist : dTilde; // std::max(dTilde, dTilde_history)
+                    const T dte = dt / mt.eta;
+                    kap = vol * dte * mt.l0 * mt.l0;
+                    int base;
+#pragma unroll
+                    for (int a = 0; a < 3; a++) cellOf(loadPos(D.P + (PX + a) * cap, p), D.odx, D.domain, base, d0[a]);
+                    c3[0] = (double)vol;                                             // sum vol w
+                    c3[1] = (double)((dt * hist) / mt.eta) * (double)vol;            // D~ part of rhs
+                    c3[2] = (double)vol * (1.0 + (double)dte * ((double)hist + 1.0)); // diagonal
+                }
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    cellOf(loadPos(D.P + (PX + a) * cap, p), D.odx, D.domain, base[a], d0[a]);
-                    bsplineFrac<double>((double)d0[a], w[a], dw[a]);
+                    double w[3], dw[3];
+                    bsplineFrac<double>((double)d0[a], w, dw);
+#pragma unroll
+                    for (int i = 0; i < 3; i++) pw[3 * a + i][t] = w[i];
                 }
-                const double cw = (double)vol;                                   // sum vol w
-                const double cr = (double)((dt * hist) / mt.eta) * (double)vol; // D~ part of rhs
-                const double cd = (double)vol * (1.0 + (double)dte * ((double)hist + 1.0)); // diagonal
-                const int lx = base[0] & (TL::BX - 1), ly = base[1] & (TL::BY - 1), lz = base[2] & (TL::BZ - 1);
 #pragma unroll
-                for (int i = 0; i < 3; i++)
+                for (int f = 0; f < 3; f++) pc[f][t] = c3[f];
+                stAt(D.Q + PDT * cap, s, hist);
 #pragma unroll
-                    for (int j = 0; j < 3; j++) {
-                        const int nrow = ((lx + i) * TL::TY + (ly + j)) * TL::TZ + lz;
-                        const double wij = w[0][i] * w[1][j];
-#pragma unroll
-                        for (int k = 0; k < 3; k++) {
-                            const double wk = wij * w[2][k];
-                            unsafeAtomicAdd(&acc[0][nrow + k], cw * wk);
-                            unsafeAtomicAdd(&acc[1][nrow + k], cr * wk);
-                            unsafeAtomicAdd(&acc[2][nrow + k], cd * wk);
-                        }
-                    }
+                for (int a = 0; a < 3; a++) stAt(D.cgp + a * cap, s, d0[a]);
+                stAt(D.cgp + 3 * cap, s, kap);
             }
-            stAt(D.Q + PDT * cap, s, hist);
+            activeCells<NB>(cs, ce, lo, hi, cl, &ncl);
+            __syncthreads();
+            // ---- lanes = (cell, i, j) stencil columns: each walks its cell's
+            // particles (one base node) and adds 3 nodes x 3 sums into the tile
+            const int per = ncl * 9;
+            for (int slot = t; slot < per; slot += WG) {
+                const int cell = cl[slot / 9], col = slot - 9 * (slot / 9);
+                const int ci = col / 3, cj = col - 3 * (col / 3);
+                const int p0 = max(cs[cell], lo) - lo, p1 = min(ce[cell], hi) - lo;
+                double a[3][3] = { { 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0 } };
+                for (int pp = p0; pp < p1; pp++) {
+                    const double wxy = pw[ci][pp] * pw[3 + cj][pp];
 #pragma unroll
-            for (int a = 0; a < 3; a++) stAt(D.cgp + a * cap, s, d0[a]);
-            stAt(D.cgp + 3 * cap, s, kap);
+                    for (int f = 0; f < 3; f++) {
+                        const double cf = pc[f][pp] * wxy;
+#pragma unroll
+                        for (int k = 0; k < 3; k++) a[f][k] += cf * pw[6 + k][pp];
+                    }
+                }
+                const int cx = cell >> (B::ybits + B::zbits), cy = (cell >> B::zbits) & (TL::BY - 1), cz = cell & (TL::BZ - 1);
+                const int nbase = ((cx + ci) * TL::TY + (cy + cj)) * TL::TZ + cz;
+#pragma unroll
+                for (int f = 0; f < 3; f++)
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        if (a[f][k] != 0.0) unsafeAtomicAdd(&acc[f][nbase + k], a[f][k]);
+            }
         }
         __syncthreads();
         flushTile(D, b, acc[0], D.cgv, ovf);          // x slot: sum vol w
@@ -2257,16 +2317,21 @@ __global__ __launch_bounds__(WG) void k_dtilde(Dev<T, AccT> D, T dt)
 }
 
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_cg(Dev<T, AccT> D)
+__global__ __launch_bounds__(CGW) void k_cg(Dev<T, AccT> D)
 {
     using B = Blk<T>;
     using TL = Tile<T>;
     constexpr int NB = TL::NB;
     __shared__ double st[TL::TN], qt[TL::TN];
-    __shared__ double red[WG / 64];
+    __shared__ double pw[18][CGW]; // per particle of the segment: w_a(i) at [3 a + i], dw_a(i) at [9 + 3 a + i]
+    __shared__ double pg[3][CGW];  // kappa G odx
+    __shared__ int cs[NB], ce[NB], cl[NB];
+    __shared__ int ncl, ticket;
+    __shared__ double red[CGW / 64];
     const int t = threadIdx.x;
     const int nwg = (int)gridDim.x;
     const int nblocks = D.counters[CNT_BLOCKS];
+    const int n = D.n;
     const int64_t cap = D.cap, cn = D.cap_nodes, nn = (int64_t)nblocks * NB;
     double* const X = D.cgv;
     double* const R = D.cgv + cn;
@@ -2274,9 +2339,9 @@ __global__ __launch_bounds__(WG) void k_cg(Dev<T, AccT> D)
     double* const Qv = D.cgv + 3 * cn;
     const double* const DG = D.cgv + 4 * cn;
     double* const part = D.cg_part;
-    const int64_t first = (int64_t)blockIdx.x * WG + t, stride = (int64_t)nwg * WG;
+    const int64_t first = (int64_t)blockIdx.x * CGW + t, stride = (int64_t)nwg * CGW;
     const double odx = (double)D.odx;
-    unsigned epoch = 0;
+    unsigned epoch = 0, tickets = 0;
     int ovf = 0;
     // r = b - A 0 = b with b_i = sum vol dt D~ / eta w + pf_i sum vol w; x = 0
     double prr = 0.0, prho = 0.0;
@@ -2289,15 +2354,15 @@ __global__ __launch_bounds__(WG) void k_cg(Dev<T, AccT> D)
         prr += r * r;
         prho += mr * r;
     }
-    prr = wgSum(prr, red);
-    prho = wgSum(prho, red);
+    prr = blockSum<CGW>(prr, red);
+    prho = blockSum<CGW>(prho, red);
     if (t == 0) {
         part[blockIdx.x] = prr;
         part[CG_MAX_WGS + blockIdx.x] = prho;
     }
     gridBarrier(D.cg_bar, (unsigned)nwg, epoch, D.counters);
-    double rr = foldPartials(part, nwg, red);
-    double rho = foldPartials(part + CG_MAX_WGS, nwg, red);
+    double rr = foldPartials<CGW>(part, nwg, red);
+    double rho = foldPartials<CGW>(part + CG_MAX_WGS, nwg, red);
     const double rel = sqrt(rr) * D.cg_rel;
     double rho_old = 0.0;
     int it = 0;
@@ -2314,13 +2379,24 @@ __global__ __launch_bounds__(WG) void k_cg(Dev<T, AccT> D)
             psq += dg * sv * sv;
         }
         gridBarrier(D.cg_bar, (unsigned)nwg, epoch, D.counters);
-        // q += K s, block by block
-        for (int b = blockIdx.x; b < nblocks; b += nwg) {
+        // q += K s, one grid block at a time, blocks handed out by a ticket
+        // counter (the block sizes vary): s of the block and its halo is staged
+        // in LDS; one thread per particle gathers G = sum_j s_j grad w_j and
+        // keeps kappa G with its weights in LDS; then lanes = (cell, i, j)
+        // stencil columns walk their cell's particles and add kappa G . grad w_i
+        // for 3 nodes into a tile flushed with one HBM atomic per node value.
+        // Every workgroup draws until it gets a ticket past the last block, so
+        // each pass uses exactly nblocks + nwg tickets.
+        for (;;) {
+            __syncthreads(); // previous block flushed, ticket slot free
+            if (t == 0) ticket = (int)(__hip_atomic_fetch_add(&D.cg_bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - tickets);
+            __syncthreads();
+            const int b = ticket;
+            if (b >= nblocks) break;
             const int bs = D.block_start[b], cnt = D.block_count[b];
             if (cnt <= 0) continue;
-            __syncthreads(); // previous block flushed
             const int* nb = D.nbr + b * 8;
-            for (int nd = t; nd < TL::TN; nd += WG) {
+            for (int nd = t; nd < TL::TN; nd += CGW) {
                 const int tx = nd / (TL::TY * TL::TZ), ty = (nd / TL::TZ) % TL::TY, tz = nd % TL::TZ;
                 const int tb = nb[((tx >= TL::BX) << 2) | ((ty >= TL::BY) << 1) | (tz >= TL::BZ)];
                 double v = 0.0;
@@ -2332,60 +2408,93 @@ __global__ __launch_bounds__(WG) void k_cg(Dev<T, AccT> D)
                 st[nd] = v;
                 qt[nd] = 0.0;
             }
-            __syncthreads();
-            for (int s = bs + t; s < bs + cnt; s += WG) {
-                const double kap = (double)ldAt(D.cgp + 3 * cap, s);
-                if (kap == 0.0) continue;
-                double w[3][3], dw[3][3];
-#pragma unroll
-                for (int a = 0; a < 3; a++) bsplineFrac<double>((double)ldAt(D.cgp + a * cap, s), w[a], dw[a]);
-                const int e = ldAt(D.sorted_cell, s) - b * NB;
-                const int lx = e >> (B::ybits + B::zbits), ly = (e >> B::zbits) & (TL::BY - 1), lz = e & (TL::BZ - 1);
-                double G0 = 0.0, G1 = 0.0, G2 = 0.0;
-#pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    double ty = 0.0, tdy = 0.0, tz = 0.0;
-#pragma unroll
-                    for (int j = 0; j < 3; j++) {
-                        const int nrow = ((lx + i) * TL::TY + (ly + j)) * TL::TZ + lz;
-                        double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-                        for (int k = 0; k < 3; k++) {
-                            const double sv = st[nrow + k];
-                            a0 += w[2][k] * sv;
-                            a1 += dw[2][k] * sv;
-                        }
-                        ty += w[1][j] * a0;
-                        tdy += dw[1][j] * a0;
-                        tz += w[1][j] * a1;
-                    }
-                    G0 += dw[0][i] * ty;
-                    G1 += w[0][i] * tdy;
-                    G2 += w[0][i] * tz;
-                }
-                G0 *= odx;
-                G1 *= odx;
-                G2 *= odx;
-                psq += kap * (G0 * G0 + G1 * G1 + G2 * G2);
-                const double g0 = kap * G0 * odx, g1 = kap * G1 * odx, g2 = kap * G2 * odx;
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-#pragma unroll
-                    for (int j = 0; j < 3; j++) {
-                        const int nrow = ((lx + i) * TL::TY + (ly + j)) * TL::TZ + lz;
-                        const double cxy = g0 * dw[0][i] * w[1][j] + g1 * w[0][i] * dw[1][j];
-                        const double cz = g2 * w[0][i] * w[1][j];
-#pragma unroll
-                        for (int k = 0; k < 3; k++) unsafeAtomicAdd(&qt[nrow + k], cxy * w[2][k] + cz * dw[2][k]);
-                    }
+            if (t < NB) {
+                cs[t] = D.cell_start[b * NB + t];
+                ce[t] = cs[t] + D.cell_count[b * NB + t];
             }
             __syncthreads();
-            flushTile(D, b, qt, Qv, ovf);
+            for (int lo = bs; lo < bs + cnt; lo += CGW) {
+                const int hi = min(lo + CGW, bs + cnt);
+                if (lo != bs) __syncthreads(); // previous chunk's columns done with pw, pg
+                const int s = lo + t;
+                if (s < hi) {
+                    const double kap = (double)ldAt(D.cgp + 3 * cap, s);
+                    double w[3][3], dw[3][3];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) bsplineFrac<double>((double)ldAt(D.cgp + a * cap, s), w[a], dw[a]);
+                    double G0 = 0.0, G1 = 0.0, G2 = 0.0;
+                    if (kap != 0.0) {
+                        const int e = ldAt(D.sorted_cell, s) - b * NB;
+                        const int lx = e >> (B::ybits + B::zbits), ly = (e >> B::zbits) & (TL::BY - 1), lz = e & (TL::BZ - 1);
+#pragma unroll
+                        for (int i = 0; i < 3; i++) {
+                            double ty = 0.0, tdy = 0.0, tz = 0.0;
+#pragma unroll
+                            for (int j = 0; j < 3; j++) {
+                                const int nrow = ((lx + i) * TL::TY + (ly + j)) * TL::TZ + lz;
+                                double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+                                for (int k = 0; k < 3; k++) {
+                                    const double sv = st[nrow + k];
+                                    a0 += w[2][k] * sv;
+                                    a1 += dw[2][k] * sv;
+                                }
+                                ty += w[1][j] * a0;
+                                tdy += dw[1][j] * a0;
+                                tz += w[1][j] * a1;
+                            }
+                            G0 += dw[0][i] * ty;
+                            G1 += w[0][i] * tdy;
+                            G2 += w[0][i] * tz;
+                        }
+                        G0 *= odx;
+                        G1 *= odx;
+                        G2 *= odx;
+                        psq += kap * (G0 * G0 + G1 * G1 + G2 * G2);
+                    }
+                    // kappa G . grad w_ijk = kappa odx (G0 dw_i w_j w_k + G1 w_i dw_j w_k + G2 w_i w_j dw_k)
+                    pg[0][t] = kap * G0 * odx;
+                    pg[1][t] = kap * G1 * odx;
+                    pg[2][t] = kap * G2 * odx;
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+#pragma unroll
+                        for (int i = 0; i < 3; i++) {
+                            pw[3 * a + i][t] = w[a][i];
+                            pw[9 + 3 * a + i][t] = dw[a][i];
+                        }
+                }
+                activeCells<NB>(cs, ce, lo, hi, cl, &ncl);
+                __syncthreads();
+                const int per = ncl * 9;
+                for (int slot = t; slot < per; slot += CGW) {
+                    const int cell = cl[slot / 9], col = slot - 9 * (slot / 9);
+                    const int ci = col / 3, cj = col - 3 * (col / 3);
+                    const int p0 = max(cs[cell], lo) - lo, p1 = min(ce[cell], hi) - lo;
+                    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+                    for (int pp = p0; pp < p1; pp++) {
+                        const double wx = pw[ci][pp], dwx = pw[9 + ci][pp], wy = pw[3 + cj][pp], dwy = pw[12 + cj][pp];
+                        const double cxy = pg[0][pp] * dwx * wy + pg[1][pp] * wx * dwy;
+                        const double cz = pg[2][pp] * wx * wy;
+                        a0 += cxy * pw[6][pp] + cz * pw[15][pp];
+                        a1 += cxy * pw[7][pp] + cz * pw[16][pp];
+                        a2 += cxy * pw[8][pp] + cz * pw[17][pp];
+                    }
+                    const int cx = cell >> (B::ybits + B::zbits), cy = (cell >> B::zbits) & (TL::BY - 1), cz = cell & (TL::BZ - 1);
+                    const int nbase = ((cx + ci) * TL::TY + (cy + cj)) * TL::TZ + cz;
+                    if (a0 != 0.0) unsafeAtomicAdd(&qt[nbase], a0);
+                    if (a1 != 0.0) unsafeAtomicAdd(&qt[nbase + 1], a1);
+                    if (a2 != 0.0) unsafeAtomicAdd(&qt[nbase + 2], a2);
+                }
+            }
+            __syncthreads();
+            flushTile<T, AccT, CGW>(D, b, qt, Qv, ovf);
         }
-        psq = wgSum(psq, red);
+        tickets += (unsigned)(nblocks + nwg);
+        psq = blockSum<CGW>(psq, red);
         if (t == 0) part[2 * CG_MAX_WGS + blockIdx.x] = psq;
         gridBarrier(D.cg_bar, (unsigned)nwg, epoch, D.counters);
-        const double sq = foldPartials(part + 2 * CG_MAX_WGS, nwg, red);
+        const double sq = foldPartials<CGW>(part + 2 * CG_MAX_WGS, nwg, red);
         const double alpha = sq != 0.0 ? rho / sq : 1.7976931348623157e308;
         // x += alpha s, r -= alpha q, and the next M^-1 r . r
         prr = 0.0;
@@ -2399,15 +2508,15 @@ __global__ __launch_bounds__(WG) void k_cg(Dev<T, AccT> D)
             prr += r * r;
             prho += (r / dg) * r;
         }
-        prr = wgSum(prr, red);
-        prho = wgSum(prho, red);
+        prr = blockSum<CGW>(prr, red);
+        prho = blockSum<CGW>(prho, red);
         if (t == 0) {
             part[blockIdx.x] = prr;
             part[CG_MAX_WGS + blockIdx.x] = prho;
         }
         gridBarrier(D.cg_bar, (unsigned)nwg, epoch, D.counters);
-        rr = foldPartials(part, nwg, red);
-        const double rho_new = foldPartials(part + CG_MAX_WGS, nwg, red);
+        rr = foldPartials<CGW>(part, nwg, red);
+        const double rho_new = foldPartials<CGW>(part + CG_MAX_WGS, nwg, red);
         const double conv = sqrt(rr);
         if (conv < CG_TOLERANCE || conv < rel || it == CG_ITERATIONS) break;
         rho_old = rho;

@@ -195,7 +195,7 @@ struct Dev {
     double* cgv;       // [5][cap_nodes] x, r, s, q, diag (x and r first hold sum vol w and the D~ part of rhs)
     T* cgp;            // [4][cap] per sorted particle: d0 x, y, z; vol dt / eta l0^2 (0: not in the solve)
     double* cg_part;   // [3][CG_MAX_WGS] per-workgroup partial sums
-    unsigned* cg_bar;  // [0] barrier arrivals, [1] iterations of the last solve
+    unsigned* cg_bar;  // [0] barrier arrivals, [1] iterations of the last solve, [2] block tickets
     // scene
     T dx, odx;
     T pos_scale; // FP32 form: metres -> position units, odx * 2^POS_BITS

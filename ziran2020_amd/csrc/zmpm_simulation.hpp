@@ -11,7 +11,10 @@
 
 #include <array>
 #include <cstdint>
+#include <cstring>
+#include <fstream>
 #include <functional>
+#include <istream>
 #include <ostream>
 #include <stdexcept>
 #include <string>
@@ -252,22 +255,79 @@ public:
         host_stale_ = false;
     }
 
-    // Flat binary state in upload order (the reference streams DataManager
-    // attributes, MpmSimulationBase.cpp:1060-1098).
+    // Restart state: the particle attributes in upload order plus the clock,
+    // as one binary record (the reference streams its DataManager attributes and
+    // scene, MpmSimulationBase.cpp:1060-1114; this is the same role in a flat
+    // layout of its own: magic, version, n, frame, time, time_since_last_frame,
+    // then each attribute array).  Materials and colliders are scene setup and
+    // are not part of it, as in the reference's restart files.
+    static constexpr char kStateMagic[8] = { 'Z', 'M', 'P', 'M', 'S', 'T', 'A', 'T' };
+    static constexpr int32_t kStateVersion = 1;
+
     virtual void writeState(std::ostream& out)
     {
         sync();
         const int64_t n = (int64_t)X.size();
+        out.write(kStateMagic, sizeof(kStateMagic));
+        out.write((const char*)&kStateVersion, sizeof(kStateVersion));
         out.write((const char*)&n, sizeof(n));
-        out.write((const char*)X.data(), n * sizeof(TV));
-        out.write((const char*)V.data(), n * sizeof(TV));
-        out.write((const char*)C.data(), n * sizeof(TM));
-        out.write((const char*)F.data(), n * sizeof(TM));
-        out.write((const char*)mass.data(), n * sizeof(double));
-        out.write((const char*)element_measure.data(), n * sizeof(double));
-        out.write((const char*)material.data(), n * sizeof(int32_t));
-        out.write((const char*)damage.data(), n * sizeof(double));
-        out.write((const char*)degradation.data(), n * sizeof(double));
+        out.write((const char*)&frame, sizeof(frame));
+        out.write((const char*)&time, sizeof(time));
+        out.write((const char*)&time_since_last_frame, sizeof(time_since_last_frame));
+        forEachAttribute([&](char* p, size_t bytes) { out.write(p, (std::streamsize)bytes); });
+        if (!out) throw ZmpmError(ZMPM_E_ARG, "writeState: write failed");
+    }
+
+    // SimulationBase::readState counterpart: replaces the particle state and the
+    // clock; the device copy is refreshed before the next substep.
+    virtual void readState(std::istream& in)
+    {
+        char magic[sizeof(kStateMagic)];
+        int32_t version = 0;
+        int64_t n = -1;
+        in.read(magic, sizeof(magic));
+        in.read((char*)&version, sizeof(version));
+        in.read((char*)&n, sizeof(n));
+        if (!in || std::memcmp(magic, kStateMagic, sizeof(magic)) != 0 || version != kStateVersion || n < 0)
+            throw ZmpmError(ZMPM_E_ARG, "readState: not a state record of this version");
+        in.read((char*)&frame, sizeof(frame));
+        in.read((char*)&time, sizeof(time));
+        in.read((char*)&time_since_last_frame, sizeof(time_since_last_frame));
+        resizeAttributes((size_t)n);
+        forEachAttribute([&](char* p, size_t bytes) { in.read(p, (std::streamsize)bytes); });
+        if (!in) throw ZmpmError(ZMPM_E_ARG, "readState: truncated state record");
+        for (int32_t m : material)
+            if (m < 0 || m >= (int32_t)materials.size()) throw ZmpmError(ZMPM_E_ARG, "readState: material index out of range");
+        host_stale_ = false;
+        dirty_ = true;
+    }
+
+    // SimulationBase::write / read / outputFileName (SimulationBase.h:148-200)
+    // and restart (:265-285): state files output_dir/restart_<frame>.dat.
+    std::string output_dir = ".";
+    virtual std::string outputFileName(const std::string& name = "restart", const std::string& suffix = ".dat") const
+    {
+        return name + "_" + std::to_string(frame) + suffix;
+    }
+    virtual void write(const std::string& filename)
+    {
+        std::ofstream f(output_dir + "/" + filename, std::ios::binary);
+        if (!f) throw ZmpmError(ZMPM_E_ARG, "write: cannot open " + output_dir + "/" + filename);
+        writeState(f);
+    }
+    virtual void read(const std::string& filename)
+    {
+        std::ifstream f(output_dir + "/" + filename, std::ios::binary);
+        if (!f) throw ZmpmError(ZMPM_E_ARG, "read: cannot open " + output_dir + "/" + filename);
+        readState(f);
+    }
+    // Reads the state written at the start of frame new_restart_frame and goes
+    // on from there.
+    virtual void restart(int new_restart_frame)
+    {
+        frame = new_restart_frame;
+        read(outputFileName());
+        if (frame != new_restart_frame) throw ZmpmError(ZMPM_E_ARG, "restart: file holds another frame");
     }
 
     zmpm_handle handle() const { return h_; }
@@ -276,6 +336,40 @@ private:
     zmpm_handle h_ = nullptr;
     bool dirty_ = true;
     bool host_stale_ = false;
+
+    template <class Fn>
+    void forEachAttribute(Fn&& fn)
+    {
+        const size_t n = X.size();
+        fn((char*)X.data(), n * sizeof(TV));
+        fn((char*)V.data(), n * sizeof(TV));
+        fn((char*)C.data(), n * sizeof(TM));
+        fn((char*)F.data(), n * sizeof(TM));
+        fn((char*)mass.data(), n * sizeof(double));
+        fn((char*)element_measure.data(), n * sizeof(double));
+        fn((char*)material.data(), n * sizeof(int32_t));
+        fn((char*)damage.data(), n * sizeof(double));
+        fn((char*)degradation.data(), n * sizeof(double));
+        fn((char*)laplacian.data(), n * sizeof(double));
+        fn((char*)plastic_state.data(), n * sizeof(double));
+        fn((char*)max_dtilde.data(), n * sizeof(double));
+    }
+
+    void resizeAttributes(size_t n)
+    {
+        X.resize(n);
+        V.resize(n);
+        C.resize(n);
+        F.resize(n);
+        mass.resize(n);
+        element_measure.resize(n);
+        material.resize(n);
+        damage.resize(n);
+        degradation.resize(n);
+        laplacian.resize(n);
+        plastic_state.resize(n);
+        max_dtilde.resize(n);
+    }
 
     zmpm_particles view()
     {

@@ -37,6 +37,7 @@ class Scene:
     v: np.ndarray = None
     C: np.ndarray = None
     info: dict = field(default_factory=dict)
+    options: dict = field(default_factory=dict)  # zmpm_scene_desc options of the workload (e.g. implicit_damage)
 
     def __post_init__(self):
         n = self.x.shape[0]
@@ -52,9 +53,11 @@ class Scene:
 
     def make(self, lib, precision=A.PREC_FP64, device=0, apic_rpic_ratio=1.0, **transfer):
         """transfer: transfer_scheme, flip_pic_ratio, mls_mpm, rpic_damping_iteration,
-        pic_damping_iteration (zmpm_scene_desc)."""
+        pic_damping_iteration, implicit_damage (zmpm_scene_desc); they extend and
+        override the workload's own options."""
         sim = A.Simulation(lib, dx=self.dx, precision=precision, domain_nodes=self.domain_nodes,
-                           gravity=self.gravity, apic_rpic_ratio=apic_rpic_ratio, device=device, **transfer)
+                           gravity=self.gravity, apic_rpic_ratio=apic_rpic_ratio, device=device,
+                           **{**self.options, **transfer})
         sim.set_materials(self.materials)
         for c in self.colliders:
             sim.add_collider(c)
@@ -265,10 +268,20 @@ def damage_slab_active(**kw):
     return damage_slab(**kw)
 
 
+def damage_slab_implicit(**kw):
+    """The active damage slab (sigma_c = 20 Pa) with the implicit phase-field solve
+    (AnisoFractureSimulation(true), examples/3DtestImplicit.h)."""
+    sc = damage_slab_active(**kw)
+    sc.name = "damage_slab_implicit"
+    sc.options = dict(implicit_damage=True)
+    return sc
+
+
 WORKLOADS = {
     "elastic_cube": elastic_cube,
     "damage_slab": damage_slab,
     "damage_slab_active": damage_slab_active,
+    "damage_slab_implicit": damage_slab_implicit,
     "sparse_spheres": sparse_spheres,
     "batched_scenes": batched_scenes,
     "two_material": two_material,
