@@ -52,6 +52,19 @@ constexpr long long FIXED_MAGIC_BITS = 0x4338000000000000LL;
 
 __device__ __forceinline__ void latch(int* counters, int bits) { atomicOr(&counters[1], bits); }
 
+// Workgroup ids are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  The logical index below gives each XCD one contiguous run of the grid's
+// work items (G2P's particle segments), so neighbours that read the same halo
+// nodes run on the same XCD and meet in its L2 (G2P FP64 ~189 -> ~164 us, FP32
+// ~104 -> ~101 us for 1M particles on MI355X).  A bijection on [0, nwg).
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcdLocal(int wg, int nwg)
+{
+    const int x = wg % NUM_XCD, q = wg / NUM_XCD;
+    const int full = nwg / NUM_XCD, rem = nwg % NUM_XCD;
+    return x * full + min(x, rem) + q;
+}
+
 // ------------------------------------------------------- particle field access
 // Element i of a field whose base (buffer + field * capacity) is uniform: the
 // byte offset is formed in 32 bits (capacity < 2^27), so the access takes the
@@ -917,7 +930,11 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
 
     // in-block order of this workgroup's blocks first (a loop of its own, so
     // its registers are free again for the scatter)
-    for (int b = blockIdx.x; !DAMP && b < nblocks; b += gridDim.x) {
+    // blocks round-robin over the XCDs: neighbouring blocks flush their shared
+    // halo nodes from different L2s (an XCD-contiguous mapping, as in G2P, made
+    // P2G ~25 % slower on MI355X); both loops take the same blocks
+    const int wgl = (int)blockIdx.x;
+    for (int b = wgl; !DAMP && b < nblocks; b += gridDim.x) {
         const int bs = D.block_start[b], cnt = D.block_count[b];
         if (cnt <= 0) { // halo-only block
             if (t < NB) {
@@ -930,7 +947,7 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
         blockOrder(D, b, bs, cnt, ska, pa, skb, pb, STAGE, hist, cur, cst);
     }
     int ovf = 0;
-    for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    for (int b = wgl; b < nblocks; b += gridDim.x) {
         __syncthreads(); // previous block flushed
         if (t < NB) {
             const int c = b * NB + t;
@@ -1714,7 +1731,7 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
     constexpr int NGT = FLIP ? 7 : 4;
     __shared__ T gt[NGT][TL::TN];
     __shared__ int seg_end;
-    const int lo = blockIdx.x * WG;
+    const int lo = xcdLocal((int)blockIdx.x, (int)gridDim.x) * WG;
     const int hi = min(lo + WG, D.n);
     const int t = threadIdx.x, s = lo + t;
     const bool live = s < hi;
@@ -1964,7 +1981,7 @@ __global__ __launch_bounds__(WG) void k_damp_g2p(Dev<T, AccT> D)
     constexpr int NB = TL::NB;
     __shared__ T gt[3][TL::TN];
     __shared__ int seg_end;
-    const int lo = blockIdx.x * WG;
+    const int lo = xcdLocal((int)blockIdx.x, (int)gridDim.x) * WG;
     const int hi = min(lo + WG, D.n);
     const int t = threadIdx.x, s = lo + t;
     const bool live = s < hi;
