@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define ZMPM_ABI_VERSION 4
+#define ZMPM_ABI_VERSION 5
 
 /* Error model.  Replaces ZIRAN_ASSERT -> std::runtime_error
  * (Lib/Ziran/CS/Util/Debug.h:19-37) and the CFL RestartException
@@ -72,11 +72,20 @@ typedef enum {
  *  DRUCKER_PRAGER_STVK_HENCKY DruckerPragerStvkHencky::projectStrain (PlasticityApplier.cpp:811-876):
  *                            friction cone alpha = sqrt(2/3) 2 sin(phi) / (3 - sin(phi)),
  *                            cohesion, tip projection with volume correction
- *                            log Jp += beta tr(eps); per-particle state: log Jp */
+ *                            log Jp += beta tr(eps); per-particle state: log Jp
+ *  SNOW                      SnowPlasticity::projectStrain (PlasticityApplier.cpp:16-50):
+ *                            singular values clamped to [1 - critical_compression,
+ *                            1 + critical_stretch], Jp <- Jp det F / det Fe clamped to
+ *                            [min_jp, max_jp], mu and lambda scaled by
+ *                            exp(hardening (Jp_old - Jp)); per-particle state: Jp (the
+ *                            material's mu, lambda are the values at Jp = 1).  Models
+ *                            whose stress is linear in (mu, lambda): fixed corotated,
+ *                            neo-Hookean. */
 typedef enum {
     ZMPM_PLASTICITY_NONE = 0,
     ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY = 1,
-    ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY = 2
+    ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY = 2,
+    ZMPM_PLASTICITY_SNOW = 3
 } zmpm_plasticity;
 
 /* Collision object types; values equal AnalyticCollisionObject::COLLISION_OBJECT_TYPE
@@ -145,11 +154,15 @@ typedef struct {
     double residual_phase;   /* k in g = (1-d)^2 (1-k) + k                                */
     double l0, sigma_crit, eta, zeta;
     double yield_stress;     /* VonMisesStvkHencky::yield_stress (initial, per particle)  */
-    double hardening;        /* VonMisesStvkHencky::xi / DruckerPragerStvkHencky::beta     */
+    double hardening;        /* VonMisesStvkHencky::xi / DruckerPragerStvkHencky::beta /
+                                SnowPlasticity::psi                                       */
     double friction_angle;   /* DruckerPragerStvkHencky friction angle (degrees)          */
     double cohesion;         /* DruckerPragerStvkHencky::cohesion                          */
     int32_t volume_correction; /* DruckerPragerStvkHencky::volume_correction              */
     int32_t reserved;
+    double critical_compression; /* SnowPlasticity::theta_c (reference default 2e-2)      */
+    double critical_stretch;     /* SnowPlasticity::theta_s (7.5e-3)                       */
+    double min_jp, max_jp;       /* SnowPlasticity::min_Jp, max_Jp (0.6, 20)               */
 } zmpm_material;
 
 /* Analytic collision object with rigid translation b(t) = b0 + dbdt * t
@@ -183,7 +196,8 @@ typedef struct {
     double* g;          /* [n] degradation g of the QR model, may be NULL (1 on upload)   */
     double* laplacian;  /* [n] damage Laplacian (download only), may be NULL              */
     double* plastic_state; /* [n] plasticity state (von Mises: yield stress, Drucker-Prager:
-                              log Jp), may be NULL (upload: the material's yield stress / 0) */
+                              log Jp, snow: Jp), may be NULL (upload: the material's yield
+                              stress / 0 / 1)                                              */
     double* max_dtilde; /* [n] AnisotropicPhaseField::maxDTilde, the history maximum of the
                            crack driving force the implicit damage solve reads
                            (AnisoFractureSimulation.h:284-337), may be NULL (0 on upload)  */

@@ -1148,6 +1148,37 @@ void druckerPragerProject(const zmpm_material& mt, double& logJp, M3& F)
         for (int c = 0; c < 3; c++) F.a[r][c] = U.a[r][0] * e[0] * V.a[c][0] + U.a[r][1] * e[1] * V.a[c][1] + U.a[r][2] * e[2] * V.a[c][2];
 }
 
+// SnowPlasticity::projectStrain (PlasticityApplier.cpp:16-50): singular values
+// clamped to [1 - theta_c, 1 + theta_s], F = U Sigma_clamped V^T, Jp <- Jp det F /
+// det Fe clamped to [min_Jp, max_Jp] (the !(<=) / !(>=) tests of the reference,
+// so a NaN lands on a bound).  The reference then scales the particle's mu and
+// lambda by exp(psi (Jp_old - Jp_new)); over a run that is exp(psi (1 - Jp)) with
+// Jp starting at 1, which snowScale gives from the stored Jp.
+void snowProject(const zmpm_material& mt, double& Jp, M3& F)
+{
+    M3 U, V;
+    double sigma[3];
+    svd3(F, U, sigma, V, kSvdTol);
+    double Fe_det = 1;
+    for (int a = 0; a < 3; a++) {
+        sigma[a] = std::max(std::min(sigma[a], 1.0 + mt.critical_stretch), 1.0 - mt.critical_compression);
+        Fe_det *= sigma[a];
+    }
+    double Jp_new = Jp * mdet(F) / Fe_det;
+    if (!(Jp_new <= mt.max_jp)) Jp_new = mt.max_jp;
+    if (!(Jp_new >= mt.min_jp)) Jp_new = mt.min_jp;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++)
+            F.a[r][c] = U.a[r][0] * sigma[0] * V.a[c][0] + U.a[r][1] * sigma[1] * V.a[c][1] + U.a[r][2] * sigma[2] * V.a[c][2];
+    Jp = Jp_new;
+}
+
+// The factor on mu and lambda of a snow particle (1 for every other material).
+double snowScale(const zmpm_material& mt, double Jp)
+{
+    return mt.plasticity == ZMPM_PLASTICITY_SNOW ? std::exp(mt.hardening * (1.0 - Jp)) : 1.0;
+}
+
 // gridToParticlesHelper<scheme, deg 2, mls> generic path (MpmSimulationBase.cpp:1412-1478)
 // and evolveStrain: F <- (I + dt gradV) F (FBasedMpmForceHelper.cpp:93-108);
 // J <- (1 + dt tr gradV) J (JBasedMpmForceHelper.cpp:79-94).  APIC_blend_RPIC:
@@ -1230,6 +1261,7 @@ int gridToParticlesPass(zmpm_context* h, double dt, double r, double flip, bool 
         const zmpm_material& mt = matOf(h, i);
         if (mt.plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY) vonMisesProject(mt.mu, mt.hardening, h->ys[i], Fn);
         if (mt.plasticity == ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY) druckerPragerProject(mt, h->ys[i], Fn);
+        if (mt.plasticity == ZMPM_PLASTICITY_SNOW) snowProject(mt, h->ys[i], Fn);
         mstore(Fn, Fp);
     }
     return cfl_flag;
@@ -1401,7 +1433,9 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
     std::vector<double> vtau((size_t)n * 9);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
-        M3 tau = vtauOf(matOf(h, i), &h->F[9 * i], h->V0[i], h->g[i]);
+        const zmpm_material& mt = matOf(h, i);
+        // stress linear in (mu, lambda): a snow particle's hardened moduli scale it
+        M3 tau = vtauOf(mt, &h->F[9 * i], h->V0[i] * snowScale(mt, h->ys[i]), h->g[i]);
         mstore(tau, &vtau[9 * i]);
     }
 
@@ -1689,10 +1723,12 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
     if (!h || count <= 0 || count > 256 || !mats) return fail(h, ZMPM_E_ARG, "bad material table");
     for (int32_t i = 0; i < count; i++) {
         const zmpm_material& m = mats[i];
-        if (m.plasticity < ZMPM_PLASTICITY_NONE || m.plasticity > ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY)
+        if (m.plasticity < ZMPM_PLASTICITY_NONE || m.plasticity > ZMPM_PLASTICITY_SNOW)
             return fail(h, ZMPM_E_ARG, "unknown plasticity");
         if (m.plasticity && m.model == ZMPM_MODEL_EQUATION_OF_STATE)
             return fail(h, ZMPM_E_ARG, "plasticity needs a model with mu");
+        if (m.plasticity == ZMPM_PLASTICITY_SNOW && m.model != ZMPM_MODEL_FIXED_COROTATED && m.model != ZMPM_MODEL_NEO_HOOKEAN)
+            return fail(h, ZMPM_E_ARG, "snow plasticity needs a stress linear in mu and lambda (fixed corotated, neo-Hookean)");
     }
     h->mats.assign(mats, mats + count);
     h->any_damage = false;
@@ -1745,7 +1781,9 @@ zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
     for (int64_t i = 0; i < n; i++) {
         const zmpm_material& mt = h->mats[(size_t)h->mat[i]];
         h->ys[i] = p->plastic_state ? p->plastic_state[i]
-                                    : (mt.plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY ? mt.yield_stress : 0.0);
+                                    : (mt.plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY ? mt.yield_stress
+                                       : mt.plasticity == ZMPM_PLASTICITY_SNOW                 ? 1.0
+                                                                                               : 0.0);
     }
     h->lap.assign(n, 0.0);
     h->dtil.assign(n, 0.0);
@@ -1946,6 +1984,13 @@ void zmpm_oracle_drucker_prager(const zmpm_material* mat, double* logJp, double*
 }
 
 // VonMisesStvkHencky::projectStrain on one deformation gradient (row-major, in place)
+void zmpm_oracle_snow(const zmpm_material* mat, double* Jp, double* F)
+{
+    M3 f = mload(F);
+    snowProject(*mat, *Jp, f);
+    mstore(f, F);
+}
+
 void zmpm_oracle_von_mises(double mu, double xi, double* yield_stress, double* F)
 {
     M3 f = mload(F);

@@ -395,6 +395,29 @@ def test_drucker_prager_plasticity(gpu, oracle, precision):
         1e-3, np.abs(o["plastic_state"]).max())
 
 
+@pytest.mark.parametrize("model", [A.MODEL_FIXED_COROTATED, A.MODEL_NEO_HOOKEAN])
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_snow_plasticity(gpu, oracle, model, precision):
+    # SnowPlasticity (PlasticityApplier.cpp:16-50, MpmInitializationHelper.h:883-965):
+    # a pre-compressed snow cube landing on the sticky floor; singular values
+    # are clamped from the first substep and the hardened moduli follow Jp
+    sc = small_cube(cells=10, seed=8)
+    sc.v[:, 1] = -1.0
+    rng = np.random.default_rng(3)
+    sc.F = (0.97 * np.eye(3)).reshape(1, 9) + 0.01 * rng.normal(size=(sc.n, 9))
+    sc.materials = [A.make_material(model, 1e4, 0.3, plasticity=A.PLASTICITY_SNOW, hardening=10.0)]
+    steps = 25
+    g = run(gpu, sc, precision, steps).download()
+    o = run(oracle, sc, A.PREC_FP64, steps).download()
+    assert np.abs(o["plastic_state"] - 1.0).max() > 1e-3  # Jp moved
+    tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+    for k in ("x", "v", "F", "plastic_state"):
+        assert normwise(g[k], o[k]) < tol, k
+    sc.materials[0].plasticity = A.PLASTICITY_NONE
+    e = run(oracle, sc, A.PREC_FP64, steps).download()
+    assert normwise(e["F"], o["F"]) > 1e-4
+
+
 @pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
 def test_calculate_dt(gpu, oracle, precision):
     sc = small_cube(cells=8)

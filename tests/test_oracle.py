@@ -899,3 +899,72 @@ def test_implicit_damage_matches_dense_solve(oracle):
         want[p_] = min(max(d0[p_], d0[p_] + acc), 1.0)
     assert np.abs(out["d"] - want).max() < 1e-6
     assert (out["d"] > d0).sum() > 10
+
+
+def snow(O, m, Jp, F):
+    O.zmpm_oracle_snow.argtypes = [C.POINTER(A.Material), DP, DP]
+    j = C.c_double(Jp)
+    f = np.ascontiguousarray(F, float).ravel().copy()
+    O.zmpm_oracle_snow(C.byref(m), C.byref(j), p(f))
+    return f.reshape(3, 3), j.value
+
+
+def test_snow_return_mapping(O):
+    # SnowPlasticity::projectStrain (PlasticityApplier.cpp:16-50): singular values
+    # clamped to [1 - theta_c, 1 + theta_s], principal frames kept, Jp <- Jp det F /
+    # det Fe clamped to [min_Jp, max_Jp]; inside the bounds F and Jp stay.
+    rng = np.random.default_rng(31)
+    m = A.make_material(A.MODEL_FIXED_COROTATED, plasticity=A.PLASTICITY_SNOW, hardening=10.0,
+                        critical_compression=0.025, critical_stretch=0.0075, min_jp=0.6, max_jp=20.0)
+    lo, hi = 1 - 0.025, 1 + 0.0075
+    for k in range(30):
+        U = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        V = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        U *= np.sign(np.linalg.det(U))
+        V *= np.sign(np.linalg.det(V))
+        s = np.sort(rng.uniform(0.9, 1.05, 3))[::-1] if k % 3 else np.sort(rng.uniform(lo + 1e-3, hi - 1e-3, 3))[::-1]
+        F = U @ np.diag(s) @ V.T
+        Jp0 = rng.uniform(0.8, 1.2)
+        Fn, Jp = snow(O, m, Jp0, F)
+        sc = np.clip(s, lo, hi)
+        assert np.abs(Fn - U @ np.diag(sc) @ V.T).max() < 1e-12
+        want = np.clip(Jp0 * np.prod(s) / np.prod(sc), 0.6, 20.0)
+        assert abs(Jp - want) < 1e-12 * want
+        if np.all((s > lo) & (s < hi)):
+            assert abs(Jp - Jp0) < 1e-13 and np.abs(Fn - F).max() < 1e-13
+    # the Jp bounds
+    F = np.diag([0.5, 0.5, 0.5])
+    _, Jp = snow(O, m, 1.0, F)
+    assert Jp == 0.6
+    _, Jp = snow(O, m, 19.9, np.diag([1.5, 1.5, 1.5]))
+    assert Jp == 20.0
+
+
+def test_snow_hardening_scales_the_moduli(oracle):
+    # A snow particle's mu and lambda are the material's times exp(psi (1 - Jp))
+    # (the reference multiplies them by exp(psi (Jp_old - Jp_new)) at every
+    # projection, from Jp = 1): with bounds wide enough that nothing is clamped,
+    # a snow cube at Jp = 0.95 moves as a plain corotated cube with those moduli.
+    sc = cube_scene(cells=4)
+    rng = np.random.default_rng(4)
+    sc.F = (np.eye(3).reshape(1, 9) + 0.02 * rng.normal(size=(sc.n, 9)))
+    psi, Jp0 = 10.0, 0.95
+    h = np.exp(psi * (1 - Jp0))
+    base = sc.materials[0]
+    snow_m = A.make_material(A.MODEL_FIXED_COROTATED, plasticity=A.PLASTICITY_SNOW, hardening=psi,
+                             critical_compression=0.9, critical_stretch=5.0, min_jp=0.1, max_jp=10.0)
+    snow_m.mu, snow_m.lam = base.mu, base.lam
+    plain = A.make_material(A.MODEL_FIXED_COROTATED)
+    plain.mu, plain.lam = base.mu * h, base.lam * h
+    outs = []
+    for mt, ps in ((snow_m, np.full(sc.n, Jp0)), (plain, None)):
+        sc.materials = [mt]
+        sim = sc.make(oracle)
+        sim.upload(sc.x, sc.v, sc.C, sc.F, sc.m, sc.V0, sc.material, plastic_state=ps)
+        sim.run(3, sc.dt, 0.0)
+        outs.append(sim.download())
+    a, b = outs
+    assert np.abs(a["plastic_state"] - Jp0).max() < 1e-12
+    for k in ("x", "v", "F"):
+        assert normwise(a[k], b[k]) < 1e-10, k
+    assert normwise(b["v"], sc.v) > 1e-6  # the stress moved the particles

@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-ZMPM_ABI_VERSION = 4
+ZMPM_ABI_VERSION = 5
 
 # zmpm_status
 OK, E_ARG, E_HIP, E_DOMAIN, E_CFL, E_NAN, E_OVERFLOW, E_NOMEM = range(8)
@@ -25,7 +25,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 MODEL_FIXED_COROTATED, MODEL_QR_STABLE_NEO_HOOKEAN, MODEL_NEO_HOOKEAN, MODEL_EQUATION_OF_STATE = range(4)
 STICKY, SLIP, SEPARATE, GHOST = 1, 2, 3, 4
 APIC_BLEND_RPIC, FLIP_BLEND_PIC = 0, 1
-PLASTICITY_NONE, PLASTICITY_VON_MISES_STVK_HENCKY, PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY = 0, 1, 2
+PLASTICITY_NONE, PLASTICITY_VON_MISES_STVK_HENCKY, PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY, PLASTICITY_SNOW = 0, 1, 2, 3
 SHAPE_HALF_SPACE, SHAPE_BOX = 0, 1
 
 D3 = C.c_double * 3
@@ -65,6 +65,8 @@ class Material(C.Structure):
         ("yield_stress", C.c_double), ("hardening", C.c_double),
         ("friction_angle", C.c_double), ("cohesion", C.c_double),
         ("volume_correction", C.c_int32), ("reserved", C.c_int32),
+        ("critical_compression", C.c_double), ("critical_stretch", C.c_double),
+        ("min_jp", C.c_double), ("max_jp", C.c_double),
     ]
 
 
@@ -109,9 +111,12 @@ def make_material(model=MODEL_FIXED_COROTATED, E=1e4, nu=0.3, *, has_damage=Fals
                   bulk=0.0, gamma=7.0, directors=((1.0, 0.0, 0.0),), alphas=(0.0,),
                   residual_phase=0.001, l0=0.0, sigma_crit=1.0, eta=0.1, zeta=1.0,
                   plasticity=PLASTICITY_NONE, yield_stress=0.0, hardening=0.0, friction_angle=30.0, cohesion=0.0,
-                  volume_correction=True) -> Material:
+                  volume_correction=True, critical_compression=2e-2, critical_stretch=7.5e-3, min_jp=0.6,
+                  max_jp=20.0) -> Material:
     """Material from Young's modulus and Poisson ratio with the reference's Lame
-    conversion (CorotatedElasticity.cpp:15-20, QRStableNeoHookean.h:61-65)."""
+    conversion (CorotatedElasticity.cpp:15-20, QRStableNeoHookean.h:61-65).  The snow
+    plasticity defaults are SnowPlasticity's constructor defaults (PlasticityApplier.h:103;
+    its psi is `hardening`)."""
     m = Material()
     m.model = model
     m.has_damage = 1 if has_damage else 0
@@ -127,6 +132,7 @@ def make_material(model=MODEL_FIXED_COROTATED, E=1e4, nu=0.3, *, has_damage=Fals
     m.residual_phase, m.l0, m.sigma_crit, m.eta, m.zeta = residual_phase, l0, sigma_crit, eta, zeta
     m.plasticity, m.yield_stress, m.hardening = plasticity, yield_stress, hardening
     m.friction_angle, m.cohesion, m.volume_correction = friction_angle, cohesion, int(bool(volume_correction))
+    m.critical_compression, m.critical_stretch, m.min_jp, m.max_jp = critical_compression, critical_stretch, min_jp, max_jp
     return m
 
 

@@ -290,6 +290,13 @@ zmpm_status uploadMaterials(zmpm_context* h)
         // DruckerPragerStvkHencky::DruckerPragerStvkHencky (PlasticityApplier.cpp:811-819)
         const double sin_phi = std::sin(m.friction_angle / 180.0 * 3.141592653);
         d.dp_alpha = (T)(std::sqrt(2.0 / 3.0) * 2.0 * sin_phi / (3.0 - sin_phi));
+        // SnowPlasticity (PlasticityApplier.cpp:16-50)
+        d.snow_lo = 1.0 - m.critical_compression;
+        d.snow_hi = 1.0 + m.critical_stretch;
+        d.min_jp = m.min_jp;
+        d.max_jp = m.max_jp;
+        d.snow_lam_lo = (float)(d.snow_lo * d.snow_lo - 1.0);
+        d.snow_lam_hi = (float)(d.snow_hi * d.snow_hi - 1.0);
     }
     HIPCHK(h, hipMemcpyAsync(h->mats_dev, md.data(), sizeof(MatDev<T>) * kMaxMaterials, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -443,8 +450,11 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
         buf[PD * cap + i] = p->d ? (T)p->d[i] : T(0);
         buf[PG * cap + i] = p->g ? (T)p->g[i] : T(1);
         const int mi = p->material[i];
-        const bool vm = mi >= 0 && mi < (int)h->mats.size() && h->mats[(size_t)mi].plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY;
-        buf[PYS * cap + i] = p->plastic_state ? (T)p->plastic_state[i] : (vm ? (T)h->mats[(size_t)mi].yield_stress : T(0));
+        const int pl = mi >= 0 && mi < (int)h->mats.size() ? h->mats[(size_t)mi].plasticity : ZMPM_PLASTICITY_NONE;
+        const T ys0 = pl == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY ? (T)h->mats[(size_t)mi].yield_stress
+            : pl == ZMPM_PLASTICITY_SNOW                          ? T(1)
+                                                                  : T(0);
+        buf[PYS * cap + i] = p->plastic_state ? (T)p->plastic_state[i] : ys0;
         buf[PDT * cap + i] = p->max_dtilde ? (T)p->max_dtilde[i] : T(0);
     }
     std::vector<int> ids(cap, 0), mat(cap, 0);
@@ -745,10 +755,13 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
     for (int i = 0; i < count; i++) {
         if (mats[i].model < 0 || mats[i].model > ZMPM_MODEL_EQUATION_OF_STATE)
             return fail(h, ZMPM_E_ARG, "unknown constitutive model");
-        if (mats[i].plasticity < ZMPM_PLASTICITY_NONE || mats[i].plasticity > ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY)
+        if (mats[i].plasticity < ZMPM_PLASTICITY_NONE || mats[i].plasticity > ZMPM_PLASTICITY_SNOW)
             return fail(h, ZMPM_E_ARG, "unknown plasticity");
         if (mats[i].plasticity && mats[i].model == ZMPM_MODEL_EQUATION_OF_STATE)
             return fail(h, ZMPM_E_ARG, "plasticity needs a model with mu");
+        if (mats[i].plasticity == ZMPM_PLASTICITY_SNOW && mats[i].model != ZMPM_MODEL_FIXED_COROTATED
+            && mats[i].model != ZMPM_MODEL_NEO_HOOKEAN)
+            return fail(h, ZMPM_E_ARG, "snow plasticity needs a stress linear in mu and lambda (fixed corotated, neo-Hookean)");
     }
     (void)hipSetDevice(h->desc.device);
     dropGraphs(h);

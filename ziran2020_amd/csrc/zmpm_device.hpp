@@ -774,6 +774,19 @@ __device__ __forceinline__ void writeBounds(unsigned int* maxbits, const float* 
     }
 }
 
+// The factor on a particle's stress from its material's plasticity: snow's
+// hardened moduli exp(psi (1 - Jp)) (the supported models' stress is linear in
+// mu and lambda); 1 otherwise.
+template <class T>
+__device__ __forceinline__ T snowScale(const MatDev<T>& mt, T jp)
+{
+    if (mt.plasticity != ZMPM_PLASTICITY_SNOW) return T(1);
+    if constexpr (sizeof(T) == 4)
+        return expf(mt.hardening * (1.f - jp));
+    else
+        return exp(mt.hardening * (1.0 - jp));
+}
+
 // Stress of the uploaded state and the P2G bounds of the first substep; with
 // bin, also the binning of the first substep (grid of max_slots workgroups,
 // striding over particles).  Run again without bin after a material change.
@@ -791,7 +804,8 @@ __global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D, int bin)
             const MatDev<T>& mt = D.mats[D.pmat[p]];
             Mat3<T> F = loadM(D.P, cap, PF, p);
             T S[6], v[3], Cm[9];
-            vtauSym(mt, F, ldAt(D.P + PVOL * cap, p), ldAt(D.P + PG * cap, p), S);
+            const T jp = mt.plasticity == ZMPM_PLASTICITY_SNOW ? ldAt(D.P + PYS * cap, p) : T(1);
+            vtauSym(mt, F, ldAt(D.P + PVOL * cap, p) * snowScale(mt, jp), ldAt(D.P + PG * cap, p), S);
 #pragma unroll
             for (int i = 0; i < 6; i++) stAt(D.P + (PS + i) * cap, p, S[i]);
 #pragma unroll
@@ -1687,7 +1701,7 @@ __device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     }
     // stress of the new state for the next substep's P2G
     T S[6];
-    rare = rare || vtauSym<true>(mt, Fn, R.V0, g, S);
+    rare = rare || vtauSym<true>(mt, Fn, R.V0 * snowScale(mt, R.ys), g, S);
     if (!rare) {
 #pragma unroll
         for (int i = 0; i < 6; i++) stAt(Q + (PS + i) * cap, s, S[i]);
@@ -1897,6 +1911,45 @@ __device__ __noinline__ bool druckerPragerProject(const MatDev<T>& mt, Mat3<T>& 
 // deferred.  von Mises: outside or near |eps_hat| = yield / (2 mu).  Drucker-
 // Prager: anything but strictly inside the cone with log Jp = 0 (where the
 // reference's rebuild of F from its SVD returns F).
+// SnowPlasticity::projectStrain (PlasticityApplier.cpp:16-50) on the stored
+// deformation, in double as vonMisesProject: singular values clamped to
+// [1 - theta_c, 1 + theta_s] (std::min / std::max order), F = U Sigma V^T,
+// Jp <- Jp det F / det Fe clamped to [min_Jp, max_Jp].  The reference also scales
+// the particle's mu and lambda by exp(psi (Jp_old - Jp_new)); snowScale gives
+// the accumulated factor exp(psi (1 - Jp)) from the stored Jp.  F is rebuilt
+// from the SVD in every case, as the reference does.  Returns true.
+template <class T>
+__device__ __noinline__ bool snowProject(const MatDev<T>& mt, Mat3<T>& Fs, T& jp)
+{
+    constexpr double id = sizeof(T) == 4 ? 1.0 : 0.0;
+    Mat3<double> F, U, V;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) F.a[i][j] = (double)Fs.a[i][j] + (i == j ? id : 0.0);
+    double sg[3];
+    svd3(F, U, sg, V);
+    double fe_det = 1.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double v = mt.snow_hi < sg[a] ? mt.snow_hi : sg[a];
+        v = v < mt.snow_lo ? mt.snow_lo : v;
+        sg[a] = v;
+        fe_det *= v;
+    }
+    double jn = (double)jp * det3(F) / fe_det;
+    if (!(jn <= mt.max_jp)) jn = mt.max_jp;
+    if (!(jn >= mt.min_jp)) jn = mt.min_jp;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            Fs.a[r][c] = (T)((U.a[r][0] * sg[0] * V.a[c][0] + U.a[r][1] * sg[1] * V.a[c][1] + U.a[r][2] * sg[2] * V.a[c][2])
+                             - (r == c ? id : 0.0));
+    jp = (T)jn;
+    return true;
+}
+
 __device__ __forceinline__ bool plasticMayProject(const MatDev<double>&, const Mat3<double>&, double) { return true; }
 __device__ __forceinline__ bool plasticMayProject(const MatDev<float>& mt, const Mat3<float>& H, float state)
 {
@@ -1907,6 +1960,8 @@ __device__ __forceinline__ bool plasticMayProject(const MatDev<float>& mt, const
     strainOfH(H, E);
     float l1, l2, l3;
     symEigvals3(E, l1, l2, l3);
+    if (mt.plasticity == ZMPM_PLASTICITY_SNOW) // a singular value may leave [1 - theta_c, 1 + theta_s]
+        return !(l3 > mt.snow_lam_lo + 4e-6f && l1 < mt.snow_lam_hi - 4e-6f);
     if (!(1.f + l3 > 1e-8f)) return true;
     const float c = dp ? mt.cohesion : 0.f;
     const float e0 = 0.5f * log1pf(l1) - c, e1 = 0.5f * log1pf(l2) - c, e2 = 0.5f * log1pf(l3) - c;
@@ -1947,10 +2002,12 @@ __global__ __launch_bounds__(WG) void k_rare(Dev<T, AccT> D, T dt)
             stAt(D.Q + PG * cap, s, g);
         }
         Mat3<T> Fn = loadM(D.Q, cap, PF, s);
+        T ys = T(0);
         if (mt.plasticity) {
-            T ys = ldAt(D.Q + PYS * cap, s);
+            ys = ldAt(D.Q + PYS * cap, s);
             const bool changed = mt.plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY ? vonMisesProject(mt, Fn, ys)
-                                                                                       : druckerPragerProject(mt, Fn, ys);
+                : mt.plasticity == ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY             ? druckerPragerProject(mt, Fn, ys)
+                                                                                         : snowProject(mt, Fn, ys);
             if (changed) {
 #pragma unroll
                 for (int k = 0; k < 9; k++) stAt(D.Q + (PF + k) * cap, s, Fn.a[k / 3][k % 3]);
@@ -1958,7 +2015,7 @@ __global__ __launch_bounds__(WG) void k_rare(Dev<T, AccT> D, T dt)
             }
         }
         T S[6];
-        vtauSym<false>(mt, Fn, ldAt(D.Q + PVOL * cap, s), g, S);
+        vtauSym<false>(mt, Fn, ldAt(D.Q + PVOL * cap, s) * snowScale(mt, ys), g, S);
 #pragma unroll
         for (int k = 0; k < 6; k++) stAt(D.Q + (PS + k) * cap, s, S[k]);
         if constexpr (std::is_same<AccT, long long>::value) forceBounds(bnd, S, dmg ? d : T(0), D.odx, D.any_damage != 0);

@@ -43,7 +43,7 @@ enum : int {
     PG = 27,  // degradation g
     PLAP = 28, // damage Laplacian (output)
     PS = 29,   // V0 * Kirchhoff stress of the current F, symmetric: xx xy xz yy yz zz
-    PYS = 35,  // plasticity state: von Mises yield stress / Drucker-Prager log Jp
+    PYS = 35,  // plasticity state: von Mises yield stress / Drucker-Prager log Jp / snow Jp
     PDT = 36,  // implicit damage: history maximum of the crack driving force D~
     NPF = 37
 };
@@ -117,6 +117,10 @@ struct MatDev {
     T hardening;       // VonMisesStvkHencky::xi / DruckerPragerStvkHencky::beta
     T dp_alpha;        // DruckerPragerStvkHencky::alpha (from the friction angle)
     T cohesion;        // DruckerPragerStvkHencky::cohesion
+    // SnowPlasticity: singular-value bounds, Jp bounds, and the bounds on the
+    // eigenvalues of F^T F - I that the float G2P gate tests (hardening is psi)
+    double snow_lo, snow_hi, min_jp, max_jp;
+    float snow_lam_lo, snow_lam_hi;
 };
 
 struct ColDev {

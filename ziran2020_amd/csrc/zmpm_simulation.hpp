@@ -68,7 +68,7 @@ public:
     std::vector<TM> C, F;
     std::vector<int32_t> material;
     std::vector<double> damage, degradation, laplacian;
-    std::vector<double> plastic_state; // von Mises yield stress / Drucker-Prager log Jp per particle
+    std::vector<double> plastic_state; // von Mises yield stress / Drucker-Prager log Jp / snow Jp per particle
     std::vector<double> max_dtilde;    // AnisotropicPhaseField::maxDTilde (implicit damage)
     // ---- scene
     std::vector<zmpm_material> materials;
@@ -120,10 +120,10 @@ public:
         degradation.push_back(1);
         laplacian.push_back(0);
         max_dtilde.push_back(0);
-        plastic_state.push_back(mat >= 0 && mat < (int)materials.size()
-                                        && materials[(size_t)mat].plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY
-                                    ? materials[(size_t)mat].yield_stress
-                                    : 0.0);
+        const int pl = mat >= 0 && mat < (int)materials.size() ? materials[(size_t)mat].plasticity : ZMPM_PLASTICITY_NONE;
+        plastic_state.push_back(pl == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY ? materials[(size_t)mat].yield_stress
+                                    : pl == ZMPM_PLASTICITY_SNOW              ? 1.0 // SnowPlasticity::Jp starts at 1
+                                                                              : 0.0);
         dirty_ = true;
     }
 
