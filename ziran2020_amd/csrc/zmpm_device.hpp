@@ -572,13 +572,19 @@ __device__ __forceinline__ void blockOrder(const Dev<T, AccT>& D, int b, int sta
             pb[dst] = pa[i];
         }
         __syncthreads();
-        // rank inside the cell; write the sorted block
+        // rank inside the cell (keys are unique); write the sorted block.  The
+        // cell's keys are read four at a time so the LDS reads overlap.
         for (int i = t; i < cnt; i += WG) {
             const uint32_t k = skb[i];
             const int c = (int)(k >> idbits);
             const int c0 = cst[c], c1 = c0 + hist[c];
             int pos = c0;
-            for (int j = c0; j < c1; j++) pos += skb[j] < k;
+            int j = c0;
+            for (; j + 4 <= c1; j += 4) {
+                const uint32_t k0 = skb[j], k1 = skb[j + 1], k2 = skb[j + 2], k3 = skb[j + 3];
+                pos += (int)(k0 < k) + (int)(k1 < k) + (int)(k2 < k) + (int)(k3 < k);
+            }
+            for (; j < c1; j++) pos += skb[j] < k;
             D.perm[start + pos] = pb[i];
             D.spid[start + pos] = (int)(k & idmask);
             D.sorted_cell[start + pos] = b * NB + c;
