@@ -544,13 +544,28 @@ __device__ __forceinline__ void blockOrder(const Dev<T, AccT>& D, int b, int sta
     const bool staged = cnt <= stage;
     if (t < NB) hist[t] = 0;
     __syncthreads();
-    // stage (key, live index) and count cells
-    for (int i = t; i < cnt; i += WG) {
-        const uint32_t k = D.tmpk[start + i];
-        atomicAdd(&hist[k >> idbits], 1);
-        if (staged) {
-            ska[i] = k;
-            pa[i] = D.tmp[start + i];
+    // stage (key, live index) and count cells; four rounds of loads in flight
+    for (int i0 = t; i0 < cnt; i0 += 4 * WG) {
+        uint32_t kk[4] = { 0u, 0u, 0u, 0u };
+        int pp[4] = { 0, 0, 0, 0 };
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * WG;
+            if (i < cnt) {
+                kk[u] = D.tmpk[start + i];
+                if (staged) pp[u] = D.tmp[start + i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * WG;
+            if (i < cnt) {
+                atomicAdd(&hist[kk[u] >> idbits], 1);
+                if (staged) {
+                    ska[i] = kk[u];
+                    pa[i] = pp[u];
+                }
+            }
         }
     }
     __syncthreads();
