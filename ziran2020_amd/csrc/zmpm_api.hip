@@ -16,7 +16,7 @@ using namespace zmpm;
 namespace {
 
 constexpr int kMaxMaterials = 256;
-constexpr int kMaxColliders = 16;
+constexpr int kMaxColliders = MAX_COLLIDERS;
 constexpr int kTimedGroups = 5;
 const char* const kTimedNames[kTimedGroups + 1] = { "bin_sort", "p2g", "grid_update", "damage_solve", "g2p", "substep" };
 

@@ -1188,13 +1188,25 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
     const int64_t nn = (int64_t)D.counters[0] * NB;
     const int64_t cn = D.cap_nodes;
-    const bool cg = D.implicit && D.any_damage;
-    for (int64_t node = (int64_t)blockIdx.x * WG + threadIdx.x; node < nn; node += (int64_t)gridDim.x * WG) {
-        if (cg) { // the damage solve's accumulators, filled by k_dtilde
+    // the collision objects in LDS: read from there the per-node loop's loads
+    // cannot alias the kernel's stores (with the table in global memory, stores
+    // of doubles anywhere in the kernel made the compiler load it per node with
+    // vector loads: ~2x slower on large grids)
+    __shared__ ColDev cols[MAX_COLLIDERS];
+    const int ncols = min(D.num_cols, MAX_COLLIDERS);
+    static_assert(sizeof(ColDev) % 8 == 0, "collider record is copied as 8-byte words");
+    for (int e = threadIdx.x; e < ncols * (int)(sizeof(ColDev) / 8); e += WG)
+        reinterpret_cast<uint64_t*>(cols)[e] = reinterpret_cast<const uint64_t*>(D.cols)[e];
+    __syncthreads();
+    if (D.implicit && D.any_damage) {
+        // the damage solve's accumulators, filled by k_dtilde
+        for (int64_t node = (int64_t)blockIdx.x * WG + threadIdx.x; node < nn; node += (int64_t)gridDim.x * WG) {
             D.cgv[node] = 0.0;
             D.cgv[cn + node] = 0.0;
             D.cgv[4 * cn + node] = 0.0;
         }
+    }
+    for (int64_t node = (int64_t)blockIdx.x * WG + threadIdx.x; node < nn; node += (int64_t)gridDim.x * WG) {
         const T m = accValue(D, GM, node, SC_M);
         T f[3];
 #pragma unroll
@@ -1223,8 +1235,8 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
             if (D.flip) D.oldv[a * cn + node] = v; // the grid's v before the update (FLIP)
             vi[a] = v + D.gravity[a] * dt + f[a] * dt_over_mi;
         }
-        for (int c = 0; c < D.num_cols; c++) {
-            const ColDev& co = D.cols[c];
+        for (int c = 0; c < ncols; c++) {
+            const ColDev& co = cols[c];
             if (co.type == ZMPM_GHOST) continue;
             T X[3];
 #pragma unroll

@@ -123,6 +123,8 @@ struct MatDev {
     float snow_lam_lo, snow_lam_hi;
 };
 
+constexpr int MAX_COLLIDERS = 16; // collision objects per handle
+
 struct ColDev {
     int shape, type;
     double origin[3], n[3], bmin[3], bmax[3], b0[3], dbdt[3], friction;
