@@ -499,14 +499,19 @@ def test_autotune_restores_the_state(gpu):
 
 
 # ------------------------------------------------------- implicit damage solve
+@pytest.mark.parametrize("transfer", ["apic", "pic"])
 @pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
-def test_implicit_damage_slab(gpu, oracle, precision):
+def test_implicit_damage_slab(gpu, oracle, precision, transfer):
     # AnisoFractureSimulation with implicit = true: the phase field by the
-    # matrix-free CG solve (AnisoFractureSimulation.h:435-680)
+    # matrix-free CG solve (AnisoFractureSimulation.h:435-680); also under
+    # FLIP_blend_PIC at ratio 0, the transfer of the AnisoMPM demos
     sc = implicit_slab()
     steps = 20
-    g = run(gpu, sc, precision, steps, implicit_damage=True).download()
-    os_ = run(oracle, sc, A.PREC_FP64, steps, implicit_damage=True)
+    kw = dict(implicit_damage=True)
+    if transfer == "pic":
+        kw.update(transfer_scheme=A.FLIP_BLEND_PIC, flip_pic_ratio=0.0)
+    g = run(gpu, sc, precision, steps, **kw).download()
+    os_ = run(oracle, sc, A.PREC_FP64, steps, **kw)
     o = os_.download()
     assert o["d"].max() > 0.5 and oracle.lib.zmpm_oracle_cg_iterations(os_.h) >= 1
     tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
