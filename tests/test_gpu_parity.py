@@ -565,3 +565,17 @@ def test_implicit_damage_many_blocks(gpu, oracle):
         assert normwise(gd[k], od[k]) < FP64_TOL * 10, k
     gg, og = g.grid(), o.grid()
     assert normwise(gg["pf"], og["pf"]) < 1e-8
+
+
+def test_invalid_options_rejected(gpu):
+    # the library validates as the oracle does (tests/test_oracle.py): bad scene
+    # flags at create, plasticity on a model it cannot act on at set_materials
+    sc = small_cube(cells=2)
+    with pytest.raises(A.ZmpmError):
+        sc.make(gpu, A.PREC_FP32, implicit_damage=2)
+    for model, plast in ((A.MODEL_QR_STABLE_NEO_HOOKEAN, A.PLASTICITY_SNOW),
+                         (A.MODEL_EQUATION_OF_STATE, A.PLASTICITY_VON_MISES_STVK_HENCKY),
+                         (A.MODEL_FIXED_COROTATED, 4)):
+        sc.materials = [A.make_material(model, plasticity=plast)]
+        with pytest.raises(A.ZmpmError):
+            sc.make(gpu, A.PREC_FP32)

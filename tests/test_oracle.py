@@ -652,10 +652,27 @@ def test_transfer_options_validated(oracle):
            dict(pic_damping_iteration=2),
            dict(apic_rpic_ratio=1.5),
            dict(rpic_damping_iteration=-1),
-           dict(transfer_scheme=7)]
+           dict(transfer_scheme=7),
+           dict(implicit_damage=2)]
     for kw in bad:
         with pytest.raises(A.ZmpmError):
             sc.make(oracle, **kw)
+
+
+def test_plasticity_model_combinations_validated(oracle):
+    # snow scales the stress by its hardened moduli, which needs a stress linear
+    # in (mu, lambda); every plasticity needs a model with mu
+    sc = _free_cube()
+    for model, plast in ((A.MODEL_QR_STABLE_NEO_HOOKEAN, A.PLASTICITY_SNOW),
+                         (A.MODEL_EQUATION_OF_STATE, A.PLASTICITY_SNOW),
+                         (A.MODEL_EQUATION_OF_STATE, A.PLASTICITY_VON_MISES_STVK_HENCKY),
+                         (A.MODEL_FIXED_COROTATED, 4)):
+        sc.materials = [A.make_material(model, plasticity=plast)]
+        with pytest.raises(A.ZmpmError):
+            sc.make(oracle)
+    for model in (A.MODEL_FIXED_COROTATED, A.MODEL_NEO_HOOKEAN):
+        sc.materials = [A.make_material(model, plasticity=A.PLASTICITY_SNOW)]
+        sc.make(oracle).close()
 
 
 def test_calculate_dt(oracle):
