@@ -176,7 +176,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "particles": scene.n, "dx": scene.dx, "dt": scene.dt,
                    "form": "FP32 particle state, int64 fixed-point P2G", "active_nodes": nn32,
-                   "launch": cfg32, "autotuned": not args.no_autotune},
+                   "launch": cfg32, "autotuned": not args.no_autotune, "options": dict(scene.options)},
     }
     # roofline of the dominant kernel stage (HIP events on the handle's stream)
     stage_ms = {k: v for k, v in st32.items() if k in ("p2g", "g2p", "grid_update")}
