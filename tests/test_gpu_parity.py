@@ -579,3 +579,34 @@ def test_invalid_options_rejected(gpu):
         sc.materials = [A.make_material(model, plasticity=plast)]
         with pytest.raises(A.ZmpmError):
             sc.make(gpu, A.PREC_FP32)
+
+
+def test_implicit_damage_without_damaged_materials(gpu):
+    # no material carries the phase field: the solve is skipped and the
+    # substep is the explicit one (bitwise in the deterministic FP32 form)
+    sc = small_cube(cells=6, seed=4)
+    a = run(gpu, sc, A.PREC_FP32, 6, implicit_damage=True).download()
+    b = run(gpu, sc, A.PREC_FP32, 6).download()
+    for k in ("x", "v", "C", "F"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_implicit_damage_mixed_materials(gpu, oracle, precision):
+    # scenes with and without the phase field in one grid: only the damaged
+    # materials' particles take part in the solve; nodes reached only by the
+    # others keep a zero diagonal
+    sc = scenes.batched_scenes(n_scenes=6, n_range=(3000, 8000), domain_nodes=256, sigma_crit=20.0)
+    sc.materials[1].eta = 1e-3
+    damaged = sc.material == 1
+    F = sc.F.reshape(-1, 3, 3)
+    F[damaged] = np.diag([1.02, 1.0, 1.0]) @ F[damaged]  # pre-stretched: the phase field grows at once
+    sc.F = F.reshape(-1, 9)
+    steps = 10
+    g = run(gpu, sc, precision, steps, implicit_damage=True).download()
+    o = run(oracle, sc, A.PREC_FP64, steps, implicit_damage=True).download()
+    assert o["d"].max() > 0.01
+    assert np.all(o["d"][~damaged] == 0.0)
+    tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+    for k in ("x", "v", "F", "d", "g", "max_dtilde"):
+        assert normwise(g[k], o[k]) < (tol if k not in ("d", "max_dtilde") else 10 * tol), k
