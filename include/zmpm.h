@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define ZMPM_ABI_VERSION 5
+#define ZMPM_ABI_VERSION 6
 
 /* Error model.  Replaces ZIRAN_ASSERT -> std::runtime_error
  * (Lib/Ziran/CS/Util/Debug.h:19-37) and the CFL RestartException
@@ -55,12 +55,16 @@ typedef enum {
  *  QR_STABLE_NEO_HOOKEAN    Physics/ConstitutiveModel/QRStableNeoHookean.h:54-213
  *  NEO_HOOKEAN              Physics/ConstitutiveModel/NeoHookean.cpp:15-48
  *  EQUATION_OF_STATE (J-based helper)  Physics/ConstitutiveModel/EquationOfState.h:47-120,
- *                           MPM/Force/JBasedMpmForceHelper.cpp:28-50,79-94        */
+ *                           MPM/Force/JBasedMpmForceHelper.cpp:28-50,79-94
+ *  QR_ANISOTROPIC           Physics/ConstitutiveModel/QRAnisotropic.h:47-169 (k_shear =
+ *                           mu, k_volume = lambda, k_fiber_isotropic = mu as setParameters
+ *                           :57-64 sets them; k_fiber_0..2 from fiber_stiffness)      */
 typedef enum {
     ZMPM_MODEL_FIXED_COROTATED = 0,
     ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN = 1,
     ZMPM_MODEL_NEO_HOOKEAN = 2,
-    ZMPM_MODEL_EQUATION_OF_STATE = 3
+    ZMPM_MODEL_EQUATION_OF_STATE = 3,
+    ZMPM_MODEL_QR_ANISOTROPIC = 4
 } zmpm_model;
 
 /* Plasticity applied after the F update (MpmSimulationBase::applyPlasticity,
@@ -143,12 +147,14 @@ typedef struct {
  * (Lib/MPM/MpmParticleHandleBase.cpp:175-222). */
 typedef struct {
     int32_t model;           /* zmpm_model                                                */
-    int32_t has_damage;      /* 1: phase-field damage updated each substep                */
-    double mu, lambda;       /* Lame parameters (corotated / QR-NH / NH)                  */
+    int32_t has_damage;      /* 1: phase-field damage updated each substep (QR-NH and
+                                QR-aniso; AnisoFractureSimulation.h:342-363 dispatches no
+                                other model on the path)                                  */
+    double mu, lambda;       /* Lame parameters (corotated / QR-NH / NH / QR-aniso)       */
     double kx, ky;           /* QRStableNeoHookean extra fibre stiffness                  */
     double bulk, gamma;      /* EquationOfState                                           */
     int32_t num_directors;   /* 1 or 2 structural directors a_0                           */
-    int32_t plasticity;      /* zmpm_plasticity (models with mu: corotated, QR-NH, NH)    */
+    int32_t plasticity;      /* zmpm_plasticity (models with mu: all but EOS)             */
     double a0[2][3];         /* structural directors (reference uses unit e1, e2)         */
     double alphas[2];
     double residual_phase;   /* k in g = (1-d)^2 (1-k) + k                                */
@@ -159,10 +165,13 @@ typedef struct {
     double friction_angle;   /* DruckerPragerStvkHencky friction angle (degrees)          */
     double cohesion;         /* DruckerPragerStvkHencky::cohesion                          */
     int32_t volume_correction; /* DruckerPragerStvkHencky::volume_correction              */
-    int32_t reserved;
+    int32_t isotropic;       /* QRAnisotropic::isotropic: every fibre term degraded and
+                                scaleFiberStiffness a no-op (QRAnisotropic.h:96-104,139-147) */
     double critical_compression; /* SnowPlasticity::theta_c (reference default 2e-2)      */
     double critical_stretch;     /* SnowPlasticity::theta_s (7.5e-3)                       */
     double min_jp, max_jp;       /* SnowPlasticity::min_Jp, max_Jp (0.6, 20)               */
+    double fiber_stiffness[3];   /* QRAnisotropic::k_fiber_0..2 (mu each after setParameters,
+                                    times scaleFiberStiffness's factor)                   */
 } zmpm_material;
 
 /* Analytic collision object with rigid translation b(t) = b0 + dbdt * t

@@ -533,6 +533,50 @@ M3 qrStablePiola(const zmpm_material& mat, double g, double kx, double ky, const
     return mmul(mmul(Q, A), mtrans(Rinv));
 }
 
+// QRAnisotropic::updateScratch + firstPiola (QRAnisotropic.h:87-169,
+// QrBasedHelper.h:28-40) with k_shear = mu, k_volume = lambda and fibre
+// stiffnesses kf (QRAnisotropic.h:57-64).  No sign convention on R: the Givens
+// sweep leaves R_00, R_11 >= 0 and R_22 with the sign of det F.  Returns P; J
+// and energy through pointers.
+M3 qrAnisoPiola(const zmpm_material& mat, double g, const double kf[3], const M3& F, double* J_out, double* psi_out)
+{
+    const double kv = mat.lambda, ks = mat.mu;
+    const bool iso = mat.isotropic != 0;
+    M3 Q, R;
+    givensQR(F, Q, R);
+    M3 Rinv = minverse(R);
+    double J = 1;
+    for (int i = 0; i < 3; i++) J *= R.a[i][i];
+    auto sqr = [](double v) { return v * v; };
+    double energy = (J >= 1) ? g * 0.5 * kv * sqr(J - 1) : 0.5 * kv * sqr(J - 1);
+    for (int i = 0; i < 3; i++)
+        for (int j = i + 1; j < 3; j++) energy += g * 0.5 * ks * sqr(R.a[i][j]);
+    for (int i = 0; i < 3; i++) {
+        const double gi = (i == 0 && !iso) ? 1.0 : g; // the first fibre is not degraded
+        energy += gi * 0.5 * kf[i] * sqr(R.a[i][i] - 1);
+    }
+    M3 D = mzero(); // dPsiHat/dR
+    for (int i = 0; i < 3; i++) D.a[i][i] += (J >= 1 ? g : 1.0) * kv * (J - 1) * J / R.a[i][i];
+    for (int i = 0; i < 3; i++)
+        for (int j = i + 1; j < 3; j++) D.a[i][j] += g * ks * R.a[i][j];
+    for (int i = 0; i < 3; i++) {
+        const double gi = (i == 0 && !iso) ? 1.0 : g;
+        D.a[i][i] += gi * kf[i] * (R.a[i][i] - 1);
+    }
+    M3 A = mmul(D, mtrans(R));
+    for (int i = 1; i < 3; i++)
+        for (int j = 0; j < i; j++) A.a[i][j] = A.a[j][i];
+    if (J_out) *J_out = J;
+    if (psi_out) *psi_out = energy;
+    return mmul(mmul(Q, A), mtrans(Rinv));
+}
+
+// Models that carry the phase field (updateDamage, AnisoFractureSimulation.h:342-363).
+bool phaseFieldModel(int model)
+{
+    return model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN || model == ZMPM_MODEL_QR_ANISOTROPIC;
+}
+
 // First Piola stress of a material at F with degradation g.
 M3 firstPiola(const zmpm_material& mat, const M3& F, double g)
 {
@@ -550,6 +594,8 @@ M3 firstPiola(const zmpm_material& mat, const M3& F, double g)
     }
     case ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN:
         return qrStablePiola(mat, g, mat.kx, mat.ky, F, nullptr, nullptr);
+    case ZMPM_MODEL_QR_ANISOTROPIC:
+        return qrAnisoPiola(mat, g, mat.fiber_stiffness, F, nullptr, nullptr);
     case ZMPM_MODEL_NEO_HOOKEAN: {
         // P = tau F^{-T} with tau from NeoHookean::kirchhoff (NeoHookean.cpp:23-48)
         double J = mdet(F);
@@ -582,6 +628,11 @@ double psiOf(const zmpm_material& mat, const M3& F, double g)
     case ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN: {
         double e = 0;
         qrStablePiola(mat, g, mat.kx, mat.ky, F, nullptr, &e);
+        return e;
+    }
+    case ZMPM_MODEL_QR_ANISOTROPIC: {
+        double e = 0;
+        qrAnisoPiola(mat, g, mat.fiber_stiffness, F, nullptr, &e);
         return e;
     }
     case ZMPM_MODEL_NEO_HOOKEAN: { // NeoHookean.cpp:31-35
@@ -671,12 +722,19 @@ inline double macaulay(double x) { return (x + std::fabs(x)) / 2; } // MathTools
 // The crack driving force D~ = zeta <phi - 1>_+ of one particle
 // (AnisoFractureSimulation.h:222-268, and :302-332 for the implicit solve) with
 // constructStructuralTensor (:365-391) and evaluateEffectiveCauchyStress
-// (QRStableNeoHookean.h:83-93).
+// (QRStableNeoHookean.h:83-93, QRAnisotropic.h:79-91).
 double dTildeOf(const zmpm_material& mat, const M3& F)
 {
-    // undegraded isotropic Cauchy stress: g = 1, kx = ky = 0
+    // undegraded isotropic Cauchy stress: g = 1 and kx = ky = 0 (QR-NH) or every
+    // fibre at k_fiber_isotropic = mu (QR-aniso)
     double J = 1;
-    M3 P = qrStablePiola(mat, 1.0, 0.0, 0.0, F, &J, nullptr);
+    M3 P;
+    if (mat.model == ZMPM_MODEL_QR_ANISOTROPIC) {
+        const double kiso[3] = { mat.mu, mat.mu, mat.mu };
+        P = qrAnisoPiola(mat, 1.0, kiso, F, &J, nullptr);
+    }
+    else
+        P = qrStablePiola(mat, 1.0, 0.0, 0.0, F, &J, nullptr);
     M3 cauchy = mmul(P, mtrans(F));
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) cauchy.a[i][j] /= J;
@@ -1295,7 +1353,7 @@ void solveDamageSystem(zmpm_context* h, double dt)
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         const zmpm_material& mt = matOf(h, i);
-        if (!mt.has_damage || mt.model != ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) continue;
+        if (!mt.has_damage || !phaseFieldModel(mt.model)) continue;
         on[i] = 1;
         const M3 F = mload(&h->F[9 * i]);
         vol[i] = h->V0[i] * mdet(F);
@@ -1540,7 +1598,7 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; i++) {
             const zmpm_material& mt = matOf(h, i);
-            if (!mt.has_damage || mt.model != ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) continue;
+            if (!mt.has_damage || !phaseFieldModel(mt.model)) continue;
             damageUpdate(mt, mload(&h->F[9 * i]), h->lap[i], dt, h->d[i], h->g[i]);
         }
     }
@@ -1723,6 +1781,7 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
     if (!h || count <= 0 || count > 256 || !mats) return fail(h, ZMPM_E_ARG, "bad material table");
     for (int32_t i = 0; i < count; i++) {
         const zmpm_material& m = mats[i];
+        if (m.model < 0 || m.model > ZMPM_MODEL_QR_ANISOTROPIC) return fail(h, ZMPM_E_ARG, "unknown constitutive model");
         if (m.plasticity < ZMPM_PLASTICITY_NONE || m.plasticity > ZMPM_PLASTICITY_SNOW)
             return fail(h, ZMPM_E_ARG, "unknown plasticity");
         if (m.plasticity && m.model == ZMPM_MODEL_EQUATION_OF_STATE)

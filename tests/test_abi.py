@@ -29,7 +29,7 @@ def test_hip_library_exports_every_symbol():
     missing = [n for n in declared() if not hasattr(lib, n)]
     assert not missing, missing
     lib.zmpm_abi_version.restype = ctypes.c_int32
-    assert lib.zmpm_abi_version() == 5
+    assert lib.zmpm_abi_version() == 6
     lib.zmpm_build_info.restype = ctypes.c_char_p
     assert b"gfx950" in lib.zmpm_build_info()
 

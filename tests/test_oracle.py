@@ -259,23 +259,64 @@ def _tau_from_strain(model, m, H, g=1.0, dtype=np.float64):
             n = np.sqrt(one + n2m1)
             tau = tau + dtype(k) * n2m1 / ((n + one) * n) * np.outer(f, f)
         return tau
+    if model == A.MODEL_QR_ANISOTROPIC:
+        return _qr_aniso_tau_from_strain(m, H, dtype(g), jm1, J)
     return mu * E + lam * np.log1p(jm1) * I
 
 
-@pytest.mark.parametrize("model", [A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN])
+def _qr_aniso_tau_from_strain(m, H, g, jm1, J, kf=None):
+    """QRAnisotropic's tau = Q A Q^T in H = F - I: Gram-Schmidt Q, R from the
+    columns of F, R_ii - 1 without cancellation, A = v I + g k_s (R R^T - diag R_ii^2)
+    + diag(c_i (R_ii - 1) R_ii)."""
+    dtype = H.dtype.type
+    one = dtype(1)
+    f = np.eye(3, dtype=dtype) + H
+    kf = [dtype(k) for k in (m.fiber_stiffness if kf is None else kf)]
+    kv, ks = dtype(m.lam), dtype(m.mu)
+    n0m1 = dtype(2) * H[0, 0] + H[:, 0] @ H[:, 0]
+    R00 = np.sqrt(one + n0m1)
+    r00m1 = n0m1 / (R00 + one)
+    q0 = f[:, 0] / R00
+    R01, R02 = q0 @ f[:, 1], q0 @ f[:, 2]
+    n1m1 = dtype(2) * H[1, 1] + H[:, 1] @ H[:, 1] - R01 * R01
+    R11 = np.sqrt(one + n1m1)
+    r11m1 = n1m1 / (R11 + one)
+    q1 = (f[:, 1] - R01 * q0) / R11
+    q2 = np.cross(q0, q1)
+    R12 = q1 @ f[:, 2]
+    pr = R00 * R11
+    R22 = J / pr
+    r22m1 = (jm1 - (r00m1 + r11m1 + r00m1 * r11m1)) / pr
+    v = (g if jm1 >= 0 else one) * kv * jm1 * J
+    gks = g * ks
+    c0 = g * kf[0] if m.isotropic else kf[0]
+    Am = np.zeros((3, 3), dtype=dtype)
+    Am[0, 0] = v + gks * (R01 * R01 + R02 * R02) + c0 * r00m1 * R00
+    Am[1, 1] = v + gks * R12 * R12 + g * kf[1] * r11m1 * R11
+    Am[2, 2] = v + g * kf[2] * r22m1 * R22
+    Am[0, 1] = Am[1, 0] = gks * (R01 * R11 + R02 * R12)
+    Am[0, 2] = Am[2, 0] = gks * R02 * R22
+    Am[1, 2] = Am[2, 1] = gks * R12 * R22
+    Q = np.stack([q0, q1, q2], axis=1)
+    return Q @ Am @ Q.T
+
+
+@pytest.mark.parametrize("model", [A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN,
+                                   A.MODEL_QR_ANISOTROPIC])
 def test_stress_from_strain_matches_reference_route(O, model):
     # The rewrite in H = F - I is the same function of F as the reference's
     # route (SVD polar for corotated, QR for QR-NH, log det for NH): checked in
     # double on rotated, stretched, sheared and nearly degenerate F with J > 0.
     rng = np.random.default_rng(21)
-    m = mat(model, kx=500.0, ky=300.0) if model == A.MODEL_QR_STABLE_NEO_HOOKEAN else mat(model)
+    m = mat(model, kx=500.0, ky=300.0, fiber_scale=(2.0, 0.5, 1.5))
+    qr = model in (A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_QR_ANISOTROPIC)
     Fs = [rand_F(rng, 0.3) for _ in range(30)] + [np.eye(3), np.diag([1.2, 1.2, 1.2]), np.diag([1.1, 1.1, 0.9]),
                                                   np.diag([1 + 1e-9, 1, 1 - 1e-9]), np.diag([3.0, 0.5, 0.4])]
     Fs += [np.linalg.qr(rng.normal(size=(3, 3)))[0] @ np.diag(1 + 1e-4 * rng.normal(size=3)) for _ in range(5)]
     for F in Fs:
         if np.linalg.det(F) <= 0.05:
             F = F @ np.diag([1, 1, -1.0])
-        for g in ((1.0, 0.4) if model == A.MODEL_QR_STABLE_NEO_HOOKEAN else (1.0,)):
+        for g in ((1.0, 0.4) if qr else (1.0,)):
             want = piola(O, m, F, g) @ F.T
             got = _tau_from_strain(model, m, F - np.eye(3), g)
             assert np.abs(got - want).max() <= 1e-9 * max(1.0, np.abs(want).max()), (F, g)
@@ -293,6 +334,69 @@ def test_stress_from_strain_keeps_float_precision_at_small_strain(O):
         assert np.abs(got32 - want).max() < 1e-5 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("isotropic", [False, True])
+def test_qr_anisotropic_piola_is_energy_gradient(O, isotropic):
+    # P from the Givens QR route is the derivative of the model's energy, with
+    # unequal fibre stiffnesses, degraded and undegraded, on both sides of J = 1
+    rng = np.random.default_rng(31)
+    m = mat(A.MODEL_QR_ANISOTROPIC, 1e4, 0.3, isotropic=isotropic, fiber_scale=(2.0, 0.5, 1.5))
+    seen = set()
+    for _ in range(30):
+        F = rand_F(rng, 0.25)
+        J = np.linalg.det(F)
+        if J <= 0.2:
+            continue
+        seen.add(J >= 1)
+        for g in (1.0, 0.4):
+            P, G = piola(O, m, F, g), fd_piola(O, m, F, g)
+            assert np.abs(P - G).max() < 1e-5 * max(1, np.abs(P).max()), (isotropic, g)
+    assert seen == {True, False}
+
+
+def test_qr_anisotropic_diagonal_closed_form(O):
+    # F = diag(s): Q = I, R = diag(s), so P = dPsiHat/dR = diag(g_v lambda (J - 1) J / s_i
+    # + c_i (s_i - 1)), c_0 = k_0 undegraded, c_1,2 = g k_1,2; R_22 < 0 included
+    for iso in (False, True):
+        m = mat(A.MODEL_QR_ANISOTROPIC, 2e4, 0.25, isotropic=iso, fiber_scale=(3.0, 0.5, 2.0))
+        kf = np.array(m.fiber_stiffness[:])
+        for sig in ([1.2, 0.9, 1.05], [0.7, 1.3, 1.0], [1.1, 0.9, -0.7], [1.3, 1.2, 1.1]):
+            S = np.array(sig)
+            J = S.prod()
+            for g in (1.0, 0.3):
+                c = g * kf
+                if not iso:
+                    c[0] = kf[0]
+                want = np.diag((g if J >= 1 else 1.0) * m.lam * (J - 1) * J / S + c * (S - 1))
+                assert np.abs(piola(O, m, np.diag(S), g) - want).max() < 1e-9 * max(1.0, np.abs(want).max())
+    # isotropic: scaleFiberStiffness is a no-op, every fibre stays at mu
+    m = mat(A.MODEL_QR_ANISOTROPIC, 2e4, 0.25, isotropic=True, fiber_scale=(3.0, 0.5, 2.0))
+    assert list(m.fiber_stiffness) == [m.mu] * 3
+
+
+def test_qr_anisotropic_stress_from_strain_keeps_float_precision(O):
+    rng = np.random.default_rng(32)
+    m = mat(A.MODEL_QR_ANISOTROPIC, fiber_scale=(2.0, 1.0, 1.0))
+    for _ in range(20):
+        H = 1e-5 * rng.normal(size=(3, 3))
+        want = piola(O, m, np.eye(3) + H, 0.7) @ (np.eye(3) + H).T
+        got32 = _tau_from_strain(A.MODEL_QR_ANISOTROPIC, m, H, 0.7, dtype=np.float32).astype(np.float64)
+        assert np.abs(got32 - want).max() < 1e-5 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("model", [A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_QR_ANISOTROPIC])
+def test_sigma_crit_from_percentage_is_the_energy_at_uniform_stretch(O, model):
+    # getSigmaCrit: the undegraded fibre-free / fibre-isotropic model's energy at (1 + p) I
+    m = mat(model, 1e4, 0.4, kx=700.0, ky=200.0, fiber_scale=(2.0, 1.0, 1.0))
+    dummy = mat(model, 1e4, 0.4)
+    for pct in (0.01, 0.15, -0.05):
+        want = psi(O, dummy, (1 + pct) * np.eye(3), 1.0)
+        assert abs(A.sigma_crit_from_percentage(m, pct) - want) < 1e-9 * max(1.0, want)
+    # QR-aniso: 0.5 lambda ((1+p)^3 - 1)^2 + 1.5 mu p^2
+    if model == A.MODEL_QR_ANISOTROPIC:
+        p_ = 0.15
+        assert abs(A.sigma_crit_from_percentage(m, p_) - (0.5 * m.lam * ((1 + p_) ** 3 - 1) ** 2 + 1.5 * m.mu * p_ ** 2)) < 1e-9
+
+
 def test_corotated_diagonal_closed_form(O):
     m = mat(A.MODEL_FIXED_COROTATED, 2e4, 0.25)
     for sig in ([1.2, 0.9, 1.05], [0.7, 1.3, 1.0], [2.0, 1.0, 0.5]):
@@ -302,11 +406,13 @@ def test_corotated_diagonal_closed_form(O):
         assert np.abs(piola(O, m, np.diag(S)) - want).max() < 1e-9
 
 
-@pytest.mark.parametrize("model", [A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN])
+@pytest.mark.parametrize("model", [A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN,
+                                   A.MODEL_QR_ANISOTROPIC])
 def test_rotation_invariance(O, model):
-    # P(QF) = Q P(F) for isotropic hyperelasticity (QR-NH is left-rotation invariant too)
+    # P(QF) = Q P(F) for isotropic hyperelasticity (the QR models are left-rotation
+    # invariant too: QF has the same R)
     rng = np.random.default_rng(5)
-    m = mat(model)
+    m = mat(model, fiber_scale=(2.0, 0.5, 1.5))
     for _ in range(10):
         F = rand_F(rng)
         Q = np.linalg.qr(rng.normal(size=(3, 3)))[0]
@@ -316,7 +422,8 @@ def test_rotation_invariance(O, model):
 
 
 def test_rest_state_is_stress_free(O):
-    for model in (A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN):
+    for model in (A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN,
+                  A.MODEL_QR_ANISOTROPIC):
         assert np.abs(piola(O, mat(model), np.eye(3))).max() < 1e-10
 
 
@@ -363,6 +470,25 @@ def test_damage_uniaxial_closed_form(O):
     d, g = damage(O, m, np.diag([s, 1, 1]), d0, lap, dt)
     assert abs(d - want_d) < 1e-12
     assert abs(g - ((1 - d) ** 2 * (1 - m.residual_phase) + m.residual_phase)) < 1e-14
+
+
+def test_damage_uniaxial_closed_form_qr_anisotropic(O):
+    # The effective stress of QRAnisotropic resets every fibre to k_fiber_isotropic
+    # = mu and g to 1 (QRAnisotropic.h:79-91): at F = diag(s, 1, 1), Q = I, R = F,
+    # sigma = diag((lambda + mu)(s - 1), lambda (s - 1), lambda (s - 1)), whatever
+    # the material's own fibre stiffnesses.
+    s, alpha = 1.01, -0.9
+    for iso, scale in ((False, (3.0, 0.5, 2.0)), (True, (1.0, 1.0, 1.0))):
+        m = mat(A.MODEL_QR_ANISOTROPIC, 1e4, 0.3, has_damage=True, alphas=(alpha,), sigma_crit=50.0, eta=0.1,
+                zeta=1.0, residual_phase=1e-3, l0=0.01, isotropic=iso, fiber_scale=scale)
+        lam, mu = m.lam, m.mu
+        sxx, syy = (lam + mu) * (s - 1), lam * (s - 1)
+        phi = ((1 + alpha) ** 2 * sxx ** 2 + 2 * syy ** 2) / m.sigma_crit ** 2
+        d0, lap, dt = 0.1, 3.0, 1e-4
+        diff = (1 - d0) * m.zeta * max(phi - 1, 0) - (d0 - m.l0 ** 2 * lap)
+        want_d = min(d0 + dt / m.eta * max(diff, 0), 1.0)
+        d, g = damage(O, m, np.diag([s, 1, 1]), d0, lap, dt)
+        assert want_d > d0 and abs(d - want_d) < 1e-12
 
 
 def test_damage_no_growth_below_threshold_and_clamp(O):

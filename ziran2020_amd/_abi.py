@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-ZMPM_ABI_VERSION = 5
+ZMPM_ABI_VERSION = 6
 
 # zmpm_status
 OK, E_ARG, E_HIP, E_DOMAIN, E_CFL, E_NAN, E_OVERFLOW, E_NOMEM = range(8)
@@ -22,7 +22,8 @@ STATUS_NAMES = {
 }
 
 PREC_FP64, PREC_FP32 = 0, 1
-MODEL_FIXED_COROTATED, MODEL_QR_STABLE_NEO_HOOKEAN, MODEL_NEO_HOOKEAN, MODEL_EQUATION_OF_STATE = range(4)
+MODEL_FIXED_COROTATED, MODEL_QR_STABLE_NEO_HOOKEAN, MODEL_NEO_HOOKEAN, MODEL_EQUATION_OF_STATE, \
+    MODEL_QR_ANISOTROPIC = range(5)
 STICKY, SLIP, SEPARATE, GHOST = 1, 2, 3, 4
 APIC_BLEND_RPIC, FLIP_BLEND_PIC = 0, 1
 PLASTICITY_NONE, PLASTICITY_VON_MISES_STVK_HENCKY, PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY, PLASTICITY_SNOW = 0, 1, 2, 3
@@ -64,9 +65,10 @@ class Material(C.Structure):
         ("l0", C.c_double), ("sigma_crit", C.c_double), ("eta", C.c_double), ("zeta", C.c_double),
         ("yield_stress", C.c_double), ("hardening", C.c_double),
         ("friction_angle", C.c_double), ("cohesion", C.c_double),
-        ("volume_correction", C.c_int32), ("reserved", C.c_int32),
+        ("volume_correction", C.c_int32), ("isotropic", C.c_int32),
         ("critical_compression", C.c_double), ("critical_stretch", C.c_double),
         ("min_jp", C.c_double), ("max_jp", C.c_double),
+        ("fiber_stiffness", D3),
     ]
 
 
@@ -112,11 +114,13 @@ def make_material(model=MODEL_FIXED_COROTATED, E=1e4, nu=0.3, *, has_damage=Fals
                   residual_phase=0.001, l0=0.0, sigma_crit=1.0, eta=0.1, zeta=1.0,
                   plasticity=PLASTICITY_NONE, yield_stress=0.0, hardening=0.0, friction_angle=30.0, cohesion=0.0,
                   volume_correction=True, critical_compression=2e-2, critical_stretch=7.5e-3, min_jp=0.6,
-                  max_jp=20.0) -> Material:
+                  max_jp=20.0, isotropic=False, fiber_scale=(1.0, 1.0, 1.0)) -> Material:
     """Material from Young's modulus and Poisson ratio with the reference's Lame
     conversion (CorotatedElasticity.cpp:15-20, QRStableNeoHookean.h:61-65).  The snow
     plasticity defaults are SnowPlasticity's constructor defaults (PlasticityApplier.h:103;
-    its psi is `hardening`)."""
+    its psi is `hardening`).  QRAnisotropic's fibre stiffnesses are mu times
+    `fiber_scale` (setParameters then scaleFiberStiffness, QRAnisotropic.h:57-64,
+    96-104, which leaves an isotropic model's fibres at mu)."""
     m = Material()
     m.model = model
     m.has_damage = 1 if has_damage else 0
@@ -133,7 +137,27 @@ def make_material(model=MODEL_FIXED_COROTATED, E=1e4, nu=0.3, *, has_damage=Fals
     m.plasticity, m.yield_stress, m.hardening = plasticity, yield_stress, hardening
     m.friction_angle, m.cohesion, m.volume_correction = friction_angle, cohesion, int(bool(volume_correction))
     m.critical_compression, m.critical_stretch, m.min_jp, m.max_jp = critical_compression, critical_stretch, min_jp, max_jp
+    m.isotropic = 1 if isotropic else 0
+    for c in range(3):
+        m.fiber_stiffness[c] = m.mu * (1.0 if isotropic else fiber_scale[c])
     return m
+
+
+def sigma_crit_from_percentage(m: Material, percentage: float) -> float:
+    """The critical stress the reference derives from a stretch percentage when it
+    attaches the phase field: the strain energy density of the undegraded,
+    fibre-free (QR-NH) or fibre-isotropic (QR-aniso) model at F = (1 + p) I
+    (MpmParticleHandleBase.cpp:175-222; QRStableNeoHookean::getSigmaCrit,
+    QRStableNeoHookean.h:66-78; QRAnisotropic::getSigmaCrit, QRAnisotropic.h:65-77).
+    At F = s I the QR factors are Q = I, R = s I and J = s^3."""
+    s = 1.0 + percentage
+    J = s ** 3
+    vol = 0.5 * m.lam * (J - 1.0) ** 2
+    if m.model == MODEL_QR_STABLE_NEO_HOOKEAN:
+        return vol + m.mu / 2.0 * (3.0 * s * s - 3.0) - m.mu * (J - 1.0)
+    if m.model == MODEL_QR_ANISOTROPIC:
+        return vol + 3.0 * 0.5 * m.mu * (s - 1.0) ** 2
+    raise ValueError("sigma_crit_from_percentage: only the QR models define getSigmaCrit")
 
 
 def half_space(origin, normal, type_=STICKY, friction=0.0) -> Collider:

@@ -684,6 +684,10 @@ __device__ __forceinline__ bool vtauSym(const MatDev<float>& mt, const Mat3<floa
         float J;
         qrStableTauH(H, mt.mu, mt.lambda, g, mt.kx, mt.ky, t6, &J);
     }
+    else if (mt.model == ZMPM_MODEL_QR_ANISOTROPIC) {
+        float J;
+        qrAnisoTauH(H, mt.lambda, mt.mu, g, mt.kf[0], mt.kf[1], mt.kf[2], mt.iso != 0, t6, &J);
+    }
     else
         neoHookeanTauH(H, mt.mu, mt.lambda, t6);
 #pragma unroll
@@ -706,9 +710,12 @@ __device__ __forceinline__ bool vtauSym(const MatDev<double>& mt, const Mat3<dou
         Mat3<T> tau;
         if (mt.model == ZMPM_MODEL_FIXED_COROTATED)
             tau = corotatedTau(F, mt.mu, mt.lambda);
-        else if (mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
+        else if (mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN || mt.model == ZMPM_MODEL_QR_ANISOTROPIC) {
             T J, t6[6];
-            qrStableTau(F, mt.mu, mt.lambda, g, mt.kx, mt.ky, t6, &J);
+            if (mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN)
+                qrStableTau(F, mt.mu, mt.lambda, g, mt.kx, mt.ky, t6, &J);
+            else
+                qrAnisoTauH(minusIdentity(F), mt.lambda, mt.mu, g, mt.kf[0], mt.kf[1], mt.kf[2], mt.iso != 0, t6, &J);
             S[0] = V0 * t6[0];
             S[1] = V0 * t6[1];
             S[2] = V0 * t6[2];
@@ -1375,10 +1382,17 @@ __global__ __launch_bounds__(WG) void k_damp_grid(Dev<T, AccT> D)
 template <class T, bool FAST = false>
 __device__ __forceinline__ bool dTildeOf(const MatDev<T>& mt, const Mat3<T>& Fs, T& dTilde)
 {
-    // undegraded Cauchy stress tau(g = 1, no fibres) / J (evaluateEffectiveCauchyStress,
-    // QRStableNeoHookean.h:83-93); tau is exactly symmetric in its closed form
+    // undegraded Cauchy stress tau / J (evaluateEffectiveCauchyStress): g = 1 and no
+    // fibres (QRStableNeoHookean.h:83-93), or every fibre at k_fiber_isotropic = mu
+    // (QRAnisotropic.h:79-91); tau is exactly symmetric in its closed form
     T J, t6[6];
-    if constexpr (sizeof(T) == 4)
+    if (mt.model == ZMPM_MODEL_QR_ANISOTROPIC) {
+        if constexpr (sizeof(T) == 4)
+            qrAnisoTauH(Fs, mt.lambda, mt.mu, 1.f, mt.mu, mt.mu, mt.mu, true, t6, &J);
+        else
+            qrAnisoTauH(minusIdentity(Fs), mt.lambda, mt.mu, T(1), mt.mu, mt.mu, mt.mu, true, t6, &J);
+    }
+    else if constexpr (sizeof(T) == 4)
         qrStableTauH(Fs, mt.mu, mt.lambda, 1.f, 0.f, 0.f, t6, &J);
     else
         qrStableTau(Fs, mt.mu, mt.lambda, T(1), T(0), T(0), t6, &J);
@@ -1699,7 +1713,7 @@ __device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     }
     else {
         const Mat3<T> F = Fn;
-        if (dmg && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
+        if (dmg && phaseFieldModel(mt.model)) {
             if (D.implicit) {
                 // damageG2PHelper (AnisoFractureSimulation.h:644-678): the solve's
                 // change, never healing, clamped at 1
@@ -2028,7 +2042,7 @@ __global__ __launch_bounds__(WG) void k_rare(Dev<T, AccT> D, T dt)
         const MatDev<T>& mt = D.mats[ldAt(D.qmat, s)];
         const bool dmg = D.any_damage && mt.has_damage;
         T d = ldAt(D.P + PD * cap, p), g = ldAt(D.P + PG * cap, p);
-        if (dmg && !D.implicit && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
+        if (dmg && !D.implicit && phaseFieldModel(mt.model)) {
             const Mat3<T> F = loadM(D.P, cap, PF, p); // F_n (FP32 form: H)
             damageUpdate<T, false>(mt, F, ldAt(D.Q + PLAP * cap, s), dt, d, g);
             stAt(D.Q + PD * cap, s, d);
@@ -2353,7 +2367,7 @@ __global__ __launch_bounds__(WG) void k_dtilde(Dev<T, AccT> D, T dt)
                 T kap = T(0);
                 T d0[3] = { T(0), T(0), T(0) };
                 double c3[3] = { 0.0, 0.0, 0.0 };
-                if (mt.has_damage && mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN) {
+                if (mt.has_damage && phaseFieldModel(mt.model)) {
                     const Mat3<T> Fs = loadM(D.P, cap, PF, p);
                     T J;
                     if constexpr (sizeof(T) == 4)

@@ -105,8 +105,10 @@ template <> struct Blk<double> {
 
 template <class T>
 struct MatDev {
-    int model, has_damage, nd, pad;
+    int model, has_damage, nd;
+    int iso;           // QRAnisotropic::isotropic
     T mu, lambda, kx, ky, bulk, gamma;
+    T kf[3];           // QRAnisotropic::k_fiber_0..2
     T a0[2][3];
     T alphas[2];
     T k, l0, sigma_crit, eta, zeta;
@@ -122,6 +124,13 @@ struct MatDev {
     double snow_lo, snow_hi, min_jp, max_jp;
     float snow_lam_lo, snow_lam_hi;
 };
+
+// Models that carry the phase field: updateDamage dispatches to the QR models
+// only (AnisoFractureSimulation.h:342-363).
+__host__ __device__ constexpr bool phaseFieldModel(int model)
+{
+    return model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN || model == ZMPM_MODEL_QR_ANISOTROPIC;
+}
 
 constexpr int MAX_COLLIDERS = 16; // collision objects per handle
 

@@ -582,12 +582,13 @@ __device__ __forceinline__ Mat3<T> neoHookeanTau(const Mat3<T>& F, T mu, T lambd
 // strain to cancellation.  The models are the same functions of F as above.
 
 // J - 1 = det(I + H) - 1 = tr H + (principal 2x2 minors of H) + det H.
-__device__ __forceinline__ float jm1OfH(const Mat3<float>& H)
+template <class T>
+__device__ __forceinline__ T jm1OfH(const Mat3<T>& H)
 {
-    const float h00 = H.a[0][0], h01 = H.a[0][1], h02 = H.a[0][2];
-    const float h10 = H.a[1][0], h11 = H.a[1][1], h12 = H.a[1][2];
-    const float h20 = H.a[2][0], h21 = H.a[2][1], h22 = H.a[2][2];
-    const float m2 = (h00 * h11 - h01 * h10) + (h00 * h22 - h02 * h20) + (h11 * h22 - h12 * h21);
+    const T h00 = H.a[0][0], h01 = H.a[0][1], h02 = H.a[0][2];
+    const T h10 = H.a[1][0], h11 = H.a[1][1], h12 = H.a[1][2];
+    const T h20 = H.a[2][0], h21 = H.a[2][1], h22 = H.a[2][2];
+    const T m2 = (h00 * h11 - h01 * h10) + (h00 * h22 - h02 * h20) + (h11 * h22 - h12 * h21);
     return (h00 + h11 + h22) + m2 + det3(H);
 }
 
@@ -603,6 +604,15 @@ __device__ __forceinline__ void strainOfH(const Mat3<float>& H, float* E)
     E[3] = 2.f * r1[1] + (r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
     E[4] = (r1[2] + r2[1]) + (r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2]);
     E[5] = 2.f * r2[2] + (r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
+}
+
+template <class T>
+__device__ __forceinline__ Mat3<T> minusIdentity(const Mat3<T>& F)
+{
+    Mat3<T> H = F;
+#pragma unroll
+    for (int i = 0; i < 3; i++) H.a[i][i] -= T(1);
+    return H;
 }
 
 __device__ __forceinline__ Mat3<float> identityPlus(const Mat3<float>& H)
@@ -732,6 +742,80 @@ __device__ __forceinline__ void qrStableTauH(const Mat3<float>& H, float mu, flo
             tau[4] += cc * f[1] * f[2];
             tau[5] += cc * f[2] * f[2];
         }
+    }
+    *Jout = J;
+}
+
+// QRAnisotropic Kirchhoff stress tau = P F^T from H = F - I, symmetric (xx xy xz
+// yy yz zz), both forms.  The reference evaluates P = Q A R^{-T} from the Givens
+// QR F = Q R (R_00, R_11 >= 0, R_22 with the sign of det F), A the symmetric
+// matrix whose upper triangle is dPsiHat/dR R^T (QRAnisotropic.h:87-169,
+// QrBasedHelper.h:28-40); so tau = Q A Q^T.  With dPsiHat/dR = diag(v / R_ii +
+// c_i (R_ii - 1)) + g k_s (strict upper triangle of R), v = g_v k_v (J - 1) J,
+// g_v = g for J >= 1 else 1, c_i = g k_i except c_0 = k_0 (undegraded first
+// fibre) when not isotropic:
+//   A = v I + g k_s (R R^T - diag(R_ii^2)) + diag(c_i (R_ii - 1) R_ii),
+// whose entries are all strain-sized.  Q and R are Gram-Schmidt on the columns
+// f_c of F (the Givens sweep's factors: q0 = f0 / |f0|, q1 the unit part of f1
+// normal to q0, q2 = q0 x q1, R_22 = J / (R_00 R_11)), and R_ii - 1 come from H
+// without cancellation:
+//   R_00^2 - 1 = 2 H_00 + |h_0|^2,  R_11^2 - 1 = 2 H_11 + |h_1|^2 - R_01^2,
+//   R_22 - 1 = ((J - 1) - (R_00 R_11 - 1)) / (R_00 R_11).
+template <class T>
+__device__ __forceinline__ void qrAnisoTauH(const Mat3<T>& H, T kv, T ks, T g, T k0, T k1, T k2, bool iso, T* tau,
+                                            T* Jout)
+{
+    const T jm1 = jm1OfH(H);
+    const T J = T(1) + jm1;
+    T h[3][3], f[3][3]; // column c of H and of F
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            h[c][r] = H.a[r][c];
+            f[c][r] = H.a[r][c] + (r == c ? T(1) : T(0));
+        }
+    const T n0m1 = T(2) * h[0][0] + (h[0][0] * h[0][0] + h[0][1] * h[0][1] + h[0][2] * h[0][2]);
+    const T R00 = dsqrt(T(1) + n0m1);
+    const T r00m1 = ddiv(n0m1, R00 + T(1));
+    const T i00 = ddiv(T(1), R00);
+    const T q0[3] = { f[0][0] * i00, f[0][1] * i00, f[0][2] * i00 };
+    const T R01 = q0[0] * f[1][0] + q0[1] * f[1][1] + q0[2] * f[1][2];
+    const T R02 = q0[0] * f[2][0] + q0[1] * f[2][1] + q0[2] * f[2][2];
+    const T n1m1 = T(2) * h[1][1] + (h[1][0] * h[1][0] + h[1][1] * h[1][1] + h[1][2] * h[1][2]) - R01 * R01;
+    const T R11 = dsqrt(T(1) + n1m1);
+    const T r11m1 = ddiv(n1m1, R11 + T(1));
+    const T i11 = ddiv(T(1), R11);
+    const T q1[3] = { (f[1][0] - R01 * q0[0]) * i11, (f[1][1] - R01 * q0[1]) * i11, (f[1][2] - R01 * q0[2]) * i11 };
+    const T q2[3] = { q0[1] * q1[2] - q0[2] * q1[1], q0[2] * q1[0] - q0[0] * q1[2], q0[0] * q1[1] - q0[1] * q1[0] };
+    const T R12 = q1[0] * f[2][0] + q1[1] * f[2][1] + q1[2] * f[2][2];
+    const T pr = R00 * R11;
+    const T prm1 = r00m1 + r11m1 + r00m1 * r11m1;
+    const T ipr = ddiv(T(1), pr);
+    const T R22 = J * ipr;
+    const T r22m1 = (jm1 - prm1) * ipr;
+    const T v = ((jm1 >= T(0)) ? g : T(1)) * kv * jm1 * J;
+    const T gks = g * ks;
+    const T c0 = iso ? g * k0 : k0;
+    T A[3][3];
+    A[0][0] = v + gks * (R01 * R01 + R02 * R02) + c0 * r00m1 * R00;
+    A[1][1] = v + gks * (R12 * R12) + g * k1 * r11m1 * R11;
+    A[2][2] = v + g * k2 * r22m1 * R22;
+    A[0][1] = A[1][0] = gks * (R01 * R11 + R02 * R12);
+    A[0][2] = A[2][0] = gks * (R02 * R22);
+    A[1][2] = A[2][1] = gks * (R12 * R22);
+    const T* q[3] = { q0, q1, q2 };
+    // B = Q A (row a: sum_i q_i[a] A_ij), tau = B Q^T
+    T B[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) B[a][j] = q[0][a] * A[0][j] + q[1][a] * A[1][j] + q[2][a] * A[2][j];
+    const int ia[6] = { 0, 0, 0, 1, 1, 2 }, ib[6] = { 0, 1, 2, 1, 2, 2 };
+#pragma unroll
+    for (int e = 0; e < 6; e++) {
+        const int a = ia[e], b = ib[e];
+        tau[e] = B[a][0] * q[0][b] + B[a][1] * q[1][b] + B[a][2] * q[2][b];
     }
     *Jout = J;
 }

@@ -133,20 +133,23 @@ def elastic_cube(n_side_cells=32, dx=1.0 / 64, ppc_axis=2, seed=1, steps=100, dt
 
 def damage_slab(n=1_000_000, dx=1.0 / 128, size=(1.0, 0.2, 0.2), seed=2, steps=200, dt=1e-4,
                 domain_nodes=256, E=1e4, nu=0.3, rho=1000.0, alpha=-0.9, sigma_crit=5e3,
-                eta=0.1, zeta=1.0, residual=0.001, pull_speed=0.05):
+                eta=0.1, zeta=1.0, residual=0.001, pull_speed=0.05, model=A.MODEL_QR_STABLE_NEO_HOOKEAN,
+                isotropic=False, fiber_scale=(1.0, 1.0, 1.0)):
     """AnisoMPM phase-field damage: n particles uniformly in a slab, per-particle fibre
     +-15 deg about x in F0, A = I + alpha (a x a), l0 = 2 dx, sigma_c = 5e3 Pa,
     QRStableNeoHookean; bottom face held by a fixed sticky box, top face pulled by
-    a sticky box moving +y (examples/paramEffects.h:129-154)."""
+    a sticky box moving +y (examples/paramEffects.h:129-154).  `model` may be the
+    other phase-field model, QRAnisotropic (as examples/orange.h:46-63 uses it, with
+    its `isotropic` flag and fibre stiffness scales)."""
     rng = np.random.default_rng(seed)
     size = np.asarray(size, float)
     lo = np.array([16 * dx, 16 * dx, 16 * dx])
     hi = lo + size
     x = uniform_box(rng, lo, hi, n)
     V0 = np.full(n, float(np.prod(size)) / n)
-    mat = A.make_material(A.MODEL_QR_STABLE_NEO_HOOKEAN, E, nu, has_damage=True, directors=((1, 0, 0),),
+    mat = A.make_material(model, E, nu, has_damage=True, directors=((1, 0, 0),),
                           alphas=(alpha,), residual_phase=residual, l0=2 * dx, sigma_crit=sigma_crit,
-                          eta=eta, zeta=zeta)
+                          eta=eta, zeta=zeta, isotropic=isotropic, fiber_scale=fiber_scale)
     m = 4 * dx
     top = A.box((lo[0] - m, hi[1] - 2 * dx, lo[2] - m), (hi[0] + m, hi[1] + 8 * dx, hi[2] + m),
                 A.STICKY, dbdt=(0.0, pull_speed, 0.0))
