@@ -58,13 +58,16 @@ typedef enum {
  *                           MPM/Force/JBasedMpmForceHelper.cpp:28-50,79-94
  *  QR_ANISOTROPIC           Physics/ConstitutiveModel/QRAnisotropic.h:47-169 (k_shear =
  *                           mu, k_volume = lambda, k_fiber_isotropic = mu as setParameters
- *                           :57-64 sets them; k_fiber_0..2 from fiber_stiffness)      */
+ *                           :57-64 sets them; k_fiber_0..2 from fiber_stiffness)
+ *  NEO_HOOKEAN_BORDEN       Physics/ConstitutiveModel/NeoHookeanBorden.cpp:15-108 (kappa =
+ *                           0.666666 mu + lambda as setLameParameters :15-21 sets it)   */
 typedef enum {
     ZMPM_MODEL_FIXED_COROTATED = 0,
     ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN = 1,
     ZMPM_MODEL_NEO_HOOKEAN = 2,
     ZMPM_MODEL_EQUATION_OF_STATE = 3,
-    ZMPM_MODEL_QR_ANISOTROPIC = 4
+    ZMPM_MODEL_QR_ANISOTROPIC = 4,
+    ZMPM_MODEL_NEO_HOOKEAN_BORDEN = 5
 } zmpm_model;
 
 /* Plasticity applied after the F update (MpmSimulationBase::applyPlasticity,
@@ -147,10 +150,10 @@ typedef struct {
  * (Lib/MPM/MpmParticleHandleBase.cpp:175-222). */
 typedef struct {
     int32_t model;           /* zmpm_model                                                */
-    int32_t has_damage;      /* 1: phase-field damage updated each substep (QR-NH and
-                                QR-aniso; AnisoFractureSimulation.h:342-363 dispatches no
-                                other model on the path)                                  */
-    double mu, lambda;       /* Lame parameters (corotated / QR-NH / NH / QR-aniso)       */
+    int32_t has_damage;      /* 1: phase-field damage updated each substep (QR-NH,
+                                QR-aniso and NH-Borden; AnisoFractureSimulation.h:342-363
+                                dispatches no other model on the path)                    */
+    double mu, lambda;       /* Lame parameters (every model but EOS)                     */
     double kx, ky;           /* QRStableNeoHookean extra fibre stiffness                  */
     double bulk, gamma;      /* EquationOfState                                           */
     int32_t num_directors;   /* 1 or 2 structural directors a_0                           */

@@ -571,10 +571,44 @@ M3 qrAnisoPiola(const zmpm_material& mat, double g, const double kf[3], const M3
     return mmul(mmul(Q, A), mtrans(Rinv));
 }
 
+// NeoHookeanBorden::kirchhoff (NeoHookeanBorden.cpp:78-92) with kappa =
+// 0.666666 mu + lambda (setLameParameters :15-21): tau = mu J^{-2/3} dev(F F^T)
+// + J kappa/2 (J - 1/J) I, the volumetric part degraded only for J >= 1.
+double bordenKappa(const zmpm_material& mat) { return 0.666666 * mat.mu + mat.lambda; }
+
+M3 bordenTau(const zmpm_material& mat, const M3& F, double g)
+{
+    const double J = mdet(F);
+    M3 B = mmul(F, mtrans(F));
+    const double tr3 = (B.a[0][0] + B.a[1][1] + B.a[2][2]) / 3.0;
+    const double sdev = mat.mu * std::pow(J, -2.0 / 3.0);
+    const double vol = J * (bordenKappa(mat) / 2 * (J - 1 / J));
+    const double gd = g, gv = (J >= 1) ? g : 1.0;
+    M3 tau;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            tau.a[i][j] = gd * sdev * (B.a[i][j] - (i == j ? tr3 : 0.0)) + (i == j ? gv * vol : 0.0);
+    return tau;
+}
+
+// NeoHookeanBorden::psi (NeoHookeanBorden.cpp:45-59)
+double bordenPsi(const zmpm_material& mat, const M3& F, double g)
+{
+    const double J = mdet(F);
+    const double a = std::pow(J, -1.0 / 3.0);
+    double trC = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) trC += (a * F.a[i][j]) * (a * F.a[i][j]);
+    const double dev = mat.mu / 2 * (trC - 3);
+    const double vol = bordenKappa(mat) / 2 * ((J * J - 1) / 2 - std::log(J));
+    return (J >= 1) ? g * (vol + dev) : g * dev + vol;
+}
+
 // Models that carry the phase field (updateDamage, AnisoFractureSimulation.h:342-363).
 bool phaseFieldModel(int model)
 {
-    return model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN || model == ZMPM_MODEL_QR_ANISOTROPIC;
+    return model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN || model == ZMPM_MODEL_QR_ANISOTROPIC
+        || model == ZMPM_MODEL_NEO_HOOKEAN_BORDEN;
 }
 
 // First Piola stress of a material at F with degradation g.
@@ -596,6 +630,14 @@ M3 firstPiola(const zmpm_material& mat, const M3& F, double g)
         return qrStablePiola(mat, g, mat.kx, mat.ky, F, nullptr, nullptr);
     case ZMPM_MODEL_QR_ANISOTROPIC:
         return qrAnisoPiola(mat, g, mat.fiber_stiffness, F, nullptr, nullptr);
+    case ZMPM_MODEL_NEO_HOOKEAN_BORDEN: {
+        // P = tau F^{-T} (HyperelasticConstitutiveModel.h:82-87 over kirchhoff)
+        const double J = mdet(F);
+        M3 FinvT = mcofactor(F);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) FinvT.a[i][j] *= 1.0 / J;
+        return mmul(bordenTau(mat, F, g), FinvT);
+    }
     case ZMPM_MODEL_NEO_HOOKEAN: {
         // P = tau F^{-T} with tau from NeoHookean::kirchhoff (NeoHookean.cpp:23-48)
         double J = mdet(F);
@@ -635,6 +677,8 @@ double psiOf(const zmpm_material& mat, const M3& F, double g)
         qrAnisoPiola(mat, g, mat.fiber_stiffness, F, nullptr, &e);
         return e;
     }
+    case ZMPM_MODEL_NEO_HOOKEAN_BORDEN:
+        return bordenPsi(mat, F, g);
     case ZMPM_MODEL_NEO_HOOKEAN: { // NeoHookean.cpp:31-35
         double J = mdet(F), lj = std::log(J), I1 = 0;
         for (int i = 0; i < 3; i++)
@@ -661,7 +705,9 @@ M3 vtauOf(const zmpm_material& mat, const double* Fp, double V0, double g)
         return tau;
     }
     M3 F = mload(Fp);
-    if (mat.model == ZMPM_MODEL_NEO_HOOKEAN) {
+    if (mat.model == ZMPM_MODEL_NEO_HOOKEAN_BORDEN)
+        tau = bordenTau(mat, F, g); // the model's own kirchhoff (FBasedMpmForceHelper.cpp:58)
+    else if (mat.model == ZMPM_MODEL_NEO_HOOKEAN) {
         // NeoHookean::kirchhoff (NeoHookean.cpp:38-43): tau = mu F F^T + (lambda logJ - mu) I
         double J = mdet(F);
         double scale = mat.lambda * std::log(J) - mat.mu;
@@ -728,14 +774,21 @@ double dTildeOf(const zmpm_material& mat, const M3& F)
     // undegraded isotropic Cauchy stress: g = 1 and kx = ky = 0 (QR-NH) or every
     // fibre at k_fiber_isotropic = mu (QR-aniso)
     double J = 1;
-    M3 P;
-    if (mat.model == ZMPM_MODEL_QR_ANISOTROPIC) {
-        const double kiso[3] = { mat.mu, mat.mu, mat.mu };
-        P = qrAnisoPiola(mat, 1.0, kiso, F, &J, nullptr);
+    M3 cauchy;
+    if (mat.model == ZMPM_MODEL_NEO_HOOKEAN_BORDEN) { // NeoHookeanBorden.cpp:96-108
+        J = mdet(F);
+        cauchy = bordenTau(mat, F, 1.0);
     }
-    else
-        P = qrStablePiola(mat, 1.0, 0.0, 0.0, F, &J, nullptr);
-    M3 cauchy = mmul(P, mtrans(F));
+    else {
+        M3 P;
+        if (mat.model == ZMPM_MODEL_QR_ANISOTROPIC) {
+            const double kiso[3] = { mat.mu, mat.mu, mat.mu };
+            P = qrAnisoPiola(mat, 1.0, kiso, F, &J, nullptr);
+        }
+        else
+            P = qrStablePiola(mat, 1.0, 0.0, 0.0, F, &J, nullptr);
+        cauchy = mmul(P, mtrans(F));
+    }
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) cauchy.a[i][j] /= J;
     M3 sym;
@@ -1781,7 +1834,7 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
     if (!h || count <= 0 || count > 256 || !mats) return fail(h, ZMPM_E_ARG, "bad material table");
     for (int32_t i = 0; i < count; i++) {
         const zmpm_material& m = mats[i];
-        if (m.model < 0 || m.model > ZMPM_MODEL_QR_ANISOTROPIC) return fail(h, ZMPM_E_ARG, "unknown constitutive model");
+        if (m.model < 0 || m.model > ZMPM_MODEL_NEO_HOOKEAN_BORDEN) return fail(h, ZMPM_E_ARG, "unknown constitutive model");
         if (m.plasticity < ZMPM_PLASTICITY_NONE || m.plasticity > ZMPM_PLASTICITY_SNOW)
             return fail(h, ZMPM_E_ARG, "unknown plasticity");
         if (m.plasticity && m.model == ZMPM_MODEL_EQUATION_OF_STATE)

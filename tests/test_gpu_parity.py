@@ -142,6 +142,21 @@ def test_qr_anisotropic_damage_slab(gpu, oracle, precision, variant):
         assert normwise(g[k], o[k]) < (tol if k != "d" else 10 * tol), k
 
 
+@pytest.mark.parametrize("implicit", [False, True])
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_neo_hookean_borden_damage_slab(gpu, oracle, precision, implicit):
+    # the slab with NeoHookeanBorden, the third model updateDamage dispatches to
+    sc = scenes.damage_slab(n=20000, dx=1.0 / 64, size=(0.5, 0.15, 0.15), domain_nodes=128, sigma_crit=20.0,
+                            pull_speed=0.5, seed=6, model=A.MODEL_NEO_HOOKEAN_BORDEN, eta=1e-3 if implicit else 0.1)
+    steps = 20
+    g = run(gpu, sc, precision, steps, implicit_damage=implicit).download()
+    o = run(oracle, sc, A.PREC_FP64, steps, implicit_damage=implicit).download()
+    assert o["d"].max() > 0.01, "scene must exercise the damage update"
+    tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+    for k in ("x", "v", "F", "d", "g"):
+        assert normwise(g[k], o[k]) < (tol if k != "d" else 10 * tol), k
+
+
 @pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
 def test_damage_rest_state_stays_undamaged(gpu, oracle, precision):
     # Stress-free rotated particles (F0 = per-particle fibre rotation, no load):

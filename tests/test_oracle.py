@@ -181,7 +181,8 @@ def rand_F(rng, s=0.2):
     return np.eye(3) + s * rng.normal(size=(3, 3))
 
 
-@pytest.mark.parametrize("model", [A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN])
+@pytest.mark.parametrize("model", [A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN,
+                                   A.MODEL_NEO_HOOKEAN_BORDEN])
 def test_piola_is_energy_gradient(O, model):
     rng = np.random.default_rng(3)
     m = mat(model, kx=0.0)
@@ -189,7 +190,7 @@ def test_piola_is_energy_gradient(O, model):
         F = rand_F(rng)
         if np.linalg.det(F) <= 0.2:
             continue
-        for g in ((1.0, 0.4) if model == A.MODEL_QR_STABLE_NEO_HOOKEAN else (1.0,)):
+        for g in ((1.0, 0.4) if model in (A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN_BORDEN) else (1.0,)):
             P, G = piola(O, m, F, g), fd_piola(O, m, F, g)
             assert np.abs(P - G).max() < 1e-5 * max(1, np.abs(P).max()), (model, g)
 
@@ -261,6 +262,12 @@ def _tau_from_strain(model, m, H, g=1.0, dtype=np.float64):
         return tau
     if model == A.MODEL_QR_ANISOTROPIC:
         return _qr_aniso_tau_from_strain(m, H, dtype(g), jm1, J)
+    if model == A.MODEL_NEO_HOOKEAN_BORDEN:
+        # g mu J^{-2/3} dev(E) + g_v kappa/2 (J - 1)(J + 1) I
+        kappa = dtype(0.666666 * m.mu + m.lam)
+        gv = dtype(g) if jm1 >= 0 else one
+        dev = E - np.trace(E) / dtype(3) * I
+        return dtype(g) * mu * J ** dtype(-2.0 / 3.0) * dev + gv * kappa / dtype(2) * jm1 * (dtype(2) + jm1) * I
     return mu * E + lam * np.log1p(jm1) * I
 
 
@@ -302,19 +309,21 @@ def _qr_aniso_tau_from_strain(m, H, g, jm1, J, kf=None):
 
 
 @pytest.mark.parametrize("model", [A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN,
-                                   A.MODEL_QR_ANISOTROPIC])
+                                   A.MODEL_QR_ANISOTROPIC, A.MODEL_NEO_HOOKEAN_BORDEN])
 def test_stress_from_strain_matches_reference_route(O, model):
     # The rewrite in H = F - I is the same function of F as the reference's
     # route (SVD polar for corotated, QR for QR-NH, log det for NH): checked in
     # double on rotated, stretched, sheared and nearly degenerate F with J > 0.
     rng = np.random.default_rng(21)
     m = mat(model, kx=500.0, ky=300.0, fiber_scale=(2.0, 0.5, 1.5))
-    qr = model in (A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_QR_ANISOTROPIC)
+    qr = model in (A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_QR_ANISOTROPIC, A.MODEL_NEO_HOOKEAN_BORDEN)
     Fs = [rand_F(rng, 0.3) for _ in range(30)] + [np.eye(3), np.diag([1.2, 1.2, 1.2]), np.diag([1.1, 1.1, 0.9]),
                                                   np.diag([1 + 1e-9, 1, 1 - 1e-9]), np.diag([3.0, 0.5, 0.4])]
     Fs += [np.linalg.qr(rng.normal(size=(3, 3)))[0] @ np.diag(1 + 1e-4 * rng.normal(size=3)) for _ in range(5)]
     for F in Fs:
         if np.linalg.det(F) <= 0.05:
+            if model == A.MODEL_NEO_HOOKEAN_BORDEN:
+                continue  # J^{-2/3} of an inverted F is not real
             F = F @ np.diag([1, 1, -1.0])
         for g in ((1.0, 0.4) if qr else (1.0,)):
             want = piola(O, m, F, g) @ F.T
@@ -383,7 +392,43 @@ def test_qr_anisotropic_stress_from_strain_keeps_float_precision(O):
         assert np.abs(got32 - want).max() < 1e-5 * np.abs(want).max()
 
 
-@pytest.mark.parametrize("model", [A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_QR_ANISOTROPIC])
+def test_neo_hookean_borden_closed_forms(O):
+    # kappa = 0.666666 mu + lambda (the reference's literal, not 2/3).  A rotated
+    # uniform stretch s Q has dev(B) = 0: tau = g_v kappa/2 (J^2 - 1) I, J = s^3.
+    # A uniaxial stretch diag(s, 1, 1): tau = g mu s^{-2/3} (s^2 - 1) diag(2, -1, -1)/3
+    # + g_v kappa/2 (s^2 - 1) I.
+    m = mat(A.MODEL_NEO_HOOKEAN_BORDEN, 2e4, 0.3)
+    kappa = 0.666666 * m.mu + m.lam
+    rng = np.random.default_rng(33)
+    Q = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    if np.linalg.det(Q) < 0:
+        Q[:, 0] *= -1
+    for s in (1.05, 0.93):
+        for g in (1.0, 0.3):
+            J = s ** 3
+            gv = g if J >= 1 else 1.0
+            tau = piola(O, m, s * Q, g) @ (s * Q).T
+            want = gv * kappa / 2 * (J * J - 1) * np.eye(3)
+            assert np.abs(tau - want).max() < 1e-11 * np.abs(want).max()
+            F = np.diag([s, 1.0, 1.0])
+            gv = g if s >= 1 else 1.0
+            want = g * m.mu * s ** (-2.0 / 3.0) * (s * s - 1) * np.diag([2.0, -1.0, -1.0]) / 3 \
+                + gv * kappa / 2 * (s * s - 1) * np.eye(3)
+            tau = piola(O, m, F, g) @ F.T
+            assert np.abs(tau - want).max() < 1e-11 * np.abs(want).max()
+
+
+def test_neo_hookean_borden_stress_from_strain_keeps_float_precision(O):
+    rng = np.random.default_rng(34)
+    m = mat(A.MODEL_NEO_HOOKEAN_BORDEN)
+    for _ in range(20):
+        H = 1e-5 * rng.normal(size=(3, 3))
+        want = piola(O, m, np.eye(3) + H, 0.6) @ (np.eye(3) + H).T
+        got32 = _tau_from_strain(A.MODEL_NEO_HOOKEAN_BORDEN, m, H, 0.6, dtype=np.float32).astype(np.float64)
+        assert np.abs(got32 - want).max() < 1e-5 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("model", [A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_QR_ANISOTROPIC, A.MODEL_NEO_HOOKEAN_BORDEN])
 def test_sigma_crit_from_percentage_is_the_energy_at_uniform_stretch(O, model):
     # getSigmaCrit: the undegraded fibre-free / fibre-isotropic model's energy at (1 + p) I
     m = mat(model, 1e4, 0.4, kx=700.0, ky=200.0, fiber_scale=(2.0, 1.0, 1.0))
@@ -407,7 +452,7 @@ def test_corotated_diagonal_closed_form(O):
 
 
 @pytest.mark.parametrize("model", [A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN,
-                                   A.MODEL_QR_ANISOTROPIC])
+                                   A.MODEL_QR_ANISOTROPIC, A.MODEL_NEO_HOOKEAN_BORDEN])
 def test_rotation_invariance(O, model):
     # P(QF) = Q P(F) for isotropic hyperelasticity (the QR models are left-rotation
     # invariant too: QF has the same R)
@@ -423,7 +468,7 @@ def test_rotation_invariance(O, model):
 
 def test_rest_state_is_stress_free(O):
     for model in (A.MODEL_FIXED_COROTATED, A.MODEL_QR_STABLE_NEO_HOOKEAN, A.MODEL_NEO_HOOKEAN,
-                  A.MODEL_QR_ANISOTROPIC):
+                  A.MODEL_QR_ANISOTROPIC, A.MODEL_NEO_HOOKEAN_BORDEN):
         assert np.abs(piola(O, mat(model), np.eye(3))).max() < 1e-10
 
 
@@ -489,6 +534,24 @@ def test_damage_uniaxial_closed_form_qr_anisotropic(O):
         want_d = min(d0 + dt / m.eta * max(diff, 0), 1.0)
         d, g = damage(O, m, np.diag([s, 1, 1]), d0, lap, dt)
         assert want_d > d0 and abs(d - want_d) < 1e-12
+
+
+def test_damage_uniaxial_closed_form_neo_hookean_borden(O):
+    # effective stress g = 1 (NeoHookeanBorden.cpp:96-108) at F = diag(s, 1, 1):
+    # sigma = (mu s^{-2/3} (s^2 - 1) diag(2, -1, -1)/3 + kappa/2 (s^2 - 1) I) / s
+    s, alpha = 1.01, -0.9
+    m = mat(A.MODEL_NEO_HOOKEAN_BORDEN, 1e4, 0.3, has_damage=True, alphas=(alpha,), sigma_crit=50.0, eta=0.1,
+            zeta=1.0, residual_phase=1e-3, l0=0.01)
+    kappa = 0.666666 * m.mu + m.lam
+    dev = m.mu * s ** (-2.0 / 3.0) * (s * s - 1) / 3
+    vol = kappa / 2 * (s * s - 1)
+    sxx, syy = (2 * dev + vol) / s, (-dev + vol) / s
+    phi = ((1 + alpha) ** 2 * max(sxx, 0) ** 2 + 2 * max(syy, 0) ** 2) / m.sigma_crit ** 2
+    d0, lap, dt = 0.1, 3.0, 1e-4
+    diff = (1 - d0) * m.zeta * max(phi - 1, 0) - (d0 - m.l0 ** 2 * lap)
+    want_d = min(d0 + dt / m.eta * max(diff, 0), 1.0)
+    d, g = damage(O, m, np.diag([s, 1, 1]), d0, lap, dt)
+    assert want_d > d0 and abs(d - want_d) < 1e-12
 
 
 def test_damage_no_growth_below_threshold_and_clamp(O):

@@ -7,6 +7,7 @@ bind a second library through :class:`ZmpmLib` to compare against it.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -23,7 +24,7 @@ STATUS_NAMES = {
 
 PREC_FP64, PREC_FP32 = 0, 1
 MODEL_FIXED_COROTATED, MODEL_QR_STABLE_NEO_HOOKEAN, MODEL_NEO_HOOKEAN, MODEL_EQUATION_OF_STATE, \
-    MODEL_QR_ANISOTROPIC = range(5)
+    MODEL_QR_ANISOTROPIC, MODEL_NEO_HOOKEAN_BORDEN = range(6)
 STICKY, SLIP, SEPARATE, GHOST = 1, 2, 3, 4
 APIC_BLEND_RPIC, FLIP_BLEND_PIC = 0, 1
 PLASTICITY_NONE, PLASTICITY_VON_MISES_STVK_HENCKY, PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY, PLASTICITY_SNOW = 0, 1, 2, 3
@@ -148,8 +149,10 @@ def sigma_crit_from_percentage(m: Material, percentage: float) -> float:
     attaches the phase field: the strain energy density of the undegraded,
     fibre-free (QR-NH) or fibre-isotropic (QR-aniso) model at F = (1 + p) I
     (MpmParticleHandleBase.cpp:175-222; QRStableNeoHookean::getSigmaCrit,
-    QRStableNeoHookean.h:66-78; QRAnisotropic::getSigmaCrit, QRAnisotropic.h:65-77).
-    At F = s I the QR factors are Q = I, R = s I and J = s^3."""
+    QRStableNeoHookean.h:66-78; QRAnisotropic::getSigmaCrit, QRAnisotropic.h:65-77),
+    or NeoHookeanBorden's energy there at g = 1 (NeoHookeanBorden.cpp:36-43, whose
+    deviatoric part vanishes at F = s I).  At F = s I the QR factors are Q = I,
+    R = s I and J = s^3."""
     s = 1.0 + percentage
     J = s ** 3
     vol = 0.5 * m.lam * (J - 1.0) ** 2
@@ -157,7 +160,10 @@ def sigma_crit_from_percentage(m: Material, percentage: float) -> float:
         return vol + m.mu / 2.0 * (3.0 * s * s - 3.0) - m.mu * (J - 1.0)
     if m.model == MODEL_QR_ANISOTROPIC:
         return vol + 3.0 * 0.5 * m.mu * (s - 1.0) ** 2
-    raise ValueError("sigma_crit_from_percentage: only the QR models define getSigmaCrit")
+    if m.model == MODEL_NEO_HOOKEAN_BORDEN:
+        kappa = 0.666666 * m.mu + m.lam
+        return kappa / 2.0 * ((J * J - 1.0) / 2.0 - math.log(J))
+    raise ValueError("sigma_crit_from_percentage: only the phase-field models define getSigmaCrit")
 
 
 def half_space(origin, normal, type_=STICKY, friction=0.0) -> Collider:

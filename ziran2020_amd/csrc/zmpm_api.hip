@@ -267,6 +267,7 @@ zmpm_status uploadMaterials(zmpm_context* h)
         d.kx = (T)m.kx;
         d.ky = (T)m.ky;
         d.iso = m.isotropic ? 1 : 0;
+        d.kappa = (T)(0.666666 * m.mu + m.lambda); // NeoHookeanBorden.cpp:15-21
         for (int c = 0; c < 3; c++) d.kf[c] = (T)m.fiber_stiffness[c];
         d.bulk = (T)m.bulk;
         d.gamma = (T)m.gamma;
@@ -755,7 +756,7 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
     if (!h) return ZMPM_E_ARG;
     if (count <= 0 || count > kMaxMaterials || !mats) return fail(h, ZMPM_E_ARG, "bad material table");
     for (int i = 0; i < count; i++) {
-        if (mats[i].model < 0 || mats[i].model > ZMPM_MODEL_QR_ANISOTROPIC)
+        if (mats[i].model < 0 || mats[i].model > ZMPM_MODEL_NEO_HOOKEAN_BORDEN)
             return fail(h, ZMPM_E_ARG, "unknown constitutive model");
         if (mats[i].plasticity < ZMPM_PLASTICITY_NONE || mats[i].plasticity > ZMPM_PLASTICITY_SNOW)
             return fail(h, ZMPM_E_ARG, "unknown plasticity");

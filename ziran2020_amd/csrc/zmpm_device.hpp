@@ -688,6 +688,10 @@ __device__ __forceinline__ bool vtauSym(const MatDev<float>& mt, const Mat3<floa
         float J;
         qrAnisoTauH(H, mt.lambda, mt.mu, g, mt.kf[0], mt.kf[1], mt.kf[2], mt.iso != 0, t6, &J);
     }
+    else if (mt.model == ZMPM_MODEL_NEO_HOOKEAN_BORDEN) {
+        float J;
+        bordenTauH(H, mt.mu, mt.kappa, g, t6, &J);
+    }
     else
         neoHookeanTauH(H, mt.mu, mt.lambda, t6);
 #pragma unroll
@@ -710,12 +714,14 @@ __device__ __forceinline__ bool vtauSym(const MatDev<double>& mt, const Mat3<dou
         Mat3<T> tau;
         if (mt.model == ZMPM_MODEL_FIXED_COROTATED)
             tau = corotatedTau(F, mt.mu, mt.lambda);
-        else if (mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN || mt.model == ZMPM_MODEL_QR_ANISOTROPIC) {
+        else if (mt.model != ZMPM_MODEL_NEO_HOOKEAN) {
             T J, t6[6];
             if (mt.model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN)
                 qrStableTau(F, mt.mu, mt.lambda, g, mt.kx, mt.ky, t6, &J);
-            else
+            else if (mt.model == ZMPM_MODEL_QR_ANISOTROPIC)
                 qrAnisoTauH(minusIdentity(F), mt.lambda, mt.mu, g, mt.kf[0], mt.kf[1], mt.kf[2], mt.iso != 0, t6, &J);
+            else
+                bordenTauH(minusIdentity(F), mt.mu, mt.kappa, g, t6, &J);
             S[0] = V0 * t6[0];
             S[1] = V0 * t6[1];
             S[2] = V0 * t6[2];
@@ -1383,14 +1389,21 @@ template <class T, bool FAST = false>
 __device__ __forceinline__ bool dTildeOf(const MatDev<T>& mt, const Mat3<T>& Fs, T& dTilde)
 {
     // undegraded Cauchy stress tau / J (evaluateEffectiveCauchyStress): g = 1 and no
-    // fibres (QRStableNeoHookean.h:83-93), or every fibre at k_fiber_isotropic = mu
-    // (QRAnisotropic.h:79-91); tau is exactly symmetric in its closed form
+    // fibres (QRStableNeoHookean.h:83-93), every fibre at k_fiber_isotropic = mu
+    // (QRAnisotropic.h:79-91), or g = 1 (NeoHookeanBorden.cpp:96-108); tau is
+    // exactly symmetric in its closed form
     T J, t6[6];
     if (mt.model == ZMPM_MODEL_QR_ANISOTROPIC) {
         if constexpr (sizeof(T) == 4)
             qrAnisoTauH(Fs, mt.lambda, mt.mu, 1.f, mt.mu, mt.mu, mt.mu, true, t6, &J);
         else
             qrAnisoTauH(minusIdentity(Fs), mt.lambda, mt.mu, T(1), mt.mu, mt.mu, mt.mu, true, t6, &J);
+    }
+    else if (mt.model == ZMPM_MODEL_NEO_HOOKEAN_BORDEN) { // NeoHookeanBorden.cpp:96-108
+        if constexpr (sizeof(T) == 4)
+            bordenTauH(Fs, mt.mu, mt.kappa, 1.f, t6, &J);
+        else
+            bordenTauH(minusIdentity(Fs), mt.mu, mt.kappa, T(1), t6, &J);
     }
     else if constexpr (sizeof(T) == 4)
         qrStableTauH(Fs, mt.mu, mt.lambda, 1.f, 0.f, 0.f, t6, &J);

@@ -109,6 +109,7 @@ struct MatDev {
     int iso;           // QRAnisotropic::isotropic
     T mu, lambda, kx, ky, bulk, gamma;
     T kf[3];           // QRAnisotropic::k_fiber_0..2
+    T kappa;           // NeoHookeanBorden::kappa
     T a0[2][3];
     T alphas[2];
     T k, l0, sigma_crit, eta, zeta;
@@ -125,11 +126,12 @@ struct MatDev {
     float snow_lam_lo, snow_lam_hi;
 };
 
-// Models that carry the phase field: updateDamage dispatches to the QR models
-// only (AnisoFractureSimulation.h:342-363).
+// Models that carry the phase field: updateDamage dispatches to these three only
+// (AnisoFractureSimulation.h:342-363).
 __host__ __device__ constexpr bool phaseFieldModel(int model)
 {
-    return model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN || model == ZMPM_MODEL_QR_ANISOTROPIC;
+    return model == ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN || model == ZMPM_MODEL_QR_ANISOTROPIC
+        || model == ZMPM_MODEL_NEO_HOOKEAN_BORDEN;
 }
 
 constexpr int MAX_COLLIDERS = 16; // collision objects per handle

@@ -593,17 +593,18 @@ __device__ __forceinline__ T jm1OfH(const Mat3<T>& H)
 }
 
 // E = F F^T - I = H + H^T + H H^T, symmetric (xx xy xz yy yz zz).
-__device__ __forceinline__ void strainOfH(const Mat3<float>& H, float* E)
+template <class T>
+__device__ __forceinline__ void strainOfH(const Mat3<T>& H, T* E)
 {
-    const float* r0 = H.a[0];
-    const float* r1 = H.a[1];
-    const float* r2 = H.a[2];
-    E[0] = 2.f * r0[0] + (r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2]);
+    const T* r0 = H.a[0];
+    const T* r1 = H.a[1];
+    const T* r2 = H.a[2];
+    E[0] = T(2) * r0[0] + (r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2]);
     E[1] = (r0[1] + r1[0]) + (r0[0] * r1[0] + r0[1] * r1[1] + r0[2] * r1[2]);
     E[2] = (r0[2] + r2[0]) + (r0[0] * r2[0] + r0[1] * r2[1] + r0[2] * r2[2]);
-    E[3] = 2.f * r1[1] + (r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
+    E[3] = T(2) * r1[1] + (r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
     E[4] = (r1[2] + r2[1]) + (r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2]);
-    E[5] = 2.f * r2[2] + (r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
+    E[5] = T(2) * r2[2] + (r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
 }
 
 template <class T>
@@ -817,6 +818,39 @@ __device__ __forceinline__ void qrAnisoTauH(const Mat3<T>& H, T kv, T ks, T g, T
         const int a = ia[e], b = ib[e];
         tau[e] = B[a][0] * q[0][b] + B[a][1] * q[1][b] + B[a][2] * q[2][b];
     }
+    *Jout = J;
+}
+
+// J^{-2/3} in double, out of line: inlined, the double pow raises the register
+// count of every kernel that holds the NeoHookeanBorden branch.
+__device__ __noinline__ double powMinusTwoThirds(double J) { return pow(J, -2.0 / 3.0); }
+
+// NeoHookeanBorden Kirchhoff stress from H = F - I, symmetric, both forms
+// (NeoHookeanBorden.cpp:78-92): mu J^{-2/3} dev(F F^T) + J kappa/2 (J - 1/J) I,
+// the deviatoric part times g and the volumetric part times g only for J >= 1.
+// dev(F F^T) = dev(E) and J kappa/2 (J - 1/J) = kappa/2 (J - 1)(J + 1) are
+// strain-sized.
+template <class T>
+__device__ __forceinline__ void bordenTauH(const Mat3<T>& H, T mu, T kappa, T g, T* tau, T* Jout)
+{
+    const T jm1 = jm1OfH(H);
+    const T J = T(1) + jm1;
+    T E[6];
+    strainOfH(H, E);
+    T jpow; // J^{-2/3}
+    if constexpr (sizeof(T) == 4)
+        jpow = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(J) * (-2.f / 3.f));
+    else
+        jpow = powMinusTwoThirds(J);
+    const T sd = g * mu * jpow;
+    const T tr3 = (E[0] + E[3] + E[5]) * (T(1) / T(3));
+    const T vol = ((jm1 >= T(0)) ? g : T(1)) * (kappa * T(0.5)) * jm1 * (T(2) + jm1);
+    tau[0] = sd * (E[0] - tr3) + vol;
+    tau[1] = sd * E[1];
+    tau[2] = sd * E[2];
+    tau[3] = sd * (E[3] - tr3) + vol;
+    tau[4] = sd * E[4];
+    tau[5] = sd * (E[5] - tr3) + vol;
     *Jout = J;
 }
 
