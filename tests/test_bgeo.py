@@ -1,0 +1,103 @@
+"""BGEO point files of the C++ facade (zmpm_bgeo.hpp): the reference's Partio
+output (PartioIO.h:166-200, AnisoVisualizer.h:38-63).  Partio is not in this
+build, so the byte layout is checked against the published BGEO v5 point layout
+restated below; parity with Partio itself is unpinned."""
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "ziran2020_amd", "csrc")
+
+DRIVER = r"""
+#include "zmpm_bgeo.hpp"
+#include <cstdio>
+#include <vector>
+int main(int argc, char** argv)
+{
+    const int n = 5;
+    std::vector<double> x(3 * n), d(n), a(3 * n);
+    for (int i = 0; i < 3 * n; i++) { x[i] = 0.1 * i - 0.3; a[i] = 1.0 / (i + 1); }
+    for (int i = 0; i < n; i++) d[i] = i * 0.25;
+    zmpm::writeBgeoPoints(argv[1], n, x.data(), {});
+    zmpm::writeBgeoPoints(argv[2], n, x.data(), { { "d", 1, d.data() }, { "a_1", 3, a.data() } });
+    zmpm::writeBgeoPoints(argv[3], 0, nullptr, {});
+    try {
+        zmpm::writeBgeoPoints(argv[3], n, x.data(), { { "position", 3, a.data() } });
+        return 2;
+    } catch (const std::invalid_argument&) {}
+    return 0;
+}
+"""
+
+
+def parse_bgeo(path):
+    """Reads a BGEO v5 point file: header, point attribute table, points, extra markers."""
+    b = open(path, "rb").read()
+    magic, ver_c, version = struct.unpack(">icI", b[:9])
+    assert magic == int.from_bytes(b"Bgeo", "big") and ver_c == b"V" and version == 5
+    npts, nprims, npg, nprg, npa, nva, nprima, nattr = struct.unpack(">8i", b[9:41])
+    assert (nprims, npg, nprg, nva, nprima, nattr) == (0, 0, 0, 0, 0, 0)
+    off = 41
+    attrs = []
+    for _ in range(npa):
+        (ln,) = struct.unpack(">h", b[off:off + 2])
+        name = b[off + 2:off + 2 + ln].decode()
+        off += 2 + ln
+        count, typ = struct.unpack(">hi", b[off:off + 6])
+        off += 6
+        defaults = struct.unpack(f">{count}i", b[off:off + 4 * count])
+        off += 4 * count
+        assert typ == 5 and all(v == 0 for v in defaults)
+        attrs.append((name, count))
+    width = 4 + sum(c for _, c in attrs)
+    pts = np.frombuffer(b[off:off + 4 * width * npts], dtype=">f4").reshape(npts, width)
+    off += 4 * width * npts
+    assert b[off:] == b"\x00\xff"
+    return pts, attrs
+
+
+@pytest.fixture(scope="module")
+def files(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    d = tmp_path_factory.mktemp("bgeo")
+    src = d / "drv.cpp"
+    src.write_text(DRIVER)
+    exe = d / "drv"
+    subprocess.run([gxx, "-std=c++17", "-O1", "-I", CSRC, str(src), "-o", str(exe)], check=True)
+    paths = [str(d / "plain.bgeo"), str(d / "attrs.bgeo"), str(d / "empty.bgeo")]
+    subprocess.run([str(exe), *paths], check=True)
+    return paths
+
+
+def test_positions_only(files):
+    pts, attrs = parse_bgeo(files[0])
+    x = (0.1 * np.arange(15) - 0.3).reshape(5, 3)
+    assert attrs == []
+    assert np.array_equal(pts[:, :3], x.astype(np.float32))
+    assert np.all(pts[:, 3] == 1.0)
+
+
+def test_attributes_follow_position(files):
+    pts, attrs = parse_bgeo(files[1])
+    assert attrs == [("d", 1), ("a_1", 3)]
+    assert np.array_equal(pts[:, 4], (np.arange(5) * 0.25).astype(np.float32))
+    assert np.array_equal(pts[:, 5:8], (1.0 / (np.arange(15) + 1)).reshape(5, 3).astype(np.float32))
+
+
+def test_exact_header_bytes(files):
+    b = open(files[0], "rb").read()
+    want = b"Bgeo" + b"V" + struct.pack(">9i", 5, 5, 0, 0, 0, 0, 0, 0, 0)
+    assert b[:len(want)] == want
+    assert len(b) == len(want) + 5 * 16 + 2
+
+
+def test_empty(files):
+    pts, attrs = parse_bgeo(files[2])
+    assert pts.shape == (0, 4) and attrs == []

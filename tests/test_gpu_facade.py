@@ -94,6 +94,10 @@ def test_cpp_restart_matches_continuous_run(tmp_path, fp32):
         assert out.returncode == 0, out.stderr
         runs.append(json.loads(out.stdout.strip().splitlines()[-1]))
     assert (tmp_path / "restart_7.dat").exists()
+    # the positions at the restart point as a BGEO point file (writePartio)
+    from test_bgeo import parse_bgeo
+    pts, attrs = parse_bgeo(str(tmp_path / "partio_7.bgeo"))
+    assert attrs == [] and pts.shape == (runs[0]["n"], 4) and np.all(np.isfinite(pts)) and np.all(pts[:, 3] == 1)
     a, b = runs
     assert a["n"] == b["n"] and abs(a["time"] - b["time"]) < 1e-15
     tol = 1e-12 if not fp32 else 1e-6

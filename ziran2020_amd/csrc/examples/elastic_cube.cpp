@@ -9,7 +9,8 @@
 // max_dt = 1/100 s) instead of the fixed substeps.
 // With a restart directory the fixed-substep run stops halfway, writes its
 // state there (write(outputFileName())), and a second simulation built from the
-// same scene setup restarts from that file and runs the other half.
+// same scene setup restarts from that file and runs the other half.  The
+// positions at the restart point also go to <restart dir>/partio_7.bgeo.
 // prints one JSON line: particle count, centre of mass, mean velocity, substeps.
 #include "zmpm_simulation.hpp"
 
@@ -87,6 +88,7 @@ int main(int argc, char** argv)
         sim.advance(steps / 2, 1e-4);
         sim.frame = 7;
         sim.output_dir = restart_dir;
+        sim.write_partio = true; // partio_7.bgeo beside the state file
         sim.write(sim.outputFileName());
         zmpm::MpmSimulationHip again;
         setup(again, cells, fp32);

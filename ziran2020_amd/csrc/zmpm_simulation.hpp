@@ -8,6 +8,7 @@
 #pragma once
 
 #include "zmpm.h"
+#include "zmpm_bgeo.hpp"
 
 #include <array>
 #include <cstdint>
@@ -238,10 +239,17 @@ public:
     }
     int substeps_last_frame = 0;
 
-    // SimulationBase::simulate (SimulationBase.h:327-354) without restart files.
+    // SimulationBase::simulate (SimulationBase.h:327-354).  With write_frames the
+    // state file of the start frame and of every finished frame is written, as
+    // the reference always does (write(outputFileName()), :332-333 and :319).
+    bool write_frames = false;
     virtual void simulate()
     {
-        while (frame < end_frame) advanceOneFrame();
+        if (write_frames) write(outputFileName());
+        while (frame < end_frame) {
+            advanceOneFrame();
+            if (write_frames) write(outputFileName());
+        }
         sync();
     }
 
@@ -314,6 +322,40 @@ public:
         std::ofstream f(output_dir + "/" + filename, std::ios::binary);
         if (!f) throw ZmpmError(ZMPM_E_ARG, "write: cannot open " + output_dir + "/" + filename);
         writeState(f);
+        if (write_partio) writePartio(output_dir + "/" + outputFileName("partio", ".bgeo"));
+    }
+
+    // MpmSimulationBase::write_partio and writePartio (PartioIO.h:166-200): the
+    // particle positions as a BGEO point file, output_dir/partio_<frame>.bgeo,
+    // next to each state file.
+    bool write_partio = false;
+    virtual void writePartio(const std::string& path)
+    {
+        sync();
+        try {
+            writeBgeoPoints(path, (int64_t)X.size(), X.empty() ? nullptr : X[0].data(), {});
+        }
+        catch (const std::exception& e) {
+            throw ZmpmError(ZMPM_E_ARG, e.what());
+        }
+    }
+
+    // The AnisoMPM visualizer's particle file (aniso_visualize_particles_vec,
+    // AnisoVisualizer.h:38-63, without its fibre attribute): positions, damage
+    // "d" and "laplacian".
+    virtual void writePartioDamage(const std::string& path)
+    {
+        sync();
+        const int64_t n = (int64_t)X.size();
+        if ((int64_t)damage.size() != n || (int64_t)laplacian.size() != n)
+            throw ZmpmError(ZMPM_E_ARG, "writePartioDamage: no phase field");
+        try {
+            writeBgeoPoints(path, n, n ? X[0].data() : nullptr,
+                            { BgeoAttribute { "d", 1, damage.data() }, BgeoAttribute { "laplacian", 1, laplacian.data() } });
+        }
+        catch (const std::exception& e) {
+            throw ZmpmError(ZMPM_E_ARG, e.what());
+        }
     }
     virtual void read(const std::string& filename)
     {
