@@ -8,7 +8,7 @@ moving sticky box, dt=1e-4 (seed 2).  Inputs are uploaded to HBM before the
 timed region; one "step" is one full substep (binning + sort, stress + P2G,
 grid update, G2P + damage + F update).
 
-Prints ONE JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+Prints ONE JSON line.  `--gpus N` with N > 1 exits: the substep runs on one GPU.
 """
 from __future__ import annotations
 
@@ -147,7 +147,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs the substep on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the substep runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 

@@ -301,7 +301,7 @@ zmpm_status zmpm_calculate_dt(zmpm_handle h, double max_dt, double* dt, double* 
  * P2G split cap (most lanes sharing one stencil column of a cell) changes how
  * a cell's particles are grouped into float partial sums before the 64-bit
  * fixed-point accumulation (results deterministic for a given value).
- * No counterpart in the reference (launch configuration of this build). */
+ * No counterpart in the reference (launch configuration of this library). */
 typedef struct {
     int32_t p2g_wgs;     /* in-block sort + P2G: workgroups over grid blocks      */
     int32_t grid_wgs;    /* grid update: workgroups over nodes                    */
@@ -320,7 +320,7 @@ zmpm_status zmpm_set_launch_config(zmpm_handle h, const zmpm_launch_config* cfg)
  * and restored after, so the simulation continues from where it was (the grid
  * and binning queries then describe the tuner's last substep).  The chosen
  * configuration is returned in *cfg (may be NULL).
- * No counterpart in the reference (launch configuration of this build). */
+ * No counterpart in the reference (launch configuration of this library). */
 zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, int32_t tune_split,
                           zmpm_launch_config* cfg);
 

@@ -1,6 +1,6 @@
 """BGEO point files of the C++ facade (zmpm_bgeo.hpp): the reference's Partio
-output (PartioIO.h:166-200, AnisoVisualizer.h:38-63).  Partio is not in this
-build, so the byte layout is checked against the published BGEO v5 point layout
+output (PartioIO.h:166-200, AnisoVisualizer.h:38-63).  Partio is not available
+here, so the byte layout is checked against the published BGEO v5 point layout
 restated below; parity with Partio itself is unpinned."""
 import os
 import shutil

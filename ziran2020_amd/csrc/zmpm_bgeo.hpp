@@ -4,8 +4,8 @@
 // (Partio::write on a "*.bgeo" name, from writePartio, Lib/Ziran/Math/
 // Geometry/PartioIO.h:166-200, called by MpmSimulationBase::writeState,
 // MpmSimulationBase.cpp:1066-1079, and by the AnisoMPM visualizer,
-// Projects/anisofracture/AnisoVisualizer.h:14-63).  Partio is not part of this
-// build; this writer emits the point-only layout Partio's BGEO writer produces:
+// Projects/anisofracture/AnisoVisualizer.h:14-63).  Partio is not linked
+// here; this writer emits the point-only layout Partio's BGEO writer produces:
 //
 //   all integers and floats big-endian
 //   int32 magic ('B' 'g' 'e' 'o'), char 'V', int32 version 5
