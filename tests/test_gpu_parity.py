@@ -609,7 +609,8 @@ def test_invalid_options_rejected(gpu):
         sc.make(gpu, A.PREC_FP32, implicit_damage=2)
     for model, plast in ((A.MODEL_QR_STABLE_NEO_HOOKEAN, A.PLASTICITY_SNOW),
                          (A.MODEL_EQUATION_OF_STATE, A.PLASTICITY_VON_MISES_STVK_HENCKY),
-                         (A.MODEL_FIXED_COROTATED, 4)):
+                         (A.MODEL_NEO_HOOKEAN_BORDEN, A.PLASTICITY_SNOW),
+                         (A.MODEL_FIXED_COROTATED, 4), (6, A.PLASTICITY_NONE)):
         sc.materials = [A.make_material(model, plasticity=plast)]
         with pytest.raises(A.ZmpmError):
             sc.make(gpu, A.PREC_FP32)

@@ -855,7 +855,9 @@ def test_plasticity_model_combinations_validated(oracle):
     for model, plast in ((A.MODEL_QR_STABLE_NEO_HOOKEAN, A.PLASTICITY_SNOW),
                          (A.MODEL_EQUATION_OF_STATE, A.PLASTICITY_SNOW),
                          (A.MODEL_EQUATION_OF_STATE, A.PLASTICITY_VON_MISES_STVK_HENCKY),
-                         (A.MODEL_FIXED_COROTATED, 4)):
+                         (A.MODEL_QR_ANISOTROPIC, A.PLASTICITY_SNOW),
+                         (A.MODEL_NEO_HOOKEAN_BORDEN, A.PLASTICITY_SNOW),
+                         (A.MODEL_FIXED_COROTATED, 4), (6, A.PLASTICITY_NONE), (-1, A.PLASTICITY_NONE)):
         sc.materials = [A.make_material(model, plasticity=plast)]
         with pytest.raises(A.ZmpmError):
             sc.make(oracle)
