@@ -142,6 +142,34 @@ def test_qr_anisotropic_damage_slab(gpu, oracle, precision, variant):
         assert normwise(g[k], o[k]) < (tol if k != "d" else 10 * tol), k
 
 
+def degraded_prestretched_cube(lib, precision, model, isotropic, g=0.3):
+    # a pre-stretched cube at a fixed degradation g (no damage update): QRAnisotropic's
+    # first fibre (stiffness 3 mu) is undegraded unless the model is isotropic
+    sc = small_cube(cells=8)
+    sc.materials = [A.make_material(model, 1e4, 0.3, isotropic=isotropic, fiber_scale=(3.0, 1.0, 1.0))]
+    n = sc.x.shape[0]
+    F = np.tile(np.diag([1.04, 0.98, 1.01]).ravel(), (n, 1))
+    sim = sc.make(lib, precision)
+    sim.upload(sc.x, sc.v, sc.C, F, sc.m, sc.V0, sc.material, g=np.full(n, g))
+    sim.run(10, sc.dt)
+    return sim.download()
+
+
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_qr_anisotropic_degraded_fibre(gpu, oracle, precision):
+    # g < 1 on every particle: the isotropic flag decides whether the first fibre's
+    # term is degraded, a difference far above the FP32 bar
+    outs = {}
+    for iso in (False, True):
+        g = degraded_prestretched_cube(gpu, precision, A.MODEL_QR_ANISOTROPIC, iso)
+        o = degraded_prestretched_cube(oracle, A.PREC_FP64, A.MODEL_QR_ANISOTROPIC, iso)
+        tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+        for k in ("x", "v", "F"):
+            assert normwise(g[k], o[k]) < tol, (iso, k)
+        outs[iso] = o
+    assert normwise(outs[True]["v"], outs[False]["v"]) > 100 * FP32_TOL
+
+
 @pytest.mark.parametrize("implicit", [False, True])
 @pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
 def test_neo_hookean_borden_damage_slab(gpu, oracle, precision, implicit):
