@@ -101,3 +101,42 @@ def test_exact_header_bytes(files):
 def test_empty(files):
     pts, attrs = parse_bgeo(files[2])
     assert pts.shape == (0, 4) and attrs == []
+
+
+SIGMA_DRIVER = r"""
+#include "zmpm_simulation.hpp"
+#include <cstdio>
+int main()
+{
+    const int models[3] = { ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN, ZMPM_MODEL_QR_ANISOTROPIC, ZMPM_MODEL_NEO_HOOKEAN_BORDEN };
+    for (int model : models)
+        for (double p : { 0.01, 0.15, -0.05 }) {
+            zmpm_material m {};
+            m.model = model;
+            m.mu = 3571.4285714285716;
+            m.lambda = 14285.714285714286;
+            std::printf("%d %.17g %.17g\n", model, p, zmpm::sigmaCritFromPercentage(m, p));
+        }
+    return 0;
+}
+"""
+
+
+def test_cpp_sigma_crit_matches_python_helper(tmp_path):
+    # the facade's getSigmaCrit counterpart gives the values of the Python helper,
+    # which tests/test_oracle.py pins to the oracle's energy at (1 + p) I
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    from ziran2020_amd import _abi as A
+    src = tmp_path / "sig.cpp"
+    src.write_text(SIGMA_DRIVER)
+    exe = tmp_path / "sig"
+    subprocess.run([gxx, "-std=c++17", "-O1", "-I", CSRC, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    lines = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    rows = [ln.split() for ln in lines if ln.strip()]
+    assert len(rows) == 9
+    for model, p, v in rows:
+        m = A.make_material(int(model), 1e4, 0.4)
+        assert abs(float(v) - A.sigma_crit_from_percentage(m, float(p))) <= 1e-12 * max(1.0, abs(float(v)))

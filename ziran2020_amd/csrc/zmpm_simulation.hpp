@@ -11,6 +11,7 @@
 #include "zmpm_bgeo.hpp"
 
 #include <array>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -30,6 +31,32 @@ struct RestartException : public std::runtime_error {
     explicit RestartException(int f)
         : std::runtime_error("particle traveled more than a grid cell"), frame(f) {}
 };
+
+// The critical stress the reference derives from a stretch percentage when it
+// attaches the phase field (addFBasedMpmForceWithAnisotropicPhaseField,
+// MpmParticleHandleBase.cpp:175-222): the undegraded model's strain energy at
+// F = (1 + p) I, where Q = I, R = (1 + p) I and J = (1 + p)^3.
+//   QRStableNeoHookean::getSigmaCrit (QRStableNeoHookean.h:66-78), no fibres;
+//   QRAnisotropic::getSigmaCrit (QRAnisotropic.h:65-77), fibres at mu;
+//   NeoHookeanBorden::getSigmaCrit (NeoHookeanBorden.cpp:36-43), g = 1, no
+//   deviatoric energy at a uniform stretch.
+inline double sigmaCritFromPercentage(const zmpm_material& m, double percentage)
+{
+    const double s = 1.0 + percentage, J = s * s * s;
+    const double vol = 0.5 * m.lambda * (J - 1.0) * (J - 1.0);
+    switch (m.model) {
+    case ZMPM_MODEL_QR_STABLE_NEO_HOOKEAN:
+        return vol + m.mu / 2.0 * (3.0 * s * s - 3.0) - m.mu * (J - 1.0);
+    case ZMPM_MODEL_QR_ANISOTROPIC:
+        return vol + 1.5 * m.mu * (s - 1.0) * (s - 1.0);
+    case ZMPM_MODEL_NEO_HOOKEAN_BORDEN: {
+        const double kappa = 0.666666 * m.mu + m.lambda;
+        return kappa / 2.0 * ((J * J - 1.0) / 2.0 - std::log(J));
+    }
+    default:
+        throw std::invalid_argument("sigmaCritFromPercentage: only the phase-field models define getSigmaCrit");
+    }
+}
 
 struct ZmpmError : public std::runtime_error {
     zmpm_status status;
