@@ -1,18 +1,20 @@
 """Benchmark: particle-substeps/s of the explicit APIC MPM substep with AnisoMPM
-damage on one MI355X, FP32 form (value) next to the FP64 plain form.
+damage on one MI355X, FP32 batched form (value) next to the FP64 plain form.
 
-Workload (default): AnisoMPM phase-field damage, 1,000,000 particles in a
-1 x 0.2 x 0.2 m slab, dx = 1/128, QRStableNeoHookean E=1e4 nu=0.3, per-particle
-fibre +-15 deg about x, alpha=-0.9, l0=2dx, sigma_c=5e3 Pa, top face pulled by a
-moving sticky box, dt=1e-4 (seed 2).  Inputs are uploaded to HBM before the
-timed region; one "step" is one full substep (binning + sort, stress + P2G,
-grid update, G2P + damage + F update).
+Workload (default, `batched_scenes`): 64 independent scenes in one launch per
+stage -- cubes of 10k..500k particles (log-uniform, seed 4, 10,366,485 in all),
+odd scenes fixed-corotated, even scenes QRStableNeoHookean with AnisoMPM
+phase-field damage, dx = 1/128, dt = 1e-4, gravity, a sticky floor; every
+substep a HIP-graph replay.  Inputs are uploaded to HBM before the timed
+region; one "step" is one full substep (binning + sort, P2G, grid update, G2P +
+damage + F update + next stress) of every scene.
 
 Prints ONE JSON line.  `--gpus N` with N > 1 exits: the substep runs on one GPU.
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -24,7 +26,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak without packed FP32: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 METRIC = "particle-substeps/sec FP32-batched vs FP64-plain on 1 MI355X; % HBM peak"
+ORACLE_LIB = os.path.join(ROOT, "oracle", "libzmpm_oracle.so")
+
+
+def source_digest():
+    """Digest of the device sources: a traffic or instruction count measured on
+    other kernels is reported as stale, not used."""
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "ziran2020_amd", "csrc")
+    for name in sorted(os.listdir(csrc)):
+        if name.endswith((".hpp", ".hip")):
+            with open(os.path.join(csrc, name), "rb") as f:
+                h.update(name.encode())
+                h.update(f.read())
+    return h.hexdigest()[:16]
 
 
 def algorithmic_bytes(stage, precision, n_particles, n_nodes, damage):
@@ -52,10 +70,50 @@ def algorithmic_bytes(stage, precision, n_particles, n_nodes, damage):
     return per_p * n_particles + per_n * n_nodes
 
 
-def cpu_baseline(scene, budget_s=12.0, max_steps=10):
-    """The FP64 oracle (CPU restatement of the reference step) on the host cores."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Threads the oracle uses: OMP_NUM_THREADS (16 on the GPU box), else the CPUs
+    this process may run on."""
+    if os.environ.get("OMP_NUM_THREADS"):
+        return int(os.environ["OMP_NUM_THREADS"])
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_sample_scene(name, scene, max_particles):
+    """A bounded sample of the workload for the CPU oracle: the smallest scenes of
+    a batch (each exactly as in the full batch), else the workload itself."""
+    from ziran2020_amd import scenes
+    if name == "batched_scenes":
+        full_counts = np.array(scene.info["counts"])
+        order = np.argsort(full_counts, kind="stable")
+        keep, tot = [], 0
+        for s in order:
+            if tot + full_counts[s] > max_particles and keep:
+                break
+            keep.append(int(s))
+            tot += int(full_counts[s])
+        sc = scenes.batched_scenes(keep=keep)
+        return sc, f"{len(keep)} of the 64 scenes ({sc.n} particles, each scene as in the full batch)"
+    return scene, f"the full {scene.n}-particle workload"
+
+
+def time_oracle(scene, threads, budget_s, max_steps):
     from ziran2020_amd import _abi as A
-    lib = A.ZmpmLib(os.path.join(ROOT, "oracle", "libzmpm_oracle.so"))
+    lib = A.ZmpmLib(ORACLE_LIB)
+    lib.lib.zmpm_oracle_set_threads(threads)
     sim = scene.make(lib, A.PREC_FP64)
     t0 = time.perf_counter()
     sim.run(1, scene.dt, 0.0)
@@ -65,10 +123,23 @@ def cpu_baseline(scene, budget_s=12.0, max_steps=10):
     sim.run(steps, scene.dt, scene.dt)
     el = time.perf_counter() - t0
     sim.close()
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": scene.n * steps / el, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} substeps of the full {scene.n}-particle workload after 1 untimed substep, "
-                      f"FP64 oracle (C++/OpenMP restatement of the reference CPU step), {threads} threads"}
+    return scene.n * steps / el, steps
+
+
+def cpu_baseline(name, scene, budget_s=10.0):
+    """The FP64 oracle (C++/OpenMP restatement of the reference CPU step) on the
+    host cores, at the available threads and at 1 thread, on bounded samples."""
+    threads = cpu_threads()
+    sc, what = cpu_sample_scene(name, scene, 1_000_000)
+    v, steps = time_oracle(sc, threads, budget_s, 10)
+    sc1, what1 = cpu_sample_scene(name, scene, 150_000)
+    v1, steps1 = time_oracle(sc1, 1, budget_s, 5)
+    return {"value": v, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} substeps of {what} after 1 untimed substep, FP64 oracle "
+                      f"(C++/OpenMP restatement of the reference CPU step), {threads} threads",
+            "value_1thread": v1,
+            "sample_1thread": f"{steps1} substeps of {what1} after 1 untimed substep, 1 thread",
+            "cpu_model": cpu_model()}
 
 
 def timed(scene, lib, precision, steps, warmup, timing_steps=50, autotune=True):
@@ -112,10 +183,34 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "dispatches": 10}), flush=True)
 
 
-def accuracy_after(scene, lib, steps=100):
+def _field_errors(a, b, keys):
+    errs = {}
+    for k in keys:
+        den = np.abs(b[k]).max()
+        errs[k] = float(np.abs(a[k] - b[k]).max() / den) if den > 0 else float(np.abs(a[k]).max())
+    return errs
+
+
+def _grid_errors(ga, gb):
+    """Normwise error of grid m and mv over the union of the active nodes
+    (matched by node coordinates; a node active in one only counts as 0 in the other)."""
+    def table(g):
+        return {tuple(c): (m, m * v) for c, m, v in zip(g["coords"].tolist(), g["m"], g["v"])}
+    ta, tb = table(ga), table(gb)
+    keys = sorted(set(ta) | set(tb))
+    z = (0.0, np.zeros(3))
+    ma = np.array([ta.get(k, z)[0] for k in keys])
+    mb = np.array([tb.get(k, z)[0] for k in keys])
+    pa = np.array([ta.get(k, z)[1] for k in keys])
+    pb = np.array([tb.get(k, z)[1] for k in keys])
+    return {"m": float(np.abs(ma - mb).max() / np.abs(mb).max()),
+            "mv": float(np.abs(pa - pb).max() / np.abs(pb).max()),
+            "nodes_only_in_one": len(keys) - len(set(ta) & set(tb))}
+
+
+def accuracy_vs_fp64(scene, lib, steps=100):
     """Per-field normwise error (max |a - b| / max |b|) of the FP32 form against
-    the FP64 plain form after exactly `steps` substeps from the same upload (the
-    FP64 form matches the FP64 CPU oracle to 1e-9 in the parity tests)."""
+    the FP64 plain GPU form after exactly `steps` substeps of the full workload."""
     from ziran2020_amd import _abi as A
     out = {}
     for prec in (A.PREC_FP32, A.PREC_FP64):
@@ -123,12 +218,58 @@ def accuracy_after(scene, lib, steps=100):
         sim.run(steps, scene.dt, 0.0)
         out[prec] = sim.download()
         sim.close()
-    errs = {}
-    for k in ("x", "v", "C", "F", "d", "g"):
-        a, b = out[A.PREC_FP32][k], out[A.PREC_FP64][k]
-        den = np.abs(b).max()
-        errs[k] = float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a).max())
-    return errs
+    return _field_errors(out[A.PREC_FP32], out[A.PREC_FP64], ("x", "v", "C", "F", "d", "g"))
+
+
+ORACLE_SAMPLES = {
+    # workload -> (sample scene, description); the oracle runs 100 substeps of it
+    "elastic_cube": (lambda S: S.elastic_cube(), "the full workload"),
+    "damage_slab": (lambda S: S.damage_slab(n=125_000), "the slab at 125,000 particles"),
+    "damage_slab_active": (lambda S: S.damage_slab_active(n=125_000), "the slab at 125,000 particles"),
+    "damage_slab_implicit": (lambda S: S.damage_slab_implicit(n=125_000), "the slab at 125,000 particles"),
+    "sparse_spheres": (lambda S: S.sparse_spheres(n=250_000), "the 20 spheres at 250,000 particles"),
+    "two_material": (lambda S: S.two_material(n=250_000), "the block and column at 250,000 particles"),
+}
+
+
+def accuracy_vs_oracle(name, scene, lib, steps=100):
+    """Per-field normwise error of the FP32 form (GPU) against the FP64 oracle
+    after exactly `steps` substeps from the same upload, on a bounded sample of
+    the workload (the oracle is the restated reference CPU step), including the
+    grid mass and momentum of the last substep."""
+    from ziran2020_amd import _abi as A, scenes
+    if name == "batched_scenes":
+        sc, what = cpu_sample_scene(name, scene, 100_000)
+    else:
+        mk, what = ORACLE_SAMPLES[name]
+        sc = mk(scenes)
+    ref = A.ZmpmLib(ORACLE_LIB)
+    ref.lib.zmpm_oracle_set_threads(cpu_threads())
+    outs, grids = [], []
+    for L, prec in ((lib, A.PREC_FP32), (ref, A.PREC_FP64)):
+        sim = sc.make(L, prec)
+        sim.run(steps, sc.dt, 0.0)
+        outs.append(sim.download())
+        grids.append(sim.grid())
+        sim.close()
+    errs = _field_errors(outs[0], outs[1], ("x", "v", "C", "F", "d", "g"))
+    errs["grid"] = _grid_errors(grids[0], grids[1])
+    return {"substeps": steps, "sample": what, "particles": sc.n, **errs}
+
+
+def profile_entry(path, key, digest):
+    """(value, note) of a per-dispatch counter figure committed under profiles/,
+    None when missing or measured on other kernel sources."""
+    if not os.path.exists(path):
+        return None, "no profile"
+    with open(path) as f:
+        table = json.load(f)
+    e = table.get(key)
+    if not isinstance(e, dict):
+        return None, "no profile for this workload"
+    if e.get("source") != digest:
+        return None, f"stale: measured on kernel sources {e.get('source')}, now {digest}"
+    return e["value"], f"measured at {e.get('commit', '?')}"
 
 
 def main():
@@ -136,9 +277,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="damage_slab")
+    ap.add_argument("--workload", default="batched_scenes")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the 100-substep accuracy runs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain MiB-sized device copy (traffic-counter calibration)")
@@ -155,8 +297,10 @@ def main():
     from ziran2020_amd import _abi as A, scenes
 
     lib = ziran2020_amd.load_library()
-    scene = scenes.WORKLOADS[args.workload]()
+    make = scenes.WORKLOADS[args.workload]
+    scene = make()
     damage = any(m.has_damage for m in scene.materials)
+    digest = source_digest()
 
     el32, st32, nn32, out32 = timed(scene, lib, A.PREC_FP32, args.steps, args.warmup, autotune=not args.no_autotune)
     cfg32 = timed.launch_config
@@ -178,35 +322,44 @@ def main():
                    "form": "FP32 particle state, int64 fixed-point P2G", "active_nodes": nn32,
                    "launch": cfg32, "autotuned": not args.no_autotune, "options": dict(scene.options)},
     }
+    if "counts" in scene.info:
+        result["config"]["scenes"] = len(scene.info["counts"])
     # roofline of the dominant kernel stage (HIP events on the handle's stream)
     stage_ms = {k: v for k, v in st32.items() if k in ("p2g", "g2p", "grid_update")}
     dom = max(stage_ms, key=stage_ms.get)
     byt = algorithmic_bytes(dom, "fp32", scene.n, nn32, damage)
     achieved = byt / (stage_ms[dom] * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get(f"{args.workload}/fp32/{dom}")
-    result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                          "algorithmic_bytes": byt, "avg_ms": stage_ms[dom]}
+    key = f"{args.workload}/fp32/{dom}"
+    traffic, traffic_note = profile_entry(os.path.join(ROOT, "profiles", "traffic.json"), key, digest)
+    valu, valu_note = profile_entry(os.path.join(ROOT, "profiles", "valu.json"), key, digest)
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note, "kernel": dom,
+            "algorithmic_bytes": byt, "avg_ms": stage_ms[dom], "kernel_sources": digest}
+    if valu is not None:
+        # SQ_INSTS_VALU (wavefront instructions per dispatch) x 64 lanes over the
+        # kernel's time, against the VALU issue peak
+        lane_ops = valu * 64 / (stage_ms[dom] * 1e-3)
+        roof["valu"] = {"lane_ops_per_s": lane_ops, "peak": VALU_PEAK_LANE_OPS,
+                        "frac": lane_ops / VALU_PEAK_LANE_OPS, "insts_per_particle": valu * 64 / scene.n,
+                        "note": valu_note}
+        roof["limiter"] = "valu" if lane_ops / VALU_PEAK_LANE_OPS > achieved / HBM_PEAK_GBS else "hbm"
+    else:
+        roof["valu"] = {"note": valu_note}
+    result["roofline"] = roof
     result["stage_ms"] = st32
     if not args.no_fp64:
         el64, st64, nn64, out64 = timed(scene, lib, A.PREC_FP64, args.steps, args.warmup,
                                         autotune=not args.no_autotune)
         v64 = scene.n * args.steps / el64
-        errs = {}
-        for k in ("x", "v", "C", "F", "d"):
-            den = np.abs(out64[k]).max()
-            errs[k] = float(np.abs(out32[k] - out64[k]).max() / den) if den > 0 else float(np.abs(out32[k]).max())
         result["fp64_plain"] = {"value": v64, "ms_per_step": 1e3 * el64 / args.steps, "stage_ms": st64,
                                 "dtype": "fp64", "launch": timed.launch_config}
         result["fp32_over_fp64"] = value / v64
-        result["fp32_vs_fp64_normwise_err"] = {"substeps": args.warmup + args.steps + min(args.steps, 50), **errs}
-        result["fp32_vs_fp64_after_100_substeps"] = accuracy_after(scene, lib, 100)
+        if not args.no_accuracy:
+            result["fp32_vs_fp64_after_100_substeps"] = accuracy_vs_fp64(scene, lib, 100)
+    if not args.no_accuracy:
+        result["fp32_vs_oracle_after_100_substeps"] = accuracy_vs_oracle(args.workload, scene, lib, 100)
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(scene)
+        result["cpu_baseline"] = cpu_baseline(args.workload, scene)
     print(json.dumps(result), flush=True)
 
 

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define ZMPM_ABI_VERSION 6
+#define ZMPM_ABI_VERSION 7
 
 /* Error model.  Replaces ZIRAN_ASSERT -> std::runtime_error
  * (Lib/Ziran/CS/Util/Debug.h:19-37) and the CFL RestartException
@@ -104,11 +104,19 @@ typedef enum {
     ZMPM_GHOST = 4
 } zmpm_collider_type;
 
-/* Level-set shapes: HalfSpace (Math/Geometry/AnalyticLevelSet.cpp:318-363) and
- * AxisAlignedAnalyticBox (AnalyticLevelSet.cpp:412-428 over AnalyticBox :545-600). */
+/* Level-set shapes (in the collision object's material space):
+ *  HALF_SPACE       HalfSpace (Math/Geometry/AnalyticLevelSet.cpp:318-363)
+ *  BOX              AxisAlignedAnalyticBox (AnalyticLevelSet.cpp:412-428 over
+ *                   AnalyticBox :545-600), normal by the reference's AutoDiff rules
+ *  SPHERE           Sphere (AnalyticLevelSet.cpp:446-522): inside when |X - c|^2 <
+ *                   r^2 (strict), normal (X - c)/|X - c|, e_x within 1e-7 of c
+ *  CAPPED_CYLINDER  CappedCylinder (AnalyticLevelSet.h:252-340): radius, height along
+ *                   its local y, its own rotation quaternion and centre            */
 typedef enum {
     ZMPM_SHAPE_HALF_SPACE = 0,
-    ZMPM_SHAPE_BOX = 1
+    ZMPM_SHAPE_BOX = 1,
+    ZMPM_SHAPE_SPHERE = 2,
+    ZMPM_SHAPE_CAPPED_CYLINDER = 3
 } zmpm_shape;
 
 /* Particle-grid transfer scheme; values equal MpmSimulationBase::TransferScheme
@@ -177,18 +185,36 @@ typedef struct {
                                     times scaleFiberStiffness's factor)                   */
 } zmpm_material;
 
-/* Analytic collision object with rigid translation b(t) = b0 + dbdt * t
- * (CollisionObject.h:144-148 setTranslation; R = I, s = 1, omega = 0). */
+/* Analytic collision object (AnalyticCollisionObject, CollisionObject.h:20-165):
+ * a level set in material space X moved rigidly to world space as
+ * x = s(t) R(t) X + b(t), with
+ *   b(t) = b0 + dbdt t                     (setTranslation, CollisionObject.h:144-148)
+ *   R(t) = rotation of the quaternion q(t) = (cos(|w| t/2), sin(|w| t/2) w/|w|) * q0
+ *                                          (setRotation / setAngularVelocity :150-160,
+ *                                           a constant angular velocity w as the
+ *                                           examples/boneTwist.h:144-170 callbacks set)
+ *   s(t) = s0 + dsdt t                     (setScaling :162-166)
+ * and object velocity v = w x (x - b) + (dsdt / s)(x - b) + dbdt
+ * (detectAndResolveCollision, CollisionObject.cpp:341-364).  A zero rotation
+ * quaternion means identity and a zero scale0 means 1, so a zero-initialised
+ * record is a static, unscaled object. */
 typedef struct {
     int32_t shape;           /* zmpm_shape                                                */
     int32_t type;            /* zmpm_collider_type                                        */
-    double origin[3];        /* half-space origin                                          */
+    double origin[3];        /* half-space origin; sphere / capped-cylinder centre        */
     double normal[3];        /* half-space outward normal (normalised by the library)     */
-    double box_min[3];       /* box corners in object space                               */
+    double box_min[3];       /* box corners in material space                             */
     double box_max[3];
     double b0[3];            /* translation at t = 0                                      */
     double dbdt[3];          /* translation velocity                                      */
     double friction;         /* SLIP / SEPARATE kinematic friction                        */
+    double radius;           /* sphere, capped cylinder                                   */
+    double height;           /* capped cylinder, along its local y axis                   */
+    double shape_rotation[4];/* capped cylinder's own rotation (w, x, y, z); 0 = identity */
+    double rotation[4];      /* object rotation q0 at t = 0 (w, x, y, z); 0 = identity    */
+    double omega[3];         /* object angular velocity                                   */
+    double scale0;           /* object scaling at t = 0 (0 = 1)                           */
+    double dsdt;             /* object scaling rate                                       */
 } zmpm_collider;
 
 /* Host particle arrays (caller-owned).  Stands for the Particles/DataManager

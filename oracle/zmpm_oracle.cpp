@@ -32,6 +32,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <omp.h>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -958,6 +959,231 @@ struct Collider {
     double n[3]; // normalised half-space normal
 };
 
+// ------------------------------------------------------- collision objects
+// Eigen's Quaternion::toRotationMatrix for q = (w, x, y, z); an all-zero q
+// stands for the identity.
+M3 quatToMatrix(const double* q)
+{
+    double w = q[0], x = q[1], y = q[2], z = q[3];
+    if (w == 0 && x == 0 && y == 0 && z == 0) w = 1;
+    const double tx = 2 * x, ty = 2 * y, tz = 2 * z;
+    const double twx = tx * w, twy = ty * w, twz = tz * w;
+    const double txx = tx * x, txy = ty * x, txz = tz * x;
+    const double tyy = ty * y, tyz = tz * y, tzz = tz * z;
+    M3 R;
+    R.a[0][0] = 1 - (tyy + tzz);
+    R.a[0][1] = txy - twz;
+    R.a[0][2] = txz + twy;
+    R.a[1][0] = txy + twz;
+    R.a[1][1] = 1 - (txx + tzz);
+    R.a[1][2] = tyz - twx;
+    R.a[2][0] = txz - twy;
+    R.a[2][1] = tyz + twx;
+    R.a[2][2] = 1 - (txx + tyy);
+    return R;
+}
+
+// State of a collision object at time t, as the updateState callbacks of the
+// examples set it (examples/boneTwist.h:144-170): b = b0 + dbdt t, a rotation
+// q(t) = (cos(|w| t / 2), sin(|w| t / 2) w / |w|) * q0 about a constant angular
+// velocity w, s = s0 + dsdt t (AnalyticCollisionObject::setTranslation /
+// setRotation / setAngularVelocity / setScaling, CollisionObject.h:144-166).
+struct ColliderAt {
+    M3 R;
+    double b[3], s;
+};
+
+ColliderAt colliderAt(const zmpm_collider& c, double t)
+{
+    ColliderAt st;
+    for (int a = 0; a < 3; a++) st.b[a] = c.b0[a] + c.dbdt[a] * t;
+    st.s = (c.scale0 == 0 ? 1.0 : c.scale0) + c.dsdt * t;
+    double q0[4] = { c.rotation[0], c.rotation[1], c.rotation[2], c.rotation[3] };
+    if (q0[0] == 0 && q0[1] == 0 && q0[2] == 0 && q0[3] == 0) q0[0] = 1;
+    const double wn = std::sqrt(c.omega[0] * c.omega[0] + c.omega[1] * c.omega[1] + c.omega[2] * c.omega[2]);
+    double q[4] = { q0[0], q0[1], q0[2], q0[3] };
+    if (wn > 0) {
+        const double ha = 0.5 * wn * t, cw = std::cos(ha), sw = std::sin(ha) / wn;
+        const double p[4] = { cw, sw * c.omega[0], sw * c.omega[1], sw * c.omega[2] };
+        // Hamilton product p * q0
+        q[0] = p[0] * q0[0] - p[1] * q0[1] - p[2] * q0[2] - p[3] * q0[3];
+        q[1] = p[0] * q0[1] + p[1] * q0[0] + p[2] * q0[3] - p[3] * q0[2];
+        q[2] = p[0] * q0[2] - p[1] * q0[3] + p[2] * q0[0] + p[3] * q0[1];
+        q[3] = p[0] * q0[3] + p[1] * q0[2] - p[2] * q0[1] + p[3] * q0[0];
+    }
+    st.R = quatToMatrix(q);
+    return st;
+}
+
+// AnalyticLevelSet::queryInside in material space X: inside flag, signed
+// distance and the material-space normal N.
+//  half space  HalfSpace::signedDistance / normal (AnalyticLevelSet.cpp:331-347)
+//  box         AxisAlignedAnalyticBox -> AnalyticBox::signedDistancePrimitive and its
+//              AutoDiff normal (AnalyticLevelSet.cpp:412-428, 562-588): d = |X - c| - h;
+//              inside (max d <= 0) the normal is sign(X_a - c_a) e_a of the first
+//              largest d_a (Eigen's maxCoeff keeps the first; std::min(dd, 0) keeps dd
+//              at dd = 0; AutoDiff abs takes sign +1 at 0, AutoDiff.h:307-311, and the
+//              norm of the clamped d has derivative 0 at 0, AutoDiff.h:319-324);
+//              outside it is sum_a max(d_a, 0) sign(X_a - c_a) e_a / |max(d, 0)|
+//  sphere      Sphere::queryInside (AnalyticLevelSet.cpp:494-511): inside when
+//              |X - c|^2 < r^2, N = (X - c)/|X - c|, e_x when |X - c| < 1e-7
+//  cylinder    CappedCylinder::signedDistancePrimitive and its AutoDiff normal
+//              (AnalyticLevelSet.h:290-320) in the primitive frame R_c^-1 (X - b_c):
+//              q = (|(x, z)|, y), d = |q| - (r, h/2); inside the normal is the
+//              gradient of the larger d (d_0 on a tie), outside that of |max(d, 0)|
+bool levelSetInside(const Collider& co, const double X[3], double& phi, double N[3])
+{
+    const zmpm_collider& c = co.c;
+    N[0] = N[1] = N[2] = 0;
+    if (c.shape == ZMPM_SHAPE_HALF_SPACE) {
+        phi = co.n[0] * (X[0] - c.origin[0]) + co.n[1] * (X[1] - c.origin[1]) + co.n[2] * (X[2] - c.origin[2]);
+        for (int a = 0; a < 3; a++) N[a] = co.n[a];
+        return phi <= 0;
+    }
+    if (c.shape == ZMPM_SHAPE_BOX) {
+        double dd[3], Xp[3];
+        for (int a = 0; a < 3; a++) {
+            const double cen = (c.box_min[a] + c.box_max[a]) / 2;
+            const double he = (c.box_max[a] - c.box_min[a]) / 2;
+            Xp[a] = X[a] - cen;
+            dd[a] = std::fabs(Xp[a]) - he;
+        }
+        int am = 0;
+        for (int a = 1; a < 3; a++)
+            if (dd[a] > dd[am]) am = a;
+        double q2 = 0;
+        for (int a = 0; a < 3; a++) q2 += dd[a] > 0 ? dd[a] * dd[a] : 0.0;
+        phi = std::min(dd[am], 0.0) + std::sqrt(q2);
+        if (q2 > 0) {
+            const double qn = std::sqrt(q2);
+            for (int a = 0; a < 3; a++) N[a] = (dd[a] > 0 ? dd[a] / qn : 0.0) * (Xp[a] < 0 ? -1.0 : 1.0);
+        }
+        else
+            N[am] = Xp[am] < 0 ? -1.0 : 1.0;
+        return phi <= 0;
+    }
+    if (c.shape == ZMPM_SHAPE_SPHERE) {
+        double tc[3], d2 = 0;
+        for (int a = 0; a < 3; a++) {
+            tc[a] = X[a] - c.origin[a];
+            d2 += tc[a] * tc[a];
+        }
+        const double dist = std::sqrt(d2);
+        phi = dist - c.radius;
+        if (dist < 1e-7)
+            N[0] = 1;
+        else
+            for (int a = 0; a < 3; a++) N[a] = (1 / dist) * tc[a];
+        return d2 < c.radius * c.radius;
+    }
+    // capped cylinder
+    const M3 Rc = quatToMatrix(c.shape_rotation);
+    const M3 Ri = minverse(Rc);
+    double Y[3], Xp[3];
+    for (int a = 0; a < 3; a++) Y[a] = X[a] - c.origin[a];
+    for (int a = 0; a < 3; a++) Xp[a] = Ri.a[a][0] * Y[0] + Ri.a[a][1] * Y[1] + Ri.a[a][2] * Y[2];
+    const double rxz = std::sqrt(Xp[0] * Xp[0] + Xp[2] * Xp[2]);
+    const double d0 = rxz - c.radius, d1 = std::fabs(Xp[1]) - 0.5 * c.height;
+    // gradients in the primitive frame
+    double g0[3] = { 0, 0, 0 }, g1[3] = { 0, Xp[1] < 0 ? -1.0 : 1.0, 0 };
+    if (rxz != 0) {
+        g0[0] = Xp[0] / rxz;
+        g0[2] = Xp[2] / rxz;
+    }
+    const double q0 = std::max(d0, 0.0), q1 = std::max(d1, 0.0);
+    const double mx = std::max(d0, d1);
+    const double qn = std::sqrt(q0 * q0 + q1 * q1);
+    phi = std::min(mx, 0.0) + qn;
+    double n[3] = { 0, 0, 0 };
+    if (!(0 < mx)) { // min(max(d0, d1), 0) = max(d0, d1), whose gradient is d0's on a tie
+        const double* g = (d0 < d1) ? g1 : g0;
+        for (int a = 0; a < 3; a++) n[a] = g[a];
+    }
+    if (qn != 0)
+        for (int a = 0; a < 3; a++) n[a] += ((d0 < 0 ? 0.0 : d0) * g0[a] + (d1 < 0 ? 0.0 : d1) * g1[a]) / qn;
+    for (int a = 0; a < 3; a++) N[a] = Rc.a[a][0] * n[0] + Rc.a[a][1] * n[1] + Rc.a[a][2] * n[2];
+    return phi <= 0;
+}
+
+// AnalyticCollisionObject::detectAndResolveCollision (CollisionObject.cpp:341-406)
+// for the object state st: true when x is inside; vi resolved; n the world normal
+// of a SLIP / SEPARATE object.
+bool detectAndResolve(const Collider& co, const ColliderAt& st, const double* xi, double* vi, double* n)
+{
+    const zmpm_collider& c = co.c;
+    double xb[3], X[3];
+    for (int a = 0; a < 3; a++) xb[a] = xi[a] - st.b[a];
+    const double os = 1 / st.s;
+    for (int a = 0; a < 3; a++) X[a] = (st.R.a[0][a] * xb[0] + st.R.a[1][a] * xb[1] + st.R.a[2][a] * xb[2]) * os;
+    double phi, N[3];
+    if (!levelSetInside(co, X, phi, N)) return false;
+    // v_object = omega x (x - b) + (dsdt / s)(x - b) + dbdt
+    double vo[3];
+    const double* w = c.omega;
+    const double cr[3] = { w[1] * xb[2] - w[2] * xb[1], w[2] * xb[0] - w[0] * xb[2], w[0] * xb[1] - w[1] * xb[0] };
+    for (int a = 0; a < 3; a++) vo[a] = cr[a] + (c.dsdt * os) * xb[a] + c.dbdt[a];
+    for (int a = 0; a < 3; a++) vi[a] -= vo[a];
+    if (c.type == ZMPM_STICKY)
+        vi[0] = vi[1] = vi[2] = 0;
+    else {
+        for (int a = 0; a < 3; a++) n[a] = st.R.a[a][0] * N[0] + st.R.a[a][1] * N[1] + st.R.a[a][2] * N[2];
+        const double dot = vi[0] * n[0] + vi[1] * n[1] + vi[2] * n[2];
+        if (c.type == ZMPM_SLIP || dot < 0) {
+            for (int a = 0; a < 3; a++) vi[a] -= n[a] * dot;
+            // kinematic friction, only against an approaching velocity
+            if (c.friction != 0 && dot < 0) {
+                const double vn = std::sqrt(vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]);
+                if (-dot * c.friction < vn)
+                    for (int a = 0; a < 3; a++) vi[a] += (vi[a] / vn) * dot * c.friction;
+                else
+                    vi[0] = vi[1] = vi[2] = 0;
+            }
+        }
+    }
+    for (int a = 0; a < 3; a++) vi[a] += vo[a];
+    return true;
+}
+
+// AnalyticCollisionObject::multiObjectCollision (CollisionObject.cpp:66-105) at
+// time t: objects in insertion order; the first STICKY hit ends the loop; the
+// normals of SLIP / SEPARATE hits are Gram-Schmidt orthonormalised into a basis
+// and the loop ends once it holds 3 of them (a hit whose normal lies in the span
+// of the earlier ones adds none).
+// The objects' states at the substep's time (the reference's updateState at
+// reinitialize, MpmSimulationBase.cpp:209-213).
+std::vector<ColliderAt> collidersAt(const std::vector<Collider>& cols, double t)
+{
+    std::vector<ColliderAt> st;
+    for (const Collider& co : cols) st.push_back(colliderAt(co.c, t));
+    return st;
+}
+
+bool multiObjectCollision(const std::vector<Collider>& cols, const std::vector<ColliderAt>& states, const double* xi,
+                          double* vi)
+{
+    bool any = false;
+    int count = 0;
+    double basis[3][3] = { { 0, 0, 0 }, { 0, 0, 0 }, { 0, 0, 0 } };
+    for (size_t k = 0; k < cols.size(); k++) {
+        const Collider& co = cols[k];
+        if (co.c.type == ZMPM_GHOST) continue;
+        double n[3] = { 0, 0, 0 };
+        if (!detectAndResolve(co, states[k], xi, vi, n)) continue;
+        any = true;
+        if (co.c.type == ZMPM_STICKY) break;
+        for (int k = 0; k < count; k++) {
+            const double dot = basis[k][0] * n[0] + basis[k][1] * n[1] + basis[k][2] * n[2];
+            for (int a = 0; a < 3; a++) n[a] -= dot * basis[k][a];
+        }
+        const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (len != 0) {
+            for (int a = 0; a < 3; a++) basis[count][a] = n[a] / len;
+            if (++count == 3) break;
+        }
+    }
+    return any;
+}
+
 } // namespace
 
 // --------------------------------------------------------------- the handle
@@ -1661,6 +1887,7 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
     {
         double dvg[3] = { h->desc.gravity[0] * dt, h->desc.gravity[1] * dt, h->desc.gravity[2] * dt };
         const std::vector<Collider>& cols = h->colliders;
+        const std::vector<ColliderAt> states = collidersAt(cols, t);
         const int NB = h->NB;
         const Layout L = h->L;
 #pragma omp parallel for schedule(static)
@@ -1674,63 +1901,7 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
                 double dt_over_mi = dt / g.m;
                 double vi[3];
                 for (int a = 0; a < 3; a++) vi[a] = g.v[a] + dvg[a] + g.new_v[a] * dt_over_mi;
-                for (const Collider& co : cols) {
-                    if (co.c.type == ZMPM_GHOST) continue;
-                    // detectAndResolveCollision (CollisionObject.cpp:341-406) with R = I, s = 1, omega = 0
-                    double bt[3], X[3];
-                    for (int a = 0; a < 3; a++) {
-                        bt[a] = co.c.b0[a] + co.c.dbdt[a] * t;
-                        X[a] = xi[a] - bt[a];
-                    }
-                    double phi, nrm[3] = { 0, 0, 0 };
-                    if (co.c.shape == ZMPM_SHAPE_HALF_SPACE) {
-                        phi = co.n[0] * (X[0] - co.c.origin[0]) + co.n[1] * (X[1] - co.c.origin[1]) + co.n[2] * (X[2] - co.c.origin[2]);
-                        for (int a = 0; a < 3; a++) nrm[a] = co.n[a];
-                    }
-                    else {
-                        // AnalyticBox::signedDistancePrimitive (AnalyticLevelSet.cpp:562-580)
-                        double dd[3], Xp[3];
-                        for (int a = 0; a < 3; a++) {
-                            double c = (co.c.box_min[a] + co.c.box_max[a]) / 2;
-                            double he = (co.c.box_max[a] - co.c.box_min[a]) / 2;
-                            Xp[a] = X[a] - c;
-                            dd[a] = std::fabs(Xp[a]) - he;
-                        }
-                        int am = 0;
-                        for (int a = 1; a < 3; a++)
-                            if (dd[a] > dd[am]) am = a;
-                        double q2 = 0;
-                        for (int a = 0; a < 3; a++) q2 += dd[a] > 0 ? dd[a] * dd[a] : 0.0;
-                        phi = std::min(dd[am], 0.0) + std::sqrt(q2);
-                        if (q2 > 0) {
-                            double qn = std::sqrt(q2);
-                            for (int a = 0; a < 3; a++) nrm[a] = (dd[a] > 0 ? dd[a] / qn : 0.0) * (Xp[a] >= 0 ? 1.0 : -1.0);
-                        }
-                        else
-                            nrm[am] = Xp[am] >= 0 ? 1.0 : -1.0;
-                    }
-                    if (!(phi <= 0)) continue;
-                    const double* vo = co.c.dbdt;
-                    if (co.c.type == ZMPM_STICKY) {
-                        for (int a = 0; a < 3; a++) vi[a] = vo[a];
-                        break; // the first sticky hit ends the loop
-                    }
-                    double vr[3] = { vi[0] - vo[0], vi[1] - vo[1], vi[2] - vo[2] };
-                    double dot = vr[0] * nrm[0] + vr[1] * nrm[1] + vr[2] * nrm[2];
-                    bool project = (co.c.type == ZMPM_SLIP) || (dot < 0);
-                    if (project) {
-                        for (int a = 0; a < 3; a++) vr[a] -= nrm[a] * dot;
-                        if (co.c.friction != 0 && dot < 0) {
-                            double vn = std::sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
-                            if (-dot * co.c.friction < vn) {
-                                for (int a = 0; a < 3; a++) vr[a] += (vr[a] / vn) * dot * co.c.friction;
-                            }
-                            else
-                                vr[0] = vr[1] = vr[2] = 0;
-                        }
-                    }
-                    for (int a = 0; a < 3; a++) vi[a] = vr[a] + vo[a];
-                }
+                multiObjectCollision(cols, states, xi, vi);
                 for (int a = 0; a < 3; a++) g.new_v[a] = vi[a];
             }
         }
@@ -1842,6 +2013,8 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
         if (m.plasticity == ZMPM_PLASTICITY_SNOW && m.model != ZMPM_MODEL_FIXED_COROTATED && m.model != ZMPM_MODEL_NEO_HOOKEAN)
             return fail(h, ZMPM_E_ARG, "snow plasticity needs a stress linear in mu and lambda (fixed corotated, neo-Hookean)");
     }
+    for (int32_t m : h->mat)
+        if (m >= count) return fail(h, ZMPM_E_ARG, "material table smaller than the uploaded particles' material indices");
     h->mats.assign(mats, mats + count);
     h->any_damage = false;
     for (auto& m : h->mats) h->any_damage = h->any_damage || m.has_damage;
@@ -1852,6 +2025,9 @@ zmpm_status zmpm_add_collider(zmpm_handle h, const zmpm_collider* c, int32_t* id
 {
     if (!h || !c) return ZMPM_E_ARG;
     if (h->colliders.size() >= 16) return fail(h, ZMPM_E_ARG, "too many colliders");
+    if (c->shape < ZMPM_SHAPE_HALF_SPACE || c->shape > ZMPM_SHAPE_CAPPED_CYLINDER) return fail(h, ZMPM_E_ARG, "unknown shape");
+    if (c->type < ZMPM_STICKY || c->type > ZMPM_GHOST) return fail(h, ZMPM_E_ARG, "unknown collider type");
+    if (!((c->scale0 == 0 ? 1.0 : c->scale0) > 0)) return fail(h, ZMPM_E_ARG, "collider scaling must be positive");
     Collider co;
     co.c = *c;
     double nn = std::sqrt(c->normal[0] * c->normal[0] + c->normal[1] * c->normal[1] + c->normal[2] * c->normal[2]);
@@ -2064,6 +2240,43 @@ zmpm_status zmpm_autotune(zmpm_handle h, double dt, double, int32_t steps, int32
 }
 
 // ------------------------------------------------ unit entry points (pinning)
+static Collider makeCollider(const zmpm_collider& c)
+{
+    Collider co;
+    co.c = c;
+    const double nn = std::sqrt(c.normal[0] * c.normal[0] + c.normal[1] * c.normal[1] + c.normal[2] * c.normal[2]);
+    for (int a = 0; a < 3; a++) co.n[a] = nn > 0 ? c.normal[a] / nn : 0.0; // HalfSpace ctor normalises
+    return co;
+}
+
+// multiObjectCollision of one grid node at x, time t; v resolved in place.
+// Returns 1 when any object collided.
+int32_t zmpm_oracle_collide(const zmpm_collider* cols, int32_t count, const double* x, double t, double* v)
+{
+    std::vector<Collider> cs;
+    for (int i = 0; i < count; i++) cs.push_back(makeCollider(cols[i]));
+    return multiObjectCollision(cs, collidersAt(cs, t), x, v) ? 1 : 0;
+}
+
+// AnalyticCollisionObject::queryInside (CollisionObject.cpp:234-249) of one
+// object at world point x, time t: inside flag, phi scaled by s, world normal R N.
+int32_t zmpm_oracle_query(const zmpm_collider* c, const double* x, double t, double* phi, double* n)
+{
+    const Collider co = makeCollider(*c);
+    const ColliderAt st = colliderAt(*c, t);
+    double xb[3], X[3], N[3];
+    for (int a = 0; a < 3; a++) xb[a] = x[a] - st.b[a];
+    for (int a = 0; a < 3; a++) X[a] = (st.R.a[0][a] * xb[0] + st.R.a[1][a] * xb[1] + st.R.a[2][a] * xb[2]) / st.s;
+    const bool inside = levelSetInside(co, X, *phi, N);
+    *phi *= st.s;
+    for (int a = 0; a < 3; a++) n[a] = st.R.a[a][0] * N[0] + st.R.a[a][1] * N[1] + st.R.a[a][2] * N[2];
+    return inside ? 1 : 0;
+}
+
+// OpenMP threads of the oracle's parallel loops (results do not depend on it)
+void zmpm_oracle_set_threads(int32_t n) { omp_set_num_threads(n > 0 ? n : 1); }
+int32_t zmpm_oracle_get_threads(void) { return omp_get_max_threads(); }
+
 // CG iterations of the handle's last implicit damage solve (-1: none yet)
 int32_t zmpm_oracle_cg_iterations(zmpm_handle h) { return h ? h->cg_iterations : -1; }
 

@@ -15,3 +15,27 @@ def run(lib, scene, precision, steps, t0=0.0, **kw):
     sim = scene.make(lib, precision, **kw)
     sim.run(steps, scene.dt, t0)
     return sim
+
+
+def read_state(path):
+    """A state record of MpmSimulationHip::writeState (zmpm_simulation.hpp): magic,
+    version, n, frame, time, time_since_last_frame, then the attribute arrays."""
+    with open(path, "rb") as f:
+        buf = f.read()
+    assert buf[:8] == b"ZMPMSTAT"
+    version, n = np.frombuffer(buf, np.int32, 1, 8)[0], int(np.frombuffer(buf, np.int64, 1, 12)[0])
+    assert version == 1
+    frame = int(np.frombuffer(buf, np.int32, 1, 20)[0])
+    time, since = np.frombuffer(buf, np.float64, 2, 24)
+    off = 40
+    out = dict(frame=frame, time=float(time), time_since_last_frame=float(since))
+    for name, width, dt in (("x", 3, np.float64), ("v", 3, np.float64), ("C", 9, np.float64), ("F", 9, np.float64),
+                            ("m", 1, np.float64), ("V0", 1, np.float64), ("material", 1, np.int32),
+                            ("d", 1, np.float64), ("g", 1, np.float64), ("laplacian", 1, np.float64),
+                            ("plastic_state", 1, np.float64), ("max_dtilde", 1, np.float64)):
+        cnt = n * width
+        a = np.frombuffer(buf, dt, cnt, off)
+        off += cnt * np.dtype(dt).itemsize
+        out[name] = a.reshape(n, width) if width > 1 else a
+    assert off == len(buf)
+    return out

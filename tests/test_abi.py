@@ -29,7 +29,9 @@ def test_hip_library_exports_every_symbol():
     missing = [n for n in declared() if not hasattr(lib, n)]
     assert not missing, missing
     lib.zmpm_abi_version.restype = ctypes.c_int32
-    assert lib.zmpm_abi_version() == 6
+    version = int(re.search(r"#define ZMPM_ABI_VERSION (\d+)", open(HEADER).read()).group(1))
+    from ziran2020_amd import _abi as A
+    assert lib.zmpm_abi_version() == version == A.ZMPM_ABI_VERSION
     lib.zmpm_build_info.restype = ctypes.c_char_p
     assert b"gfx950" in lib.zmpm_build_info()
 
@@ -45,21 +47,31 @@ def test_python_bindings_cover_the_header():
 
 
 def test_struct_layouts_match_header():
-    """ctypes mirrors of the ABI structs have the sizes a C compiler gives them."""
+    """ctypes mirrors of the ABI structs have the size and every field offset a C
+    compiler gives them (a swap of two same-sized fields fails this)."""
     import subprocess
     import tempfile
     from ziran2020_amd import _abi as A
-    src = ('#include "zmpm.h"\n#include <stdio.h>\nint main(){printf("%zu %zu %zu %zu %zu\\n", sizeof(zmpm_scene_desc),'
-           ' sizeof(zmpm_material), sizeof(zmpm_collider), sizeof(zmpm_particles), sizeof(zmpm_launch_config));'
-           ' return 0;}\n')
+    structs = [("zmpm_scene_desc", A.SceneDesc), ("zmpm_material", A.Material), ("zmpm_collider", A.Collider),
+               ("zmpm_particles", A.Particles), ("zmpm_launch_config", A.LaunchConfig)]
+    c_name = {"lam": "lambda"}  # a Python keyword in the ctypes mirror
+    lines = ['#include "zmpm.h"', "#include <stddef.h>", "#include <stdio.h>", "int main(){"]
+    want = []
+    for cname, py in structs:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        want.append(ctypes.sizeof(py))
+        for field in py._fields_:
+            f = field[0]
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {c_name.get(f, f)}));')
+            want.append(getattr(py, f).offset)
+    lines.append("return 0;}")
     with tempfile.TemporaryDirectory() as d:
         c = os.path.join(d, "s.c")
-        open(c, "w").write(src)
+        open(c, "w").write("\n".join(lines) + "\n")
         exe = os.path.join(d, "s")
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
-        sizes = list(map(int, subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()))
-    assert sizes == [ctypes.sizeof(A.SceneDesc), ctypes.sizeof(A.Material), ctypes.sizeof(A.Collider),
-                     ctypes.sizeof(A.Particles), ctypes.sizeof(A.LaunchConfig)]
+        got = list(map(int, subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()))
+    assert got == want
 
 
 def test_library_loads_without_gpu_and_fails_loudly():

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from ziran2020_amd import _abi as A, scenes
+from helpers import normwise, read_state
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -29,21 +30,55 @@ def lattice_cube(cells):
 
 
 @pytest.mark.parametrize("fp32", [0, 1])
-def test_cpp_facade_matches_oracle(oracle, fp32):
+def test_cpp_facade_matches_oracle(oracle, fp32, tmp_path):
+    # every particle's x, v, C and F as the facade's host arrays hold them after
+    # sync() (writeState), in upload order, against the oracle
     cells, steps = 8, 20
-    out = subprocess.run([EXE, str(cells), str(steps), str(fp32)], capture_output=True, text=True, timeout=120)
+    state = str(tmp_path / "s.dat")
+    out = subprocess.run([EXE, str(cells), str(steps), str(fp32), "--state", state], capture_output=True, text=True,
+                         timeout=120)
     assert out.returncode == 0, out.stderr
     got = json.loads(out.stdout.strip().splitlines()[-1])
+    st = read_state(state)
     sc = lattice_cube(cells)
-    assert got["n"] == sc.n
+    assert got["n"] == sc.n == st["x"].shape[0]
     sim = sc.make(oracle)
     sim.run(steps, 1e-4)
     o = sim.download()
+    tol = 1e-9 if not fp32 else 1e-4
+    for k in ("x", "v", "C", "F"):
+        assert normwise(st[k], o[k]) < tol, (k, normwise(st[k], o[k]))
+    assert np.array_equal(st["m"], o["m"]) and np.array_equal(st["material"], o["material"])
     com = (o["m"][:, None] * o["x"]).sum(0) / o["m"].sum()
-    vm = (o["m"][:, None] * o["v"]).sum(0) / o["m"].sum()
-    tol_x, tol_v = (1e-12, 1e-9) if not fp32 else (1e-6, 1e-4)
-    assert np.abs(np.array(got["com"]) - com).max() < tol_x
-    assert np.abs(np.array(got["mean_v"]) - vm).max() < tol_v * np.abs(vm).max()
+    assert np.abs(np.array(got["com"]) - com).max() < (1e-12 if not fp32 else 1e-6)
+
+
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_cpp_autorestart_on_the_device(oracle, fp32, tmp_path):
+    # the CFL condition reported by the device drives simulate()'s rewind with
+    # max_dt halved (MpmSimulationBase.cpp:1480-1499, SimulationBase.h:327-354):
+    # the same result as a run started at the final max_dt, and as the oracle's
+    runs = []
+    for max_dt in (1.0 / 60, 1.0 / 240):
+        state = str(tmp_path / f"s{len(runs)}.dat")
+        out = subprocess.run([EXE, "4", "0", str(fp32), "2", "-", "--autorestart", repr(max_dt), "--state", state],
+                             capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        runs.append((json.loads(out.stdout.strip().splitlines()[-1]), read_state(state)))
+    (a, sa), (b, sb) = runs
+    assert a["restarts"] == 2 and a["max_dt"] == 1.0 / 240 and b["restarts"] == 0
+    for k in ("x", "v", "C", "F"):
+        # the FP32 form is bitwise reproducible; the FP64 form's double atomics only to rounding
+        assert np.array_equal(sa[k], sb[k]) if fp32 else normwise(sa[k], sb[k]) < 1e-12, k
+    # the oracle through the same substeps (2 frames of 1/60 s at dt = 1/240)
+    sc = lattice_cube(4)
+    sc.v[:] = (0.0, -3.0, 0.0)
+    sim = sc.make(oracle)
+    sim.run(8, 1.0 / 240, 0.0)
+    o = sim.download()
+    tol = 1e-9 if not fp32 else 1e-4
+    for k in ("x", "v", "C", "F"):
+        assert normwise(sa[k], o[k]) < tol, (k, normwise(sa[k], o[k]))
 
 
 def test_cpp_frame_driver_matches_oracle(oracle):

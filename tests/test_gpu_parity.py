@@ -276,6 +276,72 @@ def test_slip_and_separate_colliders(gpu, oracle):
         assert normwise(g[k], o[k]) < FP64_TOL, k
 
 
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_corner_slip_walls_stop_after_three(gpu, oracle, precision):
+    """Nodes in a corner inside three SLIP / SEPARATE walls, then a moving STICKY box
+    over everything: multiObjectCollision (CollisionObject.cpp:66-105) skips the box
+    once three independent slip normals are in, so corner nodes keep the projected
+    velocity while the rest take the box's."""
+    sc = small_cube(cells=8)
+    dx = sc.dx
+    rng = np.random.default_rng(21)
+    sc.v = rng.normal(scale=0.2, size=sc.v.shape)
+    sc.colliders = [A.half_space((18 * dx, 0, 0), (1, 0, 0), A.SLIP),
+                    A.half_space((0, 10 * dx, 0), (0, 1, 0), A.SEPARATE),
+                    A.half_space((0, 0, 18 * dx), (0, 0, 1), A.SLIP, friction=0.2),
+                    A.box((0, 0, 0), (40 * dx, 40 * dx, 40 * dx), A.STICKY, dbdt=(0.1, 0.2, 0.3))]
+    o = run(oracle, sc, A.PREC_FP64, 1)
+    go = o.grid()
+    xi = go["coords"] * dx
+    corner = (xi[:, 0] <= 18 * dx) & (xi[:, 1] <= 10 * dx) & (xi[:, 2] <= 18 * dx)
+    assert corner.sum() > 10 and not np.any(np.all(go["new_v"][corner] == [0.1, 0.2, 0.3], axis=1))
+    assert np.all(go["new_v"][~corner] == [0.1, 0.2, 0.3])
+    g = run(gpu, sc, precision, 1)
+    gg = g.grid()
+    assert np.array_equal(gg["coords"], go["coords"]) or precision == A.PREC_FP32
+    if precision == A.PREC_FP64:
+        assert normwise(gg["new_v"], go["new_v"]) < 1e-12
+    g.run(7, sc.dt, sc.dt)
+    o.run(7, sc.dt, sc.dt)
+    a, b = g.download(), o.download()
+    tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+    for k in ("x", "v", "C", "F"):
+        assert normwise(a[k], b[k]) < tol, (k, normwise(a[k], b[k]))
+
+
+@pytest.mark.parametrize("precision", [A.PREC_FP64, A.PREC_FP32])
+def test_rotating_scaling_and_curved_colliders(gpu, oracle, precision):
+    """Moving objects as examples/boneTwist.h sets them up: a STICKY sphere rotating
+    about y while it translates, a SLIP capped cylinder with its own rotation (axis
+    along x) and friction, a SEPARATE box that grows (scaling), a GHOST that is
+    ignored -- the GPU against the oracle after 10 substeps."""
+    sc = small_cube(cells=8)
+    dx = sc.dx
+    rng = np.random.default_rng(22)
+    sc.v = rng.normal(scale=0.2, size=sc.v.shape)
+    s2 = 2 ** -0.5
+    top = 16 * dx
+    sc.colliders = [
+        A.sphere((0, 0, 0), 3 * dx, A.STICKY, b0=(20 * dx, top, 20 * dx), dbdt=(0, 0.02, 0), omega=(0, -0.7, 0)),
+        A.capped_cylinder(2 * dx, 12 * dx, shape_rotation=(s2, 0, 0, s2), center=(20 * dx, 9 * dx, 17 * dx),
+                          type_=A.SLIP, friction=0.3, omega=(0.4, 0, 0), b0=(0, 0.001, 0)),
+        A.box((-2 * dx, -2 * dx, -2 * dx), (2 * dx, 2 * dx, 2 * dx), A.SEPARATE, b0=(23 * dx, 12 * dx, 23 * dx),
+              scale0=1.0, dsdt=3.0),
+        A.sphere((20 * dx, 12 * dx, 20 * dx), 10 * dx, A.GHOST),
+        A.half_space((0, 8 * dx, 0), (0, 1, 0), A.STICKY),
+    ]
+    o = run(oracle, sc, A.PREC_FP64, 1)
+    n_hit = o.grid()["new_v"]
+    assert n_hit.shape[0] > 0
+    g = run(gpu, sc, precision, 1)
+    g.run(9, sc.dt, sc.dt)
+    o.run(9, sc.dt, sc.dt)
+    a, b = g.download(), o.download()
+    tol = FP64_TOL if precision == A.PREC_FP64 else FP32_TOL
+    for k in ("x", "v", "C", "F"):
+        assert normwise(a[k], b[k]) < tol, (k, normwise(a[k], b[k]))
+
+
 def test_empty_scene(gpu):
     sc = small_cube(cells=4)
     sim = sc.make(gpu, A.PREC_FP32)
@@ -532,6 +598,18 @@ def test_material_change_after_upload_restresses(gpu):
     b = _fields(run(gpu, sc, A.PREC_FP32, 3).download())
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_material_table_must_cover_uploaded_particles(gpu):
+    # a smaller table would leave particles reading an empty material slot
+    sc = small_cube(cells=4)
+    sc.materials = [A.make_material(A.MODEL_FIXED_COROTATED, 1e4, 0.3), A.make_material(A.MODEL_NEO_HOOKEAN, 1e4, 0.3)]
+    sc.material = (np.arange(sc.n) % 2).astype(np.int32)
+    sim = sc.make(gpu, A.PREC_FP32)
+    with pytest.raises(A.ZmpmError) as e:
+        sim.set_materials(sc.materials[:1])
+    assert e.value.status == A.E_ARG
+    sim.set_materials(sc.materials[::-1])
 
 
 def test_launch_config_does_not_change_results(gpu):

@@ -668,6 +668,224 @@ def test_moving_sticky_box_sets_object_velocity(oracle):
     assert np.all(g["new_v"][inside] == np.array([0.0, 0.5, 0.0]))
 
 
+@pytest.mark.parametrize("r", [0.0, 0.5])
+def test_apic_rpic_blend_of_an_affine_field(oracle, r):
+    """APIC/RPIC blend C = (r + 1)/2 B D^-1 + (r - 1)/2 (B D^-1)^T (MpmSimulationBase.cpp:
+    1446-1447): for an affine grid field B D^-1 = B, so r = 0 keeps the skew part of B
+    and r = 0.5 gives 0.75 B - 0.25 B^T."""
+    sc = cube_scene(cells=6)
+    sc.materials = [A.make_material(A.MODEL_FIXED_COROTATED, 0.0, 0.0)]
+    sc.colliders = []
+    sc.gravity = (0.0, 0.0, 0.0)
+    a = np.array([0.1, -0.2, 0.05])
+    B = np.array([[0.3, -0.1, 0.2], [0.05, -0.4, 0.1], [0.2, 0.0, 0.1]])
+    sc.v = a + sc.x @ B.T
+    sc.C = np.tile(B.ravel(), (sc.n, 1))
+    sim = sc.make(oracle, apic_rpic_ratio=r)
+    sim.substep(sc.dt, 0.0)
+    out = sim.download()
+    want = (r + 1) / 2 * B + (r - 1) / 2 * B.T
+    assert np.abs(out["C"] - want.ravel()).max() < 1e-10
+    assert np.abs(out["v"] - (a + sc.x @ B.T)).max() < 1e-12
+
+
+# ------------------------------------------------------------ collision objects
+def _cols(cols):
+    return (A.Collider * len(cols))(*cols)
+
+
+@pytest.fixture(scope="module")
+def CO(oracle):
+    L = oracle.lib
+    L.zmpm_oracle_collide.restype = C.c_int32
+    L.zmpm_oracle_collide.argtypes = [C.POINTER(A.Collider), C.c_int32, DP, C.c_double, DP]
+    L.zmpm_oracle_query.restype = C.c_int32
+    L.zmpm_oracle_query.argtypes = [C.POINTER(A.Collider), DP, C.c_double, DP, DP]
+    return L
+
+
+def collide(CO, cols, x, v, t=0.0):
+    v = np.array(v, float)
+    hit = CO.zmpm_oracle_collide(_cols(cols), len(cols), p(np.array(x, float)), t, p(v))
+    return bool(hit), v
+
+
+def query(CO, c, x, t=0.0):
+    phi, n = C.c_double(), np.zeros(3)
+    inside = CO.zmpm_oracle_query(C.byref(c), p(np.array(x, float)), t, C.byref(phi), p(n))
+    return bool(inside), phi.value, n
+
+
+def test_slip_removes_the_normal_component(CO):
+    """SLIP (CollisionObject.cpp:372-386): v_rel -= n (v_rel . n) in both directions,
+    relative to the object's velocity."""
+    hs = A.half_space((0, 0.5, 0), (0, 1, 0), A.SLIP)
+    x = (0.3, 0.4, 0.2)
+    assert collide(CO, [hs], x, (1, -2, 3))[1].tolist() == [1, 0, 3]
+    assert collide(CO, [hs], x, (1, 2, 3))[1].tolist() == [1, 0, 3]
+    hit, v = collide(CO, [hs], (0.3, 0.6, 0.2), (1, -2, 3))  # outside: untouched
+    assert not hit and v.tolist() == [1, -2, 3]
+    moving = A.half_space((0, 0.5, 0), (0, 1, 0), A.SLIP, dbdt=(0, 0.5, 0))
+    assert collide(CO, [moving], x, (1, -2, 3))[1].tolist() == [1, 0.5, 3]
+    tilted = A.half_space((0, 0.5, 0), (1, 1, 0), A.SLIP)  # normalised by the HalfSpace constructor
+    _, v = collide(CO, [tilted], (0.1, 0.1, 0.0), (2, 0, 0))
+    assert np.abs(v - [1, -1, 0]).max() < 1e-15
+
+
+def test_separate_touches_only_approaching_velocity(CO):
+    """SEPARATE (CollisionObject.cpp:387-401): only dot < 0 is projected."""
+    hs = A.half_space((0, 0.5, 0), (0, 1, 0), A.SEPARATE)
+    x = (0.3, 0.4, 0.2)
+    assert collide(CO, [hs], x, (1, 2, 3))[1].tolist() == [1, 2, 3]
+    assert collide(CO, [hs], x, (1, -2, 3))[1].tolist() == [1, 0, 3]
+
+
+@pytest.mark.parametrize("kind", [A.SLIP, A.SEPARATE])
+def test_coulomb_friction(CO, kind):
+    """Kinematic friction after the projection, only for dot < 0: the tangential
+    velocity shrinks by mu |v_n| (v += v/|v| dot mu) or vanishes when mu |v_n| >= |v_t|."""
+    x = (0.3, 0.4, 0.2)
+    hs = A.half_space((0, 0.5, 0), (0, 1, 0), kind, friction=0.5)
+    _, v = collide(CO, [hs], x, (3, -2, 4))  # |v_t| = 5, mu |v_n| = 1
+    assert np.abs(v - [2.4, 0, 3.2]).max() < 1e-15
+    hs3 = A.half_space((0, 0.5, 0), (0, 1, 0), kind, friction=3.0)
+    assert collide(CO, [hs3], x, (3, -2, 4))[1].tolist() == [0, 0, 0]  # mu |v_n| = 6 >= 5
+    hs25 = A.half_space((0, 0.5, 0), (0, 1, 0), kind, friction=2.5)
+    assert collide(CO, [hs25], x, (3, -2, 4))[1].tolist() == [0, 0, 0]  # mu |v_n| = 5: not below |v_t|
+    want = [3, 0, 4] if kind == A.SLIP else [3, 2, 4]  # receding: no friction
+    assert collide(CO, [hs], x, (3, 2, 4))[1].tolist() == want
+
+
+def test_sticky_ghost_and_order(CO):
+    x = (0.3, 0.4, 0.2)
+    box1 = A.box((0, 0, 0), (1, 1, 1), A.STICKY, dbdt=(1, 0, 0))
+    box2 = A.box((0, 0, 0), (1, 1, 1), A.STICKY, dbdt=(0, 1, 0))
+    ghost = A.box((0, 0, 0), (1, 1, 1), A.GHOST, dbdt=(0, 0, 1))
+    assert collide(CO, [ghost], x, (5, 6, 7)) == (False, pytest.approx([5, 6, 7]))
+    assert collide(CO, [ghost, box1, box2], x, (5, 6, 7))[1].tolist() == [1, 0, 0]  # first sticky ends the loop
+    assert collide(CO, [box2, box1], x, (5, 6, 7))[1].tolist() == [0, 1, 0]
+
+
+def test_slip_normals_stop_after_three_independent_hits(CO):
+    """multiObjectCollision (CollisionObject.cpp:66-105): SLIP / SEPARATE normals are
+    Gram-Schmidt orthonormalised; once 3 are independent the loop ends, so a later
+    object is skipped.  A normal in the span of the earlier ones is not counted."""
+    x = (0.3, 0.4, 0.2)
+    sx = A.half_space((1, 0, 0), (1, 0, 0), A.SLIP)
+    sy = A.half_space((0, 1, 0), (0, 1, 0), A.SLIP)
+    sz = A.half_space((0, 0, 1), (0, 0, 1), A.SLIP)
+    stick = A.box((0, 0, 0), (1, 1, 1), A.STICKY, dbdt=(0.25, 0.5, 0.75))
+    v0 = (1, -2, 3)
+    assert collide(CO, [sx, sy, stick], x, v0)[1].tolist() == [0.25, 0.5, 0.75]
+    assert collide(CO, [sx, sy, sz, stick], x, v0)[1].tolist() == [0, 0, 0]  # the sticky box is skipped
+    assert collide(CO, [sx, sx, sy, stick], x, v0)[1].tolist() == [0.25, 0.5, 0.75]  # repeated normal: 2 only
+    assert collide(CO, [sx, sx, sy, sz, stick], x, v0)[1].tolist() == [0, 0, 0]
+    # a SEPARATE object counts even when the velocity recedes from it
+    sepx = A.half_space((1, 0, 0), (1, 0, 0), A.SEPARATE)
+    assert collide(CO, [sepx, sy, sz, stick], x, (1, -2, 3))[1].tolist() == [1, 0, 0]
+    # non-orthogonal normals x, (x + y)/sqrt2 and (x + y + z)/sqrt3 span three dimensions:
+    # each object projects its own normal out in turn, (1,-2,3) -> (0,-2,3) -> (1,-1,3)
+    # -> (0,-2,2), and the sticky box after them is skipped
+    sxy = A.half_space((1, 1, 0), (1, 1, 0), A.SLIP)
+    sxyz = A.half_space((1, 1, 1), (1, 1, 1), A.SLIP)
+    hit, v = collide(CO, [sx, sxy, sxyz, stick], x, v0)
+    assert hit and np.abs(v - [0, -2, 2]).max() < 1e-14
+    # two of them in the span of the first two (x + y) are not counted
+    hit, v = collide(CO, [sx, sxy, sy, stick], x, v0)
+    assert hit and v.tolist() == [0.25, 0.5, 0.75]
+
+
+def test_box_normal_follows_the_autodiff_rules(CO):
+    """AnalyticBox::normal (AnalyticLevelSet.cpp:582-588): the AutoDiff gradient of
+    min(max_a d_a, 0) + |max(d, 0)| with d = |X - c| - h.  Inside, the first largest
+    d_a wins ties (Eigen's maxCoeff), abs has sign +1 at 0 (AutoDiff.h:307-311) and
+    the clamped norm has derivative 0 at 0 (AutoDiff.h:319-324)."""
+    b = A.box((0, 0, 0), (1, 1, 1), A.SLIP)
+    cases = [
+        ((0.5, 0.5, 0.875), True, -0.125, (0, 0, 1)),     # nearest face +z
+        ((0.125, 0.5, 0.5), True, -0.125, (-1, 0, 0)),    # nearest face -x
+        ((0.875, 0.875, 0.5), True, -0.125, (1, 0, 0)),   # edge: tie x/y -> x
+        ((0.5, 0.125, 0.125), True, -0.125, (0, -1, 0)),  # edge: tie y/z -> y
+        ((0.875, 0.875, 0.875), True, -0.125, (1, 0, 0)),  # corner: tie -> x
+        ((0.125, 0.875, 0.875), True, -0.125, (-1, 0, 0)),
+        ((0.5, 0.5, 0.5), True, -0.5, (1, 0, 0)),          # centre: sign +1 at 0
+        ((1.0, 0.5, 0.5), True, 0.0, (1, 0, 0)),           # on the face: phi = 0 is inside
+        ((1.25, 1.25, 0.5), False, 0.25 * 2 ** 0.5, (2 ** -0.5, 2 ** -0.5, 0)),
+        ((-0.25, 1.25, 1.5), False, (0.25 ** 2 * 2 + 0.25) ** 0.5,
+         np.array([-0.25, 0.25, 0.5]) / (0.25 ** 2 * 2 + 0.25) ** 0.5),
+    ]
+    for x, inside, phi, n in cases:
+        got_in, got_phi, got_n = query(CO, b, x)
+        assert got_in == inside and abs(got_phi - phi) < 1e-15, x
+        assert np.abs(got_n - np.asarray(n, float)).max() < 1e-15, (x, got_n)
+    # the slip projection uses it: a node near the +z face keeps its tangential velocity
+    assert collide(CO, [b], (0.5, 0.5, 0.875), (1, 2, 3))[1].tolist() == [1, 2, 0]
+
+
+def test_sphere_query(CO):
+    """Sphere::queryInside (AnalyticLevelSet.cpp:494-511): strict |X - c|^2 < r^2,
+    radial normal, e_x at the centre."""
+    sp = A.sphere((1, 1, 1), 0.5, A.SLIP)
+    assert query(CO, sp, (1.375, 1, 1)) == (True, -0.125, pytest.approx([1, 0, 0]))
+    inside, phi, n = query(CO, sp, (1.5, 1, 1))  # on the surface: outside
+    assert not inside and phi == 0.0
+    inside, phi, n = query(CO, sp, (1, 1, 1))
+    assert inside and phi == -0.5 and n.tolist() == [1, 0, 0]
+    inside, phi, n = query(CO, sp, (1, 0.75, 1.25))
+    assert inside and np.abs(n - np.array([0, -1, 1]) / 2 ** 0.5).max() < 1e-15
+    assert collide(CO, [sp], (1, 1.25, 1), (1, 2, 3))[1].tolist() == [1, 0, 3]
+
+
+def test_capped_cylinder_query(CO):
+    """CappedCylinder (AnalyticLevelSet.h:290-320): axis along the primitive y,
+    q = (|(x, z)|, y), d = |q| - (r, h/2); inside the gradient of the larger d (d_0 on
+    a tie, whose gradient is 0 on the axis), outside that of |max(d, 0)|."""
+    cy = A.capped_cylinder(0.5, 2.0, center=(1, 1, 1), type_=A.SLIP)
+    cases = [
+        ((1.25, 1, 1), True, -0.25, (1, 0, 0)),
+        ((1, 1.875, 1), True, -0.125, (0, 1, 0)),
+        ((1, 0.125, 1), True, -0.125, (0, -1, 0)),
+        ((1, 1.5, 1), True, -0.5, (0, 0, 0)),  # tie on the axis: d_0's gradient, which is 0 there
+        ((1.75, 2.25, 1), False, 0.25 * 2 ** 0.5, (2 ** -0.5, 2 ** -0.5, 0)),
+        ((1, 1, 1.75), False, 0.25, (0, 0, 1)),
+    ]
+    for x, inside, phi, n in cases:
+        got_in, got_phi, got_n = query(CO, cy, x)
+        assert got_in == inside and abs(got_phi - phi) < 1e-15, x
+        assert np.abs(got_n - np.asarray(n, float)).max() < 1e-15, (x, got_n)
+    # its own rotation: 90 degrees about z takes the primitive y axis to -x
+    s2 = 2 ** -0.5
+    cz = A.capped_cylinder(0.5, 2.0, shape_rotation=(s2, 0, 0, s2), center=(1, 1, 1), type_=A.SLIP)
+    inside, phi, n = query(CO, cz, (1.875, 1, 1))
+    assert inside and abs(phi + 0.125) < 1e-12 and np.abs(n - [1, 0, 0]).max() < 1e-12
+    inside, phi, n = query(CO, cz, (1, 1.25, 1))
+    assert inside and abs(phi + 0.25) < 1e-12 and np.abs(n - [0, 1, 0]).max() < 1e-12
+
+
+def test_moving_object_velocity_and_placement(CO):
+    """v_object = omega x (x - b) + (ds/dt / s)(x - b) + db/dt (CollisionObject.cpp:
+    350-364) and x = s R(t) X + b(t) with R(t) = exp(t [omega]x) R0."""
+    th = 0.8
+    sp = A.sphere((0, 0, 0), 0.5, A.STICKY, b0=(1, 1, 1), dbdt=(0.1, 0.2, 0.3), omega=(0, th, 0))
+    _, v = collide(CO, [sp], (1.25, 1, 1), (9, 9, 9))
+    assert np.abs(v - ([0, 0, -0.25 * th] + np.array([0.1, 0.2, 0.3]))).max() < 1e-15
+    sc = A.sphere((0, 0, 0), 0.5, A.STICKY, b0=(1, 1, 1), scale0=2.0, dsdt=0.5)
+    _, v = collide(CO, [sc], (1.5, 1, 1), (9, 9, 9))  # (ds/dt / s)(x - b) = 0.25 * 0.5
+    assert np.abs(v - [0.125, 0, 0]).max() < 1e-15
+    assert query(CO, sc, (1.875, 1, 1))[0] and not query(CO, sc, (2.125, 1, 1))[0]  # radius 0.5 * s
+    assert abs(query(CO, sc, (1.875, 1, 1))[1] + 0.125) < 1e-15  # phi scaled by s
+    # rotation composed after q0: q0 = 90 deg about x, then omega about z; a small box
+    # at material (0, 0, 1) sits at world b + Rx90 (0,0,1) = b + (0,-1,0) at t = 0 and at
+    # b + Rz90 Rx90 (0,0,1) = b + (1,0,0) once omega t = 90 degrees
+    s2 = 2 ** -0.5
+    bx = A.box((-0.1, -0.1, 0.9), (0.1, 0.1, 1.1), A.STICKY, b0=(2, 2, 2), rotation=(s2, s2, 0, 0),
+               omega=(0, 0, 2.0))
+    t90 = (np.pi / 2) / 2.0
+    assert query(CO, bx, (2, 1, 2), 0.0)[0] and not query(CO, bx, (3, 2, 2), 0.0)[0]
+    assert query(CO, bx, (3, 2, 2), t90)[0] and not query(CO, bx, (2, 1, 2), t90)[0]
+    assert not query(CO, bx, (1, 2, 2), t90)[0]  # the opposite sense of rotation
+
+
 def test_laplacian_quirk_known_answer(oracle):
     """Uniform nodal d: the reference's 3D Laplacian weight uses ddw[1](i), ddw[2](i)
     (MpmGrid.h:610-612), giving lap = 6 d (3 t^2 - 1.25) / dx^2 with t = x/dx - base_x - 1."""
@@ -846,6 +1064,19 @@ def test_transfer_options_validated(oracle):
     for kw in bad:
         with pytest.raises(A.ZmpmError):
             sc.make(oracle, **kw)
+
+
+def test_material_table_must_cover_uploaded_particles(oracle):
+    """A material table smaller than the particles' material indices is rejected
+    (those particles would otherwise read an empty slot: no stress, no status)."""
+    sc = cube_scene(cells=2)
+    sc.materials = [A.make_material(A.MODEL_FIXED_COROTATED, 1e4, 0.3), A.make_material(A.MODEL_NEO_HOOKEAN, 1e4, 0.3)]
+    sc.material = (np.arange(sc.n) % 2).astype(np.int32)
+    sim = sc.make(oracle)
+    with pytest.raises(A.ZmpmError) as e:
+        sim.set_materials(sc.materials[:1])
+    assert e.value.status == A.E_ARG
+    sim.set_materials(sc.materials[::-1])  # same size: accepted
 
 
 def test_plasticity_model_combinations_validated(oracle):

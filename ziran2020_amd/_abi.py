@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-ZMPM_ABI_VERSION = 6
+ZMPM_ABI_VERSION = 7
 
 # zmpm_status
 OK, E_ARG, E_HIP, E_DOMAIN, E_CFL, E_NAN, E_OVERFLOW, E_NOMEM = range(8)
@@ -28,7 +28,7 @@ MODEL_FIXED_COROTATED, MODEL_QR_STABLE_NEO_HOOKEAN, MODEL_NEO_HOOKEAN, MODEL_EQU
 STICKY, SLIP, SEPARATE, GHOST = 1, 2, 3, 4
 APIC_BLEND_RPIC, FLIP_BLEND_PIC = 0, 1
 PLASTICITY_NONE, PLASTICITY_VON_MISES_STVK_HENCKY, PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY, PLASTICITY_SNOW = 0, 1, 2, 3
-SHAPE_HALF_SPACE, SHAPE_BOX = 0, 1
+SHAPE_HALF_SPACE, SHAPE_BOX, SHAPE_SPHERE, SHAPE_CAPPED_CYLINDER = 0, 1, 2, 3
 
 D3 = C.c_double * 3
 
@@ -81,6 +81,9 @@ class Collider(C.Structure):
         ("box_min", D3), ("box_max", D3),
         ("b0", D3), ("dbdt", D3),
         ("friction", C.c_double),
+        ("radius", C.c_double), ("height", C.c_double),
+        ("shape_rotation", C.c_double * 4), ("rotation", C.c_double * 4),
+        ("omega", D3), ("scale0", C.c_double), ("dsdt", C.c_double),
     ]
 
 
@@ -166,21 +169,56 @@ def sigma_crit_from_percentage(m: Material, percentage: float) -> float:
     raise ValueError("sigma_crit_from_percentage: only the phase-field models define getSigmaCrit")
 
 
-def half_space(origin, normal, type_=STICKY, friction=0.0) -> Collider:
+def _motion(c, b0, dbdt, rotation, omega, scale0, dsdt):
+    """Rigid motion of a collision object: translation b0 + dbdt t, rotation
+    q(t) = (cos(|w| t/2), sin(|w| t/2) w/|w|) * rotation, scaling scale0 + dsdt t."""
+    for a in range(3):
+        c.b0[a], c.dbdt[a], c.omega[a] = b0[a], dbdt[a], omega[a]
+    for a in range(4):
+        c.rotation[a] = rotation[a]
+    c.scale0, c.dsdt = scale0, dsdt
+    return c
+
+
+def half_space(origin, normal, type_=STICKY, friction=0.0, b0=(0, 0, 0), dbdt=(0, 0, 0),
+               rotation=(1, 0, 0, 0), omega=(0, 0, 0), scale0=1.0, dsdt=0.0) -> Collider:
     c = Collider()
     c.shape, c.type, c.friction = SHAPE_HALF_SPACE, type_, friction
     for a in range(3):
         c.origin[a], c.normal[a] = origin[a], normal[a]
-    return c
+    return _motion(c, b0, dbdt, rotation, omega, scale0, dsdt)
 
 
-def box(box_min, box_max, type_=STICKY, b0=(0, 0, 0), dbdt=(0, 0, 0), friction=0.0) -> Collider:
+def box(box_min, box_max, type_=STICKY, b0=(0, 0, 0), dbdt=(0, 0, 0), friction=0.0,
+        rotation=(1, 0, 0, 0), omega=(0, 0, 0), scale0=1.0, dsdt=0.0) -> Collider:
     c = Collider()
     c.shape, c.type, c.friction = SHAPE_BOX, type_, friction
     for a in range(3):
         c.box_min[a], c.box_max[a] = box_min[a], box_max[a]
-        c.b0[a], c.dbdt[a] = b0[a], dbdt[a]
-    return c
+    return _motion(c, b0, dbdt, rotation, omega, scale0, dsdt)
+
+
+def sphere(center, radius, type_=STICKY, b0=(0, 0, 0), dbdt=(0, 0, 0), friction=0.0,
+           rotation=(1, 0, 0, 0), omega=(0, 0, 0), scale0=1.0, dsdt=0.0) -> Collider:
+    c = Collider()
+    c.shape, c.type, c.friction, c.radius = SHAPE_SPHERE, type_, friction, radius
+    for a in range(3):
+        c.origin[a] = center[a]
+    return _motion(c, b0, dbdt, rotation, omega, scale0, dsdt)
+
+
+def capped_cylinder(radius, height, shape_rotation=(1, 0, 0, 0), center=(0, 0, 0), type_=STICKY, b0=(0, 0, 0),
+                    dbdt=(0, 0, 0), friction=0.0, rotation=(1, 0, 0, 0), omega=(0, 0, 0), scale0=1.0,
+                    dsdt=0.0) -> Collider:
+    """CappedCylinder(radius, height, q, b) (AnalyticLevelSet.h:252-340): axis along the
+    primitive's y, its own rotation quaternion q = (w, x, y, z) and centre b."""
+    c = Collider()
+    c.shape, c.type, c.friction, c.radius, c.height = SHAPE_CAPPED_CYLINDER, type_, friction, radius, height
+    for a in range(3):
+        c.origin[a] = center[a]
+    for a in range(4):
+        c.shape_rotation[a] = shape_rotation[a]
+    return _motion(c, b0, dbdt, rotation, omega, scale0, dsdt)
 
 
 class LaunchConfig(C.Structure):

@@ -4,18 +4,32 @@
 // against the reference's MpmSimulationBase member names.
 //
 // usage: zmpm_elastic_cube <cells per side> <substeps> <fp32: 0|1> [frames] [restart dir]
+//                          [--state FILE] [--autorestart MAX_DT]
 // With frames > 0 the cube starts at 3 m/s downwards and the frame driver runs
 // that many frames of 1/60 s with CFL substeps (calculateDt, TimeStepping::nextDt;
 // max_dt = 1/100 s) instead of the fixed substeps.
-// With a restart directory the fixed-substep run stops halfway, writes its
-// state there (write(outputFileName())), and a second simulation built from the
-// same scene setup restarts from that file and runs the other half.  The
+// With --autorestart the frame driver instead runs simulate() with autorestart
+// on, max_dt = MAX_DT and calculateDt() returning max_dt (SimulationBase's own
+// calculateDt, SimulationBase.h:260-263): a max_dt that moves particles more
+// than dx in a substep makes the device report the CFL condition, and simulate()
+// rewinds to the last finished frame with max_dt halved until the substeps fit.
+// With a restart directory (not "-") the fixed-substep run stops halfway, writes
+// its state there (write(outputFileName())), and a second simulation built from
+// the same scene setup restarts from that file and runs the other half.  The
 // positions at the restart point also go to <restart dir>/partio_7.bgeo.
-// prints one JSON line: particle count, centre of mass, mean velocity, substeps.
+// --state FILE writes the final particle state (writeState) to FILE.
+// prints one JSON line: particle count, centre of mass, mean velocity, substeps,
+// time, restarts taken and the final max_dt.
 #include "zmpm_simulation.hpp"
 
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+
+// SimulationBase::calculateDt (SimulationBase.h:260-263): the frame driver's max_dt.
+struct FixedDtSimulation : zmpm::MpmSimulationHip {
+    double calculateDt() override { return max_dt; }
+};
 
 static void setup(zmpm::MpmSimulationHip& sim, int cells, bool fp32)
 {
@@ -45,8 +59,14 @@ static void setup(zmpm::MpmSimulationHip& sim, int cells, bool fp32)
                 sim.addParticle({ x0 + (i + 0.5) * h, y0 + (j + 0.5) * h, z0 + (k + 0.5) * h }, rho * V0, V0, mat);
 }
 
+static const char* state_path = nullptr;
+
 static int report(zmpm::MpmSimulationHip& sim, int substeps)
 {
+    if (state_path) {
+        std::ofstream f(state_path, std::ios::binary);
+        sim.writeState(f);
+    }
     sim.sync();
     double cm[3] = { 0, 0, 0 }, vm[3] = { 0, 0, 0 }, M = 0;
     for (size_t p = 0; p < sim.X.size(); p++) {
@@ -57,18 +77,43 @@ static int report(zmpm::MpmSimulationHip& sim, int substeps)
         M += sim.mass[p];
     }
     std::printf("{\"n\": %zu, \"com\": [%.17g, %.17g, %.17g], \"mean_v\": [%.17g, %.17g, %.17g], \"substeps\": %d, "
-                "\"time\": %.17g}\n",
-                sim.X.size(), cm[0] / M, cm[1] / M, cm[2] / M, vm[0] / M, vm[1] / M, vm[2] / M, substeps, sim.time);
+                "\"time\": %.17g, \"restarts\": %d, \"max_dt\": %.17g}\n",
+                sim.X.size(), cm[0] / M, cm[1] / M, cm[2] / M, vm[0] / M, vm[1] / M, vm[2] / M, substeps, sim.time,
+                sim.restarts, sim.max_dt);
     return 0;
 }
 
 int main(int argc, char** argv)
 {
-    const int cells = argc > 1 ? std::atoi(argv[1]) : 16;
-    const int steps = argc > 2 ? std::atoi(argv[2]) : 20;
-    const bool fp32 = argc > 3 && std::atoi(argv[3]) != 0;
-    const int frames = argc > 4 ? std::atoi(argv[4]) : 0;
-    const char* restart_dir = argc > 5 ? argv[5] : nullptr;
+    double autorestart_max_dt = 0;
+    int nargs = 0;
+    const char* pos[5] = { nullptr, nullptr, nullptr, nullptr, nullptr };
+    for (int i = 1; i < argc; i++) {
+        if (!std::strcmp(argv[i], "--state") && i + 1 < argc)
+            state_path = argv[++i];
+        else if (!std::strcmp(argv[i], "--autorestart") && i + 1 < argc)
+            autorestart_max_dt = std::atof(argv[++i]);
+        else if (nargs < 5)
+            pos[nargs++] = argv[i];
+    }
+    const int cells = pos[0] ? std::atoi(pos[0]) : 16;
+    const int steps = pos[1] ? std::atoi(pos[1]) : 20;
+    const bool fp32 = pos[2] && std::atoi(pos[2]) != 0;
+    const int frames = pos[3] ? std::atoi(pos[3]) : 0;
+    const char* restart_dir = pos[4] && std::strcmp(pos[4], "-") ? pos[4] : nullptr;
+    if (autorestart_max_dt > 0) {
+        FixedDtSimulation fs;
+        setup(fs, cells, fp32);
+        for (auto& v : fs.V) v = { 0, -3, 0 };
+        fs.frame_dt = 1.0 / 60;
+        fs.max_dt = autorestart_max_dt;
+        fs.max_dt_original = 1.0 / 60;
+        fs.autorestart = true;
+        fs.end_frame = frames > 0 ? frames : 1;
+        fs.initialize();
+        fs.simulate();
+        return report(fs, 0);
+    }
     zmpm::MpmSimulationHip sim;
     setup(sim, cells, fp32);
     int substeps = steps;

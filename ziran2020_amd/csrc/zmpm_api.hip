@@ -59,6 +59,7 @@ struct zmpm_context {
     void* mats_dev = nullptr;
     void* cols_dev = nullptr;
     int n = 0;
+    int max_mat = -1; // largest material index of the uploaded particles
     int64_t cap = 0;
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -324,6 +325,26 @@ zmpm_status uploadColliders(zmpm_context* h)
             d.dbdt[a] = c.dbdt[a];
         }
         d.friction = c.friction;
+        d.radius = c.radius;
+        d.height = c.height;
+        // the capped cylinder's primitive rotation and its inverse (Eigen's 3x3
+        // inverse through the cofactor matrix, as R.inverse() forms it)
+        quatToMatrix(c.shape_rotation, d.Rc);
+        const double* R = d.Rc;
+        const double cof[9] = { R[4] * R[8] - R[5] * R[7], R[5] * R[6] - R[3] * R[8], R[3] * R[7] - R[4] * R[6],
+                                R[2] * R[7] - R[1] * R[8], R[0] * R[8] - R[2] * R[6], R[1] * R[6] - R[0] * R[7],
+                                R[1] * R[5] - R[2] * R[4], R[2] * R[3] - R[0] * R[5], R[0] * R[4] - R[1] * R[3] };
+        const double inv = 1.0 / (cof[0] * R[0] + cof[1] * R[1] + cof[2] * R[2]);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) d.Rci[3 * i + j] = cof[3 * j + i] * inv;
+        const bool q_zero = c.rotation[0] == 0 && c.rotation[1] == 0 && c.rotation[2] == 0 && c.rotation[3] == 0;
+        for (int a = 0; a < 4; a++) d.q0[a] = q_zero ? (a == 0 ? 1.0 : 0.0) : c.rotation[a];
+        for (int a = 0; a < 3; a++) d.omega[a] = c.omega[a];
+        d.s0 = c.scale0 == 0 ? 1.0 : c.scale0;
+        d.dsdt = c.dsdt;
+        // translation only: x - b(t) is already material space
+        const bool identity = d.q0[0] == 1 && d.q0[1] == 0 && d.q0[2] == 0 && d.q0[3] == 0;
+        d.moving = (identity && c.omega[0] == 0 && c.omega[1] == 0 && c.omega[2] == 0 && d.s0 == 1 && c.dsdt == 0) ? 0 : 1;
     }
     HIPCHK(h, hipMemcpyAsync(h->cols_dev, cd.data(), sizeof(ColDev) * kMaxColliders, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -766,6 +787,8 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
             && mats[i].model != ZMPM_MODEL_NEO_HOOKEAN)
             return fail(h, ZMPM_E_ARG, "snow plasticity needs a stress linear in mu and lambda (fixed corotated, neo-Hookean)");
     }
+    if (h->n > 0 && h->max_mat >= count)
+        return fail(h, ZMPM_E_ARG, "material table smaller than the uploaded particles' material indices");
     (void)hipSetDevice(h->desc.device);
     dropGraphs(h);
     h->mats.assign(mats, mats + count);
@@ -787,8 +810,9 @@ zmpm_status zmpm_add_collider(zmpm_handle h, const zmpm_collider* c, int32_t* id
 {
     if (!h || !c) return ZMPM_E_ARG;
     if ((int)h->cols.size() >= kMaxColliders) return fail(h, ZMPM_E_ARG, "too many colliders");
-    if (c->shape != ZMPM_SHAPE_HALF_SPACE && c->shape != ZMPM_SHAPE_BOX) return fail(h, ZMPM_E_ARG, "unknown shape");
+    if (c->shape < ZMPM_SHAPE_HALF_SPACE || c->shape > ZMPM_SHAPE_CAPPED_CYLINDER) return fail(h, ZMPM_E_ARG, "unknown shape");
     if (c->type < ZMPM_STICKY || c->type > ZMPM_GHOST) return fail(h, ZMPM_E_ARG, "unknown collider type");
+    if (!((c->scale0 == 0 ? 1.0 : c->scale0) > 0)) return fail(h, ZMPM_E_ARG, "collider scaling must be positive");
     (void)hipSetDevice(h->desc.device);
     dropGraphs(h);
     h->cols.push_back(*c);
@@ -822,12 +846,17 @@ zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
     if (h->mats.empty()) return fail(h, ZMPM_E_ARG, "set materials before upload");
     const int bbits = h->fp32 ? 6 : 5;
     if (p->n >= ((int64_t)1 << (32 - bbits))) return fail(h, ZMPM_E_ARG, "too many particles for the key layout");
-    for (int64_t i = 0; i < p->n; i++)
+    int max_mat = -1;
+    for (int64_t i = 0; i < p->n; i++) {
         if (p->material[i] < 0 || p->material[i] >= (int32_t)h->mats.size())
             return fail(h, ZMPM_E_ARG, "material index out of range");
+        max_mat = std::max(max_mat, (int)p->material[i]);
+    }
     (void)hipSetDevice(h->desc.device);
     dropGraphs(h);
-    return h->fp32 ? uploadT<float, long long>(h, p) : uploadT<double, double>(h, p);
+    const zmpm_status s = h->fp32 ? uploadT<float, long long>(h, p) : uploadT<double, double>(h, p);
+    if (s == ZMPM_OK) h->max_mat = max_mat;
+    return s;
 }
 
 zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)

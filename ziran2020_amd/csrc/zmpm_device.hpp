@@ -1189,6 +1189,263 @@ __device__ __forceinline__ T accValue(const Dev<T, AccT>& D, int f, int64_t node
         return D.acc[f * D.cap_nodes + node];
 }
 
+// ------------------------------------------------------- collision objects
+// Per-substep state of a collision object: rotation R(t), translation b(t),
+// scaling s(t) and 1 / s (the reference's updateState at the substep's time,
+// MpmSimulationBase.cpp:209-213).
+struct ColState {
+    double R[9];
+    double b[3];
+    double s, os;
+};
+
+// Eigen's Quaternion::toRotationMatrix of q = (w, x, y, z), row-major.
+__host__ __device__ inline void quatToMatrix(const double* q, double* R)
+{
+    double w = q[0];
+    const double x = q[1], y = q[2], z = q[3];
+    if (w == 0 && x == 0 && y == 0 && z == 0) w = 1;
+    const double tx = 2 * x, ty = 2 * y, tz = 2 * z;
+    const double twx = tx * w, twy = ty * w, twz = tz * w;
+    const double txx = tx * x, txy = ty * x, txz = tz * x;
+    const double tyy = ty * y, tyz = tz * y, tzz = tz * z;
+    R[0] = 1 - (tyy + tzz);
+    R[1] = txy - twz;
+    R[2] = txz + twy;
+    R[3] = txy + twz;
+    R[4] = 1 - (txx + tzz);
+    R[5] = tyz - twx;
+    R[6] = txz - twy;
+    R[7] = tyz + twx;
+    R[8] = 1 - (txx + tyy);
+}
+
+// The object's state at time t: b = b0 + dbdt t, s = s0 + dsdt t and the
+// rotation of q(t) = (cos(|w| t/2), sin(|w| t/2) w/|w|) * q0 (constant angular
+// velocity w, as the updateState callbacks of examples/boneTwist.h:144-170 set it).
+__device__ inline void colliderState(const ColDev& c, double t, ColState& st)
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++) st.b[a] = c.b0[a] + c.dbdt[a] * t;
+    st.s = c.s0 + c.dsdt * t;
+    st.os = 1.0 / st.s;
+    double q[4] = { c.q0[0], c.q0[1], c.q0[2], c.q0[3] };
+    const double wn = sqrt(c.omega[0] * c.omega[0] + c.omega[1] * c.omega[1] + c.omega[2] * c.omega[2]);
+    if (wn > 0) {
+        const double ha = 0.5 * wn * t, cw = cos(ha), sw = sin(ha) / wn;
+        const double p[4] = { cw, sw * c.omega[0], sw * c.omega[1], sw * c.omega[2] };
+        const double* q0 = c.q0;
+        q[0] = p[0] * q0[0] - p[1] * q0[1] - p[2] * q0[2] - p[3] * q0[3];
+        q[1] = p[0] * q0[1] + p[1] * q0[0] + p[2] * q0[3] - p[3] * q0[2];
+        q[2] = p[0] * q0[2] - p[1] * q0[3] + p[2] * q0[0] + p[3] * q0[1];
+        q[3] = p[0] * q0[3] + p[1] * q0[2] - p[2] * q0[1] + p[3] * q0[0];
+    }
+    quatToMatrix(q, st.R);
+}
+
+// AnalyticLevelSet::queryInside in material space X: inside flag and the
+// material-space normal N (half space AnalyticLevelSet.cpp:331-347; the axis-
+// aligned box's AutoDiff normal :562-588 -- the first largest d_a inside, sign
+// +1 at 0, the clamped norm's derivative 0 at 0; Sphere::queryInside :494-511;
+// CappedCylinder AnalyticLevelSet.h:290-320 in its primitive frame).
+template <class T>
+__device__ __forceinline__ bool levelSetInside(const ColDev& co, const T* X, T* N)
+{
+    N[0] = N[1] = N[2] = T(0);
+    if (co.shape == ZMPM_SHAPE_HALF_SPACE) {
+        const T phi = (T)co.n[0] * (X[0] - (T)co.origin[0]) + (T)co.n[1] * (X[1] - (T)co.origin[1])
+            + (T)co.n[2] * (X[2] - (T)co.origin[2]);
+#pragma unroll
+        for (int a = 0; a < 3; a++) N[a] = (T)co.n[a];
+        return phi <= T(0);
+    }
+    if (co.shape == ZMPM_SHAPE_BOX) {
+        T dd[3], Xc[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const T c0 = (T)((co.bmin[a] + co.bmax[a]) / 2), he = (T)((co.bmax[a] - co.bmin[a]) / 2);
+            Xc[a] = X[a] - c0;
+            dd[a] = dabs(Xc[a]) - he;
+        }
+        int am = 0;
+        if (dd[1] > dd[am]) am = 1;
+        if (dd[2] > dd[am]) am = 2;
+        T q2 = T(0);
+#pragma unroll
+        for (int a = 0; a < 3; a++) q2 += dd[a] > T(0) ? dd[a] * dd[a] : T(0);
+        const T dmax = dd[am];
+        const T phi = (dmax < T(0) ? dmax : T(0)) + dsqrt(q2);
+        if (q2 > T(0)) {
+            const T qn = dsqrt(q2);
+#pragma unroll
+            for (int a = 0; a < 3; a++) N[a] = (dd[a] > T(0) ? dd[a] / qn : T(0)) * (Xc[a] < T(0) ? T(-1) : T(1));
+        }
+        else {
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                if (a == am) N[a] = Xc[a] < T(0) ? T(-1) : T(1);
+        }
+        return phi <= T(0);
+    }
+    if (co.shape == ZMPM_SHAPE_SPHERE) {
+        T tc[3], d2 = T(0);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            tc[a] = X[a] - (T)co.origin[a];
+            d2 += tc[a] * tc[a];
+        }
+        const T dist = dsqrt(d2);
+        if (dist < T(1e-7))
+            N[0] = T(1);
+        else {
+            const T id = T(1) / dist;
+#pragma unroll
+            for (int a = 0; a < 3; a++) N[a] = id * tc[a];
+        }
+        const T r = (T)co.radius;
+        return d2 < r * r;
+    }
+    // capped cylinder, in the primitive frame Rc^-1 (X - centre)
+    T Y[3], Xp[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) Y[a] = X[a] - (T)co.origin[a];
+#pragma unroll
+    for (int a = 0; a < 3; a++) Xp[a] = (T)co.Rci[3 * a] * Y[0] + (T)co.Rci[3 * a + 1] * Y[1] + (T)co.Rci[3 * a + 2] * Y[2];
+    const T rxz = dsqrt(Xp[0] * Xp[0] + Xp[2] * Xp[2]);
+    const T d0 = rxz - (T)co.radius, d1 = dabs(Xp[1]) - T(0.5) * (T)co.height;
+    T g0[3] = { T(0), T(0), T(0) };
+    const T g1y = Xp[1] < T(0) ? T(-1) : T(1);
+    if (rxz != T(0)) {
+        g0[0] = Xp[0] / rxz;
+        g0[2] = Xp[2] / rxz;
+    }
+    const T q0 = d0 < T(0) ? T(0) : d0, q1 = d1 < T(0) ? T(0) : d1;
+    const T mx = d0 < d1 ? d1 : d0;
+    const T qn = dsqrt(q0 * q0 + q1 * q1);
+    const T phi = (T(0) < mx ? T(0) : mx) + qn;
+    T n[3] = { T(0), T(0), T(0) };
+    if (!(T(0) < mx)) {
+        if (d0 < d1)
+            n[1] = g1y;
+        else {
+            n[0] = g0[0];
+            n[2] = g0[2];
+        }
+    }
+    if (qn != T(0)) {
+        const T iq = T(1) / qn;
+        n[0] += q0 * g0[0] * iq;
+        n[1] += q1 * g1y * iq;
+        n[2] += q0 * g0[2] * iq;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) N[a] = (T)co.Rc[3 * a] * n[0] + (T)co.Rc[3 * a + 1] * n[1] + (T)co.Rc[3 * a + 2] * n[2];
+    return phi <= T(0);
+}
+
+// AnalyticCollisionObject::detectAndResolveCollision (CollisionObject.cpp:341-406)
+// of grid node xi: true when the node is inside; vi resolved; n the world normal
+// of a SLIP / SEPARATE object.
+template <class T>
+__device__ __forceinline__ bool resolveCollision(const ColDev& co, const ColState& st, const T* xi, T* vi, T* n)
+{
+    T xb[3], X[3], N[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) xb[a] = xi[a] - (T)st.b[a];
+    if (co.moving) {
+        const T os = (T)st.os;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            X[a] = ((T)st.R[a] * xb[0] + (T)st.R[3 + a] * xb[1] + (T)st.R[6 + a] * xb[2]) * os;
+    }
+    else {
+#pragma unroll
+        for (int a = 0; a < 3; a++) X[a] = xb[a];
+    }
+    if (!levelSetInside(co, X, N)) return false;
+    // v_object = omega x (x - b) + (dsdt / s)(x - b) + dbdt
+    T vo[3];
+    if (co.moving) {
+        const T w0 = (T)co.omega[0], w1 = (T)co.omega[1], w2 = (T)co.omega[2];
+        const T sr = (T)(co.dsdt * st.os);
+        vo[0] = (w1 * xb[2] - w2 * xb[1]) + sr * xb[0] + (T)co.dbdt[0];
+        vo[1] = (w2 * xb[0] - w0 * xb[2]) + sr * xb[1] + (T)co.dbdt[1];
+        vo[2] = (w0 * xb[1] - w1 * xb[0]) + sr * xb[2] + (T)co.dbdt[2];
+#pragma unroll
+        for (int a = 0; a < 3; a++) n[a] = (T)st.R[3 * a] * N[0] + (T)st.R[3 * a + 1] * N[1] + (T)st.R[3 * a + 2] * N[2];
+    }
+    else {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            vo[a] = (T)co.dbdt[a];
+            n[a] = N[a];
+        }
+    }
+    if (co.type == ZMPM_STICKY) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) vi[a] = vo[a];
+        return true;
+    }
+    T vr[3] = { vi[0] - vo[0], vi[1] - vo[1], vi[2] - vo[2] };
+    const T dot = vr[0] * n[0] + vr[1] * n[1] + vr[2] * n[2];
+    if (co.type == ZMPM_SLIP || dot < T(0)) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) vr[a] -= n[a] * dot;
+        const T fr = (T)co.friction;
+        if (fr != T(0) && dot < T(0)) { // kinematic friction, only against approach
+            const T vn = dsqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+            if (-dot * fr < vn) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) vr[a] += (vr[a] / vn) * dot * fr;
+            }
+            else
+                vr[0] = vr[1] = vr[2] = T(0);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) vi[a] = vr[a] + vo[a];
+    return true;
+}
+
+// AnalyticCollisionObject::multiObjectCollision (CollisionObject.cpp:66-105):
+// objects in insertion order; the first STICKY hit ends the loop; the normals
+// of SLIP / SEPARATE hits are Gram-Schmidt orthonormalised and the loop ends
+// once 3 are independent (a normal in the span of the earlier ones adds none).
+template <class T>
+__device__ __forceinline__ void multiObjectCollision(const ColDev* cols, const ColState* st, int ncols, const T* xi,
+                                                     T* vi)
+{
+    int count = 0;
+    T b0[3] = { T(0), T(0), T(0) }, b1[3] = { T(0), T(0), T(0) };
+    for (int c = 0; c < ncols; c++) {
+        const ColDev& co = cols[c];
+        if (co.type == ZMPM_GHOST) continue;
+        T n[3];
+        if (!resolveCollision(co, st[c], xi, vi, n)) continue;
+        if (co.type == ZMPM_STICKY) break;
+        if (count >= 1) {
+            const T d = b0[0] * n[0] + b0[1] * n[1] + b0[2] * n[2];
+#pragma unroll
+            for (int a = 0; a < 3; a++) n[a] -= d * b0[a];
+        }
+        if (count >= 2) {
+            const T d = b1[0] * n[0] + b1[1] * n[1] + b1[2] * n[2];
+#pragma unroll
+            for (int a = 0; a < 3; a++) n[a] -= d * b1[a];
+        }
+        const T len = dsqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (len != T(0)) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const T u = n[a] / len;
+                if (count == 0) b0[a] = u;
+                if (count == 1) b1[a] = u;
+            }
+            if (++count == 3) break;
+        }
+    }
+}
+
 // v = mv/m (MpmSimulationBase.cpp:655-666) and gridVelocityExplicitUpdate
 // (:1179-1228) with multiObjectCollision / detectAndResolveCollision
 // (CollisionObject.cpp:66-105, 341-406) at time t; phase-field normalisation
@@ -1201,15 +1458,18 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
     const int64_t nn = (int64_t)D.counters[0] * NB;
     const int64_t cn = D.cap_nodes;
-    // the collision objects in LDS: read from there the per-node loop's loads
-    // cannot alias the kernel's stores (with the table in global memory, stores
-    // of doubles anywhere in the kernel made the compiler load it per node with
-    // vector loads: ~2x slower on large grids)
+    // the collision objects and their states at t in LDS: read from there the
+    // per-node loop's loads cannot alias the kernel's stores (with the table in
+    // global memory, stores of doubles anywhere in the kernel made the compiler
+    // load it per node with vector loads: ~2x slower on large grids)
     __shared__ ColDev cols[MAX_COLLIDERS];
+    __shared__ ColState cst[MAX_COLLIDERS];
     const int ncols = min(D.num_cols, MAX_COLLIDERS);
     static_assert(sizeof(ColDev) % 8 == 0, "collider record is copied as 8-byte words");
     for (int e = threadIdx.x; e < ncols * (int)(sizeof(ColDev) / 8); e += WG)
         reinterpret_cast<uint64_t*>(cols)[e] = reinterpret_cast<const uint64_t*>(D.cols)[e];
+    __syncthreads();
+    if ((int)threadIdx.x < ncols) colliderState(cols[threadIdx.x], t, cst[threadIdx.x]);
     __syncthreads();
     if (D.implicit && D.any_damage) {
         // the damage solve's accumulators, filled by k_dtilde
@@ -1248,72 +1508,7 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
             if (D.flip) D.oldv[a * cn + node] = v; // the grid's v before the update (FLIP)
             vi[a] = v + D.gravity[a] * dt + f[a] * dt_over_mi;
         }
-        for (int c = 0; c < ncols; c++) {
-            const ColDev& co = cols[c];
-            if (co.type == ZMPM_GHOST) continue;
-            T X[3];
-#pragma unroll
-            for (int a = 0; a < 3; a++) X[a] = xi[a] - (T)(co.b0[a] + co.dbdt[a] * t);
-            T phi, nrm[3] = { T(0), T(0), T(0) };
-            if (co.shape == ZMPM_SHAPE_HALF_SPACE) {
-                phi = (T)co.n[0] * (X[0] - (T)co.origin[0]) + (T)co.n[1] * (X[1] - (T)co.origin[1])
-                    + (T)co.n[2] * (X[2] - (T)co.origin[2]);
-#pragma unroll
-                for (int a = 0; a < 3; a++) nrm[a] = (T)co.n[a];
-            }
-            else {
-                T dd[3], Xc[3];
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    T c0 = (T)((co.bmin[a] + co.bmax[a]) / 2), he = (T)((co.bmax[a] - co.bmin[a]) / 2);
-                    Xc[a] = X[a] - c0;
-                    dd[a] = dabs(Xc[a]) - he;
-                }
-                int am = 0;
-                if (dd[1] > dd[am]) am = 1;
-                if (dd[2] > dd[am]) am = 2;
-                T q2 = T(0);
-#pragma unroll
-                for (int a = 0; a < 3; a++) q2 += dd[a] > T(0) ? dd[a] * dd[a] : T(0);
-                T dmax = dd[am];
-                phi = (dmax < T(0) ? dmax : T(0)) + dsqrt(q2);
-                if (co.type != ZMPM_STICKY) {
-                    if (q2 > T(0)) {
-                        T qn = dsqrt(q2);
-#pragma unroll
-                        for (int a = 0; a < 3; a++)
-                            nrm[a] = (dd[a] > T(0) ? dd[a] / qn : T(0)) * (Xc[a] >= T(0) ? T(1) : T(-1));
-                    }
-                    else
-                        nrm[am] = Xc[am] >= T(0) ? T(1) : T(-1);
-                }
-            }
-            if (!(phi <= T(0))) continue;
-            const T vo[3] = { (T)co.dbdt[0], (T)co.dbdt[1], (T)co.dbdt[2] };
-            if (co.type == ZMPM_STICKY) {
-#pragma unroll
-                for (int a = 0; a < 3; a++) vi[a] = vo[a];
-                break;
-            }
-            T vr[3] = { vi[0] - vo[0], vi[1] - vo[1], vi[2] - vo[2] };
-            T dot = vr[0] * nrm[0] + vr[1] * nrm[1] + vr[2] * nrm[2];
-            if (co.type == ZMPM_SLIP || dot < T(0)) {
-#pragma unroll
-                for (int a = 0; a < 3; a++) vr[a] -= nrm[a] * dot;
-                T fr = (T)co.friction;
-                if (fr != T(0) && dot < T(0)) {
-                    T vn = dsqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
-                    if (-dot * fr < vn) {
-#pragma unroll
-                        for (int a = 0; a < 3; a++) vr[a] += (vr[a] / vn) * dot * fr;
-                    }
-                    else
-                        vr[0] = vr[1] = vr[2] = T(0);
-                }
-            }
-#pragma unroll
-            for (int a = 0; a < 3; a++) vi[a] = vr[a] + vo[a];
-        }
+        multiObjectCollision(cols, cst, ncols, xi, vi);
 #pragma unroll
         for (int a = 0; a < 3; a++) D.newv[a * cn + node] = vi[a];
         if (D.any_damage) {

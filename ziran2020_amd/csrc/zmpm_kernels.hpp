@@ -136,9 +136,17 @@ __host__ __device__ constexpr bool phaseFieldModel(int model)
 
 constexpr int MAX_COLLIDERS = 16; // collision objects per handle
 
+// A collision object (AnalyticCollisionObject, CollisionObject.h:20-165) as the
+// kernels read it: its level set, its rigid motion and, for the capped cylinder,
+// the primitive's own rotation and its inverse (formed on the host).
 struct ColDev {
     int shape, type;
+    int moving; // rotation or scaling: R(t), s(t) evaluated; otherwise x - b(t) is material space
+    int pad;
     double origin[3], n[3], bmin[3], bmax[3], b0[3], dbdt[3], friction;
+    double radius, height;
+    double Rc[9], Rci[9]; // capped cylinder primitive rotation and its inverse (row-major)
+    double q0[4], omega[3], s0, dsdt;
 };
 
 struct Masks {

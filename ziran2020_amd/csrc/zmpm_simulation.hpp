@@ -10,6 +10,7 @@
 #include "zmpm.h"
 #include "zmpm_bgeo.hpp"
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -18,6 +19,7 @@
 #include <functional>
 #include <istream>
 #include <ostream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -106,11 +108,18 @@ public:
     double time = 0;
     double frame_dt = 1.0 / 24;
     int end_frame = 0;
-    // TimeStepping (Lib/Ziran/Sim/TimeStepping.h:15-33): dt bounds of the frame driver
+    // TimeStepping (Lib/Ziran/Sim/TimeStepping.h:15-33): dt bounds of the frame
+    // driver; max_dt_original keeps the constructor's value (the default frame_dt)
     double min_dt = 1e-6;
     double max_dt = 1.0 / 24;
+    double max_dt_original = 1.0 / 24;
     double time_since_last_frame = 0;
+    int start_frame = 0;
     std::vector<std::function<void(int, int, double, double)>> begin_time_step_callbacks;
+    // SimulationBase::restart_callbacks (SimulationBase.h:334-343): run once after a
+    // restart has read its frame (the CFL autorestart shrinks max_dt through one)
+    std::vector<std::function<void(int)>> restart_callbacks;
+    int restarts = 0; // restarts taken by simulate() (no counterpart in the reference)
 
     MpmSimulationHip() = default;
     MpmSimulationHip(const MpmSimulationHip&) = delete;
@@ -199,17 +208,35 @@ public:
 
     // MpmSimulationBase::advanceOneTimeStep (MpmSimulationBase.cpp:262-357):
     // one zmpm_substep on the device.  With autorestart the CFL condition is
-    // checked after the substep (one device synchronisation).
+    // checked after the substep (one device synchronisation); a particle that
+    // moved more than dx throws RestartException for the last finished frame,
+    // one frame earlier when max_dt cannot shrink any further, and otherwise
+    // leaves a restart callback that halves max_dt (no lower than min_dt)
+    // (MpmSimulationBase.cpp:1480-1499).  The device state of the violating
+    // substep is the one a restart discards.
     virtual void advanceOneTimeStep(double dt_)
     {
         reinitialize();
         check(zmpm_substep(h_, dt_, time), "zmpm_substep");
+        host_stale_ = true;
         if (autorestart) {
             zmpm_status s = zmpm_synchronize(h_);
-            if (s == ZMPM_E_CFL) throw RestartException(frame - 1);
+            if (s == ZMPM_E_CFL) {
+                // `frame` counts finished frames here: the reference's frame - 1
+                int frame_to_restart = frame;
+                if (max_dt <= min_dt) {
+                    frame_to_restart--;
+                    if (frame_to_restart < start_frame)
+                        throw ZmpmError(ZMPM_E_CFL, "Unstable initial conditions detected, shrink min_dt or change other parameters");
+                }
+                else {
+                    const double new_max_dt = std::max(max_dt / 2, min_dt);
+                    restart_callbacks.push_back([new_max_dt, this](int) { max_dt = new_max_dt; });
+                }
+                throw RestartException(frame_to_restart);
+            }
             check(s, "zmpm_synchronize");
         }
-        host_stale_ = true;
     }
 
     // `steps` substeps of size dt_ without host round trips.
@@ -231,11 +258,13 @@ public:
         return d;
     }
 
-    // TimeStepping::nextDt (TimeStepping.h:45-59): clamp to [min_dt, max_dt] and
-    // land exactly on the frame boundary without a tiny last substep.
+    // TimeStepping::nextDt (TimeStepping.h:45-59): max_dt no larger than
+    // max_dt_original, clamp to [min_dt, max_dt] and land exactly on the frame
+    // boundary without a tiny last substep.
     double nextDt(double d)
     {
         if (!(d > 0)) throw ZmpmError(ZMPM_E_ARG, "nextDt: dt must be positive");
+        max_dt = std::min(max_dt, max_dt_original);
         d = (d < min_dt) ? min_dt : (d > max_dt) ? max_dt : d;
         if (time_since_last_frame + d >= frame_dt)
             d = frame_dt - time_since_last_frame;
@@ -269,13 +298,25 @@ public:
     // SimulationBase::simulate (SimulationBase.h:327-354).  With write_frames the
     // state file of the start frame and of every finished frame is written, as
     // the reference always does (write(outputFileName()), :332-333 and :319).
+    // A RestartException from a frame rewinds to the frame it names (restart:
+    // its state file, or with write_frames off the copy kept in memory of the
+    // last two finished frames), runs the restart callbacks and carries on.
     bool write_frames = false;
     virtual void simulate()
     {
-        if (write_frames) write(outputFileName());
+        frameFinished();
         while (frame < end_frame) {
-            advanceOneFrame();
-            if (write_frames) write(outputFileName());
+            try {
+                advanceOneFrame();
+            }
+            catch (const RestartException& e) {
+                restart(e.frame);
+                for (auto& f : restart_callbacks) f(frame);
+                restart_callbacks.clear();
+                restarts++;
+                continue;
+            }
+            frameFinished();
         }
         sync();
     }
@@ -390,12 +431,22 @@ public:
         if (!f) throw ZmpmError(ZMPM_E_ARG, "read: cannot open " + output_dir + "/" + filename);
         readState(f);
     }
-    // Reads the state written at the start of frame new_restart_frame and goes
-    // on from there.
+    // SimulationBase::restart (SimulationBase.h:265-285) with the read simulate()
+    // does next: the state of frame new_restart_frame (its state file, else the
+    // in-memory copy simulate() keeps) and the clock of that frame.
     virtual void restart(int new_restart_frame)
     {
         frame = new_restart_frame;
-        read(outputFileName());
+        time = frame * frame_dt; // TimeStepping::reset (TimeStepping.h:88-92)
+        time_since_last_frame = 0;
+        bool from_memory = false;
+        for (const auto& kv : kept_frames_)
+            if (kv.first == new_restart_frame && !write_frames) {
+                std::istringstream in(kv.second);
+                readState(in);
+                from_memory = true;
+            }
+        if (!from_memory) read(outputFileName());
         if (frame != new_restart_frame) throw ZmpmError(ZMPM_E_ARG, "restart: file holds another frame");
     }
 
@@ -405,6 +456,21 @@ private:
     zmpm_handle h_ = nullptr;
     bool dirty_ = true;
     bool host_stale_ = false;
+    // state records of the last two finished frames, for autorestart without state files
+    std::vector<std::pair<int, std::string>> kept_frames_;
+
+    // End of a frame in simulate(): its state file (write_frames) and, for
+    // autorestart without files, an in-memory copy.
+    void frameFinished()
+    {
+        if (write_frames) write(outputFileName());
+        if (autorestart && !write_frames) {
+            std::ostringstream out;
+            writeState(out);
+            if (kept_frames_.size() == 2) kept_frames_.erase(kept_frames_.begin());
+            kept_frames_.emplace_back(frame, out.str());
+        }
+    }
 
     template <class Fn>
     void forEachAttribute(Fn&& fn)

@@ -195,11 +195,12 @@ def sparse_spheres(n=8_000_000, n_spheres=20, dx=1.0 / 256, seed=3, steps=100, d
 
 
 def batched_scenes(n_scenes=64, n_range=(10_000, 500_000), dx=1.0 / 128, seed=4, steps=100, dt=1e-4,
-                   domain_nodes=512, tile_cells=48, E=1e4, nu=0.3, rho=1000.0, sigma_crit=5e3):
+                   domain_nodes=512, tile_cells=48, E=1e4, nu=0.3, rho=1000.0, sigma_crit=5e3, keep=None):
     """Batched uneven scenes in one launch: n_scenes independent cubes with particle
     counts log-uniform over n_range, each in its own tile of one world grid on a
     shared sticky floor; odd scenes fixed-corotated, even scenes QRStableNeoHookean
-    with AnisoMPM damage."""
+    with AnisoMPM damage.  keep: indices of the scenes to place (the others are
+    drawn and dropped, so every kept scene is exactly its copy in the full batch)."""
     rng = np.random.default_rng(seed)
     counts = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), n_scenes)).astype(int)
     per_row = int(np.ceil(np.sqrt(n_scenes)))
@@ -208,22 +209,28 @@ def batched_scenes(n_scenes=64, n_range=(10_000, 500_000), dx=1.0 / 128, seed=4,
     mats = [A.make_material(A.MODEL_FIXED_COROTATED, E, nu),
             A.make_material(A.MODEL_QR_STABLE_NEO_HOOKEAN, E, nu, has_damage=True, alphas=(-0.9,),
                             l0=2 * dx, sigma_crit=sigma_crit)]
+    kept = []
     for s, k in enumerate(counts):
         side = (k / 8.0) ** (1.0 / 3.0) * dx  # ~8 particles per cell
         side = min(side, (tile_cells - 8) * dx)
         ix, iz = s % per_row, s // per_row
         lo = np.array([off + ix * tile_cells * dx, off, off + iz * tile_cells * dx])
-        xs.append(uniform_box(rng, lo, lo + side, k))
+        xk = uniform_box(rng, lo, lo + side, k)
         damaged = (s % 2 == 0)
+        Fk = fibre_F(rng, k) if damaged else identity_F(k)
+        if keep is not None and s not in keep:
+            continue
+        kept.append(s)
+        xs.append(xk)
         mid.append(np.full(k, 1 if damaged else 0, np.int32))
         V0.append(np.full(k, side ** 3 / k))
-        Fs.append(fibre_F(rng, k) if damaged else identity_F(k))
+        Fs.append(Fk)
     x = np.concatenate(xs)
     n = x.shape[0]
     cols = domain_walls(domain_nodes, dx) + [A.half_space((0, off, 0), (0, 1, 0), A.STICKY)]
     return Scene("batched_scenes", dx, domain_nodes, dt, steps, (0.0, -9.8, 0.0), mats, cols, x,
                  np.concatenate(mid), np.full(n, rho), np.concatenate(V0), np.concatenate(Fs),
-                 info=dict(counts=counts.tolist()))
+                 info=dict(counts=counts.tolist(), scenes=kept))
 
 
 def two_material(n=16_000_000, dx=1.0 / 256, seed=5, steps=100, dt=1e-4, domain_nodes=512,
