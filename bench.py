@@ -2,9 +2,12 @@
 damage on one MI355X, FP32 batched form (value) next to the FP64 plain form.
 
 Workload (default, `batched_scenes`): 64 independent scenes in one launch per
-stage -- cubes of 10k..500k particles (log-uniform, seed 4, 10,366,485 in all),
-odd scenes fixed-corotated, even scenes QRStableNeoHookean with AnisoMPM
-phase-field damage, dx = 1/128, dt = 1e-4, gravity, a sticky floor; every
+stage (one handle of zmpm_create_batch) -- cubes of 10k..500k particles
+(log-uniform, seed 4, 10,366,485 in all), each scene with its own dx (1/96,
+1/128, 1/160), dt (8e-5..1.25e-4), start time, tilted gravity, APIC/RPIC ratio
+and colliders (walls, a sticky / slipping / separating floor, moving spheres,
+spinning boxes, growing capped cylinders); even scenes QRStableNeoHookean with
+AnisoMPM phase-field damage, odd scenes fixed-corotated or Neo-Hookean; every
 substep a HIP-graph replay.  Inputs are uploaded to HBM before the timed
 region; one "step" is one full substep (binning + sort, P2G, grid update, G2P +
 damage + F update + next stress) of every scene.
@@ -116,11 +119,11 @@ def time_oracle(scene, threads, budget_s, max_steps):
     lib.lib.zmpm_oracle_set_threads(threads)
     sim = scene.make(lib, A.PREC_FP64)
     t0 = time.perf_counter()
-    sim.run(1, scene.dt, 0.0)
+    scene.run(sim, 1)
     one = time.perf_counter() - t0
     steps = int(max(1, min(max_steps, budget_s / max(one, 1e-6))))
     t0 = time.perf_counter()
-    sim.run(steps, scene.dt, scene.dt)
+    scene.run(sim, steps, start=1)
     el = time.perf_counter() - t0
     sim.close()
     return scene.n * steps / el, steps
@@ -149,16 +152,15 @@ def timed(scene, lib, precision, steps, warmup, timing_steps=50, autotune=True):
     min(steps, timing_steps) substeps launched one kernel at a time."""
     sim = scene.make(lib, precision)
     timed.launch_config = sim.autotune(scene.dt, 0.0, steps=4) if autotune else sim.launch_config()
-    sim.run(warmup, scene.dt, 0.0)
+    scene.run(sim, warmup)
     sim.synchronize()
-    t_start = warmup * scene.dt
     t0 = time.perf_counter()
-    sim.run(steps, scene.dt, t_start)
+    scene.run(sim, steps, start=warmup)
     sim.synchronize()
     el = time.perf_counter() - t0
     k = min(steps, timing_steps)
     sim.L.zmpm_set_timing(sim.h, 1)
-    sim.run(k, scene.dt, t_start + steps * scene.dt)
+    scene.run(sim, k, start=warmup + steps)
     sim.synchronize()
     sim.L.zmpm_set_timing(sim.h, 0)
     stages = sim.timings()
@@ -215,7 +217,7 @@ def accuracy_vs_fp64(scene, lib, steps=100):
     out = {}
     for prec in (A.PREC_FP32, A.PREC_FP64):
         sim = scene.make(lib, prec)
-        sim.run(steps, scene.dt, 0.0)
+        scene.run(sim, steps)
         out[prec] = sim.download()
         sim.close()
     return _field_errors(out[A.PREC_FP32], out[A.PREC_FP64], ("x", "v", "C", "F", "d", "g"))
@@ -248,7 +250,7 @@ def accuracy_vs_oracle(name, scene, lib, steps=100):
     outs, grids = [], []
     for L, prec in ((lib, A.PREC_FP32), (ref, A.PREC_FP64)):
         sim = sc.make(L, prec)
-        sim.run(steps, sc.dt, 0.0)
+        sc.run(sim, steps)
         outs.append(sim.download())
         grids.append(sim.grid())
         sim.close()
@@ -318,12 +320,16 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": args.workload, "particles": scene.n, "dx": scene.dx, "dt": scene.dt,
+        "config": {"workload": args.workload, "particles": scene.n,
                    "form": "FP32 particle state, int64 fixed-point P2G", "active_nodes": nn32,
-                   "launch": cfg32, "autotuned": not args.no_autotune, "options": dict(scene.options)},
+                   "launch": cfg32, "autotuned": not args.no_autotune},
     }
-    if "counts" in scene.info:
-        result["config"]["scenes"] = len(scene.info["counts"])
+    if isinstance(scene, scenes.SceneBatch):
+        result["config"].update(scenes=len(scene.scenes), dx=sorted({s.dx for s in scene.scenes}),
+                                dt=[float(scene.dts.min()), float(scene.dts.max())],
+                                launches="one per stage for all scenes (zmpm_run_batch)")
+    else:
+        result["config"].update(dx=scene.dx, dt=scene.dt, options=dict(scene.options))
     # roofline of the dominant kernel stage (HIP events on the handle's stream)
     stage_ms = {k: v for k, v in st32.items() if k in ("p2g", "g2p", "grid_update")}
     dom = max(stage_ms, key=stage_ms.get)

@@ -303,6 +303,32 @@ zmpm_status zmpm_get_binning(zmpm_handle h, int64_t* n_groups, uint64_t* keys,
 zmpm_status zmpm_get_grid(zmpm_handle h, int64_t* n_nodes, int32_t* coords,
                           double* m, double* v, double* new_v, double* pf);
 
+/* Batched independent scenes in one handle: every launch of a substep covers
+ * all of them.  No counterpart in the reference, whose simulations run one at a
+ * time (one MpmSimulationBase each); each scene keeps the reference's
+ * semantics.  descs[s] gives scene s its own dx, gravity, apic_rpic_ratio, cfl
+ * and domain_nodes; precision, device, transfer scheme, flip_pic_ratio, mls_mpm
+ * and the damping iterations must be equal in every scene, and implicit_damage
+ * needs a single scene.  Scene s lives in its own cubic region of the handle's
+ * grid (side = the largest domain_nodes, regions in page order; at most 1024
+ * nodes per axis in all): positions in and out are metres in the scene's own
+ * frame, as if it ran alone, and zmpm_get_grid / zmpm_get_binning report the
+ * handle's grid (a scene's node (i, j, k) is node origin + (i, j, k)).  The
+ * material table is shared (particles name their material); colliders belong
+ * to one scene; statuses are latched for the whole batch.  Each scene's result
+ * equals its run alone: FP32 bit for bit (per-scene fixed-point scales), FP64
+ * to the rounding of its double atomics. */
+zmpm_status zmpm_create_batch(const zmpm_scene_desc* descs, int32_t n_scenes, zmpm_handle* out);
+/* First node of scene s's region in the handle's grid (3 int32). */
+zmpm_status zmpm_batch_origin(zmpm_handle h, int32_t scene, int32_t* origin);
+/* Collision object of one scene (zmpm_add_collider adds to scene 0). */
+zmpm_status zmpm_add_scene_collider(zmpm_handle h, int32_t scene, const zmpm_collider* c, int32_t* id_out);
+/* Upload with scene[i] the scene of particle i (zmpm_upload puts all in scene 0). */
+zmpm_status zmpm_upload_batch(zmpm_handle h, const zmpm_particles* p, const int32_t* scene);
+/* `steps` substeps of every scene s with its own step dt[s] from its own time
+ * t0[s] (zmpm_run gives every scene the same dt and t0). */
+zmpm_status zmpm_run_batch(zmpm_handle h, int32_t steps, const double* dt, const double* t0);
+
 /* Per-stage device timing (HIP events on the handle's stream) of the substeps
  * run while enabled; off by default.  No counterpart in the reference, whose
  * ZIRAN_TIMER scopes (Lib/Ziran/CS/Util/Timer.h:35-60) play this role on the CPU. */
@@ -317,8 +343,9 @@ zmpm_status zmpm_get_timings(zmpm_handle h, int32_t* count, const char** names, 
  * evalMaxParticleSpeed (:1613-1638): the largest particle speed |v| and the
  * particles' bounding box, reduced on the device (synchronous).  *dt = cfl dx /
  * max speed, or max_dt when every particle is at rest (collision-object speeds
- * are not included, as in the reference).  Output pointers may be NULL;
- * min_corner / max_corner take 3 doubles. */
+ * are not included, as in the reference); for a batch the smallest of the
+ * scenes' steps and the box over the scenes' own frames.  Output pointers may
+ * be NULL; min_corner / max_corner take 3 doubles. */
 zmpm_status zmpm_calculate_dt(zmpm_handle h, double max_dt, double* dt, double* max_speed, double* min_corner,
                               double* max_corner);
 

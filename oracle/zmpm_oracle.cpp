@@ -28,6 +28,7 @@
 #include "zmpm.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1214,7 +1215,17 @@ struct zmpm_context {
     int NB = 0;
     int latched = ZMPM_OK;
     bool stepped = false;
+    // a batch (zmpm_create_batch): one single-scene context per scene, run one
+    // after the other; particles keep their upload order through `pscene`
+    std::vector<zmpm_context*> subs;
+    std::vector<std::array<int, 3>> origins;
+    std::vector<int32_t> pscene;
+    ~zmpm_context();
 };
+zmpm_context::~zmpm_context()
+{
+    for (zmpm_context* c : subs) delete c;
+}
 
 namespace {
 
@@ -2002,6 +2013,14 @@ const char* zmpm_last_error(zmpm_handle h) { return h ? h->err.c_str() : "null h
 
 zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material* mats)
 {
+    if (h && !h->subs.empty()) {
+        for (zmpm_context* c : h->subs) {
+            const zmpm_status st = zmpm_set_materials(c, count, mats);
+            if (st != ZMPM_OK) return fail(h, st, c->err);
+        }
+        h->mats.assign(mats, mats + count);
+        return ZMPM_OK;
+    }
     if (!h || count <= 0 || count > 256 || !mats) return fail(h, ZMPM_E_ARG, "bad material table");
     for (int32_t i = 0; i < count; i++) {
         const zmpm_material& m = mats[i];
@@ -2023,6 +2042,7 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
 
 zmpm_status zmpm_add_collider(zmpm_handle h, const zmpm_collider* c, int32_t* id_out)
 {
+    if (h && !h->subs.empty()) return zmpm_add_collider(h->subs[0], c, id_out);
     if (!h || !c) return ZMPM_E_ARG;
     if (h->colliders.size() >= 16) return fail(h, ZMPM_E_ARG, "too many colliders");
     if (c->shape < ZMPM_SHAPE_HALF_SPACE || c->shape > ZMPM_SHAPE_CAPPED_CYLINDER) return fail(h, ZMPM_E_ARG, "unknown shape");
@@ -2039,14 +2059,24 @@ zmpm_status zmpm_add_collider(zmpm_handle h, const zmpm_collider* c, int32_t* id
 
 zmpm_status zmpm_clear_colliders(zmpm_handle h)
 {
+    if (h && !h->subs.empty()) {
+        for (zmpm_context* c : h->subs) zmpm_clear_colliders(c);
+        return ZMPM_OK;
+    }
     if (!h) return ZMPM_E_ARG;
     h->colliders.clear();
     return ZMPM_OK;
 }
 
+zmpm_status zmpm_upload_batch(zmpm_handle h, const zmpm_particles* p, const int32_t* scene);
+
 zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
 {
     if (!h || !p || p->n < 0) return ZMPM_E_ARG;
+    if (!h->subs.empty()) {
+        std::vector<int32_t> zero((size_t)p->n, 0);
+        return zmpm_upload_batch(h, p, zero.data());
+    }
     if (p->n > 0 && (!p->x || !p->v || !p->C || !p->F || !p->m || !p->V0 || !p->material))
         return fail(h, ZMPM_E_ARG, "missing particle array");
     if (h->mats.empty()) return fail(h, ZMPM_E_ARG, "set materials before upload");
@@ -2083,8 +2113,176 @@ zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
     return ZMPM_OK;
 }
 
+zmpm_status batchDownload(zmpm_handle h, zmpm_particles* p)
+{
+    if (!p || p->n != h->n) return fail(h, ZMPM_E_ARG, "particle count mismatch");
+    zmpm_status first = ZMPM_OK;
+    std::vector<int64_t> next(h->subs.size(), 0);
+    for (size_t s = 0; s < h->subs.size(); s++) {
+        zmpm_context* c = h->subs[s];
+        if (first == ZMPM_OK && c->latched != ZMPM_OK) first = (zmpm_status)c->latched;
+    }
+    for (int64_t i = 0; i < h->n; i++) {
+        const size_t s = (size_t)h->pscene[(size_t)i];
+        const zmpm_context* c = h->subs[s];
+        const int64_t k = next[s]++;
+        for (int a = 0; a < 3; a++) {
+            if (p->x) p->x[3 * i + a] = c->x[3 * k + a];
+            if (p->v) p->v[3 * i + a] = c->v[3 * k + a];
+        }
+        for (int a = 0; a < 9; a++) {
+            if (p->C) p->C[9 * i + a] = c->C[9 * k + a];
+            if (p->F) p->F[9 * i + a] = c->F[9 * k + a];
+        }
+        if (p->m) p->m[i] = c->m[k];
+        if (p->V0) p->V0[i] = c->V0[k];
+        if (p->material) p->material[i] = c->mat[k];
+        if (p->d) p->d[i] = c->d[k];
+        if (p->g) p->g[i] = c->g[k];
+        if (p->laplacian) p->laplacian[i] = c->lap[k];
+        if (p->plastic_state) p->plastic_state[i] = c->ys[k];
+        if (p->max_dtilde) p->max_dtilde[i] = c->dtil[k];
+    }
+    return first;
+}
+
+// ---- batches: scenes as separate contexts (a batch's scenes never interact)
+zmpm_status zmpm_create_batch(const zmpm_scene_desc* descs, int32_t n_scenes, zmpm_handle* out)
+{
+    if (!descs || !out || n_scenes < 1 || n_scenes > 4096) return ZMPM_E_ARG;
+    const zmpm_scene_desc& d0 = descs[0];
+    int region = 0;
+    for (int32_t s = 0; s < n_scenes; s++) {
+        const zmpm_scene_desc& d = descs[s];
+        if (d.precision != d0.precision || d.device != d0.device || d.transfer_scheme != d0.transfer_scheme
+            || d.flip_pic_ratio != d0.flip_pic_ratio || d.mls_mpm != d0.mls_mpm
+            || d.rpic_damping_iteration != d0.rpic_damping_iteration || d.pic_damping_iteration != d0.pic_damping_iteration
+            || d.implicit_damage != d0.implicit_damage || !(d.cfl > 0))
+            return ZMPM_E_ARG;
+        region = std::max(region, d.domain_nodes);
+    }
+    if (n_scenes > 1 && d0.implicit_damage) return ZMPM_E_ARG;
+    int rd = 1;
+    while (rd * rd * rd < n_scenes) rd++;
+    int grid = 1;
+    while (grid < rd * region) grid <<= 1;
+    if (grid > 1024) return ZMPM_E_ARG;
+    auto* h = new zmpm_context();
+    h->desc = d0;
+    h->L = makeLayout(d0.precision == ZMPM_PREC_FP32 ? 6 : 7);
+    h->NB = 1 << h->L.block_bits;
+    for (int32_t s = 0; s < n_scenes; s++) {
+        zmpm_handle c = nullptr;
+        if (zmpm_create(&descs[s], &c) != ZMPM_OK) {
+            delete h;
+            return ZMPM_E_ARG;
+        }
+        h->subs.push_back(c);
+    }
+    // the regions' layout of the GPU library (page order of each region's first node)
+    std::vector<std::pair<uint64_t, int>> order;
+    for (int r = 0; r < rd * rd * rd; r++)
+        order.push_back({ linearOffset(h->L, (r / (rd * rd)) * region, ((r / rd) % rd) * region, (r % rd) * region), r });
+    std::sort(order.begin(), order.end());
+    for (int32_t s = 0; s < n_scenes; s++) {
+        const int r = order[(size_t)s].second;
+        h->origins.push_back({ (r / (rd * rd)) * region, ((r / rd) % rd) * region, (r % rd) * region });
+    }
+    *out = h;
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_batch_origin(zmpm_handle h, int32_t scene, int32_t* origin)
+{
+    if (!h || !origin) return ZMPM_E_ARG;
+    if (h->subs.empty()) {
+        if (scene != 0) return ZMPM_E_ARG;
+        origin[0] = origin[1] = origin[2] = 0;
+        return ZMPM_OK;
+    }
+    if (scene < 0 || scene >= (int32_t)h->subs.size()) return ZMPM_E_ARG;
+    for (int a = 0; a < 3; a++) origin[a] = h->origins[(size_t)scene][a];
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_add_scene_collider(zmpm_handle h, int32_t scene, const zmpm_collider* c, int32_t* id_out)
+{
+    if (!h) return ZMPM_E_ARG;
+    if (h->subs.empty()) return scene == 0 ? zmpm_add_collider(h, c, id_out) : fail(h, ZMPM_E_ARG, "no such scene");
+    if (scene < 0 || scene >= (int32_t)h->subs.size()) return fail(h, ZMPM_E_ARG, "no such scene");
+    return zmpm_add_collider(h->subs[(size_t)scene], c, id_out);
+}
+
+zmpm_status zmpm_upload_batch(zmpm_handle h, const zmpm_particles* p, const int32_t* scene)
+{
+    if (!h || !p || p->n < 0) return ZMPM_E_ARG;
+    if (p->n > 0 && !scene) return fail(h, ZMPM_E_ARG, "missing scene array");
+    const size_t ns = h->subs.empty() ? 1 : h->subs.size();
+    for (int64_t i = 0; i < p->n; i++)
+        if (scene[i] < 0 || scene[i] >= (int32_t)ns) return fail(h, ZMPM_E_ARG, "scene index out of range");
+    if (h->subs.empty()) return zmpm_upload(h, p);
+    h->pscene.assign(scene, scene + p->n);
+    h->n = p->n;
+    for (size_t s = 0; s < ns; s++) {
+        // the scene's particles in upload order
+        std::vector<int64_t> idx;
+        for (int64_t i = 0; i < p->n; i++)
+            if (scene[i] == (int32_t)s) idx.push_back(i);
+        const int64_t k = (int64_t)idx.size();
+        std::vector<double> x(3 * k), v(3 * k), C(9 * k), F(9 * k), m(k), V0(k), d(k), g(k), ys(k), dt(k);
+        std::vector<int32_t> mat(k);
+        for (int64_t j = 0; j < k; j++) {
+            const int64_t i = idx[(size_t)j];
+            for (int a = 0; a < 3; a++) {
+                x[3 * j + a] = p->x[3 * i + a];
+                v[3 * j + a] = p->v[3 * i + a];
+            }
+            for (int a = 0; a < 9; a++) {
+                C[9 * j + a] = p->C[9 * i + a];
+                F[9 * j + a] = p->F[9 * i + a];
+            }
+            m[j] = p->m[i];
+            V0[j] = p->V0[i];
+            mat[j] = p->material[i];
+            if (p->d) d[j] = p->d[i];
+            if (p->g) g[j] = p->g[i];
+            if (p->plastic_state) ys[j] = p->plastic_state[i];
+            if (p->max_dtilde) dt[j] = p->max_dtilde[i];
+        }
+        zmpm_particles q {};
+        q.n = k;
+        q.x = x.data();
+        q.v = v.data();
+        q.C = C.data();
+        q.F = F.data();
+        q.m = m.data();
+        q.V0 = V0.data();
+        q.material = mat.data();
+        q.d = p->d ? d.data() : nullptr;
+        q.g = p->g ? g.data() : nullptr;
+        q.plastic_state = p->plastic_state ? ys.data() : nullptr;
+        q.max_dtilde = p->max_dtilde ? dt.data() : nullptr;
+        const zmpm_status st = zmpm_upload(h->subs[s], &q);
+        if (st != ZMPM_OK) return fail(h, st, h->subs[s]->err);
+    }
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_run_batch(zmpm_handle h, int32_t steps, const double* dt, const double* t0)
+{
+    if (!h || steps < 0 || !dt || !t0) return ZMPM_E_ARG;
+    if (h->subs.empty()) return zmpm_run(h, steps, dt[0], t0[0]);
+    zmpm_status first = ZMPM_OK;
+    for (size_t s = 0; s < h->subs.size(); s++) {
+        const zmpm_status st = zmpm_run(h->subs[s], steps, dt[s], t0[s]);
+        if (first == ZMPM_OK) first = st;
+    }
+    return first;
+}
+
 zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
 {
+    if (h && !h->subs.empty()) return batchDownload(h, p);
     if (!h || !p || p->n != h->n) return fail(h, ZMPM_E_ARG, "particle count mismatch");
     const int64_t n = h->n;
     if (p->x) std::copy(h->x.begin(), h->x.end(), p->x);
@@ -2107,6 +2305,14 @@ zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
 zmpm_status zmpm_substep(zmpm_handle h, double dt, double t)
 {
     if (!h) return ZMPM_E_ARG;
+    if (!h->subs.empty()) {
+        zmpm_status first = ZMPM_OK;
+        for (zmpm_context* c : h->subs) {
+            const zmpm_status st = zmpm_substep(c, dt, t);
+            if (first == ZMPM_OK) first = st;
+        }
+        return first;
+    }
     if (h->latched != ZMPM_OK && h->latched != ZMPM_E_CFL) return (zmpm_status)h->latched;
     return substep(h, dt, t);
 }
@@ -2124,11 +2330,17 @@ zmpm_status zmpm_run(zmpm_handle h, int32_t steps, double dt, double t0)
 zmpm_status zmpm_synchronize(zmpm_handle h)
 {
     if (!h) return ZMPM_E_ARG;
+    if (!h->subs.empty()) {
+        for (zmpm_context* c : h->subs)
+            if (c->latched != ZMPM_OK) return (zmpm_status)c->latched;
+        return ZMPM_OK;
+    }
     return (zmpm_status)h->latched;
 }
 
 zmpm_status zmpm_get_binning(zmpm_handle h, int64_t* n_groups, uint64_t* keys, int32_t* order, int32_t* groups, uint64_t* block_offset)
 {
+    if (h && !h->subs.empty()) return fail(h, ZMPM_E_ARG, "the oracle reports a batch's binning per scene only (a handle per scene)");
     if (!h || !n_groups) return ZMPM_E_ARG;
     *n_groups = (int64_t)h->groups.size();
     if (keys) std::copy(h->keys.begin(), h->keys.end(), keys);
@@ -2144,6 +2356,7 @@ zmpm_status zmpm_get_binning(zmpm_handle h, int64_t* n_groups, uint64_t* keys, i
 
 zmpm_status zmpm_get_grid(zmpm_handle h, int64_t* n_nodes, int32_t* coords, double* m, double* v, double* new_v, double* pf)
 {
+    if (h && !h->subs.empty()) return fail(h, ZMPM_E_ARG, "the oracle reports a batch's grid per scene only (a handle per scene)");
     if (!h || !n_nodes) return ZMPM_E_ARG;
     struct Rec {
         int c[3];
@@ -2187,6 +2400,35 @@ zmpm_status zmpm_set_timing(zmpm_handle h, int32_t)
 zmpm_status zmpm_calculate_dt(zmpm_handle h, double max_dt, double* dt, double* max_speed, double* min_corner,
                               double* max_corner)
 {
+    if (h && !h->subs.empty()) {
+        // the smallest scene step, the largest speed and the box over the scenes' frames
+        if (!(max_dt > 0)) return ZMPM_E_ARG;
+        double best = max_dt, sp = 0, lo[3] = { 3.4028234663852886e38, 3.4028234663852886e38, 3.4028234663852886e38 },
+               hi[3] = { -3.4028234663852886e38, -3.4028234663852886e38, -3.4028234663852886e38 };
+        bool any = false;
+        for (zmpm_context* c : h->subs) {
+            if (c->n == 0) continue;
+            double d, s1, l[3], u[3];
+            const zmpm_status st = zmpm_calculate_dt(c, max_dt, &d, &s1, l, u);
+            if (st != ZMPM_OK) return st;
+            if (s1 != 0) {
+                best = any ? std::min(best, d) : d;
+                any = true;
+            }
+            sp = std::max(sp, s1);
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], l[a]);
+                hi[a] = std::max(hi[a], u[a]);
+            }
+        }
+        if (dt) *dt = best;
+        if (max_speed) *max_speed = sp;
+        for (int a = 0; a < 3; a++) {
+            if (min_corner) min_corner[a] = lo[a];
+            if (max_corner) max_corner[a] = hi[a];
+        }
+        return ZMPM_OK;
+    }
     if (!h || !(max_dt > 0)) return ZMPM_E_ARG;
     const double lowest = (double)std::numeric_limits<float>::lowest();
     double r[7] = { 0.0, lowest, lowest, lowest, lowest, lowest, lowest };

@@ -11,9 +11,10 @@ def normwise(a, b):
     return num / den if den > 0 else num
 
 
-def run(lib, scene, precision, steps, t0=0.0, **kw):
+def run(lib, scene, precision, steps, start=0, **kw):
+    """A handle of the scene (or scene batch) after substeps start .. start + steps - 1."""
     sim = scene.make(lib, precision, **kw)
-    sim.run(steps, scene.dt, t0)
+    scene.run(sim, steps, start)
     return sim
 
 

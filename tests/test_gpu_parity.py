@@ -226,8 +226,8 @@ def test_two_material(gpu, oracle, precision):
         assert normwise(g[k], o[k]) < tol, k
 
 
-def test_batched_scenes_small(gpu, oracle):
-    sc = scenes.batched_scenes(n_scenes=8, n_range=(500, 4000), domain_nodes=256)
+def test_shared_grid_scenes_small(gpu, oracle):
+    sc = scenes.shared_grid_scenes(n_scenes=8, n_range=(500, 4000), domain_nodes=256)
     g = run(gpu, sc, A.PREC_FP64, 5).download()
     o = run(oracle, sc, A.PREC_FP64, 5).download()
     for k in ("x", "v", "F", "d"):
@@ -237,7 +237,7 @@ def test_batched_scenes_small(gpu, oracle):
 FP32_TOL_100 = 2e-5  # FP32 form vs FP64 form after 100 substeps, every field
 
 
-@pytest.mark.parametrize("name", ["elastic_cube", "two_material", "batched_scenes"])
+@pytest.mark.parametrize("name", ["elastic_cube", "two_material", "shared_grid_scenes", "batched_scenes"])
 def test_fp32_error_after_100_substeps(gpu, name):
     # The FP64 form equals the FP64 oracle to 1e-9 (tests above); the FP32
     # form, with fixed-point positions and H = F - I, stays within 2e-5 of it
@@ -247,6 +247,8 @@ def test_fp32_error_after_100_substeps(gpu, name):
         sc = scenes.elastic_cube()
     elif name == "two_material":
         sc = scenes.two_material(n=1_000_000, dx=1.0 / 128)
+    elif name == "shared_grid_scenes":
+        sc = scenes.shared_grid_scenes(n_scenes=8)
     else:
         sc = scenes.batched_scenes(n_scenes=8)
     a = run(gpu, sc, A.PREC_FP32, 100).download()
@@ -737,7 +739,7 @@ def test_implicit_damage_mixed_materials(gpu, oracle, precision):
     # scenes with and without the phase field in one grid: only the damaged
     # materials' particles take part in the solve; nodes reached only by the
     # others keep a zero diagonal
-    sc = scenes.batched_scenes(n_scenes=6, n_range=(3000, 8000), domain_nodes=256, sigma_crit=20.0)
+    sc = scenes.shared_grid_scenes(n_scenes=6, n_range=(3000, 8000), domain_nodes=256, sigma_crit=20.0)
     sc.materials[1].eta = 1e-3
     damaged = sc.material == 1
     F = sc.F.reshape(-1, 3, 3)

@@ -238,7 +238,8 @@ class ZmpmLib:
         "zmpm_set_materials", "zmpm_add_collider", "zmpm_clear_colliders", "zmpm_upload",
         "zmpm_download", "zmpm_substep", "zmpm_run", "zmpm_synchronize", "zmpm_get_binning",
         "zmpm_get_grid", "zmpm_set_timing", "zmpm_get_timings", "zmpm_calculate_dt", "zmpm_get_launch_config",
-        "zmpm_set_launch_config", "zmpm_autotune",
+        "zmpm_set_launch_config", "zmpm_autotune", "zmpm_create_batch", "zmpm_batch_origin",
+        "zmpm_add_scene_collider", "zmpm_upload_batch", "zmpm_run_batch",
     )
 
     def __init__(self, path: str):
@@ -276,6 +277,11 @@ class ZmpmLib:
         L.zmpm_set_launch_config.argtypes = [C.c_void_p, C.POINTER(LaunchConfig)]
         L.zmpm_autotune.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                     C.POINTER(LaunchConfig)]
+        L.zmpm_create_batch.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.POINTER(C.c_void_p)]
+        L.zmpm_batch_origin.argtypes = [C.c_void_p, C.c_int32, _IP]
+        L.zmpm_add_scene_collider.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Collider), C.POINTER(C.c_int32)]
+        L.zmpm_upload_batch.argtypes = [C.c_void_p, C.POINTER(Particles), _IP]
+        L.zmpm_run_batch.argtypes = [C.c_void_p, C.c_int32, _DP, _DP]
         if L.zmpm_abi_version() != ZMPM_ABI_VERSION:
             raise RuntimeError(f"{path}: ABI version {L.zmpm_abi_version()} != {ZMPM_ABI_VERSION}")
 
@@ -284,22 +290,27 @@ class ZmpmLib:
         return self.lib.zmpm_build_info().decode()
 
 
+def scene_desc(*, dx, precision=PREC_FP64, domain_nodes=256, gravity=(0, 0, 0), apic_rpic_ratio=1.0, cfl=0.6,
+               device=0, transfer_scheme=APIC_BLEND_RPIC, flip_pic_ratio=0.0, mls_mpm=False, rpic_damping_iteration=0,
+               pic_damping_iteration=0, implicit_damage=False) -> SceneDesc:
+    d = SceneDesc()
+    d.precision, d.domain_nodes, d.dx = precision, domain_nodes, dx
+    for a in range(3):
+        d.gravity[a] = gravity[a]
+    d.apic_rpic_ratio, d.cfl, d.device = apic_rpic_ratio, cfl, device
+    d.transfer_scheme, d.flip_pic_ratio, d.mls_mpm = transfer_scheme, flip_pic_ratio, int(bool(mls_mpm))
+    d.rpic_damping_iteration, d.pic_damping_iteration = rpic_damping_iteration, pic_damping_iteration
+    d.implicit_damage = int(implicit_damage)
+    return d
+
+
 class Simulation:
     """One zmpm handle: scene, materials, colliders and particle state."""
 
-    def __init__(self, lib: ZmpmLib, *, dx, precision=PREC_FP64, domain_nodes=256, gravity=(0, 0, 0),
-                 apic_rpic_ratio=1.0, cfl=0.6, device=0, transfer_scheme=APIC_BLEND_RPIC, flip_pic_ratio=0.0,
-                 mls_mpm=False, rpic_damping_iteration=0, pic_damping_iteration=0, implicit_damage=False):
+    def __init__(self, lib: ZmpmLib, **desc):
         self.lib = lib
         self.L = lib.lib
-        d = SceneDesc()
-        d.precision, d.domain_nodes, d.dx = precision, domain_nodes, dx
-        for a in range(3):
-            d.gravity[a] = gravity[a]
-        d.apic_rpic_ratio, d.cfl, d.device = apic_rpic_ratio, cfl, device
-        d.transfer_scheme, d.flip_pic_ratio, d.mls_mpm = transfer_scheme, flip_pic_ratio, int(bool(mls_mpm))
-        d.rpic_damping_iteration, d.pic_damping_iteration = rpic_damping_iteration, pic_damping_iteration
-        d.implicit_damage = int(implicit_damage)
+        d = scene_desc(**desc)
         self.desc = d
         h = C.c_void_p()
         self._check(self.L.zmpm_create(C.byref(d), C.byref(h)), "zmpm_create", handle=False)
@@ -338,7 +349,8 @@ class Simulation:
         self._check(self.L.zmpm_add_collider(self.h, C.byref(c), C.byref(i)), "zmpm_add_collider")
         return i.value
 
-    def upload(self, x, v, Cm, F, m, V0, material, d=None, g=None, plastic_state=None, max_dtilde=None):
+    def upload(self, x, v, Cm, F, m, V0, material, d=None, g=None, plastic_state=None, max_dtilde=None,
+               scene=None):
         n = int(x.shape[0])
         arrs = dict(
             x=np.ascontiguousarray(x, np.float64).reshape(n, 3),
@@ -360,7 +372,12 @@ class Simulation:
         p.material = _ptr(arrs["material"], C.c_int32)
         p.laplacian = None
         self._keep = arrs
-        self._check(self.L.zmpm_upload(self.h, C.byref(p)), "zmpm_upload")
+        if scene is None:
+            self._check(self.L.zmpm_upload(self.h, C.byref(p)), "zmpm_upload")
+        else:
+            arrs["scene"] = np.ascontiguousarray(scene, np.int32)
+            self._check(self.L.zmpm_upload_batch(self.h, C.byref(p), _ptr(arrs["scene"], C.c_int32)),
+                        "zmpm_upload_batch")
         self.n = n
 
     def download(self, check=True):
@@ -446,3 +463,35 @@ class Simulation:
         ms = np.zeros(64)
         self._check(self.L.zmpm_get_timings(self.h, C.byref(cnt), names, _ptr(ms)), "zmpm_get_timings")
         return {names[i].decode(): float(ms[i]) for i in range(cnt.value)}
+
+
+class Batch(Simulation):
+    """One handle running several independent scenes (zmpm_create_batch): each
+    scene its own dx, dt, gravity, transfer ratio, CFL number and colliders."""
+
+    def __init__(self, lib: ZmpmLib, descs):
+        self.lib = lib
+        self.L = lib.lib
+        arr = (SceneDesc * len(descs))(*[scene_desc(**d) for d in descs])
+        self.descs = arr
+        self.desc = arr[0]
+        self.n_scenes = len(descs)
+        h = C.c_void_p()
+        self._check(self.L.zmpm_create_batch(arr, len(descs), C.byref(h)), "zmpm_create_batch", handle=False)
+        self.h = h
+        self.n = 0
+
+    def add_scene_collider(self, scene: int, c: Collider) -> int:
+        i = C.c_int32()
+        self._check(self.L.zmpm_add_scene_collider(self.h, scene, C.byref(c), C.byref(i)), "zmpm_add_scene_collider")
+        return i.value
+
+    def origin(self, scene: int):
+        o = np.zeros(3, np.int32)
+        self._check(self.L.zmpm_batch_origin(self.h, scene, _ptr(o, C.c_int32)), "zmpm_batch_origin")
+        return o
+
+    def run_batch(self, steps, dts, t0s):
+        dts = np.ascontiguousarray(dts, np.float64)
+        t0s = np.ascontiguousarray(t0s, np.float64)
+        return self._check(self.L.zmpm_run_batch(self.h, steps, _ptr(dts), _ptr(t0s)), "zmpm_run_batch")

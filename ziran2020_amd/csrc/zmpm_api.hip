@@ -4,6 +4,7 @@
 #include "zmpm_device.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -46,13 +47,27 @@ uint32_t hostBitPack(uint64_t data, uint64_t mask)
 } // namespace
 
 struct zmpm_context {
-    zmpm_scene_desc desc {};
+    zmpm_scene_desc desc {}; // scene 0's description: the options every scene shares
     std::string err;
     hipStream_t stream = nullptr;
     bool fp32 = false;
     bool any_damage = false;
     std::vector<zmpm_material> mats;
-    std::vector<zmpm_collider> cols;
+    // scenes: one (zmpm_create) or a batch (zmpm_create_batch), each in its own
+    // cubic region of the grid, regions laid out in page (Morton) order
+    std::vector<zmpm_scene_desc> scenes;
+    std::vector<std::vector<zmpm_collider>> scene_cols; // per scene, insertion order
+    int region = 0;                  // region side in nodes (the largest scene domain)
+    int region_dim = 1;              // regions per axis
+    int grid_domain = 0;             // nodes per axis of the whole grid
+    std::vector<int> region_scene;   // [region_dim^3] scene or -1
+    std::vector<std::array<int, 3>> origin; // per scene: first node of its region
+    std::vector<int32_t> pscene;     // scene of each uploaded particle (upload order)
+    std::vector<double> step_dt;     // per scene: dt the device table holds (0: none yet)
+    std::vector<unsigned char> scene_host; // host mirror of the device scene table
+    void* scenes_dev = nullptr;
+    void* regions_dev = nullptr;
+    void* colstate_dev = nullptr;
     Dev<float, long long> Df {};
     Dev<double, double> Dd {};
     std::vector<void*> allocs; // particle + grid buffers (freed on reallocation)
@@ -66,11 +81,10 @@ struct zmpm_context {
     int timed_steps = 0;
     double timed_ms[kTimedGroups + 1] = {};
     bool stepped = false;
-    double clock_host = 0;
-    // captured two-substep graphs, keyed by live buffer and dt
+    std::vector<double> clock_host; // per scene: next and running substep time (upload source)
+    // captured two-substep graphs, keyed by live buffer (dt and t are device state)
     struct Graph {
         const void* live;
-        double dt;
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
@@ -141,7 +155,8 @@ zmpm_status allocate(zmpm_context* h, int n)
     Dev<T, AccT> D {};
     using B = Blk<T>;
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
-    const int64_t dom = h->desc.domain_nodes;
+    const int64_t dom = h->grid_domain;
+    const int64_t ns = (int64_t)h->scenes.size();
     const int64_t cap = std::max(n, 1);
     D.cap = cap;
     D.domain = (int)dom;
@@ -181,13 +196,13 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.cell_count, D.cap_nodes);
     ALLOC(D.cell_start, D.cap_nodes);
     ALLOC(D.counters, 8);
-    ALLOC(D.clock, 2);
+    ALLOC(D.clock, 2 * ns);
     ALLOC(D.acc, (int64_t)NGF * D.cap_nodes);
     ALLOC(D.newv, 3 * D.cap_nodes);
     ALLOC(D.oldv, 3 * D.cap_nodes);
     ALLOC(D.pfn, D.cap_nodes);
-    ALLOC(D.scales, NSC);
-    ALLOC(D.maxbits, (int64_t)NSC * BOUND_SLOTS);
+    ALLOC(D.scales, NSC * ns);
+    ALLOC(D.maxbits, (int64_t)NSC * BOUND_SLOTS * ns);
     ALLOC(D.extent, (int64_t)(EXTENT_WGS + 1) * 8);
     if (h->desc.implicit_damage) {
         ALLOC(D.cgv, 5 * D.cap_nodes);
@@ -216,14 +231,57 @@ zmpm_status allocate(zmpm_context* h, int n)
     HIPCHK(h, hipMemsetAsync(D.sorted_cell, 0, cap * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.spid, 0, cap * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.counters, 0, 8 * sizeof(int), h->stream));
-    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * BOUND_SLOTS * sizeof(unsigned int), h->stream));
-    float ones[NSC] = { 1.f, 1.f, 1.f, 1.f, 1.f };
-    HIPCHK(h, hipMemcpyAsync(D.scales, ones, sizeof(ones), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, ns * NSC * BOUND_SLOTS * sizeof(unsigned int), h->stream));
+    const std::vector<float> ones((size_t)(NSC * ns), 1.f);
+    HIPCHK(h, hipMemcpyAsync(D.scales, ones.data(), ones.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemsetAsync(D.clock, 0, 2 * ns * sizeof(double), h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     D.masks.x = spreadMask(B::data_bits, 0);
     D.masks.y = spreadMask(B::data_bits, 1);
     D.masks.z = spreadMask(B::data_bits, 2);
     devOf<T, AccT>(h) = D;
     h->cap = cap;
+    return ZMPM_OK;
+}
+
+// Collision objects of every scene, grouped by scene in insertion order.
+size_t colliderCount(const zmpm_context* h)
+{
+    size_t c = 0;
+    for (const auto& v : h->scene_cols) c += v.size();
+    return c;
+}
+
+// The device scene table from the scene descriptions and the current per-scene
+// dt (the host mirror stays alive for the asynchronous copy).
+template <class T>
+zmpm_status uploadScenes(zmpm_context* h)
+{
+    const size_t ns = h->scenes.size();
+    h->scene_host.assign(ns * sizeof(SceneDev<T>), 0);
+    SceneDev<T>* sd = reinterpret_cast<SceneDev<T>*>(h->scene_host.data());
+    int col0 = 0;
+    for (size_t s = 0; s < ns; s++) {
+        const zmpm_scene_desc& d = h->scenes[s];
+        SceneDev<T>& e = sd[s];
+        e.dx = (T)d.dx;
+        e.ddx = d.dx;
+        e.odx = T(1) / (T)d.dx;
+        e.pos_scale = (T)((double)(1 << POS_BITS) / d.dx);
+        e.Dinv = T(4) / ((T)d.dx * (T)d.dx);
+        for (int a = 0; a < 3; a++) e.gravity[a] = (T)d.gravity[a];
+        e.rpic = (T)d.apic_rpic_ratio;
+        e.cfl = (T)d.cfl;
+        e.dt = (T)h->step_dt[s];
+        e.ddt = h->step_dt[s];
+        e.inv_cfl_dx = 1.0 / (d.cfl * d.dx);
+        for (int a = 0; a < 3; a++) e.origin[a] = h->origin[s][a];
+        e.domain = d.domain_nodes;
+        e.col0 = col0;
+        e.ncol = (int)h->scene_cols[s].size();
+        col0 += e.ncol;
+    }
+    HIPCHK(h, hipMemcpyAsync(h->scenes_dev, sd, ns * sizeof(SceneDev<T>), hipMemcpyHostToDevice, h->stream));
     return ZMPM_OK;
 }
 
@@ -233,17 +291,18 @@ void refreshTables(zmpm_context* h)
     Dev<T, AccT>& D = devOf<T, AccT>(h);
     D.mats = (const MatDev<T>*)h->mats_dev;
     D.cols = (const ColDev*)h->cols_dev;
-    D.num_cols = (int)h->cols.size();
+    D.colstate = (ColState*)h->colstate_dev;
+    D.num_cols = (int)colliderCount(h);
+    D.sc = (const SceneDev<T>*)h->scenes_dev;
+    D.nscenes = (int)h->scenes.size();
+    D.region_scene = (const int*)h->regions_dev;
+    D.region_dim = h->region_dim;
+    int rs = 0;
+    while ((1 << rs) < h->region) rs++;
+    D.region_shift = rs;
     D.any_damage = h->any_damage ? 1 : 0;
     D.any_plastic = 0;
     for (const auto& m : h->mats) D.any_plastic |= m.plasticity != ZMPM_PLASTICITY_NONE ? 1 : 0;
-    D.dx = (T)h->desc.dx;
-    D.odx = T(1) / (T)h->desc.dx;
-    D.pos_scale = (T)((double)(1 << POS_BITS) / h->desc.dx);
-    D.Dinv = T(4) / ((T)h->desc.dx * (T)h->desc.dx);
-    for (int a = 0; a < 3; a++) D.gravity[a] = (T)h->desc.gravity[a];
-    D.rpic = (T)h->desc.apic_rpic_ratio;
-    D.cfl = (T)h->desc.cfl;
     D.p2g_split = h->p2g_split;
     D.flip = h->desc.transfer_scheme == ZMPM_FLIP_BLEND_PIC ? 1 : 0;
     D.mls = h->desc.mls_mpm ? 1 : 0;
@@ -251,6 +310,7 @@ void refreshTables(zmpm_context* h)
     D.implicit = h->desc.implicit_damage ? 1 : 0;
     // ConjugateGradient's relative tolerance for its scalar type (ConjugateGradient.h:70)
     D.cg_rel = std::is_same<T, float>::value ? 1e-6 : 1e-12;
+    (void)uploadScenes<T>(h);
 }
 
 template <class T>
@@ -309,10 +369,14 @@ zmpm_status uploadMaterials(zmpm_context* h)
 
 zmpm_status uploadColliders(zmpm_context* h)
 {
-    std::vector<ColDev> cd(kMaxColliders);
-    for (size_t i = 0; i < h->cols.size(); i++) {
-        const zmpm_collider& c = h->cols[i];
+    std::vector<ColDev> cd(std::max<size_t>(colliderCount(h), 1));
+    std::vector<std::pair<int, const zmpm_collider*>> flat;
+    for (size_t sc = 0; sc < h->scene_cols.size(); sc++)
+        for (const zmpm_collider& c : h->scene_cols[sc]) flat.push_back({ (int)sc, &c });
+    for (size_t i = 0; i < flat.size(); i++) {
+        const zmpm_collider& c = *flat[i].second;
         ColDev& d = cd[i];
+        d.scene = flat[i].first;
         d.shape = c.shape;
         d.type = c.type;
         double nn = std::sqrt(c.normal[0] * c.normal[0] + c.normal[1] * c.normal[1] + c.normal[2] * c.normal[2]);
@@ -346,7 +410,8 @@ zmpm_status uploadColliders(zmpm_context* h)
         const bool identity = d.q0[0] == 1 && d.q0[1] == 0 && d.q0[2] == 0 && d.q0[3] == 0;
         d.moving = (identity && c.omega[0] == 0 && c.omega[1] == 0 && c.omega[2] == 0 && d.s0 == 1 && c.dsdt == 0) ? 0 : 1;
     }
-    HIPCHK(h, hipMemcpyAsync(h->cols_dev, cd.data(), sizeof(ColDev) * kMaxColliders, hipMemcpyHostToDevice, h->stream));
+    if (!flat.empty())
+        HIPCHK(h, hipMemcpyAsync(h->cols_dev, cd.data(), sizeof(ColDev) * flat.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return ZMPM_OK;
 }
@@ -359,7 +424,7 @@ int gridFor(int64_t items, int per, int maxg)
 }
 
 template <class T, class AccT>
-void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
+void launchStep(zmpm_context* h, hipEvent_t* ev)
 {
     Dev<T, AccT>& D = devOf<T, AccT>(h);
     hipStream_t st = h->stream;
@@ -367,26 +432,26 @@ void launchStep(zmpm_context* h, double dt, hipEvent_t* ev)
     if (ev) (void)hipEventRecord(ev[0], st);
     k_page_scan<T, AccT><<<(int)((D.num_pages + PSCAN_TILE - 1) / PSCAN_TILE), WG, 0, st>>>(D);
     const int prep = (int)std::min<int64_t>(D.cap_blocks, h->prep_wgs);
-    k_scatter<T, AccT><<<gp + prep + 1, WG, 0, st>>>(D, dt, gp, prep);
+    k_scatter<T, AccT><<<gp + prep + 1, WG, 0, st>>>(D, gp, prep);
     if (ev) (void)hipEventRecord(ev[1], st);
     if (D.mls)
         k_p2g<T, AccT, P2G_MLS><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
     else
         k_p2g<T, AccT, P2G_STD><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[2], st);
-    k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D, (T)dt);
+    k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
     if (ev) (void)hipEventRecord(ev[3], st);
     if (D.implicit && D.any_damage) {
         // AnisoFractureSimulation::solveDamageSystem (AnisoFractureSimulation.h:435-612)
-        k_dtilde<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D, (T)dt);
+        k_dtilde<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
         k_cg<T, AccT><<<D.cg_wgs, CGW, 0, st>>>(D);
     }
     if (ev) (void)hipEventRecord(ev[4], st);
     if (D.flip)
-        k_g2p<T, AccT, true><<<D.max_slots, WG, 0, st>>>(D, (T)dt);
+        k_g2p<T, AccT, true><<<D.max_slots, WG, 0, st>>>(D);
     else
-        k_g2p<T, AccT, false><<<D.max_slots, WG, 0, st>>>(D, (T)dt);
-    k_rare<T, AccT><<<std::min(D.max_slots, 1024), WG, 0, st>>>(D, (T)dt);
+        k_g2p<T, AccT, false><<<D.max_slots, WG, 0, st>>>(D);
+    k_rare<T, AccT><<<std::min(D.max_slots, 1024), WG, 0, st>>>(D);
     std::swap(D.P, D.Q);
     std::swap(D.pid, D.qid);
     std::swap(D.pmat, D.qmat);
@@ -441,7 +506,7 @@ double decodePos(int X, double dx)
 }
 
 template <class T, class AccT>
-zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
+zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p, const int32_t* scene)
 {
     const int n = (int)p->n;
     zmpm_status s;
@@ -454,14 +519,23 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
     const int64_t cap = D.cap;
     constexpr bool F32 = std::is_same<T, float>::value;
     std::vector<T> buf((size_t)NPF * cap, T(0));
+    h->pscene.assign((size_t)n, 0);
     for (int i = 0; i < n; i++) {
+        const int sc = scene ? scene[i] : 0;
+        h->pscene[(size_t)i] = sc;
+        const zmpm_scene_desc& sd = h->scenes[(size_t)sc];
         for (int a = 0; a < 3; a++) {
             if constexpr (F32) {
-                const int X = encodePos(p->x[3 * i + a], h->desc.dx);
+                // position units of the whole grid: the scene's region starts at its origin
+                int X = encodePos(p->x[3 * i + a], sd.dx);
+                if (X != POS_NAN) {
+                    const int64_t Xg = (int64_t)X + ((int64_t)h->origin[(size_t)sc][a] << POS_BITS);
+                    X = (int)std::min<int64_t>(std::max<int64_t>(Xg, POS_MIN), POS_MAX);
+                }
                 std::memcpy(&buf[(PX + a) * cap + i], &X, sizeof(int));
             }
             else
-                buf[(PX + a) * cap + i] = (T)p->x[3 * i + a];
+                buf[(PX + a) * cap + i] = (T)p->x[3 * i + a]; // metres in the scene's frame
             buf[(PV + a) * cap + i] = (T)p->v[3 * i + a];
         }
         for (int k = 0; k < 9; k++) {
@@ -484,7 +558,7 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
     std::vector<int> ids(cap, 0), mat(cap, 0);
     for (int i = 0; i < n; i++) {
         ids[i] = i;
-        mat[i] = p->material[i];
+        mat[i] = p->material[i] | (h->pscene[(size_t)i] << PMAT_SCENE_SHIFT);
     }
     HIPCHK(h, hipMemcpyAsync(D.P, buf.data(), buf.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(D.pid, ids.data(), cap * sizeof(int), hipMemcpyHostToDevice, h->stream));
@@ -495,7 +569,7 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p)
     // the page flags and counts the last G2P may have left for a substep that
     // did not run
     D.max_slots = std::max(1, (n + WG - 1) / WG);
-    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * BOUND_SLOTS * sizeof(unsigned int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, h->scenes.size() * NSC * BOUND_SLOTS * sizeof(unsigned int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.touched, 0, D.num_pages * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.page_count, 0, D.num_pages * sizeof(int), h->stream));
     if (n > 0) {
@@ -512,7 +586,7 @@ template <class T, class AccT>
 zmpm_status restress(zmpm_context* h)
 {
     Dev<T, AccT>& D = devOf<T, AccT>(h);
-    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, NSC * BOUND_SLOTS * sizeof(unsigned int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.maxbits, 0, h->scenes.size() * NSC * BOUND_SLOTS * sizeof(unsigned int), h->stream));
     k_stress_init<T, AccT><<<D.max_slots, WG, 0, h->stream>>>(D, 0);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipGetLastError());
@@ -534,12 +608,15 @@ zmpm_status downloadT(zmpm_context* h, zmpm_particles* p)
     for (int s = 0; s < n; s++) {
         const int i = ids[s];
         if (i < 0 || i >= n) return fail(h, ZMPM_E_HIP, "corrupt particle id table");
+        const int sc = mat[s] >> PMAT_SCENE_SHIFT;
+        if (sc < 0 || sc >= (int)h->scenes.size()) return fail(h, ZMPM_E_HIP, "corrupt particle scene");
         for (int a = 0; a < 3; a++) {
             if (p->x) {
                 if constexpr (std::is_same<T, float>::value) {
                     int X;
                     std::memcpy(&X, &buf[(PX + a) * cap + s], sizeof(int));
-                    p->x[3 * i + a] = decodePos(X, h->desc.dx);
+                    if (X != POS_NAN) X -= h->origin[(size_t)sc][a] << POS_BITS;
+                    p->x[3 * i + a] = decodePos(X, h->scenes[(size_t)sc].dx);
                 }
                 else
                     p->x[3 * i + a] = (double)buf[(PX + a) * cap + s];
@@ -554,7 +631,7 @@ zmpm_status downloadT(zmpm_context* h, zmpm_particles* p)
         }
         if (p->m) p->m[i] = (double)buf[PM * cap + s];
         if (p->V0) p->V0[i] = (double)buf[PVOL * cap + s];
-        if (p->material) p->material[i] = mat[s];
+        if (p->material) p->material[i] = mat[s] & PMAT_MAT_MASK;
         if (p->d) p->d[i] = (double)buf[PD * cap + s];
         if (p->g) p->g[i] = (double)buf[PG * cap + s];
         if (p->laplacian) p->laplacian[i] = (double)buf[PLAP * cap + s];
@@ -632,7 +709,8 @@ zmpm_status gridT(zmpm_context* h, int64_t* n_nodes, int32_t* coords, double* m,
     std::vector<int> bpage(nb);
     std::vector<AccT> acc((size_t)7 * nn);
     std::vector<T> nv((size_t)3 * nn), pfn(nn);
-    float sc[NSC] = { 1, 1, 1, 1, 1 };
+    const size_t ns = h->scenes.size();
+    std::vector<float> sc(NSC * ns, 1.f);
     HIPCHK(h, hipMemcpy(bpage.data(), D.block_page, nb * sizeof(int), hipMemcpyDeviceToHost));
     for (int f = 0; f < 7; f++)
         HIPCHK(h, hipMemcpy(acc.data() + f * nn, D.acc + f * D.cap_nodes, nn * sizeof(AccT), hipMemcpyDeviceToHost));
@@ -640,22 +718,32 @@ zmpm_status gridT(zmpm_context* h, int64_t* n_nodes, int32_t* coords, double* m,
         HIPCHK(h, hipMemcpy(nv.data() + a * nn, D.newv + a * D.cap_nodes, nn * sizeof(T), hipMemcpyDeviceToHost));
     if (h->any_damage) HIPCHK(h, hipMemcpy(pfn.data(), D.pfn, nn * sizeof(T), hipMemcpyDeviceToHost));
     // scales of the last substep's P2G (they are replaced only at the start of the next one)
-    if (std::is_same<AccT, long long>::value) HIPCHK(h, hipMemcpy(sc, D.scales, sizeof(sc), hipMemcpyDeviceToHost));
-    auto val = [&](int f, int64_t node, int g) -> double {
-        if (std::is_same<AccT, long long>::value) return (double)acc[f * nn + node] / (double)sc[g];
-        return (double)acc[f * nn + node];
+    if (std::is_same<AccT, long long>::value)
+        HIPCHK(h, hipMemcpy(sc.data(), D.scales, sc.size() * sizeof(float), hipMemcpyDeviceToHost));
+    // scene of a node from its region
+    auto sceneAt = [&](const uint32_t* c) -> int {
+        const int rd = h->region_dim;
+        const int r0 = (int)c[0] / h->region, r1 = (int)c[1] / h->region, r2 = (int)c[2] / h->region;
+        if (r0 >= rd || r1 >= rd || r2 >= rd) return 0;
+        return std::max(0, h->region_scene[(size_t)((r0 * rd + r1) * rd + r2)]);
     };
     struct Rec {
         uint32_t c[3];
         int64_t node;
+        int scene;
+    };
+    auto val = [&](int f, const Rec& r, int g) -> double {
+        if (std::is_same<AccT, long long>::value) return (double)acc[f * nn + r.node] / (double)sc[(size_t)r.scene * NSC + g];
+        return (double)acc[f * nn + r.node];
     };
     std::vector<Rec> recs;
     for (int64_t node = 0; node < nn; node++) {
-        double mm = val(GM, node, SC_M);
-        if (mm == 0) continue;
         const int b = (int)(node / NB), e = (int)(node % NB);
         uint64_t off = ((uint64_t)(uint32_t)bpage[b] << 12) | ((uint64_t)e << B::data_bits);
-        recs.push_back(Rec { { hostBitPack(off, D.masks.x), hostBitPack(off, D.masks.y), hostBitPack(off, D.masks.z) }, node });
+        Rec r { { hostBitPack(off, D.masks.x), hostBitPack(off, D.masks.y), hostBitPack(off, D.masks.z) }, node, 0 };
+        r.scene = sceneAt(r.c);
+        if (val(GM, r, SC_M) == 0) continue;
+        recs.push_back(r);
     }
     std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) {
         if (a.c[0] != b.c[0]) return a.c[0] < b.c[0];
@@ -665,12 +753,12 @@ zmpm_status gridT(zmpm_context* h, int64_t* n_nodes, int32_t* coords, double* m,
     *n_nodes = (int64_t)recs.size();
     for (size_t q = 0; q < recs.size(); q++) {
         const int64_t node = recs[q].node;
-        const double mm = val(GM, node, SC_M);
+        const double mm = val(GM, recs[q], SC_M);
         if (coords)
             for (int a = 0; a < 3; a++) coords[3 * q + a] = (int32_t)recs[q].c[a];
         if (m) m[q] = mm;
         if (v)
-            for (int a = 0; a < 3; a++) v[3 * q + a] = val(GMV + a, node, SC_MV) / mm;
+            for (int a = 0; a < 3; a++) v[3 * q + a] = val(GMV + a, recs[q], SC_MV) / mm;
         if (new_v)
             for (int a = 0; a < 3; a++) new_v[3 * q + a] = (double)nv[a * nn + node];
         if (pf) pf[q] = h->any_damage ? (double)pfn[node] : 0.0;
@@ -693,8 +781,9 @@ std::vector<std::pair<void*, size_t>> liveState(zmpm_context* h)
              { D.pmat, (size_t)D.cap * sizeof(int) },        { D.key, (size_t)D.cap * sizeof(uint64_t) },
              { D.rank, (size_t)D.cap * sizeof(int) },
              { D.touched, (size_t)D.num_pages * sizeof(int) }, { D.page_count, (size_t)D.num_pages * sizeof(int) },
-             { D.maxbits, (size_t)NSC * BOUND_SLOTS * sizeof(unsigned) }, { D.scales, NSC * sizeof(float) },
-             { D.counters, 8 * sizeof(int) },                { D.clock, 2 * sizeof(double) } };
+             { D.maxbits, h->scenes.size() * NSC * BOUND_SLOTS * sizeof(unsigned) },
+             { D.scales, h->scenes.size() * NSC * sizeof(float) },
+             { D.counters, 8 * sizeof(int) },                { D.clock, 2 * h->scenes.size() * sizeof(double) } };
 }
 
 template <class T, class AccT>
@@ -715,29 +804,99 @@ const char* zmpm_build_info(void)
     return "zmpm HIP library: explicit APIC MPM substep + AnisoMPM damage; device code gfx950";
 }
 
-zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
+} // extern "C"
+
+namespace {
+
+// A scene description the reference's assertions accept
+// (MpmSimulationBase::initialize / advanceOneTimeStep, MpmSimulationBase.cpp:108-128, 319, 335).
+bool validScene(const zmpm_scene_desc* d)
 {
-    if (!desc || !out) return ZMPM_E_ARG;
-    const int dn = desc->domain_nodes;
-    if (dn < 64 || dn > 1024 || (dn & (dn - 1)) || !(desc->dx > 0)) return ZMPM_E_ARG;
-    if (desc->precision != ZMPM_PREC_FP32 && desc->precision != ZMPM_PREC_FP64) return ZMPM_E_ARG;
-    if (desc->transfer_scheme != ZMPM_APIC_BLEND_RPIC && desc->transfer_scheme != ZMPM_FLIP_BLEND_PIC) return ZMPM_E_ARG;
-    // MpmSimulationBase::initialize / advanceOneTimeStep assertions (MpmSimulationBase.cpp:108-128, 319, 335)
-    const bool apic = desc->transfer_scheme == ZMPM_APIC_BLEND_RPIC;
-    if (desc->mls_mpm && !apic) return ZMPM_E_ARG;
-    if (apic && !(desc->apic_rpic_ratio >= 0 && desc->apic_rpic_ratio <= 1)) return ZMPM_E_ARG;
-    if (desc->rpic_damping_iteration < 0 || desc->pic_damping_iteration < 0) return ZMPM_E_ARG;
-    if ((desc->rpic_damping_iteration > 0 && !apic) || (desc->pic_damping_iteration > 0 && apic)) return ZMPM_E_ARG;
-    if (desc->implicit_damage != 0 && desc->implicit_damage != 1) return ZMPM_E_ARG;
+    const int dn = d->domain_nodes;
+    if (dn < 64 || dn > 1024 || (dn & (dn - 1)) || !(d->dx > 0) || !(d->cfl > 0)) return false;
+    if (d->precision != ZMPM_PREC_FP32 && d->precision != ZMPM_PREC_FP64) return false;
+    if (d->transfer_scheme != ZMPM_APIC_BLEND_RPIC && d->transfer_scheme != ZMPM_FLIP_BLEND_PIC) return false;
+    const bool apic = d->transfer_scheme == ZMPM_APIC_BLEND_RPIC;
+    if (d->mls_mpm && !apic) return false;
+    if (apic && !(d->apic_rpic_ratio >= 0 && d->apic_rpic_ratio <= 1)) return false;
+    if (d->rpic_damping_iteration < 0 || d->pic_damping_iteration < 0) return false;
+    if ((d->rpic_damping_iteration > 0 && !apic) || (d->pic_damping_iteration > 0 && apic)) return false;
+    if (d->implicit_damage != 0 && d->implicit_damage != 1) return false;
+    return true;
+}
+
+// The options every scene of a batch shares (one launch sequence runs them all).
+bool sameSharedOptions(const zmpm_scene_desc& a, const zmpm_scene_desc& b)
+{
+    return a.precision == b.precision && a.device == b.device && a.transfer_scheme == b.transfer_scheme
+        && a.flip_pic_ratio == b.flip_pic_ratio && a.mls_mpm == b.mls_mpm
+        && a.rpic_damping_iteration == b.rpic_damping_iteration && a.pic_damping_iteration == b.pic_damping_iteration
+        && a.implicit_damage == b.implicit_damage;
+}
+
+uint64_t hostDeposit(uint32_t v, uint64_t mask)
+{
+    uint64_t r = 0;
+    int bit = 0;
+    for (int b = 0; b < 64; b++)
+        if (mask & (1ull << b)) {
+            if (v & (1u << bit)) r |= 1ull << b;
+            bit++;
+        }
+    return r;
+}
+
+zmpm_status createScenes(const zmpm_scene_desc* descs, int32_t n_scenes, zmpm_handle* out)
+{
+    if (!descs || !out || n_scenes < 1 || n_scenes > 4096) return ZMPM_E_ARG;
+    for (int s = 0; s < n_scenes; s++) {
+        if (!validScene(&descs[s])) return ZMPM_E_ARG;
+        if (!sameSharedOptions(descs[0], descs[s])) return ZMPM_E_ARG;
+    }
+    // the implicit damage solve couples every node of the handle: one scene only
+    if (n_scenes > 1 && descs[0].implicit_damage) return ZMPM_E_ARG;
+    // regions: cubes of the largest scene domain, region_dim per axis
+    int region = 0;
+    for (int s = 0; s < n_scenes; s++) region = std::max(region, descs[s].domain_nodes);
+    int rd = 1;
+    while (rd * rd * rd < n_scenes) rd++;
+    int grid = 1;
+    while (grid < rd * region) grid <<= 1;
+    if (grid > 1024) return ZMPM_E_ARG; // fixed-point positions address at most 1024 nodes per axis
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ZMPM_E_HIP;
-    if (desc->device < 0 || desc->device >= ndev) return ZMPM_E_ARG;
-    if (hipSetDevice(desc->device) != hipSuccess) return ZMPM_E_HIP;
+    const int device = descs[0].device;
+    if (device < 0 || device >= ndev) return ZMPM_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return ZMPM_E_HIP;
     auto* h = new zmpm_context();
-    h->desc = *desc;
-    h->fp32 = desc->precision == ZMPM_PREC_FP32;
+    h->desc = descs[0];
+    h->scenes.assign(descs, descs + n_scenes);
+    h->scene_cols.assign((size_t)n_scenes, {});
+    h->step_dt.assign((size_t)n_scenes, 0.0);
+    h->region = region;
+    h->region_dim = rd;
+    h->grid_domain = grid;
+    h->fp32 = descs[0].precision == ZMPM_PREC_FP32;
+    // regions in page (Morton) order of their first node, so a scene's grid
+    // blocks are contiguous in the dense block order; scene s takes the s-th
+    const int log2s = h->fp32 ? 6 : 7;
+    const uint64_t mx = spreadMask(log2s, 0), my = spreadMask(log2s, 1), mz = spreadMask(log2s, 2);
+    std::vector<std::pair<uint64_t, int>> order;
+    for (int r = 0; r < rd * rd * rd; r++) {
+        const int r0 = r / (rd * rd), r1 = (r / rd) % rd, r2 = r % rd;
+        const uint64_t off = hostDeposit((uint32_t)(r0 * region), mx) | hostDeposit((uint32_t)(r1 * region), my)
+            | hostDeposit((uint32_t)(r2 * region), mz);
+        order.push_back({ off, r });
+    }
+    std::sort(order.begin(), order.end());
+    h->region_scene.assign((size_t)(rd * rd * rd), -1);
+    h->origin.assign((size_t)n_scenes, { 0, 0, 0 });
+    for (int s = 0; s < n_scenes; s++) {
+        const int r = order[(size_t)s].second;
+        h->region_scene[(size_t)r] = s;
+        h->origin[(size_t)s] = { (r / (rd * rd)) * region, ((r / rd) % rd) * region, (r % rd) * region };
+    }
     if (const char* e = std::getenv("ZMPM_NO_GRAPH")) h->use_graphs = std::atoi(e) == 0;
-    if (const char* e = std::getenv("ZMPM_P2G_SPLIT")) h->p2g_split = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("ZMPM_P2G_WGS")) h->p2g_wgs = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("ZMPM_GRID_WGS")) h->grid_wgs = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("ZMPM_PREP_WGS")) h->prep_wgs = std::max(1, std::atoi(e));
@@ -746,14 +905,44 @@ zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
         return ZMPM_E_HIP;
     }
     const size_t msz = h->fp32 ? sizeof(MatDev<float>) : sizeof(MatDev<double>);
-    if (hipMalloc(&h->mats_dev, msz * kMaxMaterials) != hipSuccess
-        || hipMalloc(&h->cols_dev, sizeof(ColDev) * kMaxColliders) != hipSuccess) {
-        if (h->mats_dev) (void)hipFree(h->mats_dev);
+    const size_t ssz = h->fp32 ? sizeof(SceneDev<float>) : sizeof(SceneDev<double>);
+    const size_t ncol = (size_t)kMaxColliders * (size_t)n_scenes;
+    if (hipMalloc(&h->mats_dev, msz * kMaxMaterials) != hipSuccess || hipMalloc(&h->cols_dev, sizeof(ColDev) * ncol) != hipSuccess
+        || hipMalloc(&h->colstate_dev, sizeof(ColState) * ncol) != hipSuccess
+        || hipMalloc(&h->scenes_dev, ssz * (size_t)n_scenes) != hipSuccess
+        || hipMalloc(&h->regions_dev, sizeof(int) * h->region_scene.size()) != hipSuccess
+        || hipMemcpy(h->regions_dev, h->region_scene.data(), sizeof(int) * h->region_scene.size(), hipMemcpyHostToDevice)
+            != hipSuccess) {
+        (void)hipGetLastError();
+        for (void* q : { h->mats_dev, h->cols_dev, h->colstate_dev, h->scenes_dev, h->regions_dev })
+            if (q) (void)hipFree(q);
         (void)hipStreamDestroy(h->stream);
         delete h;
         return ZMPM_E_NOMEM;
     }
+    if (h->fp32)
+        refreshTables<float, long long>(h);
+    else
+        refreshTables<double, double>(h);
     *out = h;
+    return ZMPM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out) { return createScenes(desc, 1, out); }
+
+zmpm_status zmpm_create_batch(const zmpm_scene_desc* descs, int32_t n_scenes, zmpm_handle* out)
+{
+    return createScenes(descs, n_scenes, out);
+}
+
+zmpm_status zmpm_batch_origin(zmpm_handle h, int32_t scene, int32_t* origin)
+{
+    if (!h || !origin || scene < 0 || scene >= (int32_t)h->scenes.size()) return ZMPM_E_ARG;
+    for (int a = 0; a < 3; a++) origin[a] = h->origin[(size_t)scene][a];
     return ZMPM_OK;
 }
 
@@ -763,14 +952,18 @@ zmpm_status zmpm_destroy(zmpm_handle h)
     (void)hipSetDevice(h->desc.device);
     freeAll(h);
     for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
-    if (h->mats_dev) (void)hipFree(h->mats_dev);
-    if (h->cols_dev) (void)hipFree(h->cols_dev);
+    for (void* q : { h->mats_dev, h->cols_dev, h->colstate_dev, h->scenes_dev, h->regions_dev })
+        if (q) (void)hipFree(q);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return ZMPM_OK;
 }
 
-const char* zmpm_last_error(zmpm_handle h) { return h ? h->err.c_str() : "null handle"; }
+const char* zmpm_last_error(zmpm_handle h)
+{
+    if (!h) return "null handle";
+    return h->err.c_str();
+}
 
 zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material* mats)
 {
@@ -806,31 +999,39 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
     return ZMPM_OK;
 }
 
-zmpm_status zmpm_add_collider(zmpm_handle h, const zmpm_collider* c, int32_t* id_out)
+zmpm_status zmpm_add_scene_collider(zmpm_handle h, int32_t scene, const zmpm_collider* c, int32_t* id_out)
 {
     if (!h || !c) return ZMPM_E_ARG;
-    if ((int)h->cols.size() >= kMaxColliders) return fail(h, ZMPM_E_ARG, "too many colliders");
+    if (scene < 0 || scene >= (int32_t)h->scenes.size()) return fail(h, ZMPM_E_ARG, "no such scene");
+    auto& cols = h->scene_cols[(size_t)scene];
+    if ((int)cols.size() >= kMaxColliders) return fail(h, ZMPM_E_ARG, "too many colliders");
     if (c->shape < ZMPM_SHAPE_HALF_SPACE || c->shape > ZMPM_SHAPE_CAPPED_CYLINDER) return fail(h, ZMPM_E_ARG, "unknown shape");
     if (c->type < ZMPM_STICKY || c->type > ZMPM_GHOST) return fail(h, ZMPM_E_ARG, "unknown collider type");
     if (!((c->scale0 == 0 ? 1.0 : c->scale0) > 0)) return fail(h, ZMPM_E_ARG, "collider scaling must be positive");
     (void)hipSetDevice(h->desc.device);
     dropGraphs(h);
-    h->cols.push_back(*c);
+    cols.push_back(*c);
     zmpm_status s = uploadColliders(h);
     if (s != ZMPM_OK) return s;
     if (h->fp32)
         refreshTables<float, long long>(h);
     else
         refreshTables<double, double>(h);
-    if (id_out) *id_out = (int32_t)h->cols.size() - 1;
+    if (id_out) *id_out = (int32_t)cols.size() - 1;
     return ZMPM_OK;
+}
+
+zmpm_status zmpm_add_collider(zmpm_handle h, const zmpm_collider* c, int32_t* id_out)
+{
+    return zmpm_add_scene_collider(h, 0, c, id_out);
 }
 
 zmpm_status zmpm_clear_colliders(zmpm_handle h)
 {
     if (!h) return ZMPM_E_ARG;
+    (void)hipSetDevice(h->desc.device);
     dropGraphs(h);
-    h->cols.clear();
+    for (auto& v : h->scene_cols) v.clear();
     if (h->fp32)
         refreshTables<float, long long>(h);
     else
@@ -838,7 +1039,7 @@ zmpm_status zmpm_clear_colliders(zmpm_handle h)
     return ZMPM_OK;
 }
 
-zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
+static zmpm_status uploadScenesParticles(zmpm_handle h, const zmpm_particles* p, const int32_t* scene)
 {
     if (!h || !p || p->n < 0) return ZMPM_E_ARG;
     if (p->n > 0 && (!p->x || !p->v || !p->C || !p->F || !p->m || !p->V0 || !p->material))
@@ -851,12 +1052,22 @@ zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
         if (p->material[i] < 0 || p->material[i] >= (int32_t)h->mats.size())
             return fail(h, ZMPM_E_ARG, "material index out of range");
         max_mat = std::max(max_mat, (int)p->material[i]);
+        if (scene && (scene[i] < 0 || scene[i] >= (int32_t)h->scenes.size()))
+            return fail(h, ZMPM_E_ARG, "scene index out of range");
     }
     (void)hipSetDevice(h->desc.device);
     dropGraphs(h);
-    const zmpm_status s = h->fp32 ? uploadT<float, long long>(h, p) : uploadT<double, double>(h, p);
+    const zmpm_status s = h->fp32 ? uploadT<float, long long>(h, p, scene) : uploadT<double, double>(h, p, scene);
     if (s == ZMPM_OK) h->max_mat = max_mat;
     return s;
+}
+
+zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p) { return uploadScenesParticles(h, p, nullptr); }
+
+zmpm_status zmpm_upload_batch(zmpm_handle h, const zmpm_particles* p, const int32_t* scene)
+{
+    if (!scene && p && p->n > 0) return fail(h, ZMPM_E_ARG, "missing scene array");
+    return uploadScenesParticles(h, p, scene);
 }
 
 zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
@@ -872,11 +1083,11 @@ zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
 
 // Two substeps captured into one HIP graph (the particle buffers ping-pong, so
 // a pair returns to the starting buffer); cached per live buffer and dt.
-static zmpm_status graphFor(zmpm_handle h, double dt, hipGraphExec_t* out)
+static zmpm_status graphFor(zmpm_handle h, hipGraphExec_t* out)
 {
     const void* live = h->fp32 ? (const void*)h->Df.P : (const void*)h->Dd.P;
     for (auto& g : h->graphs)
-        if (g.live == live && g.dt == dt) {
+        if (g.live == live) {
             *out = g.exec;
             return ZMPM_OK;
         }
@@ -886,24 +1097,37 @@ static zmpm_status graphFor(zmpm_handle h, double dt, hipGraphExec_t* out)
         return fail(h, ZMPM_E_HIP, "hipStreamBeginCapture");
     for (int k = 0; k < 2; k++) {
         if (h->fp32)
-            launchStep<float, long long>(h, dt, nullptr);
+            launchStep<float, long long>(h, nullptr);
         else
-            launchStep<double, double>(h, dt, nullptr);
+            launchStep<double, double>(h, nullptr);
     }
     if (hipStreamEndCapture(h->stream, &graph) != hipSuccess) return fail(h, ZMPM_E_HIP, "hipStreamEndCapture");
     hipGraphExec_t exec = nullptr;
     hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(h, ZMPM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    h->graphs.push_back({ live, dt, exec });
+    h->graphs.push_back({ live, exec });
     *out = exec;
     return ZMPM_OK;
 }
 
-static zmpm_status runSteps(zmpm_handle h, int32_t steps, double dt, double t0)
+// `steps` substeps of every scene: scene s advances by dt[s] per substep from
+// t0[s] (its device clock); the scene table is refreshed when a dt changes.
+static zmpm_status runSteps(zmpm_handle h, int32_t steps, const double* dt, const double* t0)
 {
     if (h->n == 0) return ZMPM_OK;
     (void)hipSetDevice(h->desc.device);
+    const size_t ns = h->scenes.size();
+    bool dt_changed = false;
+    for (size_t s = 0; s < ns; s++)
+        if (h->step_dt[s] != dt[s]) {
+            h->step_dt[s] = dt[s];
+            dt_changed = true;
+        }
+    if (dt_changed) {
+        const zmpm_status st = h->fp32 ? uploadScenes<float>(h) : uploadScenes<double>(h);
+        if (st != ZMPM_OK) return st;
+    }
     hipEvent_t* ev = nullptr;
     if (h->timing) {
         const size_t need = (size_t)steps * (kTimedGroups + 1);
@@ -913,15 +1137,16 @@ static zmpm_status runSteps(zmpm_handle h, int32_t steps, double dt, double t0)
             h->events.push_back(e);
         }
     }
-    // the device clock starts at t0; every substep advances it by dt
-    h->clock_host = t0;
+    // each scene's device clock starts at its t0; every substep advances it by its dt
+    h->clock_host.assign(2 * ns, 0.0);
+    for (size_t s = 0; s < ns; s++) h->clock_host[2 * s] = h->clock_host[2 * s + 1] = t0[s];
     double* clock = h->fp32 ? h->Df.clock : h->Dd.clock;
-    if (hipMemcpyAsync(clock, &h->clock_host, sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess)
+    if (hipMemcpyAsync(clock, h->clock_host.data(), 2 * ns * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess)
         return fail(h, ZMPM_E_HIP, "clock upload");
     int32_t s = 0;
     if (h->use_graphs && !h->timing && steps >= 2) {
         hipGraphExec_t exec = nullptr;
-        zmpm_status st = graphFor(h, dt, &exec);
+        zmpm_status st = graphFor(h, &exec);
         if (st != ZMPM_OK) return st;
         for (; s + 2 <= steps; s += 2)
             if (hipGraphLaunch(exec, h->stream) != hipSuccess) return fail(h, ZMPM_E_HIP, "hipGraphLaunch");
@@ -929,9 +1154,9 @@ static zmpm_status runSteps(zmpm_handle h, int32_t steps, double dt, double t0)
     for (; s < steps; s++) {
         if (h->timing) ev = &h->events[(size_t)s * (kTimedGroups + 1)];
         if (h->fp32)
-            launchStep<float, long long>(h, dt, ev);
+            launchStep<float, long long>(h, ev);
         else
-            launchStep<double, double>(h, dt, ev);
+            launchStep<double, double>(h, ev);
     }
     h->stepped = true;
     hipError_t e = hipGetLastError();
@@ -955,15 +1180,30 @@ static zmpm_status runSteps(zmpm_handle h, int32_t steps, double dt, double t0)
     return ZMPM_OK;
 }
 
+// Every scene with the same dt and start time.
+static zmpm_status runUniform(zmpm_handle h, int32_t steps, double dt, double t0)
+{
+    const std::vector<double> dts(h->scenes.size(), dt), t0s(h->scenes.size(), t0);
+    return runSteps(h, steps, dts.data(), t0s.data());
+}
+
 zmpm_status zmpm_substep(zmpm_handle h, double dt, double t)
 {
     if (!h || !(dt > 0)) return ZMPM_E_ARG;
-    return runSteps(h, 1, dt, t);
+    return runUniform(h, 1, dt, t);
 }
 
 zmpm_status zmpm_run(zmpm_handle h, int32_t steps, double dt, double t0)
 {
     if (!h || steps < 0 || !(dt > 0)) return ZMPM_E_ARG;
+    return runUniform(h, steps, dt, t0);
+}
+
+zmpm_status zmpm_run_batch(zmpm_handle h, int32_t steps, const double* dt, const double* t0)
+{
+    if (!h || steps < 0 || !dt || !t0) return ZMPM_E_ARG;
+    for (size_t s = 0; s < h->scenes.size(); s++)
+        if (!(dt[s] > 0)) return fail(h, ZMPM_E_ARG, "every scene's dt must be positive");
     return runSteps(h, steps, dt, t0);
 }
 
@@ -1009,12 +1249,12 @@ zmpm_status zmpm_calculate_dt(zmpm_handle h, double max_dt, double* dt, double* 
     double* ext = h->fp32 ? h->Df.extent : h->Dd.extent;
     if (h->n > 0 && ext) {
         if (h->fp32) {
-            k_extent<float, long long><<<EXTENT_WGS, WG, 0, h->stream>>>(h->Df, 0);
-            k_extent<float, long long><<<1, WG, 0, h->stream>>>(h->Df, 1);
+            k_extent<float, long long><<<EXTENT_WGS, WG, 0, h->stream>>>(h->Df, 0, EXTENT_WGS, nullptr, 0.0, 0.0);
+            k_extent<float, long long><<<1, WG, 0, h->stream>>>(h->Df, 1, EXTENT_WGS, nullptr, 0.0, 0.0);
         }
         else {
-            k_extent<double, double><<<EXTENT_WGS, WG, 0, h->stream>>>(h->Dd, 0);
-            k_extent<double, double><<<1, WG, 0, h->stream>>>(h->Dd, 1);
+            k_extent<double, double><<<EXTENT_WGS, WG, 0, h->stream>>>(h->Dd, 0, EXTENT_WGS, nullptr, 0.0, 0.0);
+            k_extent<double, double><<<1, WG, 0, h->stream>>>(h->Dd, 1, EXTENT_WGS, nullptr, 0.0, 0.0);
         }
         HIPCHK(h, hipGetLastError());
         HIPCHK(h, hipMemcpyAsync(e, ext + EXTENT_WGS * 8, 8 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1025,7 +1265,14 @@ zmpm_status zmpm_calculate_dt(zmpm_handle h, double max_dt, double* dt, double* 
         if (max_corner) max_corner[a] = e[1 + a];
         if (min_corner) min_corner[a] = -e[4 + a];
     }
-    if (dt) *dt = e[0] != 0 ? h->desc.cfl * h->desc.dx / e[0] : max_dt;
+    // one scene: cfl dx / max speed (MpmSimulationBase.cpp:1127-1151); a batch:
+    // the smallest scene step, 1 / max(speed / (cfl dx))
+    if (dt) {
+        if (h->scenes.size() == 1)
+            *dt = e[0] != 0 ? h->desc.cfl * h->desc.dx / e[0] : max_dt;
+        else
+            *dt = e[7] != 0 ? 1.0 / e[7] : max_dt;
+    }
     return ZMPM_OK;
 }
 
@@ -1109,7 +1356,7 @@ zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, in
     h->timing = true;
     zmpm_status st = ZMPM_OK;
     // one untimed substep first (first-touch of the configuration's launches)
-    if ((st = runSteps(h, 1, dt, t)) != ZMPM_OK) goto done;
+    if ((st = runUniform(h, 1, dt, t)) != ZMPM_OK) goto done;
     t += dt;
     for (Knob& k : knobs) {
         int best = *k.value;
@@ -1120,7 +1367,7 @@ zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, in
                 refreshTables<float, long long>(h);
             else
                 refreshTables<double, double>(h);
-            if ((st = runSteps(h, steps, dt, t)) != ZMPM_OK) goto done;
+            if ((st = runUniform(h, steps, dt, t)) != ZMPM_OK) goto done;
             t += steps * dt;
             const double ms = h->timed_ms[k.stage] / h->timed_steps;
             if (ms < best_ms) {

@@ -102,13 +102,15 @@ __device__ __forceinline__ void storePos(float* base, int i, int X) { stAt(base,
 __device__ __forceinline__ void storePos(double* base, int i, double x) { stAt(base, i, x); }
 
 // Base node of one axis, floor(x / dx - 1/2) (baseNode<2>, BSplines.h:16-20;
-// the key build of MpmSimulationBase.cpp:1545), and the offset from it in
+// the key build of MpmSimulationBase.cpp:1545), as a node of the handle's grid
+// (the scene's region starts at node `origin`), and the offset from it in
 // cells, d0 = x / dx - base in [0.5, 1.5).  Returns ST_NAN / ST_DOMAIN when the
-// position is not finite or the base node leaves [0, domain - 4); base is then
-// clamped into that range (and d0 set to 1) so callers index inside the grid.
-// FP64 form: x * (1/dx) - 0.5 in double without FMA, truncated (int_floor for
-// t >= 0, MathTools.h:15-25).
-__device__ __forceinline__ int cellOf(double x, double odx, int domain, int& base, double& d0)
+// position is not finite or the base node leaves [0, domain - 4) of the scene;
+// base is then clamped into that range (and d0 set to 1) so callers index
+// inside the scene's region.
+// FP64 form: x (metres in the scene's frame) * (1/dx) - 0.5 in double without
+// FMA, truncated (int_floor for t >= 0, MathTools.h:15-25).
+__device__ __forceinline__ int cellOf(double x, double odx, int origin, int domain, int& base, double& d0)
 {
     const double xs = mul_rn(x, odx);
     double t = sub_rn(xs, 0.5);
@@ -122,26 +124,41 @@ __device__ __forceinline__ int cellOf(double x, double odx, int domain, int& bas
         bits |= ST_DOMAIN;
         t = (t < 0.0) ? 0.0 : hi - 1.0;
     }
-    base = (int)t;
-    d0 = bits ? 1.0 : xs - double(base);
+    const int b = (int)t;
+    d0 = bits ? 1.0 : xs - double(b);
+    base = b + origin;
     return bits;
 }
 
-// FP32 form: the same floor, exact on the fixed-point position; d0 has at most
-// POS_BITS + 1 significant bits, so it is exact in float too.
-__device__ __forceinline__ int cellOf(int X, float, int domain, int& base, float& d0)
+// FP32 form: the same floor, exact on the fixed-point position (held in the
+// handle's grid already); d0 has at most POS_BITS + 1 significant bits, so it is
+// exact in float too.
+__device__ __forceinline__ int cellOf(int X, float, int origin, int domain, int& base, float& d0)
 {
     const int bits0 = (X == POS_NAN) ? ST_NAN : 0;
     int bits = bits0;
     const int t = bits0 ? 0 : X - POS_HALF;
-    int b = t >> POS_BITS; // arithmetic shift: floor
+    int b = (t >> POS_BITS) - origin; // arithmetic shift: floor
     if (!(b >= 0 && b < domain - 4)) {
         bits |= ST_DOMAIN;
         b = (b < 0) ? 0 : domain - 5;
     }
-    base = b;
-    d0 = bits ? 1.f : (float)(X - (b << POS_BITS)) * (1.f / (float)(1 << POS_BITS));
+    base = b + origin;
+    d0 = bits ? 1.f : (float)(X - (base << POS_BITS)) * (1.f / (float)(1 << POS_BITS));
     return bits;
+}
+
+// The scene of a particle (material word) and of a grid node (its region).
+__device__ __forceinline__ int sceneOf(int pmat) { return pmat >> PMAT_SCENE_SHIFT; }
+__device__ __forceinline__ int matOf(int pmat) { return pmat & PMAT_MAT_MASK; }
+
+template <class T, class AccT>
+__device__ __forceinline__ int sceneOfNode(const Dev<T, AccT>& D, uint32_t ix, uint32_t iy, uint32_t iz)
+{
+    const uint32_t rd = (uint32_t)D.region_dim;
+    const uint32_t rx = ix >> D.region_shift, ry = iy >> D.region_shift, rz = iz >> D.region_shift;
+    if (rx >= rd || ry >= rd || rz >= rd) return -1;
+    return D.region_scene[(rx * rd + ry) * rd + rz];
 }
 
 // Every third bit: b9..b0 -> bit 3*i (block coordinates < 1024).
@@ -272,18 +289,19 @@ __device__ __forceinline__ uint64_t scanPair(uint64_t w)
 // runs inside a wavefront: one count atomic and one activation per run.  Every
 // lane of the wavefront calls it (active or not).
 template <class T, class AccT>
-__device__ __forceinline__ void binParticle(const Dev<T, AccT>& D, int p, bool active, const PosT<T>* x)
+__device__ __forceinline__ void binParticle(const Dev<T, AccT>& D, int p, bool active, const PosT<T>* x, int scene)
 {
     using B = Blk<T>;
     const int lane = threadIdx.x & 63;
     uint32_t page = 0xffffffffu - (uint32_t)lane; // inactive lanes never join a run
     int bb[3] = { 0, 0, 0 };
     if (active) {
+        const SceneDev<T>& sd = D.sc[scene];
         int bits = 0;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             T d0;
-            bits |= cellOf(x[a], D.odx, D.domain, bb[a], d0);
+            bits |= cellOf(x[a], sd.odx, sd.origin[a], sd.domain, bb[a], d0);
         }
         if (bits) latch(D.counters, bits);
         uint32_t elem;
@@ -441,35 +459,90 @@ __global__ __launch_bounds__(WG) void k_page_scan(Dev<T, AccT> D)
     }
 }
 
-// Start of a substep: device clock and (float form) the fixed-point scales of
-// this substep's P2G from the bounds the previous G2P (or the upload) wrote, one
-// slot per workgroup: 2^(40 - e) with max = f 2^e, f in [0.5, 1).
-template <class T, class AccT>
-__device__ __forceinline__ void beginStep(const Dev<T, AccT>& D, double dt)
+// Eigen's Quaternion::toRotationMatrix of q = (w, x, y, z), row-major.
+__host__ __device__ inline void quatToMatrix(const double* q, double* R)
 {
-    // clock[0] = time of the next substep, clock[1] = this substep's t; the
-    // rare-path queue of this substep's G2P starts empty
-    if (threadIdx.x == 0) {
-        D.counters[CNT_RARE] = 0;
-        D.clock[1] = D.clock[0];
-        D.clock[0] = D.clock[0] + dt;
+    double w = q[0];
+    const double x = q[1], y = q[2], z = q[3];
+    if (w == 0 && x == 0 && y == 0 && z == 0) w = 1;
+    const double tx = 2 * x, ty = 2 * y, tz = 2 * z;
+    const double twx = tx * w, twy = ty * w, twz = tz * w;
+    const double txx = tx * x, txy = ty * x, txz = tz * x;
+    const double tyy = ty * y, tyz = tz * y, tzz = tz * z;
+    R[0] = 1 - (tyy + tzz);
+    R[1] = txy - twz;
+    R[2] = txz + twy;
+    R[3] = txy + twz;
+    R[4] = 1 - (txx + tzz);
+    R[5] = tyz - twx;
+    R[6] = txz - twy;
+    R[7] = tyz + twx;
+    R[8] = 1 - (txx + tyy);
+}
+
+// The object's state at time t: b = b0 + dbdt t, s = s0 + dsdt t and the
+// rotation of q(t) = (cos(|w| t/2), sin(|w| t/2) w/|w|) * q0 (constant angular
+// velocity w, as the updateState callbacks of examples/boneTwist.h:144-170 set it).
+__device__ inline void colliderState(const ColDev& c, double t, ColState& st)
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++) st.b[a] = c.b0[a] + c.dbdt[a] * t;
+    st.s = c.s0 + c.dsdt * t;
+    st.os = 1.0 / st.s;
+    double q[4] = { c.q0[0], c.q0[1], c.q0[2], c.q0[3] };
+    const double wn = sqrt(c.omega[0] * c.omega[0] + c.omega[1] * c.omega[1] + c.omega[2] * c.omega[2]);
+    if (wn > 0) {
+        const double ha = 0.5 * wn * t, cw = cos(ha), sw = sin(ha) / wn;
+        const double p[4] = { cw, sw * c.omega[0], sw * c.omega[1], sw * c.omega[2] };
+        const double* q0 = c.q0;
+        q[0] = p[0] * q0[0] - p[1] * q0[1] - p[2] * q0[2] - p[3] * q0[3];
+        q[1] = p[0] * q0[1] + p[1] * q0[0] + p[2] * q0[3] - p[3] * q0[2];
+        q[2] = p[0] * q0[2] - p[1] * q0[3] + p[2] * q0[0] + p[3] * q0[1];
+        q[3] = p[0] * q0[3] + p[1] * q0[2] - p[2] * q0[1] + p[3] * q0[0];
+    }
+    quatToMatrix(q, st.R);
+}
+
+// Start of a substep: each scene's clock and (float form) the fixed-point
+// scales of this substep's P2G per scene from the bounds the previous G2P (or
+// the upload) wrote, one slot per workgroup: 2^(40 - e) with max = f 2^e,
+// f in [0.5, 1).
+__device__ __forceinline__ float boundScale(float m, float fallback)
+{
+    if (!(m > 0.f && isfinite(m))) return fallback;
+    int e;
+    frexpf(m, &e);
+    int se = (int)FIXED_TARGET_EXP - e;
+    se = se < -120 ? -120 : (se > 120 ? 120 : se);
+    return ldexpf(1.0f, se);
+}
+
+template <class T, class AccT>
+__device__ __forceinline__ void beginStep(const Dev<T, AccT>& D)
+{
+    // clock[2 s] = time of the scene's next substep, clock[2 s + 1] = this
+    // substep's t; the rare-path queue of this substep's G2P starts empty
+    if (threadIdx.x == 0) D.counters[CNT_RARE] = 0;
+    for (int sc = threadIdx.x; sc < D.nscenes; sc += WG) {
+        D.clock[2 * sc + 1] = D.clock[2 * sc];
+        D.clock[2 * sc] = D.clock[2 * sc] + D.sc[sc].ddt;
+    }
+    __syncthreads();
+    // the collision objects at their scene's time t_n (the grid update sees them
+    // before they move, MpmSimulationBase.cpp:291-293)
+    for (int c = threadIdx.x; c < D.num_cols; c += WG) {
+        const ColDev co = D.cols[c];
+        colliderState(co, D.clock[2 * co.scene + 1], D.colstate[c]);
     }
     if constexpr (!std::is_same<AccT, long long>::value) return;
-    // one wavefront per scale group over the BOUND_SLOTS slots, which are
-    // cleared for this substep's G2P
+    // one wavefront per (scene, scale group) over the BOUND_SLOTS slots, which
+    // are cleared for this substep's G2P
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
-    float m = 0.f;
-    for (int k = g; k < NSC; k += WG / 64) {
+    for (int k = g; k < D.nscenes * NSC; k += WG / 64) {
         const float v = __uint_as_float(D.maxbits[k * BOUND_SLOTS + lane]);
         D.maxbits[k * BOUND_SLOTS + lane] = 0u;
-        m = waveMax(v);
-        if (lane == 0 && m > 0.f && isfinite(m)) {
-            int e;
-            frexpf(m, &e);
-            int se = (int)FIXED_TARGET_EXP - e;
-            se = se < -120 ? -120 : (se > 120 ? 120 : se);
-            D.scales[k] = ldexpf(1.0f, se);
-        }
+        const float m = waveMax(v);
+        if (lane == 0 && m > 0.f && isfinite(m)) D.scales[k] = boundScale(m, 1.f);
     }
 }
 
@@ -479,7 +552,7 @@ __device__ __forceinline__ void beginStep(const Dev<T, AccT>& D, double dt)
 // workgroups: zero the accumulators of every block's nodes and build the
 // forward-neighbour block table.  The last workgroup starts the substep.
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D, double dt, int part_wgs, int prep_wgs)
+__global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D, int part_wgs, int prep_wgs)
 {
     using B = Blk<T>;
     constexpr int bbits = B::xbits + B::ybits + B::zbits;
@@ -499,7 +572,7 @@ __global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D, double dt, int p
         return;
     }
     if ((int)blockIdx.x == part_wgs + prep_wgs) {
-        beginStep(D, dt);
+        beginStep(D);
         return;
     }
     const int nb = D.counters[CNT_BLOCKS];
@@ -786,25 +859,25 @@ __device__ __forceinline__ void particleBounds(float* b, T m, const T* v, const 
     forceBounds(b, S, d, odx, dmg);
 }
 
-// Workgroup maximum of the per-thread bounds, folded into slot
-// blockIdx % BOUND_SLOTS of each scale group with an atomic max on the float
-// bits (non-negative floats order like their bit patterns).
-__device__ __forceinline__ void writeBounds(unsigned int* maxbits, const float* b)
+// The per-thread bounds folded into the bound slots of each thread's scene:
+// per wavefront and scene (one pass per distinct scene in the wavefront), an
+// atomic max on the float bits (non-negative floats order like their bit
+// patterns) into slot blockIdx % BOUND_SLOTS of each scale group.  Threads with
+// no particle pass scene < 0.
+__device__ __forceinline__ void writeBounds(unsigned int* maxbits, const float* b, int scene)
 {
-    __shared__ float wmax[NSC][WG / 64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    uint64_t left = __ballot(scene >= 0);
+    while (left) {
+        const int s0 = __shfl(scene, __ffsll((unsigned long long)left) - 1);
+        const bool mine = scene == s0;
+        left &= ~__ballot(mine);
 #pragma unroll
-    for (int g = 0; g < NSC; g++) {
-        const float mx = waveMax(b[g]);
-        if (lane == 0) wmax[g][wv] = mx;
-    }
-    __syncthreads();
-    if (threadIdx.x < NSC) {
-        float mx = 0.f;
-#pragma unroll
-        for (int w = 0; w < WG / 64; w++) mx = fmaxf(mx, wmax[threadIdx.x][w]);
-        if (mx > 0.f || !(mx == mx))
-            atomicMax(&maxbits[threadIdx.x * BOUND_SLOTS + (int)(blockIdx.x % BOUND_SLOTS)], __float_as_uint(mx));
+        for (int g = 0; g < NSC; g++) {
+            const float mx = waveMax(mine ? b[g] : 0.f);
+            if (lane == 0 && (mx > 0.f || !(mx == mx)))
+                atomicMax(&maxbits[(s0 * NSC + g) * BOUND_SLOTS + (int)(blockIdx.x % BOUND_SLOTS)], __float_as_uint(mx));
+        }
     }
 }
 
@@ -827,15 +900,17 @@ __device__ __forceinline__ T snowScale(const MatDev<T>& mt, T jp)
 template <class T, class AccT>
 __global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D, int bin)
 {
-    float b[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
     const int64_t cap = D.cap;
     if (bin) openPageScan(D);
     for (int base = blockIdx.x * WG; base < D.n; base += gridDim.x * WG) {
+        float b[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
         const int p = base + threadIdx.x;
         const bool active = p < D.n;
         PosT<T> x[3] = { 0, 0, 0 };
+        const int pm = active ? D.pmat[p] : 0, scene = sceneOf(pm);
         if (active) {
-            const MatDev<T>& mt = D.mats[D.pmat[p]];
+            const SceneDev<T>& sd = D.sc[scene];
+            const MatDev<T>& mt = D.mats[matOf(pm)];
             Mat3<T> F = loadM(D.P, cap, PF, p);
             T S[6], v[3], Cm[9];
             const T jp = mt.plasticity == ZMPM_PLASTICITY_SNOW ? ldAt(D.P + PYS * cap, p) : T(1);
@@ -850,11 +925,22 @@ __global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D, int bin)
 #pragma unroll
             for (int i = 0; i < 9; i++) Cm[i] = ldAt(D.P + (PC + i) * cap, p);
             const bool dmg = D.any_damage && mt.has_damage;
-            particleBounds(b, ldAt(D.P + PM * cap, p), v, Cm, S, dmg ? ldAt(D.P + PD * cap, p) : T(0), D.dx, D.odx, D.any_damage != 0);
+            particleBounds(b, ldAt(D.P + PM * cap, p), v, Cm, S, dmg ? ldAt(D.P + PD * cap, p) : T(0), sd.dx, sd.odx,
+                           D.any_damage != 0);
         }
-        if (bin) binParticle(D, p, active, x);
+        if (bin) binParticle(D, p, active, x, scene);
+        if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, b, active ? scene : -1);
     }
-    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, b);
+}
+
+// Scene of dense block b (its region).
+template <class T, class AccT>
+__device__ __forceinline__ int blockScene(const Dev<T, AccT>& D, int b)
+{
+    using B = Blk<T>;
+    uint32_t Bx, By, Bz;
+    blockOfPage<T>((uint32_t)D.block_page[b], Bx, By, Bz);
+    return max(0, sceneOfNode(D, Bx << B::xbits, By << B::ybits, Bz << B::zbits));
 }
 
 // Block-local LDS tile geometry: a block of BX x BY x BZ nodes plus the +2
@@ -919,7 +1005,7 @@ __device__ __forceinline__ void loadP2G(const Dev<T, AccT>& D, int s, bool dmg, 
 #pragma unroll
     for (int i = 0; i < 6; i++) r.S[i] = DAMP ? T(0) : ldAt(P + (PS + i) * cap, p);
     r.m = ldAt(P + PM * cap, p);
-    r.d = !DAMP && dmg && D.mats[D.pmat[p]].has_damage ? ldAt(P + PD * cap, p) : T(0);
+    r.d = !DAMP && dmg && D.mats[matOf(D.pmat[p])].has_damage ? ldAt(P + PD * cap, p) : T(0);
 }
 
 // particlesToGridWithForceHelper<scheme, deg 2, mls> (MpmSimulationBase.cpp:791-854).
@@ -973,8 +1059,6 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
     const int nq = DAMP ? GFO : (dmg ? NGF : GPF);
     const int64_t cap = D.cap, cn = D.cap_nodes;
     const int nblocks = D.counters[0];
-    const float* sc = D.scales;
-    const float scl[NGF] = { sc[SC_M], sc[SC_MV], sc[SC_MV], sc[SC_MV], sc[SC_F], sc[SC_F], sc[SC_F], sc[SC_PF], sc[SC_PW] };
 
     // in-block order of this workgroup's blocks first (a loop of its own, so
     // its registers are free again for the scatter)
@@ -1006,6 +1090,10 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
         const int bs = cs[0], be = ce[NB - 1];
         if (be <= bs) continue; // halo-only block
         for (int e = t; e < nq * TL::TN; e += WG) (&tile[0][0])[e] = AccT(0);
+        // the block's scene: its dx and fixed-point scales
+        const SceneDev<T>& sd = D.sc[blockScene(D, b)];
+        const float* sc = D.scales + blockScene(D, b) * NSC;
+        const float scl[NGF] = { sc[SC_M], sc[SC_MV], sc[SC_MV], sc[SC_MV], sc[SC_F], sc[SC_F], sc[SC_F], sc[SC_PF], sc[SC_PW] };
         bool have = false;
         RawP2G<T> raw;
         for (int lo = bs; lo < be; lo += CH) {
@@ -1020,15 +1108,15 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     T d0;
-                    cellOf(raw.x[a], D.odx, D.domain, base[a], d0); // errors latched by the binning
+                    cellOf(raw.x[a], sd.odx, sd.origin[a], sd.domain, base[a], d0); // errors latched by the binning
                     bsplineFrac(d0, w[a], dw[a]);
                     if constexpr (sizeof(T) == 4)
-                        xb[a] = -d0 * D.dx; // x_base - x_p
+                        xb[a] = -d0 * sd.dx; // x_base - x_p
                     else
-                        xb[a] = T(base[a]) * D.dx - raw.x[a];
+                        xb[a] = T(base[a] - sd.origin[a]) * sd.dx - raw.x[a];
                     if constexpr (MLS) // D^-1 dx w (x_i - x_p) in place of dw
 #pragma unroll
-                        for (int i = 0; i < 3; i++) dw[a][i] = (D.Dinv * D.dx) * w[a][i] * (xb[a] + T(i) * D.dx);
+                        for (int i = 0; i < 3; i++) dw[a][i] = (sd.Dinv * sd.dx) * w[a][i] * (xb[a] + T(i) * sd.dx);
                 }
                 T mC[9], a3[3];
 #pragma unroll
@@ -1038,8 +1126,8 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
                     a3[r] = mass * raw.v[r] + mC[3 * r] * xb[0] + mC[3 * r + 1] * xb[1] + mC[3 * r + 2] * xb[2];
                 T Sv[6];
 #pragma unroll
-                for (int i = 0; i < 6; i++) Sv[i] = raw.S[i] * D.odx; // float form stores -S below
-                const T dx = D.dx;
+                for (int i = 0; i < 6; i++) Sv[i] = raw.S[i] * sd.odx; // float form stores -S below
+                const T dx = sd.dx;
                 q[0][t] = V4<T> { mass, a3[0], a3[1], a3[2] };
                 q[1][t] = V4<T> { mC[0] * dx, mC[1] * dx, mC[2] * dx, mC[3] * dx };
                 q[2][t] = V4<T> { mC[4] * dx, mC[5] * dx, mC[6] * dx, mC[7] * dx };
@@ -1181,67 +1269,15 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
 }
 
 template <class T, class AccT>
-__device__ __forceinline__ T accValue(const Dev<T, AccT>& D, int f, int64_t node, int sg)
+__device__ __forceinline__ T accValue(const Dev<T, AccT>& D, int f, int64_t node, const float* sc, int sg)
 {
     if constexpr (std::is_same<AccT, long long>::value)
-        return (T)((double)D.acc[f * D.cap_nodes + node] * (1.0 / (double)D.scales[sg]));
+        return (T)((double)D.acc[f * D.cap_nodes + node] * (1.0 / (double)sc[sg]));
     else
         return D.acc[f * D.cap_nodes + node];
 }
 
 // ------------------------------------------------------- collision objects
-// Per-substep state of a collision object: rotation R(t), translation b(t),
-// scaling s(t) and 1 / s (the reference's updateState at the substep's time,
-// MpmSimulationBase.cpp:209-213).
-struct ColState {
-    double R[9];
-    double b[3];
-    double s, os;
-};
-
-// Eigen's Quaternion::toRotationMatrix of q = (w, x, y, z), row-major.
-__host__ __device__ inline void quatToMatrix(const double* q, double* R)
-{
-    double w = q[0];
-    const double x = q[1], y = q[2], z = q[3];
-    if (w == 0 && x == 0 && y == 0 && z == 0) w = 1;
-    const double tx = 2 * x, ty = 2 * y, tz = 2 * z;
-    const double twx = tx * w, twy = ty * w, twz = tz * w;
-    const double txx = tx * x, txy = ty * x, txz = tz * x;
-    const double tyy = ty * y, tyz = tz * y, tzz = tz * z;
-    R[0] = 1 - (tyy + tzz);
-    R[1] = txy - twz;
-    R[2] = txz + twy;
-    R[3] = txy + twz;
-    R[4] = 1 - (txx + tzz);
-    R[5] = tyz - twx;
-    R[6] = txz - twy;
-    R[7] = tyz + twx;
-    R[8] = 1 - (txx + tyy);
-}
-
-// The object's state at time t: b = b0 + dbdt t, s = s0 + dsdt t and the
-// rotation of q(t) = (cos(|w| t/2), sin(|w| t/2) w/|w|) * q0 (constant angular
-// velocity w, as the updateState callbacks of examples/boneTwist.h:144-170 set it).
-__device__ inline void colliderState(const ColDev& c, double t, ColState& st)
-{
-#pragma unroll
-    for (int a = 0; a < 3; a++) st.b[a] = c.b0[a] + c.dbdt[a] * t;
-    st.s = c.s0 + c.dsdt * t;
-    st.os = 1.0 / st.s;
-    double q[4] = { c.q0[0], c.q0[1], c.q0[2], c.q0[3] };
-    const double wn = sqrt(c.omega[0] * c.omega[0] + c.omega[1] * c.omega[1] + c.omega[2] * c.omega[2]);
-    if (wn > 0) {
-        const double ha = 0.5 * wn * t, cw = cos(ha), sw = sin(ha) / wn;
-        const double p[4] = { cw, sw * c.omega[0], sw * c.omega[1], sw * c.omega[2] };
-        const double* q0 = c.q0;
-        q[0] = p[0] * q0[0] - p[1] * q0[1] - p[2] * q0[2] - p[3] * q0[3];
-        q[1] = p[0] * q0[1] + p[1] * q0[0] + p[2] * q0[3] - p[3] * q0[2];
-        q[2] = p[0] * q0[2] - p[1] * q0[3] + p[2] * q0[0] + p[3] * q0[1];
-        q[3] = p[0] * q0[3] + p[1] * q0[2] - p[2] * q0[1] + p[3] * q0[0];
-    }
-    quatToMatrix(q, st.R);
-}
 
 // AnalyticLevelSet::queryInside in material space X: inside flag and the
 // material-space normal N (half space AnalyticLevelSet.cpp:331-347; the axis-
@@ -1448,29 +1484,25 @@ __device__ __forceinline__ void multiObjectCollision(const ColDev* cols, const C
 
 // v = mv/m (MpmSimulationBase.cpp:655-666) and gridVelocityExplicitUpdate
 // (:1179-1228) with multiObjectCollision / detectAndResolveCollision
-// (CollisionObject.cpp:66-105, 341-406) at time t; phase-field normalisation
-// (AnisoFractureSimulation.h:177-179).
+// (CollisionObject.cpp:66-105, 341-406) at the scene's time t; phase-field
+// normalisation (AnisoFractureSimulation.h:177-179).  Each node takes its
+// scene's dx, dt, gravity and collision objects.
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
+__global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D)
 {
-    const double t = D.clock[1]; // collision objects at t_n (MpmSimulationBase.cpp:291-293)
     using B = Blk<T>;
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
     const int64_t nn = (int64_t)D.counters[0] * NB;
     const int64_t cn = D.cap_nodes;
-    // the collision objects and their states at t in LDS: read from there the
-    // per-node loop's loads cannot alias the kernel's stores (with the table in
-    // global memory, stores of doubles anywhere in the kernel made the compiler
-    // load it per node with vector loads: ~2x slower on large grids)
+    // the collision objects of the workgroup's scene and their states in LDS:
+    // read from there the per-node loop's loads cannot alias the kernel's stores
+    // (with the table in global memory, stores of doubles anywhere in the kernel
+    // made the compiler load it per node with vector loads: ~2x slower on large
+    // grids).  A scene's blocks are contiguous, so a workgroup's nodes rarely
+    // span two scenes; those that do read the tables in global memory.
     __shared__ ColDev cols[MAX_COLLIDERS];
     __shared__ ColState cst[MAX_COLLIDERS];
-    const int ncols = min(D.num_cols, MAX_COLLIDERS);
-    static_assert(sizeof(ColDev) % 8 == 0, "collider record is copied as 8-byte words");
-    for (int e = threadIdx.x; e < ncols * (int)(sizeof(ColDev) / 8); e += WG)
-        reinterpret_cast<uint64_t*>(cols)[e] = reinterpret_cast<const uint64_t*>(D.cols)[e];
-    __syncthreads();
-    if ((int)threadIdx.x < ncols) colliderState(cols[threadIdx.x], t, cst[threadIdx.x]);
-    __syncthreads();
+    static_assert(sizeof(ColDev) % 8 == 0 && sizeof(ColState) % 8 == 0, "collider records are copied as 8-byte words");
     if (D.implicit && D.any_damage) {
         // the damage solve's accumulators, filled by k_dtilde
         for (int64_t node = (int64_t)blockIdx.x * WG + threadIdx.x; node < nn; node += (int64_t)gridDim.x * WG) {
@@ -1479,11 +1511,32 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
             D.cgv[4 * cn + node] = 0.0;
         }
     }
-    for (int64_t node = (int64_t)blockIdx.x * WG + threadIdx.x; node < nn; node += (int64_t)gridDim.x * WG) {
-        const T m = accValue(D, GM, node, SC_M);
+    int staged = -1, nstaged = 0; // the scene whose objects are in LDS (the same in every thread)
+    for (int64_t base = (int64_t)blockIdx.x * WG; base < nn; base += (int64_t)gridDim.x * WG) {
+        const int s_first = blockScene(D, (int)(base / NB));
+        const int s_last = blockScene(D, (int)((min(base + WG, nn) - 1) / NB));
+        const bool uniform = s_first == s_last;
+        if (uniform && s_first != staged) {
+            __syncthreads(); // the previous scene's objects are no longer read
+            const SceneDev<T>& sd0 = D.sc[s_first];
+            nstaged = min(sd0.ncol, MAX_COLLIDERS);
+            for (int e = threadIdx.x; e < nstaged * (int)(sizeof(ColDev) / 8); e += WG)
+                reinterpret_cast<uint64_t*>(cols)[e] = reinterpret_cast<const uint64_t*>(D.cols + sd0.col0)[e];
+            for (int e = threadIdx.x; e < nstaged * (int)(sizeof(ColState) / 8); e += WG)
+                reinterpret_cast<uint64_t*>(cst)[e] = reinterpret_cast<const uint64_t*>(D.colstate + sd0.col0)[e];
+            __syncthreads();
+            staged = s_first;
+        }
+        const int64_t node = base + threadIdx.x;
+        if (node >= nn) continue;
+        const int b = (int)(node / NB), e = (int)(node % NB);
+        const int scene = uniform ? s_first : blockScene(D, b);
+        const SceneDev<T>& sd = D.sc[scene];
+        const float* sc = D.scales + scene * NSC;
+        const T m = accValue(D, GM, node, sc, SC_M);
         T f[3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) f[a] = accValue(D, GFO + a, node, SC_F);
+        for (int a = 0; a < 3; a++) f[a] = accValue(D, GFO + a, node, sc, SC_F);
         if (m == T(0)) {
 #pragma unroll
             for (int a = 0; a < 3; a++) D.newv[a * cn + node] = f[a];
@@ -1493,27 +1546,33 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D, T dt)
             if (D.any_damage) D.pfn[node] = T(0);
             continue;
         }
-        const int b = (int)(node / NB), e = (int)(node % NB);
         uint32_t Bx, By, Bz;
         blockOfPage<T>((uint32_t)D.block_page[b], Bx, By, Bz);
-        const uint32_t ix = (Bx << B::xbits) | (uint32_t)(e >> (B::ybits + B::zbits));
-        const uint32_t iy = (By << B::ybits) | (uint32_t)((e >> B::zbits) & ((1 << B::ybits) - 1));
-        const uint32_t iz = (Bz << B::zbits) | (uint32_t)(e & ((1 << B::zbits) - 1));
-        const T xi[3] = { T(ix) * D.dx, T(iy) * D.dx, T(iz) * D.dx };
+        const int ix = (int)((Bx << B::xbits) | (uint32_t)(e >> (B::ybits + B::zbits))) - sd.origin[0];
+        const int iy = (int)((By << B::ybits) | (uint32_t)((e >> B::zbits) & ((1 << B::ybits) - 1))) - sd.origin[1];
+        const int iz = (int)((Bz << B::zbits) | (uint32_t)(e & ((1 << B::zbits) - 1))) - sd.origin[2];
+        // collision objects in double in both forms: a node on an object's
+        // surface (a wall at a whole number of cells) is classified as in double
+        const double xi[3] = { ix * sd.ddx, iy * sd.ddx, iz * sd.ddx };
+        const T dt = sd.dt;
         const T dt_over_mi = dt / m;
         T vi[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            T v = accValue(D, GMV + a, node, SC_MV) / m;
+            T v = accValue(D, GMV + a, node, sc, SC_MV) / m;
             if (D.flip) D.oldv[a * cn + node] = v; // the grid's v before the update (FLIP)
-            vi[a] = v + D.gravity[a] * dt + f[a] * dt_over_mi;
+            vi[a] = v + sd.gravity[a] * dt + f[a] * dt_over_mi;
         }
-        multiObjectCollision(cols, cst, ncols, xi, vi);
+        double vd[3] = { vi[0], vi[1], vi[2] };
+        if (uniform)
+            multiObjectCollision(cols, cst, nstaged, xi, vd);
+        else
+            multiObjectCollision(D.cols + sd.col0, D.colstate + sd.col0, min(sd.ncol, MAX_COLLIDERS), xi, vd);
 #pragma unroll
-        for (int a = 0; a < 3; a++) D.newv[a * cn + node] = vi[a];
+        for (int a = 0; a < 3; a++) D.newv[a * cn + node] = (T)vd[a];
         if (D.any_damage) {
-            const T pw = accValue(D, GPW, node, SC_PW);
-            D.pfn[node] = pw != T(0) ? accValue(D, GPF, node, SC_PF) / pw : T(0);
+            const T pw = accValue(D, GPW, node, sc, SC_PW);
+            D.pfn[node] = pw != T(0) ? accValue(D, GPF, node, sc, SC_PF) / pw : T(0);
         }
     }
 }
@@ -1531,16 +1590,12 @@ __global__ __launch_bounds__(WG) void k_damp_prep(Dev<T, AccT> D)
     const int t = threadIdx.x;
     if constexpr (std::is_same<AccT, long long>::value) {
         if (blockIdx.x == gridDim.x - 1) {
-            const int lane = t & 63, g = t >> 6;
-            if (g == SC_M || g == SC_MV) {
-                const float m = waveMax(__uint_as_float(D.maxbits[g * BOUND_SLOTS + lane]));
-                if (lane == 0 && m > 0.f && isfinite(m)) {
-                    int e;
-                    frexpf(m, &e);
-                    int se = (int)FIXED_TARGET_EXP - e;
-                    se = se < -120 ? -120 : (se > 120 ? 120 : se);
-                    D.scales[g] = ldexpf(1.0f, se);
-                }
+            // one wavefront per (scene, mass or momentum group)
+            const int lane = t & 63;
+            for (int k = t >> 6; k < 2 * D.nscenes; k += WG / 64) {
+                const int g = (k & 1) ? SC_MV : SC_M, sc = k >> 1;
+                const float m = waveMax(__uint_as_float(D.maxbits[(sc * NSC + g) * BOUND_SLOTS + lane]));
+                if (lane == 0 && m > 0.f && isfinite(m)) D.scales[sc * NSC + g] = boundScale(m, 1.f);
             }
             return;
         }
@@ -1562,10 +1617,11 @@ __global__ __launch_bounds__(WG) void k_damp_grid(Dev<T, AccT> D)
     const int64_t nn = (int64_t)D.counters[0] * NB;
     const int64_t cn = D.cap_nodes;
     for (int64_t node = (int64_t)blockIdx.x * WG + threadIdx.x; node < nn; node += (int64_t)gridDim.x * WG) {
-        const T m = accValue(D, GM, node, SC_M);
+        const float* sc = D.scales + blockScene(D, (int)(node / NB)) * NSC;
+        const T m = accValue(D, GM, node, sc, SC_M);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            const T v = m != T(0) ? accValue(D, GMV + a, node, SC_MV) / m : T(0);
+            const T v = m != T(0) ? accValue(D, GMV + a, node, sc, SC_MV) / m : T(0);
             D.newv[a * cn + node] = v;
             if (D.flip) D.oldv[a * cn + node] = v;
         }
@@ -1698,24 +1754,24 @@ struct G2PAcc {
 // the stencil's tile position from that cell of the segment's block (the damping
 // passes: this substep's binning, moved particle); otherwise from the particle.
 template <class T, class AccT, bool FLIP = false>
-__device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const T (*gt)[Tile<T>::TN], const PosT<T>* Xp,
-                                          bool dmg, G2PAcc<T>& A, int cell = -1)
+__device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const SceneDev<T>& sd, const T (*gt)[Tile<T>::TN],
+                                          const PosT<T>* Xp, bool dmg, G2PAcc<T>& A, int cell = -1)
 {
     using TL = Tile<T>;
-    const T odx = D.odx, dx = D.dx;
+    const T odx = sd.odx, dx = sd.dx;
     T w[3][3], dw[3][3], off[3][3];
     int base[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         T d0;
-        cellOf(Xp[a], odx, D.domain, base[a], d0);
+        cellOf(Xp[a], odx, sd.origin[a], sd.domain, base[a], d0);
         bsplineFrac(d0, w[a], dw[a]);
 #pragma unroll
         for (int i = 0; i < 3; i++) { // x_i - x_p
             if constexpr (sizeof(T) == 4)
                 off[a][i] = (T(i) - d0) * dx;
             else
-                off[a][i] = T(base[a] + i) * dx - Xp[a];
+                off[a][i] = T(base[a] - sd.origin[a] + i) * dx - Xp[a];
         }
     }
     const T wzo[3] = { w[2][0] * off[2][0], w[2][1] * off[2][1], w[2][2] * off[2][2] };
@@ -1828,17 +1884,18 @@ __device__ __forceinline__ void loadG2P(const Dev<T, AccT>& D, int s, int p, Raw
 // Returns true for a particle deferred to k_rare (its d, g and stress are
 // completed there).
 template <class T, class AccT, bool FLIP = false>
-__device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const RawG2P<T>& R, T dt, const G2PAcc<T>& A,
-                                          const MatDev<T>& mt, float* bnd, PosT<T>* xnew)
+__device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const RawG2P<T>& R, const SceneDev<T>& sd,
+                                          const G2PAcc<T>& A, const MatDev<T>& mt, float* bnd, PosT<T>* xnew)
 {
     bool rare = false;
     constexpr bool FIXED = std::is_same<AccT, long long>::value;
     const int64_t cap = D.cap;
     T* Q = D.Q;
     const bool dmg = D.any_damage && mt.has_damage;
-    const T dx = D.dx;
-    const T Dinv = D.Dinv;
-    const T r = D.rpic;
+    const T dt = sd.dt;
+    const T dx = sd.dx;
+    const T Dinv = sd.Dinv;
+    const T r = sd.rpic;
     const T ca = (r + T(1)) * T(0.5), cb = (r - T(1)) * T(0.5);
     // grad v of evolveStrain: sum v_i grad(w)^T, or C under MLS-MPM (APIC)
     T gvF[3][3];
@@ -1887,7 +1944,7 @@ __device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const Ra
                 // fixed point: exact integer add of the rounded increment; a
                 // non-finite increment marks the position (the binning latches
                 // it), an out-of-range one saturates (the domain check rejects it)
-                const float u = inc[a] * D.pos_scale;
+                const float u = inc[a] * sd.pos_scale;
                 int xn = POS_NAN;
                 if (u == u && R.x[a] != POS_NAN) {
                     const int du = __float2int_rn(fminf(fmaxf(u, -1073741824.f), 1073741824.f));
@@ -1960,7 +2017,7 @@ __device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     if (!rare) {
 #pragma unroll
         for (int i = 0; i < 6; i++) stAt(Q + (PS + i) * cap, s, S[i]);
-        if constexpr (FIXED) forceBounds(bnd, S, dmg ? d : T(0), D.odx, D.any_damage != 0);
+        if constexpr (FIXED) forceBounds(bnd, S, dmg ? d : T(0), sd.odx, D.any_damage != 0);
     }
     return rare;
 }
@@ -1993,7 +2050,7 @@ __device__ __forceinline__ void stageTile(const Dev<T, AccT>& D, int b, T (*gt)[
 }
 
 template <class T, class AccT, bool FLIP>
-__global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
+__global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D)
 {
     using TL = Tile<T>;
     constexpr int NB = TL::NB;
@@ -2024,6 +2081,8 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
         A.gv[i / 3][i % 3] = T(0);
     }
     A.lap = T(0);
+    const int scene = live ? sceneOf(R.mid) : 0;
+    const SceneDev<T>& sd = D.sc[scene];
     int seg_lo = lo;
     while (seg_lo < hi) {
         const int b = D.sorted_cell[seg_lo] / NB;
@@ -2035,13 +2094,13 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
         stageTile(D, b, gt, nfield, dmg);
         __syncthreads();
         const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
-        if (live && s >= seg_lo && s < seg_hi) g2pGather<T, AccT, FLIP>(D, gt, R.x, dmg, A);
+        if (live && s >= seg_lo && s < seg_hi) g2pGather<T, AccT, FLIP>(D, sd, gt, R.x, dmg, A);
         seg_lo = seg_hi;
     }
     float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
     PosT<T> xnew[3] = { 0, 0, 0 };
     bool rare = false;
-    if (live) rare = g2pFinish<T, AccT, FLIP>(D, s, R, dt, A, D.mats[R.mid], bnd, xnew);
+    if (live) rare = g2pFinish<T, AccT, FLIP>(D, s, R, sd, A, D.mats[matOf(R.mid)], bnd, xnew);
     // deferred particles to the rare-path queue, one atomic per wavefront
     const uint64_t rm = __ballot(rare);
     if (rm) {
@@ -2053,8 +2112,8 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D, T dt)
     }
     // binning of the coming substep at the new positions (live index s)
     openPageScan(D);
-    binParticle(D, s, live, xnew);
-    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd);
+    binParticle(D, s, live, xnew, scene);
+    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd, live ? scene : -1);
 }
 
 // VonMisesStvkHencky::projectStrain (PlasticityApplier.cpp:1480-1516) on the
@@ -2237,45 +2296,52 @@ __device__ __forceinline__ bool plasticMayProject(const MatDev<float>& mt, const
 // (F_{n+1}, g_{n+1}) with every branch of the models, and its force bound.  Runs
 // after k_g2p of the same substep, before the buffers swap.
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_rare(Dev<T, AccT> D, T dt)
+__global__ __launch_bounds__(WG) void k_rare(Dev<T, AccT> D)
 {
     const int cnt = D.counters[CNT_RARE];
     const int64_t cap = D.cap;
-    float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
-    for (int i = blockIdx.x * WG + threadIdx.x; i < cnt; i += gridDim.x * WG) {
-        const int s = ldAt(D.rare, i);
-        if ((unsigned)s >= (unsigned)D.n) continue;
-        int p = ldAt(D.perm, s);
-        if ((unsigned)p >= (unsigned)D.n) p = 0;
-        const MatDev<T>& mt = D.mats[ldAt(D.qmat, s)];
-        const bool dmg = D.any_damage && mt.has_damage;
-        T d = ldAt(D.P + PD * cap, p), g = ldAt(D.P + PG * cap, p);
-        if (dmg && !D.implicit && phaseFieldModel(mt.model)) {
-            const Mat3<T> F = loadM(D.P, cap, PF, p); // F_n (FP32 form: H)
-            damageUpdate<T, false>(mt, F, ldAt(D.Q + PLAP * cap, s), dt, d, g);
-            stAt(D.Q + PD * cap, s, d);
-            stAt(D.Q + PG * cap, s, g);
-        }
-        Mat3<T> Fn = loadM(D.Q, cap, PF, s);
-        T ys = T(0);
-        if (mt.plasticity) {
-            ys = ldAt(D.Q + PYS * cap, s);
-            const bool changed = mt.plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY ? vonMisesProject(mt, Fn, ys)
-                : mt.plasticity == ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY             ? druckerPragerProject(mt, Fn, ys)
-                                                                                         : snowProject(mt, Fn, ys);
-            if (changed) {
-#pragma unroll
-                for (int k = 0; k < 9; k++) stAt(D.Q + (PF + k) * cap, s, Fn.a[k / 3][k % 3]);
-                stAt(D.Q + PYS * cap, s, ys);
+    for (int base = blockIdx.x * WG; base < cnt; base += gridDim.x * WG) {
+        float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
+        const int i = base + threadIdx.x;
+        int s = i < cnt ? ldAt(D.rare, i) : -1;
+        if ((unsigned)s >= (unsigned)D.n) s = -1;
+        int scene = -1;
+        if (s >= 0) {
+            int p = ldAt(D.perm, s);
+            if ((unsigned)p >= (unsigned)D.n) p = 0;
+            const int pm = ldAt(D.qmat, s);
+            scene = sceneOf(pm);
+            const SceneDev<T>& sd = D.sc[scene];
+            const MatDev<T>& mt = D.mats[matOf(pm)];
+            const bool dmg = D.any_damage && mt.has_damage;
+            T d = ldAt(D.P + PD * cap, p), g = ldAt(D.P + PG * cap, p);
+            if (dmg && !D.implicit && phaseFieldModel(mt.model)) {
+                const Mat3<T> F = loadM(D.P, cap, PF, p); // F_n (FP32 form: H)
+                damageUpdate<T, false>(mt, F, ldAt(D.Q + PLAP * cap, s), sd.dt, d, g);
+                stAt(D.Q + PD * cap, s, d);
+                stAt(D.Q + PG * cap, s, g);
             }
-        }
-        T S[6];
-        vtauSym<false>(mt, Fn, ldAt(D.Q + PVOL * cap, s) * snowScale(mt, ys), g, S);
+            Mat3<T> Fn = loadM(D.Q, cap, PF, s);
+            T ys = T(0);
+            if (mt.plasticity) {
+                ys = ldAt(D.Q + PYS * cap, s);
+                const bool changed = mt.plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY ? vonMisesProject(mt, Fn, ys)
+                    : mt.plasticity == ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY             ? druckerPragerProject(mt, Fn, ys)
+                                                                                             : snowProject(mt, Fn, ys);
+                if (changed) {
 #pragma unroll
-        for (int k = 0; k < 6; k++) stAt(D.Q + (PS + k) * cap, s, S[k]);
-        if constexpr (std::is_same<AccT, long long>::value) forceBounds(bnd, S, dmg ? d : T(0), D.odx, D.any_damage != 0);
+                    for (int k = 0; k < 9; k++) stAt(D.Q + (PF + k) * cap, s, Fn.a[k / 3][k % 3]);
+                    stAt(D.Q + PYS * cap, s, ys);
+                }
+            }
+            T S[6];
+            vtauSym<false>(mt, Fn, ldAt(D.Q + PVOL * cap, s) * snowScale(mt, ys), g, S);
+#pragma unroll
+            for (int k = 0; k < 6; k++) stAt(D.Q + (PS + k) * cap, s, S[k]);
+            if constexpr (std::is_same<AccT, long long>::value) forceBounds(bnd, S, dmg ? d : T(0), sd.odx, D.any_damage != 0);
+        }
+        if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd, scene);
     }
-    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd);
 }
 
 // Damping pass G2P, gridToParticles(0) (MpmSimulationBase.cpp:328, 344) with
@@ -2301,13 +2367,15 @@ __global__ __launch_bounds__(WG) void k_damp_g2p(Dev<T, AccT> D)
     T* P = D.P;
     PosT<T> x[3] = { 0, 0, 0 };
     T m = T(0);
-    int cell = 0;
+    int cell = 0, scene = 0;
     if (live) {
 #pragma unroll
         for (int a = 0; a < 3; a++) x[a] = loadPos(P + (PX + a) * cap, s);
         m = ldAt(P + PM * cap, s);
         cell = ldAt(D.sorted_cell, s) & (NB - 1);
+        scene = sceneOf(ldAt(D.pmat, s));
     }
+    const SceneDev<T>& sd = D.sc[scene];
     G2PAcc<T> A;
 #pragma unroll
     for (int a = 0; a < 3; a++) A.picV[a] = A.oldV[a] = T(0);
@@ -2328,13 +2396,13 @@ __global__ __launch_bounds__(WG) void k_damp_g2p(Dev<T, AccT> D)
         stageTile(D, b, gt, 3, false);
         __syncthreads();
         const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
-        if (live && s >= seg_lo && s < seg_hi) g2pGather<T, AccT, false>(D, gt, x, false, A, cell);
+        if (live && s >= seg_lo && s < seg_hi) g2pGather<T, AccT, false>(D, sd, gt, x, false, A, cell);
         seg_lo = seg_hi;
     }
     float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
     if (live) {
         T Cn[9];
-        const T Dinv = D.Dinv;
+        const T Dinv = sd.Dinv;
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -2344,9 +2412,9 @@ __global__ __launch_bounds__(WG) void k_damp_g2p(Dev<T, AccT> D)
         for (int a = 0; a < 3; a++) stAt(P + (PV + a) * cap, s, A.picV[a]);
 #pragma unroll
         for (int i = 0; i < 9; i++) stAt(P + (PC + i) * cap, s, Cn[i]);
-        if constexpr (std::is_same<AccT, long long>::value) momentumBounds(bnd, m, A.picV, Cn, D.dx);
+        if constexpr (std::is_same<AccT, long long>::value) momentumBounds(bnd, m, A.picV, Cn, sd.dx);
     }
-    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd);
+    if constexpr (std::is_same<AccT, long long>::value) writeBounds(D.maxbits, bnd, live ? scene : -1);
 }
 
 // evalMaxParticleSpeed (MpmSimulationBase.cpp:1613-1638): per workgroup the
@@ -2374,34 +2442,56 @@ __device__ __forceinline__ double blockMax(double v)
 }
 
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_extent(Dev<T, AccT> D, int fold)
+__global__ __launch_bounds__(WG) void k_extent(Dev<T, AccT> D, int fold, int rows, double* out, double max_dt,
+                                                double cfl_dx)
 {
     const double lowest = -3.4028234663852886e38; // std::numeric_limits<float>::lowest()
-    double r[7] = { 0.0, lowest, lowest, lowest, lowest, lowest, lowest };
+    double r[8] = { 0.0, lowest, lowest, lowest, lowest, lowest, lowest, 0.0 };
     if (fold) {
-        for (int w = threadIdx.x; w < EXTENT_WGS; w += WG)
+        for (int w = threadIdx.x; w < rows; w += WG)
 #pragma unroll
-            for (int k = 0; k < 7; k++) r[k] = fmax(r[k], D.extent[w * 8 + k]);
+            for (int k = 0; k < 8; k++) r[k] = fmax(r[k], D.extent[w * 8 + k]);
     }
     else {
         const int64_t cap = D.cap;
         for (int p = blockIdx.x * WG + threadIdx.x; p < D.n; p += gridDim.x * WG) {
+            const SceneDev<T>& sd = D.sc[sceneOf(ldAt(D.pmat, p))];
             const T v0 = ldAt(D.P + (PV + 0) * cap, p), v1 = ldAt(D.P + (PV + 1) * cap, p), v2 = ldAt(D.P + (PV + 2) * cap, p);
-            r[0] = fmax(r[0], (double)dsqrt(v0 * v0 + v1 * v1 + v2 * v2));
+            const double speed = (double)dsqrt(v0 * v0 + v1 * v1 + v2 * v2);
+            r[0] = fmax(r[0], speed);
+            r[7] = fmax(r[7], speed * sd.inv_cfl_dx);
 #pragma unroll
             for (int a = 0; a < 3; a++) {
-                const double x = posMetres(loadPos(D.P + (PX + a) * cap, p), (double)D.dx);
+                // in the scene's own frame
+                PosT<T> X = loadPos(D.P + (PX + a) * cap, p);
+                if constexpr (sizeof(T) == 4)
+                    if (X != POS_NAN) X -= sd.origin[a] << POS_BITS;
+                const double x = posMetres(X, sd.ddx);
                 r[1 + a] = fmax(r[1 + a], x);
                 r[4 + a] = fmax(r[4 + a], -x);
             }
         }
     }
 #pragma unroll
-    for (int k = 0; k < 7; k++) r[k] = blockMax(r[k]);
+    for (int k = 0; k < 8; k++) r[k] = blockMax(r[k]);
     if (threadIdx.x == 0) {
-        const int row = fold ? EXTENT_WGS : blockIdx.x;
+        const int row = fold ? rows : blockIdx.x;
 #pragma unroll
-        for (int k = 0; k < 7; k++) D.extent[row * 8 + k] = r[k];
+        for (int k = 0; k < 8; k++) D.extent[row * 8 + k] = r[k];
+        // the folded result as calculateDt returns it: dt, speed, min corner, max
+        // corner (one scene: cfl dx / speed; a batch: the smallest scene step)
+        if (fold && out) {
+            if (D.nscenes == 1)
+                out[0] = r[0] != 0.0 ? cfl_dx / r[0] : max_dt;
+            else
+                out[0] = r[7] != 0.0 ? 1.0 / r[7] : max_dt;
+            out[1] = r[0];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                out[2 + a] = -r[4 + a];
+                out[5 + a] = r[1 + a];
+            }
+        }
     }
 }
 
@@ -2535,8 +2625,11 @@ __device__ __forceinline__ void activeCells(const int* cs, const int* ce, int lo
 }
 
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_dtilde(Dev<T, AccT> D, T dt)
+__global__ __launch_bounds__(WG) void k_dtilde(Dev<T, AccT> D)
 {
+    // the implicit solve runs on one-scene handles only
+    const SceneDev<T>& sd = D.sc[0];
+    const T dt = sd.dt;
     using B = Blk<T>;
     using TL = Tile<T>;
     constexpr int NB = TL::NB;
@@ -2570,7 +2663,7 @@ __global__ __launch_bounds__(WG) void k_dtilde(Dev<T, AccT> D, T dt)
             if (s < hi) {
                 int p = ldAt(D.perm, s);
                 if ((unsigned)p >= (unsigned)D.n) p = 0;
-                const MatDev<T>& mt = D.mats[ldAt(D.pmat, p)];
+                const MatDev<T>& mt = D.mats[matOf(ldAt(D.pmat, p))];
                 T hist = ldAt(D.P + PDT * cap, p);
                 T kap = T(0);
                 T d0[3] = { T(0), T(0), T(0) };
@@ -2590,7 +2683,7 @@ __global__ __launch_bounds__(WG) void k_dtilde(Dev<T, AccT> D, T dt)
                     kap = vol * dte * mt.l0 * mt.l0;
                     int base;
 #pragma unroll
-                    for (int a = 0; a < 3; a++) cellOf(loadPos(D.P + (PX + a) * cap, p), D.odx, D.domain, base, d0[a]);
+                    for (int a = 0; a < 3; a++) cellOf(loadPos(D.P + (PX + a) * cap, p), sd.odx, sd.origin[a], sd.domain, base, d0[a]);
                     c3[0] = (double)vol;                                             // sum vol w
                     c3[1] = (double)((dt * hist) / mt.eta) * (double)vol;            // D~ part of rhs
                     c3[2] = (double)vol * (1.0 + (double)dte * ((double)hist + 1.0)); // diagonal
@@ -2669,7 +2762,7 @@ __global__ __launch_bounds__(CGW) void k_cg(Dev<T, AccT> D)
     const double* const DG = D.cgv + 4 * cn;
     double* const part = D.cg_part;
     const int64_t first = (int64_t)blockIdx.x * CGW + t, stride = (int64_t)nwg * CGW;
-    const double odx = (double)D.odx;
+    const double odx = (double)D.sc[0].odx; // the implicit solve runs on one-scene handles only
     unsigned epoch = 0, tickets = 0;
     int ovf = 0;
     // r = b - A 0 = b with b_i = sum vol dt D~ / eta w + pf_i sum vol w; x = 0

@@ -142,16 +142,51 @@ constexpr int MAX_COLLIDERS = 16; // collision objects per handle
 struct ColDev {
     int shape, type;
     int moving; // rotation or scaling: R(t), s(t) evaluated; otherwise x - b(t) is material space
-    int pad;
+    int scene;  // the scene it belongs to
     double origin[3], n[3], bmin[3], bmax[3], b0[3], dbdt[3], friction;
     double radius, height;
     double Rc[9], Rci[9]; // capped cylinder primitive rotation and its inverse (row-major)
     double q0[4], omega[3], s0, dsdt;
 };
 
+// Per-substep state of a collision object: rotation R(t), translation b(t),
+// scaling s(t) and 1 / s at its scene's time (the reference's updateState at the
+// substep's time, MpmSimulationBase.cpp:209-213).
+struct ColState {
+    double R[9];
+    double b[3];
+    double s, os;
+};
+
 struct Masks {
     uint64_t x, y, z;
 };
+
+// One scene of a handle: a batch holds several independent scenes, each in its
+// own cubic region of the handle's grid (side `region` nodes, a power of two,
+// at `origin`), with its own dx, dt, gravity, transfer ratio, CFL number,
+// collision objects and clock.  Particles carry their scene in the high bits
+// of their material word (PMAT_SCENE_SHIFT); a grid block's scene follows from
+// its coordinates (region_scene).
+template <class T>
+struct SceneDev {
+    T dx, odx;
+    T pos_scale;   // FP32 form: metres -> position units, odx * 2^POS_BITS
+    T Dinv;        // 4 / dx^2 (MpmSimulationBase.cpp:119-122)
+    T gravity[3];
+    T rpic;        // apic_rpic_ratio
+    T cfl;
+    T dt;          // this substep's dt
+    double ddt;    // the same in double (the clock)
+    double ddx;    // dx in double: node positions for the collision objects
+    double inv_cfl_dx; // 1 / (cfl dx): a particle speed times this is 1 / its CFL step
+    int origin[3]; // first node of the scene's region
+    int domain;    // the scene's own addressable nodes per axis (base nodes in [0, domain - 4))
+    int col0, ncol; // its collision objects
+};
+
+constexpr int PMAT_SCENE_SHIFT = 8; // material index in bits 0-7, scene above
+constexpr int PMAT_MAT_MASK = 0xff;
 
 // Kernel-visible state of one handle.
 template <class T, class AccT>
@@ -186,7 +221,7 @@ struct Dev {
     int* cell_count;   // [cap_blocks*NB]
     int* cell_start;   // [cap_blocks*NB]
     int* counters;     // [0] num_blocks, [1] status bits, [2] scan epoch, [3] scan ticket, [4] rare count
-    double* clock;     // [0] time of the next substep, [1] time of the running substep
+    double* clock;     // per scene: [2 s] time of the next substep, [2 s + 1] time of the running substep
     int cap_blocks;
     int domain;
     Masks masks;
@@ -196,14 +231,15 @@ struct Dev {
     T* oldv;           // [3][cap_nodes] v = mv/m before the update (FLIP_blend_PIC)
     T* pfn;            // [cap_nodes]
     int64_t cap_nodes;
-    // fixed-point scales (float form only)
-    float* scales;     // [NSC] current scale 2^e
-    unsigned int* maxbits; // [NSC][64] max |contribution| bound (float bits), atomic-max slots
+    // fixed-point scales (float form only), per scene
+    float* scales;     // [scenes][NSC] current scale 2^e
+    unsigned int* maxbits; // [scenes][NSC][BOUND_SLOTS] max |contribution| bound (float bits), atomic-max slots
     double* extent;        // [EXTENT_WGS + 1][8] particle speed / bounding-box partial maxima
     int max_slots;         // workgroups of the particle-order launches (G2P / upload)
     // tables
     const MatDev<T>* mats;
-    const ColDev* cols;
+    const ColDev* cols;  // grouped by scene (SceneDev::col0, ncol)
+    ColState* colstate;  // [num_cols] states at the running substep's time (written by k_scatter)
     int num_cols;
     int any_damage;
     int any_plastic;
@@ -221,13 +257,12 @@ struct Dev {
     T* cgp;            // [4][cap] per sorted particle: d0 x, y, z; vol dt / eta l0^2 (0: not in the solve)
     double* cg_part;   // [3][CG_MAX_WGS] per-workgroup partial sums
     unsigned* cg_bar;  // [0] barrier arrivals, [1] iterations of the last solve, [2] block tickets
-    // scene
-    T dx, odx;
-    T pos_scale; // FP32 form: metres -> position units, odx * 2^POS_BITS
-    T Dinv; // 4 / dx^2 (MpmSimulationBase.cpp:119-122)
-    T gravity[3];
-    T rpic;
-    T cfl;
+    // scenes
+    const SceneDev<T>* sc;     // [nscenes]
+    int nscenes;
+    const int* region_scene;   // [region_dim^3] scene of each region, -1 for none
+    int region_shift;          // log2 of the region side in nodes
+    int region_dim;            // regions per axis
 };
 
 } // namespace zmpm

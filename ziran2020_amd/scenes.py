@@ -38,6 +38,7 @@ class Scene:
     C: np.ndarray = None
     info: dict = field(default_factory=dict)
     options: dict = field(default_factory=dict)  # zmpm_scene_desc options of the workload (e.g. implicit_damage)
+    t0: float = 0.0  # simulation time of the first substep
 
     def __post_init__(self):
         n = self.x.shape[0]
@@ -51,18 +52,83 @@ class Scene:
     def n(self):
         return int(self.x.shape[0])
 
-    def make(self, lib, precision=A.PREC_FP64, device=0, apic_rpic_ratio=1.0, **transfer):
-        """transfer: transfer_scheme, flip_pic_ratio, mls_mpm, rpic_damping_iteration,
-        pic_damping_iteration, implicit_damage (zmpm_scene_desc); they extend and
-        override the workload's own options."""
-        sim = A.Simulation(lib, dx=self.dx, precision=precision, domain_nodes=self.domain_nodes,
-                           gravity=self.gravity, apic_rpic_ratio=apic_rpic_ratio, device=device,
-                           **{**self.options, **transfer})
+    def desc(self, precision=A.PREC_FP64, device=0, **transfer):
+        """zmpm_scene_desc fields of the scene.  transfer: apic_rpic_ratio,
+        transfer_scheme, flip_pic_ratio, mls_mpm, rpic_damping_iteration,
+        pic_damping_iteration, implicit_damage; they extend and override the
+        workload's own options."""
+        return dict(dx=self.dx, precision=precision, domain_nodes=self.domain_nodes, gravity=self.gravity,
+                    device=device, **{**self.options, **transfer})
+
+    def make(self, lib, precision=A.PREC_FP64, device=0, **transfer):
+        sim = A.Simulation(lib, **self.desc(precision, device, **transfer))
         sim.set_materials(self.materials)
         for c in self.colliders:
             sim.add_collider(c)
         sim.upload(self.x, self.v, self.C, self.F, self.m, self.V0, self.material)
         return sim
+
+    def run(self, sim, steps, start=0):
+        """Substeps start .. start + steps - 1 of the scene (time t0 + start dt)."""
+        return sim.run(steps, self.dt, self.t0 + start * self.dt)
+
+
+@dataclass
+class SceneBatch:
+    """Independent scenes run in one handle (zmpm_create_batch): each keeps its
+    own dx, dt, start time, gravity, transfer ratio, domain and colliders; the
+    material table is shared (every scene's `materials` is the same list)."""
+    name: str
+    scenes: list
+    steps: int
+    info: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return sum(s.n for s in self.scenes)
+
+    @property
+    def materials(self):
+        return self.scenes[0].materials
+
+    @property
+    def dts(self):
+        return np.array([s.dt for s in self.scenes])
+
+    @property
+    def t0s(self):
+        return np.array([s.t0 for s in self.scenes])
+
+    @property
+    def dt(self):
+        """The smallest scene step (for reports; each scene runs its own)."""
+        return float(self.dts.min())
+
+    @property
+    def scene_of(self):
+        return np.concatenate([np.full(s.n, i, np.int32) for i, s in enumerate(self.scenes)])
+
+    def cat(self, field_):
+        return np.concatenate([getattr(s, field_) for s in self.scenes])
+
+    def subset(self, keep):
+        return SceneBatch(self.name, [self.scenes[i] for i in keep], self.steps,
+                          {**self.info, "scenes": [self.info.get("scenes", list(range(len(self.scenes))))[i]
+                                                   for i in keep]})
+
+    def make(self, lib, precision=A.PREC_FP64, device=0, **transfer):
+        assert all(s.materials is self.materials for s in self.scenes), "scenes of a batch share one material table"
+        sim = A.Batch(lib, [s.desc(precision, device, **transfer) for s in self.scenes])
+        sim.set_materials(self.materials)
+        for i, s in enumerate(self.scenes):
+            for c in s.colliders:
+                sim.add_scene_collider(i, c)
+        sim.upload(self.cat("x"), self.cat("v"), self.cat("C"), self.cat("F"), self.cat("m"), self.cat("V0"),
+                   self.cat("material"), scene=self.scene_of)
+        return sim
+
+    def run(self, sim, steps, start=0):
+        return sim.run_batch(steps, self.dts, self.t0s + start * self.dts)
 
 
 def domain_walls(domain_nodes, dx, type_=A.STICKY):
@@ -194,13 +260,79 @@ def sparse_spheres(n=8_000_000, n_spheres=20, dx=1.0 / 256, seed=3, steps=100, d
                  info=dict(radii=rs, E=list(Es)))
 
 
-def batched_scenes(n_scenes=64, n_range=(10_000, 500_000), dx=1.0 / 128, seed=4, steps=100, dt=1e-4,
+def batched_scenes(n_scenes=64, n_range=(10_000, 500_000), seed=4, steps=100, domain_nodes=64, E=1e4, nu=0.3,
+                   rho=1000.0, sigma_crit=5e3, keep=None):
+    """Batched uneven scenes in one launch per stage: n_scenes independent scenes
+    (a SceneBatch), particle counts log-uniform over n_range; scene s has its own
+    dx (1/96, 1/128 or 1/160 by s mod 3), dt (8e-5 .. 1.25e-4), start time,
+    gravity (-9.8 m/s^2 tilted up to 15 degrees), APIC/RPIC ratio, initial
+    velocity and colliders: the six sticky domain walls at 5 dx, a floor at
+    10 dx that is sticky, slipping with friction, separating or frictionless
+    slip (s mod 4), and on some scenes a moving sphere, a rotating box or a
+    growing capped cylinder.  Even scenes are QRStableNeoHookean with AnisoMPM
+    damage (l0 = 2 dx of the scene, fibres +-15 degrees about x), odd scenes
+    fixed-corotated or Neo-Hookean; ~8 particles per cell in a cube of at most
+    40 cells.  keep: indices of the scenes to place (the others are drawn and
+    dropped, so every kept scene is exactly its copy in the full batch)."""
+    rng = np.random.default_rng(seed)
+    dxs = (1.0 / 96, 1.0 / 128, 1.0 / 160)
+    mats = []
+    for dx in dxs:  # material 3k + {0, 1, 2} for the scenes of dx option k
+        mats += [A.make_material(A.MODEL_FIXED_COROTATED, E, nu),
+                 A.make_material(A.MODEL_QR_STABLE_NEO_HOOKEAN, E, nu, has_damage=True, alphas=(-0.9,),
+                                 l0=2 * dx, sigma_crit=sigma_crit),
+                 A.make_material(A.MODEL_NEO_HOOKEAN, 2 * E, 0.35)]
+    counts = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), n_scenes)).astype(int)
+    out, kept = [], []
+    for s, k in enumerate(counts):
+        opt = s % 3
+        dx = dxs[opt]
+        dt = float(rng.uniform(0.8e-4, 1.25e-4))
+        t0 = float(rng.uniform(0.0, 0.05))
+        tilt, az = np.deg2rad(rng.uniform(0, 15)), rng.uniform(0, 2 * np.pi)
+        gravity = (9.8 * np.sin(tilt) * np.cos(az), -9.8 * np.cos(tilt), 9.8 * np.sin(tilt) * np.sin(az))
+        rpic = float(rng.choice([1.0, 0.5, 0.0]))
+        cells = min((k / 8.0) ** (1.0 / 3.0), 40.0)
+        side = cells * dx
+        lo = np.array([12 * dx, 10 * dx, 12 * dx]) + rng.uniform(0, 1, 3) * np.array([4, 2, 4]) * dx
+        xk = uniform_box(rng, lo, lo + side, k)
+        v0 = rng.uniform(-0.3, 0.3, 3)
+        damaged = (s % 2 == 0)
+        Fk = fibre_F(rng, k) if damaged else identity_F(k)
+        mat = 3 * opt + (1 if damaged else int(rng.integers(0, 2)) * 2)
+        floor_type, friction = [(A.STICKY, 0.0), (A.SLIP, 0.3), (A.SEPARATE, 0.1), (A.SLIP, 0.0)][s % 4]
+        cols = domain_walls(domain_nodes, dx) + [A.half_space((0, 10 * dx, 0), (0, 1, 0), floor_type,
+                                                              friction=friction)]
+        top = lo + side
+        ctr = lo + side / 2
+        if s % 5 == 0:  # a sphere falling onto the top face
+            r = 0.3 * side
+            cols.append(A.sphere((0, 0, 0), r, A.SLIP, b0=(ctr[0], top[1] + r + 2 * dx, ctr[2]),
+                                 dbdt=(0.0, -2.0, 0.0)))
+        elif s % 5 == 1:  # a box spinning about y, pressing on one edge
+            h = 0.2 * side
+            cols.append(A.box((-h, -h, -h), (h, h, h), A.STICKY, b0=(lo[0], top[1], ctr[2]), omega=(0.0, 4.0, 0.0)))
+        elif s % 5 == 2:  # a growing capped cylinder beside the cube
+            cols.append(A.capped_cylinder(2 * dx, side, center=(0, 0, 0), type_=A.SEPARATE,
+                                          b0=(top[0] + 4 * dx, ctr[1], ctr[2]), scale0=1.0, dsdt=5.0))
+        if keep is not None and s not in keep:
+            continue
+        kept.append(s)
+        n = xk.shape[0]
+        V0 = np.full(n, side ** 3 / n)
+        out.append(Scene(f"scene{s}", dx, domain_nodes, dt, steps, gravity, mats, cols, xk,
+                         np.full(n, mat, np.int32), np.full(n, rho), V0, Fk, v=np.tile(v0, (n, 1)),
+                         options=dict(apic_rpic_ratio=rpic), t0=t0))
+    return SceneBatch("batched_scenes", out, steps, info=dict(counts=counts.tolist(), scenes=kept))
+
+
+def shared_grid_scenes(n_scenes=64, n_range=(10_000, 500_000), dx=1.0 / 128, seed=4, steps=100, dt=1e-4,
                    domain_nodes=512, tile_cells=48, E=1e4, nu=0.3, rho=1000.0, sigma_crit=5e3, keep=None):
-    """Batched uneven scenes in one launch: n_scenes independent cubes with particle
-    counts log-uniform over n_range, each in its own tile of one world grid on a
-    shared sticky floor; odd scenes fixed-corotated, even scenes QRStableNeoHookean
-    with AnisoMPM damage.  keep: indices of the scenes to place (the others are
-    drawn and dropped, so every kept scene is exactly its copy in the full batch)."""
+    """Uneven cubes in one world grid (one scene, one dx and dt): n_scenes cubes
+    with particle counts log-uniform over n_range, each in its own tile on a
+    shared sticky floor; odd cubes fixed-corotated, even cubes QRStableNeoHookean
+    with AnisoMPM damage.  keep: indices of the cubes to place (the others are
+    drawn and dropped, so every kept cube is exactly its copy in the full set)."""
     rng = np.random.default_rng(seed)
     counts = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), n_scenes)).astype(int)
     per_row = int(np.ceil(np.sqrt(n_scenes)))
@@ -228,7 +360,7 @@ def batched_scenes(n_scenes=64, n_range=(10_000, 500_000), dx=1.0 / 128, seed=4,
     x = np.concatenate(xs)
     n = x.shape[0]
     cols = domain_walls(domain_nodes, dx) + [A.half_space((0, off, 0), (0, 1, 0), A.STICKY)]
-    return Scene("batched_scenes", dx, domain_nodes, dt, steps, (0.0, -9.8, 0.0), mats, cols, x,
+    return Scene("shared_grid_scenes", dx, domain_nodes, dt, steps, (0.0, -9.8, 0.0), mats, cols, x,
                  np.concatenate(mid), np.full(n, rho), np.concatenate(V0), np.concatenate(Fs),
                  info=dict(counts=counts.tolist(), scenes=kept))
 
@@ -294,5 +426,6 @@ WORKLOADS = {
     "damage_slab_implicit": damage_slab_implicit,
     "sparse_spheres": sparse_spheres,
     "batched_scenes": batched_scenes,
+    "shared_grid_scenes": shared_grid_scenes,
     "two_material": two_material,
 }
