@@ -349,6 +349,20 @@ zmpm_status zmpm_get_timings(zmpm_handle h, int32_t* count, const char** names, 
 zmpm_status zmpm_calculate_dt(zmpm_handle h, double max_dt, double* dt, double* max_speed, double* min_corner,
                               double* max_corner);
 
+/* zmpm_calculate_dt without allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The same device
+ * reduction as zmpm_calculate_dt, asynchronous on the handle's stream: its
+ * per-workgroup partial maxima go to a fixed static device array (1025 rows of
+ * 8 doubles, 64.1 KiB per GPU, held until the process ends) instead of the
+ * handle's buffer, and the result to `out`, caller-owned DEVICE memory of 8
+ * doubles: dt, the largest particle speed, min corner[3], max corner[3].
+ * reduction_wgs (1..1024) sets the workgroups of the first pass; larger values
+ * are rejected with ZMPM_E_ARG.  Calls nothing that allocates.  Every call on
+ * one GPU shares the array (each call overwrites the rows it uses): calls that
+ * overlap on one GPU -- other handles' streams, other threads -- race. */
+zmpm_status zmpm_calculate_dt_noalloc(zmpm_handle h, double max_dt, int32_t reduction_wgs, double* out);
+
 /* Launch configuration of the grid-stride kernels (workgroups of 256 threads)
  * and the P2G lane split.  The workgroup counts do not change any result; the
  * P2G split cap (most lanes sharing one stencil column of a cell) changes how

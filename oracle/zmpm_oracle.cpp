@@ -2449,6 +2449,16 @@ zmpm_status zmpm_calculate_dt(zmpm_handle h, double max_dt, double* dt, double* 
     return ZMPM_OK;
 }
 
+// The allocation-free variant has no device here: `out` is host memory of 8
+// doubles (dt, max speed, min corner[3], max corner[3]); reduction_wgs is
+// checked against the same limit.
+zmpm_status zmpm_calculate_dt_noalloc(zmpm_handle h, double max_dt, int32_t reduction_wgs, double* out)
+{
+    if (!h || !(max_dt > 0) || !out) return ZMPM_E_ARG;
+    if (reduction_wgs < 1 || reduction_wgs > 1024) return fail(h, ZMPM_E_ARG, "reduction_wgs must be in [1, 1024]");
+    return zmpm_calculate_dt(h, max_dt, &out[0], &out[1], &out[2], &out[5]);
+}
+
 zmpm_status zmpm_get_timings(zmpm_handle h, int32_t* count, const char**, double*)
 {
     if (!h || !count) return ZMPM_E_ARG;

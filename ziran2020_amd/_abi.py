@@ -239,7 +239,7 @@ class ZmpmLib:
         "zmpm_download", "zmpm_substep", "zmpm_run", "zmpm_synchronize", "zmpm_get_binning",
         "zmpm_get_grid", "zmpm_set_timing", "zmpm_get_timings", "zmpm_calculate_dt", "zmpm_get_launch_config",
         "zmpm_set_launch_config", "zmpm_autotune", "zmpm_create_batch", "zmpm_batch_origin",
-        "zmpm_add_scene_collider", "zmpm_upload_batch", "zmpm_run_batch",
+        "zmpm_add_scene_collider", "zmpm_upload_batch", "zmpm_run_batch", "zmpm_calculate_dt_noalloc",
     )
 
     def __init__(self, path: str):
@@ -282,6 +282,7 @@ class ZmpmLib:
         L.zmpm_add_scene_collider.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Collider), C.POINTER(C.c_int32)]
         L.zmpm_upload_batch.argtypes = [C.c_void_p, C.POINTER(Particles), _IP]
         L.zmpm_run_batch.argtypes = [C.c_void_p, C.c_int32, _DP, _DP]
+        L.zmpm_calculate_dt_noalloc.argtypes = [C.c_void_p, C.c_double, C.c_int32, _DP]
         if L.zmpm_abi_version() != ZMPM_ABI_VERSION:
             raise RuntimeError(f"{path}: ABI version {L.zmpm_abi_version()} != {ZMPM_ABI_VERSION}")
 

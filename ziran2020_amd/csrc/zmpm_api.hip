@@ -9,10 +9,20 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace zmpm;
+
+// Scratch of zmpm_calculate_dt_noalloc: the per-workgroup partial maxima of its
+// first reduction pass (one row of 8 doubles per workgroup) and the folded row,
+// at most NOALLOC_MAX_WGS + 1 rows (64.1 KiB), one copy per GPU held until the
+// process ends.  Host code never passes this name to a kernel or a copy: it
+// stands for a host-side shadow; the device address comes from
+// hipGetSymbolAddress once per device (noallocScratch below).
+constexpr int NOALLOC_MAX_WGS = 1024;
+static __device__ __align__(256) double scratch[(NOALLOC_MAX_WGS + 1) * 8];
 
 namespace {
 
@@ -49,6 +59,7 @@ uint32_t hostBitPack(uint64_t data, uint64_t mask)
 struct zmpm_context {
     zmpm_scene_desc desc {}; // scene 0's description: the options every scene shares
     std::string err;
+    const char* err_static = nullptr; // message of the allocation-free entry point
     hipStream_t stream = nullptr;
     bool fp32 = false;
     bool any_damage = false;
@@ -98,8 +109,38 @@ namespace {
 
 zmpm_status fail(zmpm_context* h, zmpm_status s, const std::string& msg)
 {
-    if (h) h->err = msg;
+    if (h) {
+        h->err = msg;
+        h->err_static = nullptr;
+    }
     return s;
+}
+
+// Error of the allocation-free entry point: a static message, no string copy.
+zmpm_status failStatic(zmpm_context* h, zmpm_status s, const char* msg)
+{
+    if (h) h->err_static = msg;
+    return s;
+}
+
+// Device address of `scratch` on each device, taken once per device (with that
+// device current) by zmpm_create.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+zmpm_status setUpNoallocScratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return ZMPM_E_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return ZMPM_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return ZMPM_E_HIP;
+    }
+    g_scratch[device] = static_cast<double*>(p);
+    return ZMPM_OK;
 }
 
 #define HIPCHK(h, x)                                                                            \
@@ -868,6 +909,9 @@ zmpm_status createScenes(const zmpm_scene_desc* descs, int32_t n_scenes, zmpm_ha
     const int device = descs[0].device;
     if (device < 0 || device >= ndev) return ZMPM_E_ARG;
     if (hipSetDevice(device) != hipSuccess) return ZMPM_E_HIP;
+    // the allocation-free entry point's scratch address on this device (a
+    // failure here only disables that entry point)
+    (void)setUpNoallocScratch(device);
     auto* h = new zmpm_context();
     h->desc = descs[0];
     h->scenes.assign(descs, descs + n_scenes);
@@ -962,7 +1006,7 @@ zmpm_status zmpm_destroy(zmpm_handle h)
 const char* zmpm_last_error(zmpm_handle h)
 {
     if (!h) return "null handle";
-    return h->err.c_str();
+    return h->err_static ? h->err_static : h->err.c_str();
 }
 
 zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material* mats)
@@ -1273,6 +1317,35 @@ zmpm_status zmpm_calculate_dt(zmpm_handle h, double max_dt, double* dt, double* 
         else
             *dt = e[7] != 0 ? 1.0 / e[7] : max_dt;
     }
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_calculate_dt_noalloc(zmpm_handle h, double max_dt, int32_t reduction_wgs, double* out)
+{
+    if (!h) return ZMPM_E_ARG;
+    if (!(max_dt > 0) || !out) return failStatic(h, ZMPM_E_ARG, "max_dt must be positive and out a device buffer");
+    if (reduction_wgs < 1 || reduction_wgs > NOALLOC_MAX_WGS)
+        return failStatic(h, ZMPM_E_ARG, "reduction_wgs must be in [1, 1024]: the static scratch holds 1024 partial rows");
+    if (hipSetDevice(h->desc.device) != hipSuccess) return failStatic(h, ZMPM_E_HIP, "hipSetDevice failed");
+    double* scr = (h->desc.device >= 0 && h->desc.device < kMaxDevices) ? g_scratch[h->desc.device] : nullptr;
+    if (!scr) return failStatic(h, ZMPM_E_HIP, "static scratch not set up on this device");
+    const double cfl_dx = h->desc.cfl * h->desc.dx;
+    // the handle's kernel state with the partial maxima in the static scratch
+    if (h->fp32) {
+        Dev<float, long long> D = h->Df;
+        D.extent = scr;
+        if (h->n == 0) D.n = 0;
+        k_extent<float, long long><<<reduction_wgs, WG, 0, h->stream>>>(D, 0, reduction_wgs, nullptr, max_dt, cfl_dx);
+        k_extent<float, long long><<<1, WG, 0, h->stream>>>(D, 1, reduction_wgs, out, max_dt, cfl_dx);
+    }
+    else {
+        Dev<double, double> D = h->Dd;
+        D.extent = scr;
+        if (h->n == 0) D.n = 0;
+        k_extent<double, double><<<reduction_wgs, WG, 0, h->stream>>>(D, 0, reduction_wgs, nullptr, max_dt, cfl_dx);
+        k_extent<double, double><<<1, WG, 0, h->stream>>>(D, 1, reduction_wgs, out, max_dt, cfl_dx);
+    }
+    if (hipGetLastError() != hipSuccess) return failStatic(h, ZMPM_E_HIP, "kernel launch failed");
     return ZMPM_OK;
 }
 
