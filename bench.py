@@ -196,18 +196,23 @@ def _field_errors(a, b, keys):
 def _grid_errors(ga, gb):
     """Normwise error of grid m and mv over the union of the active nodes
     (matched by node coordinates; a node active in one only counts as 0 in the other)."""
-    def table(g):
-        return {tuple(c): (m, m * v) for c, m, v in zip(g["coords"].tolist(), g["m"], g["v"])}
-    ta, tb = table(ga), table(gb)
-    keys = sorted(set(ta) | set(tb))
-    z = (0.0, np.zeros(3))
-    ma = np.array([ta.get(k, z)[0] for k in keys])
-    mb = np.array([tb.get(k, z)[0] for k in keys])
-    pa = np.array([ta.get(k, z)[1] for k in keys])
-    pb = np.array([tb.get(k, z)[1] for k in keys])
+    def keyed(g):
+        c = g["coords"].astype(np.int64)
+        return (c[:, 0] << 40) | (c[:, 1] << 20) | c[:, 2]
+    ka, kb = keyed(ga), keyed(gb)
+    keys = np.union1d(ka, kb)
+
+    def spread(k, g):
+        idx = np.searchsorted(keys, k)
+        m, mv = np.zeros(keys.size), np.zeros((keys.size, 3))
+        m[idx] = g["m"]
+        mv[idx] = g["m"][:, None] * g["v"]
+        return m, mv
+    ma, pa = spread(ka, ga)
+    mb, pb = spread(kb, gb)
     return {"m": float(np.abs(ma - mb).max() / np.abs(mb).max()),
             "mv": float(np.abs(pa - pb).max() / np.abs(pb).max()),
-            "nodes_only_in_one": len(keys) - len(set(ta) & set(tb))}
+            "nodes_only_in_one": int(keys.size - np.intersect1d(ka, kb).size)}
 
 
 def accuracy_vs_fp64(scene, lib, steps=100):

@@ -11,6 +11,29 @@ def normwise(a, b):
     return num / den if den > 0 else num
 
 
+def grid_errors(ga, gb):
+    """Normwise error of grid mass and momentum (m v) of grid ga against gb over
+    the union of their active nodes, matched by node coordinates (a node active
+    in one only counts as 0 in the other)."""
+    def keyed(g):
+        c = g["coords"].astype(np.int64)
+        return (c[:, 0] << 40) | (c[:, 1] << 20) | c[:, 2]
+    ka, kb = keyed(ga), keyed(gb)
+    keys = np.union1d(ka, kb)
+    def spread(k, g):
+        idx = np.searchsorted(keys, k)
+        m = np.zeros(keys.size)
+        mv = np.zeros((keys.size, 3))
+        m[idx] = g["m"]
+        mv[idx] = g["m"][:, None] * g["v"]
+        return m, mv
+    ma, pa = spread(ka, ga)
+    mb, pb = spread(kb, gb)
+    return {"m": float(np.abs(ma - mb).max() / np.abs(mb).max()),
+            "mv": float(np.abs(pa - pb).max() / np.abs(pb).max()),
+            "nodes_only_in_one": int(keys.size - np.intersect1d(ka, kb).size)}
+
+
 def run(lib, scene, precision, steps, start=0, **kw):
     """A handle of the scene (or scene batch) after substeps start .. start + steps - 1."""
     sim = scene.make(lib, precision, **kw)

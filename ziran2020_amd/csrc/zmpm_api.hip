@@ -100,6 +100,7 @@ struct zmpm_context {
     };
     std::vector<Graph> graphs;
     bool use_graphs = true;
+    hipError_t launch_err = hipSuccess; // a failed launch that returns its error (the cooperative k_cg)
     int p2g_split = 64;
     // workgroups of the grid-stride launches (results do not depend on them)
     int p2g_wgs = 2048, grid_wgs = 2048, prep_wgs = 2048;
@@ -260,6 +261,9 @@ zmpm_status allocate(zmpm_context* h, int n)
         HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cg<T, AccT>, CGW, 0));
         HIPCHK(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->desc.device));
         if (occ < 1) return fail(h, ZMPM_E_HIP, "damage solve kernel cannot be resident");
+        int coop = 0;
+        HIPCHK(h, hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, h->desc.device));
+        if (!coop) return fail(h, ZMPM_E_HIP, "the implicit damage solve needs cooperative launches");
         D.cg_wgs = std::max(1, std::min(CG_MAX_WGS, cus));
         HIPCHK(h, hipMemsetAsync(D.cg_bar, 0, 4 * sizeof(unsigned), h->stream));
     }
@@ -485,7 +489,13 @@ void launchStep(zmpm_context* h, hipEvent_t* ev)
     if (D.implicit && D.any_damage) {
         // AnisoFractureSimulation::solveDamageSystem (AnisoFractureSimulation.h:435-612)
         k_dtilde<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
-        k_cg<T, AccT><<<D.cg_wgs, CGW, 0, st>>>(D);
+        // a cooperative launch: the runtime starts it only with every workgroup
+        // resident, which its grid barriers need (work of other streams or
+        // processes cannot leave some of them waiting to start)
+        void* args[] = { &D };
+        const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_cg<T, AccT>), dim3(D.cg_wgs),
+                                                        dim3(CGW), args, 0, st);
+        if (e != hipSuccess) h->launch_err = e;
     }
     if (ev) (void)hipEventRecord(ev[4], st);
     if (D.flip)
@@ -1188,7 +1198,10 @@ static zmpm_status runSteps(zmpm_handle h, int32_t steps, const double* dt, cons
     if (hipMemcpyAsync(clock, h->clock_host.data(), 2 * ns * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess)
         return fail(h, ZMPM_E_HIP, "clock upload");
     int32_t s = 0;
-    if (h->use_graphs && !h->timing && steps >= 2) {
+    // substeps with the implicit damage solve run eagerly: its cooperative
+    // launch is not captured into graphs
+    const bool coop = h->fp32 ? (h->Df.implicit && h->Df.any_damage) : (h->Dd.implicit && h->Dd.any_damage);
+    if (h->use_graphs && !h->timing && steps >= 2 && !coop) {
         hipGraphExec_t exec = nullptr;
         zmpm_status st = graphFor(h, &exec);
         if (st != ZMPM_OK) return st;
@@ -1204,6 +1217,8 @@ static zmpm_status runSteps(zmpm_handle h, int32_t steps, const double* dt, cons
     }
     h->stepped = true;
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = h->launch_err;
+    h->launch_err = hipSuccess;
     if (e != hipSuccess) return fail(h, ZMPM_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     if (h->timing) {
         if (hipStreamSynchronize(h->stream) != hipSuccess) return fail(h, ZMPM_E_HIP, "stream sync");
