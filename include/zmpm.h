@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define ZMPM_ABI_VERSION 7
+#define ZMPM_ABI_VERSION 8
 
 /* Error model.  Replaces ZIRAN_ASSERT -> std::runtime_error
  * (Lib/Ziran/CS/Util/Debug.h:19-37) and the CFL RestartException
@@ -183,7 +183,23 @@ typedef struct {
     double min_jp, max_jp;       /* SnowPlasticity::min_Jp, max_Jp (0.6, 20)               */
     double fiber_stiffness[3];   /* QRAnisotropic::k_fiber_0..2 (mu each after setParameters,
                                     times scaleFiberStiffness's factor)                   */
+    int32_t fj_mixed;        /* zmpm_fj_phase: 0 = the model's own force helper; SOLID /
+                                FLUID = FJMixedMpmForceHelper<model, QuadraticVolumePenalty>
+                                (Lib/MPM/Force/FJMixedMpmForceHelper.h:16-278) as
+                                addFJMixedMpmForce adds it (MpmParticleHandleBase.cpp:
+                                811-834): fixed-corotated or neo-Hookean F model, a
+                                per-particle J, no phase field; a FLUID phase zeroes
+                                the F model's mu and lambda                              */
+    double j_lambda;         /* QuadraticVolumePenalty::lambda of the J model
+                                (psi = lambda/2 (J - 1)^2)                               */
 } zmpm_material;
+
+/* Material phase of an FJ-mixed material (MATERIAL_PHASE_ENUM). */
+typedef enum {
+    ZMPM_FJ_NONE = 0,
+    ZMPM_FJ_SOLID = 1,
+    ZMPM_FJ_FLUID = 2
+} zmpm_fj_phase;
 
 /* Analytic collision object (AnalyticCollisionObject, CollisionObject.h:20-165):
  * a level set in material space X moved rigidly to world space as
@@ -239,6 +255,8 @@ typedef struct {
     double* max_dtilde; /* [n] AnisotropicPhaseField::maxDTilde, the history maximum of the
                            crack driving force the implicit damage solve reads
                            (AnisoFractureSimulation.h:284-337), may be NULL (0 on upload)  */
+    double* J;          /* [n] the "J" attribute of FJ-mixed particles
+                           (FJMixedMpmForceHelper.h:36), may be NULL (1 on upload)        */
 } zmpm_particles;
 
 typedef struct zmpm_context* zmpm_handle;
@@ -257,6 +275,12 @@ const char* zmpm_last_error(zmpm_handle h);
  * model attribute added by addFBasedMpmForce* / addJBasedMpmForce
  * (MpmParticleHandleBase.cpp:175-222,796-809). */
 zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material* mats);
+
+/* FJMixedMpmForceHelper::state (FJMixedMpmForceHelper.h:38-40), the helper's
+ * static SOLVE_F / SOLVE_J switch, for the FJ-mixed particles of this handle:
+ * 0 = SOLVE_F (default; updateState adds the F model's V0 tau(F)), 1 = SOLVE_J
+ * (adds V0 lambda_J (J - 1) J I, :98-121).  Takes effect at the next substep. */
+zmpm_status zmpm_set_fj_state(zmpm_handle h, int32_t state);
 
 /* Collision objects (<= 16).  Replaces MpmSimulationBase::addCollisionObject
  * (MpmSimulationBase.cpp:87-91); evaluation order = insertion order, as in

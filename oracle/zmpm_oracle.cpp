@@ -1199,6 +1199,10 @@ struct zmpm_context {
     // particle state (index order, never reordered)
     int64_t n = 0;
     std::vector<double> x, v, C, F, m, V0, d, g, lap, ys, dtil;
+    // FJ-mixed particles: the "J" attribute and the J model's lambda (per
+    // particle, as the reference keeps one QuadraticVolumePenalty per particle)
+    std::vector<double> J, jl;
+    int fj_state = 0; // FJMixedMpmForceHelper::state: 0 SOLVE_F, 1 SOLVE_J
     std::vector<int32_t> mat;
     int cg_iterations = -1; // last implicit damage solve
     // binning of the last substep
@@ -1601,12 +1605,21 @@ int gridToParticlesPass(zmpm_context* h, double dt, double r, double flip, bool 
             Fp[0] = (1 + dt * (gv[0] + gv[4] + gv[8])) * Fp[0];
             continue;
         }
+        const zmpm_material& mt = matOf(h, i);
         M3 A;
         for (int a = 0; a < 3; a++)
             for (int c = 0; c < 3; c++) A.a[a][c] = (a == c ? 1.0 : 0.0) + dt * gv[3 * a + c];
-        M3 Fn = mmul(A, mload(Fp));
+        M3 Fn = mload(Fp);
+        if (mt.fj_mixed == ZMPM_FJ_SOLID) {
+            // FJMixedMpmForceHelper::evolveStrain (FJMixedMpmForceHelper.h:168-187): F and
+            // J = (1 + dt tr gradV) J, then the J model takes the F model's lambda
+            Fn = mmul(A, Fn);
+            h->J[i] = (1 + dt * (gv[0] + gv[4] + gv[8])) * h->J[i];
+            h->jl[i] = mt.lambda;
+        }
+        else if (mt.fj_mixed != ZMPM_FJ_FLUID) // a fluid phase keeps F, J and lambda_J
+            Fn = mmul(A, Fn);
         // applyPlasticity (MpmSimulationBase.cpp:1517-1529)
-        const zmpm_material& mt = matOf(h, i);
         if (mt.plasticity == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY) vonMisesProject(mt.mu, mt.hardening, h->ys[i], Fn);
         if (mt.plasticity == ZMPM_PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY) druckerPragerProject(mt, h->ys[i], Fn);
         if (mt.plasticity == ZMPM_PLASTICITY_SNOW) snowProject(mt, h->ys[i], Fn);
@@ -1782,8 +1795,24 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         const zmpm_material& mt = matOf(h, i);
-        // stress linear in (mu, lambda): a snow particle's hardened moduli scale it
-        M3 tau = vtauOf(mt, &h->F[9 * i], h->V0[i] * snowScale(mt, h->ys[i]), h->g[i]);
+        M3 tau;
+        if (mt.fj_mixed && h->fj_state == 1) {
+            // FJMixedMpmForceHelper::updateState, SOLVE_J (FJMixedMpmForceHelper.h:106-121):
+            // vtau += V0 dpsi/dJ J I, QuadraticVolumePenalty dpsi/dJ = lambda (J - 1)
+            const double dpsi_dJ = h->jl[i] * (h->J[i] - 1);
+            tau = mzero();
+            tau.a[0][0] = tau.a[1][1] = tau.a[2][2] = h->V0[i] * dpsi_dJ * h->J[i];
+        }
+        else if (mt.fj_mixed) {
+            // SOLVE_F (:98-105): the F model's kirchhoff; addFJMixedMpmForce zeroes
+            // a fluid's mu and lambda (MpmParticleHandleBase.cpp:818-821)
+            zmpm_material fm = mt;
+            if (mt.fj_mixed == ZMPM_FJ_FLUID) fm.mu = fm.lambda = 0;
+            tau = vtauOf(fm, &h->F[9 * i], h->V0[i], h->g[i]);
+        }
+        else
+            // stress linear in (mu, lambda): a snow particle's hardened moduli scale it
+            tau = vtauOf(mt, &h->F[9 * i], h->V0[i] * snowScale(mt, h->ys[i]), h->g[i]);
         mstore(tau, &vtau[9 * i]);
     }
 
@@ -2011,6 +2040,15 @@ zmpm_status zmpm_destroy(zmpm_handle h)
 
 const char* zmpm_last_error(zmpm_handle h) { return h ? h->err.c_str() : "null handle"; }
 
+zmpm_status zmpm_set_fj_state(zmpm_handle h, int32_t state)
+{
+    if (!h) return ZMPM_E_ARG;
+    if (state != 0 && state != 1) return fail(h, ZMPM_E_ARG, "FJ state is 0 (SOLVE_F) or 1 (SOLVE_J)");
+    h->fj_state = state;
+    for (zmpm_context* c : h->subs) c->fj_state = state;
+    return ZMPM_OK;
+}
+
 zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material* mats)
 {
     if (h && !h->subs.empty()) {
@@ -2031,6 +2069,12 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
             return fail(h, ZMPM_E_ARG, "plasticity needs a model with mu");
         if (m.plasticity == ZMPM_PLASTICITY_SNOW && m.model != ZMPM_MODEL_FIXED_COROTATED && m.model != ZMPM_MODEL_NEO_HOOKEAN)
             return fail(h, ZMPM_E_ARG, "snow plasticity needs a stress linear in mu and lambda (fixed corotated, neo-Hookean)");
+        if (m.fj_mixed < ZMPM_FJ_NONE || m.fj_mixed > ZMPM_FJ_FLUID) return fail(h, ZMPM_E_ARG, "unknown FJ-mixed phase");
+        if (m.fj_mixed
+            && ((m.model != ZMPM_MODEL_FIXED_COROTATED && m.model != ZMPM_MODEL_NEO_HOOKEAN) || m.has_damage
+                || m.plasticity == ZMPM_PLASTICITY_SNOW))
+            return fail(h, ZMPM_E_ARG,
+                        "an FJ-mixed material has a fixed-corotated or neo-Hookean F model, no phase field and no snow plasticity");
     }
     for (int32_t m : h->mat)
         if (m >= count) return fail(h, ZMPM_E_ARG, "material table smaller than the uploaded particles' material indices");
@@ -2106,6 +2150,11 @@ zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
     h->lap.assign(n, 0.0);
     h->dtil.assign(n, 0.0);
     if (p->max_dtilde) h->dtil.assign(p->max_dtilde, p->max_dtilde + n);
+    // addFJMixedMpmForce: J = 1 and the J model's own lambda
+    h->J.assign(n, 1.0);
+    if (p->J) h->J.assign(p->J, p->J + n);
+    h->jl.resize(n);
+    for (int64_t i = 0; i < n; i++) h->jl[i] = h->mats[(size_t)h->mat[i]].j_lambda;
     if (p->d) h->d.assign(p->d, p->d + n);
     if (p->g) h->g.assign(p->g, p->g + n);
     h->latched = ZMPM_OK;
@@ -2142,6 +2191,7 @@ zmpm_status batchDownload(zmpm_handle h, zmpm_particles* p)
         if (p->laplacian) p->laplacian[i] = c->lap[k];
         if (p->plastic_state) p->plastic_state[i] = c->ys[k];
         if (p->max_dtilde) p->max_dtilde[i] = c->dtil[k];
+        if (p->J) p->J[i] = c->J[k];
     }
     return first;
 }
@@ -2229,7 +2279,7 @@ zmpm_status zmpm_upload_batch(zmpm_handle h, const zmpm_particles* p, const int3
         for (int64_t i = 0; i < p->n; i++)
             if (scene[i] == (int32_t)s) idx.push_back(i);
         const int64_t k = (int64_t)idx.size();
-        std::vector<double> x(3 * k), v(3 * k), C(9 * k), F(9 * k), m(k), V0(k), d(k), g(k), ys(k), dt(k);
+        std::vector<double> x(3 * k), v(3 * k), C(9 * k), F(9 * k), m(k), V0(k), d(k), g(k), ys(k), dt(k), Jv(k, 1.0);
         std::vector<int32_t> mat(k);
         for (int64_t j = 0; j < k; j++) {
             const int64_t i = idx[(size_t)j];
@@ -2248,6 +2298,7 @@ zmpm_status zmpm_upload_batch(zmpm_handle h, const zmpm_particles* p, const int3
             if (p->g) g[j] = p->g[i];
             if (p->plastic_state) ys[j] = p->plastic_state[i];
             if (p->max_dtilde) dt[j] = p->max_dtilde[i];
+            if (p->J) Jv[j] = p->J[i];
         }
         zmpm_particles q {};
         q.n = k;
@@ -2262,6 +2313,7 @@ zmpm_status zmpm_upload_batch(zmpm_handle h, const zmpm_particles* p, const int3
         q.g = p->g ? g.data() : nullptr;
         q.plastic_state = p->plastic_state ? ys.data() : nullptr;
         q.max_dtilde = p->max_dtilde ? dt.data() : nullptr;
+        q.J = p->J ? Jv.data() : nullptr;
         const zmpm_status st = zmpm_upload(h->subs[s], &q);
         if (st != ZMPM_OK) return fail(h, st, h->subs[s]->err);
     }
@@ -2297,6 +2349,7 @@ zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
     if (p->laplacian) std::copy(h->lap.begin(), h->lap.end(), p->laplacian);
     if (p->plastic_state) std::copy(h->ys.begin(), h->ys.end(), p->plastic_state);
     if (p->max_dtilde) std::copy(h->dtil.begin(), h->dtil.end(), p->max_dtilde);
+    if (p->J) std::copy(h->J.begin(), h->J.end(), p->J);
     (void)n;
     zmpm_status s = (zmpm_status)h->latched;
     return s;

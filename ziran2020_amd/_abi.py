@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-ZMPM_ABI_VERSION = 7
+ZMPM_ABI_VERSION = 8
 
 # zmpm_status
 OK, E_ARG, E_HIP, E_DOMAIN, E_CFL, E_NAN, E_OVERFLOW, E_NOMEM = range(8)
@@ -29,6 +29,8 @@ STICKY, SLIP, SEPARATE, GHOST = 1, 2, 3, 4
 APIC_BLEND_RPIC, FLIP_BLEND_PIC = 0, 1
 PLASTICITY_NONE, PLASTICITY_VON_MISES_STVK_HENCKY, PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY, PLASTICITY_SNOW = 0, 1, 2, 3
 SHAPE_HALF_SPACE, SHAPE_BOX, SHAPE_SPHERE, SHAPE_CAPPED_CYLINDER = 0, 1, 2, 3
+FJ_NONE, FJ_SOLID, FJ_FLUID = 0, 1, 2
+SOLVE_F, SOLVE_J = 0, 1
 
 D3 = C.c_double * 3
 
@@ -70,6 +72,7 @@ class Material(C.Structure):
         ("critical_compression", C.c_double), ("critical_stretch", C.c_double),
         ("min_jp", C.c_double), ("max_jp", C.c_double),
         ("fiber_stiffness", D3),
+        ("fj_mixed", C.c_int32), ("j_lambda", C.c_double),
     ]
 
 
@@ -97,6 +100,7 @@ class Particles(C.Structure):
         ("x", _DP), ("v", _DP), ("C", _DP), ("F", _DP),
         ("m", _DP), ("V0", _DP), ("material", _IP),
         ("d", _DP), ("g", _DP), ("laplacian", _DP), ("plastic_state", _DP), ("max_dtilde", _DP),
+        ("J", _DP),
     ]
 
 
@@ -118,13 +122,16 @@ def make_material(model=MODEL_FIXED_COROTATED, E=1e4, nu=0.3, *, has_damage=Fals
                   residual_phase=0.001, l0=0.0, sigma_crit=1.0, eta=0.1, zeta=1.0,
                   plasticity=PLASTICITY_NONE, yield_stress=0.0, hardening=0.0, friction_angle=30.0, cohesion=0.0,
                   volume_correction=True, critical_compression=2e-2, critical_stretch=7.5e-3, min_jp=0.6,
-                  max_jp=20.0, isotropic=False, fiber_scale=(1.0, 1.0, 1.0)) -> Material:
+                  max_jp=20.0, isotropic=False, fiber_scale=(1.0, 1.0, 1.0), fj_mixed=FJ_NONE,
+                  j_E=None, j_nu=None) -> Material:
     """Material from Young's modulus and Poisson ratio with the reference's Lame
     conversion (CorotatedElasticity.cpp:15-20, QRStableNeoHookean.h:61-65).  The snow
     plasticity defaults are SnowPlasticity's constructor defaults (PlasticityApplier.h:103;
     its psi is `hardening`).  QRAnisotropic's fibre stiffnesses are mu times
     `fiber_scale` (setParameters then scaleFiberStiffness, QRAnisotropic.h:57-64,
-    96-104, which leaves an isotropic model's fibres at mu)."""
+    96-104, which leaves an isotropic model's fibres at mu).  fj_mixed (FJ_SOLID /
+    FJ_FLUID) makes it FJMixedMpmForceHelper<model, QuadraticVolumePenalty(j_E, j_nu)>
+    (addFJMixedMpmForce, MpmParticleHandleBase.cpp:811-834); j_E, j_nu default to E, nu."""
     m = Material()
     m.model = model
     m.has_damage = 1 if has_damage else 0
@@ -144,6 +151,10 @@ def make_material(model=MODEL_FIXED_COROTATED, E=1e4, nu=0.3, *, has_damage=Fals
     m.isotropic = 1 if isotropic else 0
     for c in range(3):
         m.fiber_stiffness[c] = m.mu * (1.0 if isotropic else fiber_scale[c])
+    m.fj_mixed = fj_mixed
+    jE = E if j_E is None else j_E
+    jnu = nu if j_nu is None else j_nu
+    m.j_lambda = jE * jnu / ((1.0 + jnu) * (1.0 - 2.0 * jnu))  # QuadraticVolumePenalty::setMaterialParameters
     return m
 
 
@@ -240,6 +251,7 @@ class ZmpmLib:
         "zmpm_get_grid", "zmpm_set_timing", "zmpm_get_timings", "zmpm_calculate_dt", "zmpm_get_launch_config",
         "zmpm_set_launch_config", "zmpm_autotune", "zmpm_create_batch", "zmpm_batch_origin",
         "zmpm_add_scene_collider", "zmpm_upload_batch", "zmpm_run_batch", "zmpm_calculate_dt_noalloc",
+        "zmpm_set_fj_state",
     )
 
     def __init__(self, path: str):
@@ -283,6 +295,7 @@ class ZmpmLib:
         L.zmpm_upload_batch.argtypes = [C.c_void_p, C.POINTER(Particles), _IP]
         L.zmpm_run_batch.argtypes = [C.c_void_p, C.c_int32, _DP, _DP]
         L.zmpm_calculate_dt_noalloc.argtypes = [C.c_void_p, C.c_double, C.c_int32, _DP]
+        L.zmpm_set_fj_state.argtypes = [C.c_void_p, C.c_int32]
         if L.zmpm_abi_version() != ZMPM_ABI_VERSION:
             raise RuntimeError(f"{path}: ABI version {L.zmpm_abi_version()} != {ZMPM_ABI_VERSION}")
 
@@ -345,13 +358,17 @@ class Simulation:
         arr = (Material * len(mats))(*mats)
         self._check(self.L.zmpm_set_materials(self.h, len(mats), arr), "zmpm_set_materials")
 
+    def set_fj_state(self, state):
+        """FJMixedMpmForceHelper::state: SOLVE_F (0) or SOLVE_J (1)."""
+        self._check(self.L.zmpm_set_fj_state(self.h, state), "zmpm_set_fj_state")
+
     def add_collider(self, c: Collider) -> int:
         i = C.c_int32()
         self._check(self.L.zmpm_add_collider(self.h, C.byref(c), C.byref(i)), "zmpm_add_collider")
         return i.value
 
     def upload(self, x, v, Cm, F, m, V0, material, d=None, g=None, plastic_state=None, max_dtilde=None,
-               scene=None):
+               scene=None, J=None):
         n = int(x.shape[0])
         arrs = dict(
             x=np.ascontiguousarray(x, np.float64).reshape(n, 3),
@@ -365,10 +382,11 @@ class Simulation:
             g=None if g is None else np.ascontiguousarray(g, np.float64),
             plastic_state=None if plastic_state is None else np.ascontiguousarray(plastic_state, np.float64),
             max_dtilde=None if max_dtilde is None else np.ascontiguousarray(max_dtilde, np.float64),
+            J=None if J is None else np.ascontiguousarray(J, np.float64),
         )
         p = Particles()
         p.n = n
-        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "plastic_state", "max_dtilde"):
+        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "plastic_state", "max_dtilde", "J"):
             setattr(p, k, _ptr(arrs[k]))
         p.material = _ptr(arrs["material"], C.c_int32)
         p.laplacian = None
@@ -385,10 +403,11 @@ class Simulation:
         n = self.n
         out = dict(x=np.zeros((n, 3)), v=np.zeros((n, 3)), C=np.zeros((n, 9)), F=np.zeros((n, 9)),
                    m=np.zeros(n), V0=np.zeros(n), material=np.zeros(n, np.int32), d=np.zeros(n),
-                   g=np.zeros(n), laplacian=np.zeros(n), plastic_state=np.zeros(n), max_dtilde=np.zeros(n))
+                   g=np.zeros(n), laplacian=np.zeros(n), plastic_state=np.zeros(n), max_dtilde=np.zeros(n),
+                   J=np.zeros(n))
         p = Particles()
         p.n = n
-        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "laplacian", "plastic_state", "max_dtilde"):
+        for k in ("x", "v", "C", "F", "m", "V0", "d", "g", "laplacian", "plastic_state", "max_dtilde", "J"):
             setattr(p, k, _ptr(out[k]))
         p.material = _ptr(out["material"], C.c_int32)
         st = self.L.zmpm_download(self.h, C.byref(p))

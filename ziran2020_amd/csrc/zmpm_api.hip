@@ -100,6 +100,7 @@ struct zmpm_context {
     };
     std::vector<Graph> graphs;
     bool use_graphs = true;
+    int fj_state = 0;       // FJMixedMpmForceHelper::state: 0 SOLVE_F, 1 SOLVE_J
     hipError_t launch_err = hipSuccess; // a failed launch that returns its error (the cooperative k_cg)
     int p2g_split = 64;
     // workgroups of the grid-stride launches (results do not depend on them)
@@ -347,7 +348,12 @@ void refreshTables(zmpm_context* h)
     D.region_shift = rs;
     D.any_damage = h->any_damage ? 1 : 0;
     D.any_plastic = 0;
-    for (const auto& m : h->mats) D.any_plastic |= m.plasticity != ZMPM_PLASTICITY_NONE ? 1 : 0;
+    D.any_fj = 0;
+    for (const auto& m : h->mats) {
+        D.any_plastic |= m.plasticity != ZMPM_PLASTICITY_NONE ? 1 : 0;
+        D.any_fj |= m.fj_mixed != ZMPM_FJ_NONE ? 1 : 0;
+    }
+    D.fj_solve_j = h->fj_state;
     D.p2g_split = h->p2g_split;
     D.flip = h->desc.transfer_scheme == ZMPM_FLIP_BLEND_PIC ? 1 : 0;
     D.mls = h->desc.mls_mpm ? 1 : 0;
@@ -406,6 +412,11 @@ zmpm_status uploadMaterials(zmpm_context* h)
         d.max_jp = m.max_jp;
         d.snow_lam_lo = (float)(d.snow_lo * d.snow_lo - 1.0);
         d.snow_lam_hi = (float)(d.snow_hi * d.snow_hi - 1.0);
+        // addFJMixedMpmForce (MpmParticleHandleBase.cpp:811-834): a fluid phase
+        // zeroes the F model's mu and lambda
+        d.fj = m.fj_mixed;
+        d.jl = (T)m.j_lambda;
+        if (m.fj_mixed == ZMPM_FJ_FLUID) d.mu = d.lambda = T(0);
     }
     HIPCHK(h, hipMemcpyAsync(h->mats_dev, md.data(), sizeof(MatDev<T>) * kMaxMaterials, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -605,6 +616,9 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p, const int32_t* sce
                                                                   : T(0);
         buf[PYS * cap + i] = p->plastic_state ? (T)p->plastic_state[i] : ys0;
         buf[PDT * cap + i] = p->max_dtilde ? (T)p->max_dtilde[i] : T(0);
+        // J (addFJMixedMpmForce starts it at 1); the FP32 form keeps J - 1
+        const double J = p->J ? p->J[i] : 1.0;
+        buf[PJ * cap + i] = F32 ? (T)(J - 1.0) : (T)J;
     }
     std::vector<int> ids(cap, 0), mat(cap, 0);
     for (int i = 0; i < n; i++) {
@@ -625,7 +639,7 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p, const int32_t* sce
     HIPCHK(h, hipMemsetAsync(D.page_count, 0, D.num_pages * sizeof(int), h->stream));
     if (n > 0) {
         // stress of the uploaded state and the first substep's fixed-point bounds
-        k_stress_init<T, AccT><<<D.max_slots, WG, 0, h->stream>>>(D, 1);
+        k_stress_init<T, AccT><<<D.max_slots, WG, 0, h->stream>>>(D, 1, 1);
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipGetLastError());
@@ -638,7 +652,7 @@ zmpm_status restress(zmpm_context* h)
 {
     Dev<T, AccT>& D = devOf<T, AccT>(h);
     HIPCHK(h, hipMemsetAsync(D.maxbits, 0, h->scenes.size() * NSC * BOUND_SLOTS * sizeof(unsigned int), h->stream));
-    k_stress_init<T, AccT><<<D.max_slots, WG, 0, h->stream>>>(D, 0);
+    k_stress_init<T, AccT><<<D.max_slots, WG, 0, h->stream>>>(D, 0, h->stepped ? 0 : 1);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipGetLastError());
     return ZMPM_OK;
@@ -688,6 +702,7 @@ zmpm_status downloadT(zmpm_context* h, zmpm_particles* p)
         if (p->laplacian) p->laplacian[i] = (double)buf[PLAP * cap + s];
         if (p->plastic_state) p->plastic_state[i] = (double)buf[PYS * cap + s];
         if (p->max_dtilde) p->max_dtilde[i] = h->desc.implicit_damage ? (double)buf[PDT * cap + s] : 0.0;
+        if (p->J) p->J[i] = (double)buf[PJ * cap + s] + (std::is_same<T, float>::value ? 1.0 : 0.0);
     }
     return ZMPM_OK;
 }
@@ -1033,6 +1048,13 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
         if (mats[i].plasticity == ZMPM_PLASTICITY_SNOW && mats[i].model != ZMPM_MODEL_FIXED_COROTATED
             && mats[i].model != ZMPM_MODEL_NEO_HOOKEAN)
             return fail(h, ZMPM_E_ARG, "snow plasticity needs a stress linear in mu and lambda (fixed corotated, neo-Hookean)");
+        if (mats[i].fj_mixed < ZMPM_FJ_NONE || mats[i].fj_mixed > ZMPM_FJ_FLUID)
+            return fail(h, ZMPM_E_ARG, "unknown FJ-mixed phase");
+        if (mats[i].fj_mixed
+            && ((mats[i].model != ZMPM_MODEL_FIXED_COROTATED && mats[i].model != ZMPM_MODEL_NEO_HOOKEAN)
+                || mats[i].has_damage || mats[i].plasticity == ZMPM_PLASTICITY_SNOW))
+            return fail(h, ZMPM_E_ARG,
+                        "an FJ-mixed material has a fixed-corotated or neo-Hookean F model, no phase field and no snow plasticity");
     }
     if (h->n > 0 && h->max_mat >= count)
         return fail(h, ZMPM_E_ARG, "material table smaller than the uploaded particles' material indices");
@@ -1049,6 +1071,23 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
         refreshTables<double, double>(h);
     // particles already on the device: their stored stress and the next P2G's
     // fixed-point bounds follow the new table (the binning stays)
+    if (h->n > 0) return h->fp32 ? restress<float, long long>(h) : restress<double, double>(h);
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_set_fj_state(zmpm_handle h, int32_t state)
+{
+    if (!h) return ZMPM_E_ARG;
+    if (state != 0 && state != 1) return fail(h, ZMPM_E_ARG, "FJ state is 0 (SOLVE_F) or 1 (SOLVE_J)");
+    if (state == h->fj_state) return ZMPM_OK;
+    (void)hipSetDevice(h->desc.device);
+    dropGraphs(h);
+    h->fj_state = state;
+    if (h->fp32)
+        refreshTables<float, long long>(h);
+    else
+        refreshTables<double, double>(h);
+    // the stored stress of the particles on the device follows the new state
     if (h->n > 0) return h->fp32 ? restress<float, long long>(h) : restress<double, double>(h);
     return ZMPM_OK;
 }

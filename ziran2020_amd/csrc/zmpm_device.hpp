@@ -851,6 +851,32 @@ __device__ __forceinline__ void forceBounds(float* b, const T* S, T d, T odx, bo
     }
 }
 
+// FJMixedMpmForceHelper::updateState under SOLVE_J (FJMixedMpmForceHelper.h:
+// 106-121): V0 dpsi/dJ J I with QuadraticVolumePenalty's dpsi/dJ = lambda_J (J - 1)
+// (QuadraticVolumePenalty.h).  lambda_J is the J model's own until the first
+// evolveStrain, which sets it to the F model's lambda (:186) -- except for fluid
+// particles, whose evolveStrain returns before that copy (FJ-mixed materials
+// have no snow hardening, so the F model's lambda is the table's).  jm: J (FP64
+// form) or J - 1 (FP32 form).
+template <class T>
+__device__ __forceinline__ void fjSolveJStress(const MatDev<T>& mt, bool first, T V0, T jm, T* S)
+{
+    const bool own = first || mt.fj == ZMPM_FJ_FLUID;
+    const T lam = own ? mt.jl : mt.lambda;
+    T jm1, J;
+    if constexpr (sizeof(T) == 4) {
+        jm1 = jm;
+        J = T(1) + jm;
+    }
+    else {
+        jm1 = jm - T(1);
+        J = jm;
+    }
+    const T p = V0 * (lam * jm1) * J;
+    S[0] = S[3] = S[5] = p;
+    S[1] = S[2] = S[4] = T(0);
+}
+
 template <class T>
 __device__ __forceinline__ void particleBounds(float* b, T m, const T* v, const T* Cm, const T* S, T d, T dx, T odx,
                                                bool dmg)
@@ -866,19 +892,58 @@ __device__ __forceinline__ void particleBounds(float* b, T m, const T* v, const 
 // no particle pass scene < 0.
 __device__ __forceinline__ void writeBounds(unsigned int* maxbits, const float* b, int scene)
 {
-    const int lane = threadIdx.x & 63;
-    uint64_t left = __ballot(scene >= 0);
-    while (left) {
-        const int s0 = __shfl(scene, __ffsll((unsigned long long)left) - 1);
-        const bool mine = scene == s0;
-        left &= ~__ballot(mine);
+    __shared__ float wmax[NSC][WG / 64];
+    __shared__ int wscene[WG / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int slot = (int)(blockIdx.x % BOUND_SLOTS);
+    const uint64_t act = __ballot(scene >= 0);
+    const int s0 = act ? __shfl(scene, __ffsll((unsigned long long)act) - 1) : -1;
+    if (!__ballot(scene >= 0 && scene != s0)) {
+        // the wavefront's particles are in one scene (or none): its maxima go to
+        // the workgroup's fold below
 #pragma unroll
         for (int g = 0; g < NSC; g++) {
-            const float mx = waveMax(mine ? b[g] : 0.f);
-            if (lane == 0 && (mx > 0.f || !(mx == mx)))
-                atomicMax(&maxbits[(s0 * NSC + g) * BOUND_SLOTS + (int)(blockIdx.x % BOUND_SLOTS)], __float_as_uint(mx));
+            const float mx = waveMax(b[g]);
+            if (lane == 0) wmax[g][wv] = mx;
         }
+        if (lane == 0) wscene[wv] = s0;
     }
+    else {
+        // a wavefront across scenes' blocks: one pass per scene, its own atomics
+        uint64_t left = act;
+        while (left) {
+            const int sc = __shfl(scene, __ffsll((unsigned long long)left) - 1);
+            const bool mine = scene == sc;
+            left &= ~__ballot(mine);
+#pragma unroll
+            for (int g = 0; g < NSC; g++) {
+                const float mx = waveMax(mine ? b[g] : 0.f);
+                if (lane == 0 && (mx > 0.f || !(mx == mx)))
+                    atomicMax(&maxbits[(sc * NSC + g) * BOUND_SLOTS + slot], __float_as_uint(mx));
+            }
+        }
+        if (lane == 0) wscene[wv] = -1;
+    }
+    __syncthreads();
+    // one atomic per group for each run of wavefronts of one scene (one run in practice)
+    if (threadIdx.x < NSC) {
+        const int g = threadIdx.x;
+        int sc = -1;
+        float mx = 0.f;
+#pragma unroll
+        for (int w = 0; w < WG / 64; w++) {
+            const int ws = wscene[w];
+            if (ws < 0) continue;
+            if (sc >= 0 && ws != sc) {
+                if (mx > 0.f || !(mx == mx)) atomicMax(&maxbits[(sc * NSC + g) * BOUND_SLOTS + slot], __float_as_uint(mx));
+                mx = 0.f;
+            }
+            sc = ws;
+            mx = fmaxf(mx, wmax[g][w]);
+        }
+        if (sc >= 0 && (mx > 0.f || !(mx == mx))) atomicMax(&maxbits[(sc * NSC + g) * BOUND_SLOTS + slot], __float_as_uint(mx));
+    }
+    __syncthreads(); // wmax / wscene reusable by the next call
 }
 
 // The factor on a particle's stress from its material's plasticity: snow's
@@ -898,7 +963,7 @@ __device__ __forceinline__ T snowScale(const MatDev<T>& mt, T jp)
 // bin, also the binning of the first substep (grid of max_slots workgroups,
 // striding over particles).  Run again without bin after a material change.
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D, int bin)
+__global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D, int bin, int first)
 {
     const int64_t cap = D.cap;
     if (bin) openPageScan(D);
@@ -914,7 +979,10 @@ __global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D, int bin)
             Mat3<T> F = loadM(D.P, cap, PF, p);
             T S[6], v[3], Cm[9];
             const T jp = mt.plasticity == ZMPM_PLASTICITY_SNOW ? ldAt(D.P + PYS * cap, p) : T(1);
-            vtauSym(mt, F, ldAt(D.P + PVOL * cap, p) * snowScale(mt, jp), ldAt(D.P + PG * cap, p), S);
+            if (D.fj_solve_j && mt.fj)
+                fjSolveJStress(mt, first != 0, ldAt(D.P + PVOL * cap, p), ldAt(D.P + PJ * cap, p), S);
+            else
+                vtauSym(mt, F, ldAt(D.P + PVOL * cap, p) * snowScale(mt, jp), ldAt(D.P + PG * cap, p), S);
 #pragma unroll
             for (int i = 0; i < 6; i++) stAt(D.P + (PS + i) * cap, p, S[i]);
 #pragma unroll
@@ -933,10 +1001,11 @@ __global__ __launch_bounds__(WG) void k_stress_init(Dev<T, AccT> D, int bin)
     }
 }
 
-// Scene of dense block b (its region).
+// Scene of dense block b (its region); a handle of one scene needs no lookup.
 template <class T, class AccT>
 __device__ __forceinline__ int blockScene(const Dev<T, AccT>& D, int b)
 {
+    if (D.nscenes == 1) return 0;
     using B = Blk<T>;
     uint32_t Bx, By, Bz;
     blockOfPage<T>((uint32_t)D.block_page[b], Bx, By, Bz);
@@ -1031,8 +1100,12 @@ __device__ __forceinline__ void loadP2G(const Dev<T, AccT>& D, int s, bool dmg, 
 // int64 fixed point in LDS and HBM, so the result does not depend on scheduling.
 enum : int { P2G_STD = 0, P2G_MLS = 1, P2G_DAMP = 2 };
 
+// Waves per SIMD the P2G kernel is compiled for: the float form fits 4 (at most
+// 128 VGPRs), the double form 3.
+template <class T> struct P2GWaves { static constexpr int value = sizeof(T) == 4 ? 4 : 3; };
+
 template <class T, class AccT, int VAR = P2G_STD>
-__global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
+__global__ __launch_bounds__(WG, P2GWaves<T>::value) void k_p2g(Dev<T, AccT> D)
 {
     constexpr bool DAMP = VAR == P2G_DAMP, MLS = VAR == P2G_MLS;
     using B = Blk<T>;
@@ -1091,8 +1164,9 @@ __global__ __launch_bounds__(WG) void k_p2g(Dev<T, AccT> D)
         if (be <= bs) continue; // halo-only block
         for (int e = t; e < nq * TL::TN; e += WG) (&tile[0][0])[e] = AccT(0);
         // the block's scene: its dx and fixed-point scales
-        const SceneDev<T>& sd = D.sc[blockScene(D, b)];
-        const float* sc = D.scales + blockScene(D, b) * NSC;
+        const int bscene = blockScene(D, b);
+        const SceneDev<T>& sd = D.sc[bscene];
+        const float* sc = D.scales + bscene * NSC;
         const float scl[NGF] = { sc[SC_M], sc[SC_MV], sc[SC_MV], sc[SC_MV], sc[SC_F], sc[SC_F], sc[SC_F], sc[SC_PF], sc[SC_PW] };
         bool have = false;
         RawP2G<T> raw;
@@ -1857,6 +1931,7 @@ struct RawG2P {
     T F[9], m, V0, d, g;
     T v[3]; // FLIP_blend_PIC only
     T ys;   // plasticity state (when any material has plasticity)
+    T J;    // FJ-mixed J (FP32 form: J - 1), when any material is FJ-mixed
     int mid, id;
 };
 
@@ -1876,6 +1951,7 @@ __device__ __forceinline__ void loadG2P(const Dev<T, AccT>& D, int s, int p, Raw
     r.mid = ldAt(D.pmat, p);
     r.id = ldAt(D.spid, s);
     r.ys = D.any_plastic ? ldAt(P + PYS * cap, p) : T(0);
+    r.J = D.any_fj ? ldAt(P + PJ * cap, p) : T(0);
     if constexpr (FLIP)
 #pragma unroll
         for (int a = 0; a < 3; a++) r.v[a] = ldAt(P + (PV + a) * cap, p);
@@ -1970,13 +2046,25 @@ __device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     // F <- (I + dt grad v) F; J <- (1 + dt tr grad v) J.  FP32 form on H = F - I:
     // H <- H + G + G H with G = dt grad v; J - 1 <- (J - 1) + dt tr(grad v) J.
     const T dtr = dt * (gvF[0][0] + gvF[1][1] + gvF[2][2]);
+    // FJMixedMpmForceHelper::evolveStrain (FJMixedMpmForceHelper.h:168-187): F and
+    // J = (1 + dt tr grad v) J of a solid; a fluid's are left as they are
+    T jn = R.J;
+    if (D.any_fj) {
+        if (mt.fj == ZMPM_FJ_SOLID) {
+            if constexpr (sizeof(T) == 4)
+                jn = jn + dtr * (1.f + jn);
+            else
+                jn = (T(1) + dtr) * jn;
+        }
+        stAt(Q + PJ * cap, s, jn);
+    }
     if (mt.model == ZMPM_MODEL_EQUATION_OF_STATE) {
         if constexpr (sizeof(T) == 4)
             Fn.a[0][0] = Fn.a[0][0] + dtr * (1.f + Fn.a[0][0]);
         else
             Fn.a[0][0] = (T(1) + dtr) * Fn.a[0][0];
     }
-    else {
+    else if (!D.any_fj || mt.fj != ZMPM_FJ_FLUID) {
         const Mat3<T> F = Fn;
         if (dmg && phaseFieldModel(mt.model)) {
             if (D.implicit) {
@@ -2013,7 +2101,10 @@ __device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     }
     // stress of the new state for the next substep's P2G
     T S[6];
-    rare = rare || vtauSym<true>(mt, Fn, R.V0 * snowScale(mt, R.ys), g, S);
+    if (D.fj_solve_j && mt.fj)
+        fjSolveJStress(mt, false, R.V0, jn, S);
+    else
+        rare = rare || vtauSym<true>(mt, Fn, R.V0 * snowScale(mt, R.ys), g, S);
     if (!rare) {
 #pragma unroll
         for (int i = 0; i < 6; i++) stAt(Q + (PS + i) * cap, s, S[i]);
@@ -2082,7 +2173,6 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D)
     }
     A.lap = T(0);
     const int scene = live ? sceneOf(R.mid) : 0;
-    const SceneDev<T>& sd = D.sc[scene];
     int seg_lo = lo;
     while (seg_lo < hi) {
         const int b = D.sorted_cell[seg_lo] / NB;
@@ -2094,13 +2184,21 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D)
         stageTile(D, b, gt, nfield, dmg);
         __syncthreads();
         const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
-        if (live && s >= seg_lo && s < seg_hi) g2pGather<T, AccT, FLIP>(D, sd, gt, R.x, dmg, A);
+        // the segment's particles are in its block's scene (a uniform index)
+        if (live && s >= seg_lo && s < seg_hi) g2pGather<T, AccT, FLIP>(D, D.sc[blockScene(D, b)], gt, R.x, dmg, A);
         seg_lo = seg_hi;
     }
     float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };
     PosT<T> xnew[3] = { 0, 0, 0 };
     bool rare = false;
-    if (live) rare = g2pFinish<T, AccT, FLIP>(D, s, R, sd, A, D.mats[matOf(R.mid)], bnd, xnew);
+    // a wavefront of one scene (all but those across two scenes' blocks) reads
+    // the scene's table with scalar loads through a wave-uniform index
+    const int s_first = __builtin_amdgcn_readfirstlane(scene);
+    if (!__ballot(live && scene != s_first)) {
+        if (live) rare = g2pFinish<T, AccT, FLIP>(D, s, R, D.sc[s_first], A, D.mats[matOf(R.mid)], bnd, xnew);
+    }
+    else if (live)
+        rare = g2pFinish<T, AccT, FLIP>(D, s, R, D.sc[scene], A, D.mats[matOf(R.mid)], bnd, xnew);
     // deferred particles to the rare-path queue, one atomic per wavefront
     const uint64_t rm = __ballot(rare);
     if (rm) {
@@ -2335,7 +2433,10 @@ __global__ __launch_bounds__(WG) void k_rare(Dev<T, AccT> D)
                 }
             }
             T S[6];
-            vtauSym<false>(mt, Fn, ldAt(D.Q + PVOL * cap, s) * snowScale(mt, ys), g, S);
+            if (D.fj_solve_j && mt.fj)
+                fjSolveJStress(mt, false, ldAt(D.Q + PVOL * cap, s), ldAt(D.Q + PJ * cap, s), S);
+            else
+                vtauSym<false>(mt, Fn, ldAt(D.Q + PVOL * cap, s) * snowScale(mt, ys), g, S);
 #pragma unroll
             for (int k = 0; k < 6; k++) stAt(D.Q + (PS + k) * cap, s, S[k]);
             if constexpr (std::is_same<AccT, long long>::value) forceBounds(bnd, S, dmg ? d : T(0), sd.odx, D.any_damage != 0);
@@ -2396,7 +2497,7 @@ __global__ __launch_bounds__(WG) void k_damp_g2p(Dev<T, AccT> D)
         stageTile(D, b, gt, 3, false);
         __syncthreads();
         const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
-        if (live && s >= seg_lo && s < seg_hi) g2pGather<T, AccT, false>(D, sd, gt, x, false, A, cell);
+        if (live && s >= seg_lo && s < seg_hi) g2pGather<T, AccT, false>(D, D.sc[blockScene(D, b)], gt, x, false, A, cell);
         seg_lo = seg_hi;
     }
     float bnd[NSC] = { 0.f, 0.f, 0.f, 0.f, 0.f };

@@ -45,7 +45,8 @@ enum : int {
     PS = 29,   // V0 * Kirchhoff stress of the current F, symmetric: xx xy xz yy yz zz
     PYS = 35,  // plasticity state: von Mises yield stress / Drucker-Prager log Jp / snow Jp
     PDT = 36,  // implicit damage: history maximum of the crack driving force D~
-    NPF = 37
+    PJ = 37,   // FJ-mixed particles: J (FP32 form: J - 1)
+    NPF = 38
 };
 
 // FP32-form position: X = round(x / dx * 2^POS_BITS) as int32.  Domains are at
@@ -124,6 +125,10 @@ struct MatDev {
     // eigenvalues of F^T F - I that the float G2P gate tests (hardening is psi)
     double snow_lo, snow_hi, min_jp, max_jp;
     float snow_lam_lo, snow_lam_hi;
+    // FJMixedMpmForceHelper: ZMPM_FJ_SOLID / ZMPM_FJ_FLUID (a fluid's mu and
+    // lambda above are 0) and the J model's own lambda
+    int fj;
+    T jl;
 };
 
 // Models that carry the phase field: updateDamage dispatches to these three only
@@ -243,6 +248,8 @@ struct Dev {
     int num_cols;
     int any_damage;
     int any_plastic;
+    int any_fj;        // some material is FJ-mixed: G2P carries J
+    int fj_solve_j;    // FJMixedMpmForceHelper::state == SOLVE_J
     // launch parameters
     int p2g_split; // most lanes sharing one stencil column of a cell in P2G (>= 1)
     // transfer scheme (MpmSimulationBase::transfer_scheme, mls_mpm, flip_pic_ratio)

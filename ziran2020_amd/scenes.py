@@ -39,6 +39,7 @@ class Scene:
     info: dict = field(default_factory=dict)
     options: dict = field(default_factory=dict)  # zmpm_scene_desc options of the workload (e.g. implicit_damage)
     t0: float = 0.0  # simulation time of the first substep
+    J: np.ndarray = None  # FJ-mixed particles' J (None: 1)
 
     def __post_init__(self):
         n = self.x.shape[0]
@@ -65,7 +66,7 @@ class Scene:
         sim.set_materials(self.materials)
         for c in self.colliders:
             sim.add_collider(c)
-        sim.upload(self.x, self.v, self.C, self.F, self.m, self.V0, self.material)
+        sim.upload(self.x, self.v, self.C, self.F, self.m, self.V0, self.material, J=self.J)
         return sim
 
     def run(self, sim, steps, start=0):
@@ -123,8 +124,11 @@ class SceneBatch:
         for i, s in enumerate(self.scenes):
             for c in s.colliders:
                 sim.add_scene_collider(i, c)
+        J = None
+        if any(s.J is not None for s in self.scenes):
+            J = np.concatenate([s.J if s.J is not None else np.ones(s.n) for s in self.scenes])
         sim.upload(self.cat("x"), self.cat("v"), self.cat("C"), self.cat("F"), self.cat("m"), self.cat("V0"),
-                   self.cat("material"), scene=self.scene_of)
+                   self.cat("material"), scene=self.scene_of, J=J)
         return sim
 
     def run(self, sim, steps, start=0):
