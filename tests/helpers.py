@@ -48,7 +48,7 @@ def read_state(path):
         buf = f.read()
     assert buf[:8] == b"ZMPMSTAT"
     version, n = np.frombuffer(buf, np.int32, 1, 8)[0], int(np.frombuffer(buf, np.int64, 1, 12)[0])
-    assert version == 1
+    assert version == 2
     frame = int(np.frombuffer(buf, np.int32, 1, 20)[0])
     time, since = np.frombuffer(buf, np.float64, 2, 24)
     off = 40
@@ -56,7 +56,7 @@ def read_state(path):
     for name, width, dt in (("x", 3, np.float64), ("v", 3, np.float64), ("C", 9, np.float64), ("F", 9, np.float64),
                             ("m", 1, np.float64), ("V0", 1, np.float64), ("material", 1, np.int32),
                             ("d", 1, np.float64), ("g", 1, np.float64), ("laplacian", 1, np.float64),
-                            ("plastic_state", 1, np.float64), ("max_dtilde", 1, np.float64)):
+                            ("plastic_state", 1, np.float64), ("max_dtilde", 1, np.float64), ("J", 1, np.float64)):
         cnt = n * width
         a = np.frombuffer(buf, dt, cnt, off)
         off += cnt * np.dtype(dt).itemsize

@@ -85,6 +85,9 @@ public:
     // AnisoFractureSimulation::implicit (AnisoFractureSimulation.h:61): phase
     // field by the CG solve of solveDamageSystem instead of the explicit update
     bool implicit = false;
+    // FJMixedMpmForceHelper::state (FJMixedMpmForceHelper.h:38-40): 0 SOLVE_F,
+    // 1 SOLVE_J; setFJState applies it to a running simulation
+    int fj_state = 0;
     TV gravity { 0, 0, 0 };
     bool autorestart = true;
     // ---- device configuration (no counterpart in the reference)
@@ -100,6 +103,7 @@ public:
     std::vector<double> damage, degradation, laplacian;
     std::vector<double> plastic_state; // von Mises yield stress / Drucker-Prager log Jp / snow Jp per particle
     std::vector<double> max_dtilde;    // AnisotropicPhaseField::maxDTilde (implicit damage)
+    std::vector<double> J;             // the "J" attribute of FJ-mixed particles (FJMixedMpmForceHelper.h:36)
     // ---- scene
     std::vector<zmpm_material> materials;
     std::vector<zmpm_collider> collision_objects;
@@ -136,6 +140,12 @@ public:
         dirty_ = true;
     }
 
+    void setFJState(int state)
+    {
+        fj_state = state;
+        if (h_) check(zmpm_set_fj_state(h_, state), "zmpm_set_fj_state");
+    }
+
     int addMaterial(const zmpm_material& m)
     {
         materials.push_back(m);
@@ -157,6 +167,7 @@ public:
         degradation.push_back(1);
         laplacian.push_back(0);
         max_dtilde.push_back(0);
+        J.push_back(1); // addFJMixedMpmForce starts J at 1
         const int pl = mat >= 0 && mat < (int)materials.size() ? materials[(size_t)mat].plasticity : ZMPM_PLASTICITY_NONE;
         plastic_state.push_back(pl == ZMPM_PLASTICITY_VON_MISES_STVK_HENCKY ? materials[(size_t)mat].yield_stress
                                     : pl == ZMPM_PLASTICITY_SNOW              ? 1.0 // SnowPlasticity::Jp starts at 1
@@ -188,6 +199,7 @@ public:
         d.implicit_damage = implicit ? 1 : 0;
         check(zmpm_create(&d, &h_), "zmpm_create");
         check(zmpm_set_materials(h_, (int32_t)materials.size(), materials.data()), "zmpm_set_materials");
+        check(zmpm_set_fj_state(h_, fj_state), "zmpm_set_fj_state");
         for (const auto& c : collision_objects) check(zmpm_add_collider(h_, &c, nullptr), "zmpm_add_collider");
         upload();
     }
@@ -338,7 +350,7 @@ public:
     // then each attribute array).  Materials and colliders are scene setup and
     // are not part of it, as in the reference's restart files.
     static constexpr char kStateMagic[8] = { 'Z', 'M', 'P', 'M', 'S', 'T', 'A', 'T' };
-    static constexpr int32_t kStateVersion = 1;
+    static constexpr int32_t kStateVersion = 2;
 
     virtual void writeState(std::ostream& out)
     {
@@ -488,6 +500,7 @@ private:
         fn((char*)laplacian.data(), n * sizeof(double));
         fn((char*)plastic_state.data(), n * sizeof(double));
         fn((char*)max_dtilde.data(), n * sizeof(double));
+        fn((char*)J.data(), n * sizeof(double));
     }
 
     void resizeAttributes(size_t n)
@@ -504,6 +517,7 @@ private:
         laplacian.resize(n);
         plastic_state.resize(n);
         max_dtilde.resize(n);
+        J.resize(n);
     }
 
     zmpm_particles view()
@@ -522,6 +536,7 @@ private:
         p.laplacian = laplacian.data();
         p.plastic_state = plastic_state.data();
         p.max_dtilde = max_dtilde.data();
+        p.J = J.data();
         return p;
     }
 
