@@ -149,6 +149,9 @@ typedef struct {
                                 129-141): 1 = the phase field of the damaged particles by the
                                 matrix-free CG solve of solveDamageSystem (:435-632), 0 = the
                                 explicit Laplacian update                                 */
+    int32_t ignore_collision_objects; /* MpmSimulationBase::ignoreCollisionObject
+                                (MpmSimulationBase.cpp:1217-1221): the grid update leaves
+                                the velocities the collision objects would change         */
 } zmpm_scene_desc;
 
 /* One material = one constitutive-model parameter set shared by its particles,
@@ -326,6 +329,24 @@ zmpm_status zmpm_get_binning(zmpm_handle h, int64_t* n_groups, uint64_t* keys,
  * Pass NULL arrays to query the count only. */
 zmpm_status zmpm_get_grid(zmpm_handle h, int64_t* n_nodes, int32_t* coords,
                           double* m, double* v, double* new_v, double* pf);
+
+/* External body force of a scene, MpmInitializationHelper::addExternalBodyForce
+ * (Lib/MPM/MpmInitializationHelper.h:1436-1444) appended to MpmSimulationBase::fext:
+ * f(x, t) = scale (sin(frequency t) + 1) direction, added as v_i += dt f to every
+ * grid node with mass after gravity and the internal forces, before the collision
+ * objects, at the scene's time t of the substep (MpmSimulationBase.cpp:1208-1212).
+ * At most 4 per scene, applied in the order added. */
+zmpm_status zmpm_add_body_force(zmpm_handle h, int32_t scene, double scale, double frequency,
+                                const double* direction);
+
+/* MpmSimulationBase::explicit_velocity_field (MpmSimulationBase.h:124, set by
+ * MpmInitializationHelper::addExplicitVelocityField :1416-1419) as the affine map
+ * v(x) = a + B x (B row-major): when set, the grid update writes new_v = a + B x_i
+ * at every node instead of the force update and the collision objects
+ * (MpmSimulationBase.cpp:1187-1195).  The reference takes any function of (m, x,
+ * v); the device takes the affine family, which holds the rigid motions and
+ * uniform strain rates.  a == NULL removes the field. */
+zmpm_status zmpm_set_velocity_field(zmpm_handle h, int32_t scene, const double* a, const double* B);
 
 /* Batched independent scenes in one handle: every launch of a substep covers
  * all of them.  No counterpart in the reference, whose simulations run one at a

@@ -1203,6 +1203,14 @@ struct zmpm_context {
     // particle, as the reference keeps one QuadraticVolumePenalty per particle)
     std::vector<double> J, jl;
     int fj_state = 0; // FJMixedMpmForceHelper::state: 0 SOLVE_F, 1 SOLVE_J
+    // MpmSimulationBase::fext as addExternalBodyForce builds its entries
+    // (scale, frequency, direction), and explicit_velocity_field as v = a + B x
+    struct BodyForce {
+        double scale, freq, dir[3];
+    };
+    std::vector<BodyForce> fext;
+    bool vfield = false;
+    double va[3] = { 0, 0, 0 }, vB[9] = {};
     std::vector<int32_t> mat;
     int cg_iterations = -1; // last implicit damage solve
     // binning of the last substep
@@ -1938,10 +1946,25 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
                 uint64_t off = (h->block_pages[b] << 12) | ((uint64_t)e << L.data_bits);
                 int node[3] = { (int)bitPack(off, L.xmask), (int)bitPack(off, L.ymask), (int)bitPack(off, L.zmask) };
                 double xi[3] = { node[0] * dx, node[1] * dx, node[2] * dx };
+                if (h->vfield) {
+                    // explicit_velocity_field(mi, xi, vi) (MpmSimulationBase.cpp:1187-1195)
+                    for (int a = 0; a < 3; a++)
+                        g.new_v[a] = h->va[a] + (h->vB[3 * a] * xi[0] + h->vB[3 * a + 1] * xi[1] + h->vB[3 * a + 2] * xi[2]);
+                    continue;
+                }
                 double dt_over_mi = dt / g.m;
                 double vi[3];
                 for (int a = 0; a < 3; a++) vi[a] = g.v[a] + dvg[a] + g.new_v[a] * dt_over_mi;
+                // fext(x, t) (:1208-1212), addExternalBodyForce's
+                // f = scale (sin(frequency t) + 1) direction (MpmInitializationHelper.h:1436-1444)
+                for (const auto& ff : h->fext) {
+                    const double amp = ff.scale * (std::sin(ff.freq * t) + 1);
+                    for (int a = 0; a < 3; a++) vi[a] += dt * (amp * ff.dir[a]);
+                }
+                double old_vi[3] = { vi[0], vi[1], vi[2] };
                 multiObjectCollision(cols, states, xi, vi);
+                if (h->desc.ignore_collision_objects) // ignoreCollisionObject (:1217-1221)
+                    for (int a = 0; a < 3; a++) vi[a] = old_vi[a];
                 for (int a = 0; a < 3; a++) g.new_v[a] = vi[a];
             }
         }
@@ -2039,6 +2062,37 @@ zmpm_status zmpm_destroy(zmpm_handle h)
 }
 
 const char* zmpm_last_error(zmpm_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+zmpm_status zmpm_add_body_force(zmpm_handle h, int32_t scene, double scale, double frequency, const double* direction)
+{
+    if (!h) return ZMPM_E_ARG;
+    if (!h->subs.empty()) {
+        if (scene < 0 || scene >= (int32_t)h->subs.size()) return fail(h, ZMPM_E_ARG, "no such scene");
+        const zmpm_status st = zmpm_add_body_force(h->subs[(size_t)scene], 0, scale, frequency, direction);
+        return st == ZMPM_OK ? st : fail(h, st, h->subs[(size_t)scene]->err);
+    }
+    if (scene != 0) return fail(h, ZMPM_E_ARG, "no such scene");
+    if (!direction || !std::isfinite(scale) || !std::isfinite(frequency)) return fail(h, ZMPM_E_ARG, "bad body force");
+    if (h->fext.size() >= 4) return fail(h, ZMPM_E_ARG, "at most 4 body forces per scene");
+    h->fext.push_back({ scale, frequency, { direction[0], direction[1], direction[2] } });
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_set_velocity_field(zmpm_handle h, int32_t scene, const double* a, const double* B)
+{
+    if (!h) return ZMPM_E_ARG;
+    if (!h->subs.empty()) {
+        if (scene < 0 || scene >= (int32_t)h->subs.size()) return fail(h, ZMPM_E_ARG, "no such scene");
+        const zmpm_status st = zmpm_set_velocity_field(h->subs[(size_t)scene], 0, a, B);
+        return st == ZMPM_OK ? st : fail(h, st, h->subs[(size_t)scene]->err);
+    }
+    if (scene != 0) return fail(h, ZMPM_E_ARG, "no such scene");
+    if (a && !B) return fail(h, ZMPM_E_ARG, "the velocity field needs B");
+    h->vfield = a != nullptr;
+    for (int k = 0; k < 3; k++) h->va[k] = a ? a[k] : 0.0;
+    for (int k = 0; k < 9; k++) h->vB[k] = a ? B[k] : 0.0;
+    return ZMPM_OK;
+}
 
 zmpm_status zmpm_set_fj_state(zmpm_handle h, int32_t state)
 {

@@ -50,6 +50,7 @@ class SceneDesc(C.Structure):
         ("rpic_damping_iteration", C.c_int32),
         ("pic_damping_iteration", C.c_int32),
         ("implicit_damage", C.c_int32),
+        ("ignore_collision_objects", C.c_int32),
     ]
 
 
@@ -251,7 +252,7 @@ class ZmpmLib:
         "zmpm_get_grid", "zmpm_set_timing", "zmpm_get_timings", "zmpm_calculate_dt", "zmpm_get_launch_config",
         "zmpm_set_launch_config", "zmpm_autotune", "zmpm_create_batch", "zmpm_batch_origin",
         "zmpm_add_scene_collider", "zmpm_upload_batch", "zmpm_run_batch", "zmpm_calculate_dt_noalloc",
-        "zmpm_set_fj_state",
+        "zmpm_set_fj_state", "zmpm_add_body_force", "zmpm_set_velocity_field",
     )
 
     def __init__(self, path: str):
@@ -296,6 +297,8 @@ class ZmpmLib:
         L.zmpm_run_batch.argtypes = [C.c_void_p, C.c_int32, _DP, _DP]
         L.zmpm_calculate_dt_noalloc.argtypes = [C.c_void_p, C.c_double, C.c_int32, _DP]
         L.zmpm_set_fj_state.argtypes = [C.c_void_p, C.c_int32]
+        L.zmpm_add_body_force.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, _DP]
+        L.zmpm_set_velocity_field.argtypes = [C.c_void_p, C.c_int32, _DP, _DP]
         if L.zmpm_abi_version() != ZMPM_ABI_VERSION:
             raise RuntimeError(f"{path}: ABI version {L.zmpm_abi_version()} != {ZMPM_ABI_VERSION}")
 
@@ -306,7 +309,7 @@ class ZmpmLib:
 
 def scene_desc(*, dx, precision=PREC_FP64, domain_nodes=256, gravity=(0, 0, 0), apic_rpic_ratio=1.0, cfl=0.6,
                device=0, transfer_scheme=APIC_BLEND_RPIC, flip_pic_ratio=0.0, mls_mpm=False, rpic_damping_iteration=0,
-               pic_damping_iteration=0, implicit_damage=False) -> SceneDesc:
+               pic_damping_iteration=0, implicit_damage=False, ignore_collision_objects=False) -> SceneDesc:
     d = SceneDesc()
     d.precision, d.domain_nodes, d.dx = precision, domain_nodes, dx
     for a in range(3):
@@ -315,6 +318,7 @@ def scene_desc(*, dx, precision=PREC_FP64, domain_nodes=256, gravity=(0, 0, 0), 
     d.transfer_scheme, d.flip_pic_ratio, d.mls_mpm = transfer_scheme, flip_pic_ratio, int(bool(mls_mpm))
     d.rpic_damping_iteration, d.pic_damping_iteration = rpic_damping_iteration, pic_damping_iteration
     d.implicit_damage = int(implicit_damage)
+    d.ignore_collision_objects = int(bool(ignore_collision_objects))
     return d
 
 
@@ -357,6 +361,17 @@ class Simulation:
     def set_materials(self, mats):
         arr = (Material * len(mats))(*mats)
         self._check(self.L.zmpm_set_materials(self.h, len(mats), arr), "zmpm_set_materials")
+
+    def add_body_force(self, scale, frequency, direction, scene=0):
+        """addExternalBodyForce: f(t) = scale (sin(frequency t) + 1) direction."""
+        d = np.ascontiguousarray(direction, np.float64)
+        self._check(self.L.zmpm_add_body_force(self.h, scene, scale, frequency, _ptr(d)), "zmpm_add_body_force")
+
+    def set_velocity_field(self, a=None, B=None, scene=0):
+        """explicit_velocity_field as v(x) = a + B x; a=None removes it."""
+        av = None if a is None else np.ascontiguousarray(a, np.float64)
+        Bv = None if B is None else np.ascontiguousarray(B, np.float64).reshape(9)
+        self._check(self.L.zmpm_set_velocity_field(self.h, scene, _ptr(av), _ptr(Bv)), "zmpm_set_velocity_field")
 
     def set_fj_state(self, state):
         """FJMixedMpmForceHelper::state: SOLVE_F (0) or SOLVE_J (1)."""

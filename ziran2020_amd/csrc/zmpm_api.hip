@@ -77,6 +77,18 @@ struct zmpm_context {
     std::vector<double> step_dt;     // per scene: dt the device table holds (0: none yet)
     std::vector<unsigned char> scene_host; // host mirror of the device scene table
     void* scenes_dev = nullptr;
+    void* bfval_dev = nullptr; // body forces at the running substep's time, per scene
+    // per scene: external body forces (scale, frequency, direction) and the
+    // explicit velocity field (enabled, a, B)
+    struct BodyForce {
+        double scale, freq, dir[3];
+    };
+    struct VelocityField {
+        bool on = false;
+        double a[3] {}, B[9] {};
+    };
+    std::vector<std::vector<BodyForce>> body_forces;
+    std::vector<VelocityField> vfields;
     void* regions_dev = nullptr;
     void* colstate_dev = nullptr;
     Dev<float, long long> Df {};
@@ -326,6 +338,18 @@ zmpm_status uploadScenes(zmpm_context* h)
         e.col0 = col0;
         e.ncol = (int)h->scene_cols[s].size();
         col0 += e.ncol;
+        e.ignore_cols = d.ignore_collision_objects ? 1 : 0;
+        e.nbf = (int)h->body_forces[s].size();
+        for (int k = 0; k < e.nbf; k++) {
+            const auto& bf = h->body_forces[s][(size_t)k];
+            e.bf_scale[k] = bf.scale;
+            e.bf_freq[k] = bf.freq;
+            for (int a = 0; a < 3; a++) e.bf_dir[k][a] = bf.dir[a];
+        }
+        const auto& vf = h->vfields[s];
+        e.vfield = vf.on ? 1 : 0;
+        for (int a = 0; a < 3; a++) e.va[a] = vf.a[a];
+        for (int k = 0; k < 9; k++) e.vB[k] = vf.B[k];
     }
     HIPCHK(h, hipMemcpyAsync(h->scenes_dev, sd, ns * sizeof(SceneDev<T>), hipMemcpyHostToDevice, h->stream));
     return ZMPM_OK;
@@ -340,6 +364,7 @@ void refreshTables(zmpm_context* h)
     D.colstate = (ColState*)h->colstate_dev;
     D.num_cols = (int)colliderCount(h);
     D.sc = (const SceneDev<T>*)h->scenes_dev;
+    D.bfval = (double*)h->bfval_dev;
     D.nscenes = (int)h->scenes.size();
     D.region_scene = (const int*)h->regions_dev;
     D.region_dim = h->region_dim;
@@ -941,6 +966,8 @@ zmpm_status createScenes(const zmpm_scene_desc* descs, int32_t n_scenes, zmpm_ha
     h->desc = descs[0];
     h->scenes.assign(descs, descs + n_scenes);
     h->scene_cols.assign((size_t)n_scenes, {});
+    h->body_forces.assign((size_t)n_scenes, {});
+    h->vfields.assign((size_t)n_scenes, {});
     h->step_dt.assign((size_t)n_scenes, 0.0);
     h->region = region;
     h->region_dim = rd;
@@ -979,11 +1006,12 @@ zmpm_status createScenes(const zmpm_scene_desc* descs, int32_t n_scenes, zmpm_ha
     if (hipMalloc(&h->mats_dev, msz * kMaxMaterials) != hipSuccess || hipMalloc(&h->cols_dev, sizeof(ColDev) * ncol) != hipSuccess
         || hipMalloc(&h->colstate_dev, sizeof(ColState) * ncol) != hipSuccess
         || hipMalloc(&h->scenes_dev, ssz * (size_t)n_scenes) != hipSuccess
+        || hipMalloc(&h->bfval_dev, sizeof(double) * 3 * MAX_BODY_FORCES * (size_t)n_scenes) != hipSuccess
         || hipMalloc(&h->regions_dev, sizeof(int) * h->region_scene.size()) != hipSuccess
         || hipMemcpy(h->regions_dev, h->region_scene.data(), sizeof(int) * h->region_scene.size(), hipMemcpyHostToDevice)
             != hipSuccess) {
         (void)hipGetLastError();
-        for (void* q : { h->mats_dev, h->cols_dev, h->colstate_dev, h->scenes_dev, h->regions_dev })
+        for (void* q : { h->mats_dev, h->cols_dev, h->colstate_dev, h->scenes_dev, h->regions_dev, h->bfval_dev })
             if (q) (void)hipFree(q);
         (void)hipStreamDestroy(h->stream);
         delete h;
@@ -1021,7 +1049,7 @@ zmpm_status zmpm_destroy(zmpm_handle h)
     (void)hipSetDevice(h->desc.device);
     freeAll(h);
     for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
-    for (void* q : { h->mats_dev, h->cols_dev, h->colstate_dev, h->scenes_dev, h->regions_dev })
+    for (void* q : { h->mats_dev, h->cols_dev, h->colstate_dev, h->scenes_dev, h->regions_dev, h->bfval_dev })
         if (q) (void)hipFree(q);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -1072,6 +1100,41 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
     // particles already on the device: their stored stress and the next P2G's
     // fixed-point bounds follow the new table (the binning stays)
     if (h->n > 0) return h->fp32 ? restress<float, long long>(h) : restress<double, double>(h);
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_add_body_force(zmpm_handle h, int32_t scene, double scale, double frequency, const double* direction)
+{
+    if (!h) return ZMPM_E_ARG;
+    if (scene < 0 || scene >= (int32_t)h->scenes.size()) return fail(h, ZMPM_E_ARG, "no such scene");
+    if (!direction || !std::isfinite(scale) || !std::isfinite(frequency)) return fail(h, ZMPM_E_ARG, "bad body force");
+    auto& list = h->body_forces[(size_t)scene];
+    if ((int)list.size() >= MAX_BODY_FORCES) return fail(h, ZMPM_E_ARG, "at most 4 body forces per scene");
+    (void)hipSetDevice(h->desc.device);
+    dropGraphs(h);
+    list.push_back({ scale, frequency, { direction[0], direction[1], direction[2] } });
+    if (h->fp32)
+        refreshTables<float, long long>(h);
+    else
+        refreshTables<double, double>(h);
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_set_velocity_field(zmpm_handle h, int32_t scene, const double* a, const double* B)
+{
+    if (!h) return ZMPM_E_ARG;
+    if (scene < 0 || scene >= (int32_t)h->scenes.size()) return fail(h, ZMPM_E_ARG, "no such scene");
+    if (a && !B) return fail(h, ZMPM_E_ARG, "the velocity field needs B");
+    (void)hipSetDevice(h->desc.device);
+    dropGraphs(h);
+    auto& vf = h->vfields[(size_t)scene];
+    vf.on = a != nullptr;
+    for (int k = 0; k < 3; k++) vf.a[k] = a ? a[k] : 0.0;
+    for (int k = 0; k < 9; k++) vf.B[k] = a ? B[k] : 0.0;
+    if (h->fp32)
+        refreshTables<float, long long>(h);
+    else
+        refreshTables<double, double>(h);
     return ZMPM_OK;
 }
 

@@ -534,6 +534,16 @@ __device__ __forceinline__ void beginStep(const Dev<T, AccT>& D)
         const ColDev co = D.cols[c];
         colliderState(co, D.clock[2 * co.scene + 1], D.colstate[c]);
     }
+    // the external body forces at the scene's time t_n
+    // (addExternalBodyForce, MpmInitializationHelper.h:1436-1444)
+    for (int e = threadIdx.x; e < D.nscenes * MAX_BODY_FORCES; e += WG) {
+        const int sc = e / MAX_BODY_FORCES, k = e - sc * MAX_BODY_FORCES;
+        const SceneDev<T>& sd = D.sc[sc];
+        if (k >= sd.nbf) continue;
+        const double amp = sd.bf_scale[k] * (sin(sd.bf_freq[k] * D.clock[2 * sc + 1]) + 1.0);
+#pragma unroll
+        for (int a = 0; a < 3; a++) D.bfval[3 * e + a] = amp * sd.bf_dir[k][a];
+    }
     if constexpr (!std::is_same<AccT, long long>::value) return;
     // one wavefront per (scene, scale group) over the BOUND_SLOTS slots, which
     // are cleared for this substep's G2P
@@ -1637,11 +1647,25 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D)
             if (D.flip) D.oldv[a * cn + node] = v; // the grid's v before the update (FLIP)
             vi[a] = v + sd.gravity[a] * dt + f[a] * dt_over_mi;
         }
+        // external body forces, in the order added (MpmSimulationBase.cpp:1208-1212)
+        for (int k = 0; k < sd.nbf; k++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) vi[a] += dt * (T)D.bfval[3 * (scene * MAX_BODY_FORCES + k) + a];
         double vd[3] = { vi[0], vi[1], vi[2] };
-        if (uniform)
+        if (sd.ignore_cols) {
+            // ignoreCollisionObject (:1217-1221): the objects leave v as it is
+        }
+        else if (uniform)
             multiObjectCollision(cols, cst, nstaged, xi, vd);
         else
             multiObjectCollision(D.cols + sd.col0, D.colstate + sd.col0, min(sd.ncol, MAX_COLLIDERS), xi, vd);
+        if (sd.vfield) {
+            // explicit_velocity_field (:1187-1195): new_v = a + B x_i at every node
+            // with mass, in place of the update above
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                vd[a] = sd.va[a] + (sd.vB[3 * a] * xi[0] + sd.vB[3 * a + 1] * xi[1] + sd.vB[3 * a + 2] * xi[2]);
+        }
 #pragma unroll
         for (int a = 0; a < 3; a++) D.newv[a * cn + node] = (T)vd[a];
         if (D.any_damage) {

@@ -140,6 +140,7 @@ __host__ __device__ constexpr bool phaseFieldModel(int model)
 }
 
 constexpr int MAX_COLLIDERS = 16; // collision objects per handle
+constexpr int MAX_BODY_FORCES = 4; // external body forces per scene
 
 // A collision object (AnalyticCollisionObject, CollisionObject.h:20-165) as the
 // kernels read it: its level set, its rigid motion and, for the capped cylinder,
@@ -188,6 +189,14 @@ struct SceneDev {
     int origin[3]; // first node of the scene's region
     int domain;    // the scene's own addressable nodes per axis (base nodes in [0, domain - 4))
     int col0, ncol; // its collision objects
+    int ignore_cols; // MpmSimulationBase::ignoreCollisionObject
+    // external body forces (MpmSimulationBase::fext as addExternalBodyForce makes
+    // them): f(t) = scale (sin(frequency t) + 1) direction, evaluated per substep
+    int nbf;
+    double bf_scale[MAX_BODY_FORCES], bf_freq[MAX_BODY_FORCES], bf_dir[MAX_BODY_FORCES][3];
+    // explicit velocity field v(x) = a + B x (row-major B), when vfield != 0
+    int vfield;
+    double va[3], vB[9];
 };
 
 constexpr int PMAT_SCENE_SHIFT = 8; // material index in bits 0-7, scene above
@@ -245,6 +254,7 @@ struct Dev {
     const MatDev<T>* mats;
     const ColDev* cols;  // grouped by scene (SceneDev::col0, ncol)
     ColState* colstate;  // [num_cols] states at the running substep's time (written by k_scatter)
+    double* bfval;       // [nscenes][MAX_BODY_FORCES][3] body forces at the running substep's time
     int num_cols;
     int any_damage;
     int any_plastic;

@@ -40,6 +40,8 @@ class Scene:
     options: dict = field(default_factory=dict)  # zmpm_scene_desc options of the workload (e.g. implicit_damage)
     t0: float = 0.0  # simulation time of the first substep
     J: np.ndarray = None  # FJ-mixed particles' J (None: 1)
+    body_forces: list = field(default_factory=list)  # (scale, frequency, direction): addExternalBodyForce
+    velocity_field: tuple = None  # (a, B): explicit_velocity_field v = a + B x
 
     def __post_init__(self):
         n = self.x.shape[0]
@@ -64,10 +66,20 @@ class Scene:
     def make(self, lib, precision=A.PREC_FP64, device=0, **transfer):
         sim = A.Simulation(lib, **self.desc(precision, device, **transfer))
         sim.set_materials(self.materials)
-        for c in self.colliders:
-            sim.add_collider(c)
+        self.add_scene_setup(sim, 0, batch=False)
         sim.upload(self.x, self.v, self.C, self.F, self.m, self.V0, self.material, J=self.J)
         return sim
+
+    def add_scene_setup(self, sim, scene, batch):
+        for c in self.colliders:
+            if batch:
+                sim.add_scene_collider(scene, c)
+            else:
+                sim.add_collider(c)
+        for scale, freq, direction in self.body_forces:
+            sim.add_body_force(scale, freq, direction, scene=scene)
+        if self.velocity_field is not None:
+            sim.set_velocity_field(*self.velocity_field, scene=scene)
 
     def run(self, sim, steps, start=0):
         """Substeps start .. start + steps - 1 of the scene (time t0 + start dt)."""
@@ -122,8 +134,7 @@ class SceneBatch:
         sim = A.Batch(lib, [s.desc(precision, device, **transfer) for s in self.scenes])
         sim.set_materials(self.materials)
         for i, s in enumerate(self.scenes):
-            for c in s.colliders:
-                sim.add_scene_collider(i, c)
+            s.add_scene_setup(sim, i, batch=True)
         J = None
         if any(s.J is not None for s in self.scenes):
             J = np.concatenate([s.J if s.J is not None else np.ones(s.n) for s in self.scenes])
