@@ -308,6 +308,43 @@ zmpm_status zmpm_substep(zmpm_handle h, double dt, double t);
 /* `steps` substeps t0, t0+dt, ...; the GPU library replays a captured HIP graph. */
 zmpm_status zmpm_run(zmpm_handle h, int32_t steps, double dt, double t0);
 
+/* Stages of one substep, for a host that replaces one of the virtual steps of
+ * MpmSimulationBase::advanceOneTimeStep (MpmSimulationBase.h:152-227) and keeps
+ * the device for the others.  zmpm_stage(h, s, dt, t) runs stage s of the
+ * substep at time t with step dt; the stages of a substep run in this order,
+ * each once, with the same dt and t, and together equal zmpm_substep(h, dt, t):
+ *   SORT          beginTimeStep + sortParticlesAndPolluteGrid (:238-245, 1531-1573)
+ *   P2G           forcesUpdateParticleState + particlesToGrid (:953-979, 618-667)
+ *                 with the AnisoMPM phase-field transfer (AnisoFractureSimulation.h:
+ *                 129-141); zmpm_get_grid then reports m and v = mv / m
+ *   GRID_UPDATE   gridVelocityExplicitUpdate (:1179-1228); zmpm_get_grid then
+ *                 reports new_v, which zmpm_set_grid_velocity may replace
+ *   DAMAGE_SOLVE  solveDamageSystem (AnisoFractureSimulation.h:435-612) when
+ *                 implicit_damage is set (else nothing)
+ *   G2P           gridToParticles + evolveStrain + the damage and plasticity
+ *                 updates + the damping passes (:1412-1529, 318-348)
+ * The particle state changes at G2P only (the explicit damage update, which the
+ * reference runs inside particlesToGrid, reads only F_n and the phase-field
+ * Laplacian, so it is applied there).  Between the stages of a substep only the
+ * queries (zmpm_get_grid, zmpm_get_binning, zmpm_download) and
+ * zmpm_set_grid_velocity are allowed; the other entry points return ZMPM_E_ARG
+ * until its G2P stage has run.  A single-scene handle only. */
+typedef enum {
+    ZMPM_STAGE_SORT = 0,
+    ZMPM_STAGE_P2G = 1,
+    ZMPM_STAGE_GRID_UPDATE = 2,
+    ZMPM_STAGE_DAMAGE_SOLVE = 3,
+    ZMPM_STAGE_G2P = 4
+} zmpm_stage_id;
+zmpm_status zmpm_stage(zmpm_handle h, int32_t stage, double dt, double t);
+
+/* Replace the updated grid velocities (new_v, 3 doubles per node) of the
+ * substep in progress, after its GRID_UPDATE stage and before its G2P stage:
+ * n_nodes and the node order are those zmpm_get_grid reports (nodes with mass,
+ * ordered by (x, y, z) node coordinates).  For a host override of
+ * gridVelocityExplicitUpdate. */
+zmpm_status zmpm_set_grid_velocity(zmpm_handle h, int64_t n_nodes, const double* new_v);
+
 /* Wait for queued work; returns the first latched device condition. */
 zmpm_status zmpm_synchronize(zmpm_handle h);
 

@@ -1211,6 +1211,9 @@ struct zmpm_context {
     std::vector<BodyForce> fext;
     bool vfield = false;
     double va[3] = { 0, 0, 0 }, vB[9] = {};
+    // zmpm_stage: the next stage of the substep in progress (0 = none), its dt and t
+    int stage_next = 0;
+    double stage_dt = 0, stage_t = 0;
     std::vector<int32_t> mat;
     int cg_iterations = -1; // last implicit damage solve
     // binning of the last substep
@@ -1790,13 +1793,26 @@ void solveDamageSystem(zmpm_context* h, double dt)
     }
 }
 
-zmpm_status substep(zmpm_context* h, double dt, double t)
+// One explicit substep (MpmSimulationBase::advanceOneTimeStep), or its stages
+// first..last (zmpm_stage): 0 sortParticlesAndPolluteGrid, 1 forcesUpdateParticleState
+// + particlesToGrid with the AnisoMPM damage of its override (the explicit update
+// or solveDamageSystem, AnisoFractureSimulation.h:129-141), 2
+// gridVelocityExplicitUpdate, 3 (nothing: the damage solve is part of 1 here),
+// 4 gridToParticles + evolveStrain + applyPlasticity + the damping passes.
+zmpm_status substep(zmpm_context* h, double dt, double t, int first = 0, int last = 4)
 {
     if (h->n == 0) return ZMPM_OK;
-    zmpm_status st = sortAndPollute(h);
-    if (st != ZMPM_OK) return st;
+    if (first <= 0 && 0 <= last) {
+        zmpm_status st = sortAndPollute(h);
+        if (st != ZMPM_OK) return st;
+    }
     const int64_t n = h->n;
     const double dx = h->desc.dx;
+    const int ng = (int)h->groups.size();
+    const bool apic = h->desc.transfer_scheme == ZMPM_APIC_BLEND_RPIC;
+    const bool mls = h->desc.mls_mpm != 0;
+    const double D_inverse = 4 / (dx * dx);
+    if (first <= 1 && 1 <= last) {
 
     // forcesUpdateParticleState (MpmSimulationBase.cpp:953-979)
     std::vector<double> vtau((size_t)n * 9);
@@ -1826,10 +1842,6 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
 
     // particlesToGridWithForceHelper<scheme, deg 2, mls> generic path (MpmSimulationBase.cpp:791-854):
     // C = mass C for APIC_blend_RPIC, 0 for FLIP_blend_PIC; MLS force S (x_i - x_p) w D^-1
-    const int ng = (int)h->groups.size();
-    const bool apic = h->desc.transfer_scheme == ZMPM_APIC_BLEND_RPIC;
-    const bool mls = h->desc.mls_mpm != 0;
-    const double D_inverse = 4 / (dx * dx);
     for (uint64_t color = 0; color < 8; color++) {
 #pragma omp parallel for schedule(dynamic, 4)
         for (int gi = 0; gi < ng; gi++) {
@@ -1930,9 +1942,10 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
         }
     }
 
+    }
     // gridVelocityExplicitUpdate (MpmSimulationBase.cpp:1179-1228) with
     // multiObjectCollision (CollisionObject.cpp:66-105) at time t
-    {
+    if (first <= 2 && 2 <= last) {
         double dvg[3] = { h->desc.gravity[0] * dt, h->desc.gravity[1] * dt, h->desc.gravity[2] * dt };
         const std::vector<Collider>& cols = h->colliders;
         const std::vector<ColliderAt> states = collidersAt(cols, t);
@@ -1970,6 +1983,7 @@ zmpm_status substep(zmpm_context* h, double dt, double t)
         }
     }
 
+    if (!(first <= 4 && 4 <= last)) return ZMPM_OK;
     int cfl_flag = gridToParticlesPass(h, dt, h->desc.apic_rpic_ratio, h->desc.flip_pic_ratio, false);
 
     // damping passes (MpmSimulationBase.cpp:318-348): grid m, v cleared on the
@@ -2105,6 +2119,7 @@ zmpm_status zmpm_set_fj_state(zmpm_handle h, int32_t state)
 
 zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material* mats)
 {
+    if (h && h->stage_next) return fail(h, ZMPM_E_ARG, "a substep is in progress (zmpm_stage)");
     if (h && !h->subs.empty()) {
         for (zmpm_context* c : h->subs) {
             const zmpm_status st = zmpm_set_materials(c, count, mats);
@@ -2171,6 +2186,7 @@ zmpm_status zmpm_upload_batch(zmpm_handle h, const zmpm_particles* p, const int3
 zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
 {
     if (!h || !p || p->n < 0) return ZMPM_E_ARG;
+    if (h->stage_next) return fail(h, ZMPM_E_ARG, "a substep is in progress (zmpm_stage)");
     if (!h->subs.empty()) {
         std::vector<int32_t> zero((size_t)p->n, 0);
         return zmpm_upload_batch(h, p, zero.data());
@@ -2409,9 +2425,28 @@ zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
     return s;
 }
 
+zmpm_status zmpm_stage(zmpm_handle h, int32_t stage, double dt, double t)
+{
+    if (!h || stage < ZMPM_STAGE_SORT || stage > ZMPM_STAGE_G2P || !(dt > 0)) return ZMPM_E_ARG;
+    if (!h->subs.empty()) return fail(h, ZMPM_E_ARG, "stages run on a single-scene handle");
+    if (stage != h->stage_next) return fail(h, ZMPM_E_ARG, "stages run in order: SORT, P2G, GRID_UPDATE, DAMAGE_SOLVE, G2P");
+    if (stage == ZMPM_STAGE_SORT) {
+        if (h->latched != ZMPM_OK && h->latched != ZMPM_E_CFL) return (zmpm_status)h->latched;
+        h->stage_dt = dt;
+        h->stage_t = t;
+    }
+    else if (dt != h->stage_dt || t != h->stage_t)
+        return fail(h, ZMPM_E_ARG, "the stages of a substep share its dt and t");
+    const zmpm_status st = substep(h, dt, t, stage, stage);
+    h->stage_next = stage == ZMPM_STAGE_G2P ? 0 : stage + 1;
+    if (st != ZMPM_OK) h->stage_next = 0;
+    return st;
+}
+
 zmpm_status zmpm_substep(zmpm_handle h, double dt, double t)
 {
     if (!h) return ZMPM_E_ARG;
+    if (h->stage_next) return fail(h, ZMPM_E_ARG, "a substep is in progress (zmpm_stage)");
     if (!h->subs.empty()) {
         zmpm_status first = ZMPM_OK;
         for (zmpm_context* c : h->subs) {
@@ -2427,6 +2462,7 @@ zmpm_status zmpm_substep(zmpm_handle h, double dt, double t)
 zmpm_status zmpm_run(zmpm_handle h, int32_t steps, double dt, double t0)
 {
     if (!h || steps < 0) return ZMPM_E_ARG;
+    if (h->stage_next) return fail(h, ZMPM_E_ARG, "a substep is in progress (zmpm_stage)");
     for (int32_t s = 0; s < steps; s++) {
         zmpm_status st = zmpm_substep(h, dt, t0 + s * dt);
         if (st != ZMPM_OK) return st;
@@ -2461,28 +2497,48 @@ zmpm_status zmpm_get_binning(zmpm_handle h, int64_t* n_groups, uint64_t* keys, i
     return ZMPM_OK;
 }
 
-zmpm_status zmpm_get_grid(zmpm_handle h, int64_t* n_nodes, int32_t* coords, double* m, double* v, double* new_v, double* pf)
+// The grid nodes with mass, ordered by (x, y, z) node coordinates.
+struct GridRec {
+    int c[3];
+    Node* g;
+};
+std::vector<GridRec> gridRecords(zmpm_context* h)
 {
-    if (h && !h->subs.empty()) return fail(h, ZMPM_E_ARG, "the oracle reports a batch's grid per scene only (a handle per scene)");
-    if (!h || !n_nodes) return ZMPM_E_ARG;
-    struct Rec {
-        int c[3];
-        const Node* g;
-    };
-    std::vector<Rec> recs;
+    std::vector<GridRec> recs;
     for (size_t b = 0; b < h->block_pages.size(); b++)
         for (int e = 0; e < h->NB; e++) {
-            const Node& g = h->nodes[b * h->NB + e];
+            Node& g = h->nodes[b * h->NB + e];
             if (g.idx < 0) continue;
             uint64_t off = (h->block_pages[b] << 12) | ((uint64_t)e << h->L.data_bits);
-            Rec r { { (int)bitPack(off, h->L.xmask), (int)bitPack(off, h->L.ymask), (int)bitPack(off, h->L.zmask) }, &g };
+            GridRec r { { (int)bitPack(off, h->L.xmask), (int)bitPack(off, h->L.ymask), (int)bitPack(off, h->L.zmask) }, &g };
             recs.push_back(r);
         }
-    std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) {
+    std::sort(recs.begin(), recs.end(), [](const GridRec& a, const GridRec& b) {
         if (a.c[0] != b.c[0]) return a.c[0] < b.c[0];
         if (a.c[1] != b.c[1]) return a.c[1] < b.c[1];
         return a.c[2] < b.c[2];
     });
+    return recs;
+}
+
+zmpm_status zmpm_set_grid_velocity(zmpm_handle h, int64_t n_nodes, const double* new_v)
+{
+    if (!h || !new_v) return ZMPM_E_ARG;
+    if (!h->subs.empty()) return fail(h, ZMPM_E_ARG, "stages run on a single-scene handle");
+    if (h->stage_next != ZMPM_STAGE_DAMAGE_SOLVE && h->stage_next != ZMPM_STAGE_G2P)
+        return fail(h, ZMPM_E_ARG, "the grid velocities are replaced between the GRID_UPDATE and G2P stages");
+    std::vector<GridRec> recs = gridRecords(h);
+    if (n_nodes != (int64_t)recs.size()) return fail(h, ZMPM_E_ARG, "node count differs from zmpm_get_grid's");
+    for (size_t q = 0; q < recs.size(); q++)
+        for (int a = 0; a < 3; a++) recs[q].g->new_v[a] = new_v[3 * q + a];
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_get_grid(zmpm_handle h, int64_t* n_nodes, int32_t* coords, double* m, double* v, double* new_v, double* pf)
+{
+    if (h && !h->subs.empty()) return fail(h, ZMPM_E_ARG, "the oracle reports a batch's grid per scene only (a handle per scene)");
+    if (!h || !n_nodes) return ZMPM_E_ARG;
+    std::vector<GridRec> recs = gridRecords(h);
     *n_nodes = (int64_t)recs.size();
     for (size_t q = 0; q < recs.size(); q++) {
         const Node& g = *recs[q].g;

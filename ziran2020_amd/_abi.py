@@ -30,6 +30,7 @@ APIC_BLEND_RPIC, FLIP_BLEND_PIC = 0, 1
 PLASTICITY_NONE, PLASTICITY_VON_MISES_STVK_HENCKY, PLASTICITY_DRUCKER_PRAGER_STVK_HENCKY, PLASTICITY_SNOW = 0, 1, 2, 3
 SHAPE_HALF_SPACE, SHAPE_BOX, SHAPE_SPHERE, SHAPE_CAPPED_CYLINDER = 0, 1, 2, 3
 FJ_NONE, FJ_SOLID, FJ_FLUID = 0, 1, 2
+STAGE_SORT, STAGE_P2G, STAGE_GRID_UPDATE, STAGE_DAMAGE_SOLVE, STAGE_G2P = range(5)
 SOLVE_F, SOLVE_J = 0, 1
 
 D3 = C.c_double * 3
@@ -252,7 +253,8 @@ class ZmpmLib:
         "zmpm_get_grid", "zmpm_set_timing", "zmpm_get_timings", "zmpm_calculate_dt", "zmpm_get_launch_config",
         "zmpm_set_launch_config", "zmpm_autotune", "zmpm_create_batch", "zmpm_batch_origin",
         "zmpm_add_scene_collider", "zmpm_upload_batch", "zmpm_run_batch", "zmpm_calculate_dt_noalloc",
-        "zmpm_set_fj_state", "zmpm_add_body_force", "zmpm_set_velocity_field",
+        "zmpm_set_fj_state", "zmpm_add_body_force", "zmpm_set_velocity_field", "zmpm_stage",
+        "zmpm_set_grid_velocity",
     )
 
     def __init__(self, path: str):
@@ -299,6 +301,8 @@ class ZmpmLib:
         L.zmpm_set_fj_state.argtypes = [C.c_void_p, C.c_int32]
         L.zmpm_add_body_force.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, _DP]
         L.zmpm_set_velocity_field.argtypes = [C.c_void_p, C.c_int32, _DP, _DP]
+        L.zmpm_stage.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double]
+        L.zmpm_set_grid_velocity.argtypes = [C.c_void_p, C.c_int64, _DP]
         if L.zmpm_abi_version() != ZMPM_ABI_VERSION:
             raise RuntimeError(f"{path}: ABI version {L.zmpm_abi_version()} != {ZMPM_ABI_VERSION}")
 
@@ -433,6 +437,15 @@ class Simulation:
 
     def substep(self, dt, t):
         return self._check(self.L.zmpm_substep(self.h, dt, t), "zmpm_substep")
+
+    def stage(self, stage, dt, t):
+        """One stage of the substep at t (zmpm_stage): STAGE_SORT .. STAGE_G2P in order."""
+        return self._check(self.L.zmpm_stage(self.h, stage, dt, t), "zmpm_stage")
+
+    def set_grid_velocity(self, new_v):
+        """Replace new_v of the grid nodes zmpm_get_grid reports (between GRID_UPDATE and G2P)."""
+        nv = np.ascontiguousarray(new_v, np.float64).reshape(-1, 3)
+        self._check(self.L.zmpm_set_grid_velocity(self.h, nv.shape[0], _ptr(nv)), "zmpm_set_grid_velocity")
 
     def run(self, steps, dt, t0=0.0):
         return self._check(self.L.zmpm_run(self.h, steps, dt, t0), "zmpm_run")

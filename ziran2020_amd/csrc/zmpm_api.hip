@@ -114,6 +114,9 @@ struct zmpm_context {
     bool use_graphs = true;
     int fj_state = 0;       // FJMixedMpmForceHelper::state: 0 SOLVE_F, 1 SOLVE_J
     hipError_t launch_err = hipSuccess; // a failed launch that returns its error (the cooperative k_cg)
+    // zmpm_stage: the next stage of the substep in progress (0 = none), its dt and t
+    int stage_next = 0;
+    double stage_dt = 0, stage_t = 0;
     int p2g_split = 64;
     // workgroups of the grid-stride launches (results do not depend on them)
     int p2g_wgs = 2048, grid_wgs = 2048, prep_wgs = 2048;
@@ -128,6 +131,14 @@ zmpm_status fail(zmpm_context* h, zmpm_status s, const std::string& msg)
         h->err_static = nullptr;
     }
     return s;
+}
+
+// Entry points that change the state: refused between the stages of a substep.
+bool midSubstep(zmpm_context* h)
+{
+    if (h->stage_next == 0) return false;
+    (void)fail(h, ZMPM_E_ARG, "a substep is in progress (zmpm_stage): run its stages to G2P first");
+    return true;
 }
 
 // Error of the allocation-free entry point: a static message, no string copy.
@@ -504,55 +515,78 @@ int gridFor(int64_t items, int per, int maxg)
     return (int)std::min<int64_t>(g, maxg);
 }
 
+// The kernels of one substep stage (zmpm_stage_id); launchStep runs all five.
 template <class T, class AccT>
-void launchStep(zmpm_context* h, hipEvent_t* ev)
+void launchStage(zmpm_context* h, int stage)
 {
     Dev<T, AccT>& D = devOf<T, AccT>(h);
     hipStream_t st = h->stream;
-    const int gp = gridFor(D.n, WG, 1 << 30);
-    if (ev) (void)hipEventRecord(ev[0], st);
-    k_page_scan<T, AccT><<<(int)((D.num_pages + PSCAN_TILE - 1) / PSCAN_TILE), WG, 0, st>>>(D);
-    const int prep = (int)std::min<int64_t>(D.cap_blocks, h->prep_wgs);
-    k_scatter<T, AccT><<<gp + prep + 1, WG, 0, st>>>(D, gp, prep);
-    if (ev) (void)hipEventRecord(ev[1], st);
-    if (D.mls)
-        k_p2g<T, AccT, P2G_MLS><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
-    else
-        k_p2g<T, AccT, P2G_STD><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
-    if (ev) (void)hipEventRecord(ev[2], st);
-    k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
-    if (ev) (void)hipEventRecord(ev[3], st);
-    if (D.implicit && D.any_damage) {
-        // AnisoFractureSimulation::solveDamageSystem (AnisoFractureSimulation.h:435-612)
-        k_dtilde<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
-        // a cooperative launch: the runtime starts it only with every workgroup
-        // resident, which its grid barriers need (work of other streams or
-        // processes cannot leave some of them waiting to start)
-        void* args[] = { &D };
-        const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_cg<T, AccT>), dim3(D.cg_wgs),
-                                                        dim3(CGW), args, 0, st);
-        if (e != hipSuccess) h->launch_err = e;
+    switch (stage) {
+    case ZMPM_STAGE_SORT: {
+        const int gp = gridFor(D.n, WG, 1 << 30);
+        k_page_scan<T, AccT><<<(int)((D.num_pages + PSCAN_TILE - 1) / PSCAN_TILE), WG, 0, st>>>(D);
+        const int prep = (int)std::min<int64_t>(D.cap_blocks, h->prep_wgs);
+        k_scatter<T, AccT><<<gp + prep + 1, WG, 0, st>>>(D, gp, prep);
+        break;
     }
-    if (ev) (void)hipEventRecord(ev[4], st);
-    if (D.flip)
-        k_g2p<T, AccT, true><<<D.max_slots, WG, 0, st>>>(D);
-    else
-        k_g2p<T, AccT, false><<<D.max_slots, WG, 0, st>>>(D);
-    k_rare<T, AccT><<<std::min(D.max_slots, 1024), WG, 0, st>>>(D);
-    std::swap(D.P, D.Q);
-    std::swap(D.pid, D.qid);
-    std::swap(D.pmat, D.qmat);
-    // RPIC / PIC damping passes (MpmSimulationBase.cpp:318-348) on the moved
-    // particles, through this substep's binning
-    const int damping = D.flip ? h->desc.pic_damping_iteration : h->desc.rpic_damping_iteration;
-    constexpr bool FIXED = std::is_same<AccT, long long>::value;
-    for (int it = 0; it < damping; it++) {
-        k_damp_prep<T, AccT><<<gridFor(D.cap_nodes, WG, h->prep_wgs) + (FIXED ? 1 : 0), WG, 0, st>>>(D);
-        k_p2g<T, AccT, P2G_DAMP><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
-        k_damp_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
-        k_damp_g2p<T, AccT><<<D.max_slots, WG, 0, st>>>(D);
+    case ZMPM_STAGE_P2G:
+        if (D.mls)
+            k_p2g<T, AccT, P2G_MLS><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
+        else
+            k_p2g<T, AccT, P2G_STD><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
+        break;
+    case ZMPM_STAGE_GRID_UPDATE:
+        k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
+        break;
+    case ZMPM_STAGE_DAMAGE_SOLVE:
+        if (D.implicit && D.any_damage) {
+            // AnisoFractureSimulation::solveDamageSystem (AnisoFractureSimulation.h:435-612)
+            k_dtilde<T, AccT><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
+            // a cooperative launch: the runtime starts it only with every workgroup
+            // resident, which its grid barriers need (work of other streams or
+            // processes cannot leave some of them waiting to start)
+            void* args[] = { &D };
+            const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_cg<T, AccT>),
+                                                            dim3(D.cg_wgs), dim3(CGW), args, 0, st);
+            if (e != hipSuccess) h->launch_err = e;
+        }
+        break;
+    case ZMPM_STAGE_G2P: {
+        if (D.flip)
+            k_g2p<T, AccT, true><<<D.max_slots, WG, 0, st>>>(D);
+        else
+            k_g2p<T, AccT, false><<<D.max_slots, WG, 0, st>>>(D);
+        k_rare<T, AccT><<<std::min(D.max_slots, 1024), WG, 0, st>>>(D);
+        std::swap(D.P, D.Q);
+        std::swap(D.pid, D.qid);
+        std::swap(D.pmat, D.qmat);
+        // RPIC / PIC damping passes (MpmSimulationBase.cpp:318-348) on the moved
+        // particles, through this substep's binning
+        const int damping = D.flip ? h->desc.pic_damping_iteration : h->desc.rpic_damping_iteration;
+        constexpr bool FIXED = std::is_same<AccT, long long>::value;
+        for (int it = 0; it < damping; it++) {
+            k_damp_prep<T, AccT><<<gridFor(D.cap_nodes, WG, h->prep_wgs) + (FIXED ? 1 : 0), WG, 0, st>>>(D);
+            k_p2g<T, AccT, P2G_DAMP><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
+            k_damp_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
+            k_damp_g2p<T, AccT><<<D.max_slots, WG, 0, st>>>(D);
+        }
+        break;
     }
-    if (ev) (void)hipEventRecord(ev[5], st);
+    default:
+        break;
+    }
+}
+
+// One substep: the five stages, with the timing events between them when given
+// (events 0..5 bracket sort, P2G, grid update, damage solve, G2P).
+template <class T, class AccT>
+void launchStep(zmpm_context* h, hipEvent_t* ev)
+{
+    for (int stage = ZMPM_STAGE_SORT; stage <= ZMPM_STAGE_G2P; stage++) {
+        if (ev) (void)hipEventRecord(ev[stage], h->stream);
+        launchStage<T, AccT>(h, stage);
+    }
+    if (ev) (void)hipEventRecord(ev[5], h->stream);
 }
 
 zmpm_status deviceStatus(zmpm_context* h, bool sync)
@@ -741,7 +775,8 @@ zmpm_status binningT(zmpm_context* h, int64_t* n_groups, uint64_t* keys, int32_t
     constexpr int bbits = B::xbits + B::ybits + B::zbits;
     constexpr int NB = 1 << bbits;
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (!h->stepped) {
+    // the in-block order is complete after a substep's P2G
+    if (!h->stepped && h->stage_next <= ZMPM_STAGE_P2G) {
         *n_groups = 0;
         return ZMPM_OK;
     }
@@ -783,34 +818,52 @@ zmpm_status binningT(zmpm_context* h, int64_t* n_groups, uint64_t* keys, int32_t
     return ZMPM_OK;
 }
 
+// The grid of the substep in progress or last run, on the host: the nodes with
+// mass ordered by (x, y, z) node coordinates (the order zmpm_get_grid reports).
 template <class T, class AccT>
-zmpm_status gridT(zmpm_context* h, int64_t* n_nodes, int32_t* coords, double* m, double* v, double* new_v, double* pf)
+struct HostGrid {
+    struct Rec {
+        uint32_t c[3];
+        int64_t node;
+        int scene;
+    };
+    int64_t nn = 0;
+    std::vector<AccT> acc;
+    std::vector<T> nv, pfn;
+    std::vector<float> sc;
+    std::vector<Rec> recs;
+    double val(int f, const Rec& r, int g) const
+    {
+        if (std::is_same<AccT, long long>::value) return (double)acc[f * nn + r.node] / (double)sc[(size_t)r.scene * NSC + g];
+        return (double)acc[f * nn + r.node];
+    }
+};
+
+template <class T, class AccT>
+zmpm_status collectGrid(zmpm_context* h, HostGrid<T, AccT>& G)
 {
     Dev<T, AccT>& D = devOf<T, AccT>(h);
     using B = Blk<T>;
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (!h->stepped) {
-        *n_nodes = 0;
-        return ZMPM_OK;
-    }
     int nb = 0;
     HIPCHK(h, hipMemcpy(&nb, D.counters, sizeof(int), hipMemcpyDeviceToHost));
     const int64_t nn = (int64_t)nb * NB;
+    G.nn = nn;
     std::vector<int> bpage(nb);
-    std::vector<AccT> acc((size_t)7 * nn);
-    std::vector<T> nv((size_t)3 * nn), pfn(nn);
+    G.acc.assign((size_t)7 * nn, AccT(0));
+    G.nv.assign((size_t)3 * nn, T(0));
+    G.pfn.assign((size_t)nn, T(0));
     const size_t ns = h->scenes.size();
-    std::vector<float> sc(NSC * ns, 1.f);
+    G.sc.assign(NSC * ns, 1.f);
     HIPCHK(h, hipMemcpy(bpage.data(), D.block_page, nb * sizeof(int), hipMemcpyDeviceToHost));
     for (int f = 0; f < 7; f++)
-        HIPCHK(h, hipMemcpy(acc.data() + f * nn, D.acc + f * D.cap_nodes, nn * sizeof(AccT), hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(G.acc.data() + f * nn, D.acc + f * D.cap_nodes, nn * sizeof(AccT), hipMemcpyDeviceToHost));
     for (int a = 0; a < 3; a++)
-        HIPCHK(h, hipMemcpy(nv.data() + a * nn, D.newv + a * D.cap_nodes, nn * sizeof(T), hipMemcpyDeviceToHost));
-    if (h->any_damage) HIPCHK(h, hipMemcpy(pfn.data(), D.pfn, nn * sizeof(T), hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(G.nv.data() + a * nn, D.newv + a * D.cap_nodes, nn * sizeof(T), hipMemcpyDeviceToHost));
+    if (h->any_damage) HIPCHK(h, hipMemcpy(G.pfn.data(), D.pfn, nn * sizeof(T), hipMemcpyDeviceToHost));
     // scales of the last substep's P2G (they are replaced only at the start of the next one)
     if (std::is_same<AccT, long long>::value)
-        HIPCHK(h, hipMemcpy(sc.data(), D.scales, sc.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(G.sc.data(), D.scales, G.sc.size() * sizeof(float), hipMemcpyDeviceToHost));
     // scene of a node from its region
     auto sceneAt = [&](const uint32_t* c) -> int {
         const int rd = h->region_dim;
@@ -818,42 +871,67 @@ zmpm_status gridT(zmpm_context* h, int64_t* n_nodes, int32_t* coords, double* m,
         if (r0 >= rd || r1 >= rd || r2 >= rd) return 0;
         return std::max(0, h->region_scene[(size_t)((r0 * rd + r1) * rd + r2)]);
     };
-    struct Rec {
-        uint32_t c[3];
-        int64_t node;
-        int scene;
-    };
-    auto val = [&](int f, const Rec& r, int g) -> double {
-        if (std::is_same<AccT, long long>::value) return (double)acc[f * nn + r.node] / (double)sc[(size_t)r.scene * NSC + g];
-        return (double)acc[f * nn + r.node];
-    };
-    std::vector<Rec> recs;
+    using Rec = typename HostGrid<T, AccT>::Rec;
+    G.recs.clear();
     for (int64_t node = 0; node < nn; node++) {
         const int b = (int)(node / NB), e = (int)(node % NB);
         uint64_t off = ((uint64_t)(uint32_t)bpage[b] << 12) | ((uint64_t)e << B::data_bits);
         Rec r { { hostBitPack(off, D.masks.x), hostBitPack(off, D.masks.y), hostBitPack(off, D.masks.z) }, node, 0 };
         r.scene = sceneAt(r.c);
-        if (val(GM, r, SC_M) == 0) continue;
-        recs.push_back(r);
+        if (G.val(GM, r, SC_M) == 0) continue;
+        G.recs.push_back(r);
     }
-    std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) {
+    std::sort(G.recs.begin(), G.recs.end(), [](const Rec& a, const Rec& b) {
         if (a.c[0] != b.c[0]) return a.c[0] < b.c[0];
         if (a.c[1] != b.c[1]) return a.c[1] < b.c[1];
         return a.c[2] < b.c[2];
     });
-    *n_nodes = (int64_t)recs.size();
-    for (size_t q = 0; q < recs.size(); q++) {
-        const int64_t node = recs[q].node;
-        const double mm = val(GM, recs[q], SC_M);
+    return ZMPM_OK;
+}
+
+template <class T, class AccT>
+zmpm_status gridT(zmpm_context* h, int64_t* n_nodes, int32_t* coords, double* m, double* v, double* new_v, double* pf)
+{
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    // a grid exists after a substep's P2G (a whole substep or its P2G stage)
+    if (!h->stepped && h->stage_next <= ZMPM_STAGE_P2G) {
+        *n_nodes = 0;
+        return ZMPM_OK;
+    }
+    HostGrid<T, AccT> G;
+    const zmpm_status st = collectGrid<T, AccT>(h, G);
+    if (st != ZMPM_OK) return st;
+    const int64_t nn = G.nn;
+    *n_nodes = (int64_t)G.recs.size();
+    for (size_t q = 0; q < G.recs.size(); q++) {
+        const int64_t node = G.recs[q].node;
+        const double mm = G.val(GM, G.recs[q], SC_M);
         if (coords)
-            for (int a = 0; a < 3; a++) coords[3 * q + a] = (int32_t)recs[q].c[a];
+            for (int a = 0; a < 3; a++) coords[3 * q + a] = (int32_t)G.recs[q].c[a];
         if (m) m[q] = mm;
         if (v)
-            for (int a = 0; a < 3; a++) v[3 * q + a] = val(GMV + a, recs[q], SC_MV) / mm;
+            for (int a = 0; a < 3; a++) v[3 * q + a] = G.val(GMV + a, G.recs[q], SC_MV) / mm;
         if (new_v)
-            for (int a = 0; a < 3; a++) new_v[3 * q + a] = (double)nv[a * nn + node];
-        if (pf) pf[q] = h->any_damage ? (double)pfn[node] : 0.0;
+            for (int a = 0; a < 3; a++) new_v[3 * q + a] = (double)G.nv[a * nn + node];
+        if (pf) pf[q] = h->any_damage ? (double)G.pfn[node] : 0.0;
     }
+    return ZMPM_OK;
+}
+
+template <class T, class AccT>
+zmpm_status setGridVelocityT(zmpm_context* h, int64_t n_nodes, const double* new_v)
+{
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HostGrid<T, AccT> G;
+    const zmpm_status st = collectGrid<T, AccT>(h, G);
+    if (st != ZMPM_OK) return st;
+    if (n_nodes != (int64_t)G.recs.size()) return fail(h, ZMPM_E_ARG, "node count differs from zmpm_get_grid's");
+    const int64_t nn = G.nn;
+    for (size_t q = 0; q < G.recs.size(); q++)
+        for (int a = 0; a < 3; a++) G.nv[a * nn + G.recs[q].node] = (T)new_v[3 * q + a];
+    Dev<T, AccT>& D = devOf<T, AccT>(h);
+    for (int a = 0; a < 3; a++)
+        HIPCHK(h, hipMemcpy(D.newv + a * D.cap_nodes, G.nv.data() + a * nn, nn * sizeof(T), hipMemcpyHostToDevice));
     return ZMPM_OK;
 }
 
@@ -1065,6 +1143,7 @@ const char* zmpm_last_error(zmpm_handle h)
 zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material* mats)
 {
     if (!h) return ZMPM_E_ARG;
+    if (midSubstep(h)) return ZMPM_E_ARG;
     if (count <= 0 || count > kMaxMaterials || !mats) return fail(h, ZMPM_E_ARG, "bad material table");
     for (int i = 0; i < count; i++) {
         if (mats[i].model < 0 || mats[i].model > ZMPM_MODEL_NEO_HOOKEAN_BORDEN)
@@ -1106,6 +1185,7 @@ zmpm_status zmpm_set_materials(zmpm_handle h, int32_t count, const zmpm_material
 zmpm_status zmpm_add_body_force(zmpm_handle h, int32_t scene, double scale, double frequency, const double* direction)
 {
     if (!h) return ZMPM_E_ARG;
+    if (midSubstep(h)) return ZMPM_E_ARG;
     if (scene < 0 || scene >= (int32_t)h->scenes.size()) return fail(h, ZMPM_E_ARG, "no such scene");
     if (!direction || !std::isfinite(scale) || !std::isfinite(frequency)) return fail(h, ZMPM_E_ARG, "bad body force");
     auto& list = h->body_forces[(size_t)scene];
@@ -1123,6 +1203,7 @@ zmpm_status zmpm_add_body_force(zmpm_handle h, int32_t scene, double scale, doub
 zmpm_status zmpm_set_velocity_field(zmpm_handle h, int32_t scene, const double* a, const double* B)
 {
     if (!h) return ZMPM_E_ARG;
+    if (midSubstep(h)) return ZMPM_E_ARG;
     if (scene < 0 || scene >= (int32_t)h->scenes.size()) return fail(h, ZMPM_E_ARG, "no such scene");
     if (a && !B) return fail(h, ZMPM_E_ARG, "the velocity field needs B");
     (void)hipSetDevice(h->desc.device);
@@ -1141,6 +1222,7 @@ zmpm_status zmpm_set_velocity_field(zmpm_handle h, int32_t scene, const double* 
 zmpm_status zmpm_set_fj_state(zmpm_handle h, int32_t state)
 {
     if (!h) return ZMPM_E_ARG;
+    if (midSubstep(h)) return ZMPM_E_ARG;
     if (state != 0 && state != 1) return fail(h, ZMPM_E_ARG, "FJ state is 0 (SOLVE_F) or 1 (SOLVE_J)");
     if (state == h->fj_state) return ZMPM_OK;
     (void)hipSetDevice(h->desc.device);
@@ -1158,6 +1240,7 @@ zmpm_status zmpm_set_fj_state(zmpm_handle h, int32_t state)
 zmpm_status zmpm_add_scene_collider(zmpm_handle h, int32_t scene, const zmpm_collider* c, int32_t* id_out)
 {
     if (!h || !c) return ZMPM_E_ARG;
+    if (midSubstep(h)) return ZMPM_E_ARG;
     if (scene < 0 || scene >= (int32_t)h->scenes.size()) return fail(h, ZMPM_E_ARG, "no such scene");
     auto& cols = h->scene_cols[(size_t)scene];
     if ((int)cols.size() >= kMaxColliders) return fail(h, ZMPM_E_ARG, "too many colliders");
@@ -1198,6 +1281,7 @@ zmpm_status zmpm_clear_colliders(zmpm_handle h)
 static zmpm_status uploadScenesParticles(zmpm_handle h, const zmpm_particles* p, const int32_t* scene)
 {
     if (!h || !p || p->n < 0) return ZMPM_E_ARG;
+    if (midSubstep(h)) return ZMPM_E_ARG;
     if (p->n > 0 && (!p->x || !p->v || !p->C || !p->F || !p->m || !p->V0 || !p->material))
         return fail(h, ZMPM_E_ARG, "missing particle array");
     if (h->mats.empty()) return fail(h, ZMPM_E_ARG, "set materials before upload");
@@ -1269,10 +1353,9 @@ static zmpm_status graphFor(zmpm_handle h, hipGraphExec_t* out)
 
 // `steps` substeps of every scene: scene s advances by dt[s] per substep from
 // t0[s] (its device clock); the scene table is refreshed when a dt changes.
-static zmpm_status runSteps(zmpm_handle h, int32_t steps, const double* dt, const double* t0)
+// Every scene's step dt[s] in the scene table and its device clock at t0[s].
+static zmpm_status setClocks(zmpm_handle h, const double* dt, const double* t0)
 {
-    if (h->n == 0) return ZMPM_OK;
-    (void)hipSetDevice(h->desc.device);
     const size_t ns = h->scenes.size();
     bool dt_changed = false;
     for (size_t s = 0; s < ns; s++)
@@ -1284,6 +1367,20 @@ static zmpm_status runSteps(zmpm_handle h, int32_t steps, const double* dt, cons
         const zmpm_status st = h->fp32 ? uploadScenes<float>(h) : uploadScenes<double>(h);
         if (st != ZMPM_OK) return st;
     }
+    // each scene's device clock starts at its t0; every substep advances it by its dt
+    h->clock_host.assign(2 * ns, 0.0);
+    for (size_t s = 0; s < ns; s++) h->clock_host[2 * s] = h->clock_host[2 * s + 1] = t0[s];
+    double* clock = h->fp32 ? h->Df.clock : h->Dd.clock;
+    if (hipMemcpyAsync(clock, h->clock_host.data(), 2 * ns * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        return fail(h, ZMPM_E_HIP, "clock upload");
+    return ZMPM_OK;
+}
+
+static zmpm_status runSteps(zmpm_handle h, int32_t steps, const double* dt, const double* t0)
+{
+    if (midSubstep(h)) return ZMPM_E_ARG;
+    if (h->n == 0) return ZMPM_OK;
+    (void)hipSetDevice(h->desc.device);
     hipEvent_t* ev = nullptr;
     if (h->timing) {
         const size_t need = (size_t)steps * (kTimedGroups + 1);
@@ -1293,12 +1390,10 @@ static zmpm_status runSteps(zmpm_handle h, int32_t steps, const double* dt, cons
             h->events.push_back(e);
         }
     }
-    // each scene's device clock starts at its t0; every substep advances it by its dt
-    h->clock_host.assign(2 * ns, 0.0);
-    for (size_t s = 0; s < ns; s++) h->clock_host[2 * s] = h->clock_host[2 * s + 1] = t0[s];
-    double* clock = h->fp32 ? h->Df.clock : h->Dd.clock;
-    if (hipMemcpyAsync(clock, h->clock_host.data(), 2 * ns * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess)
-        return fail(h, ZMPM_E_HIP, "clock upload");
+    {
+        const zmpm_status st = setClocks(h, dt, t0);
+        if (st != ZMPM_OK) return st;
+    }
     int32_t s = 0;
     // substeps with the implicit damage solve run eagerly: its cooperative
     // launch is not captured into graphs
@@ -1358,6 +1453,52 @@ zmpm_status zmpm_run(zmpm_handle h, int32_t steps, double dt, double t0)
 {
     if (!h || steps < 0 || !(dt > 0)) return ZMPM_E_ARG;
     return runUniform(h, steps, dt, t0);
+}
+
+zmpm_status zmpm_stage(zmpm_handle h, int32_t stage, double dt, double t)
+{
+    if (!h || stage < ZMPM_STAGE_SORT || stage > ZMPM_STAGE_G2P || !(dt > 0)) return ZMPM_E_ARG;
+    if (h->scenes.size() != 1) return fail(h, ZMPM_E_ARG, "stages run on a single-scene handle");
+    if (stage != h->stage_next)
+        return fail(h, ZMPM_E_ARG, "stages run in order: SORT, P2G, GRID_UPDATE, DAMAGE_SOLVE, G2P");
+    if (stage == ZMPM_STAGE_SORT) {
+        h->stage_dt = dt;
+        h->stage_t = t;
+    }
+    else if (dt != h->stage_dt || t != h->stage_t)
+        return fail(h, ZMPM_E_ARG, "the stages of a substep share its dt and t");
+    h->stage_next = stage == ZMPM_STAGE_G2P ? 0 : stage + 1;
+    if (h->n == 0) return ZMPM_OK;
+    (void)hipSetDevice(h->desc.device);
+    if (stage == ZMPM_STAGE_SORT) {
+        const zmpm_status st = setClocks(h, &dt, &t);
+        if (st != ZMPM_OK) {
+            h->stage_next = 0;
+            return st;
+        }
+    }
+    if (h->fp32)
+        launchStage<float, long long>(h, stage);
+    else
+        launchStage<double, double>(h, stage);
+    if (stage == ZMPM_STAGE_G2P) h->stepped = true;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = h->launch_err;
+    h->launch_err = hipSuccess;
+    if (e != hipSuccess) {
+        h->stage_next = 0;
+        return fail(h, ZMPM_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    }
+    return ZMPM_OK;
+}
+
+zmpm_status zmpm_set_grid_velocity(zmpm_handle h, int64_t n_nodes, const double* new_v)
+{
+    if (!h || !new_v) return ZMPM_E_ARG;
+    if (h->stage_next != ZMPM_STAGE_DAMAGE_SOLVE && h->stage_next != ZMPM_STAGE_G2P)
+        return fail(h, ZMPM_E_ARG, "the grid velocities are replaced between the GRID_UPDATE and G2P stages");
+    (void)hipSetDevice(h->desc.device);
+    return h->fp32 ? setGridVelocityT<float, long long>(h, n_nodes, new_v) : setGridVelocityT<double, double>(h, n_nodes, new_v);
 }
 
 zmpm_status zmpm_run_batch(zmpm_handle h, int32_t steps, const double* dt, const double* t0)
@@ -1493,6 +1634,7 @@ zmpm_status zmpm_get_launch_config(zmpm_handle h, zmpm_launch_config* cfg)
 zmpm_status zmpm_set_launch_config(zmpm_handle h, const zmpm_launch_config* cfg)
 {
     if (!h || !cfg) return ZMPM_E_ARG;
+    if (midSubstep(h)) return ZMPM_E_ARG;
     if (cfg->p2g_wgs < 1 || cfg->grid_wgs < 1 || cfg->prep_wgs < 1 || cfg->p2g_split < 1)
         return fail(h, ZMPM_E_ARG, "launch configuration entries must be >= 1");
     (void)hipSetDevice(h->desc.device);
@@ -1512,6 +1654,7 @@ zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, in
                           zmpm_launch_config* cfg)
 {
     if (!h || !(dt > 0) || steps < 1) return ZMPM_E_ARG;
+    if (midSubstep(h)) return ZMPM_E_ARG;
     if (h->n == 0) return cfg ? zmpm_get_launch_config(h, cfg) : ZMPM_OK;
     (void)hipSetDevice(h->desc.device);
     // snapshot of the state so tuning leaves the simulation where it was
