@@ -445,33 +445,51 @@ zmpm_status zmpm_calculate_dt(zmpm_handle h, double max_dt, double* dt, double* 
  * overlap on one GPU -- other handles' streams, other threads -- race. */
 zmpm_status zmpm_calculate_dt_noalloc(zmpm_handle h, double max_dt, int32_t reduction_wgs, double* out);
 
-/* Launch configuration of the grid-stride kernels (workgroups of 256 threads)
- * and the P2G lane split.  The workgroup counts do not change any result; the
- * P2G split cap (most lanes sharing one stencil column of a cell) changes how
- * a cell's particles are grouped into float partial sums before the 64-bit
- * fixed-point accumulation (results deterministic for a given value).
+/* Launch configuration of the grid-stride kernels (workgroups of 256 threads),
+ * the compiled kernel variants and the P2G lane split.  The workgroup counts
+ * and the variants do not change any result; the P2G split cap (most lanes
+ * sharing one stencil column of a cell) changes how a cell's particles are
+ * grouped into float partial sums before the 64-bit fixed-point accumulation
+ * (results deterministic for a given value; the autotuner never changes it).
  * No counterpart in the reference (launch configuration of this library). */
 typedef struct {
     int32_t p2g_wgs;     /* in-block sort + P2G: workgroups over grid blocks      */
     int32_t grid_wgs;    /* grid update: workgroups over nodes                    */
     int32_t prep_wgs;    /* accumulator zeroing / neighbour table                 */
     int32_t p2g_split;   /* P2G lanes per stencil column, cap (>= 1)              */
-    int32_t reserved[4];
+    int32_t variant;     /* compiled variants, bits: 1 = P2G built for one wave per
+                            SIMD fewer, 2 = G2P segments round-robin over the XCDs,
+                            4 = P2G blocks XCD by XCD                              */
+    int32_t reserved[3];
 } zmpm_launch_config;
 
 zmpm_status zmpm_get_launch_config(zmpm_handle h, zmpm_launch_config* cfg);
 zmpm_status zmpm_set_launch_config(zmpm_handle h, const zmpm_launch_config* cfg);
 
 /* Autotuner: times candidate workgroup counts for each grid-stride kernel on
- * the uploaded scene (coordinate descent, `steps` event-timed substeps per
- * candidate, colliders at times from t0) and keeps the fastest; with
- * tune_split != 0 also the P2G split cap.  The particle state is saved before
- * and restored after, so the simulation continues from where it was (the grid
- * and binning queries then describe the tuner's last substep).  The chosen
- * configuration is returned in *cfg (may be NULL).
+ * the uploaded scene (coordinate descent, each candidate over two rounds of
+ * `steps` event-timed substeps, its faster round kept, colliders at times from
+ * t0) and keeps the fastest; with tune_variants != 0 also the compiled kernel
+ * variants.  Nothing it changes changes a result.  The particle state is
+ * saved before and restored after, so the simulation continues from where it
+ * was (the grid and binning queries then describe the tuner's last substep).
+ * The chosen configuration is returned in *cfg (may be NULL).  In a batch
+ * every scene steps with dt while tuning.
  * No counterpart in the reference (launch configuration of this library). */
-zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, int32_t tune_split,
+zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, int32_t tune_variants,
                           zmpm_launch_config* cfg);
+
+/* zmpm_autotune with a cache file of tuned configurations, one text line per
+ * (device, workload) key: the device's gfx target and compute-unit count, the
+ * precision, the number of scenes, the particle count rounded to a quarter
+ * power of two, and the options that select kernels (damage, implicit solve,
+ * transfer scheme, MLS, damping, FJ-mixed, plasticity).  A key found in the
+ * file applies its configuration without timing (*from_cache = 1); otherwise
+ * the tuner runs and appends its result.  The file is created when missing;
+ * processes tuning the same file at once may both append (the first line of
+ * a key is the one read). */
+zmpm_status zmpm_autotune_cached(zmpm_handle h, double dt, double t0, int32_t steps, int32_t tune_variants,
+                                 const char* cache_path, zmpm_launch_config* cfg, int32_t* from_cache);
 
 #ifdef __cplusplus
 }

@@ -1192,7 +1192,7 @@ struct zmpm_context {
     zmpm_scene_desc desc;
     Layout L;
     std::string err;
-    zmpm_launch_config launch { 1, 1, 1, 1, { 0, 0, 0, 0 } };
+    zmpm_launch_config launch { 1, 1, 1, 1, 0, { 0, 0, 0 } };
     std::vector<zmpm_material> mats;
     std::vector<Collider> colliders;
     bool any_damage = false;
@@ -2652,6 +2652,16 @@ zmpm_status zmpm_autotune(zmpm_handle h, double dt, double, int32_t steps, int32
     if (!h || !(dt > 0) || steps < 1) return ZMPM_E_ARG;
     if (cfg) *cfg = h->launch;
     return ZMPM_OK;
+}
+
+// The oracle has nothing to tune or cache: the configuration it keeps, never
+// from a cache file.
+zmpm_status zmpm_autotune_cached(zmpm_handle h, double dt, double t0, int32_t steps, int32_t tune_variants,
+                                 const char* cache_path, zmpm_launch_config* cfg, int32_t* from_cache)
+{
+    if (!cache_path) return ZMPM_E_ARG;
+    if (from_cache) *from_cache = 0;
+    return zmpm_autotune(h, dt, t0, steps, tune_variants, cfg);
 }
 
 // ------------------------------------------------ unit entry points (pinning)

@@ -89,3 +89,14 @@ def test_library_loads_without_gpu_and_fails_loudly():
     with pytest.raises(A.ZmpmError) as e:
         A.Simulation(lib, dx=0.01)
     assert e.value.status == A.E_HIP
+
+
+def test_oracle_autotune_cached_keeps_its_configuration(oracle, tmp_path):
+    # the oracle has no launches: the cached tuner returns its configuration and
+    # never reports a cache hit (nor writes the file)
+    from ziran2020_amd import scenes
+    sc = scenes.elastic_cube(n_side_cells=2)
+    sim = sc.make(oracle)
+    cfg, hit = sim.autotune_cached(str(tmp_path / "t.txt"), sc.dt)
+    assert not hit and cfg == sim.launch_config()
+    assert not (tmp_path / "t.txt").exists()

@@ -614,15 +614,47 @@ def test_material_table_must_cover_uploaded_particles(gpu):
     sim.set_materials(sc.materials[::-1])
 
 
-def test_launch_config_does_not_change_results(gpu):
-    # the workgroup counts of the grid-stride kernels are free: FP32 bit-exact
+@pytest.mark.parametrize("variant", range(8))
+def test_launch_config_does_not_change_results(gpu, variant):
+    # everything the autotuner may change is free: the workgroup counts of the
+    # grid-stride kernels and every compiled variant (P2G occupancy and block
+    # order, G2P segment order) leave the FP32 form bit-exact
     sc = small_slab(n=8000)
     a = _fields(run(gpu, sc, A.PREC_FP32, 4).download())
     sim = sc.make(gpu, A.PREC_FP32)
-    sim.set_launch_config(p2g_wgs=37, grid_wgs=5, prep_wgs=3)
-    assert sim.launch_config()["p2g_wgs"] == 37
+    sim.set_launch_config(p2g_wgs=37, grid_wgs=5, prep_wgs=3, variant=variant)
+    assert sim.launch_config()["p2g_wgs"] == 37 and sim.launch_config()["variant"] == variant
     sim.run(4, sc.dt, 0.0)
     b = _fields(sim.download())
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_launch_config_rejects_unknown_variants(gpu):
+    sim = small_slab(n=2000).make(gpu, A.PREC_FP32)
+    with pytest.raises(A.ZmpmError):
+        sim.set_launch_config(variant=8)
+
+
+def test_autotune_cache_file(gpu, tmp_path):
+    # the first tuning of a workload writes its line; a second handle of the same
+    # workload reads it back without timing; another precision is another key
+    path = str(tmp_path / "tune.txt")
+    sc = small_slab(n=8000)
+    s1 = sc.make(gpu, A.PREC_FP32)
+    c1, hit1 = s1.autotune_cached(path, sc.dt, steps=1)
+    s2 = sc.make(gpu, A.PREC_FP32)
+    c2, hit2 = s2.autotune_cached(path, sc.dt, steps=1)
+    assert not hit1 and hit2 and c1 == c2 and s2.launch_config() == c1
+    s3 = sc.make(gpu, A.PREC_FP64)
+    _, hit3 = s3.autotune_cached(path, sc.dt, steps=1)
+    assert not hit3
+    lines = open(path).read().splitlines()
+    assert len(lines) == 2 and "gfx950" in lines[0] and "/fp32/" in lines[0] and "/fp64/" in lines[1]
+    # a cached configuration leaves the results as they are
+    s2.run(3, sc.dt, 0.0)
+    b = _fields(s2.download())
+    a = _fields(run(gpu, sc, A.PREC_FP32, 3).download())
     for k in a:
         assert np.array_equal(a[k], b[k]), k
 
@@ -632,7 +664,7 @@ def test_autotune_restores_the_state(gpu):
     sc = small_slab(n=8000)
     sim = sc.make(gpu, A.PREC_FP32)
     cfg = sim.autotune(sc.dt, 0.0, steps=2)
-    assert all(v >= 1 for v in cfg.values())
+    assert all(v >= 1 for k, v in cfg.items() if k != "variant") and 0 <= cfg["variant"] <= 7
     sim.run(4, sc.dt, 0.0)
     a = _fields(sim.download())
     b = _fields(run(gpu, sc, A.PREC_FP32, 4).download())

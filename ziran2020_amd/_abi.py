@@ -237,10 +237,14 @@ def capped_cylinder(radius, height, shape_rotation=(1, 0, 0, 0), center=(0, 0, 0
 class LaunchConfig(C.Structure):
     """zmpm_launch_config (include/zmpm.h)."""
     _fields_ = [("p2g_wgs", C.c_int32), ("grid_wgs", C.c_int32),
-                ("prep_wgs", C.c_int32), ("p2g_split", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("prep_wgs", C.c_int32), ("p2g_split", C.c_int32), ("variant", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
     def as_dict(self):
-        return {k: int(getattr(self, k)) for k in ("p2g_wgs", "grid_wgs", "prep_wgs", "p2g_split")}
+        return {k: int(getattr(self, k)) for k in ("p2g_wgs", "grid_wgs", "prep_wgs", "p2g_split", "variant")}
+
+
+VARIANT_P2G_LOW_OCC, VARIANT_G2P_RR, VARIANT_P2G_XCD = 1, 2, 4
 
 
 class ZmpmLib:
@@ -254,7 +258,7 @@ class ZmpmLib:
         "zmpm_set_launch_config", "zmpm_autotune", "zmpm_create_batch", "zmpm_batch_origin",
         "zmpm_add_scene_collider", "zmpm_upload_batch", "zmpm_run_batch", "zmpm_calculate_dt_noalloc",
         "zmpm_set_fj_state", "zmpm_add_body_force", "zmpm_set_velocity_field", "zmpm_stage",
-        "zmpm_set_grid_velocity",
+        "zmpm_set_grid_velocity", "zmpm_autotune_cached",
     )
 
     def __init__(self, path: str):
@@ -303,6 +307,8 @@ class ZmpmLib:
         L.zmpm_set_velocity_field.argtypes = [C.c_void_p, C.c_int32, _DP, _DP]
         L.zmpm_stage.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double]
         L.zmpm_set_grid_velocity.argtypes = [C.c_void_p, C.c_int64, _DP]
+        L.zmpm_autotune_cached.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_char_p,
+                                           C.POINTER(LaunchConfig), C.POINTER(C.c_int32)]
         if L.zmpm_abi_version() != ZMPM_ABI_VERSION:
             raise RuntimeError(f"{path}: ABI version {L.zmpm_abi_version()} != {ZMPM_ABI_VERSION}")
 
@@ -499,11 +505,21 @@ class Simulation:
             setattr(cfg, k, int(v))
         self._check(self.L.zmpm_set_launch_config(self.h, C.byref(cfg)), "zmpm_set_launch_config")
 
-    def autotune(self, dt, t0=0.0, steps=4, tune_split=False):
-        """Tune the launch configuration on this scene (advances the simulation)."""
+    def autotune(self, dt, t0=0.0, steps=4, variants=True):
+        """Tune the launch configuration on this scene (the state is restored)."""
         cfg = LaunchConfig()
-        self._check(self.L.zmpm_autotune(self.h, dt, t0, steps, int(bool(tune_split)), C.byref(cfg)), "zmpm_autotune")
+        self._check(self.L.zmpm_autotune(self.h, dt, t0, steps, int(bool(variants)), C.byref(cfg)), "zmpm_autotune")
         return cfg.as_dict()
+
+    def autotune_cached(self, cache_path, dt, t0=0.0, steps=4, variants=True):
+        """autotune through a cache file keyed by device and workload; returns
+        (configuration, whether it came from the file)."""
+        cfg = LaunchConfig()
+        hit = C.c_int32(0)
+        self._check(self.L.zmpm_autotune_cached(self.h, dt, t0, steps, int(bool(variants)),
+                                                os.fsencode(cache_path), C.byref(cfg), C.byref(hit)),
+                    "zmpm_autotune_cached")
+        return cfg.as_dict(), bool(hit.value)
 
     def timings(self):
         cnt = C.c_int32(64)

@@ -8,6 +8,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <limits>
 #include <mutex>
 #include <string>
@@ -29,6 +31,11 @@ namespace {
 constexpr int kMaxMaterials = 256;
 constexpr int kMaxColliders = MAX_COLLIDERS;
 constexpr int kTimedGroups = 5;
+// compiled kernel variants the launch configuration selects
+constexpr int VARIANT_P2G_LOW_OCC = 1; // P2G compiled for one wave per SIMD fewer
+constexpr int VARIANT_G2P_RR = 2;      // G2P segments round-robin over the XCDs
+constexpr int VARIANT_P2G_XCD = 4;     // P2G blocks XCD by XCD
+constexpr int VARIANT_ALL = 7;
 const char* const kTimedNames[kTimedGroups + 1] = { "bin_sort", "p2g", "grid_update", "damage_solve", "g2p", "substep" };
 
 uint64_t spreadMask(int log2_struct, int axis)
@@ -120,6 +127,7 @@ struct zmpm_context {
     int p2g_split = 64;
     // workgroups of the grid-stride launches (results do not depend on them)
     int p2g_wgs = 2048, grid_wgs = 2048, prep_wgs = 2048;
+    int variant = 0; // compiled kernel variants (VARIANT_* bits), none changes a result
 };
 
 namespace {
@@ -515,6 +523,26 @@ int gridFor(int64_t items, int per, int maxg)
     return (int)std::min<int64_t>(g, maxg);
 }
 
+// P2G of the substep: the MLS form, or the standard form in the compiled
+// variant the launch configuration selects (VARIANT_* bits).
+template <class T, class AccT>
+void launchP2G(zmpm_context* h, const Dev<T, AccT>& D, hipStream_t st)
+{
+    const int wgs = (int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs);
+    constexpr int OCC = P2GWaves<T>::value;
+    const int v = h->variant;
+    if (D.mls)
+        k_p2g<T, AccT, P2G_MLS><<<wgs, WG, 0, st>>>(D);
+    else if ((v & VARIANT_P2G_LOW_OCC) && (v & VARIANT_P2G_XCD))
+        k_p2g<T, AccT, P2G_STD, OCC - 1, true><<<wgs, WG, 0, st>>>(D);
+    else if (v & VARIANT_P2G_LOW_OCC)
+        k_p2g<T, AccT, P2G_STD, OCC - 1, false><<<wgs, WG, 0, st>>>(D);
+    else if (v & VARIANT_P2G_XCD)
+        k_p2g<T, AccT, P2G_STD, OCC, true><<<wgs, WG, 0, st>>>(D);
+    else
+        k_p2g<T, AccT, P2G_STD><<<wgs, WG, 0, st>>>(D);
+}
+
 // The kernels of one substep stage (zmpm_stage_id); launchStep runs all five.
 template <class T, class AccT>
 void launchStage(zmpm_context* h, int stage)
@@ -530,10 +558,7 @@ void launchStage(zmpm_context* h, int stage)
         break;
     }
     case ZMPM_STAGE_P2G:
-        if (D.mls)
-            k_p2g<T, AccT, P2G_MLS><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
-        else
-            k_p2g<T, AccT, P2G_STD><<<(int)std::min<int64_t>(D.cap_blocks, h->p2g_wgs), WG, 0, st>>>(D);
+        launchP2G<T, AccT>(h, D, st);
         break;
     case ZMPM_STAGE_GRID_UPDATE:
         k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
@@ -554,6 +579,8 @@ void launchStage(zmpm_context* h, int stage)
     case ZMPM_STAGE_G2P: {
         if (D.flip)
             k_g2p<T, AccT, true><<<D.max_slots, WG, 0, st>>>(D);
+        else if (h->variant & VARIANT_G2P_RR)
+            k_g2p<T, AccT, false, true><<<D.max_slots, WG, 0, st>>>(D);
         else
             k_g2p<T, AccT, false><<<D.max_slots, WG, 0, st>>>(D);
         k_rare<T, AccT><<<std::min(D.max_slots, 1024), WG, 0, st>>>(D);
@@ -1628,6 +1655,7 @@ zmpm_status zmpm_get_launch_config(zmpm_handle h, zmpm_launch_config* cfg)
     cfg->grid_wgs = h->grid_wgs;
     cfg->prep_wgs = h->prep_wgs;
     cfg->p2g_split = h->p2g_split;
+    cfg->variant = h->variant;
     return ZMPM_OK;
 }
 
@@ -1637,12 +1665,14 @@ zmpm_status zmpm_set_launch_config(zmpm_handle h, const zmpm_launch_config* cfg)
     if (midSubstep(h)) return ZMPM_E_ARG;
     if (cfg->p2g_wgs < 1 || cfg->grid_wgs < 1 || cfg->prep_wgs < 1 || cfg->p2g_split < 1)
         return fail(h, ZMPM_E_ARG, "launch configuration entries must be >= 1");
+    if (cfg->variant < 0 || cfg->variant > VARIANT_ALL) return fail(h, ZMPM_E_ARG, "unknown kernel variant bits");
     (void)hipSetDevice(h->desc.device);
     dropGraphs(h);
     h->p2g_wgs = cfg->p2g_wgs;
     h->grid_wgs = cfg->grid_wgs;
     h->prep_wgs = cfg->prep_wgs;
     h->p2g_split = cfg->p2g_split;
+    h->variant = cfg->variant;
     if (h->fp32)
         refreshTables<float, long long>(h);
     else
@@ -1650,7 +1680,7 @@ zmpm_status zmpm_set_launch_config(zmpm_handle h, const zmpm_launch_config* cfg)
     return ZMPM_OK;
 }
 
-zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, int32_t tune_split,
+zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, int32_t tune_variants,
                           zmpm_launch_config* cfg)
 {
     if (!h || !(dt > 0) || steps < 1) return ZMPM_E_ARG;
@@ -1674,41 +1704,49 @@ zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, in
     const Dev<float, long long> df = h->Df;
     const Dev<double, double> dd = h->Dd;
     double t = t0;
-    // stage (0 bin/sort, 1 p2g, 2 grid update) each knob is charged with
+    // a knob: the bits `mask` of *value, the timed stage it is charged with (0
+    // bin/sort, 1 p2g, 2 grid update, 4 g2p) and its candidates; none of them
+    // changes a result (the P2G lane split, which does, is not tuned)
     struct Knob {
         int* value;
+        int mask;
         int stage;
         std::vector<int> candidates;
     };
     std::vector<Knob> knobs = {
-        { &h->p2g_wgs, 1, { 512, 1024, 2048, 4096 } },
-        { &h->prep_wgs, 0, { 256, 512, 1024, 2048 } },
-        { &h->grid_wgs, 2, { 256, 512, 1024, 2048, 4096 } },
+        { &h->p2g_wgs, ~0, 1, { 512, 1024, 2048, 4096 } },
+        { &h->prep_wgs, ~0, 0, { 256, 512, 1024, 2048 } },
+        { &h->grid_wgs, ~0, 2, { 256, 512, 1024, 2048, 4096 } },
     };
-    if (tune_split) knobs.push_back({ &h->p2g_split, 1, { 1, 2, 4, 64 } });
+    if (tune_variants) {
+        knobs.push_back({ &h->variant, VARIANT_P2G_LOW_OCC, 1, { 0, VARIANT_P2G_LOW_OCC } });
+        knobs.push_back({ &h->variant, VARIANT_P2G_XCD, 1, { 0, VARIANT_P2G_XCD } });
+        knobs.push_back({ &h->variant, VARIANT_G2P_RR, 4, { 0, VARIANT_G2P_RR } });
+    }
     h->timing = true;
     zmpm_status st = ZMPM_OK;
     // one untimed substep first (first-touch of the configuration's launches)
     if ((st = runUniform(h, 1, dt, t)) != ZMPM_OK) goto done;
     t += dt;
     for (Knob& k : knobs) {
-        int best = *k.value;
-        double best_ms = 1e300;
-        for (int c : k.candidates) {
-            *k.value = c;
-            if (h->fp32)
-                refreshTables<float, long long>(h);
-            else
-                refreshTables<double, double>(h);
-            if ((st = runUniform(h, steps, dt, t)) != ZMPM_OK) goto done;
-            t += steps * dt;
-            const double ms = h->timed_ms[k.stage] / h->timed_steps;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = c;
+        // each candidate timed in two rounds, its best round kept (one round
+        // can catch a clock or cache transient)
+        std::vector<double> best_of(k.candidates.size(), 1e300);
+        for (int round = 0; round < 2; round++)
+            for (size_t ci = 0; ci < k.candidates.size(); ci++) {
+                *k.value = (*k.value & ~k.mask) | (k.candidates[ci] & k.mask);
+                if (h->fp32)
+                    refreshTables<float, long long>(h);
+                else
+                    refreshTables<double, double>(h);
+                if ((st = runUniform(h, steps, dt, t)) != ZMPM_OK) goto done;
+                t += steps * dt;
+                best_of[ci] = std::min(best_of[ci], h->timed_ms[k.stage] / h->timed_steps);
             }
-        }
-        *k.value = best;
+        size_t best = 0;
+        for (size_t ci = 1; ci < k.candidates.size(); ci++)
+            if (best_of[ci] < best_of[best]) best = ci;
+        *k.value = (*k.value & ~k.mask) | (k.candidates[best] & k.mask);
     }
 done:
     h->timing = timing;
@@ -1733,6 +1771,60 @@ done:
     dropGraphs(h);
     if (st != ZMPM_OK) return st;
     if (cfg) return zmpm_get_launch_config(h, cfg);
+    return ZMPM_OK;
+}
+
+// Key of the tuning cache: the device and what selects the work and the kernels.
+static std::string tuneKey(zmpm_handle h, int32_t tune_variants)
+{
+    hipDeviceProp_t prop {};
+    (void)hipGetDeviceProperties(&prop, h->desc.device);
+    std::string arch = prop.gcnArchName;
+    const size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch.resize(colon);
+    bool plastic = false, fj = false;
+    for (const auto& m : h->mats) {
+        plastic = plastic || m.plasticity != ZMPM_PLASTICITY_NONE;
+        fj = fj || m.fj_mixed != ZMPM_FJ_NONE;
+    }
+    std::ostringstream k;
+    k << arch << "/cu" << prop.multiProcessorCount << "/" << (h->fp32 ? "fp32" : "fp64") << "/s" << h->scenes.size()
+      << "/n" << std::lround(4.0 * std::log2((double)std::max(h->n, 1))) << "/d" << (h->any_damage ? 1 : 0) << "/i"
+      << h->desc.implicit_damage << "/t" << h->desc.transfer_scheme << "/m" << h->desc.mls_mpm << "/r"
+      << h->desc.rpic_damping_iteration << "/p" << h->desc.pic_damping_iteration << "/j" << (fj ? 1 : 0) << "/y"
+      << (plastic ? 1 : 0) << "/v" << (tune_variants ? 1 : 0);
+    return k.str();
+}
+
+zmpm_status zmpm_autotune_cached(zmpm_handle h, double dt, double t0, int32_t steps, int32_t tune_variants,
+                                 const char* cache_path, zmpm_launch_config* cfg, int32_t* from_cache)
+{
+    if (!h || !cache_path || !(dt > 0) || steps < 1) return ZMPM_E_ARG;
+    if (midSubstep(h)) return ZMPM_E_ARG;
+    if (from_cache) *from_cache = 0;
+    const std::string key = tuneKey(h, tune_variants);
+    {
+        std::ifstream in(cache_path);
+        std::string line;
+        while (std::getline(in, line)) {
+            std::istringstream ls(line);
+            std::string k;
+            zmpm_launch_config c {};
+            if (!(ls >> k >> c.p2g_wgs >> c.grid_wgs >> c.prep_wgs >> c.variant) || k != key) continue;
+            c.p2g_split = h->p2g_split;
+            const zmpm_status st = zmpm_set_launch_config(h, &c);
+            if (st != ZMPM_OK) continue; // a malformed line: tune instead
+            if (from_cache) *from_cache = 1;
+            return cfg ? zmpm_get_launch_config(h, cfg) : ZMPM_OK;
+        }
+    }
+    zmpm_launch_config c {};
+    const zmpm_status st = zmpm_autotune(h, dt, t0, steps, tune_variants, &c);
+    if (st != ZMPM_OK) return st;
+    // best effort: a cache that cannot be written only costs the next run a tuning
+    std::ofstream out(cache_path, std::ios::app);
+    out << key << ' ' << c.p2g_wgs << ' ' << c.grid_wgs << ' ' << c.prep_wgs << ' ' << c.variant << '\n';
+    if (cfg) *cfg = c;
     return ZMPM_OK;
 }
 

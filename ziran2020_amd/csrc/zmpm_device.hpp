@@ -1110,12 +1110,14 @@ __device__ __forceinline__ void loadP2G(const Dev<T, AccT>& D, int s, bool dmg, 
 // int64 fixed point in LDS and HBM, so the result does not depend on scheduling.
 enum : int { P2G_STD = 0, P2G_MLS = 1, P2G_DAMP = 2 };
 
-// Waves per SIMD the P2G kernel is compiled for: the float form fits 4 (at most
-// 128 VGPRs), the double form 3.
+// Waves per SIMD the P2G kernel is compiled for by default: the float form fits
+// 4 (at most 128 VGPRs), the double form 3.  The autotuner also times one wave
+// fewer (more registers) and an XCD-contiguous block order; no variant changes
+// a result (the in-block order fixes every sum).
 template <class T> struct P2GWaves { static constexpr int value = sizeof(T) == 4 ? 4 : 3; };
 
-template <class T, class AccT, int VAR = P2G_STD>
-__global__ __launch_bounds__(WG, P2GWaves<T>::value) void k_p2g(Dev<T, AccT> D)
+template <class T, class AccT, int VAR = P2G_STD, int OCC = P2GWaves<T>::value, bool XCD = false>
+__global__ __launch_bounds__(WG, OCC) void k_p2g(Dev<T, AccT> D)
 {
     constexpr bool DAMP = VAR == P2G_DAMP, MLS = VAR == P2G_MLS;
     using B = Blk<T>;
@@ -1145,10 +1147,11 @@ __global__ __launch_bounds__(WG, P2GWaves<T>::value) void k_p2g(Dev<T, AccT> D)
 
     // in-block order of this workgroup's blocks first (a loop of its own, so
     // its registers are free again for the scatter)
-    // blocks round-robin over the XCDs: neighbouring blocks flush their shared
-    // halo nodes from different L2s (an XCD-contiguous mapping, as in G2P, made
-    // P2G ~25 % slower on MI355X); both loops take the same blocks
-    const int wgl = (int)blockIdx.x;
+    // blocks round-robin over the XCDs by default: neighbouring blocks flush
+    // their shared halo nodes from different L2s (an XCD-contiguous mapping, as
+    // in G2P, made P2G ~25 % slower on the damage slab); both loops take the
+    // same blocks
+    const int wgl = XCD ? xcdLocal((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
     for (int b = wgl; !DAMP && b < nblocks; b += gridDim.x) {
         const int bs = D.block_start[b], cnt = D.block_count[b];
         if (cnt <= 0) { // halo-only block
@@ -2164,7 +2167,9 @@ __device__ __forceinline__ void stageTile(const Dev<T, AccT>& D, int b, T (*gt)[
     }
 }
 
-template <class T, class AccT, bool FLIP>
+// RR: workgroups take particle segments round-robin over the XCDs instead of
+// XCD by XCD (an autotuner variant; results are the same).
+template <class T, class AccT, bool FLIP, bool RR = false>
 __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D)
 {
     using TL = Tile<T>;
@@ -2172,7 +2177,7 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D)
     constexpr int NGT = FLIP ? 7 : 4;
     __shared__ T gt[NGT][TL::TN];
     __shared__ int seg_end;
-    const int lo = xcdLocal((int)blockIdx.x, (int)gridDim.x) * WG;
+    const int lo = (RR ? (int)blockIdx.x : xcdLocal((int)blockIdx.x, (int)gridDim.x)) * WG;
     const int hi = min(lo + WG, D.n);
     const int t = threadIdx.x, s = lo + t;
     const bool live = s < hi;
