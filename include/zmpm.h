@@ -131,8 +131,12 @@ typedef enum {
 typedef struct {
     int32_t precision;       /* zmpm_precision                                            */
     int32_t domain_nodes;    /* grid nodes per axis that may be addressed: power of two in
-                                [64, 1024] (the reference SPGrid is 4096^3,
-                                MpmGrid.h:111-112).  Sizes the dense page table.         */
+                                [64, 4096], the reference SPGrid's range (MpmGrid.h:111-112).
+                                Sizes the page flags and table (4 B per page each; 12 GB
+                                at 4096 in the FP32 form, 24 GB in the FP64 form), which
+                                each substep visits only where particles are.  The FP32
+                                form's fixed-point positions resolve dx / 2^21 up to 1024
+                                nodes, dx / 2^19 above.                                   */
     double dx;               /* MpmSimulationBase::dx                                     */
     double gravity[3];       /* MpmSimulationBase::gravity                                */
     double apic_rpic_ratio;  /* MpmSimulationBase::apic_rpic_ratio (default 1)            */
@@ -392,7 +396,7 @@ zmpm_status zmpm_set_velocity_field(zmpm_handle h, int32_t scene, const double* 
  * and domain_nodes; precision, device, transfer scheme, flip_pic_ratio, mls_mpm
  * and the damping iterations must be equal in every scene, and implicit_damage
  * needs a single scene.  Scene s lives in its own cubic region of the handle's
- * grid (side = the largest domain_nodes, regions in page order; at most 1024
+ * grid (side = the largest domain_nodes, regions in page order; at most 4096
  * nodes per axis in all): positions in and out are metres in the scene's own
  * frame, as if it ran alone, and zmpm_get_grid / zmpm_get_binning report the
  * handle's grid (a scene's node (i, j, k) is node origin + (i, j, k)).  The

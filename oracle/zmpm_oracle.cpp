@@ -1250,20 +1250,21 @@ zmpm_status fail(zmpm_context* h, zmpm_status s, const std::string& msg)
     return s;
 }
 
-// FP32 layout position grid: x / dx in units of 2^-kPosBits, clamped to
-// [-2^30, 2^31 - 2] units (outside the domain either way).
-constexpr int kPosBits = 21;
+// FP32 layout position grid: x / dx in units of 2^-pb, pb = 21 for domains of
+// up to 1024 nodes per axis and 19 up to 4096, clamped to [-2^30, 2^31 - 2]
+// units (outside the domain either way).
+inline int posBitsOf(int domain_nodes) { return domain_nodes <= 1024 ? 21 : 19; }
 
-inline double posUnits(double x, double dx)
+inline double posUnits(double x, double dx, int pb)
 {
-    const double u = std::nearbyint(x / dx * (double)(1 << kPosBits));
+    const double u = std::nearbyint(x / dx * (double)(1 << pb));
     return std::min(std::max(u, -1073741824.0), 2147483646.0);
 }
 
-inline double quantizePos(double x, double dx)
+inline double quantizePos(double x, double dx, int pb)
 {
     if (!std::isfinite(x)) return x;
-    return posUnits(x, dx) * dx / (double)(1 << kPosBits);
+    return posUnits(x, dx, pb) * dx / (double)(1 << pb);
 }
 
 // Base node of particle i in the handle's precision arithmetic; false if the
@@ -1281,10 +1282,11 @@ bool baseNodeOf(const zmpm_context* h, int64_t i, int* b, bool* nan_out)
         }
         double s;
         if (h->desc.precision == ZMPM_PREC_FP32) {
-            // floor(X / 2^21 - 1/2) of the integer X, exactly
-            const int64_t X = (int64_t)posUnits(xv, h->desc.dx);
-            const int64_t t = X - (int64_t(1) << (kPosBits - 1));
-            b[a] = (int)((t >= 0) ? (t >> kPosBits) : -((-t + (int64_t(1) << kPosBits) - 1) >> kPosBits));
+            // floor(X / 2^pb - 1/2) of the integer X, exactly
+            const int pb = posBitsOf(h->desc.domain_nodes);
+            const int64_t X = (int64_t)posUnits(xv, h->desc.dx, pb);
+            const int64_t t = X - (int64_t(1) << (pb - 1));
+            b[a] = (int)((t >= 0) ? (t >> pb) : -((-t + (int64_t(1) << pb) - 1) >> pb));
         }
         else {
             double odx = 1 / h->desc.dx;
@@ -1301,7 +1303,10 @@ bool baseNodeOf(const zmpm_context* h, int64_t i, int* b, bool* nan_out)
 // Index-space position in the handle's precision (BSplineWeights::compute, MpmGrid.h:74-79).
 inline double indexSpace(const zmpm_context* h, double xv)
 {
-    if (h->desc.precision == ZMPM_PREC_FP32) return posUnits(xv, h->desc.dx) / (double)(1 << kPosBits);
+    if (h->desc.precision == ZMPM_PREC_FP32) {
+        const int pb = posBitsOf(h->desc.domain_nodes);
+        return posUnits(xv, h->desc.dx, pb) / (double)(1 << pb);
+    }
     return xv * (1 / h->desc.dx);
 }
 
@@ -1604,7 +1609,7 @@ int gridToParticlesPass(zmpm_context* h, double dt, double r, double flip, bool 
         double inc[3] = { dt * picV[0], dt * picV[1], dt * picV[2] };
         for (int a = 0; a < 3; a++) Xp[a] += inc[a];
         if (fp32_layout)
-            for (int a = 0; a < 3; a++) Xp[a] = quantizePos(Xp[a], dx);
+            for (int a = 0; a < 3; a++) Xp[a] = quantizePos(Xp[a], dx, posBitsOf(h->desc.domain_nodes));
         double incn = inc[0] * inc[0] + inc[1] * inc[1] + inc[2] * inc[2];
         if (incn > dx2) cfl_flag |= 1;
     }
@@ -2051,7 +2056,7 @@ zmpm_status zmpm_create(const zmpm_scene_desc* desc, zmpm_handle* out)
 {
     if (!desc || !out) return ZMPM_E_ARG;
     int dn = desc->domain_nodes;
-    if (dn < 64 || dn > 1024 || (dn & (dn - 1)) || !(desc->dx > 0)) return ZMPM_E_ARG;
+    if (dn < 64 || dn > 4096 || (dn & (dn - 1)) || !(desc->dx > 0)) return ZMPM_E_ARG;
     if (desc->precision != ZMPM_PREC_FP32 && desc->precision != ZMPM_PREC_FP64) return ZMPM_E_ARG;
     if (desc->transfer_scheme != ZMPM_APIC_BLEND_RPIC && desc->transfer_scheme != ZMPM_FLIP_BLEND_PIC) return ZMPM_E_ARG;
     // MpmSimulationBase::initialize / advanceOneTimeStep assertions (MpmSimulationBase.cpp:108-128, 319, 335)
@@ -2200,7 +2205,7 @@ zmpm_status zmpm_upload(zmpm_handle h, const zmpm_particles* p)
     h->n = n;
     h->x.assign(p->x, p->x + 3 * n);
     if (h->desc.precision == ZMPM_PREC_FP32)
-        for (double& xv : h->x) xv = quantizePos(xv, h->desc.dx);
+        for (double& xv : h->x) xv = quantizePos(xv, h->desc.dx, posBitsOf(h->desc.domain_nodes));
     h->v.assign(p->v, p->v + 3 * n);
     h->C.assign(p->C, p->C + 9 * n);
     h->F.assign(p->F, p->F + 9 * n);
@@ -2286,7 +2291,7 @@ zmpm_status zmpm_create_batch(const zmpm_scene_desc* descs, int32_t n_scenes, zm
     while (rd * rd * rd < n_scenes) rd++;
     int grid = 1;
     while (grid < rd * region) grid <<= 1;
-    if (grid > 1024) return ZMPM_E_ARG;
+    if (grid > 4096) return ZMPM_E_ARG;
     auto* h = new zmpm_context();
     h->desc = d0;
     h->L = makeLayout(d0.precision == ZMPM_PREC_FP32 ? 6 : 7);

@@ -97,10 +97,12 @@ def test_batch_validation(lib_name, request):
     assert _create(lib, mixed) == A.E_ARG  # one precision per handle
     assert _create(lib, _descs(2, implicit_damage=True)) == A.E_ARG  # the implicit solve needs one scene
     assert _create(lib, _descs(0)) == A.E_ARG
-    # regions of the largest domain, ceil(cbrt(n)) per axis, at most 1024 nodes per axis
-    assert _create(lib, [A.scene_desc(dx=1.0 / 64, domain_nodes=512) for _ in range(8)]) == A.OK
-    assert _create(lib, [A.scene_desc(dx=1.0 / 64, domain_nodes=512) for _ in range(9)]) == A.E_ARG
-    assert _create(lib, [A.scene_desc(dx=1.0 / 64, domain_nodes=1024) for _ in range(2)]) == A.E_ARG
+    # regions of the largest domain, ceil(cbrt(n)) per axis, at most 4096 nodes per axis
+    assert _create(lib, [A.scene_desc(dx=1.0 / 64, domain_nodes=512) for _ in range(64)]) == A.OK
+    assert _create(lib, [A.scene_desc(dx=1.0 / 64, domain_nodes=2048) for _ in range(8)]) == A.OK
+    assert _create(lib, [A.scene_desc(dx=1.0 / 64, domain_nodes=2048) for _ in range(9)]) == A.E_ARG
+    assert _create(lib, [A.scene_desc(dx=1.0 / 64, domain_nodes=4096) for _ in range(2)]) == A.E_ARG
+    assert _create(lib, [A.scene_desc(dx=1.0 / 64, domain_nodes=8192)]) == A.E_ARG
 
 
 def test_batch_upload_rejects_bad_scene(oracle):

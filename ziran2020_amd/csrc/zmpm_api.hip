@@ -31,6 +31,7 @@ namespace {
 constexpr int kMaxMaterials = 256;
 constexpr int kMaxColliders = MAX_COLLIDERS;
 constexpr int kTimedGroups = 5;
+constexpr int kPageScanWgs = 1024; // persistent page-scan workgroups (at most one per super page)
 // compiled kernel variants the launch configuration selects
 constexpr int VARIANT_P2G_LOW_OCC = 1; // P2G compiled for one wave per SIMD fewer
 constexpr int VARIANT_G2P_RR = 2;      // G2P segments round-robin over the XCDs
@@ -263,6 +264,11 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.page_table, D.num_pages);
     const int64_t scan_tiles = (D.num_pages + PSCAN_TILE - 1) / PSCAN_TILE;
     ALLOC(D.scan_state, scan_tiles);
+    D.num_super = scan_tiles; // one page-scan tile per super page
+    const int64_t super_tiles = (D.num_super + PSCAN_TILE - 1) / PSCAN_TILE;
+    ALLOC(D.super_touched, D.num_super);
+    ALLOC(D.super_list, D.num_super);
+    ALLOC(D.super_state, super_tiles);
     ALLOC(D.block_page, cap_blocks);
     ALLOC(D.block_start, cap_blocks);
     ALLOC(D.block_count, cap_blocks);
@@ -302,6 +308,8 @@ zmpm_status allocate(zmpm_context* h, int n)
     HIPCHK(h, hipMemsetAsync(D.touched, 0, D.num_pages * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.page_count, 0, D.num_pages * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.scan_state, 0, scan_tiles * sizeof(uint64_t), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.super_touched, 0, D.num_super * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.super_state, 0, super_tiles * sizeof(uint64_t), h->stream));
     HIPCHK(h, hipMemsetAsync(D.tmp, 0, cap * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.tmpk, 0, cap * sizeof(uint32_t), h->stream));
     HIPCHK(h, hipMemsetAsync(D.perm, 0, cap * sizeof(int), h->stream));
@@ -344,7 +352,8 @@ zmpm_status uploadScenes(zmpm_context* h)
         e.dx = (T)d.dx;
         e.ddx = d.dx;
         e.odx = T(1) / (T)d.dx;
-        e.pos_scale = (T)((double)(1 << POS_BITS) / d.dx);
+        e.pos_bits = posBits(d.domain_nodes);
+        e.pos_scale = (T)((double)(1 << e.pos_bits) / d.dx);
         e.Dinv = T(4) / ((T)d.dx * (T)d.dx);
         for (int a = 0; a < 3; a++) e.gravity[a] = (T)d.gravity[a];
         e.rpic = (T)d.apic_rpic_ratio;
@@ -552,7 +561,9 @@ void launchStage(zmpm_context* h, int stage)
     switch (stage) {
     case ZMPM_STAGE_SORT: {
         const int gp = gridFor(D.n, WG, 1 << 30);
-        k_page_scan<T, AccT><<<(int)((D.num_pages + PSCAN_TILE - 1) / PSCAN_TILE), WG, 0, st>>>(D);
+        // the touched super pages in order, then the scan of their pages
+        k_super_scan<T, AccT><<<(int)((D.num_super + PSCAN_TILE - 1) / PSCAN_TILE), WG, 0, st>>>(D);
+        k_page_scan<T, AccT><<<(int)std::min<int64_t>(D.num_super, kPageScanWgs), WG, 0, st>>>(D);
         const int prep = (int)std::min<int64_t>(D.cap_blocks, h->prep_wgs);
         k_scatter<T, AccT><<<gp + prep + 1, WG, 0, st>>>(D, gp, prep);
         break;
@@ -635,22 +646,22 @@ zmpm_status deviceStatus(zmpm_context* h, bool sync)
     return ZMPM_OK;
 }
 
-// FP32-form position in fixed point, X = round(x / dx * 2^POS_BITS): POS_NAN
-// for a non-finite x, clamped to [POS_MIN, POS_MAX] (outside the domain either
-// way, which the first substep reports).
-int encodePos(double x, double dx)
+// FP32-form position in fixed point, X = round(x / dx * 2^pb) in the scene's
+// frame: POS_NAN for a non-finite x, clamped to [POS_MIN, POS_MAX] (outside the
+// domain either way, which the first substep reports).
+int encodePos(double x, double dx, int pb)
 {
     if (!std::isfinite(x)) return POS_NAN;
-    const double u = std::nearbyint(x / dx * (double)(1 << POS_BITS));
+    const double u = std::nearbyint(x / dx * (double)(1 << pb));
     if (!(u >= (double)POS_MIN)) return POS_MIN;
     if (!(u <= (double)POS_MAX)) return POS_MAX;
     return (int)u;
 }
 
-double decodePos(int X, double dx)
+double decodePos(int X, double dx, int pb)
 {
     if (X == POS_NAN) return std::numeric_limits<double>::quiet_NaN();
-    return (double)X * dx / (double)(1 << POS_BITS);
+    return (double)X * dx / (double)(1 << pb);
 }
 
 template <class T, class AccT>
@@ -674,12 +685,7 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p, const int32_t* sce
         const zmpm_scene_desc& sd = h->scenes[(size_t)sc];
         for (int a = 0; a < 3; a++) {
             if constexpr (F32) {
-                // position units of the whole grid: the scene's region starts at its origin
-                int X = encodePos(p->x[3 * i + a], sd.dx);
-                if (X != POS_NAN) {
-                    const int64_t Xg = (int64_t)X + ((int64_t)h->origin[(size_t)sc][a] << POS_BITS);
-                    X = (int)std::min<int64_t>(std::max<int64_t>(Xg, POS_MIN), POS_MAX);
-                }
+                const int X = encodePos(p->x[3 * i + a], sd.dx, posBits(sd.domain_nodes));
                 std::memcpy(&buf[(PX + a) * cap + i], &X, sizeof(int));
             }
             else
@@ -723,6 +729,7 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p, const int32_t* sce
     HIPCHK(h, hipMemsetAsync(D.maxbits, 0, h->scenes.size() * NSC * BOUND_SLOTS * sizeof(unsigned int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.touched, 0, D.num_pages * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.page_count, 0, D.num_pages * sizeof(int), h->stream));
+    HIPCHK(h, hipMemsetAsync(D.super_touched, 0, D.num_super * sizeof(int), h->stream));
     if (n > 0) {
         // stress of the uploaded state and the first substep's fixed-point bounds
         k_stress_init<T, AccT><<<D.max_slots, WG, 0, h->stream>>>(D, 1, 1);
@@ -766,8 +773,8 @@ zmpm_status downloadT(zmpm_context* h, zmpm_particles* p)
                 if constexpr (std::is_same<T, float>::value) {
                     int X;
                     std::memcpy(&X, &buf[(PX + a) * cap + s], sizeof(int));
-                    if (X != POS_NAN) X -= h->origin[(size_t)sc][a] << POS_BITS;
-                    p->x[3 * i + a] = decodePos(X, h->scenes[(size_t)sc].dx);
+                    const zmpm_scene_desc& sd = h->scenes[(size_t)sc];
+                    p->x[3 * i + a] = decodePos(X, sd.dx, posBits(sd.domain_nodes));
                 }
                 else
                     p->x[3 * i + a] = (double)buf[(PX + a) * cap + s];
@@ -977,6 +984,7 @@ std::vector<std::pair<void*, size_t>> liveState(zmpm_context* h)
              { D.pmat, (size_t)D.cap * sizeof(int) },        { D.key, (size_t)D.cap * sizeof(uint64_t) },
              { D.rank, (size_t)D.cap * sizeof(int) },
              { D.touched, (size_t)D.num_pages * sizeof(int) }, { D.page_count, (size_t)D.num_pages * sizeof(int) },
+             { D.super_touched, (size_t)D.num_super * sizeof(int) },
              { D.maxbits, h->scenes.size() * NSC * BOUND_SLOTS * sizeof(unsigned) },
              { D.scales, h->scenes.size() * NSC * sizeof(float) },
              { D.counters, 8 * sizeof(int) },                { D.clock, 2 * h->scenes.size() * sizeof(double) } };
@@ -986,7 +994,8 @@ template <class T, class AccT>
 void clearScanState(zmpm_context* h)
 {
     Dev<T, AccT>& D = devOf<T, AccT>(h);
-    (void)hipMemsetAsync(D.scan_state, 0, (D.num_pages + PSCAN_TILE - 1) / PSCAN_TILE * sizeof(uint64_t), h->stream);
+    (void)hipMemsetAsync(D.scan_state, 0, D.num_super * sizeof(uint64_t), h->stream);
+    (void)hipMemsetAsync(D.super_state, 0, (D.num_super + PSCAN_TILE - 1) / PSCAN_TILE * sizeof(uint64_t), h->stream);
 }
 
 } // namespace
@@ -1009,7 +1018,7 @@ namespace {
 bool validScene(const zmpm_scene_desc* d)
 {
     const int dn = d->domain_nodes;
-    if (dn < 64 || dn > 1024 || (dn & (dn - 1)) || !(d->dx > 0) || !(d->cfl > 0)) return false;
+    if (dn < 64 || dn > 4096 || (dn & (dn - 1)) || !(d->dx > 0) || !(d->cfl > 0)) return false;
     if (d->precision != ZMPM_PREC_FP32 && d->precision != ZMPM_PREC_FP64) return false;
     if (d->transfer_scheme != ZMPM_APIC_BLEND_RPIC && d->transfer_scheme != ZMPM_FLIP_BLEND_PIC) return false;
     const bool apic = d->transfer_scheme == ZMPM_APIC_BLEND_RPIC;
@@ -1058,7 +1067,7 @@ zmpm_status createScenes(const zmpm_scene_desc* descs, int32_t n_scenes, zmpm_ha
     while (rd * rd * rd < n_scenes) rd++;
     int grid = 1;
     while (grid < rd * region) grid <<= 1;
-    if (grid > 1024) return ZMPM_E_ARG; // fixed-point positions address at most 1024 nodes per axis
+    if (grid > 4096) return ZMPM_E_ARG; // the SPGrid range (MpmGrid.h:111-112)
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ZMPM_E_HIP;
     const int device = descs[0].device;

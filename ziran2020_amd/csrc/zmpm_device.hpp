@@ -110,7 +110,7 @@ __device__ __forceinline__ void storePos(double* base, int i, double x) { stAt(b
 // inside the scene's region.
 // FP64 form: x (metres in the scene's frame) * (1/dx) - 0.5 in double without
 // FMA, truncated (int_floor for t >= 0, MathTools.h:15-25).
-__device__ __forceinline__ int cellOf(double x, double odx, int origin, int domain, int& base, double& d0)
+__device__ __forceinline__ int cellOf(double x, double odx, int origin, int domain, int& base, double& d0, int)
 {
     const double xs = mul_rn(x, odx);
     double t = sub_rn(xs, 0.5);
@@ -130,21 +130,21 @@ __device__ __forceinline__ int cellOf(double x, double odx, int origin, int doma
     return bits;
 }
 
-// FP32 form: the same floor, exact on the fixed-point position (held in the
-// handle's grid already); d0 has at most POS_BITS + 1 significant bits, so it is
-// exact in float too.
-__device__ __forceinline__ int cellOf(int X, float, int origin, int domain, int& base, float& d0)
+// FP32 form: the same floor, exact on the fixed-point position X (scene frame,
+// pb fraction bits); d0 has at most pb + 1 significant bits, so it is exact in
+// float too.
+__device__ __forceinline__ int cellOf(int X, float, int origin, int domain, int& base, float& d0, int pb)
 {
     const int bits0 = (X == POS_NAN) ? ST_NAN : 0;
     int bits = bits0;
-    const int t = bits0 ? 0 : X - POS_HALF;
-    int b = (t >> POS_BITS) - origin; // arithmetic shift: floor
+    const int t = bits0 ? 0 : X - (1 << (pb - 1)); // half a cell below: floor(X / 2^pb - 1/2)
+    int b = t >> pb;                               // arithmetic shift: floor
     if (!(b >= 0 && b < domain - 4)) {
         bits |= ST_DOMAIN;
         b = (b < 0) ? 0 : domain - 5;
     }
     base = b + origin;
-    d0 = bits ? 1.f : (float)(X - (base << POS_BITS)) * (1.f / (float)(1 << POS_BITS));
+    d0 = bits ? 1.f : (float)(X - (b << pb)) * __int_as_float((127 - pb) << 23); // times 2^-pb, exact
     return bits;
 }
 
@@ -161,25 +161,28 @@ __device__ __forceinline__ int sceneOfNode(const Dev<T, AccT>& D, uint32_t ix, u
     return D.region_scene[(rx * rd + ry) * rd + rz];
 }
 
-// Every third bit: b9..b0 -> bit 3*i (block coordinates < 1024).
-__device__ __forceinline__ uint32_t part1by2(uint32_t x)
+// Every third bit: b10..b0 -> bit 3*i (block coordinates < 2048, the 4096-node
+// domain in the double form's 2-node x blocks).
+__device__ __forceinline__ uint32_t part1by2(uint32_t v)
 {
-    x &= 0x3ffu;
-    x = (x | (x << 16)) & 0x030000ffu;
-    x = (x | (x << 8)) & 0x0300f00fu;
-    x = (x | (x << 4)) & 0x030c30c3u;
-    x = (x | (x << 2)) & 0x09249249u;
-    return x;
+    uint64_t x = v & 0x7ffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return (uint32_t)x;
 }
 
-__device__ __forceinline__ uint32_t compact1by2(uint32_t x)
+__device__ __forceinline__ uint32_t compact1by2(uint32_t v)
 {
-    x &= 0x09249249u;
-    x = (x | (x >> 2)) & 0x030c30c3u;
-    x = (x | (x >> 4)) & 0x0300f00fu;
-    x = (x | (x >> 8)) & 0x030000ffu;
-    x = (x | (x >> 16)) & 0x000003ffu;
-    return x;
+    uint64_t x = v & 0x49249249u;
+    x = (x | (x >> 2)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x >> 4)) & 0x100f00f00f00f00full;
+    x = (x | (x >> 8)) & 0x1f0000ff0000ffull;
+    x = (x | (x >> 16)) & 0x1f00000000ffffull;
+    x = (x | (x >> 32)) & 0x7ffull;
+    return (uint32_t)x;
 }
 
 // SPGrid page index of block coordinates: the page bits of Linear_Offset
@@ -261,7 +264,7 @@ __device__ __forceinline__ uint64_t waveInclusiveScan64(uint64_t v)
 // reference's unique 64-bit keys (MpmSimulationBase.cpp:1552).
 
 // counters[] slots
-enum : int { CNT_BLOCKS = 0, CNT_STATUS = 1, CNT_EPOCH = 2, CNT_TICKET = 3, CNT_RARE = 4 };
+enum : int { CNT_BLOCKS = 0, CNT_STATUS = 1, CNT_EPOCH = 2, CNT_TICKET = 3, CNT_RARE = 4, CNT_SUPER = 5, CNT_TICKET2 = 6 };
 
 constexpr int PSCAN_ITEMS = 8;
 constexpr int PSCAN_TILE = WG * PSCAN_ITEMS; // pages per scan tile
@@ -301,7 +304,7 @@ __device__ __forceinline__ void binParticle(const Dev<T, AccT>& D, int p, bool a
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             T d0;
-            bits |= cellOf(x[a], sd.odx, sd.origin[a], sd.domain, bb[a], d0);
+            bits |= cellOf(x[a], sd.odx, sd.origin[a], sd.domain, bb[a], d0, sd.pos_bits);
         }
         if (bits) latch(D.counters, bits);
         uint32_t elem;
@@ -324,7 +327,11 @@ __device__ __forceinline__ void binParticle(const Dev<T, AccT>& D, int p, bool a
 #pragma unroll
             for (int b = 0; b < 2; b++)
 #pragma unroll
-                for (int c = 0; c < 2; c++) D.touched[pageOfBlock<T>(D.masks, Bx + a, By + b, Bz + c)] = 1;
+                for (int c = 0; c < 2; c++) {
+                    const uint32_t pg = pageOfBlock<T>(D.masks, Bx + a, By + b, Bz + c);
+                    D.touched[pg] = 1;
+                    D.super_touched[pg / PSCAN_TILE] = 1;
+                }
     }
     slot = __shfl(slot, start);
     if (active) stAt(D.rank, p, slot + (lane - start));
@@ -340,122 +347,201 @@ __device__ __forceinline__ void openPageScan(const Dev<T, AccT>& D)
         const int e = D.counters[CNT_EPOCH];
         D.counters[CNT_EPOCH] = e % 511 + 1; // the coming substep's page-scan epoch
         D.counters[CNT_TICKET] = 0;          // page-scan tile tickets
+        D.counters[CNT_TICKET2] = 0;         // super-page-scan tile tickets
+    }
+}
+
+// Decoupled look-back of a single-pass scan, run by wavefront 0 of the
+// workgroup holding `tile`: publishes the tile's aggregate, adds its
+// predecessors' published prefixes (64 per poll; lane l reads tile - 1 - l, the
+// prefix is the sum up to the nearest inclusive word), publishes the inclusive
+// prefix and returns the exclusive one.  A tile waits only on tiles that took
+// earlier tickets, which are running; the wait is also bounded (SCAN_SPIN_LIMIT
+// polls, then ST_SCAN is latched).
+template <class T, class AccT>
+__device__ __forceinline__ uint64_t lookBack(const Dev<T, AccT>& D, uint64_t* state, int tile, uint64_t total,
+                                             unsigned epoch)
+{
+    const int lane = threadIdx.x & 63;
+    uint64_t pre = 0;
+    if (tile == 0) {
+        if (lane == 0) __hip_atomic_store(&state[0], scanWord(epoch, 2, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return 0;
+    }
+    if (lane == 0) __hip_atomic_store(&state[tile], scanWord(epoch, 1, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int base_j = tile - 1, spins = 0;
+    while (true) {
+        const int j = base_j - lane;
+        uint64_t w = scanWord(epoch, 2, 0); // before tile 0: an inclusive zero
+        if (j >= 0) w = __hip_atomic_load(&state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned we = (unsigned)(w >> 55), wf = (unsigned)(w >> 53) & 3u;
+        const bool valid = we == epoch && wf != 0;
+        const uint64_t incl = __ballot(valid && wf == 2);
+        const int first = incl ? __ffsll((unsigned long long)incl) - 1 : 63;
+        const uint64_t need = (first == 63) ? ~0ull : ((2ull << first) - 1ull);
+        if (__ballot(!valid) & need) { // a predecessor up to `first` is not published yet
+            if (++spins > SCAN_SPIN_LIMIT) {
+                if (lane == 0) latch(D.counters, ST_SCAN);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        uint64_t v = lane <= first ? scanPair(w) : 0ull;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+        pre += v;
+        if (incl) break;
+        base_j -= 64;
+    }
+    if (lane == 0) __hip_atomic_store(&state[tile], scanWord(epoch, 2, pre + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return pre;
+}
+
+// Exclusive workgroup scan of one value per thread; *total gets the sum.
+__device__ __forceinline__ uint64_t blockExclusiveScan(uint64_t mine, uint64_t* wsum, uint64_t* total)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t inc = waveInclusiveScan64(mine);
+    if (lane == 63) wsum[wv] = inc;
+    __syncthreads();
+    uint64_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < WG / 64; w++) {
+        if (w < wv) off += wsum[w];
+        tot += wsum[w];
+    }
+    *total = tot;
+    return off + inc - mine;
+}
+
+// Compaction of the touched super pages in index order (one tile of
+// PSCAN_TILE super pages per workgroup, tiles in ticket order): super_list
+// and their count (CNT_SUPER); the flags are cleared.  Its last tile also
+// zeroes the block count, which the page scan sets when any page is active.
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_super_scan(Dev<T, AccT> D)
+{
+    __shared__ int s_tile;
+    __shared__ uint64_t s_wsum[WG / 64];
+    __shared__ uint64_t s_prefix;
+    const int t = threadIdx.x, wv = t >> 6;
+    const int64_t ns = D.num_super;
+    const int ntiles = (int)((ns + PSCAN_TILE - 1) / PSCAN_TILE);
+    const unsigned epoch = (unsigned)D.counters[CNT_EPOCH];
+    if (t == 0) s_tile = atomicAdd(&D.counters[CNT_TICKET2], 1);
+    __syncthreads();
+    const int tile = s_tile;
+    if (tile >= ntiles) return;
+    const int64_t base = (int64_t)tile * PSCAN_TILE + (int64_t)t * PSCAN_ITEMS;
+    uint32_t fl[PSCAN_ITEMS];
+    uint64_t mine = 0;
+#pragma unroll
+    for (int k = 0; k < PSCAN_ITEMS; k++) {
+        const int64_t sp = base + k;
+        fl[k] = sp < ns && D.super_touched[sp] != 0;
+        mine += fl[k];
+    }
+    uint64_t total = 0;
+    uint64_t ex = blockExclusiveScan(mine, s_wsum, &total);
+    if (wv == 0) {
+        const uint64_t pre = lookBack(D, D.super_state, tile, total, epoch);
+        if ((t & 63) == 0) {
+            s_prefix = pre;
+            if (tile == ntiles - 1) {
+                D.counters[CNT_SUPER] = (int)(pre + total);
+                D.counters[CNT_BLOCKS] = 0;
+            }
+        }
+    }
+    __syncthreads();
+    ex += s_prefix;
+#pragma unroll
+    for (int k = 0; k < PSCAN_ITEMS; k++) {
+        const int64_t sp = base + k;
+        if (fl[k]) {
+            D.super_list[ex] = (int)sp;
+            D.super_touched[sp] = 0;
+        }
+        ex += fl[k];
     }
 }
 
 // Single-pass scan (tiles taken in ticket order, each adding its predecessors'
-// published prefixes: decoupled look-back) over the pages of the pair
-// (activated, particle count).  Epilogue per page: dense block id or -1 in the
-// page table, and for each block its page, first sorted position and particle
-// count.  The flags and counts are cleared for the next substep.  A tile waits
-// only on tiles that took earlier tickets, which are running; the wait is also
-// bounded (SCAN_SPIN_LIMIT polls, then ST_SCAN is latched).
+// published prefixes) of the pair (activated, particle count) over the pages of
+// the touched super pages, one super page per tile, in page order.  Persistent:
+// each workgroup takes tickets until the super pages run out.  Epilogue per
+// page: dense block id or -1 in the page table, and for each block its page,
+// first sorted position and particle count.  The flags and counts are cleared
+// for the next substep.  Pages of untouched super pages keep stale table
+// entries, which nothing reads: every page a particle or a block's neighbour
+// lookup names was touched.
 template <class T, class AccT>
 __global__ __launch_bounds__(WG) void k_page_scan(Dev<T, AccT> D)
 {
     __shared__ int s_tile;
     __shared__ uint64_t s_wsum[WG / 64];
     __shared__ uint64_t s_prefix;
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int t = threadIdx.x, wv = t >> 6;
     const int64_t np = D.num_pages;
-    const int ntiles = (int)((np + PSCAN_TILE - 1) / PSCAN_TILE);
+    const int ntiles = D.counters[CNT_SUPER];
     const unsigned epoch = (unsigned)D.counters[CNT_EPOCH];
-    if (t == 0) s_tile = atomicAdd(&D.counters[CNT_TICKET], 1);
-    __syncthreads();
-    const int tile = s_tile;
-    if (tile >= ntiles) return;
-    const int64_t base = (int64_t)tile * PSCAN_TILE + (int64_t)t * PSCAN_ITEMS;
-    uint32_t fl[PSCAN_ITEMS], ct[PSCAN_ITEMS];
-    uint64_t mine = 0;
+    for (;;) {
+        if (t == 0) s_tile = atomicAdd(&D.counters[CNT_TICKET], 1);
+        __syncthreads();
+        const int tile = s_tile;
+        __syncthreads(); // s_tile read by every thread before the next ticket
+        if (tile >= ntiles) return;
+        const int64_t base = (int64_t)D.super_list[tile] * PSCAN_TILE + (int64_t)t * PSCAN_ITEMS;
+        uint32_t fl[PSCAN_ITEMS], ct[PSCAN_ITEMS];
+        uint64_t mine = 0;
 #pragma unroll
-    for (int k = 0; k < PSCAN_ITEMS; k++) {
-        const int64_t pg = base + k;
-        fl[k] = 0;
-        ct[k] = 0;
-        if (pg < np) {
-            fl[k] = D.touched[pg] != 0;
-            ct[k] = (uint32_t)D.page_count[pg];
+        for (int k = 0; k < PSCAN_ITEMS; k++) {
+            const int64_t pg = base + k;
+            fl[k] = 0;
+            ct[k] = 0;
+            if (pg < np) {
+                fl[k] = D.touched[pg] != 0;
+                ct[k] = (uint32_t)D.page_count[pg];
+            }
+            mine += ((uint64_t)fl[k] << 32) | ct[k];
         }
-        mine += ((uint64_t)fl[k] << 32) | ct[k];
-    }
-    const uint64_t inc = waveInclusiveScan64(mine);
-    if (lane == 63) s_wsum[wv] = inc;
-    __syncthreads();
-    uint64_t off = 0, total = 0;
+        uint64_t total = 0;
+        uint64_t ex = blockExclusiveScan(mine, s_wsum, &total);
+        if (wv == 0) {
+            const uint64_t pre = lookBack(D, D.scan_state, tile, total, epoch);
+            if ((t & 63) == 0) {
+                s_prefix = pre;
+                if (tile == ntiles - 1) {
+                    const int64_t blocks = (int64_t)((pre + total) >> 32);
+                    if (blocks > D.cap_blocks) latch(D.counters, ST_CAPACITY);
+                    D.counters[CNT_BLOCKS] = (int)min<int64_t>(blocks, D.cap_blocks);
+                }
+            }
+        }
+        __syncthreads();
+        ex += s_prefix;
 #pragma unroll
-    for (int w = 0; w < WG / 64; w++) {
-        if (w < wv) off += s_wsum[w];
-        total += s_wsum[w];
-    }
-    // wavefront 0 publishes and looks back 64 predecessors per poll: lane l reads
-    // tile - 1 - l; the prefix is the sum up to the nearest inclusive word
-    if (wv == 0) {
-        uint64_t pre = 0;
-        if (tile == 0) {
-            if (lane == 0)
-                __hip_atomic_store(&D.scan_state[0], scanWord(epoch, 2, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        else {
-            if (lane == 0)
-                __hip_atomic_store(&D.scan_state[tile], scanWord(epoch, 1, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int base_j = tile - 1, spins = 0;
-            while (true) {
-                const int j = base_j - lane;
-                uint64_t w = scanWord(epoch, 2, 0); // before tile 0: an inclusive zero
-                if (j >= 0) w = __hip_atomic_load(&D.scan_state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned we = (unsigned)(w >> 55), wf = (unsigned)(w >> 53) & 3u;
-                const bool valid = we == epoch && wf != 0;
-                const uint64_t incl = __ballot(valid && wf == 2);
-                const int first = incl ? __ffsll((unsigned long long)incl) - 1 : 63;
-                const uint64_t need = (first == 63) ? ~0ull : ((2ull << first) - 1ull);
-                if (__ballot(!valid) & need) { // a predecessor up to `first` is not published yet
-                    if (++spins > SCAN_SPIN_LIMIT) {
-                        if (lane == 0) latch(D.counters, ST_SCAN);
-                        break;
+        for (int k = 0; k < PSCAN_ITEMS; k++) {
+            const int64_t pg = base + k;
+            if (pg < np) {
+                int id = -1;
+                if (fl[k]) {
+                    const int bid = (int)(ex >> 32);
+                    D.touched[pg] = 0;
+                    if (bid < D.cap_blocks) {
+                        id = bid;
+                        D.block_page[bid] = (int)pg;
+                        D.block_start[bid] = (int)(ex & 0xffffffffull);
+                        D.block_count[bid] = (int)ct[k];
                     }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
                 }
-                uint64_t v = lane <= first ? scanPair(w) : 0ull;
-#pragma unroll
-                for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-                pre += v;
-                if (incl) break;
-                base_j -= 64;
+                if (ct[k]) D.page_count[pg] = 0;
+                D.page_table[pg] = id;
             }
-            if (lane == 0)
-                __hip_atomic_store(&D.scan_state[tile], scanWord(epoch, 2, pre + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ex += ((uint64_t)fl[k] << 32) | ct[k];
         }
-        if (lane == 0) {
-            s_prefix = pre;
-            if (tile == ntiles - 1) {
-                const int64_t blocks = (int64_t)((pre + total) >> 32);
-                if (blocks > D.cap_blocks) latch(D.counters, ST_CAPACITY);
-                D.counters[CNT_BLOCKS] = (int)min<int64_t>(blocks, D.cap_blocks);
-            }
-        }
-    }
-    __syncthreads();
-    uint64_t ex = s_prefix + off + inc - mine;
-#pragma unroll
-    for (int k = 0; k < PSCAN_ITEMS; k++) {
-        const int64_t pg = base + k;
-        if (pg < np) {
-            int id = -1;
-            if (fl[k]) {
-                const int bid = (int)(ex >> 32);
-                D.touched[pg] = 0;
-                if (bid < D.cap_blocks) {
-                    id = bid;
-                    D.block_page[bid] = (int)pg;
-                    D.block_start[bid] = (int)(ex & 0xffffffffull);
-                    D.block_count[bid] = (int)ct[k];
-                }
-            }
-            if (ct[k]) D.page_count[pg] = 0;
-            D.page_table[pg] = id;
-        }
-        ex += ((uint64_t)fl[k] << 32) | ct[k];
+        __syncthreads(); // s_wsum and s_prefix free for the next tile
     }
 }
 
@@ -1195,7 +1281,7 @@ __global__ __launch_bounds__(WG, OCC) void k_p2g(Dev<T, AccT> D)
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     T d0;
-                    cellOf(raw.x[a], sd.odx, sd.origin[a], sd.domain, base[a], d0); // errors latched by the binning
+                    cellOf(raw.x[a], sd.odx, sd.origin[a], sd.domain, base[a], d0, sd.pos_bits); // errors latched by the binning
                     bsplineFrac(d0, w[a], dw[a]);
                     if constexpr (sizeof(T) == 4)
                         xb[a] = -d0 * sd.dx; // x_base - x_p
@@ -1865,7 +1951,7 @@ __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const SceneDev<
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         T d0;
-        cellOf(Xp[a], odx, sd.origin[a], sd.domain, base[a], d0);
+        cellOf(Xp[a], odx, sd.origin[a], sd.domain, base[a], d0, sd.pos_bits);
         bsplineFrac(d0, w[a], dw[a]);
 #pragma unroll
         for (int i = 0; i < 3; i++) { // x_i - x_p
@@ -2554,8 +2640,8 @@ __global__ __launch_bounds__(WG) void k_damp_g2p(Dev<T, AccT> D)
 // reference's initial float lowest.
 constexpr int EXTENT_WGS = 256;
 
-__device__ __forceinline__ double posMetres(int X, double dx) { return (double)X * dx * (1.0 / (double)(1 << POS_BITS)); }
-__device__ __forceinline__ double posMetres(double x, double) { return x; }
+__device__ __forceinline__ double posMetres(int X, double dx, int pb) { return (double)X * dx * (1.0 / (double)(1 << pb)); }
+__device__ __forceinline__ double posMetres(double x, double, int) { return x; }
 
 __device__ __forceinline__ double blockMax(double v)
 {
@@ -2593,10 +2679,8 @@ __global__ __launch_bounds__(WG) void k_extent(Dev<T, AccT> D, int fold, int row
 #pragma unroll
             for (int a = 0; a < 3; a++) {
                 // in the scene's own frame
-                PosT<T> X = loadPos(D.P + (PX + a) * cap, p);
-                if constexpr (sizeof(T) == 4)
-                    if (X != POS_NAN) X -= sd.origin[a] << POS_BITS;
-                const double x = posMetres(X, sd.ddx);
+                const PosT<T> X = loadPos(D.P + (PX + a) * cap, p);
+                const double x = posMetres(X, sd.ddx, sd.pos_bits);
                 r[1 + a] = fmax(r[1 + a], x);
                 r[4 + a] = fmax(r[4 + a], -x);
             }
@@ -2813,7 +2897,8 @@ __global__ __launch_bounds__(WG) void k_dtilde(Dev<T, AccT> D)
                     kap = vol * dte * mt.l0 * mt.l0;
                     int base;
 #pragma unroll
-                    for (int a = 0; a < 3; a++) cellOf(loadPos(D.P + (PX + a) * cap, p), sd.odx, sd.origin[a], sd.domain, base, d0[a]);
+                    for (int a = 0; a < 3; a++)
+                        cellOf(loadPos(D.P + (PX + a) * cap, p), sd.odx, sd.origin[a], sd.domain, base, d0[a], sd.pos_bits);
                     c3[0] = (double)vol;                                             // sum vol w
                     c3[1] = (double)((dt * hist) / mt.eta) * (double)vol;            // D~ part of rhs
                     c3[2] = (double)vol * (1.0 + (double)dte * ((double)hist + 1.0)); // diagonal

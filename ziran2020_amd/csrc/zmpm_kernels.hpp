@@ -24,7 +24,7 @@ namespace zmpm {
 
 // particle scalar fields (field-major SoA)
 //   FP64 form: x in metres and F, as the reference stores them.
-//   FP32 form: x as 32-bit fixed point in units of dx / 2^POS_BITS (the bits of
+//   FP32 form: x as 32-bit fixed point in units of dx / 2^pos_bits (the bits of
 //   an int32 in the float slot), and H = F - I instead of F (EOS: J - 1 in
 //   PF+0).  A float x carries ~|x| 2^-24 of rounding, which near 1 m is 1e-5 of
 //   a cell at dx = 1/256, and a float F near I loses the strain F - I to
@@ -49,13 +49,13 @@ enum : int {
     NPF = 38
 };
 
-// FP32-form position: X = round(x / dx * 2^POS_BITS) as int32.  Domains are at
-// most 1024 nodes per axis, so every in-domain X is below 2^31.  POS_NAN marks
-// a non-finite position (an upload of NaN / Inf or a non-finite advection);
+// FP32-form position: X = round(x / dx * 2^b) as int32 in the scene's own
+// frame, b = posBits(the scene's domain): 21 for domains of up to 1024 nodes per
+// axis, 19 up to 4096, so every in-domain X is below 2^31.  POS_NAN marks a
+// non-finite position (an upload of NaN / Inf or a non-finite advection);
 // out-of-range values are clamped to [POS_MIN, POS_MAX], where the binning's
 // domain check rejects them.
-constexpr int POS_BITS = 21;
-constexpr int POS_HALF = 1 << (POS_BITS - 1); // half a cell: base = floor(X / 2^POS_BITS - 1/2)
+__host__ __device__ constexpr int posBits(int domain_nodes) { return domain_nodes <= 1024 ? 21 : 19; }
 constexpr int POS_NAN = (int)0x80000000;
 constexpr int POS_MIN = -(1 << 30);
 constexpr int POS_MAX = 0x7ffffffe;
@@ -177,7 +177,8 @@ struct Masks {
 template <class T>
 struct SceneDev {
     T dx, odx;
-    T pos_scale;   // FP32 form: metres -> position units, odx * 2^POS_BITS
+    T pos_scale;   // FP32 form: metres -> position units, odx * 2^pos_bits
+    int pos_bits;  // FP32 form: fraction bits of the fixed-point position (posBits)
     T Dinv;        // 4 / dx^2 (MpmSimulationBase.cpp:119-122)
     T gravity[3];
     T rpic;        // apic_rpic_ratio
@@ -228,6 +229,13 @@ struct Dev {
     int* page_table;   // per page: dense block id or -1
     int64_t num_pages;
     uint64_t* scan_state; // per page-scan tile: published prefix (see k_page_scan)
+    // super pages (runs of PSCAN_TILE consecutive pages, one page-scan tile each):
+    // flags set with the page flags, compacted in order into super_list, so the
+    // page scan visits only the touched ones
+    int64_t num_super;
+    int* super_touched;
+    int* super_list;
+    uint64_t* super_state; // per super-page-scan tile: published prefix
     int* block_page;   // dense block id -> page index
     int* block_start;  // dense block id -> first sorted position
     int* block_count;  // dense block id -> particles
