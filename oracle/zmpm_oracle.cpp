@@ -2541,9 +2541,25 @@ zmpm_status zmpm_set_grid_velocity(zmpm_handle h, int64_t n_nodes, const double*
 
 zmpm_status zmpm_get_grid(zmpm_handle h, int64_t* n_nodes, int32_t* coords, double* m, double* v, double* new_v, double* pf)
 {
-    if (h && !h->subs.empty()) return fail(h, ZMPM_E_ARG, "the oracle reports a batch's grid per scene only (a handle per scene)");
     if (!h || !n_nodes) return ZMPM_E_ARG;
-    std::vector<GridRec> recs = gridRecords(h);
+    std::vector<GridRec> recs;
+    if (!h->subs.empty()) {
+        // a batch: every scene's nodes at its region's origin in the handle's grid
+        for (size_t sc = 0; sc < h->subs.size(); sc++) {
+            std::vector<GridRec> r = gridRecords(h->subs[sc]);
+            for (GridRec& g : r)
+                for (int a = 0; a < 3; a++) g.c[a] += h->origins[sc][a];
+            recs.insert(recs.end(), r.begin(), r.end());
+        }
+        std::sort(recs.begin(), recs.end(), [](const GridRec& a, const GridRec& b) {
+            if (a.c[0] != b.c[0]) return a.c[0] < b.c[0];
+            if (a.c[1] != b.c[1]) return a.c[1] < b.c[1];
+            return a.c[2] < b.c[2];
+        });
+    }
+    else
+        recs = gridRecords(h);
+    const bool any_damage = h->subs.empty() ? h->any_damage : h->subs[0]->any_damage;
     *n_nodes = (int64_t)recs.size();
     for (size_t q = 0; q < recs.size(); q++) {
         const Node& g = *recs[q].g;
@@ -2554,7 +2570,7 @@ zmpm_status zmpm_get_grid(zmpm_handle h, int64_t* n_nodes, int32_t* coords, doub
             for (int a = 0; a < 3; a++) v[3 * q + a] = g.v[a];
         if (new_v)
             for (int a = 0; a < 3; a++) new_v[3 * q + a] = g.new_v[a];
-        if (pf) pf[q] = h->any_damage ? g.pf : 0.0;
+        if (pf) pf[q] = any_damage ? g.pf : 0.0;
     }
     return ZMPM_OK;
 }
