@@ -436,8 +436,8 @@ def test_damage_slab_pic(gpu, oracle, precision):
 @pytest.mark.parametrize("precision", [A.PREC_FP32, A.PREC_FP64])
 @pytest.mark.parametrize("transfer", ["apic", "flip_damped", "implicit_damage"])
 def test_graph_replays_match_eager_substeps(gpu, precision, transfer):
-    """zmpm_run replays a captured two-substep HIP graph; every replay, with host
-    downloads in between, matches eager one-at-a-time substeps (bitwise for the
+    """zmpm_run replays captured HIP graphs of eight and of two substeps; every
+    replay, with host downloads in between, matches eager one-at-a-time substeps (bitwise for the
     deterministic FP32 form, to FP64 rounding otherwise; the implicit damage
     solve sums in double with HBM atomics, so there the FP32 form agrees to float
     rounding)."""
@@ -461,15 +461,19 @@ def test_graph_replays_match_eager_substeps(gpu, precision, transfer):
                 assert np.array_equal(a[k], b[k]), (rep, k)
             else:
                 assert normwise(a[k], b[k]) < tol, (rep, k)
-    # an odd step count ends on the other buffer; the next run captures its own graph
-    g.run(3, sc.dt, t)
-    for k in range(3):
-        e.substep(sc.dt, t + k * sc.dt)
-    a, b = g.download(), e.download()
-    if exact:
-        assert np.array_equal(a["x"], b["x"])
-    else:
-        assert normwise(a["x"], b["x"]) < tol
+    # an odd step count ends on the other buffer; the next run captures its own
+    # graphs; 11 = one eight-substep graph, a pair and one eager substep
+    for n in (3, 11, 11):
+        g.run(n, sc.dt, t)
+        for k in range(n):
+            e.substep(sc.dt, t + k * sc.dt)
+        t += n * sc.dt
+        a, b = g.download(), e.download()
+        for k in ("x", "v", "F"):
+            if exact:
+                assert np.array_equal(a[k], b[k]), (n, k)
+            else:
+                assert normwise(a[k], b[k]) < tol, (n, k)
 
 
 @pytest.mark.parametrize("case", ["corotated", "corotated_hardening", "damage_slab"])

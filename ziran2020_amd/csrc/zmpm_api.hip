@@ -32,6 +32,7 @@ constexpr int kMaxMaterials = 256;
 constexpr int kMaxColliders = MAX_COLLIDERS;
 constexpr int kTimedGroups = 5;
 constexpr int kPageScanWgs = 1024; // persistent page-scan workgroups (at most one per super page)
+constexpr int kGraphSteps = 8;     // substeps per captured graph of zmpm_run (even)
 // compiled kernel variants the launch configuration selects
 constexpr int VARIANT_P2G_LOW_OCC = 1; // P2G compiled for one wave per SIMD fewer
 constexpr int VARIANT_G2P_RR = 2;      // G2P segments round-robin over the XCDs
@@ -113,9 +114,11 @@ struct zmpm_context {
     double timed_ms[kTimedGroups + 1] = {};
     bool stepped = false;
     std::vector<double> clock_host; // per scene: next and running substep time (upload source)
-    // captured two-substep graphs, keyed by live buffer (dt and t are device state)
+    // captured graphs of kGraphSteps or two substeps, keyed by live buffer and
+    // length (dt and t are device state)
     struct Graph {
         const void* live;
+        int steps;
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
@@ -1357,13 +1360,14 @@ zmpm_status zmpm_download(zmpm_handle h, zmpm_particles* p)
     return s != ZMPM_OK ? s : st;
 }
 
-// Two substeps captured into one HIP graph (the particle buffers ping-pong, so
-// a pair returns to the starting buffer); cached per live buffer and dt.
-static zmpm_status graphFor(zmpm_handle h, hipGraphExec_t* out)
+// `n` (even) substeps captured into one HIP graph (the particle buffers
+// ping-pong, so an even count returns to the starting buffer); cached per live
+// buffer and n.
+static zmpm_status graphFor(zmpm_handle h, int n, hipGraphExec_t* out)
 {
     const void* live = h->fp32 ? (const void*)h->Df.P : (const void*)h->Dd.P;
     for (auto& g : h->graphs)
-        if (g.live == live) {
+        if (g.live == live && g.steps == n) {
             *out = g.exec;
             return ZMPM_OK;
         }
@@ -1371,7 +1375,7 @@ static zmpm_status graphFor(zmpm_handle h, hipGraphExec_t* out)
     hipGraph_t graph = nullptr;
     if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return fail(h, ZMPM_E_HIP, "hipStreamBeginCapture");
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < n; k++) {
         if (h->fp32)
             launchStep<float, long long>(h, nullptr);
         else
@@ -1382,7 +1386,7 @@ static zmpm_status graphFor(zmpm_handle h, hipGraphExec_t* out)
     hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(h, ZMPM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    h->graphs.push_back({ live, exec });
+    h->graphs.push_back({ live, n, exec });
     *out = exec;
     return ZMPM_OK;
 }
@@ -1435,11 +1439,15 @@ static zmpm_status runSteps(zmpm_handle h, int32_t steps, const double* dt, cons
     // launch is not captured into graphs
     const bool coop = h->fp32 ? (h->Df.implicit && h->Df.any_damage) : (h->Dd.implicit && h->Dd.any_damage);
     if (h->use_graphs && !h->timing && steps >= 2 && !coop) {
-        hipGraphExec_t exec = nullptr;
-        zmpm_status st = graphFor(h, &exec);
-        if (st != ZMPM_OK) return st;
-        for (; s + 2 <= steps; s += 2)
-            if (hipGraphLaunch(exec, h->stream) != hipSuccess) return fail(h, ZMPM_E_HIP, "hipGraphLaunch");
+        // runs of kGraphSteps substeps, then pairs, replayed from captured graphs
+        for (const int n : { kGraphSteps, 2 }) {
+            if (s + n > steps) continue;
+            hipGraphExec_t exec = nullptr;
+            zmpm_status st = graphFor(h, n, &exec);
+            if (st != ZMPM_OK) return st;
+            for (; s + n <= steps; s += n)
+                if (hipGraphLaunch(exec, h->stream) != hipSuccess) return fail(h, ZMPM_E_HIP, "hipGraphLaunch");
+        }
     }
     for (; s < steps; s++) {
         if (h->timing) ev = &h->events[(size_t)s * (kTimedGroups + 1)];
