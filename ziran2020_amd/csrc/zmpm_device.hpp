@@ -52,6 +52,17 @@ constexpr long long FIXED_MAGIC_BITS = 0x4338000000000000LL;
 
 __device__ __forceinline__ void latch(int* counters, int bits) { atomicOr(&counters[1], bits); }
 
+// Workgroup barrier ordering LDS only: the workgroup's global loads and
+// atomics stay in flight across it (a full __syncthreads waits for them), so a
+// prefetch issued before it keeps hiding its latency.  Used where threads
+// exchange data through LDS alone.
+__device__ __forceinline__ void ldsBarrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Workgroup ids are dispatched round-robin over the 8 XCDs (each with its own
 // L2).  The logical index below gives each XCD one contiguous run of the grid's
 // work items (G2P's particle segments), so neighbours that read the same halo
@@ -712,7 +723,7 @@ __device__ __forceinline__ void blockOrder(const Dev<T, AccT>& D, int b, int sta
     const int t = threadIdx.x;
     const bool staged = cnt <= stage;
     if (t < NB) hist[t] = 0;
-    __syncthreads();
+    ldsBarrier();
     // stage (key, live index) and count cells; four rounds of loads in flight
     for (int i0 = t; i0 < cnt; i0 += 4 * WG) {
         uint32_t kk[4] = { 0u, 0u, 0u, 0u };
@@ -737,7 +748,7 @@ __device__ __forceinline__ void blockOrder(const Dev<T, AccT>& D, int b, int sta
             }
         }
     }
-    __syncthreads();
+    ldsBarrier();
     if (t < 64) {
         const int v = t < NB ? hist[t] : 0;
         const int ex = waveInclusiveScan(v) - v;
@@ -746,7 +757,7 @@ __device__ __forceinline__ void blockOrder(const Dev<T, AccT>& D, int b, int sta
             cur[t] = ex;
         }
     }
-    __syncthreads();
+    ldsBarrier();
     if (staged) {
         // group by cell
         for (int i = t; i < cnt; i += WG) {
@@ -755,7 +766,7 @@ __device__ __forceinline__ void blockOrder(const Dev<T, AccT>& D, int b, int sta
             skb[dst] = k;
             pb[dst] = pa[i];
         }
-        __syncthreads();
+        ldsBarrier();
         // rank inside the cell (keys are unique); write the sorted block.  The
         // cell's keys are read four at a time so the LDS reads overlap.
         for (int i = t; i < cnt; i += WG) {
@@ -810,9 +821,9 @@ __device__ __forceinline__ void blockOrder(const Dev<T, AccT>& D, int b, int sta
         D.cell_start[b * NB + t] = start + cst[t];
         D.cell_count[b * NB + t] = hist[t];
     }
-    // the sorted block is read back (perm) by this workgroup
-    __threadfence_block();
-    __syncthreads();
+    // LDS free for the next block; the sorted block (perm, cell starts) is read
+    // back by this workgroup after the full barrier that ends its ordering loop
+    ldsBarrier();
 }
 
 // ------------------------------------------------------------- P2G helpers
@@ -1215,7 +1226,7 @@ __global__ __launch_bounds__(WG, OCC) void k_p2g(Dev<T, AccT> D)
     __shared__ AccT tile[NGF][TL::TN];
     __shared__ int cs[NB], ce[NB], cl[NB];
     __shared__ int hist[NB], cur[NB], cst[NB];
-    __shared__ int ncl;
+    __shared__ int ncl, nbl[8], nxr[2];
     // the in-block sort stages (key, live index) in the payload area and the
     // grouped copy in the tile area, both free until the block's first chunk
     constexpr int SA = (int)(sizeof(q) / 8), SB = (int)(sizeof(tile) / 8);
@@ -1247,18 +1258,40 @@ __global__ __launch_bounds__(WG, OCC) void k_p2g(Dev<T, AccT> D)
             }
             continue;
         }
-        __syncthreads(); // LDS of the previous block's order done
+        ldsBarrier(); // LDS of the previous block's order done
         blockOrder(D, b, bs, cnt, ska, pa, skb, pb, STAGE, hist, cur, cst);
     }
+    // the order just written (perm, cell starts) is read below: a full barrier;
+    // the loop below exchanges data through LDS only
+    __syncthreads();
     int ovf = 0;
-    for (int b = wgl; b < nblocks; b += gridDim.x) {
-        __syncthreads(); // previous block flushed
+    bool have = false; // raw holds this thread's particle of the coming chunk
+    RawP2G<T> raw;
+    // cell ranges and neighbour table of the workgroup's next block, loaded a
+    // block ahead (thread t < NB: cell t; threads 64..71: neighbour t - 64)
+    int ns = 0, ne = 0, nn = -1;
+    auto fetchBlock = [&](int bb) {
+        if (bb >= nblocks) return;
         if (t < NB) {
-            const int c = b * NB + t;
-            cs[t] = D.cell_start[c];
-            ce[t] = D.cell_start[c] + D.cell_count[c];
+            const int c = bb * NB + t;
+            ns = D.cell_start[c];
+            ne = ns + D.cell_count[c];
         }
-        __syncthreads();
+        else if (t >= 64 && t < 72)
+            nn = D.nbr[bb * 8 + t - 64];
+    };
+    fetchBlock(wgl);
+    for (int b = wgl; b < nblocks; b += gridDim.x) {
+        ldsBarrier(); // previous block flushed
+        if (t < NB) {
+            cs[t] = ns;
+            ce[t] = ne;
+        }
+        else if (t >= 64 && t < 72)
+            nbl[t - 64] = nn;
+        const int bn = b + (int)gridDim.x;
+        fetchBlock(bn);
+        ldsBarrier();
         const int bs = cs[0], be = ce[NB - 1];
         if (be <= bs) continue; // halo-only block
         for (int e = t; e < nq * TL::TN; e += WG) (&tile[0][0])[e] = AccT(0);
@@ -1267,11 +1300,9 @@ __global__ __launch_bounds__(WG, OCC) void k_p2g(Dev<T, AccT> D)
         const SceneDev<T>& sd = D.sc[bscene];
         const float* sc = D.scales + bscene * NSC;
         const float scl[NGF] = { sc[SC_M], sc[SC_MV], sc[SC_MV], sc[SC_MV], sc[SC_F], sc[SC_F], sc[SC_F], sc[SC_PF], sc[SC_PW] };
-        bool have = false;
-        RawP2G<T> raw;
         for (int lo = bs; lo < be; lo += CH) {
             const int hi = min(lo + CH, be);
-            __syncthreads(); // previous chunk's phase B done with the payload
+            ldsBarrier(); // previous chunk's phase B done with the payload
             // ---- phase A: payload of particle lo + t from the prefetched fields
             if (t < CH && lo + t < hi) {
                 if (!have) loadP2G<T, AccT, DAMP>(D, lo + t, dmg, raw);
@@ -1327,13 +1358,25 @@ __global__ __launch_bounds__(WG, OCC) void k_p2g(Dev<T, AccT> D)
                 if (act) cl[__popcll(m & ((1ull << t) - 1))] = t;
                 if (t == 0) ncl = __popcll(m);
             }
-            __syncthreads();
-            // float form: prefetch the next chunk of this block while phase B runs
-            // (the double form keeps its registers for phase B)
+            // the next block's particle range, for the prefetch of its first chunk
+            if (sizeof(T) == 4 && hi == be && bn < nblocks) {
+                if (t == 0) nxr[0] = ns;
+                if (t == NB - 1) nxr[1] = ne;
+            }
+            ldsBarrier();
+            // float form: prefetch the next chunk of this block, or after its last
+            // chunk the first chunk of the next block, while phase B runs (the
+            // double form keeps its registers for phase B)
             have = false;
-            if (sizeof(T) == 4 && t < CH && hi + t < be) {
-                loadP2G<T, AccT, DAMP>(D, hi + t, dmg, raw);
-                have = true;
+            if (sizeof(T) == 4 && t < CH) {
+                if (hi + t < be) {
+                    loadP2G<T, AccT, DAMP>(D, hi + t, dmg, raw);
+                    have = true;
+                }
+                else if (hi == be && bn < nblocks && nxr[0] + t < nxr[1]) {
+                    loadP2G<T, AccT, DAMP>(D, nxr[0] + t, dmg, raw);
+                    have = true;
+                }
             }
             // ---- phase B: lanes = (sub-lane, cell, i, j); R sub-lanes share a column
             const int per = ncl * 9;
@@ -1415,16 +1458,15 @@ __global__ __launch_bounds__(WG, OCC) void k_p2g(Dev<T, AccT> D)
                 if (FIXED && !(chk <= FIXED_LANE_LIMIT)) ovf = 1;
             }
         }
-        __syncthreads();
+        ldsBarrier();
         // ---- flush the tile: one HBM atomic per non-zero node value
-        const int* nb = D.nbr + b * 8;
         for (int e = t; e < nq * TL::TN; e += WG) {
             const int f = e / TL::TN, nd = e - TL::TN * (e / TL::TN);
             const AccT v = tile[f][nd];
             if (v == AccT(0)) continue;
             const int tx = nd / (TL::TY * TL::TZ), ty = (nd / TL::TZ) % TL::TY, tz = nd % TL::TZ;
             const int which = ((tx >= TL::BX) << 2) | ((ty >= TL::BY) << 1) | (tz >= TL::BZ);
-            const int tb = nb[which];
+            const int tb = nbl[which];
             if (tb < 0) {
                 ovf |= 2;
                 continue;
