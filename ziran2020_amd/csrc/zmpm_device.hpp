@@ -1983,7 +1983,7 @@ struct G2PAcc {
 // the stencil's tile position from that cell of the segment's block (the damping
 // passes: this substep's binning, moved particle); otherwise from the particle.
 template <class T, class AccT, bool FLIP = false>
-__device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const SceneDev<T>& sd, const T (*gt)[Tile<T>::TN],
+__device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const SceneDev<T>& sd, const V4<T> (*gt)[Tile<T>::TN],
                                           const PosT<T>* Xp, bool dmg, G2PAcc<T>& A, int cell = -1)
 {
     using TL = Tile<T>;
@@ -2026,16 +2026,23 @@ __device__ __forceinline__ void g2pGather(const Dev<T, AccT>& D, const SceneDev<
             T p0 = T(0), p1 = T(0);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
+                const V4<T> n4 = gt[0][nrow + k];
+                const T nvs[3] = { n4.x, n4.y, n4.z };
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    const T nv = gt[a][nrow + k];
+                    const T nv = nvs[a];
                     s0[a] += w[2][k] * nv;
                     s1[a] += dw[2][k] * nv;
                     s2[a] += wzo[k] * nv;
-                    if constexpr (FLIP) so[a] += w[2][k] * gt[4 + a][nrow + k];
+                }
+                if constexpr (FLIP) {
+                    const V4<T> o4 = gt[1][nrow + k];
+                    so[0] += w[2][k] * o4.x;
+                    so[1] += w[2][k] * o4.y;
+                    so[2] += w[2][k] * o4.z;
                 }
                 if (dmg) {
-                    const T ph = gt[3][nrow + k];
+                    const T ph = n4.w;
                     p0 += w[2][k] * ph;
                     p1 += ph;
                 }
@@ -2269,29 +2276,42 @@ __device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const Ra
 }
 
 // Grid fields of block b's tile (the block and its forward halo, through the
-// neighbour table) into gt: field f < nf is 0-2 new velocity, 3 phase field
-// (staged only with dmg), 4-6 the grid's velocity before the update (FLIP).
+// neighbour table) into gt, four to a node record: field f < nf is 0-2 new
+// velocity, 3 phase field (staged only with dmg) in gt[0]; 4-6 the grid's
+// velocity before the update (FLIP) in gt[1].
 template <class T, class AccT>
-__device__ __forceinline__ void stageTile(const Dev<T, AccT>& D, int b, T (*gt)[Tile<T>::TN], int nf, bool dmg)
+__device__ __forceinline__ void stageTile(const Dev<T, AccT>& D, int b, V4<T> (*gt)[Tile<T>::TN], int nf, bool dmg)
 {
     using B = Blk<T>;
     using TL = Tile<T>;
     constexpr int NB = TL::NB;
     const int64_t cn = D.cap_nodes;
+    // the neighbour table row, read once (a uniform address) and selected per node
     const int* nb = D.nbr + b * 8;
+    int nbv[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) nbv[i] = nb[i];
     for (int e = threadIdx.x; e < nf * TL::TN; e += WG) {
         const int f = e / TL::TN, nd = e - TL::TN * (e / TL::TN);
         if (f == 3 && !dmg) continue;
         const int tx = nd / (TL::TY * TL::TZ), ty = (nd / TL::TZ) % TL::TY, tz = nd % TL::TZ;
         const int which = ((tx >= TL::BX) << 2) | ((ty >= TL::BY) << 1) | (tz >= TL::BZ);
-        const int tb = nb[which];
+        int tb = nbv[0];
+#pragma unroll
+        for (int i = 1; i < 8; i++) tb = which == i ? nbv[i] : tb;
         T v = T(0);
         if (tb >= 0) {
             const int elem = ((tx & (TL::BX - 1)) << (B::ybits + B::zbits)) | ((ty & (TL::BY - 1)) << B::zbits) | (tz & (TL::BZ - 1));
             const int64_t node = (int64_t)tb * NB + elem;
             v = f < 3 ? D.newv[f * cn + node] : (f == 3 ? D.pfn[node] : D.oldv[(f - 4) * cn + node]);
         }
-        gt[f][nd] = v;
+        V4<T>& r = gt[f >> 2][nd];
+        switch (f & 3) {
+        case 0: r.x = v; break;
+        case 1: r.y = v; break;
+        case 2: r.z = v; break;
+        default: r.w = v; break;
+        }
     }
 }
 
@@ -2303,7 +2323,7 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D)
     using TL = Tile<T>;
     constexpr int NB = TL::NB;
     constexpr int NGT = FLIP ? 7 : 4;
-    __shared__ T gt[NGT][TL::TN];
+    __shared__ V4<T> gt[(NGT + 3) / 4][TL::TN];
     __shared__ int seg_end;
     const int lo = (RR ? (int)blockIdx.x : xcdLocal((int)blockIdx.x, (int)gridDim.x)) * WG;
     const int hi = min(lo + WG, D.n);
@@ -2333,13 +2353,15 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D)
     int seg_lo = lo;
     while (seg_lo < hi) {
         const int b = D.sorted_cell[seg_lo] / NB;
-        __syncthreads(); // previous segment's gathers done
+        // LDS-only barriers: the particle loads issued above stay in flight
+        // while the tile is staged
+        ldsBarrier(); // previous segment's gathers done
         if (t == 0) {
             const int c = b * NB + NB - 1;
             seg_end = D.cell_start[c] + D.cell_count[c];
         }
         stageTile(D, b, gt, nfield, dmg);
-        __syncthreads();
+        ldsBarrier();
         const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
         // the segment's particles are in its block's scene (a uniform index)
         if (live && s >= seg_lo && s < seg_hi) g2pGather<T, AccT, FLIP>(D, D.sc[blockScene(D, b)], gt, R.x, dmg, A);
@@ -2615,7 +2637,7 @@ __global__ __launch_bounds__(WG) void k_damp_g2p(Dev<T, AccT> D)
 {
     using TL = Tile<T>;
     constexpr int NB = TL::NB;
-    __shared__ T gt[3][TL::TN];
+    __shared__ V4<T> gt[1][TL::TN];
     __shared__ int seg_end;
     const int lo = xcdLocal((int)blockIdx.x, (int)gridDim.x) * WG;
     const int hi = min(lo + WG, D.n);
