@@ -281,9 +281,8 @@ zmpm_status allocate(zmpm_context* h, int n)
     ALLOC(D.counters, 8);
     ALLOC(D.clock, 2 * ns);
     ALLOC(D.acc, (int64_t)NGF * D.cap_nodes);
-    ALLOC(D.newv, 3 * D.cap_nodes);
-    ALLOC(D.oldv, 3 * D.cap_nodes);
-    ALLOC(D.pfn, D.cap_nodes);
+    ALLOC(D.newv, 4 * D.cap_nodes);
+    ALLOC(D.oldv, 4 * D.cap_nodes);
     ALLOC(D.scales, NSC * ns);
     ALLOC(D.maxbits, (int64_t)NSC * BOUND_SLOTS * ns);
     ALLOC(D.extent, (int64_t)(EXTENT_WGS + 1) * 8);
@@ -866,7 +865,7 @@ struct HostGrid {
     };
     int64_t nn = 0;
     std::vector<AccT> acc;
-    std::vector<T> nv, pfn;
+    std::vector<T> nv; // [nn][4] node records: new velocity, phase field
     std::vector<float> sc;
     std::vector<Rec> recs;
     double val(int f, const Rec& r, int g) const
@@ -888,16 +887,13 @@ zmpm_status collectGrid(zmpm_context* h, HostGrid<T, AccT>& G)
     G.nn = nn;
     std::vector<int> bpage(nb);
     G.acc.assign((size_t)7 * nn, AccT(0));
-    G.nv.assign((size_t)3 * nn, T(0));
-    G.pfn.assign((size_t)nn, T(0));
+    G.nv.assign((size_t)4 * nn, T(0));
     const size_t ns = h->scenes.size();
     G.sc.assign(NSC * ns, 1.f);
     HIPCHK(h, hipMemcpy(bpage.data(), D.block_page, nb * sizeof(int), hipMemcpyDeviceToHost));
     for (int f = 0; f < 7; f++)
         HIPCHK(h, hipMemcpy(G.acc.data() + f * nn, D.acc + f * D.cap_nodes, nn * sizeof(AccT), hipMemcpyDeviceToHost));
-    for (int a = 0; a < 3; a++)
-        HIPCHK(h, hipMemcpy(G.nv.data() + a * nn, D.newv + a * D.cap_nodes, nn * sizeof(T), hipMemcpyDeviceToHost));
-    if (h->any_damage) HIPCHK(h, hipMemcpy(G.pfn.data(), D.pfn, nn * sizeof(T), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(G.nv.data(), D.newv, 4 * nn * sizeof(T), hipMemcpyDeviceToHost));
     // scales of the last substep's P2G (they are replaced only at the start of the next one)
     if (std::is_same<AccT, long long>::value)
         HIPCHK(h, hipMemcpy(G.sc.data(), D.scales, G.sc.size() * sizeof(float), hipMemcpyDeviceToHost));
@@ -938,7 +934,6 @@ zmpm_status gridT(zmpm_context* h, int64_t* n_nodes, int32_t* coords, double* m,
     HostGrid<T, AccT> G;
     const zmpm_status st = collectGrid<T, AccT>(h, G);
     if (st != ZMPM_OK) return st;
-    const int64_t nn = G.nn;
     *n_nodes = (int64_t)G.recs.size();
     for (size_t q = 0; q < G.recs.size(); q++) {
         const int64_t node = G.recs[q].node;
@@ -949,8 +944,8 @@ zmpm_status gridT(zmpm_context* h, int64_t* n_nodes, int32_t* coords, double* m,
         if (v)
             for (int a = 0; a < 3; a++) v[3 * q + a] = G.val(GMV + a, G.recs[q], SC_MV) / mm;
         if (new_v)
-            for (int a = 0; a < 3; a++) new_v[3 * q + a] = (double)G.nv[a * nn + node];
-        if (pf) pf[q] = h->any_damage ? (double)G.pfn[node] : 0.0;
+            for (int a = 0; a < 3; a++) new_v[3 * q + a] = (double)G.nv[4 * node + a];
+        if (pf) pf[q] = h->any_damage ? (double)G.nv[4 * node + 3] : 0.0;
     }
     return ZMPM_OK;
 }
@@ -965,10 +960,9 @@ zmpm_status setGridVelocityT(zmpm_context* h, int64_t n_nodes, const double* new
     if (n_nodes != (int64_t)G.recs.size()) return fail(h, ZMPM_E_ARG, "node count differs from zmpm_get_grid's");
     const int64_t nn = G.nn;
     for (size_t q = 0; q < G.recs.size(); q++)
-        for (int a = 0; a < 3; a++) G.nv[a * nn + G.recs[q].node] = (T)new_v[3 * q + a];
+        for (int a = 0; a < 3; a++) G.nv[4 * G.recs[q].node + a] = (T)new_v[3 * q + a];
     Dev<T, AccT>& D = devOf<T, AccT>(h);
-    for (int a = 0; a < 3; a++)
-        HIPCHK(h, hipMemcpy(D.newv + a * D.cap_nodes, G.nv.data() + a * nn, nn * sizeof(T), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(D.newv, G.nv.data(), 4 * nn * sizeof(T), hipMemcpyHostToDevice));
     return ZMPM_OK;
 }
 

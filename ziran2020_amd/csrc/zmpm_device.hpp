@@ -1135,6 +1135,14 @@ struct alignas(4 * sizeof(T)) V4 {
     T x, y, z, w;
 };
 
+// The four-field record of grid node `node` in a [nodes][4] array (the grid's
+// new and old velocities: one aligned 16- or 32-byte access per node).
+template <class T>
+__device__ __forceinline__ V4<T>& nodeRec(T* a, int64_t node)
+{
+    return reinterpret_cast<V4<T>*>(a)[node];
+}
+
 // Adds a lane's partial sum into the LDS tile.  Float form: v * scale (exact:
 // the scale is a power of two) rounded to the nearest integer as int64 -- added
 // in double onto 1.5 x 2^52 it lands in the low mantissa bits (exact for
@@ -1753,12 +1761,8 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D)
 #pragma unroll
         for (int a = 0; a < 3; a++) f[a] = accValue(D, GFO + a, node, sc, SC_F);
         if (m == T(0)) {
-#pragma unroll
-            for (int a = 0; a < 3; a++) D.newv[a * cn + node] = f[a];
-            if (D.flip)
-#pragma unroll
-                for (int a = 0; a < 3; a++) D.oldv[a * cn + node] = T(0);
-            if (D.any_damage) D.pfn[node] = T(0);
+            nodeRec(D.newv, node) = V4<T> { f[0], f[1], f[2], T(0) };
+            if (D.flip) nodeRec(D.oldv, node) = V4<T> { T(0), T(0), T(0), T(0) };
             continue;
         }
         uint32_t Bx, By, Bz;
@@ -1775,7 +1779,7 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D)
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             T v = accValue(D, GMV + a, node, sc, SC_MV) / m;
-            if (D.flip) D.oldv[a * cn + node] = v; // the grid's v before the update (FLIP)
+            if (D.flip) D.oldv[4 * node + a] = v; // the grid's v before the update (FLIP)
             vi[a] = v + sd.gravity[a] * dt + f[a] * dt_over_mi;
         }
         // external body forces, in the order added (MpmSimulationBase.cpp:1208-1212)
@@ -1797,12 +1801,12 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D)
             for (int a = 0; a < 3; a++)
                 vd[a] = sd.va[a] + (sd.vB[3 * a] * xi[0] + sd.vB[3 * a + 1] * xi[1] + sd.vB[3 * a + 2] * xi[2]);
         }
-#pragma unroll
-        for (int a = 0; a < 3; a++) D.newv[a * cn + node] = (T)vd[a];
+        T pf = T(0);
         if (D.any_damage) {
             const T pw = accValue(D, GPW, node, sc, SC_PW);
-            D.pfn[node] = pw != T(0) ? accValue(D, GPF, node, sc, SC_PF) / pw : T(0);
+            pf = pw != T(0) ? accValue(D, GPF, node, sc, SC_PF) / pw : T(0);
         }
+        nodeRec(D.newv, node) = V4<T> { (T)vd[0], (T)vd[1], (T)vd[2], pf };
     }
 }
 
@@ -1844,15 +1848,14 @@ __global__ __launch_bounds__(WG) void k_damp_grid(Dev<T, AccT> D)
     using B = Blk<T>;
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
     const int64_t nn = (int64_t)D.counters[0] * NB;
-    const int64_t cn = D.cap_nodes;
     for (int64_t node = (int64_t)blockIdx.x * WG + threadIdx.x; node < nn; node += (int64_t)gridDim.x * WG) {
         const float* sc = D.scales + blockScene(D, (int)(node / NB)) * NSC;
         const T m = accValue(D, GM, node, sc, SC_M);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             const T v = m != T(0) ? accValue(D, GMV + a, node, sc, SC_MV) / m : T(0);
-            D.newv[a * cn + node] = v;
-            if (D.flip) D.oldv[a * cn + node] = v;
+            D.newv[4 * node + a] = v;
+            if (D.flip) D.oldv[4 * node + a] = v;
         }
     }
 }
@@ -2275,43 +2278,35 @@ __device__ __forceinline__ bool g2pFinish(const Dev<T, AccT>& D, int s, const Ra
     return rare;
 }
 
-// Grid fields of block b's tile (the block and its forward halo, through the
-// neighbour table) into gt, four to a node record: field f < nf is 0-2 new
-// velocity, 3 phase field (staged only with dmg) in gt[0]; 4-6 the grid's
-// velocity before the update (FLIP) in gt[1].
+// Grid node records of block b's tile (the block and its forward halo, through
+// the neighbour table) into gt: gt[0] the new velocity and phase field, and
+// with nrec = 2 gt[1] the grid's velocity before the update (FLIP).  A node
+// outside the grid stages zeros.
 template <class T, class AccT>
-__device__ __forceinline__ void stageTile(const Dev<T, AccT>& D, int b, V4<T> (*gt)[Tile<T>::TN], int nf, bool dmg)
+__device__ __forceinline__ void stageTile(const Dev<T, AccT>& D, int b, V4<T> (*gt)[Tile<T>::TN], int nrec)
 {
     using B = Blk<T>;
     using TL = Tile<T>;
     constexpr int NB = TL::NB;
-    const int64_t cn = D.cap_nodes;
     // the neighbour table row, read once (a uniform address) and selected per node
     const int* nb = D.nbr + b * 8;
     int nbv[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) nbv[i] = nb[i];
-    for (int e = threadIdx.x; e < nf * TL::TN; e += WG) {
-        const int f = e / TL::TN, nd = e - TL::TN * (e / TL::TN);
-        if (f == 3 && !dmg) continue;
+    for (int e = threadIdx.x; e < nrec * TL::TN; e += WG) {
+        const int r = e / TL::TN, nd = e - TL::TN * (e / TL::TN);
         const int tx = nd / (TL::TY * TL::TZ), ty = (nd / TL::TZ) % TL::TY, tz = nd % TL::TZ;
         const int which = ((tx >= TL::BX) << 2) | ((ty >= TL::BY) << 1) | (tz >= TL::BZ);
         int tb = nbv[0];
 #pragma unroll
         for (int i = 1; i < 8; i++) tb = which == i ? nbv[i] : tb;
-        T v = T(0);
+        V4<T> v { T(0), T(0), T(0), T(0) };
         if (tb >= 0) {
             const int elem = ((tx & (TL::BX - 1)) << (B::ybits + B::zbits)) | ((ty & (TL::BY - 1)) << B::zbits) | (tz & (TL::BZ - 1));
             const int64_t node = (int64_t)tb * NB + elem;
-            v = f < 3 ? D.newv[f * cn + node] : (f == 3 ? D.pfn[node] : D.oldv[(f - 4) * cn + node]);
+            v = nodeRec(r == 0 ? D.newv : D.oldv, node);
         }
-        V4<T>& r = gt[f >> 2][nd];
-        switch (f & 3) {
-        case 0: r.x = v; break;
-        case 1: r.y = v; break;
-        case 2: r.z = v; break;
-        default: r.w = v; break;
-        }
+        gt[r][nd] = v;
     }
 }
 
@@ -2334,7 +2329,6 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D)
     // (stored only for damaged ones), so no per-particle material load has to
     // complete before the gathers
     const bool dmg = D.any_damage != 0;
-    const int nfield = FLIP ? 7 : (dmg ? 4 : 3);
     if (live) {
         int p = ldAt(D.perm, s);
         if ((unsigned)p >= (unsigned)D.n) p = 0;
@@ -2360,7 +2354,7 @@ __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D)
             const int c = b * NB + NB - 1;
             seg_end = D.cell_start[c] + D.cell_count[c];
         }
-        stageTile(D, b, gt, nfield, dmg);
+        stageTile(D, b, gt, FLIP ? 2 : 1);
         ldsBarrier();
         const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
         // the segment's particles are in its block's scene (a uniform index)
@@ -2673,7 +2667,7 @@ __global__ __launch_bounds__(WG) void k_damp_g2p(Dev<T, AccT> D)
             const int c = b * NB + NB - 1;
             seg_end = D.cell_start[c] + D.cell_count[c];
         }
-        stageTile(D, b, gt, 3, false);
+        stageTile(D, b, gt, 1);
         __syncthreads();
         const int seg_hi = max(seg_lo + 1, min(hi, seg_end));
         if (live && s >= seg_lo && s < seg_hi) g2pGather<T, AccT, false>(D, D.sc[blockScene(D, b)], gt, x, false, A, cell);
@@ -3048,7 +3042,7 @@ __global__ __launch_bounds__(CGW) void k_cg(Dev<T, AccT> D)
     double prr = 0.0, prho = 0.0;
     for (int64_t node = first; node < nn; node += stride) {
         const double dg = DG[node];
-        const double r = dg > 0.0 ? R[node] + (double)D.pfn[node] * X[node] : 0.0;
+        const double r = dg > 0.0 ? R[node] + (double)D.newv[4 * node + 3] * X[node] : 0.0;
         X[node] = 0.0;
         R[node] = r;
         const double mr = dg > 0.0 ? r / dg : 0.0;
@@ -3226,7 +3220,7 @@ __global__ __launch_bounds__(CGW) void k_cg(Dev<T, AccT> D)
     // g.phase_field = x - phase_field: the change G2P transfers back
     for (int64_t node = first; node < nn; node += stride) {
         const double dg = DG[node];
-        D.pfn[node] = dg > 0.0 ? (T)(X[node] - (double)D.pfn[node]) : T(0);
+        D.newv[4 * node + 3] = dg > 0.0 ? (T)(X[node] - (double)D.newv[4 * node + 3]) : T(0);
     }
     if (blockIdx.x == 0 && t == 0) D.cg_bar[1] = (unsigned)it;
     if (ovf) latch(D.counters, ST_CAPACITY);

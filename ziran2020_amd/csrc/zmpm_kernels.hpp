@@ -249,9 +249,8 @@ struct Dev {
     Masks masks;
     // grid
     AccT* acc;         // [NGF][cap_nodes]
-    T* newv;           // [3][cap_nodes]
-    T* oldv;           // [3][cap_nodes] v = mv/m before the update (FLIP_blend_PIC)
-    T* pfn;            // [cap_nodes]
+    T* newv;           // [cap_nodes][4]: new velocity (0-2) and phase field (3), one record per node
+    T* oldv;           // [cap_nodes][4]: v = mv/m before the update (0-2; FLIP_blend_PIC)
     int64_t cap_nodes;
     // fixed-point scales (float form only), per scene
     float* scales;     // [scenes][NSC] current scale 2^e
