@@ -1172,18 +1172,19 @@ struct RawP2G {
     T v[3], C[9], S[6], m, d;
 };
 
+// pos_only: the position alone (the weights' half of the P2G payload).
 template <class T, class AccT, bool DAMP = false>
-__device__ __forceinline__ void loadP2G(const Dev<T, AccT>& D, int s, bool dmg, RawP2G<T>& r)
+__device__ __forceinline__ void loadP2G(const Dev<T, AccT>& D, int s, bool dmg, RawP2G<T>& r, bool pos_only = false)
 {
     int p = DAMP ? s : ldAt(D.perm, s);
     if ((unsigned)p >= (unsigned)D.n) p = 0;
     const int64_t cap = D.cap;
     const T* P = D.P;
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        r.x[a] = loadPos(P + (PX + a) * cap, p);
-        r.v[a] = ldAt(P + (PV + a) * cap, p);
-    }
+    for (int a = 0; a < 3; a++) r.x[a] = loadPos(P + (PX + a) * cap, p);
+    if (pos_only) return;
+#pragma unroll
+    for (int a = 0; a < 3; a++) r.v[a] = ldAt(P + (PV + a) * cap, p);
 #pragma unroll
     for (int i = 0; i < 9; i++) r.C[i] = ldAt(P + (PC + i) * cap, p);
 #pragma unroll
@@ -1205,7 +1206,8 @@ __device__ __forceinline__ void loadP2G(const Dev<T, AccT>& D, int s, bool dmg, 
 //
 // One workgroup per grid block (grid-stride over blocks).  The block's sorted
 // particles are taken CHUNK at a time, counted from the block's first particle.
-// Phase A: one thread per particle writes its P2G payload to LDS.  Phase B: the
+// Phase A: two threads per particle (its weights; its momentum, affine and
+// stress terms) write its P2G payload to LDS.  Phase B: the
 // cells holding particles of the chunk are compacted; lanes are (sub-lane,
 // cell, i, j) columns of the 27-node stencil; each lane walks every R-th
 // particle of its cell (all share one base node), accumulates 3 nodes x 9
@@ -1245,6 +1247,13 @@ __global__ __launch_bounds__(WG, OCC) void k_p2g(Dev<T, AccT> D)
     int* const pb = reinterpret_cast<int*>(&tile[0][0]) + SB;
 
     const int t = threadIdx.x;
+    // phase A: particle pt of the chunk.  Float form: two threads per particle,
+    // its weights (t < CH) and its other terms (t >= CH); the double form, short
+    // of registers, computes both in thread t < CH
+    constexpr bool SPLIT = sizeof(T) == 4;
+    static_assert(WG == 2 * CH, "two threads per chunk particle in phase A");
+    const int pt = SPLIT ? (t & (CH - 1)) : t;
+    const bool doW = !SPLIT || t < CH, doM = !SPLIT || t >= CH;
     const bool dmg = !DAMP && D.any_damage != 0;
     const int nq = DAMP ? GFO : (dmg ? NGF : GPF);
     const int64_t cap = D.cap, cn = D.cap_nodes;
@@ -1311,51 +1320,59 @@ __global__ __launch_bounds__(WG, OCC) void k_p2g(Dev<T, AccT> D)
         for (int lo = bs; lo < be; lo += CH) {
             const int hi = min(lo + CH, be);
             ldsBarrier(); // previous chunk's phase B done with the payload
-            // ---- phase A: payload of particle lo + t from the prefetched fields
-            if (t < CH && lo + t < hi) {
-                if (!have) loadP2G<T, AccT, DAMP>(D, lo + t, dmg, raw);
-                const T mass = raw.m;
-                T w[3][3], dw[3][3], xb[3];
+            // ---- phase A: payload of particle lo + pt from the prefetched fields:
+            // its B-spline weights (doW) and its momentum, affine and stress terms (doM)
+            if (pt < CH && lo + pt < hi) {
+                if (!have) loadP2G<T, AccT, DAMP>(D, lo + pt, dmg, raw, !doM);
+                T d0s[3], xb[3];
                 int base[3];
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    T d0;
-                    cellOf(raw.x[a], sd.odx, sd.origin[a], sd.domain, base[a], d0, sd.pos_bits); // errors latched by the binning
-                    bsplineFrac(d0, w[a], dw[a]);
+                    cellOf(raw.x[a], sd.odx, sd.origin[a], sd.domain, base[a], d0s[a], sd.pos_bits); // errors latched by the binning
                     if constexpr (sizeof(T) == 4)
-                        xb[a] = -d0 * sd.dx; // x_base - x_p
+                        xb[a] = -d0s[a] * sd.dx; // x_base - x_p
                     else
                         xb[a] = T(base[a] - sd.origin[a]) * sd.dx - raw.x[a];
-                    if constexpr (MLS) // D^-1 dx w (x_i - x_p) in place of dw
-#pragma unroll
-                        for (int i = 0; i < 3; i++) dw[a][i] = (sd.Dinv * sd.dx) * w[a][i] * (xb[a] + T(i) * sd.dx);
                 }
-                T mC[9], a3[3];
+                if (doW) {
+                    T w[3][3], dw[3][3];
 #pragma unroll
-                for (int i = 0; i < 9; i++) mC[i] = D.flip ? T(0) : mass * raw.C[i];
+                    for (int a = 0; a < 3; a++) {
+                        bsplineFrac(d0s[a], w[a], dw[a]);
+                        if constexpr (MLS) // D^-1 dx w (x_i - x_p) in place of dw
 #pragma unroll
-                for (int r = 0; r < 3; r++)
-                    a3[r] = mass * raw.v[r] + mC[3 * r] * xb[0] + mC[3 * r + 1] * xb[1] + mC[3 * r + 2] * xb[2];
-                T Sv[6];
+                            for (int i = 0; i < 3; i++) dw[a][i] = (sd.Dinv * sd.dx) * w[a][i] * (xb[a] + T(i) * sd.dx);
+                    }
+                    q[5][pt] = V4<T> { w[2][0], w[2][1], w[2][2], T(0) };
+                    q[6][pt] = V4<T> { dw[2][0], dw[2][1], dw[2][2], T(0) };
 #pragma unroll
-                for (int i = 0; i < 6; i++) Sv[i] = raw.S[i] * sd.odx; // float form stores -S below
-                const T dx = sd.dx;
-                q[0][t] = V4<T> { mass, a3[0], a3[1], a3[2] };
-                q[1][t] = V4<T> { mC[0] * dx, mC[1] * dx, mC[2] * dx, mC[3] * dx };
-                q[2][t] = V4<T> { mC[4] * dx, mC[5] * dx, mC[6] * dx, mC[7] * dx };
-                // the force is -V0 tau grad(w): the float form accumulates it
-                // with FMAs of the negated stress
-                const T sg = sizeof(T) == 4 ? T(-1) : T(1);
-                q[3][t] = V4<T> { mC[8] * dx, sg * Sv[0], sg * Sv[1], sg * Sv[2] };
-                q[4][t] = V4<T> { sg * Sv[3], sg * Sv[4], sg * Sv[5], raw.d };
-                q[5][t] = V4<T> { w[2][0], w[2][1], w[2][2], T(0) };
-                q[6][t] = V4<T> { dw[2][0], dw[2][1], dw[2][2], T(0) };
+                    for (int i = 0; i < 3; i++) {
+                        wl[0][pt][i] = w[0][i];
+                        wl[1][pt][i] = dw[0][i];
+                        wl[2][pt][i] = w[1][i];
+                        wl[3][pt][i] = dw[1][i];
+                    }
+                }
+                if (doM) {
+                    const T mass = raw.m;
+                    T mC[9], a3[3];
 #pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    wl[0][t][i] = w[0][i];
-                    wl[1][t][i] = dw[0][i];
-                    wl[2][t][i] = w[1][i];
-                    wl[3][t][i] = dw[1][i];
+                    for (int i = 0; i < 9; i++) mC[i] = D.flip ? T(0) : mass * raw.C[i];
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+                        a3[r] = mass * raw.v[r] + mC[3 * r] * xb[0] + mC[3 * r + 1] * xb[1] + mC[3 * r + 2] * xb[2];
+                    T Sv[6];
+#pragma unroll
+                    for (int i = 0; i < 6; i++) Sv[i] = raw.S[i] * sd.odx; // float form stores -S below
+                    const T dx = sd.dx;
+                    q[0][pt] = V4<T> { mass, a3[0], a3[1], a3[2] };
+                    q[1][pt] = V4<T> { mC[0] * dx, mC[1] * dx, mC[2] * dx, mC[3] * dx };
+                    q[2][pt] = V4<T> { mC[4] * dx, mC[5] * dx, mC[6] * dx, mC[7] * dx };
+                    // the force is -V0 tau grad(w): the float form accumulates it
+                    // with FMAs of the negated stress
+                    const T sg = sizeof(T) == 4 ? T(-1) : T(1);
+                    q[3][pt] = V4<T> { mC[8] * dx, sg * Sv[0], sg * Sv[1], sg * Sv[2] };
+                    q[4][pt] = V4<T> { sg * Sv[3], sg * Sv[4], sg * Sv[5], raw.d };
                 }
             }
             // ---- cells holding particles of this chunk (NB <= 64: one wavefront)
@@ -1376,13 +1393,13 @@ __global__ __launch_bounds__(WG, OCC) void k_p2g(Dev<T, AccT> D)
             // chunk the first chunk of the next block, while phase B runs (the
             // double form keeps its registers for phase B)
             have = false;
-            if (sizeof(T) == 4 && t < CH) {
-                if (hi + t < be) {
-                    loadP2G<T, AccT, DAMP>(D, hi + t, dmg, raw);
+            if (sizeof(T) == 4) {
+                if (hi + pt < be) {
+                    loadP2G<T, AccT, DAMP>(D, hi + pt, dmg, raw, !doM);
                     have = true;
                 }
-                else if (hi == be && bn < nblocks && nxr[0] + t < nxr[1]) {
-                    loadP2G<T, AccT, DAMP>(D, nxr[0] + t, dmg, raw);
+                else if (hi == be && bn < nblocks && nxr[0] + pt < nxr[1]) {
+                    loadP2G<T, AccT, DAMP>(D, nxr[0] + pt, dmg, raw, !doM);
                     have = true;
                 }
             }
