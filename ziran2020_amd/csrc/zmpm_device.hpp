@@ -653,11 +653,12 @@ __device__ __forceinline__ void beginStep(const Dev<T, AccT>& D)
     }
 }
 
-// Workgroups [0, part_wgs): particle p -> block_start[block] + its slot, with
-// its in-block sort key (cell, original index) beside it (the order inside a
-// block is fixed by P2G's blockOrder).  The next prep_wgs
-// workgroups: zero the accumulators of every block's nodes and build the
-// forward-neighbour block table.  The last workgroup starts the substep.
+// Workgroups [0, prep_wgs): zero the accumulators of every block's nodes and
+// build the forward-neighbour block table (first, so their block loops overlap
+// the particle workgroups).  Workgroup prep_wgs starts the substep.  The rest,
+// part_wgs of them: particle p -> block_start[block] + its slot, with its
+// in-block sort key (cell, original index) beside it (the order inside a block
+// is fixed by P2G's blockOrder).
 template <class T, class AccT>
 __global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D, int part_wgs, int prep_wgs)
 {
@@ -665,8 +666,8 @@ __global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D, int part_wgs, in
     constexpr int bbits = B::xbits + B::ybits + B::zbits;
     constexpr int NB = 1 << bbits;
     const int t = threadIdx.x;
-    if ((int)blockIdx.x < part_wgs) {
-        const int p = blockIdx.x * WG + t;
+    if ((int)blockIdx.x > prep_wgs) {
+        const int p = ((int)blockIdx.x - prep_wgs - 1) * WG + t;
         if (p >= D.n) return;
         const uint64_t key = D.key[p];
         const int bid = D.page_table[key >> bbits];
@@ -678,13 +679,13 @@ __global__ __launch_bounds__(WG) void k_scatter(Dev<T, AccT> D, int part_wgs, in
         }
         return;
     }
-    if ((int)blockIdx.x == part_wgs + prep_wgs) {
+    if ((int)blockIdx.x == prep_wgs) {
         beginStep(D);
         return;
     }
     const int nb = D.counters[CNT_BLOCKS];
     const int nf = D.any_damage ? NGF : GPF;
-    for (int b = (int)blockIdx.x - part_wgs; b < nb; b += prep_wgs) {
+    for (int b = (int)blockIdx.x; b < nb; b += prep_wgs) {
         for (int e = t; e < nf * NB; e += WG) {
             const int f = e / NB, nd = e % NB;
             D.acc[f * D.cap_nodes + (int64_t)b * NB + nd] = AccT(0);
@@ -2328,7 +2329,8 @@ __device__ __forceinline__ void stageTile(const Dev<T, AccT>& D, int b, V4<T> (*
 }
 
 // RR: workgroups take particle segments round-robin over the XCDs instead of
-// XCD by XCD (an autotuner variant; results are the same).
+// XCD by XCD (an autotuner variant; results are the same).  (Held to 4 waves
+// per SIMD, the float form spills and runs ~7 % slower than at its natural 3.)
 template <class T, class AccT, bool FLIP, bool RR = false>
 __global__ __launch_bounds__(WG) void k_g2p(Dev<T, AccT> D)
 {
