@@ -130,7 +130,7 @@ struct zmpm_context {
     double stage_dt = 0, stage_t = 0;
     int p2g_split = 64;
     // workgroups of the grid-stride launches (results do not depend on them)
-    int p2g_wgs = 2048, grid_wgs = 2048, prep_wgs = 2048;
+    int p2g_wgs = 2048, grid_wgs = 2048, prep_wgs = 8192;
     int variant = 0; // compiled kernel variants (VARIANT_* bits), none changes a result
 };
 
@@ -1726,7 +1726,7 @@ zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, in
     };
     std::vector<Knob> knobs = {
         { &h->p2g_wgs, ~0, 1, { 512, 1024, 2048, 4096 } },
-        { &h->prep_wgs, ~0, 0, { 256, 512, 1024, 2048 } },
+        { &h->prep_wgs, ~0, 0, { 512, 2048, 8192 } },
         { &h->grid_wgs, ~0, 2, { 256, 512, 1024, 2048, 4096 } },
     };
     if (tune_variants) {
@@ -1786,6 +1786,9 @@ done:
 }
 
 // Key of the tuning cache: the device and what selects the work and the kernels.
+// Raised when the kernels change enough that cached tunings should be redone.
+constexpr int kTuneRevision = 2;
+
 static std::string tuneKey(zmpm_handle h, int32_t tune_variants)
 {
     hipDeviceProp_t prop {};
@@ -1803,7 +1806,7 @@ static std::string tuneKey(zmpm_handle h, int32_t tune_variants)
       << "/n" << std::lround(4.0 * std::log2((double)std::max(h->n, 1))) << "/d" << (h->any_damage ? 1 : 0) << "/i"
       << h->desc.implicit_damage << "/t" << h->desc.transfer_scheme << "/m" << h->desc.mls_mpm << "/r"
       << h->desc.rpic_damping_iteration << "/p" << h->desc.pic_damping_iteration << "/j" << (fj ? 1 : 0) << "/y"
-      << (plastic ? 1 : 0) << "/v" << (tune_variants ? 1 : 0);
+      << (plastic ? 1 : 0) << "/v" << (tune_variants ? 1 : 0) << "/k" << kTuneRevision;
     return k.str();
 }
 
