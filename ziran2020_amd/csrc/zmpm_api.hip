@@ -131,6 +131,7 @@ struct zmpm_context {
     int p2g_split = 64;
     // workgroups of the grid-stride launches (results do not depend on them)
     int p2g_wgs = 2048, grid_wgs = 2048, prep_wgs = 8192;
+    int rare_wgs = 1024; // k_rare's grid: its workgroups resident at once on the device
     int variant = 0; // compiled kernel variants (VARIANT_* bits), none changes a result
 };
 
@@ -306,6 +307,14 @@ zmpm_status allocate(zmpm_context* h, int n)
         if (!coop) return fail(h, ZMPM_E_HIP, "the implicit damage solve needs cooperative launches");
         D.cg_wgs = std::max(1, std::min(CG_MAX_WGS, cus));
         HIPCHK(h, hipMemsetAsync(D.cg_bar, 0, 4 * sizeof(unsigned), h->stream));
+    }
+    {
+        // k_rare walks its queue grid-stride: one resident round of workgroups
+        // serves any queue, and an empty one costs a single round of starts
+        int occ = 0, cus = 0;
+        HIPCHK(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_rare<T, AccT>, WG, 0));
+        HIPCHK(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->desc.device));
+        h->rare_wgs = std::max(1, occ * cus);
     }
     HIPCHK(h, hipMemsetAsync(D.touched, 0, D.num_pages * sizeof(int), h->stream));
     HIPCHK(h, hipMemsetAsync(D.page_count, 0, D.num_pages * sizeof(int), h->stream));
@@ -596,7 +605,7 @@ void launchStage(zmpm_context* h, int stage)
             k_g2p<T, AccT, false, true><<<D.max_slots, WG, 0, st>>>(D);
         else
             k_g2p<T, AccT, false><<<D.max_slots, WG, 0, st>>>(D);
-        k_rare<T, AccT><<<std::min(D.max_slots, 1024), WG, 0, st>>>(D);
+        k_rare<T, AccT><<<std::min(D.max_slots, h->rare_wgs), WG, 0, st>>>(D);
         std::swap(D.P, D.Q);
         std::swap(D.pid, D.qid);
         std::swap(D.pmat, D.qmat);
