@@ -634,6 +634,31 @@ def test_launch_config_does_not_change_results(gpu, variant):
         assert np.array_equal(a[k], b[k]), k
 
 
+@pytest.mark.parametrize("precision", [A.PREC_FP32, A.PREC_FP64])
+@pytest.mark.parametrize("transfer", ["apic", "mls", "rpic_damped"])
+def test_p2g_many_blocks_per_workgroup(gpu, precision, transfer):
+    # P2G workgroups walk their blocks with the next block's cell ranges,
+    # neighbour row and first particle chunk loaded ahead: one or seven
+    # workgroups for ~100 blocks must give what one block per workgroup gives
+    # (FP32 bit for bit; FP64 to its atomic-order bar)
+    kw = {"apic": {}, "mls": dict(mls_mpm=True), "rpic_damped": dict(rpic_damping_iteration=2)}[transfer]
+    sc = small_slab(n=40000)
+    ref = sc.make(gpu, precision, **kw)
+    ref.set_launch_config(p2g_wgs=4096)
+    ref.run(4, sc.dt, 0.0)
+    a = _fields(ref.download())
+    for wgs in (1, 7):
+        sim = sc.make(gpu, precision, **kw)
+        sim.set_launch_config(p2g_wgs=wgs)
+        sim.run(4, sc.dt, 0.0)
+        b = _fields(sim.download())
+        for k in a:
+            if precision == A.PREC_FP32:
+                assert np.array_equal(a[k], b[k]), (wgs, k)
+            else:
+                assert normwise(a[k], b[k]) < 1e-12, (wgs, k, normwise(a[k], b[k]))
+
+
 def test_launch_config_rejects_unknown_variants(gpu):
     sim = small_slab(n=2000).make(gpu, A.PREC_FP32)
     with pytest.raises(A.ZmpmError):
