@@ -1688,13 +1688,16 @@ __device__ __forceinline__ bool resolveCollision(const ColDev& co, const ColStat
 // objects in insertion order; the first STICKY hit ends the loop; the normals
 // of SLIP / SEPARATE hits are Gram-Schmidt orthonormalised and the loop ends
 // once 3 are independent (a normal in the span of the earlier ones adds none).
+// mask: bit c clear for objects that hold no node of the node's block (they
+// would leave vi as it is).
 template <class T>
 __device__ __forceinline__ void multiObjectCollision(const ColDev* cols, const ColState* st, int ncols, const T* xi,
-                                                     T* vi)
+                                                     T* vi, uint32_t mask)
 {
     int count = 0;
     T b0[3] = { T(0), T(0), T(0) }, b1[3] = { T(0), T(0), T(0) };
     for (int c = 0; c < ncols; c++) {
+        if (!((mask >> c) & 1u)) continue;
         const ColDev& co = cols[c];
         if (co.type == ZMPM_GHOST) continue;
         T n[3];
@@ -1723,13 +1726,67 @@ __device__ __forceinline__ void multiObjectCollision(const ColDev* cols, const C
     }
 }
 
+// Whether collision object co may hold a node of the block whose first node is
+// i0 (scene frame, node units; ext nodes per axis): the block's bounding sphere,
+// taken to the object's material space (a rotation, then 1 / s), against the
+// level set's inside region grown by its radius.  False only when no node of
+// the block is inside, so an object it rules out leaves every node as it is.
+template <class T>
+__device__ __forceinline__ bool colliderMayHold(const ColDev& co, const ColState& st, const int* i0, const int* ext,
+                                                double h)
+{
+    if (co.type == ZMPM_GHOST) return false;
+    double c[3], r2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double e = 0.5 * (ext[a] - 1) * h;
+        c[a] = i0[a] * h + e - st.b[a];
+        r2 += e * e;
+    }
+    // the margin covers the rounding of the per-node level-set evaluation
+    double r = sqrt(r2) * 1.001 + 1e-9 * h;
+    double X[3];
+    if (co.moving) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) X[a] = (st.R[a] * c[0] + st.R[3 + a] * c[1] + st.R[6 + a] * c[2]) * st.os;
+        r *= fabs(st.os);
+    }
+    else {
+#pragma unroll
+        for (int a = 0; a < 3; a++) X[a] = c[a];
+    }
+    if (co.shape == ZMPM_SHAPE_HALF_SPACE) {
+        const double phi = co.n[0] * (X[0] - co.origin[0]) + co.n[1] * (X[1] - co.origin[1]) + co.n[2] * (X[2] - co.origin[2]);
+        return phi <= r * sqrt(co.n[0] * co.n[0] + co.n[1] * co.n[1] + co.n[2] * co.n[2]);
+    }
+    if (co.shape == ZMPM_SHAPE_BOX) {
+        bool in = true;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            in = in && fabs(X[a] - 0.5 * (co.bmin[a] + co.bmax[a])) <= 0.5 * (co.bmax[a] - co.bmin[a]) + r;
+        return in;
+    }
+    if (co.shape == ZMPM_SHAPE_SPHERE) {
+        double d2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) d2 += (X[a] - co.origin[a]) * (X[a] - co.origin[a]);
+        return sqrt(d2) <= co.radius + r;
+    }
+    double Y[3], Xp[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) Y[a] = X[a] - co.origin[a];
+#pragma unroll
+    for (int a = 0; a < 3; a++) Xp[a] = co.Rci[3 * a] * Y[0] + co.Rci[3 * a + 1] * Y[1] + co.Rci[3 * a + 2] * Y[2];
+    return sqrt(Xp[0] * Xp[0] + Xp[2] * Xp[2]) <= co.radius + r && fabs(Xp[1]) <= 0.5 * co.height + r;
+}
+
 // v = mv/m (MpmSimulationBase.cpp:655-666) and gridVelocityExplicitUpdate
 // (:1179-1228) with multiObjectCollision / detectAndResolveCollision
 // (CollisionObject.cpp:66-105, 341-406) at the scene's time t; phase-field
 // normalisation (AnisoFractureSimulation.h:177-179).  Each node takes its
 // scene's dx, dt, gravity and collision objects.
 template <class T, class AccT>
-__global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D)
+__global__ __launch_bounds__(WG, 4) void k_grid(Dev<T, AccT> D)
 {
     using B = Blk<T>;
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
@@ -1752,8 +1809,16 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D)
             D.cgv[4 * cn + node] = 0.0;
         }
     }
+    // per block of the iteration, the objects that may hold one of its nodes
+    // (bit c of the block's word; two words so the next iteration's need no
+    // second barrier)
+    constexpr int BPI = WG / NB;
+    static_assert(MAX_COLLIDERS == 16 && BPI * MAX_COLLIDERS <= WG, "16 object lanes per block of an iteration");
+    __shared__ uint32_t bmask[2][BPI];
+    const int nblocks = D.counters[0];
     int staged = -1, nstaged = 0; // the scene whose objects are in LDS (the same in every thread)
-    for (int64_t base = (int64_t)blockIdx.x * WG; base < nn; base += (int64_t)gridDim.x * WG) {
+    int it = 0;
+    for (int64_t base = (int64_t)blockIdx.x * WG; base < nn; base += (int64_t)gridDim.x * WG, it ^= 1) {
         const int s_first = blockScene(D, (int)(base / NB));
         const int s_last = blockScene(D, (int)((min(base + WG, nn) - 1) / NB));
         const bool uniform = s_first == s_last;
@@ -1768,9 +1833,32 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D)
             __syncthreads();
             staged = s_first;
         }
+        {
+            // thread (lb, c): block lb of the iteration, object c of its scene
+            const int lb = (int)threadIdx.x / MAX_COLLIDERS, c = (int)threadIdx.x % MAX_COLLIDERS;
+            const int bb = (int)(base / NB) + lb;
+            bool hold = false;
+            if (lb < BPI && bb < nblocks) {
+                const int s = uniform ? s_first : blockScene(D, bb);
+                const SceneDev<T>& sb = D.sc[s];
+                if (!sb.ignore_cols && c < min(sb.ncol, MAX_COLLIDERS)) {
+                    uint32_t Bx, By, Bz;
+                    blockOfPage<T>((uint32_t)D.block_page[bb], Bx, By, Bz);
+                    const int i0[3] = { (int)(Bx << B::xbits) - sb.origin[0], (int)(By << B::ybits) - sb.origin[1],
+                                        (int)(Bz << B::zbits) - sb.origin[2] };
+                    const int ext[3] = { 1 << B::xbits, 1 << B::ybits, 1 << B::zbits };
+                    hold = uniform ? colliderMayHold<T>(cols[c], cst[c], i0, ext, sb.ddx)
+                                   : colliderMayHold<T>(D.cols[sb.col0 + c], D.colstate[sb.col0 + c], i0, ext, sb.ddx);
+                }
+            }
+            const uint64_t bal = __ballot(hold);
+            if (lb < BPI && c == 0) bmask[it][lb] = (uint32_t)(bal >> (threadIdx.x & 48)) & 0xffffu;
+        }
+        ldsBarrier();
         const int64_t node = base + threadIdx.x;
         if (node >= nn) continue;
         const int b = (int)(node / NB), e = (int)(node % NB);
+        const uint32_t cmask = bmask[it][threadIdx.x / NB];
         const int scene = uniform ? s_first : blockScene(D, b);
         const SceneDev<T>& sd = D.sc[scene];
         const float* sc = D.scales + scene * NSC;
@@ -1809,9 +1897,9 @@ __global__ __launch_bounds__(WG) void k_grid(Dev<T, AccT> D)
             // ignoreCollisionObject (:1217-1221): the objects leave v as it is
         }
         else if (uniform)
-            multiObjectCollision(cols, cst, nstaged, xi, vd);
+            multiObjectCollision(cols, cst, nstaged, xi, vd, cmask);
         else
-            multiObjectCollision(D.cols + sd.col0, D.colstate + sd.col0, min(sd.ncol, MAX_COLLIDERS), xi, vd);
+            multiObjectCollision(D.cols + sd.col0, D.colstate + sd.col0, min(sd.ncol, MAX_COLLIDERS), xi, vd, cmask);
         if (sd.vfield) {
             // explicit_velocity_field (:1187-1195): new_v = a + B x_i at every node
             // with mass, in place of the update above
