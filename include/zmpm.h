@@ -463,7 +463,9 @@ typedef struct {
     int32_t p2g_split;   /* P2G lanes per stencil column, cap (>= 1)              */
     int32_t variant;     /* compiled variants, bits: 1 = P2G built for one wave per
                             SIMD fewer, 2 = G2P segments round-robin over the XCDs,
-                            4 = P2G blocks XCD by XCD                              */
+                            4 = P2G blocks XCD by XCD, 8 = collision objects
+                            resolved in a launch after the grid update (default
+                            for batches) instead of inside it                     */
     int32_t reserved[3];
 } zmpm_launch_config;
 

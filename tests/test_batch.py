@@ -158,3 +158,26 @@ def test_gpu_batch_calculate_dt(gpu, oracle):
     dg, do = g.calculate_dt(1e-3), o.calculate_dt(1e-3)
     assert abs(dg["dt"] - do["dt"]) <= 1e-12 * do["dt"]
     assert np.allclose(dg["min_corner"], do["min_corner"], rtol=0, atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [A.PREC_FP32, A.PREC_FP64])
+def test_gpu_batch_collision_objects_fused_or_split(gpu, precision):
+    # the collision objects (moving sphere, rotating box, growing capped
+    # cylinder; sticky, slipping, separating and frictional floors) resolved
+    # inside the grid update or in a launch of their own after it, over the
+    # blocks each object may reach: the same particles (FP32 bit for bit, FP64
+    # to its atomic-order bar)
+    b = small_batch(n_scenes=10, n_range=(2000, 20000))
+    outs = []
+    for variant in (0, A.VARIANT_GRID_SPLIT):
+        sim = b.make(gpu, precision)
+        sim.set_launch_config(variant=variant)
+        assert sim.launch_config()["variant"] == variant
+        b.run(sim, 20)
+        outs.append(sim.download())
+    for k in FIELDS:
+        if precision == A.PREC_FP32:
+            assert np.array_equal(outs[0][k], outs[1][k]), k
+        else:
+            assert normwise(outs[0][k], outs[1][k]) < 1e-9, (k, normwise(outs[0][k], outs[1][k]))

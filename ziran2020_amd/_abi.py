@@ -244,7 +244,7 @@ class LaunchConfig(C.Structure):
         return {k: int(getattr(self, k)) for k in ("p2g_wgs", "grid_wgs", "prep_wgs", "p2g_split", "variant")}
 
 
-VARIANT_P2G_LOW_OCC, VARIANT_G2P_RR, VARIANT_P2G_XCD = 1, 2, 4
+VARIANT_P2G_LOW_OCC, VARIANT_G2P_RR, VARIANT_P2G_XCD, VARIANT_GRID_SPLIT = 1, 2, 4, 8
 
 
 class ZmpmLib:

@@ -37,7 +37,8 @@ constexpr int kGraphSteps = 8;     // substeps per captured graph of zmpm_run (e
 constexpr int VARIANT_P2G_LOW_OCC = 1; // P2G compiled for one wave per SIMD fewer
 constexpr int VARIANT_G2P_RR = 2;      // G2P segments round-robin over the XCDs
 constexpr int VARIANT_P2G_XCD = 4;     // P2G blocks XCD by XCD
-constexpr int VARIANT_ALL = 7;
+constexpr int VARIANT_GRID_SPLIT = 8;  // collision objects in a launch after the grid update
+constexpr int VARIANT_ALL = 15;
 const char* const kTimedNames[kTimedGroups + 1] = { "bin_sort", "p2g", "grid_update", "damage_solve", "g2p", "substep" };
 
 uint64_t spreadMask(int log2_struct, int axis)
@@ -133,6 +134,7 @@ struct zmpm_context {
     int p2g_wgs = 2048, grid_wgs = 2048, prep_wgs = 8192;
     int rare_wgs = 1024; // k_rare's grid: its workgroups resident at once on the device
     int variant = 0; // compiled kernel variants (VARIANT_* bits), none changes a result
+    bool launch_set = false; // configured by zmpm_set_launch_config or the autotuner
 };
 
 namespace {
@@ -583,7 +585,12 @@ void launchStage(zmpm_context* h, int stage)
         launchP2G<T, AccT>(h, D, st);
         break;
     case ZMPM_STAGE_GRID_UPDATE:
-        k_grid<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
+        if (h->variant & VARIANT_GRID_SPLIT) {
+            k_grid<T, AccT, false><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
+            if (D.num_cols > 0) k_collide<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
+        }
+        else
+            k_grid<T, AccT, true><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
         break;
     case ZMPM_STAGE_DAMAGE_SOLVE:
         if (D.implicit && D.any_damage) {
@@ -685,6 +692,11 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p, const int32_t* sce
     Dev<T, AccT>& D = devOf<T, AccT>(h);
     D.n = n;
     h->n = n;
+    // unless configured: collision objects in a launch of their own on grids
+    // large enough for the grid update's doubled occupancy to outweigh the
+    // second launch (batches; single scenes from 4M particles)
+    if (!h->launch_set)
+        h->variant = (h->variant & ~VARIANT_GRID_SPLIT) | (h->scenes.size() > 1 || n >= (1 << 22) ? VARIANT_GRID_SPLIT : 0);
     refreshTables<T, AccT>(h);
     const int64_t cap = D.cap;
     constexpr bool F32 = std::is_same<T, float>::value;
@@ -1693,6 +1705,7 @@ zmpm_status zmpm_set_launch_config(zmpm_handle h, const zmpm_launch_config* cfg)
     h->prep_wgs = cfg->prep_wgs;
     h->p2g_split = cfg->p2g_split;
     h->variant = cfg->variant;
+    h->launch_set = true;
     if (h->fp32)
         refreshTables<float, long long>(h);
     else
@@ -1742,6 +1755,7 @@ zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, in
         knobs.push_back({ &h->variant, VARIANT_P2G_LOW_OCC, 1, { 0, VARIANT_P2G_LOW_OCC } });
         knobs.push_back({ &h->variant, VARIANT_P2G_XCD, 1, { 0, VARIANT_P2G_XCD } });
         knobs.push_back({ &h->variant, VARIANT_G2P_RR, 4, { 0, VARIANT_G2P_RR } });
+        knobs.push_back({ &h->variant, VARIANT_GRID_SPLIT, 2, { 0, VARIANT_GRID_SPLIT } });
     }
     h->timing = true;
     zmpm_status st = ZMPM_OK;
@@ -1770,6 +1784,7 @@ zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, in
     }
 done:
     h->timing = timing;
+    h->launch_set = true;
     if (st == ZMPM_OK) st = deviceStatus(h, true);
     // restore the state (into the buffers live before tuning); page-scan tile
     // words of the tuning substeps are cleared
@@ -1796,7 +1811,7 @@ done:
 
 // Key of the tuning cache: the device and what selects the work and the kernels.
 // Raised when the kernels change enough that cached tunings should be redone.
-constexpr int kTuneRevision = 2;
+constexpr int kTuneRevision = 3;
 
 static std::string tuneKey(zmpm_handle h, int32_t tune_variants)
 {

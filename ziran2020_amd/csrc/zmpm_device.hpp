@@ -1781,26 +1781,171 @@ __device__ __forceinline__ bool colliderMayHold(const ColDev& co, const ColState
 }
 
 // v = mv/m (MpmSimulationBase.cpp:655-666) and gridVelocityExplicitUpdate
-// (:1179-1228) with multiObjectCollision / detectAndResolveCollision
-// (CollisionObject.cpp:66-105, 341-406) at the scene's time t; phase-field
-// normalisation (AnisoFractureSimulation.h:177-179).  Each node takes its
-// scene's dx, dt, gravity and collision objects.
+// (:1179-1228) of one node at its scene's time t, without the collision
+// objects; phase-field normalisation (AnisoFractureSimulation.h:177-179).  A
+// node without mass gets its record (the force, for the host's grid view) and
+// false; otherwise v and pf are returned for the caller to store, and col tells
+// whether the scene's objects apply (not ignored, no explicit velocity field).
 template <class T, class AccT>
-__global__ __launch_bounds__(WG, 4) void k_grid(Dev<T, AccT> D)
+__device__ __forceinline__ bool gridNode(const Dev<T, AccT>& D, int64_t node, int b, int scene, T* vi, T& pf, bool& col)
+{
+    using B = Blk<T>;
+    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
+    const SceneDev<T>& sd = D.sc[scene];
+    const float* sc = D.scales + scene * NSC;
+    const T m = accValue(D, GM, node, sc, SC_M);
+    T f[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) f[a] = accValue(D, GFO + a, node, sc, SC_F);
+    if (m == T(0)) {
+        nodeRec(D.newv, node) = V4<T> { f[0], f[1], f[2], T(0) };
+        if (D.flip) nodeRec(D.oldv, node) = V4<T> { T(0), T(0), T(0), T(0) };
+        return false;
+    }
+    const T dt = sd.dt;
+    const T dt_over_mi = dt / m;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        T v = accValue(D, GMV + a, node, sc, SC_MV) / m;
+        if (D.flip) D.oldv[4 * node + a] = v; // the grid's v before the update (FLIP)
+        vi[a] = v + sd.gravity[a] * dt + f[a] * dt_over_mi;
+    }
+    // external body forces, in the order added (MpmSimulationBase.cpp:1208-1212)
+    for (int k = 0; k < sd.nbf; k++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) vi[a] += dt * (T)D.bfval[3 * (scene * MAX_BODY_FORCES + k) + a];
+    col = !sd.ignore_cols && !sd.vfield; // ignoreCollisionObject (:1217-1221)
+    if (sd.vfield) {
+        // explicit_velocity_field (:1187-1195): new_v = a + B x_i at every node
+        // with mass, in place of the update above (the objects do not apply)
+        const int e = (int)(node % NB);
+        uint32_t Bx, By, Bz;
+        blockOfPage<T>((uint32_t)D.block_page[b], Bx, By, Bz);
+        const int ix = (int)((Bx << B::xbits) | (uint32_t)(e >> (B::ybits + B::zbits))) - sd.origin[0];
+        const int iy = (int)((By << B::ybits) | (uint32_t)((e >> B::zbits) & ((1 << B::ybits) - 1))) - sd.origin[1];
+        const int iz = (int)((Bz << B::zbits) | (uint32_t)(e & ((1 << B::zbits) - 1))) - sd.origin[2];
+        const double xi[3] = { ix * sd.ddx, iy * sd.ddx, iz * sd.ddx };
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            vi[a] = (T)(sd.va[a] + (sd.vB[3 * a] * xi[0] + sd.vB[3 * a + 1] * xi[1] + sd.vB[3 * a + 2] * xi[2]));
+    }
+    pf = T(0);
+    if (D.any_damage) {
+        const T pw = accValue(D, GPW, node, sc, SC_PW);
+        pf = pw != T(0) ? accValue(D, GPF, node, sc, SC_PF) / pw : T(0);
+    }
+    return true;
+}
+
+// The collision objects of a grid-update workgroup: those of its scene staged
+// in LDS, read from there so the per-node loop's loads cannot alias the
+// kernel's stores (with the table in global memory, stores of doubles anywhere
+// in the kernel made the compiler load it per node with vector loads: ~2x
+// slower on large grids); a scene's blocks are contiguous, so a workgroup's
+// nodes rarely span two scenes, and those that do read the tables in global
+// memory.  Per block of the iteration (WG nodes), the objects that may hold one
+// of its nodes: bit c of the block's word, two words so the next iteration's
+// need no second barrier.
+template <class T>
+struct ColLds {
+    static constexpr int NB = 1 << (Blk<T>::xbits + Blk<T>::ybits + Blk<T>::zbits);
+    static constexpr int BPI = WG / NB; // blocks per iteration
+    ColDev cols[MAX_COLLIDERS];
+    ColState cst[MAX_COLLIDERS];
+    uint32_t bmask[2][BPI];
+};
+static_assert(sizeof(ColDev) % 8 == 0 && sizeof(ColState) % 8 == 0, "collider records are copied as 8-byte words");
+static_assert(MAX_COLLIDERS == 16, "16 object lanes per block of an iteration");
+
+// Called by every thread for the iteration of nodes [base, base + WG) (it:
+// the iteration's parity): stages the objects of its scene when all its nodes
+// are in one (staged / nstaged: the scene in LDS and its object count, the same
+// in every thread), returns that first scene and whether all nodes are in it,
+// and leaves the block words in L; it ends on a barrier.
+template <class T, class AccT>
+__device__ __forceinline__ void collidePrepare(const Dev<T, AccT>& D, ColLds<T>& L, int64_t base, int64_t nn, int it,
+                                               int& staged, int& nstaged, int& first, bool& same)
+{
+    using B = Blk<T>;
+    constexpr int NB = ColLds<T>::NB, BPI = ColLds<T>::BPI;
+    const int nblocks = (int)(nn / NB);
+    const int s_first = blockScene(D, (int)(base / NB));
+    const int s_last = blockScene(D, (int)((min(base + WG, nn) - 1) / NB));
+    const bool uniform = s_first == s_last;
+    if (uniform && s_first != staged) {
+        __syncthreads(); // the previous scene's objects are no longer read
+        const SceneDev<T>& sd0 = D.sc[s_first];
+        nstaged = min(sd0.ncol, MAX_COLLIDERS);
+        for (int e = threadIdx.x; e < nstaged * (int)(sizeof(ColDev) / 8); e += WG)
+            reinterpret_cast<uint64_t*>(L.cols)[e] = reinterpret_cast<const uint64_t*>(D.cols + sd0.col0)[e];
+        for (int e = threadIdx.x; e < nstaged * (int)(sizeof(ColState) / 8); e += WG)
+            reinterpret_cast<uint64_t*>(L.cst)[e] = reinterpret_cast<const uint64_t*>(D.colstate + sd0.col0)[e];
+        __syncthreads();
+        staged = s_first;
+    }
+    // thread (lb, c): block lb of the iteration, object c of its scene
+    const int lb = (int)threadIdx.x / MAX_COLLIDERS, c = (int)threadIdx.x % MAX_COLLIDERS;
+    const int bb = (int)(base / NB) + lb;
+    bool hold = false;
+    if (lb < BPI && bb < nblocks) {
+        const int s = uniform ? s_first : blockScene(D, bb);
+        const SceneDev<T>& sb = D.sc[s];
+        if (!sb.ignore_cols && !sb.vfield && c < min(sb.ncol, MAX_COLLIDERS)) {
+            uint32_t Bx, By, Bz;
+            blockOfPage<T>((uint32_t)D.block_page[bb], Bx, By, Bz);
+            const int i0[3] = { (int)(Bx << B::xbits) - sb.origin[0], (int)(By << B::ybits) - sb.origin[1],
+                                (int)(Bz << B::zbits) - sb.origin[2] };
+            const int ext[3] = { 1 << B::xbits, 1 << B::ybits, 1 << B::zbits };
+            hold = uniform ? colliderMayHold<T>(L.cols[c], L.cst[c], i0, ext, sb.ddx)
+                           : colliderMayHold<T>(D.cols[sb.col0 + c], D.colstate[sb.col0 + c], i0, ext, sb.ddx);
+        }
+    }
+    const uint64_t bal = __ballot(hold);
+    if (lb < BPI && c == 0) L.bmask[it][lb] = (uint32_t)(bal >> (threadIdx.x & 48)) & 0xffffu;
+    first = s_first;
+    same = uniform;
+    ldsBarrier();
+}
+
+// The collision objects of the block words in L on node `node` (block b, of
+// scene `scene`) with velocity v: multiObjectCollision / detectAndResolveCollision
+// (CollisionObject.cpp:66-105, 341-406) at the scene's time t, in double in
+// both forms (a node on an object's surface, a wall at a whole number of
+// cells, is classified as in double).
+template <class T, class AccT>
+__device__ __forceinline__ void collideNode(const Dev<T, AccT>& D, const ColLds<T>& L, int nstaged, bool uniform,
+                                            uint32_t cmask, int64_t node, int b, int scene, T* vi)
+{
+    using B = Blk<T>;
+    constexpr int NB = ColLds<T>::NB;
+    const int e = (int)(node % NB);
+    const SceneDev<T>& sd = D.sc[scene];
+    uint32_t Bx, By, Bz;
+    blockOfPage<T>((uint32_t)D.block_page[b], Bx, By, Bz);
+    const int ix = (int)((Bx << B::xbits) | (uint32_t)(e >> (B::ybits + B::zbits))) - sd.origin[0];
+    const int iy = (int)((By << B::ybits) | (uint32_t)((e >> B::zbits) & ((1 << B::ybits) - 1))) - sd.origin[1];
+    const int iz = (int)((Bz << B::zbits) | (uint32_t)(e & ((1 << B::zbits) - 1))) - sd.origin[2];
+    const double xi[3] = { ix * sd.ddx, iy * sd.ddx, iz * sd.ddx };
+    double vd[3] = { vi[0], vi[1], vi[2] };
+    if (uniform)
+        multiObjectCollision(L.cols, L.cst, nstaged, xi, vd, cmask);
+    else
+        multiObjectCollision(D.cols + sd.col0, D.colstate + sd.col0, min(sd.ncol, MAX_COLLIDERS), xi, vd, cmask);
+#pragma unroll
+    for (int a = 0; a < 3; a++) vi[a] = (T)vd[a];
+}
+
+// The grid update (gridNode); with COL also its collision objects (collideNode)
+// on the nodes of the blocks they may reach, in this one launch.  Without COL
+// k_collide applies them next: a second launch, but this kernel then needs no
+// double-precision object code and runs at twice the occupancy.
+template <class T, class AccT, bool COL>
+__global__ __launch_bounds__(WG, COL ? 4 : 1) void k_grid(Dev<T, AccT> D)
 {
     using B = Blk<T>;
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
     const int64_t nn = (int64_t)D.counters[0] * NB;
     const int64_t cn = D.cap_nodes;
-    // the collision objects of the workgroup's scene and their states in LDS:
-    // read from there the per-node loop's loads cannot alias the kernel's stores
-    // (with the table in global memory, stores of doubles anywhere in the kernel
-    // made the compiler load it per node with vector loads: ~2x slower on large
-    // grids).  A scene's blocks are contiguous, so a workgroup's nodes rarely
-    // span two scenes; those that do read the tables in global memory.
-    __shared__ ColDev cols[MAX_COLLIDERS];
-    __shared__ ColState cst[MAX_COLLIDERS];
-    static_assert(sizeof(ColDev) % 8 == 0 && sizeof(ColState) % 8 == 0, "collider records are copied as 8-byte words");
     if (D.implicit && D.any_damage) {
         // the damage solve's accumulators, filled by k_dtilde
         for (int64_t node = (int64_t)blockIdx.x * WG + threadIdx.x; node < nn; node += (int64_t)gridDim.x * WG) {
@@ -1809,110 +1954,64 @@ __global__ __launch_bounds__(WG, 4) void k_grid(Dev<T, AccT> D)
             D.cgv[4 * cn + node] = 0.0;
         }
     }
-    // per block of the iteration, the objects that may hold one of its nodes
-    // (bit c of the block's word; two words so the next iteration's need no
-    // second barrier)
-    constexpr int BPI = WG / NB;
-    static_assert(MAX_COLLIDERS == 16 && BPI * MAX_COLLIDERS <= WG, "16 object lanes per block of an iteration");
-    __shared__ uint32_t bmask[2][BPI];
-    const int nblocks = D.counters[0];
-    int staged = -1, nstaged = 0; // the scene whose objects are in LDS (the same in every thread)
-    int it = 0;
+    if constexpr (!COL) {
+        for (int64_t node = (int64_t)blockIdx.x * WG + threadIdx.x; node < nn; node += (int64_t)gridDim.x * WG) {
+            const int b = (int)(node / NB);
+            T vi[3], pf;
+            bool col;
+            if (gridNode(D, node, b, blockScene(D, b), vi, pf, col)) nodeRec(D.newv, node) = V4<T> { vi[0], vi[1], vi[2], pf };
+        }
+    }
+    else {
+        __shared__ ColLds<T> L;
+        int staged = -1, nstaged = 0, it = 0;
+        for (int64_t base = (int64_t)blockIdx.x * WG; base < nn; base += (int64_t)gridDim.x * WG, it ^= 1) {
+            int s_first;
+            bool uniform;
+            collidePrepare(D, L, base, nn, it, staged, nstaged, s_first, uniform);
+            const int64_t node = base + threadIdx.x;
+            if (node >= nn) continue;
+            const int b = (int)(node / NB);
+            const int scene = uniform ? s_first : blockScene(D, b);
+            T vi[3], pf;
+            bool col = false;
+            if (!gridNode(D, node, b, scene, vi, pf, col)) continue;
+            const uint32_t cmask = L.bmask[it][threadIdx.x / NB];
+            if (col && cmask != 0u) collideNode(D, L, nstaged, uniform, cmask, node, b, scene, vi);
+            nodeRec(D.newv, node) = V4<T> { vi[0], vi[1], vi[2], pf };
+        }
+    }
+}
+
+// The collision objects of the grid update after k_grid<COL = false>: the
+// nodes with mass of the blocks some object may reach resolve those objects on
+// the velocity k_grid stored (in the float form a float, widened to double as
+// the fused update widens it, so the result is the one of resolving inside the
+// update).
+template <class T, class AccT>
+__global__ __launch_bounds__(WG) void k_collide(Dev<T, AccT> D)
+{
+    using B = Blk<T>;
+    constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
+    const int64_t nn = (int64_t)D.counters[0] * NB;
+    __shared__ ColLds<T> L;
+    int staged = -1, nstaged = 0, it = 0;
     for (int64_t base = (int64_t)blockIdx.x * WG; base < nn; base += (int64_t)gridDim.x * WG, it ^= 1) {
-        const int s_first = blockScene(D, (int)(base / NB));
-        const int s_last = blockScene(D, (int)((min(base + WG, nn) - 1) / NB));
-        const bool uniform = s_first == s_last;
-        if (uniform && s_first != staged) {
-            __syncthreads(); // the previous scene's objects are no longer read
-            const SceneDev<T>& sd0 = D.sc[s_first];
-            nstaged = min(sd0.ncol, MAX_COLLIDERS);
-            for (int e = threadIdx.x; e < nstaged * (int)(sizeof(ColDev) / 8); e += WG)
-                reinterpret_cast<uint64_t*>(cols)[e] = reinterpret_cast<const uint64_t*>(D.cols + sd0.col0)[e];
-            for (int e = threadIdx.x; e < nstaged * (int)(sizeof(ColState) / 8); e += WG)
-                reinterpret_cast<uint64_t*>(cst)[e] = reinterpret_cast<const uint64_t*>(D.colstate + sd0.col0)[e];
-            __syncthreads();
-            staged = s_first;
-        }
-        {
-            // thread (lb, c): block lb of the iteration, object c of its scene
-            const int lb = (int)threadIdx.x / MAX_COLLIDERS, c = (int)threadIdx.x % MAX_COLLIDERS;
-            const int bb = (int)(base / NB) + lb;
-            bool hold = false;
-            if (lb < BPI && bb < nblocks) {
-                const int s = uniform ? s_first : blockScene(D, bb);
-                const SceneDev<T>& sb = D.sc[s];
-                if (!sb.ignore_cols && c < min(sb.ncol, MAX_COLLIDERS)) {
-                    uint32_t Bx, By, Bz;
-                    blockOfPage<T>((uint32_t)D.block_page[bb], Bx, By, Bz);
-                    const int i0[3] = { (int)(Bx << B::xbits) - sb.origin[0], (int)(By << B::ybits) - sb.origin[1],
-                                        (int)(Bz << B::zbits) - sb.origin[2] };
-                    const int ext[3] = { 1 << B::xbits, 1 << B::ybits, 1 << B::zbits };
-                    hold = uniform ? colliderMayHold<T>(cols[c], cst[c], i0, ext, sb.ddx)
-                                   : colliderMayHold<T>(D.cols[sb.col0 + c], D.colstate[sb.col0 + c], i0, ext, sb.ddx);
-                }
-            }
-            const uint64_t bal = __ballot(hold);
-            if (lb < BPI && c == 0) bmask[it][lb] = (uint32_t)(bal >> (threadIdx.x & 48)) & 0xffffu;
-        }
-        ldsBarrier();
+        int s_first;
+        bool uniform;
+        collidePrepare(D, L, base, nn, it, staged, nstaged, s_first, uniform);
         const int64_t node = base + threadIdx.x;
         if (node >= nn) continue;
-        const int b = (int)(node / NB), e = (int)(node % NB);
-        const uint32_t cmask = bmask[it][threadIdx.x / NB];
+        const uint32_t cmask = L.bmask[it][threadIdx.x / NB];
+        if (cmask == 0u) continue;
+        if (D.acc[GM * D.cap_nodes + node] == AccT(0)) continue; // no mass: the record holds the force
+        const int b = (int)(node / NB);
         const int scene = uniform ? s_first : blockScene(D, b);
-        const SceneDev<T>& sd = D.sc[scene];
-        const float* sc = D.scales + scene * NSC;
-        const T m = accValue(D, GM, node, sc, SC_M);
-        T f[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) f[a] = accValue(D, GFO + a, node, sc, SC_F);
-        if (m == T(0)) {
-            nodeRec(D.newv, node) = V4<T> { f[0], f[1], f[2], T(0) };
-            if (D.flip) nodeRec(D.oldv, node) = V4<T> { T(0), T(0), T(0), T(0) };
-            continue;
-        }
-        uint32_t Bx, By, Bz;
-        blockOfPage<T>((uint32_t)D.block_page[b], Bx, By, Bz);
-        const int ix = (int)((Bx << B::xbits) | (uint32_t)(e >> (B::ybits + B::zbits))) - sd.origin[0];
-        const int iy = (int)((By << B::ybits) | (uint32_t)((e >> B::zbits) & ((1 << B::ybits) - 1))) - sd.origin[1];
-        const int iz = (int)((Bz << B::zbits) | (uint32_t)(e & ((1 << B::zbits) - 1))) - sd.origin[2];
-        // collision objects in double in both forms: a node on an object's
-        // surface (a wall at a whole number of cells) is classified as in double
-        const double xi[3] = { ix * sd.ddx, iy * sd.ddx, iz * sd.ddx };
-        const T dt = sd.dt;
-        const T dt_over_mi = dt / m;
-        T vi[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            T v = accValue(D, GMV + a, node, sc, SC_MV) / m;
-            if (D.flip) D.oldv[4 * node + a] = v; // the grid's v before the update (FLIP)
-            vi[a] = v + sd.gravity[a] * dt + f[a] * dt_over_mi;
-        }
-        // external body forces, in the order added (MpmSimulationBase.cpp:1208-1212)
-        for (int k = 0; k < sd.nbf; k++)
-#pragma unroll
-            for (int a = 0; a < 3; a++) vi[a] += dt * (T)D.bfval[3 * (scene * MAX_BODY_FORCES + k) + a];
-        double vd[3] = { vi[0], vi[1], vi[2] };
-        if (sd.ignore_cols) {
-            // ignoreCollisionObject (:1217-1221): the objects leave v as it is
-        }
-        else if (uniform)
-            multiObjectCollision(cols, cst, nstaged, xi, vd, cmask);
-        else
-            multiObjectCollision(D.cols + sd.col0, D.colstate + sd.col0, min(sd.ncol, MAX_COLLIDERS), xi, vd, cmask);
-        if (sd.vfield) {
-            // explicit_velocity_field (:1187-1195): new_v = a + B x_i at every node
-            // with mass, in place of the update above
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-                vd[a] = sd.va[a] + (sd.vB[3 * a] * xi[0] + sd.vB[3 * a + 1] * xi[1] + sd.vB[3 * a + 2] * xi[2]);
-        }
-        T pf = T(0);
-        if (D.any_damage) {
-            const T pw = accValue(D, GPW, node, sc, SC_PW);
-            pf = pw != T(0) ? accValue(D, GPF, node, sc, SC_PF) / pw : T(0);
-        }
-        nodeRec(D.newv, node) = V4<T> { (T)vd[0], (T)vd[1], (T)vd[2], pf };
+        V4<T>& rec = nodeRec(D.newv, node);
+        const V4<T> v0 = rec;
+        T vi[3] = { v0.x, v0.y, v0.z };
+        collideNode(D, L, nstaged, uniform, cmask, node, b, scene, vi);
+        rec = V4<T> { vi[0], vi[1], vi[2], v0.w };
     }
 }
 
