@@ -587,7 +587,7 @@ void launchStage(zmpm_context* h, int stage)
     case ZMPM_STAGE_GRID_UPDATE:
         if (h->variant & VARIANT_GRID_SPLIT) {
             k_grid<T, AccT, false><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
-            if (D.num_cols > 0) k_collide<T, AccT><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);
+            if (D.num_cols > 0) k_collide<T, AccT><<<gridFor(D.cap_blocks, WG / MAX_COLLIDERS, h->grid_wgs), WG, 0, st>>>(D);
         }
         else
             k_grid<T, AccT, true><<<gridFor(D.cap_nodes, WG, h->grid_wgs), WG, 0, st>>>(D);

@@ -1983,35 +1983,89 @@ __global__ __launch_bounds__(WG, COL ? 4 : 1) void k_grid(Dev<T, AccT> D)
     }
 }
 
-// The collision objects of the grid update after k_grid<COL = false>: the
-// nodes with mass of the blocks some object may reach resolve those objects on
-// the velocity k_grid stored (in the float form a float, widened to double as
-// the fused update widens it, so the result is the one of resolving inside the
-// update).
+// The collision objects of the grid update after k_grid<COL = false>, by
+// groups of 16 blocks: lanes (block, object) test whether the object may hold
+// a node of the block (reading the object tables from global memory, where 16
+// lanes share each record); then, only when some block of the group is
+// reached, the objects of the group's scene are staged in LDS and the nodes
+// with mass of the reached blocks resolve them on the velocity k_grid stored
+// (in the float form a float, widened to double as the fused update widens it,
+// so the result is the one of resolving inside the update).
 template <class T, class AccT>
 __global__ __launch_bounds__(WG) void k_collide(Dev<T, AccT> D)
 {
     using B = Blk<T>;
     constexpr int NB = 1 << (B::xbits + B::ybits + B::zbits);
-    const int64_t nn = (int64_t)D.counters[0] * NB;
-    __shared__ ColLds<T> L;
-    int staged = -1, nstaged = 0, it = 0;
-    for (int64_t base = (int64_t)blockIdx.x * WG; base < nn; base += (int64_t)gridDim.x * WG, it ^= 1) {
-        int s_first;
-        bool uniform;
-        collidePrepare(D, L, base, nn, it, staged, nstaged, s_first, uniform);
-        const int64_t node = base + threadIdx.x;
-        if (node >= nn) continue;
-        const uint32_t cmask = L.bmask[it][threadIdx.x / NB];
-        if (cmask == 0u) continue;
-        if (D.acc[GM * D.cap_nodes + node] == AccT(0)) continue; // no mass: the record holds the force
-        const int b = (int)(node / NB);
-        const int scene = uniform ? s_first : blockScene(D, b);
-        V4<T>& rec = nodeRec(D.newv, node);
-        const V4<T> v0 = rec;
-        T vi[3] = { v0.x, v0.y, v0.z };
-        collideNode(D, L, nstaged, uniform, cmask, node, b, scene, vi);
-        rec = V4<T> { vi[0], vi[1], vi[2], v0.w };
+    constexpr int GB = WG / MAX_COLLIDERS; // blocks per group
+    static_assert(MAX_COLLIDERS == 16 && (GB * NB) % WG == 0, "a group is whole iterations of WG nodes");
+    const int nblocks = D.counters[0];
+    __shared__ ColDev cols[MAX_COLLIDERS];
+    __shared__ ColState cst[MAX_COLLIDERS];
+    __shared__ uint32_t gmask[GB];
+    __shared__ int any;
+    int staged = -1, nstaged = 0; // the scene whose objects are in LDS (the same in every thread)
+    for (int g0 = (int)blockIdx.x * GB; g0 < nblocks; g0 += (int)gridDim.x * GB) {
+        const int s_first = blockScene(D, g0);
+        const bool uniform = s_first == blockScene(D, min(g0 + GB, nblocks) - 1);
+        // thread (lb, c): block g0 + lb, object c of its scene
+        const int lb = (int)threadIdx.x / MAX_COLLIDERS, c = (int)threadIdx.x % MAX_COLLIDERS;
+        const int bb = g0 + lb;
+        bool hold = false;
+        if (bb < nblocks) {
+            const int sc = uniform ? s_first : blockScene(D, bb);
+            const SceneDev<T>& sb = D.sc[sc];
+            if (!sb.ignore_cols && !sb.vfield && c < min(sb.ncol, MAX_COLLIDERS)) {
+                uint32_t Bx, By, Bz;
+                blockOfPage<T>((uint32_t)D.block_page[bb], Bx, By, Bz);
+                const int i0[3] = { (int)(Bx << B::xbits) - sb.origin[0], (int)(By << B::ybits) - sb.origin[1],
+                                    (int)(Bz << B::zbits) - sb.origin[2] };
+                const int ext[3] = { 1 << B::xbits, 1 << B::ybits, 1 << B::zbits };
+                hold = colliderMayHold<T>(D.cols[sb.col0 + c], D.colstate[sb.col0 + c], i0, ext, sb.ddx);
+            }
+        }
+        const uint64_t bal = __ballot(hold);
+        __syncthreads(); // the previous group's words and flag are no longer read
+        if (threadIdx.x == 0) any = 0;
+        ldsBarrier();
+        if ((threadIdx.x & 15) == 0) gmask[lb] = (uint32_t)(bal >> (threadIdx.x & 48)) & 0xffffu;
+        if ((threadIdx.x & 63) == 0 && bal != 0) any = 1;
+        ldsBarrier();
+        if (!any) continue; // the same in every thread
+        if (uniform && s_first != staged) {
+            const SceneDev<T>& sd0 = D.sc[s_first];
+            nstaged = min(sd0.ncol, MAX_COLLIDERS);
+            for (int e = threadIdx.x; e < nstaged * (int)(sizeof(ColDev) / 8); e += WG)
+                reinterpret_cast<uint64_t*>(cols)[e] = reinterpret_cast<const uint64_t*>(D.cols + sd0.col0)[e];
+            for (int e = threadIdx.x; e < nstaged * (int)(sizeof(ColState) / 8); e += WG)
+                reinterpret_cast<uint64_t*>(cst)[e] = reinterpret_cast<const uint64_t*>(D.colstate + sd0.col0)[e];
+            __syncthreads();
+            staged = s_first;
+        }
+        const int64_t gn = (int64_t)min(g0 + GB, nblocks) * NB;
+        for (int64_t node = (int64_t)g0 * NB + threadIdx.x; node < gn; node += WG) {
+            const int b = (int)(node / NB);
+            const uint32_t cmask = gmask[b - g0];
+            if (cmask == 0u) continue;
+            if (D.acc[GM * D.cap_nodes + node] == AccT(0)) continue; // no mass: the record holds the force
+            const int e = (int)(node % NB);
+            const int scene = uniform ? s_first : blockScene(D, b);
+            const SceneDev<T>& sd = D.sc[scene];
+            uint32_t Bx, By, Bz;
+            blockOfPage<T>((uint32_t)D.block_page[b], Bx, By, Bz);
+            const int ix = (int)((Bx << B::xbits) | (uint32_t)(e >> (B::ybits + B::zbits))) - sd.origin[0];
+            const int iy = (int)((By << B::ybits) | (uint32_t)((e >> B::zbits) & ((1 << B::ybits) - 1))) - sd.origin[1];
+            const int iz = (int)((Bz << B::zbits) | (uint32_t)(e & ((1 << B::zbits) - 1))) - sd.origin[2];
+            // in double in both forms, as the fused update
+            const double xi[3] = { ix * sd.ddx, iy * sd.ddx, iz * sd.ddx };
+            V4<T>& rec = nodeRec(D.newv, node);
+            const V4<T> v0 = rec;
+            double vd[3] = { v0.x, v0.y, v0.z };
+            if (uniform)
+                multiObjectCollision(cols, cst, nstaged, xi, vd, cmask);
+            else
+                multiObjectCollision(D.cols + sd.col0, D.colstate + sd.col0, min(sd.ncol, MAX_COLLIDERS), xi, vd, cmask);
+            rec = V4<T> { (T)vd[0], (T)vd[1], (T)vd[2], v0.w };
+        }
     }
 }
 
