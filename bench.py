@@ -97,7 +97,9 @@ def cpu_threads():
 
 def cpu_sample_scene(name, scene, max_particles):
     """A bounded sample of the workload for the CPU oracle: the smallest scenes of
-    a batch (each exactly as in the full batch), else the workload itself."""
+    a batch (each exactly as in the full batch), else the workload itself, or
+    the workload's accuracy sample (its geometry with fewer particles) when the
+    workload is above max_particles and the sample is smaller."""
     from ziran2020_amd import scenes
     if name == "batched_scenes":
         full_counts = np.array(scene.info["counts"])
@@ -110,6 +112,11 @@ def cpu_sample_scene(name, scene, max_particles):
             tot += int(full_counts[s])
         sc = scenes.batched_scenes(keep=keep)
         return sc, f"{len(keep)} of the 64 scenes ({sc.n} particles, each scene as in the full batch)"
+    if scene.n > max_particles and name in ORACLE_SAMPLES:
+        mk, what, _ = ORACLE_SAMPLES[name]
+        sc = mk(scenes)
+        if sc.n < scene.n:
+            return sc, what
     return scene, f"the full {scene.n}-particle workload"
 
 
@@ -229,13 +236,17 @@ def accuracy_vs_fp64(scene, lib, steps=100):
 
 
 ORACLE_SAMPLES = {
-    # workload -> (sample scene, description); the oracle runs 100 substeps of it
-    "elastic_cube": (lambda S: S.elastic_cube(), "the full workload"),
-    "damage_slab": (lambda S: S.damage_slab(n=125_000), "the slab at 125,000 particles"),
-    "damage_slab_active": (lambda S: S.damage_slab_active(n=125_000), "the slab at 125,000 particles"),
-    "damage_slab_implicit": (lambda S: S.damage_slab_implicit(n=125_000), "the slab at 125,000 particles"),
-    "sparse_spheres": (lambda S: S.sparse_spheres(n=250_000), "the 20 spheres at 250,000 particles"),
-    "two_material": (lambda S: S.two_material(n=250_000), "the block and column at 250,000 particles"),
+    # workload -> (sample scene, description, grid blocks the library reserves
+    # or None for its default); the oracle runs 100 substeps of it.  The sparse
+    # samples keep the workload's geometry with fewer particles, under one per
+    # cell: nearly every particle opens its own grid block, more than the
+    # library's default reserve for that particle count.
+    "elastic_cube": (lambda S: S.elastic_cube(), "the full workload", None),
+    "damage_slab": (lambda S: S.damage_slab(n=125_000), "the slab at 125,000 particles", None),
+    "damage_slab_active": (lambda S: S.damage_slab_active(n=125_000), "the slab at 125,000 particles", None),
+    "damage_slab_implicit": (lambda S: S.damage_slab_implicit(n=125_000), "the slab at 125,000 particles", None),
+    "sparse_spheres": (lambda S: S.sparse_spheres(n=250_000), "the 20 spheres at 250,000 particles", 250_000),
+    "two_material": (lambda S: S.two_material(n=250_000), "the block and column at 250,000 particles", 250_000),
 }
 
 
@@ -245,16 +256,28 @@ def accuracy_vs_oracle(name, scene, lib, steps=100):
     the workload (the oracle is the restated reference CPU step), including the
     grid mass and momentum of the last substep."""
     from ziran2020_amd import _abi as A, scenes
+    blocks = None
     if name == "batched_scenes":
         sc, what = cpu_sample_scene(name, scene, 100_000)
     else:
-        mk, what = ORACLE_SAMPLES[name]
+        mk, what, blocks = ORACLE_SAMPLES[name]
         sc = mk(scenes)
     ref = A.ZmpmLib(ORACLE_LIB)
     ref.lib.zmpm_oracle_set_threads(cpu_threads())
     outs, grids = [], []
     for L, prec in ((lib, A.PREC_FP32), (ref, A.PREC_FP64)):
-        sim = sc.make(L, prec)
+        # the library reads its block reserve from the environment at upload
+        old = os.environ.get("ZMPM_BLOCK_CAPACITY")
+        if blocks and L is lib:
+            os.environ["ZMPM_BLOCK_CAPACITY"] = str(blocks)
+        try:
+            sim = sc.make(L, prec)
+        finally:
+            if blocks and L is lib:
+                if old is None:
+                    del os.environ["ZMPM_BLOCK_CAPACITY"]
+                else:
+                    os.environ["ZMPM_BLOCK_CAPACITY"] = old
         sc.run(sim, steps)
         outs.append(sim.download())
         grids.append(sim.grid())
@@ -277,6 +300,11 @@ def profile_entry(path, key, digest):
     if e.get("source") != digest:
         return None, f"stale: measured on kernel sources {e.get('source')}, now {digest}"
     return e["value"], f"measured at {e.get('commit', '?')}"
+
+
+def progress(msg):
+    """One line on stderr as each leg of the run ends (the JSON line is stdout's)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -312,6 +340,7 @@ def main():
     el32, st32, nn32, out32 = timed(scene, lib, A.PREC_FP32, args.steps, args.warmup, autotune=not args.no_autotune)
     cfg32 = timed.launch_config
     value = scene.n * args.steps / el32
+    progress(f"FP32 form timed ({args.workload}, {scene.n} particles)")
     result = {
         "metric": METRIC,
         "value": value,
@@ -362,15 +391,19 @@ def main():
         el64, st64, nn64, out64 = timed(scene, lib, A.PREC_FP64, args.steps, args.warmup,
                                         autotune=not args.no_autotune)
         v64 = scene.n * args.steps / el64
+        progress("FP64 form timed")
         result["fp64_plain"] = {"value": v64, "ms_per_step": 1e3 * el64 / args.steps, "stage_ms": st64,
                                 "dtype": "fp64", "launch": timed.launch_config}
         result["fp32_over_fp64"] = value / v64
         if not args.no_accuracy:
             result["fp32_vs_fp64_after_100_substeps"] = accuracy_vs_fp64(scene, lib, 100)
+            progress("FP32 against FP64 after 100 substeps done")
     if not args.no_accuracy:
         result["fp32_vs_oracle_after_100_substeps"] = accuracy_vs_oracle(args.workload, scene, lib, 100)
+        progress("FP32 against the oracle after 100 substeps done")
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args.workload, scene)
+        progress("CPU baseline done")
     print(json.dumps(result), flush=True)
 
 
