@@ -181,3 +181,18 @@ def test_gpu_batch_collision_objects_fused_or_split(gpu, precision):
             assert np.array_equal(outs[0][k], outs[1][k]), k
         else:
             assert normwise(outs[0][k], outs[1][k]) < 1e-9, (k, normwise(outs[0][k], outs[1][k]))
+
+
+@pytest.mark.gpu
+def test_gpu_grid_update_form_chosen_at_upload(gpu):
+    # until a launch configuration is set, an upload chooses where the
+    # collision objects are resolved: a launch of their own for a batch, inside
+    # the grid update for a small single scene; a set configuration is kept
+    b = small_batch(n_scenes=3)
+    assert b.make(gpu, A.PREC_FP32).launch_config()["variant"] & A.VARIANT_GRID_SPLIT
+    s = b.scenes[0]
+    one = s.make(gpu, A.PREC_FP32)
+    assert not one.launch_config()["variant"] & A.VARIANT_GRID_SPLIT
+    one.set_launch_config(variant=A.VARIANT_GRID_SPLIT)
+    one.upload(s.x, s.v, s.C, s.F, s.m, s.V0, s.material, J=s.J)
+    assert one.launch_config()["variant"] == A.VARIANT_GRID_SPLIT
