@@ -35,16 +35,41 @@ METRIC = "particle-substeps/sec FP32-batched vs FP64-plain on 1 MI355X; % HBM pe
 ORACLE_LIB = os.path.join(ROOT, "oracle", "libzmpm_oracle.so")
 
 
+def code_only(text):
+    """C++ source without its comments and blank lines (string and character
+    literals kept as written)."""
+    out, i, n = [], 0, len(text)
+    while i < n:
+        c = text[i]
+        if c in "\"'":
+            j = i + 1
+            while j < n and text[j] != c:
+                j += 2 if text[j] == "\\" else 1
+            out.append(text[i:j + 1])
+            i = j + 1
+        elif text.startswith("//", i):
+            while i < n and text[i] != "\n":
+                i += 1
+        elif text.startswith("/*", i):
+            k = text.find("*/", i + 2)
+            i = n if k < 0 else k + 2
+        else:
+            out.append(c)
+            i += 1
+    lines = (ln.rstrip() for ln in "".join(out).split("\n"))
+    return "\n".join(ln for ln in lines if ln)
+
+
 def source_digest():
-    """Digest of the device sources: a traffic or instruction count measured on
-    other kernels is reported as stale, not used."""
+    """Digest of the device sources' code (comments aside): a traffic or
+    instruction count measured on other kernels is reported as stale, not used."""
     h = hashlib.sha256()
     csrc = os.path.join(ROOT, "ziran2020_amd", "csrc")
     for name in sorted(os.listdir(csrc)):
         if name.endswith((".hpp", ".hip")):
-            with open(os.path.join(csrc, name), "rb") as f:
+            with open(os.path.join(csrc, name), encoding="utf-8") as f:
                 h.update(name.encode())
-                h.update(f.read())
+                h.update(code_only(f.read()).encode())
     return h.hexdigest()[:16]
 
 

@@ -41,3 +41,16 @@ def test_grid_errors_helper():
     e = bench._grid_errors(ga, gb)
     assert e["nodes_only_in_one"] == 3 and e["m"] > 0
     assert bench._grid_errors(ga, ga)["m"] == 0.0
+
+
+def test_source_digest_ignores_comments_only():
+    # profile figures are tied to the kernel code: comments and blank lines do
+    # not change the digest, code (string literals included) does
+    a = 'int f(int x) { return x + 1; } // one\nconst char* s = "a // b";\n'
+    b = '/* doc */\nint f(int x) { return x + 1; }\n\n// two\nconst char* s = "a // b";   \n'
+    c = 'int f(int x) { return x + 2; }\nconst char* s = "a // b";\n'
+    d = 'int f(int x) { return x + 1; }\nconst char* s = "a // c";\n'
+    assert bench.code_only(a) == bench.code_only(b)
+    assert bench.code_only(a) != bench.code_only(c)
+    assert bench.code_only(a) != bench.code_only(d)
+    assert len(bench.source_digest()) == 16

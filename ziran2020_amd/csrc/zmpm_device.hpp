@@ -4,6 +4,7 @@
 // One substep (Ziran MpmSimulationBase::advanceOneTimeStep,
 // Lib/MPM/MpmSimulationBase.cpp:262-357) is this launch sequence on the
 // handle's stream:
+//   k_super_scan     touched super pages in order (single-pass compaction)
 //   k_page_scan      dense block ids in page (Morton) order, page table, block
 //                    ranges (single-pass scan)
 //   k_scatter        particles to their block ranges; zero block accumulators,
@@ -13,8 +14,10 @@
 //                    (-> reference order), then APIC scatter of m, mv, f
 //                    (stress read from the particle) and of the damage phase
 //                    field
-//   k_grid           v = mv/m, gravity + force, collision objects, phase-field
-//                    normalisation
+//   k_grid           v = mv/m, gravity + force, body forces, phase-field
+//                    normalisation; the collision objects inside it (small
+//                    grids) or in k_collide after it, on the blocks they may
+//                    reach
 //   k_dtilde, k_cg   (implicit damage only) crack driving force history and the
 //                    phase-field CG solve
 //   k_g2p            APIC gather, x update, CFL flag, damage Laplacian and
@@ -22,6 +25,8 @@
 //                    P2G's fixed-point bounds; writes particles in sorted order
 //                    and bins them for the next substep (binParticle: key,
 //                    page activation, per-page counts)
+//   k_rare           the particles k_g2p deferred (full damage update,
+//                    near-inversion stress, plasticity return mappings)
 #pragma once
 #include "zmpm_kernels.hpp"
 
