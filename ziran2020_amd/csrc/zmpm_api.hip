@@ -132,6 +132,7 @@ struct zmpm_context {
     int p2g_split = 64;
     // workgroups of the grid-stride launches (results do not depend on them)
     int p2g_wgs = 2048, grid_wgs = 2048, prep_wgs = 8192;
+    bool p2g_wgs_env = false; // P2G workgroups set by ZMPM_P2G_WGS
     int rare_wgs = 1024; // k_rare's grid: its workgroups resident at once on the device
     int variant = 0; // compiled kernel variants (VARIANT_* bits), none changes a result
     bool launch_set = false; // configured by zmpm_set_launch_config or the autotuner
@@ -695,8 +696,17 @@ zmpm_status uploadT(zmpm_context* h, const zmpm_particles* p, const int32_t* sce
     // unless configured: collision objects in a launch of their own on grids
     // large enough for the grid update's doubled occupancy to outweigh the
     // second launch (batches; single scenes from 4M particles)
-    if (!h->launch_set)
+    if (!h->launch_set) {
         h->variant = (h->variant & ~VARIANT_GRID_SPLIT) | (h->scenes.size() > 1 || n >= (1 << 22) ? VARIANT_GRID_SPLIT : 0);
+        // P2G workgroups: on large grids several times the resident ones, so
+        // the blocks' uneven work evens out across the device (one per ~1k
+        // particles, 2048 to 16384)
+        if (!h->p2g_wgs_env) {
+            int w = 2048;
+            while (w < 16384 && (int64_t)w * 2 * 512 <= (int64_t)n) w *= 2;
+            h->p2g_wgs = w;
+        }
+    }
     refreshTables<T, AccT>(h);
     const int64_t cap = D.cap;
     constexpr bool F32 = std::is_same<T, float>::value;
@@ -1125,7 +1135,10 @@ zmpm_status createScenes(const zmpm_scene_desc* descs, int32_t n_scenes, zmpm_ha
         h->origin[(size_t)s] = { (r / (rd * rd)) * region, ((r / rd) % rd) * region, (r % rd) * region };
     }
     if (const char* e = std::getenv("ZMPM_NO_GRAPH")) h->use_graphs = std::atoi(e) == 0;
-    if (const char* e = std::getenv("ZMPM_P2G_WGS")) h->p2g_wgs = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("ZMPM_P2G_WGS")) {
+        h->p2g_wgs = std::max(1, std::atoi(e));
+        h->p2g_wgs_env = true;
+    }
     if (const char* e = std::getenv("ZMPM_GRID_WGS")) h->grid_wgs = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("ZMPM_PREP_WGS")) h->prep_wgs = std::max(1, std::atoi(e));
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1747,7 +1760,7 @@ zmpm_status zmpm_autotune(zmpm_handle h, double dt, double t0, int32_t steps, in
         std::vector<int> candidates;
     };
     std::vector<Knob> knobs = {
-        { &h->p2g_wgs, ~0, 1, { 512, 1024, 2048, 4096 } },
+        { &h->p2g_wgs, ~0, 1, { 2048, 4096, 8192, 16384, 32768 } },
         { &h->prep_wgs, ~0, 0, { 512, 2048, 8192 } },
         { &h->grid_wgs, ~0, 2, { 256, 512, 1024, 2048, 4096 } },
     };
@@ -1811,7 +1824,7 @@ done:
 
 // Key of the tuning cache: the device and what selects the work and the kernels.
 // Raised when the kernels change enough that cached tunings should be redone.
-constexpr int kTuneRevision = 3;
+constexpr int kTuneRevision = 4;
 
 static std::string tuneKey(zmpm_handle h, int32_t tune_variants)
 {
